@@ -1,0 +1,188 @@
+"""Seeded synthetic scenes: the workloads the contact pipeline is measured and checked on.
+
+Each scene is a list of body specs, dicts with the keys the reference's
+``world.addBody`` options use (``src/world.js:189-218`` DEFAULTS, ``docs/API.md``):
+``type``, ``shape``, ``size`` / ``radius`` / ``height``, ``position``, ``rotation``,
+``velocity``, ``angularVelocity``, ``friction``, ``restitution``, ``density``, ``mass``.
+A key left out takes the reference's default. Every number is a double produced here
+from a splitmix64 stream (doubles as ``(x >> 11) * 2**-53``, Gaussians by the Marsaglia
+polar method), so the same seed gives the same scene bit for bit in any language that
+reads the JSON this module writes.
+
+Scenes:
+  pyramid      210 unit boxes in 20 rows on a static 40x1x40 ground (test/contact.test.js:45-64)
+  rain         spheres raining into a walled 20x20 floor
+  tower        a box tower, each level turned about y, on a 10x1x10 ground
+  pile_world   one world of the batched workload: a tower or a sphere pile
+  throw        a thrown box landing and hitting a 6x6 wall of unit boxes
+"""
+from __future__ import annotations
+
+import json
+import math
+import sys
+
+_MASK = (1 << 64) - 1
+
+
+class SplitMix64:
+    """splitmix64 (Steele, Lea, Flood 2014): the generator every scene draws from."""
+
+    def __init__(self, seed: int):
+        self.state = seed & _MASK
+
+    def next_u64(self) -> int:
+        self.state = (self.state + 0x9E3779B97F4A7C15) & _MASK
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _MASK
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _MASK
+        return z ^ (z >> 31)
+
+    def uniform(self) -> float:
+        """A double in [0, 1): the top 53 bits."""
+        return (self.next_u64() >> 11) * (2.0 ** -53)
+
+    def between(self, lo: float, hi: float) -> float:
+        return lo + (hi - lo) * self.uniform()
+
+    def gauss(self, sigma: float = 1.0) -> float:
+        """Marsaglia's polar method; the second value of each pair is dropped."""
+        while True:
+            u = 2.0 * self.uniform() - 1.0
+            v = 2.0 * self.uniform() - 1.0
+            s = u * u + v * v
+            if 0.0 < s < 1.0:
+                return sigma * u * math.sqrt(-2.0 * math.log(s) / s)
+
+
+def _ground(size, top=0.0, friction=None):
+    spec = {"type": "static", "shape": "box", "size": list(size), "position": [0.0, top - size[1] * 0.5, 0.0]}
+    if friction is not None:
+        spec["friction"] = friction
+    return spec
+
+
+def pyramid(rows: int = 20):
+    """Rows of unit boxes, row k holding rows-k boxes 1 m apart, centred, at y = 0.5 + k.
+
+    Boxes have friction 0.6, density 1000, restitution 0; the ground is a static
+    40x1x40 box whose top is at y = 0.
+    """
+    bodies = [_ground((40.0, 1.0, 40.0))]
+    for row in range(rows):
+        for k in range(rows - row):
+            bodies.append({"shape": "box", "size": [1.0, 1.0, 1.0], "friction": 0.6,
+                           "position": [k - (rows - 1 - row) / 2, 0.5 + row, 0.0]})
+    return bodies
+
+
+def _walls(half: float, height: float):
+    """Four static walls 1 m thick and `height` tall around a floor 2*half across."""
+    h = height * 0.5
+    return [
+        {"type": "static", "shape": "box", "size": [1.0, height, 2 * half], "position": [half + 0.5, h, 0.0]},
+        {"type": "static", "shape": "box", "size": [1.0, height, 2 * half], "position": [-half - 0.5, h, 0.0]},
+        {"type": "static", "shape": "box", "size": [2 * half, height, 1.0], "position": [0.0, h, half + 0.5]},
+        {"type": "static", "shape": "box", "size": [2 * half, height, 1.0], "position": [0.0, h, -half - 0.5]},
+    ]
+
+
+def rain(count: int = 2000, seed: int = 2, spread: float = 5.0, lo: float = 2.0, hi: float = 42.0):
+    """Spheres of radius U[0.1, 0.3], restitution U[0, 0.7], friction 0.5, density 1000,
+    centres uniform in x, z in [-spread, spread], y in [lo, hi], at rest, over a static
+    20x1x20 floor walled by four static 1x6x20 boxes."""
+    rng = SplitMix64(seed)
+    bodies = [_ground((20.0, 1.0, 20.0))] + _walls(10.0, 6.0)
+    for _ in range(count):
+        r = rng.between(0.1, 0.3)
+        e = rng.between(0.0, 0.7)
+        x = rng.between(-spread, spread)
+        y = rng.between(lo, hi)
+        z = rng.between(-spread, spread)
+        bodies.append({"shape": "sphere", "radius": r, "restitution": e, "friction": 0.5, "position": [x, y, z]})
+    return bodies
+
+
+def _yaw(theta: float):
+    """The unit quaternion [x, y, z, w] turning by theta about +y."""
+    return [0.0, math.sin(theta * 0.5), 0.0, math.cos(theta * 0.5)]
+
+
+def tower(height: int, rng: SplitMix64, ground=(10.0, 1.0, 10.0)):
+    """`height` unit boxes stacked, each level turned U[0, 15] degrees about y."""
+    bodies = [_ground(ground)]
+    for k in range(height):
+        theta = math.radians(rng.between(0.0, 15.0))
+        bodies.append({"shape": "box", "size": [1.0, 1.0, 1.0], "position": [0.0, 0.5 + k, 0.0],
+                       "rotation": _yaw(theta)})
+    return bodies
+
+
+def sphere_pile(count: int, rng: SplitMix64):
+    """`count` spheres as in rain(), over a static 10x1x10 ground, dropped from y in [1, 1 + count/25]."""
+    bodies = [_ground((10.0, 1.0, 10.0))]
+    top = 1.0 + count / 25.0
+    for _ in range(count):
+        r = rng.between(0.1, 0.3)
+        e = rng.between(0.0, 0.7)
+        x = rng.between(-2.5, 2.5)
+        y = rng.between(1.0, top)
+        z = rng.between(-2.5, 2.5)
+        bodies.append({"shape": "sphere", "radius": r, "restitution": e, "friction": 0.5, "position": [x, y, z]})
+    return bodies
+
+
+def batched(worlds: int = 4096, seed: int = 4, lo: int = 8, hi: int = 512):
+    """Independent worlds, body counts log-uniform in [lo, hi]; even worlds box towers of
+    height U{4..20}, odd worlds sphere piles of the world's count. Returns a list of scenes."""
+    rng = SplitMix64(seed)
+    out = []
+    for w in range(worlds):
+        n = int(math.floor(math.exp(rng.between(math.log(lo), math.log(hi + 1)))))
+        if w % 2 == 0:
+            height = 4 + int(rng.uniform() * 17)
+            out.append(tower(height, rng))
+        else:
+            out.append(sphere_pile(n, rng))
+    return out
+
+
+def throw(rng: SplitMix64, dt_wall: float = 4.0):
+    """A box of half-extents U[0.2, 0.5]^3 and mass U[1, 10] kg thrown with horizontal
+    velocity N(0, 2^2), upward 3 m/s and spin N(0, 1)^3, over a ground of friction
+    U[0.2, 0.9] and restitution U[0, 0.5], towards a 6x6 wall of unit boxes 4 m away."""
+    half = [rng.between(0.2, 0.5) for _ in range(3)]
+    mass = rng.between(1.0, 10.0)
+    mu = rng.between(0.2, 0.9)
+    e = rng.between(0.0, 0.5)
+    vx, vz = rng.gauss(2.0), rng.gauss(2.0)
+    w = [rng.gauss(1.0) for _ in range(3)]
+    ground = {"type": "static", "shape": "box", "size": [20.0, 1.0, 20.0], "position": [0.0, -0.5, 0.0],
+              "friction": mu, "restitution": e}
+    bodies = [ground]
+    for row in range(6):
+        for col in range(6):
+            bodies.append({"shape": "box", "size": [1.0, 1.0, 1.0], "position": [dt_wall, 0.5 + row, col - 2.5]})
+    bodies.append({"shape": "box", "size": [2 * h for h in half], "mass": mass, "position": [0.0, 1.0, 0.0],
+                   "velocity": [vx, 3.0, vz], "angularVelocity": w})
+    return bodies
+
+
+SCENES = {
+    "pyramid": lambda: pyramid(),
+    "rain": lambda: rain(),
+    "rain_small": lambda: rain(count=150, seed=2, spread=3.0, lo=0.5, hi=6.0),
+    "tower": lambda: tower(12, SplitMix64(40)),
+    "pile_small": lambda: sphere_pile(60, SplitMix64(41)),
+    "throw": lambda: throw(SplitMix64(5)),
+}
+
+
+def main(argv):
+    name = argv[1]
+    json.dump(SCENES[name](), sys.stdout)
+    sys.stdout.write("\n")
+
+
+if __name__ == "__main__":
+    main(sys.argv)
