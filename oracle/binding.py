@@ -1,0 +1,120 @@
+"""ORACLE — test infrastructure only. ctypes binding of _build/liboracle.so (oracle.h)."""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "liboracle.so")
+
+_D = ctypes.POINTER(ctypes.c_double)
+_U8 = ctypes.POINTER(ctypes.c_uint8)
+_I32 = ctypes.POINTER(ctypes.c_int32)
+_U32 = ctypes.POINTER(ctypes.c_uint32)
+
+# (struct field, kind, fixture column, numbers per unit, unit) in oracle.h order.
+BODY = [
+    ("position", _D, "position", 3), ("rotation", _D, "rotation", 4), ("velocity", _D, "velocity", 3),
+    ("angular", _D, "angular", 3), ("dims", _D, "dims", 3), ("inv_mass", _D, "invMass", 1),
+    ("inertia", _D, "inertia", 3), ("inv_inertia", _D, "invInertia", 3), ("inv_i", _D, "invI", 6),
+    ("axes", _D, "axes", 9), ("tumbled", _D, "tumbled", 3), ("ceiling", _D, "ceiling", 1),
+    ("gravity_scale", _D, "gravityScale", 1), ("damping", _D, "damping", 1),
+    ("angular_damping", _D, "angularDamping", 1), ("start", _D, "start", 3), ("start_q", _D, "startQ", 4),
+    ("type", _U8, "type", 1), ("shape", _U8, "shape", 1), ("frozen", _U8, "frozen", 1), ("lock", _U8, "lock", 1),
+    ("framed", _U8, "framed", 1), ("awake", _U8, "awake", 1), ("joint_head", _I32, "jointHead", 1),
+]
+MANIFOLD = [
+    ("a", _I32, "a", 1), ("b", _I32, "b", 1), ("pair", _I32, "pair", 1), ("sub", _I32, "sub", 1),
+    ("row", _I32, "row", 1), ("points", _U8, "points", 1), ("sliding", _U8, "sliding", 1),
+    ("normal", _D, "normal", 3), ("t1", _D, "t1", 3), ("t2", _D, "t2", 3), ("friction", _D, "friction", 1),
+    ("restitution", _D, "restitution", 1), ("tolerance", _D, "tolerance", 1), ("patch", _D, "patch", 1),
+    ("roll_impulse1", _D, "rollImpulse1", 1), ("roll_impulse2", _D, "rollImpulse2", 1),
+    ("spin_impulse", _D, "spinImpulse", 1), ("tangent_impulse1", _D, "tangentImpulse1", 1),
+    ("tangent_impulse2", _D, "tangentImpulse2", 1), ("drift1", _D, "drift1", 1), ("drift2", _D, "drift2", 1),
+    ("separation", _D, "separation", 4), ("point", _D, "point", 12), ("shift", _D, "shift", 12),
+    ("normal_impulse", _D, "normalImpulse", 4), ("max_normal_impulse", _D, "maxNormalImpulse", 4),
+    ("relative_velocity", _D, "relativeVelocity", 4), ("gap", _D, "gap", 1), ("now", _D, "now", 4),
+    ("ra", _D, "ra", 12), ("rb", _D, "rb", 12), ("la", _D, "la", 12), ("lb", _D, "lb", 12),
+    ("adjusted", _D, "adjusted", 4), ("normal_mass", _D, "normalMass", 4), ("jac", _D, "jac", 48),
+    ("block", _D, "block", 16), ("lever", _D, "lever", 4), ("anchor_a", _D, "anchorA", 3),
+    ("anchor_b", _D, "anchorB", 3), ("fjac", _D, "fjac", 24), ("tangent_mass", _D, "tangentMass", 3),
+    ("roll_mass1", _D, "rollMass1", 1), ("roll_mass2", _D, "rollMass2", 1), ("spin_mass", _D, "spinMass", 1),
+    ("pid", _U32, "pid", 4),
+]
+PAIR = [("pair_points", _U8, "points", 1), ("pair_pid", _U32, "pid", 4),
+        ("pair_normal_impulse", _D, "normalImpulse", 4), ("pair_twist", _D, "twist", 3),
+        ("pair_friction", _D, "friction", 3), ("pair_slip", _D, "slip", 3)]
+
+_NP = {_D: np.float64, _U8: np.uint8, _I32: np.int32, _U32: np.uint32}
+
+
+class OracleSoft(ctypes.Structure):
+    _fields_ = ([(n, ctypes.c_int32) for n in ("top", "awake_count", "substeps", "count", "solved", "pair_count")]
+                + [("dt", ctypes.c_double), ("gravity", ctypes.c_double * 3)]
+                + [(f, k) for f, k, _, _ in BODY[:17]] + [(f, k) for f, k, _, _ in BODY[17:23]]
+                + [("joint_head", _I32), ("awake_list", _I32)]
+                + [("starts", _I32)] + [(f, k) for f, k, _, _ in MANIFOLD[:5]] + [(f, k) for f, k, _, _ in MANIFOLD[5:7]]
+                + [(f, k) for f, k, _, _ in MANIFOLD[7:45]] + [("pid", _U32)]
+                + [(f, k) for f, k, _, _ in PAIR])
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError(f"oracle not built: run make -C {HERE}")
+        _lib = ctypes.CDLL(LIB)
+        _lib.oracle_soft_step.argtypes = [ctypes.POINTER(OracleSoft)]
+        _lib.oracle_soft_step.restype = ctypes.c_int
+    return _lib
+
+
+def columns(fx):
+    """The soft step's columns from a fixture's 'in.*' arrays, as fresh numpy arrays by
+    fixture column name: bodies as ``b.<name>``, manifolds ``c.<name>``, pairs ``p.<name>``."""
+    meta = fx.meta
+    top, count, pairs = meta["top"], meta["count"], meta["pairs"]
+    cols = {}
+    for _, kind, name, per in BODY:
+        src = fx.get("in." + name)
+        cols["b." + name] = src.copy() if src is not None else np.zeros(top * per, _NP[kind])
+    for _, kind, name, per in MANIFOLD:
+        src = fx.get("in.c." + name)
+        cols["c." + name] = src.copy() if src is not None else np.zeros(count * per, _NP[kind])
+    for _, kind, name, per in PAIR:
+        cols["p." + name] = np.zeros(max(pairs, 1) * per, _NP[kind])
+    cols["awake_list"] = fx["in.awakeList"].copy()
+    cols["starts"] = fx["in.starts"].copy()
+    cols["gravity"] = fx["in.gravity"].copy()
+    return cols
+
+
+def soft_step(fx, cols=None):
+    """Runs oracle_soft_step on a fixture's input state; returns the columns after it."""
+    meta = fx.meta
+    cols = columns(fx) if cols is None else cols
+    st = OracleSoft()
+    st.top, st.awake_count, st.substeps = meta["top"], meta["awake"], meta["substeps"]
+    st.count, st.solved, st.pair_count = meta["count"], meta["solved"], meta["pairs"]
+    st.dt = meta["dt"]
+    for k in range(3):
+        st.gravity[k] = cols["gravity"][k]
+
+    def ptr(a, kind):
+        return a.ctypes.data_as(kind)
+
+    for f, kind, name, _ in BODY:
+        setattr(st, f, ptr(cols["b." + name], kind))
+    for f, kind, name, _ in MANIFOLD:
+        setattr(st, f, ptr(cols["c." + name], kind))
+    for f, kind, name, _ in PAIR:
+        setattr(st, f, ptr(cols["p." + name], kind))
+    st.awake_list = ptr(cols["awake_list"], _I32)
+    st.starts = ptr(cols["starts"], _I32)
+    rc = lib().oracle_soft_step(ctypes.byref(st))
+    if rc != 0:
+        raise RuntimeError(f"oracle_soft_step returned {rc}")
+    return cols

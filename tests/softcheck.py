@@ -1,0 +1,94 @@
+"""Bitwise comparison of the soft step's outputs against the reference's fixtures.
+
+Compares only what the reference's soft step writes: manifold columns on the solved
+rows, per-point columns on the points each manifold has, the K block on manifolds of
+two or more points, rolling masses on manifolds of at most two, and on the pair list
+the entries store() keeps. Unused slots of the reference's arrays hold whatever an
+earlier step left there, which nothing reads.
+"""
+import numpy as np
+
+PER_POINT = {"normalImpulse": 1, "maxNormalImpulse": 1, "relativeVelocity": 1, "now": 1, "adjusted": 1,
+             "normalMass": 1, "lever": 1, "ra": 3, "rb": 3, "la": 3, "lb": 3, "jac": 12}
+PER_MANIFOLD = {"gap": 1, "tangentImpulse1": 1, "tangentImpulse2": 1, "spinImpulse": 1, "drift1": 1, "drift2": 1,
+                "sliding": 1, "anchorA": 3, "anchorB": 3, "fjac": 24, "tangentMass": 3, "spinMass": 1}
+ROLL = ["rollImpulse1", "rollImpulse2", "rollMass1", "rollMass2"]
+BODY = ["position", "rotation", "velocity", "angular", "tumbled", "ceiling", "invI", "start", "startQ"]
+
+
+def bits(a):
+    a = np.ascontiguousarray(a)
+    return a.view(np.uint64) if a.dtype == np.float64 else a
+
+
+def first_diff(name, got, want, index=None):
+    g, w = bits(got), bits(want)
+    if g.shape != w.shape:
+        return f"{name}: shape {g.shape} != {w.shape}"
+    bad = np.nonzero(g != w)[0]
+    if len(bad) == 0:
+        return None
+    i = bad[0]
+    where = index[i] if index is not None else i
+    return f"{name}: {len(bad)} differ, first at {where}: got {got[i]!r} want {want[i]!r}"
+
+
+def point_index(points, count, per, solved):
+    idx = []
+    for m in range(solved):
+        for k in range(points[m]):
+            base = (4 * m + k) * per
+            idx.extend(range(base, base + per))
+    return np.array(idx, dtype=np.int64)
+
+
+def compare(fx, got):
+    """Differences between `got` (columns by fixture name: b.*, c.*, p.*) and the fixture's out.*; [] if none."""
+    meta = fx.meta
+    solved, count = meta["solved"], meta["count"]
+    points = fx["in.c.points"].astype(np.int64)
+    errs = []
+    for name in BODY:
+        e = first_diff("body." + name, got["b." + name], fx["out." + name])
+        if e:
+            errs.append(e)
+    for name, per in PER_POINT.items():
+        idx = point_index(points, count, per, solved)
+        if len(idx):
+            e = first_diff("c." + name, got["c." + name][idx], fx["out.c." + name][idx], idx)
+            if e:
+                errs.append(e)
+    for name, per in PER_MANIFOLD.items():
+        n = solved * per
+        e = first_diff("c." + name, got["c." + name][:n], fx["out.c." + name][:n])
+        if e:
+            errs.append(e)
+    rows = [m for m in range(solved) if points[m] <= 2]
+    for name in ROLL:
+        e = first_diff("c." + name, got["c." + name][rows], fx["out.c." + name][rows], rows)
+        if e:
+            errs.append(e)
+    bidx = np.array([16 * m + 4 * k + l for m in range(solved) if points[m] >= 2
+                     for k in range(points[m]) for l in range(points[m])], dtype=np.int64)
+    if len(bidx):
+        e = first_diff("c.block", got["c.block"][bidx], fx["out.c.block"][bidx], bidx)
+        if e:
+            errs.append(e)
+    pair = fx["in.c.pair"]
+    pidx = [pair[m] for m in range(solved)]
+    e = first_diff("pairs.points", got["p.points"][pidx], fx["out.pairs.points"][pidx], pidx)
+    if e:
+        errs.append(e)
+    for name in ("twist", "friction", "slip"):
+        idx = np.array([3 * p + k for p in pidx for k in range(3)], dtype=np.int64)
+        if len(idx):
+            e = first_diff("pairs." + name, got["p." + name][idx], fx["out.pairs." + name][idx], idx)
+            if e:
+                errs.append(e)
+    idx = np.array([4 * pair[m] + k for m in range(solved) for k in range(points[m])], dtype=np.int64)
+    for name in ("pid", "normalImpulse"):
+        if len(idx):
+            e = first_diff("pairs." + name, got["p." + name][idx], fx["out.pairs." + name][idx], idx)
+            if e:
+                errs.append(e)
+    return errs

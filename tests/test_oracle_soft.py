@@ -1,0 +1,37 @@
+"""The oracle's soft step against the reference's own (tests/golden fixtures): bit for bit.
+
+Each fixture holds the reference's world and manifolds as contact.js inertias() received
+them and as store() left them (src/world.js:961-1031); the oracle runs the same phase
+from the same state and must leave every written column with the same bits.
+"""
+import numpy as np
+import pytest
+
+import fixtures
+from oracle import binding
+from softcheck import compare
+
+SOFT = fixtures.find("soft")
+
+
+@pytest.mark.parametrize("path", SOFT, ids=[p.split("/")[-1].replace(".bmfx.gz", "") for p in SOFT])
+def test_soft_step_matches_reference(path):
+    fx = fixtures.load(path)
+    got = binding.soft_step(fx)
+    errs = compare(fx, got)
+    assert not errs, "\n".join(errs)
+
+
+def test_fixtures_present():
+    assert len(SOFT) >= 15
+
+
+@pytest.mark.parametrize("column,index", [("c.friction", 0), ("c.tolerance", 3), ("b.velocity", 7)])
+def test_a_planted_mistake_is_caught(column, index):
+    """One ulp changed in one input column must show up in the comparison."""
+    fx = fixtures.load(fixtures.find("soft", "pyramid")[1])
+    cols = binding.columns(fx)
+    a = cols[column]
+    a.view(np.uint64)[index] += 1
+    got = binding.soft_step(fx, cols)
+    assert compare(fx, got), f"an ulp in {column}[{index}] went unnoticed"
