@@ -1,0 +1,276 @@
+// The C ABI of include/bearing_mi355x.h: contexts, HBM residency of a batch's
+// columns, and the soft-step launch.
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/bearing_mi355x.h"
+#include "soft.h"
+
+using bm::SoftDev;
+
+namespace {
+
+enum Unit { WORLD1, WORLD, STARTS, WORLD3, BODY, AWAKE, MANIFOLD, PAIR };
+enum Dir { IN = 1, IO = 3 };
+
+struct Column {
+  const char* name;
+  size_t host;  // offsetof in bm_columns
+  size_t dev;   // offsetof in SoftDev
+  size_t elem;  // bytes per number
+  int per;      // numbers per unit
+  Unit unit;
+  Dir dir;
+};
+
+#define COL(n, e, p, u, d) {#n, offsetof(bm_columns, n), offsetof(SoftDev, n), e, p, u, d}
+const Column COLUMNS[] = {
+    COL(body_offset, 4, 1, WORLD1, IN), COL(manifold_offset, 4, 1, WORLD1, IN), COL(pair_offset, 4, 1, WORLD1, IN),
+    COL(awake_offset, 4, 1, WORLD1, IN), COL(starts, 4, 1, STARTS, IN), COL(solved, 4, 1, WORLD, IN),
+    COL(substeps, 4, 1, WORLD, IN), COL(dt, 8, 1, WORLD, IN), COL(gravity, 8, 1, WORLD3, IN),
+    COL(position, 8, 3, BODY, IO), COL(rotation, 8, 4, BODY, IO), COL(velocity, 8, 3, BODY, IO),
+    COL(angular, 8, 3, BODY, IO), COL(start, 8, 3, BODY, IO), COL(start_q, 8, 4, BODY, IO),
+    COL(inv_i, 8, 6, BODY, IO), COL(tumbled, 8, 3, BODY, IO), COL(ceiling, 8, 1, BODY, IO),
+    COL(dims, 8, 3, BODY, IN), COL(inv_mass, 8, 1, BODY, IN), COL(inertia, 8, 3, BODY, IN),
+    COL(inv_inertia, 8, 3, BODY, IN), COL(axes, 8, 9, BODY, IN), COL(gravity_scale, 8, 1, BODY, IN),
+    COL(damping, 8, 1, BODY, IN), COL(angular_damping, 8, 1, BODY, IN), COL(type, 1, 1, BODY, IN),
+    COL(shape, 1, 1, BODY, IN), COL(frozen, 1, 1, BODY, IN), COL(lock, 1, 1, BODY, IN), COL(framed, 1, 1, BODY, IN),
+    COL(awake, 1, 1, BODY, IN), COL(joint_head, 4, 1, BODY, IN), COL(awake_list, 4, 1, AWAKE, IN),
+    COL(a, 4, 1, MANIFOLD, IN), COL(b, 4, 1, MANIFOLD, IN), COL(pair, 4, 1, MANIFOLD, IN),
+    COL(sub, 4, 1, MANIFOLD, IN), COL(row, 4, 1, MANIFOLD, IN), COL(points, 1, 1, MANIFOLD, IN),
+    COL(sliding, 1, 1, MANIFOLD, IO), COL(normal, 8, 3, MANIFOLD, IN), COL(t1, 8, 3, MANIFOLD, IN),
+    COL(t2, 8, 3, MANIFOLD, IN), COL(friction, 8, 1, MANIFOLD, IN), COL(restitution, 8, 1, MANIFOLD, IN),
+    COL(tolerance, 8, 1, MANIFOLD, IN), COL(patch, 8, 1, MANIFOLD, IN), COL(roll_impulse1, 8, 1, MANIFOLD, IO),
+    COL(roll_impulse2, 8, 1, MANIFOLD, IO), COL(spin_impulse, 8, 1, MANIFOLD, IO),
+    COL(tangent_impulse1, 8, 1, MANIFOLD, IO), COL(tangent_impulse2, 8, 1, MANIFOLD, IO),
+    COL(drift1, 8, 1, MANIFOLD, IO), COL(drift2, 8, 1, MANIFOLD, IO), COL(separation, 8, 4, MANIFOLD, IN),
+    COL(point, 8, 12, MANIFOLD, IN), COL(shift, 8, 12, MANIFOLD, IN), COL(normal_impulse, 8, 4, MANIFOLD, IO),
+    COL(max_normal_impulse, 8, 4, MANIFOLD, IO), COL(relative_velocity, 8, 4, MANIFOLD, IO),
+    COL(gap, 8, 1, MANIFOLD, IO), COL(now, 8, 4, MANIFOLD, IO), COL(ra, 8, 12, MANIFOLD, IO),
+    COL(rb, 8, 12, MANIFOLD, IO), COL(la, 8, 12, MANIFOLD, IO), COL(lb, 8, 12, MANIFOLD, IO),
+    COL(adjusted, 8, 4, MANIFOLD, IO), COL(normal_mass, 8, 4, MANIFOLD, IO), COL(jac, 8, 48, MANIFOLD, IO),
+    COL(block, 8, 16, MANIFOLD, IO), COL(lever, 8, 4, MANIFOLD, IO), COL(anchor_a, 8, 3, MANIFOLD, IO),
+    COL(anchor_b, 8, 3, MANIFOLD, IO), COL(fjac, 8, 24, MANIFOLD, IO), COL(tangent_mass, 8, 3, MANIFOLD, IO),
+    COL(roll_mass1, 8, 1, MANIFOLD, IO), COL(roll_mass2, 8, 1, MANIFOLD, IO), COL(spin_mass, 8, 1, MANIFOLD, IO),
+    COL(pid, 4, 4, MANIFOLD, IN), COL(pair_points, 1, 1, PAIR, IO), COL(pair_pid, 4, 4, PAIR, IO),
+    COL(pair_normal_impulse, 8, 4, PAIR, IO), COL(pair_twist, 8, 3, PAIR, IO), COL(pair_friction, 8, 3, PAIR, IO),
+    COL(pair_slip, 8, 3, PAIR, IO),
+};
+#undef COL
+constexpr int NCOL = sizeof(COLUMNS) / sizeof(COLUMNS[0]);
+
+}  // namespace
+
+struct bm_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t begin = nullptr, end = nullptr;
+  std::string error;
+  int32_t n_worlds = 0, bodies = 0, awake = 0, manifolds = 0, pairs = 0;
+  void* buffer[NCOL] = {};
+  size_t capacity[NCOL] = {};
+  size_t bytes[NCOL] = {};
+  SoftDev dev{};
+  bool uploaded = false;
+  float last_ms = 0;
+};
+
+namespace {
+
+int fail(bm_context* ctx, int code, const std::string& message) {
+  if (ctx) ctx->error = message;
+  return code;
+}
+
+#define HIP_OR_FAIL(ctx, call, where)                                                             \
+  do {                                                                                            \
+    hipError_t e_ = (call);                                                                       \
+    if (e_ != hipSuccess) return fail(ctx, BM_ERR_DEVICE, std::string(where) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+size_t units(const bm_context* ctx, Unit u) {
+  switch (u) {
+    case WORLD1: return (size_t)ctx->n_worlds + 1;
+    case WORLD: return (size_t)ctx->n_worlds;
+    case STARTS: return (size_t)ctx->n_worlds * BM_STARTS;
+    case WORLD3: return (size_t)ctx->n_worlds * 3;
+    case BODY: return (size_t)ctx->bodies;
+    case AWAKE: return (size_t)ctx->awake;
+    case MANIFOLD: return (size_t)ctx->manifolds;
+    case PAIR: return (size_t)ctx->pairs;
+  }
+  return 0;
+}
+
+template <typename T>
+T field(const void* base, size_t off) {
+  T v;
+  std::memcpy(&v, static_cast<const char*>(base) + off, sizeof(T));
+  return v;
+}
+
+// Checks a batch's offset tables and the indices inside its columns, so no kernel
+// can read or write outside the buffers it is given.
+int check_batch(bm_context* ctx, const bm_columns* h) {
+  const int n = h->n_worlds;
+  if (n <= 0) return fail(ctx, BM_ERR_ARGUMENT, "bm_upload: n_worlds must be 1 or more");
+  const int32_t* tables[4] = {h->body_offset, h->manifold_offset, h->pair_offset, h->awake_offset};
+  const char* names[4] = {"body_offset", "manifold_offset", "pair_offset", "awake_offset"};
+  for (int t = 0; t < 4; t++) {
+    if (!tables[t]) return fail(ctx, BM_ERR_ARGUMENT, std::string("bm_upload: ") + names[t] + " is null");
+    if (tables[t][0] != 0) return fail(ctx, BM_ERR_ARGUMENT, std::string("bm_upload: ") + names[t] + "[0] must be 0");
+    for (int w = 0; w < n; w++)
+      if (tables[t][w + 1] < tables[t][w])
+        return fail(ctx, BM_ERR_ARGUMENT, std::string("bm_upload: ") + names[t] + " must not decrease");
+  }
+  if (h->awake_offset[n] > 0 && !h->awake_list) return fail(ctx, BM_ERR_ARGUMENT, "bm_upload: awake_list is null");
+  if (h->manifold_offset[n] > 0 && (!h->a || !h->b || !h->pair || !h->sub || !h->row || !h->points))
+    return fail(ctx, BM_ERR_ARGUMENT, "bm_upload: a manifold index column is null");
+  for (int w = 0; w < n; w++) {
+    const int nb = h->body_offset[w + 1] - h->body_offset[w];
+    const int nm = h->manifold_offset[w + 1] - h->manifold_offset[w];
+    const int np = h->pair_offset[w + 1] - h->pair_offset[w];
+    const int moff = h->manifold_offset[w], aoff = h->awake_offset[w];
+    const int32_t* st = h->starts + BM_STARTS * w;
+    if (h->solved[w] < 0 || h->solved[w] > nm) return fail(ctx, BM_ERR_ARGUMENT, "bm_upload: solved out of range");
+    if (h->substeps[w] < 1) return fail(ctx, BM_ERR_ARGUMENT, "bm_upload: substeps must be 1 or more");
+    if (st[0] != 0) return fail(ctx, BM_ERR_ARGUMENT, "bm_upload: starts[0] must be 0");
+    for (int k = 0; k + 1 < BM_STARTS; k++)
+      if (st[k + 1] < st[k] || st[k + 1] > nm) return fail(ctx, BM_ERR_ARGUMENT, "bm_upload: starts out of order");
+    if (st[BM_COLOURS + 1] != h->solved[w]) return fail(ctx, BM_ERR_ARGUMENT, "bm_upload: solved must equal the resting run's start");
+    for (int i = aoff; i < h->awake_offset[w + 1]; i++)
+      if (h->awake_list[i] < 0 || h->awake_list[i] >= nb) return fail(ctx, BM_ERR_ARGUMENT, "bm_upload: awake_list slot out of range");
+    for (int m = moff; m < moff + nm; m++) {
+      if (h->a[m] < 0 || h->a[m] >= nb || h->b[m] < 0 || h->b[m] >= nb)
+        return fail(ctx, BM_ERR_ARGUMENT, "bm_upload: manifold body slot out of range");
+      if (h->pair[m] < 0 || h->pair[m] >= np) return fail(ctx, BM_ERR_ARGUMENT, "bm_upload: manifold pair out of range");
+      if (h->row[m] < 0 || h->row[m] >= nm) return fail(ctx, BM_ERR_ARGUMENT, "bm_upload: row out of range");
+      if (h->points[m] < 1 || h->points[m] > 4) return fail(ctx, BM_ERR_ARGUMENT, "bm_upload: a manifold has 1 to 4 points");
+      if (m - moff < h->solved[w] && h->sub[m] >= 0)
+        return fail(ctx, BM_ERR_UNSUPPORTED, "bm_upload: manifolds of mesh and compound pairs are not supported");
+    }
+  }
+  return BM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bm_create(bm_context** out, int32_t device) {
+  if (!out) return BM_ERR_ARGUMENT;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0) return BM_ERR_DEVICE;
+  bm_context* ctx = new bm_context();
+  ctx->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess
+      || hipEventCreate(&ctx->begin) != hipSuccess || hipEventCreate(&ctx->end) != hipSuccess) {
+    delete ctx;
+    return BM_ERR_DEVICE;
+  }
+  *out = ctx;
+  return BM_OK;
+}
+
+int bm_destroy(bm_context* ctx) {
+  if (!ctx) return BM_OK;
+  (void)hipSetDevice(ctx->device);
+  for (int c = 0; c < NCOL; c++)
+    if (ctx->buffer[c]) (void)hipFree(ctx->buffer[c]);
+  if (ctx->begin) (void)hipEventDestroy(ctx->begin);
+  if (ctx->end) (void)hipEventDestroy(ctx->end);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return BM_OK;
+}
+
+const char* bm_last_error(const bm_context* ctx) { return ctx ? ctx->error.c_str() : "bm: no context"; }
+
+double bm_last_kernel_ms(const bm_context* ctx) { return ctx ? ctx->last_ms : 0.0; }
+
+int bm_upload(bm_context* ctx, const bm_columns* h) {
+  if (!ctx || !h) return fail(ctx, BM_ERR_ARGUMENT, "bm_upload: null argument");
+  ctx->error.clear();
+  ctx->uploaded = false;
+  for (int c = 0; c < NCOL; c++) {
+    const Column& col = COLUMNS[c];
+    if (col.unit <= WORLD3 && !field<const void*>(h, col.host))
+      return fail(ctx, BM_ERR_ARGUMENT, std::string("bm_upload: ") + col.name + " is null");
+  }
+  if (int rc = check_batch(ctx, h)) return rc;
+  ctx->n_worlds = h->n_worlds;
+  ctx->bodies = h->body_offset[h->n_worlds];
+  ctx->manifolds = h->manifold_offset[h->n_worlds];
+  ctx->pairs = h->pair_offset[h->n_worlds];
+  ctx->awake = h->awake_offset[h->n_worlds];
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->device), "bm_upload: hipSetDevice");
+  for (int c = 0; c < NCOL; c++) {
+    const Column& col = COLUMNS[c];
+    const size_t bytes = units(ctx, col.unit) * col.per * col.elem;
+    ctx->bytes[c] = bytes;
+    const void* src = field<const void*>(h, col.host);
+    if (bytes > 0 && !src) return fail(ctx, BM_ERR_ARGUMENT, std::string("bm_upload: ") + col.name + " is null");
+    if (bytes > ctx->capacity[c]) {
+      if (ctx->buffer[c]) HIP_OR_FAIL(ctx, hipFree(ctx->buffer[c]), "bm_upload: hipFree");
+      ctx->buffer[c] = nullptr;
+      size_t cap = 256;
+      while (cap < bytes) cap *= 2;
+      HIP_OR_FAIL(ctx, hipMalloc(&ctx->buffer[c], cap), "bm_upload: hipMalloc");
+      ctx->capacity[c] = cap;
+    }
+    if (bytes > 0)
+      HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->buffer[c], src, bytes, hipMemcpyHostToDevice, ctx->stream),
+                  "bm_upload: hipMemcpyAsync");
+    void* p = ctx->buffer[c];
+    std::memcpy(reinterpret_cast<char*>(&ctx->dev) + col.dev, &p, sizeof(void*));
+  }
+  ctx->dev.n_worlds = ctx->n_worlds;
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_upload: hipStreamSynchronize");
+  ctx->uploaded = true;
+  return BM_OK;
+}
+
+int bm_soft_step(bm_context* ctx) {
+  if (!ctx) return BM_ERR_ARGUMENT;
+  if (!ctx->uploaded) return fail(ctx, BM_ERR_ARGUMENT, "bm_soft_step: no batch uploaded");
+  ctx->error.clear();
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->device), "bm_soft_step: hipSetDevice");
+  HIP_OR_FAIL(ctx, hipEventRecord(ctx->begin, ctx->stream), "bm_soft_step: hipEventRecord");
+  bm::launch_soft_step(ctx->dev, ctx->n_worlds, 256, ctx->stream);
+  HIP_OR_FAIL(ctx, hipGetLastError(), "bm_soft_step: launch");
+  HIP_OR_FAIL(ctx, hipEventRecord(ctx->end, ctx->stream), "bm_soft_step: hipEventRecord");
+  HIP_OR_FAIL(ctx, hipEventSynchronize(ctx->end), "bm_soft_step: kernel");
+  HIP_OR_FAIL(ctx, hipEventElapsedTime(&ctx->last_ms, ctx->begin, ctx->end), "bm_soft_step: hipEventElapsedTime");
+  return BM_OK;
+}
+
+int bm_download(bm_context* ctx, bm_columns* h) {
+  if (!ctx || !h) return fail(ctx, BM_ERR_ARGUMENT, "bm_download: null argument");
+  if (!ctx->uploaded) return fail(ctx, BM_ERR_ARGUMENT, "bm_download: no batch uploaded");
+  ctx->error.clear();
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->device), "bm_download: hipSetDevice");
+  for (int c = 0; c < NCOL; c++) {
+    const Column& col = COLUMNS[c];
+    if (col.dir != IO || ctx->bytes[c] == 0) continue;
+    void* dst = field<void*>(h, col.host);
+    if (!dst) return fail(ctx, BM_ERR_ARGUMENT, std::string("bm_download: ") + col.name + " is null");
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(dst, ctx->buffer[c], ctx->bytes[c], hipMemcpyDeviceToHost, ctx->stream),
+                "bm_download: hipMemcpyAsync");
+  }
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_download: hipStreamSynchronize");
+  return BM_OK;
+}
+
+int bm_soft_step_host(bm_context* ctx, bm_columns* h) {
+  if (int rc = bm_upload(ctx, h)) return rc;
+  if (int rc = bm_soft_step(ctx)) return rc;
+  return bm_download(ctx, h);
+}
+
+}  // extern "C"

@@ -1,0 +1,36 @@
+// Device-side view of a batch (the columns of bearing_mi355x.h, in HBM).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define BM_STARTS_DEV 35  // colours, overflow, resting, end (src/contact.js:160)
+
+namespace bm {
+
+struct SoftDev {
+  int n_worlds;
+  const int *body_offset, *manifold_offset, *pair_offset, *awake_offset;
+  const int *starts, *solved, *substeps;
+  const double *dt, *gravity;
+  double *position, *rotation, *velocity, *angular, *start, *start_q, *inv_i, *tumbled, *ceiling;
+  const double *dims, *inv_mass, *inertia, *inv_inertia, *axes, *gravity_scale, *damping, *angular_damping;
+  const uint8_t *type, *shape, *frozen, *lock, *framed, *awake;
+  const int *joint_head, *awake_list;
+  const int *a, *b, *pair, *sub, *row;
+  const uint8_t* points;
+  uint8_t* sliding;
+  const double *normal, *t1, *t2, *friction, *restitution, *tolerance, *patch;
+  double *roll_impulse1, *roll_impulse2, *spin_impulse, *tangent_impulse1, *tangent_impulse2, *drift1, *drift2;
+  const double *separation, *point, *shift;
+  double *normal_impulse, *max_normal_impulse, *relative_velocity, *gap, *now, *ra, *rb, *la, *lb, *adjusted,
+      *normal_mass, *jac, *block, *lever, *anchor_a, *anchor_b, *fjac, *tangent_mass, *roll_mass1, *roll_mass2,
+      *spin_mass;
+  const uint32_t* pid;
+  uint8_t* pair_points;
+  uint32_t* pair_pid;
+  double *pair_normal_impulse, *pair_twist, *pair_friction, *pair_slip;
+};
+
+void launch_soft_step(const SoftDev& d, int grid, int block, hipStream_t stream);
+
+}  // namespace bm

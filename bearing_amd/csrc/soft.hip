@@ -1,0 +1,880 @@
+// The soft step on an MI355X: one workgroup per world, persistent across the
+// whole phase (inertias, prepare, every substep's integrate / warm start / solve /
+// relax, restitution, store), with a workgroup barrier between colours.
+//
+// Why this shape. A step's solve is 8 substeps x 3 sweeps x (colours + overflow)
+// dependent sweeps (about 216 for nine colours), each a few hundred manifolds for a
+// small world: a kernel launch per sweep would cost more than the sweep. Inside one
+// workgroup a sweep ends at a barrier instead. A batch of worlds is a grid of
+// workgroups, one world each, which fills the 256 CUs when there are thousands of
+// worlds. Each thread takes whole manifolds (up to four points, the reference's
+// block LCP is per manifold), gathering its two bodies' velocities from HBM and
+// scattering them back; within a colour no two manifolds share a body the solve
+// moves (src/graph.js:1-16), so the scatters never race, and static, frozen and
+// kinematic bodies (inverse mass 0) are never written (src/contact.js:2096-2104).
+//
+// Arithmetic is the reference's, operation for operation, in doubles, built with
+// -ffp-contract=off: the results are the reference's bits. Each function cites
+// the lines it follows.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "jsmath.h"
+#include "soft.h"
+
+namespace bm {
+
+constexpr int P = 4;      // points per manifold, src/contact.js:128
+constexpr int J = 12;     // turning terms per point, src/contact.js:139
+constexpr int FJ = 24;
+constexpr int DYNAMIC = 0;
+constexpr int SPHERE_SHAPE = 1, CAPSULE_SHAPE = 2;
+constexpr int COLOURS = 32, OVERFLOW = COLOURS;
+constexpr double STIFFNESS = 1.0 / 8;               // src/world.js:115
+constexpr double DAMPING_RATIO = 10;                // src/world.js:118
+constexpr double SPIN_TURN = 3.141592653589793;     // src/world.js:171
+constexpr double SPIN_DISC = 2.0 / 3;               // src/contact.js:146
+constexpr double COMPLIANCE = 1.0 / 1048576;        // 2^-20, src/forest.js:67
+constexpr double ROUNDED = 1.0 / 17592186044416.0;  // 2^-44, src/world.js:2809
+constexpr double MATH_PI = 3.141592653589793;
+
+struct Soft {
+  double bias_rate, mass_scale, impulse_scale;
+};
+
+// src/contact.js:1590-1597
+__device__ inline Soft make_soft(double hertz, double zeta, double h) {
+  const double omega = 2 * MATH_PI * hertz;
+  const double a1 = 2 * zeta + h * omega, a2 = h * omega * a1, a3 = 1 / (1 + a2);
+  return Soft{omega / a1, a2 * a3, a3};
+}
+
+// src/contact.js:1520-1549 (lockedInverse)
+__device__ inline void locked_inverse(const double* ax, const double* I, int lock, double* out) {
+  const double i0 = I[0], i1 = I[1], i2 = I[2];
+  const double x0 = ax[0], x1 = ax[1], x2 = ax[2];
+  const double y0 = ax[3], y1 = ax[4], y2 = ax[5];
+  const double z0 = ax[6], z1 = ax[7], z2 = ax[8];
+  double M[6];
+  M[0] = x0 * x0 * i0 + y0 * y0 * i1 + z0 * z0 * i2;
+  M[1] = x0 * x1 * i0 + y0 * y1 * i1 + z0 * z1 * i2;
+  M[2] = x0 * x2 * i0 + y0 * y2 * i1 + z0 * z2 * i2;
+  M[3] = x1 * x1 * i0 + y1 * y1 * i1 + z1 * z1 * i2;
+  M[4] = x1 * x2 * i0 + y1 * y2 * i1 + z1 * z2 * i2;
+  M[5] = x2 * x2 * i0 + y2 * y2 * i1 + z2 * z2 * i2;
+  for (int k = 0; k < 6; k++) out[k] = 0;
+  int first = -1, second = -1;
+  for (int k = 0; k < 3; k++) {
+    if ((lock >> k) & 1) continue;
+    if (first < 0) first = k; else second = k;
+  }
+  if (first < 0) return;
+  // Entry (r, c) of a symmetric 3x3 in xx xy xz yy yz zz: diagonal 0, 3, 5.
+  const int ff = first == 0 ? 0 : first == 1 ? 3 : 5;
+  if (second < 0) {
+    if (M[ff] > 0) out[ff] = 1 / M[ff];
+    return;
+  }
+  const int ss = second == 1 ? 3 : 5;
+  const int fs = first == 0 ? (second == 1 ? 1 : 2) : 4;
+  const double det = M[ff] * M[ss] - M[fs] * M[fs];
+  if (!(det > 0)) return;
+  out[ff] = M[ss] / det; out[ss] = M[ff] / det; out[fs] = -M[fs] / det;
+}
+
+// src/contact.js:1302-1310 and 1493-1508: R diag(1/I) R^T for an awake dynamic body.
+__device__ inline void inertia_of(const SoftDev& d, int g) {
+  double* out = d.inv_i + 6 * g;
+  if (d.frozen[g] != 0 || d.type[g] != DYNAMIC) {
+    for (int k = 0; k < 6; k++) out[k] = 0;
+    return;
+  }
+  const double* ax = d.axes + 9 * g;
+  if (d.lock[g] != 0) {
+    double o[6];
+    locked_inverse(ax, d.inertia + 3 * g, d.lock[g], o);
+    for (int k = 0; k < 6; k++) out[k] = o[k];
+    return;
+  }
+  const double* inv = d.inv_inertia + 3 * g;
+  const double i0 = inv[0], i1 = inv[1], i2 = inv[2];
+  const double x0 = ax[0], x1 = ax[1], x2 = ax[2];
+  const double y0 = ax[3], y1 = ax[4], y2 = ax[5];
+  const double z0 = ax[6], z1 = ax[7], z2 = ax[8];
+  out[0] = x0 * x0 * i0 + y0 * y0 * i1 + z0 * z0 * i2;
+  out[1] = x0 * x1 * i0 + y0 * y1 * i1 + z0 * z1 * i2;
+  out[2] = x0 * x2 * i0 + y0 * y2 * i1 + z0 * z2 * i2;
+  out[3] = x1 * x1 * i0 + y1 * y1 * i1 + z1 * z1 * i2;
+  out[4] = x1 * x2 * i0 + y1 * y2 * i1 + z1 * z2 * i2;
+  out[5] = x2 * x2 * i0 + y2 * y2 * i1 + z2 * z2 * i2;
+}
+
+// src/contact.js:1709-1722 (turning, along)
+__device__ __forceinline__ double along(const double* I, double x, double y, double z) {
+  return I[0] * x * x + I[3] * y * y + I[5] * z * z + 2 * (I[1] * x * y + I[2] * x * z + I[4] * y * z);
+}
+__device__ __forceinline__ double turning(const double* Ia, const double* Ib, double x, double y, double z) {
+  const double k = along(Ia, x, y, z) + along(Ib, x, y, z);
+  return k > 0 ? 1 / k : 0;
+}
+
+// src/contact.js:1565-1581 (lever): the 12 turning terms into jac, the effective mass back.
+__device__ inline double lever(double* jac, const double* Ia, const double* Ib, double ima, double imb, double rax,
+                               double ray, double raz, double rbx, double rby, double rbz, double dx, double dy,
+                               double dz) {
+  const double cax = ray * dz - raz * dy, cay = raz * dx - rax * dz, caz = rax * dy - ray * dx;
+  const double cbx = rby * dz - rbz * dy, cby = rbz * dx - rbx * dz, cbz = rbx * dy - rby * dx;
+  const double iax = Ia[0] * cax + Ia[1] * cay + Ia[2] * caz;
+  const double iay = Ia[1] * cax + Ia[3] * cay + Ia[4] * caz;
+  const double iaz = Ia[2] * cax + Ia[4] * cay + Ia[5] * caz;
+  const double ibx = Ib[0] * cbx + Ib[1] * cby + Ib[2] * cbz;
+  const double iby = Ib[1] * cbx + Ib[3] * cby + Ib[4] * cbz;
+  const double ibz = Ib[2] * cbx + Ib[4] * cby + Ib[5] * cbz;
+  jac[0] = cax; jac[1] = cay; jac[2] = caz;
+  jac[3] = cbx; jac[4] = cby; jac[5] = cbz;
+  jac[6] = iax; jac[7] = iay; jac[8] = iaz;
+  jac[9] = ibx; jac[10] = iby; jac[11] = ibz;
+  const double k = ima + imb + cax * iax + cay * iay + caz * iaz + cbx * ibx + cby * iby + cbz * ibz;
+  return k > 0 ? 1 / k : 0;
+}
+
+// src/contact.js:1199-1220 (core) for a whole body: anchor in its frame into out;
+// returns the anchor's world offset at the start.
+__device__ inline V3 core(const SoftDev& d, int g, double x, double y, double z, double* out) {
+  const int shape = d.shape[g];
+  if (shape == SPHERE_SHAPE) {
+    out[0] = 0; out[1] = 0; out[2] = 0;
+    return V3{0, 0, 0};
+  }
+  const double* q = d.rotation + 4 * g;
+  V3 v = rotate(q, x, y, z, true);
+  if (shape != CAPSULE_SHAPE) {
+    out[0] = v.x; out[1] = v.y; out[2] = v.z;
+    return V3{x, y, z};
+  }
+  const double half = (d.dims[3 * g + 1] - 2 * d.dims[3 * g]) * 0.5;
+  const double c = v.y < -half ? -half : v.y > half ? half : v.y;
+  out[0] = 0; out[1] = c; out[2] = 0;
+  return rotate(q, 0, c, 0, false);
+}
+
+// src/contact.js:1346-1445 (prepareRows) for manifold row m (global), bodies ga, gb.
+__device__ void prepare_row(const SoftDev& d, int m, int ga, int gb) {
+  const double* Ia = d.inv_i + 6 * ga;
+  const double* Ib = d.inv_i + 6 * gb;
+  const double *pa = d.position + 3 * ga, *pb = d.position + 3 * gb;
+  const double *va = d.velocity + 3 * ga, *vb = d.velocity + 3 * gb;
+  const double *wa = d.angular + 3 * ga, *wb = d.angular + 3 * gb;
+  const double ima = d.inv_mass[ga], imb = d.inv_mass[gb];
+  const int m3 = 3 * m;
+  const double nx = d.normal[m3], ny = d.normal[m3 + 1], nz = d.normal[m3 + 2];
+  const int count = d.points[m];
+  if (count <= 2) {
+    d.roll_mass1[m] = turning(Ia, Ib, d.t1[m3], d.t1[m3 + 1], d.t1[m3 + 2]);
+    d.roll_mass2[m] = turning(Ia, Ib, d.t2[m3], d.t2[m3 + 1], d.t2[m3 + 2]);
+  }
+  d.spin_mass[m] = turning(Ia, Ib, nx, ny, nz);
+  const double tol = d.tolerance[m];
+  double weight = 0, ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0;
+  double ex = 0, ey = 0, ez = 0, fx = 0, fy = 0, fz = 0;
+  for (int k = 0; k < count; k++) {
+    const int i = m * P + k, i3 = 3 * i;
+    const double px = d.point[i3], py = d.point[i3 + 1], pz = d.point[i3 + 2];
+    const double rax = px - pa[0], ray = py - pa[1], raz = pz - pa[2];
+    const double rbx = px + d.shift[i3] - pb[0], rby = py + d.shift[i3 + 1] - pb[1], rbz = pz + d.shift[i3 + 2] - pb[2];
+    d.ra[i3] = rax; d.ra[i3 + 1] = ray; d.ra[i3 + 2] = raz;
+    d.rb[i3] = rbx; d.rb[i3 + 1] = rby; d.rb[i3 + 2] = rbz;
+    const V3 ca = core(d, ga, rax, ray, raz, d.la + i3);
+    const V3 cb = core(d, gb, rbx, rby, rbz, d.lb + i3);
+    d.adjusted[i] = d.separation[i] - ((cb.x - ca.x) * nx + (cb.y - ca.y) * ny + (cb.z - ca.z) * nz);
+    d.normal_mass[i] = lever(d.jac + J * i, Ia, Ib, ima, imb, rax, ray, raz, rbx, rby, rbz, nx, ny, nz);
+    d.max_normal_impulse[i] = 0;
+    const double s = d.separation[i] / tol, near = s <= 1 ? 1 : s >= 2 ? 0 : 2 - s;
+    weight += near;
+    ax += near * rax; ay += near * ray; az += near * raz; bx += near * rbx; by += near * rby; bz += near * rbz;
+    ex += rax; ey += ray; ez += raz; fx += rbx; fy += rby; fz += rbz;
+    const double dvx = vb[0] + wb[1] * rbz - wb[2] * rby - va[0] - wa[1] * raz + wa[2] * ray;
+    const double dvy = vb[1] + wb[2] * rbx - wb[0] * rbz - va[1] - wa[2] * rax + wa[0] * raz;
+    const double dvz = vb[2] + wb[0] * rby - wb[1] * rbx - va[2] - wa[0] * ray + wa[1] * rax;
+    d.relative_velocity[i] = dvx * nx + dvy * ny + dvz * nz;
+  }
+  if (count >= 2) {
+    const double* jac = d.jac + J * (m * P);
+    double* K = d.block + m * P * P;
+    const double s = ima + imb;
+    for (int k = 0; k < count; k++) {
+      const double* jk = jac + J * k;
+      for (int l = 0; l <= k; l++) {
+        const double* jl = jac + J * l;
+        const double e = s + jk[0] * jl[6] + jk[1] * jl[7] + jk[2] * jl[8] + jk[3] * jl[9] + jk[4] * jl[10]
+            + jk[5] * jl[11];
+        K[P * k + l] = e; K[P * l + k] = e;
+      }
+    }
+  }
+  if (weight == 0) { weight = count; ax = ex; ay = ey; az = ez; bx = fx; by = fy; bz = fz; }
+  ax /= weight; ay /= weight; az /= weight; bx /= weight; by /= weight; bz /= weight;
+  d.anchor_a[m3] = ax; d.anchor_a[m3 + 1] = ay; d.anchor_a[m3 + 2] = az;
+  d.anchor_b[m3] = bx; d.anchor_b[m3 + 1] = by; d.anchor_b[m3 + 2] = bz;
+  for (int k = 0; k < count; k++) {
+    const int i3 = 3 * (m * P + k);
+    double dx = d.ra[i3] - ax, dy = d.ra[i3 + 1] - ay, dz = d.ra[i3 + 2] - az;
+    const double al = dx * nx + dy * ny + dz * nz;
+    dx -= al * nx; dy -= al * ny; dz -= al * nz;
+    d.lever[m * P + k] = dsqrt(dx * dx + dy * dy + dz * dz);
+  }
+  double* f = d.fjac + FJ * m;
+  lever(f, Ia, Ib, ima, imb, ax, ay, az, bx, by, bz, d.t1[m3], d.t1[m3 + 1], d.t1[m3 + 2]);
+  lever(f + 12, Ia, Ib, ima, imb, ax, ay, az, bx, by, bz, d.t2[m3], d.t2[m3 + 1], d.t2[m3 + 2]);
+  const double k11 = ima + imb + f[0] * f[6] + f[1] * f[7] + f[2] * f[8] + f[3] * f[9] + f[4] * f[10] + f[5] * f[11];
+  const double k22 = ima + imb + f[12] * f[18] + f[13] * f[19] + f[14] * f[20] + f[15] * f[21] + f[16] * f[22]
+      + f[17] * f[23];
+  const double k12 = f[12] * f[6] + f[13] * f[7] + f[14] * f[8] + f[15] * f[9] + f[16] * f[10] + f[17] * f[11];
+  const double det = k11 * k22 - k12 * k12;
+  double* tm = d.tangent_mass + m3;
+  if (det > 0) {
+    tm[0] = k22 / det; tm[1] = -k12 / det; tm[2] = k11 / det;
+  } else {
+    tm[0] = 0; tm[1] = 0; tm[2] = 0;
+  }
+}
+
+// The two bodies' velocities, held in registers for a manifold (src/contact.js:1897-1908).
+struct Pair {
+  double vax, vay, vaz, wax, way, waz, vbx, vby, vbz, wbx, wby, wbz;
+};
+
+__device__ __forceinline__ Pair gather(const SoftDev& d, int ga, int gb) {
+  const double *va = d.velocity + 3 * ga, *wa = d.angular + 3 * ga;
+  const double *vb = d.velocity + 3 * gb, *wb = d.angular + 3 * gb;
+  return Pair{va[0], va[1], va[2], wa[0], wa[1], wa[2], vb[0], vb[1], vb[2], wb[0], wb[1], wb[2]};
+}
+
+// src/contact.js:2096-2104 (writeBack): a body that cannot move is never written.
+__device__ __forceinline__ void scatter(const SoftDev& d, int ga, int gb, double ima, double imb, const Pair& s) {
+  if (ima != 0) {
+    double *v = d.velocity + 3 * ga, *w = d.angular + 3 * ga;
+    v[0] = s.vax; v[1] = s.vay; v[2] = s.vaz; w[0] = s.wax; w[1] = s.way; w[2] = s.waz;
+  }
+  if (imb != 0) {
+    double *v = d.velocity + 3 * gb, *w = d.angular + 3 * gb;
+    v[0] = s.vbx; v[1] = s.vby; v[2] = s.vbz; w[0] = s.wbx; w[1] = s.wby; w[2] = s.wbz;
+  }
+}
+
+// src/contact.js:1616-1663 (warmRows) for row m.
+__device__ void warm_row(const SoftDev& d, int m, int ga, int gb) {
+  const double ima = d.inv_mass[ga], imb = d.inv_mass[gb];
+  const int m3 = 3 * m;
+  const double nx = d.normal[m3], ny = d.normal[m3 + 1], nz = d.normal[m3 + 2];
+  const double t1x = d.t1[m3], t1y = d.t1[m3 + 1], t1z = d.t1[m3 + 2];
+  const double t2x = d.t2[m3], t2y = d.t2[m3 + 1], t2z = d.t2[m3 + 2];
+  Pair s = gather(d, ga, gb);
+  const int count = d.points[m];
+  for (int k = 0; k < count; k++) {
+    const int i = m * P + k;
+    const double* jac = d.jac + J * i;
+    const double ln = d.normal_impulse[i];
+    const double px = nx * ln, py = ny * ln, pz = nz * ln;
+    s.vax -= ima * px; s.vay -= ima * py; s.vaz -= ima * pz;
+    s.vbx += imb * px; s.vby += imb * py; s.vbz += imb * pz;
+    s.wax -= jac[6] * ln; s.way -= jac[7] * ln; s.waz -= jac[8] * ln;
+    s.wbx += jac[9] * ln; s.wby += jac[10] * ln; s.wbz += jac[11] * ln;
+  }
+  {
+    const double* fj = d.fjac + FJ * m;
+    const double l1 = d.tangent_impulse1[m], l2 = d.tangent_impulse2[m];
+    const double px = t1x * l1 + t2x * l2, py = t1y * l1 + t2y * l2, pz = t1z * l1 + t2z * l2;
+    s.vax -= ima * px; s.vay -= ima * py; s.vaz -= ima * pz;
+    s.vbx += imb * px; s.vby += imb * py; s.vbz += imb * pz;
+    s.wax -= fj[6] * l1 + fj[18] * l2; s.way -= fj[7] * l1 + fj[19] * l2; s.waz -= fj[8] * l1 + fj[20] * l2;
+    s.wbx += fj[9] * l1 + fj[21] * l2; s.wby += fj[10] * l1 + fj[22] * l2; s.wbz += fj[11] * l1 + fj[23] * l2;
+  }
+  {
+    const double r1 = d.roll_impulse1[m], r2 = d.roll_impulse2[m], sp = d.spin_impulse[m];
+    const double *Ia = d.inv_i + 6 * ga, *Ib = d.inv_i + 6 * gb;
+    const double lx = t1x * r1 + t2x * r2 + nx * sp, ly = t1y * r1 + t2y * r2 + ny * sp, lz = t1z * r1 + t2z * r2 + nz * sp;
+    s.wax -= Ia[0] * lx + Ia[1] * ly + Ia[2] * lz;
+    s.way -= Ia[1] * lx + Ia[3] * ly + Ia[4] * lz;
+    s.waz -= Ia[2] * lx + Ia[4] * ly + Ia[5] * lz;
+    s.wbx += Ib[0] * lx + Ib[1] * ly + Ib[2] * lz;
+    s.wby += Ib[1] * lx + Ib[3] * ly + Ib[4] * lz;
+    s.wbz += Ib[2] * lx + Ib[4] * ly + Ib[5] * lz;
+  }
+  scatter(d, ga, gb, ima, imb, s);
+}
+
+// src/contact.js:1672-1724 (pushes): Gaussian elimination without pivoting on the
+// points of `set`; the rest 0. The four-point case is the reference's unrolled one.
+__device__ inline void pushes(const double* K, int count, int set, const double* give, const double* wanted,
+                              double* solved) {
+  if (set == 15) {
+    const double g0 = 1 + give[0], g1 = 1 + give[1], g2 = 1 + give[2], g3 = 1 + give[3];
+    double a00 = K[0] * g0, a01 = K[1], a02 = K[2], a03 = K[3];
+    double a10 = K[4], a11 = K[5] * g1, a12 = K[6], a13 = K[7];
+    double a20 = K[8], a21 = K[9], a22 = K[10] * g2, a23 = K[11];
+    double a30 = K[12], a31 = K[13], a32 = K[14], a33 = K[15] * g3;
+    double r0 = wanted[0], r1 = wanted[1], r2 = wanted[2], r3 = wanted[3], f;
+    f = a10 / a00; a11 -= f * a01; a12 -= f * a02; a13 -= f * a03; r1 -= f * r0;
+    f = a20 / a00; a21 -= f * a01; a22 -= f * a02; a23 -= f * a03; r2 -= f * r0;
+    f = a30 / a00; a31 -= f * a01; a32 -= f * a02; a33 -= f * a03; r3 -= f * r0;
+    f = a21 / a11; a22 -= f * a12; a23 -= f * a13; r2 -= f * r1;
+    f = a31 / a11; a32 -= f * a12; a33 -= f * a13; r3 -= f * r1;
+    f = a32 / a22; a33 -= f * a23; r3 -= f * r2;
+    const double x3 = r3 / a33;
+    const double x2 = (r2 - a23 * x3) / a22;
+    const double x1 = (r1 - a12 * x2 - a13 * x3) / a11;
+    const double x0 = (r0 - a01 * x1 - a02 * x2 - a03 * x3) / a00;
+    solved[0] = x0; solved[1] = x1; solved[2] = x2; solved[3] = x3;
+    return;
+  }
+  double A[P * P], R[P], X[P];
+  int in[P];
+  int n = 0;
+  for (int k = 0; k < count; k++) {
+    solved[k] = 0;
+    if ((set >> k) & 1) in[n++] = k;
+  }
+  for (int r = 0; r < n; r++) {
+    const int kr = in[r];
+    for (int s = 0; s < n; s++) A[r * P + s] = K[P * kr + in[s]];
+    A[r * P + r] *= 1 + give[kr];
+    R[r] = wanted[kr];
+  }
+  for (int col = 0; col < n; col++) {
+    const double pivot = A[col * P + col];
+    for (int r = col + 1; r < n; r++) {
+      const double f = A[r * P + col] / pivot;
+      for (int s = col + 1; s < n; s++) A[r * P + s] -= f * A[col * P + s];
+      R[r] -= f * R[col];
+    }
+  }
+  for (int r = n - 1; r >= 0; r--) {
+    double x = R[r];
+    for (int s = r + 1; s < n; s++) x -= A[r * P + s] * X[s];
+    X[r] = x / A[r * P + r];
+    solved[in[r]] = X[r];
+  }
+}
+
+// src/contact.js:1805-2090 (solveRows) for row m.
+__device__ void solve_row(const SoftDev& d, const Soft& soft, double h, bool bias, bool known, int m, int ga, int gb) {
+  const double inv = 1 / h;
+  const double springy = soft.impulse_scale / soft.mass_scale + COMPLIANCE;
+  double target[P], mass_scale[P], impulse_scale[P], give[P], closing_now[P], wanted[P], solved[P];
+  const int m3 = 3 * m;
+  const double nx = d.normal[m3], ny = d.normal[m3 + 1], nz = d.normal[m3 + 2];
+  const double max_bias = d.tolerance[m] * inv;
+  const int count = d.points[m];
+  double dn = 0;
+  V3 na{0, 0, 0}, nb{0, 0, 0};
+  if (!known) {
+    const double *pa = d.position + 3 * ga, *pb = d.position + 3 * gb;
+    const double *sa = d.start + 3 * ga, *sb = d.start + 3 * gb;
+    const double dx = (pb[0] - sb[0]) - (pa[0] - sa[0]);
+    const double dy = (pb[1] - sb[1]) - (pa[1] - sa[1]);
+    const double dz = (pb[2] - sb[2]) - (pa[2] - sa[2]);
+    dn = dx * nx + dy * ny + dz * nz;
+    na = rotate(d.rotation + 4 * ga, nx, ny, nz, true);
+    nb = rotate(d.rotation + 4 * gb, nx, ny, nz, true);
+  }
+  double least = __builtin_inf();
+  for (int k = 0; k < count; k++) {
+    const int i = m * P + k, i3 = 3 * i;
+    double s;
+    if (known) {
+      s = d.now[i];
+    } else {
+      s = dn + (d.lb[i3] * nb.x + d.lb[i3 + 1] * nb.y + d.lb[i3 + 2] * nb.z)
+          - (d.la[i3] * na.x + d.la[i3 + 1] * na.y + d.la[i3 + 2] * na.z) + d.adjusted[i];
+      if (!bias) d.now[i] = s;
+    }
+    if (s < least) least = s;
+    if (s > 0) {
+      target[k] = s * inv; mass_scale[k] = 1; impulse_scale[k] = 0; give[k] = COMPLIANCE;
+    } else if (bias) {
+      target[k] = js_max(soft.bias_rate * s, -max_bias);
+      mass_scale[k] = soft.mass_scale; impulse_scale[k] = soft.impulse_scale; give[k] = springy;
+    } else {
+      target[k] = 0; mass_scale[k] = 1; impulse_scale[k] = 0; give[k] = COMPLIANCE;
+    }
+  }
+  d.gap[m] = least;
+  const double ima = d.inv_mass[ga], imb = d.inv_mass[gb];
+  const double* K = d.block + m * P * P;
+  Pair s = gather(d, ga, gb);
+  double impulse[P];
+  for (int k = 0; k < count; k++) impulse[k] = d.normal_impulse[m * P + k];
+  if (count == 1 || !(K[0] > 0)) {
+    for (int k = 0; k < count; k++) {
+      const int i = m * P + k;
+      const double* jac = d.jac + J * i;
+      const double vn = (s.vbx - s.vax) * nx + (s.vby - s.vay) * ny + (s.vbz - s.vaz) * nz
+          + s.wbx * jac[3] + s.wby * jac[4] + s.wbz * jac[5] - s.wax * jac[0] - s.way * jac[1] - s.waz * jac[2];
+      const double old = impulse[k];
+      double next = old - d.normal_mass[i] * mass_scale[k] * (vn + target[k]) - impulse_scale[k] * old;
+      if (next < 0) next = 0;
+      closing_now[k] = vn; solved[k] = next;
+    }
+  } else {
+    double Kl[P * P];
+    for (int k = 0; k < count; k++)
+      for (int l = 0; l < count; l++) Kl[P * k + l] = K[P * k + l];
+    for (int k = 0; k < count; k++) {
+      const double* jac = d.jac + J * (m * P + k);
+      closing_now[k] = (s.vbx - s.vax) * nx + (s.vby - s.vay) * ny + (s.vbz - s.vaz) * nz
+          + s.wbx * jac[3] + s.wby * jac[4] + s.wbz * jac[5] - s.wax * jac[0] - s.way * jac[1] - s.waz * jac[2];
+    }
+    int set = 0;
+    for (int k = 0; k < count; k++) {
+      double r = -closing_now[k] - target[k];
+      for (int l = 0; l < count; l++) r += Kl[P * k + l] * impulse[l];
+      wanted[k] = r;
+      if (r > 0) set |= 1 << k;
+    }
+    for (int round = 0;; round++) {
+      pushes(Kl, count, set, give, wanted, solved);
+      int flip = -1;
+      for (int k = 0; k < count && flip < 0; k++) {
+        if ((set >> k) & 1) {
+          if (solved[k] < 0) flip = k;
+        } else {
+          double r = -wanted[k];
+          for (int l = 0; l < count; l++) r += Kl[P * k + l] * solved[l];
+          if (r < 0) flip = k;
+        }
+      }
+      if (flip < 0 || round == (1 << count)) break;
+      set ^= 1 << flip;
+    }
+  }
+  for (int k = 0; k < count; k++) {
+    const int i = m * P + k;
+    const double* jac = d.jac + J * i;
+    const double old = impulse[k], next = solved[k] > 0 ? solved[k] : 0;
+    const double dd = next - old;
+    impulse[k] = next;
+    d.normal_impulse[i] = next;
+    const double most = d.max_normal_impulse[i];
+    if (dd > 0 && most == 0) d.relative_velocity[i] = closing_now[k];
+    if (dd > most) d.max_normal_impulse[i] = dd;
+    const double px = nx * dd, py = ny * dd, pz = nz * dd;
+    s.vax -= ima * px; s.vay -= ima * py; s.vaz -= ima * pz;
+    s.vbx += imb * px; s.vby += imb * py; s.vbz += imb * pz;
+    s.wax -= jac[6] * dd; s.way -= jac[7] * dd; s.waz -= jac[8] * dd;
+    s.wbx += jac[9] * dd; s.wby += jac[10] * dd; s.wbz += jac[11] * dd;
+  }
+  const double mu = d.friction[m];
+  const double *Ia = d.inv_i + 6 * ga, *Ib = d.inv_i + 6 * gb;
+  const double t1x = d.t1[m3], t1y = d.t1[m3 + 1], t1z = d.t1[m3 + 2];
+  const double t2x = d.t2[m3], t2y = d.t2[m3 + 1], t2z = d.t2[m3 + 2];
+  double pressed = 0, twisted = 0;
+  for (int k = 0; k < count; k++) {
+    pressed += impulse[k];
+    twisted += d.lever[m * P + k] * impulse[k];
+  }
+  {
+    const double* f = d.fjac + FJ * m;
+    const double* g = f + 12;
+    const double dvx = s.vbx - s.vax, dvy = s.vby - s.vay, dvz = s.vbz - s.vaz;
+    double v1 = dvx * t1x + dvy * t1y + dvz * t1z
+        + s.wbx * f[3] + s.wby * f[4] + s.wbz * f[5] - s.wax * f[0] - s.way * f[1] - s.waz * f[2];
+    double v2 = dvx * t2x + dvy * t2y + dvz * t2z
+        + s.wbx * g[3] + s.wby * g[4] + s.wbz * g[5] - s.wax * g[0] - s.way * g[1] - s.waz * g[2];
+    const double s1 = d.drift1[m], s2 = d.drift2[m];
+    if (bias && (s1 != 0 || s2 != 0)) {
+      double b1 = soft.bias_rate * s1, b2 = soft.bias_rate * s2;
+      const double bb = b1 * b1 + b2 * b2;
+      if (bb > max_bias * max_bias) {
+        const double scale = max_bias / dsqrt(bb);
+        b1 *= scale; b2 *= scale;
+      }
+      v1 += b1; v2 += b2;
+    }
+    const double* tm = d.tangent_mass + m3;
+    const double o1 = d.tangent_impulse1[m], o2 = d.tangent_impulse2[m];
+    double n1 = o1 - (tm[0] * v1 + tm[1] * v2), n2 = o2 - (tm[1] * v1 + tm[2] * v2);
+    const double limit = mu * pressed, l2 = n1 * n1 + n2 * n2;
+    if (l2 > limit * limit) {
+      // Newton's method on the trust-region multiplier, as the reference does (src/contact.js:1987-2030).
+      const double ma = tm[0], mb = tm[1], mc = tm[2];
+      double al = 0, x = n1, y = n2, ll = l2;
+      while (limit > 0) {
+        const double ab = al * mb, det = (1 + al * ma) * (1 + al * mc) - ab * ab;
+        const double px = ma * x + mb * y, py = mb * x + mc * y;
+        const double wx = ((1 + al * mc) * px - ab * py) / det, wy = ((1 + al * ma) * py - ab * px) / det;
+        const double lw = x * wx + y * wy;
+        if (!(lw > 0)) break;
+        const double next = al + (dsqrt(ll) / limit - 1) * ll / lw;
+        if (!(next > al)) break;
+        al = next;
+        const double ab2 = al * mb, d2 = (1 + al * ma) * (1 + al * mc) - ab2 * ab2;
+        x = ((1 + al * mc) * n1 - ab2 * n2) / d2; y = ((1 + al * ma) * n2 - ab2 * n1) / d2;
+        ll = x * x + y * y;
+        if (!(ll > limit * limit)) break;
+      }
+      const double scale = limit / dsqrt(ll);
+      n1 = x * scale; n2 = y * scale;
+      d.sliding[m] = 1;
+    } else {
+      d.sliding[m] = 0;
+    }
+    d.tangent_impulse1[m] = n1; d.tangent_impulse2[m] = n2;
+    const double d1 = n1 - o1, d2 = n2 - o2;
+    const double px = t1x * d1 + t2x * d2, py = t1y * d1 + t2y * d2, pz = t1z * d1 + t2z * d2;
+    s.vax -= ima * px; s.vay -= ima * py; s.vaz -= ima * pz;
+    s.vbx += imb * px; s.vby += imb * py; s.vbz += imb * pz;
+    s.wax -= f[6] * d1 + g[6] * d2; s.way -= f[7] * d1 + g[7] * d2; s.waz -= f[8] * d1 + g[8] * d2;
+    s.wbx += f[9] * d1 + g[9] * d2; s.wby += f[10] * d1 + g[10] * d2; s.wbz += f[11] * d1 + g[11] * d2;
+  }
+  if (count <= 2) {
+    const double most = d.patch[m] * pressed;
+    const double ux = s.wbx - s.wax, uy = s.wby - s.way, uz = s.wbz - s.waz;
+    const double r1 = d.roll_impulse1[m], r2 = d.roll_impulse2[m];
+    double n1 = r1 - d.roll_mass1[m] * (ux * t1x + uy * t1y + uz * t1z);
+    double n2 = r2 - d.roll_mass2[m] * (ux * t2x + uy * t2y + uz * t2z);
+    const double l2 = n1 * n1 + n2 * n2;
+    if (l2 > most * most) {
+      const double scale = most / dsqrt(l2);
+      n1 *= scale; n2 *= scale;
+    }
+    d.roll_impulse1[m] = n1; d.roll_impulse2[m] = n2;
+    const double d1 = n1 - r1, d2 = n2 - r2;
+    const double lx = t1x * d1 + t2x * d2, ly = t1y * d1 + t2y * d2, lz = t1z * d1 + t2z * d2;
+    s.wax -= Ia[0] * lx + Ia[1] * ly + Ia[2] * lz;
+    s.way -= Ia[1] * lx + Ia[3] * ly + Ia[4] * lz;
+    s.waz -= Ia[2] * lx + Ia[4] * ly + Ia[5] * lz;
+    s.wbx += Ib[0] * lx + Ib[1] * ly + Ib[2] * lz;
+    s.wby += Ib[1] * lx + Ib[3] * ly + Ib[4] * lz;
+    s.wbz += Ib[2] * lx + Ib[4] * ly + Ib[5] * lz;
+  }
+  {
+    const double limit = count == 1 ? SPIN_DISC * mu * d.patch[m] * pressed : mu * twisted;
+    const double s0 = d.spin_impulse[m];
+    double s1 = s0 - d.spin_mass[m] * ((s.wbx - s.wax) * nx + (s.wby - s.way) * ny + (s.wbz - s.waz) * nz);
+    if (s1 > limit) s1 = limit; else if (s1 < -limit) s1 = -limit;
+    d.spin_impulse[m] = s1;
+    const double ds = s1 - s0, lx = nx * ds, ly = ny * ds, lz = nz * ds;
+    s.wax -= Ia[0] * lx + Ia[1] * ly + Ia[2] * lz;
+    s.way -= Ia[1] * lx + Ia[3] * ly + Ia[4] * lz;
+    s.waz -= Ia[2] * lx + Ia[4] * ly + Ia[5] * lz;
+    s.wbx += Ib[0] * lx + Ib[1] * ly + Ib[2] * lz;
+    s.wby += Ib[1] * lx + Ib[3] * ly + Ib[4] * lz;
+    s.wbz += Ib[2] * lx + Ib[4] * ly + Ib[5] * lz;
+  }
+  scatter(d, ga, gb, ima, imb, s);
+}
+
+// src/world.js:1495-1530 (_accelerate) for one awake body.
+__device__ void accelerate_body(const SoftDev& d, int g, double h, int substeps, const double* grav) {
+  if (d.type[g] != DYNAMIC || d.frozen[g] != 0) return;
+  const double gx = grav[0] * h, gy = grav[1] * h, gz = grav[2] * h;
+  const double most = SPIN_TURN / (h * substeps);
+  double *v = d.velocity + 3 * g, *w = d.angular + 3 * g;
+  const double sc = d.gravity_scale[g];
+  v[0] += gx * sc; v[1] += gy * sc; v[2] += gz * sc;
+  if (d.damping[g] != 0) {
+    const double k = 1 / (1 + h * d.damping[g]);
+    v[0] *= k; v[1] *= k; v[2] *= k;
+  }
+  if (d.angular_damping[g] != 0) {
+    const double k = 1 / (1 + h * d.angular_damping[g]);
+    w[0] *= k; w[1] *= k; w[2] *= k;
+  }
+  if (d.shape[g] != SPHERE_SHAPE || d.framed[g] == 1) {
+    const double w2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+    if (w2 > most * most) {
+      const double k = most / dsqrt(w2);
+      w[0] *= k; w[1] *= k; w[2] *= k;
+    }
+  }
+}
+
+// src/world.js:2820-2828 (advance): normalize(q + h (w, 0) q), returned.
+struct Q4 {
+  double x, y, z, w;
+};
+__device__ __forceinline__ Q4 advance(const Q4& q, double wx, double wy, double wz, double h) {
+  const double nx = q.x + h * (wx * q.w + wy * q.z - wz * q.y);
+  const double ny = q.y + h * (wy * q.w + wz * q.x - wx * q.z);
+  const double nz = q.z + h * (wz * q.w + wx * q.y - wy * q.x);
+  const double nw = q.w - h * (wx * q.x + wy * q.y + wz * q.z);
+  const double l = 1 / dsqrt(nx * nx + ny * ny + nz * nz + nw * nw);
+  return Q4{nx * l, ny * l, nz * l, nw * l};
+}
+
+// src/world.js:2831-2834 (angular): R I^-1 R^T L.
+__device__ __forceinline__ V3 angular_of(const Q4& q, const double* inv, double lx, double ly, double lz) {
+  const V3 b = rotate(q.x, q.y, q.z, q.w, lx, ly, lz, true);
+  return rotate(q.x, q.y, q.z, q.w, b.x * inv[0], b.y * inv[1], b.z * inv[2], false);
+}
+
+// src/world.js:2777-2805 (untwist)
+__device__ inline Q4 untwist(const Q4& q, const double* inv, double lx, double ly, double lz, double most) {
+  const V3 B = rotate(q.x, q.y, q.z, q.w, lx, ly, lz, true);
+  const double bx = B.x, by = B.y, bz = B.z, ll = bx * bx + by * by + bz * bz;
+  double x = bx, y = by, z = bz;
+  for (int round = 0; round < 2; round++) {
+    const double gx = x * inv[0], gy = y * inv[1], gz = z * inv[2];
+    const double over = x * gx + y * gy + z * gz - most;
+    if (!(over > 0)) break;
+    const double k = (gx * x + gy * y + gz * z) / ll, tx = gx - k * x, ty = gy - k * y, tz = gz - k * z;
+    const double tt = tx * tx + ty * ty + tz * tz;
+    if (!(tt > 0)) break;
+    const double a = 0.5 * over / tt;
+    x -= a * tx; y -= a * ty; z -= a * tz;
+    const double back = dsqrt(ll / (x * x + y * y + z * z));
+    x *= back; y *= back; z *= back;
+  }
+  const double cx = y * bz - z * by, cy = z * bx - x * bz, cz = x * by - y * bx;
+  const double cw = ll + x * bx + y * by + z * bz;
+  const double l = dsqrt(cx * cx + cy * cy + cz * cz + cw * cw);
+  if (!(l > 0)) return q;
+  const double tx = cx / l, ty = cy / l, tz = cz / l, tw = cw / l;
+  // q (x) T, src/core/math.js:231-238
+  const double ox = q.w * tx + q.x * tw + q.y * tz - q.z * ty;
+  const double oy = q.w * ty + q.y * tw + q.z * tx - q.x * tz;
+  const double oz = q.w * tz + q.z * tw + q.x * ty - q.y * tx;
+  const double ow = q.w * tw - q.x * tx - q.y * ty - q.z * tz;
+  const double n = 1 / dsqrt(ox * ox + oy * oy + oz * oz + ow * ow);
+  return Q4{ox * n, oy * n, oz * n, ow * n};
+}
+
+// src/world.js:2739-2763 (tumble): q and w of body g on by dt, carrying its angular momentum.
+__device__ void tumble(const SoftDev& d, int g, double dt) {
+  double* qp = d.rotation + 4 * g;
+  double* w = d.angular + 3 * g;
+  const double* I = d.inertia + 3 * g;
+  const double* inv = d.inv_inertia + 3 * g;
+  double* left = d.tumbled + 3 * g;
+  Q4 q{qp[0], qp[1], qp[2], qp[3]};
+  const double w0 = w[0], w1 = w[1], w2 = w[2];
+  const V3 b = rotate(q.x, q.y, q.z, q.w, w0, w1, w2, true);
+  const V3 L = rotate(q.x, q.y, q.z, q.w, b.x * I[0], b.y * I[1], b.z * I[2], false);
+  const double lx = L.x, ly = L.y, lz = L.z;
+  if (w0 != left[0] || w1 != left[1] || w2 != left[2]) d.ceiling[g] = w0 * lx + w1 * ly + w2 * lz;
+  const double most = d.ceiling[g];
+  const Q4 q4 = advance(q, w0, w1, w2, dt * 0.25);
+  V3 v = angular_of(q4, inv, lx, ly, lz);
+  q = advance(q, v.x, v.y, v.z, dt * 0.5);
+  v = angular_of(q, inv, lx, ly, lz);
+  if (v.x * lx + v.y * ly + v.z * lz > most * (1 + ROUNDED)) {
+    q = untwist(q, inv, lx, ly, lz, most);
+    v = angular_of(q, inv, lx, ly, lz);
+  }
+  qp[0] = q.x; qp[1] = q.y; qp[2] = q.z; qp[3] = q.w;
+  w[0] = v.x; w[1] = v.y; w[2] = v.z;
+  left[0] = v.x; left[1] = v.y; left[2] = v.z;
+}
+
+// src/world.js:1533-1563 (_move) for one awake body.
+__device__ void move_body(const SoftDev& d, int g, double h) {
+  if (d.frozen[g] != 0) return;
+  double *p = d.position + 3 * g, *v = d.velocity + 3 * g, *w = d.angular + 3 * g;
+  p[0] += v[0] * h; p[1] += v[1] * h; p[2] += v[2] * h;
+  const double wx = w[0], wy = w[1], wz = w[2];
+  const bool turning = wx != 0 || wy != 0 || wz != 0;
+  const double* I = d.inertia + 3 * g;
+  if (turning && d.type[g] == DYNAMIC && d.lock[g] == 0 && d.joint_head[g] == 0 && (I[0] != I[1] || I[1] != I[2])) {
+    tumble(d, g, h);
+  } else if (turning) {
+    double* qp = d.rotation + 4 * g;
+    const Q4 q = advance(Q4{qp[0], qp[1], qp[2], qp[3]}, wx, wy, wz, h * 0.5);
+    qp[0] = q.x; qp[1] = q.y; qp[2] = q.z; qp[3] = q.w;
+  }
+}
+
+// src/contact.js:2229-2251 (restitution) for one world, in the order narrow() found
+// the manifolds; one thread, since bounces on a shared body follow each other.
+__device__ void restitution_world(const SoftDev& d, int moff, int boff, int count, int solved, double gravity) {
+  for (int f = 0; f < count; f++) {
+    const int lm = d.row[moff + f];
+    const int m = moff + lm;
+    const double e = d.restitution[m];
+    if (e == 0 || lm >= solved) continue;
+    const double threshold = dsqrt(2 * gravity * d.tolerance[m]);
+    const int ga = boff + d.a[m], gb = boff + d.b[m];
+    const int m3 = 3 * m;
+    const double nx = d.normal[m3], ny = d.normal[m3 + 1], nz = d.normal[m3 + 2];
+    const double ima = d.inv_mass[ga], imb = d.inv_mass[gb];
+    for (int k = 0; k < d.points[m]; k++) {
+      const int i = m * P + k;
+      if (d.relative_velocity[i] > -threshold || d.max_normal_impulse[i] == 0) continue;
+      const double* jac = d.jac + J * i;
+      double *va = d.velocity + 3 * ga, *wa = d.angular + 3 * ga;
+      double *vb = d.velocity + 3 * gb, *wb = d.angular + 3 * gb;
+      const double vn = (vb[0] - va[0]) * nx + (vb[1] - va[1]) * ny + (vb[2] - va[2]) * nz
+          + wb[0] * jac[3] + wb[1] * jac[4] + wb[2] * jac[5] - wa[0] * jac[0] - wa[1] * jac[1] - wa[2] * jac[2];
+      const double old = d.normal_impulse[i];
+      double next = old - d.normal_mass[i] * (vn + e * d.relative_velocity[i]);
+      if (next < 0) next = 0;
+      const double l = next - old;
+      d.normal_impulse[i] = next;
+      // push(), src/contact.js:1743-1749: every component written, as the reference does.
+      const double px = nx * l, py = ny * l, pz = nz * l;
+      va[0] -= ima * px; va[1] -= ima * py; va[2] -= ima * pz;
+      vb[0] += imb * px; vb[1] += imb * py; vb[2] += imb * pz;
+      wa[0] -= jac[6] * l; wa[1] -= jac[7] * l; wa[2] -= jac[8] * l;
+      wb[0] += jac[9] * l; wb[1] += jac[10] * l; wb[2] += jac[11] * l;
+    }
+  }
+}
+
+// src/contact.js:1168-1182 (turned): body g's move over the step and its turn vector.
+__device__ inline void turned(const SoftDev& d, int g, double* T) {
+  const double *p = d.position + 3 * g, *st = d.start + 3 * g, *q = d.rotation + 4 * g, *q0 = d.start_q + 4 * g;
+  const double ax = q[0], ay = q[1], az = q[2], aw = q[3];
+  const double bx = -q0[0], by = -q0[1], bz = -q0[2], bw = q0[3];
+  double ux = aw * bx + ax * bw + ay * bz - az * by;
+  double uy = aw * by + ay * bw + az * bx - ax * bz;
+  double uz = aw * bz + az * bw + ax * by - ay * bx;
+  double uw = aw * bw - ax * bx - ay * by - az * bz;
+  if (uw < 0) { ux = -ux; uy = -uy; uz = -uz; uw = -uw; }
+  const double sine = dsqrt(ux * ux + uy * uy + uz * uz);
+  const double k = sine == 0 ? 0 : 2 * js_atan2(sine, uw) / sine;
+  T[0] = p[0] - st[0]; T[1] = p[1] - st[1]; T[2] = p[2] - st[2];
+  T[3] = ux * k; T[4] = uy * k; T[5] = uz * k;
+}
+
+// src/contact.js:1122-1153 (slipped) then 1091-1115 (keep) for row m, into pair gp.
+__device__ void store_row(const SoftDev& d, int m, int ga, int gb, int gp) {
+  const int m3 = 3 * m, count = d.points[m];
+  double pressed = 0;
+  bool impact = false;
+  for (int k = 0; k < count; k++) {
+    const int i = m * P + k, i3 = 3 * i;
+    pressed += d.normal_impulse[i];
+    if (d.shift[i3] != 0 || d.shift[i3 + 1] != 0 || d.shift[i3 + 2] != 0) impact = true;
+  }
+  const double t1x = d.t1[m3], t1y = d.t1[m3 + 1], t1z = d.t1[m3 + 2];
+  const double t2x = d.t2[m3], t2y = d.t2[m3 + 1], t2z = d.t2[m3 + 2];
+  if (d.sliding[m] != 0 || pressed == 0 || impact) {
+    d.drift1[m] = 0; d.drift2[m] = 0;
+  } else {
+    double T[6];
+    double dx = 0, dy = 0, dz = 0;
+    if (d.awake[gb] != 0) {
+      turned(d, gb, T);
+      const double rx = d.anchor_b[m3], ry = d.anchor_b[m3 + 1], rz = d.anchor_b[m3 + 2];
+      dx = T[0] + T[4] * rz - T[5] * ry;
+      dy = T[1] + T[5] * rx - T[3] * rz;
+      dz = T[2] + T[3] * ry - T[4] * rx;
+    }
+    if (d.awake[ga] != 0) {
+      turned(d, ga, T);
+      const double rx = d.anchor_a[m3], ry = d.anchor_a[m3 + 1], rz = d.anchor_a[m3 + 2];
+      dx -= T[0] + T[4] * rz - T[5] * ry;
+      dy -= T[1] + T[5] * rx - T[3] * rz;
+      dz -= T[2] + T[3] * ry - T[4] * rx;
+    }
+    d.drift1[m] += dx * t1x + dy * t1y + dz * t1z;
+    d.drift2[m] += dx * t2x + dy * t2y + dz * t2z;
+  }
+  const int p3 = 3 * gp;
+  d.pair_points[gp] = (uint8_t)count;
+  const double r1 = d.roll_impulse1[m], r2 = d.roll_impulse2[m], sp = d.spin_impulse[m];
+  d.pair_twist[p3] = t1x * r1 + t2x * r2 + d.normal[m3] * sp;
+  d.pair_twist[p3 + 1] = t1y * r1 + t2y * r2 + d.normal[m3 + 1] * sp;
+  d.pair_twist[p3 + 2] = t1z * r1 + t2z * r2 + d.normal[m3 + 2] * sp;
+  const double l1 = d.tangent_impulse1[m], l2 = d.tangent_impulse2[m];
+  d.pair_friction[p3] = t1x * l1 + t2x * l2;
+  d.pair_friction[p3 + 1] = t1y * l1 + t2y * l2;
+  d.pair_friction[p3 + 2] = t1z * l1 + t2z * l2;
+  const double d1 = d.drift1[m], d2 = d.drift2[m];
+  d.pair_slip[p3] = t1x * d1 + t2x * d2;
+  d.pair_slip[p3 + 1] = t1y * d1 + t2y * d2;
+  d.pair_slip[p3 + 2] = t1z * d1 + t2z * d2;
+  for (int k = 0; k < count; k++) {
+    d.pair_pid[P * gp + k] = d.pid[m * P + k];
+    d.pair_normal_impulse[P * gp + k] = d.normal_impulse[m * P + k];
+  }
+}
+
+// One sweep (warm start, solve or relax) over a world's colours, then its overflow in order.
+template <int KIND>
+__device__ __forceinline__ void sweep(const SoftDev& d, const Soft& soft, double h, bool known, int moff, int boff,
+                                      const int* starts) {
+  for (int c = 0; c < COLOURS; c++) {
+    const int from = starts[c], to = starts[c + 1];
+    if (from == to) continue;
+    for (int lm = from + (int)threadIdx.x; lm < to; lm += blockDim.x) {
+      const int m = moff + lm, ga = boff + d.a[m], gb = boff + d.b[m];
+      if (KIND == 0) warm_row(d, m, ga, gb);
+      else solve_row(d, soft, h, KIND == 1, known, m, ga, gb);
+    }
+    __syncthreads();
+  }
+  const int from = starts[OVERFLOW], to = starts[OVERFLOW + 1];
+  if (from < to) {
+    if (threadIdx.x == 0) {
+      for (int lm = from; lm < to; lm++) {
+        const int m = moff + lm, ga = boff + d.a[m], gb = boff + d.b[m];
+        if (KIND == 0) warm_row(d, m, ga, gb);
+        else solve_row(d, soft, h, KIND == 1, known, m, ga, gb);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// src/world.js:961-1031 for the worlds blockIdx.x, blockIdx.x + gridDim.x, ...
+__global__ void __launch_bounds__(256) soft_step_kernel(SoftDev d) {
+  __shared__ int starts[BM_STARTS_DEV];
+  for (int wld = blockIdx.x; wld < d.n_worlds; wld += gridDim.x) {
+    const int boff = d.body_offset[wld], moff = d.manifold_offset[wld], poff = d.pair_offset[wld];
+    const int aoff = d.awake_offset[wld], awake = d.awake_offset[wld + 1] - aoff;
+    const int count = d.manifold_offset[wld + 1] - moff, solved = d.solved[wld];
+    const int substeps = d.substeps[wld];
+    const double h = d.dt[wld] / substeps;
+    const double* grav = d.gravity + 3 * wld;
+    __syncthreads();
+    for (int k = threadIdx.x; k < BM_STARTS_DEV; k += blockDim.x) starts[k] = d.starts[BM_STARTS_DEV * wld + k];
+    const bool touching = count > 0;
+    for (int i = threadIdx.x; i < awake; i += blockDim.x) {
+      const int g = boff + d.awake_list[aoff + i];
+      if (touching) inertia_of(d, g);
+      for (int k = 0; k < 3; k++) d.start[3 * g + k] = d.position[3 * g + k];
+      for (int k = 0; k < 4; k++) d.start_q[4 * g + k] = d.rotation[4 * g + k];
+    }
+    __syncthreads();
+    Soft soft{0, 1, 0};
+    if (touching) {
+      for (int lm = threadIdx.x; lm < solved; lm += blockDim.x) {
+        const int m = moff + lm;
+        prepare_row(d, m, boff + d.a[m], boff + d.b[m]);
+      }
+      soft = make_soft(STIFFNESS / h, DAMPING_RATIO, h);
+      __syncthreads();
+    }
+    for (int sub = 0; sub < substeps; sub++) {
+      for (int i = threadIdx.x; i < awake; i += blockDim.x) accelerate_body(d, boff + d.awake_list[aoff + i], h, substeps, grav);
+      __syncthreads();
+      if (touching) {
+        sweep<0>(d, soft, h, false, moff, boff, starts);
+        sweep<1>(d, soft, h, sub > 0, moff, boff, starts);
+      }
+      for (int i = threadIdx.x; i < awake; i += blockDim.x) move_body(d, boff + d.awake_list[aoff + i], h);
+      __syncthreads();
+      if (touching) sweep<2>(d, soft, h, false, moff, boff, starts);
+    }
+    if (touching) {
+      if (threadIdx.x == 0) {
+        restitution_world(d, moff, boff, count, solved,
+                          dsqrt(grav[0] * grav[0] + grav[1] * grav[1] + grav[2] * grav[2]));
+      }
+      __syncthreads();
+      for (int f = threadIdx.x; f < count; f += blockDim.x) {
+        const int lm = d.row[moff + f];
+        if (lm >= solved) continue;
+        const int m = moff + lm;
+        store_row(d, m, boff + d.a[m], boff + d.b[m], poff + d.pair[m]);
+      }
+    }
+  }
+}
+
+void launch_soft_step(const SoftDev& d, int grid, int block, hipStream_t stream) {
+  hipLaunchKernelGGL(soft_step_kernel, dim3(grid), dim3(block), 0, stream, d);
+}
+
+}  // namespace bm

@@ -1,0 +1,172 @@
+"""ctypes binding of the C ABI in include/bearing_mi355x.h.
+
+``Batch`` packs one or more worlds' columns (the reference's own layout, by slot,
+manifold, point slot and pair) into the flat arrays and offset tables the library
+takes, runs the soft step on the device, and splits the results back per world.
+There is no CPU fallback: without the built library, or without a GPU, every call
+raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "libbearing_mi355x.so")
+STARTS = 35
+
+_D = ctypes.POINTER(ctypes.c_double)
+_U8 = ctypes.POINTER(ctypes.c_uint8)
+_I32 = ctypes.POINTER(ctypes.c_int32)
+_U32 = ctypes.POINTER(ctypes.c_uint32)
+_NP = {_D: np.float64, _U8: np.uint8, _I32: np.int32, _U32: np.uint32}
+
+# bm_columns after n_worlds, in header order: (field, kind, unit, numbers per unit).
+# Units: W1 (worlds + 1), W, S (35 per world), W3, B (body), A (awake), M (manifold), P (pair).
+FIELDS = [
+    ("body_offset", _I32, "W1", 1), ("manifold_offset", _I32, "W1", 1), ("pair_offset", _I32, "W1", 1),
+    ("awake_offset", _I32, "W1", 1), ("starts", _I32, "S", 1), ("solved", _I32, "W", 1), ("substeps", _I32, "W", 1),
+    ("dt", _D, "W", 1), ("gravity", _D, "W3", 1),
+    ("position", _D, "B", 3), ("rotation", _D, "B", 4), ("velocity", _D, "B", 3), ("angular", _D, "B", 3),
+    ("start", _D, "B", 3), ("start_q", _D, "B", 4), ("inv_i", _D, "B", 6), ("tumbled", _D, "B", 3),
+    ("ceiling", _D, "B", 1), ("dims", _D, "B", 3), ("inv_mass", _D, "B", 1), ("inertia", _D, "B", 3),
+    ("inv_inertia", _D, "B", 3), ("axes", _D, "B", 9), ("gravity_scale", _D, "B", 1), ("damping", _D, "B", 1),
+    ("angular_damping", _D, "B", 1), ("type", _U8, "B", 1), ("shape", _U8, "B", 1), ("frozen", _U8, "B", 1),
+    ("lock", _U8, "B", 1), ("framed", _U8, "B", 1), ("awake", _U8, "B", 1), ("joint_head", _I32, "B", 1),
+    ("awake_list", _I32, "A", 1),
+    ("a", _I32, "M", 1), ("b", _I32, "M", 1), ("pair", _I32, "M", 1), ("sub", _I32, "M", 1), ("row", _I32, "M", 1),
+    ("points", _U8, "M", 1), ("sliding", _U8, "M", 1), ("normal", _D, "M", 3), ("t1", _D, "M", 3),
+    ("t2", _D, "M", 3), ("friction", _D, "M", 1), ("restitution", _D, "M", 1), ("tolerance", _D, "M", 1),
+    ("patch", _D, "M", 1), ("roll_impulse1", _D, "M", 1), ("roll_impulse2", _D, "M", 1),
+    ("spin_impulse", _D, "M", 1), ("tangent_impulse1", _D, "M", 1), ("tangent_impulse2", _D, "M", 1),
+    ("drift1", _D, "M", 1), ("drift2", _D, "M", 1), ("separation", _D, "M", 4), ("point", _D, "M", 12),
+    ("shift", _D, "M", 12), ("normal_impulse", _D, "M", 4), ("max_normal_impulse", _D, "M", 4),
+    ("relative_velocity", _D, "M", 4), ("gap", _D, "M", 1), ("now", _D, "M", 4), ("ra", _D, "M", 12),
+    ("rb", _D, "M", 12), ("la", _D, "M", 12), ("lb", _D, "M", 12), ("adjusted", _D, "M", 4),
+    ("normal_mass", _D, "M", 4), ("jac", _D, "M", 48), ("block", _D, "M", 16), ("lever", _D, "M", 4),
+    ("anchor_a", _D, "M", 3), ("anchor_b", _D, "M", 3), ("fjac", _D, "M", 24), ("tangent_mass", _D, "M", 3),
+    ("roll_mass1", _D, "M", 1), ("roll_mass2", _D, "M", 1), ("spin_mass", _D, "M", 1), ("pid", _U32, "M", 4),
+    ("pair_points", _U8, "P", 1), ("pair_pid", _U32, "P", 4), ("pair_normal_impulse", _D, "P", 4),
+    ("pair_twist", _D, "P", 3), ("pair_friction", _D, "P", 3), ("pair_slip", _D, "P", 3),
+]
+PER_WORLD_UNITS = ("B", "A", "M", "P")
+
+
+class Columns(ctypes.Structure):
+    _fields_ = [("n_worlds", ctypes.c_int32)] + [(f, k) for f, k, _, _ in FIELDS]
+
+
+EXPORTS = ["bm_create", "bm_destroy", "bm_last_error", "bm_upload", "bm_soft_step", "bm_download",
+           "bm_soft_step_host", "bm_last_kernel_ms"]
+
+_lib = None
+
+
+def load():
+    """The built library, with its signatures set; raises if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError(f"bearing_amd: {LIB} is not built (run __graft_entry__.build())")
+        lib = ctypes.CDLL(LIB)
+        vp = ctypes.c_void_p
+        lib.bm_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int32]
+        lib.bm_destroy.argtypes = [vp]
+        lib.bm_last_error.argtypes = [vp]
+        lib.bm_last_error.restype = ctypes.c_char_p
+        for name in ("bm_upload", "bm_download", "bm_soft_step_host"):
+            getattr(lib, name).argtypes = [vp, ctypes.POINTER(Columns)]
+        lib.bm_soft_step.argtypes = [vp]
+        lib.bm_last_kernel_ms.argtypes = [vp]
+        lib.bm_last_kernel_ms.restype = ctypes.c_double
+        _lib = lib
+    return _lib
+
+
+class BearingError(RuntimeError):
+    pass
+
+
+class Context:
+    """One bm_context on a HIP device."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        self.ptr = ctypes.c_void_p()
+        rc = self.lib.bm_create(ctypes.byref(self.ptr), device)
+        if rc != 0:
+            raise BearingError(f"bm_create: no HIP device {device} (status {rc})")
+
+    def check(self, rc, what):
+        if rc != 0:
+            raise BearingError(f"{what}: status {rc}: {self.lib.bm_last_error(self.ptr).decode()}")
+
+    def close(self):
+        if self.ptr:
+            self.lib.bm_destroy(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Worlds' soft-step columns, concatenated. Each world is a dict by bm_columns field
+    name (per-body, per-manifold, per-pair arrays plus 'awake_list', 'starts', 'solved',
+    'substeps', 'dt', 'gravity')."""
+
+    def __init__(self, worlds):
+        self.worlds = worlds
+        n = len(worlds)
+        counts = {u: [] for u in PER_WORLD_UNITS}
+        for w in worlds:
+            counts["B"].append(len(w["type"]))
+            counts["A"].append(len(w["awake_list"]))
+            counts["M"].append(len(w["points"]))
+            counts["P"].append(len(w["pair_points"]))
+        self.offsets = {u: np.concatenate([[0], np.cumsum(c)]).astype(np.int32) for u, c in counts.items()}
+        self.arrays = {}
+        for f, kind, unit, per in FIELDS:
+            dt = _NP[kind]
+            if unit == "W1":
+                key = {"body_offset": "B", "manifold_offset": "M", "pair_offset": "P", "awake_offset": "A"}[f]
+                arr = self.offsets[key]
+            elif unit in ("W", "W3", "S"):
+                arr = np.concatenate([np.atleast_1d(np.asarray(w[f], dtype=dt)) for w in worlds]).astype(dt)
+            else:
+                arr = np.concatenate([np.asarray(w[f], dtype=dt).reshape(-1) for w in worlds]).astype(dt)
+                expect = int(self.offsets[unit][-1]) * per
+                if arr.size != expect:
+                    raise ValueError(f"Batch: {f} has {arr.size} numbers, expected {expect}")
+            self.arrays[f] = np.ascontiguousarray(arr)
+        self.n = n
+        self.struct = Columns()
+        self.struct.n_worlds = n
+        for f, kind, _, _ in FIELDS:
+            setattr(self.struct, f, self.arrays[f].ctypes.data_as(kind))
+
+    def world(self, i):
+        """World i's columns after the step (views into the batch arrays)."""
+        out = {}
+        for f, kind, unit, per in FIELDS:
+            if unit in PER_WORLD_UNITS:
+                o = self.offsets[unit]
+                out[f] = self.arrays[f][o[i] * per:o[i + 1] * per]
+        return out
+
+    def upload(self, ctx):
+        ctx.check(ctx.lib.bm_upload(ctx.ptr, ctypes.byref(self.struct)), "bm_upload")
+
+    def download(self, ctx):
+        ctx.check(ctx.lib.bm_download(ctx.ptr, ctypes.byref(self.struct)), "bm_download")
+
+    def step(self, ctx):
+        """bm_soft_step on the uploaded batch; returns the kernel's milliseconds."""
+        ctx.check(ctx.lib.bm_soft_step(ctx.ptr), "bm_soft_step")
+        return ctx.lib.bm_last_kernel_ms(ctx.ptr)
+
+    def step_host(self, ctx):
+        ctx.check(ctx.lib.bm_soft_step_host(ctx.ptr, ctypes.byref(self.struct)), "bm_soft_step_host")

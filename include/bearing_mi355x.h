@@ -1,0 +1,145 @@
+/*
+ * bearing_mi355x — the contact pipeline of the Bearing physics engine on one AMD
+ * Instinct MI355X (gfx950). C ABI: plain pointers and sizes, int status codes,
+ * no C++ exceptions or torch types across it.
+ *
+ * The reference (bearing-physics, JavaScript) keeps a world as typed-array
+ * columns by body slot (src/world.js:228-311), a step's manifolds as columns by
+ * manifold and by point (src/contact.js:149-258, four point slots a manifold)
+ * and the pair list's remembered impulses by pair (src/pairs.js:40-110). This
+ * interface takes exactly those columns, in exactly that layout, so an N-API
+ * binding hands the reference's own typed arrays through unchanged.
+ *
+ * A batch holds one or more independent worlds. Column arrays of a batch are
+ * the worlds' columns one after another; the offset tables say where each
+ * world's bodies, manifolds, pairs and awake list begin. Slots, manifold
+ * numbers and pair numbers inside the columns (a, b, pair, row, awake_list)
+ * are each world's own, as the reference numbers them.
+ *
+ * Threading: a context is used by one thread at a time; two contexts are
+ * independent. Host pointers are read or written only during the call that
+ * takes them and never kept.
+ */
+#ifndef BEARING_MI355X_H
+#define BEARING_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BM_OK 0
+#define BM_ERR_ARGUMENT 1   /* a size, pointer or offset table is invalid */
+#define BM_ERR_DEVICE 2     /* a HIP call failed; bm_last_error says which */
+#define BM_ERR_UNSUPPORTED 3 /* a manifold of a mesh or compound pair (sub >= 0) */
+
+#define BM_COLOURS 32            /* src/graph.js:67 */
+#define BM_STARTS (BM_COLOURS + 3) /* colour runs, overflow, resting, end: src/contact.js:160 */
+
+/* The columns of a batch of worlds, host or device side. Arrays marked [in]
+ * are read, [out] written, [io] both. Per-unit widths follow the reference. */
+typedef struct bm_columns {
+  int32_t n_worlds;
+  /* offset tables, n_worlds + 1 entries each */
+  const int32_t* body_offset;
+  const int32_t* manifold_offset;
+  const int32_t* pair_offset;
+  const int32_t* awake_offset;
+  /* per world */
+  const int32_t* starts;     /* [in] BM_STARTS per world: contacts.starts after colour() */
+  const int32_t* solved;     /* [in] contacts.solved: rows before the resting run */
+  const int32_t* substeps;   /* [in] world.substeps */
+  const double* dt;          /* [in] the step's dt */
+  const double* gravity;     /* [in] 3 per world */
+  /* bodies, by slot (src/world.js COLUMNS) */
+  double* position;          /* [io] 3 */
+  double* rotation;          /* [io] 4 */
+  double* velocity;          /* [io] 3 */
+  double* angular;           /* [io] 3 */
+  double* start;             /* [out] 3: where each awake body began the step */
+  double* start_q;           /* [out] 4 */
+  double* inv_i;             /* [io] 6: world inverse inertia, written for awake bodies */
+  double* tumbled;           /* [io] 3 */
+  double* ceiling;           /* [io] 1 */
+  const double* dims;        /* [in] 3 */
+  const double* inv_mass;    /* [in] 1 */
+  const double* inertia;     /* [in] 3 */
+  const double* inv_inertia; /* [in] 3 */
+  const double* axes;        /* [in] 9: the rotation's columns as _bounds left them */
+  const double* gravity_scale, *damping, *angular_damping; /* [in] 1 each */
+  const uint8_t* type, *shape, *frozen, *lock, *framed, *awake; /* [in] 1 each */
+  const int32_t* joint_head; /* [in] 1 */
+  const int32_t* awake_list; /* [in] awake_offset-indexed */
+  /* manifolds, by row as colour() laid them out (src/graph.js arrange) */
+  const int32_t* a, *b, *pair, *sub, *row; /* [in] 1 */
+  const uint8_t* points;     /* [in] 1 */
+  uint8_t* sliding;          /* [out] 1 */
+  const double* normal, *t1, *t2; /* [in] 3 */
+  const double* friction, *restitution, *tolerance, *patch; /* [in] 1 */
+  double* roll_impulse1, *roll_impulse2, *spin_impulse, *tangent_impulse1, *tangent_impulse2; /* [io] 1 */
+  double* drift1, *drift2;   /* [io] 1 */
+  const double* separation;  /* [in] 4 */
+  const double* point, *shift; /* [in] 12 */
+  double* normal_impulse, *max_normal_impulse, *relative_velocity; /* [io] 4 */
+  double* gap;               /* [io] 1 */
+  double* now;               /* [out] 4 */
+  double* ra, *rb, *la, *lb; /* [out] 12 */
+  double* adjusted, *normal_mass; /* [out] 4 */
+  double* jac;               /* [out] 48 */
+  double* block;             /* [out] 16 */
+  double* lever;             /* [out] 4 */
+  double* anchor_a, *anchor_b; /* [out] 3 */
+  double* fjac;              /* [out] 24 */
+  double* tangent_mass;      /* [out] 3 */
+  double* roll_mass1, *roll_mass2, *spin_mass; /* [out] 1 */
+  const uint32_t* pid;       /* [in] 4 */
+  /* what the pair list keeps, by pair (src/pairs.js) */
+  uint8_t* pair_points;      /* [out] 1 */
+  uint32_t* pair_pid;        /* [out] 4 */
+  double* pair_normal_impulse; /* [out] 4 */
+  double* pair_twist, *pair_friction, *pair_slip; /* [out] 3 */
+} bm_columns;
+
+typedef struct bm_context bm_context;
+
+/* A context on HIP device `device`, with room grown as uploads need it. */
+int bm_create(bm_context** out, int32_t device);
+int bm_destroy(bm_context* ctx);
+/* The last error's message, "caller: ..." as the reference words its errors
+ * (src/core/check.js:15-22); "" after a success. Owned by the context. */
+const char* bm_last_error(const bm_context* ctx);
+
+/* Copies a batch's columns into the context's device memory (HBM). */
+int bm_upload(bm_context* ctx, const bm_columns* host);
+
+/*
+ * The soft step on the device, on the uploaded batch: for each world what
+ * src/world.js step() runs from inertias() to store() (src/world.js:961-1031)
+ * for a world with no joints —
+ *   contact.js inertias (1302), the start poses (world.js:964-974),
+ *   prepare (1337-1445), makeSoft (1590-1597),
+ *   `substeps` times: world._accelerate (1495), warmStart (1604-1663),
+ *   solve with bias (1782-2090), world._move with tumble/advance
+ *   (1533-1563, 2739-2834), solve relaxing;
+ *   restitution (2229-2251), store (1067-1182).
+ * Colours run one after another, each colour's manifolds in parallel, the
+ * overflow in order (src/graph.js:1-60). Results are the reference's bits.
+ */
+int bm_soft_step(bm_context* ctx);
+
+/* Copies every [out]/[io] column of the uploaded batch back into `host`. */
+int bm_download(bm_context* ctx, bm_columns* host);
+
+/* bm_upload + bm_soft_step + bm_download: what a binding of world.step's
+ * contact phase calls with the world's own arrays. */
+int bm_soft_step_host(bm_context* ctx, bm_columns* host);
+
+/* Milliseconds the last bm_soft_step's kernel took on the device (HIP events). */
+double bm_last_kernel_ms(const bm_context* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BEARING_MI355X_H */

@@ -26,6 +26,8 @@ JOBS = {
     "tower": dict(scene="tower", dt=1 / 60, steps=120, soft=[1, 10, 40], colour=[10], trace=5, every=30),
     "throw": dict(scene="throw", dt=1 / 120, steps=120, soft=[30, 60, 90, 119], colour=[60], trace=70, every=30),
     "pile_small": dict(scene="pile_small", dt=1 / 60, steps=150, soft=[30, 70, 110], colour=[70], trace=100, every=50),
+    # The 2,000-sphere rain at full size: its solve phase mid-landing is the bench's workload.
+    "rain": dict(scene="rain", dt=1 / 60, steps=900, soft=[300], colour=[], trace=-1, every=300),
 }
 
 
@@ -45,5 +47,5 @@ def run(name):
 
 
 if __name__ == "__main__":
-    for name in sys.argv[1:] or list(JOBS):
+    for name in sys.argv[1:] or [j for j in JOBS if j != "rain"]:
         run(name)

@@ -92,3 +92,46 @@ def compare(fx, got):
             if e:
                 errs.append(e)
     return errs
+
+
+def snake(name):
+    out = []
+    for ch in name:
+        if ch.isupper():
+            out.append("_" + ch.lower())
+        else:
+            out.append(ch)
+    return "".join(out)
+
+
+def world_from_columns(fx, cols):
+    """A bm_columns world dict (bearing_amd.native.Batch) from oracle-style columns of a fixture."""
+    meta = fx.meta
+    w = {}
+    for key, arr in cols.items():
+        if key.startswith("b."):
+            w[snake(key[2:])] = arr
+        elif key.startswith("c."):
+            w[snake(key[2:])] = arr
+        elif key.startswith("p."):
+            w["pair_" + snake(key[2:])] = arr
+    w["awake_list"] = cols["awake_list"]
+    w["starts"] = cols["starts"]
+    w["gravity"] = cols["gravity"]
+    w["solved"] = np.int32(meta["solved"])
+    w["substeps"] = np.int32(meta["substeps"])
+    w["dt"] = np.float64(meta["dt"])
+    return w
+
+
+def columns_from_world(w):
+    """The inverse of world_from_columns for the output columns: names as compare() reads them."""
+    from oracle.binding import BODY, MANIFOLD, PAIR
+    out = {}
+    for _, _, name, _ in BODY:
+        out["b." + name] = w[snake(name)]
+    for _, _, name, _ in MANIFOLD:
+        out["c." + name] = w[snake(name)]
+    for _, _, name, _ in PAIR:
+        out["p." + name] = w["pair_" + snake(name)]
+    return out
