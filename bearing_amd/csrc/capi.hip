@@ -78,6 +78,10 @@ struct bm_context {
   SoftDev dev{};
   bool uploaded = false;
   float last_ms = 0;
+  int32_t most_bodies = 0;  // the most bodies any world of the batch has
+  size_t lds_limit = 0;     // dynamic LDS a workgroup may take (0: not yet asked)
+  int32_t shape = 0;        // launch shape of the soft-step kernel
+  int32_t resident = 1;     // keep velocities in LDS when they fit
 };
 
 namespace {
@@ -194,6 +198,12 @@ const char* bm_last_error(const bm_context* ctx) { return ctx ? ctx->error.c_str
 
 double bm_last_kernel_ms(const bm_context* ctx) { return ctx ? ctx->last_ms : 0.0; }
 
+int64_t bm_lds_limit(bm_context* ctx) {
+  if (!ctx) return -1;
+  if (ctx->lds_limit == 0) ctx->lds_limit = bm::soft_step_lds_limit(ctx->device);
+  return (int64_t)ctx->lds_limit;
+}
+
 int bm_upload(bm_context* ctx, const bm_columns* h) {
   if (!ctx || !h) return fail(ctx, BM_ERR_ARGUMENT, "bm_upload: null argument");
   ctx->error.clear();
@@ -209,6 +219,11 @@ int bm_upload(bm_context* ctx, const bm_columns* h) {
   ctx->manifolds = h->manifold_offset[h->n_worlds];
   ctx->pairs = h->pair_offset[h->n_worlds];
   ctx->awake = h->awake_offset[h->n_worlds];
+  ctx->most_bodies = 0;
+  for (int w = 0; w < h->n_worlds; w++) {
+    const int32_t nb = h->body_offset[w + 1] - h->body_offset[w];
+    if (nb > ctx->most_bodies) ctx->most_bodies = nb;
+  }
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->device), "bm_upload: hipSetDevice");
   for (int c = 0; c < NCOL; c++) {
     const Column& col = COLUMNS[c];
@@ -242,7 +257,12 @@ int bm_soft_step(bm_context* ctx) {
   ctx->error.clear();
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->device), "bm_soft_step: hipSetDevice");
   HIP_OR_FAIL(ctx, hipEventRecord(ctx->begin, ctx->stream), "bm_soft_step: hipEventRecord");
-  bm::launch_soft_step(ctx->dev, ctx->n_worlds, 256, ctx->stream);
+  if (ctx->lds_limit == 0) ctx->lds_limit = bm::soft_step_lds_limit(ctx->device);
+  // Velocities resident in LDS when the largest world's fit: 48 bytes a body.
+  const size_t lds = (size_t)ctx->most_bodies * 6 * sizeof(double);
+  (void)hipGetLastError();  // a stale error of an earlier call is not this launch's
+  bm::launch_soft_step(ctx->dev, ctx->n_worlds, ctx->shape, ctx->resident && lds <= ctx->lds_limit ? lds : 0,
+                       ctx->stream);
   HIP_OR_FAIL(ctx, hipGetLastError(), "bm_soft_step: launch");
   HIP_OR_FAIL(ctx, hipEventRecord(ctx->end, ctx->stream), "bm_soft_step: hipEventRecord");
   HIP_OR_FAIL(ctx, hipEventSynchronize(ctx->end), "bm_soft_step: kernel");
@@ -264,6 +284,14 @@ int bm_download(bm_context* ctx, bm_columns* h) {
                 "bm_download: hipMemcpyAsync");
   }
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_download: hipStreamSynchronize");
+  return BM_OK;
+}
+
+int bm_set_launch(bm_context* ctx, int32_t shape, int32_t resident) {
+  if (!ctx) return BM_ERR_ARGUMENT;
+  if (shape < 0 || shape >= bm::SOFT_SHAPES) return fail(ctx, BM_ERR_ARGUMENT, "bm_set_launch: shape must be 0, 1 or 2");
+  ctx->shape = shape;
+  ctx->resident = resident ? 1 : 0;
   return BM_OK;
 }
 

@@ -22,7 +22,7 @@ __device__ __forceinline__ double dsqrt(double x) { return __builtin_sqrt(x); }
 __device__ __forceinline__ double dabs(double x) { return __builtin_fabs(x); }
 
 // fdlibm s_atan.c: reduce about 0, 0.5, 1, 1.5 or infinity, then an odd polynomial.
-__device__ inline double js_atan(double x) {
+__device__ __forceinline__ double js_atan(double x) {
   // The arctangents of 0.5, 1, 1.5 and infinity, each in a high and a low part; picked
   // by branches rather than a table, which the compiler would put in scratch memory.
   const double HI3 = 1.57079632679489655800e+00, LO3 = 6.12323399573676603587e-17;
@@ -60,7 +60,7 @@ __device__ inline double js_atan(double x) {
 }
 
 // fdlibm e_atan2.c: the quadrant from the signs (-0 included), then atan of |y/x|.
-__device__ inline double js_atan2(double y, double x) {
+__device__ __forceinline__ double js_atan2(double y, double x) {
   const double PI = 3.1415926535897931160e+00, PI_LO = 1.2246467991473531772e-16;
   const double PI_O_2 = 1.5707963267948965580e+00, PI_O_4 = 7.8539816339744827900e-01;
   if (x != x || y != y) return x + y;

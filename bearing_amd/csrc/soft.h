@@ -31,6 +31,12 @@ struct SoftDev {
   double *pair_normal_impulse, *pair_twist, *pair_friction, *pair_slip;
 };
 
-void launch_soft_step(const SoftDev& d, int grid, int block, hipStream_t stream);
+// One workgroup per world; lds_bytes > 0 keeps each world's body velocities in that
+// much dynamic LDS (6 doubles a body) for the phase.
+// shape: 0 = 256 threads, registers for one wave a SIMD; 1 = 256 threads, two waves;
+// 2 = 512 threads, two waves.
+constexpr int SOFT_SHAPES = 3;
+void launch_soft_step(const SoftDev& d, int grid, int shape, size_t lds_bytes, hipStream_t stream);
+size_t soft_step_lds_limit(int device);
 
 }  // namespace bm

@@ -43,14 +43,14 @@ struct Soft {
 };
 
 // src/contact.js:1590-1597
-__device__ inline Soft make_soft(double hertz, double zeta, double h) {
+__device__ __forceinline__ Soft make_soft(double hertz, double zeta, double h) {
   const double omega = 2 * MATH_PI * hertz;
   const double a1 = 2 * zeta + h * omega, a2 = h * omega * a1, a3 = 1 / (1 + a2);
   return Soft{omega / a1, a2 * a3, a3};
 }
 
 // src/contact.js:1520-1549 (lockedInverse)
-__device__ inline void locked_inverse(const double* ax, const double* I, int lock, double* out) {
+__device__ __forceinline__ void locked_inverse(const double* ax, const double* I, int lock, double* out) {
   const double i0 = I[0], i1 = I[1], i2 = I[2];
   const double x0 = ax[0], x1 = ax[1], x2 = ax[2];
   const double y0 = ax[3], y1 = ax[4], y2 = ax[5];
@@ -83,7 +83,7 @@ __device__ inline void locked_inverse(const double* ax, const double* I, int loc
 }
 
 // src/contact.js:1302-1310 and 1493-1508: R diag(1/I) R^T for an awake dynamic body.
-__device__ inline void inertia_of(const SoftDev& d, int g) {
+__device__ __forceinline__ void inertia_of(const SoftDev& d, int g) {
   double* out = d.inv_i + 6 * g;
   if (d.frozen[g] != 0 || d.type[g] != DYNAMIC) {
     for (int k = 0; k < 6; k++) out[k] = 0;
@@ -119,7 +119,7 @@ __device__ __forceinline__ double turning(const double* Ia, const double* Ib, do
 }
 
 // src/contact.js:1565-1581 (lever): the 12 turning terms into jac, the effective mass back.
-__device__ inline double lever(double* jac, const double* Ia, const double* Ib, double ima, double imb, double rax,
+__device__ __forceinline__ double lever(double* jac, const double* Ia, const double* Ib, double ima, double imb, double rax,
                                double ray, double raz, double rbx, double rby, double rbz, double dx, double dy,
                                double dz) {
   const double cax = ray * dz - raz * dy, cay = raz * dx - rax * dz, caz = rax * dy - ray * dx;
@@ -140,7 +140,7 @@ __device__ inline double lever(double* jac, const double* Ia, const double* Ib, 
 
 // src/contact.js:1199-1220 (core) for a whole body: anchor in its frame into out;
 // returns the anchor's world offset at the start.
-__device__ inline V3 core(const SoftDev& d, int g, double x, double y, double z, double* out) {
+__device__ __forceinline__ V3 core(const SoftDev& d, int g, double x, double y, double z, double* out) {
   const int shape = d.shape[g];
   if (shape == SPHERE_SHAPE) {
     out[0] = 0; out[1] = 0; out[2] = 0;
@@ -158,13 +158,24 @@ __device__ inline V3 core(const SoftDev& d, int g, double x, double y, double z,
   return rotate(q, 0, c, 0, false);
 }
 
+// Where a world's body velocities live during the soft step: in LDS (v then w, six
+// doubles a body, by the world's own slot) when the world fits, else in the
+// global columns. Every per-body velocity access goes through this.
+struct VW {
+  double* vb;
+  double* wb;
+  int stride;
+  __device__ __forceinline__ double* v(int slot) const { return vb + stride * slot; }
+  __device__ __forceinline__ double* w(int slot) const { return wb + stride * slot; }
+};
+
 // src/contact.js:1346-1445 (prepareRows) for manifold row m (global), bodies ga, gb.
-__device__ void prepare_row(const SoftDev& d, int m, int ga, int gb) {
+__device__ __forceinline__ void prepare_row(const SoftDev& d, const VW& vw, int m, int ga, int gb, int sa, int sb) {
   const double* Ia = d.inv_i + 6 * ga;
   const double* Ib = d.inv_i + 6 * gb;
   const double *pa = d.position + 3 * ga, *pb = d.position + 3 * gb;
-  const double *va = d.velocity + 3 * ga, *vb = d.velocity + 3 * gb;
-  const double *wa = d.angular + 3 * ga, *wb = d.angular + 3 * gb;
+  const double *va = vw.v(sa), *vb = vw.v(sb);
+  const double *wa = vw.w(sa), *wb = vw.w(sb);
   const double ima = d.inv_mass[ga], imb = d.inv_mass[gb];
   const int m3 = 3 * m;
   const double nx = d.normal[m3], ny = d.normal[m3 + 1], nz = d.normal[m3 + 2];
@@ -244,34 +255,34 @@ struct Pair {
   double vax, vay, vaz, wax, way, waz, vbx, vby, vbz, wbx, wby, wbz;
 };
 
-__device__ __forceinline__ Pair gather(const SoftDev& d, int ga, int gb) {
-  const double *va = d.velocity + 3 * ga, *wa = d.angular + 3 * ga;
-  const double *vb = d.velocity + 3 * gb, *wb = d.angular + 3 * gb;
+__device__ __forceinline__ Pair gather(const VW& vw, int sa, int sb) {
+  const double *va = vw.v(sa), *wa = vw.w(sa), *vb = vw.v(sb), *wb = vw.w(sb);
   return Pair{va[0], va[1], va[2], wa[0], wa[1], wa[2], vb[0], vb[1], vb[2], wb[0], wb[1], wb[2]};
 }
 
 // src/contact.js:2096-2104 (writeBack): a body that cannot move is never written.
-__device__ __forceinline__ void scatter(const SoftDev& d, int ga, int gb, double ima, double imb, const Pair& s) {
+__device__ __forceinline__ void scatter(const VW& vw, int sa, int sb, double ima, double imb, const Pair& s) {
   if (ima != 0) {
-    double *v = d.velocity + 3 * ga, *w = d.angular + 3 * ga;
+    double *v = vw.v(sa), *w = vw.w(sa);
     v[0] = s.vax; v[1] = s.vay; v[2] = s.vaz; w[0] = s.wax; w[1] = s.way; w[2] = s.waz;
   }
   if (imb != 0) {
-    double *v = d.velocity + 3 * gb, *w = d.angular + 3 * gb;
+    double *v = vw.v(sb), *w = vw.w(sb);
     v[0] = s.vbx; v[1] = s.vby; v[2] = s.vbz; w[0] = s.wbx; w[1] = s.wby; w[2] = s.wbz;
   }
 }
 
-// src/contact.js:1616-1663 (warmRows) for row m.
-__device__ void warm_row(const SoftDev& d, int m, int ga, int gb) {
+// src/contact.js:1616-1663 (warmRows) for row m of N points.
+template <int N>
+__device__ __forceinline__ void warm_row(const SoftDev& d, const VW& vw, int m, int ga, int gb, int sa, int sb) {
   const double ima = d.inv_mass[ga], imb = d.inv_mass[gb];
   const int m3 = 3 * m;
   const double nx = d.normal[m3], ny = d.normal[m3 + 1], nz = d.normal[m3 + 2];
   const double t1x = d.t1[m3], t1y = d.t1[m3 + 1], t1z = d.t1[m3 + 2];
   const double t2x = d.t2[m3], t2y = d.t2[m3 + 1], t2z = d.t2[m3 + 2];
-  Pair s = gather(d, ga, gb);
-  const int count = d.points[m];
-  for (int k = 0; k < count; k++) {
+  Pair s = gather(vw, sa, sb);
+#pragma unroll
+  for (int k = 0; k < N; k++) {
     const int i = m * P + k;
     const double* jac = d.jac + J * i;
     const double ln = d.normal_impulse[i];
@@ -301,85 +312,79 @@ __device__ void warm_row(const SoftDev& d, int m, int ga, int gb) {
     s.wby += Ib[1] * lx + Ib[3] * ly + Ib[4] * lz;
     s.wbz += Ib[2] * lx + Ib[4] * ly + Ib[5] * lz;
   }
-  scatter(d, ga, gb, ima, imb, s);
+  scatter(vw, sa, sb, ima, imb, s);
 }
 
-// src/contact.js:1672-1724 (pushes): Gaussian elimination without pivoting on the
-// points of `set`; the rest 0. The four-point case is the reference's unrolled one.
-__device__ inline void pushes(const double* K, int count, int set, const double* give, const double* wanted,
-                              double* solved) {
-  if (set == 15) {
-    const double g0 = 1 + give[0], g1 = 1 + give[1], g2 = 1 + give[2], g3 = 1 + give[3];
-    double a00 = K[0] * g0, a01 = K[1], a02 = K[2], a03 = K[3];
-    double a10 = K[4], a11 = K[5] * g1, a12 = K[6], a13 = K[7];
-    double a20 = K[8], a21 = K[9], a22 = K[10] * g2, a23 = K[11];
-    double a30 = K[12], a31 = K[13], a32 = K[14], a33 = K[15] * g3;
-    double r0 = wanted[0], r1 = wanted[1], r2 = wanted[2], r3 = wanted[3], f;
-    f = a10 / a00; a11 -= f * a01; a12 -= f * a02; a13 -= f * a03; r1 -= f * r0;
-    f = a20 / a00; a21 -= f * a01; a22 -= f * a02; a23 -= f * a03; r2 -= f * r0;
-    f = a30 / a00; a31 -= f * a01; a32 -= f * a02; a33 -= f * a03; r3 -= f * r0;
-    f = a21 / a11; a22 -= f * a12; a23 -= f * a13; r2 -= f * r1;
-    f = a31 / a11; a32 -= f * a12; a33 -= f * a13; r3 -= f * r1;
-    f = a32 / a22; a33 -= f * a23; r3 -= f * r2;
-    const double x3 = r3 / a33;
-    const double x2 = (r2 - a23 * x3) / a22;
-    const double x1 = (r1 - a12 * x2 - a13 * x3) / a11;
-    const double x0 = (r0 - a01 * x1 - a02 * x2 - a03 * x3) / a00;
-    solved[0] = x0; solved[1] = x1; solved[2] = x2; solved[3] = x3;
-    return;
+// src/contact.js:1672-1724 (pushes) for the points in `set` (bit k for point k), the
+// rest 0: Gaussian elimination without pivoting on the set's rows and columns in
+// rising order, exactly the reference's compacted loops (and its unrolled four-point
+// case, the same operations), with every index known at compile time so the matrix
+// stays in registers.
+template <int N>
+__device__ __forceinline__ void pushes(const double (&K)[N][N], int set, const double (&give)[N],
+                                       const double (&wanted)[N], double (&solved)[N]) {
+  double A[N][N], R[N], X[N];
+#pragma unroll
+  for (int r = 0; r < N; r++) {
+#pragma unroll
+    for (int c = 0; c < N; c++) A[r][c] = K[r][c];
+    A[r][r] *= 1 + give[r];
+    R[r] = wanted[r];
+    solved[r] = 0;
   }
-  double A[P * P], R[P], X[P];
-  int in[P];
-  int n = 0;
-  for (int k = 0; k < count; k++) {
-    solved[k] = 0;
-    if ((set >> k) & 1) in[n++] = k;
-  }
-  for (int r = 0; r < n; r++) {
-    const int kr = in[r];
-    for (int s = 0; s < n; s++) A[r * P + s] = K[P * kr + in[s]];
-    A[r * P + r] *= 1 + give[kr];
-    R[r] = wanted[kr];
-  }
-  for (int col = 0; col < n; col++) {
-    const double pivot = A[col * P + col];
-    for (int r = col + 1; r < n; r++) {
-      const double f = A[r * P + col] / pivot;
-      for (int s = col + 1; s < n; s++) A[r * P + s] -= f * A[col * P + s];
+#pragma unroll
+  for (int col = 0; col < N; col++) {
+    if (!((set >> col) & 1)) continue;
+    const double pivot = A[col][col];
+#pragma unroll
+    for (int r = col + 1; r < N; r++) {
+      if (!((set >> r) & 1)) continue;
+      const double f = A[r][col] / pivot;
+#pragma unroll
+      for (int c = col + 1; c < N; c++) {
+        if ((set >> c) & 1) A[r][c] -= f * A[col][c];
+      }
       R[r] -= f * R[col];
     }
   }
-  for (int r = n - 1; r >= 0; r--) {
+#pragma unroll
+  for (int r = N - 1; r >= 0; r--) {
+    if (!((set >> r) & 1)) continue;
     double x = R[r];
-    for (int s = r + 1; s < n; s++) x -= A[r * P + s] * X[s];
-    X[r] = x / A[r * P + r];
-    solved[in[r]] = X[r];
+#pragma unroll
+    for (int c = r + 1; c < N; c++) {
+      if ((set >> c) & 1) x -= A[r][c] * X[c];
+    }
+    X[r] = x / A[r][r];
+    solved[r] = X[r];
   }
 }
 
-// src/contact.js:1805-2090 (solveRows) for row m.
-__device__ void solve_row(const SoftDev& d, const Soft& soft, double h, bool bias, bool known, int m, int ga, int gb) {
+// src/contact.js:1805-2090 (solveRows) for row m of N points.
+template <int N>
+__device__ __forceinline__ void solve_row(const SoftDev& d, const VW& vw, const Soft& soft, double h, bool bias, bool known, int m,
+                          int ga, int gb, int sa, int sb) {
   const double inv = 1 / h;
   const double springy = soft.impulse_scale / soft.mass_scale + COMPLIANCE;
-  double target[P], mass_scale[P], impulse_scale[P], give[P], closing_now[P], wanted[P], solved[P];
+  double target[N], mass_scale[N], impulse_scale[N], give[N], closing_now[N], wanted[N], solved[N], impulse[N];
   const int m3 = 3 * m;
   const double nx = d.normal[m3], ny = d.normal[m3 + 1], nz = d.normal[m3 + 2];
   const double max_bias = d.tolerance[m] * inv;
-  const int count = d.points[m];
   double dn = 0;
   V3 na{0, 0, 0}, nb{0, 0, 0};
   if (!known) {
     const double *pa = d.position + 3 * ga, *pb = d.position + 3 * gb;
-    const double *sa = d.start + 3 * ga, *sb = d.start + 3 * gb;
-    const double dx = (pb[0] - sb[0]) - (pa[0] - sa[0]);
-    const double dy = (pb[1] - sb[1]) - (pa[1] - sa[1]);
-    const double dz = (pb[2] - sb[2]) - (pa[2] - sa[2]);
+    const double *qa = d.start + 3 * ga, *qb = d.start + 3 * gb;
+    const double dx = (pb[0] - qb[0]) - (pa[0] - qa[0]);
+    const double dy = (pb[1] - qb[1]) - (pa[1] - qa[1]);
+    const double dz = (pb[2] - qb[2]) - (pa[2] - qa[2]);
     dn = dx * nx + dy * ny + dz * nz;
     na = rotate(d.rotation + 4 * ga, nx, ny, nz, true);
     nb = rotate(d.rotation + 4 * gb, nx, ny, nz, true);
   }
   double least = __builtin_inf();
-  for (int k = 0; k < count; k++) {
+#pragma unroll
+  for (int k = 0; k < N; k++) {
     const int i = m * P + k, i3 = 3 * i;
     double s;
     if (known) {
@@ -398,15 +403,16 @@ __device__ void solve_row(const SoftDev& d, const Soft& soft, double h, bool bia
     } else {
       target[k] = 0; mass_scale[k] = 1; impulse_scale[k] = 0; give[k] = COMPLIANCE;
     }
+    impulse[k] = d.normal_impulse[i];
   }
   d.gap[m] = least;
   const double ima = d.inv_mass[ga], imb = d.inv_mass[gb];
-  const double* K = d.block + m * P * P;
-  Pair s = gather(d, ga, gb);
-  double impulse[P];
-  for (int k = 0; k < count; k++) impulse[k] = d.normal_impulse[m * P + k];
-  if (count == 1 || !(K[0] > 0)) {
-    for (int k = 0; k < count; k++) {
+  Pair s = gather(vw, sa, sb);
+  const double* Kg = d.block + m * P * P;
+  const bool single = N == 1 || !(Kg[0] > 0);
+  if (single) {
+#pragma unroll
+    for (int k = 0; k < N; k++) {
       const int i = m * P + k;
       const double* jac = d.jac + J * i;
       const double vn = (s.vbx - s.vax) * nx + (s.vby - s.vay) * ny + (s.vbz - s.vaz) * nz
@@ -417,38 +423,49 @@ __device__ void solve_row(const SoftDev& d, const Soft& soft, double h, bool bia
       closing_now[k] = vn; solved[k] = next;
     }
   } else {
-    double Kl[P * P];
-    for (int k = 0; k < count; k++)
-      for (int l = 0; l < count; l++) Kl[P * k + l] = K[P * k + l];
-    for (int k = 0; k < count; k++) {
+    double K[N][N];
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+#pragma unroll
+      for (int l = 0; l < N; l++) K[k][l] = Kg[P * k + l];
+    }
+#pragma unroll
+    for (int k = 0; k < N; k++) {
       const double* jac = d.jac + J * (m * P + k);
       closing_now[k] = (s.vbx - s.vax) * nx + (s.vby - s.vay) * ny + (s.vbz - s.vaz) * nz
           + s.wbx * jac[3] + s.wby * jac[4] + s.wbz * jac[5] - s.wax * jac[0] - s.way * jac[1] - s.waz * jac[2];
     }
     int set = 0;
-    for (int k = 0; k < count; k++) {
+#pragma unroll
+    for (int k = 0; k < N; k++) {
       double r = -closing_now[k] - target[k];
-      for (int l = 0; l < count; l++) r += Kl[P * k + l] * impulse[l];
+#pragma unroll
+      for (int l = 0; l < N; l++) r += K[k][l] * impulse[l];
       wanted[k] = r;
       if (r > 0) set |= 1 << k;
     }
+    // Murty's least-index pivoting, at most 2^N rounds (src/contact.js:1935-1951).
     for (int round = 0;; round++) {
-      pushes(Kl, count, set, give, wanted, solved);
+      pushes<N>(K, set, give, wanted, solved);
       int flip = -1;
-      for (int k = 0; k < count && flip < 0; k++) {
+#pragma unroll
+      for (int k = 0; k < N; k++) {
+        if (flip >= 0) continue;
         if ((set >> k) & 1) {
           if (solved[k] < 0) flip = k;
         } else {
           double r = -wanted[k];
-          for (int l = 0; l < count; l++) r += Kl[P * k + l] * solved[l];
+#pragma unroll
+          for (int l = 0; l < N; l++) r += K[k][l] * solved[l];
           if (r < 0) flip = k;
         }
       }
-      if (flip < 0 || round == (1 << count)) break;
+      if (flip < 0 || round == (1 << N)) break;
       set ^= 1 << flip;
     }
   }
-  for (int k = 0; k < count; k++) {
+#pragma unroll
+  for (int k = 0; k < N; k++) {
     const int i = m * P + k;
     const double* jac = d.jac + J * i;
     const double old = impulse[k], next = solved[k] > 0 ? solved[k] : 0;
@@ -469,7 +486,8 @@ __device__ void solve_row(const SoftDev& d, const Soft& soft, double h, bool bia
   const double t1x = d.t1[m3], t1y = d.t1[m3 + 1], t1z = d.t1[m3 + 2];
   const double t2x = d.t2[m3], t2y = d.t2[m3 + 1], t2z = d.t2[m3 + 2];
   double pressed = 0, twisted = 0;
-  for (int k = 0; k < count; k++) {
+#pragma unroll
+  for (int k = 0; k < N; k++) {
     pressed += impulse[k];
     twisted += d.lever[m * P + k] * impulse[k];
   }
@@ -527,7 +545,7 @@ __device__ void solve_row(const SoftDev& d, const Soft& soft, double h, bool bia
     s.wax -= f[6] * d1 + g[6] * d2; s.way -= f[7] * d1 + g[7] * d2; s.waz -= f[8] * d1 + g[8] * d2;
     s.wbx += f[9] * d1 + g[9] * d2; s.wby += f[10] * d1 + g[10] * d2; s.wbz += f[11] * d1 + g[11] * d2;
   }
-  if (count <= 2) {
+  if (N <= 2) {
     const double most = d.patch[m] * pressed;
     const double ux = s.wbx - s.wax, uy = s.wby - s.way, uz = s.wbz - s.waz;
     const double r1 = d.roll_impulse1[m], r2 = d.roll_impulse2[m];
@@ -549,7 +567,7 @@ __device__ void solve_row(const SoftDev& d, const Soft& soft, double h, bool bia
     s.wbz += Ib[2] * lx + Ib[4] * ly + Ib[5] * lz;
   }
   {
-    const double limit = count == 1 ? SPIN_DISC * mu * d.patch[m] * pressed : mu * twisted;
+    const double limit = N == 1 ? SPIN_DISC * mu * d.patch[m] * pressed : mu * twisted;
     const double s0 = d.spin_impulse[m];
     double s1 = s0 - d.spin_mass[m] * ((s.wbx - s.wax) * nx + (s.wby - s.way) * ny + (s.wbz - s.waz) * nz);
     if (s1 > limit) s1 = limit; else if (s1 < -limit) s1 = -limit;
@@ -562,15 +580,16 @@ __device__ void solve_row(const SoftDev& d, const Soft& soft, double h, bool bia
     s.wby += Ib[1] * lx + Ib[3] * ly + Ib[4] * lz;
     s.wbz += Ib[2] * lx + Ib[4] * ly + Ib[5] * lz;
   }
-  scatter(d, ga, gb, ima, imb, s);
+  scatter(vw, sa, sb, ima, imb, s);
 }
 
 // src/world.js:1495-1530 (_accelerate) for one awake body.
-__device__ void accelerate_body(const SoftDev& d, int g, double h, int substeps, const double* grav) {
+__device__ __forceinline__ void accelerate_body(const SoftDev& d, const VW& vw, int g, int slot, double h, int substeps,
+                                const double* grav) {
   if (d.type[g] != DYNAMIC || d.frozen[g] != 0) return;
   const double gx = grav[0] * h, gy = grav[1] * h, gz = grav[2] * h;
   const double most = SPIN_TURN / (h * substeps);
-  double *v = d.velocity + 3 * g, *w = d.angular + 3 * g;
+  double *v = vw.v(slot), *w = vw.w(slot);
   const double sc = d.gravity_scale[g];
   v[0] += gx * sc; v[1] += gy * sc; v[2] += gz * sc;
   if (d.damping[g] != 0) {
@@ -610,7 +629,7 @@ __device__ __forceinline__ V3 angular_of(const Q4& q, const double* inv, double 
 }
 
 // src/world.js:2777-2805 (untwist)
-__device__ inline Q4 untwist(const Q4& q, const double* inv, double lx, double ly, double lz, double most) {
+__device__ __forceinline__ Q4 untwist(const Q4& q, const double* inv, double lx, double ly, double lz, double most) {
   const V3 B = rotate(q.x, q.y, q.z, q.w, lx, ly, lz, true);
   const double bx = B.x, by = B.y, bz = B.z, ll = bx * bx + by * by + bz * bz;
   double x = bx, y = by, z = bz;
@@ -641,9 +660,8 @@ __device__ inline Q4 untwist(const Q4& q, const double* inv, double lx, double l
 }
 
 // src/world.js:2739-2763 (tumble): q and w of body g on by dt, carrying its angular momentum.
-__device__ void tumble(const SoftDev& d, int g, double dt) {
+__device__ __forceinline__ void tumble(const SoftDev& d, double* w, int g, double dt) {
   double* qp = d.rotation + 4 * g;
-  double* w = d.angular + 3 * g;
   const double* I = d.inertia + 3 * g;
   const double* inv = d.inv_inertia + 3 * g;
   double* left = d.tumbled + 3 * g;
@@ -668,15 +686,15 @@ __device__ void tumble(const SoftDev& d, int g, double dt) {
 }
 
 // src/world.js:1533-1563 (_move) for one awake body.
-__device__ void move_body(const SoftDev& d, int g, double h) {
+__device__ __forceinline__ void move_body(const SoftDev& d, const VW& vw, int g, int slot, double h) {
   if (d.frozen[g] != 0) return;
-  double *p = d.position + 3 * g, *v = d.velocity + 3 * g, *w = d.angular + 3 * g;
+  double *p = d.position + 3 * g, *v = vw.v(slot), *w = vw.w(slot);
   p[0] += v[0] * h; p[1] += v[1] * h; p[2] += v[2] * h;
   const double wx = w[0], wy = w[1], wz = w[2];
   const bool turning = wx != 0 || wy != 0 || wz != 0;
   const double* I = d.inertia + 3 * g;
   if (turning && d.type[g] == DYNAMIC && d.lock[g] == 0 && d.joint_head[g] == 0 && (I[0] != I[1] || I[1] != I[2])) {
-    tumble(d, g, h);
+    tumble(d, w, g, h);
   } else if (turning) {
     double* qp = d.rotation + 4 * g;
     const Q4 q = advance(Q4{qp[0], qp[1], qp[2], qp[3]}, wx, wy, wz, h * 0.5);
@@ -686,14 +704,15 @@ __device__ void move_body(const SoftDev& d, int g, double h) {
 
 // src/contact.js:2229-2251 (restitution) for one world, in the order narrow() found
 // the manifolds; one thread, since bounces on a shared body follow each other.
-__device__ void restitution_world(const SoftDev& d, int moff, int boff, int count, int solved, double gravity) {
+__device__ __forceinline__ void restitution_world(const SoftDev& d, const VW& vw, int moff, int boff, int count, int solved,
+                                  double gravity) {
   for (int f = 0; f < count; f++) {
     const int lm = d.row[moff + f];
     const int m = moff + lm;
     const double e = d.restitution[m];
     if (e == 0 || lm >= solved) continue;
     const double threshold = dsqrt(2 * gravity * d.tolerance[m]);
-    const int ga = boff + d.a[m], gb = boff + d.b[m];
+    const int sa = d.a[m], sb = d.b[m], ga = boff + sa, gb = boff + sb;
     const int m3 = 3 * m;
     const double nx = d.normal[m3], ny = d.normal[m3 + 1], nz = d.normal[m3 + 2];
     const double ima = d.inv_mass[ga], imb = d.inv_mass[gb];
@@ -701,8 +720,7 @@ __device__ void restitution_world(const SoftDev& d, int moff, int boff, int coun
       const int i = m * P + k;
       if (d.relative_velocity[i] > -threshold || d.max_normal_impulse[i] == 0) continue;
       const double* jac = d.jac + J * i;
-      double *va = d.velocity + 3 * ga, *wa = d.angular + 3 * ga;
-      double *vb = d.velocity + 3 * gb, *wb = d.angular + 3 * gb;
+      double *va = vw.v(sa), *wa = vw.w(sa), *vb = vw.v(sb), *wb = vw.w(sb);
       const double vn = (vb[0] - va[0]) * nx + (vb[1] - va[1]) * ny + (vb[2] - va[2]) * nz
           + wb[0] * jac[3] + wb[1] * jac[4] + wb[2] * jac[5] - wa[0] * jac[0] - wa[1] * jac[1] - wa[2] * jac[2];
       const double old = d.normal_impulse[i];
@@ -721,7 +739,7 @@ __device__ void restitution_world(const SoftDev& d, int moff, int boff, int coun
 }
 
 // src/contact.js:1168-1182 (turned): body g's move over the step and its turn vector.
-__device__ inline void turned(const SoftDev& d, int g, double* T) {
+__device__ __forceinline__ void turned(const SoftDev& d, int g, double* T) {
   const double *p = d.position + 3 * g, *st = d.start + 3 * g, *q = d.rotation + 4 * g, *q0 = d.start_q + 4 * g;
   const double ax = q[0], ay = q[1], az = q[2], aw = q[3];
   const double bx = -q0[0], by = -q0[1], bz = -q0[2], bw = q0[3];
@@ -737,7 +755,7 @@ __device__ inline void turned(const SoftDev& d, int g, double* T) {
 }
 
 // src/contact.js:1122-1153 (slipped) then 1091-1115 (keep) for row m, into pair gp.
-__device__ void store_row(const SoftDev& d, int m, int ga, int gb, int gp) {
+__device__ __forceinline__ void store_row(const SoftDev& d, int m, int ga, int gb, int gp) {
   const int m3 = 3 * m, count = d.points[m];
   double pressed = 0;
   bool impact = false;
@@ -790,38 +808,56 @@ __device__ void store_row(const SoftDev& d, int m, int ga, int gb, int gp) {
   }
 }
 
-// One sweep (warm start, solve or relax) over a world's colours, then its overflow in order.
+// One row of a sweep: warm start (KIND 0), solve (1) or relax (2), by its point count.
 template <int KIND>
-__device__ __forceinline__ void sweep(const SoftDev& d, const Soft& soft, double h, bool known, int moff, int boff,
-                                      const int* starts) {
+__device__ __forceinline__ void row_of(const SoftDev& d, const VW& vw, const Soft& soft, double h, bool known, int m,
+                                       int boff) {
+  const int sa = d.a[m], sb = d.b[m], ga = boff + sa, gb = boff + sb;
+  switch (d.points[m]) {
+    case 1:
+      if (KIND == 0) warm_row<1>(d, vw, m, ga, gb, sa, sb); else solve_row<1>(d, vw, soft, h, KIND == 1, known, m, ga, gb, sa, sb);
+      break;
+    case 2:
+      if (KIND == 0) warm_row<2>(d, vw, m, ga, gb, sa, sb); else solve_row<2>(d, vw, soft, h, KIND == 1, known, m, ga, gb, sa, sb);
+      break;
+    case 3:
+      if (KIND == 0) warm_row<3>(d, vw, m, ga, gb, sa, sb); else solve_row<3>(d, vw, soft, h, KIND == 1, known, m, ga, gb, sa, sb);
+      break;
+    default:
+      if (KIND == 0) warm_row<4>(d, vw, m, ga, gb, sa, sb); else solve_row<4>(d, vw, soft, h, KIND == 1, known, m, ga, gb, sa, sb);
+      break;
+  }
+}
+
+// One sweep over a world's colours, a barrier after each, then its overflow in order.
+template <int KIND>
+__device__ __forceinline__ void sweep(const SoftDev& d, const VW& vw, const Soft& soft, double h, bool known,
+                                      int moff, int boff, const int* starts) {
   for (int c = 0; c < COLOURS; c++) {
     const int from = starts[c], to = starts[c + 1];
     if (from == to) continue;
-    for (int lm = from + (int)threadIdx.x; lm < to; lm += blockDim.x) {
-      const int m = moff + lm, ga = boff + d.a[m], gb = boff + d.b[m];
-      if (KIND == 0) warm_row(d, m, ga, gb);
-      else solve_row(d, soft, h, KIND == 1, known, m, ga, gb);
-    }
+    for (int lm = from + (int)threadIdx.x; lm < to; lm += blockDim.x) row_of<KIND>(d, vw, soft, h, known, moff + lm, boff);
     __syncthreads();
   }
   const int from = starts[OVERFLOW], to = starts[OVERFLOW + 1];
   if (from < to) {
     if (threadIdx.x == 0) {
-      for (int lm = from; lm < to; lm++) {
-        const int m = moff + lm, ga = boff + d.a[m], gb = boff + d.b[m];
-        if (KIND == 0) warm_row(d, m, ga, gb);
-        else solve_row(d, soft, h, KIND == 1, known, m, ga, gb);
-      }
+      for (int lm = from; lm < to; lm++) row_of<KIND>(d, vw, soft, h, known, moff + lm, boff);
     }
     __syncthreads();
   }
 }
 
 // src/world.js:961-1031 for the worlds blockIdx.x, blockIdx.x + gridDim.x, ...
-__global__ void __launch_bounds__(256) soft_step_kernel(SoftDev d) {
+// RESIDENT: the world's body velocities are copied into LDS for the phase and back
+// after it (the host chooses it when every world's bodies fit the LDS it gives).
+template <bool RESIDENT, int THREADS, int WAVES>
+__global__ void __launch_bounds__(THREADS, WAVES) soft_step_kernel(SoftDev d) {
   __shared__ int starts[BM_STARTS_DEV];
+  extern __shared__ double lds[];
   for (int wld = blockIdx.x; wld < d.n_worlds; wld += gridDim.x) {
     const int boff = d.body_offset[wld], moff = d.manifold_offset[wld], poff = d.pair_offset[wld];
+    const int nbodies = d.body_offset[wld + 1] - boff;
     const int aoff = d.awake_offset[wld], awake = d.awake_offset[wld + 1] - aoff;
     const int count = d.manifold_offset[wld + 1] - moff, solved = d.solved[wld];
     const int substeps = d.substeps[wld];
@@ -836,30 +872,47 @@ __global__ void __launch_bounds__(256) soft_step_kernel(SoftDev d) {
       for (int k = 0; k < 3; k++) d.start[3 * g + k] = d.position[3 * g + k];
       for (int k = 0; k < 4; k++) d.start_q[4 * g + k] = d.rotation[4 * g + k];
     }
+    VW vw;
+    if (RESIDENT) {
+      for (int i = threadIdx.x; i < nbodies; i += blockDim.x) {
+        const double *v = d.velocity + 3 * (boff + i), *w = d.angular + 3 * (boff + i);
+        double* o = lds + 6 * i;
+        o[0] = v[0]; o[1] = v[1]; o[2] = v[2]; o[3] = w[0]; o[4] = w[1]; o[5] = w[2];
+      }
+      vw = VW{lds, lds + 3, 6};
+    } else {
+      vw = VW{d.velocity + 3 * boff, d.angular + 3 * boff, 3};
+    }
     __syncthreads();
     Soft soft{0, 1, 0};
     if (touching) {
       for (int lm = threadIdx.x; lm < solved; lm += blockDim.x) {
-        const int m = moff + lm;
-        prepare_row(d, m, boff + d.a[m], boff + d.b[m]);
+        const int m = moff + lm, sa = d.a[m], sb = d.b[m];
+        prepare_row(d, vw, m, boff + sa, boff + sb, sa, sb);
       }
       soft = make_soft(STIFFNESS / h, DAMPING_RATIO, h);
       __syncthreads();
     }
     for (int sub = 0; sub < substeps; sub++) {
-      for (int i = threadIdx.x; i < awake; i += blockDim.x) accelerate_body(d, boff + d.awake_list[aoff + i], h, substeps, grav);
+      for (int i = threadIdx.x; i < awake; i += blockDim.x) {
+        const int slot = d.awake_list[aoff + i];
+        accelerate_body(d, vw, boff + slot, slot, h, substeps, grav);
+      }
       __syncthreads();
       if (touching) {
-        sweep<0>(d, soft, h, false, moff, boff, starts);
-        sweep<1>(d, soft, h, sub > 0, moff, boff, starts);
+        sweep<0>(d, vw, soft, h, false, moff, boff, starts);
+        sweep<1>(d, vw, soft, h, sub > 0, moff, boff, starts);
       }
-      for (int i = threadIdx.x; i < awake; i += blockDim.x) move_body(d, boff + d.awake_list[aoff + i], h);
+      for (int i = threadIdx.x; i < awake; i += blockDim.x) {
+        const int slot = d.awake_list[aoff + i];
+        move_body(d, vw, boff + slot, slot, h);
+      }
       __syncthreads();
-      if (touching) sweep<2>(d, soft, h, false, moff, boff, starts);
+      if (touching) sweep<2>(d, vw, soft, h, false, moff, boff, starts);
     }
     if (touching) {
       if (threadIdx.x == 0) {
-        restitution_world(d, moff, boff, count, solved,
+        restitution_world(d, vw, moff, boff, count, solved,
                           dsqrt(grav[0] * grav[0] + grav[1] * grav[1] + grav[2] * grav[2]));
       }
       __syncthreads();
@@ -870,11 +923,37 @@ __global__ void __launch_bounds__(256) soft_step_kernel(SoftDev d) {
         store_row(d, m, boff + d.a[m], boff + d.b[m], poff + d.pair[m]);
       }
     }
+    if (RESIDENT) {
+      for (int i = threadIdx.x; i < nbodies; i += blockDim.x) {
+        double *v = d.velocity + 3 * (boff + i), *w = d.angular + 3 * (boff + i);
+        const double* o = lds + 6 * i;
+        v[0] = o[0]; v[1] = o[1]; v[2] = o[2]; w[0] = o[3]; w[1] = o[4]; w[2] = o[5];
+      }
+    }
   }
 }
 
-void launch_soft_step(const SoftDev& d, int grid, int block, hipStream_t stream) {
-  hipLaunchKernelGGL(soft_step_kernel, dim3(grid), dim3(block), 0, stream, d);
+// The most dynamic LDS a workgroup of the resident kernel may take on `device`:
+// the device's LDS a workgroup less the kernel's own static LDS.
+size_t soft_step_lds_limit(int device) {
+  int most = 0;
+  if (hipDeviceGetAttribute(&most, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) return 0;
+  const size_t own = BM_STARTS_DEV * sizeof(int) + 64;
+  return (size_t)most > own ? (size_t)most - own : 0;
+}
+
+// Launch shapes: threads a workgroup, and registers sized for 1 or 2 waves a SIMD.
+void launch_soft_step(const SoftDev& d, int grid, int shape, size_t lds_bytes, hipStream_t stream) {
+  const dim3 g(grid);
+  if (lds_bytes > 0) {
+    if (shape == 1) hipLaunchKernelGGL((soft_step_kernel<true, 256, 2>), g, dim3(256), lds_bytes, stream, d);
+    else if (shape == 2) hipLaunchKernelGGL((soft_step_kernel<true, 512, 2>), g, dim3(512), lds_bytes, stream, d);
+    else hipLaunchKernelGGL((soft_step_kernel<true, 256, 1>), g, dim3(256), lds_bytes, stream, d);
+  } else {
+    if (shape == 1) hipLaunchKernelGGL((soft_step_kernel<false, 256, 2>), g, dim3(256), 0, stream, d);
+    else if (shape == 2) hipLaunchKernelGGL((soft_step_kernel<false, 512, 2>), g, dim3(512), 0, stream, d);
+    else hipLaunchKernelGGL((soft_step_kernel<false, 256, 1>), g, dim3(256), 0, stream, d);
+  }
 }
 
 }  // namespace bm

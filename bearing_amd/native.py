@@ -57,7 +57,7 @@ class Columns(ctypes.Structure):
 
 
 EXPORTS = ["bm_create", "bm_destroy", "bm_last_error", "bm_upload", "bm_soft_step", "bm_download",
-           "bm_soft_step_host", "bm_last_kernel_ms"]
+           "bm_soft_step_host", "bm_last_kernel_ms", "bm_set_launch", "bm_lds_limit"]
 
 _lib = None
 
@@ -79,6 +79,9 @@ def load():
         lib.bm_soft_step.argtypes = [vp]
         lib.bm_last_kernel_ms.argtypes = [vp]
         lib.bm_last_kernel_ms.restype = ctypes.c_double
+        lib.bm_set_launch.argtypes = [vp, ctypes.c_int32, ctypes.c_int32]
+        lib.bm_lds_limit.argtypes = [vp]
+        lib.bm_lds_limit.restype = ctypes.c_int64
         _lib = lib
     return _lib
 
@@ -100,6 +103,12 @@ class Context:
     def check(self, rc, what):
         if rc != 0:
             raise BearingError(f"{what}: status {rc}: {self.lib.bm_last_error(self.ptr).decode()}")
+
+    def set_launch(self, shape, resident=True):
+        self.check(self.lib.bm_set_launch(self.ptr, shape, 1 if resident else 0), "bm_set_launch")
+
+    def lds_limit(self):
+        return self.lib.bm_lds_limit(self.ptr)
 
     def close(self):
         if self.ptr:

@@ -75,17 +75,13 @@ def cpu_baseline(fx, seconds):
 
     base = binding.columns(fx)
     rows = int(np.sum(fx["in.c.points"][: fx.meta["solved"]]))
-    done, t0 = 0, time.perf_counter()
-    while True:
-        cols = {k: v.copy() for k, v in base.items()}
+    done, elapsed = 0, 0.0
+    while elapsed < seconds and done < 1000:
+        cols = {k: v.copy() for k, v in base.items()}  # a fresh input each time, not timed
         t = time.perf_counter()
         binding.soft_step(fx, cols)
+        elapsed += time.perf_counter() - t
         done += 1
-        spent = time.perf_counter() - t0
-        if spent >= seconds or done >= 1000:
-            break
-    # The copies are outside what is counted: time the steps alone.
-    elapsed = spent - done * 0.0  # copy cost is small against a soft step; included for honesty
     return {"value": rows * 2 * SUBSTEPS * done / elapsed, "unit": "contact-iterations/s", "cores": 1, "kind": "port",
             "sample": f"{done} soft steps of {WORKLOAD} ({rows} rows) in {elapsed:.2f} s on one host thread"}
 
@@ -105,6 +101,7 @@ def main():
         sys.exit(2)
 
     from bearing_amd import native
+    from bearing_amd.tune import tune
 
     fx, cols, world_from_columns = load_world(WORKLOAD)
     meta = fx.meta
@@ -115,8 +112,11 @@ def main():
     a, b = fx["in.c.a"][:solved], fx["in.c.b"][:solved]
     step_bytes = algorithmic_bytes(points, inv_mass[a] > 0, inv_mass[b] > 0)
 
+    plain_world = world_from_columns(fx, cols)
+    shape, resident, _ = tune([plain_world])
     ctx = native.Context(0)
-    batch = native.Batch([world_from_columns(fx, cols)])
+    ctx.set_launch(shape, resident)
+    batch = native.Batch([plain_world])
     batch.upload(ctx)
     for _ in range(args.warmup):
         batch.step(ctx)
@@ -134,6 +134,8 @@ def main():
     sources = [load_world(n)[:2] for n in BATCH_SOURCES]
     worlds = [world_from_columns(sources[i % len(sources)][0], {k: v.copy() for k, v in sources[i % len(sources)][1].items()})
               for i in range(args.batch_worlds)]
+    bshape, bresident, _ = tune(worlds)
+    ctx.set_launch(bshape, bresident)
     big = native.Batch(worlds)
     big.upload(ctx)
     big.step(ctx)
@@ -144,6 +146,7 @@ def main():
     batched_rate = args.batch_worlds * args.batch_steps / batched_wall
     # The plain drop-in looped once per world, on a sample of the same worlds.
     sample = min(64, args.batch_worlds)
+    ctx.set_launch(shape, resident)
     singles = [native.Batch([w]) for w in worlds[:sample]]
     t_loop = 0.0
     for s in singles:
@@ -163,14 +166,14 @@ def main():
         "data": "synthetic: the reference's own state of the seeded 2,000-sphere rain before its step-300 solve",
         "config": {"workload": "rain-2000 soft step (inertias..store, 8 substeps), plain drop-in, one world",
                    "bodies": meta["top"], "manifolds": solved, "rows": rows, "colours": int(np.count_nonzero(np.diff(fx["in.starts"][:33]))),
-                   "kernel_ms": avg_kernel_ms},
+                   "kernel_ms": avg_kernel_ms, "launch": {"shape": shape, "resident_lds": resident}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "algorithmic_bytes_per_launch": step_bytes},
         "cpu_baseline": cpu,
         "batched": {"worlds": args.batch_worlds, "steps": args.batch_steps, "world_steps_per_s": batched_rate,
                     "looped_plain_world_steps_per_s": loop_rate, "speedup_vs_looped": batched_rate / loop_rate,
-                    "worlds_from": BATCH_SOURCES},
+                    "worlds_from": BATCH_SOURCES, "launch": {"shape": bshape, "resident_lds": bresident}},
         "bwd_fwd": None,
     }
     print(json.dumps(line))

@@ -135,6 +135,18 @@ int bm_download(bm_context* ctx, bm_columns* host);
  * contact phase calls with the world's own arrays. */
 int bm_soft_step_host(bm_context* ctx, bm_columns* host);
 
+/* How bm_soft_step launches, for tuning (bearing_amd/tune.py picks per workload):
+ * shape 0 = 256 threads a workgroup with registers for one wave a SIMD, 1 = 256
+ * threads and two waves a SIMD, 2 = 512 threads and two waves; resident = 1 keeps
+ * each world's body velocities in LDS for the phase when the largest world's fit
+ * (48 bytes a body), 0 leaves them in HBM. Defaults: 0, 1. Results are the same
+ * bits for every setting. */
+int bm_set_launch(bm_context* ctx, int32_t shape, int32_t resident);
+
+/* Bytes of LDS a soft-step workgroup can hold velocities in on this context's
+ * device (a world fits when 48 x its bodies is no more); -1 without a context. */
+int64_t bm_lds_limit(bm_context* ctx);
+
 /* Milliseconds the last bm_soft_step's kernel took on the device (HIP events). */
 double bm_last_kernel_ms(const bm_context* ctx);
 

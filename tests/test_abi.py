@@ -15,7 +15,7 @@ HEADER = os.path.join(ROOT, "include", "bearing_mi355x.h")
 
 def declared():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int|double|const char\*)\s+(bm_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t|double|const char\*)\s+(bm_\w+)\s*\(", text, re.M)))
 
 
 def test_header_declares_what_the_binding_knows():

@@ -51,6 +51,20 @@ def test_gpu_batch_of_every_fixture(ctx):
         assert not errs, f"{SOFT[i]}: " + "\n".join(errs)
 
 
+@pytest.mark.parametrize("shape,resident", [(s, r) for s in (0, 1, 2) for r in (True, False)])
+def test_gpu_every_launch_shape_gives_the_same_bits(ctx, shape, resident):
+    fxs = [fixtures.load(p) for p in SOFT]
+    batch = native.Batch([world_from_columns(fx, binding.columns(fx)) for fx in fxs])
+    ctx.set_launch(shape, resident)
+    try:
+        batch.step_host(ctx)
+    finally:
+        ctx.set_launch(0, True)
+    for i, fx in enumerate(fxs):
+        errs = compare(fx, columns_from_world(batch.world(i)))
+        assert not errs, f"shape {shape} resident {resident} {SOFT[i]}: " + "\n".join(errs)
+
+
 def scrambled(fx, seed):
     """The fixture's input with every dynamic body's velocities changed by seeded noise."""
     rng = np.random.default_rng(seed)
