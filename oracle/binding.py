@@ -72,6 +72,25 @@ def lib():
     return _lib
 
 
+def colour(a, b, top, type_, frozen, frozen_count):
+    """oracle_colour: (colour, order, row, starts, solved) for manifolds a[m]-b[m]."""
+    lib_ = lib()
+    n = len(a)
+    a = np.ascontiguousarray(a, np.int32)
+    b = np.ascontiguousarray(b, np.int32)
+    type_ = np.ascontiguousarray(type_, np.uint8)
+    frozen = np.ascontiguousarray(frozen, np.uint8)
+    col, order, row = (np.zeros(max(n, 1), np.int32) for _ in range(3))
+    starts = np.zeros(35, np.int32)
+    solved = ctypes.c_int32()
+    f = lib_.oracle_colour
+    f.restype = ctypes.c_int
+    f(n, a.ctypes.data_as(_I32), b.ctypes.data_as(_I32), top, type_.ctypes.data_as(_U8), frozen.ctypes.data_as(_U8),
+      frozen_count, col.ctypes.data_as(_I32), order.ctypes.data_as(_I32), row.ctypes.data_as(_I32),
+      starts.ctypes.data_as(_I32), ctypes.byref(solved))
+    return col[:n], order[:n], row[:n], starts, solved.value
+
+
 def columns(fx):
     """The soft step's columns from a fixture's 'in.*' arrays, as fresh numpy arrays by
     fixture column name: bodies as ``b.<name>``, manifolds ``c.<name>``, pairs ``p.<name>``."""

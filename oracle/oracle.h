@@ -37,6 +37,14 @@ typedef struct OracleSoft {
    a compound's, sub >= 0). */
 int oracle_soft_step(OracleSoft* w);
 
+/* src/graph.js:105-188: each manifold's colour (COLOURS = 32, 32 = overflow, 33 =
+   resting), the rows' order (row o holds manifold order[o]), the way back (row), the
+   35 run starts, and solved (the resting run's start). count manifolds between slots
+   A[m] and B[m]; top slots in use. */
+int oracle_colour(int32_t count, const int32_t* A, const int32_t* B, int32_t top, const uint8_t* type,
+                  const uint8_t* frozen, int32_t frozen_count, int32_t* colour, int32_t* order, int32_t* row,
+                  int32_t* starts, int32_t* solved);
+
 #ifdef __cplusplus
 }
 #endif

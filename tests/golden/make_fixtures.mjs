@@ -147,18 +147,21 @@ function main() {
     }
   };
   const hashes = [], every = job.every || 0, kept = {};
-  const counts = { awake: new Int32Array(job.steps), contacts: new Int32Array(job.steps) };
+  const counts = { awake: new Int32Array(job.steps), contacts: new Int32Array(job.steps),
+    frozen: new Int32Array(job.steps) };
   for (let s = 1; s <= job.steps; s++) {
     world.step(job.dt);
     const t = world._transforms.subarray(0, 7 * world._slots.top);
     hashes.push(createHash('sha256').update(Buffer.from(t.buffer, t.byteOffset, t.byteLength)).digest('hex'));
     counts.awake[s - 1] = world._awakeCount;
     counts.contacts[s - 1] = world._contacts.count;
+    counts.frozen[s - 1] = world._frozenCount;
     if (every > 0 && s % every === 0) kept[`transforms.${s}`] = t.slice();
   }
   const final = {};
   bodies(world, ['_position', '_rotation', '_velocity', '_angular'], 'final.', final);
-  writeFileSync(`${job.out}.traj.bmfx.gz`, pack(Object.assign({ 'awake': counts.awake, 'contacts': counts.contacts },
+  writeFileSync(`${job.out}.traj.bmfx.gz`, pack(Object.assign({ 'awake': counts.awake, 'contacts': counts.contacts,
+    'frozen': counts.frozen },
     kept, final), { steps: job.steps, dt: job.dt, hashes, bodies: scene.length, impacts: world._impacts,
     catches: world._catches, solverSteps: world._solverSteps }));
 }

@@ -21,9 +21,10 @@ DATA = os.path.join(HERE, "data")
 
 # name: scene, dt, steps, soft-phase steps, colour steps, trace step, transforms every
 JOBS = {
-    "pyramid": dict(scene="pyramid", dt=1 / 60, steps=600, soft=[1, 2, 30, 55], colour=[1, 30], trace=3, every=100),
+    "pyramid": dict(scene="pyramid", dt=1 / 60, steps=600, soft=[1, 2, 30, 55, 63], colour=[1, 30, 62, 66], trace=3,
+                    every=100),
     "rain_small": dict(scene="rain_small", dt=1 / 60, steps=240, soft=[20, 40, 60, 90], colour=[60], trace=45, every=60),
-    "tower": dict(scene="tower", dt=1 / 60, steps=120, soft=[1, 10, 40], colour=[10], trace=5, every=30),
+    "tower": dict(scene="tower", dt=1 / 60, steps=120, soft=[1, 10, 40, 64], colour=[10, 63], trace=5, every=30),
     "throw": dict(scene="throw", dt=1 / 120, steps=120, soft=[30, 60, 90, 119], colour=[60], trace=70, every=30),
     "pile_small": dict(scene="pile_small", dt=1 / 60, steps=150, soft=[30, 70, 110], colour=[70], trace=100, every=50),
     # The 2,000-sphere rain at full size: its solve phase mid-landing is the bench's workload.
