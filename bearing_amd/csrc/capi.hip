@@ -6,9 +6,11 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/bearing_mi355x.h"
+#include "colour.h"
 #include "soft.h"
 
 using bm::SoftDev;
@@ -31,7 +33,7 @@ struct Column {
 #define COL(n, e, p, u, d) {#n, offsetof(bm_columns, n), offsetof(SoftDev, n), e, p, u, d}
 const Column COLUMNS[] = {
     COL(body_offset, 4, 1, WORLD1, IN), COL(manifold_offset, 4, 1, WORLD1, IN), COL(pair_offset, 4, 1, WORLD1, IN),
-    COL(awake_offset, 4, 1, WORLD1, IN), COL(starts, 4, 1, STARTS, IN), COL(solved, 4, 1, WORLD, IN),
+    COL(awake_offset, 4, 1, WORLD1, IN), COL(starts, 4, 1, STARTS, IO), COL(solved, 4, 1, WORLD, IO),
     COL(substeps, 4, 1, WORLD, IN), COL(dt, 8, 1, WORLD, IN), COL(gravity, 8, 1, WORLD3, IN),
     COL(position, 8, 3, BODY, IO), COL(rotation, 8, 4, BODY, IO), COL(velocity, 8, 3, BODY, IO),
     COL(angular, 8, 3, BODY, IO), COL(start, 8, 3, BODY, IO), COL(start_q, 8, 4, BODY, IO),
@@ -41,15 +43,16 @@ const Column COLUMNS[] = {
     COL(damping, 8, 1, BODY, IN), COL(angular_damping, 8, 1, BODY, IN), COL(type, 1, 1, BODY, IN),
     COL(shape, 1, 1, BODY, IN), COL(frozen, 1, 1, BODY, IN), COL(lock, 1, 1, BODY, IN), COL(framed, 1, 1, BODY, IN),
     COL(awake, 1, 1, BODY, IN), COL(joint_head, 4, 1, BODY, IN), COL(awake_list, 4, 1, AWAKE, IN),
-    COL(a, 4, 1, MANIFOLD, IN), COL(b, 4, 1, MANIFOLD, IN), COL(pair, 4, 1, MANIFOLD, IN),
-    COL(sub, 4, 1, MANIFOLD, IN), COL(row, 4, 1, MANIFOLD, IN), COL(points, 1, 1, MANIFOLD, IN),
-    COL(sliding, 1, 1, MANIFOLD, IO), COL(normal, 8, 3, MANIFOLD, IN), COL(t1, 8, 3, MANIFOLD, IN),
-    COL(t2, 8, 3, MANIFOLD, IN), COL(friction, 8, 1, MANIFOLD, IN), COL(restitution, 8, 1, MANIFOLD, IN),
-    COL(tolerance, 8, 1, MANIFOLD, IN), COL(patch, 8, 1, MANIFOLD, IN), COL(roll_impulse1, 8, 1, MANIFOLD, IO),
+    COL(a, 4, 1, MANIFOLD, IO), COL(b, 4, 1, MANIFOLD, IO), COL(pair, 4, 1, MANIFOLD, IO),
+    COL(sub, 4, 1, MANIFOLD, IO), COL(row, 4, 1, MANIFOLD, IO), COL(colour, 4, 1, MANIFOLD, IO),
+    COL(order, 4, 1, MANIFOLD, IO), COL(points, 1, 1, MANIFOLD, IO),
+    COL(sliding, 1, 1, MANIFOLD, IO), COL(normal, 8, 3, MANIFOLD, IO), COL(t1, 8, 3, MANIFOLD, IO),
+    COL(t2, 8, 3, MANIFOLD, IO), COL(friction, 8, 1, MANIFOLD, IO), COL(restitution, 8, 1, MANIFOLD, IO),
+    COL(tolerance, 8, 1, MANIFOLD, IO), COL(patch, 8, 1, MANIFOLD, IO), COL(roll_impulse1, 8, 1, MANIFOLD, IO),
     COL(roll_impulse2, 8, 1, MANIFOLD, IO), COL(spin_impulse, 8, 1, MANIFOLD, IO),
     COL(tangent_impulse1, 8, 1, MANIFOLD, IO), COL(tangent_impulse2, 8, 1, MANIFOLD, IO),
-    COL(drift1, 8, 1, MANIFOLD, IO), COL(drift2, 8, 1, MANIFOLD, IO), COL(separation, 8, 4, MANIFOLD, IN),
-    COL(point, 8, 12, MANIFOLD, IN), COL(shift, 8, 12, MANIFOLD, IN), COL(normal_impulse, 8, 4, MANIFOLD, IO),
+    COL(drift1, 8, 1, MANIFOLD, IO), COL(drift2, 8, 1, MANIFOLD, IO), COL(separation, 8, 4, MANIFOLD, IO),
+    COL(point, 8, 12, MANIFOLD, IO), COL(shift, 8, 12, MANIFOLD, IO), COL(normal_impulse, 8, 4, MANIFOLD, IO),
     COL(max_normal_impulse, 8, 4, MANIFOLD, IO), COL(relative_velocity, 8, 4, MANIFOLD, IO),
     COL(gap, 8, 1, MANIFOLD, IO), COL(now, 8, 4, MANIFOLD, IO), COL(ra, 8, 12, MANIFOLD, IO),
     COL(rb, 8, 12, MANIFOLD, IO), COL(la, 8, 12, MANIFOLD, IO), COL(lb, 8, 12, MANIFOLD, IO),
@@ -57,12 +60,32 @@ const Column COLUMNS[] = {
     COL(block, 8, 16, MANIFOLD, IO), COL(lever, 8, 4, MANIFOLD, IO), COL(anchor_a, 8, 3, MANIFOLD, IO),
     COL(anchor_b, 8, 3, MANIFOLD, IO), COL(fjac, 8, 24, MANIFOLD, IO), COL(tangent_mass, 8, 3, MANIFOLD, IO),
     COL(roll_mass1, 8, 1, MANIFOLD, IO), COL(roll_mass2, 8, 1, MANIFOLD, IO), COL(spin_mass, 8, 1, MANIFOLD, IO),
-    COL(pid, 4, 4, MANIFOLD, IN), COL(pair_points, 1, 1, PAIR, IO), COL(pair_pid, 4, 4, PAIR, IO),
+    COL(pid, 4, 4, MANIFOLD, IO), COL(pair_points, 1, 1, PAIR, IO), COL(pair_pid, 4, 4, PAIR, IO),
     COL(pair_normal_impulse, 8, 4, PAIR, IO), COL(pair_twist, 8, 3, PAIR, IO), COL(pair_friction, 8, 3, PAIR, IO),
     COL(pair_slip, 8, 3, PAIR, IO),
 };
 #undef COL
 constexpr int NCOL = sizeof(COLUMNS) / sizeof(COLUMNS[0]);
+
+// The columns narrow() writes, which colour()'s arrange moves to their rows (src/graph.js:83-91).
+const char* const ARRANGED[] = {"a", "b", "pair", "sub", "points", "colour", "pid", "normal", "t1", "t2",
+                                "friction", "restitution", "tolerance", "patch", "roll_impulse1", "roll_impulse2",
+                                "spin_impulse", "tangent_impulse1", "tangent_impulse2", "drift1", "drift2",
+                                "separation", "normal_impulse", "point", "shift"};
+constexpr int NARRANGED = sizeof(ARRANGED) / sizeof(ARRANGED[0]);
+static_assert(NARRANGED <= bm::ARRANGE_MAX, "arrange takes at most ARRANGE_MAX columns");
+
+int column_index(const char* name) {
+  for (int c = 0; c < NCOL; c++)
+    if (std::strcmp(COLUMNS[c].name, name) == 0) return c;
+  return -1;
+}
+
+// Device scratch the colouring uses, grown as batches need.
+struct Scratch {
+  void* p = nullptr;
+  size_t cap = 0;
+};
 
 }  // namespace
 
@@ -82,6 +105,11 @@ struct bm_context {
   size_t lds_limit = 0;     // dynamic LDS a workgroup may take (0: not yet asked)
   int32_t shape = 0;        // launch shape of the soft-step kernel
   int32_t resident = 1;     // keep velocities in LDS when they fit
+  bool laid_out = false;    // rows are laid out by colour (checked at upload, or by bm_colour)
+  void* spare[NCOL] = {};   // arrange's second buffers
+  size_t spare_cap[NCOL] = {};
+  Scratch deg, adj, keys, key_offset, world_of;
+  std::vector<int32_t> key_offsets;
 };
 
 namespace {
@@ -140,25 +168,50 @@ int check_batch(bm_context* ctx, const bm_columns* h) {
     const int nm = h->manifold_offset[w + 1] - h->manifold_offset[w];
     const int np = h->pair_offset[w + 1] - h->pair_offset[w];
     const int moff = h->manifold_offset[w], aoff = h->awake_offset[w];
-    const int32_t* st = h->starts + BM_STARTS * w;
-    if (h->solved[w] < 0 || h->solved[w] > nm) return fail(ctx, BM_ERR_ARGUMENT, "bm_upload: solved out of range");
     if (h->substeps[w] < 1) return fail(ctx, BM_ERR_ARGUMENT, "bm_upload: substeps must be 1 or more");
-    if (st[0] != 0) return fail(ctx, BM_ERR_ARGUMENT, "bm_upload: starts[0] must be 0");
-    for (int k = 0; k + 1 < BM_STARTS; k++)
-      if (st[k + 1] < st[k] || st[k + 1] > nm) return fail(ctx, BM_ERR_ARGUMENT, "bm_upload: starts out of order");
-    if (st[BM_COLOURS + 1] != h->solved[w]) return fail(ctx, BM_ERR_ARGUMENT, "bm_upload: solved must equal the resting run's start");
     for (int i = aoff; i < h->awake_offset[w + 1]; i++)
       if (h->awake_list[i] < 0 || h->awake_list[i] >= nb) return fail(ctx, BM_ERR_ARGUMENT, "bm_upload: awake_list slot out of range");
     for (int m = moff; m < moff + nm; m++) {
       if (h->a[m] < 0 || h->a[m] >= nb || h->b[m] < 0 || h->b[m] >= nb)
         return fail(ctx, BM_ERR_ARGUMENT, "bm_upload: manifold body slot out of range");
       if (h->pair[m] < 0 || h->pair[m] >= np) return fail(ctx, BM_ERR_ARGUMENT, "bm_upload: manifold pair out of range");
-      if (h->row[m] < 0 || h->row[m] >= nm) return fail(ctx, BM_ERR_ARGUMENT, "bm_upload: row out of range");
       if (h->points[m] < 1 || h->points[m] > 4) return fail(ctx, BM_ERR_ARGUMENT, "bm_upload: a manifold has 1 to 4 points");
-      if (m - moff < h->solved[w] && h->sub[m] >= 0)
+      if (h->sub[m] >= 0)
         return fail(ctx, BM_ERR_UNSUPPORTED, "bm_upload: manifolds of mesh and compound pairs are not supported");
     }
   }
+  return BM_OK;
+}
+
+// Whether starts, solved and row describe rows laid out by colour: runs in order within
+// each world, solved at the resting run, row a permutation of each world's rows.
+bool laid_out(const bm_columns* h) {
+  if (!h->starts || !h->solved || !h->row) return false;
+  std::vector<char> seen;
+  for (int w = 0; w < h->n_worlds; w++) {
+    const int nm = h->manifold_offset[w + 1] - h->manifold_offset[w], moff = h->manifold_offset[w];
+    const int32_t* st = h->starts + BM_STARTS * w;
+    if (st[0] != 0 || st[BM_STARTS - 1] != nm || st[BM_COLOURS + 1] != h->solved[w]) return false;
+    for (int k = 0; k + 1 < BM_STARTS; k++)
+      if (st[k + 1] < st[k]) return false;
+    seen.assign(nm, 0);
+    for (int m = moff; m < moff + nm; m++) {
+      const int r = h->row[m];
+      if (r < 0 || r >= nm || seen[r]) return false;
+      seen[r] = 1;
+    }
+  }
+  return true;
+}
+
+int grow(bm_context* ctx, Scratch& s, size_t bytes, const char* what) {
+  if (bytes <= s.cap) return BM_OK;
+  if (s.p) (void)hipFree(s.p);
+  s.p = nullptr;
+  size_t cap = 256;
+  while (cap < bytes) cap *= 2;
+  if (hipMalloc(&s.p, cap) != hipSuccess) return fail(ctx, BM_ERR_DEVICE, std::string(what) + ": hipMalloc");
+  s.cap = cap;
   return BM_OK;
 }
 
@@ -185,8 +238,12 @@ int bm_create(bm_context** out, int32_t device) {
 int bm_destroy(bm_context* ctx) {
   if (!ctx) return BM_OK;
   (void)hipSetDevice(ctx->device);
-  for (int c = 0; c < NCOL; c++)
+  for (int c = 0; c < NCOL; c++) {
     if (ctx->buffer[c]) (void)hipFree(ctx->buffer[c]);
+    if (ctx->spare[c]) (void)hipFree(ctx->spare[c]);
+  }
+  for (Scratch* s : {&ctx->deg, &ctx->adj, &ctx->keys, &ctx->key_offset, &ctx->world_of})
+    if (s->p) (void)hipFree(s->p);
   if (ctx->begin) (void)hipEventDestroy(ctx->begin);
   if (ctx->end) (void)hipEventDestroy(ctx->end);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -214,6 +271,7 @@ int bm_upload(bm_context* ctx, const bm_columns* h) {
       return fail(ctx, BM_ERR_ARGUMENT, std::string("bm_upload: ") + col.name + " is null");
   }
   if (int rc = check_batch(ctx, h)) return rc;
+  ctx->laid_out = laid_out(h);
   ctx->n_worlds = h->n_worlds;
   ctx->bodies = h->body_offset[h->n_worlds];
   ctx->manifolds = h->manifold_offset[h->n_worlds];
@@ -255,6 +313,8 @@ int bm_soft_step(bm_context* ctx) {
   if (!ctx) return BM_ERR_ARGUMENT;
   if (!ctx->uploaded) return fail(ctx, BM_ERR_ARGUMENT, "bm_soft_step: no batch uploaded");
   ctx->error.clear();
+  if (!ctx->laid_out)
+    return fail(ctx, BM_ERR_ARGUMENT, "bm_soft_step: the batch's rows are not laid out by colour (call bm_colour)");
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->device), "bm_soft_step: hipSetDevice");
   HIP_OR_FAIL(ctx, hipEventRecord(ctx->begin, ctx->stream), "bm_soft_step: hipEventRecord");
   if (ctx->lds_limit == 0) ctx->lds_limit = bm::soft_step_lds_limit(ctx->device);
@@ -284,6 +344,97 @@ int bm_download(bm_context* ctx, bm_columns* h) {
                 "bm_download: hipMemcpyAsync");
   }
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_download: hipStreamSynchronize");
+  return BM_OK;
+}
+
+int bm_colour(bm_context* ctx) {
+  if (!ctx) return BM_ERR_ARGUMENT;
+  if (!ctx->uploaded) return fail(ctx, BM_ERR_ARGUMENT, "bm_colour: no batch uploaded");
+  ctx->error.clear();
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->device), "bm_colour: hipSetDevice");
+  const int n = ctx->n_worlds;
+  // Each world's sort width: a power of two at least its manifold count.
+  std::vector<int32_t> moff(n + 1);
+  HIP_OR_FAIL(ctx, hipMemcpy(moff.data(), ctx->buffer[column_index("manifold_offset")], (n + 1) * sizeof(int32_t),
+                             hipMemcpyDeviceToHost), "bm_colour: hipMemcpy");
+  ctx->key_offsets.assign(n + 1, 0);
+  for (int w = 0; w < n; w++) {
+    int width = 1;
+    while (width < moff[w + 1] - moff[w]) width *= 2;
+    ctx->key_offsets[w + 1] = ctx->key_offsets[w] + width;
+  }
+  const size_t manifolds = ctx->manifolds;
+  if (int rc = grow(ctx, ctx->deg, ((size_t)ctx->bodies + n) * sizeof(int32_t), "bm_colour")) return rc;
+  if (int rc = grow(ctx, ctx->adj, 2 * manifolds * sizeof(int32_t) + 4, "bm_colour")) return rc;
+  if (int rc = grow(ctx, ctx->keys, (size_t)ctx->key_offsets[n] * sizeof(uint64_t), "bm_colour")) return rc;
+  if (int rc = grow(ctx, ctx->key_offset, (n + 1) * sizeof(int32_t), "bm_colour")) return rc;
+  if (int rc = grow(ctx, ctx->world_of, manifolds * sizeof(int32_t) + 4, "bm_colour")) return rc;
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->key_offset.p, ctx->key_offsets.data(), (n + 1) * sizeof(int32_t),
+                                  hipMemcpyHostToDevice, ctx->stream), "bm_colour: hipMemcpyAsync");
+  auto buf = [&](const char* name) { return ctx->buffer[column_index(name)]; };
+  bm::ColourDev cd{};
+  cd.n_worlds = n;
+  cd.body_offset = static_cast<const int*>(buf("body_offset"));
+  cd.manifold_offset = static_cast<const int*>(buf("manifold_offset"));
+  cd.a = static_cast<const int*>(buf("a"));
+  cd.b = static_cast<const int*>(buf("b"));
+  cd.type = static_cast<const uint8_t*>(buf("type"));
+  cd.frozen = static_cast<const uint8_t*>(buf("frozen"));
+  cd.colour = static_cast<int*>(buf("colour"));
+  cd.order = static_cast<int*>(buf("order"));
+  cd.row = static_cast<int*>(buf("row"));
+  cd.starts = static_cast<int*>(buf("starts"));
+  cd.solved = static_cast<int*>(buf("solved"));
+  cd.deg = static_cast<int*>(ctx->deg.p);
+  cd.adj = static_cast<int*>(ctx->adj.p);
+  cd.keys = static_cast<uint64_t*>(ctx->keys.p);
+  cd.key_offset = static_cast<const int*>(ctx->key_offset.p);
+  (void)hipGetLastError();
+  bm::launch_world_of(cd.manifold_offset, n, static_cast<int*>(ctx->world_of.p), ctx->stream);
+  bm::launch_colour(cd, ctx->stream);
+  HIP_OR_FAIL(ctx, hipGetLastError(), "bm_colour: launch");
+  // Arrange: every column narrow() writes, into its spare buffer, then the two swap.
+  bm::ArrangeDev ad{};
+  ad.manifolds = (long)manifolds;
+  ad.manifold_offset = cd.manifold_offset;
+  ad.order = cd.order;
+  ad.world_of = static_cast<const int*>(ctx->world_of.p);
+  int idx[NARRANGED];
+  for (int k = 0; k < NARRANGED; k++) {
+    const int c = column_index(ARRANGED[k]);
+    idx[k] = c;
+    if (ctx->spare_cap[c] < ctx->capacity[c]) {
+      if (ctx->spare[c]) HIP_OR_FAIL(ctx, hipFree(ctx->spare[c]), "bm_colour: hipFree");
+      ctx->spare[c] = nullptr;
+      HIP_OR_FAIL(ctx, hipMalloc(&ctx->spare[c], ctx->capacity[c]), "bm_colour: hipMalloc");
+      ctx->spare_cap[c] = ctx->capacity[c];
+    }
+    ad.columns[k] = bm::ArrangeColumn{ctx->buffer[c], ctx->spare[c], (int)COLUMNS[c].elem, COLUMNS[c].per};
+  }
+  bm::launch_arrange(ad, NARRANGED, ctx->stream);
+  HIP_OR_FAIL(ctx, hipGetLastError(), "bm_colour: arrange");
+  for (int k = 0; k < NARRANGED; k++) {
+    const int c = idx[k];
+    std::swap(ctx->buffer[c], ctx->spare[c]);
+    std::swap(ctx->capacity[c], ctx->spare_cap[c]);
+    void* p = ctx->buffer[c];
+    std::memcpy(reinterpret_cast<char*>(&ctx->dev) + COLUMNS[c].dev, &p, sizeof(void*));
+  }
+  bm::RestDev rd{};
+  rd.manifolds = (long)manifolds;
+  rd.manifold_offset = cd.manifold_offset;
+  rd.solved = cd.solved;
+  rd.world_of = ad.world_of;
+  rd.points = static_cast<const uint8_t*>(buf("points"));
+  rd.separation = static_cast<const double*>(buf("separation"));
+  rd.normal_impulse = static_cast<const double*>(buf("normal_impulse"));
+  rd.max_normal_impulse = static_cast<double*>(buf("max_normal_impulse"));
+  rd.relative_velocity = static_cast<double*>(buf("relative_velocity"));
+  rd.gap = static_cast<double*>(buf("gap"));
+  bm::launch_rest(rd, ctx->stream);
+  HIP_OR_FAIL(ctx, hipGetLastError(), "bm_colour: rest");
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_colour: kernel");
+  ctx->laid_out = true;
   return BM_OK;
 }
 

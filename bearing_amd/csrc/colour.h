@@ -1,0 +1,51 @@
+// Device-side views for colouring, arranging rows and rest() (colour.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "soft.h"
+
+namespace bm {
+
+struct ColourDev {
+  int n_worlds;
+  const int *body_offset, *manifold_offset;
+  const int *a, *b;                // narrow order, each world's own slots
+  const uint8_t *type, *frozen;
+  int* colour;                     // out, narrow order
+  int *order, *row, *starts, *solved;
+  int* deg;                        // scratch: bodies + n_worlds
+  int* adj;                        // scratch: 2 per manifold
+  uint64_t* keys;                  // scratch: key_offset[n_worlds]
+  const int* key_offset;           // per world, a power of two at least its manifolds
+};
+
+struct ArrangeColumn {
+  const void* from;
+  void* to;
+  int bytes;  // 1, 4 or 8
+  int per;    // numbers per manifold
+};
+
+constexpr int ARRANGE_MAX = 32;
+
+struct ArrangeDev {
+  long manifolds;
+  const int *manifold_offset, *order, *world_of;
+  ArrangeColumn columns[ARRANGE_MAX];
+};
+
+struct RestDev {
+  long manifolds;
+  const int *manifold_offset, *solved, *world_of;
+  const uint8_t* points;
+  const double *separation, *normal_impulse;
+  double *max_normal_impulse, *relative_velocity, *gap;
+};
+
+void launch_colour(const ColourDev& d, hipStream_t stream);
+void launch_world_of(const int* manifold_offset, int n_worlds, int* world_of, hipStream_t stream);
+void launch_arrange(const ArrangeDev& d, int columns, hipStream_t stream);
+void launch_rest(const RestDev& d, hipStream_t stream);
+
+}  // namespace bm

@@ -1,0 +1,210 @@
+// Constraint-graph colouring on the MI355X (src/graph.js:105-214), one workgroup per world.
+//
+// The reference colours manifolds greedily in the order narrow() found them: each takes
+// the lowest of 32 colours that no earlier manifold on one of its moving bodies holds
+// (or the overflow when all 32 are held), so a manifold's colour depends only on the
+// colours of the earlier manifolds it shares a moving body with. That makes it a
+// priority fixed point: a manifold whose earlier neighbours all have their colours can
+// take its own, the same one the serial pass gives it. Each round every undecided
+// manifold checks its earlier neighbours (through a per-body list built with LDS-free
+// atomics in HBM scratch) and decides if it can; rounds run until none is left, as
+// many as the longest chain of dependent manifolds, not as many as manifolds.
+//
+// The rows are then laid out as the reference's stable counting sorts leave them: by
+// colour, then by b's slot, then by the order found. That key (colour, b, m) is unique,
+// so a bitonic sort of it in scratch gives exactly that order. Resting rows (between
+// frozen or static bodies, src/graph.js RESTING) go last and get rest()'s values
+// (src/contact.js:1318-1329). Then every column narrow() writes is moved to its row
+// (src/graph.js:178-214), through spare buffers.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "colour.h"
+
+namespace bm {
+
+namespace {
+constexpr int DYNAMIC = 0, STATIC = 1;
+constexpr int COLOURS = 32, OVERFLOW = 32, RESTING = 33;
+constexpr int UNDECIDED = -1;
+
+__device__ __forceinline__ int lowest_free(int32_t held) {
+  const int32_t bit = ~held & (held + 1);
+  return 31 - __clz((uint32_t)bit);
+}
+}  // namespace
+
+// One workgroup a world. Scratch per world (at the world's manifold / body offsets):
+// deg[nb + 1] (per body), adj[2 * count], keys[pow2 >= count] (64-bit), state[count].
+__global__ void __launch_bounds__(256) colour_kernel(ColourDev d) {
+  __shared__ int flag;
+  __shared__ int runs[RESTING + 2];
+  for (int wld = blockIdx.x; wld < d.n_worlds; wld += gridDim.x) {
+    const int boff = d.body_offset[wld], moff = d.manifold_offset[wld];
+    const int nb = d.body_offset[wld + 1] - boff, n = d.manifold_offset[wld + 1] - moff;
+    const int* A = d.a + moff;
+    const int* B = d.b + moff;
+    int* colour = d.colour + moff;
+    int* deg = d.deg + boff + wld;           // nb + 1 entries a world
+    int* adj = d.adj + 2 * moff;             // 2 per manifold
+    uint64_t* keys = d.keys + d.key_offset[wld];
+    const int width = d.key_offset[wld + 1] - d.key_offset[wld];
+    const uint8_t* type = d.type + boff;
+    const uint8_t* frozen = d.frozen + boff;
+    __syncthreads();
+    for (int s = threadIdx.x; s <= nb; s += blockDim.x) deg[s] = 0;
+    for (int k = threadIdx.x; k < RESTING + 2; k += blockDim.x) runs[k] = 0;
+    __syncthreads();
+    // Which ends move, and the resting rows (src/graph.js:114-124).
+    for (int m = threadIdx.x; m < n; m += blockDim.x) {
+      const int a = A[m], b = B[m];
+      const bool fa = frozen[a] != 0, fb = frozen[b] != 0;
+      if ((fa || type[a] == STATIC) && (fb || type[b] == STATIC)) {
+        colour[m] = RESTING;
+        continue;
+      }
+      colour[m] = UNDECIDED;
+      if (type[a] == DYNAMIC && !fa) atomicAdd(&deg[a + 1], 1);
+      if (type[b] == DYNAMIC && !fb) atomicAdd(&deg[b + 1], 1);
+    }
+    __syncthreads();
+    // Body lists' starts: an inclusive scan of deg[1..nb], one wave strided.
+    if (threadIdx.x == 0) {
+      for (int s = 1; s <= nb; s++) deg[s] += deg[s - 1];
+    }
+    __syncthreads();
+    // Fill: deg[s] walks up to the next body's start.
+    for (int m = threadIdx.x; m < n; m += blockDim.x) {
+      if (colour[m] == RESTING) continue;
+      const int a = A[m], b = B[m];
+      if (type[a] == DYNAMIC && frozen[a] == 0) adj[atomicAdd(&deg[a], 1)] = m;
+      if (type[b] == DYNAMIC && frozen[b] == 0) adj[atomicAdd(&deg[b], 1)] = m;
+    }
+    __syncthreads();
+    // deg[s] is now body s's end, body s-1's end its start (deg[-1] read as 0).
+    do {
+      __syncthreads();
+      if (threadIdx.x == 0) flag = 0;
+      __syncthreads();
+      for (int m = threadIdx.x; m < n; m += blockDim.x) {
+        if (__hip_atomic_load(&colour[m], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != UNDECIDED) continue;
+        const int a = A[m], b = B[m];
+        int32_t held = 0;
+        bool ready = true;
+        for (int end = 0; end < 2 && ready; end++) {
+          const int s = end == 0 ? a : b;
+          if (type[s] != DYNAMIC || frozen[s] != 0) continue;
+          const int from = s == 0 ? 0 : deg[s - 1], to = deg[s];
+          for (int k = from; k < to; k++) {
+            const int o = adj[k];
+            if (o >= m) continue;
+            const int c = __hip_atomic_load(&colour[o], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (c == UNDECIDED) { ready = false; break; }
+            if (c < COLOURS) held |= (int32_t)(1u << c);
+          }
+        }
+        if (!ready) { flag = 1; continue; }
+        __hip_atomic_store(&colour[m], held == -1 ? OVERFLOW : lowest_free(held), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+      __syncthreads();
+    } while (flag);
+    // Keys (colour, b, m), padded past n with the largest key; runs counted per colour.
+    for (int i = threadIdx.x; i < width; i += blockDim.x) {
+      if (i < n) {
+        const int c = colour[i];
+        keys[i] = ((uint64_t)c << 56) | ((uint64_t)(uint32_t)B[i] << 32) | (uint32_t)i;
+        atomicAdd(&runs[c + 1], 1);
+      } else {
+        keys[i] = ~0ull;
+      }
+    }
+    __syncthreads();
+    // Bitonic sort of `width` (a power of two) keys.
+    for (int size = 2; size <= width; size <<= 1) {
+      for (int stride = size >> 1; stride > 0; stride >>= 1) {
+        for (int i = threadIdx.x; i < width; i += blockDim.x) {
+          const int j = i ^ stride;
+          if (j > i) {
+            const uint64_t x = keys[i], y = keys[j];
+            const bool up = (i & size) == 0;
+            if ((x > y) == up) { keys[i] = y; keys[j] = x; }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    int* order = d.order + moff;
+    int* row = d.row + moff;
+    for (int o = threadIdx.x; o < n; o += blockDim.x) {
+      const int m = (int)(uint32_t)(keys[o] & 0xffffffffu);
+      order[o] = m;
+      row[m] = o;
+    }
+    if (threadIdx.x == 0) {
+      int* starts = d.starts + BM_STARTS_DEV * wld;
+      starts[0] = 0;
+      for (int k = 0; k <= RESTING; k++) starts[k + 1] = starts[k] + runs[k + 1];
+      d.solved[wld] = starts[RESTING];
+    }
+    __syncthreads();
+  }
+}
+
+// Moves column `c` of every world's manifolds to their rows: to[o] = from[order[o]].
+__global__ void arrange_kernel(ArrangeDev d) {
+  const int col = blockIdx.y;
+  const ArrangeColumn c = d.columns[col];
+  const long total = (long)d.manifolds * c.per;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    const long o = i / c.per, k = i - o * c.per;
+    const int w = d.world_of[o];
+    const long m = d.manifold_offset[w] + d.order[o];
+    if (c.bytes == 8) static_cast<uint64_t*>(c.to)[i] = static_cast<const uint64_t*>(c.from)[m * c.per + k];
+    else if (c.bytes == 4) static_cast<uint32_t*>(c.to)[i] = static_cast<const uint32_t*>(c.from)[m * c.per + k];
+    else static_cast<uint8_t*>(c.to)[i] = static_cast<const uint8_t*>(c.from)[m * c.per + k];
+  }
+}
+
+// rest() (src/contact.js:1318-1329) on each world's resting rows, after they are laid out.
+__global__ void rest_kernel(RestDev d) {
+  const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= d.manifolds) return;
+  const int w = d.world_of[m];
+  if (m - d.manifold_offset[w] < d.solved[w]) return;
+  double least = __builtin_inf();
+  for (int k = 0; k < d.points[m]; k++) {
+    const long i = 4 * m + k;
+    if (d.separation[i] < least) least = d.separation[i];
+    d.max_normal_impulse[i] = d.normal_impulse[i];
+    d.relative_velocity[i] = 0;
+  }
+  d.gap[m] = least;
+}
+
+// Each manifold's world, for the flat kernels above.
+__global__ void world_of_kernel(const int* manifold_offset, int n_worlds, int* world_of) {
+  const int w = blockIdx.x;
+  if (w >= n_worlds) return;
+  for (int m = manifold_offset[w] + threadIdx.x; m < manifold_offset[w + 1]; m += blockDim.x) world_of[m] = w;
+}
+
+void launch_colour(const ColourDev& d, hipStream_t stream) {
+  hipLaunchKernelGGL(colour_kernel, dim3(d.n_worlds), dim3(256), 0, stream, d);
+}
+
+void launch_world_of(const int* manifold_offset, int n_worlds, int* world_of, hipStream_t stream) {
+  hipLaunchKernelGGL(world_of_kernel, dim3(n_worlds), dim3(256), 0, stream, manifold_offset, n_worlds, world_of);
+}
+
+void launch_arrange(const ArrangeDev& d, int columns, hipStream_t stream) {
+  if (d.manifolds == 0) return;
+  hipLaunchKernelGGL(arrange_kernel, dim3(256, columns), dim3(256), 0, stream, d);
+}
+
+void launch_rest(const RestDev& d, hipStream_t stream) {
+  if (d.manifolds == 0) return;
+  hipLaunchKernelGGL(rest_kernel, dim3((d.manifolds + 255) / 256), dim3(256), 0, stream, d);
+}
+
+}  // namespace bm

@@ -16,7 +16,7 @@ struct SoftDev {
   const double *dims, *inv_mass, *inertia, *inv_inertia, *axes, *gravity_scale, *damping, *angular_damping;
   const uint8_t *type, *shape, *frozen, *lock, *framed, *awake;
   const int *joint_head, *awake_list;
-  const int *a, *b, *pair, *sub, *row;
+  const int *a, *b, *pair, *sub, *row, *colour, *order;
   const uint8_t* points;
   uint8_t* sliding;
   const double *normal, *t1, *t2, *friction, *restitution, *tolerance, *patch;

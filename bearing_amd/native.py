@@ -35,7 +35,7 @@ FIELDS = [
     ("lock", _U8, "B", 1), ("framed", _U8, "B", 1), ("awake", _U8, "B", 1), ("joint_head", _I32, "B", 1),
     ("awake_list", _I32, "A", 1),
     ("a", _I32, "M", 1), ("b", _I32, "M", 1), ("pair", _I32, "M", 1), ("sub", _I32, "M", 1), ("row", _I32, "M", 1),
-    ("points", _U8, "M", 1), ("sliding", _U8, "M", 1), ("normal", _D, "M", 3), ("t1", _D, "M", 3),
+    ("colour", _I32, "M", 1), ("order", _I32, "M", 1), ("points", _U8, "M", 1), ("sliding", _U8, "M", 1), ("normal", _D, "M", 3), ("t1", _D, "M", 3),
     ("t2", _D, "M", 3), ("friction", _D, "M", 1), ("restitution", _D, "M", 1), ("tolerance", _D, "M", 1),
     ("patch", _D, "M", 1), ("roll_impulse1", _D, "M", 1), ("roll_impulse2", _D, "M", 1),
     ("spin_impulse", _D, "M", 1), ("tangent_impulse1", _D, "M", 1), ("tangent_impulse2", _D, "M", 1),
@@ -57,7 +57,7 @@ class Columns(ctypes.Structure):
 
 
 EXPORTS = ["bm_create", "bm_destroy", "bm_last_error", "bm_upload", "bm_soft_step", "bm_download",
-           "bm_soft_step_host", "bm_last_kernel_ms", "bm_set_launch", "bm_lds_limit"]
+           "bm_soft_step_host", "bm_last_kernel_ms", "bm_set_launch", "bm_lds_limit", "bm_colour"]
 
 _lib = None
 
@@ -77,6 +77,7 @@ def load():
         for name in ("bm_upload", "bm_download", "bm_soft_step_host"):
             getattr(lib, name).argtypes = [vp, ctypes.POINTER(Columns)]
         lib.bm_soft_step.argtypes = [vp]
+        lib.bm_colour.argtypes = [vp]
         lib.bm_last_kernel_ms.argtypes = [vp]
         lib.bm_last_kernel_ms.restype = ctypes.c_double
         lib.bm_set_launch.argtypes = [vp, ctypes.c_int32, ctypes.c_int32]
@@ -176,6 +177,10 @@ class Batch:
         """bm_soft_step on the uploaded batch; returns the kernel's milliseconds."""
         ctx.check(ctx.lib.bm_soft_step(ctx.ptr), "bm_soft_step")
         return ctx.lib.bm_last_kernel_ms(ctx.ptr)
+
+    def colour(self, ctx):
+        """bm_colour on the uploaded batch (manifolds in the order narrow() found them)."""
+        ctx.check(ctx.lib.bm_colour(ctx.ptr), "bm_colour")
 
     def step_host(self, ctx):
         ctx.check(ctx.lib.bm_soft_step_host(ctx.ptr, ctypes.byref(self.struct)), "bm_soft_step_host")

@@ -47,8 +47,8 @@ typedef struct bm_columns {
   const int32_t* pair_offset;
   const int32_t* awake_offset;
   /* per world */
-  const int32_t* starts;     /* [in] BM_STARTS per world: contacts.starts after colour() */
-  const int32_t* solved;     /* [in] contacts.solved: rows before the resting run */
+  int32_t* starts;           /* [io] BM_STARTS per world: contacts.starts after colour() */
+  int32_t* solved;           /* [io] contacts.solved: rows before the resting run */
   const int32_t* substeps;   /* [in] world.substeps */
   const double* dt;          /* [in] the step's dt */
   const double* gravity;     /* [in] 3 per world */
@@ -71,16 +71,20 @@ typedef struct bm_columns {
   const uint8_t* type, *shape, *frozen, *lock, *framed, *awake; /* [in] 1 each */
   const int32_t* joint_head; /* [in] 1 */
   const int32_t* awake_list; /* [in] awake_offset-indexed */
-  /* manifolds, by row as colour() laid them out (src/graph.js arrange) */
-  const int32_t* a, *b, *pair, *sub, *row; /* [in] 1 */
-  const uint8_t* points;     /* [in] 1 */
+  /* manifolds, by row as colour() laid them out (src/graph.js arrange); before
+   * bm_colour, in the order narrow() found them */
+  int32_t* a, *b, *pair, *sub; /* [io] 1 (moved to their rows by bm_colour) */
+  int32_t* row;              /* [io] 1: the row each found manifold went to */
+  int32_t* colour;           /* [io] 1: colour of the manifold in each row (32 overflow, 33 resting) */
+  int32_t* order;            /* [io] 1: which found manifold each row holds */
+  uint8_t* points;           /* [io] 1 */
   uint8_t* sliding;          /* [out] 1 */
-  const double* normal, *t1, *t2; /* [in] 3 */
-  const double* friction, *restitution, *tolerance, *patch; /* [in] 1 */
+  double* normal, *t1, *t2;  /* [io] 3 */
+  double* friction, *restitution, *tolerance, *patch; /* [io] 1 */
   double* roll_impulse1, *roll_impulse2, *spin_impulse, *tangent_impulse1, *tangent_impulse2; /* [io] 1 */
   double* drift1, *drift2;   /* [io] 1 */
-  const double* separation;  /* [in] 4 */
-  const double* point, *shift; /* [in] 12 */
+  double* separation;        /* [io] 4 */
+  double* point, *shift;     /* [io] 12 */
   double* normal_impulse, *max_normal_impulse, *relative_velocity; /* [io] 4 */
   double* gap;               /* [io] 1 */
   double* now;               /* [out] 4 */
@@ -93,7 +97,7 @@ typedef struct bm_columns {
   double* fjac;              /* [out] 24 */
   double* tangent_mass;      /* [out] 3 */
   double* roll_mass1, *roll_mass2, *spin_mass; /* [out] 1 */
-  const uint32_t* pid;       /* [in] 4 */
+  uint32_t* pid;             /* [io] 4 */
   /* what the pair list keeps, by pair (src/pairs.js) */
   uint8_t* pair_points;      /* [out] 1 */
   uint32_t* pair_pid;        /* [out] 4 */
@@ -110,8 +114,18 @@ int bm_destroy(bm_context* ctx);
  * (src/core/check.js:15-22); "" after a success. Owned by the context. */
 const char* bm_last_error(const bm_context* ctx);
 
-/* Copies a batch's columns into the context's device memory (HBM). */
+/* Copies a batch's columns into the context's device memory (HBM). Indices are
+ * checked (slots, pairs, point counts); if starts, solved and row describe a
+ * layout by colour the batch is ready for bm_soft_step, else bm_colour lays it out. */
 int bm_upload(bm_context* ctx, const bm_columns* host);
+
+/* src/graph.js:105-214 and src/contact.js:1318-1329 on the device, for each world of
+ * the uploaded batch whose manifolds are in the order narrow() found them: colour
+ * each (the reference's greedy lowest-free colour of 32, overflow, resting), lay the
+ * rows out colour by colour and by b's slot within a colour, move every column narrow()
+ * writes to its row, set row / order / starts / solved, and give resting rows rest()'s
+ * values. Bit-identical to the reference's colour(), arrange() and rest(). */
+int bm_colour(bm_context* ctx);
 
 /*
  * The soft step on the device, on the uploaded batch: for each world what

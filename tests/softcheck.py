@@ -115,6 +115,8 @@ def world_from_columns(fx, cols):
             w[snake(key[2:])] = arr
         elif key.startswith("p."):
             w["pair_" + snake(key[2:])] = arr
+    w["colour"] = fx["in.c.colour"].astype(np.int32).copy()
+    w["order"] = fx["in.c.order"].astype(np.int32).copy()
     w["awake_list"] = cols["awake_list"]
     w["starts"] = cols["starts"]
     w["gravity"] = cols["gravity"]
@@ -134,4 +136,30 @@ def columns_from_world(w):
         out["c." + name] = w[snake(name)]
     for _, _, name, _ in PAIR:
         out["p." + name] = w["pair_" + snake(name)]
+    return out
+
+
+# The columns narrow() writes, which colour() moves to their rows (src/graph.js:83-91).
+ARRANGED = ["a", "b", "pair", "sub", "points", "colour", "pid", "normal", "t1", "t2", "friction", "restitution",
+            "tolerance", "patch", "roll_impulse1", "roll_impulse2", "spin_impulse", "tangent_impulse1",
+            "tangent_impulse2", "drift1", "drift2", "separation", "normal_impulse", "point", "shift"]
+
+
+def narrow_order(world, seed=0):
+    """A laid-out world dict back in the order narrow() found its manifolds (row m of the
+    result is found manifold m), with the layout columns and rest()'s outputs scrambled."""
+    rng = np.random.default_rng(seed)
+    row = np.asarray(world["row"], np.int64)
+    n = len(row)
+    out = dict(world)
+    for name in ARRANGED:
+        col = np.asarray(world[name])
+        per = col.size // n if n else 1
+        out[name] = col.reshape(n, per)[row].reshape(-1).copy()
+    for name in ("row", "order"):
+        out[name] = rng.integers(-5, n + 5, n).astype(np.int32)
+    out["starts"] = rng.integers(0, n + 1, 35).astype(np.int32)
+    out["solved"] = np.int32(rng.integers(0, n + 1))
+    for name in ("max_normal_impulse", "relative_velocity", "gap"):
+        out[name] = rng.normal(size=np.asarray(world[name]).size)
     return out

@@ -13,7 +13,7 @@ import pytest
 import fixtures
 from bearing_amd import native
 from oracle import binding
-from softcheck import columns_from_world, compare, world_from_columns
+from softcheck import ARRANGED, columns_from_world, compare, narrow_order, world_from_columns
 
 pytestmark = pytest.mark.gpu
 
@@ -91,3 +91,48 @@ def test_gpu_matches_oracle_on_scrambled_worlds(ctx, seed):
             same = np.array_equal(g.view(np.uint64) if g.dtype == np.float64 else g,
                                   w.view(np.uint64) if w.dtype == np.float64 else w)
             assert same, f"{SOFT[i]} seed {seed}: {key} differs from the oracle"
+
+
+@pytest.mark.parametrize("path", SOFT, ids=[p.split("/")[-1].replace(".bmfx.gz", "") for p in SOFT])
+def test_gpu_colour_then_soft_step_matches_reference(ctx, path):
+    """From the manifolds in the order narrow() found them: bm_colour must lay the rows out
+    as the reference's colour() / arrange() / rest() did, and the soft step after it must
+    end where the reference's did."""
+    print("fixture", path, flush=True)
+    fx = fixtures.load(path)
+    laid = world_from_columns(fx, binding.columns(fx))
+    batch = native.Batch([narrow_order(laid, seed=len(path))])
+    batch.upload(ctx)
+    batch.colour(ctx)
+    batch.download(ctx)
+    got = batch.world(0)
+    n = fx.meta["count"]
+    for name in ARRANGED + ["row", "order"]:
+        g, w = np.asarray(got[name]), np.asarray(laid[name])
+        assert np.array_equal(g.view(np.uint64) if g.dtype == np.float64 else g,
+                              w.view(np.uint64) if w.dtype == np.float64 else w), f"{name} laid out differently"
+    assert np.array_equal(batch.arrays["starts"], fx["in.starts"])
+    assert int(batch.arrays["solved"][0]) == fx.meta["solved"]
+    solved = fx.meta["solved"]
+    pts = laid["points"]
+    for name in ("max_normal_impulse", "relative_velocity"):
+        idx = [4 * m + k for m in range(solved, n) for k in range(pts[m])]
+        assert np.array_equal(np.asarray(got[name])[idx].view(np.uint64), np.asarray(laid[name])[idx].view(np.uint64))
+    assert np.array_equal(np.asarray(got["gap"])[solved:n].view(np.uint64), np.asarray(laid["gap"])[solved:n].view(np.uint64))
+    batch.step(ctx)
+    batch.download(ctx)
+    errs = compare(fx, columns_from_world(batch.world(0)))
+    assert not errs, "\n".join(errs)
+
+
+def test_gpu_colour_of_a_batch(ctx):
+    fxs = [fixtures.load(p) for p in SOFT]
+    laid = [world_from_columns(fx, binding.columns(fx)) for fx in fxs]
+    batch = native.Batch([narrow_order(w, seed=i) for i, w in enumerate(laid)])
+    batch.upload(ctx)
+    batch.colour(ctx)
+    batch.step(ctx)
+    batch.download(ctx)
+    for i, fx in enumerate(fxs):
+        errs = compare(fx, columns_from_world(batch.world(i)))
+        assert not errs, f"{SOFT[i]}: " + "\n".join(errs)
