@@ -367,9 +367,28 @@ __device__ __forceinline__ void solve_row(const SoftDev& d, const VW& vw, const 
   const double inv = 1 / h;
   const double springy = soft.impulse_scale / soft.mass_scale + COMPLIANCE;
   double target[N], mass_scale[N], impulse_scale[N], give[N], closing_now[N], wanted[N], solved[N], impulse[N];
+  double now[N], pushed[N];
+  uint32_t first = 0, larger = 0;
+  // Every input of the row is read before anything is written: the columns are
+  // distinct arrays, and with no store in between the loads can all be in flight
+  // together instead of one dependent round trip after another.
+  const double tol = d.tolerance[m], mu = d.friction[m], patch = d.patch[m];
+  const double s1_in = d.drift1[m], s2_in = d.drift2[m];
+  const double o1 = d.tangent_impulse1[m], o2 = d.tangent_impulse2[m];
+  const double r1_in = N <= 2 ? d.roll_impulse1[m] : 0, r2_in = N <= 2 ? d.roll_impulse2[m] : 0;
+  const double rm1 = N <= 2 ? d.roll_mass1[m] : 0, rm2 = N <= 2 ? d.roll_mass2[m] : 0;
+  const double s0_in = d.spin_impulse[m], spin_mass = d.spin_mass[m];
+  const double tm0 = d.tangent_mass[3 * m], tm1 = d.tangent_mass[3 * m + 1], tm2 = d.tangent_mass[3 * m + 2];
+  double most_in[N], lever_in[N], nmass[N];
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    most_in[k] = d.max_normal_impulse[m * P + k];
+    lever_in[k] = d.lever[m * P + k];
+    nmass[k] = d.normal_mass[m * P + k];
+  }
   const int m3 = 3 * m;
   const double nx = d.normal[m3], ny = d.normal[m3 + 1], nz = d.normal[m3 + 2];
-  const double max_bias = d.tolerance[m] * inv;
+  const double max_bias = tol * inv;
   double dn = 0;
   V3 na{0, 0, 0}, nb{0, 0, 0};
   if (!known) {
@@ -392,8 +411,8 @@ __device__ __forceinline__ void solve_row(const SoftDev& d, const VW& vw, const 
     } else {
       s = dn + (d.lb[i3] * nb.x + d.lb[i3 + 1] * nb.y + d.lb[i3 + 2] * nb.z)
           - (d.la[i3] * na.x + d.la[i3 + 1] * na.y + d.la[i3 + 2] * na.z) + d.adjusted[i];
-      if (!bias) d.now[i] = s;
     }
+    now[k] = s;
     if (s < least) least = s;
     if (s > 0) {
       target[k] = s * inv; mass_scale[k] = 1; impulse_scale[k] = 0; give[k] = COMPLIANCE;
@@ -405,7 +424,6 @@ __device__ __forceinline__ void solve_row(const SoftDev& d, const VW& vw, const 
     }
     impulse[k] = d.normal_impulse[i];
   }
-  d.gap[m] = least;
   const double ima = d.inv_mass[ga], imb = d.inv_mass[gb];
   Pair s = gather(vw, sa, sb);
   const double* Kg = d.block + m * P * P;
@@ -418,7 +436,7 @@ __device__ __forceinline__ void solve_row(const SoftDev& d, const VW& vw, const 
       const double vn = (s.vbx - s.vax) * nx + (s.vby - s.vay) * ny + (s.vbz - s.vaz) * nz
           + s.wbx * jac[3] + s.wby * jac[4] + s.wbz * jac[5] - s.wax * jac[0] - s.way * jac[1] - s.waz * jac[2];
       const double old = impulse[k];
-      double next = old - d.normal_mass[i] * mass_scale[k] * (vn + target[k]) - impulse_scale[k] * old;
+      double next = old - nmass[k] * mass_scale[k] * (vn + target[k]) - impulse_scale[k] * old;
       if (next < 0) next = 0;
       closing_now[k] = vn; solved[k] = next;
     }
@@ -471,17 +489,15 @@ __device__ __forceinline__ void solve_row(const SoftDev& d, const VW& vw, const 
     const double old = impulse[k], next = solved[k] > 0 ? solved[k] : 0;
     const double dd = next - old;
     impulse[k] = next;
-    d.normal_impulse[i] = next;
-    const double most = d.max_normal_impulse[i];
-    if (dd > 0 && most == 0) d.relative_velocity[i] = closing_now[k];
-    if (dd > most) d.max_normal_impulse[i] = dd;
+    // The first push this step records the closing speed; the largest push is kept.
+    if (dd > 0 && most_in[k] == 0) first |= 1u << k;
+    if (dd > most_in[k]) { larger |= 1u << k; pushed[k] = dd; }
     const double px = nx * dd, py = ny * dd, pz = nz * dd;
     s.vax -= ima * px; s.vay -= ima * py; s.vaz -= ima * pz;
     s.vbx += imb * px; s.vby += imb * py; s.vbz += imb * pz;
     s.wax -= jac[6] * dd; s.way -= jac[7] * dd; s.waz -= jac[8] * dd;
     s.wbx += jac[9] * dd; s.wby += jac[10] * dd; s.wbz += jac[11] * dd;
   }
-  const double mu = d.friction[m];
   const double *Ia = d.inv_i + 6 * ga, *Ib = d.inv_i + 6 * gb;
   const double t1x = d.t1[m3], t1y = d.t1[m3 + 1], t1z = d.t1[m3 + 2];
   const double t2x = d.t2[m3], t2y = d.t2[m3 + 1], t2z = d.t2[m3 + 2];
@@ -489,8 +505,10 @@ __device__ __forceinline__ void solve_row(const SoftDev& d, const VW& vw, const 
 #pragma unroll
   for (int k = 0; k < N; k++) {
     pressed += impulse[k];
-    twisted += d.lever[m * P + k] * impulse[k];
+    twisted += lever_in[k] * impulse[k];
   }
+  uint8_t sliding = 0;
+  double t1_out, t2_out, r1_out = 0, r2_out = 0, s_out;
   {
     const double* f = d.fjac + FJ * m;
     const double* g = f + 12;
@@ -499,7 +517,7 @@ __device__ __forceinline__ void solve_row(const SoftDev& d, const VW& vw, const 
         + s.wbx * f[3] + s.wby * f[4] + s.wbz * f[5] - s.wax * f[0] - s.way * f[1] - s.waz * f[2];
     double v2 = dvx * t2x + dvy * t2y + dvz * t2z
         + s.wbx * g[3] + s.wby * g[4] + s.wbz * g[5] - s.wax * g[0] - s.way * g[1] - s.waz * g[2];
-    const double s1 = d.drift1[m], s2 = d.drift2[m];
+    const double s1 = s1_in, s2 = s2_in;
     if (bias && (s1 != 0 || s2 != 0)) {
       double b1 = soft.bias_rate * s1, b2 = soft.bias_rate * s2;
       const double bb = b1 * b1 + b2 * b2;
@@ -509,13 +527,11 @@ __device__ __forceinline__ void solve_row(const SoftDev& d, const VW& vw, const 
       }
       v1 += b1; v2 += b2;
     }
-    const double* tm = d.tangent_mass + m3;
-    const double o1 = d.tangent_impulse1[m], o2 = d.tangent_impulse2[m];
-    double n1 = o1 - (tm[0] * v1 + tm[1] * v2), n2 = o2 - (tm[1] * v1 + tm[2] * v2);
+    double n1 = o1 - (tm0 * v1 + tm1 * v2), n2 = o2 - (tm1 * v1 + tm2 * v2);
     const double limit = mu * pressed, l2 = n1 * n1 + n2 * n2;
     if (l2 > limit * limit) {
       // Newton's method on the trust-region multiplier, as the reference does (src/contact.js:1987-2030).
-      const double ma = tm[0], mb = tm[1], mc = tm[2];
+      const double ma = tm0, mb = tm1, mc = tm2;
       double al = 0, x = n1, y = n2, ll = l2;
       while (limit > 0) {
         const double ab = al * mb, det = (1 + al * ma) * (1 + al * mc) - ab * ab;
@@ -533,11 +549,9 @@ __device__ __forceinline__ void solve_row(const SoftDev& d, const VW& vw, const 
       }
       const double scale = limit / dsqrt(ll);
       n1 = x * scale; n2 = y * scale;
-      d.sliding[m] = 1;
-    } else {
-      d.sliding[m] = 0;
+      sliding = 1;
     }
-    d.tangent_impulse1[m] = n1; d.tangent_impulse2[m] = n2;
+    t1_out = n1; t2_out = n2;
     const double d1 = n1 - o1, d2 = n2 - o2;
     const double px = t1x * d1 + t2x * d2, py = t1y * d1 + t2y * d2, pz = t1z * d1 + t2z * d2;
     s.vax -= ima * px; s.vay -= ima * py; s.vaz -= ima * pz;
@@ -546,17 +560,17 @@ __device__ __forceinline__ void solve_row(const SoftDev& d, const VW& vw, const 
     s.wbx += f[9] * d1 + g[9] * d2; s.wby += f[10] * d1 + g[10] * d2; s.wbz += f[11] * d1 + g[11] * d2;
   }
   if (N <= 2) {
-    const double most = d.patch[m] * pressed;
+    const double most = patch * pressed;
     const double ux = s.wbx - s.wax, uy = s.wby - s.way, uz = s.wbz - s.waz;
-    const double r1 = d.roll_impulse1[m], r2 = d.roll_impulse2[m];
-    double n1 = r1 - d.roll_mass1[m] * (ux * t1x + uy * t1y + uz * t1z);
-    double n2 = r2 - d.roll_mass2[m] * (ux * t2x + uy * t2y + uz * t2z);
+    const double r1 = r1_in, r2 = r2_in;
+    double n1 = r1 - rm1 * (ux * t1x + uy * t1y + uz * t1z);
+    double n2 = r2 - rm2 * (ux * t2x + uy * t2y + uz * t2z);
     const double l2 = n1 * n1 + n2 * n2;
     if (l2 > most * most) {
       const double scale = most / dsqrt(l2);
       n1 *= scale; n2 *= scale;
     }
-    d.roll_impulse1[m] = n1; d.roll_impulse2[m] = n2;
+    r1_out = n1; r2_out = n2;
     const double d1 = n1 - r1, d2 = n2 - r2;
     const double lx = t1x * d1 + t2x * d2, ly = t1y * d1 + t2y * d2, lz = t1z * d1 + t2z * d2;
     s.wax -= Ia[0] * lx + Ia[1] * ly + Ia[2] * lz;
@@ -567,11 +581,11 @@ __device__ __forceinline__ void solve_row(const SoftDev& d, const VW& vw, const 
     s.wbz += Ib[2] * lx + Ib[4] * ly + Ib[5] * lz;
   }
   {
-    const double limit = N == 1 ? SPIN_DISC * mu * d.patch[m] * pressed : mu * twisted;
-    const double s0 = d.spin_impulse[m];
-    double s1 = s0 - d.spin_mass[m] * ((s.wbx - s.wax) * nx + (s.wby - s.way) * ny + (s.wbz - s.waz) * nz);
+    const double limit = N == 1 ? SPIN_DISC * mu * patch * pressed : mu * twisted;
+    const double s0 = s0_in;
+    double s1 = s0 - spin_mass * ((s.wbx - s.wax) * nx + (s.wby - s.way) * ny + (s.wbz - s.waz) * nz);
     if (s1 > limit) s1 = limit; else if (s1 < -limit) s1 = -limit;
-    d.spin_impulse[m] = s1;
+    s_out = s1;
     const double ds = s1 - s0, lx = nx * ds, ly = ny * ds, lz = nz * ds;
     s.wax -= Ia[0] * lx + Ia[1] * ly + Ia[2] * lz;
     s.way -= Ia[1] * lx + Ia[3] * ly + Ia[4] * lz;
@@ -581,6 +595,24 @@ __device__ __forceinline__ void solve_row(const SoftDev& d, const VW& vw, const 
     s.wbz += Ib[2] * lx + Ib[4] * ly + Ib[5] * lz;
   }
   scatter(vw, sa, sb, ima, imb, s);
+  // The row's results, written last.
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    const int i = m * P + k;
+    if (!known && !bias) d.now[i] = now[k];
+    d.normal_impulse[i] = impulse[k];
+    if ((first >> k) & 1) d.relative_velocity[i] = closing_now[k];
+    if ((larger >> k) & 1) d.max_normal_impulse[i] = pushed[k];
+  }
+  d.gap[m] = least;
+  d.sliding[m] = sliding;
+  d.tangent_impulse1[m] = t1_out;
+  d.tangent_impulse2[m] = t2_out;
+  if (N <= 2) {
+    d.roll_impulse1[m] = r1_out;
+    d.roll_impulse2[m] = r2_out;
+  }
+  d.spin_impulse[m] = s_out;
 }
 
 // src/world.js:1495-1530 (_accelerate) for one awake body.
@@ -703,38 +735,81 @@ __device__ __forceinline__ void move_body(const SoftDev& d, const VW& vw, int g,
 }
 
 // src/contact.js:2229-2251 (restitution) for one world, in the order narrow() found
-// the manifolds; one thread, since bounces on a shared body follow each other.
-__device__ __forceinline__ void restitution_world(const SoftDev& d, const VW& vw, int moff, int boff, int count, int solved,
-                                  double gravity) {
-  for (int f = 0; f < count; f++) {
-    const int lm = d.row[moff + f];
-    const int m = moff + lm;
-    const double e = d.restitution[m];
-    if (e == 0 || lm >= solved) continue;
-    const double threshold = dsqrt(2 * gravity * d.tolerance[m]);
-    const int sa = d.a[m], sb = d.b[m], ga = boff + sa, gb = boff + sb;
-    const int m3 = 3 * m;
-    const double nx = d.normal[m3], ny = d.normal[m3 + 1], nz = d.normal[m3 + 2];
-    const double ima = d.inv_mass[ga], imb = d.inv_mass[gb];
-    for (int k = 0; k < d.points[m]; k++) {
-      const int i = m * P + k;
-      if (d.relative_velocity[i] > -threshold || d.max_normal_impulse[i] == 0) continue;
-      const double* jac = d.jac + J * i;
-      double *va = vw.v(sa), *wa = vw.w(sa), *vb = vw.v(sb), *wb = vw.w(sb);
-      const double vn = (vb[0] - va[0]) * nx + (vb[1] - va[1]) * ny + (vb[2] - va[2]) * nz
-          + wb[0] * jac[3] + wb[1] * jac[4] + wb[2] * jac[5] - wa[0] * jac[0] - wa[1] * jac[1] - wa[2] * jac[2];
-      const double old = d.normal_impulse[i];
-      double next = old - d.normal_mass[i] * (vn + e * d.relative_velocity[i]);
-      if (next < 0) next = 0;
-      const double l = next - old;
-      d.normal_impulse[i] = next;
-      // push(), src/contact.js:1743-1749: every component written, as the reference does.
-      const double px = nx * l, py = ny * l, pz = nz * l;
-      va[0] -= ima * px; va[1] -= ima * py; va[2] -= ima * pz;
-      vb[0] += imb * px; vb[1] += imb * py; vb[2] += imb * pz;
-      wa[0] -= jac[6] * l; wa[1] -= jac[7] * l; wa[2] -= jac[8] * l;
-      wb[0] += jac[9] * l; wb[1] += jac[10] * l; wb[2] += jac[11] * l;
+// the manifolds. Bounces on a shared body follow each other, so one lane applies them
+// in that order; but which rows bounce at all (restitution, a point that pushed and
+// met faster than the cut-off) doesn't change while they are applied, so the whole
+// workgroup finds them first, a chunk of rows at a time, and compacts them in order
+// into LDS for that lane. At rest almost no row bounces.
+constexpr int BOUNCE_CHUNK = 512;
+
+__device__ __forceinline__ bool bounces(const SoftDev& d, int m, int lm, int solved, double gravity) {
+  const double e = d.restitution[m];
+  if (e == 0 || lm >= solved) return false;
+  const double threshold = dsqrt(2 * gravity * d.tolerance[m]);
+  for (int k = 0; k < d.points[m]; k++) {
+    const int i = m * P + k;
+    if (!(d.relative_velocity[i] > -threshold || d.max_normal_impulse[i] == 0)) return true;
+  }
+  return false;
+}
+
+__device__ __forceinline__ void bounce_row(const SoftDev& d, const VW& vw, int m, int boff, double gravity) {
+  const double e = d.restitution[m];
+  const double threshold = dsqrt(2 * gravity * d.tolerance[m]);
+  const int sa = d.a[m], sb = d.b[m], ga = boff + sa, gb = boff + sb;
+  const int m3 = 3 * m;
+  const double nx = d.normal[m3], ny = d.normal[m3 + 1], nz = d.normal[m3 + 2];
+  const double ima = d.inv_mass[ga], imb = d.inv_mass[gb];
+  for (int k = 0; k < d.points[m]; k++) {
+    const int i = m * P + k;
+    if (d.relative_velocity[i] > -threshold || d.max_normal_impulse[i] == 0) continue;
+    const double* jac = d.jac + J * i;
+    double *va = vw.v(sa), *wa = vw.w(sa), *vb = vw.v(sb), *wb = vw.w(sb);
+    const double vn = (vb[0] - va[0]) * nx + (vb[1] - va[1]) * ny + (vb[2] - va[2]) * nz
+        + wb[0] * jac[3] + wb[1] * jac[4] + wb[2] * jac[5] - wa[0] * jac[0] - wa[1] * jac[1] - wa[2] * jac[2];
+    const double old = d.normal_impulse[i];
+    double next = old - d.normal_mass[i] * (vn + e * d.relative_velocity[i]);
+    if (next < 0) next = 0;
+    const double l = next - old;
+    d.normal_impulse[i] = next;
+    // push(), src/contact.js:1743-1749: every component written, as the reference does.
+    const double px = nx * l, py = ny * l, pz = nz * l;
+    va[0] -= ima * px; va[1] -= ima * py; va[2] -= ima * pz;
+    vb[0] += imb * px; vb[1] += imb * py; vb[2] += imb * pz;
+    wa[0] -= jac[6] * l; wa[1] -= jac[7] * l; wa[2] -= jac[8] * l;
+    wb[0] += jac[9] * l; wb[1] += jac[10] * l; wb[2] += jac[11] * l;
+  }
+}
+
+__device__ __forceinline__ void restitution_world(const SoftDev& d, const VW& vw, int moff, int boff, int count,
+                                                  int solved, double gravity, int* list, int* waves) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = (blockDim.x + 63) >> 6;
+  for (int base = 0; base < count; base += BOUNCE_CHUNK) {
+    const int end = base + BOUNCE_CHUNK < count ? base + BOUNCE_CHUNK : count;
+    // Each thread looks at consecutive rows f of the chunk; a wave's hits are counted by ballot.
+    int total = 0;
+    for (int f0 = base; f0 < end; f0 += blockDim.x) {
+      const int f = f0 + (int)threadIdx.x;
+      int m = -1;
+      bool hit = false;
+      if (f < end) {
+        const int lm = d.row[moff + f];
+        m = moff + lm;
+        hit = bounces(d, m, lm, solved, gravity);
+      }
+      const uint64_t mask = __ballot(hit);
+      if (lane == 0) waves[wave] = __popcll(mask);
+      __syncthreads();
+      int before = total;
+      for (int k = 0; k < wave; k++) before += waves[k];
+      if (hit) list[before + __popcll(mask & ((1ull << lane) - 1))] = m;
+      for (int k = 0; k < nwaves; k++) total += waves[k];
+      __syncthreads();
     }
+    if (threadIdx.x == 0) {
+      for (int k = 0; k < total; k++) bounce_row(d, vw, list[k], boff, gravity);
+    }
+    __syncthreads();
   }
 }
 
@@ -854,6 +929,8 @@ __device__ __forceinline__ void sweep(const SoftDev& d, const VW& vw, const Soft
 template <bool RESIDENT, int THREADS, int WAVES>
 __global__ void __launch_bounds__(THREADS, WAVES) soft_step_kernel(SoftDev d) {
   __shared__ int starts[BM_STARTS_DEV];
+  __shared__ int bounce_list[BOUNCE_CHUNK];
+  __shared__ int bounce_waves[THREADS / 64];
   extern __shared__ double lds[];
   for (int wld = blockIdx.x; wld < d.n_worlds; wld += gridDim.x) {
     const int boff = d.body_offset[wld], moff = d.manifold_offset[wld], poff = d.pair_offset[wld];
@@ -911,11 +988,8 @@ __global__ void __launch_bounds__(THREADS, WAVES) soft_step_kernel(SoftDev d) {
       if (touching) sweep<2>(d, vw, soft, h, false, moff, boff, starts);
     }
     if (touching) {
-      if (threadIdx.x == 0) {
-        restitution_world(d, vw, moff, boff, count, solved,
-                          dsqrt(grav[0] * grav[0] + grav[1] * grav[1] + grav[2] * grav[2]));
-      }
-      __syncthreads();
+      restitution_world(d, vw, moff, boff, count, solved, dsqrt(grav[0] * grav[0] + grav[1] * grav[1] + grav[2] * grav[2]),
+                        bounce_list, bounce_waves);
       for (int f = threadIdx.x; f < count; f += blockDim.x) {
         const int lm = d.row[moff + f];
         if (lm >= solved) continue;
@@ -938,7 +1012,7 @@ __global__ void __launch_bounds__(THREADS, WAVES) soft_step_kernel(SoftDev d) {
 size_t soft_step_lds_limit(int device) {
   int most = 0;
   if (hipDeviceGetAttribute(&most, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) return 0;
-  const size_t own = BM_STARTS_DEV * sizeof(int) + 64;
+  const size_t own = (BM_STARTS_DEV + BOUNCE_CHUNK + 8) * sizeof(int) + 64;
   return (size_t)most > own ? (size_t)most - own : 0;
 }
 

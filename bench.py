@@ -94,6 +94,9 @@ def main():
     ap.add_argument("--batch-worlds", type=int, default=4096)
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--plain-launch", default=None,
+                    help="shape,resident for the plain world (default: autotuned, bearing_amd/tune.py)")
+    ap.add_argument("--batched-launch", default=None, help="shape,resident for the batch (default: autotuned)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -113,7 +116,10 @@ def main():
     step_bytes = algorithmic_bytes(points, inv_mass[a] > 0, inv_mass[b] > 0)
 
     plain_world = world_from_columns(fx, cols)
-    shape, resident, _ = tune([plain_world])
+    if args.plain_launch:
+        shape, resident = (int(x) for x in args.plain_launch.split(","))
+    else:
+        shape, resident, _ = tune([plain_world])
     ctx = native.Context(0)
     ctx.set_launch(shape, resident)
     batch = native.Batch([plain_world])
@@ -134,7 +140,10 @@ def main():
     sources = [load_world(n)[:2] for n in BATCH_SOURCES]
     worlds = [world_from_columns(sources[i % len(sources)][0], {k: v.copy() for k, v in sources[i % len(sources)][1].items()})
               for i in range(args.batch_worlds)]
-    bshape, bresident, _ = tune(worlds)
+    if args.batched_launch:
+        bshape, bresident = (int(x) for x in args.batched_launch.split(","))
+    else:
+        bshape, bresident, _ = tune(worlds)
     ctx.set_launch(bshape, bresident)
     big = native.Batch(worlds)
     big.upload(ctx)
@@ -144,9 +153,8 @@ def main():
         big.step(ctx)
     batched_wall = time.perf_counter() - t0
     batched_rate = args.batch_worlds * args.batch_steps / batched_wall
-    # The plain drop-in looped once per world, on a sample of the same worlds.
+    # The same kernel looped once per world (one world a launch), on a sample of the same worlds.
     sample = min(64, args.batch_worlds)
-    ctx.set_launch(shape, resident)
     singles = [native.Batch([w]) for w in worlds[:sample]]
     t_loop = 0.0
     for s in singles:
@@ -166,14 +174,14 @@ def main():
         "data": "synthetic: the reference's own state of the seeded 2,000-sphere rain before its step-300 solve",
         "config": {"workload": "rain-2000 soft step (inertias..store, 8 substeps), plain drop-in, one world",
                    "bodies": meta["top"], "manifolds": solved, "rows": rows, "colours": int(np.count_nonzero(np.diff(fx["in.starts"][:33]))),
-                   "kernel_ms": avg_kernel_ms, "launch": {"shape": shape, "resident_lds": resident}},
+                   "kernel_ms": avg_kernel_ms, "launch": {"shape": shape, "resident_lds": bool(resident)}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "algorithmic_bytes_per_launch": step_bytes},
         "cpu_baseline": cpu,
         "batched": {"worlds": args.batch_worlds, "steps": args.batch_steps, "world_steps_per_s": batched_rate,
                     "looped_plain_world_steps_per_s": loop_rate, "speedup_vs_looped": batched_rate / loop_rate,
-                    "worlds_from": BATCH_SOURCES, "launch": {"shape": bshape, "resident_lds": bresident}},
+                    "worlds_from": BATCH_SOURCES, "launch": {"shape": bshape, "resident_lds": bool(bresident)}},
         "bwd_fwd": None,
     }
     print(json.dumps(line))
