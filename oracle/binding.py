@@ -137,3 +137,26 @@ def soft_step(fx, cols=None):
     if rc != 0:
         raise RuntimeError(f"oracle_soft_step returned {rc}")
     return cols
+
+
+def collide(shapes_a, shapes_b, margin, tolerance):
+    """oracle_collide on flat shape records (kind, centre 3, axes 9, half 3, weight 3):
+    (count, normal, points, separations, ids) per pair."""
+    n = len(margin)
+    a = np.ascontiguousarray(shapes_a, np.float64)
+    b = np.ascontiguousarray(shapes_b, np.float64)
+    margin = np.ascontiguousarray(margin, np.float64)
+    tolerance = np.ascontiguousarray(tolerance, np.float64)
+    count = np.zeros(n, np.int32)
+    normal = np.zeros(3 * n)
+    points = np.zeros(12 * n)
+    separations = np.zeros(4 * n)
+    ids = np.zeros(4 * n, np.uint32)
+    f = lib().oracle_collide
+    f.restype = ctypes.c_int
+    rc = f(n, a.ctypes.data_as(_D), b.ctypes.data_as(_D), margin.ctypes.data_as(_D), tolerance.ctypes.data_as(_D),
+           count.ctypes.data_as(_I32), normal.ctypes.data_as(_D), points.ctypes.data_as(_D),
+           separations.ctypes.data_as(_D), ids.ctypes.data_as(_U32))
+    if rc != n:
+        raise ValueError(f"oracle_collide: pair {-rc - 1} is of a kind not restated")
+    return count, normal, points, separations, ids

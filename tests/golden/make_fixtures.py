@@ -47,6 +47,108 @@ def run(name):
                     os.path.join(HERE, "make_fixtures.mjs"), job_path], check=True)
 
 
+def collide_pairs(seed, n):
+    """Seeded shape pairs in the configurations the narrowphase branches on: spheres near
+    and in boxes, boxes stacked flat (exactly and nearly), tilted, turned about the normal,
+    edge on edge, corner down, and apart within and beyond the margin. One record a pair:
+    per shape kind (0 sphere, 2 box), centre 3, unit quaternion 4, half 3, weight 3; then
+    margin, tolerance."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    recs = []
+
+    def quat(kind):
+        if kind == "id":
+            return [0.0, 0.0, 0.0, 1.0]
+        if kind == "yaw":
+            t = rng.uniform(0, 2 * np.pi)
+            return [0.0, np.sin(t / 2), 0.0, np.cos(t / 2)]
+        if kind == "tilt":
+            ax = rng.normal(size=3)
+            ax /= np.linalg.norm(ax)
+            t = rng.normal(0, 0.02)
+            return list(ax * np.sin(t / 2)) + [np.cos(t / 2)]
+        v = rng.normal(size=4)
+        return list(v / np.linalg.norm(v))
+
+    def weight():
+        return [0.0, -9.81, 0.0] if rng.random() < 0.7 else [0.0, 0.0, 0.0]
+
+    for k in range(n):
+        case = k % 8
+        margin = float(rng.choice([0.0, rng.uniform(0, 0.05), rng.uniform(0, 0.4)]))
+        if case == 0:   # sphere - sphere
+            ra, rb = rng.uniform(0.1, 1, 2)
+            d = rng.normal(size=3)
+            d /= np.linalg.norm(d)
+            dist = ra + rb + rng.uniform(-0.3, 0.3) * (ra + rb)
+            if rng.random() < 0.05:
+                dist = 0.0
+            ca = rng.uniform(-3, 3, 3)
+            a = [0, *ca, 0, 0, 0, 1, ra, 0, 0, *weight()]
+            b = [0, *(ca + d * dist), 0, 0, 0, 1, rb, 0, 0, *weight()]
+            tol = min(ra, rb) * 2 / 200
+        elif case in (1, 2):   # sphere - box (either order)
+            h = rng.uniform(0.1, 1.5, 3)
+            r = rng.uniform(0.05, 0.6)
+            qb = quat(rng.choice(["id", "yaw", "any"]))
+            local = rng.uniform(-1.4, 1.4, 3) * (h + r)
+            cb = rng.uniform(-2, 2, 3)
+            R = np.array(_axes(qb)).reshape(3, 3).T
+            cs = cb + R @ local
+            sph = [0, *cs, 0, 0, 0, 1, r, 0, 0, *weight()]
+            box = [2, *cb, *qb, *h, *weight()]
+            a, b = (sph, box) if case == 1 else (box, sph)
+            tol = min(2 * r, 2 * h.min()) / 200
+        else:   # box - box
+            ha = rng.uniform(0.1, 2.0, 3)
+            hb = rng.uniform(0.1, 1.0, 3)
+            if case == 3:   # stacked flat, maybe exactly
+                qa, qb = [0.0, 0, 0, 1], quat(rng.choice(["id", "yaw"]))
+                off = np.array([rng.uniform(-1, 1) * ha[0], 0, rng.uniform(-1, 1) * ha[2]])
+                if rng.random() < 0.5:
+                    off = np.round(off * 4) / 4
+                gap = float(rng.choice([0.0, rng.uniform(-0.02, 0.02), rng.uniform(-0.2, 0.2)]))
+                ca = rng.uniform(-2, 2, 3)
+                cb = ca + off + np.array([0, ha[1] + hb[1] + gap, 0])
+                if rng.random() < 0.3:
+                    ha, hb = np.ones(3) * 0.5, np.ones(3) * 0.5
+                    cb = ca + np.array([float(rng.choice([0.0, 0.5, 1.0])), 1.0 + gap, 0.0])
+            else:
+                qa = quat(["tilt", "any", "yaw", "tilt", "any"][case - 4])
+                qb = quat(["tilt", "any", "any", "any", "tilt"][case - 4])
+                d = rng.normal(size=3)
+                d /= np.linalg.norm(d)
+                reach = np.linalg.norm(ha) * 0.8 + np.linalg.norm(hb) * 0.8
+                ca = rng.uniform(-2, 2, 3)
+                cb = ca + d * reach * rng.uniform(0.5, 1.3)
+            a = [2, *ca, *qa, *ha, *weight()]
+            b = [2, *cb, *qb, *hb, *weight()]
+            tol = 2 * min(ha.min(), hb.min()) / 200
+        recs.append([float(x) for x in a] + [float(x) for x in b] + [margin, tol])
+    return np.array(recs, np.float64)
+
+
+def _axes(q):
+    x, y, z, w = q
+    return [1 - 2 * (y * y + z * z), 2 * (x * y + z * w), 2 * (x * z - y * w),
+            2 * (x * y - z * w), 1 - 2 * (x * x + z * z), 2 * (y * z + x * w),
+            2 * (x * z + y * w), 2 * (y * z - x * w), 1 - 2 * (x * x + y * y)]
+
+
+def run_collide(seed=2026, n=4000):
+    pairs = collide_pairs(seed, n)
+    path = os.path.join(DATA, "collide.in.bin")
+    pairs.tofile(path)
+    subprocess.run(["node", "--no-warnings", "--experimental-loader", os.path.join(HERE, "lower.mjs"),
+                    os.path.join(HERE, "make_collide.mjs"), path, os.path.join(DATA, f"collide.{seed}.bmfx.gz")],
+                   check=True)
+    os.remove(path)
+
+
 if __name__ == "__main__":
+    if sys.argv[1:] == ["collide"]:
+        run_collide()
+        sys.exit(0)
     for name in sys.argv[1:] or [j for j in JOBS if j != "rain"]:
         run(name)

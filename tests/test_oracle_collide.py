@@ -1,0 +1,119 @@
+"""The oracle's narrowphase (spheres and boxes) against the reference.
+
+1. The reference's own hand-worked cases (test/collide.test.js:61-199), to its tolerance of 1e-9.
+2. collide() as the reference computed it on 4,000 seeded pairs (tests/golden collide.*):
+   counts, normals, points, separations and feature ids, bit for bit.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import fixtures
+from collidecheck import compare_manifolds
+from oracle import binding
+
+COLLIDE = fixtures.find("collide") if hasattr(fixtures, "find") else []
+
+
+def test_collide_matches_reference_bit_for_bit():
+    paths = sorted(__import__("glob").glob(fixtures.DATA + "/collide.*.bmfx.gz"))
+    assert paths
+    for path in paths:
+        fx = fixtures.load(path)
+        sh = fx["shapes"].reshape(-1, 2, 19)
+        got = binding.collide(sh[:, 0].copy(), sh[:, 1].copy(), fx["margin"], fx["tolerance"])
+        errs = compare_manifolds(fx, *got)
+        assert not errs, "\n".join(errs)
+
+
+def shape(kind, size=(1, 1, 1), radius=0.5, position=(0, 0, 0), rotation=(0, 0, 0, 1)):
+    """A placed shape record, as the reference's test helper make() builds one."""
+    x, y, z, w = rotation
+    axes = [1 - 2 * (y * y + z * z), 2 * (x * y + z * w), 2 * (x * z - y * w),
+            2 * (x * y - z * w), 1 - 2 * (x * x + z * z), 2 * (y * z + x * w),
+            2 * (x * z + y * w), 2 * (y * z - x * w), 1 - 2 * (x * x + y * y)]
+    half = [size[0] / 2, size[1] / 2, size[2] / 2] if kind == "box" else [radius, 0, 0]
+    return [2.0 if kind == "box" else 0.0, *position, *axes, *half, 0, 0, 0]
+
+
+def turn(axis, radians):
+    l = math.sqrt(sum(a * a for a in axis))
+    s = math.sin(radians / 2)
+    return [a / l * s for a in axis] + [math.cos(radians / 2)]
+
+
+def run(a, b, margin=0.2, tolerance=0.005):
+    count, normal, points, seps, ids = binding.collide(np.array([a]), np.array([b]), [margin], [tolerance])
+    return int(count[0]), normal, [(points[3 * k:3 * k + 3], seps[k]) for k in range(count[0])]
+
+
+def same_points(got, expected):
+    assert len(got) == len(expected)
+    left = list(got)
+    for p, s in expected:
+        i = next(i for i, (g, _) in enumerate(left) if np.linalg.norm(g - np.array(p)) < 1e-9)
+        assert abs(left[i][1] - s) <= 1e-9
+        left.pop(i)
+
+
+def test_hand_worked_sphere_sphere():
+    a = shape("sphere", radius=1)
+    n, normal, pts = run(a, shape("sphere", radius=0.5, position=(2, 0, 0)), 1)
+    assert np.allclose(normal, [1, 0, 0], atol=1e-9)
+    same_points(pts, [([1.25, 0, 0], 0.5)])
+    assert run(a, shape("sphere", radius=0.5, position=(3, 0, 0)), 0.5)[0] == 0
+    same_points(run(a, shape("sphere", radius=0.5, position=(1.2, 0, 0)))[2], [([0.85, 0, 0], -0.3)])
+    n, normal, pts = run(a, shape("sphere", radius=0.5))
+    assert np.allclose(normal, [0, 1, 0])
+    same_points(pts, [([0, 0.25, 0], -1.5)])
+
+
+def test_hand_worked_sphere_box():
+    box = shape("box", size=(2, 2, 2))
+    n, normal, pts = run(shape("sphere", position=(0, 1.6, 0)), box)
+    assert np.allclose(normal, [0, -1, 0])
+    same_points(pts, [([0, 1.05, 0], 0.1)])
+    n, normal, pts = run(shape("sphere", position=(1.3, 1.3, 0)), box)
+    assert np.allclose(normal, [-math.sqrt(0.5), -math.sqrt(0.5), 0], atol=1e-9)
+    assert abs(pts[0][1] - (0.3 * math.sqrt(2) - 0.5)) <= 1e-9
+    n, normal, pts = run(shape("sphere", position=(1.2, 1.2, 1.2)), box)
+    assert abs(pts[0][1] - (0.2 * math.sqrt(3) - 0.5)) <= 1e-9
+    n, normal, pts = run(shape("sphere", position=(0, 0.9, 0)), box)
+    assert np.allclose(normal, [0, -1, 0])
+    same_points(pts, [([0, 0.7, 0], -0.6)])
+    n, normal, pts = run(box, shape("sphere", position=(0, 1.6, 0)))
+    assert np.allclose(normal, [0, 1, 0])
+    same_points(pts, [([0, 1.05, 0], 0.1)])
+    n, normal, pts = run(shape("sphere", position=(0, 1.6, 0)), shape("box", size=(2, 2, 2), rotation=turn([0, 1, 0], 0.7)))
+    same_points(pts, [([0, 1.05, 0], 0.1)])
+
+
+def test_hand_worked_box_box():
+    a = shape("box", size=(2, 2, 2))
+    n, normal, pts = run(a, shape("box", size=(2, 2, 2), position=(0, 2.1, 0)))
+    assert np.allclose(normal, [0, 1, 0])
+    same_points(pts, [([-1, 1.05, -1], 0.1), ([1, 1.05, -1], 0.1), ([1, 1.05, 1], 0.1), ([-1, 1.05, 1], 0.1)])
+    big, small = shape("box", size=(4, 1, 4)), shape("box", size=(1, 1, 1), position=(0, 1, 0))
+    corners = [[-0.5, 0.5, -0.5], [0.5, 0.5, -0.5], [0.5, 0.5, 0.5], [-0.5, 0.5, 0.5]]
+    n, normal, pts = run(big, small)
+    assert np.allclose(normal, [0, 1, 0])
+    same_points(pts, [(p, 0) for p in corners])
+    n, normal, pts = run(small, big)
+    assert np.allclose(normal, [0, -1, 0])
+    same_points(pts, [(p, 0) for p in corners])
+    n, normal, pts = run(a, shape("box", size=(2, 2, 2), position=(0, 2.05, 0), rotation=turn([0, 1, 0], math.pi / 4)))
+    assert np.allclose(normal, [0, 1, 0]) and n == 4
+    for p, s in pts:
+        assert abs(s - 0.05) <= 1e-9 and abs(p[0]) <= 1 + 1e-9 and abs(p[2]) <= 1 + 1e-9
+    ridge = math.sqrt(2)
+    n, normal, pts = run(shape("box", size=(2, 2, 2), rotation=turn([0, 0, 1], math.pi / 4)),
+                         shape("box", size=(2, 2, 2), position=(0, 2.8, 0), rotation=turn([1, 0, 0], math.pi / 4)))
+    assert np.allclose(normal, [0, 1, 0], atol=1e-9)
+    same_points(pts, [([0, 1.4, 0], 2.8 - 2 * ridge)])
+    up = [1 / math.sqrt(3)] * 3
+    corner = turn([up[2], 0, -up[0]], math.acos(up[1]) + math.pi)
+    n, normal, pts = run(big, shape("box", size=(1, 1, 1), position=(0, 0.5 + math.sqrt(3) / 2 + 0.01, 0), rotation=corner), 0.05)
+    assert n == 1
+    assert abs(pts[0][1] - 0.01) <= 1e-12
+    assert np.allclose(pts[0][0], [0, 0.505, 0], atol=1e-12)
