@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/bearing_mi355x.h"
+#include "collide.h"
 #include "colour.h"
 #include "soft.h"
 
@@ -109,6 +110,7 @@ struct bm_context {
   void* spare[NCOL] = {};   // arrange's second buffers
   size_t spare_cap[NCOL] = {};
   Scratch deg, adj, keys, key_offset, world_of;
+  Scratch pairs_in, pairs_out;
   std::vector<int32_t> key_offsets;
 };
 
@@ -242,7 +244,8 @@ int bm_destroy(bm_context* ctx) {
     if (ctx->buffer[c]) (void)hipFree(ctx->buffer[c]);
     if (ctx->spare[c]) (void)hipFree(ctx->spare[c]);
   }
-  for (Scratch* s : {&ctx->deg, &ctx->adj, &ctx->keys, &ctx->key_offset, &ctx->world_of})
+  for (Scratch* s : {&ctx->deg, &ctx->adj, &ctx->keys, &ctx->key_offset, &ctx->world_of, &ctx->pairs_in,
+                     &ctx->pairs_out})
     if (s->p) (void)hipFree(s->p);
   if (ctx->begin) (void)hipEventDestroy(ctx->begin);
   if (ctx->end) (void)hipEventDestroy(ctx->end);
@@ -435,6 +438,58 @@ int bm_colour(bm_context* ctx) {
   HIP_OR_FAIL(ctx, hipGetLastError(), "bm_colour: rest");
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_colour: kernel");
   ctx->laid_out = true;
+  return BM_OK;
+}
+
+int bm_collide_host(bm_context* ctx, int32_t n, const double* a, const double* b, const double* margin,
+                    const double* tolerance, int32_t* count, double* normal, double* points, double* separations,
+                    uint32_t* ids) {
+  if (!ctx) return BM_ERR_ARGUMENT;
+  ctx->error.clear();
+  if (n < 0) return fail(ctx, BM_ERR_ARGUMENT, "bm_collide_host: n must be 0 or more");
+  if (n == 0) return BM_OK;
+  if (!a || !b || !margin || !tolerance || !count || !normal || !points || !separations || !ids)
+    return fail(ctx, BM_ERR_ARGUMENT, "bm_collide_host: null argument");
+  for (int32_t k = 0; k < n; k++) {
+    const double ka = a[19 * (size_t)k], kb = b[19 * (size_t)k];
+    if (!(ka == 0 || ka == 1 || ka == 2 || ka == 3) || !(kb == 0 || kb == 1 || kb == 2 || kb == 3))
+      return fail(ctx, BM_ERR_ARGUMENT, "bm_collide_host: a shape's kind must be 0, 1, 2 or 3");
+  }
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->device), "bm_collide_host: hipSetDevice");
+  const size_t shapes = 19 * sizeof(double) * (size_t)n, scalars = sizeof(double) * (size_t)n;
+  const size_t in_bytes = 2 * shapes + 2 * scalars;
+  const size_t out_bytes = (size_t)n * (sizeof(int32_t) + 3 * 8 + 12 * 8 + 4 * 8 + 4 * 4) + 64;
+  if (int rc = grow(ctx, ctx->pairs_in, in_bytes, "bm_collide_host")) return rc;
+  if (int rc = grow(ctx, ctx->pairs_out, out_bytes, "bm_collide_host")) return rc;
+  char* in = static_cast<char*>(ctx->pairs_in.p);
+  char* out = static_cast<char*>(ctx->pairs_out.p);
+  bm::CollideDev d{};
+  d.n = n;
+  d.a = reinterpret_cast<const double*>(in);
+  d.b = reinterpret_cast<const double*>(in + shapes);
+  d.margin = reinterpret_cast<const double*>(in + 2 * shapes);
+  d.tolerance = reinterpret_cast<const double*>(in + 2 * shapes + scalars);
+  d.normal = reinterpret_cast<double*>(out);
+  d.points = reinterpret_cast<double*>(out + 3 * 8 * (size_t)n);
+  d.separations = reinterpret_cast<double*>(out + 15 * 8 * (size_t)n);
+  d.count = reinterpret_cast<int32_t*>(out + 19 * 8 * (size_t)n);
+  d.ids = reinterpret_cast<uint32_t*>(out + 19 * 8 * (size_t)n + 4 * (size_t)n);
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(const_cast<double*>(d.a), a, shapes, hipMemcpyHostToDevice, ctx->stream), "bm_collide_host: copy");
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(const_cast<double*>(d.b), b, shapes, hipMemcpyHostToDevice, ctx->stream), "bm_collide_host: copy");
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(const_cast<double*>(d.margin), margin, scalars, hipMemcpyHostToDevice, ctx->stream), "bm_collide_host: copy");
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(const_cast<double*>(d.tolerance), tolerance, scalars, hipMemcpyHostToDevice, ctx->stream), "bm_collide_host: copy");
+  (void)hipGetLastError();
+  HIP_OR_FAIL(ctx, hipEventRecord(ctx->begin, ctx->stream), "bm_collide_host: hipEventRecord");
+  bm::launch_collide(d, ctx->stream);
+  HIP_OR_FAIL(ctx, hipGetLastError(), "bm_collide_host: launch");
+  HIP_OR_FAIL(ctx, hipEventRecord(ctx->end, ctx->stream), "bm_collide_host: hipEventRecord");
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(count, d.count, 4 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream), "bm_collide_host: copy");
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(normal, d.normal, 24 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream), "bm_collide_host: copy");
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(points, d.points, 96 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream), "bm_collide_host: copy");
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(separations, d.separations, 32 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream), "bm_collide_host: copy");
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(ids, d.ids, 16 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream), "bm_collide_host: copy");
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_collide_host: kernel");
+  HIP_OR_FAIL(ctx, hipEventElapsedTime(&ctx->last_ms, ctx->begin, ctx->end), "bm_collide_host: hipEventElapsedTime");
   return BM_OK;
 }
 

@@ -1,0 +1,19 @@
+// Device-side view of a batch of shape pairs for the narrowphase (collide.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bm {
+
+struct CollideDev {
+  int n;
+  const double *a, *b;             // shape records: kind, centre 3, axes 9, half 3, weight 3
+  const double *margin, *tolerance;
+  int* count;                      // -1 for a pair of kinds not handled (capsules, hulls)
+  double *normal, *points, *separations;
+  uint32_t* ids;
+};
+
+void launch_collide(const CollideDev& d, hipStream_t stream);
+
+}  // namespace bm

@@ -57,7 +57,7 @@ class Columns(ctypes.Structure):
 
 
 EXPORTS = ["bm_create", "bm_destroy", "bm_last_error", "bm_upload", "bm_soft_step", "bm_download",
-           "bm_soft_step_host", "bm_last_kernel_ms", "bm_set_launch", "bm_lds_limit", "bm_colour"]
+           "bm_soft_step_host", "bm_last_kernel_ms", "bm_set_launch", "bm_lds_limit", "bm_colour", "bm_collide_host"]
 
 _lib = None
 
@@ -78,6 +78,7 @@ def load():
             getattr(lib, name).argtypes = [vp, ctypes.POINTER(Columns)]
         lib.bm_soft_step.argtypes = [vp]
         lib.bm_colour.argtypes = [vp]
+        lib.bm_collide_host.argtypes = [vp, ctypes.c_int32, _D, _D, _D, _D, _I32, _D, _D, _D, _U32]
         lib.bm_last_kernel_ms.argtypes = [vp]
         lib.bm_last_kernel_ms.restype = ctypes.c_double
         lib.bm_set_launch.argtypes = [vp, ctypes.c_int32, ctypes.c_int32]
@@ -121,6 +122,24 @@ class Context:
             self.close()
         except Exception:
             pass
+
+
+def collide(ctx, a, b, margin, tolerance):
+    """bm_collide_host on flat shape records: (count, normal, points, separations, ids) per pair."""
+    n = len(margin)
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    margin = np.ascontiguousarray(margin, np.float64)
+    tolerance = np.ascontiguousarray(tolerance, np.float64)
+    count = np.zeros(n, np.int32)
+    normal = np.zeros(3 * n)
+    points = np.zeros(12 * n)
+    separations = np.zeros(4 * n)
+    ids = np.zeros(4 * n, np.uint32)
+    p = lambda x, k: x.ctypes.data_as(k)
+    ctx.check(ctx.lib.bm_collide_host(ctx.ptr, n, p(a, _D), p(b, _D), p(margin, _D), p(tolerance, _D), p(count, _I32),
+                                      p(normal, _D), p(points, _D), p(separations, _D), p(ids, _U32)), "bm_collide_host")
+    return count, normal, points, separations, ids
 
 
 class Batch:

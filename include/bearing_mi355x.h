@@ -149,6 +149,20 @@ int bm_download(bm_context* ctx, bm_columns* host);
  * contact phase calls with the world's own arrays. */
 int bm_soft_step_host(bm_context* ctx, bm_columns* host);
 
+/*
+ * The narrowphase on the device, one pair a lane: src/collide.js collide(a, b,
+ * margin, tolerance, out) (204-242) for spheres and boxes (349-403, 713-1229), the
+ * shapes of pair k given as records a[19k..], b[19k..] = kind (0 sphere, 2 box),
+ * centre 3, axes 9 (the shape's x, y, z axes in world space), half 3 (box: half
+ * extents; sphere: radius), weight 3 (gravity times gravityScale for a dynamic
+ * body, else 0; reducePoints reads it). Out, per pair: count (0-4; -1 for a
+ * capsule or hull pair, not handled), normal 3 (a to b), points 12, separations 4,
+ * ids 4 (the reference's feature ids). Bit-identical to the reference's collide().
+ */
+int bm_collide_host(bm_context* ctx, int32_t n, const double* a, const double* b, const double* margin,
+                    const double* tolerance, int32_t* count, double* normal, double* points, double* separations,
+                    uint32_t* ids);
+
 /* How bm_soft_step launches, for tuning (bearing_amd/tune.py picks per workload):
  * shape 0 = 256 threads a workgroup with registers for one wave a SIMD, 1 = 256
  * threads and two waves a SIMD, 2 = 512 threads and two waves; resident = 1 keeps
