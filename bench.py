@@ -97,7 +97,18 @@ def main():
     ap.add_argument("--plain-launch", default=None,
                     help="shape,resident for the plain world (default: autotuned, bearing_amd/tune.py)")
     ap.add_argument("--batched-launch", default=None, help="shape,resident for the batch (default: autotuned)")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only copy this many MiB device to device 5 times (torch), to calibrate HBM byte counters")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        x = torch.ones(args.copy_probe * 1024 * 1024 // 8, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_mib": args.copy_probe, "bytes_read": x.numel() * 8, "bytes_written": y.numel() * 8}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(worlds shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
@@ -167,6 +178,15 @@ def main():
     ctx.close()
 
     cpu = cpu_baseline(fx, args.cpu_seconds)
+    # HBM bytes per launch of the plain kernel, from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+    # recorded in profiles/ (scaled by a copy kernel of known size); null if not measured
+    # for this launch shape.
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "soft_step_rain2000_traffic.json")
+    if os.path.exists(tpath):
+        rec = json.load(open(tpath))
+        if rec.get("kernel") == f"soft_step_kernel<{'true' if resident else 'false'}, {512 if shape == 2 else 256}, {1 if shape == 0 else 2}>":
+            traffic = rec["bytes"]
     line = {
         "metric": "contact-iters/s", "value": value, "unit": "contact-iterations/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
@@ -176,7 +196,7 @@ def main():
                    "bodies": meta["top"], "manifolds": solved, "rows": rows, "colours": int(np.count_nonzero(np.diff(fx["in.starts"][:33]))),
                    "kernel_ms": avg_kernel_ms, "launch": {"shape": shape, "resident_lds": bool(resident)}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": step_bytes},
         "cpu_baseline": cpu,
         "batched": {"worlds": args.batch_worlds, "steps": args.batch_steps, "world_steps_per_s": batched_rate,
