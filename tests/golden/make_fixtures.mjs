@@ -103,6 +103,11 @@ function main() {
   const scene = JSON.parse(readFileSync(job.scene, 'utf8'));
   const world = new World({ workers: 0 });
   for (const spec of scene) world.addBody(spec);
+  // The bodies as addBody resolved them, before any step: what a binding hands the device world.
+  const init = {};
+  bodies(world, ['_position', '_rotation', '_velocity', '_angular', '_dims', '_mass', '_invMass', '_inertia',
+    '_invInertia', '_type', '_shape', '_framed', '_lock', '_gravityScale', '_damping', '_angularDamping', '_size',
+    '_reach', '_friction', '_restitution', '_tumbled', '_ceiling'], 'init.', init);
   const softAt = new Set(job.soft || []), colourAt = new Set(job.colour || []), traceAt = job.trace || -1;
   let rec = null, meta = null, trace = null, traceCount = 0, colourRec = null;
   const snap = (tag) => {
@@ -162,7 +167,7 @@ function main() {
   bodies(world, ['_position', '_rotation', '_velocity', '_angular'], 'final.', final);
   writeFileSync(`${job.out}.traj.bmfx.gz`, pack(Object.assign({ 'awake': counts.awake, 'contacts': counts.contacts,
     'frozen': counts.frozen },
-    kept, final), { steps: job.steps, dt: job.dt, hashes, bodies: scene.length, impacts: world._impacts,
+    kept, final, init), { steps: job.steps, dt: job.dt, hashes, bodies: scene.length, impacts: world._impacts,
     catches: world._catches, solverSteps: world._solverSteps }));
 }
 
