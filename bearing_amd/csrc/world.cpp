@@ -1,0 +1,1135 @@
+// A whole world stepped as the reference steps it (src/world.js step, 880-1072), for
+// bodies that are boxes and spheres, no joints, characters, sensors or compounds.
+//
+// Which parts run where. The order-defining serial bookkeeping stays on the host, in the
+// reference's exact order: the AABB trees and fat boxes (src/broadphase.js, src/tree.js),
+// the pair list with its swap-on-remove and awake/asleep runs and manifold cache
+// (src/pairs.js), the narrow driver's visit order, reuse decisions, emit / carry /
+// remember and mid-step wakes (src/contact.js:296-1036), freezing, touched and sleeping
+// islands (src/world.js:1116-1343). The device does the per-pair narrowphase (collide,
+// in batches between the points where a result can wake an island), the colouring and
+// row layout, and the whole soft step (colour.hip, collide.hip, soft.hip). Every
+// arithmetic step is the reference's, so a world's transforms are the reference's bits.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/bearing_mi355x.h"
+#include "tree.h"
+
+namespace {
+
+constexpr int P = 4;
+constexpr int DYNAMIC = 0, STATIC = 1, KINEMATIC = 2, CHARACTER = 3;
+constexpr int BOX = 0, SPHERE = 1;
+constexpr int MOVING_TREE = 1, STATIC_TREE = 0;
+constexpr double PIXEL = 1.0 / 200;      // src/contact.js:125
+constexpr double FAT = 1.0 / 4;          // src/broadphase.js:32
+constexpr double AHEAD = 16;             // src/broadphase.js:54
+constexpr double SLEEP_TIME = 1;         // src/world.js:154
+constexpr int STILL = 1, QUIET = 2, CALM = 4, HANDLED = 8, THAWING = 16;  // src/world.js:159
+
+inline double js_max(double a, double b) {
+  if (a != a || b != b) return NAN;
+  if (a > b) return a;
+  if (b > a) return b;
+  return std::signbit(a) ? b : a;
+}
+inline double js_min(double a, double b) {
+  if (a != a || b != b) return NAN;
+  if (a < b) return a;
+  if (b < a) return b;
+  return std::signbit(a) ? a : b;
+}
+
+// src/core/math.js:223-228
+void axes_of(const double* q, double* ax) {
+  const double x = q[0], y = q[1], z = q[2], w = q[3];
+  ax[0] = 1 - 2 * (y * y + z * z); ax[1] = 2 * (x * y + z * w); ax[2] = 2 * (x * z - y * w);
+  ax[3] = 2 * (x * y - z * w); ax[4] = 1 - 2 * (x * x + z * z); ax[5] = 2 * (y * z + x * w);
+  ax[6] = 2 * (x * z + y * w); ax[7] = 2 * (y * z - x * w); ax[8] = 1 - 2 * (x * x + y * y);
+}
+
+// The pair list (src/pairs.js:40-378): [0, active) awake, [active, count) asleep.
+struct Pairs {
+  int count = 0, active = 0;
+  std::vector<int32_t> a, b, cache;
+  std::vector<uint8_t> points, touching;
+  std::vector<uint32_t> pid;
+  std::vector<double> normal_impulse, friction, slip, twist;
+  std::unordered_map<uint64_t, int> index;
+  // The manifold cache (REUSE, src/contact.js:387-467), a row per cached pair.
+  int cached = 0;
+  std::vector<int32_t> owner;
+  std::vector<uint8_t> cache_points;
+  std::vector<double> cache_pose, cache_loss, cache_normal, cache_local;
+  std::vector<uint32_t> cache_id;
+
+  static uint64_t key(int x, int y) { return (uint64_t)x * 16777216ull + (uint64_t)y; }
+
+  void grow(int n) {
+    if ((int)a.size() >= n) return;
+    size_t c = a.size() < 16 ? 16 : a.size();
+    while ((int)c < n) c *= 2;
+    a.resize(c); b.resize(c); cache.resize(c); points.resize(c); touching.resize(c);
+    pid.resize(P * c); normal_impulse.resize(P * c); friction.resize(3 * c); slip.resize(3 * c); twist.resize(3 * c);
+  }
+  void cache_grow(int n) {
+    if ((int)owner.size() >= n) return;
+    size_t c = owner.size() < 16 ? 16 : owner.size();
+    while ((int)c < n) c *= 2;
+    owner.resize(c); cache_points.resize(c); cache_pose.resize(7 * c); cache_loss.resize(c);
+    cache_normal.resize(3 * c); cache_local.resize(6 * P * c); cache_id.resize(P * c);
+  }
+  // src/pairs.js:243-251
+  int cache_of(int p) {
+    if (cache[p] != 0) return cache[p] - 1;
+    const int r = cached++;
+    cache_grow(cached);
+    owner[r] = p;
+    cache[p] = r + 1;
+    return r;
+  }
+  // src/pairs.js:253-268
+  void uncache(int p) {
+    const int r = cache[p] - 1, last = --cached;
+    cache[p] = 0;
+    if (r == last) return;
+    const int q = owner[last];
+    owner[r] = q;
+    cache[q] = r + 1;
+    cache_points[r] = cache_points[last];
+    std::memmove(&cache_pose[7 * r], &cache_pose[7 * last], 7 * sizeof(double));
+    cache_loss[r] = cache_loss[last];
+    std::memmove(&cache_normal[3 * r], &cache_normal[3 * last], 3 * sizeof(double));
+    std::memmove(&cache_local[6 * P * r], &cache_local[6 * P * last], 6 * P * sizeof(double));
+    std::memmove(&cache_id[P * r], &cache_id[P * last], P * sizeof(uint32_t));
+  }
+  // src/pairs.js:271-275 (events are not kept: the flag alone)
+  void part(int p) { touching[p] = 0; }
+  // src/pairs.js:234-240
+  void forget(int p) {
+    points[p] = 0;
+    if (cache[p] != 0) uncache(p);
+    part(p);
+  }
+  // src/pairs.js:303-318
+  void move(int from, int to) {
+    a[to] = a[from]; b[to] = b[from];
+    points[to] = points[from]; touching[to] = touching[from]; cache[to] = cache[from];
+    if (cache[to] != 0) owner[cache[to] - 1] = to;
+    for (int k = 0; k < P; k++) { pid[P * to + k] = pid[P * from + k]; normal_impulse[P * to + k] = normal_impulse[P * from + k]; }
+    for (int k = 0; k < 3; k++) {
+      friction[3 * to + k] = friction[3 * from + k]; slip[3 * to + k] = slip[3 * from + k]; twist[3 * to + k] = twist[3 * from + k];
+    }
+    index[key(a[to], b[to])] = to;
+  }
+  // src/pairs.js:321-334
+  void swap(int i, int j) {
+    if (i == j) return;
+    std::swap(a[i], a[j]); std::swap(b[i], b[j]);
+    std::swap(points[i], points[j]); std::swap(touching[i], touching[j]); std::swap(cache[i], cache[j]);
+    if (cache[i] != 0) owner[cache[i] - 1] = i;
+    if (cache[j] != 0) owner[cache[j] - 1] = j;
+    for (int k = 0; k < P; k++) { std::swap(pid[P * i + k], pid[P * j + k]); std::swap(normal_impulse[P * i + k], normal_impulse[P * j + k]); }
+    for (int k = 0; k < 3; k++) {
+      std::swap(friction[3 * i + k], friction[3 * j + k]); std::swap(slip[3 * i + k], slip[3 * j + k]);
+      std::swap(twist[3 * i + k], twist[3 * j + k]);
+    }
+    index[key(a[i], b[i])] = i;
+    index[key(a[j], b[j])] = j;
+  }
+  void wake(int p) { swap(p, active); active++; }
+  void sleep(int p) { active--; swap(p, active); }
+  // src/pairs.js:217-232
+  void remove(int p) {
+    part(p);
+    if (cache[p] != 0) uncache(p);
+    index.erase(key(a[p], b[p]));
+    if (p < active) {
+      const int last = --active;
+      if (p != last) move(last, p);
+      p = last;
+    }
+    const int end = --count;
+    if (p != end) move(end, p);
+  }
+};
+
+// A manifold as collide() or reuse leaves it (src/collide.js Manifold).
+struct Manifold {
+  int count = 0;
+  double normal[3] = {0, 0, 0};
+  double points[12] = {};
+  double separations[4] = {};
+  uint32_t ids[4] = {};
+  double loss = 0;
+};
+
+// This step's manifolds in the order narrow() found them (src/contact.js Contacts).
+struct Contacts {
+  int count = 0;
+  std::vector<int32_t> a, b, pair, sub, row, colour, order;
+  std::vector<uint8_t> points, sliding;
+  std::vector<double> normal, t1, t2, friction, restitution, tolerance, patch, roll_impulse1, roll_impulse2, spin_impulse,
+      tangent_impulse1, tangent_impulse2, drift1, drift2, separation, point, shift, normal_impulse, max_normal_impulse,
+      relative_velocity, gap, now, ra, rb, la, lb, adjusted, normal_mass, jac, block, lever, anchor_a, anchor_b, fjac,
+      tangent_mass, roll_mass1, roll_mass2, spin_mass;
+  std::vector<uint32_t> pid;
+  void ensure(int n) {
+    if ((int)a.size() >= n) return;
+    size_t c = a.size() < 16 ? 16 : a.size();
+    while ((int)c < n) c *= 2;
+    for (auto* v : {&a, &b, &pair, &sub, &row, &colour, &order}) v->resize(c);
+    points.resize(c); sliding.resize(c);
+    for (auto* v : {&friction, &restitution, &tolerance, &patch, &roll_impulse1, &roll_impulse2, &spin_impulse,
+                    &tangent_impulse1, &tangent_impulse2, &drift1, &drift2, &gap, &roll_mass1, &roll_mass2, &spin_mass})
+      v->resize(c);
+    for (auto* v : {&normal, &t1, &t2, &anchor_a, &anchor_b, &tangent_mass}) v->resize(3 * c);
+    for (auto* v : {&separation, &normal_impulse, &max_normal_impulse, &relative_velocity, &now, &adjusted, &normal_mass,
+                    &lever})
+      v->resize(P * c);
+    for (auto* v : {&point, &shift, &ra, &rb, &la, &lb}) v->resize(12 * c);
+    jac.resize(48 * c); block.resize(16 * c); fjac.resize(24 * c);
+    pid.resize(P * c);
+  }
+};
+
+}  // namespace
+
+struct bm_world {
+  bm_context* ctx = nullptr;
+  std::string error;
+  double gravity[3] = {0, -9.81, 0};
+  int substeps = 8;
+  bool sleeping = true;
+  int top = 0;
+  // Body columns by slot (src/world.js:228-311), the subset this path reads and writes.
+  std::vector<double> position, rotation, velocity, angular, dims, mass, inv_mass, inertia, inv_inertia, inv_i, axes,
+      box, motion, start, start_q, tumbled, ceiling, gravity_scale, damping, angular_damping, size, reach, friction,
+      restitution, still, rest, least;
+  std::vector<uint8_t> type, shape, frozen, calm, lock, framed, awake, fast;
+  std::vector<int32_t> joint_head, in_awake, awake_list, island, island_next, parent, slept;
+  std::vector<float> transforms;
+  int awake_count = 0, frozen_count = 0, slept_count = 0;
+  bool still_ready = false, wake_pending = false;
+  double step_dt = 0;
+  int step_count = 0;
+  long impacts = 0, solver_steps = 0;
+  // Broadphase (src/broadphase.js:56-209).
+  bm::Tree trees[2];
+  std::vector<int32_t> leaf, moved;
+  std::vector<int8_t> which;
+  std::vector<uint8_t> is_moved;
+  Pairs pairs;
+  Contacts c;
+  // Narrowphase batches for the device.
+  std::vector<double> batch_a, batch_b, batch_margin, batch_tolerance, batch_normal, batch_points, batch_seps;
+  std::vector<int32_t> batch_count;
+  std::vector<uint32_t> batch_ids;
+};
+
+namespace {
+
+int fail(bm_world* w, int code, const std::string& m) {
+  w->error = m;
+  return code;
+}
+
+// ---- bodies ----
+
+// src/world.js:1580-1630 (_boundsOf) for a box or a sphere.
+void bounds_of(bm_world* w, int s, double dt, double gl) {
+  const int s3 = 3 * s, s4 = 4 * s, a9 = 9 * s, b6 = 6 * s;
+  double* ax = &w->axes[a9];
+  axes_of(&w->rotation[s4], ax);
+  double m = 0;
+  if (w->type[s] != STATIC) {
+    const double* v = &w->velocity[s3];
+    const double* om = &w->angular[s3];
+    const double speed = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2])
+        + (w->type[s] == DYNAMIC ? gl * std::fabs(w->gravity_scale[s]) * dt : 0)
+        + std::sqrt(om[0] * om[0] + om[1] * om[1] + om[2] * om[2]) * w->reach[s];
+    m = speed * dt;
+  }
+  w->motion[s] = m;
+  double ex, ey, ez;
+  if (w->shape[s] == BOX) {
+    const double hx = w->dims[s3] * 0.5, hy = w->dims[s3 + 1] * 0.5, hz = w->dims[s3 + 2] * 0.5;
+    ex = std::fabs(ax[0]) * hx + std::fabs(ax[3]) * hy + std::fabs(ax[6]) * hz;
+    ey = std::fabs(ax[1]) * hx + std::fabs(ax[4]) * hy + std::fabs(ax[7]) * hz;
+    ez = std::fabs(ax[2]) * hx + std::fabs(ax[5]) * hy + std::fabs(ax[8]) * hz;
+  } else {
+    ex = ey = ez = w->dims[s3];
+  }
+  const double grow = m + w->size[s] * PIXEL;
+  const double cx = w->position[s3] + 0, cy = w->position[s3 + 1] + 0, cz = w->position[s3 + 2] + 0;
+  double* box = &w->box[b6];
+  box[0] = cx - ex - grow; box[1] = cy - ey - grow; box[2] = cz - ez - grow;
+  box[3] = cx + ex + grow; box[4] = cy + ey + grow; box[5] = cz + ez + grow;
+}
+
+double gravity_length(const bm_world* w) {
+  const double* g = w->gravity;
+  return std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+}
+
+// src/world.js:1301-1308 (_settle)
+void settle(bm_world* w, int s) {
+  w->still[s] = 0;
+  for (int k = 0; k < 3; k++) w->rest[7 * s + k] = w->position[3 * s + k];
+  for (int k = 0; k < 4; k++) w->rest[7 * s + 3 + k] = w->rotation[4 * s + k];
+}
+
+// src/world.js:1290-1296 (_awaken)
+void awaken(bm_world* w, int s) {
+  if (w->in_awake[s] != 0) return;
+  w->awake_list[w->awake_count] = s;
+  w->in_awake[s] = ++w->awake_count;
+  w->awake[s] = 1;
+  settle(w, s);
+}
+
+// src/world.js:1311-1320 (_unawaken)
+void unawaken(bm_world* w, int s) {
+  const int i = w->in_awake[s] - 1;
+  if (i < 0) return;
+  const int last = w->awake_list[--w->awake_count];
+  w->awake_list[i] = last;
+  w->in_awake[last] = i + 1;
+  w->in_awake[s] = 0;
+  w->awake[s] = 0;
+}
+
+// src/world.js:1175-1180 (_thaw)
+void thaw(bm_world* w, int s, bool afresh) {
+  w->frozen[s] = 0;
+  w->frozen_count--;
+  w->inv_mass[s] = 1 / w->mass[s];
+  if (afresh) settle(w, s);
+}
+
+// src/world.js:1326-1343 (_wake)
+void wake(bm_world* w, int s) {
+  if (w->island[s] == 0) {
+    w->calm[s] |= HANDLED;
+    if (w->frozen[s] != 0) thaw(w, s, true);
+    return;
+  }
+  const double dt = w->step_dt;
+  const double gl = dt > 0 ? gravity_length(w) : 0;
+  for (int b = w->island[s] - 1; b >= 0;) {
+    const int next = w->island_next[b] - 1;
+    w->island[b] = 0;
+    w->island_next[b] = 0;
+    awaken(w, b);
+    if (dt > 0) bounds_of(w, b, dt, gl);
+    b = next;
+  }
+  w->wake_pending = true;
+}
+
+// ---- broadphase (src/broadphase.js) ----
+
+void fat(bm_world* w, int s, int tree, double time, double* out) {
+  const double* box = &w->box[6 * s];
+  if (tree != MOVING_TREE) {
+    for (int k = 0; k < 6; k++) out[k] = box[k];
+    return;
+  }
+  const double m = w->size[s] * FAT;
+  for (int k = 0; k < 3; k++) {
+    const double d = w->velocity[3 * s + k] * time;
+    out[k] = box[k] - m + (d < 0 ? d : 0);
+    out[k + 3] = box[k + 3] + m + (d > 0 ? d : 0);
+  }
+}
+
+void mark(bm_world* w, int s) {
+  if (w->is_moved[s]) return;
+  w->is_moved[s] = 1;
+  w->moved.push_back(s);
+}
+
+void broadphase_add(bm_world* w, int s) {
+  const int t = w->type[s] == STATIC ? STATIC_TREE : MOVING_TREE;
+  double out[6];
+  fat(w, s, t, 0, out);
+  w->leaf[s] = w->trees[t].insert(s, out);
+  w->which[s] = (int8_t)t;
+  mark(w, s);
+}
+
+// src/broadphase.js:119-124 (settle: a body fallen asleep keeps a fat box with no lead)
+void broadphase_settle(bm_world* w, int s) {
+  if (w->which[s] != MOVING_TREE) return;
+  double out[6];
+  fat(w, s, MOVING_TREE, 0, out);
+  w->trees[MOVING_TREE].move(w->leaf[s], out);
+}
+
+// src/broadphase.js:140-159
+void broadphase_update(bm_world* w, double dt) {
+  bm::Tree& tree = w->trees[MOVING_TREE];
+  for (int i = 0; i < w->awake_count; i++) {
+    const int s = w->awake_list[i];
+    if (w->which[s] != MOVING_TREE) continue;
+    const double* fb = &tree.box[6 * w->leaf[s]];
+    const double* b = &w->box[6 * s];
+    if (b[0] >= fb[0] && b[1] >= fb[1] && b[2] >= fb[2] && b[3] <= fb[3] && b[4] <= fb[4] && b[5] <= fb[5]) continue;
+    double out[6];
+    fat(w, s, MOVING_TREE, AHEAD * dt, out);
+    tree.move(w->leaf[s], out);
+    mark(w, s);
+  }
+}
+
+bool overlap(bm_world* w, int a, int b) {
+  const double* fa = &w->trees[w->which[a]].box[6 * w->leaf[a]];
+  const double* fb = &w->trees[w->which[b]].box[6 * w->leaf[b]];
+  return fa[0] <= fb[3] && fa[3] >= fb[0] && fa[1] <= fb[4] && fa[4] >= fb[1] && fa[2] <= fb[5] && fa[5] >= fb[2];
+}
+
+// src/pairs.js:180-214 (found), for bodies with no sensors, layers or joints.
+void found(bm_world* w, int x, int y) {
+  if (w->type[x] != DYNAMIC && w->type[y] != DYNAMIC) return;
+  if (w->type[y] == STATIC) wake(w, x);
+  else if (w->type[x] == STATIC) wake(w, y);
+  const int a = x < y ? x : y, b = x < y ? y : x;
+  Pairs& pr = w->pairs;
+  if (pr.index.count(Pairs::key(a, b))) return;
+  const int p = pr.count;
+  pr.grow(p + 1);
+  pr.a[p] = a; pr.b[p] = b; pr.points[p] = 0; pr.touching[p] = 0; pr.cache[p] = 0;
+  pr.index[Pairs::key(a, b)] = p;
+  pr.count++;
+  pr.wake(p);
+}
+
+// src/broadphase.js:161-179
+void candidates(bm_world* w) {
+  for (size_t i = 0; i < w->moved.size(); i++) {
+    const int a = w->moved[i];
+    w->is_moved[a] = 0;
+    const int t = w->which[a];
+    if (t < 0) continue;
+    double q[6];
+    std::memcpy(q, &w->trees[t].box[6 * w->leaf[a]], sizeof q);
+    auto visit = [&](int b) {
+      if (b != a) found(w, a, b);
+    };
+    if (w->type[a] != STATIC) w->trees[STATIC_TREE].query(q, visit);
+    w->trees[MOVING_TREE].query(q, visit);
+  }
+  w->moved.clear();
+}
+
+// ---- narrow driver (src/contact.js:296-1036) ----
+
+// src/contact.js:686-693 (moving)
+double moving(const bm_world* w, int a, int b) {
+  const double* v = w->velocity.data();
+  const double* om = w->angular.data();
+  const double* g = w->gravity;
+  const int a3 = 3 * a, b3 = 3 * b;
+  const double ga = w->type[a] == DYNAMIC ? w->gravity_scale[a] : 0, gb = w->type[b] == DYNAMIC ? w->gravity_scale[b] : 0;
+  const double dx = v[b3] - v[a3], dy = v[b3 + 1] - v[a3 + 1], dz = v[b3 + 2] - v[a3 + 2];
+  const double gs = gb - ga, gl = std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]) * std::fabs(gs);
+  const double turning = std::sqrt(om[a3] * om[a3] + om[a3 + 1] * om[a3 + 1] + om[a3 + 2] * om[a3 + 2]) * w->reach[a]
+      + std::sqrt(om[b3] * om[b3] + om[b3 + 1] * om[b3 + 1] + om[b3 + 2] * om[b3 + 2]) * w->reach[b];
+  return (std::sqrt(dx * dx + dy * dy + dz * dz) + 0.5 * gl * w->step_dt + turning) * w->step_dt;
+}
+
+// src/contact.js:609-622 (relative): the smaller body's pose in the larger's frame, into pose; returns the smaller.
+int relative(const bm_world* w, int a, int b, double* pose) {
+  const int h = w->reach[a] >= w->reach[b] ? a : b, g = h == a ? b : a;
+  const double* pos = w->position.data();
+  const double* A = w->axes.data();
+  const double* q = w->rotation.data();
+  const int h3 = 3 * h, g3 = 3 * g, h9 = 9 * h, h4 = 4 * h, g4 = 4 * g;
+  const double dx = pos[g3] - pos[h3], dy = pos[g3 + 1] - pos[h3 + 1], dz = pos[g3 + 2] - pos[h3 + 2];
+  pose[0] = A[h9] * dx + A[h9 + 1] * dy + A[h9 + 2] * dz;
+  pose[1] = A[h9 + 3] * dx + A[h9 + 4] * dy + A[h9 + 5] * dz;
+  pose[2] = A[h9 + 6] * dx + A[h9 + 7] * dy + A[h9 + 8] * dz;
+  const double ax = -q[h4], ay = -q[h4 + 1], az = -q[h4 + 2], aw = q[h4 + 3];
+  const double bx = q[g4], by = q[g4 + 1], bz = q[g4 + 2], bw = q[g4 + 3];
+  pose[3] = aw * bx + ax * bw + ay * bz - az * by;
+  pose[4] = aw * by + ay * bw + az * bx - ax * bz;
+  pose[5] = aw * bz + az * bw + ax * by - ay * bx;
+  pose[6] = aw * bw - ax * bx - ay * by - az * bz;
+  return g;
+}
+
+// src/contact.js:518-537 (keeps)
+bool keeps(const bm_world* w, int p, int a, int b, double tolerance) {
+  const Pairs& pr = w->pairs;
+  const int r = pr.cache[p] - 1;
+  const double* pose = &pr.cache_pose[7 * r];
+  double now[7];
+  const int g = relative(w, a, b, now);
+  const double ex = now[0] - pose[0], ey = now[1] - pose[1], ez = now[2] - pose[2];
+  const double cx = -pose[3], cy = -pose[4], cz = -pose[5], cw = pose[6];
+  const double rx = now[3], ry = now[4], rz = now[5], rw = now[6];
+  const double ux = cw * rx + cx * rw + cy * rz - cz * ry;
+  const double uy = cw * ry + cy * rw + cz * rx - cx * rz;
+  const double uz = cw * rz + cz * rw + cx * ry - cy * rx;
+  const double drift = std::sqrt(ex * ex + ey * ey + ez * ez) + 2 * std::sqrt(ux * ux + uy * uy + uz * uz) * w->reach[g];
+  if (pr.cache_loss[r] + 2 * drift > tolerance) return false;
+  if (w->joint_head[a] != 0 || w->joint_head[b] != 0) return false;
+  return pr.touching[p] == 1 || drift + moving(w, a, b) <= tolerance;
+}
+
+double rounded(const bm_world* w, int s) { return w->shape[s] == SPHERE ? w->dims[3 * s] : 0; }
+
+double least(const Manifold& m) {
+  double s = INFINITY;
+  for (int k = 0; k < m.count; k++)
+    if (m.separations[k] < s) s = m.separations[k];
+  return s;
+}
+
+// src/contact.js:468-510 (reused)
+bool reused(const bm_world* w, int p, int a, int b, double tolerance, double margin, Manifold& m) {
+  const Pairs& pr = w->pairs;
+  if (pr.cache[p] == 0 || !keeps(w, p, a, b, tolerance)) return false;
+  const int r = pr.cache[p] - 1, count = pr.cache_points[r];
+  const double* pos = w->position.data();
+  const double* A = w->axes.data();
+  const int a3 = 3 * a, b3 = 3 * b, a9 = 9 * a, b9 = 9 * b;
+  const double* ln = &pr.cache_normal[3 * r];
+  const double lx = ln[0], ly = ln[1], lz = ln[2];
+  const double nx = A[a9] * lx + A[a9 + 3] * ly + A[a9 + 6] * lz;
+  const double ny = A[a9 + 1] * lx + A[a9 + 4] * ly + A[a9 + 7] * lz;
+  const double nz = A[a9 + 2] * lx + A[a9 + 5] * ly + A[a9 + 8] * lz;
+  m.normal[0] = nx; m.normal[1] = ny; m.normal[2] = nz;
+  const double ra = rounded(w, a), rb = -rounded(w, b);
+  m.count = count;
+  for (int k = 0; k < count; k++) {
+    const double* L = &pr.cache_local[6 * (P * r + k)];
+    const double ax = L[0], ay = L[1], az = L[2], bx = L[3], by = L[4], bz = L[5];
+    const double pax = pos[a3] + A[a9] * ax + A[a9 + 3] * ay + A[a9 + 6] * az + nx * ra;
+    const double pay = pos[a3 + 1] + A[a9 + 1] * ax + A[a9 + 4] * ay + A[a9 + 7] * az + ny * ra;
+    const double paz = pos[a3 + 2] + A[a9 + 2] * ax + A[a9 + 5] * ay + A[a9 + 8] * az + nz * ra;
+    const double pbx = pos[b3] + A[b9] * bx + A[b9 + 3] * by + A[b9 + 6] * bz + nx * rb;
+    const double pby = pos[b3 + 1] + A[b9 + 1] * bx + A[b9 + 4] * by + A[b9 + 7] * bz + ny * rb;
+    const double pbz = pos[b3 + 2] + A[b9 + 2] * bx + A[b9 + 5] * by + A[b9 + 8] * bz + nz * rb;
+    const double s = (pbx - pax) * nx + (pby - pay) * ny + (pbz - paz) * nz, half = 0.5 * s;
+    m.separations[k] = s;
+    if (rb != 0) {
+      m.points[3 * k] = pbx - nx * half; m.points[3 * k + 1] = pby - ny * half; m.points[3 * k + 2] = pbz - nz * half;
+    } else if (ra != 0) {
+      m.points[3 * k] = pax + nx * half; m.points[3 * k + 1] = pay + ny * half; m.points[3 * k + 2] = paz + nz * half;
+    } else {
+      m.points[3 * k] = (pax + pbx) * 0.5; m.points[3 * k + 1] = (pay + pby) * 0.5; m.points[3 * k + 2] = (paz + pbz) * 0.5;
+    }
+    m.ids[k] = pr.cache_id[P * r + k];
+  }
+  if (least(m) > margin) return false;
+  return true;
+}
+
+// src/contact.js:581-585 (cored) for a box or a sphere.
+void cored(const bm_world* w, int s, double* local) {
+  if (w->shape[s] == SPHERE) { local[0] = 0; local[1] = 0; local[2] = 0; }
+}
+
+// src/contact.js:540-570 (remember), for a manifold not found at the moment of impact.
+void remember(bm_world* w, int p, int a, int b, const Manifold& m) {
+  Pairs& pr = w->pairs;
+  const int count = m.count, r = pr.cache_of(p);
+  pr.cache_points[r] = (uint8_t)count;
+  double pose[7];
+  relative(w, a, b, pose);
+  for (int k = 0; k < 7; k++) pr.cache_pose[7 * r + k] = pose[k];
+  pr.cache_loss[r] = m.loss;
+  const double* pos = w->position.data();
+  const double* A = w->axes.data();
+  const int a3 = 3 * a, b3 = 3 * b, a9 = 9 * a, b9 = 9 * b;
+  const double nx = m.normal[0], ny = m.normal[1], nz = m.normal[2];
+  double* ln = &pr.cache_normal[3 * r];
+  ln[0] = A[a9] * nx + A[a9 + 1] * ny + A[a9 + 2] * nz;
+  ln[1] = A[a9 + 3] * nx + A[a9 + 4] * ny + A[a9 + 5] * nz;
+  ln[2] = A[a9 + 6] * nx + A[a9 + 7] * ny + A[a9 + 8] * nz;
+  for (int k = 0; k < count; k++) {
+    double* L = &pr.cache_local[6 * (P * r + k)];
+    const double half = 0.5 * m.separations[k];
+    const double px = m.points[3 * k], py = m.points[3 * k + 1], pz = m.points[3 * k + 2];
+    const double ax = px - nx * half - pos[a3], ay = py - ny * half - pos[a3 + 1], az = pz - nz * half - pos[a3 + 2];
+    const double bx = px + nx * half - pos[b3], by = py + ny * half - pos[b3 + 1], bz = pz + nz * half - pos[b3 + 2];
+    L[0] = A[a9] * ax + A[a9 + 1] * ay + A[a9 + 2] * az;
+    L[1] = A[a9 + 3] * ax + A[a9 + 4] * ay + A[a9 + 5] * az;
+    L[2] = A[a9 + 6] * ax + A[a9 + 7] * ay + A[a9 + 8] * az;
+    L[3] = A[b9] * bx + A[b9 + 1] * by + A[b9 + 2] * bz;
+    L[4] = A[b9 + 3] * bx + A[b9 + 4] * by + A[b9 + 5] * bz;
+    L[5] = A[b9 + 6] * bx + A[b9 + 7] * by + A[b9 + 8] * bz;
+    cored(w, a, L);
+    cored(w, b, L + 3);
+    pr.cache_id[P * r + k] = m.ids[k];
+  }
+}
+
+// src/contact.js:1283-1292 (basis)
+void basis(Contacts& c, int m) {
+  const double nx = c.normal[3 * m], ny = c.normal[3 * m + 1], nz = c.normal[3 * m + 2];
+  const double ax = std::fabs(nx), ay = std::fabs(ny), az = std::fabs(nz);
+  double tx, ty, tz;
+  if (ax <= ay && ax <= az) { tx = 0; ty = nz; tz = -ny; }
+  else if (ay <= az) { tx = -nz; ty = 0; tz = nx; }
+  else { tx = ny; ty = -nx; tz = 0; }
+  const double l = 1 / std::sqrt(tx * tx + ty * ty + tz * tz);
+  tx *= l; ty *= l; tz *= l;
+  c.t1[3 * m] = tx; c.t1[3 * m + 1] = ty; c.t1[3 * m + 2] = tz;
+  c.t2[3 * m] = ny * tz - nz * ty; c.t2[3 * m + 1] = nz * tx - nx * tz; c.t2[3 * m + 2] = nx * ty - ny * tx;
+}
+
+// src/contact.js:889-921 (emit), sign 1, no sub-manifold, not shifted.
+int emit(bm_world* w, int p, int a, int b, const Manifold& mf, double tolerance) {
+  Contacts& c = w->c;
+  const int m = c.count++;
+  c.ensure(c.count);
+  c.a[m] = a; c.b[m] = b; c.pair[m] = p; c.sub[m] = -1;
+  c.points[m] = (uint8_t)mf.count;
+  for (int k = 0; k < 3; k++) c.normal[3 * m + k] = mf.normal[k];
+  c.friction[m] = std::sqrt(w->friction[a] * w->friction[b]);
+  c.restitution[m] = js_max(w->restitution[a], w->restitution[b]);
+  c.tolerance[m] = tolerance;
+  c.patch[m] = PIXEL * (w->type[a] != DYNAMIC ? w->size[b] : w->type[b] != DYNAMIC ? w->size[a]
+                                                                                   : js_min(w->size[a], w->size[b]));
+  c.roll_impulse1[m] = 0; c.roll_impulse2[m] = 0; c.spin_impulse[m] = 0;
+  c.tangent_impulse1[m] = 0; c.tangent_impulse2[m] = 0; c.drift1[m] = 0; c.drift2[m] = 0;
+  for (int k = 0; k < mf.count; k++) {
+    const int i = m * P + k;
+    c.pid[i] = mf.ids[k];
+    c.separation[i] = mf.separations[k];
+    for (int j = 0; j < 3; j++) c.point[3 * i + j] = mf.points[3 * k + j];
+    for (int j = 0; j < 3; j++) c.shift[3 * i + j] = 0;
+    c.normal_impulse[i] = 0;
+  }
+  basis(c, m);
+  return m;
+}
+
+// src/contact.js:999-1036 (carry) from the pair list.
+void carry(bm_world* w, int p, int m) {
+  const Pairs& pr = w->pairs;
+  Contacts& c = w->c;
+  const int remembered = pr.points[p];
+  if (remembered == 0) return;
+  const int m3 = 3 * m;
+  const double t1x = c.t1[m3], t1y = c.t1[m3 + 1], t1z = c.t1[m3 + 2];
+  const double t2x = c.t2[m3], t2y = c.t2[m3 + 1], t2z = c.t2[m3 + 2];
+  const int count = c.points[m], p3 = 3 * p;
+  const double lx = pr.twist[p3], ly = pr.twist[p3 + 1], lz = pr.twist[p3 + 2];
+  if (count <= 2) {
+    c.roll_impulse1[m] = lx * t1x + ly * t1y + lz * t1z;
+    c.roll_impulse2[m] = lx * t2x + ly * t2y + lz * t2z;
+  }
+  c.spin_impulse[m] = lx * c.normal[m3] + ly * c.normal[m3 + 1] + lz * c.normal[m3 + 2];
+  const double fx = pr.friction[p3], fy = pr.friction[p3 + 1], fz = pr.friction[p3 + 2];
+  c.tangent_impulse1[m] = fx * t1x + fy * t1y + fz * t1z;
+  c.tangent_impulse2[m] = fx * t2x + fy * t2y + fz * t2z;
+  bool same = false;
+  for (int k = 0; k < count; k++) {
+    const int i = m * P + k;
+    const uint32_t id = c.pid[i];
+    for (int j = 0; j < remembered; j++) {
+      const int o = p * P + j;
+      if (pr.pid[o] != id) continue;
+      c.normal_impulse[i] = pr.normal_impulse[o];
+      same = true;
+      break;
+    }
+  }
+  if (same) {
+    const double sx = pr.slip[p3], sy = pr.slip[p3 + 1], sz = pr.slip[p3 + 2];
+    c.drift1[m] = sx * t1x + sy * t1y + sz * t1z;
+    c.drift2[m] = sx * t2x + sy * t2y + sz * t2z;
+  }
+}
+
+// A shape record for the device narrowphase (src/contact.js:1244-1276 load).
+void load(const bm_world* w, int s, double* r) {
+  r[0] = w->shape[s] == BOX ? 2.0 : 0.0;
+  for (int k = 0; k < 3; k++) r[1 + k] = w->position[3 * s + k];
+  for (int k = 0; k < 9; k++) r[4 + k] = w->axes[9 * s + k];
+  if (w->shape[s] == BOX) {
+    for (int k = 0; k < 3; k++) r[13 + k] = w->dims[3 * s + k] * 0.5;
+  } else {
+    r[13] = w->dims[3 * s]; r[14] = 0; r[15] = 0;
+  }
+  const double scale = w->type[s] == DYNAMIC ? w->gravity_scale[s] : 0;
+  for (int k = 0; k < 3; k++) r[16 + k] = w->gravity[k] * scale;
+}
+
+// A pair visited by narrow(): what find() decided for it, in the order visited.
+struct Visit {
+  int p, a, b;
+  double tolerance;
+  int kind;  // 0: no manifold (already forgotten); 1: reused, in m; 2: met, from the device batch at `slot`
+  int slot;
+  Manifold m;
+};
+
+// Runs the pending device narrowphase batch and finishes every pending visit in order:
+// src/contact.js:355-363 (meet's result: forget or remember) and 318-322 (wake, emit, carry).
+int flush(bm_world* w, std::vector<Visit>& pending) {
+  const int n = (int)w->batch_margin.size();
+  if (n > 0) {
+    w->batch_count.resize(n);
+    w->batch_normal.resize(3 * n);
+    w->batch_points.resize(12 * n);
+    w->batch_seps.resize(4 * n);
+    w->batch_ids.resize(4 * n);
+    const int rc = bm_collide_host(w->ctx, n, w->batch_a.data(), w->batch_b.data(), w->batch_margin.data(),
+                                   w->batch_tolerance.data(), w->batch_count.data(), w->batch_normal.data(),
+                                   w->batch_points.data(), w->batch_seps.data(), w->batch_ids.data());
+    if (rc != BM_OK) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx));
+  }
+  for (Visit& v : pending) {
+    if (v.kind == 0) continue;
+    if (v.kind == 2) {
+      const int k = v.slot;
+      Manifold& m = v.m;
+      m.count = w->batch_count[k];
+      if (m.count < 0) return fail(w, BM_ERR_UNSUPPORTED, "step: a pair of shapes the device narrowphase does not handle");
+      m.loss = 0;
+      for (int i = 0; i < 3; i++) m.normal[i] = w->batch_normal[3 * k + i];
+      for (int i = 0; i < 12; i++) m.points[i] = w->batch_points[12 * k + i];
+      for (int i = 0; i < 4; i++) m.separations[i] = w->batch_seps[4 * k + i];
+      for (int i = 0; i < 4; i++) m.ids[i] = w->batch_ids[4 * k + i];
+      if (m.count == 0) {
+        w->pairs.forget(v.p);
+        continue;
+      }
+      // meet() (src/contact.js:627-640): not touching and closing by more than the
+      // tolerance asks for a time of impact.
+      const double moves = least(m) > v.tolerance ? moving(w, v.a, v.b) : 0;
+      if (moves > v.tolerance)
+        return fail(w, BM_ERR_UNSUPPORTED, "step: a pair needs a time of impact, which this world does not compute yet");
+      remember(w, v.p, v.a, v.b, m);
+    }
+    if (w->awake[v.a] != w->awake[v.b]) wake(w, w->awake[v.a] ? v.b : v.a);
+    const int mi = emit(w, v.p, v.a, v.b, v.m, v.tolerance);
+    carry(w, v.p, mi);
+  }
+  pending.clear();
+  w->batch_a.clear(); w->batch_b.clear(); w->batch_margin.clear(); w->batch_tolerance.clear();
+  return BM_OK;
+}
+
+// A body that may be woken by its pair's manifold: dynamic and asleep.
+bool sleeper(const bm_world* w, int s) { return w->type[s] == DYNAMIC && w->island[s] != 0; }
+
+// src/contact.js:296-327 (narrow), from pair `from` on.
+int narrow(bm_world* w, int from) {
+  Pairs& pr = w->pairs;
+  std::vector<Visit> pending;
+  for (int p = from; p < pr.active;) {
+    const int a = pr.a[p], b = pr.b[p];
+    if (!overlap(w, a, b)) {
+      pr.remove(p);
+      continue;
+    }
+    const double tolerance = js_min(w->size[a], w->size[b]) * PIXEL;
+    // A visit whose manifold could wake an island needs every earlier one finished first.
+    if (sleeper(w, a) || sleeper(w, b)) {
+      if (int rc = flush(w, pending)) return rc;
+    }
+    Visit v{p, a, b, tolerance, 0, -1, Manifold{}};
+    const double* ba = &w->box[6 * a];
+    const double* bb = &w->box[6 * b];
+    if (ba[0] > bb[3] || bb[0] > ba[3] || ba[1] > bb[4] || bb[1] > ba[4] || ba[2] > bb[5] || bb[2] > ba[5]) {
+      pr.forget(p);
+    } else {
+      const double margin = w->motion[a] + w->motion[b] + tolerance;
+      if (reused(w, p, a, b, tolerance, margin, v.m)) {
+        v.kind = 1;
+      } else {
+        v.kind = 2;
+        v.slot = (int)w->batch_margin.size();
+        double ra[19], rb[19];
+        load(w, a, ra);
+        load(w, b, rb);
+        w->batch_a.insert(w->batch_a.end(), ra, ra + 19);
+        w->batch_b.insert(w->batch_b.end(), rb, rb + 19);
+        w->batch_margin.push_back(margin);
+        w->batch_tolerance.push_back(tolerance);
+      }
+    }
+    pending.push_back(v);
+    if (sleeper(w, a) || sleeper(w, b)) {
+      if (int rc = flush(w, pending)) return rc;
+    }
+    p++;
+  }
+  return flush(w, pending);
+}
+
+// src/world.js:1369-1382 (_syncPairs), with no bodies removed and no static body moved by hand.
+void sync_pairs(bm_world* w) {
+  Pairs& pr = w->pairs;
+  while (w->wake_pending) {
+    w->wake_pending = false;
+    for (int p = pr.active; p < pr.count; p++) {
+      if (w->awake[pr.a[p]] == 1 || w->awake[pr.b[p]] == 1) pr.wake(p);
+    }
+  }
+}
+
+// ---- freezing and sleeping ----
+
+void join(std::vector<int32_t>& parent, const std::vector<uint8_t>& type, int a, int b);
+int root_of(std::vector<int32_t>& parent, int s) {
+  while (parent[s] != s) {
+    parent[s] = parent[parent[s]];
+    s = parent[s];
+  }
+  return s;
+}
+// src/world.js:2304-2309
+void join(std::vector<int32_t>& parent, const std::vector<uint8_t>& type, int a, int b) {
+  if (type[a] != DYNAMIC || type[b] != DYNAMIC) return;
+  const int ra = root_of(parent, a), rb = root_of(parent, b);
+  if (ra < rb) parent[rb] = ra;
+  else if (rb < ra) parent[ra] = rb;
+}
+
+// src/world.js:1116-1172 (_freeze)
+void freeze(bm_world* w) {
+  const int n = w->awake_count, count = w->c.count;
+  const int* A = w->c.a.data();
+  const int* B = w->c.b.data();
+  auto& calm = w->calm;
+  auto& type = w->type;
+  for (int i = 0; i < n; i++) {
+    const int s = w->awake_list[i];
+    calm[s] = w->still[s] >= SLEEP_TIME && (calm[s] & HANDLED) == 0 ? (STILL | QUIET | CALM) : 0;
+  }
+  for (int m = 0; m < count; m++) {
+    const int a = A[m], b = B[m];
+    if (type[b] != STATIC && (calm[b] & STILL) == 0) calm[a] &= ~(QUIET | CALM);
+    if (type[a] != STATIC && (calm[a] & STILL) == 0) calm[b] &= ~(QUIET | CALM);
+  }
+  for (int m = 0; m < count; m++) {
+    const int a = A[m], b = B[m];
+    if (type[b] != STATIC && (calm[b] & QUIET) == 0) calm[a] &= ~CALM;
+    if (type[a] != STATIC && (calm[a] & QUIET) == 0) calm[b] &= ~CALM;
+  }
+  if (w->frozen_count > 0) {
+    auto& parent = w->parent;
+    for (int i = 0; i < n; i++) parent[w->awake_list[i]] = w->awake_list[i];
+    for (int m = 0; m < count; m++)
+      if (w->frozen[A[m]] != 0 && w->frozen[B[m]] != 0) join(parent, type, A[m], B[m]);
+    bool thawing = false;
+    for (int i = 0; i < n; i++) {
+      const int s = w->awake_list[i];
+      if (w->frozen[s] != 0 && (calm[s] & CALM) == 0) { calm[root_of(parent, s)] |= THAWING; thawing = true; }
+    }
+    if (thawing) {
+      for (int i = 0; i < n; i++) {
+        const int s = w->awake_list[i];
+        if (w->frozen[s] != 0 && (calm[root_of(parent, s)] & THAWING) != 0) calm[s] &= ~CALM;
+      }
+    }
+  }
+  for (int i = 0; i < n; i++) {
+    const int s = w->awake_list[i];
+    const bool fr = (calm[s] & CALM) != 0 && type[s] == DYNAMIC && w->joint_head[s] == 0;
+    if (fr && w->frozen[s] == 0) {
+      w->frozen[s] = 2;
+      w->frozen_count++;
+      w->inv_mass[s] = 0;
+      for (int k = 0; k < 3; k++) { w->velocity[3 * s + k] = 0; w->angular[3 * s + k] = 0; }
+    } else if (!fr && w->frozen[s] != 0) {
+      thaw(w, s, true);
+    }
+  }
+}
+
+// src/world.js:1271-1286 (_fallAsleep)
+void fall_asleep(bm_world* w, int s, int r) {
+  if (w->frozen[s] != 0) thaw(w, s, false);
+  for (int k = 0; k < 3; k++) { w->velocity[3 * s + k] = 0; w->angular[3 * s + k] = 0; }
+  unawaken(w, s);
+  w->island[s] = r + 1;
+  if (s != r) {
+    w->island_next[s] = w->island_next[r];
+    w->island_next[r] = s + 1;
+  }
+  bounds_of(w, s, 0, 0);
+  broadphase_settle(w, s);
+  w->slept[w->slept_count++] = s;
+}
+
+// src/world.js:1196-1268 (_sleep), for worlds with no kinematic bodies, characters or joints.
+void sleep_step(bm_world* w, double dt) {
+  const double* p = w->position.data();
+  const double* q = w->rotation.data();
+  bool ready = false;
+  for (int i = 0; i < w->awake_count; i++) {
+    const int s = w->awake_list[i], s3 = 3 * s, s4 = 4 * s, r7 = 7 * s;
+    const double* rest = &w->rest[r7];
+    const double dx = p[s3] - rest[0], dy = p[s3 + 1] - rest[1], dz = p[s3 + 2] - rest[2];
+    const double qx = q[s4], qy = q[s4 + 1], qz = q[s4 + 2], qw = q[s4 + 3];
+    const double ax = rest[3], ay = rest[4], az = rest[5], aw = rest[6];
+    const double tx = aw * qx - qw * ax - (qy * az - qz * ay);
+    const double ty = aw * qy - qw * ay - (qz * ax - qx * az);
+    const double tz = aw * qz - qw * az - (qx * ay - qy * ax);
+    const double strayed = std::sqrt(dx * dx + dy * dy + dz * dz) + 2 * w->reach[s] * std::sqrt(tx * tx + ty * ty + tz * tz);
+    if (strayed < w->size[s] * PIXEL) {
+      const double before = w->still[s];
+      w->still[s] += dt;
+      if (w->still[s] >= SLEEP_TIME) ready = true;
+      if (std::floor(w->still[s] / SLEEP_TIME) != std::floor(before / SLEEP_TIME)) {
+        for (int k = 0; k < 3; k++) w->rest[r7 + k] = p[s3 + k];
+        for (int k = 0; k < 4; k++) w->rest[r7 + 3 + k] = q[s4 + k];
+      }
+    } else {
+      settle(w, s);
+    }
+  }
+  w->still_ready = ready;
+  if (!ready) return;
+  auto& parent = w->parent;
+  const int n = w->awake_count;
+  for (int i = 0; i < n; i++) parent[w->awake_list[i]] = w->awake_list[i];
+  for (int m = 0; m < w->c.count; m++) join(parent, w->type, w->c.a[m], w->c.b[m]);
+  for (int i = 0; i < n; i++) w->least[w->awake_list[i]] = INFINITY;
+  for (int i = 0; i < n; i++) {
+    const int s = w->awake_list[i], r = root_of(parent, s);
+    if (w->still[s] < w->least[r]) w->least[r] = w->still[s];
+  }
+  bool slept = false;
+  for (int i = n - 1; i >= 0; i--) {
+    const int s = w->awake_list[i], r = root_of(parent, s);
+    if (w->least[r] < SLEEP_TIME) continue;
+    fall_asleep(w, s, r);
+    slept = true;
+  }
+  if (!slept) return;
+  Pairs& pr = w->pairs;
+  for (int pp = 0; pp < pr.active;) {
+    if (w->awake[pr.a[pp]] == 0 && w->awake[pr.b[pp]] == 0) pr.sleep(pp);
+    else pp++;
+  }
+}
+
+// src/contact.js:2178-2213 (touched): which pairs touched, the flag only (no events kept).
+void touched(bm_world* w) {
+  Contacts& c = w->c;
+  Pairs& pr = w->pairs;
+  for (int f = 0; f < c.count;) {
+    const int p = c.pair[c.row[f]];
+    int hit = -1;
+    for (; f < c.count && c.pair[c.row[f]] == p; f++) {
+      const int m = c.row[f];
+      if (hit >= 0) continue;
+      bool meets = c.gap[m] <= c.tolerance[m];
+      for (int k = 0; k < c.points[m] && !meets; k++) meets = c.max_normal_impulse[m * P + k] > 0;
+      if (meets) hit = m;
+    }
+    const uint8_t now = hit >= 0 ? 1 : 0;
+    if (now == pr.touching[p]) continue;
+    pr.touching[p] = now;
+  }
+}
+
+// The device phase: colour, rest, inertias .. store (src/world.js:956-1031), through the batch ABI.
+int device_phase(bm_world* w, double dt) {
+  Contacts& c = w->c;
+  Pairs& pr = w->pairs;
+  c.ensure(c.count > 0 ? c.count : 1);
+  pr.grow(pr.count > 0 ? pr.count : 1);
+  const int32_t body_offset[2] = {0, w->top}, manifold_offset[2] = {0, c.count}, pair_offset[2] = {0, pr.count},
+                awake_offset[2] = {0, w->awake_count};
+  int32_t starts[BM_STARTS] = {0}, solved = 0, substeps = w->substeps;
+  double dts = dt;
+  bm_columns h{};
+  h.n_worlds = 1;
+  h.body_offset = body_offset; h.manifold_offset = manifold_offset; h.pair_offset = pair_offset;
+  h.awake_offset = awake_offset;
+  h.starts = starts; h.solved = &solved; h.substeps = &substeps; h.dt = &dts; h.gravity = w->gravity;
+  h.position = w->position.data(); h.rotation = w->rotation.data(); h.velocity = w->velocity.data();
+  h.angular = w->angular.data(); h.start = w->start.data(); h.start_q = w->start_q.data(); h.inv_i = w->inv_i.data();
+  h.tumbled = w->tumbled.data(); h.ceiling = w->ceiling.data(); h.dims = w->dims.data(); h.inv_mass = w->inv_mass.data();
+  h.inertia = w->inertia.data(); h.inv_inertia = w->inv_inertia.data(); h.axes = w->axes.data();
+  h.gravity_scale = w->gravity_scale.data(); h.damping = w->damping.data(); h.angular_damping = w->angular_damping.data();
+  h.type = w->type.data(); h.shape = w->shape.data(); h.frozen = w->frozen.data(); h.lock = w->lock.data();
+  h.framed = w->framed.data(); h.awake = w->awake.data(); h.joint_head = w->joint_head.data();
+  h.awake_list = w->awake_list.data();
+  h.a = c.a.data(); h.b = c.b.data(); h.pair = c.pair.data(); h.sub = c.sub.data(); h.row = c.row.data();
+  h.colour = c.colour.data(); h.order = c.order.data(); h.points = c.points.data(); h.sliding = c.sliding.data();
+  h.normal = c.normal.data(); h.t1 = c.t1.data(); h.t2 = c.t2.data(); h.friction = c.friction.data();
+  h.restitution = c.restitution.data(); h.tolerance = c.tolerance.data(); h.patch = c.patch.data();
+  h.roll_impulse1 = c.roll_impulse1.data(); h.roll_impulse2 = c.roll_impulse2.data(); h.spin_impulse = c.spin_impulse.data();
+  h.tangent_impulse1 = c.tangent_impulse1.data(); h.tangent_impulse2 = c.tangent_impulse2.data();
+  h.drift1 = c.drift1.data(); h.drift2 = c.drift2.data(); h.separation = c.separation.data(); h.point = c.point.data();
+  h.shift = c.shift.data(); h.normal_impulse = c.normal_impulse.data(); h.max_normal_impulse = c.max_normal_impulse.data();
+  h.relative_velocity = c.relative_velocity.data(); h.gap = c.gap.data(); h.now = c.now.data(); h.ra = c.ra.data();
+  h.rb = c.rb.data(); h.la = c.la.data(); h.lb = c.lb.data(); h.adjusted = c.adjusted.data();
+  h.normal_mass = c.normal_mass.data(); h.jac = c.jac.data(); h.block = c.block.data(); h.lever = c.lever.data();
+  h.anchor_a = c.anchor_a.data(); h.anchor_b = c.anchor_b.data(); h.fjac = c.fjac.data();
+  h.tangent_mass = c.tangent_mass.data(); h.roll_mass1 = c.roll_mass1.data(); h.roll_mass2 = c.roll_mass2.data();
+  h.spin_mass = c.spin_mass.data(); h.pid = c.pid.data();
+  h.pair_points = pr.points.data(); h.pair_pid = pr.pid.data(); h.pair_normal_impulse = pr.normal_impulse.data();
+  h.pair_twist = pr.twist.data(); h.pair_friction = pr.friction.data(); h.pair_slip = pr.slip.data();
+  if (int rc = bm_upload(w->ctx, &h)) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx));
+  if (int rc = bm_colour(w->ctx)) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx));
+  if (int rc = bm_soft_step(w->ctx)) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx));
+  if (int rc = bm_download(w->ctx, &h)) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx));
+  return BM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bm_world_create(bm_world** out, int32_t device, const double* gravity, int32_t substeps, int32_t sleeping) {
+  if (!out) return BM_ERR_ARGUMENT;
+  *out = nullptr;
+  bm_world* w = new bm_world();
+  if (int rc = bm_create(&w->ctx, device)) {
+    delete w;
+    return rc;
+  }
+  if (gravity)
+    for (int k = 0; k < 3; k++) w->gravity[k] = gravity[k];
+  w->substeps = substeps > 0 ? substeps : 8;
+  w->sleeping = sleeping != 0;
+  *out = w;
+  return BM_OK;
+}
+
+int bm_world_destroy(bm_world* w) {
+  if (!w) return BM_OK;
+  bm_destroy(w->ctx);
+  delete w;
+  return BM_OK;
+}
+
+const char* bm_world_error(const bm_world* w) { return w ? w->error.c_str() : "bm_world: no world"; }
+
+int bm_world_add(bm_world* w, const bm_bodies* b) {
+  if (!w || !b) return BM_ERR_ARGUMENT;
+  w->error.clear();
+  if (b->n < 0) return fail(w, BM_ERR_ARGUMENT, "addBody: n must be 0 or more");
+  for (int i = 0; i < b->n; i++) {
+    if (!b->type || !b->shape || !b->lock || !b->position || !b->rotation || !b->velocity || !b->angular || !b->dims ||
+        !b->inertia || !b->inv_inertia || !b->mass || !b->inv_mass || !b->gravity_scale || !b->damping ||
+        !b->angular_damping || !b->size || !b->reach || !b->friction || !b->restitution)
+      return fail(w, BM_ERR_ARGUMENT, "addBody: every column is needed");
+    if (b->shape[i] != BOX && b->shape[i] != SPHERE)
+      return fail(w, BM_ERR_UNSUPPORTED, "addBody: this world takes boxes and spheres");
+    if (b->type[i] != DYNAMIC && b->type[i] != STATIC)
+      return fail(w, BM_ERR_UNSUPPORTED, "addBody: this world takes dynamic and static bodies");
+  }
+  const int first = w->top, top = w->top + b->n;
+  auto grow = [&](auto& v, int per) { v.resize((size_t)per * top); };
+  for (auto* v : {&w->position, &w->velocity, &w->angular, &w->dims, &w->inertia, &w->inv_inertia, &w->start, &w->tumbled})
+    grow(*v, 3);
+  for (auto* v : {&w->rotation, &w->start_q}) grow(*v, 4);
+  grow(w->inv_i, 6); grow(w->axes, 9); grow(w->box, 6); grow(w->rest, 7);
+  for (auto* v : {&w->mass, &w->inv_mass, &w->motion, &w->ceiling, &w->gravity_scale, &w->damping, &w->angular_damping,
+                  &w->size, &w->reach, &w->friction, &w->restitution, &w->still, &w->least})
+    grow(*v, 1);
+  for (auto* v : {&w->type, &w->shape, &w->frozen, &w->calm, &w->lock, &w->framed, &w->awake, &w->fast, &w->is_moved})
+    grow(*v, 1);
+  for (auto* v : {&w->joint_head, &w->in_awake, &w->awake_list, &w->island, &w->island_next, &w->parent, &w->slept,
+                  &w->leaf})
+    grow(*v, 1);
+  w->which.resize(top, -1);
+  w->transforms.resize(7 * (size_t)top);
+  w->top = top;
+  for (int i = 0; i < b->n; i++) {
+    const int s = first + i;
+    for (int k = 0; k < 3; k++) {
+      w->position[3 * s + k] = b->position[3 * i + k];
+      w->velocity[3 * s + k] = b->velocity[3 * i + k];
+      w->angular[3 * s + k] = b->angular[3 * i + k];
+      w->dims[3 * s + k] = b->dims[3 * i + k];
+      w->inertia[3 * s + k] = b->inertia[3 * i + k];
+      w->inv_inertia[3 * s + k] = b->inv_inertia[3 * i + k];
+      w->tumbled[3 * s + k] = NAN;  // never equal to any spin (src/world.js:2260)
+    }
+    for (int k = 0; k < 4; k++) w->rotation[4 * s + k] = b->rotation[4 * i + k];
+    w->mass[s] = b->mass[i]; w->inv_mass[s] = b->inv_mass[i];
+    w->gravity_scale[s] = b->gravity_scale[i]; w->damping[s] = b->damping[i];
+    w->angular_damping[s] = b->angular_damping[i]; w->size[s] = b->size[i]; w->reach[s] = b->reach[i];
+    w->friction[s] = b->friction[i]; w->restitution[s] = b->restitution[i];
+    w->type[s] = b->type[i]; w->shape[s] = b->shape[i]; w->lock[s] = b->lock[i];
+    // addBody (src/world.js:563-578): placed, bounded, filed, awake unless static.
+    for (int k = 0; k < 7; k++) w->transforms[7 * s + k] = (float)(k < 3 ? w->position[3 * s + k] : w->rotation[4 * s + k - 3]);
+    for (int k = 0; k < 3; k++) w->start[3 * s + k] = w->position[3 * s + k];
+    bounds_of(w, s, 0, 0);
+    broadphase_add(w, s);
+    if (w->type[s] != STATIC) awaken(w, s);
+  }
+  return BM_OK;
+}
+
+// src/world.js:880-1072 (step), for the bodies above.
+int bm_world_step(bm_world* w, double dt) {
+  if (!w) return BM_ERR_ARGUMENT;
+  w->error.clear();
+  if (!(dt > 0) || dt == INFINITY) return fail(w, BM_ERR_ARGUMENT, "step: dt must be a positive number of seconds");
+  ++w->step_count;
+  if (w->wake_pending) sync_pairs(w);
+  w->slept_count = 0;
+  w->step_dt = dt;
+  const double gl = gravity_length(w);
+  for (int i = 0; i < w->awake_count; i++) bounds_of(w, w->awake_list[i], dt, gl);
+  broadphase_update(w, dt);
+  candidates(w);
+  w->c.count = 0;
+  int from = 0;
+  do {
+    if (w->wake_pending) sync_pairs(w);
+    if (int rc = narrow(w, from)) return rc;
+    from = w->pairs.active;
+  } while (w->wake_pending);
+  const int touching = w->c.count;
+  if (w->sleeping && (w->frozen_count > 0 || w->still_ready)) freeze(w);
+  if (touching > 0) w->solver_steps++;
+  if (int rc = device_phase(w, dt)) return rc;
+  // _catch (src/world.js:1683-1743): a body whose centre moved more than half its size.
+  for (int i = 0; i < w->awake_count; i++) {
+    const int s = w->awake_list[i], s3 = 3 * s;
+    if (w->type[s] != DYNAMIC) continue;
+    const double dx = w->position[s3] - w->start[s3], dy = w->position[s3 + 1] - w->start[s3 + 1],
+                 dz = w->position[s3 + 2] - w->start[s3 + 2], half = 0.5 * w->size[s];
+    if (dx * dx + dy * dy + dz * dz > half * half)
+      return fail(w, BM_ERR_UNSUPPORTED, "step: a fast body would be swept (_catch), which this world does not do yet");
+  }
+  if (touching > 0) touched(w);
+  for (int i = 0; i < w->awake_count; i++) {
+    const int s = w->awake_list[i];
+    if (w->frozen[s] == 1) continue;
+    if (w->frozen[s] == 2) w->frozen[s] = 1;
+    for (int k = 0; k < 3; k++) w->transforms[7 * s + k] = (float)w->position[3 * s + k];
+    for (int k = 0; k < 4; k++) w->transforms[7 * s + 3 + k] = (float)w->rotation[4 * s + k];
+  }
+  w->step_dt = 0;
+  if (w->sleeping) sleep_step(w, dt);
+  return BM_OK;
+}
+
+int bm_world_read(const bm_world* w, bm_world_state* out) {
+  if (!w || !out) return BM_ERR_ARGUMENT;
+  out->top = w->top;
+  out->awake = w->awake_count;
+  out->contacts = w->c.count;
+  out->pairs = w->pairs.count;
+  out->frozen = w->frozen_count;
+  out->impacts = w->impacts;
+  if (out->transforms) std::memcpy(out->transforms, w->transforms.data(), 7 * sizeof(float) * w->top);
+  if (out->position) std::memcpy(out->position, w->position.data(), 3 * sizeof(double) * w->top);
+  if (out->rotation) std::memcpy(out->rotation, w->rotation.data(), 4 * sizeof(double) * w->top);
+  if (out->velocity) std::memcpy(out->velocity, w->velocity.data(), 3 * sizeof(double) * w->top);
+  if (out->angular) std::memcpy(out->angular, w->angular.data(), 3 * sizeof(double) * w->top);
+  return BM_OK;
+}
+
+}  // extern "C"

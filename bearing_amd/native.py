@@ -57,7 +57,8 @@ class Columns(ctypes.Structure):
 
 
 EXPORTS = ["bm_create", "bm_destroy", "bm_last_error", "bm_upload", "bm_soft_step", "bm_download",
-           "bm_soft_step_host", "bm_last_kernel_ms", "bm_set_launch", "bm_lds_limit", "bm_colour", "bm_collide_host"]
+           "bm_soft_step_host", "bm_last_kernel_ms", "bm_set_launch", "bm_lds_limit", "bm_colour", "bm_collide_host",
+           "bm_world_create", "bm_world_destroy", "bm_world_error", "bm_world_add", "bm_world_step", "bm_world_read"]
 
 _lib = None
 
@@ -84,8 +85,36 @@ def load():
         lib.bm_set_launch.argtypes = [vp, ctypes.c_int32, ctypes.c_int32]
         lib.bm_lds_limit.argtypes = [vp]
         lib.bm_lds_limit.restype = ctypes.c_int64
+        world_signatures(lib)
         _lib = lib
     return _lib
+
+
+class Bodies(ctypes.Structure):
+    """bm_bodies: addBody's resolved columns, one entry a body."""
+    _fields_ = [("n", ctypes.c_int32), ("type", _U8), ("shape", _U8), ("lock", _U8)] + [
+        (f, _D) for f in ("position", "rotation", "velocity", "angular", "dims", "inertia", "inv_inertia", "mass",
+                          "inv_mass", "gravity_scale", "damping", "angular_damping", "size", "reach", "friction",
+                          "restitution")]
+
+
+class WorldState(ctypes.Structure):
+    _fields_ = [("top", ctypes.c_int32), ("awake", ctypes.c_int32), ("contacts", ctypes.c_int32),
+                ("pairs", ctypes.c_int32), ("frozen", ctypes.c_int32), ("impacts", ctypes.c_int64),
+                ("transforms", ctypes.POINTER(ctypes.c_float)), ("position", _D), ("rotation", _D), ("velocity", _D),
+                ("angular", _D)]
+
+
+def world_signatures(lib):
+    """Sets the bm_world_* signatures on a library that exports them."""
+    vp = ctypes.c_void_p
+    lib.bm_world_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int32, _D, ctypes.c_int32, ctypes.c_int32]
+    lib.bm_world_destroy.argtypes = [vp]
+    lib.bm_world_error.argtypes = [vp]
+    lib.bm_world_error.restype = ctypes.c_char_p
+    lib.bm_world_add.argtypes = [vp, ctypes.POINTER(Bodies)]
+    lib.bm_world_step.argtypes = [vp, ctypes.c_double]
+    lib.bm_world_read.argtypes = [vp, ctypes.POINTER(WorldState)]
 
 
 class BearingError(RuntimeError):
@@ -203,3 +232,79 @@ class Batch:
 
     def step_host(self, ctx):
         ctx.check(ctx.lib.bm_soft_step_host(ctx.ptr, ctypes.byref(self.struct)), "bm_soft_step_host")
+
+
+# addBody's resolved columns (bm_bodies field, numbers a body, dtype).
+BODY_COLUMNS = [("type", 1, np.uint8), ("shape", 1, np.uint8), ("lock", 1, np.uint8), ("position", 3, np.float64),
+                ("rotation", 4, np.float64), ("velocity", 3, np.float64), ("angular", 3, np.float64),
+                ("dims", 3, np.float64), ("inertia", 3, np.float64), ("inv_inertia", 3, np.float64),
+                ("mass", 1, np.float64), ("inv_mass", 1, np.float64), ("gravity_scale", 1, np.float64),
+                ("damping", 1, np.float64), ("angular_damping", 1, np.float64), ("size", 1, np.float64),
+                ("reach", 1, np.float64), ("friction", 1, np.float64), ("restitution", 1, np.float64)]
+
+
+class World:
+    """A whole world stepped as the reference's World.step steps it (bm_world_*).
+
+    `lib` is the library to bind, the built one by default."""
+
+    def __init__(self, device=0, gravity=(0.0, -9.81, 0.0), substeps=8, sleeping=True, lib=None):
+        if lib is None:
+            lib = load()
+        else:
+            world_signatures(lib)
+        self.lib = lib
+        self.ptr = ctypes.c_void_p()
+        g = np.ascontiguousarray(gravity, np.float64)
+        rc = lib.bm_world_create(ctypes.byref(self.ptr), device, g.ctypes.data_as(_D), substeps, 1 if sleeping else 0)
+        if rc != 0:
+            raise BearingError(f"bm_world_create: no HIP device {device} (status {rc})")
+
+    def check(self, rc, what):
+        if rc != 0:
+            raise BearingError(f"{what}: status {rc}: {self.lib.bm_world_error(self.ptr).decode()}")
+
+    def add(self, columns):
+        """addBody for each body of `columns` (name -> array, names as BODY_COLUMNS)."""
+        n = len(columns["type"])
+        keep, b = [], Bodies(n=n)
+        for name, per, dtype in BODY_COLUMNS:
+            arr = np.ascontiguousarray(columns[name], dtype)
+            if arr.size != per * n:
+                raise ValueError(f"{name}: {arr.size} numbers for {n} bodies")
+            keep.append(arr)
+            setattr(b, name, arr.ctypes.data_as(_U8 if dtype == np.uint8 else _D))
+        self.check(self.lib.bm_world_add(self.ptr, ctypes.byref(b)), "bm_world_add")
+
+    def step(self, dt):
+        self.check(self.lib.bm_world_step(self.ptr, dt), "bm_world_step")
+
+    def state(self):
+        s = WorldState()
+        self.check(self.lib.bm_world_read(self.ptr, ctypes.byref(s)), "bm_world_read")
+        return s
+
+    def read(self):
+        """The counts, and transforms (float32, 7 a slot), position, rotation, velocity, angular."""
+        top = self.state().top
+        out = {"transforms": np.zeros(7 * top, np.float32), "position": np.zeros(3 * top), "rotation": np.zeros(4 * top),
+               "velocity": np.zeros(3 * top), "angular": np.zeros(3 * top)}
+        s = WorldState()
+        s.transforms = out["transforms"].ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        for name in ("position", "rotation", "velocity", "angular"):
+            setattr(s, name, out[name].ctypes.data_as(_D))
+        self.check(self.lib.bm_world_read(self.ptr, ctypes.byref(s)), "bm_world_read")
+        for name in ("top", "awake", "contacts", "pairs", "frozen", "impacts"):
+            out[name] = getattr(s, name)
+        return out
+
+    def close(self):
+        if self.ptr:
+            self.lib.bm_world_destroy(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -178,6 +178,50 @@ int64_t bm_lds_limit(bm_context* ctx);
 /* Milliseconds the last bm_soft_step's kernel took on the device (HIP events). */
 double bm_last_kernel_ms(const bm_context* ctx);
 
+/* ---- a whole world, stepped as src/world.js step (880-1072) steps it ----
+ *
+ * For worlds of dynamic and static boxes and spheres, with no joints, sensors,
+ * characters or compounds. The host keeps the reference's order-defining serial
+ * state (the broadphase trees, the pair list, the narrow driver's order, the
+ * sleeping islands); the device runs each step's narrowphase batches, colouring
+ * and soft step. Transforms match the reference's bit for bit. A step that would
+ * need what this path does not compute yet (a time of impact, src/impact.js, or
+ * the continuous catch, src/world.js:1683) fails with BM_ERR_UNSUPPORTED and a
+ * message, rather than going on differently. */
+typedef struct bm_world bm_world;
+
+/* Bodies as addBody resolves them (src/world.js:563-578, _write 2152-2273), one
+ * entry a body: the columns the reference keeps by slot. type: 0 dynamic,
+ * 1 static; shape: 0 box (dims = full size), 1 sphere (dims[0] = radius). */
+typedef struct bm_bodies {
+  int32_t n;
+  const uint8_t *type, *shape, *lock;
+  const double *position, *rotation, *velocity, *angular, *dims, *inertia, *inv_inertia; /* 3, 4, 3, 3, 3, 3, 3 */
+  const double *mass, *inv_mass, *gravity_scale, *damping, *angular_damping, *size, *reach, *friction,
+      *restitution;
+} bm_bodies;
+
+/* What bm_world_read copies out; any pointer may be NULL. transforms: 7 floats a
+ * slot (position, rotation), as the reference's Float32Array world.transforms. */
+typedef struct bm_world_state {
+  int32_t top, awake, contacts, pairs, frozen;
+  int64_t impacts;
+  float* transforms;
+  double *position, *rotation, *velocity, *angular;
+} bm_world_state;
+
+/* new World({gravity, substeps, sleeping}) (src/world.js:314): gravity may be NULL
+ * for (0, -9.81, 0); substeps 0 for 8. */
+int bm_world_create(bm_world** out, int32_t device, const double* gravity, int32_t substeps, int32_t sleeping);
+int bm_world_destroy(bm_world* world);
+/* The last error's message, "" after a call that succeeded. */
+const char* bm_world_error(const bm_world* world);
+/* addBody (src/world.js:563) for each body, slots in order. */
+int bm_world_add(bm_world* world, const bm_bodies* bodies);
+/* world.step(dt) (src/world.js:880). */
+int bm_world_step(bm_world* world, double dt);
+int bm_world_read(const bm_world* world, bm_world_state* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,128 @@
+// TEST ONLY. The library's device calls (bm_upload, bm_colour, bm_soft_step,
+// bm_download, bm_collide_host) answered on the CPU by the oracle, so the host half of
+// bm_world (bearing_amd/csrc/world.cpp: trees, pair list, narrow order, freezing and
+// sleeping) can be run against the reference's trajectories on a machine with no GPU.
+// The product library never links this: there each of these calls runs on the device.
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/bearing_mi355x.h"
+#include "../../oracle/oracle.h"
+
+extern "C" int oracle_collide(int n, const double* a, const double* b, const double* margin, const double* tolerance,
+                              int32_t* count, double* normal, double* points, double* separations, uint32_t* ids);
+
+struct bm_context {
+  bm_columns h;
+  std::string error;
+};
+
+namespace {
+// The columns narrow() writes, moved to their rows by colour() (src/graph.js:178-214).
+template <class T>
+void arrange(T* col, int per, const int32_t* order, int n) {
+  std::vector<T> from(col, col + (size_t)per * n);
+  for (int o = 0; o < n; o++)
+    for (int k = 0; k < per; k++) col[(size_t)o * per + k] = from[(size_t)order[o] * per + k];
+}
+template <class T>
+T* mut(const T* p) { return const_cast<T*>(p); }
+}  // namespace
+
+extern "C" {
+
+int bm_create(bm_context** out, int32_t) {
+  *out = new bm_context();
+  return BM_OK;
+}
+int bm_destroy(bm_context* ctx) {
+  delete ctx;
+  return BM_OK;
+}
+const char* bm_last_error(const bm_context* ctx) { return ctx->error.c_str(); }
+int bm_upload(bm_context* ctx, const bm_columns* host) {
+  if (host->n_worlds != 1) return BM_ERR_ARGUMENT;
+  ctx->h = *host;
+  return BM_OK;
+}
+int bm_download(bm_context*, bm_columns*) { return BM_OK; }
+
+int bm_colour(bm_context* ctx) {
+  bm_columns& h = ctx->h;
+  const int n = h.manifold_offset[1], top = h.body_offset[1];
+  int frozen = 0;
+  for (int s = 0; s < top; s++) frozen += h.frozen[s] != 0;
+  oracle_colour(n, h.a, h.b, top, h.type, h.frozen, frozen, h.colour, h.order, h.row, h.starts, h.solved);
+  const int32_t* order = h.order;
+  arrange(h.a, 1, order, n); arrange(h.b, 1, order, n); arrange(h.pair, 1, order, n); arrange(h.sub, 1, order, n);
+  arrange(h.points, 1, order, n); arrange(h.colour, 1, order, n); arrange(h.pid, 4, order, n);
+  arrange(h.normal, 3, order, n); arrange(h.t1, 3, order, n); arrange(h.t2, 3, order, n);
+  arrange(h.friction, 1, order, n); arrange(h.restitution, 1, order, n); arrange(h.tolerance, 1, order, n);
+  arrange(h.patch, 1, order, n); arrange(h.roll_impulse1, 1, order, n); arrange(h.roll_impulse2, 1, order, n);
+  arrange(h.spin_impulse, 1, order, n); arrange(h.tangent_impulse1, 1, order, n);
+  arrange(h.tangent_impulse2, 1, order, n); arrange(h.drift1, 1, order, n); arrange(h.drift2, 1, order, n);
+  arrange(h.separation, 4, order, n); arrange(h.normal_impulse, 4, order, n); arrange(h.point, 12, order, n);
+  arrange(h.shift, 12, order, n);
+  // rest() (src/contact.js:1318-1329)
+  for (int m = *h.solved; m < n; m++) {
+    double least = __builtin_inf();
+    for (int k = 0; k < h.points[m]; k++) {
+      const int i = 4 * m + k;
+      if (h.separation[i] < least) least = h.separation[i];
+      h.max_normal_impulse[i] = h.normal_impulse[i];
+      h.relative_velocity[i] = 0;
+    }
+    h.gap[m] = least;
+  }
+  return BM_OK;
+}
+
+int bm_soft_step(bm_context* ctx) {
+  bm_columns& h = ctx->h;
+  OracleSoft o{};
+  o.top = h.body_offset[1];
+  o.awake_count = h.awake_offset[1];
+  o.substeps = *h.substeps;
+  o.count = h.manifold_offset[1];
+  o.solved = *h.solved;
+  o.pair_count = h.pair_offset[1];
+  o.dt = *h.dt;
+  for (int k = 0; k < 3; k++) o.gravity[k] = h.gravity[k];
+  o.position = h.position; o.rotation = h.rotation; o.velocity = h.velocity; o.angular = h.angular;
+  o.dims = mut(h.dims); o.inv_mass = mut(h.inv_mass); o.inertia = mut(h.inertia); o.inv_inertia = mut(h.inv_inertia);
+  o.inv_i = h.inv_i; o.axes = mut(h.axes); o.tumbled = h.tumbled; o.ceiling = h.ceiling;
+  o.gravity_scale = mut(h.gravity_scale); o.damping = mut(h.damping); o.angular_damping = mut(h.angular_damping);
+  o.start = h.start; o.start_q = h.start_q;
+  o.type = mut(h.type); o.shape = mut(h.shape); o.frozen = mut(h.frozen); o.lock = mut(h.lock);
+  o.framed = mut(h.framed); o.awake = mut(h.awake); o.joint_head = mut(h.joint_head);
+  o.awake_list = mut(h.awake_list);
+  o.starts = h.starts; o.a = h.a; o.b = h.b; o.pair = h.pair; o.sub = h.sub; o.row = h.row;
+  o.points = h.points; o.sliding = h.sliding;
+  o.normal = h.normal; o.t1 = h.t1; o.t2 = h.t2; o.friction = h.friction; o.restitution = h.restitution;
+  o.tolerance = h.tolerance; o.patch = h.patch; o.roll_impulse1 = h.roll_impulse1; o.roll_impulse2 = h.roll_impulse2;
+  o.spin_impulse = h.spin_impulse; o.tangent_impulse1 = h.tangent_impulse1; o.tangent_impulse2 = h.tangent_impulse2;
+  o.drift1 = h.drift1; o.drift2 = h.drift2; o.separation = h.separation; o.point = h.point; o.shift = h.shift;
+  o.normal_impulse = h.normal_impulse; o.max_normal_impulse = h.max_normal_impulse;
+  o.relative_velocity = h.relative_velocity; o.gap = h.gap; o.now = h.now; o.ra = h.ra; o.rb = h.rb; o.la = h.la;
+  o.lb = h.lb; o.adjusted = h.adjusted; o.normal_mass = h.normal_mass; o.jac = h.jac; o.block = h.block;
+  o.lever = h.lever; o.anchor_a = h.anchor_a; o.anchor_b = h.anchor_b; o.fjac = h.fjac;
+  o.tangent_mass = h.tangent_mass; o.roll_mass1 = h.roll_mass1; o.roll_mass2 = h.roll_mass2;
+  o.spin_mass = h.spin_mass; o.pid = h.pid;
+  o.pair_points = h.pair_points; o.pair_pid = h.pair_pid; o.pair_normal_impulse = h.pair_normal_impulse;
+  o.pair_twist = h.pair_twist; o.pair_friction = h.pair_friction; o.pair_slip = h.pair_slip;
+  if (oracle_soft_step(&o) != 0) {
+    ctx->error = "oracle soft step: a manifold it does not cover";
+    return BM_ERR_UNSUPPORTED;
+  }
+  return BM_OK;
+}
+
+int bm_collide_host(bm_context*, int32_t n, const double* a, const double* b, const double* margin,
+                    const double* tolerance, int32_t* count, double* normal, double* points, double* separations,
+                    uint32_t* ids) {
+  oracle_collide(n, a, b, margin, tolerance, count, normal, points, separations, ids);
+  return BM_OK;
+}
+
+}  // extern "C"

@@ -58,3 +58,29 @@ def test_create_without_a_gpu_fails_loudly():
         pass
     with pytest.raises(native.BearingError):
         native.Context(0)
+
+
+@pytest.mark.parametrize("struct, mirror", [("bm_bodies", native.Bodies), ("bm_world_state", native.WorldState)])
+def test_world_structs_match_the_header(tmp_path, struct, mirror):
+    fields = [f for f, _ in mirror._fields_]
+    src = tmp_path / "layout.c"
+    body = "".join(f'  printf("%s %zu\\n", "{f}", offsetof({struct}, {f}));\n' for f in fields)
+    src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "bearing_mi355x.h"\nint main(void) {\n'
+                   f'  printf("sizeof %zu\\n", sizeof({struct}));\n{body}  return 0;\n}}\n')
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.dirname(HEADER), str(src), "-o", str(exe)], check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True).stdout.splitlines())
+    assert int(got["sizeof"]) == ctypes.sizeof(mirror)
+    for f in fields:
+        assert int(got[f]) == getattr(mirror, f).offset, f
+
+
+def test_world_create_without_a_gpu_fails_loudly():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is present")
+    except ImportError:
+        pass
+    with pytest.raises(native.BearingError):
+        native.World(0)
