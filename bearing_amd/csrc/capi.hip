@@ -493,6 +493,61 @@ int bm_collide_host(bm_context* ctx, int32_t n, const double* a, const double* b
   return BM_OK;
 }
 
+int bm_meet_host(bm_context* ctx, int32_t n, const double* a, const double* b, const double* pair, int32_t* count,
+                 double* normal, double* points, double* separations, uint32_t* ids, double* shift, int32_t* flags) {
+  if (!ctx) return BM_ERR_ARGUMENT;
+  ctx->error.clear();
+  if (n < 0) return fail(ctx, BM_ERR_ARGUMENT, "bm_meet_host: n must be 0 or more");
+  if (n == 0) return BM_OK;
+  if (!a || !b || !pair || !count || !normal || !points || !separations || !ids || !shift || !flags)
+    return fail(ctx, BM_ERR_ARGUMENT, "bm_meet_host: null argument");
+  for (int32_t k = 0; k < n; k++) {
+    const double ka = a[32 * (size_t)k], kb = b[32 * (size_t)k];
+    if (!(ka == 0 || ka == 1 || ka == 2 || ka == 3) || !(kb == 0 || kb == 1 || kb == 2 || kb == 3))
+      return fail(ctx, BM_ERR_ARGUMENT, "bm_meet_host: a shape's kind must be 0, 1, 2 or 3");
+    if (!(pair[6 * (size_t)k + 2] > 0)) return fail(ctx, BM_ERR_ARGUMENT, "bm_meet_host: dt must be positive");
+  }
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->device), "bm_meet_host: hipSetDevice");
+  const size_t bodies = 32 * sizeof(double) * (size_t)n, pairs = 6 * sizeof(double) * (size_t)n;
+  const size_t in_bytes = 2 * bodies + pairs;
+  const size_t out_bytes = (size_t)n * (3 * 8 + 12 * 8 + 4 * 8 + 12 * 8 + 4 + 4 + 4 * 4) + 64;
+  if (int rc = grow(ctx, ctx->pairs_in, in_bytes, "bm_meet_host")) return rc;
+  if (int rc = grow(ctx, ctx->pairs_out, out_bytes, "bm_meet_host")) return rc;
+  char* in = static_cast<char*>(ctx->pairs_in.p);
+  char* out = static_cast<char*>(ctx->pairs_out.p);
+  const size_t N = (size_t)n;
+  bm::MeetDev d{};
+  d.n = n;
+  d.a = reinterpret_cast<const double*>(in);
+  d.b = reinterpret_cast<const double*>(in + bodies);
+  d.pair = reinterpret_cast<const double*>(in + 2 * bodies);
+  d.normal = reinterpret_cast<double*>(out);
+  d.points = reinterpret_cast<double*>(out + 24 * N);
+  d.separations = reinterpret_cast<double*>(out + 120 * N);
+  d.shift = reinterpret_cast<double*>(out + 152 * N);
+  d.count = reinterpret_cast<int32_t*>(out + 248 * N);
+  d.flags = reinterpret_cast<int32_t*>(out + 252 * N);
+  d.ids = reinterpret_cast<uint32_t*>(out + 256 * N);
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(const_cast<double*>(d.a), a, bodies, hipMemcpyHostToDevice, ctx->stream), "bm_meet_host: copy");
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(const_cast<double*>(d.b), b, bodies, hipMemcpyHostToDevice, ctx->stream), "bm_meet_host: copy");
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(const_cast<double*>(d.pair), pair, pairs, hipMemcpyHostToDevice, ctx->stream), "bm_meet_host: copy");
+  (void)hipGetLastError();
+  HIP_OR_FAIL(ctx, hipEventRecord(ctx->begin, ctx->stream), "bm_meet_host: hipEventRecord");
+  bm::launch_meet(d, ctx->stream);
+  HIP_OR_FAIL(ctx, hipGetLastError(), "bm_meet_host: launch");
+  HIP_OR_FAIL(ctx, hipEventRecord(ctx->end, ctx->stream), "bm_meet_host: hipEventRecord");
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(count, d.count, 4 * N, hipMemcpyDeviceToHost, ctx->stream), "bm_meet_host: copy");
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(flags, d.flags, 4 * N, hipMemcpyDeviceToHost, ctx->stream), "bm_meet_host: copy");
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(normal, d.normal, 24 * N, hipMemcpyDeviceToHost, ctx->stream), "bm_meet_host: copy");
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(points, d.points, 96 * N, hipMemcpyDeviceToHost, ctx->stream), "bm_meet_host: copy");
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(separations, d.separations, 32 * N, hipMemcpyDeviceToHost, ctx->stream), "bm_meet_host: copy");
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(ids, d.ids, 16 * N, hipMemcpyDeviceToHost, ctx->stream), "bm_meet_host: copy");
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(shift, d.shift, 96 * N, hipMemcpyDeviceToHost, ctx->stream), "bm_meet_host: copy");
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_meet_host: kernel");
+  HIP_OR_FAIL(ctx, hipEventElapsedTime(&ctx->last_ms, ctx->begin, ctx->end), "bm_meet_host: hipEventElapsedTime");
+  return BM_OK;
+}
+
 int bm_set_launch(bm_context* ctx, int32_t shape, int32_t resident) {
   if (!ctx) return BM_ERR_ARGUMENT;
   if (shape < 0 || shape >= bm::SOFT_SHAPES) return fail(ctx, BM_ERR_ARGUMENT, "bm_set_launch: shape must be 0, 1 or 2");

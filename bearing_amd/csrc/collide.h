@@ -16,4 +16,16 @@ struct CollideDev {
 
 void launch_collide(const CollideDev& d, hipStream_t stream);
 
+struct MeetDev {
+  int n;
+  const double *a, *b;  // body records: the shape record (19), rotation 4, velocity 3, spin 3,
+                        // gravity scale, reach, flags (1 dynamic, 2 spin capped)
+  const double* pair;   // margin, tolerance, dt, gravity 3
+  int *count, *flags;   // flags: 1 shifted, 2 a time of impact was asked
+  double *normal, *points, *separations, *shift;
+  uint32_t* ids;
+};
+
+void launch_meet(const MeetDev& d, hipStream_t stream);
+
 }  // namespace bm

@@ -91,6 +91,76 @@ __device__ __forceinline__ double js_atan2(double y, double x) {
   }
 }
 
+// Sine and cosine as src/core/math.js:26-123 computes them, fdlibm's published method
+// (k_sin.c, k_cos.c, the medium case of e_rem_pio2.c): x less the nearest multiple n of
+// pi/2 in 33-bit pieces (each product with n exact), as a head and a tail, then the
+// minimax polynomials on [-pi/4, pi/4], chosen and signed by n mod 4. Math.round's
+// halves-up rounding of x * 2/pi is floor, then up when the remainder is at least 0.5.
+struct Reduced {
+  double head, tail;
+  int quadrant;
+};
+__device__ __forceinline__ Reduced reduce_pio2(double x) {
+  const double n0 = __builtin_floor(x * 6.36619772367581382433e-01);
+  const double n = x * 6.36619772367581382433e-01 - n0 >= 0.5 ? n0 + 1 : n0;
+  double r = x - n * 1.57079632673412561417e+00;
+  double w = n * 6.07710050650619224932e-11;
+  double head = r - w;
+  const double limit = dabs(x);
+  if (dabs(head) < limit * 3.0517578125e-05) {
+    double t = r;
+    w = n * 6.07710050630396597660e-11;
+    r = t - w;
+    w = n * 2.02226624879595063154e-21 - ((t - r) - w);
+    head = r - w;
+    if (dabs(head) < limit * 8.881784197001252e-16) {
+      t = r;
+      w = n * 2.02226624871116645580e-21;
+      r = t - w;
+      w = n * 8.47842766036889956997e-32 - ((t - r) - w);
+      head = r - w;
+    }
+  }
+  Reduced out;
+  out.head = head;
+  out.tail = (r - head) - w;
+  out.quadrant = (int)((long long)n & 3);
+  return out;
+}
+__device__ __forceinline__ double kernel_sin(double x, double y) {
+  const double z = x * x, v = z * x;
+  const double r = 8.33333333332248946124e-03 +
+      z * (-1.98412698298579493134e-04 + z * (2.75573137070700676789e-06 +
+      z * (-2.50507602534068634195e-08 + z * 1.58969099521155010221e-10)));
+  return x - ((z * (0.5 * y - v * r) - y) - v * -1.66666666666666324348e-01);
+}
+__device__ __forceinline__ double kernel_cos(double x, double y) {
+  const double z = x * x;
+  const double r = z * (4.16666666666666019037e-02 + z * (-1.38888888888741095749e-03 +
+      z * (2.48015872894767294178e-05 + z * (-2.75573143513906633035e-07 +
+      z * (2.08757232129817482790e-09 + z * -1.13596475577881948265e-11)))));
+  const double hz = 0.5 * z, w = 1 - hz;
+  return w + (((1 - w) - hz) + (z * r - x * y));
+}
+__device__ __forceinline__ double js_sin(double x) {
+  const Reduced r = reduce_pio2(x);
+  switch (r.quadrant) {
+    case 0: return kernel_sin(r.head, r.tail);
+    case 1: return kernel_cos(r.head, r.tail);
+    case 2: return -kernel_sin(r.head, r.tail);
+    default: return -kernel_cos(r.head, r.tail);
+  }
+}
+__device__ __forceinline__ double js_cos(double x) {
+  const Reduced r = reduce_pio2(x);
+  switch (r.quadrant) {
+    case 0: return kernel_cos(r.head, r.tail);
+    case 1: return -kernel_sin(r.head, r.tail);
+    case 2: return -kernel_cos(r.head, r.tail);
+    default: return kernel_sin(r.head, r.tail);
+  }
+}
+
 // Rotation by a unit quaternion (x, y, z, w), or by its conjugate:
 // v' = v + w t + u x t, t = 2 u x v (src/core/math.js:212-220).
 struct V3 {

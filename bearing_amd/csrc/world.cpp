@@ -23,7 +23,7 @@
 namespace {
 
 constexpr int P = 4;
-constexpr int DYNAMIC = 0, STATIC = 1, KINEMATIC = 2, CHARACTER = 3;
+constexpr int DYNAMIC = 0, STATIC = 1;
 constexpr int BOX = 0, SPHERE = 1;
 constexpr int MOVING_TREE = 1, STATIC_TREE = 0;
 constexpr double PIXEL = 1.0 / 200;      // src/contact.js:125
@@ -227,8 +227,8 @@ struct bm_world {
   Pairs pairs;
   Contacts c;
   // Narrowphase batches for the device.
-  std::vector<double> batch_a, batch_b, batch_margin, batch_tolerance, batch_normal, batch_points, batch_seps;
-  std::vector<int32_t> batch_count;
+  std::vector<double> batch_a, batch_b, batch_pair, batch_normal, batch_points, batch_seps, batch_shift;
+  std::vector<int32_t> batch_count, batch_flags;
   std::vector<uint32_t> batch_ids;
 };
 
@@ -585,8 +585,9 @@ void basis(Contacts& c, int m) {
   c.t2[3 * m] = ny * tz - nz * ty; c.t2[3 * m + 1] = nz * tx - nx * tz; c.t2[3 * m + 2] = nx * ty - ny * tx;
 }
 
-// src/contact.js:889-921 (emit), sign 1, no sub-manifold, not shifted.
-int emit(bm_world* w, int p, int a, int b, const Manifold& mf, double tolerance) {
+// src/contact.js:889-921 (emit), sign 1, no sub-manifold; `shift` (3 a point) for a
+// manifold met at the moment of impact, else null.
+int emit(bm_world* w, int p, int a, int b, const Manifold& mf, double tolerance, const double* shift) {
   Contacts& c = w->c;
   const int m = c.count++;
   c.ensure(c.count);
@@ -605,7 +606,7 @@ int emit(bm_world* w, int p, int a, int b, const Manifold& mf, double tolerance)
     c.pid[i] = mf.ids[k];
     c.separation[i] = mf.separations[k];
     for (int j = 0; j < 3; j++) c.point[3 * i + j] = mf.points[3 * k + j];
-    for (int j = 0; j < 3; j++) c.shift[3 * i + j] = 0;
+    for (int j = 0; j < 3; j++) c.shift[3 * i + j] = shift ? shift[3 * k + j] : 0;
     c.normal_impulse[i] = 0;
   }
   basis(c, m);
@@ -650,7 +651,10 @@ void carry(bm_world* w, int p, int m) {
   }
 }
 
-// A shape record for the device narrowphase (src/contact.js:1244-1276 load).
+// A body record for the device meet (src/contact.js:1244-1276 load, and what moving()
+// and src/impact.js start read): the shape (kind, centre 3, axes 9, half 3, weight 3),
+// rotation 4, velocity 3, spin 3, gravity scale (0 unless dynamic), reach, flags
+// (1 dynamic, 2 spin capped at pi/dt: dynamic and not a plain sphere).
 void load(const bm_world* w, int s, double* r) {
   r[0] = w->shape[s] == BOX ? 2.0 : 0.0;
   for (int k = 0; k < 3; k++) r[1 + k] = w->position[3 * s + k];
@@ -662,6 +666,13 @@ void load(const bm_world* w, int s, double* r) {
   }
   const double scale = w->type[s] == DYNAMIC ? w->gravity_scale[s] : 0;
   for (int k = 0; k < 3; k++) r[16 + k] = w->gravity[k] * scale;
+  for (int k = 0; k < 4; k++) r[19 + k] = w->rotation[4 * s + k];
+  for (int k = 0; k < 3; k++) r[23 + k] = w->velocity[3 * s + k];
+  for (int k = 0; k < 3; k++) r[26 + k] = w->angular[3 * s + k];
+  r[29] = scale;
+  r[30] = w->reach[s];
+  const bool dynamic = w->type[s] == DYNAMIC;
+  r[31] = (dynamic ? 1 : 0) | (dynamic && (w->shape[s] != SPHERE || w->framed[s] == 1) ? 2 : 0);
 }
 
 // A pair visited by narrow(): what find() decided for it, in the order visited.
@@ -676,47 +687,54 @@ struct Visit {
 // Runs the pending device narrowphase batch and finishes every pending visit in order:
 // src/contact.js:355-363 (meet's result: forget or remember) and 318-322 (wake, emit, carry).
 int flush(bm_world* w, std::vector<Visit>& pending) {
-  const int n = (int)w->batch_margin.size();
+  const int n = (int)w->batch_pair.size() / 6;
   if (n > 0) {
     w->batch_count.resize(n);
+    w->batch_flags.resize(n);
     w->batch_normal.resize(3 * n);
     w->batch_points.resize(12 * n);
     w->batch_seps.resize(4 * n);
     w->batch_ids.resize(4 * n);
-    const int rc = bm_collide_host(w->ctx, n, w->batch_a.data(), w->batch_b.data(), w->batch_margin.data(),
-                                   w->batch_tolerance.data(), w->batch_count.data(), w->batch_normal.data(),
-                                   w->batch_points.data(), w->batch_seps.data(), w->batch_ids.data());
+    w->batch_shift.resize(12 * n);
+    const int rc = bm_meet_host(w->ctx, n, w->batch_a.data(), w->batch_b.data(), w->batch_pair.data(),
+                                w->batch_count.data(), w->batch_normal.data(), w->batch_points.data(),
+                                w->batch_seps.data(), w->batch_ids.data(), w->batch_shift.data(),
+                                w->batch_flags.data());
     if (rc != BM_OK) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx));
   }
   for (Visit& v : pending) {
     if (v.kind == 0) continue;
+    const double* shift = nullptr;
     if (v.kind == 2) {
+      // find() after meet() (src/contact.js:355-363).
       const int k = v.slot;
       Manifold& m = v.m;
       m.count = w->batch_count[k];
       if (m.count < 0) return fail(w, BM_ERR_UNSUPPORTED, "step: a pair of shapes the device narrowphase does not handle");
+      if (w->batch_flags[k] & 2) w->impacts++;
+      if (m.count == 0) {
+        w->pairs.forget(v.p);
+        continue;
+      }
       m.loss = 0;
       for (int i = 0; i < 3; i++) m.normal[i] = w->batch_normal[3 * k + i];
       for (int i = 0; i < 12; i++) m.points[i] = w->batch_points[12 * k + i];
       for (int i = 0; i < 4; i++) m.separations[i] = w->batch_seps[4 * k + i];
       for (int i = 0; i < 4; i++) m.ids[i] = w->batch_ids[4 * k + i];
-      if (m.count == 0) {
-        w->pairs.forget(v.p);
-        continue;
+      if (w->batch_flags[k] & 1) {
+        // A manifold met at the moment of impact is not kept (src/contact.js:541).
+        if (w->pairs.cache[v.p] != 0) w->pairs.uncache(v.p);
+        shift = &w->batch_shift[12 * k];
+      } else {
+        remember(w, v.p, v.a, v.b, m);
       }
-      // meet() (src/contact.js:627-640): not touching and closing by more than the
-      // tolerance asks for a time of impact.
-      const double moves = least(m) > v.tolerance ? moving(w, v.a, v.b) : 0;
-      if (moves > v.tolerance)
-        return fail(w, BM_ERR_UNSUPPORTED, "step: a pair needs a time of impact, which this world does not compute yet");
-      remember(w, v.p, v.a, v.b, m);
     }
     if (w->awake[v.a] != w->awake[v.b]) wake(w, w->awake[v.a] ? v.b : v.a);
-    const int mi = emit(w, v.p, v.a, v.b, v.m, v.tolerance);
+    const int mi = emit(w, v.p, v.a, v.b, v.m, v.tolerance, shift);
     carry(w, v.p, mi);
   }
   pending.clear();
-  w->batch_a.clear(); w->batch_b.clear(); w->batch_margin.clear(); w->batch_tolerance.clear();
+  w->batch_a.clear(); w->batch_b.clear(); w->batch_pair.clear();
   return BM_OK;
 }
 
@@ -749,14 +767,14 @@ int narrow(bm_world* w, int from) {
         v.kind = 1;
       } else {
         v.kind = 2;
-        v.slot = (int)w->batch_margin.size();
-        double ra[19], rb[19];
+        v.slot = (int)w->batch_pair.size() / 6;
+        double ra[32], rb[32];
         load(w, a, ra);
         load(w, b, rb);
-        w->batch_a.insert(w->batch_a.end(), ra, ra + 19);
-        w->batch_b.insert(w->batch_b.end(), rb, rb + 19);
-        w->batch_margin.push_back(margin);
-        w->batch_tolerance.push_back(tolerance);
+        w->batch_a.insert(w->batch_a.end(), ra, ra + 32);
+        w->batch_b.insert(w->batch_b.end(), rb, rb + 32);
+        const double pair[6] = {margin, tolerance, w->step_dt, w->gravity[0], w->gravity[1], w->gravity[2]};
+        w->batch_pair.insert(w->batch_pair.end(), pair, pair + 6);
       }
     }
     pending.push_back(v);

@@ -57,7 +57,7 @@ class Columns(ctypes.Structure):
 
 
 EXPORTS = ["bm_create", "bm_destroy", "bm_last_error", "bm_upload", "bm_soft_step", "bm_download",
-           "bm_soft_step_host", "bm_last_kernel_ms", "bm_set_launch", "bm_lds_limit", "bm_colour", "bm_collide_host",
+           "bm_soft_step_host", "bm_last_kernel_ms", "bm_set_launch", "bm_lds_limit", "bm_colour", "bm_collide_host", "bm_meet_host",
            "bm_world_create", "bm_world_destroy", "bm_world_error", "bm_world_add", "bm_world_step", "bm_world_read"]
 
 _lib = None
@@ -80,6 +80,7 @@ def load():
         lib.bm_soft_step.argtypes = [vp]
         lib.bm_colour.argtypes = [vp]
         lib.bm_collide_host.argtypes = [vp, ctypes.c_int32, _D, _D, _D, _D, _I32, _D, _D, _D, _U32]
+        lib.bm_meet_host.argtypes = [vp, ctypes.c_int32, _D, _D, _D, _I32, _D, _D, _D, _U32, _D, _I32]
         lib.bm_last_kernel_ms.argtypes = [vp]
         lib.bm_last_kernel_ms.restype = ctypes.c_double
         lib.bm_set_launch.argtypes = [vp, ctypes.c_int32, ctypes.c_int32]
@@ -169,6 +170,23 @@ def collide(ctx, a, b, margin, tolerance):
     ctx.check(ctx.lib.bm_collide_host(ctx.ptr, n, p(a, _D), p(b, _D), p(margin, _D), p(tolerance, _D), p(count, _I32),
                                       p(normal, _D), p(points, _D), p(separations, _D), p(ids, _U32)), "bm_collide_host")
     return count, normal, points, separations, ids
+
+
+def meet(ctx, a, b, pair):
+    """bm_meet_host on body records (32 numbers) and pair records (margin, tolerance, dt,
+    gravity 3): (count, normal, points, separations, ids, shift, flags) per pair."""
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    pair = np.ascontiguousarray(pair, np.float64)
+    n = pair.size // 6
+    count, flags = np.zeros(n, np.int32), np.zeros(n, np.int32)
+    normal, points, separations = np.zeros(3 * n), np.zeros(12 * n), np.zeros(4 * n)
+    ids, shift = np.zeros(4 * n, np.uint32), np.zeros(12 * n)
+    ctx.check(ctx.lib.bm_meet_host(ctx.ptr, n, a.ctypes.data_as(_D), b.ctypes.data_as(_D), pair.ctypes.data_as(_D),
+                                   count.ctypes.data_as(_I32), normal.ctypes.data_as(_D), points.ctypes.data_as(_D),
+                                   separations.ctypes.data_as(_D), ids.ctypes.data_as(_U32), shift.ctypes.data_as(_D),
+                                   flags.ctypes.data_as(_I32)), "bm_meet_host")
+    return count, normal, points, separations, ids, shift, flags
 
 
 class Batch:

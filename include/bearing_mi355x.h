@@ -163,6 +163,23 @@ int bm_collide_host(bm_context* ctx, int32_t n, const double* a, const double* b
                     const double* tolerance, int32_t* count, double* normal, double* points, double* separations,
                     uint32_t* ids);
 
+/*
+ * The narrow driver's meet on the device, one pair a lane: src/contact.js meet
+ * (627-640) for two whole bodies. collide() as above; when the pair is apart by
+ * more than the tolerance and moving() (686-693) says it closes by more within the
+ * step, the time of impact (src/impact.js:63-89, conservative advancement over
+ * src/gjk.js closest), collide() again where the bodies meet with the margin left,
+ * and back() (712-732) carrying the points to the step's start.
+ * Body records, 32 numbers: the shape record above (19), rotation 4, velocity 3,
+ * spin 3, gravity scale (0 unless dynamic), reach, flags (1 dynamic, 2 spin capped at
+ * pi/dt: dynamic and not a plain sphere). Pair records, 6 numbers: margin,
+ * tolerance, dt, gravity 3. Out as bm_collide_host, plus shift 12 a pair (b's end
+ * from a's at the start, src/contact.js SHIFT) and flags (1 shifted: met at the
+ * moment of impact; 2 a time of impact was asked, world._impacts).
+ */
+int bm_meet_host(bm_context* ctx, int32_t n, const double* a, const double* b, const double* pair, int32_t* count,
+                 double* normal, double* points, double* separations, uint32_t* ids, double* shift, int32_t* flags);
+
 /* How bm_soft_step launches, for tuning (bearing_amd/tune.py picks per workload):
  * shape 0 = 256 threads a workgroup with registers for one wave a SIMD, 1 = 256
  * threads and two waves a SIMD, 2 = 512 threads and two waves; resident = 1 keeps

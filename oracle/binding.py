@@ -160,3 +160,55 @@ def collide(shapes_a, shapes_b, margin, tolerance):
     if rc != n:
         raise ValueError(f"oracle_collide: pair {-rc - 1} is of a kind not restated")
     return count, normal, points, separations, ids
+
+
+def _f64(x):
+    return np.ascontiguousarray(x, np.float64)
+
+
+def closest(shapes_a, shapes_b):
+    """oracle_closest on shape records: (distance, nearest point on a, on b) per pair."""
+    a, b = _f64(shapes_a), _f64(shapes_b)
+    n = a.size // 19
+    distance, near_a, near_b = np.zeros(n), np.zeros(3 * n), np.zeros(3 * n)
+    lib().oracle_closest(n, a.ctypes.data_as(_D), b.ctypes.data_as(_D), distance.ctypes.data_as(_D),
+                         near_a.ctypes.data_as(_D), near_b.ctypes.data_as(_D))
+    return distance, near_a, near_b
+
+
+def impact(bodies_a, bodies_b, dt, tolerance):
+    """oracle_impact on body records (32 numbers each): (t, posed a, posed b) per pair,
+    the posed shapes as centre 3 and axes 9."""
+    a, b, dt, tolerance = _f64(bodies_a), _f64(bodies_b), _f64(dt), _f64(tolerance)
+    n = len(dt)
+    t, pa, pb = np.zeros(n), np.zeros(12 * n), np.zeros(12 * n)
+    lib().oracle_impact(n, a.ctypes.data_as(_D), b.ctypes.data_as(_D), dt.ctypes.data_as(_D),
+                        tolerance.ctypes.data_as(_D), t.ctypes.data_as(_D), pa.ctypes.data_as(_D),
+                        pb.ctypes.data_as(_D))
+    return t, pa, pb
+
+
+def meet(bodies_a, bodies_b, pair):
+    """oracle_meet on body records and pair scalars (margin, tolerance, dt, gravity 3):
+    (count, normal, points, separations, ids, shift, flags) per pair."""
+    a, b, pair = _f64(bodies_a), _f64(bodies_b), _f64(pair)
+    n = pair.size // 6
+    count, flags = np.zeros(n, np.int32), np.zeros(n, np.int32)
+    normal, points, separations = np.zeros(3 * n), np.zeros(12 * n), np.zeros(4 * n)
+    ids, shift = np.zeros(4 * n, np.uint32), np.zeros(12 * n)
+    f = lib().oracle_meet
+    f.restype = ctypes.c_int
+    rc = f(n, a.ctypes.data_as(_D), b.ctypes.data_as(_D), pair.ctypes.data_as(_D), count.ctypes.data_as(_I32),
+           normal.ctypes.data_as(_D), points.ctypes.data_as(_D), separations.ctypes.data_as(_D),
+           ids.ctypes.data_as(_U32), shift.ctypes.data_as(_D), flags.ctypes.data_as(_I32))
+    if rc != n:
+        raise ValueError(f"oracle_meet: pair {-rc - 1} is of a kind not restated")
+    return count, normal, points, separations, ids, shift, flags
+
+
+def sincos(x):
+    """oracle_sincos: the reference's sine and cosine of each x."""
+    x = _f64(x)
+    s, c = np.zeros(x.size), np.zeros(x.size)
+    lib().oracle_sincos(x.size, x.ctypes.data_as(_D), s.ctypes.data_as(_D), c.ctypes.data_as(_D))
+    return s, c

@@ -9,6 +9,7 @@
 // on seeded random pairs (tests/golden, collide.*).
 #include <cmath>
 #include <cstdint>
+#include <cstring>
 
 #include "jsmath.h"
 
@@ -555,6 +556,394 @@ int collide(const Shape& a, const Shape& b, double margin, double tolerance, Man
   return out.count;
 }
 
+
+// ---- distance between cores, src/gjk.js ----
+
+constexpr double PROGRESS = 1.0 / 1073741824;  // 2^-30, src/gjk.js:47
+constexpr int GJK_ROUNDS = 64;                  // src/gjk.js:50
+constexpr double FLAT = 1.0 / 4503599627370496; // 2^-52, src/gjk.js:57
+const uint8_t FACES[16] = {0, 1, 2, 3, 0, 1, 3, 2, 0, 2, 3, 1, 1, 2, 3, 0};
+
+// src/gjk.js:66-68 (radius): what the core leaves out of a round shape.
+double radius(const Shape& s) { return s.kind == SPHERE || s.kind == CAPSULE ? s.half[0] : 0; }
+
+// src/gjk.js:70-107 (support) for a sphere's, a capsule's or a box's core.
+void gjk_support(const Shape& s, double x, double y, double z, double* out) {
+  const double* c = s.center;
+  const double* a = s.axes;
+  const double* h = s.half;
+  if (s.kind == SPHERE) {
+    out[0] = c[0]; out[1] = c[1]; out[2] = c[2];
+  } else if (s.kind == CAPSULE) {
+    const double e = x * a[3] + y * a[4] + z * a[5] >= 0 ? h[1] : -h[1];
+    out[0] = c[0] + e * a[3]; out[1] = c[1] + e * a[4]; out[2] = c[2] + e * a[5];
+  } else {
+    const double ex = x * a[0] + y * a[1] + z * a[2] >= 0 ? h[0] : -h[0];
+    const double ey = x * a[3] + y * a[4] + z * a[5] >= 0 ? h[1] : -h[1];
+    const double ez = x * a[6] + y * a[7] + z * a[8] >= 0 ? h[2] : -h[2];
+    out[0] = c[0] + ex * a[0] + ey * a[3] + ez * a[6];
+    out[1] = c[1] + ex * a[1] + ey * a[4] + ez * a[7];
+    out[2] = c[2] + ex * a[2] + ey * a[5] + ez * a[8];
+  }
+}
+
+// The simplex of src/gjk.js (W, PA, PB, L, count) and its reduction (172-277).
+struct Simplex {
+  double W[12], PA[12], PB[12], L[4];
+  int count;
+
+  void move(int to, int from) {
+    if (to == from) return;
+    for (int c = 0; c < 3; c++) {
+      W[3 * to + c] = W[3 * from + c]; PA[3 * to + c] = PA[3 * from + c]; PB[3 * to + c] = PB[3 * from + c];
+    }
+  }
+  void keep(int i, int j, int k, double li, double lj, double lk, int n) {
+    move(0, i); L[0] = li;
+    if (n > 1) { move(1, j); L[1] = lj; }
+    if (n > 2) { move(2, k); L[2] = lk; }
+    count = n;
+  }
+  void segment(int i, int j) {
+    const double ax = W[3 * i], ay = W[3 * i + 1], az = W[3 * i + 2];
+    const double ex = W[3 * j] - ax, ey = W[3 * j + 1] - ay, ez = W[3 * j + 2] - az;
+    const double ee = ex * ex + ey * ey + ez * ez;
+    const double t = ee > 0 ? -(ax * ex + ay * ey + az * ez) / ee : 0;
+    if (t <= 0) keep(i, i, i, 1, 0, 0, 1);
+    else if (t >= 1) keep(j, j, j, 1, 0, 0, 1);
+    else keep(i, j, j, 1 - t, t, 0, 2);
+  }
+  // Ericson's closest point on a triangle to the origin, by its Voronoi regions.
+  void triangle(int i, int j, int k) {
+    const double ax = W[3 * i], ay = W[3 * i + 1], az = W[3 * i + 2];
+    const double bx = W[3 * j], by = W[3 * j + 1], bz = W[3 * j + 2];
+    const double cx = W[3 * k], cy = W[3 * k + 1], cz = W[3 * k + 2];
+    const double abx = bx - ax, aby = by - ay, abz = bz - az, acx = cx - ax, acy = cy - ay, acz = cz - az;
+    const double d1 = -(abx * ax + aby * ay + abz * az), d2 = -(acx * ax + acy * ay + acz * az);
+    if (d1 <= 0 && d2 <= 0) return keep(i, i, i, 1, 0, 0, 1);
+    const double d3 = -(abx * bx + aby * by + abz * bz), d4 = -(acx * bx + acy * by + acz * bz);
+    if (d3 >= 0 && d4 <= d3) return keep(j, j, j, 1, 0, 0, 1);
+    const double vc = d1 * d4 - d3 * d2;
+    if (vc <= 0 && d1 >= 0 && d3 <= 0) {
+      const double v = d1 / (d1 - d3);
+      return keep(i, j, j, 1 - v, v, 0, 2);
+    }
+    const double d5 = -(abx * cx + aby * cy + abz * cz), d6 = -(acx * cx + acy * cy + acz * cz);
+    if (d6 >= 0 && d5 <= d6) return keep(k, k, k, 1, 0, 0, 1);
+    const double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0 && d2 >= 0 && d6 <= 0) {
+      const double w = d2 / (d2 - d6);
+      return keep(i, k, k, 1 - w, w, 0, 2);
+    }
+    const double va = d3 * d6 - d5 * d4;
+    if (va <= 0 && d4 - d3 >= 0 && d5 - d6 >= 0) {
+      const double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+      return keep(j, k, k, 1 - w, w, 0, 2);
+    }
+    const double sc = 1 / (va + vb + vc), v = vb * sc, w = vc * sc;
+    keep(i, j, k, 1 - v - w, v, w, 3);
+  }
+  // The face of the tetrahedron nearest the origin, or false when the origin is inside.
+  bool tetrahedron() {
+    double best = INFINITY;
+    int best_face = -1;
+    double w[12], pa[12], pb[12];
+    std::memcpy(w, W, sizeof w); std::memcpy(pa, PA, sizeof pa); std::memcpy(pb, PB, sizeof pb);
+    bool outside = false;
+    for (int f = 0; f < 4; f++) {
+      const int i = FACES[4 * f], j = FACES[4 * f + 1], k = FACES[4 * f + 2], o = FACES[4 * f + 3];
+      const double ax = w[3 * i], ay = w[3 * i + 1], az = w[3 * i + 2];
+      const double ux = w[3 * j] - ax, uy = w[3 * j + 1] - ay, uz = w[3 * j + 2] - az;
+      const double vx = w[3 * k] - ax, vy = w[3 * k + 1] - ay, vz = w[3 * k + 2] - az;
+      const double nx = uy * vz - uz * vy, ny = uz * vx - ux * vz, nz = ux * vy - uy * vx;
+      const double to_origin = -(nx * ax + ny * ay + nz * az);
+      const double ox = w[3 * o] - ax, oy = w[3 * o + 1] - ay, oz = w[3 * o + 2] - az;
+      const double to_fourth = nx * ox + ny * oy + nz * oz;
+      const bool flat = to_fourth * to_fourth <= FLAT * (nx * nx + ny * ny + nz * nz) * (ox * ox + oy * oy + oz * oz);
+      if (!flat && to_origin * to_fourth >= 0) continue;
+      outside = true;
+      std::memcpy(W, w, sizeof w); std::memcpy(PA, pa, sizeof pa); std::memcpy(PB, pb, sizeof pb);
+      count = 4;
+      triangle(i, j, k);
+      double px = 0, py = 0, pz = 0;
+      for (int m = 0; m < count; m++) {
+        px += L[m] * W[3 * m]; py += L[m] * W[3 * m + 1]; pz += L[m] * W[3 * m + 2];
+      }
+      const double d = px * px + py * py + pz * pz;
+      if (d < best) { best = d; best_face = f; }
+    }
+    if (!outside) return false;
+    const int i = FACES[4 * best_face], j = FACES[4 * best_face + 1], k = FACES[4 * best_face + 2];
+    std::memcpy(W, w, sizeof w); std::memcpy(PA, pa, sizeof pa); std::memcpy(PB, pb, sizeof pb);
+    count = 4;
+    triangle(i, j, k);
+    return true;
+  }
+  bool reduce() {
+    if (count == 2) { segment(0, 1); return true; }
+    if (count == 3) { triangle(0, 1, 2); return true; }
+    return tetrahedron();
+  }
+};
+
+// src/gjk.js:114-155 (closest): the distance between two cores and the nearest points on each.
+double gjk_closest(const Shape& a, const Shape& b, double* near_a, double* near_b) {
+  Simplex S;
+  double sa[3], sb[3];
+  double vx = a.center[0] - b.center[0], vy = a.center[1] - b.center[1], vz = a.center[2] - b.center[2];
+  if (vx == 0 && vy == 0 && vz == 0) vx = 1;
+  gjk_support(a, -vx, -vy, -vz, sa);
+  gjk_support(b, vx, vy, vz, sb);
+  S.W[0] = sa[0] - sb[0]; S.W[1] = sa[1] - sb[1]; S.W[2] = sa[2] - sb[2];
+  S.PA[0] = sa[0]; S.PA[1] = sa[1]; S.PA[2] = sa[2];
+  S.PB[0] = sb[0]; S.PB[1] = sb[1]; S.PB[2] = sb[2];
+  S.count = 1;
+  S.L[0] = 1;
+  vx = S.W[0]; vy = S.W[1]; vz = S.W[2];
+  for (int round = 0; round < GJK_ROUNDS; round++) {
+    const double vv = vx * vx + vy * vy + vz * vz;
+    if (vv == 0) break;
+    gjk_support(a, -vx, -vy, -vz, sa);
+    gjk_support(b, vx, vy, vz, sb);
+    const double wx = sa[0] - sb[0], wy = sa[1] - sb[1], wz = sa[2] - sb[2];
+    if (vv - (vx * wx + vy * wy + vz * wz) <= PROGRESS * vv) break;
+    bool repeat = false;
+    for (int i = 0; i < S.count; i++)
+      if (S.W[3 * i] == wx && S.W[3 * i + 1] == wy && S.W[3 * i + 2] == wz) repeat = true;
+    if (repeat) break;
+    const int i3 = 3 * S.count;
+    S.W[i3] = wx; S.W[i3 + 1] = wy; S.W[i3 + 2] = wz;
+    S.PA[i3] = sa[0]; S.PA[i3 + 1] = sa[1]; S.PA[i3 + 2] = sa[2];
+    S.PB[i3] = sb[0]; S.PB[i3 + 1] = sb[1]; S.PB[i3 + 2] = sb[2];
+    S.count++;
+    if (!S.reduce()) { vx = vy = vz = 0; break; }
+    vx = 0; vy = 0; vz = 0;
+    for (int i = 0; i < S.count; i++) {
+      vx += S.L[i] * S.W[3 * i]; vy += S.L[i] * S.W[3 * i + 1]; vz += S.L[i] * S.W[3 * i + 2];
+    }
+  }
+  double ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0;
+  for (int i = 0; i < S.count; i++) {
+    ax += S.L[i] * S.PA[3 * i]; ay += S.L[i] * S.PA[3 * i + 1]; az += S.L[i] * S.PA[3 * i + 2];
+    bx += S.L[i] * S.PB[3 * i]; by += S.L[i] * S.PB[3 * i + 1]; bz += S.L[i] * S.PB[3 * i + 2];
+  }
+  near_a[0] = ax; near_a[1] = ay; near_a[2] = az;
+  near_b[0] = bx; near_b[1] = by; near_b[2] = bz;
+  return std::sqrt(vx * vx + vy * vy + vz * vz);
+}
+
+// ---- time of impact, src/impact.js ----
+
+constexpr int IMPACT_ROUNDS = 32;   // src/impact.js:31
+constexpr double UNSURE = -2;       // src/impact.js:34
+constexpr double SPIN_CAP = 3.141592653589793;  // pi, src/impact.js:28
+
+// What meet() reads of a body besides its shape: its pose, motion and gravity.
+struct Motion {
+  double q[4], v[3], w[3];
+  double gravity_scale;  // 0 unless dynamic (src/contact.js:687)
+  double reach;
+  bool dynamic, capped;  // capped: dynamic, and not a plain sphere (src/impact.js:99)
+};
+
+// src/impact.js:91-111 (start) for both bodies: positions, velocities, the pull of
+// gravity and spins, the spin cut to pi/dt for all but a plain sphere.
+struct Flight {
+  double P[6], V[6], G[6], WV[6], Q[8];
+};
+void start(Flight& F, const Shape& s, const Motion& m, int o, double dt) {
+  for (int k = 0; k < 3; k++) {
+    F.P[o + k] = s.center[k];
+    F.V[o + k] = m.v[k];
+    F.G[o + k] = s.weight[k];
+    F.WV[o + k] = m.w[k];
+  }
+  if (m.capped) {
+    const double most = SPIN_CAP / dt;
+    const double w2 = F.WV[o] * F.WV[o] + F.WV[o + 1] * F.WV[o + 1] + F.WV[o + 2] * F.WV[o + 2];
+    if (w2 > most * most) {
+      const double k = most / std::sqrt(w2);
+      F.WV[o] *= k; F.WV[o + 1] *= k; F.WV[o + 2] *= k;
+    }
+  }
+  const int q4 = o == 0 ? 0 : 4;
+  for (int k = 0; k < 4; k++) F.Q[q4 + k] = m.q[k];
+}
+
+// src/impact.js:141-149 (turned): (x, y, z, w) times q, normalized.
+void turned(double x, double y, double z, double w, const double* q, double* out) {
+  const double bx = q[0], by = q[1], bz = q[2], bw = q[3];
+  const double rx = w * bx + x * bw + y * bz - z * by;
+  const double ry = w * by + y * bw + z * bx - x * bz;
+  const double rz = w * bz + z * bw + x * by - y * bx;
+  const double rw = w * bw - x * bx - y * by - z * bz;
+  const double l = 1 / std::sqrt(rx * rx + ry * ry + rz * rz + rw * rw);
+  out[0] = rx * l; out[1] = ry * l; out[2] = rz * l; out[3] = rw * l;
+}
+
+// src/impact.js:125-139 (posed): where body o is at time t, falling and turning at its spin.
+void posed(const Flight& F, int o, double t, double* c, double* q) {
+  c[0] = F.P[o] + (F.V[o] + 0.5 * F.G[o] * t) * t;
+  c[1] = F.P[o + 1] + (F.V[o + 1] + 0.5 * F.G[o + 1] * t) * t;
+  c[2] = F.P[o + 2] + (F.V[o + 2] + 0.5 * F.G[o + 2] * t) * t;
+  const int q4 = o == 0 ? 0 : 4;
+  const double wx = F.WV[o], wy = F.WV[o + 1], wz = F.WV[o + 2], w = std::sqrt(wx * wx + wy * wy + wz * wz);
+  if (w == 0 || t == 0) {
+    for (int k = 0; k < 4; k++) q[k] = F.Q[q4 + k];
+    return;
+  }
+  const double half = 0.5 * w * t, k = oracle::js_sin(half) / w;
+  turned(wx * k, wy * k, wz * k, oracle::js_cos(half), F.Q + q4, q);
+}
+
+// src/collide.js:82-91 (Shape.place)
+void place(Shape& s, const double* c, const double* q) {
+  s.center[0] = c[0]; s.center[1] = c[1]; s.center[2] = c[2];
+  const double x = q[0], y = q[1], z = q[2], w = q[3];
+  double* a = s.axes;
+  a[0] = 1 - 2 * (y * y + z * z); a[1] = 2 * (x * y + z * w); a[2] = 2 * (x * z - y * w);
+  a[3] = 2 * (x * y - z * w); a[4] = 1 - 2 * (x * x + z * z); a[5] = 2 * (y * z + x * w);
+  a[6] = 2 * (x * z + y * w); a[7] = 2 * (y * z - x * w); a[8] = 1 - 2 * (x * x + y * y);
+}
+
+// src/impact.js:63-89 (impact): conservative advancement of the cores over [0, dt).
+// t, -1 (they cannot meet within the step) or UNSURE; sa and sb are left posed at the last t tried.
+double impact(Flight& F, Shape& sa, Shape& sb, const Motion& ma, const Motion& mb, double dt, double tolerance) {
+  start(F, sa, ma, 0, dt);
+  start(F, sb, mb, 3, dt);
+  const double ra = radius(sa), rb = radius(sb);
+  const double* WV = F.WV;
+  const double spin = std::sqrt(WV[0] * WV[0] + WV[1] * WV[1] + WV[2] * WV[2]) * ma.reach
+      + std::sqrt(WV[3] * WV[3] + WV[4] * WV[4] + WV[5] * WV[5]) * mb.reach;
+  const double gx = F.G[3] - F.G[0], gy = F.G[4] - F.G[1], gz = F.G[5] - F.G[2];
+  const double gl = std::sqrt(gx * gx + gy * gy + gz * gz);
+  double t = 0;
+  for (int round = 0; round < IMPACT_ROUNDS; round++) {
+    double c[3], q[4];
+    posed(F, 0, t, c, q);
+    place(sa, c, q);
+    posed(F, 3, t, c, q);
+    place(sb, c, q);
+    double na[3], nb[3];
+    const double d = gjk_closest(sa, sb, na, nb), gap = d - ra - rb;
+    if (gap <= tolerance) return t;
+    const double nx = (nb[0] - na[0]) / d, ny = (nb[1] - na[1]) / d, nz = (nb[2] - na[2]) / d;
+    const double vx = F.V[3] - F.V[0] + gx * t, vy = F.V[4] - F.V[1] + gy * t, vz = F.V[5] - F.V[2] + gz * t;
+    const double rate = -(vx * nx + vy * ny + vz * nz) + gl * (dt - t) + spin;
+    if (!(rate > 0)) return -1;
+    t += gap / rate;
+    if (t >= dt) return -1;
+  }
+  return UNSURE;
+}
+
+// src/contact.js:686-693 (moving): how far the pair may close within the step.
+double moving(const Motion& ma, const Motion& mb, const double* g, double dt) {
+  const double dx = mb.v[0] - ma.v[0], dy = mb.v[1] - ma.v[1], dz = mb.v[2] - ma.v[2];
+  const double gs = mb.gravity_scale - ma.gravity_scale, gl = std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]) * std::fabs(gs);
+  const double turning = std::sqrt(ma.w[0] * ma.w[0] + ma.w[1] * ma.w[1] + ma.w[2] * ma.w[2]) * ma.reach
+      + std::sqrt(mb.w[0] * mb.w[0] + mb.w[1] * mb.w[1] + mb.w[2] * mb.w[2]) * mb.reach;
+  return (std::sqrt(dx * dx + dy * dy + dz * dz) + 0.5 * gl * dt + turning) * dt;
+}
+
+double least_separation(const Manifold& m) {
+  double s = INFINITY;
+  for (int k = 0; k < m.count; k++)
+    if (m.separations[k] < s) s = m.separations[k];
+  return s;
+}
+
+// src/contact.js:740-761 (carried) for a whole body: a point held at body-relative
+// (x, y, z) at the moment of impact, turned qt, to where it is at the step's start, turned q.
+void carried(const Shape& s, const double* qt, const double* q, double x, double y, double z, double* out) {
+  if (s.kind == SPHERE) { out[0] = x; out[1] = y; out[2] = z; return; }
+  oracle::rotate(qt, x, y, z, true, out);
+  if (s.kind != CAPSULE) {
+    oracle::rotate(q, out[0], out[1], out[2], false, out);
+    return;
+  }
+  const double half = s.half[1];
+  const double c = out[1] < -half ? -half : out[1] > half ? half : out[1];
+  oracle::rotate(qt, 0, c, 0, false, out);
+  const double rx = x - out[0], ry = y - out[1], rz = z - out[2];
+  oracle::rotate(q, 0, c, 0, false, out);
+  out[0] += rx; out[1] += ry; out[2] += rz;
+}
+
+// src/contact.js:712-732 (back): a manifold found at the moment of impact, its points
+// carried back with their bodies to the step's start; SHIFT keeps how far b's end is
+// from a's there.
+void back(const Flight& F, const Shape& sa, const Shape& sb, const Motion& ma, const Motion& mb, double t,
+          Manifold& mf, double* shift) {
+  double CA[3], QA[4], CB[3], QB[4];
+  posed(F, 0, t, CA, QA);
+  posed(F, 3, t, CB, QB);
+  const double* pa = F.P;
+  const double* pb = F.P + 3;
+  const double nx = mf.normal[0], ny = mf.normal[1], nz = mf.normal[2];
+  for (int k = 0; k < mf.count; k++) {
+    const int k3 = 3 * k;
+    const double half = 0.5 * mf.separations[k];
+    const double px = mf.points[k3], py = mf.points[k3 + 1], pz = mf.points[k3 + 2];
+    double v3[3];
+    carried(sa, QA, ma.q, px - nx * half - CA[0], py - ny * half - CA[1], pz - nz * half - CA[2], v3);
+    const double ax = pa[0] + v3[0], ay = pa[1] + v3[1], az = pa[2] + v3[2];
+    carried(sb, QB, mb.q, px + nx * half - CB[0], py + ny * half - CB[1], pz + nz * half - CB[2], v3);
+    const double bx = pb[0] + v3[0] - ax, by = pb[1] + v3[1] - ay, bz = pb[2] + v3[2] - az;
+    mf.separations[k] = bx * nx + by * ny + bz * nz;
+    mf.points[k3] = ax; mf.points[k3 + 1] = ay; mf.points[k3 + 2] = az;
+    shift[k3] = bx; shift[k3 + 1] = by; shift[k3 + 2] = bz;
+  }
+}
+
+// src/contact.js:627-640 (meet) for two whole bodies: the manifold now, or, when the pair
+// is apart and closing faster than the tolerance, the one at the moment of impact carried
+// back (shifted). Returns the point count (0: none), or -1 for a kind not restated here.
+int meet(Shape sa, Shape sb, const Motion& ma, const Motion& mb, const double* gravity, double dt, double margin,
+         double tolerance, Manifold& out, double* shift, bool& shifted, bool& asked) {
+  shifted = false;
+  asked = false;
+  const int n = collide(sa, sb, margin, tolerance, out);
+  if (n <= 0) return n;
+  const double moves = least_separation(out) > tolerance ? moving(ma, mb, gravity, dt) : 0;
+  if (moves > tolerance) {
+    Flight F;
+    const double t = impact(F, sa, sb, ma, mb, dt, tolerance);
+    asked = true;
+    const double ahead = t >= 0 ? moves * (1 - t / dt) + 2 * tolerance : 0;
+    if (t == -1) return 0;
+    if (t != UNSURE) {
+      const int m = collide(sa, sb, ahead, tolerance, out);
+      if (m <= 0) return m;
+      back(F, sa, sb, ma, mb, t, out, shift);
+      shifted = true;
+    }
+  }
+  return out.count;
+}
+
+void read_shape(const double* r, Shape& s) {
+  s.kind = (int)r[0];
+  for (int i = 0; i < 3; i++) s.center[i] = r[1 + i];
+  for (int i = 0; i < 9; i++) s.axes[i] = r[4 + i];
+  for (int i = 0; i < 3; i++) s.half[i] = r[13 + i];
+  for (int i = 0; i < 3; i++) s.weight[i] = r[16 + i];
+}
+
+// A body record for meet: the shape record (19), then q 4, v 3, w 3, gravity scale
+// (0 unless dynamic), reach, flags (1 dynamic, 2 spin capped): 32 numbers.
+void read_body(const double* r, Shape& s, Motion& m) {
+  read_shape(r, s);
+  for (int i = 0; i < 4; i++) m.q[i] = r[19 + i];
+  for (int i = 0; i < 3; i++) m.v[i] = r[23 + i];
+  for (int i = 0; i < 3; i++) m.w[i] = r[26 + i];
+  m.gravity_scale = r[29];
+  m.reach = r[30];
+  const int flags = (int)r[31];
+  m.dynamic = (flags & 1) != 0;
+  m.capped = (flags & 2) != 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -585,6 +974,68 @@ int oracle_collide(int n, const double* a, const double* b, const double* margin
     for (int i = 0; i < 4; i++) ids[4 * k + i] = m.ids[i];
   }
   return n;
+}
+
+/* src/gjk.js:114-155 on whole shapes (19-number records): distance and nearest points. */
+void oracle_closest(int n, const double* a, const double* b, double* distance, double* near_a, double* near_b) {
+  for (int k = 0; k < n; k++) {
+    Shape sa, sb;
+    read_shape(a + 19 * k, sa);
+    read_shape(b + 19 * k, sb);
+    distance[k] = gjk_closest(sa, sb, near_a + 3 * k, near_b + 3 * k);
+  }
+}
+
+/* src/impact.js:63-89 on body records (32 numbers, see read_body); t, and the two
+   shapes' centres and axes where it left them (12 numbers a shape). */
+void oracle_impact(int n, const double* a, const double* b, const double* dt, const double* tolerance, double* t,
+                   double* posed_a, double* posed_b) {
+  for (int k = 0; k < n; k++) {
+    Shape sa, sb;
+    Motion ma, mb;
+    read_body(a + 32 * k, sa, ma);
+    read_body(b + 32 * k, sb, mb);
+    Flight F;
+    t[k] = impact(F, sa, sb, ma, mb, dt[k], tolerance[k]);
+    for (int i = 0; i < 3; i++) { posed_a[12 * k + i] = sa.center[i]; posed_b[12 * k + i] = sb.center[i]; }
+    for (int i = 0; i < 9; i++) { posed_a[12 * k + 3 + i] = sa.axes[i]; posed_b[12 * k + 3 + i] = sb.axes[i]; }
+  }
+}
+
+/* src/contact.js:627-640 (meet) on body records (32 numbers) and per-pair scalars
+   (margin, tolerance, dt, gravity 3: 6 numbers). Results as oracle_collide's, plus the
+   shift (12 a pair) and flags (1 shifted, 2 a time of impact was asked). Returns n, or
+   -(k+1) for pair k of a kind not restated here. */
+int oracle_meet(int n, const double* a, const double* b, const double* pair, int32_t* count, double* normal,
+                double* points, double* separations, uint32_t* ids, double* shift, int32_t* flags) {
+  for (int k = 0; k < n; k++) {
+    Shape sa, sb;
+    Motion ma, mb;
+    read_body(a + 32 * k, sa, ma);
+    read_body(b + 32 * k, sb, mb);
+    const double* pr = pair + 6 * k;
+    Manifold m{};
+    double sh[12] = {0};
+    bool shifted = false, asked = false;
+    const int c = meet(sa, sb, ma, mb, pr + 3, pr[2], pr[0], pr[1], m, sh, shifted, asked);
+    if (c < 0) return -(k + 1);
+    count[k] = c;
+    flags[k] = (shifted ? 1 : 0) | (asked ? 2 : 0);
+    for (int i = 0; i < 3; i++) normal[3 * k + i] = m.normal[i];
+    for (int i = 0; i < 12; i++) points[12 * k + i] = m.points[i];
+    for (int i = 0; i < 4; i++) separations[4 * k + i] = m.separations[i];
+    for (int i = 0; i < 4; i++) ids[4 * k + i] = m.ids[i];
+    for (int i = 0; i < 12; i++) shift[12 * k + i] = sh[i];
+  }
+  return n;
+}
+
+/* src/core/math.js:102-122, for checking against the reference's own values. */
+void oracle_sincos(int n, const double* x, double* s, double* c) {
+  for (int k = 0; k < n; k++) {
+    s[k] = oracle::js_sin(x[k]);
+    c[k] = oracle::js_cos(x[k]);
+  }
 }
 
 }  // extern "C"

@@ -107,6 +107,91 @@ inline double js_atan2(double y, double x) {
   }
 }
 
+// Math.round(x): the nearest integer, halves toward +infinity, -0 for -0.5 <= x < 0.
+inline double js_round(double x) {
+  double r = std::floor(x);
+  if (x - r >= 0.5) r += 1;
+  if (r == 0 && (x < 0 || std::signbit(x))) return -0.0;
+  return r;
+}
+
+// Sine and cosine as src/core/math.js:26-123 computes them, fdlibm's published method
+// (k_sin.c, k_cos.c and the medium case of e_rem_pio2.c, Sun Microsystems, 1993):
+// x less the nearest multiple n of pi/2, taken off in 33-bit pieces of pi/2 so each
+// product with n is exact, as a head and a tail; then the minimax polynomials for
+// sine or cosine on [-pi/4, pi/4], chosen and signed by n mod 4.
+struct Reduced {
+  double head, tail;
+  int quadrant;
+};
+inline Reduced reduce_pio2(double x) {
+  const double INV_PIO2 = 6.36619772367581382433e-01;
+  const double PIO2_1 = 1.57079632673412561417e+00, PIO2_1T = 6.07710050650619224932e-11;
+  const double PIO2_2 = 6.07710050630396597660e-11, PIO2_2T = 2.02226624879595063154e-21;
+  const double PIO2_3 = 2.02226624871116645580e-21, PIO2_3T = 8.47842766036889956997e-32;
+  const double ROUND_2 = 3.0517578125e-05, ROUND_3 = 8.881784197001252e-16;  // 2^-15, 2^-50
+  const double n = js_round(x * INV_PIO2);
+  double r = x - n * PIO2_1;
+  double w = n * PIO2_1T;
+  double head = r - w;
+  const double limit = std::fabs(x);
+  if (std::fabs(head) < limit * ROUND_2) {
+    double t = r;
+    w = n * PIO2_2;
+    r = t - w;
+    w = n * PIO2_2T - ((t - r) - w);
+    head = r - w;
+    if (std::fabs(head) < limit * ROUND_3) {
+      t = r;
+      w = n * PIO2_3;
+      r = t - w;
+      w = n * PIO2_3T - ((t - r) - w);
+      head = r - w;
+    }
+  }
+  Reduced out;
+  out.head = head;
+  out.tail = (r - head) - w;
+  // n & 3 of a JavaScript number: ToInt32, two's complement.
+  out.quadrant = (int)((int64_t)n & 3);
+  return out;
+}
+inline double kernel_sin(double x, double y) {
+  const double S1 = -1.66666666666666324348e-01, S2 = 8.33333333332248946124e-03;
+  const double S3 = -1.98412698298579493134e-04, S4 = 2.75573137070700676789e-06;
+  const double S5 = -2.50507602534068634195e-08, S6 = 1.58969099521155010221e-10;
+  const double z = x * x, v = z * x;
+  const double r = S2 + z * (S3 + z * (S4 + z * (S5 + z * S6)));
+  return x - ((z * (0.5 * y - v * r) - y) - v * S1);
+}
+inline double kernel_cos(double x, double y) {
+  const double C1 = 4.16666666666666019037e-02, C2 = -1.38888888888741095749e-03;
+  const double C3 = 2.48015872894767294178e-05, C4 = -2.75573143513906633035e-07;
+  const double C5 = 2.08757232129817482790e-09, C6 = -1.13596475577881948265e-11;
+  const double z = x * x;
+  const double r = z * (C1 + z * (C2 + z * (C3 + z * (C4 + z * (C5 + z * C6)))));
+  const double hz = 0.5 * z, w = 1 - hz;
+  return w + (((1 - w) - hz) + (z * r - x * y));
+}
+inline double js_sin(double x) {
+  const Reduced r = reduce_pio2(x);
+  switch (r.quadrant) {
+    case 0: return kernel_sin(r.head, r.tail);
+    case 1: return kernel_cos(r.head, r.tail);
+    case 2: return -kernel_sin(r.head, r.tail);
+    default: return -kernel_cos(r.head, r.tail);
+  }
+}
+inline double js_cos(double x) {
+  const Reduced r = reduce_pio2(x);
+  switch (r.quadrant) {
+    case 0: return kernel_cos(r.head, r.tail);
+    case 1: return -kernel_sin(r.head, r.tail);
+    case 2: return -kernel_cos(r.head, r.tail);
+    default: return kernel_sin(r.head, r.tail);
+  }
+}
+
 // src/core/math.js:212-220: v' = v + w t + u x t with t = 2 u x v; `inverse` uses the conjugate.
 inline void rotate(const double* q, double x, double y, double z, bool inverse, double* out) {
   const double s = inverse ? -1 : 1;

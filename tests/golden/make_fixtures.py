@@ -146,9 +146,81 @@ def run_collide(seed=2026, n=4000):
     os.remove(path)
 
 
+def impact_pairs(seed, n):
+    """Seeded pairs closing on each other within about a step, for the time of impact:
+    spheres and boxes, slow and fast, spinning below and above the pi/dt cap, the second
+    body dynamic or static, some already within the tolerance and some parting. One record
+    a pair: per body kind (0 sphere, 2 box), centre 3, unit quaternion 4, half 3, velocity 3,
+    spin 3, gravity scale, dynamic (1 or 0), reach; then dt, tolerance, gravity 3."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    recs = []
+
+    def quat():
+        if rng.random() < 0.3:
+            return [0.0, 0.0, 0.0, 1.0]
+        v = rng.normal(size=4)
+        return list(v / np.linalg.norm(v))
+
+    def body(kind, centre, dynamic):
+        if kind == 0:
+            r = rng.uniform(0.05, 0.6)
+            half, reach, size = [r, 0.0, 0.0], r, 2 * r
+            q = [0.0, 0.0, 0.0, 1.0]
+        else:
+            h = rng.uniform(0.05, 1.0, 3)
+            half, reach, size = list(h), float(np.linalg.norm(h)), 2 * float(h.min())
+            q = quat()
+        spin = rng.choice([0.0, 1.0, 20.0, 400.0]) if dynamic else 0.0
+        w = list(rng.normal(0, spin, 3)) if spin > 0 else [0.0, 0.0, 0.0]
+        return {"kind": kind, "c": np.asarray(centre, float), "q": q, "half": half, "w": w, "reach": reach,
+                "size": size, "dyn": dynamic, "gs": float(rng.choice([1.0, 1.0, 0.5])) if dynamic else 0.0}
+
+    for k in range(n):
+        kinds = [(0, 0), (0, 2), (2, 0), (2, 2)][k % 4]
+        dt = float(rng.choice([1 / 60, 1 / 120]))
+        a = body(kinds[0], rng.uniform(-5, 5, 3), True)
+        b = body(kinds[1], [0, 0, 0], rng.random() < 0.7)
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        speed = float(np.exp(rng.uniform(np.log(0.5), np.log(80))))
+        case = rng.random()
+        if case < 0.1:
+            gap0 = rng.uniform(-0.01, 0.002)     # about touching
+        elif case < 0.2:
+            gap0 = rng.uniform(2, 4) * speed * dt  # too far to meet
+        else:
+            gap0 = rng.uniform(-0.5, 1.0) * speed * dt
+        b["c"] = a["c"] + d * (a["reach"] + b["reach"] + gap0)
+        towards = d + rng.normal(0, 0.1, 3)
+        towards /= np.linalg.norm(towards)
+        va = towards * speed * (-1 if rng.random() < 0.1 else 1)
+        vb = rng.normal(0, 1, 3) if b["dyn"] else np.zeros(3)
+        tol = min(a["size"], b["size"]) / 200
+        rec = []
+        for x, v in ((a, va), (b, vb)):
+            rec += [x["kind"], *x["c"], *x["q"], *x["half"], *v, *x["w"], x["gs"], 1.0 if x["dyn"] else 0.0, x["reach"]]
+        rec += [dt, tol, 0.0, -9.81, 0.0]
+        recs.append(rec)
+    return np.asarray(recs, np.float64)
+
+
+def run_impact(seed=2026, n=2000):
+    pairs = impact_pairs(seed, n)
+    path = os.path.join(DATA, "impact.in.bin")
+    pairs.tofile(path)
+    subprocess.run(["node", "--no-warnings", "--experimental-loader", os.path.join(HERE, "lower.mjs"),
+                    os.path.join(HERE, "make_impact.mjs"), path, os.path.join(DATA, f"impact.{seed}.bmfx.gz")],
+                   check=True)
+    os.remove(path)
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["collide"]:
         run_collide()
+        sys.exit(0)
+    if sys.argv[1:] == ["impact"]:
+        run_impact()
         sys.exit(0)
     for name in sys.argv[1:] or [j for j in JOBS if j != "rain"]:
         run(name)

@@ -12,6 +12,9 @@
 
 extern "C" int oracle_collide(int n, const double* a, const double* b, const double* margin, const double* tolerance,
                               int32_t* count, double* normal, double* points, double* separations, uint32_t* ids);
+extern "C" int oracle_meet(int n, const double* a, const double* b, const double* pair, int32_t* count,
+                           double* normal, double* points, double* separations, uint32_t* ids, double* shift,
+                           int32_t* flags);
 
 struct bm_context {
   bm_columns h;
@@ -122,6 +125,15 @@ int bm_collide_host(bm_context*, int32_t n, const double* a, const double* b, co
                     const double* tolerance, int32_t* count, double* normal, double* points, double* separations,
                     uint32_t* ids) {
   oracle_collide(n, a, b, margin, tolerance, count, normal, points, separations, ids);
+  return BM_OK;
+}
+
+int bm_meet_host(bm_context* ctx, int32_t n, const double* a, const double* b, const double* pair, int32_t* count,
+                 double* normal, double* points, double* separations, uint32_t* ids, double* shift, int32_t* flags) {
+  if (oracle_meet(n, a, b, pair, count, normal, points, separations, ids, shift, flags) != n) {
+    ctx->error = "oracle meet: a pair of kinds it does not cover";
+    return BM_ERR_UNSUPPORTED;
+  }
   return BM_OK;
 }
 

@@ -15,7 +15,7 @@ def traj(name):
     return fixtures.load(os.path.join(fixtures.DATA, f"{name}.traj.bmfx.gz"))
 
 
-@pytest.mark.parametrize("name", ["pyramid", "tower"])
+@pytest.mark.parametrize("name", ["pyramid", "tower", "throw"])
 def test_gpu_world_matches_reference_trajectory(name):
     print("trajectory", name, flush=True)
     fx = traj(name)
@@ -27,10 +27,10 @@ def test_gpu_world_matches_reference_trajectory(name):
     assert matched == fx.meta["steps"]
 
 
-def test_gpu_world_stops_where_it_would_need_a_time_of_impact():
+def test_gpu_world_stops_where_it_would_sweep_a_fast_body():
     fx = traj("rain_small")
     w = native.World(0)
     w.add(init_columns(fx))
-    with pytest.raises(native.BearingError, match="time of impact"):
+    with pytest.raises(native.BearingError, match="swept"):
         run(w, fx)
     w.close()
