@@ -971,6 +971,7 @@ __global__ void meet_kernel(MeetDev d) {
   const double* pr = d.pair + 6 * (long)k;
   narrow::Manifold m;
   m.count = 0;
+  for (int i = 0; i < 3; i++) m.normal[i] = 0;
   double shift[12];
   for (int i = 0; i < 12; i++) shift[i] = 0;
   int flags = 0;
