@@ -17,6 +17,8 @@ class Tree {
   std::vector<int> parent, child1, child2, height, item;
   std::vector<int> stack;
 
+  std::vector<double> at;
+
   int insert(int it, const double* b);
   void remove(int leaf);
   void move(int leaf, const double* b);
@@ -39,7 +41,64 @@ class Tree {
     }
   }
 
+  // src/tree.js:288-334 (raycast): the ray from o along d, boxes grown by g, nearer child
+  // first; visit(item) returns the new maxT. Returns the last maxT.
+  template <class F>
+  double raycast(const double* o, const double* d, double max_t, const double* g, F&& visit) {
+    if (root == NIL) return max_t;
+    const double ix = 1 / d[0], iy = 1 / d[1], iz = 1 / d[2];
+    const double inv[3] = {ix, iy, iz};
+    const double t = enter(root, o, d, inv, g, max_t);
+    if (t < 0) return max_t;
+    stack.clear();
+    at.clear();
+    stack.push_back(root);
+    at.push_back(t);
+    while (!stack.empty()) {
+      const int n = stack.back();
+      const double tn = at.back();
+      stack.pop_back();
+      at.pop_back();
+      if (tn > max_t) continue;
+      if (height[n] == 0) {
+        max_t = visit(item[n]);
+        continue;
+      }
+      const int c1 = child1[n], c2 = child2[n];
+      const double t1 = enter(c1, o, d, inv, g, max_t), t2 = enter(c2, o, d, inv, g, max_t);
+      if (t1 >= 0 && t2 >= 0) {
+        if (t2 < t1) {
+          stack.push_back(c1); at.push_back(t1); stack.push_back(c2); at.push_back(t2);
+        } else {
+          stack.push_back(c2); at.push_back(t2); stack.push_back(c1); at.push_back(t1);
+        }
+      } else if (t1 >= 0) {
+        stack.push_back(c1); at.push_back(t1);
+      } else if (t2 >= 0) {
+        stack.push_back(c2); at.push_back(t2);
+      }
+    }
+    return max_t;
+  }
+
  private:
+  // src/tree.js:336-367 (enter): where the ray enters node n's box grown by g, or -1.
+  double enter(int n, const double* o, const double* d, const double* inv, const double* g, double max_t) const {
+    const double* b = &box[6 * n];
+    double t0 = 0, t1 = max_t;
+    for (int k = 0; k < 3; k++) {
+      if (d[k] == 0) {
+        if (o[k] < b[k] - g[k] || o[k] > b[k + 3] + g[k]) return -1;
+      } else {
+        double near = (b[k] - g[k] - o[k]) * inv[k], far = (b[k + 3] + g[k] - o[k]) * inv[k];
+        if (near > far) { const double s = near; near = far; far = s; }
+        if (near > t0) t0 = near;
+        if (far < t1) t1 = far;
+        if (t0 > t1) return -1;
+      }
+    }
+    return t0;
+  }
   int alloc();
   void release(int n);
   void insert_leaf(int leaf);

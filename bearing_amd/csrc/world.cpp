@@ -53,6 +53,16 @@ void axes_of(const double* q, double* ax) {
   ax[6] = 2 * (x * z + y * w); ax[7] = 2 * (y * z - x * w); ax[8] = 1 - 2 * (x * x + y * y);
 }
 
+// src/core/math.js:211-220 (rotate): v' = v + w t + u x t with t = 2 u x v; `inverse` turns by the conjugate.
+void rotate(const double* q, double x, double y, double z, bool inverse, double* out) {
+  const double sg = inverse ? -1 : 1;
+  const double ux = q[0] * sg, uy = q[1] * sg, uz = q[2] * sg, w = q[3];
+  const double tx = 2 * (uy * z - uz * y), ty = 2 * (uz * x - ux * z), tz = 2 * (ux * y - uy * x);
+  out[0] = x + w * tx + (uy * tz - uz * ty);
+  out[1] = y + w * ty + (uz * tx - ux * tz);
+  out[2] = z + w * tz + (ux * ty - uy * tx);
+}
+
 // The pair list (src/pairs.js:40-378): [0, active) awake, [active, count) asleep.
 struct Pairs {
   int count = 0, active = 0;
@@ -211,14 +221,18 @@ struct bm_world {
   std::vector<double> position, rotation, velocity, angular, dims, mass, inv_mass, inertia, inv_inertia, inv_i, axes,
       box, motion, start, start_q, tumbled, ceiling, gravity_scale, damping, angular_damping, size, reach, friction,
       restitution, still, rest, least;
-  std::vector<uint8_t> type, shape, frozen, calm, lock, framed, awake, fast;
+  std::vector<uint8_t> type, shape, frozen, calm, lock, framed, awake, fast, caught;
   std::vector<int32_t> joint_head, in_awake, awake_list, island, island_next, parent, slept;
   std::vector<float> transforms;
   int awake_count = 0, frozen_count = 0, slept_count = 0;
   bool still_ready = false, wake_pending = false;
   double step_dt = 0;
   int step_count = 0;
-  long impacts = 0, solver_steps = 0;
+  long impacts = 0, solver_steps = 0, catches = 0, catch_steps = 0;
+  // Awake dynamic bodies outside their fat boxes, listed once a step when a sweep asks
+  // (src/query.js:138-167); strays_at is the step they were listed in.
+  std::vector<int32_t> strays;
+  int strays_at = -1;
   // Broadphase (src/broadphase.js:56-209).
   bm::Tree trees[2];
   std::vector<int32_t> leaf, moved;
@@ -935,6 +949,425 @@ void sleep_step(bm_world* w, double dt) {
   }
 }
 
+// ---- the continuous catch: src/world.js:1683-1743 (_catch), src/query.js:355-445 and
+// 607-642 (advance, sweep), src/gjk.js (closest), src/contact.js:1755-1776 (gapOf) ----
+
+constexpr int CORE_SPHERE = 0, CORE_CAPSULE = 1, CORE_BOX = 2;  // src/collide.js:44 kinds
+
+// A shape as the sweep places it: kind, centre, axes (x, y, z columns), half extents.
+struct Core {
+  int kind;
+  double center[3], axes[9], half[3], weight[3];
+};
+
+double core_radius(const Core& s) { return s.kind == CORE_SPHERE || s.kind == CORE_CAPSULE ? s.half[0] : 0; }
+
+// src/gjk.js:70-107 (support) for a sphere's, a capsule's or a box's core.
+void core_support(const Core& s, double x, double y, double z, double* out) {
+  const double* c = s.center;
+  const double* a = s.axes;
+  const double* h = s.half;
+  if (s.kind == CORE_SPHERE) {
+    out[0] = c[0]; out[1] = c[1]; out[2] = c[2];
+  } else if (s.kind == CORE_CAPSULE) {
+    const double e = x * a[3] + y * a[4] + z * a[5] >= 0 ? h[1] : -h[1];
+    out[0] = c[0] + e * a[3]; out[1] = c[1] + e * a[4]; out[2] = c[2] + e * a[5];
+  } else {
+    const double ex = x * a[0] + y * a[1] + z * a[2] >= 0 ? h[0] : -h[0];
+    const double ey = x * a[3] + y * a[4] + z * a[5] >= 0 ? h[1] : -h[1];
+    const double ez = x * a[6] + y * a[7] + z * a[8] >= 0 ? h[2] : -h[2];
+    out[0] = c[0] + ex * a[0] + ey * a[3] + ez * a[6];
+    out[1] = c[1] + ex * a[1] + ey * a[4] + ez * a[7];
+    out[2] = c[2] + ex * a[2] + ey * a[5] + ez * a[8];
+  }
+}
+
+// GJK's simplex and its reduction to the feature nearest the origin (src/gjk.js:172-277).
+struct Simplex {
+  double W[12], PA[12], PB[12], L[4];
+  int count;
+  void move(int to, int from) {
+    if (to == from) return;
+    for (int c = 0; c < 3; c++) {
+      W[3 * to + c] = W[3 * from + c]; PA[3 * to + c] = PA[3 * from + c]; PB[3 * to + c] = PB[3 * from + c];
+    }
+  }
+  void keep(int i, int j, int k, double li, double lj, double lk, int n) {
+    move(0, i); L[0] = li;
+    if (n > 1) { move(1, j); L[1] = lj; }
+    if (n > 2) { move(2, k); L[2] = lk; }
+    count = n;
+  }
+  void segment(int i, int j) {
+    const double ax = W[3 * i], ay = W[3 * i + 1], az = W[3 * i + 2];
+    const double ex = W[3 * j] - ax, ey = W[3 * j + 1] - ay, ez = W[3 * j + 2] - az;
+    const double ee = ex * ex + ey * ey + ez * ez;
+    const double t = ee > 0 ? -(ax * ex + ay * ey + az * ez) / ee : 0;
+    if (t <= 0) keep(i, i, i, 1, 0, 0, 1);
+    else if (t >= 1) keep(j, j, j, 1, 0, 0, 1);
+    else keep(i, j, j, 1 - t, t, 0, 2);
+  }
+  void triangle(int i, int j, int k) {
+    const double ax = W[3 * i], ay = W[3 * i + 1], az = W[3 * i + 2];
+    const double bx = W[3 * j], by = W[3 * j + 1], bz = W[3 * j + 2];
+    const double cx = W[3 * k], cy = W[3 * k + 1], cz = W[3 * k + 2];
+    const double abx = bx - ax, aby = by - ay, abz = bz - az, acx = cx - ax, acy = cy - ay, acz = cz - az;
+    const double d1 = -(abx * ax + aby * ay + abz * az), d2 = -(acx * ax + acy * ay + acz * az);
+    if (d1 <= 0 && d2 <= 0) return keep(i, i, i, 1, 0, 0, 1);
+    const double d3 = -(abx * bx + aby * by + abz * bz), d4 = -(acx * bx + acy * by + acz * bz);
+    if (d3 >= 0 && d4 <= d3) return keep(j, j, j, 1, 0, 0, 1);
+    const double vc = d1 * d4 - d3 * d2;
+    if (vc <= 0 && d1 >= 0 && d3 <= 0) {
+      const double v = d1 / (d1 - d3);
+      return keep(i, j, j, 1 - v, v, 0, 2);
+    }
+    const double d5 = -(abx * cx + aby * cy + abz * cz), d6 = -(acx * cx + acy * cy + acz * cz);
+    if (d6 >= 0 && d5 <= d6) return keep(k, k, k, 1, 0, 0, 1);
+    const double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0 && d2 >= 0 && d6 <= 0) {
+      const double w = d2 / (d2 - d6);
+      return keep(i, k, k, 1 - w, w, 0, 2);
+    }
+    const double va = d3 * d6 - d5 * d4;
+    if (va <= 0 && d4 - d3 >= 0 && d5 - d6 >= 0) {
+      const double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+      return keep(j, k, k, 1 - w, w, 0, 2);
+    }
+    const double sc = 1 / (va + vb + vc), v = vb * sc, w = vc * sc;
+    keep(i, j, k, 1 - v - w, v, w, 3);
+  }
+  bool tetrahedron() {
+    static const uint8_t FACES[16] = {0, 1, 2, 3, 0, 1, 3, 2, 0, 2, 3, 1, 1, 2, 3, 0};  // src/gjk.js:60
+    const double FLAT = 1.0 / 4503599627370496;                                       // 2^-52
+    double best = INFINITY;
+    int best_face = -1;
+    double w[12], pa[12], pb[12];
+    std::memcpy(w, W, sizeof w); std::memcpy(pa, PA, sizeof pa); std::memcpy(pb, PB, sizeof pb);
+    bool outside = false;
+    for (int f = 0; f < 4; f++) {
+      const int i = FACES[4 * f], j = FACES[4 * f + 1], k = FACES[4 * f + 2], o = FACES[4 * f + 3];
+      const double ax = w[3 * i], ay = w[3 * i + 1], az = w[3 * i + 2];
+      const double ux = w[3 * j] - ax, uy = w[3 * j + 1] - ay, uz = w[3 * j + 2] - az;
+      const double vx = w[3 * k] - ax, vy = w[3 * k + 1] - ay, vz = w[3 * k + 2] - az;
+      const double nx = uy * vz - uz * vy, ny = uz * vx - ux * vz, nz = ux * vy - uy * vx;
+      const double to_origin = -(nx * ax + ny * ay + nz * az);
+      const double ox = w[3 * o] - ax, oy = w[3 * o + 1] - ay, oz = w[3 * o + 2] - az;
+      const double to_fourth = nx * ox + ny * oy + nz * oz;
+      const bool flat = to_fourth * to_fourth <= FLAT * (nx * nx + ny * ny + nz * nz) * (ox * ox + oy * oy + oz * oz);
+      if (!flat && to_origin * to_fourth >= 0) continue;
+      outside = true;
+      std::memcpy(W, w, sizeof w); std::memcpy(PA, pa, sizeof pa); std::memcpy(PB, pb, sizeof pb);
+      count = 4;
+      triangle(i, j, k);
+      double px = 0, py = 0, pz = 0;
+      for (int m = 0; m < count; m++) {
+        px += L[m] * W[3 * m]; py += L[m] * W[3 * m + 1]; pz += L[m] * W[3 * m + 2];
+      }
+      const double d = px * px + py * py + pz * pz;
+      if (d < best) { best = d; best_face = f; }
+    }
+    if (!outside) return false;
+    const int i = FACES[4 * best_face], j = FACES[4 * best_face + 1], k = FACES[4 * best_face + 2];
+    std::memcpy(W, w, sizeof w); std::memcpy(PA, pa, sizeof pa); std::memcpy(PB, pb, sizeof pb);
+    count = 4;
+    triangle(i, j, k);
+    return true;
+  }
+  bool reduce() {
+    if (count == 2) { segment(0, 1); return true; }
+    if (count == 3) { triangle(0, 1, 2); return true; }
+    return tetrahedron();
+  }
+};
+
+// src/gjk.js:114-155 (closest)
+double closest(const Core& a, const Core& b, double* near_a, double* near_b) {
+  const double PROGRESS = 1.0 / 1073741824;  // 2^-30
+  Simplex S;
+  double sa[3], sb[3];
+  double vx = a.center[0] - b.center[0], vy = a.center[1] - b.center[1], vz = a.center[2] - b.center[2];
+  if (vx == 0 && vy == 0 && vz == 0) vx = 1;
+  core_support(a, -vx, -vy, -vz, sa);
+  core_support(b, vx, vy, vz, sb);
+  for (int c = 0; c < 3; c++) { S.W[c] = sa[c] - sb[c]; S.PA[c] = sa[c]; S.PB[c] = sb[c]; }
+  S.count = 1;
+  S.L[0] = 1;
+  vx = S.W[0]; vy = S.W[1]; vz = S.W[2];
+  for (int round = 0; round < 64; round++) {
+    const double vv = vx * vx + vy * vy + vz * vz;
+    if (vv == 0) break;
+    core_support(a, -vx, -vy, -vz, sa);
+    core_support(b, vx, vy, vz, sb);
+    const double wx = sa[0] - sb[0], wy = sa[1] - sb[1], wz = sa[2] - sb[2];
+    if (vv - (vx * wx + vy * wy + vz * wz) <= PROGRESS * vv) break;
+    bool repeat = false;
+    for (int i = 0; i < S.count; i++)
+      if (S.W[3 * i] == wx && S.W[3 * i + 1] == wy && S.W[3 * i + 2] == wz) repeat = true;
+    if (repeat) break;
+    const int i3 = 3 * S.count;
+    S.W[i3] = wx; S.W[i3 + 1] = wy; S.W[i3 + 2] = wz;
+    for (int c = 0; c < 3; c++) { S.PA[i3 + c] = sa[c]; S.PB[i3 + c] = sb[c]; }
+    S.count++;
+    if (!S.reduce()) { vx = vy = vz = 0; break; }
+    vx = 0; vy = 0; vz = 0;
+    for (int i = 0; i < S.count; i++) {
+      vx += S.L[i] * S.W[3 * i]; vy += S.L[i] * S.W[3 * i + 1]; vz += S.L[i] * S.W[3 * i + 2];
+    }
+  }
+  double ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0;
+  for (int i = 0; i < S.count; i++) {
+    ax += S.L[i] * S.PA[3 * i]; ay += S.L[i] * S.PA[3 * i + 1]; az += S.L[i] * S.PA[3 * i + 2];
+    bx += S.L[i] * S.PB[3 * i]; by += S.L[i] * S.PB[3 * i + 1]; bz += S.L[i] * S.PB[3 * i + 2];
+  }
+  near_a[0] = ax; near_a[1] = ay; near_a[2] = az;
+  near_b[0] = bx; near_b[1] = by; near_b[2] = bz;
+  return std::sqrt(vx * vx + vy * vy + vz * vz);
+}
+
+// src/collide.js:82-91 (Shape.place)
+void place_at(Core& s, double x, double y, double z, const double* q) {
+  s.center[0] = x; s.center[1] = y; s.center[2] = z;
+  axes_of(q, s.axes);
+}
+
+// src/query.js:91-95 (place): slot s's shape where it is now.
+void place_body(const bm_world* w, int s, Core& out) {
+  const int s3 = 3 * s;
+  if (w->shape[s] == BOX) {
+    out.kind = CORE_BOX;
+    for (int k = 0; k < 3; k++) out.half[k] = w->dims[s3 + k] * 0.5;
+  } else {
+    out.kind = CORE_SPHERE;
+    out.half[0] = w->dims[s3]; out.half[1] = 0; out.half[2] = 0;
+  }
+  const double scale = w->type[s] == DYNAMIC ? w->gravity_scale[s] : 0;
+  for (int k = 0; k < 3; k++) out.weight[k] = w->gravity[k] * scale;
+  place_at(out, w->position[s3], w->position[s3 + 1], w->position[s3 + 2], &w->rotation[4 * s]);
+}
+
+// src/query.js:114-128 (extents) for a sphere or a box.
+void extents(const Core& s, double* out) {
+  const double* a = s.axes;
+  const double* h = s.half;
+  if (s.kind == CORE_SPHERE) {
+    out[0] = out[1] = out[2] = h[0];
+  } else {
+    for (int k = 0; k < 3; k++)
+      out[k] = std::fabs(a[k]) * h[0] + std::fabs(a[3 + k]) * h[1] + std::fabs(a[6 + k]) * h[2];
+  }
+}
+
+// src/query.js:138-167 (strays), listed once a step.
+const std::vector<int32_t>& strays(bm_world* w) {
+  if (w->strays_at == w->step_count) return w->strays;
+  w->strays.clear();
+  const double* fat = w->trees[MOVING_TREE].box.data();
+  for (int i = 0; i < w->awake_count; i++) {
+    const int s = w->awake_list[i];
+    if (w->type[s] != DYNAMIC) continue;
+    const int o = 6 * w->leaf[s];
+    Core body;
+    place_body(w, s, body);
+    double ext[3];
+    extents(body, ext);
+    const double* c = body.center;
+    if (c[0] - ext[0] < fat[o] || c[1] - ext[1] < fat[o + 1] || c[2] - ext[2] < fat[o + 2] || c[0] + ext[0] > fat[o + 3] ||
+        c[1] + ext[1] > fat[o + 4] || c[2] + ext[2] > fat[o + 5])
+      w->strays.push_back(s);
+  }
+  w->strays_at = w->step_count;
+  return w->strays;
+}
+
+double core_size(const Core& s) {
+  if (s.kind == CORE_SPHERE || s.kind == CORE_CAPSULE) return 2 * s.half[0];
+  return 2 * js_min(js_min(s.half[0], s.half[1]), s.half[2]);
+}
+
+// What advance() found: where along the sweep, the normal (body towards cast) and the point.
+struct Hit {
+  double normal[3], point[3];
+};
+
+// src/query.js:361-406 (advance): conservative advancement of `cast` from o along d against
+// b, up to max_t; the distance, or -1. A core already meeting b asks the device narrowphase
+// which way out, as the reference asks collide(). rc is set when that call fails.
+double advance(bm_world* w, Core& cast, const Core& b, double scale, const double* o, const double* d, double max_t,
+               bool sliding, Hit& hit, int& rc) {
+  const double ra = core_radius(cast), rb = core_radius(b), tolerance = scale * PIXEL;
+  const double meeting = scale / 67108864;  // 2^-26
+  double t = 0;
+  for (int i = 0; i < 64; i++) {
+    cast.center[0] = o[0] + t * d[0]; cast.center[1] = o[1] + t * d[1]; cast.center[2] = o[2] + t * d[2];
+    double na[3], nb[3];
+    const double dist = closest(cast, b, na, nb), gap = dist - ra - rb;
+    if (gap < tolerance) {
+      if (sliding && i == 0 && dist > meeting) {
+        const double opening = ((na[0] - nb[0]) * d[0] + (na[1] - nb[1]) * d[1] + (na[2] - nb[2]) * d[2]) / dist;
+        if (opening >= 0 || gap + opening * max_t >= js_min(gap, 0) - meeting) return -1;
+      }
+      if (dist > meeting) {
+        const double nx = (na[0] - nb[0]) / dist, ny = (na[1] - nb[1]) / dist, nz = (na[2] - nb[2]) / dist;
+        hit.normal[0] = nx; hit.normal[1] = ny; hit.normal[2] = nz;
+        hit.point[0] = nb[0] + nx * rb; hit.point[1] = nb[1] + ny * rb; hit.point[2] = nb[2] + nz * rb;
+        return t;
+      }
+      double ra_rec[19], rb_rec[19];
+      const Core* both[2] = {&cast, &b};
+      double* recs[2] = {ra_rec, rb_rec};
+      for (int k = 0; k < 2; k++) {
+        const Core& c = *both[k];
+        double* r = recs[k];
+        r[0] = c.kind;
+        for (int j = 0; j < 3; j++) r[1 + j] = c.center[j];
+        for (int j = 0; j < 9; j++) r[4 + j] = c.axes[j];
+        for (int j = 0; j < 3; j++) r[13 + j] = c.half[j];
+        for (int j = 0; j < 3; j++) r[16 + j] = c.weight[j];
+      }
+      const double zero = 0;
+      int32_t count = 0;
+      double normal[3], points[12], seps[4];
+      uint32_t ids[4];
+      rc = bm_collide_host(w->ctx, 1, ra_rec, rb_rec, &zero, &tolerance, &count, normal, points, seps, ids);
+      if (rc != BM_OK) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx)), -1;
+      if (count > 0) {
+        for (int k = 0; k < 3; k++) { hit.normal[k] = -normal[k]; hit.point[k] = points[k]; }
+      } else {
+        for (int k = 0; k < 3; k++) { hit.normal[k] = -d[k]; hit.point[k] = cast.center[k]; }
+      }
+      return t;
+    }
+    const double closing = -((na[0] - nb[0]) * d[0] + (na[1] - nb[1]) * d[1] + (na[2] - nb[2]) * d[2]) / dist;
+    if (closing <= 0) return -1;
+    t += gap / closing;
+    if (t >= max_t) return -1;
+  }
+  return -1;
+}
+
+// src/query.js:607-642 (sweep, sweepVisit) for a world of plain bodies: the first slot
+// the cast meets sliding from its centre along unit d for up to max_t, or -1; out gets
+// the distance, the normal and the point. Slots `skip` marks are passed over.
+int sweep(bm_world* w, Core& cast, int self, const double* d, double max_t, double* out, const uint8_t* skip,
+          int skipped, int& rc) {
+  const double o[3] = {cast.center[0], cast.center[1], cast.center[2]};
+  double g[3];
+  extents(cast, g);
+  int best = -1;
+  Hit best_hit{};
+  auto visit = [&](int s) -> double {
+    if (rc != BM_OK || skip[s] != 0 || s == self) return max_t;
+    Core body;
+    place_body(w, s, body);
+    const double own = w->size[s], scale = own > 0 ? js_min(own, core_size(cast)) : core_size(cast);
+    Hit hit;
+    const double t = advance(w, cast, body, scale, o, d, max_t, true, hit, rc);
+    if (t < 0) return max_t;
+    best = s;
+    max_t = t;
+    best_hit = hit;
+    return max_t;
+  };
+  for (int k = 0; k < 2; k++) {
+    if (k == MOVING_TREE && w->trees[k].leaves == skipped) continue;
+    max_t = w->trees[k].raycast(o, d, max_t, g, visit);
+  }
+  for (int s : strays(w)) visit(s);
+  cast.center[0] = o[0]; cast.center[1] = o[1]; cast.center[2] = o[2];
+  if (best >= 0) {
+    out[0] = max_t;
+    for (int k = 0; k < 3; k++) { out[1 + k] = best_hit.normal[k]; out[4 + k] = best_hit.point[k]; }
+  }
+  return best;
+}
+
+// src/contact.js:1755-1776 (gapOf): manifold m's least separation where its bodies are now.
+double gap_of(const bm_world* w, int m) {
+  const Contacts& c = w->c;
+  const double* p = w->position.data();
+  const double* start = w->start.data();
+  const int a = c.a[m], b = c.b[m], a3 = 3 * a, b3 = 3 * b, m3 = 3 * m;
+  const double nx = c.normal[m3], ny = c.normal[m3 + 1], nz = c.normal[m3 + 2];
+  const double dx = (p[b3] - start[b3]) - (p[a3] - start[a3]);
+  const double dy = (p[b3 + 1] - start[b3 + 1]) - (p[a3 + 1] - start[a3 + 1]);
+  const double dz = (p[b3 + 2] - start[b3 + 2]) - (p[a3 + 2] - start[a3 + 2]);
+  const double dn = dx * nx + dy * ny + dz * nz;
+  double na[3], nb[3];
+  rotate(&w->rotation[4 * a], nx, ny, nz, true, na);
+  rotate(&w->rotation[4 * b], nx, ny, nz, true, nb);
+  double least_gap = INFINITY;
+  for (int k = 0; k < c.points[m]; k++) {
+    const int i = m * P + k, i3 = 3 * i;
+    const double s = dn + (c.lb[i3] * nb[0] + c.lb[i3 + 1] * nb[1] + c.lb[i3 + 2] * nb[2])
+        - (c.la[i3] * na[0] + c.la[i3 + 1] * na[1] + c.la[i3 + 2] * na[2]) + c.adjusted[i];
+    if (s < least_gap) least_gap = s;
+  }
+  return least_gap;
+}
+
+// src/world.js:1683-1743 (_catch), for a world with no joints.
+int catch_fast(bm_world* w) {
+  const int n = w->awake_count;
+  const int* list = w->awake_list.data();
+  double* p = w->position.data();
+  double* q = w->rotation.data();
+  const double* st = w->start.data();
+  const double* sq = w->start_q.data();
+  int count = 0;
+  for (int i = 0; i < n; i++) {
+    const int s = list[i], s3 = 3 * s;
+    if (w->type[s] != DYNAMIC) continue;
+    const double dx = p[s3] - st[s3], dy = p[s3 + 1] - st[s3 + 1], dz = p[s3 + 2] - st[s3 + 2], half = 0.5 * w->size[s];
+    if (dx * dx + dy * dy + dz * dz > half * half) { w->fast[s] = 1; count++; }
+  }
+  if (count == 0) return BM_OK;
+  w->catch_steps++;
+  int caught = 0, rc = BM_OK;
+  for (int i = 0; i < n; i++) {
+    const int s = list[i], s3 = 3 * s, s4 = 4 * s;
+    if (w->fast[s] != 1) continue;
+    const double dx = p[s3] - st[s3], dy = p[s3 + 1] - st[s3 + 1], dz = p[s3 + 2] - st[s3 + 2];
+    const double length = std::sqrt(dx * dx + dy * dy + dz * dz);
+    Core core{};
+    core.kind = CORE_SPHERE;
+    core.half[0] = 0.125 * w->size[s];
+    const double identity[4] = {0, 0, 0, 1};
+    place_at(core, st[s3], st[s3 + 1], st[s3 + 2], identity);
+    const double dir[3] = {dx / length, dy / length, dz / length};
+    double swept[7];
+    const int hit = sweep(w, core, s, dir, length, swept, w->fast.data(), count, rc);
+    if (rc != BM_OK) return rc;
+    if (hit < 0) continue;
+    const double f = swept[0] / length;
+    p[s3] = st[s3] + dx * f; p[s3 + 1] = st[s3 + 1] + dy * f; p[s3 + 2] = st[s3 + 2] + dz * f;
+    const double sign = sq[s4] * q[s4] + sq[s4 + 1] * q[s4 + 1] + sq[s4 + 2] * q[s4 + 2] + sq[s4 + 3] * q[s4 + 3] < 0 ? -1 : 1;
+    double turned[4], l = 0;
+    for (int k = 0; k < 4; k++) {
+      const double v = (1 - f) * sq[s4 + k] + f * sign * q[s4 + k];
+      turned[k] = v;
+      l += v * v;
+    }
+    l = 1 / std::sqrt(l);
+    for (int k = 0; k < 4; k++) q[s4 + k] = turned[k] * l;
+    const double nx = swept[1], ny = swept[2], nz = swept[3];
+    double* v = w->velocity.data();
+    const double into = v[s3] * nx + v[s3 + 1] * ny + v[s3 + 2] * nz;
+    if (into < 0) {
+      const double k = (1 + js_max(w->restitution[s], w->restitution[hit])) * into;
+      v[s3] -= k * nx; v[s3 + 1] -= k * ny; v[s3 + 2] -= k * nz;
+    }
+    w->caught[s] = 1;
+    w->catches++;
+    caught++;
+  }
+  for (int i = 0; i < n; i++) w->fast[list[i]] = 0;
+  if (caught == 0) return BM_OK;
+  for (int m = 0; m < w->c.count; m++)
+    if (w->caught[w->c.a[m]] == 1 || w->caught[w->c.b[m]] == 1) w->c.gap[m] = gap_of(w, m);
+  for (int i = 0; i < n; i++) w->caught[list[i]] = 0;
+  return BM_OK;
+}
+
 // src/contact.js:2178-2213 (touched): which pairs touched, the flag only (no events kept).
 void touched(bm_world* w) {
   Contacts& c = w->c;
@@ -1053,7 +1486,8 @@ int bm_world_add(bm_world* w, const bm_bodies* b) {
   for (auto* v : {&w->mass, &w->inv_mass, &w->motion, &w->ceiling, &w->gravity_scale, &w->damping, &w->angular_damping,
                   &w->size, &w->reach, &w->friction, &w->restitution, &w->still, &w->least})
     grow(*v, 1);
-  for (auto* v : {&w->type, &w->shape, &w->frozen, &w->calm, &w->lock, &w->framed, &w->awake, &w->fast, &w->is_moved})
+  for (auto* v : {&w->type, &w->shape, &w->frozen, &w->calm, &w->lock, &w->framed, &w->awake, &w->fast, &w->caught,
+                  &w->is_moved})
     grow(*v, 1);
   for (auto* v : {&w->joint_head, &w->in_awake, &w->awake_list, &w->island, &w->island_next, &w->parent, &w->slept,
                   &w->leaf})
@@ -1112,15 +1546,8 @@ int bm_world_step(bm_world* w, double dt) {
   if (w->sleeping && (w->frozen_count > 0 || w->still_ready)) freeze(w);
   if (touching > 0) w->solver_steps++;
   if (int rc = device_phase(w, dt)) return rc;
-  // _catch (src/world.js:1683-1743): a body whose centre moved more than half its size.
-  for (int i = 0; i < w->awake_count; i++) {
-    const int s = w->awake_list[i], s3 = 3 * s;
-    if (w->type[s] != DYNAMIC) continue;
-    const double dx = w->position[s3] - w->start[s3], dy = w->position[s3 + 1] - w->start[s3 + 1],
-                 dz = w->position[s3 + 2] - w->start[s3 + 2], half = 0.5 * w->size[s];
-    if (dx * dx + dy * dy + dz * dz > half * half)
-      return fail(w, BM_ERR_UNSUPPORTED, "step: a fast body would be swept (_catch), which this world does not do yet");
-  }
+  w->strays_at = -1;
+  if (int rc = catch_fast(w)) return rc;
   if (touching > 0) touched(w);
   for (int i = 0; i < w->awake_count; i++) {
     const int s = w->awake_list[i];
@@ -1142,6 +1569,7 @@ int bm_world_read(const bm_world* w, bm_world_state* out) {
   out->pairs = w->pairs.count;
   out->frozen = w->frozen_count;
   out->impacts = w->impacts;
+  out->catches = w->catches;
   if (out->transforms) std::memcpy(out->transforms, w->transforms.data(), 7 * sizeof(float) * w->top);
   if (out->position) std::memcpy(out->position, w->position.data(), 3 * sizeof(double) * w->top);
   if (out->rotation) std::memcpy(out->rotation, w->rotation.data(), 4 * sizeof(double) * w->top);

@@ -102,6 +102,7 @@ class Bodies(ctypes.Structure):
 class WorldState(ctypes.Structure):
     _fields_ = [("top", ctypes.c_int32), ("awake", ctypes.c_int32), ("contacts", ctypes.c_int32),
                 ("pairs", ctypes.c_int32), ("frozen", ctypes.c_int32), ("impacts", ctypes.c_int64),
+                ("catches", ctypes.c_int64),
                 ("transforms", ctypes.POINTER(ctypes.c_float)), ("position", _D), ("rotation", _D), ("velocity", _D),
                 ("angular", _D)]
 
@@ -312,7 +313,7 @@ class World:
         for name in ("position", "rotation", "velocity", "angular"):
             setattr(s, name, out[name].ctypes.data_as(_D))
         self.check(self.lib.bm_world_read(self.ptr, ctypes.byref(s)), "bm_world_read")
-        for name in ("top", "awake", "contacts", "pairs", "frozen", "impacts"):
+        for name in ("top", "awake", "contacts", "pairs", "frozen", "impacts", "catches"):
             out[name] = getattr(s, name)
         return out
 

@@ -15,6 +15,9 @@ Scenes:
   tower        a box tower, each level turned about y, on a 10x1x10 ground
   pile_world   one world of the batched workload: a tower or a sphere pile
   throw        a thrown box landing and hitting a 6x6 wall of unit boxes
+  darts        small balls and rods fired at 20-200 m/s at a 2 cm static plate and the
+               floor: bodies the step would put through them, which the continuous catch
+               puts back (src/world.js:1636-1681)
 """
 from __future__ import annotations
 
@@ -168,6 +171,33 @@ def throw(rng: SplitMix64, dt_wall: float = 4.0):
     return bodies
 
 
+def darts(rng: SplitMix64, count: int = 64):
+    """`count` small bodies, balls of radius U[0.05, 0.15] and rods of 0.06 x 0.06 x
+    U[0.2, 0.6], fired from x in [-2, 0] at U[20, 200] m/s towards a static plate 2 cm thick,
+    4 m tall and 6 m wide at x = 3, every fourth one down at the floor instead; rods turned
+    any way and spinning N(0, 60). A static 20x1x20 floor. Restitution U[0, 0.5]."""
+    bodies = [_ground((20.0, 1.0, 20.0)),
+              {"type": "static", "shape": "box", "size": [0.02, 4.0, 6.0], "position": [3.0, 2.0, 0.0]}]
+    for k in range(count):
+        speed = rng.between(20.0, 200.0)
+        pos = [rng.between(-2.0, 0.0), rng.between(0.8, 3.2), rng.between(-2.0, 2.0)]
+        if k % 4 == 3:
+            vel = [rng.gauss(1.0), -speed, rng.gauss(1.0)]
+        else:
+            vel = [speed, rng.gauss(1.0), rng.gauss(1.0)]
+        e = rng.between(0.0, 0.5)
+        if k % 2 == 0:
+            bodies.append({"shape": "sphere", "radius": rng.between(0.05, 0.15), "position": pos, "velocity": vel,
+                           "restitution": e})
+        else:
+            q = [rng.gauss(1.0) for _ in range(4)]
+            n = math.sqrt(sum(x * x for x in q))
+            bodies.append({"shape": "box", "size": [0.06, 0.06, rng.between(0.2, 0.6)], "position": pos,
+                           "rotation": [x / n for x in q], "velocity": vel,
+                           "angularVelocity": [rng.gauss(60.0) for _ in range(3)], "restitution": e})
+    return bodies
+
+
 SCENES = {
     "pyramid": lambda: pyramid(),
     "rain": lambda: rain(),
@@ -175,6 +205,7 @@ SCENES = {
     "tower": lambda: tower(12, SplitMix64(40)),
     "pile_small": lambda: sphere_pile(60, SplitMix64(41)),
     "throw": lambda: throw(SplitMix64(5)),
+    "darts": lambda: darts(SplitMix64(7)),
 }
 
 

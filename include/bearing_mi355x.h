@@ -202,9 +202,8 @@ double bm_last_kernel_ms(const bm_context* ctx);
  * state (the broadphase trees, the pair list, the narrow driver's order, the
  * sleeping islands); the device runs each step's narrowphase batches, colouring
  * and soft step. Transforms match the reference's bit for bit. A step that would
- * need what this path does not compute yet (a time of impact, src/impact.js, or
- * the continuous catch, src/world.js:1683) fails with BM_ERR_UNSUPPORTED and a
- * message, rather than going on differently. */
+ * need what this path does not handle fails with BM_ERR_UNSUPPORTED and a message,
+ * rather than going on differently. */
 typedef struct bm_world bm_world;
 
 /* Bodies as addBody resolves them (src/world.js:563-578, _write 2152-2273), one
@@ -222,7 +221,7 @@ typedef struct bm_bodies {
  * slot (position, rotation), as the reference's Float32Array world.transforms. */
 typedef struct bm_world_state {
   int32_t top, awake, contacts, pairs, frozen;
-  int64_t impacts;
+  int64_t impacts, catches; /* world._impacts, world._catches */
   float* transforms;
   double *position, *rotation, *velocity, *angular;
 } bm_world_state;

@@ -27,6 +27,7 @@ JOBS = {
     "tower": dict(scene="tower", dt=1 / 60, steps=120, soft=[1, 10, 40, 64], colour=[10, 63], trace=5, every=30),
     "throw": dict(scene="throw", dt=1 / 120, steps=120, soft=[30, 60, 90, 119], colour=[60], trace=70, every=30),
     "pile_small": dict(scene="pile_small", dt=1 / 60, steps=150, soft=[30, 70, 110], colour=[70], trace=100, every=50),
+    "darts": dict(scene="darts", dt=1 / 60, steps=90, soft=[2, 20], colour=[], trace=-1, every=30),
     # The 2,000-sphere rain at full size: its solve phase mid-landing is the bench's workload.
     "rain": dict(scene="rain", dt=1 / 60, steps=900, soft=[300], colour=[], trace=-1, every=300),
 }

@@ -15,22 +15,26 @@ def traj(name):
     return fixtures.load(os.path.join(fixtures.DATA, f"{name}.traj.bmfx.gz"))
 
 
-@pytest.mark.parametrize("name", ["pyramid", "tower", "throw"])
+@pytest.mark.parametrize("name", ["pyramid", "tower", "throw", "rain_small", "pile_small", "darts"])
 def test_gpu_world_matches_reference_trajectory(name):
     print("trajectory", name, flush=True)
     fx = traj(name)
     w = native.World(0)
     w.add(init_columns(fx))
     matched, why = run(w, fx)
+    state = w.read()
     w.close()
     assert why is None, why
     assert matched == fx.meta["steps"]
+    assert (state["impacts"], state["catches"]) == (fx.meta["impacts"], fx.meta["catches"])
 
 
-def test_gpu_world_stops_where_it_would_sweep_a_fast_body():
-    fx = traj("rain_small")
+def test_gpu_world_refuses_what_it_does_not_handle():
+    # A capsule (shape 2) is not a shape this world takes: addBody fails loudly.
+    fx = traj("tower")
+    cols = init_columns(fx)
+    cols["shape"] = cols["shape"].copy()
+    cols["shape"][3] = 2
     w = native.World(0)
-    w.add(init_columns(fx))
-    with pytest.raises(native.BearingError, match="swept"):
-        run(w, fx)
-    w.close()
+    with pytest.raises(native.BearingError, match="boxes and spheres"):
+        w.add(cols)
