@@ -96,9 +96,21 @@ struct bm_context {
   hipEvent_t begin = nullptr, end = nullptr;
   std::string error;
   int32_t n_worlds = 0, bodies = 0, awake = 0, manifolds = 0, pairs = 0;
-  void* buffer[NCOL] = {};
-  size_t capacity[NCOL] = {};
+  // Every column lives in one device arena at a 256-byte aligned offset, the columns
+  // arrange() moves first, so a batch goes up and comes back in one copy each way
+  // through a pinned staging buffer.
+  void* buffer[NCOL] = {};   // arena + offset[c]
+  size_t offset[NCOL] = {};
   size_t bytes[NCOL] = {};
+  void* arena = nullptr;
+  size_t arena_cap = 0, total = 0, arranged_end = 0;
+  void* spare_arena = nullptr;  // arrange()'s output, copied back over the arranged block
+  size_t spare_cap = 0;
+  void* stage = nullptr;        // pinned host staging for uploads and downloads
+  size_t stage_cap = 0;
+  void* pstage = nullptr;       // pinned host staging for narrowphase batches
+  size_t pstage_cap = 0;
+  std::vector<int32_t> moff_host;
   SoftDev dev{};
   bool uploaded = false;
   float last_ms = 0;
@@ -107,8 +119,6 @@ struct bm_context {
   int32_t shape = 0;        // launch shape of the soft-step kernel
   int32_t resident = 1;     // keep velocities in LDS when they fit
   bool laid_out = false;    // rows are laid out by colour (checked at upload, or by bm_colour)
-  void* spare[NCOL] = {};   // arrange's second buffers
-  size_t spare_cap[NCOL] = {};
   Scratch deg, adj, keys, key_offset, world_of;
   Scratch pairs_in, pairs_out;
   std::vector<int32_t> key_offsets;
@@ -206,6 +216,49 @@ bool laid_out(const bm_columns* h) {
   return true;
 }
 
+// Device memory of at least `bytes` at *p (capacity *cap), contents not kept.
+int grow_device(bm_context* ctx, void** p, size_t* cap, size_t bytes, const char* what) {
+  if (bytes <= *cap && *p) return BM_OK;
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  size_t c = 65536;
+  while (c < bytes) c *= 2;
+  if (hipMalloc(p, c) != hipSuccess) return fail(ctx, BM_ERR_DEVICE, std::string(what) + ": hipMalloc");
+  *cap = c;
+  return BM_OK;
+}
+
+// Pinned host memory of at least `bytes` at *p (capacity *cap), contents not kept.
+int grow_pinned(bm_context* ctx, void** p, size_t* cap, size_t bytes, const char* what) {
+  if (bytes <= *cap && *p) return BM_OK;
+  if (*p) (void)hipHostFree(*p);
+  *p = nullptr;
+  size_t c = 65536;
+  while (c < bytes) c *= 2;
+  if (hipHostMalloc(p, c, hipHostMallocDefault) != hipSuccess)
+    return fail(ctx, BM_ERR_DEVICE, std::string(what) + ": hipHostMalloc");
+  *cap = c;
+  return BM_OK;
+}
+
+// The arena's column order: the arranged columns first, then the rest.
+struct ArenaOrder {
+  int order[NCOL];
+  int arranged;
+  ArenaOrder() {
+    int k = 0;
+    bool taken[NCOL] = {};
+    for (const char* name : ARRANGED) {
+      const int c = column_index(name);
+      order[k++] = c;
+      taken[c] = true;
+    }
+    arranged = k;
+    for (int c = 0; c < NCOL; c++)
+      if (!taken[c]) order[k++] = c;
+  }
+};
+
 int grow(bm_context* ctx, Scratch& s, size_t bytes, const char* what) {
   if (bytes <= s.cap) return BM_OK;
   if (s.p) (void)hipFree(s.p);
@@ -240,10 +293,10 @@ int bm_create(bm_context** out, int32_t device) {
 int bm_destroy(bm_context* ctx) {
   if (!ctx) return BM_OK;
   (void)hipSetDevice(ctx->device);
-  for (int c = 0; c < NCOL; c++) {
-    if (ctx->buffer[c]) (void)hipFree(ctx->buffer[c]);
-    if (ctx->spare[c]) (void)hipFree(ctx->spare[c]);
-  }
+  if (ctx->arena) (void)hipFree(ctx->arena);
+  if (ctx->spare_arena) (void)hipFree(ctx->spare_arena);
+  if (ctx->stage) (void)hipHostFree(ctx->stage);
+  if (ctx->pstage) (void)hipHostFree(ctx->pstage);
   for (Scratch* s : {&ctx->deg, &ctx->adj, &ctx->keys, &ctx->key_offset, &ctx->world_of, &ctx->pairs_in,
                      &ctx->pairs_out})
     if (s->p) (void)hipFree(s->p);
@@ -286,26 +339,33 @@ int bm_upload(bm_context* ctx, const bm_columns* h) {
     if (nb > ctx->most_bodies) ctx->most_bodies = nb;
   }
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->device), "bm_upload: hipSetDevice");
-  for (int c = 0; c < NCOL; c++) {
+  static const ArenaOrder ORDER;
+  size_t at = 0;
+  for (int k = 0; k < NCOL; k++) {
+    const int c = ORDER.order[k];
     const Column& col = COLUMNS[c];
     const size_t bytes = units(ctx, col.unit) * col.per * col.elem;
+    if (bytes > 0 && !field<const void*>(h, col.host))
+      return fail(ctx, BM_ERR_ARGUMENT, std::string("bm_upload: ") + col.name + " is null");
     ctx->bytes[c] = bytes;
-    const void* src = field<const void*>(h, col.host);
-    if (bytes > 0 && !src) return fail(ctx, BM_ERR_ARGUMENT, std::string("bm_upload: ") + col.name + " is null");
-    if (bytes > ctx->capacity[c]) {
-      if (ctx->buffer[c]) HIP_OR_FAIL(ctx, hipFree(ctx->buffer[c]), "bm_upload: hipFree");
-      ctx->buffer[c] = nullptr;
-      size_t cap = 256;
-      while (cap < bytes) cap *= 2;
-      HIP_OR_FAIL(ctx, hipMalloc(&ctx->buffer[c], cap), "bm_upload: hipMalloc");
-      ctx->capacity[c] = cap;
-    }
-    if (bytes > 0)
-      HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->buffer[c], src, bytes, hipMemcpyHostToDevice, ctx->stream),
-                  "bm_upload: hipMemcpyAsync");
-    void* p = ctx->buffer[c];
+    ctx->offset[c] = at;
+    at += (bytes + 255) & ~(size_t)255;
+    if (k + 1 == ORDER.arranged) ctx->arranged_end = at;
+  }
+  ctx->total = at;
+  if (int rc = grow_device(ctx, &ctx->arena, &ctx->arena_cap, at, "bm_upload")) return rc;
+  if (int rc = grow_pinned(ctx, &ctx->stage, &ctx->stage_cap, at, "bm_upload")) return rc;
+  char* stage = static_cast<char*>(ctx->stage);
+  for (int c = 0; c < NCOL; c++) {
+    const Column& col = COLUMNS[c];
+    if (ctx->bytes[c] > 0) std::memcpy(stage + ctx->offset[c], field<const void*>(h, col.host), ctx->bytes[c]);
+    void* p = static_cast<char*>(ctx->arena) + ctx->offset[c];
+    ctx->buffer[c] = p;
     std::memcpy(reinterpret_cast<char*>(&ctx->dev) + col.dev, &p, sizeof(void*));
   }
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->arena, ctx->stage, ctx->total, hipMemcpyHostToDevice, ctx->stream),
+              "bm_upload: hipMemcpyAsync");
+  ctx->moff_host.assign(h->manifold_offset, h->manifold_offset + h->n_worlds + 1);
   ctx->dev.n_worlds = ctx->n_worlds;
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_upload: hipStreamSynchronize");
   ctx->uploaded = true;
@@ -338,15 +398,17 @@ int bm_download(bm_context* ctx, bm_columns* h) {
   if (!ctx->uploaded) return fail(ctx, BM_ERR_ARGUMENT, "bm_download: no batch uploaded");
   ctx->error.clear();
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->device), "bm_download: hipSetDevice");
-  for (int c = 0; c < NCOL; c++) {
-    const Column& col = COLUMNS[c];
-    if (col.dir != IO || ctx->bytes[c] == 0) continue;
-    void* dst = field<void*>(h, col.host);
-    if (!dst) return fail(ctx, BM_ERR_ARGUMENT, std::string("bm_download: ") + col.name + " is null");
-    HIP_OR_FAIL(ctx, hipMemcpyAsync(dst, ctx->buffer[c], ctx->bytes[c], hipMemcpyDeviceToHost, ctx->stream),
-                "bm_download: hipMemcpyAsync");
-  }
+  for (int c = 0; c < NCOL; c++)
+    if (COLUMNS[c].dir == IO && ctx->bytes[c] > 0 && !field<void*>(h, COLUMNS[c].host))
+      return fail(ctx, BM_ERR_ARGUMENT, std::string("bm_download: ") + COLUMNS[c].name + " is null");
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->stage, ctx->arena, ctx->total, hipMemcpyDeviceToHost, ctx->stream),
+              "bm_download: hipMemcpyAsync");
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_download: hipStreamSynchronize");
+  const char* stage = static_cast<const char*>(ctx->stage);
+  for (int c = 0; c < NCOL; c++) {
+    if (COLUMNS[c].dir != IO || ctx->bytes[c] == 0) continue;
+    std::memcpy(field<void*>(h, COLUMNS[c].host), stage + ctx->offset[c], ctx->bytes[c]);
+  }
   return BM_OK;
 }
 
@@ -357,9 +419,7 @@ int bm_colour(bm_context* ctx) {
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->device), "bm_colour: hipSetDevice");
   const int n = ctx->n_worlds;
   // Each world's sort width: a power of two at least its manifold count.
-  std::vector<int32_t> moff(n + 1);
-  HIP_OR_FAIL(ctx, hipMemcpy(moff.data(), ctx->buffer[column_index("manifold_offset")], (n + 1) * sizeof(int32_t),
-                             hipMemcpyDeviceToHost), "bm_colour: hipMemcpy");
+  const std::vector<int32_t>& moff = ctx->moff_host;
   ctx->key_offsets.assign(n + 1, 0);
   for (int w = 0; w < n; w++) {
     int width = 1;
@@ -396,33 +456,24 @@ int bm_colour(bm_context* ctx) {
   bm::launch_world_of(cd.manifold_offset, n, static_cast<int*>(ctx->world_of.p), ctx->stream);
   bm::launch_colour(cd, ctx->stream);
   HIP_OR_FAIL(ctx, hipGetLastError(), "bm_colour: launch");
-  // Arrange: every column narrow() writes, into its spare buffer, then the two swap.
+  // Arrange: every column narrow() writes, into the spare arena at the same offsets, then
+  // the arranged block is copied back in one piece.
+  if (int rc = grow_device(ctx, &ctx->spare_arena, &ctx->spare_cap, ctx->arranged_end, "bm_colour")) return rc;
   bm::ArrangeDev ad{};
   ad.manifolds = (long)manifolds;
   ad.manifold_offset = cd.manifold_offset;
   ad.order = cd.order;
   ad.world_of = static_cast<const int*>(ctx->world_of.p);
-  int idx[NARRANGED];
   for (int k = 0; k < NARRANGED; k++) {
     const int c = column_index(ARRANGED[k]);
-    idx[k] = c;
-    if (ctx->spare_cap[c] < ctx->capacity[c]) {
-      if (ctx->spare[c]) HIP_OR_FAIL(ctx, hipFree(ctx->spare[c]), "bm_colour: hipFree");
-      ctx->spare[c] = nullptr;
-      HIP_OR_FAIL(ctx, hipMalloc(&ctx->spare[c], ctx->capacity[c]), "bm_colour: hipMalloc");
-      ctx->spare_cap[c] = ctx->capacity[c];
-    }
-    ad.columns[k] = bm::ArrangeColumn{ctx->buffer[c], ctx->spare[c], (int)COLUMNS[c].elem, COLUMNS[c].per};
+    ad.columns[k] = bm::ArrangeColumn{ctx->buffer[c], static_cast<char*>(ctx->spare_arena) + ctx->offset[c],
+                                      (int)COLUMNS[c].elem, COLUMNS[c].per};
   }
   bm::launch_arrange(ad, NARRANGED, ctx->stream);
   HIP_OR_FAIL(ctx, hipGetLastError(), "bm_colour: arrange");
-  for (int k = 0; k < NARRANGED; k++) {
-    const int c = idx[k];
-    std::swap(ctx->buffer[c], ctx->spare[c]);
-    std::swap(ctx->capacity[c], ctx->spare_cap[c]);
-    void* p = ctx->buffer[c];
-    std::memcpy(reinterpret_cast<char*>(&ctx->dev) + COLUMNS[c].dev, &p, sizeof(void*));
-  }
+  if (manifolds > 0)
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->arena, ctx->spare_arena, ctx->arranged_end, hipMemcpyDeviceToDevice,
+                                    ctx->stream), "bm_colour: hipMemcpyAsync");
   bm::RestDev rd{};
   rd.manifolds = (long)manifolds;
   rd.manifold_offset = cd.manifold_offset;
@@ -474,21 +525,28 @@ int bm_collide_host(bm_context* ctx, int32_t n, const double* a, const double* b
   d.separations = reinterpret_cast<double*>(out + 15 * 8 * (size_t)n);
   d.count = reinterpret_cast<int32_t*>(out + 19 * 8 * (size_t)n);
   d.ids = reinterpret_cast<uint32_t*>(out + 19 * 8 * (size_t)n + 4 * (size_t)n);
-  HIP_OR_FAIL(ctx, hipMemcpyAsync(const_cast<double*>(d.a), a, shapes, hipMemcpyHostToDevice, ctx->stream), "bm_collide_host: copy");
-  HIP_OR_FAIL(ctx, hipMemcpyAsync(const_cast<double*>(d.b), b, shapes, hipMemcpyHostToDevice, ctx->stream), "bm_collide_host: copy");
-  HIP_OR_FAIL(ctx, hipMemcpyAsync(const_cast<double*>(d.margin), margin, scalars, hipMemcpyHostToDevice, ctx->stream), "bm_collide_host: copy");
-  HIP_OR_FAIL(ctx, hipMemcpyAsync(const_cast<double*>(d.tolerance), tolerance, scalars, hipMemcpyHostToDevice, ctx->stream), "bm_collide_host: copy");
+  // One copy each way through pinned staging: inputs as laid out on the device, then the outputs.
+  if (int rc = grow_pinned(ctx, &ctx->pstage, &ctx->pstage_cap, in_bytes + out_bytes, "bm_collide_host")) return rc;
+  char* st = static_cast<char*>(ctx->pstage);
+  std::memcpy(st, a, shapes);
+  std::memcpy(st + shapes, b, shapes);
+  std::memcpy(st + 2 * shapes, margin, scalars);
+  std::memcpy(st + 2 * shapes + scalars, tolerance, scalars);
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(in, st, in_bytes, hipMemcpyHostToDevice, ctx->stream), "bm_collide_host: copy");
   (void)hipGetLastError();
   HIP_OR_FAIL(ctx, hipEventRecord(ctx->begin, ctx->stream), "bm_collide_host: hipEventRecord");
   bm::launch_collide(d, ctx->stream);
   HIP_OR_FAIL(ctx, hipGetLastError(), "bm_collide_host: launch");
   HIP_OR_FAIL(ctx, hipEventRecord(ctx->end, ctx->stream), "bm_collide_host: hipEventRecord");
-  HIP_OR_FAIL(ctx, hipMemcpyAsync(count, d.count, 4 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream), "bm_collide_host: copy");
-  HIP_OR_FAIL(ctx, hipMemcpyAsync(normal, d.normal, 24 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream), "bm_collide_host: copy");
-  HIP_OR_FAIL(ctx, hipMemcpyAsync(points, d.points, 96 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream), "bm_collide_host: copy");
-  HIP_OR_FAIL(ctx, hipMemcpyAsync(separations, d.separations, 32 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream), "bm_collide_host: copy");
-  HIP_OR_FAIL(ctx, hipMemcpyAsync(ids, d.ids, 16 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream), "bm_collide_host: copy");
+  const size_t produced = 19 * 8 * (size_t)n + 4 * (size_t)n + 16 * (size_t)n;
+  char* back = st + in_bytes;
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(back, out, produced, hipMemcpyDeviceToHost, ctx->stream), "bm_collide_host: copy");
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_collide_host: kernel");
+  std::memcpy(normal, back, 24 * (size_t)n);
+  std::memcpy(points, back + 3 * 8 * (size_t)n, 96 * (size_t)n);
+  std::memcpy(separations, back + 15 * 8 * (size_t)n, 32 * (size_t)n);
+  std::memcpy(count, back + 19 * 8 * (size_t)n, 4 * (size_t)n);
+  std::memcpy(ids, back + 19 * 8 * (size_t)n + 4 * (size_t)n, 16 * (size_t)n);
   HIP_OR_FAIL(ctx, hipEventElapsedTime(&ctx->last_ms, ctx->begin, ctx->end), "bm_collide_host: hipEventElapsedTime");
   return BM_OK;
 }
@@ -528,22 +586,28 @@ int bm_meet_host(bm_context* ctx, int32_t n, const double* a, const double* b, c
   d.count = reinterpret_cast<int32_t*>(out + 248 * N);
   d.flags = reinterpret_cast<int32_t*>(out + 252 * N);
   d.ids = reinterpret_cast<uint32_t*>(out + 256 * N);
-  HIP_OR_FAIL(ctx, hipMemcpyAsync(const_cast<double*>(d.a), a, bodies, hipMemcpyHostToDevice, ctx->stream), "bm_meet_host: copy");
-  HIP_OR_FAIL(ctx, hipMemcpyAsync(const_cast<double*>(d.b), b, bodies, hipMemcpyHostToDevice, ctx->stream), "bm_meet_host: copy");
-  HIP_OR_FAIL(ctx, hipMemcpyAsync(const_cast<double*>(d.pair), pair, pairs, hipMemcpyHostToDevice, ctx->stream), "bm_meet_host: copy");
+  // One copy each way through pinned staging.
+  if (int rc = grow_pinned(ctx, &ctx->pstage, &ctx->pstage_cap, in_bytes + out_bytes, "bm_meet_host")) return rc;
+  char* st = static_cast<char*>(ctx->pstage);
+  std::memcpy(st, a, bodies);
+  std::memcpy(st + bodies, b, bodies);
+  std::memcpy(st + 2 * bodies, pair, pairs);
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(in, st, in_bytes, hipMemcpyHostToDevice, ctx->stream), "bm_meet_host: copy");
   (void)hipGetLastError();
   HIP_OR_FAIL(ctx, hipEventRecord(ctx->begin, ctx->stream), "bm_meet_host: hipEventRecord");
   bm::launch_meet(d, ctx->stream);
   HIP_OR_FAIL(ctx, hipGetLastError(), "bm_meet_host: launch");
   HIP_OR_FAIL(ctx, hipEventRecord(ctx->end, ctx->stream), "bm_meet_host: hipEventRecord");
-  HIP_OR_FAIL(ctx, hipMemcpyAsync(count, d.count, 4 * N, hipMemcpyDeviceToHost, ctx->stream), "bm_meet_host: copy");
-  HIP_OR_FAIL(ctx, hipMemcpyAsync(flags, d.flags, 4 * N, hipMemcpyDeviceToHost, ctx->stream), "bm_meet_host: copy");
-  HIP_OR_FAIL(ctx, hipMemcpyAsync(normal, d.normal, 24 * N, hipMemcpyDeviceToHost, ctx->stream), "bm_meet_host: copy");
-  HIP_OR_FAIL(ctx, hipMemcpyAsync(points, d.points, 96 * N, hipMemcpyDeviceToHost, ctx->stream), "bm_meet_host: copy");
-  HIP_OR_FAIL(ctx, hipMemcpyAsync(separations, d.separations, 32 * N, hipMemcpyDeviceToHost, ctx->stream), "bm_meet_host: copy");
-  HIP_OR_FAIL(ctx, hipMemcpyAsync(ids, d.ids, 16 * N, hipMemcpyDeviceToHost, ctx->stream), "bm_meet_host: copy");
-  HIP_OR_FAIL(ctx, hipMemcpyAsync(shift, d.shift, 96 * N, hipMemcpyDeviceToHost, ctx->stream), "bm_meet_host: copy");
+  char* back = st + in_bytes;
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(back, out, 272 * N, hipMemcpyDeviceToHost, ctx->stream), "bm_meet_host: copy");
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_meet_host: kernel");
+  std::memcpy(normal, back, 24 * N);
+  std::memcpy(points, back + 24 * N, 96 * N);
+  std::memcpy(separations, back + 120 * N, 32 * N);
+  std::memcpy(shift, back + 152 * N, 96 * N);
+  std::memcpy(count, back + 248 * N, 4 * N);
+  std::memcpy(flags, back + 252 * N, 4 * N);
+  std::memcpy(ids, back + 256 * N, 16 * N);
   HIP_OR_FAIL(ctx, hipEventElapsedTime(&ctx->last_ms, ctx->begin, ctx->end), "bm_meet_host: hipEventElapsedTime");
   return BM_OK;
 }

@@ -86,6 +86,43 @@ def cpu_baseline(fx, seconds):
             "sample": f"{done} soft steps of {WORKLOAD} ({rows} rows) in {elapsed:.2f} s on one host thread"}
 
 
+WORLD_RUNS = ["pyramid", "rain_small", "darts"]
+
+
+def world_run(name, lib=None):
+    """A whole reference trajectory stepped through bm_world (broadphase, narrow driver,
+    device meet / colour / soft step, catch, sleep): wall time per step and whether every
+    step's transforms are the reference's bits (checked after the timed run)."""
+    import hashlib
+
+    import fixtures
+    from bearing_amd import native
+    from worldcheck import init_columns
+
+    fx = fixtures.load(os.path.join(fixtures.DATA, f"{name}.traj.bmfx.gz"))
+    meta = fx.meta
+    w = native.World(0, lib=lib)
+    w.add(init_columns(fx))
+    out = {"transforms": np.zeros(7 * int(meta["bodies"]), np.float32)}
+    from bearing_amd.native import WorldState
+    import ctypes
+    kept = []
+    state = WorldState()
+    state.transforms = out["transforms"].ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    t0 = time.perf_counter()
+    for _ in range(meta["steps"]):
+        w.step(meta["dt"])
+        w.lib.bm_world_read(w.ptr, ctypes.byref(state))
+        kept.append(out["transforms"].tobytes())
+    wall = time.perf_counter() - t0
+    w.close()
+    exact = all(hashlib.sha256(t).hexdigest() == h for t, h in zip(kept, meta["hashes"]))
+    awake = int(np.count_nonzero(fx["awake"]))
+    return {"steps": meta["steps"], "bodies": meta["bodies"], "awake_steps": awake, "ms_per_step": wall * 1000 / meta["steps"],
+            "ms_per_awake_step": wall * 1000 / max(awake, 1), "world_steps_per_s": meta["steps"] / wall,
+            "bit_exact": bool(exact)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,6 +134,8 @@ def main():
     ap.add_argument("--plain-launch", default=None,
                     help="shape,resident for the plain world (default: autotuned, bearing_amd/tune.py)")
     ap.add_argument("--batched-launch", default=None, help="shape,resident for the batch (default: autotuned)")
+    ap.add_argument("--world", default=None,
+                    help="only step this reference trajectory whole through bm_world and print its timing")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy this many MiB device to device 5 times (torch), to calibrate HBM byte counters")
     args = ap.parse_args()
@@ -113,6 +152,10 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(worlds shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    if args.world:
+        print(json.dumps({"world": args.world, **world_run(args.world)}))
+        return
 
     from bearing_amd import native
     from bearing_amd.tune import tune
@@ -177,6 +220,16 @@ def main():
     loop_rate = sample / t_loop
     ctx.close()
 
+    worlds_run = {name: world_run(name) for name in WORLD_RUNS}
+    # The same whole steps on the host with the oracle standing in for every device call
+    # (tests/hostworld): the port's one-thread time for the same trajectories.
+    import ctypes
+    host_lib = os.path.join(ROOT, "tests", "hostworld", "_build", "libhostworld.so")
+    if os.path.exists(host_lib):
+        lib = ctypes.CDLL(host_lib)
+        for name in WORLD_RUNS:
+            worlds_run[name]["cpu_port_ms_per_step"] = world_run(name, lib)["ms_per_step"]
+
     cpu = cpu_baseline(fx, args.cpu_seconds)
     # HBM bytes per launch of the plain kernel, from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs
     # recorded in profiles/ (scaled by a copy kernel of known size); null if not measured
@@ -202,6 +255,7 @@ def main():
         "batched": {"worlds": args.batch_worlds, "steps": args.batch_steps, "world_steps_per_s": batched_rate,
                     "looped_plain_world_steps_per_s": loop_rate, "speedup_vs_looped": batched_rate / loop_rate,
                     "worlds_from": BATCH_SOURCES, "launch": {"shape": bshape, "resident_lds": bool(bresident)}},
+        "world": worlds_run,
         "bwd_fwd": None,
     }
     print(json.dumps(line))
