@@ -86,7 +86,7 @@ def cpu_baseline(fx, seconds):
             "sample": f"{done} soft steps of {WORKLOAD} ({rows} rows) in {elapsed:.2f} s on one host thread"}
 
 
-WORLD_RUNS = ["pyramid", "rain_small", "darts"]
+WORLD_RUNS = ["pyramid", "rain", "darts"]
 
 
 def world_run(name, lib=None):
