@@ -179,6 +179,15 @@ struct Manifold {
   double loss = 0;
 };
 
+// A pair visited by narrow(): what find() decided for it, in the order visited.
+struct Visit {
+  int p, a, b;
+  double tolerance;
+  int kind;  // 0: the boxes apart (forget); 1: the kept manifold, in m; 2: met on the device, batch row `slot`
+  int slot;
+  Manifold m;
+};
+
 // This step's manifolds in the order narrow() found them (src/contact.js Contacts).
 struct Contacts {
   int count = 0;
@@ -244,6 +253,10 @@ struct bm_world {
   std::vector<double> batch_a, batch_b, batch_pair, batch_normal, batch_points, batch_seps, batch_shift;
   std::vector<int32_t> batch_count, batch_flags;
   std::vector<uint32_t> batch_ids;
+  std::vector<Visit> visits;
+  // Bodies woken while a narrow pass's results are taken (their bounds changed then).
+  std::vector<uint8_t> fresh;
+  std::vector<int32_t> woken;
 };
 
 namespace {
@@ -341,6 +354,10 @@ void wake(bm_world* w, int s) {
     w->island_next[b] = 0;
     awaken(w, b);
     if (dt > 0) bounds_of(w, b, dt, gl);
+    if (!w->fresh[b]) {
+      w->fresh[b] = 1;
+      w->woken.push_back(b);
+    }
     b = next;
   }
   w->wake_pending = true;
@@ -689,115 +706,124 @@ void load(const bm_world* w, int s, double* r) {
   r[31] = (dynamic ? 1 : 0) | (dynamic && (w->shape[s] != SPHERE || w->framed[s] == 1) ? 2 : 0);
 }
 
-// A pair visited by narrow(): what find() decided for it, in the order visited.
-struct Visit {
-  int p, a, b;
-  double tolerance;
-  int kind;  // 0: no manifold (already forgotten); 1: reused, in m; 2: met, from the device batch at `slot`
-  int slot;
-  Manifold m;
-};
-
-// Runs the pending device narrowphase batch and finishes every pending visit in order:
-// src/contact.js:355-363 (meet's result: forget or remember) and 318-322 (wake, emit, carry).
-int flush(bm_world* w, std::vector<Visit>& pending) {
+// Runs the narrowphase batch gathered so far on the device (bm_meet_host) and clears it.
+int run_batch(bm_world* w) {
   const int n = (int)w->batch_pair.size() / 6;
-  if (n > 0) {
-    w->batch_count.resize(n);
-    w->batch_flags.resize(n);
-    w->batch_normal.resize(3 * n);
-    w->batch_points.resize(12 * n);
-    w->batch_seps.resize(4 * n);
-    w->batch_ids.resize(4 * n);
-    w->batch_shift.resize(12 * n);
-    const int rc = bm_meet_host(w->ctx, n, w->batch_a.data(), w->batch_b.data(), w->batch_pair.data(),
-                                w->batch_count.data(), w->batch_normal.data(), w->batch_points.data(),
-                                w->batch_seps.data(), w->batch_ids.data(), w->batch_shift.data(),
-                                w->batch_flags.data());
-    if (rc != BM_OK) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx));
-  }
-  for (Visit& v : pending) {
-    if (v.kind == 0) continue;
-    const double* shift = nullptr;
-    if (v.kind == 2) {
-      // find() after meet() (src/contact.js:355-363).
-      const int k = v.slot;
-      Manifold& m = v.m;
-      m.count = w->batch_count[k];
-      if (m.count < 0) return fail(w, BM_ERR_UNSUPPORTED, "step: a pair of shapes the device narrowphase does not handle");
-      if (w->batch_flags[k] & 2) w->impacts++;
-      if (m.count == 0) {
-        w->pairs.forget(v.p);
-        continue;
-      }
-      m.loss = 0;
-      for (int i = 0; i < 3; i++) m.normal[i] = w->batch_normal[3 * k + i];
-      for (int i = 0; i < 12; i++) m.points[i] = w->batch_points[12 * k + i];
-      for (int i = 0; i < 4; i++) m.separations[i] = w->batch_seps[4 * k + i];
-      for (int i = 0; i < 4; i++) m.ids[i] = w->batch_ids[4 * k + i];
-      if (w->batch_flags[k] & 1) {
-        // A manifold met at the moment of impact is not kept (src/contact.js:541).
-        if (w->pairs.cache[v.p] != 0) w->pairs.uncache(v.p);
-        shift = &w->batch_shift[12 * k];
-      } else {
-        remember(w, v.p, v.a, v.b, m);
-      }
-    }
-    if (w->awake[v.a] != w->awake[v.b]) wake(w, w->awake[v.a] ? v.b : v.a);
-    const int mi = emit(w, v.p, v.a, v.b, v.m, v.tolerance, shift);
-    carry(w, v.p, mi);
-  }
-  pending.clear();
+  if (n == 0) return BM_OK;
+  w->batch_count.resize(n);
+  w->batch_flags.resize(n);
+  w->batch_normal.resize(3 * n);
+  w->batch_points.resize(12 * n);
+  w->batch_seps.resize(4 * n);
+  w->batch_ids.resize(4 * n);
+  w->batch_shift.resize(12 * n);
+  const int rc = bm_meet_host(w->ctx, n, w->batch_a.data(), w->batch_b.data(), w->batch_pair.data(),
+                              w->batch_count.data(), w->batch_normal.data(), w->batch_points.data(),
+                              w->batch_seps.data(), w->batch_ids.data(), w->batch_shift.data(), w->batch_flags.data());
   w->batch_a.clear(); w->batch_b.clear(); w->batch_pair.clear();
+  if (rc != BM_OK) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx));
   return BM_OK;
 }
 
-// A body that may be woken by its pair's manifold: dynamic and asleep.
-bool sleeper(const bm_world* w, int s) { return w->type[s] == DYNAMIC && w->island[s] != 0; }
+// find() up to the narrowphase (src/contact.js:334-354): the box test, the kept manifold
+// (REUSE), or a meet record appended to the device batch.
+void prepare_visit(bm_world* w, Visit& v) {
+  const int a = v.a, b = v.b;
+  v.kind = 0;
+  v.slot = -1;
+  const double* ba = &w->box[6 * a];
+  const double* bb = &w->box[6 * b];
+  if (ba[0] > bb[3] || bb[0] > ba[3] || ba[1] > bb[4] || bb[1] > ba[4] || ba[2] > bb[5] || bb[2] > ba[5]) return;
+  const double margin = w->motion[a] + w->motion[b] + v.tolerance;
+  if (reused(w, v.p, a, b, v.tolerance, margin, v.m)) {
+    v.kind = 1;
+    return;
+  }
+  v.kind = 2;
+  v.slot = (int)w->batch_pair.size() / 6;
+  double ra[32], rb[32];
+  load(w, a, ra);
+  load(w, b, rb);
+  w->batch_a.insert(w->batch_a.end(), ra, ra + 32);
+  w->batch_b.insert(w->batch_b.end(), rb, rb + 32);
+  const double pair[6] = {margin, v.tolerance, w->step_dt, w->gravity[0], w->gravity[1], w->gravity[2]};
+  w->batch_pair.insert(w->batch_pair.end(), pair, pair + 6);
+}
 
-// src/contact.js:296-327 (narrow), from pair `from` on.
+// The rest of find() and narrow()'s own steps for one visit, in the reference's order:
+// src/contact.js:339-363 (forget, or remember what meet() found) and 318-322 (wake,
+// emit, carry). Reads the device batch's results for a met pair.
+int finish_visit(bm_world* w, Visit& v) {
+  if (v.kind == 0) {
+    w->pairs.forget(v.p);
+    return BM_OK;
+  }
+  const double* shift = nullptr;
+  if (v.kind == 2) {
+    const int k = v.slot;
+    Manifold& m = v.m;
+    m.count = w->batch_count[k];
+    if (m.count < 0) return fail(w, BM_ERR_UNSUPPORTED, "step: a pair of shapes the device narrowphase does not handle");
+    if (w->batch_flags[k] & 2) w->impacts++;
+    if (m.count == 0) {
+      w->pairs.forget(v.p);
+      return BM_OK;
+    }
+    m.loss = 0;
+    for (int i = 0; i < 3; i++) m.normal[i] = w->batch_normal[3 * k + i];
+    for (int i = 0; i < 12; i++) m.points[i] = w->batch_points[12 * k + i];
+    for (int i = 0; i < 4; i++) m.separations[i] = w->batch_seps[4 * k + i];
+    for (int i = 0; i < 4; i++) m.ids[i] = w->batch_ids[4 * k + i];
+    if (w->batch_flags[k] & 1) {
+      // A manifold met at the moment of impact is not kept (src/contact.js:541).
+      if (w->pairs.cache[v.p] != 0) w->pairs.uncache(v.p);
+      shift = &w->batch_shift[12 * k];
+    } else {
+      remember(w, v.p, v.a, v.b, m);
+    }
+  }
+  if (w->awake[v.a] != w->awake[v.b]) wake(w, w->awake[v.a] ? v.b : v.a);
+  const int mi = emit(w, v.p, v.a, v.b, v.m, v.tolerance, shift);
+  carry(w, v.p, mi);
+  return BM_OK;
+}
+
+// src/contact.js:296-327 (narrow), from pair `from` on. Pairs that no longer overlap go as
+// the reference drops them; every other pair's narrowphase runs in one device batch; then
+// the results are taken in the pairs' order. A pair whose manifold wakes a sleeping island
+// changes the bounds of the bodies it wakes, so a later pair with a body woken that way
+// is looked at again from where things stand then, as the reference, which visits pairs
+// one by one, would see it.
 int narrow(bm_world* w, int from) {
   Pairs& pr = w->pairs;
-  std::vector<Visit> pending;
+  std::vector<Visit>& visits = w->visits;
+  visits.clear();
   for (int p = from; p < pr.active;) {
     const int a = pr.a[p], b = pr.b[p];
     if (!overlap(w, a, b)) {
       pr.remove(p);
       continue;
     }
-    const double tolerance = js_min(w->size[a], w->size[b]) * PIXEL;
-    // A visit whose manifold could wake an island needs every earlier one finished first.
-    if (sleeper(w, a) || sleeper(w, b)) {
-      if (int rc = flush(w, pending)) return rc;
-    }
-    Visit v{p, a, b, tolerance, 0, -1, Manifold{}};
-    const double* ba = &w->box[6 * a];
-    const double* bb = &w->box[6 * b];
-    if (ba[0] > bb[3] || bb[0] > ba[3] || ba[1] > bb[4] || bb[1] > ba[4] || ba[2] > bb[5] || bb[2] > ba[5]) {
-      pr.forget(p);
-    } else {
-      const double margin = w->motion[a] + w->motion[b] + tolerance;
-      if (reused(w, p, a, b, tolerance, margin, v.m)) {
-        v.kind = 1;
-      } else {
-        v.kind = 2;
-        v.slot = (int)w->batch_pair.size() / 6;
-        double ra[32], rb[32];
-        load(w, a, ra);
-        load(w, b, rb);
-        w->batch_a.insert(w->batch_a.end(), ra, ra + 32);
-        w->batch_b.insert(w->batch_b.end(), rb, rb + 32);
-        const double pair[6] = {margin, tolerance, w->step_dt, w->gravity[0], w->gravity[1], w->gravity[2]};
-        w->batch_pair.insert(w->batch_pair.end(), pair, pair + 6);
-      }
-    }
-    pending.push_back(v);
-    if (sleeper(w, a) || sleeper(w, b)) {
-      if (int rc = flush(w, pending)) return rc;
-    }
+    Visit v{p, a, b, js_min(w->size[a], w->size[b]) * PIXEL, 0, -1, Manifold{}};
+    prepare_visit(w, v);
+    visits.push_back(v);
     p++;
   }
-  return flush(w, pending);
+  if (int rc = run_batch(w)) return rc;
+  for (int s : w->woken) w->fresh[s] = 0;
+  w->woken.clear();
+  for (Visit& v : visits) {
+    if (w->fresh[v.a] || w->fresh[v.b]) {
+      prepare_visit(w, v);
+      if (v.kind == 2) {
+        if (int rc = run_batch(w)) return rc;
+      }
+    }
+    if (int rc = finish_visit(w, v)) return rc;
+  }
+  for (int s : w->woken) w->fresh[s] = 0;
+  w->woken.clear();
+  return BM_OK;
 }
 
 // src/world.js:1369-1382 (_syncPairs), with no bodies removed and no static body moved by hand.
@@ -1487,7 +1513,7 @@ int bm_world_add(bm_world* w, const bm_bodies* b) {
                   &w->size, &w->reach, &w->friction, &w->restitution, &w->still, &w->least})
     grow(*v, 1);
   for (auto* v : {&w->type, &w->shape, &w->frozen, &w->calm, &w->lock, &w->framed, &w->awake, &w->fast, &w->caught,
-                  &w->is_moved})
+                  &w->is_moved, &w->fresh})
     grow(*v, 1);
   for (auto* v : {&w->joint_head, &w->in_awake, &w->awake_list, &w->island, &w->island_next, &w->parent, &w->slept,
                   &w->leaf})
