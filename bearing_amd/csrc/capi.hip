@@ -110,7 +110,9 @@ struct bm_context {
   size_t stage_cap = 0;
   void* pstage = nullptr;       // pinned host staging for narrowphase batches
   size_t pstage_cap = 0;
-  std::vector<int32_t> moff_host;
+  std::vector<int32_t> moff_host, boff_host;
+  size_t colour_lds_limit = 0;  // dynamic LDS a colouring workgroup may take (0: not yet asked)
+  int32_t colour_in_lds = 1;    // colour worlds that fit in LDS there
   SoftDev dev{};
   bool uploaded = false;
   float last_ms = 0;
@@ -366,6 +368,7 @@ int bm_upload(bm_context* ctx, const bm_columns* h) {
   HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->arena, ctx->stage, ctx->total, hipMemcpyHostToDevice, ctx->stream),
               "bm_upload: hipMemcpyAsync");
   ctx->moff_host.assign(h->manifold_offset, h->manifold_offset + h->n_worlds + 1);
+  ctx->boff_host.assign(h->body_offset, h->body_offset + h->n_worlds + 1);
   ctx->dev.n_worlds = ctx->n_worlds;
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_upload: hipStreamSynchronize");
   ctx->uploaded = true;
@@ -452,9 +455,18 @@ int bm_colour(bm_context* ctx) {
   cd.adj = static_cast<int*>(ctx->adj.p);
   cd.keys = static_cast<uint64_t*>(ctx->keys.p);
   cd.key_offset = static_cast<const int*>(ctx->key_offset.p);
+  // Colour in LDS when every world's data fits there.
+  if (ctx->colour_lds_limit == 0) ctx->colour_lds_limit = bm::colour_lds_limit(ctx->device);
+  size_t lds = 0;
+  for (int w = 0; w < n; w++) {
+    const size_t need = bm::colour_lds_bytes(ctx->boff_host[w + 1] - ctx->boff_host[w], moff[w + 1] - moff[w],
+                                             ctx->key_offsets[w + 1] - ctx->key_offsets[w]);
+    if (need > lds) lds = need;
+  }
+  cd.in_lds = ctx->colour_in_lds && lds <= ctx->colour_lds_limit ? 1 : 0;
   (void)hipGetLastError();
   bm::launch_world_of(cd.manifold_offset, n, static_cast<int*>(ctx->world_of.p), ctx->stream);
-  bm::launch_colour(cd, ctx->stream);
+  bm::launch_colour(cd, lds, ctx->stream);
   HIP_OR_FAIL(ctx, hipGetLastError(), "bm_colour: launch");
   // Arrange: every column narrow() writes, into the spare arena at the same offsets, then
   // the arranged block is copied back in one piece.
@@ -617,6 +629,12 @@ int bm_set_launch(bm_context* ctx, int32_t shape, int32_t resident) {
   if (shape < 0 || shape >= bm::SOFT_SHAPES) return fail(ctx, BM_ERR_ARGUMENT, "bm_set_launch: shape must be 0, 1 or 2");
   ctx->shape = shape;
   ctx->resident = resident ? 1 : 0;
+  return BM_OK;
+}
+
+int bm_set_colour_lds(bm_context* ctx, int32_t allow) {
+  if (!ctx) return BM_ERR_ARGUMENT;
+  ctx->colour_in_lds = allow ? 1 : 0;
   return BM_OK;
 }
 

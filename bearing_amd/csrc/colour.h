@@ -18,7 +18,20 @@ struct ColourDev {
   int* adj;                        // scratch: 2 per manifold
   uint64_t* keys;                  // scratch: key_offset[n_worlds]
   const int* key_offset;           // per world, a power of two at least its manifolds
+  int in_lds;                      // 1: each world's colouring and sort run in dynamic LDS
 };
+
+// Dynamic LDS a world takes when coloured in LDS: first the held masks, the manifolds' ends
+// and the moving flags, which the sort keys then overwrite, then the colours.
+__host__ __device__ inline size_t colour_lds_front(int bodies, int manifolds, int width) {
+  const size_t colouring = 4 * (size_t)(bodies + (bodies & 1)) + 8 * (size_t)manifolds + (size_t)bodies;
+  const size_t sorting = 8 * (size_t)width;
+  return ((colouring > sorting ? colouring : sorting) + 15) & ~(size_t)15;
+}
+inline size_t colour_lds_bytes(int bodies, int manifolds, int width) {
+  return colour_lds_front(bodies, manifolds, width) + 4 * (size_t)manifolds;
+}
+size_t colour_lds_limit(int device);
 
 struct ArrangeColumn {
   const void* from;
@@ -43,7 +56,7 @@ struct RestDev {
   double *max_normal_impulse, *relative_velocity, *gap;
 };
 
-void launch_colour(const ColourDev& d, hipStream_t stream);
+void launch_colour(const ColourDev& d, size_t lds_bytes, hipStream_t stream);
 void launch_world_of(const int* manifold_offset, int n_worlds, int* world_of, hipStream_t stream);
 void launch_arrange(const ArrangeDev& d, int columns, hipStream_t stream);
 void launch_rest(const RestDev& d, hipStream_t stream);

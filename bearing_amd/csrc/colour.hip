@@ -10,6 +10,11 @@
 // atomics in HBM scratch) and decides if it can; rounds run until none is left, as
 // many as the longest chain of dependent manifolds, not as many as manifolds.
 //
+// A world whose data fits in LDS is coloured the reference's own way instead, one manifold
+// after another by one lane, with the held masks, the ends and the colours in LDS; every
+// step of that loop is an LDS access, so a long chain of dependent manifolds costs an LDS
+// round trip each rather than a pass of the whole workgroup through HBM.
+//
 // The rows are then laid out as the reference's stable counting sorts leave them: by
 // colour, then by b's slot, then by the order found. That key (colour, b, m) is unique,
 // so a bitonic sort of it in scratch gives exactly that order. Resting rows (between
@@ -39,7 +44,116 @@ __device__ __forceinline__ int lowest_free(int32_t held) {
 __global__ void __launch_bounds__(256) colour_kernel(ColourDev d) {
   __shared__ int flag;
   __shared__ int runs[RESTING + 2];
+  extern __shared__ __align__(16) unsigned char lds[];
   for (int wld = blockIdx.x; wld < d.n_worlds; wld += gridDim.x) {
+    if (d.in_lds) {
+      const int boff = d.body_offset[wld], moff = d.manifold_offset[wld];
+      const int nb = d.body_offset[wld + 1] - boff, n = d.manifold_offset[wld + 1] - moff;
+      const int width = d.key_offset[wld + 1] - d.key_offset[wld];
+      // The keys take the place of held, ends and moving once the colours are known.
+      uint64_t* keys = reinterpret_cast<uint64_t*>(lds);
+      int32_t* held = reinterpret_cast<int32_t*>(lds);
+      int2* ends = reinterpret_cast<int2*>(held + nb + (nb & 1));
+      uint8_t* moving = reinterpret_cast<uint8_t*>(ends + n);
+      int* col = reinterpret_cast<int*>(lds + colour_lds_front(nb, n, width));
+      const int* A = d.a + moff;
+      const int* B = d.b + moff;
+      const uint8_t* type = d.type + boff;
+      const uint8_t* frozen = d.frozen + boff;
+      __syncthreads();
+      for (int s = threadIdx.x; s < nb; s += blockDim.x) {
+        held[s] = 0;
+        moving[s] = type[s] == DYNAMIC && frozen[s] == 0;
+      }
+      for (int k = threadIdx.x; k < RESTING + 2; k += blockDim.x) runs[k] = 0;
+      __syncthreads();
+      for (int m = threadIdx.x; m < n; m += blockDim.x) {
+        const int a = A[m], b = B[m];
+        ends[m] = make_int2(a, b);
+        // Resting rows: between frozen or static bodies (src/graph.js:114-124).
+        const bool rest_a = frozen[a] != 0 || type[a] == STATIC, rest_b = frozen[b] != 0 || type[b] == STATIC;
+        col[m] = rest_a && rest_b ? RESTING : UNDECIDED;
+      }
+      __syncthreads();
+      // src/graph.js:125-140, in order, by the first wave: 64 manifolds at a time, each
+      // lane fetching one's ends and flags, then the wave walking them one by one with the
+      // ends read out of lane registers, so each step waits only on its two held masks.
+      if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        for (int base = 0; base < n; base += 64) {
+          const int m = base + lane;
+          int ea = 0, eb = 0, flags = 0;
+          if (m < n) {
+            const int2 e = ends[m];
+            ea = e.x;
+            eb = e.y;
+            flags = (moving[e.x] ? 1 : 0) | (moving[e.y] ? 2 : 0) | (col[m] == RESTING ? 4 : 0);
+          }
+          int mine = RESTING;
+          const int count = n - base < 64 ? n - base : 64;
+          for (int j = 0; j < count; j++) {
+            const int f = __builtin_amdgcn_readlane(flags, j);
+            if (f & 4) continue;
+            const int a = __builtin_amdgcn_readlane(ea, j), b = __builtin_amdgcn_readlane(eb, j);
+            const int32_t h = ((f & 1) ? held[a] : 0) | ((f & 2) ? held[b] : 0);
+            int c;
+            if (h == -1) {
+              c = OVERFLOW;
+            } else {
+              const int32_t bit = ~h & (h + 1);
+              c = 31 - __clz((uint32_t)bit);
+              if (lane == 0) {
+                if (f & 1) held[a] |= bit;
+                if (f & 2) held[b] |= bit;
+              }
+            }
+            if (lane == j) mine = c;
+          }
+          if (m < n) col[m] = mine;
+        }
+      }
+      __syncthreads();
+      int* colour = d.colour + moff;
+      for (int i = threadIdx.x; i < width; i += blockDim.x) {
+        if (i < n) {
+          const int c = col[i];
+          colour[i] = c;
+          keys[i] = ((uint64_t)c << 56) | ((uint64_t)(uint32_t)B[i] << 32) | (uint32_t)i;
+          atomicAdd(&runs[c + 1], 1);
+        } else {
+          keys[i] = ~0ull;
+        }
+      }
+      __syncthreads();
+      for (int size = 2; size <= width; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+          for (int i = threadIdx.x; i < width; i += blockDim.x) {
+            const int j = i ^ stride;
+            if (j > i) {
+              const uint64_t x = keys[i], y = keys[j];
+              const bool up = (i & size) == 0;
+              if ((x > y) == up) { keys[i] = y; keys[j] = x; }
+            }
+          }
+          __syncthreads();
+        }
+      }
+      int* order = d.order + moff;
+      int* row = d.row + moff;
+      for (int o = threadIdx.x; o < n; o += blockDim.x) {
+        const int m = (int)(uint32_t)(keys[o] & 0xffffffffu);
+        order[o] = m;
+        row[m] = o;
+      }
+      if (threadIdx.x == 0) {
+        int* starts = d.starts + BM_STARTS_DEV * wld;
+        starts[0] = 0;
+        for (int k = 0; k <= RESTING; k++) starts[k + 1] = starts[k] + runs[k + 1];
+        d.solved[wld] = starts[RESTING];
+      }
+      __syncthreads();
+      continue;
+    }
     const int boff = d.body_offset[wld], moff = d.manifold_offset[wld];
     const int nb = d.body_offset[wld + 1] - boff, n = d.manifold_offset[wld + 1] - moff;
     const int* A = d.a + moff;
@@ -189,8 +303,15 @@ __global__ void world_of_kernel(const int* manifold_offset, int n_worlds, int* w
   for (int m = manifold_offset[w] + threadIdx.x; m < manifold_offset[w + 1]; m += blockDim.x) world_of[m] = w;
 }
 
-void launch_colour(const ColourDev& d, hipStream_t stream) {
-  hipLaunchKernelGGL(colour_kernel, dim3(d.n_worlds), dim3(256), 0, stream, d);
+void launch_colour(const ColourDev& d, size_t lds_bytes, hipStream_t stream) {
+  hipLaunchKernelGGL(colour_kernel, dim3(d.n_worlds), dim3(256), d.in_lds ? lds_bytes : 0, stream, d);
+}
+
+size_t colour_lds_limit(int device) {
+  int most = 0;
+  if (hipDeviceGetAttribute(&most, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) return 0;
+  const size_t own = (RESTING + 2 + 1) * sizeof(int) + 64;
+  return (size_t)most > own ? (size_t)most - own : 0;
 }
 
 void launch_world_of(const int* manifold_offset, int n_worlds, int* world_of, hipStream_t stream) {

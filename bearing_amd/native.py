@@ -57,7 +57,7 @@ class Columns(ctypes.Structure):
 
 
 EXPORTS = ["bm_create", "bm_destroy", "bm_last_error", "bm_upload", "bm_soft_step", "bm_download",
-           "bm_soft_step_host", "bm_last_kernel_ms", "bm_set_launch", "bm_lds_limit", "bm_colour", "bm_collide_host", "bm_meet_host",
+           "bm_soft_step_host", "bm_last_kernel_ms", "bm_set_launch", "bm_lds_limit", "bm_colour", "bm_collide_host", "bm_meet_host", "bm_set_colour_lds",
            "bm_world_create", "bm_world_destroy", "bm_world_error", "bm_world_add", "bm_world_step", "bm_world_read"]
 
 _lib = None
@@ -84,6 +84,7 @@ def load():
         lib.bm_last_kernel_ms.argtypes = [vp]
         lib.bm_last_kernel_ms.restype = ctypes.c_double
         lib.bm_set_launch.argtypes = [vp, ctypes.c_int32, ctypes.c_int32]
+        lib.bm_set_colour_lds.argtypes = [vp, ctypes.c_int32]
         lib.bm_lds_limit.argtypes = [vp]
         lib.bm_lds_limit.restype = ctypes.c_int64
         world_signatures(lib)
@@ -139,6 +140,9 @@ class Context:
 
     def set_launch(self, shape, resident=True):
         self.check(self.lib.bm_set_launch(self.ptr, shape, 1 if resident else 0), "bm_set_launch")
+
+    def set_colour_lds(self, allow=True):
+        self.check(self.lib.bm_set_colour_lds(self.ptr, 1 if allow else 0), "bm_set_colour_lds")
 
     def lds_limit(self):
         return self.lib.bm_lds_limit(self.ptr)

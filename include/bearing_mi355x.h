@@ -188,6 +188,13 @@ int bm_meet_host(bm_context* ctx, int32_t n, const double* a, const double* b, c
  * bits for every setting. */
 int bm_set_launch(bm_context* ctx, int32_t shape, int32_t resident);
 
+/* How bm_colour runs: allow = 1 (the default) colours each world in one workgroup's LDS,
+ * one manifold after another as the reference does, when every world of the batch fits
+ * there (8 bytes a sort key, 13 a manifold, 5 a body); allow = 0, or a world too large
+ * for LDS, takes the HBM path, a priority fixed point over each manifold's earlier
+ * neighbours. Both give the reference's colours. */
+int bm_set_colour_lds(bm_context* ctx, int32_t allow);
+
 /* Bytes of LDS a soft-step workgroup can hold velocities in on this context's
  * device (a world fits when 48 x its bodies is no more); -1 without a context. */
 int64_t bm_lds_limit(bm_context* ctx);

@@ -94,11 +94,13 @@ def test_gpu_matches_oracle_on_scrambled_worlds(ctx, seed):
 
 
 @pytest.mark.parametrize("path", SOFT, ids=[p.split("/")[-1].replace(".bmfx.gz", "") for p in SOFT])
-def test_gpu_colour_then_soft_step_matches_reference(ctx, path):
+@pytest.mark.parametrize("in_lds", [True, False], ids=["lds", "hbm"])
+def test_gpu_colour_then_soft_step_matches_reference(ctx, path, in_lds):
     """From the manifolds in the order narrow() found them: bm_colour must lay the rows out
     as the reference's colour() / arrange() / rest() did, and the soft step after it must
-    end where the reference's did."""
-    print("fixture", path, flush=True)
+    end where the reference's did; both ways of colouring."""
+    print("fixture", path, in_lds, flush=True)
+    ctx.set_colour_lds(in_lds)
     fx = fixtures.load(path)
     laid = world_from_columns(fx, binding.columns(fx))
     batch = native.Batch([narrow_order(laid, seed=len(path))])
@@ -125,7 +127,9 @@ def test_gpu_colour_then_soft_step_matches_reference(ctx, path):
     assert not errs, "\n".join(errs)
 
 
-def test_gpu_colour_of_a_batch(ctx):
+@pytest.mark.parametrize("in_lds", [True, False], ids=["lds", "hbm"])
+def test_gpu_colour_of_a_batch(ctx, in_lds):
+    ctx.set_colour_lds(in_lds)
     fxs = [fixtures.load(p) for p in SOFT]
     laid = [world_from_columns(fx, binding.columns(fx)) for fx in fxs]
     batch = native.Batch([narrow_order(w, seed=i) for i, w in enumerate(laid)])
