@@ -1,5 +1,5 @@
 // A whole world stepped as the reference steps it (src/world.js step, 880-1072), for
-// bodies that are boxes and spheres, no joints, characters, sensors or compounds.
+// bodies that are boxes, spheres and capsules, no joints, characters, sensors or compounds.
 //
 // Which parts run where. The order-defining serial bookkeeping stays on the host, in the
 // reference's exact order: the AABB trees and fat boxes (src/broadphase.js, src/tree.js),
@@ -24,7 +24,7 @@ namespace {
 
 constexpr int P = 4;
 constexpr int DYNAMIC = 0, STATIC = 1;
-constexpr int BOX = 0, SPHERE = 1;
+constexpr int BOX = 0, SPHERE = 1, CAPSULE = 2;
 constexpr int MOVING_TREE = 1, STATIC_TREE = 0;
 constexpr double PIXEL = 1.0 / 200;      // src/contact.js:125
 constexpr double FAT = 1.0 / 4;          // src/broadphase.js:32
@@ -289,8 +289,11 @@ void bounds_of(bm_world* w, int s, double dt, double gl) {
     ex = std::fabs(ax[0]) * hx + std::fabs(ax[3]) * hy + std::fabs(ax[6]) * hz;
     ey = std::fabs(ax[1]) * hx + std::fabs(ax[4]) * hy + std::fabs(ax[7]) * hz;
     ez = std::fabs(ax[2]) * hx + std::fabs(ax[5]) * hy + std::fabs(ax[8]) * hz;
-  } else {
+  } else if (w->shape[s] == SPHERE) {
     ex = ey = ez = w->dims[s3];
+  } else {
+    const double r = w->dims[s3], hl = (w->dims[s3 + 1] - 2 * r) * 0.5;
+    ex = std::fabs(ax[3]) * hl + r; ey = std::fabs(ax[4]) * hl + r; ez = std::fabs(ax[5]) * hl + r;
   }
   const double grow = m + w->size[s] * PIXEL;
   const double cx = w->position[s3] + 0, cy = w->position[s3 + 1] + 0, cz = w->position[s3 + 2] + 0;
@@ -513,7 +516,8 @@ bool keeps(const bm_world* w, int p, int a, int b, double tolerance) {
   return pr.touching[p] == 1 || drift + moving(w, a, b) <= tolerance;
 }
 
-double rounded(const bm_world* w, int s) { return w->shape[s] == SPHERE ? w->dims[3 * s] : 0; }
+// src/contact.js:573-576 (rounded): a ball's or a capsule's radius, which its kept core leaves out.
+double rounded(const bm_world* w, int s) { return w->shape[s] == SPHERE || w->shape[s] == CAPSULE ? w->dims[3 * s] : 0; }
 
 double least(const Manifold& m) {
   double s = INFINITY;
@@ -562,9 +566,13 @@ bool reused(const bm_world* w, int p, int a, int b, double tolerance, double mar
   return true;
 }
 
-// src/contact.js:581-585 (cored) for a box or a sphere.
+// src/contact.js:578-586 (cored): a ball's kept end is its centre, a capsule's the nearest
+// point of its axis.
 void cored(const bm_world* w, int s, double* local) {
-  if (w->shape[s] == SPHERE) { local[0] = 0; local[1] = 0; local[2] = 0; }
+  if (w->shape[s] == SPHERE) { local[0] = 0; local[1] = 0; local[2] = 0; return; }
+  if (w->shape[s] != CAPSULE) return;
+  const double half = (w->dims[3 * s + 1] - 2 * w->dims[3 * s]) * 0.5, y = local[1];
+  local[0] = 0; local[1] = y < -half ? -half : y > half ? half : y; local[2] = 0;
 }
 
 // src/contact.js:540-570 (remember), for a manifold not found at the moment of impact.
@@ -687,11 +695,13 @@ void carry(bm_world* w, int p, int m) {
 // rotation 4, velocity 3, spin 3, gravity scale (0 unless dynamic), reach, flags
 // (1 dynamic, 2 spin capped at pi/dt: dynamic and not a plain sphere).
 void load(const bm_world* w, int s, double* r) {
-  r[0] = w->shape[s] == BOX ? 2.0 : 0.0;
+  r[0] = w->shape[s] == BOX ? 2.0 : w->shape[s] == CAPSULE ? 1.0 : 0.0;
   for (int k = 0; k < 3; k++) r[1 + k] = w->position[3 * s + k];
   for (int k = 0; k < 9; k++) r[4 + k] = w->axes[9 * s + k];
   if (w->shape[s] == BOX) {
     for (int k = 0; k < 3; k++) r[13 + k] = w->dims[3 * s + k] * 0.5;
+  } else if (w->shape[s] == CAPSULE) {
+    r[13] = w->dims[3 * s]; r[14] = (w->dims[3 * s + 1] - 2 * w->dims[3 * s]) * 0.5; r[15] = 0;
   } else {
     r[13] = w->dims[3 * s]; r[14] = 0; r[15] = 0;
   }
@@ -1162,6 +1172,9 @@ void place_body(const bm_world* w, int s, Core& out) {
   if (w->shape[s] == BOX) {
     out.kind = CORE_BOX;
     for (int k = 0; k < 3; k++) out.half[k] = w->dims[s3 + k] * 0.5;
+  } else if (w->shape[s] == CAPSULE) {
+    out.kind = CORE_CAPSULE;
+    out.half[0] = w->dims[s3]; out.half[1] = (w->dims[s3 + 1] - 2 * w->dims[s3]) * 0.5; out.half[2] = 0;
   } else {
     out.kind = CORE_SPHERE;
     out.half[0] = w->dims[s3]; out.half[1] = 0; out.half[2] = 0;
@@ -1171,12 +1184,14 @@ void place_body(const bm_world* w, int s, Core& out) {
   place_at(out, w->position[s3], w->position[s3 + 1], w->position[s3 + 2], &w->rotation[4 * s]);
 }
 
-// src/query.js:114-128 (extents) for a sphere or a box.
+// src/query.js:114-128 (extents) for a sphere, a capsule or a box.
 void extents(const Core& s, double* out) {
   const double* a = s.axes;
   const double* h = s.half;
   if (s.kind == CORE_SPHERE) {
     out[0] = out[1] = out[2] = h[0];
+  } else if (s.kind == CORE_CAPSULE) {
+    for (int k = 0; k < 3; k++) out[k] = std::fabs(a[3 + k]) * h[1] + h[0];
   } else {
     for (int k = 0; k < 3; k++)
       out[k] = std::fabs(a[k]) * h[0] + std::fabs(a[3 + k]) * h[1] + std::fabs(a[6 + k]) * h[2];
@@ -1498,8 +1513,8 @@ int bm_world_add(bm_world* w, const bm_bodies* b) {
         !b->inertia || !b->inv_inertia || !b->mass || !b->inv_mass || !b->gravity_scale || !b->damping ||
         !b->angular_damping || !b->size || !b->reach || !b->friction || !b->restitution)
       return fail(w, BM_ERR_ARGUMENT, "addBody: every column is needed");
-    if (b->shape[i] != BOX && b->shape[i] != SPHERE)
-      return fail(w, BM_ERR_UNSUPPORTED, "addBody: this world takes boxes and spheres");
+    if (b->shape[i] != BOX && b->shape[i] != SPHERE && b->shape[i] != CAPSULE)
+      return fail(w, BM_ERR_UNSUPPORTED, "addBody: this world takes boxes, spheres and capsules");
     if (b->type[i] != DYNAMIC && b->type[i] != STATIC)
       return fail(w, BM_ERR_UNSUPPORTED, "addBody: this world takes dynamic and static bodies");
   }

@@ -15,6 +15,7 @@ Scenes:
   tower        a box tower, each level turned about y, on a 10x1x10 ground
   pile_world   one world of the batched workload: a tower or a sphere pile
   throw        a thrown box landing and hitting a 6x6 wall of unit boxes
+  capsules     capsules, boxes and balls dropped in a heap on a floor
   darts        small balls and rods fired at 20-200 m/s at a 2 cm static plate and the
                floor: bodies the step would put through them, which the continuous catch
                puts back (src/world.js:1636-1681)
@@ -198,6 +199,26 @@ def darts(rng: SplitMix64, count: int = 64):
     return bodies
 
 
+def capsule_heap(rng: SplitMix64, count: int = 48):
+    """`count` capsules (radius U[0.05, 0.2], half-length U[0.1, 0.5], turned any way), with
+    a box of sides U[0.2, 0.8] and a ball of radius U[0.1, 0.3] after every sixth, dropped at
+    rest from x, z in [-2, 2], y in [0.5, 6] onto a static 20x1x20 floor."""
+    bodies = [_ground((20.0, 1.0, 20.0))]
+    for k in range(count):
+        pos = [rng.between(-2.0, 2.0), rng.between(0.5, 6.0), rng.between(-2.0, 2.0)]
+        q = [rng.gauss(1.0) for _ in range(4)]
+        n = math.sqrt(sum(x * x for x in q))
+        r = rng.between(0.05, 0.2)
+        bodies.append({"shape": "capsule", "radius": r, "height": 2 * r + 2 * rng.between(0.1, 0.5),
+                       "position": pos, "rotation": [x / n for x in q]})
+        if k % 6 == 5:
+            bodies.append({"shape": "box", "size": [rng.between(0.2, 0.8) for _ in range(3)],
+                           "position": [rng.between(-2.0, 2.0), rng.between(0.5, 6.0), rng.between(-2.0, 2.0)]})
+            bodies.append({"shape": "sphere", "radius": rng.between(0.1, 0.3),
+                           "position": [rng.between(-2.0, 2.0), rng.between(0.5, 6.0), rng.between(-2.0, 2.0)]})
+    return bodies
+
+
 SCENES = {
     "pyramid": lambda: pyramid(),
     "rain": lambda: rain(),
@@ -206,6 +227,7 @@ SCENES = {
     "pile_small": lambda: sphere_pile(60, SplitMix64(41)),
     "throw": lambda: throw(SplitMix64(5)),
     "darts": lambda: darts(SplitMix64(7)),
+    "capsules": lambda: capsule_heap(SplitMix64(9)),
 }
 
 

@@ -204,7 +204,7 @@ double bm_last_kernel_ms(const bm_context* ctx);
 
 /* ---- a whole world, stepped as src/world.js step (880-1072) steps it ----
  *
- * For worlds of dynamic and static boxes and spheres, with no joints, sensors,
+ * For worlds of dynamic and static boxes, spheres and capsules, with no joints, sensors,
  * characters or compounds. The host keeps the reference's order-defining serial
  * state (the broadphase trees, the pair list, the narrow driver's order, the
  * sleeping islands); the device runs each step's narrowphase batches, colouring
@@ -215,7 +215,8 @@ typedef struct bm_world bm_world;
 
 /* Bodies as addBody resolves them (src/world.js:563-578, _write 2152-2273), one
  * entry a body: the columns the reference keeps by slot. type: 0 dynamic,
- * 1 static; shape: 0 box (dims = full size), 1 sphere (dims[0] = radius). */
+ * 1 static; shape: 0 box (dims = full size), 1 sphere (dims[0] = radius),
+ * 2 capsule (dims[0] = radius, dims[1] = height end to end). */
 typedef struct bm_bodies {
   int32_t n;
   const uint8_t *type, *shape, *lock;

@@ -44,6 +44,7 @@ struct Scratch {
   double unit[3];
   double seg_s, seg_t;
   double seg_a[6], seg_b[6];
+  double local[6], edge[6];
   double world[3];
   double poly_a[24], poly_b[24];
   int feat_a[8], feat_b[8], line_a[8], line_b[8];
@@ -172,6 +173,272 @@ void sphere_box(Scratch& S, const Shape& a, const Shape& b, double margin, Manif
   const double separation = -(depth + r);
   const double t = r + separation * 0.5;
   add_point(out, a.center[0] + nx * t, a.center[1] + ny * t, a.center[2] + nz * t, separation, 0);
+}
+
+// src/collide.js:271-281 (perpendicular): a unit vector at right angles to v, into unit.
+void perpendicular(Scratch& S, double vx, double vy, double vz) {
+  const double ax = std::fabs(vx), ay = std::fabs(vy), az = std::fabs(vz);
+  if (ax <= ay && ax <= az) normalize(S, 0, vz, -vy);
+  else if (ay <= az) normalize(S, -vz, 0, vx);
+  else normalize(S, vy, -vx, 0);
+}
+
+// src/collide.js:308-312 (segment): a capsule's core segment.
+void segment(const Shape& c, double* out) {
+  const double h = c.half[1];
+  const double* y = c.axes;
+  out[0] = c.center[0] - y[3] * h; out[1] = c.center[1] - y[4] * h; out[2] = c.center[2] - y[5] * h;
+  out[3] = c.center[0] + y[3] * h; out[4] = c.center[1] + y[4] * h; out[5] = c.center[2] + y[5] * h;
+}
+
+// src/collide.js:314-319 (nearestOnSegment)
+double nearest_on_segment(const double* s, double x, double y, double z) {
+  const double dx = s[3] - s[0], dy = s[4] - s[1], dz = s[5] - s[2];
+  const double dd = dx * dx + dy * dy + dz * dz;
+  if (dd == 0) return 0;
+  return clamp01(((x - s[0]) * dx + (y - s[1]) * dy + (z - s[2]) * dz) / dd);
+}
+
+// src/collide.js:357-364 (sphereCapsule)
+void sphere_capsule(Scratch& S, const Shape& a, const Shape& b, double margin, Manifold& out) {
+  const double* c = a.center;
+  segment(b, S.seg_b);
+  const double* sb = S.seg_b;
+  const double t = nearest_on_segment(sb, c[0], c[1], c[2]);
+  const double qx = sb[0] + (sb[3] - sb[0]) * t, qy = sb[1] + (sb[4] - sb[1]) * t, qz = sb[2] + (sb[5] - sb[2]) * t;
+  perpendicular(S, b.axes[3], b.axes[4], b.axes[5]);
+  balls(S, c[0], c[1], c[2], a.half[0], qx, qy, qz, b.half[0], S.unit[0], S.unit[1], S.unit[2], margin, out, 0);
+}
+
+// src/collide.js:424-482 (capsuleCapsule): the closest points; for near-parallel cores the
+// overlap of b's segment with a's, clipped, for two points.
+void capsule_capsule(Scratch& S, const Shape& a, const Shape& b, double margin, double tolerance, Manifold& out) {
+  double* sa = S.seg_a;
+  double* sb = S.seg_b;
+  segment(a, sa);
+  segment(b, sb);
+  closest_segments(S, sa, sb);
+  const double s = S.seg_s, t = S.seg_t;
+  const double pax = sa[0] + (sa[3] - sa[0]) * s, pay = sa[1] + (sa[4] - sa[1]) * s, paz = sa[2] + (sa[5] - sa[2]) * s;
+  const double pbx = sb[0] + (sb[3] - sb[0]) * t, pby = sb[1] + (sb[4] - sb[1]) * t, pbz = sb[2] + (sb[5] - sb[2]) * t;
+  const double ra = a.half[0], rb = b.half[0];
+  const double* ya = a.axes;
+  const double* yb = b.axes;
+  if (normalize(S, ya[4] * yb[5] - ya[5] * yb[4], ya[5] * yb[3] - ya[3] * yb[5], ya[3] * yb[4] - ya[4] * yb[3]) == 0)
+    perpendicular(S, ya[3], ya[4], ya[5]);
+  const double fx = S.unit[0], fy = S.unit[1], fz = S.unit[2];
+  if (!balls(S, pax, pay, paz, ra, pbx, pby, pbz, rb, fx, fy, fz, margin, out, 0)) return;
+  const double closest = out.separations[0];
+  const double cx = out.points[0], cy = out.points[1], cz = out.points[2];
+  out.count = 0;
+  const double h = a.half[1];
+  if (h > 0 && b.half[1] > 0) {
+    const double k0 = (sb[0] - a.center[0]) * ya[3] + (sb[1] - a.center[1]) * ya[4] + (sb[2] - a.center[2]) * ya[5];
+    const double k1 = (sb[3] - sb[0]) * ya[3] + (sb[4] - sb[1]) * ya[4] + (sb[5] - sb[2]) * ya[5];
+    double u0 = 0, u1 = 1;
+    if (k1 != 0) {
+      const double ua = (-h - k0) / k1, ub = (h - k0) / k1;
+      u0 = js_max(0, js_min(ua, ub));
+      u1 = js_min(1, js_max(ua, ub));
+    } else if (k0 < -h || k0 > h) {
+      u1 = -1;
+    }
+    if (u1 > u0) {
+      const double* n = out.normal;
+      double deepest = INFINITY;
+      for (int e = 0; e < 2; e++) {
+        const double u = e == 0 ? u0 : u1;
+        const double bx = sb[0] + (sb[3] - sb[0]) * u, by = sb[1] + (sb[4] - sb[1]) * u, bz = sb[2] + (sb[5] - sb[2]) * u;
+        const double along = js_max(-h, js_min(h, k0 + k1 * u));
+        const double ax = a.center[0] + ya[3] * along, ay = a.center[1] + ya[4] * along, az = a.center[2] + ya[5] * along;
+        const double gx = bx - ax, gy = by - ay, gz = bz - az;
+        const double normal_gap = gx * n[0] + gy * n[1] + gz * n[2];
+        const double separation = normal_gap - ra - rb;
+        if (separation > margin) continue;
+        if (gx * gx + gy * gy + gz * gz - normal_gap * normal_gap > tolerance * tolerance) continue;
+        deepest = js_min(deepest, separation);
+        const double m = ra + separation * 0.5;
+        add_point(out, ax + n[0] * m, ay + n[1] * m, az + n[2] * m, separation, 1 + e);
+      }
+      if (out.count > 0 && closest < deepest) add_point(out, cx, cy, cz, closest, 0);
+      if (out.count > 0) return;
+    }
+  }
+  add_point(out, cx, cy, cz, closest, 0);
+}
+
+void to_world(Scratch& S, const Shape& b, double x, double y, double z);
+
+// src/collide.js:559-575 (segmentHitsBox): whether the segment (box frame) passes through the box.
+bool segment_hits_box(const double* s, const double* h) {
+  double t0 = 0, t1 = 1;
+  for (int i = 0; i < 3; i++) {
+    const double p = s[i], d = s[3 + i] - s[i];
+    if (d == 0) {
+      if (p < -h[i] || p > h[i]) return false;
+      continue;
+    }
+    double ta = (-h[i] - p) / d, tb = (h[i] - p) / d;
+    if (ta > tb) { const double x = ta; ta = tb; tb = x; }
+    if (ta > t0) t0 = ta;
+    if (tb < t1) t1 = tb;
+    if (t0 > t1) return false;
+  }
+  return true;
+}
+
+// src/collide.js:587-618 (clipToFace): the capsule's core (S.local, box frame) clipped to
+// the face `axis` on side `facing`, one or two points.
+bool clip_to_face(Scratch& S, const Shape& b, int axis, double facing, double r, double margin, Manifold& out,
+                  uint32_t id_base) {
+  const double* h = b.half;
+  const double* local = S.local;
+  const int j = (axis + 1) % 3, k = (axis + 2) % 3;
+  double u0 = 0, u1 = 1;
+  for (int pass = 0; pass < 2; pass++) {
+    const int c = pass == 0 ? j : k;
+    const double p = local[c], d = local[3 + c] - local[c];
+    if (d == 0) {
+      if (p < -h[c] || p > h[c]) return false;
+      continue;
+    }
+    double ta = (-h[c] - p) / d, tb = (h[c] - p) / d;
+    if (ta > tb) { const double x = ta; ta = tb; tb = x; }
+    if (ta > u0) u0 = ta;
+    if (tb < u1) u1 = tb;
+  }
+  if (u0 > u1) return false;
+  const double* ax = b.axes;
+  set_normal(out, -facing * ax[3 * axis], -facing * ax[3 * axis + 1], -facing * ax[3 * axis + 2]);
+  const int face = 2 * axis + (facing > 0 ? 1 : 0);
+  for (int e = 0; e < (u1 > u0 ? 2 : 1); e++) {
+    const double u = e == 0 ? u0 : u1;
+    const double px = local[0] + (local[3] - local[0]) * u, py = local[1] + (local[4] - local[1]) * u,
+                 pz = local[2] + (local[5] - local[2]) * u;
+    const double above = facing * (axis == 0 ? px : axis == 1 ? py : pz) - h[axis];
+    const double separation = above - r;
+    if (separation > margin) continue;
+    const double off = facing * (separation * 0.5 - above);
+    to_world(S, b, axis == 0 ? px + off : px, axis == 1 ? py + off : py, axis == 2 ? pz + off : pz);
+    add_point(out, S.world[0], S.world[1], S.world[2], separation, id_base + 4 * face + 1 + e);
+  }
+  return out.count > 0;
+}
+
+// src/collide.js:630-676 (capsuleBoxDeep): the core passes through the box; separating axes
+// over the box's faces and the core crossed with its edges, a face preferred within the tolerance.
+void capsule_box_deep(Scratch& S, const Shape& a, const Shape& b, double margin, double tolerance, Manifold& out) {
+  const double* h = b.half;
+  const double r = a.half[0];
+  const double* local = S.local;
+  double* edge = S.edge;
+  const double dx = local[3] - local[0], dy = local[4] - local[1], dz = local[5] - local[2];
+  double best_sep = -INFINITY, nx = 0, ny = 0, nz = 0;
+  int best_axis = -1;
+  for (int c = 0; c < 6; c++) {
+    double x = 0, y = 0, z = 0;
+    if (c < 3) {
+      if (c == 0) x = 1; else if (c == 1) y = 1; else z = 1;
+    } else {
+      const int e = c - 3;
+      const double l = normalize(S, e == 0 ? 0 : e == 1 ? -dz : dy, e == 0 ? dz : e == 1 ? 0 : -dx,
+                                 e == 0 ? -dy : e == 1 ? dx : 0);
+      if (!(l > PARALLEL * std::sqrt(dx * dx + dy * dy + dz * dz))) continue;
+      x = S.unit[0]; y = S.unit[1]; z = S.unit[2];
+    }
+    const double sup = h[0] * std::fabs(x) + h[1] * std::fabs(y) + h[2] * std::fabs(z);
+    const double p0 = local[0] * x + local[1] * y + local[2] * z, p1 = local[3] * x + local[4] * y + local[5] * z;
+    const double lo = js_min(p0, p1), hi = js_max(p0, p1);
+    const double plus = lo - sup, minus = -sup - hi;
+    const double sep = js_max(plus, minus) - r;
+    if (sep > best_sep + (c >= 3 && best_axis < 3 ? tolerance : 0)) {
+      best_sep = sep; best_axis = c;
+      const double sg = plus >= minus ? -1 : 1;
+      nx = x * sg; ny = y * sg; nz = z * sg;
+    }
+  }
+  if (best_axis < 3) {
+    const double facing = -(best_axis == 0 ? nx : best_axis == 1 ? ny : nz);
+    clip_to_face(S, b, best_axis, sign(facing), r, margin, out, 0);
+    return;
+  }
+  const int i = best_axis - 3, j = (i + 1) % 3, k = (i + 2) % 3;
+  const double mj = j == 0 ? -nx : j == 1 ? -ny : -nz, mk = k == 0 ? -nx : k == 1 ? -ny : -nz;
+  edge[i] = -h[i]; edge[3 + i] = h[i];
+  edge[j] = edge[3 + j] = sign(mj) * h[j];
+  edge[k] = edge[3 + k] = sign(mk) * h[k];
+  closest_segments(S, local, edge);
+  const double px = local[0] + dx * S.seg_s, py = local[1] + dy * S.seg_s, pz = local[2] + dz * S.seg_s;
+  const double ex = edge[0] + (edge[3] - edge[0]) * S.seg_t, ey = edge[1] + (edge[4] - edge[1]) * S.seg_t,
+               ez = edge[2] + (edge[5] - edge[2]) * S.seg_t;
+  const double separation = (ex - px) * nx + (ey - py) * ny + (ez - pz) * nz - r;
+  const double* ax = b.axes;
+  set_normal(out, ax[0] * nx + ax[3] * ny + ax[6] * nz, ax[1] * nx + ax[4] * ny + ax[7] * nz,
+             ax[2] * nx + ax[5] * ny + ax[8] * nz);
+  const double mm = r + separation * 0.5;
+  to_world(S, b, px + nx * mm, py + ny * mm, pz + nz * mm);
+  add_point(out, S.world[0], S.world[1], S.world[2], separation, 32 + i);
+}
+
+// src/collide.js:497-550 (capsuleBox): the core in the box's frame; through the box, the
+// deep case; else the closest of its ends and the box's twelve edges, and on a face the
+// core clipped to it.
+void capsule_box(Scratch& S, const Shape& a, const Shape& b, double margin, double tolerance, Manifold& out) {
+  double* sa = S.seg_a;
+  double* local = S.local;
+  double* edge = S.edge;
+  segment(a, sa);
+  const double* bx = b.axes;
+  const double* h = b.half;
+  const double r = a.half[0];
+  for (int e = 0; e < 2; e++) {
+    const double dx = sa[3 * e] - b.center[0], dy = sa[3 * e + 1] - b.center[1], dz = sa[3 * e + 2] - b.center[2];
+    local[3 * e] = dx * bx[0] + dy * bx[1] + dz * bx[2];
+    local[3 * e + 1] = dx * bx[3] + dy * bx[4] + dz * bx[5];
+    local[3 * e + 2] = dx * bx[6] + dy * bx[7] + dz * bx[8];
+  }
+  if (segment_hits_box(local, h)) {
+    capsule_box_deep(S, a, b, margin, tolerance, out);
+    return;
+  }
+  double best = INFINITY, sx = 0, sy = 0, sz = 0, qx = 0, qy = 0, qz = 0;
+  for (int e = 0; e < 2; e++) {
+    const double px = local[3 * e], py = local[3 * e + 1], pz = local[3 * e + 2];
+    const double cx = js_max(-h[0], js_min(h[0], px)), cy = js_max(-h[1], js_min(h[1], py)),
+                 cz = js_max(-h[2], js_min(h[2], pz));
+    const double d = (px - cx) * (px - cx) + (py - cy) * (py - cy) + (pz - cz) * (pz - cz);
+    if (d < best) { best = d; sx = px; sy = py; sz = pz; qx = cx; qy = cy; qz = cz; }
+  }
+  for (int i = 0; i < 3; i++) {
+    const int j = (i + 1) % 3, k = (i + 2) % 3;
+    for (int c = 0; c < 4; c++) {
+      edge[i] = -h[i]; edge[3 + i] = h[i];
+      edge[j] = edge[3 + j] = (c & 1) ? h[j] : -h[j];
+      edge[k] = edge[3 + k] = (c & 2) ? h[k] : -h[k];
+      closest_segments(S, local, edge);
+      const double px = local[0] + (local[3] - local[0]) * S.seg_s, py = local[1] + (local[4] - local[1]) * S.seg_s,
+                   pz = local[2] + (local[5] - local[2]) * S.seg_s;
+      const double ex = edge[0] + (edge[3] - edge[0]) * S.seg_t, ey = edge[1] + (edge[4] - edge[1]) * S.seg_t,
+                   ez = edge[2] + (edge[5] - edge[2]) * S.seg_t;
+      const double d = (px - ex) * (px - ex) + (py - ey) * (py - ey) + (pz - ez) * (pz - ez);
+      if (d < best) { best = d; sx = px; sy = py; sz = pz; qx = ex; qy = ey; qz = ez; }
+    }
+  }
+  const double dist = normalize(S, qx - sx, qy - sy, qz - sz);
+  if (dist - r > margin) return;
+  const double nlx = S.unit[0], nly = S.unit[1], nlz = S.unit[2];
+  set_normal(out, bx[0] * nlx + bx[3] * nly + bx[6] * nlz, bx[1] * nlx + bx[4] * nly + bx[7] * nlz,
+             bx[2] * nlx + bx[5] * nly + bx[8] * nlz);
+  const bool on_x = std::fabs(qx) == h[0], on_y = std::fabs(qy) == h[1], on_z = std::fabs(qz) == h[2];
+  if ((int)on_x + (int)on_y + (int)on_z == 1) {
+    const int axis = on_x ? 0 : on_y ? 1 : 2;
+    const double facing = sign(axis == 0 ? qx : axis == 1 ? qy : qz);
+    if (clip_to_face(S, b, axis, facing, r, margin, out, 0)) return;
+  }
+  const double separation = dist - r;
+  const double m = r + separation * 0.5;
+  to_world(S, b, sx + nlx * m, sy + nly * m, sz + nlz * m);
+  add_point(out, S.world[0], S.world[1], S.world[2], separation, 0);
 }
 
 // src/collide.js:840-844
@@ -549,7 +816,10 @@ int collide(const Shape& a, const Shape& b, double margin, double tolerance, Man
   const Shape& lo = swap ? b : a;
   const Shape& hi = swap ? a : b;
   if (lo.kind == SPHERE && hi.kind == SPHERE) sphere_sphere(S, lo, hi, margin, out);
+  else if (lo.kind == SPHERE && hi.kind == CAPSULE) sphere_capsule(S, lo, hi, margin, out);
   else if (lo.kind == SPHERE && hi.kind == BOX) sphere_box(S, lo, hi, margin, out);
+  else if (lo.kind == CAPSULE && hi.kind == CAPSULE) capsule_capsule(S, lo, hi, margin, tolerance, out);
+  else if (lo.kind == CAPSULE && hi.kind == BOX) capsule_box(S, lo, hi, margin, tolerance, out);
   else if (lo.kind == BOX && hi.kind == BOX) box_box(S, lo, hi, margin, tolerance, out);
   else return -1;
   if (swap) { out.normal[0] = -out.normal[0]; out.normal[1] = -out.normal[1]; out.normal[2] = -out.normal[2]; }

@@ -28,6 +28,7 @@ JOBS = {
     "throw": dict(scene="throw", dt=1 / 120, steps=120, soft=[30, 60, 90, 119], colour=[60], trace=70, every=30),
     "pile_small": dict(scene="pile_small", dt=1 / 60, steps=150, soft=[30, 70, 110], colour=[70], trace=100, every=50),
     "darts": dict(scene="darts", dt=1 / 60, steps=90, soft=[2, 20], colour=[], trace=-1, every=30),
+    "capsules": dict(scene="capsules", dt=1 / 60, steps=240, soft=[60, 120], colour=[120], trace=-1, every=60),
     # The 2,000-sphere rain at full size: its solve phase mid-landing is the bench's workload.
     "rain": dict(scene="rain", dt=1 / 60, steps=900, soft=[300], colour=[], trace=-1, every=300),
 }
@@ -137,12 +138,88 @@ def _axes(q):
             2 * (x * z + y * w), 2 * (y * z - x * w), 1 - 2 * (x * x + y * y)]
 
 
-def run_collide(seed=2026, n=4000):
-    pairs = collide_pairs(seed, n)
-    path = os.path.join(DATA, "collide.in.bin")
+def capsule_pairs(seed, n):
+    """Seeded pairs with a capsule (kind 1: half = radius, half its segment's length): against
+    a sphere (both orders), a capsule (crossing, near-parallel and parallel, end to end) and a
+    box (resting on a face, across an edge, through it, off a corner). Records as
+    collide_pairs'."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    recs = []
+
+    def quat(kind):
+        if kind == "id":
+            return [0.0, 0.0, 0.0, 1.0]
+        if kind == "flat":   # the capsule's axis (its y) turned to lie along x or z
+            t = np.pi / 2
+            ax = [0.0, 0.0, 1.0] if rng.random() < 0.5 else [1.0, 0.0, 0.0]
+            ax = np.array(ax)
+            if rng.random() < 0.5:
+                t += rng.normal(0, 0.01)
+            return list(ax * np.sin(t / 2)) + [np.cos(t / 2)]
+        v = rng.normal(size=4)
+        return list(v / np.linalg.norm(v))
+
+    def weight():
+        return [0.0, -9.81, 0.0] if rng.random() < 0.7 else [0.0, 0.0, 0.0]
+
+    def capsule(centre, q):
+        r = rng.uniform(0.05, 0.4)
+        hl = rng.uniform(0.0, 0.8) if rng.random() < 0.9 else 0.0
+        return [1, *centre, *q, r, hl, 0.0, *weight()], r, hl
+
+    for k in range(n):
+        case = k % 6
+        margin = float(rng.choice([0.0, rng.uniform(0, 0.05), rng.uniform(0, 0.4)]))
+        if case in (0, 1):   # sphere - capsule
+            cc = rng.uniform(-2, 2, 3)
+            cap, r, hl = capsule(cc, quat(rng.choice(["id", "flat", "any"])))
+            rs = rng.uniform(0.05, 0.5)
+            d = rng.normal(size=3)
+            d /= np.linalg.norm(d)
+            cs = cc + d * (r + rs + hl) * rng.uniform(0.3, 1.3)
+            sph = [0, *cs, 0, 0, 0, 1, rs, 0, 0, *weight()]
+            a, b = (sph, cap) if case == 0 else (cap, sph)
+            tol = min(2 * r, 2 * rs) / 200
+        elif case == 2:      # capsule - capsule
+            ca = rng.uniform(-2, 2, 3)
+            qa = quat(rng.choice(["id", "flat", "any"]))
+            a, ra, ha = capsule(ca, qa)
+            qb = qa if rng.random() < 0.4 else quat(rng.choice(["id", "flat", "any"]))
+            if rng.random() < 0.3:   # nearly parallel
+                qb = list(np.array(qa) + rng.normal(0, 1e-4, 4))
+                qb = list(np.array(qb) / np.linalg.norm(qb))
+            off = rng.normal(size=3)
+            off /= np.linalg.norm(off)
+            b, rb, hb = capsule(ca + off * rng.uniform(0.0, 1.2) * (ra + 0.4 + ha), qb)
+            tol = min(2 * ra, 2 * rb) / 200
+        else:                # capsule - box
+            h = rng.uniform(0.1, 1.5, 3)
+            qb = quat(rng.choice(["id", "id", "any"]))
+            cb = rng.uniform(-2, 2, 3)
+            R = np.array(_axes(qb)).reshape(3, 3).T
+            qc = quat(rng.choice(["flat", "any", "id"]))
+            local = rng.uniform(-1.3, 1.3, 3) * (h + 0.3)
+            if case == 3:    # resting on the top face, lying flat
+                local = np.array([rng.uniform(-1, 1) * h[0], h[1] + rng.uniform(-0.05, 0.3), rng.uniform(-1, 1) * h[2]])
+                qc = quat("flat")
+            cap, r, hl = capsule(cb + R @ local, qc)
+            if case == 3:
+                cap[2] += r * 0.9
+            box = [2, *cb, *qb, *h, *weight()]
+            a, b = (cap, box) if rng.random() < 0.8 else (box, cap)
+            tol = min(2 * r, 2 * h.min()) / 200
+        recs.append(a + b + [margin, tol])
+    return np.asarray(recs, np.float64)
+
+
+def run_collide(seed=2026, n=4000, capsules=False):
+    pairs = capsule_pairs(seed, n) if capsules else collide_pairs(seed, n)
+    name = "collide_capsule" if capsules else "collide"
+    path = os.path.join(DATA, name + ".in.bin")
     pairs.tofile(path)
     subprocess.run(["node", "--no-warnings", "--experimental-loader", os.path.join(HERE, "lower.mjs"),
-                    os.path.join(HERE, "make_collide.mjs"), path, os.path.join(DATA, f"collide.{seed}.bmfx.gz")],
+                    os.path.join(HERE, "make_collide.mjs"), path, os.path.join(DATA, f"{name}.{seed}.bmfx.gz")],
                    check=True)
     os.remove(path)
 
@@ -219,6 +296,9 @@ def run_impact(seed=2026, n=2000):
 if __name__ == "__main__":
     if sys.argv[1:] == ["collide"]:
         run_collide()
+        sys.exit(0)
+    if sys.argv[1:] == ["collide_capsules"]:
+        run_collide(n=3000, capsules=True)
         sys.exit(0)
     if sys.argv[1:] == ["impact"]:
         run_impact()

@@ -15,7 +15,7 @@ def traj(name):
     return fixtures.load(os.path.join(fixtures.DATA, f"{name}.traj.bmfx.gz"))
 
 
-@pytest.mark.parametrize("name", ["pyramid", "tower", "throw", "rain_small", "pile_small", "darts", "rain"])
+@pytest.mark.parametrize("name", ["pyramid", "tower", "throw", "rain_small", "pile_small", "darts", "capsules", "rain"])
 def test_gpu_world_matches_reference_trajectory(name):
     print("trajectory", name, flush=True)
     fx = traj(name)
@@ -30,11 +30,11 @@ def test_gpu_world_matches_reference_trajectory(name):
 
 
 def test_gpu_world_refuses_what_it_does_not_handle():
-    # A capsule (shape 2) is not a shape this world takes: addBody fails loudly.
+    # A hull (shape 3) is not a shape this world takes: addBody fails loudly.
     fx = traj("tower")
     cols = init_columns(fx)
     cols["shape"] = cols["shape"].copy()
-    cols["shape"][3] = 2
+    cols["shape"][3] = 3
     w = native.World(0)
-    with pytest.raises(native.BearingError, match="boxes and spheres"):
+    with pytest.raises(native.BearingError, match="boxes, spheres and capsules"):
         w.add(cols)

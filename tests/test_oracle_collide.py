@@ -1,8 +1,9 @@
-"""The oracle's narrowphase (spheres and boxes) against the reference.
+"""The oracle's narrowphase (spheres, capsules and boxes) against the reference.
 
 1. The reference's own hand-worked cases (test/collide.test.js:61-199), to its tolerance of 1e-9.
-2. collide() as the reference computed it on 4,000 seeded pairs (tests/golden collide.*):
-   counts, normals, points, separations and feature ids, bit for bit.
+2. collide() as the reference computed it on 4,000 seeded sphere/box pairs (tests/golden
+   collide.*) and 3,000 pairs with a capsule (collide_capsule.*): counts, normals, points,
+   separations and feature ids, bit for bit.
 """
 import math
 
@@ -27,14 +28,17 @@ def test_collide_matches_reference_bit_for_bit():
         assert not errs, "\n".join(errs)
 
 
-def shape(kind, size=(1, 1, 1), radius=0.5, position=(0, 0, 0), rotation=(0, 0, 0, 1)):
+def shape(kind, size=(1, 1, 1), radius=0.5, height=2.0, position=(0, 0, 0), rotation=(0, 0, 0, 1)):
     """A placed shape record, as the reference's test helper make() builds one."""
     x, y, z, w = rotation
     axes = [1 - 2 * (y * y + z * z), 2 * (x * y + z * w), 2 * (x * z - y * w),
             2 * (x * y - z * w), 1 - 2 * (x * x + z * z), 2 * (y * z + x * w),
             2 * (x * z + y * w), 2 * (y * z - x * w), 1 - 2 * (x * x + y * y)]
-    half = [size[0] / 2, size[1] / 2, size[2] / 2] if kind == "box" else [radius, 0, 0]
-    return [2.0 if kind == "box" else 0.0, *position, *axes, *half, 0, 0, 0]
+    if kind == "box":
+        return [2.0, *position, *axes, size[0] / 2, size[1] / 2, size[2] / 2, 0, 0, 0]
+    if kind == "capsule":
+        return [1.0, *position, *axes, radius, (height - 2 * radius) / 2, 0, 0, 0, 0]
+    return [0.0, *position, *axes, radius, 0, 0, 0, 0, 0]
 
 
 def turn(axis, radians):
@@ -117,3 +121,64 @@ def test_hand_worked_box_box():
     assert n == 1
     assert abs(pts[0][1] - 0.01) <= 1e-12
     assert np.allclose(pts[0][0], [0, 0.505, 0], atol=1e-12)
+
+
+def test_capsules_match_reference_bit_for_bit():
+    paths = sorted(__import__("glob").glob(fixtures.DATA + "/collide_capsule.*.bmfx.gz"))
+    assert paths
+    for path in paths:
+        fx = fixtures.load(path)
+        sh = fx["shapes"].reshape(-1, 2, 19)
+        got = binding.collide(sh[:, 0].copy(), sh[:, 1].copy(), fx["margin"], fx["tolerance"])
+        errs = compare_manifolds(fx, *got)
+        assert not errs, "\n".join(errs)
+        # every capsule pairing, and the manifold sizes each can have, are in the vectors
+        kinds = set(map(tuple, sh[:, :, 0].astype(int)))
+        assert {(0, 1), (1, 0), (1, 1), (1, 2), (2, 1)} <= kinds
+        assert int(fx["count"].max()) == 3
+
+
+def test_hand_worked_sphere_capsule():
+    # test/collide.test.js:99-112
+    capsule = shape("capsule", radius=0.5, height=2)
+    n, normal, pts = run(shape("sphere", position=(1.2, 0, 0)), capsule)
+    assert np.allclose(normal, [-1, 0, 0], atol=1e-9)
+    same_points(pts, [([0.6, 0, 0], 0.2)])
+    n, normal, pts = run(shape("sphere", position=(0, 1.8, 0)), capsule, margin=0.5)
+    assert np.allclose(normal, [0, -1, 0], atol=1e-9)
+    same_points(pts, [([0, 1.15, 0], 0.3)])
+    n, normal, pts = run(shape("sphere", position=(0, 0, 0)), capsule)
+    assert np.allclose(normal, [0, 0, -1], atol=1e-9)
+    assert abs(pts[0][1] + 1) <= 1e-9
+
+
+def test_hand_worked_capsule_capsule():
+    # test/collide.test.js:114-126
+    a = shape("capsule")
+    n, normal, pts = run(a, shape("capsule", position=(1.1, 0, 0)))
+    assert np.allclose(normal, [1, 0, 0], atol=1e-9)
+    same_points(pts, [([0.55, -0.5, 0], 0.1), ([0.55, 0.5, 0], 0.1)])
+    n, normal, pts = run(a, shape("capsule", position=(1.1, 0, 0), rotation=turn([1, 0, 0], math.pi / 2)))
+    same_points(pts, [([0.55, 0, 0], 0.1)])
+    n, normal, pts = run(a, shape("capsule", position=(0, 2.1, 0)))
+    assert np.allclose(normal, [0, 1, 0], atol=1e-9)
+    same_points(pts, [([0, 1.05, 0], 0.1)])
+    n, normal, pts = run(a, shape("capsule", position=(1.1, 0.6, 0)))
+    same_points(pts, [([0.55, 0.1, 0], 0.1), ([0.55, 0.5, 0], 0.1)])
+
+
+def test_hand_worked_capsule_box():
+    # test/collide.test.js:128-144
+    box = shape("box", size=(4, 1, 4))
+    lying = turn([0, 0, 1], -math.pi / 2)
+    n, normal, pts = run(shape("capsule", position=(0, 1.1, 0), rotation=lying), box)
+    assert np.allclose(normal, [0, -1, 0], atol=1e-9)
+    same_points(pts, [([-0.5, 0.55, 0], 0.1), ([0.5, 0.55, 0], 0.1)])
+    n, normal, pts = run(shape("capsule", position=(0, 1.6, 0)), box)
+    same_points(pts, [([0, 0.55, 0], 0.1)])
+    n, normal, pts = run(shape("capsule", position=(2.3, 0.8, 0), rotation=turn([1, 0, 0], math.pi / 2)), box)
+    assert np.allclose(normal, [-math.sqrt(0.5), -math.sqrt(0.5), 0], atol=1e-9)
+    assert n == 1 and abs(pts[0][1] - (0.3 * math.sqrt(2) - 0.5)) <= 1e-9
+    n, normal, pts = run(shape("capsule", radius=0.3, height=6, position=(0, 0.3, 0), rotation=lying), box)
+    assert np.allclose(normal, [0, -1, 0], atol=1e-9)
+    same_points(pts, [([-2, 0.25, 0], -0.5), ([2, 0.25, 0], -0.5)])
