@@ -212,3 +212,19 @@ def sincos(x):
     s, c = np.zeros(x.size), np.zeros(x.size)
     lib().oracle_sincos(x.size, x.ctypes.data_as(_D), s.ctypes.data_as(_D), c.ctypes.data_as(_D))
     return s, c
+
+
+def hulls(fx):
+    """oracle_hulls with a fixture's hull table (hullV, hullF, hullL, vertices, start, loops, planes)."""
+    i32 = lambda x: np.ascontiguousarray(x, np.int32)
+    hv, hf, hl = i32(fx["hullV"]), i32(fx["hullF"]), i32(fx["hullL"])
+    v, st, lp, pl = _f64(fx["vertices"]), i32(fx["start"]), i32(fx["loops"]), _f64(fx["planes"])
+    lib().oracle_hulls(len(hv) - 1, hv.ctypes.data_as(_I32), hf.ctypes.data_as(_I32), hl.ctypes.data_as(_I32),
+                       v.ctypes.data_as(_D), st.ctypes.data_as(_I32), lp.ctypes.data_as(_I32), pl.ctypes.data_as(_D))
+
+
+def losses(n):
+    """The last collide() call's Manifold.loss per pair."""
+    out = np.zeros(n)
+    lib().oracle_losses(out.ctypes.data_as(_D))
+    return out

@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <vector>
 
 #include "jsmath.h"
 
@@ -18,9 +19,11 @@ namespace {
 using oracle::js_max;
 using oracle::js_min;
 
-constexpr int SPHERE = 0, CAPSULE = 1, BOX = 2;
+constexpr int SPHERE = 0, CAPSULE = 1, BOX = 2, HULL = 3;
 constexpr int MAX_POINTS = 4;
 constexpr double PARALLEL = 7.450580596923828e-9;  // 2^-27, src/collide.js:686
+
+struct Hull;
 
 struct Shape {
   int kind;
@@ -28,6 +31,7 @@ struct Shape {
   double axes[9];
   double half[3];
   double weight[3];
+  const Hull* hull = nullptr;  // a hull's tables, in its body's frame
 };
 
 struct Manifold {
@@ -56,6 +60,9 @@ struct Scratch {
   double foot[3];
   const Shape* shape_a;
   const Shape* shape_b;
+  // The list reducePoints is working on (src/collide.js:1077): kept, or a hull face's.
+  const double* rk;
+  const double* rs;
 };
 
 const int CORNER_P[4] = {-1, 1, 1, -1}, CORNER_Q[4] = {-1, -1, 1, 1};
@@ -549,8 +556,8 @@ int clip_polygon(const double* src, const int* sf, const int* sl, int count, dou
 
 // src/collide.js:1116-1141 (four)
 void four(Scratch& S, int n, int a, double nx, double ny, double nz, double pressed, double touching) {
-  const double* kept = S.kept;
-  const double* sep = S.kept_sep;
+  const double* kept = S.rk;
+  const double* sep = S.rs;
   int b = -1;
   double far = -1;
   for (int pass = 0; pass < 3 && b < 0; pass++) {
@@ -586,7 +593,7 @@ bool foot(Scratch& S, const Shape& s, double nx, double ny, double nz) {
   const double along = w[0] * nx + w[1] * ny + w[2] * nz;
   const int o = S.four[0];
   if (along == 0) return false;
-  const double t = ((S.kept[3 * o] - c[0]) * nx + (S.kept[3 * o + 1] - c[1]) * ny + (S.kept[3 * o + 2] - c[2]) * nz) / along;
+  const double t = ((S.rk[3 * o] - c[0]) * nx + (S.rk[3 * o + 1] - c[1]) * ny + (S.rk[3 * o + 2] - c[2]) * nz) / along;
   if (!(t > 0)) return false;
   S.foot[0] = c[0] + w[0] * t; S.foot[1] = c[1] + w[1] * t; S.foot[2] = c[2] + w[2] * t;
   return true;
@@ -600,8 +607,8 @@ bool rests(Scratch& S, const Shape& s, double nx, double ny, double nz) {
   for (int e = 1; e <= 4; e++) {
     const int to = S.four[e & 3];
     if (to < 0) continue;
-    const double ax = S.kept[3 * from] - px, ay = S.kept[3 * from + 1] - py, az = S.kept[3 * from + 2] - pz;
-    const double bx = S.kept[3 * to] - px, by = S.kept[3 * to + 1] - py, bz = S.kept[3 * to + 2] - pz;
+    const double ax = S.rk[3 * from] - px, ay = S.rk[3 * from + 1] - py, az = S.rk[3 * from + 2] - pz;
+    const double bx = S.rk[3 * to] - px, by = S.rk[3 * to + 1] - py, bz = S.rk[3 * to + 2] - pz;
     if ((ay * bz - az * by) * nx + (az * bx - ax * bz) * ny + (ax * by - ay * bx) * nz < 0) return false;
     from = to;
   }
@@ -613,8 +620,8 @@ int under(Scratch& S, int n, double nx, double ny, double nz, double pressed) {
   int start = -1;
   double near = INFINITY;
   for (int k = 0; k < n; k++) {
-    if (S.kept_sep[k] > pressed) continue;
-    const double x = S.kept[3 * k] - S.foot[0], y = S.kept[3 * k + 1] - S.foot[1], z = S.kept[3 * k + 2] - S.foot[2];
+    if (S.rs[k] > pressed) continue;
+    const double x = S.rk[3 * k] - S.foot[0], y = S.rk[3 * k + 1] - S.foot[1], z = S.rk[3 * k + 2] - S.foot[2];
     const double along = x * nx + y * ny + z * nz;
     const double d = x * x + y * y + z * z - along * along;
     if (d < near) { near = d; start = k; }
@@ -622,11 +629,11 @@ int under(Scratch& S, int n, double nx, double ny, double nz, double pressed) {
   return start;
 }
 
-// src/collide.js:1077-1104 (reducePoints)
-void reduce_points(Scratch& S, int n, double nx, double ny, double nz, double tolerance, Manifold& out) {
-  const double* kept = S.kept;
-  const double* sep = S.kept_sep;
-  const uint32_t* id = S.kept_id;
+// src/collide.js:1077-1104 (reducePoints), on any list of n points.
+void reduce_list(Scratch& S, const double* kept, const double* sep, const uint32_t* id, int n, double nx, double ny,
+                 double nz, double tolerance, Manifold& out) {
+  S.rk = kept;
+  S.rs = sep;
   if (n <= MAX_POINTS) {
     for (int k = 0; k < n; k++) add_point(out, kept[3 * k], kept[3 * k + 1], kept[3 * k + 2], sep[k], id[k]);
     return;
@@ -654,6 +661,11 @@ void reduce_points(Scratch& S, int n, double nx, double ny, double nz, double to
   add_point(out, kept[3 * b], kept[3 * b + 1], kept[3 * b + 2], sep[b], id[b]);
   if (c >= 0) add_point(out, kept[3 * c], kept[3 * c + 1], kept[3 * c + 2], sep[c], id[c]);
   if (d >= 0) add_point(out, kept[3 * d], kept[3 * d + 1], kept[3 * d + 2], sep[d], id[d]);
+}
+
+// src/collide.js:1072-1074 (reduce): the box pairs' list.
+void reduce_points(Scratch& S, int n, double nx, double ny, double nz, double tolerance, Manifold& out) {
+  reduce_list(S, S.kept, S.kept_sep, S.kept_id, n, nx, ny, nz, tolerance, out);
 }
 
 // src/collide.js:884-945 (faceContact)
@@ -805,6 +817,577 @@ void box_box(Scratch& S, const Shape& a, const Shape& b, double margin, double t
   if (out.count == 0 && js_max(js_max(face_a, face_b), edge_sep) > 0) box_box_closest(S, a, b, margin, out);
 }
 
+// ---- hulls: src/hull.js:36-131 (the tables), src/collide.js:1262-1882 (the pairs) ----
+
+// A hull's tables in its own frame, with what src/hull.js's constructor derives from them:
+// each edge once (from the first face to meet it), its two faces, the box round the
+// corners, the reach, the thinnest size and the inscribed ball.
+struct Hull {
+  int V = 0, F = 0, E = 0;
+  std::vector<double> vertices, planes;  // 3V; 4F: unit outward normal, distance
+  std::vector<int> start, loops;         // F + 1; corners counter-clockwise from outside
+  std::vector<int> edges, sides;         // 2E each
+  double lo[3], hi[3], reach = 0, size = 0, inner = 0;
+};
+
+// src/hull.js:45-80 (the constructor's derived tables).
+void derive(Hull& h) {
+  h.F = (int)h.start.size() - 1;
+  h.V = (int)h.vertices.size() / 3;
+  const int V = h.V;
+  std::vector<int> owner((size_t)V * V, -1);
+  h.edges.clear();
+  h.sides.clear();
+  for (int f = 0; f < h.F; f++) {
+    for (int i = h.start[f]; i < h.start[f + 1]; i++) {
+      const int u = h.loops[i], v = h.loops[i + 1 < h.start[f + 1] ? i + 1 : h.start[f]];
+      const int back = owner[(size_t)v * V + u];
+      if (back < 0) owner[(size_t)u * V + v] = f;
+      else { h.edges.push_back(v); h.edges.push_back(u); h.sides.push_back(back); h.sides.push_back(f); }
+    }
+  }
+  h.E = (int)h.edges.size() / 2;
+  const double* v = h.vertices.data();
+  for (int k = 0; k < 3; k++) { h.lo[k] = INFINITY; h.hi[k] = -INFINITY; }
+  double reach = 0;
+  for (int i = 0; i < V; i++) {
+    const double x = v[3 * i], y = v[3 * i + 1], z = v[3 * i + 2];
+    h.lo[0] = js_min(h.lo[0], x); h.lo[1] = js_min(h.lo[1], y); h.lo[2] = js_min(h.lo[2], z);
+    h.hi[0] = js_max(h.hi[0], x); h.hi[1] = js_max(h.hi[1], y); h.hi[2] = js_max(h.hi[2], z);
+    reach = js_max(reach, std::sqrt(x * x + y * y + z * z));
+  }
+  h.reach = reach;
+  const double* p = h.planes.data();
+  double size = INFINITY;
+  for (int f = 0; f < h.F; f++) {
+    const double nx = p[4 * f], ny = p[4 * f + 1], nz = p[4 * f + 2];
+    double low = INFINITY;
+    for (int i = 0; i < V; i++) low = js_min(low, nx * v[3 * i] + ny * v[3 * i + 1] + nz * v[3 * i + 2]);
+    size = js_min(size, p[4 * f + 3] - low);
+  }
+  h.size = size;
+  double inner = INFINITY;
+  for (int f = 0; f < h.F; f++) {
+    const double nx = p[4 * f], ny = p[4 * f + 1], nz = p[4 * f + 2];
+    for (int i = h.start[f]; i < h.start[f + 1]; i++) {
+      const int c = 3 * h.loops[i];
+      inner = js_min(inner, nx * v[c] + ny * v[c + 1] + nz * v[c + 2]);
+    }
+  }
+  h.inner = inner;
+}
+
+// src/collide.js:1298-1299 (CUBE): src/hull.js build() of the unit cube's corners, as the
+// reference's build() lays it out (corners in input order, faces by the order quickhull
+// finished them).
+const Hull& cube() {
+  static Hull h = [] {
+    Hull c;
+    c.vertices = {-0.5, -0.5, -0.5, 0.5, -0.5, -0.5, -0.5, 0.5, -0.5, 0.5, 0.5, -0.5,
+                  -0.5, -0.5, 0.5, 0.5, -0.5, 0.5, -0.5, 0.5, 0.5, 0.5, 0.5, 0.5};
+    c.start = {0, 4, 8, 12, 16, 20, 24};
+    c.loops = {0, 1, 5, 4, 0, 4, 6, 2, 0, 2, 3, 1, 1, 3, 7, 5, 4, 5, 7, 6, 2, 6, 7, 3};
+    c.planes = {0, -1, 0, 0.5, -1, 0, 0, 0.5, 0, 0, -1, 0.5, 1, 0, 0, 0.5, 0, 0, 1, 0.5, 0, 1, 0, 0.5};
+    derive(c);
+    return c;
+  }();
+  return h;
+}
+
+constexpr double LOOSE = 1.0 / 65536;     // 2^-16, src/collide.js:1270
+constexpr double NEARLY = 1.0 / 67108864; // 2^-26, src/collide.js:1278
+constexpr int CUT = 1 << 20;              // src/collide.js:1541
+
+// src/collide.js:1286-1295 (reaches)
+bool reaches(double a1, double a2, double c, double bound) {
+  const double hi = a1 > a2 ? a1 : a2, lo = a1 > a2 ? a2 : a1;
+  if (!(hi < bound)) return true;
+  const double s = 1 - c * c;
+  if (!(s >= NEARLY)) return true;
+  const double u = lo - c * hi;
+  return u >= 0 && !(std::sqrt(hi * hi + u * u / s) < bound);
+}
+
+// src/collide.js:1309-1388 (Placed): a hull, or a box as the cube scaled, placed in the world.
+struct Placed {
+  const Hull* hull = nullptr;
+  std::vector<double> v, planes, arcs;
+  double center[3], inner = 0, reach = 0;
+
+  void set(const Shape& s) {
+    const bool box = s.kind == BOX;
+    const Hull& h = box ? cube() : *s.hull;
+    hull = &h;
+    const int V = h.V, F = h.F, E = h.E;
+    v.assign(3 * V, 0);
+    planes.assign(4 * F, 0);
+    arcs.assign(3 * E, 0);
+    const double* A = s.axes;
+    const double* c = s.center;
+    const double* half = s.half;
+    center[0] = c[0]; center[1] = c[1]; center[2] = c[2];
+    inner = box ? js_min(js_min(half[0], half[1]), half[2]) : h.inner;
+    reach = box ? std::sqrt(half[0] * half[0] + half[1] * half[1] + half[2] * half[2]) : h.reach;
+    for (int i = 0; i < V; i++) {
+      double x = h.vertices[3 * i], y = h.vertices[3 * i + 1], z = h.vertices[3 * i + 2];
+      if (box) { x *= 2 * half[0]; y *= 2 * half[1]; z *= 2 * half[2]; }
+      v[3 * i] = c[0] + A[0] * x + A[3] * y + A[6] * z;
+      v[3 * i + 1] = c[1] + A[1] * x + A[4] * y + A[7] * z;
+      v[3 * i + 2] = c[2] + A[2] * x + A[5] * y + A[8] * z;
+    }
+    for (int f = 0; f < F; f++) {
+      const double x = h.planes[4 * f], y = h.planes[4 * f + 1], z = h.planes[4 * f + 2];
+      const double nx = A[0] * x + A[3] * y + A[6] * z, ny = A[1] * x + A[4] * y + A[7] * z, nz = A[2] * x + A[5] * y + A[8] * z;
+      const double d = box ? std::fabs(x) * half[0] + std::fabs(y) * half[1] + std::fabs(z) * half[2] : h.planes[4 * f + 3];
+      planes[4 * f] = nx; planes[4 * f + 1] = ny; planes[4 * f + 2] = nz;
+      planes[4 * f + 3] = d + nx * c[0] + ny * c[1] + nz * c[2];
+    }
+    const double* p = planes.data();
+    for (int e = 0; e < E; e++) {
+      const int l = 4 * h.sides[2 * e], r = 4 * h.sides[2 * e + 1];
+      arcs[3 * e] = p[r + 1] * p[l + 2] - p[r + 2] * p[l + 1];
+      arcs[3 * e + 1] = p[r + 2] * p[l] - p[r] * p[l + 2];
+      arcs[3 * e + 2] = p[r] * p[l + 1] - p[r + 1] * p[l];
+    }
+  }
+
+  double lowest(double nx, double ny, double nz) const {
+    double m = INFINITY;
+    for (int i = 0; i < hull->V; i++) {
+      const double d = v[3 * i] * nx + v[3 * i + 1] * ny + v[3 * i + 2] * nz;
+      if (d < m) m = d;
+    }
+    return m;
+  }
+};
+
+// The hulls shape records name by number (a hull's record holds its number in half[0]),
+// set by oracle_hulls; and the last oracle_collide's losses.
+std::vector<Hull> g_hulls;
+std::vector<double> g_loss;
+
+void resolve(Shape& s) {
+  s.hull = s.kind == HULL ? &g_hulls.at((size_t)s.half[0]) : nullptr;
+}
+
+// src/collide.js:1401-1404 (mix): a point's id from the features that made it.
+uint32_t mix(uint32_t kind, uint32_t i, uint32_t j, uint32_t k) {
+  return ((kind + 1) * 0x9e3779b1u) ^ ((i + 1) * 0x85ebca77u) ^ ((j + 1) * 0xc2b2ae3du) ^ ((k + 1) * 0x27d4eb2fu);
+}
+
+double gjk_closest(const Shape& a, const Shape& b, double* near_a, double* near_b);
+
+// src/collide.js:1751-1759 (nearest): apart, the cores' closest points by GJK, one point.
+double nearest(Scratch& S, const Shape& a, const Shape& b, double ra, double margin, Manifold& out) {
+  double na[3], nb[3];
+  const double d = gjk_closest(a, b, na, nb);
+  if (!(d > 0) || d - ra > margin) return d;
+  const double nx = (nb[0] - na[0]) / d, ny = (nb[1] - na[1]) / d, nz = (nb[2] - na[2]) / d;
+  const double separation = d - ra, t = ra + separation * 0.5;
+  set_normal(out, nx, ny, nz);
+  add_point(out, na[0] + nx * t, na[1] + ny * t, na[2] + nz * t, separation, mix(3, 0, 0, 0));
+  return d;
+}
+
+// src/collide.js:1570-1581 (named): a clipped point's id from the features of a and of b.
+uint32_t named(int f, bool flipped, int rf, int inc) {
+  int g;
+  if (f < CUT) g = flipped ? f : CUT + f;
+  else if (f < 2 * CUT) {
+    const int x = (f - CUT) >> 1, line = x >> 12, plane = x & 4095;
+    g = 2 * CUT + (flipped ? 4096 * line + plane : 4096 * plane + line);
+  } else {
+    const int x = (f - 2 * CUT) >> 1, p = x >> 12, q = x & 4095, c = q == p + 1 ? q : p;
+    g = flipped ? CUT + c : c;
+  }
+  return flipped ? mix(0, inc, rf, g) : mix(0, rf, inc, g);
+}
+
+// src/collide.js:1698-1725 (clipSides): the polygon clipped to (x, y, z).m <= offset.
+int clip_sides(const double* src, const int* sf, const int* sl, int count, double* dst, int* df, int* dl, double mx,
+               double my, double mz, double offset, int plane, double slack, double length) {
+  int n = 0;
+  for (int v = 0; v < count; v++) {
+    const int w = v + 1 < count ? v + 1 : 0;
+    const double ax = src[3 * v], ay = src[3 * v + 1], az = src[3 * v + 2];
+    const double bx = src[3 * w], by = src[3 * w + 1], bz = src[3 * w + 2];
+    const double da = ax * mx + ay * my + az * mz - offset, db = bx * mx + by * my + bz * mz - offset;
+    const int line = sl[v];
+    if (da <= slack) {
+      if (da > 0) {
+        const double t = da / (length * length);
+        dst[3 * n] = ax - mx * t; dst[3 * n + 1] = ay - my * t; dst[3 * n + 2] = az - mz * t;
+      } else {
+        dst[3 * n] = ax; dst[3 * n + 1] = ay; dst[3 * n + 2] = az;
+      }
+      df[n] = sf[v];
+      dl[n] = da > 0 && db > slack ? CUT + plane : line;
+      n++;
+    }
+    if ((da <= 0 && db > slack) || (da > slack && db <= 0)) {
+      const double t = da / (da - db);
+      dst[3 * n] = ax + (bx - ax) * t; dst[3 * n + 1] = ay + (by - ay) * t; dst[3 * n + 2] = az + (bz - az) * t;
+      df[n] = line < CUT ? CUT + 2 * (line * 4096 + plane)
+                         : 2 * CUT + 2 * ((line - CUT < plane ? line - CUT : plane) * 4096 + (line - CUT > plane ? line - CUT : plane));
+      dl[n] = da <= 0 ? CUT + plane : line;
+      n++;
+    }
+  }
+  return n;
+}
+
+// src/collide.js:1592-1689 (hullFace): face rf of R as the reference against I's face most
+// nearly opposite it, clipped; whether it added points.
+bool hull_face(Scratch& S, const Placed& R, int rf, const Placed& I, bool flipped, double margin, double tolerance,
+               Manifold& out, bool sure) {
+  const double* rp = R.planes.data();
+  const double* ip = I.planes.data();
+  const Hull& rh = *R.hull;
+  const Hull& ih = *I.hull;
+  const double nx = rp[4 * rf], ny = rp[4 * rf + 1], nz = rp[4 * rf + 2], d = rp[4 * rf + 3];
+  const double* iv = I.v.data();
+  int low = 0;
+  double lowest = INFINITY;
+  for (int i = 0; i < ih.V; i++) {
+    const double dot = iv[3 * i] * nx + iv[3 * i + 1] * ny + iv[3 * i + 2] * nz;
+    if (dot < lowest) { lowest = dot; low = i; }
+  }
+  int inc = 0;
+  double least = INFINITY;
+  for (int f = 0; f < ih.F; f++) {
+    const double dot = ip[4 * f] * nx + ip[4 * f + 1] * ny + ip[4 * f + 2] * nz;
+    if (!(dot < least)) continue;
+    bool at = false;
+    for (int j = ih.start[f]; j < ih.start[f + 1] && !at; j++) at = ih.loops[j] == low;
+    if (at) { least = dot; inc = f; }
+  }
+  const int i0 = ih.start[inc], m = ih.start[inc + 1] - i0, r0 = rh.start[rf], k = rh.start[rf + 1] - r0;
+  const int cap = m + k + 1;
+  std::vector<double> clip_p(3 * cap), clip_q(3 * cap), kept_p(3 * cap), kept_s(cap);
+  std::vector<int> feat_p(cap), feat_q(cap), line_p(cap), line_q(cap);
+  std::vector<uint32_t> kept_i(cap);
+  const double* rv = R.v.data();
+  for (int j = 0; j < m; j++) {
+    const int v = 3 * ih.loops[i0 + j];
+    clip_p[3 * j] = iv[v]; clip_p[3 * j + 1] = iv[v + 1]; clip_p[3 * j + 2] = iv[v + 2];
+    feat_p[j] = j; line_p[j] = j;
+  }
+  int count = m, src = 0;
+  for (int j = 0; j < k && count > 0; j++) {
+    const int p = 3 * rh.loops[r0 + j], q = 3 * rh.loops[r0 + (j + 1 < k ? j + 1 : 0)];
+    const double ex = rv[q] - rv[p], ey = rv[q + 1] - rv[p + 1], ez = rv[q + 2] - rv[p + 2];
+    const double mx = ey * nz - ez * ny, my = ez * nx - ex * nz, mz = ex * ny - ey * nx;
+    const double offset = mx * rv[p] + my * rv[p + 1] + mz * rv[p + 2];
+    const double length = std::sqrt(mx * mx + my * my + mz * mz);
+    count = src == 0 ? clip_sides(clip_p.data(), feat_p.data(), line_p.data(), count, clip_q.data(), feat_q.data(),
+                                  line_q.data(), mx, my, mz, offset, j, tolerance * length, length)
+                     : clip_sides(clip_q.data(), feat_q.data(), line_q.data(), count, clip_p.data(), feat_p.data(),
+                                  line_p.data(), mx, my, mz, offset, j, tolerance * length, length);
+    src = 1 - src;
+  }
+  const double* poly = src == 0 ? clip_p.data() : clip_q.data();
+  const int* feat = src == 0 ? feat_p.data() : feat_q.data();
+  int n = 0;
+  for (int j = 0; j < count; j++) {
+    const double x = poly[3 * j], y = poly[3 * j + 1], z = poly[3 * j + 2];
+    const double separation = x * nx + y * ny + z * nz - d;
+    if (separation > margin) continue;
+    const double h = separation * 0.5;
+    kept_p[3 * n] = x - nx * h; kept_p[3 * n + 1] = y - ny * h; kept_p[3 * n + 2] = z - nz * h;
+    kept_s[n] = separation;
+    kept_i[n] = named(feat[j], flipped, rf, inc);
+    n++;
+  }
+  const double s = lowest - d;
+  if (s <= tolerance) {
+    double deepest = INFINITY;
+    for (int j = 0; j < n; j++)
+      if (kept_s[j] < deepest) deepest = kept_s[j];
+    if (deepest > s + tolerance) {
+      if (!sure) return false;
+      int j = 0;
+      while (ih.loops[i0 + j] != low) j++;
+      const int v = 3 * low;
+      const double h = s * 0.5;
+      kept_p[3 * n] = iv[v] - nx * h; kept_p[3 * n + 1] = iv[v + 1] - ny * h; kept_p[3 * n + 2] = iv[v + 2] - nz * h;
+      kept_s[n] = s;
+      kept_i[n] = named(j, flipped, rf, inc);
+      n++;
+    }
+  }
+  if (n == 0) return false;
+  if (flipped) set_normal(out, -nx, -ny, -nz);
+  else set_normal(out, nx, ny, nz);
+  reduce_list(S, kept_p.data(), kept_s.data(), kept_i.data(), n, nx, ny, nz, tolerance, out);
+  if (s <= tolerance) {
+    double kept = INFINITY;
+    for (int j = 0; j < out.count; j++)
+      if (out.separations[j] < kept) kept = out.separations[j];
+    if (kept - s > out.loss) out.loss = kept - s;
+  }
+  return true;
+}
+
+// src/collide.js:1728-1744 (hullEdges): edge i of A against edge j of B, the one point.
+void hull_edges(Scratch& S, const Placed& A, int i, const Placed& B, int j, double nx, double ny, double nz,
+                double margin, Manifold& out) {
+  const int u = 3 * A.hull->edges[2 * i], w = 3 * A.hull->edges[2 * i + 1];
+  const int p = 3 * B.hull->edges[2 * j], q = 3 * B.hull->edges[2 * j + 1];
+  double* sa = S.seg_a;
+  double* sb = S.seg_b;
+  for (int k = 0; k < 3; k++) {
+    sa[k] = A.v[u + k]; sa[3 + k] = A.v[w + k];
+    sb[k] = B.v[p + k]; sb[3 + k] = B.v[q + k];
+  }
+  closest_segments(S, sa, sb);
+  const double ss = S.seg_s, tt = S.seg_t;
+  const double pax = sa[0] + (sa[3] - sa[0]) * ss, pay = sa[1] + (sa[4] - sa[1]) * ss, paz = sa[2] + (sa[5] - sa[2]) * ss;
+  const double pbx = sb[0] + (sb[3] - sb[0]) * tt, pby = sb[1] + (sb[4] - sb[1]) * tt, pbz = sb[2] + (sb[5] - sb[2]) * tt;
+  const double separation = (pbx - pax) * nx + (pby - pay) * ny + (pbz - paz) * nz;
+  if (separation > margin) return;
+  if (separation > 0 && (ss == 0 || ss == 1 || tt == 0 || tt == 1)) return;
+  set_normal(out, nx, ny, nz);
+  add_point(out, (pax + pbx) * 0.5, (pay + pby) * 0.5, (paz + pbz) * 0.5, separation, mix(2, i, j, 0));
+}
+
+// src/collide.js:1406-1536 (hullHull): a hull, or a box as the cube, against a hull.
+void hull_hull(Scratch& S, const Shape& a, const Shape& b, double margin, double tolerance, Manifold& out) {
+  Placed A, B;
+  A.set(a);
+  B.set(b);
+  const Hull& ha = *A.hull;
+  const Hull& hb = *B.hull;
+  const double* pa = A.planes.data();
+  const double* pb = B.planes.data();
+  const double* ca = A.center;
+  const double* cb = B.center;
+  const int FA = ha.F, FB = hb.F;
+  const double loose = LOOSE * (std::fabs(ca[0]) + std::fabs(ca[1]) + std::fabs(ca[2]) + std::fabs(cb[0]) + std::fabs(cb[1])
+                                + std::fabs(cb[2]) + A.reach + B.reach);
+  const double ra = A.inner > loose ? A.inner - loose : -INFINITY, rb = B.inner > loose ? B.inner - loose : -INFINITY;
+  std::vector<double> above_a(FA), above_b(FB);
+  int top = 0;
+  for (int f = 0; f < FA; f++) {
+    above_a[f] = pa[4 * f] * cb[0] + pa[4 * f + 1] * cb[1] + pa[4 * f + 2] * cb[2] - pa[4 * f + 3];
+    if (above_a[f] > above_a[top]) top = f;
+  }
+  const double first_a = B.lowest(pa[4 * top], pa[4 * top + 1], pa[4 * top + 2]) - pa[4 * top + 3];
+  if (first_a > margin) return;
+  double face_a = -INFINITY;
+  int fa = 0;
+  for (int f = 0; f < FA; f++) {
+    double sep = first_a;
+    if (f != top) {
+      if (above_a[f] - rb < js_max(face_a, first_a)) continue;
+      sep = B.lowest(pa[4 * f], pa[4 * f + 1], pa[4 * f + 2]) - pa[4 * f + 3];
+      if (sep > margin) return;
+    }
+    if (sep > face_a) { face_a = sep; fa = f; }
+  }
+  top = 0;
+  for (int f = 0; f < FB; f++) {
+    above_b[f] = pb[4 * f] * ca[0] + pb[4 * f + 1] * ca[1] + pb[4 * f + 2] * ca[2] - pb[4 * f + 3];
+    if (above_b[f] > above_b[top]) top = f;
+  }
+  const double first_b = A.lowest(pb[4 * top], pb[4 * top + 1], pb[4 * top + 2]) - pb[4 * top + 3];
+  if (first_b > margin) return;
+  double face_b = -INFINITY;
+  int fb = 0;
+  for (int f = 0; f < FB; f++) {
+    double sep = first_b;
+    if (f != top) {
+      if (above_b[f] - ra < js_max(js_max(face_b, first_b), face_a)) continue;
+      sep = A.lowest(pb[4 * f], pb[4 * f + 1], pb[4 * f + 2]) - pb[4 * f + 3];
+      if (sep > margin) return;
+    }
+    if (sep > face_b) { face_b = sep; fb = f; }
+  }
+  const int* ea = ha.edges.data();
+  const int* sa = ha.sides.data();
+  const int* eb = hb.edges.data();
+  const int* sb = hb.sides.data();
+  const double* va = A.v.data();
+  const double* vb = B.v.data();
+  const double* arc_a = A.arcs.data();
+  const double* arc_b = B.arcs.data();
+  const int EA = ha.E, EB = hb.E;
+  std::vector<double> side(FB);
+  const double best = js_max(face_a, face_b);
+  std::vector<uint8_t> reach_b(EB);
+  int reachable = 0;
+  for (int j = 0; j < EB; j++) {
+    const int l = 4 * sb[2 * j], r = 4 * sb[2 * j + 1], p = 3 * eb[2 * j];
+    const double dx = ca[0] - vb[p], dy = ca[1] - vb[p + 1], dz = ca[2] - vb[p + 2];
+    const bool can = reaches(pb[l] * dx + pb[l + 1] * dy + pb[l + 2] * dz, pb[r] * dx + pb[r + 1] * dy + pb[r + 2] * dz,
+                             pb[l] * pb[r] + pb[l + 1] * pb[r + 1] + pb[l + 2] * pb[r + 2], best + ra);
+    reach_b[j] = can ? 1 : 0;
+    if (can) reachable++;
+  }
+  double edge_sep = -INFINITY, enx = 0, eny = 0, enz = 0;
+  int ei = -1, ej = -1;
+  for (int i = 0; i < EA && reachable > 0; i++) {
+    const int l = 4 * sa[2 * i], r = 4 * sa[2 * i + 1];
+    const int u = 3 * ea[2 * i], w = 3 * ea[2 * i + 1];
+    const double dx = cb[0] - va[u], dy = cb[1] - va[u + 1], dz = cb[2] - va[u + 2];
+    if (!reaches(pa[l] * dx + pa[l + 1] * dy + pa[l + 2] * dz, pa[r] * dx + pa[r + 1] * dy + pa[r + 2] * dz,
+                 pa[l] * pa[r] + pa[l + 1] * pa[r + 1] + pa[l + 2] * pa[r + 2], best + rb))
+      continue;
+    const double ax = arc_a[3 * i], ay = arc_a[3 * i + 1], az = arc_a[3 * i + 2];
+    const double ux = va[w] - va[u], uy = va[w + 1] - va[u + 1], uz = va[w + 2] - va[u + 2];
+    const double ul = std::sqrt(ux * ux + uy * uy + uz * uz);
+    for (int g = 0; g < FB; g++) side[g] = pb[4 * g] * ax + pb[4 * g + 1] * ay + pb[4 * g + 2] * az;
+    for (int j = 0; j < EB; j++) {
+      if (reach_b[j] == 0) continue;
+      const double cba = side[sb[2 * j]], dba = side[sb[2 * j + 1]];
+      if (!(cba * dba < 0)) continue;
+      const double bx = arc_b[3 * j], by = arc_b[3 * j + 1], bz = arc_b[3 * j + 2];
+      const double adc = pa[l] * bx + pa[l + 1] * by + pa[l + 2] * bz, bdc = pa[r] * bx + pa[r + 1] * by + pa[r + 2] * bz;
+      if (!(adc * bdc < 0 && cba * bdc < 0)) continue;
+      const int p = 3 * eb[2 * j], q = 3 * eb[2 * j + 1];
+      const double tx = vb[q] - vb[p], ty = vb[q + 1] - vb[p + 1], tz = vb[q + 2] - vb[p + 2];
+      const double len = normalize(S, uy * tz - uz * ty, uz * tx - ux * tz, ux * ty - uy * tx);
+      if (!(len > PARALLEL * ul * std::sqrt(tx * tx + ty * ty + tz * tz))) continue;
+      double nx = S.unit[0], ny = S.unit[1], nz = S.unit[2];
+      if (nx * (va[u] - ca[0]) + ny * (va[u + 1] - ca[1]) + nz * (va[u + 2] - ca[2]) < 0) { nx = -nx; ny = -ny; nz = -nz; }
+      const double sep = nx * (vb[p] - va[u]) + ny * (vb[p + 1] - va[u + 1]) + nz * (vb[p + 2] - va[u + 2]);
+      if (sep > margin) return;
+      if (sep > edge_sep) { edge_sep = sep; ei = i; ej = j; enx = nx; eny = ny; enz = nz; }
+    }
+  }
+  const bool edge_best = ei >= 0 && edge_sep > face_a && edge_sep > face_b, b_best = !edge_best && face_b > face_a;
+  if (ei >= 0 && edge_sep > js_max(face_a, face_b) + tolerance) {
+    hull_edges(S, A, ei, B, ej, enx, eny, enz, margin, out);
+  } else {
+    const bool on_b = face_b > face_a + tolerance, sure = !edge_best && on_b == b_best;
+    if (hull_face(S, on_b ? B : A, on_b ? fb : fa, on_b ? A : B, on_b, margin, tolerance, out, sure)) {
+      const double behind = js_max(js_max(face_a, face_b), edge_sep) - (on_b ? face_b : face_a);
+      if (behind > out.loss) out.loss = behind;
+    } else if (!sure) {
+      if (edge_best) hull_edges(S, A, ei, B, ej, enx, eny, enz, margin, out);
+      else hull_face(S, b_best ? B : A, b_best ? fb : fa, b_best ? A : B, b_best, margin, tolerance, out, true);
+    }
+  }
+  if (out.count == 0 && js_max(js_max(face_a, face_b), edge_sep) > 0) nearest(S, a, b, 0, margin, out);
+}
+
+// src/collide.js:1766-1781 (sphereHull)
+void sphere_hull(Scratch& S, const Shape& a, const Shape& b, double margin, Manifold& out) {
+  Placed B;
+  B.set(b);
+  const double* p = B.planes.data();
+  const double* c = a.center;
+  const double r = a.half[0];
+  double most = -INFINITY;
+  int f = 0;
+  for (int k = 0; k < B.hull->F; k++) {
+    const double sep = p[4 * k] * c[0] + p[4 * k + 1] * c[1] + p[4 * k + 2] * c[2] - p[4 * k + 3];
+    if (sep > most) { most = sep; f = k; }
+  }
+  if (most - r > margin) return;
+  if (most > 0 && nearest(S, a, b, r, margin, out) > 0) return;
+  const double nx = -p[4 * f], ny = -p[4 * f + 1], nz = -p[4 * f + 2];
+  const double separation = most - r, t = r + separation * 0.5;
+  set_normal(out, nx, ny, nz);
+  add_point(out, c[0] + nx * t, c[1] + ny * t, c[2] + nz * t, separation, mix(4, f, 0, 0));
+}
+
+// src/collide.js:1855-1882 (clipToHullFace): the capsule's core (S.seg_a) clipped to face f's sides.
+bool clip_to_hull_face(Scratch& S, const Placed& B, int f, double r, double margin, Manifold& out) {
+  const double* p = B.planes.data();
+  const Hull& h = *B.hull;
+  const double* v = B.v.data();
+  const double* sa = S.seg_a;
+  const double nx = p[4 * f], ny = p[4 * f + 1], nz = p[4 * f + 2];
+  const int i0 = h.start[f], k = h.start[f + 1] - i0;
+  double u0 = 0, u1 = 1;
+  for (int j = 0; j < k; j++) {
+    const int sv = 3 * h.loops[i0 + j], q = 3 * h.loops[i0 + (j + 1 < k ? j + 1 : 0)];
+    const double ex = v[q] - v[sv], ey = v[q + 1] - v[sv + 1], ez = v[q + 2] - v[sv + 2];
+    const double mx = ey * nz - ez * ny, my = ez * nx - ex * nz, mz = ex * ny - ey * nx;
+    const double offset = mx * v[sv] + my * v[sv + 1] + mz * v[sv + 2];
+    const double d0 = sa[0] * mx + sa[1] * my + sa[2] * mz - offset, d1 = sa[3] * mx + sa[4] * my + sa[5] * mz - offset;
+    if (d0 > 0 && d1 > 0) return false;
+    if (d0 > 0) u0 = js_max(u0, d0 / (d0 - d1));
+    else if (d1 > 0) u1 = js_min(u1, d0 / (d0 - d1));
+  }
+  if (u0 > u1) return false;
+  set_normal(out, -nx, -ny, -nz);
+  for (int e = 0; e < (u1 > u0 ? 2 : 1); e++) {
+    const double u = e == 0 ? u0 : u1;
+    const double x = sa[0] + (sa[3] - sa[0]) * u, y = sa[1] + (sa[4] - sa[1]) * u, z = sa[2] + (sa[5] - sa[2]) * u;
+    const double above = x * nx + y * ny + z * nz - p[4 * f + 3], separation = above - r;
+    if (separation > margin) continue;
+    const double t = (above + r) * 0.5;
+    add_point(out, x - nx * t, y - ny * t, z - nz * t, separation, mix(6, f, e, 0));
+  }
+  return out.count > 0;
+}
+
+// src/collide.js:1790-1847 (capsuleHull)
+void capsule_hull(Scratch& S, const Shape& a, const Shape& b, double margin, double tolerance, Manifold& out) {
+  Placed B;
+  B.set(b);
+  const double* p = B.planes.data();
+  const Hull& h = *B.hull;
+  const int F = h.F;
+  const double r = a.half[0];
+  double* sa = S.seg_a;
+  double* sb = S.seg_b;
+  segment(a, sa);
+  double na[3], nb[3];
+  const double d = gjk_closest(a, b, na, nb);
+  if (d > 0) {
+    if (d - r > margin) return;
+    const double nx = (nb[0] - na[0]) / d, ny = (nb[1] - na[1]) / d, nz = (nb[2] - na[2]) / d;
+    int f = 0;
+    double most = -INFINITY;
+    for (int k = 0; k < F; k++) {
+      const double dot = -(p[4 * k] * nx + p[4 * k + 1] * ny + p[4 * k + 2] * nz);
+      if (dot > most) { most = dot; f = k; }
+    }
+    if (1 - most <= PARALLEL && clip_to_hull_face(S, B, f, r, margin, out)) return;
+    nearest(S, a, b, r, margin, out);
+    return;
+  }
+  const double dx = sa[3] - sa[0], dy = sa[4] - sa[1], dz = sa[5] - sa[2];
+  const double dl = std::sqrt(dx * dx + dy * dy + dz * dz);
+  double best_sep = -INFINITY, nx = 0, ny = 0, nz = 0;
+  int best_face = -1, edge_i = -1;
+  for (int k = 0; k < F; k++) {
+    const double mx = p[4 * k], my = p[4 * k + 1], mz = p[4 * k + 2];
+    const double sep = js_min(sa[0] * mx + sa[1] * my + sa[2] * mz, sa[3] * mx + sa[4] * my + sa[5] * mz) - p[4 * k + 3] - r;
+    if (sep > best_sep) { best_sep = sep; best_face = k; }
+  }
+  const int* e = h.edges.data();
+  const double* v = B.v.data();
+  for (int i = 0; i < h.E; i++) {
+    const int u = 3 * e[2 * i], w = 3 * e[2 * i + 1];
+    const double tx = v[w] - v[u], ty = v[w + 1] - v[u + 1], tz = v[w + 2] - v[u + 2];
+    const double l = normalize(S, dy * tz - dz * ty, dz * tx - dx * tz, dx * ty - dy * tx);
+    if (!(l > PARALLEL * dl * std::sqrt(tx * tx + ty * ty + tz * tz))) continue;
+    double mx = S.unit[0], my = S.unit[1], mz = S.unit[2];
+    if (mx * (v[u] - B.center[0]) + my * (v[u + 1] - B.center[1]) + mz * (v[u + 2] - B.center[2]) < 0) { mx = -mx; my = -my; mz = -mz; }
+    const double top = -B.lowest(-mx, -my, -mz);
+    if (top - (mx * v[u] + my * v[u + 1] + mz * v[u + 2]) > tolerance) continue;
+    const double sep = js_min(sa[0] * mx + sa[1] * my + sa[2] * mz, sa[3] * mx + sa[4] * my + sa[5] * mz) - r - top;
+    if (sep > best_sep + tolerance) { best_sep = sep; edge_i = i; nx = mx; ny = my; nz = mz; }
+  }
+  if (best_sep > margin) return;
+  if (edge_i < 0) {
+    clip_to_hull_face(S, B, best_face, r, margin, out);
+    return;
+  }
+  const int u = 3 * e[2 * edge_i], w = 3 * e[2 * edge_i + 1];
+  for (int k = 0; k < 3; k++) { sb[k] = v[u + k]; sb[3 + k] = v[w + k]; }
+  closest_segments(S, sa, sb);
+  const double px = sa[0] + dx * S.seg_s, py = sa[1] + dy * S.seg_s, pz = sa[2] + dz * S.seg_s;
+  const double qx = sb[0] + (sb[3] - sb[0]) * S.seg_t, qy = sb[1] + (sb[4] - sb[1]) * S.seg_t, qz = sb[2] + (sb[5] - sb[2]) * S.seg_t;
+  const double separation = (px - qx) * nx + (py - qy) * ny + (pz - qz) * nz - r;
+  set_normal(out, -nx, -ny, -nz);
+  const double t = r + separation * 0.5;
+  add_point(out, px - nx * t, py - ny * t, pz - nz * t, separation, mix(5, edge_i, 0, 0));
+}
+
 // src/collide.js:204-242 (collide, pair), for spheres and boxes.
 int collide(const Shape& a, const Shape& b, double margin, double tolerance, Manifold& out) {
   Scratch S;
@@ -821,6 +1404,9 @@ int collide(const Shape& a, const Shape& b, double margin, double tolerance, Man
   else if (lo.kind == CAPSULE && hi.kind == CAPSULE) capsule_capsule(S, lo, hi, margin, tolerance, out);
   else if (lo.kind == CAPSULE && hi.kind == BOX) capsule_box(S, lo, hi, margin, tolerance, out);
   else if (lo.kind == BOX && hi.kind == BOX) box_box(S, lo, hi, margin, tolerance, out);
+  else if (lo.kind == SPHERE && hi.kind == HULL) sphere_hull(S, lo, hi, margin, out);
+  else if (lo.kind == CAPSULE && hi.kind == HULL) capsule_hull(S, lo, hi, margin, tolerance, out);
+  else if ((lo.kind == BOX || lo.kind == HULL) && hi.kind == HULL) hull_hull(S, lo, hi, margin, tolerance, out);
   else return -1;
   if (swap) { out.normal[0] = -out.normal[0]; out.normal[1] = -out.normal[1]; out.normal[2] = -out.normal[2]; }
   return out.count;
@@ -844,6 +1430,20 @@ void gjk_support(const Shape& s, double x, double y, double z, double* out) {
   const double* h = s.half;
   if (s.kind == SPHERE) {
     out[0] = c[0]; out[1] = c[1]; out[2] = c[2];
+  } else if (s.kind == HULL) {
+    const std::vector<double>& v = s.hull->vertices;
+    const int V = s.hull->V;
+    const double lx = x * a[0] + y * a[1] + z * a[2], ly = x * a[3] + y * a[4] + z * a[5], lz = x * a[6] + y * a[7] + z * a[8];
+    int best = 0;
+    double most = -INFINITY;
+    for (int i = 0; i < V; i++) {
+      const double d = v[3 * i] * lx + v[3 * i + 1] * ly + v[3 * i + 2] * lz;
+      if (d > most) { most = d; best = i; }
+    }
+    const double px = v[3 * best], py = v[3 * best + 1], pz = v[3 * best + 2];
+    out[0] = c[0] + px * a[0] + py * a[3] + pz * a[6];
+    out[1] = c[1] + px * a[1] + py * a[4] + pz * a[7];
+    out[2] = c[2] + px * a[2] + py * a[5] + pz * a[8];
   } else if (s.kind == CAPSULE) {
     const double e = x * a[3] + y * a[4] + z * a[5] >= 0 ? h[1] : -h[1];
     out[0] = c[0] + e * a[3]; out[1] = c[1] + e * a[4]; out[2] = c[2] + e * a[5];
@@ -1198,6 +1798,7 @@ void read_shape(const double* r, Shape& s) {
   for (int i = 0; i < 9; i++) s.axes[i] = r[4 + i];
   for (int i = 0; i < 3; i++) s.half[i] = r[13 + i];
   for (int i = 0; i < 3; i++) s.weight[i] = r[16 + i];
+  resolve(s);
 }
 
 // A body record for meet: the shape record (19), then q 4, v 3, w 3, gravity scale
@@ -1234,9 +1835,12 @@ int oracle_collide(int n, const double* a, const double* b, const double* margin
       for (int i = 0; i < 9; i++) sh[t]->axes[i] = r[4 + i];
       for (int i = 0; i < 3; i++) sh[t]->half[i] = r[13 + i];
       for (int i = 0; i < 3; i++) sh[t]->weight[i] = r[16 + i];
+      resolve(*sh[t]);
     }
     Manifold m{};
+    if (k == 0) g_loss.assign(n, 0);
     if (collide(sa, sb, margin[k], tolerance[k], m) < 0) return -(k + 1);
+    g_loss[k] = m.loss;
     count[k] = m.count;
     for (int i = 0; i < 3; i++) normal[3 * k + i] = m.normal[i];
     for (int i = 0; i < 12; i++) points[12 * k + i] = m.points[i];
@@ -1244,6 +1848,29 @@ int oracle_collide(int n, const double* a, const double* b, const double* margin
     for (int i = 0; i < 4; i++) ids[4 * k + i] = m.ids[i];
   }
   return n;
+}
+
+/* The hull table: n hulls, hull h's corners [hull_v[h], hull_v[h+1]) of `vertices` (3
+   each), its faces [hull_f[h], hull_f[h+1]) of `planes` (4 each) with their loop starts in
+   `start` (one a face, counted from the hull's first loop entry), its loops
+   [hull_l[h], hull_l[h+1]) of `loops` (corner numbers of the hull). */
+void oracle_hulls(int n, const int32_t* hull_v, const int32_t* hull_f, const int32_t* hull_l, const double* vertices,
+                  const int32_t* start, const int32_t* loops, const double* planes) {
+  g_hulls.assign(n, Hull());
+  for (int h = 0; h < n; h++) {
+    Hull& x = g_hulls[h];
+    x.vertices.assign(vertices + 3 * hull_v[h], vertices + 3 * hull_v[h + 1]);
+    x.planes.assign(planes + 4 * hull_f[h], planes + 4 * hull_f[h + 1]);
+    x.start.assign(start + hull_f[h], start + hull_f[h + 1]);
+    x.start.push_back(hull_l[h + 1] - hull_l[h]);
+    x.loops.assign(loops + hull_l[h], loops + hull_l[h + 1]);
+    derive(x);
+  }
+}
+
+/* The last oracle_collide's losses (src/collide.js Manifold.loss), one a pair. */
+void oracle_losses(double* out) {
+  for (size_t k = 0; k < g_loss.size(); k++) out[k] = g_loss[k];
 }
 
 /* src/gjk.js:114-155 on whole shapes (19-number records): distance and nearest points. */

@@ -213,6 +213,64 @@ def capsule_pairs(seed, n):
     return np.asarray(recs, np.float64)
 
 
+def hull_pairs(seed, n):
+    """Seeded pairs with a hull (16 points N(0, 0.3^2) each, as the mixed pile's rocks):
+    hull on hull, box on hull, sphere on hull and capsule on hull, either order, apart within
+    and beyond the margin, touching and overlapping, at any turn. JSON-ready dicts."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+
+    def quat():
+        if rng.random() < 0.2:
+            return [0.0, 0.0, 0.0, 1.0]
+        v = rng.normal(size=4)
+        return list(v / np.linalg.norm(v))
+
+    def weight():
+        return [0.0, -9.81, 0.0] if rng.random() < 0.7 else [0.0, 0.0, 0.0]
+
+    def shape(kind):
+        if kind == 3:
+            pts = rng.normal(0, 0.3, (16, 3))
+            return {"kind": 3, "points": [float(x) for x in pts.reshape(-1)], "reach": float(np.linalg.norm(pts, axis=1).max())}
+        if kind == 2:
+            h = rng.uniform(0.1, 1.0, 3)
+            return {"kind": 2, "half": [float(x) for x in h], "reach": float(np.linalg.norm(h))}
+        if kind == 1:
+            r, hl = rng.uniform(0.05, 0.3), rng.uniform(0.0, 0.6)
+            return {"kind": 1, "half": [r, hl, 0.0], "reach": r + hl}
+        r = rng.uniform(0.05, 0.5)
+        return {"kind": 0, "half": [r, 0.0, 0.0], "reach": r}
+
+    pairs = []
+    kinds = [(3, 3), (3, 3), (2, 3), (3, 2), (0, 3), (3, 0), (1, 3), (3, 1)]
+    for k in range(n):
+        ka, kb = kinds[k % len(kinds)]
+        a, b = shape(ka), shape(kb)
+        ca = rng.uniform(-3, 3, 3)
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        cb = ca + d * (a["reach"] + b["reach"]) * rng.uniform(0.2, 1.0)
+        for s, c in ((a, ca), (b, cb)):
+            s["center"] = [float(x) for x in c]
+            s["q"] = quat() if s["kind"] != 0 else [0.0, 0.0, 0.0, 1.0]
+            s["weight"] = weight()
+            s.pop("reach")
+        margin = float(rng.choice([0.0, rng.uniform(0, 0.05), rng.uniform(0, 0.3)]))
+        pairs.append({"a": a, "b": b, "margin": margin})
+    return pairs
+
+
+def run_hulls(seed=2026, n=1200):
+    path = os.path.join(DATA, "hulls.in.json")
+    with open(path, "w") as f:
+        json.dump(hull_pairs(seed, n), f)
+    subprocess.run(["node", "--no-warnings", "--experimental-loader", os.path.join(HERE, "lower.mjs"),
+                    os.path.join(HERE, "make_hulls.mjs"), path, os.path.join(DATA, f"collide_hull.{seed}.bmfx.gz")],
+                   check=True)
+    os.remove(path)
+
+
 def run_collide(seed=2026, n=4000, capsules=False):
     pairs = capsule_pairs(seed, n) if capsules else collide_pairs(seed, n)
     name = "collide_capsule" if capsules else "collide"
@@ -296,6 +354,9 @@ def run_impact(seed=2026, n=2000):
 if __name__ == "__main__":
     if sys.argv[1:] == ["collide"]:
         run_collide()
+        sys.exit(0)
+    if sys.argv[1:] == ["collide_hulls"]:
+        run_hulls()
         sys.exit(0)
     if sys.argv[1:] == ["collide_capsules"]:
         run_collide(n=3000, capsules=True)

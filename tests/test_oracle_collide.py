@@ -182,3 +182,33 @@ def test_hand_worked_capsule_box():
     n, normal, pts = run(shape("capsule", radius=0.3, height=6, position=(0, 0.3, 0), rotation=lying), box)
     assert np.allclose(normal, [0, -1, 0], atol=1e-9)
     same_points(pts, [([-2, 0.25, 0], -0.5), ([2, 0.25, 0], -0.5)])
+
+
+def test_hulls_match_reference_bit_for_bit():
+    # 1,200 pairs with a hull built by the reference's own src/hull.js build(): hull on hull,
+    # box on hull (the box as the reference's CUBE), sphere and capsule on hull, either order;
+    # manifolds and Manifold.loss (what REUSE keeps a hull manifold by) bit for bit.
+    path = fixtures.DATA + "/collide_hull.2026.bmfx.gz"
+    fx = fixtures.load(path)
+    binding.hulls(fx)
+    sh = fx["shapes"].reshape(-1, 2, 19)
+    got = binding.collide(sh[:, 0].copy(), sh[:, 1].copy(), fx["margin"], fx["tolerance"])
+    errs = compare_manifolds(fx, *got)
+    assert not errs, "\n".join(errs)
+    loss = binding.losses(len(fx["margin"]))
+    assert np.array_equal(loss.view(np.uint64), np.asarray(fx["loss"]).view(np.uint64))
+    assert (np.asarray(fx["loss"]) > 0).sum() > 10 and int(fx["count"].max()) == 4
+    kinds = set(map(tuple, sh[:, :, 0].astype(int)))
+    assert {(3, 3), (2, 3), (3, 2), (0, 3), (3, 0), (1, 3), (3, 1)} <= kinds
+
+
+def test_cube_hull_is_the_references():
+    # The cube the oracle and the device place a box as against a hull is hull 0 of the
+    # fixture, the reference's build() of the unit cube's corners.
+    fx = fixtures.load(fixtures.DATA + "/collide_hull.2026.bmfx.gz")
+    assert fx["vertices"][:24].tolist() == [-0.5, -0.5, -0.5, 0.5, -0.5, -0.5, -0.5, 0.5, -0.5, 0.5, 0.5, -0.5,
+                                             -0.5, -0.5, 0.5, 0.5, -0.5, 0.5, -0.5, 0.5, 0.5, 0.5, 0.5, 0.5]
+    assert fx["start"][:6].tolist() == [0, 4, 8, 12, 16, 20] and fx["hullL"][1] == 24
+    assert fx["loops"][:24].tolist() == [0, 1, 5, 4, 0, 4, 6, 2, 0, 2, 3, 1, 1, 3, 7, 5, 4, 5, 7, 6, 2, 6, 7, 3]
+    assert fx["planes"][:24].tolist() == [0, -1, 0, 0.5, -1, 0, 0, 0.5, 0, 0, -1, 0.5, 1, 0, 0, 0.5, 0, 0, 1, 0.5,
+                                           0, 1, 0, 0.5]
