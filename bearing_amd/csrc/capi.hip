@@ -2,6 +2,7 @@
 // columns, and the soft-step launch.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstddef>
 #include <cstdio>
 #include <cstring>
@@ -123,6 +124,11 @@ struct bm_context {
   bool laid_out = false;    // rows are laid out by colour (checked at upload, or by bm_colour)
   Scratch deg, adj, keys, key_offset, world_of;
   Scratch pairs_in, pairs_out;
+  // The hull table (bm_set_hulls): ints (info, start, loops, edges, sides) and doubles
+  // (dinfo, vertices, planes) on the device, and where each part starts.
+  Scratch hull_ints, hull_doubles;
+  bm::HullTable hulls{};
+  int32_t hull_count = 0;
   std::vector<int32_t> key_offsets;
 };
 
@@ -261,6 +267,52 @@ struct ArenaOrder {
   }
 };
 
+// Whether a shape record (kind first, a hull's number at 13) names a hull the table holds.
+bool hull_ok(const bm_context* ctx, const double* r) {
+  if (r[0] != 3) return true;
+  const double h = r[13];
+  return h >= 0 && h < ctx->hull_count && h == (double)(int32_t)h;
+}
+
+inline double host_min(double a, double b) {
+  if (a < b) return a;
+  if (b < a) return b;
+  return std::signbit(a) ? a : b;
+}
+inline double host_max(double a, double b) {
+  if (a > b) return a;
+  if (b > a) return b;
+  return std::signbit(a) ? b : a;
+}
+
+// src/hull.js:45-80 (the Hull constructor): each edge once, from the first face to reach
+// it, with its two faces; the inscribed ball (inner) and the reach.
+void derive_hull(int V, int F, const int32_t* start, const int32_t* loops, const double* v, const double* p,
+                 std::vector<int32_t>& edges, std::vector<int32_t>& sides, double& inner, double& reach) {
+  std::vector<int32_t> owner((size_t)V * V, -1);
+  for (int f = 0; f < F; f++) {
+    for (int i = start[f]; i < start[f + 1]; i++) {
+      const int u = loops[i], w = loops[i + 1 < start[f + 1] ? i + 1 : start[f]];
+      const int back = owner[(size_t)w * V + u];
+      if (back < 0) owner[(size_t)u * V + w] = f;
+      else { edges.push_back(w); edges.push_back(u); sides.push_back(back); sides.push_back(f); }
+    }
+  }
+  reach = 0;
+  for (int i = 0; i < V; i++) {
+    const double x = v[3 * i], y = v[3 * i + 1], z = v[3 * i + 2];
+    reach = host_max(reach, std::sqrt(x * x + y * y + z * z));
+  }
+  inner = INFINITY;
+  for (int f = 0; f < F; f++) {
+    const double nx = p[4 * f], ny = p[4 * f + 1], nz = p[4 * f + 2];
+    for (int i = start[f]; i < start[f + 1]; i++) {
+      const int c = 3 * loops[i];
+      inner = host_min(inner, nx * v[c] + ny * v[c + 1] + nz * v[c + 2]);
+    }
+  }
+}
+
 int grow(bm_context* ctx, Scratch& s, size_t bytes, const char* what) {
   if (bytes <= s.cap) return BM_OK;
   if (s.p) (void)hipFree(s.p);
@@ -301,6 +353,8 @@ int bm_destroy(bm_context* ctx) {
   if (ctx->pstage) (void)hipHostFree(ctx->pstage);
   for (Scratch* s : {&ctx->deg, &ctx->adj, &ctx->keys, &ctx->key_offset, &ctx->world_of, &ctx->pairs_in,
                      &ctx->pairs_out})
+    if (s->p) (void)hipFree(s->p);
+  for (Scratch* s : {&ctx->hull_ints, &ctx->hull_doubles})
     if (s->p) (void)hipFree(s->p);
   if (ctx->begin) (void)hipEventDestroy(ctx->begin);
   if (ctx->end) (void)hipEventDestroy(ctx->end);
@@ -506,7 +560,7 @@ int bm_colour(bm_context* ctx) {
 
 int bm_collide_host(bm_context* ctx, int32_t n, const double* a, const double* b, const double* margin,
                     const double* tolerance, int32_t* count, double* normal, double* points, double* separations,
-                    uint32_t* ids) {
+                    uint32_t* ids, double* loss) {
   if (!ctx) return BM_ERR_ARGUMENT;
   ctx->error.clear();
   if (n < 0) return fail(ctx, BM_ERR_ARGUMENT, "bm_collide_host: n must be 0 or more");
@@ -517,11 +571,13 @@ int bm_collide_host(bm_context* ctx, int32_t n, const double* a, const double* b
     const double ka = a[19 * (size_t)k], kb = b[19 * (size_t)k];
     if (!(ka == 0 || ka == 1 || ka == 2 || ka == 3) || !(kb == 0 || kb == 1 || kb == 2 || kb == 3))
       return fail(ctx, BM_ERR_ARGUMENT, "bm_collide_host: a shape's kind must be 0, 1, 2 or 3");
+    if (!hull_ok(ctx, a + 19 * (size_t)k) || !hull_ok(ctx, b + 19 * (size_t)k))
+      return fail(ctx, BM_ERR_ARGUMENT, "bm_collide_host: a hull's number is not in the hull table (bm_set_hulls)");
   }
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->device), "bm_collide_host: hipSetDevice");
   const size_t shapes = 19 * sizeof(double) * (size_t)n, scalars = sizeof(double) * (size_t)n;
   const size_t in_bytes = 2 * shapes + 2 * scalars;
-  const size_t out_bytes = (size_t)n * (sizeof(int32_t) + 3 * 8 + 12 * 8 + 4 * 8 + 4 * 4) + 64;
+  const size_t out_bytes = 184 * (size_t)n + 64;
   if (int rc = grow(ctx, ctx->pairs_in, in_bytes, "bm_collide_host")) return rc;
   if (int rc = grow(ctx, ctx->pairs_out, out_bytes, "bm_collide_host")) return rc;
   char* in = static_cast<char*>(ctx->pairs_in.p);
@@ -537,6 +593,8 @@ int bm_collide_host(bm_context* ctx, int32_t n, const double* a, const double* b
   d.separations = reinterpret_cast<double*>(out + 15 * 8 * (size_t)n);
   d.count = reinterpret_cast<int32_t*>(out + 19 * 8 * (size_t)n);
   d.ids = reinterpret_cast<uint32_t*>(out + 19 * 8 * (size_t)n + 4 * (size_t)n);
+  d.loss = reinterpret_cast<double*>(out + 176 * (size_t)n);
+  d.hulls = ctx->hulls;
   // One copy each way through pinned staging: inputs as laid out on the device, then the outputs.
   if (int rc = grow_pinned(ctx, &ctx->pstage, &ctx->pstage_cap, in_bytes + out_bytes, "bm_collide_host")) return rc;
   char* st = static_cast<char*>(ctx->pstage);
@@ -550,7 +608,7 @@ int bm_collide_host(bm_context* ctx, int32_t n, const double* a, const double* b
   bm::launch_collide(d, ctx->stream);
   HIP_OR_FAIL(ctx, hipGetLastError(), "bm_collide_host: launch");
   HIP_OR_FAIL(ctx, hipEventRecord(ctx->end, ctx->stream), "bm_collide_host: hipEventRecord");
-  const size_t produced = 19 * 8 * (size_t)n + 4 * (size_t)n + 16 * (size_t)n;
+  const size_t produced = 184 * (size_t)n;
   char* back = st + in_bytes;
   HIP_OR_FAIL(ctx, hipMemcpyAsync(back, out, produced, hipMemcpyDeviceToHost, ctx->stream), "bm_collide_host: copy");
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_collide_host: kernel");
@@ -558,13 +616,15 @@ int bm_collide_host(bm_context* ctx, int32_t n, const double* a, const double* b
   std::memcpy(points, back + 3 * 8 * (size_t)n, 96 * (size_t)n);
   std::memcpy(separations, back + 15 * 8 * (size_t)n, 32 * (size_t)n);
   std::memcpy(count, back + 19 * 8 * (size_t)n, 4 * (size_t)n);
+  if (loss) std::memcpy(loss, back + 176 * (size_t)n, 8 * (size_t)n);
   std::memcpy(ids, back + 19 * 8 * (size_t)n + 4 * (size_t)n, 16 * (size_t)n);
   HIP_OR_FAIL(ctx, hipEventElapsedTime(&ctx->last_ms, ctx->begin, ctx->end), "bm_collide_host: hipEventElapsedTime");
   return BM_OK;
 }
 
 int bm_meet_host(bm_context* ctx, int32_t n, const double* a, const double* b, const double* pair, int32_t* count,
-                 double* normal, double* points, double* separations, uint32_t* ids, double* shift, int32_t* flags) {
+                 double* normal, double* points, double* separations, uint32_t* ids, double* shift, int32_t* flags,
+                 double* loss) {
   if (!ctx) return BM_ERR_ARGUMENT;
   ctx->error.clear();
   if (n < 0) return fail(ctx, BM_ERR_ARGUMENT, "bm_meet_host: n must be 0 or more");
@@ -575,12 +635,14 @@ int bm_meet_host(bm_context* ctx, int32_t n, const double* a, const double* b, c
     const double ka = a[32 * (size_t)k], kb = b[32 * (size_t)k];
     if (!(ka == 0 || ka == 1 || ka == 2 || ka == 3) || !(kb == 0 || kb == 1 || kb == 2 || kb == 3))
       return fail(ctx, BM_ERR_ARGUMENT, "bm_meet_host: a shape's kind must be 0, 1, 2 or 3");
+    if (!hull_ok(ctx, a + 32 * (size_t)k) || !hull_ok(ctx, b + 32 * (size_t)k))
+      return fail(ctx, BM_ERR_ARGUMENT, "bm_meet_host: a hull's number is not in the hull table (bm_set_hulls)");
     if (!(pair[6 * (size_t)k + 2] > 0)) return fail(ctx, BM_ERR_ARGUMENT, "bm_meet_host: dt must be positive");
   }
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->device), "bm_meet_host: hipSetDevice");
   const size_t bodies = 32 * sizeof(double) * (size_t)n, pairs = 6 * sizeof(double) * (size_t)n;
   const size_t in_bytes = 2 * bodies + pairs;
-  const size_t out_bytes = (size_t)n * (3 * 8 + 12 * 8 + 4 * 8 + 12 * 8 + 4 + 4 + 4 * 4) + 64;
+  const size_t out_bytes = 280 * (size_t)n + 64;
   if (int rc = grow(ctx, ctx->pairs_in, in_bytes, "bm_meet_host")) return rc;
   if (int rc = grow(ctx, ctx->pairs_out, out_bytes, "bm_meet_host")) return rc;
   char* in = static_cast<char*>(ctx->pairs_in.p);
@@ -598,6 +660,8 @@ int bm_meet_host(bm_context* ctx, int32_t n, const double* a, const double* b, c
   d.count = reinterpret_cast<int32_t*>(out + 248 * N);
   d.flags = reinterpret_cast<int32_t*>(out + 252 * N);
   d.ids = reinterpret_cast<uint32_t*>(out + 256 * N);
+  d.loss = reinterpret_cast<double*>(out + 272 * N);
+  d.hulls = ctx->hulls;
   // One copy each way through pinned staging.
   if (int rc = grow_pinned(ctx, &ctx->pstage, &ctx->pstage_cap, in_bytes + out_bytes, "bm_meet_host")) return rc;
   char* st = static_cast<char*>(ctx->pstage);
@@ -611,7 +675,7 @@ int bm_meet_host(bm_context* ctx, int32_t n, const double* a, const double* b, c
   HIP_OR_FAIL(ctx, hipGetLastError(), "bm_meet_host: launch");
   HIP_OR_FAIL(ctx, hipEventRecord(ctx->end, ctx->stream), "bm_meet_host: hipEventRecord");
   char* back = st + in_bytes;
-  HIP_OR_FAIL(ctx, hipMemcpyAsync(back, out, 272 * N, hipMemcpyDeviceToHost, ctx->stream), "bm_meet_host: copy");
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(back, out, 280 * N, hipMemcpyDeviceToHost, ctx->stream), "bm_meet_host: copy");
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_meet_host: kernel");
   std::memcpy(normal, back, 24 * N);
   std::memcpy(points, back + 24 * N, 96 * N);
@@ -620,7 +684,88 @@ int bm_meet_host(bm_context* ctx, int32_t n, const double* a, const double* b, c
   std::memcpy(count, back + 248 * N, 4 * N);
   std::memcpy(flags, back + 252 * N, 4 * N);
   std::memcpy(ids, back + 256 * N, 16 * N);
+  if (loss) std::memcpy(loss, back + 272 * N, 8 * N);
   HIP_OR_FAIL(ctx, hipEventElapsedTime(&ctx->last_ms, ctx->begin, ctx->end), "bm_meet_host: hipEventElapsedTime");
+  return BM_OK;
+}
+
+int bm_set_hulls(bm_context* ctx, int32_t n, const int32_t* hull_v, const int32_t* hull_f, const int32_t* hull_l,
+                 const double* vertices, const int32_t* start, const int32_t* loops, const double* planes) {
+  if (!ctx) return BM_ERR_ARGUMENT;
+  ctx->error.clear();
+  if (n < 0 || (n > 0 && (!hull_v || !hull_f || !hull_l || !vertices || !start || !loops || !planes)))
+    return fail(ctx, BM_ERR_ARGUMENT, "bm_set_hulls: null argument");
+  // The reference's unit cube (src/collide.js:1298, as src/hull.js build() lays it out), last.
+  static const double CUBE_V[24] = {-0.5, -0.5, -0.5, 0.5, -0.5, -0.5, -0.5, 0.5, -0.5, 0.5, 0.5, -0.5,
+                                    -0.5, -0.5, 0.5, 0.5, -0.5, 0.5, -0.5, 0.5, 0.5, 0.5, 0.5, 0.5};
+  static const int32_t CUBE_S[6] = {0, 4, 8, 12, 16, 20};
+  static const int32_t CUBE_L[24] = {0, 1, 5, 4, 0, 4, 6, 2, 0, 2, 3, 1, 1, 3, 7, 5, 4, 5, 7, 6, 2, 6, 7, 3};
+  static const double CUBE_P[24] = {0, -1, 0, 0.5, -1, 0, 0, 0.5, 0, 0, -1, 0.5, 1, 0, 0, 0.5, 0, 0, 1, 0.5, 0, 1, 0, 0.5};
+  std::vector<int32_t> info, ints_start, ints_loops, ints_edges, ints_sides;
+  std::vector<double> dinfo, dv, dp;
+  for (int h = 0; h <= n; h++) {
+    const bool cube = h == n;
+    const int V = cube ? 8 : hull_v[h + 1] - hull_v[h], F = cube ? 6 : hull_f[h + 1] - hull_f[h];
+    const int L = cube ? 24 : hull_l[h + 1] - hull_l[h];
+    if (V < 4 || F < 4 || F > bm::HULL_MAX_FACES || L < 3 * F)
+      return fail(ctx, BM_ERR_UNSUPPORTED, "bm_set_hulls: a hull has 4 to 64 faces of 3 or more corners");
+    const double* v = cube ? CUBE_V : vertices + 3 * (size_t)hull_v[h];
+    const double* p = cube ? CUBE_P : planes + 4 * (size_t)hull_f[h];
+    const int32_t* lp = cube ? CUBE_L : loops + hull_l[h];
+    std::vector<int32_t> st(F + 1);
+    for (int f = 0; f < F; f++) st[f] = cube ? CUBE_S[f] : start[hull_f[h] + f];
+    st[F] = L;
+    for (int f = 0; f < F; f++) {
+      if (st[f] < 0 || st[f + 1] < st[f] + 3 || st[f + 1] > L)
+        return fail(ctx, BM_ERR_ARGUMENT, "bm_set_hulls: a face's loop is out of order or too short");
+      if (st[f + 1] - st[f] > bm::HULL_MAX_CORNERS)
+        return fail(ctx, BM_ERR_UNSUPPORTED, "bm_set_hulls: a face has more than 16 corners");
+    }
+    for (int i = 0; i < L; i++)
+      if (lp[i] < 0 || lp[i] >= V) return fail(ctx, BM_ERR_ARGUMENT, "bm_set_hulls: a loop names no corner");
+    std::vector<int32_t> edges, sides;
+    double inner, reach;
+    derive_hull(V, F, st.data(), lp, v, p, edges, sides, inner, reach);
+    const int E = (int)edges.size() / 2;
+    if (E > bm::HULL_MAX_EDGES) return fail(ctx, BM_ERR_UNSUPPORTED, "bm_set_hulls: a hull has more than 96 edges");
+    info.insert(info.end(), {(int32_t)(dv.size() / 3), (int32_t)(dp.size() / 4), (int32_t)ints_start.size(),
+                             (int32_t)ints_loops.size(), (int32_t)(ints_edges.size() / 2), V, F, E});
+    dinfo.push_back(inner);
+    dinfo.push_back(reach);
+    dv.insert(dv.end(), v, v + 3 * V);
+    dp.insert(dp.end(), p, p + 4 * F);
+    ints_start.insert(ints_start.end(), st.begin(), st.end());
+    ints_loops.insert(ints_loops.end(), lp, lp + L);
+    ints_edges.insert(ints_edges.end(), edges.begin(), edges.end());
+    ints_sides.insert(ints_sides.end(), sides.begin(), sides.end());
+  }
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->device), "bm_set_hulls: hipSetDevice");
+  std::vector<int32_t> ints;
+  const size_t i_start = info.size(), i_loops = i_start + ints_start.size(), i_edges = i_loops + ints_loops.size(),
+               i_sides = i_edges + ints_edges.size();
+  for (auto* part : {&info, &ints_start, &ints_loops, &ints_edges, &ints_sides}) ints.insert(ints.end(), part->begin(), part->end());
+  std::vector<double> dbl;
+  const size_t d_v = dinfo.size(), d_p = d_v + dv.size();
+  for (auto* part : {&dinfo, &dv, &dp}) dbl.insert(dbl.end(), part->begin(), part->end());
+  if (int rc = grow(ctx, ctx->hull_ints, ints.size() * sizeof(int32_t) + 16, "bm_set_hulls")) return rc;
+  if (int rc = grow(ctx, ctx->hull_doubles, dbl.size() * sizeof(double) + 16, "bm_set_hulls")) return rc;
+  HIP_OR_FAIL(ctx, hipMemcpy(ctx->hull_ints.p, ints.data(), ints.size() * sizeof(int32_t), hipMemcpyHostToDevice),
+              "bm_set_hulls: copy");
+  HIP_OR_FAIL(ctx, hipMemcpy(ctx->hull_doubles.p, dbl.data(), dbl.size() * sizeof(double), hipMemcpyHostToDevice),
+              "bm_set_hulls: copy");
+  const int32_t* I = static_cast<const int32_t*>(ctx->hull_ints.p);
+  const double* D = static_cast<const double*>(ctx->hull_doubles.p);
+  ctx->hulls.n = n + 1;
+  ctx->hulls.cube = n;
+  ctx->hulls.info = I;
+  ctx->hulls.start = I + i_start;
+  ctx->hulls.loops = I + i_loops;
+  ctx->hulls.edges = I + i_edges;
+  ctx->hulls.sides = I + i_sides;
+  ctx->hulls.dinfo = D;
+  ctx->hulls.vertices = D + d_v;
+  ctx->hulls.planes = D + d_p;
+  ctx->hull_count = n;
   return BM_OK;
 }
 

@@ -5,13 +5,26 @@
 
 namespace bm {
 
+// The hulls shape records name by number: each hull's tables in its body's frame
+// (src/hull.js Hull), with the reference's unit cube (src/collide.js CUBE) among them.
+constexpr int HULL_MAX_FACES = 64, HULL_MAX_EDGES = 96, HULL_MAX_CORNERS = 16;
+struct HullTable {
+  int n, cube;                 // hulls; the unit cube's number
+  const int* info;             // 8 a hull: vertex, plane, start, loop and edge offsets; V, F, E
+  const double* dinfo;         // 2 a hull: inner, reach
+  const double *vertices, *planes;
+  const int *start, *loops, *edges, *sides;  // start: F + 1 a hull, counted in the hull's loops
+};
+
 struct CollideDev {
   int n;
-  const double *a, *b;             // shape records: kind, centre 3, axes 9, half 3, weight 3
+  const double *a, *b;             // shape records: kind, centre 3, axes 9, half 3 (a hull: its number), weight 3
   const double *margin, *tolerance;
   int* count;                      // -1 for a pair of kinds not handled (capsules, hulls)
   double *normal, *points, *separations;
   uint32_t* ids;
+  double* loss;                    // Manifold.loss: how far a hull face falls short of the best axis
+  HullTable hulls;
 };
 
 void launch_collide(const CollideDev& d, hipStream_t stream);
@@ -24,6 +37,8 @@ struct MeetDev {
   int *count, *flags;   // flags: 1 shifted, 2 a time of impact was asked
   double *normal, *points, *separations, *shift;
   uint32_t* ids;
+  double* loss;
+  HullTable hulls;
 };
 
 void launch_meet(const MeetDev& d, hipStream_t stream);

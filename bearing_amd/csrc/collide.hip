@@ -22,7 +22,7 @@ __device__ __forceinline__ double js_min(double a, double b) {
   return __builtin_signbit(a) ? a : b;
 }
 
-constexpr int SPHERE = 0, CAPSULE = 1, BOX = 2;
+constexpr int SPHERE = 0, CAPSULE = 1, BOX = 2, HULL = 3;
 constexpr int MAX_POINTS = 4;
 constexpr double PARALLEL = 7.450580596923828e-9;  // 2^-27, src/collide.js:686
 
@@ -32,6 +32,8 @@ struct Shape {
   double axes[9];
   double half[3];
   double weight[3];
+  int hull;                 // a hull's number in the table
+  const HullTable* table;
 };
 
 struct Manifold {
@@ -60,6 +62,9 @@ struct Scratch {
   double foot[3];
   const Shape* shape_a;
   const Shape* shape_b;
+  // The list reducePoints is working on (src/collide.js:1077): kept, or a hull face's.
+  const double* rk;
+  const double* rs;
 };
 
 __device__ constexpr int CORNER_P[4] = {-1, 1, 1, -1}, CORNER_Q[4] = {-1, -1, 1, 1};
@@ -554,8 +559,8 @@ __device__ __forceinline__ int clip_polygon(const double* src, const int* sf, co
 
 // src/collide.js:1116-1141 (four)
 __device__ __forceinline__ void four(Scratch& S, int n, int a, double nx, double ny, double nz, double pressed, double touching) {
-  const double* kept = S.kept;
-  const double* sep = S.kept_sep;
+  const double* kept = S.rk;
+  const double* sep = S.rs;
   int b = -1;
   double far = -1;
   for (int pass = 0; pass < 3 && b < 0; pass++) {
@@ -591,7 +596,7 @@ __device__ __forceinline__ bool foot(Scratch& S, const Shape& s, double nx, doub
   const double along = w[0] * nx + w[1] * ny + w[2] * nz;
   const int o = S.four[0];
   if (along == 0) return false;
-  const double t = ((S.kept[3 * o] - c[0]) * nx + (S.kept[3 * o + 1] - c[1]) * ny + (S.kept[3 * o + 2] - c[2]) * nz) / along;
+  const double t = ((S.rk[3 * o] - c[0]) * nx + (S.rk[3 * o + 1] - c[1]) * ny + (S.rk[3 * o + 2] - c[2]) * nz) / along;
   if (!(t > 0)) return false;
   S.foot[0] = c[0] + w[0] * t; S.foot[1] = c[1] + w[1] * t; S.foot[2] = c[2] + w[2] * t;
   return true;
@@ -605,8 +610,8 @@ __device__ __forceinline__ bool rests(Scratch& S, const Shape& s, double nx, dou
   for (int e = 1; e <= 4; e++) {
     const int to = S.four[e & 3];
     if (to < 0) continue;
-    const double ax = S.kept[3 * from] - px, ay = S.kept[3 * from + 1] - py, az = S.kept[3 * from + 2] - pz;
-    const double bx = S.kept[3 * to] - px, by = S.kept[3 * to + 1] - py, bz = S.kept[3 * to + 2] - pz;
+    const double ax = S.rk[3 * from] - px, ay = S.rk[3 * from + 1] - py, az = S.rk[3 * from + 2] - pz;
+    const double bx = S.rk[3 * to] - px, by = S.rk[3 * to + 1] - py, bz = S.rk[3 * to + 2] - pz;
     if ((ay * bz - az * by) * nx + (az * bx - ax * bz) * ny + (ax * by - ay * bx) * nz < 0) return false;
     from = to;
   }
@@ -618,8 +623,8 @@ __device__ __forceinline__ int under(Scratch& S, int n, double nx, double ny, do
   int start = -1;
   double near = __builtin_inf();
   for (int k = 0; k < n; k++) {
-    if (S.kept_sep[k] > pressed) continue;
-    const double x = S.kept[3 * k] - S.foot[0], y = S.kept[3 * k + 1] - S.foot[1], z = S.kept[3 * k + 2] - S.foot[2];
+    if (S.rs[k] > pressed) continue;
+    const double x = S.rk[3 * k] - S.foot[0], y = S.rk[3 * k + 1] - S.foot[1], z = S.rk[3 * k + 2] - S.foot[2];
     const double along = x * nx + y * ny + z * nz;
     const double d = x * x + y * y + z * z - along * along;
     if (d < near) { near = d; start = k; }
@@ -628,10 +633,10 @@ __device__ __forceinline__ int under(Scratch& S, int n, double nx, double ny, do
 }
 
 // src/collide.js:1077-1104 (reducePoints)
-__device__ __forceinline__ void reduce_points(Scratch& S, int n, double nx, double ny, double nz, double tolerance, Manifold& out) {
-  const double* kept = S.kept;
-  const double* sep = S.kept_sep;
-  const uint32_t* id = S.kept_id;
+__device__ __forceinline__ void reduce_list(Scratch& S, const double* kept, const double* sep, const uint32_t* id, int n, double nx, double ny,
+                                                double nz, double tolerance, Manifold& out) {
+  S.rk = kept;
+  S.rs = sep;
   if (n <= MAX_POINTS) {
     for (int k = 0; k < n; k++) add_point(out, kept[3 * k], kept[3 * k + 1], kept[3 * k + 2], sep[k], id[k]);
     return;
@@ -659,6 +664,12 @@ __device__ __forceinline__ void reduce_points(Scratch& S, int n, double nx, doub
   add_point(out, kept[3 * b], kept[3 * b + 1], kept[3 * b + 2], sep[b], id[b]);
   if (c >= 0) add_point(out, kept[3 * c], kept[3 * c + 1], kept[3 * c + 2], sep[c], id[c]);
   if (d >= 0) add_point(out, kept[3 * d], kept[3 * d + 1], kept[3 * d + 2], sep[d], id[d]);
+}
+
+// src/collide.js:1072-1074 (reduce): the box pairs' list.
+__device__ __forceinline__ void reduce_points(Scratch& S, int n, double nx, double ny, double nz, double tolerance,
+                                              Manifold& out) {
+  reduce_list(S, S.kept, S.kept_sep, S.kept_id, n, nx, ny, nz, tolerance, out);
 }
 
 // src/collide.js:884-945 (faceContact)
@@ -810,6 +821,531 @@ __device__ __forceinline__ void box_box(Scratch& S, const Shape& a, const Shape&
   if (out.count == 0 && js_max(js_max(face_a, face_b), edge_sep) > 0) box_box_closest(S, a, b, margin, out);
 }
 
+// ---- hulls: src/collide.js:1262-1882 ----
+
+constexpr double LOOSE = 1.0 / 65536;      // 2^-16, src/collide.js:1270
+constexpr double NEARLY = 1.0 / 67108864;  // 2^-26, src/collide.js:1278
+constexpr int CUT = 1 << 20;               // src/collide.js:1541
+
+// src/collide.js:1286-1295 (reaches)
+__device__ __forceinline__ bool reaches(double a1, double a2, double c, double bound) {
+  const double hi = a1 > a2 ? a1 : a2, lo = a1 > a2 ? a2 : a1;
+  if (!(hi < bound)) return true;
+  const double s = 1 - c * c;
+  if (!(s >= NEARLY)) return true;
+  const double u = lo - c * hi;
+  return u >= 0 && !(dsqrt(hi * hi + u * u / s) < bound);
+}
+
+// src/collide.js:1309-1388 (Placed): a hull, or a box as the unit cube scaled, placed in the
+// world. Corners, planes and edge arcs are worked out where they are read, each the same
+// operations in the same order as the reference's set(), so every read has set()'s bits
+// and nothing is kept in per-lane memory.
+struct Placed {
+  const double *hv, *hp;
+  const int *start, *loops, *edges, *sides;
+  int V, F, E;
+  bool box;
+  double c[3], A[9], half[3];
+  double inner, reach;
+
+  __device__ __forceinline__ void set(const Shape& s) {
+    const HullTable& T = *s.table;
+    box = s.kind == BOX;
+    const int h = box ? T.cube : s.hull;
+    const int* info = T.info + 8 * h;
+    hv = T.vertices + 3 * info[0];
+    hp = T.planes + 4 * info[1];
+    start = T.start + info[2];
+    loops = T.loops + info[3];
+    edges = T.edges + 2 * info[4];
+    sides = T.sides + 2 * info[4];
+    V = info[5]; F = info[6]; E = info[7];
+    for (int k = 0; k < 3; k++) { c[k] = s.center[k]; half[k] = s.half[k]; }
+    for (int k = 0; k < 9; k++) A[k] = s.axes[k];
+    inner = box ? js_min(js_min(half[0], half[1]), half[2]) : T.dinfo[2 * h];
+    reach = box ? dsqrt(half[0] * half[0] + half[1] * half[1] + half[2] * half[2]) : T.dinfo[2 * h + 1];
+  }
+  __device__ __forceinline__ void vert(int i, double* o) const {
+    double x = hv[3 * i], y = hv[3 * i + 1], z = hv[3 * i + 2];
+    if (box) { x *= 2 * half[0]; y *= 2 * half[1]; z *= 2 * half[2]; }
+    o[0] = c[0] + A[0] * x + A[3] * y + A[6] * z;
+    o[1] = c[1] + A[1] * x + A[4] * y + A[7] * z;
+    o[2] = c[2] + A[2] * x + A[5] * y + A[8] * z;
+  }
+  __device__ __forceinline__ void plane(int f, double* o) const {
+    const double x = hp[4 * f], y = hp[4 * f + 1], z = hp[4 * f + 2];
+    const double nx = A[0] * x + A[3] * y + A[6] * z, ny = A[1] * x + A[4] * y + A[7] * z, nz = A[2] * x + A[5] * y + A[8] * z;
+    const double d = box ? dabs(x) * half[0] + dabs(y) * half[1] + dabs(z) * half[2] : hp[4 * f + 3];
+    o[0] = nx; o[1] = ny; o[2] = nz;
+    o[3] = d + nx * c[0] + ny * c[1] + nz * c[2];
+  }
+  __device__ __forceinline__ void arc(int e, double* o) const {
+    double l[4], r[4];
+    plane(sides[2 * e], l);
+    plane(sides[2 * e + 1], r);
+    o[0] = r[1] * l[2] - r[2] * l[1];
+    o[1] = r[2] * l[0] - r[0] * l[2];
+    o[2] = r[0] * l[1] - r[1] * l[0];
+  }
+  __device__ __forceinline__ double lowest(double nx, double ny, double nz) const {
+    double m = __builtin_inf();
+    for (int i = 0; i < V; i++) {
+      double v[3];
+      vert(i, v);
+      const double d = v[0] * nx + v[1] * ny + v[2] * nz;
+      if (d < m) m = d;
+    }
+    return m;
+  }
+};
+
+// src/collide.js:1401-1404 (mix)
+__device__ __forceinline__ uint32_t mix(uint32_t kind, uint32_t i, uint32_t j, uint32_t k) {
+  return ((kind + 1) * 0x9e3779b1u) ^ ((i + 1) * 0x85ebca77u) ^ ((j + 1) * 0xc2b2ae3du) ^ ((k + 1) * 0x27d4eb2fu);
+}
+
+__device__ __forceinline__ double gjk_closest(const Shape& a, const Shape& b, double* near_a, double* near_b);
+
+// src/collide.js:1751-1759 (nearest)
+__device__ __forceinline__ double nearest(Scratch& S, const Shape& a, const Shape& b, double ra, double margin,
+                                          Manifold& out) {
+  double na[3], nb[3];
+  const double d = gjk_closest(a, b, na, nb);
+  if (!(d > 0) || d - ra > margin) return d;
+  const double nx = (nb[0] - na[0]) / d, ny = (nb[1] - na[1]) / d, nz = (nb[2] - na[2]) / d;
+  const double separation = d - ra, t = ra + separation * 0.5;
+  set_normal(out, nx, ny, nz);
+  add_point(out, na[0] + nx * t, na[1] + ny * t, na[2] + nz * t, separation, mix(3, 0, 0, 0));
+  return d;
+}
+
+// src/collide.js:1570-1581 (named)
+__device__ __forceinline__ uint32_t named(int f, bool flipped, int rf, int inc) {
+  int g;
+  if (f < CUT) g = flipped ? f : CUT + f;
+  else if (f < 2 * CUT) {
+    const int x = (f - CUT) >> 1, line = x >> 12, plane = x & 4095;
+    g = 2 * CUT + (flipped ? 4096 * line + plane : 4096 * plane + line);
+  } else {
+    const int x = (f - 2 * CUT) >> 1, p = x >> 12, q = x & 4095, c = q == p + 1 ? q : p;
+    g = flipped ? CUT + c : c;
+  }
+  return flipped ? mix(0, inc, rf, g) : mix(0, rf, inc, g);
+}
+
+// src/collide.js:1698-1725 (clipSides)
+__device__ __forceinline__ int clip_sides(const double* src, const int* sf, const int* sl, int count, double* dst,
+                                          int* df, int* dl, double mx, double my, double mz, double offset, int plane,
+                                          double slack, double length, int cap) {
+  int n = 0;
+  // A convex polygon cut by a plane gains at most one corner, so `cap` (the reference sizes
+  // its arrays the same) is never reached; the check keeps every write inside the arrays.
+  for (int v = 0; v < count && n + 2 <= cap; v++) {
+    const int w = v + 1 < count ? v + 1 : 0;
+    const double ax = src[3 * v], ay = src[3 * v + 1], az = src[3 * v + 2];
+    const double bx = src[3 * w], by = src[3 * w + 1], bz = src[3 * w + 2];
+    const double da = ax * mx + ay * my + az * mz - offset, db = bx * mx + by * my + bz * mz - offset;
+    const int line = sl[v];
+    if (da <= slack) {
+      if (da > 0) {
+        const double t = da / (length * length);
+        dst[3 * n] = ax - mx * t; dst[3 * n + 1] = ay - my * t; dst[3 * n + 2] = az - mz * t;
+      } else {
+        dst[3 * n] = ax; dst[3 * n + 1] = ay; dst[3 * n + 2] = az;
+      }
+      df[n] = sf[v];
+      dl[n] = da > 0 && db > slack ? CUT + plane : line;
+      n++;
+    }
+    if ((da <= 0 && db > slack) || (da > slack && db <= 0)) {
+      const double t = da / (da - db);
+      dst[3 * n] = ax + (bx - ax) * t; dst[3 * n + 1] = ay + (by - ay) * t; dst[3 * n + 2] = az + (bz - az) * t;
+      df[n] = line < CUT ? CUT + 2 * (line * 4096 + plane)
+                         : 2 * CUT + 2 * ((line - CUT < plane ? line - CUT : plane) * 4096 + (line - CUT > plane ? line - CUT : plane));
+      dl[n] = da <= 0 ? CUT + plane : line;
+      n++;
+    }
+  }
+  return n;
+}
+
+constexpr int CLIP_MAX = 2 * HULL_MAX_CORNERS + 2;
+
+// src/collide.js:1592-1689 (hullFace)
+__device__ __forceinline__ bool hull_face(Scratch& S, const Placed& R, int rf, const Placed& I, bool flipped,
+                                          double margin, double tolerance, Manifold& out, bool sure) {
+  double rpf[4];
+  R.plane(rf, rpf);
+  const double nx = rpf[0], ny = rpf[1], nz = rpf[2], d = rpf[3];
+  int low = 0;
+  double lowest = __builtin_inf();
+  for (int i = 0; i < I.V; i++) {
+    double v[3];
+    I.vert(i, v);
+    const double dot = v[0] * nx + v[1] * ny + v[2] * nz;
+    if (dot < lowest) { lowest = dot; low = i; }
+  }
+  int inc = 0;
+  double least = __builtin_inf();
+  for (int f = 0; f < I.F; f++) {
+    double pf[4];
+    I.plane(f, pf);
+    const double dot = pf[0] * nx + pf[1] * ny + pf[2] * nz;
+    if (!(dot < least)) continue;
+    bool at = false;
+    for (int j = I.start[f]; j < I.start[f + 1] && !at; j++) at = I.loops[j] == low;
+    if (at) { least = dot; inc = f; }
+  }
+  const int i0 = I.start[inc], m = I.start[inc + 1] - i0, r0 = R.start[rf], k = R.start[rf + 1] - r0;
+  double clip_p[3 * CLIP_MAX], clip_q[3 * CLIP_MAX], kept_p[3 * (CLIP_MAX + 1)], kept_s[CLIP_MAX + 1];
+  int feat_p[CLIP_MAX], feat_q[CLIP_MAX], line_p[CLIP_MAX], line_q[CLIP_MAX];
+  uint32_t kept_i[CLIP_MAX + 1];
+  for (int j = 0; j < m; j++) {
+    I.vert(I.loops[i0 + j], &clip_p[3 * j]);
+    feat_p[j] = j; line_p[j] = j;
+  }
+  int count = m, src = 0;
+  for (int j = 0; j < k && count > 0; j++) {
+    double rp[3], rq[3];
+    R.vert(R.loops[r0 + j], rp);
+    R.vert(R.loops[r0 + (j + 1 < k ? j + 1 : 0)], rq);
+    const double ex = rq[0] - rp[0], ey = rq[1] - rp[1], ez = rq[2] - rp[2];
+    const double mx = ey * nz - ez * ny, my = ez * nx - ex * nz, mz = ex * ny - ey * nx;
+    const double offset = mx * rp[0] + my * rp[1] + mz * rp[2];
+    const double length = dsqrt(mx * mx + my * my + mz * mz);
+    count = src == 0 ? clip_sides(clip_p, feat_p, line_p, count, clip_q, feat_q, line_q, mx, my, mz, offset, j,
+                                  tolerance * length, length, CLIP_MAX)
+                     : clip_sides(clip_q, feat_q, line_q, count, clip_p, feat_p, line_p, mx, my, mz, offset, j,
+                                  tolerance * length, length, CLIP_MAX);
+    src = 1 - src;
+  }
+  const double* poly = src == 0 ? clip_p : clip_q;
+  const int* feat = src == 0 ? feat_p : feat_q;
+  int n = 0;
+  for (int j = 0; j < count; j++) {
+    const double x = poly[3 * j], y = poly[3 * j + 1], z = poly[3 * j + 2];
+    const double separation = x * nx + y * ny + z * nz - d;
+    if (separation > margin) continue;
+    const double h = separation * 0.5;
+    kept_p[3 * n] = x - nx * h; kept_p[3 * n + 1] = y - ny * h; kept_p[3 * n + 2] = z - nz * h;
+    kept_s[n] = separation;
+    kept_i[n] = named(feat[j], flipped, rf, inc);
+    n++;
+  }
+  const double s = lowest - d;
+  if (s <= tolerance) {
+    double deepest = __builtin_inf();
+    for (int j = 0; j < n; j++)
+      if (kept_s[j] < deepest) deepest = kept_s[j];
+    if (deepest > s + tolerance) {
+      if (!sure) return false;
+      int j = 0;
+      while (I.loops[i0 + j] != low) j++;
+      double v[3];
+      I.vert(low, v);
+      const double h = s * 0.5;
+      kept_p[3 * n] = v[0] - nx * h; kept_p[3 * n + 1] = v[1] - ny * h; kept_p[3 * n + 2] = v[2] - nz * h;
+      kept_s[n] = s;
+      kept_i[n] = named(j, flipped, rf, inc);
+      n++;
+    }
+  }
+  if (n == 0) return false;
+  if (flipped) set_normal(out, -nx, -ny, -nz);
+  else set_normal(out, nx, ny, nz);
+  reduce_list(S, kept_p, kept_s, kept_i, n, nx, ny, nz, tolerance, out);
+  if (s <= tolerance) {
+    double kept = __builtin_inf();
+    for (int j = 0; j < out.count; j++)
+      if (out.separations[j] < kept) kept = out.separations[j];
+    if (kept - s > out.loss) out.loss = kept - s;
+  }
+  return true;
+}
+
+// src/collide.js:1728-1744 (hullEdges)
+__device__ __forceinline__ void hull_edges(Scratch& S, const Placed& A, int i, const Placed& B, int j, double nx,
+                                           double ny, double nz, double margin, Manifold& out) {
+  double* sa = S.seg_a;
+  double* sb = S.seg_b;
+  A.vert(A.edges[2 * i], sa);
+  A.vert(A.edges[2 * i + 1], sa + 3);
+  B.vert(B.edges[2 * j], sb);
+  B.vert(B.edges[2 * j + 1], sb + 3);
+  closest_segments(S, sa, sb);
+  const double ss = S.seg_s, tt = S.seg_t;
+  const double pax = sa[0] + (sa[3] - sa[0]) * ss, pay = sa[1] + (sa[4] - sa[1]) * ss, paz = sa[2] + (sa[5] - sa[2]) * ss;
+  const double pbx = sb[0] + (sb[3] - sb[0]) * tt, pby = sb[1] + (sb[4] - sb[1]) * tt, pbz = sb[2] + (sb[5] - sb[2]) * tt;
+  const double separation = (pbx - pax) * nx + (pby - pay) * ny + (pbz - paz) * nz;
+  if (separation > margin) return;
+  if (separation > 0 && (ss == 0 || ss == 1 || tt == 0 || tt == 1)) return;
+  set_normal(out, nx, ny, nz);
+  add_point(out, (pax + pbx) * 0.5, (pay + pby) * 0.5, (paz + pbz) * 0.5, separation, mix(2, i, j, 0));
+}
+
+// src/collide.js:1406-1536 (hullHull)
+__device__ __forceinline__ void hull_hull(Scratch& S, const Shape& a, const Shape& b, double margin, double tolerance,
+                                          Manifold& out) {
+  Placed A, B;
+  A.set(a);
+  B.set(b);
+  const double* ca = A.c;
+  const double* cb = B.c;
+  const int FA = A.F, FB = B.F;
+  const double loose = LOOSE * (dabs(ca[0]) + dabs(ca[1]) + dabs(ca[2]) + dabs(cb[0]) + dabs(cb[1]) + dabs(cb[2])
+                                + A.reach + B.reach);
+  const double ra = A.inner > loose ? A.inner - loose : -__builtin_inf();
+  const double rb = B.inner > loose ? B.inner - loose : -__builtin_inf();
+  double above_a[HULL_MAX_FACES], above_b[HULL_MAX_FACES];
+  int top = 0;
+  for (int f = 0; f < FA; f++) {
+    double pf[4];
+    A.plane(f, pf);
+    above_a[f] = pf[0] * cb[0] + pf[1] * cb[1] + pf[2] * cb[2] - pf[3];
+    if (above_a[f] > above_a[top]) top = f;
+  }
+  double pt[4];
+  A.plane(top, pt);
+  const double first_a = B.lowest(pt[0], pt[1], pt[2]) - pt[3];
+  if (first_a > margin) return;
+  double face_a = -__builtin_inf();
+  int fa = 0;
+  for (int f = 0; f < FA; f++) {
+    double sep = first_a;
+    if (f != top) {
+      if (above_a[f] - rb < js_max(face_a, first_a)) continue;
+      double pf[4];
+      A.plane(f, pf);
+      sep = B.lowest(pf[0], pf[1], pf[2]) - pf[3];
+      if (sep > margin) return;
+    }
+    if (sep > face_a) { face_a = sep; fa = f; }
+  }
+  top = 0;
+  for (int f = 0; f < FB; f++) {
+    double pf[4];
+    B.plane(f, pf);
+    above_b[f] = pf[0] * ca[0] + pf[1] * ca[1] + pf[2] * ca[2] - pf[3];
+    if (above_b[f] > above_b[top]) top = f;
+  }
+  B.plane(top, pt);
+  const double first_b = A.lowest(pt[0], pt[1], pt[2]) - pt[3];
+  if (first_b > margin) return;
+  double face_b = -__builtin_inf();
+  int fb = 0;
+  for (int f = 0; f < FB; f++) {
+    double sep = first_b;
+    if (f != top) {
+      if (above_b[f] - ra < js_max(js_max(face_b, first_b), face_a)) continue;
+      double pf[4];
+      B.plane(f, pf);
+      sep = A.lowest(pf[0], pf[1], pf[2]) - pf[3];
+      if (sep > margin) return;
+    }
+    if (sep > face_b) { face_b = sep; fb = f; }
+  }
+  const int EA = A.E, EB = B.E;
+  double side[HULL_MAX_FACES];
+  const double best = js_max(face_a, face_b);
+  bool reach_b[HULL_MAX_EDGES];
+  int reachable = 0;
+  for (int j = 0; j < EB; j++) {
+    double l[4], r[4], p[3];
+    B.plane(B.sides[2 * j], l);
+    B.plane(B.sides[2 * j + 1], r);
+    B.vert(B.edges[2 * j], p);
+    const double dx = ca[0] - p[0], dy = ca[1] - p[1], dz = ca[2] - p[2];
+    const bool can = reaches(l[0] * dx + l[1] * dy + l[2] * dz, r[0] * dx + r[1] * dy + r[2] * dz,
+                             l[0] * r[0] + l[1] * r[1] + l[2] * r[2], best + ra);
+    reach_b[j] = can;
+    if (can) reachable++;
+  }
+  double edge_sep = -__builtin_inf(), enx = 0, eny = 0, enz = 0;
+  int ei = -1, ej = -1;
+  for (int i = 0; i < EA && reachable > 0; i++) {
+    double l[4], r[4], u[3], w[3];
+    A.plane(A.sides[2 * i], l);
+    A.plane(A.sides[2 * i + 1], r);
+    A.vert(A.edges[2 * i], u);
+    A.vert(A.edges[2 * i + 1], w);
+    const double dx = cb[0] - u[0], dy = cb[1] - u[1], dz = cb[2] - u[2];
+    if (!reaches(l[0] * dx + l[1] * dy + l[2] * dz, r[0] * dx + r[1] * dy + r[2] * dz,
+                 l[0] * r[0] + l[1] * r[1] + l[2] * r[2], best + rb))
+      continue;
+    double ar[3];
+    A.arc(i, ar);
+    const double ax = ar[0], ay = ar[1], az = ar[2];
+    const double ux = w[0] - u[0], uy = w[1] - u[1], uz = w[2] - u[2];
+    const double ul = dsqrt(ux * ux + uy * uy + uz * uz);
+    for (int g = 0; g < FB; g++) {
+      double pf[4];
+      B.plane(g, pf);
+      side[g] = pf[0] * ax + pf[1] * ay + pf[2] * az;
+    }
+    for (int j = 0; j < EB; j++) {
+      if (!reach_b[j]) continue;
+      const double cba = side[B.sides[2 * j]], dba = side[B.sides[2 * j + 1]];
+      if (!(cba * dba < 0)) continue;
+      double br[3];
+      B.arc(j, br);
+      const double bx = br[0], by = br[1], bz = br[2];
+      const double adc = l[0] * bx + l[1] * by + l[2] * bz, bdc = r[0] * bx + r[1] * by + r[2] * bz;
+      if (!(adc * bdc < 0 && cba * bdc < 0)) continue;
+      double p[3], q[3];
+      B.vert(B.edges[2 * j], p);
+      B.vert(B.edges[2 * j + 1], q);
+      const double tx = q[0] - p[0], ty = q[1] - p[1], tz = q[2] - p[2];
+      const double len = normalize(S, uy * tz - uz * ty, uz * tx - ux * tz, ux * ty - uy * tx);
+      if (!(len > PARALLEL * ul * dsqrt(tx * tx + ty * ty + tz * tz))) continue;
+      double nx = S.unit[0], ny = S.unit[1], nz = S.unit[2];
+      if (nx * (u[0] - ca[0]) + ny * (u[1] - ca[1]) + nz * (u[2] - ca[2]) < 0) { nx = -nx; ny = -ny; nz = -nz; }
+      const double sep = nx * (p[0] - u[0]) + ny * (p[1] - u[1]) + nz * (p[2] - u[2]);
+      if (sep > margin) return;
+      if (sep > edge_sep) { edge_sep = sep; ei = i; ej = j; enx = nx; eny = ny; enz = nz; }
+    }
+  }
+  const bool edge_best = ei >= 0 && edge_sep > face_a && edge_sep > face_b, b_best = !edge_best && face_b > face_a;
+  if (ei >= 0 && edge_sep > js_max(face_a, face_b) + tolerance) {
+    hull_edges(S, A, ei, B, ej, enx, eny, enz, margin, out);
+  } else {
+    const bool on_b = face_b > face_a + tolerance, sure = !edge_best && on_b == b_best;
+    if (hull_face(S, on_b ? B : A, on_b ? fb : fa, on_b ? A : B, on_b, margin, tolerance, out, sure)) {
+      const double behind = js_max(js_max(face_a, face_b), edge_sep) - (on_b ? face_b : face_a);
+      if (behind > out.loss) out.loss = behind;
+    } else if (!sure) {
+      if (edge_best) hull_edges(S, A, ei, B, ej, enx, eny, enz, margin, out);
+      else hull_face(S, b_best ? B : A, b_best ? fb : fa, b_best ? A : B, b_best, margin, tolerance, out, true);
+    }
+  }
+  if (out.count == 0 && js_max(js_max(face_a, face_b), edge_sep) > 0) nearest(S, a, b, 0, margin, out);
+}
+
+// src/collide.js:1766-1781 (sphereHull)
+__device__ __forceinline__ void sphere_hull(Scratch& S, const Shape& a, const Shape& b, double margin, Manifold& out) {
+  Placed B;
+  B.set(b);
+  const double* c = a.center;
+  const double r = a.half[0];
+  double most = -__builtin_inf();
+  int f = 0;
+  for (int k = 0; k < B.F; k++) {
+    double pf[4];
+    B.plane(k, pf);
+    const double sep = pf[0] * c[0] + pf[1] * c[1] + pf[2] * c[2] - pf[3];
+    if (sep > most) { most = sep; f = k; }
+  }
+  if (most - r > margin) return;
+  if (most > 0 && nearest(S, a, b, r, margin, out) > 0) return;
+  double pf[4];
+  B.plane(f, pf);
+  const double nx = -pf[0], ny = -pf[1], nz = -pf[2];
+  const double separation = most - r, t = r + separation * 0.5;
+  set_normal(out, nx, ny, nz);
+  add_point(out, c[0] + nx * t, c[1] + ny * t, c[2] + nz * t, separation, mix(4, f, 0, 0));
+}
+
+// src/collide.js:1855-1882 (clipToHullFace), the capsule's core in S.seg_a.
+__device__ __forceinline__ bool clip_to_hull_face(Scratch& S, const Placed& B, int f, double r, double margin,
+                                                  Manifold& out) {
+  const double* sa = S.seg_a;
+  double pf[4];
+  B.plane(f, pf);
+  const double nx = pf[0], ny = pf[1], nz = pf[2];
+  const int i0 = B.start[f], k = B.start[f + 1] - i0;
+  double u0 = 0, u1 = 1;
+  for (int j = 0; j < k; j++) {
+    double sv[3], q[3];
+    B.vert(B.loops[i0 + j], sv);
+    B.vert(B.loops[i0 + (j + 1 < k ? j + 1 : 0)], q);
+    const double ex = q[0] - sv[0], ey = q[1] - sv[1], ez = q[2] - sv[2];
+    const double mx = ey * nz - ez * ny, my = ez * nx - ex * nz, mz = ex * ny - ey * nx;
+    const double offset = mx * sv[0] + my * sv[1] + mz * sv[2];
+    const double d0 = sa[0] * mx + sa[1] * my + sa[2] * mz - offset, d1 = sa[3] * mx + sa[4] * my + sa[5] * mz - offset;
+    if (d0 > 0 && d1 > 0) return false;
+    if (d0 > 0) u0 = js_max(u0, d0 / (d0 - d1));
+    else if (d1 > 0) u1 = js_min(u1, d0 / (d0 - d1));
+  }
+  if (u0 > u1) return false;
+  set_normal(out, -nx, -ny, -nz);
+  for (int e = 0; e < (u1 > u0 ? 2 : 1); e++) {
+    const double u = e == 0 ? u0 : u1;
+    const double x = sa[0] + (sa[3] - sa[0]) * u, y = sa[1] + (sa[4] - sa[1]) * u, z = sa[2] + (sa[5] - sa[2]) * u;
+    const double above = x * nx + y * ny + z * nz - pf[3], separation = above - r;
+    if (separation > margin) continue;
+    const double t = (above + r) * 0.5;
+    add_point(out, x - nx * t, y - ny * t, z - nz * t, separation, mix(6, f, e, 0));
+  }
+  return out.count > 0;
+}
+
+// src/collide.js:1790-1847 (capsuleHull)
+__device__ __forceinline__ void capsule_hull(Scratch& S, const Shape& a, const Shape& b, double margin,
+                                             double tolerance, Manifold& out) {
+  Placed B;
+  B.set(b);
+  const double r = a.half[0];
+  double* sa = S.seg_a;
+  double* sb = S.seg_b;
+  segment(a, sa);
+  double na[3], nb[3];
+  const double d = gjk_closest(a, b, na, nb);
+  if (d > 0) {
+    if (d - r > margin) return;
+    const double nx = (nb[0] - na[0]) / d, ny = (nb[1] - na[1]) / d, nz = (nb[2] - na[2]) / d;
+    int f = 0;
+    double most = -__builtin_inf();
+    for (int k = 0; k < B.F; k++) {
+      double pf[4];
+      B.plane(k, pf);
+      const double dot = -(pf[0] * nx + pf[1] * ny + pf[2] * nz);
+      if (dot > most) { most = dot; f = k; }
+    }
+    if (1 - most <= PARALLEL && clip_to_hull_face(S, B, f, r, margin, out)) return;
+    nearest(S, a, b, r, margin, out);
+    return;
+  }
+  const double dx = sa[3] - sa[0], dy = sa[4] - sa[1], dz = sa[5] - sa[2];
+  const double dl = dsqrt(dx * dx + dy * dy + dz * dz);
+  double best_sep = -__builtin_inf(), nx = 0, ny = 0, nz = 0;
+  int best_face = -1, edge_i = -1;
+  for (int k = 0; k < B.F; k++) {
+    double pf[4];
+    B.plane(k, pf);
+    const double mx = pf[0], my = pf[1], mz = pf[2];
+    const double sep = js_min(sa[0] * mx + sa[1] * my + sa[2] * mz, sa[3] * mx + sa[4] * my + sa[5] * mz) - pf[3] - r;
+    if (sep > best_sep) { best_sep = sep; best_face = k; }
+  }
+  for (int i = 0; i < B.E; i++) {
+    double u[3], w[3];
+    B.vert(B.edges[2 * i], u);
+    B.vert(B.edges[2 * i + 1], w);
+    const double tx = w[0] - u[0], ty = w[1] - u[1], tz = w[2] - u[2];
+    const double l = normalize(S, dy * tz - dz * ty, dz * tx - dx * tz, dx * ty - dy * tx);
+    if (!(l > PARALLEL * dl * dsqrt(tx * tx + ty * ty + tz * tz))) continue;
+    double mx = S.unit[0], my = S.unit[1], mz = S.unit[2];
+    if (mx * (u[0] - B.c[0]) + my * (u[1] - B.c[1]) + mz * (u[2] - B.c[2]) < 0) { mx = -mx; my = -my; mz = -mz; }
+    const double top = -B.lowest(-mx, -my, -mz);
+    if (top - (mx * u[0] + my * u[1] + mz * u[2]) > tolerance) continue;
+    const double sep = js_min(sa[0] * mx + sa[1] * my + sa[2] * mz, sa[3] * mx + sa[4] * my + sa[5] * mz) - r - top;
+    if (sep > best_sep + tolerance) { best_sep = sep; edge_i = i; nx = mx; ny = my; nz = mz; }
+  }
+  if (best_sep > margin) return;
+  if (edge_i < 0) {
+    clip_to_hull_face(S, B, best_face, r, margin, out);
+    return;
+  }
+  B.vert(B.edges[2 * edge_i], sb);
+  B.vert(B.edges[2 * edge_i + 1], sb + 3);
+  closest_segments(S, sa, sb);
+  const double px = sa[0] + dx * S.seg_s, py = sa[1] + dy * S.seg_s, pz = sa[2] + dz * S.seg_s;
+  const double qx = sb[0] + (sb[3] - sb[0]) * S.seg_t, qy = sb[1] + (sb[4] - sb[1]) * S.seg_t, qz = sb[2] + (sb[5] - sb[2]) * S.seg_t;
+  const double separation = (px - qx) * nx + (py - qy) * ny + (pz - qz) * nz - r;
+  set_normal(out, -nx, -ny, -nz);
+  const double t = r + separation * 0.5;
+  add_point(out, px - nx * t, py - ny * t, pz - nz * t, separation, mix(5, edge_i, 0, 0));
+}
+
 // src/collide.js:204-242 (collide, pair), for spheres and boxes.
 __device__ __forceinline__ int collide(const Shape& a, const Shape& b, double margin, double tolerance, Manifold& out) {
   Scratch S;
@@ -826,6 +1362,9 @@ __device__ __forceinline__ int collide(const Shape& a, const Shape& b, double ma
   else if (lo.kind == CAPSULE && hi.kind == CAPSULE) capsule_capsule(S, lo, hi, margin, tolerance, out);
   else if (lo.kind == CAPSULE && hi.kind == BOX) capsule_box(S, lo, hi, margin, tolerance, out);
   else if (lo.kind == BOX && hi.kind == BOX) box_box(S, lo, hi, margin, tolerance, out);
+  else if (lo.kind == SPHERE && hi.kind == HULL) sphere_hull(S, lo, hi, margin, out);
+  else if (lo.kind == CAPSULE && hi.kind == HULL) capsule_hull(S, lo, hi, margin, tolerance, out);
+  else if ((lo.kind == BOX || lo.kind == HULL) && hi.kind == HULL) hull_hull(S, lo, hi, margin, tolerance, out);
   else return -1;
   if (swap) { out.normal[0] = -out.normal[0]; out.normal[1] = -out.normal[1]; out.normal[2] = -out.normal[2]; }
   return out.count;
@@ -853,6 +1392,21 @@ __device__ __forceinline__ void gjk_support(const Shape& s, double x, double y, 
   const double* h = s.half;
   if (s.kind == SPHERE) {
     out[0] = c[0]; out[1] = c[1]; out[2] = c[2];
+  } else if (s.kind == HULL) {
+    const int* info = s.table->info + 8 * s.hull;
+    const double* v = s.table->vertices + 3 * info[0];
+    const int V = info[5];
+    const double lx = x * a[0] + y * a[1] + z * a[2], ly = x * a[3] + y * a[4] + z * a[5], lz = x * a[6] + y * a[7] + z * a[8];
+    int best = 0;
+    double most = -__builtin_inf();
+    for (int i = 0; i < V; i++) {
+      const double d = v[3 * i] * lx + v[3 * i + 1] * ly + v[3 * i + 2] * lz;
+      if (d > most) { most = d; best = i; }
+    }
+    const double px = v[3 * best], py = v[3 * best + 1], pz = v[3 * best + 2];
+    out[0] = c[0] + px * a[0] + py * a[3] + pz * a[6];
+    out[1] = c[1] + px * a[1] + py * a[4] + pz * a[7];
+    out[2] = c[2] + px * a[2] + py * a[5] + pz * a[8];
   } else if (s.kind == CAPSULE) {
     const double e = x * a[3] + y * a[4] + z * a[5] >= 0 ? h[1] : -h[1];
     out[0] = c[0] + e * a[3]; out[1] = c[1] + e * a[4]; out[2] = c[2] + e * a[5];
@@ -1207,6 +1761,8 @@ __global__ void collide_kernel(CollideDev d) {
     for (int i = 0; i < 9; i++) s[t].axes[i] = r[4 + i];
     for (int i = 0; i < 3; i++) s[t].half[i] = r[13 + i];
     for (int i = 0; i < 3; i++) s[t].weight[i] = r[16 + i];
+    s[t].hull = s[t].kind == narrow::HULL ? (int)r[13] : -1;
+    s[t].table = &d.hulls;
   }
   narrow::Manifold m;
   m.count = 0;
@@ -1216,6 +1772,7 @@ __global__ void collide_kernel(CollideDev d) {
   for (int i = 0; i < 3 * n; i++) d.points[12 * (long)k + i] = m.points[i];
   for (int i = 0; i < n; i++) d.separations[4 * (long)k + i] = m.separations[i];
   for (int i = 0; i < n; i++) d.ids[4 * (long)k + i] = m.ids[i];
+  d.loss[k] = m.loss;
 }
 
 // One pair a lane: body records (32 numbers) and pair records (margin, tolerance, dt,
@@ -1236,6 +1793,8 @@ __global__ void meet_kernel(MeetDev d) {
     for (int i = 0; i < 4; i++) mo[t].q[i] = r[19 + i];
     for (int i = 0; i < 3; i++) mo[t].v[i] = r[23 + i];
     for (int i = 0; i < 3; i++) mo[t].w[i] = r[26 + i];
+    s[t].hull = s[t].kind == narrow::HULL ? (int)r[13] : -1;
+    s[t].table = &d.hulls;
     mo[t].gravity_scale = r[29];
     mo[t].reach = r[30];
     mo[t].capped = (((int)r[31]) & 2) != 0;
@@ -1255,6 +1814,7 @@ __global__ void meet_kernel(MeetDev d) {
   for (int i = 0; i < n; i++) d.separations[4 * (long)k + i] = m.separations[i];
   for (int i = 0; i < n; i++) d.ids[4 * (long)k + i] = m.ids[i];
   for (int i = 0; i < 12; i++) d.shift[12 * (long)k + i] = shift[i];
+  d.loss[k] = m.loss;
 }
 
 void launch_meet(const MeetDev& d, hipStream_t stream) {

@@ -250,7 +250,7 @@ struct bm_world {
   Pairs pairs;
   Contacts c;
   // Narrowphase batches for the device.
-  std::vector<double> batch_a, batch_b, batch_pair, batch_normal, batch_points, batch_seps, batch_shift;
+  std::vector<double> batch_a, batch_b, batch_pair, batch_normal, batch_points, batch_seps, batch_shift, batch_loss;
   std::vector<int32_t> batch_count, batch_flags;
   std::vector<uint32_t> batch_ids;
   std::vector<Visit> visits;
@@ -727,9 +727,11 @@ int run_batch(bm_world* w) {
   w->batch_seps.resize(4 * n);
   w->batch_ids.resize(4 * n);
   w->batch_shift.resize(12 * n);
+  w->batch_loss.resize(n);
   const int rc = bm_meet_host(w->ctx, n, w->batch_a.data(), w->batch_b.data(), w->batch_pair.data(),
                               w->batch_count.data(), w->batch_normal.data(), w->batch_points.data(),
-                              w->batch_seps.data(), w->batch_ids.data(), w->batch_shift.data(), w->batch_flags.data());
+                              w->batch_seps.data(), w->batch_ids.data(), w->batch_shift.data(), w->batch_flags.data(),
+                              w->batch_loss.data());
   w->batch_a.clear(); w->batch_b.clear(); w->batch_pair.clear();
   if (rc != BM_OK) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx));
   return BM_OK;
@@ -779,7 +781,7 @@ int finish_visit(bm_world* w, Visit& v) {
       w->pairs.forget(v.p);
       return BM_OK;
     }
-    m.loss = 0;
+    m.loss = w->batch_loss[k];
     for (int i = 0; i < 3; i++) m.normal[i] = w->batch_normal[3 * k + i];
     for (int i = 0; i < 12; i++) m.points[i] = w->batch_points[12 * k + i];
     for (int i = 0; i < 4; i++) m.separations[i] = w->batch_seps[4 * k + i];
@@ -1269,7 +1271,7 @@ double advance(bm_world* w, Core& cast, const Core& b, double scale, const doubl
       int32_t count = 0;
       double normal[3], points[12], seps[4];
       uint32_t ids[4];
-      rc = bm_collide_host(w->ctx, 1, ra_rec, rb_rec, &zero, &tolerance, &count, normal, points, seps, ids);
+      rc = bm_collide_host(w->ctx, 1, ra_rec, rb_rec, &zero, &tolerance, &count, normal, points, seps, ids, nullptr);
       if (rc != BM_OK) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx)), -1;
       if (count > 0) {
         for (int k = 0; k < 3; k++) { hit.normal[k] = -normal[k]; hit.point[k] = points[k]; }

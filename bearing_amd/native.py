@@ -57,7 +57,7 @@ class Columns(ctypes.Structure):
 
 
 EXPORTS = ["bm_create", "bm_destroy", "bm_last_error", "bm_upload", "bm_soft_step", "bm_download",
-           "bm_soft_step_host", "bm_last_kernel_ms", "bm_set_launch", "bm_lds_limit", "bm_colour", "bm_collide_host", "bm_meet_host", "bm_set_colour_lds",
+           "bm_soft_step_host", "bm_last_kernel_ms", "bm_set_launch", "bm_lds_limit", "bm_colour", "bm_collide_host", "bm_meet_host", "bm_set_colour_lds", "bm_set_hulls",
            "bm_world_create", "bm_world_destroy", "bm_world_error", "bm_world_add", "bm_world_step", "bm_world_read"]
 
 _lib = None
@@ -79,8 +79,9 @@ def load():
             getattr(lib, name).argtypes = [vp, ctypes.POINTER(Columns)]
         lib.bm_soft_step.argtypes = [vp]
         lib.bm_colour.argtypes = [vp]
-        lib.bm_collide_host.argtypes = [vp, ctypes.c_int32, _D, _D, _D, _D, _I32, _D, _D, _D, _U32]
-        lib.bm_meet_host.argtypes = [vp, ctypes.c_int32, _D, _D, _D, _I32, _D, _D, _D, _U32, _D, _I32]
+        lib.bm_collide_host.argtypes = [vp, ctypes.c_int32, _D, _D, _D, _D, _I32, _D, _D, _D, _U32, _D]
+        lib.bm_meet_host.argtypes = [vp, ctypes.c_int32, _D, _D, _D, _I32, _D, _D, _D, _U32, _D, _I32, _D]
+        lib.bm_set_hulls.argtypes = [vp, ctypes.c_int32, _I32, _I32, _I32, _D, _I32, _I32, _D]
         lib.bm_last_kernel_ms.argtypes = [vp]
         lib.bm_last_kernel_ms.restype = ctypes.c_double
         lib.bm_set_launch.argtypes = [vp, ctypes.c_int32, ctypes.c_int32]
@@ -159,8 +160,21 @@ class Context:
             pass
 
 
-def collide(ctx, a, b, margin, tolerance):
-    """bm_collide_host on flat shape records: (count, normal, points, separations, ids) per pair."""
+def set_hulls(ctx, table):
+    """bm_set_hulls with a hull table: a dict of hullV, hullF, hullL (offsets, n + 1 each),
+    vertices, start, loops, planes."""
+    i32 = lambda x: np.ascontiguousarray(x, np.int32)
+    f64 = lambda x: np.ascontiguousarray(x, np.float64)
+    hv, hf, hl = i32(table["hullV"]), i32(table["hullF"]), i32(table["hullL"])
+    v, st, lp, pl = f64(table["vertices"]), i32(table["start"]), i32(table["loops"]), f64(table["planes"])
+    ctx.check(ctx.lib.bm_set_hulls(ctx.ptr, len(hv) - 1, hv.ctypes.data_as(_I32), hf.ctypes.data_as(_I32),
+                                   hl.ctypes.data_as(_I32), v.ctypes.data_as(_D), st.ctypes.data_as(_I32),
+                                   lp.ctypes.data_as(_I32), pl.ctypes.data_as(_D)), "bm_set_hulls")
+
+
+def collide(ctx, a, b, margin, tolerance, with_loss=False):
+    """bm_collide_host on flat shape records: (count, normal, points, separations, ids) per
+    pair, and each manifold's loss after them when asked."""
     n = len(margin)
     a = np.ascontiguousarray(a, np.float64)
     b = np.ascontiguousarray(b, np.float64)
@@ -171,9 +185,13 @@ def collide(ctx, a, b, margin, tolerance):
     points = np.zeros(12 * n)
     separations = np.zeros(4 * n)
     ids = np.zeros(4 * n, np.uint32)
+    loss = np.zeros(n)
     p = lambda x, k: x.ctypes.data_as(k)
     ctx.check(ctx.lib.bm_collide_host(ctx.ptr, n, p(a, _D), p(b, _D), p(margin, _D), p(tolerance, _D), p(count, _I32),
-                                      p(normal, _D), p(points, _D), p(separations, _D), p(ids, _U32)), "bm_collide_host")
+                                      p(normal, _D), p(points, _D), p(separations, _D), p(ids, _U32), p(loss, _D)),
+              "bm_collide_host")
+    if with_loss:
+        return count, normal, points, separations, ids, loss
     return count, normal, points, separations, ids
 
 
@@ -186,12 +204,12 @@ def meet(ctx, a, b, pair):
     n = pair.size // 6
     count, flags = np.zeros(n, np.int32), np.zeros(n, np.int32)
     normal, points, separations = np.zeros(3 * n), np.zeros(12 * n), np.zeros(4 * n)
-    ids, shift = np.zeros(4 * n, np.uint32), np.zeros(12 * n)
+    ids, shift, loss = np.zeros(4 * n, np.uint32), np.zeros(12 * n), np.zeros(n)
     ctx.check(ctx.lib.bm_meet_host(ctx.ptr, n, a.ctypes.data_as(_D), b.ctypes.data_as(_D), pair.ctypes.data_as(_D),
                                    count.ctypes.data_as(_I32), normal.ctypes.data_as(_D), points.ctypes.data_as(_D),
                                    separations.ctypes.data_as(_D), ids.ctypes.data_as(_U32), shift.ctypes.data_as(_D),
-                                   flags.ctypes.data_as(_I32)), "bm_meet_host")
-    return count, normal, points, separations, ids, shift, flags
+                                   flags.ctypes.data_as(_I32), loss.ctypes.data_as(_D)), "bm_meet_host")
+    return count, normal, points, separations, ids, shift, flags, loss
 
 
 class Batch:

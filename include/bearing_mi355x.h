@@ -157,11 +157,12 @@ int bm_soft_step_host(bm_context* ctx, bm_columns* host);
  * extents; sphere: radius), weight 3 (gravity times gravityScale for a dynamic
  * body, else 0; reducePoints reads it). Out, per pair: count (0-4; -1 for a
  * capsule or hull pair, not handled), normal 3 (a to b), points 12, separations 4,
- * ids 4 (the reference's feature ids). Bit-identical to the reference's collide().
+ * ids 4 (the reference's feature ids), loss (Manifold.loss; may be NULL). Bit-identical
+ * to the reference's collide(). Hulls (kind 3) are those of the table bm_set_hulls set.
  */
 int bm_collide_host(bm_context* ctx, int32_t n, const double* a, const double* b, const double* margin,
                     const double* tolerance, int32_t* count, double* normal, double* points, double* separations,
-                    uint32_t* ids);
+                    uint32_t* ids, double* loss);
 
 /*
  * The narrow driver's meet on the device, one pair a lane: src/contact.js meet
@@ -178,7 +179,18 @@ int bm_collide_host(bm_context* ctx, int32_t n, const double* a, const double* b
  * moment of impact; 2 a time of impact was asked, world._impacts).
  */
 int bm_meet_host(bm_context* ctx, int32_t n, const double* a, const double* b, const double* pair, int32_t* count,
-                 double* normal, double* points, double* separations, uint32_t* ids, double* shift, int32_t* flags);
+                 double* normal, double* points, double* separations, uint32_t* ids, double* shift, int32_t* flags,
+                 double* loss);
+
+/* The hulls shape records name by number (a hull's record: kind 3, its number in place of
+ * half[0]): n hulls in their bodies' frames, as src/hull.js's Hull holds them. Hull h's
+ * corners are vertices[3 hull_v[h] .. 3 hull_v[h+1]), its faces' planes (unit outward
+ * normal, distance) planes[4 hull_f[h] .. ), its face f's corners loops[hull_l[h] +
+ * start[hull_f[h] + f] ..] counter-clockwise from outside. Each edge, its faces, the
+ * inscribed ball and the reach are derived as src/hull.js:45-80 derives them. Up to 64
+ * faces, 96 edges and 16 corners a face. The context keeps the table until the next call. */
+int bm_set_hulls(bm_context* ctx, int32_t n, const int32_t* hull_v, const int32_t* hull_f, const int32_t* hull_l,
+                 const double* vertices, const int32_t* start, const int32_t* loops, const double* planes);
 
 /* How bm_soft_step launches, for tuning (bearing_amd/tune.py picks per workload):
  * shape 0 = 256 threads a workgroup with registers for one wave a SIMD, 1 = 256

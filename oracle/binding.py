@@ -195,15 +195,15 @@ def meet(bodies_a, bodies_b, pair):
     n = pair.size // 6
     count, flags = np.zeros(n, np.int32), np.zeros(n, np.int32)
     normal, points, separations = np.zeros(3 * n), np.zeros(12 * n), np.zeros(4 * n)
-    ids, shift = np.zeros(4 * n, np.uint32), np.zeros(12 * n)
+    ids, shift, loss = np.zeros(4 * n, np.uint32), np.zeros(12 * n), np.zeros(n)
     f = lib().oracle_meet
     f.restype = ctypes.c_int
     rc = f(n, a.ctypes.data_as(_D), b.ctypes.data_as(_D), pair.ctypes.data_as(_D), count.ctypes.data_as(_I32),
            normal.ctypes.data_as(_D), points.ctypes.data_as(_D), separations.ctypes.data_as(_D),
-           ids.ctypes.data_as(_U32), shift.ctypes.data_as(_D), flags.ctypes.data_as(_I32))
+           ids.ctypes.data_as(_U32), shift.ctypes.data_as(_D), flags.ctypes.data_as(_I32), loss.ctypes.data_as(_D))
     if rc != n:
         raise ValueError(f"oracle_meet: pair {-rc - 1} is of a kind not restated")
-    return count, normal, points, separations, ids, shift, flags
+    return count, normal, points, separations, ids, shift, flags, loss
 
 
 def sincos(x):

@@ -1904,7 +1904,7 @@ void oracle_impact(int n, const double* a, const double* b, const double* dt, co
    shift (12 a pair) and flags (1 shifted, 2 a time of impact was asked). Returns n, or
    -(k+1) for pair k of a kind not restated here. */
 int oracle_meet(int n, const double* a, const double* b, const double* pair, int32_t* count, double* normal,
-                double* points, double* separations, uint32_t* ids, double* shift, int32_t* flags) {
+                double* points, double* separations, uint32_t* ids, double* shift, int32_t* flags, double* loss) {
   for (int k = 0; k < n; k++) {
     Shape sa, sb;
     Motion ma, mb;
@@ -1923,6 +1923,7 @@ int oracle_meet(int n, const double* a, const double* b, const double* pair, int
     for (int i = 0; i < 4; i++) separations[4 * k + i] = m.separations[i];
     for (int i = 0; i < 4; i++) ids[4 * k + i] = m.ids[i];
     for (int i = 0; i < 12; i++) shift[12 * k + i] = sh[i];
+    if (loss) loss[k] = m.loss;
   }
   return n;
 }

@@ -12,9 +12,12 @@
 
 extern "C" int oracle_collide(int n, const double* a, const double* b, const double* margin, const double* tolerance,
                               int32_t* count, double* normal, double* points, double* separations, uint32_t* ids);
+extern "C" void oracle_losses(double* out);
 extern "C" int oracle_meet(int n, const double* a, const double* b, const double* pair, int32_t* count,
                            double* normal, double* points, double* separations, uint32_t* ids, double* shift,
-                           int32_t* flags);
+                           int32_t* flags, double* loss);
+extern "C" void oracle_hulls(int n, const int32_t* hull_v, const int32_t* hull_f, const int32_t* hull_l,
+                             const double* vertices, const int32_t* start, const int32_t* loops, const double* planes);
 
 struct bm_context {
   bm_columns h;
@@ -123,14 +126,22 @@ int bm_soft_step(bm_context* ctx) {
 
 int bm_collide_host(bm_context*, int32_t n, const double* a, const double* b, const double* margin,
                     const double* tolerance, int32_t* count, double* normal, double* points, double* separations,
-                    uint32_t* ids) {
+                    uint32_t* ids, double* loss) {
   oracle_collide(n, a, b, margin, tolerance, count, normal, points, separations, ids);
+  if (loss) oracle_losses(loss);
+  return BM_OK;
+}
+
+int bm_set_hulls(bm_context*, int32_t n, const int32_t* hull_v, const int32_t* hull_f, const int32_t* hull_l,
+                 const double* vertices, const int32_t* start, const int32_t* loops, const double* planes) {
+  oracle_hulls(n, hull_v, hull_f, hull_l, vertices, start, loops, planes);
   return BM_OK;
 }
 
 int bm_meet_host(bm_context* ctx, int32_t n, const double* a, const double* b, const double* pair, int32_t* count,
-                 double* normal, double* points, double* separations, uint32_t* ids, double* shift, int32_t* flags) {
-  if (oracle_meet(n, a, b, pair, count, normal, points, separations, ids, shift, flags) != n) {
+                 double* normal, double* points, double* separations, uint32_t* ids, double* shift, int32_t* flags,
+                 double* loss) {
+  if (oracle_meet(n, a, b, pair, count, normal, points, separations, ids, shift, flags, loss) != n) {
     ctx->error = "oracle meet: a pair of kinds it does not cover";
     return BM_ERR_UNSUPPORTED;
   }

@@ -84,3 +84,46 @@ def test_gpu_capsules_match_oracle_on_fresh_pairs(ctx, seed):
     errs = compare_manifolds(fx, *got)
     assert not errs, "\n".join(errs)
     assert (want[0] > 0).sum() > 5000
+
+
+
+def test_gpu_hulls_match_reference(ctx):
+    # Hulls built by the reference's own src/hull.js build(); manifolds and loss bit for bit.
+    fx = fixtures.load(fixtures.DATA + "/collide_hull.2026.bmfx.gz")
+    native.set_hulls(ctx, fx)
+    sh = fx["shapes"].reshape(-1, 2, 19)
+    got = native.collide(ctx, sh[:, 0].copy(), sh[:, 1].copy(), fx["margin"], fx["tolerance"], with_loss=True)
+    errs = compare_manifolds(fx, *got[:5])
+    assert not errs, "\n".join(errs)
+    assert np.array_equal(got[5].view(np.uint64), np.asarray(fx["loss"]).view(np.uint64))
+
+
+@pytest.mark.parametrize("seed", [27, 28])
+def test_gpu_hulls_match_oracle_on_fresh_placements(ctx, seed):
+    # The same hulls, placed afresh: every shape moved and turned at random, pairs brought
+    # within reach of each other.
+    fx = fixtures.load(fixtures.DATA + "/collide_hull.2026.bmfx.gz")
+    native.set_hulls(ctx, fx)
+    binding.hulls(fx)
+    from make_fixtures import _axes
+    rng = np.random.default_rng(seed)
+    sh = fx["shapes"].reshape(-1, 2, 19).copy()
+    n = len(sh)
+    for t in range(2):
+        q = rng.normal(size=(n, 4))
+        q /= np.linalg.norm(q, axis=1)[:, None]
+        turn = sh[:, t, 0] != 0
+        sh[turn, t, 4:13] = np.array([_axes(x) for x in q[turn]])
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    sh[:, 1, 1:4] = sh[:, 0, 1:4] + d * rng.uniform(0.0, 0.9, (n, 1))
+    margin = rng.uniform(0, 0.3, n)
+    tol = np.asarray(fx["tolerance"])
+    want = binding.collide(sh[:, 0].copy(), sh[:, 1].copy(), margin, tol)
+    want_loss = binding.losses(n)
+    got = native.collide(ctx, sh[:, 0].copy(), sh[:, 1].copy(), margin, tol, with_loss=True)
+    ref = {"count": want[0], "normal": want[1], "points": want[2], "separations": want[3], "ids": want[4]}
+    errs = compare_manifolds(ref, *got[:5])
+    assert not errs, "\n".join(errs)
+    assert np.array_equal(got[5].view(np.uint64), want_loss.view(np.uint64))
+    assert (want[0] > 0).sum() > 300

@@ -28,7 +28,7 @@ def bits(a):
 
 
 def compare(got, want):
-    names = ["count", "normal", "points", "separations", "ids", "shift", "flags"]
+    names = ["count", "normal", "points", "separations", "ids", "shift", "flags", "loss"]
     count = want[0]
     mask = {"points": 12, "separations": 4, "ids": 4}
     for name, g, w in zip(names, got, want):
