@@ -1,5 +1,5 @@
 // A whole world stepped as the reference steps it (src/world.js step, 880-1072), for
-// bodies that are boxes, spheres and capsules, no joints, characters, sensors or compounds.
+// bodies that are boxes, spheres, capsules and hulls, no joints, characters, sensors or compounds.
 //
 // Which parts run where. The order-defining serial bookkeeping stays on the host, in the
 // reference's exact order: the AABB trees and fat boxes (src/broadphase.js, src/tree.js),
@@ -24,7 +24,7 @@ namespace {
 
 constexpr int P = 4;
 constexpr int DYNAMIC = 0, STATIC = 1;
-constexpr int BOX = 0, SPHERE = 1, CAPSULE = 2;
+constexpr int BOX = 0, SPHERE = 1, CAPSULE = 2, HULL = 3;
 constexpr int MOVING_TREE = 1, STATIC_TREE = 0;
 constexpr double PIXEL = 1.0 / 200;      // src/contact.js:125
 constexpr double FAT = 1.0 / 4;          // src/broadphase.js:32
@@ -231,6 +231,17 @@ struct bm_world {
       box, motion, start, start_q, tumbled, ceiling, gravity_scale, damping, angular_damping, size, reach, friction,
       restitution, still, rest, least;
   std::vector<uint8_t> type, shape, frozen, calm, lock, framed, awake, fast, caught;
+  // A hull body's hull (its number in the table), and a framed body's offset and frame.
+  std::vector<int32_t> hull;
+  std::vector<double> offset, frame;
+  // The hull table, as the host reads it (corners for the catch's sweeps, each hull's box
+  // in its own frame and its thinnest extent); the device holds its own copy.
+  struct HostHull {
+    int v0, V;
+    double lo[3], hi[3], size;
+  };
+  std::vector<HostHull> hulls;
+  std::vector<double> hull_vertices;
   std::vector<int32_t> joint_head, in_awake, awake_list, island, island_next, parent, slept;
   std::vector<float> transforms;
   int awake_count = 0, frozen_count = 0, slept_count = 0;
@@ -266,9 +277,39 @@ int fail(bm_world* w, int code, const std::string& m) {
   return code;
 }
 
+// src/world.js:1917-1930 (_pose): slot s placed at its own origin and turned its own way,
+// from the step's centre of mass and principal axes; the product is src/core/math.js:230-237
+// (quatMultiply) of the rotation by the frame's inverse.
+void pose_of(const bm_world* w, int s, double* out) {
+  const double* p = &w->position[3 * s];
+  const double* q = &w->rotation[4 * s];
+  if (w->framed[s] == 0) {
+    out[0] = p[0]; out[1] = p[1]; out[2] = p[2];
+    out[3] = q[0]; out[4] = q[1]; out[5] = q[2]; out[6] = q[3];
+    return;
+  }
+  const double* o = &w->offset[3 * s];
+  const double* f = &w->frame[4 * s];
+  double v[3];
+  rotate(q, o[0], o[1], o[2], false, v);
+  out[0] = p[0] - v[0]; out[1] = p[1] - v[1]; out[2] = p[2] - v[2];
+  const double ax = q[0], ay = q[1], az = q[2], aw = q[3];
+  const double bx = -f[0], by = -f[1], bz = -f[2], bw = f[3];
+  out[3] = aw * bx + ax * bw + ay * bz - az * by;
+  out[4] = aw * by + ay * bw + az * bx - ax * bz;
+  out[5] = aw * bz + az * bw + ax * by - ay * bx;
+  out[6] = aw * bw - ax * bx - ay * by - az * bz;
+}
+
+void write_transform(bm_world* w, int s) {
+  double pose[7];
+  pose_of(w, s, pose);
+  for (int k = 0; k < 7; k++) w->transforms[7 * s + k] = (float)pose[k];
+}
+
 // ---- bodies ----
 
-// src/world.js:1580-1630 (_boundsOf) for a box or a sphere.
+// src/world.js:1580-1630 (_boundsOf) for a box, a sphere, a capsule or a hull.
 void bounds_of(bm_world* w, int s, double dt, double gl) {
   const int s3 = 3 * s, s4 = 4 * s, a9 = 9 * s, b6 = 6 * s;
   double* ax = &w->axes[a9];
@@ -283,9 +324,18 @@ void bounds_of(bm_world* w, int s, double dt, double gl) {
     m = speed * dt;
   }
   w->motion[s] = m;
-  double ex, ey, ez;
-  if (w->shape[s] == BOX) {
-    const double hx = w->dims[s3] * 0.5, hy = w->dims[s3 + 1] * 0.5, hz = w->dims[s3 + 2] * 0.5;
+  double ex, ey, ez, ox = 0, oy = 0, oz = 0;
+  if (w->shape[s] == BOX || w->shape[s] == HULL) {
+    double hx = w->dims[s3] * 0.5, hy = w->dims[s3 + 1] * 0.5, hz = w->dims[s3 + 2] * 0.5;
+    if (w->shape[s] == HULL) {
+      // The box round its corners in its own frame, turned into the world's.
+      const auto& h = w->hulls[w->hull[s]];
+      const double cx = (h.lo[0] + h.hi[0]) * 0.5, cy = (h.lo[1] + h.hi[1]) * 0.5, cz = (h.lo[2] + h.hi[2]) * 0.5;
+      hx = (h.hi[0] - h.lo[0]) * 0.5; hy = (h.hi[1] - h.lo[1]) * 0.5; hz = (h.hi[2] - h.lo[2]) * 0.5;
+      ox = ax[0] * cx + ax[3] * cy + ax[6] * cz;
+      oy = ax[1] * cx + ax[4] * cy + ax[7] * cz;
+      oz = ax[2] * cx + ax[5] * cy + ax[8] * cz;
+    }
     ex = std::fabs(ax[0]) * hx + std::fabs(ax[3]) * hy + std::fabs(ax[6]) * hz;
     ey = std::fabs(ax[1]) * hx + std::fabs(ax[4]) * hy + std::fabs(ax[7]) * hz;
     ez = std::fabs(ax[2]) * hx + std::fabs(ax[5]) * hy + std::fabs(ax[8]) * hz;
@@ -296,7 +346,7 @@ void bounds_of(bm_world* w, int s, double dt, double gl) {
     ex = std::fabs(ax[3]) * hl + r; ey = std::fabs(ax[4]) * hl + r; ez = std::fabs(ax[5]) * hl + r;
   }
   const double grow = m + w->size[s] * PIXEL;
-  const double cx = w->position[s3] + 0, cy = w->position[s3 + 1] + 0, cz = w->position[s3 + 2] + 0;
+  const double cx = w->position[s3] + ox, cy = w->position[s3 + 1] + oy, cz = w->position[s3 + 2] + oz;
   double* box = &w->box[b6];
   box[0] = cx - ex - grow; box[1] = cy - ey - grow; box[2] = cz - ez - grow;
   box[3] = cx + ex + grow; box[4] = cy + ey + grow; box[5] = cz + ez + grow;
@@ -695,10 +745,12 @@ void carry(bm_world* w, int p, int m) {
 // rotation 4, velocity 3, spin 3, gravity scale (0 unless dynamic), reach, flags
 // (1 dynamic, 2 spin capped at pi/dt: dynamic and not a plain sphere).
 void load(const bm_world* w, int s, double* r) {
-  r[0] = w->shape[s] == BOX ? 2.0 : w->shape[s] == CAPSULE ? 1.0 : 0.0;
+  r[0] = w->shape[s] == BOX ? 2.0 : w->shape[s] == CAPSULE ? 1.0 : w->shape[s] == HULL ? 3.0 : 0.0;
   for (int k = 0; k < 3; k++) r[1 + k] = w->position[3 * s + k];
   for (int k = 0; k < 9; k++) r[4 + k] = w->axes[9 * s + k];
-  if (w->shape[s] == BOX) {
+  if (w->shape[s] == HULL) {
+    r[13] = w->hull[s]; r[14] = 0; r[15] = 0;
+  } else if (w->shape[s] == BOX) {
     for (int k = 0; k < 3; k++) r[13 + k] = w->dims[3 * s + k] * 0.5;
   } else if (w->shape[s] == CAPSULE) {
     r[13] = w->dims[3 * s]; r[14] = (w->dims[3 * s + 1] - 2 * w->dims[3 * s]) * 0.5; r[15] = 0;
@@ -990,23 +1042,41 @@ void sleep_step(bm_world* w, double dt) {
 // ---- the continuous catch: src/world.js:1683-1743 (_catch), src/query.js:355-445 and
 // 607-642 (advance, sweep), src/gjk.js (closest), src/contact.js:1755-1776 (gapOf) ----
 
-constexpr int CORE_SPHERE = 0, CORE_CAPSULE = 1, CORE_BOX = 2;  // src/collide.js:44 kinds
+constexpr int CORE_SPHERE = 0, CORE_CAPSULE = 1, CORE_BOX = 2, CORE_HULL = 3;  // src/collide.js:44 kinds
 
-// A shape as the sweep places it: kind, centre, axes (x, y, z columns), half extents.
+// A shape as the sweep places it: kind, centre, axes (x, y, z columns), half extents; a
+// hull's number, corners (V of them, in its own frame) and thinnest extent.
 struct Core {
   int kind;
   double center[3], axes[9], half[3], weight[3];
+  int hull = -1, V = 0;
+  const double* vertices = nullptr;
+  double hull_size = 0;
 };
 
 double core_radius(const Core& s) { return s.kind == CORE_SPHERE || s.kind == CORE_CAPSULE ? s.half[0] : 0; }
 
-// src/gjk.js:70-107 (support) for a sphere's, a capsule's or a box's core.
+// src/gjk.js:70-107 (support) for a sphere's, a capsule's, a box's or a hull's core.
 void core_support(const Core& s, double x, double y, double z, double* out) {
   const double* c = s.center;
   const double* a = s.axes;
   const double* h = s.half;
   if (s.kind == CORE_SPHERE) {
     out[0] = c[0]; out[1] = c[1]; out[2] = c[2];
+  } else if (s.kind == CORE_HULL) {
+    // The corner farthest along, in the hull's own frame; the first of equals.
+    const double* v = s.vertices;
+    const double lx = x * a[0] + y * a[1] + z * a[2], ly = x * a[3] + y * a[4] + z * a[5], lz = x * a[6] + y * a[7] + z * a[8];
+    int best = 0;
+    double most = -INFINITY;
+    for (int i = 0; i < s.V; i++) {
+      const double d = v[3 * i] * lx + v[3 * i + 1] * ly + v[3 * i + 2] * lz;
+      if (d > most) { most = d; best = i; }
+    }
+    const double px = v[3 * best], py = v[3 * best + 1], pz = v[3 * best + 2];
+    out[0] = c[0] + px * a[0] + py * a[3] + pz * a[6];
+    out[1] = c[1] + px * a[1] + py * a[4] + pz * a[7];
+    out[2] = c[2] + px * a[2] + py * a[5] + pz * a[8];
   } else if (s.kind == CORE_CAPSULE) {
     const double e = x * a[3] + y * a[4] + z * a[5] >= 0 ? h[1] : -h[1];
     out[0] = c[0] + e * a[3]; out[1] = c[1] + e * a[4]; out[2] = c[2] + e * a[5];
@@ -1171,7 +1241,16 @@ void place_at(Core& s, double x, double y, double z, const double* q) {
 // src/query.js:91-95 (place): slot s's shape where it is now.
 void place_body(const bm_world* w, int s, Core& out) {
   const int s3 = 3 * s;
-  if (w->shape[s] == BOX) {
+  out.hull = -1;
+  if (w->shape[s] == HULL) {
+    const auto& h = w->hulls[w->hull[s]];
+    out.kind = CORE_HULL;
+    out.hull = w->hull[s];
+    out.V = h.V;
+    out.vertices = &w->hull_vertices[3 * (size_t)h.v0];
+    out.hull_size = h.size;
+    out.half[0] = out.half[1] = out.half[2] = 0;
+  } else if (w->shape[s] == BOX) {
     out.kind = CORE_BOX;
     for (int k = 0; k < 3; k++) out.half[k] = w->dims[s3 + k] * 0.5;
   } else if (w->shape[s] == CAPSULE) {
@@ -1186,7 +1265,17 @@ void place_body(const bm_world* w, int s, Core& out) {
   place_at(out, w->position[s3], w->position[s3 + 1], w->position[s3 + 2], &w->rotation[4 * s]);
 }
 
-// src/query.js:114-128 (extents) for a sphere, a capsule or a box.
+// src/collide.js:168-177 (reachAlong) for a hull: how far its corners reach along unit (x, y, z).
+double hull_reach(const Core& s, double x, double y, double z) {
+  const double* A = s.axes;
+  const double* v = s.vertices;
+  const double lx = A[0] * x + A[1] * y + A[2] * z, ly = A[3] * x + A[4] * y + A[5] * z, lz = A[6] * x + A[7] * y + A[8] * z;
+  double most = -INFINITY;
+  for (int i = 0; i < s.V; i++) most = js_max(most, v[3 * i] * lx + v[3 * i + 1] * ly + v[3 * i + 2] * lz);
+  return most;
+}
+
+// src/query.js:114-128 (extents) for a sphere, a capsule, a box or a hull.
 void extents(const Core& s, double* out) {
   const double* a = s.axes;
   const double* h = s.half;
@@ -1194,6 +1283,10 @@ void extents(const Core& s, double* out) {
     out[0] = out[1] = out[2] = h[0];
   } else if (s.kind == CORE_CAPSULE) {
     for (int k = 0; k < 3; k++) out[k] = std::fabs(a[3 + k]) * h[1] + h[0];
+  } else if (s.kind == CORE_HULL) {
+    for (int k = 0; k < 3; k++)
+      out[k] = js_max(hull_reach(s, k == 0 ? 1 : 0, k == 1 ? 1 : 0, k == 2 ? 1 : 0),
+                      hull_reach(s, k == 0 ? -1 : 0, k == 1 ? -1 : 0, k == 2 ? -1 : 0));
   } else {
     for (int k = 0; k < 3; k++)
       out[k] = std::fabs(a[k]) * h[0] + std::fabs(a[3 + k]) * h[1] + std::fabs(a[6 + k]) * h[2];
@@ -1224,6 +1317,7 @@ const std::vector<int32_t>& strays(bm_world* w) {
 
 double core_size(const Core& s) {
   if (s.kind == CORE_SPHERE || s.kind == CORE_CAPSULE) return 2 * s.half[0];
+  if (s.kind == CORE_HULL) return s.hull_size;
   return 2 * js_min(js_min(s.half[0], s.half[1]), s.half[2]);
 }
 
@@ -1265,6 +1359,7 @@ double advance(bm_world* w, Core& cast, const Core& b, double scale, const doubl
         for (int j = 0; j < 3; j++) r[1 + j] = c.center[j];
         for (int j = 0; j < 9; j++) r[4 + j] = c.axes[j];
         for (int j = 0; j < 3; j++) r[13 + j] = c.half[j];
+        if (c.kind == CORE_HULL) r[13] = c.hull;
         for (int j = 0; j < 3; j++) r[16 + j] = c.weight[j];
       }
       const double zero = 0;
@@ -1515,8 +1610,15 @@ int bm_world_add(bm_world* w, const bm_bodies* b) {
         !b->inertia || !b->inv_inertia || !b->mass || !b->inv_mass || !b->gravity_scale || !b->damping ||
         !b->angular_damping || !b->size || !b->reach || !b->friction || !b->restitution)
       return fail(w, BM_ERR_ARGUMENT, "addBody: every column is needed");
-    if (b->shape[i] != BOX && b->shape[i] != SPHERE && b->shape[i] != CAPSULE)
-      return fail(w, BM_ERR_UNSUPPORTED, "addBody: this world takes boxes, spheres and capsules");
+    if (b->shape[i] != BOX && b->shape[i] != SPHERE && b->shape[i] != CAPSULE && b->shape[i] != HULL)
+      return fail(w, BM_ERR_UNSUPPORTED, "addBody: this world takes boxes, spheres, capsules and hulls");
+    if (b->shape[i] == HULL && (!b->hull || b->hull[i] < 0 || b->hull[i] >= (int)w->hulls.size()))
+      return fail(w, BM_ERR_ARGUMENT, "addBody: a hull body names no hull of the table (bm_world_set_hulls)");
+    const bool framed = b->framed && b->framed[i] != 0;
+    if (framed && (!b->offset || !b->frame))
+      return fail(w, BM_ERR_ARGUMENT, "addBody: a framed body needs its offset and frame");
+    if (framed && b->shape[i] != HULL)
+      return fail(w, BM_ERR_UNSUPPORTED, "addBody: a framed box, sphere or capsule has parts, which this world does not take");
     if (b->type[i] != DYNAMIC && b->type[i] != STATIC)
       return fail(w, BM_ERR_UNSUPPORTED, "addBody: this world takes dynamic and static bodies");
   }
@@ -1524,7 +1626,8 @@ int bm_world_add(bm_world* w, const bm_bodies* b) {
   auto grow = [&](auto& v, int per) { v.resize((size_t)per * top); };
   for (auto* v : {&w->position, &w->velocity, &w->angular, &w->dims, &w->inertia, &w->inv_inertia, &w->start, &w->tumbled})
     grow(*v, 3);
-  for (auto* v : {&w->rotation, &w->start_q}) grow(*v, 4);
+  for (auto* v : {&w->rotation, &w->start_q, &w->frame}) grow(*v, 4);
+  grow(w->offset, 3); grow(w->hull, 1);
   grow(w->inv_i, 6); grow(w->axes, 9); grow(w->box, 6); grow(w->rest, 7);
   for (auto* v : {&w->mass, &w->inv_mass, &w->motion, &w->ceiling, &w->gravity_scale, &w->damping, &w->angular_damping,
                   &w->size, &w->reach, &w->friction, &w->restitution, &w->still, &w->least})
@@ -1555,13 +1658,54 @@ int bm_world_add(bm_world* w, const bm_bodies* b) {
     w->angular_damping[s] = b->angular_damping[i]; w->size[s] = b->size[i]; w->reach[s] = b->reach[i];
     w->friction[s] = b->friction[i]; w->restitution[s] = b->restitution[i];
     w->type[s] = b->type[i]; w->shape[s] = b->shape[i]; w->lock[s] = b->lock[i];
+    w->hull[s] = b->shape[i] == HULL ? b->hull[i] : -1;
+    w->framed[s] = b->framed && b->framed[i] != 0 ? 1 : 0;
+    for (int k = 0; k < 3; k++) w->offset[3 * s + k] = w->framed[s] ? b->offset[3 * i + k] : 0;
+    for (int k = 0; k < 4; k++) w->frame[4 * s + k] = w->framed[s] ? b->frame[4 * i + k] : (k == 3 ? 1 : 0);
     // addBody (src/world.js:563-578): placed, bounded, filed, awake unless static.
-    for (int k = 0; k < 7; k++) w->transforms[7 * s + k] = (float)(k < 3 ? w->position[3 * s + k] : w->rotation[4 * s + k - 3]);
+    write_transform(w, s);
     for (int k = 0; k < 3; k++) w->start[3 * s + k] = w->position[3 * s + k];
     bounds_of(w, s, 0, 0);
     broadphase_add(w, s);
     if (w->type[s] != STATIC) awaken(w, s);
   }
+  return BM_OK;
+}
+
+int bm_world_set_hulls(bm_world* w, int32_t n, const int32_t* hull_v, const int32_t* hull_f, const int32_t* hull_l,
+                       const double* vertices, const int32_t* start, const int32_t* loops, const double* planes) {
+  if (!w) return BM_ERR_ARGUMENT;
+  w->error.clear();
+  for (int s = 0; s < w->top; s++)
+    if (w->shape[s] == HULL && w->hull[s] >= n)
+      return fail(w, BM_ERR_ARGUMENT, "bm_world_set_hulls: a body already added names a hull the table lacks");
+  // The device's copy first: it checks the table's layout and limits.
+  if (int rc = bm_set_hulls(w->ctx, n, hull_v, hull_f, hull_l, vertices, start, loops, planes))
+    return fail(w, rc, bm_last_error(w->ctx));
+  std::vector<bm_world::HostHull> hulls(n);
+  std::vector<double> hv;
+  for (int h = 0; h < n; h++) {
+    auto& H = hulls[h];
+    H.v0 = (int)(hv.size() / 3);
+    H.V = hull_v[h + 1] - hull_v[h];
+    const double* v = vertices + 3 * (size_t)hull_v[h];
+    hv.insert(hv.end(), v, v + 3 * H.V);
+    // src/hull.js:63-83: its box in its own frame, and its thinnest extent across a face.
+    for (int k = 0; k < 3; k++) { H.lo[k] = INFINITY; H.hi[k] = -INFINITY; }
+    for (int i = 0; i < H.V; i++)
+      for (int k = 0; k < 3; k++) { H.lo[k] = js_min(H.lo[k], v[3 * i + k]); H.hi[k] = js_max(H.hi[k], v[3 * i + k]); }
+    double size = INFINITY;
+    const double* p = planes + 4 * (size_t)hull_f[h];
+    for (int f = 0; f < hull_f[h + 1] - hull_f[h]; f++) {
+      const double nx = p[4 * f], ny = p[4 * f + 1], nz = p[4 * f + 2];
+      double low = INFINITY;
+      for (int i = 0; i < H.V; i++) low = js_min(low, nx * v[3 * i] + ny * v[3 * i + 1] + nz * v[3 * i + 2]);
+      size = js_min(size, p[4 * f + 3] - low);
+    }
+    H.size = size;
+  }
+  w->hulls.swap(hulls);
+  w->hull_vertices.swap(hv);
   return BM_OK;
 }
 
@@ -1596,8 +1740,7 @@ int bm_world_step(bm_world* w, double dt) {
     const int s = w->awake_list[i];
     if (w->frozen[s] == 1) continue;
     if (w->frozen[s] == 2) w->frozen[s] = 1;
-    for (int k = 0; k < 3; k++) w->transforms[7 * s + k] = (float)w->position[3 * s + k];
-    for (int k = 0; k < 4; k++) w->transforms[7 * s + 3 + k] = (float)w->rotation[4 * s + k];
+    write_transform(w, s);
   }
   w->step_dt = 0;
   if (w->sleeping) sleep_step(w, dt);

@@ -58,7 +58,8 @@ class Columns(ctypes.Structure):
 
 EXPORTS = ["bm_create", "bm_destroy", "bm_last_error", "bm_upload", "bm_soft_step", "bm_download",
            "bm_soft_step_host", "bm_last_kernel_ms", "bm_set_launch", "bm_lds_limit", "bm_colour", "bm_collide_host", "bm_meet_host", "bm_set_colour_lds", "bm_set_hulls",
-           "bm_world_create", "bm_world_destroy", "bm_world_error", "bm_world_add", "bm_world_step", "bm_world_read"]
+           "bm_world_create", "bm_world_destroy", "bm_world_error", "bm_world_add", "bm_world_step", "bm_world_read",
+           "bm_world_set_hulls"]
 
 _lib = None
 
@@ -98,7 +99,7 @@ class Bodies(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int32), ("type", _U8), ("shape", _U8), ("lock", _U8)] + [
         (f, _D) for f in ("position", "rotation", "velocity", "angular", "dims", "inertia", "inv_inertia", "mass",
                           "inv_mass", "gravity_scale", "damping", "angular_damping", "size", "reach", "friction",
-                          "restitution")]
+                          "restitution")] + [("hull", _I32), ("framed", _U8), ("offset", _D), ("frame", _D)]
 
 
 class WorldState(ctypes.Structure):
@@ -119,6 +120,7 @@ def world_signatures(lib):
     lib.bm_world_add.argtypes = [vp, ctypes.POINTER(Bodies)]
     lib.bm_world_step.argtypes = [vp, ctypes.c_double]
     lib.bm_world_read.argtypes = [vp, ctypes.POINTER(WorldState)]
+    lib.bm_world_set_hulls.argtypes = [vp, ctypes.c_int32, _I32, _I32, _I32, _D, _I32, _I32, _D]
 
 
 class BearingError(RuntimeError):
@@ -281,7 +283,10 @@ BODY_COLUMNS = [("type", 1, np.uint8), ("shape", 1, np.uint8), ("lock", 1, np.ui
                 ("dims", 3, np.float64), ("inertia", 3, np.float64), ("inv_inertia", 3, np.float64),
                 ("mass", 1, np.float64), ("inv_mass", 1, np.float64), ("gravity_scale", 1, np.float64),
                 ("damping", 1, np.float64), ("angular_damping", 1, np.float64), ("size", 1, np.float64),
-                ("reach", 1, np.float64), ("friction", 1, np.float64), ("restitution", 1, np.float64)]
+                ("reach", 1, np.float64), ("friction", 1, np.float64), ("restitution", 1, np.float64),
+                ("hull", 1, np.int32), ("framed", 1, np.uint8), ("offset", 3, np.float64), ("frame", 4, np.float64)]
+# Columns a world of unframed boxes, spheres and capsules can leave out (passed as NULL).
+OPTIONAL_COLUMNS = {"hull", "framed", "offset", "frame"}
 
 
 class World:
@@ -310,12 +315,25 @@ class World:
         n = len(columns["type"])
         keep, b = [], Bodies(n=n)
         for name, per, dtype in BODY_COLUMNS:
+            if name in OPTIONAL_COLUMNS and columns.get(name) is None:
+                continue
             arr = np.ascontiguousarray(columns[name], dtype)
             if arr.size != per * n:
                 raise ValueError(f"{name}: {arr.size} numbers for {n} bodies")
             keep.append(arr)
-            setattr(b, name, arr.ctypes.data_as(_U8 if dtype == np.uint8 else _D))
+            setattr(b, name, arr.ctypes.data_as(_U8 if dtype == np.uint8 else _I32 if dtype == np.int32 else _D))
         self.check(self.lib.bm_world_add(self.ptr, ctypes.byref(b)), "bm_world_add")
+
+    def set_hulls(self, table):
+        """bm_world_set_hulls with a hull table: a dict of hullV, hullF, hullL (offsets, n + 1
+        each), vertices, start, loops, planes, as set_hulls takes."""
+        i32 = lambda x: np.ascontiguousarray(x, np.int32)
+        f64 = lambda x: np.ascontiguousarray(x, np.float64)
+        hv, hf, hl = i32(table["hullV"]), i32(table["hullF"]), i32(table["hullL"])
+        v, st, lp, pl = f64(table["vertices"]), i32(table["start"]), i32(table["loops"]), f64(table["planes"])
+        self.check(self.lib.bm_world_set_hulls(self.ptr, len(hv) - 1, hv.ctypes.data_as(_I32), hf.ctypes.data_as(_I32),
+                                               hl.ctypes.data_as(_I32), v.ctypes.data_as(_D), st.ctypes.data_as(_I32),
+                                               lp.ctypes.data_as(_I32), pl.ctypes.data_as(_D)), "bm_world_set_hulls")
 
     def step(self, dt):
         self.check(self.lib.bm_world_step(self.ptr, dt), "bm_world_step")

@@ -16,6 +16,8 @@ Scenes:
   pile_world   one world of the batched workload: a tower or a sphere pile
   throw        a thrown box landing and hitting a 6x6 wall of unit boxes
   capsules     capsules, boxes and balls dropped in a heap on a floor
+  rocks        hulls of random points with boxes, balls and capsules dropped in a heap, and
+               small balls fired through two static hull boulders
   darts        small balls and rods fired at 20-200 m/s at a 2 cm static plate and the
                floor: bodies the step would put through them, which the continuous catch
                puts back (src/world.js:1636-1681)
@@ -219,6 +221,46 @@ def capsule_heap(rng: SplitMix64, count: int = 48):
     return bodies
 
 
+def _cloud(rng: SplitMix64, points: int, sigma: float):
+    """A hull's vertices: `points` Gaussian points (sigma per axis), flat x, y, z."""
+    return [rng.gauss(sigma) for _ in range(3 * points)]
+
+
+def rocks(rng: SplitMix64, count: int = 40):
+    """`count` rocks, each the hull of 12 Gaussian points (sigma U[0.15, 0.35]), turned any
+    way and dropped at rest from x, z in [-2.5, 2.5], y in [0.5, 7]; a box of sides
+    U[0.2, 0.6], a ball of radius U[0.1, 0.25] and a capsule after every fifth; two static
+    boulders (hulls of 16 points, sigma 0.5) on a static 20x1x20 floor, and four balls of
+    radius 0.03 fired at each boulder at 40-70 m/s, which the step would put through it."""
+    bodies = [_ground((20.0, 1.0, 20.0))]
+    boulders = [(-3.5, 0.5, 0.0), (3.5, 0.5, 0.0)]
+    for c in boulders:
+        bodies.append({"type": "static", "shape": "hull", "vertices": _cloud(rng, 16, 0.5), "position": list(c)})
+    for k in range(count):
+        pos = [rng.between(-2.5, 2.5), rng.between(0.5, 7.0), rng.between(-2.5, 2.5)]
+        q = [rng.gauss(1.0) for _ in range(4)]
+        n = math.sqrt(sum(x * x for x in q))
+        bodies.append({"shape": "hull", "vertices": _cloud(rng, 12, rng.between(0.15, 0.35)), "position": pos,
+                       "rotation": [x / n for x in q]})
+        if k % 5 == 4:
+            bodies.append({"shape": "box", "size": [rng.between(0.2, 0.6) for _ in range(3)],
+                           "position": [rng.between(-2.5, 2.5), rng.between(0.5, 7.0), rng.between(-2.5, 2.5)]})
+            bodies.append({"shape": "sphere", "radius": rng.between(0.1, 0.25),
+                           "position": [rng.between(-2.5, 2.5), rng.between(0.5, 7.0), rng.between(-2.5, 2.5)]})
+            r = rng.between(0.05, 0.15)
+            bodies.append({"shape": "capsule", "radius": r, "height": 2 * r + 2 * rng.between(0.1, 0.4),
+                           "position": [rng.between(-2.5, 2.5), rng.between(0.5, 7.0), rng.between(-2.5, 2.5)]})
+    for bx, by, bz in boulders:
+        side = -1.0 if bx < 0 else 1.0
+        for j in range(4):
+            y = by + rng.between(-0.2, 0.3)
+            z = bz + rng.between(-0.25, 0.25)
+            speed = rng.between(40.0, 70.0)
+            bodies.append({"shape": "sphere", "radius": 0.03, "position": [bx + side * 3.0, y, z],
+                           "velocity": [-side * speed, 0.0, 0.0]})
+    return bodies
+
+
 SCENES = {
     "pyramid": lambda: pyramid(),
     "rain": lambda: rain(),
@@ -228,6 +270,7 @@ SCENES = {
     "throw": lambda: throw(SplitMix64(5)),
     "darts": lambda: darts(SplitMix64(7)),
     "capsules": lambda: capsule_heap(SplitMix64(9)),
+    "rocks": lambda: rocks(SplitMix64(11)),
 }
 
 

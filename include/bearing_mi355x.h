@@ -216,7 +216,7 @@ double bm_last_kernel_ms(const bm_context* ctx);
 
 /* ---- a whole world, stepped as src/world.js step (880-1072) steps it ----
  *
- * For worlds of dynamic and static boxes, spheres and capsules, with no joints, sensors,
+ * For worlds of dynamic and static boxes, spheres, capsules and hulls, with no joints, sensors,
  * characters or compounds. The host keeps the reference's order-defining serial
  * state (the broadphase trees, the pair list, the narrow driver's order, the
  * sleeping islands); the device runs each step's narrowphase batches, colouring
@@ -228,13 +228,23 @@ typedef struct bm_world bm_world;
 /* Bodies as addBody resolves them (src/world.js:563-578, _write 2152-2273), one
  * entry a body: the columns the reference keeps by slot. type: 0 dynamic,
  * 1 static; shape: 0 box (dims = full size), 1 sphere (dims[0] = radius),
- * 2 capsule (dims[0] = radius, dims[1] = height end to end). */
+ * 2 capsule (dims[0] = radius, dims[1] = height end to end), 3 hull (hull = its
+ * number in the table bm_world_set_hulls set, in the frame below).
+ * position and rotation are the step's (world._position, _rotation): a framed body's
+ * centre of mass and principal axes. framed, offset (3) and frame (4) are
+ * world._framed, _offset and _frame (src/world.js:2199-2205): the transforms give a
+ * framed body's own origin and turn back as src/world.js:1917-1930 (_pose) does. A
+ * framed box, sphere or capsule is a body with parts, which this world does not take.
+ * hull may be NULL when no body is a hull; framed, offset and frame when none is framed. */
 typedef struct bm_bodies {
   int32_t n;
   const uint8_t *type, *shape, *lock;
   const double *position, *rotation, *velocity, *angular, *dims, *inertia, *inv_inertia; /* 3, 4, 3, 3, 3, 3, 3 */
   const double *mass, *inv_mass, *gravity_scale, *damping, *angular_damping, *size, *reach, *friction,
       *restitution;
+  const int32_t* hull;
+  const uint8_t* framed;
+  const double *offset, *frame; /* 3, 4 */
 } bm_bodies;
 
 /* What bm_world_read copies out; any pointer may be NULL. transforms: 7 floats a
@@ -254,6 +264,12 @@ int bm_world_destroy(bm_world* world);
 const char* bm_world_error(const bm_world* world);
 /* addBody (src/world.js:563) for each body, slots in order. */
 int bm_world_add(bm_world* world, const bm_bodies* bodies);
+/* The hulls the world's hull bodies name (their world._inner hulls, src/hull.js Hull),
+ * in the layout and with the limits of bm_set_hulls. Replaces the table; fails if a body
+ * already added names a hull the new table lacks. */
+int bm_world_set_hulls(bm_world* world, int32_t n, const int32_t* hull_v, const int32_t* hull_f,
+                       const int32_t* hull_l, const double* vertices, const int32_t* start, const int32_t* loops,
+                       const double* planes);
 /* world.step(dt) (src/world.js:880). */
 int bm_world_step(bm_world* world, double dt);
 int bm_world_read(const bm_world* world, bm_world_state* out);

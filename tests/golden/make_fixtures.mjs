@@ -108,6 +108,31 @@ function main() {
   bodies(world, ['_position', '_rotation', '_velocity', '_angular', '_dims', '_mass', '_invMass', '_inertia',
     '_invInertia', '_type', '_shape', '_framed', '_lock', '_gravityScale', '_damping', '_angularDamping', '_size',
     '_reach', '_friction', '_restitution', '_tumbled', '_ceiling'], 'init.', init);
+  // Hull bodies: each one's hull as the step holds it (world._inner, moved to its centre of
+  // mass and principal axes), numbered in slot order, and the frame that gives its origin back.
+  const top0 = world._slots.top;
+  if (Array.from(world._shape.subarray(0, top0)).some((k) => k === 3)) {
+    const hullOf = new Int32Array(top0).fill(-1);
+    const t = { vertices: [], start: [], loops: [], planes: [], hullV: [0], hullF: [0], hullL: [0] };
+    for (let s = 0; s < top0; s++) {
+      if (world._shape[s] !== 3) continue;
+      const h = world._inner[s];
+      hullOf[s] = t.hullV.length - 1;
+      for (const x of h.vertices) t.vertices.push(x);
+      for (let f = 0; f < h.start.length - 1; f++) t.start.push(h.start[f]);
+      for (const x of h.loops) t.loops.push(x);
+      for (const x of h.planes) t.planes.push(x);
+      t.hullV.push(t.vertices.length / 3);
+      t.hullF.push(t.planes.length / 4);
+      t.hullL.push(t.loops.length);
+    }
+    init['init.hull'] = hullOf;
+    init['init.offset'] = copy(world._offset, 3 * top0);
+    init['init.frame'] = copy(world._frame, 4 * top0);
+    init['hulls.vertices'] = Float64Array.from(t.vertices);
+    init['hulls.planes'] = Float64Array.from(t.planes);
+    for (const k of ['start', 'loops', 'hullV', 'hullF', 'hullL']) init['hulls.' + k] = Int32Array.from(t[k]);
+  }
   const softAt = new Set(job.soft || []), colourAt = new Set(job.colour || []), traceAt = job.trace || -1;
   let rec = null, meta = null, trace = null, traceCount = 0, colourRec = null;
   const snap = (tag) => {

@@ -29,6 +29,7 @@ JOBS = {
     "pile_small": dict(scene="pile_small", dt=1 / 60, steps=150, soft=[30, 70, 110], colour=[70], trace=100, every=50),
     "darts": dict(scene="darts", dt=1 / 60, steps=90, soft=[2, 20], colour=[], trace=-1, every=30),
     "capsules": dict(scene="capsules", dt=1 / 60, steps=240, soft=[60, 120], colour=[120], trace=-1, every=60),
+    "rocks": dict(scene="rocks", dt=1 / 60, steps=240, soft=[60, 120], colour=[120], trace=-1, every=60),
     # The 2,000-sphere rain at full size: its solve phase mid-landing is the bench's workload.
     "rain": dict(scene="rain", dt=1 / 60, steps=900, soft=[300], colour=[], trace=-1, every=300),
 }

@@ -6,7 +6,7 @@ import pytest
 
 import fixtures
 from bearing_amd import native
-from worldcheck import init_columns, run
+from worldcheck import init_columns, make_world, refuses, run
 
 pytestmark = pytest.mark.gpu
 
@@ -15,12 +15,11 @@ def traj(name):
     return fixtures.load(os.path.join(fixtures.DATA, f"{name}.traj.bmfx.gz"))
 
 
-@pytest.mark.parametrize("name", ["pyramid", "tower", "throw", "rain_small", "pile_small", "darts", "capsules", "rain"])
+@pytest.mark.parametrize("name", ["pyramid", "tower", "throw", "rain_small", "pile_small", "darts", "capsules", "rocks", "rain"])
 def test_gpu_world_matches_reference_trajectory(name):
     print("trajectory", name, flush=True)
     fx = traj(name)
-    w = native.World(0)
-    w.add(init_columns(fx))
+    w = make_world(fx, device=0)
     matched, why = run(w, fx)
     state = w.read()
     w.close()
@@ -30,11 +29,4 @@ def test_gpu_world_matches_reference_trajectory(name):
 
 
 def test_gpu_world_refuses_what_it_does_not_handle():
-    # A hull (shape 3) is not a shape this world takes: addBody fails loudly.
-    fx = traj("tower")
-    cols = init_columns(fx)
-    cols["shape"] = cols["shape"].copy()
-    cols["shape"][3] = 3
-    w = native.World(0)
-    with pytest.raises(native.BearingError, match="boxes, spheres and capsules"):
-        w.add(cols)
+    refuses(lambda: native.World(0))

@@ -10,7 +10,7 @@ import pytest
 
 import fixtures
 from bearing_amd import native
-from worldcheck import init_columns, run
+from worldcheck import init_columns, make_world, refuses, run
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "hostworld", "_build", "libhostworld.so")
@@ -26,11 +26,10 @@ def traj(name):
     return fixtures.load(os.path.join(fixtures.DATA, f"{name}.traj.bmfx.gz"))
 
 
-@pytest.mark.parametrize("name", ["pyramid", "tower", "throw", "rain_small", "pile_small", "darts", "capsules", "rain"])
+@pytest.mark.parametrize("name", ["pyramid", "tower", "throw", "rain_small", "pile_small", "darts", "capsules", "rocks", "rain"])
 def test_host_world_matches_reference_trajectory(lib, name):
     fx = traj(name)
-    w = native.World(lib=lib)
-    w.add(init_columns(fx))
+    w = make_world(fx, lib=lib)
     matched, why = run(w, fx)
     assert why is None, why
     assert matched == fx.meta["steps"]
@@ -41,11 +40,4 @@ def test_host_world_matches_reference_trajectory(lib, name):
 
 
 def test_host_world_refuses_what_it_does_not_handle(lib):
-    # A hull (shape 3) is not a shape this world takes: addBody fails loudly.
-    fx = traj("tower")
-    cols = init_columns(fx)
-    cols["shape"] = cols["shape"].copy()
-    cols["shape"][3] = 3
-    w = native.World(lib=lib)
-    with pytest.raises(native.BearingError, match="boxes, spheres and capsules"):
-        w.add(cols)
+    refuses(lambda: native.World(lib=lib))

@@ -2,6 +2,9 @@
 world.transforms with the reference's (SHA-256 of the Float32Array, per step), and its
 awake, contact and frozen counts."""
 import hashlib
+import os
+
+import pytest
 
 import numpy as np
 
@@ -13,9 +16,27 @@ CAMEL = {"inv_inertia": "invInertia", "inv_mass": "invMass", "gravity_scale": "g
 
 
 def init_columns(fx):
-    """addBody's resolved columns of a trajectory fixture, as World.add takes them."""
+    """addBody's resolved columns of a trajectory fixture, as World.add takes them; the
+    hull and frame columns only where the fixture has hull bodies."""
     return {name: np.asarray(fx["init." + CAMEL.get(name, name)]).astype(dtype)
-            for name, _, dtype in native.BODY_COLUMNS}
+            for name, _, dtype in native.BODY_COLUMNS if "init." + CAMEL.get(name, name) in fx}
+
+
+def hull_table(fx):
+    """The hull table of a trajectory fixture with hull bodies, or None."""
+    if "hulls.hullV" not in fx:
+        return None
+    return {k: fx["hulls." + k] for k in ("hullV", "hullF", "hullL", "vertices", "start", "loops", "planes")}
+
+
+def make_world(fx, **kw):
+    """A native.World holding a trajectory fixture's bodies (and hulls), as addBody left them."""
+    w = native.World(**kw)
+    table = hull_table(fx)
+    if table is not None:
+        w.set_hulls(table)
+    w.add(init_columns(fx))
+    return w
 
 
 def run(world, fx, steps=None):
@@ -32,3 +53,29 @@ def run(world, fx, steps=None):
             return s - 1, f"step {s}: transforms {'match' if got == meta['hashes'][s - 1] else 'differ'}, " \
                           f"awake/contacts/frozen {counts}, reference {want}"
     return n, None
+
+
+def refuses(new_world):
+    """addBody fails loudly on what a world does not take: a compound (shape 4), a hull
+    naming no hull of the table, and a framed box (a body with parts)."""
+    import fixtures
+    fx = fixtures.load(os.path.join(fixtures.DATA, "tower.traj.bmfx.gz"))
+    cols = init_columns(fx)
+    cols["shape"] = cols["shape"].copy()
+    cols["shape"][3] = 4
+    w = new_world()
+    with pytest.raises(native.BearingError, match="boxes, spheres, capsules and hulls"):
+        w.add(cols)
+    cols["shape"][3] = 3
+    cols["hull"] = np.zeros(len(cols["shape"]), np.int32)
+    with pytest.raises(native.BearingError, match="names no hull"):
+        w.add(cols)
+    cols = init_columns(fx)
+    cols["framed"] = cols["framed"].copy()
+    cols["framed"][3] = 1
+    cols["offset"] = np.zeros(3 * len(cols["shape"]))
+    cols["frame"] = np.tile([0.0, 0.0, 0.0, 1.0], len(cols["shape"]))
+    with pytest.raises(native.BearingError, match="has parts"):
+        w.add(cols)
+    assert w.read()["top"] == 0
+    w.close()
