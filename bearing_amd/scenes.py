@@ -221,6 +221,28 @@ def capsule_heap(rng: SplitMix64, count: int = 48):
     return bodies
 
 
+def value_noise(rng: SplitMix64, columns: int, rows: int, spacing: float = 1.0, amplitude: float = 2.0,
+                cell: float = 8.0):
+    """Heights for a heightfield, row by row: bilinear value noise, lattice values
+    U[-1, 1] * amplitude every `cell` metres (row-major lattice, drawn in order)."""
+    nx = int(math.ceil((columns - 1) * spacing / cell)) + 1
+    nz = int(math.ceil((rows - 1) * spacing / cell)) + 1
+    lattice = [[rng.between(-1.0, 1.0) * amplitude for _ in range(nx)] for _ in range(nz)]
+    heights = []
+    for j in range(rows):
+        z = j * spacing / cell
+        j0 = min(int(math.floor(z)), nz - 2)
+        fz = z - j0
+        for i in range(columns):
+            x = i * spacing / cell
+            i0 = min(int(math.floor(x)), nx - 2)
+            fx = x - i0
+            a, b = lattice[j0][i0], lattice[j0][i0 + 1]
+            c, d = lattice[j0 + 1][i0], lattice[j0 + 1][i0 + 1]
+            heights.append((a * (1.0 - fx) + b * fx) * (1.0 - fz) + (c * (1.0 - fx) + d * fx) * fz)
+    return heights
+
+
 def _cloud(rng: SplitMix64, points: int, sigma: float):
     """A hull's vertices: `points` Gaussian points (sigma per axis), flat x, y, z."""
     return [rng.gauss(sigma) for _ in range(3 * points)]

@@ -228,3 +228,46 @@ def losses(n):
     out = np.zeros(n)
     lib().oracle_losses(out.ctypes.data_as(_D))
     return out
+
+
+def fields(columns, rows, spacing, heights):
+    """oracle_fields: heightfields by number, field f with columns[f] x rows[f] samples
+    spacing[f] apart, heights[f] its samples row by row."""
+    n = len(columns)
+    cols, rws = np.ascontiguousarray(columns, np.int32), np.ascontiguousarray(rows, np.int32)
+    sp = _f64(spacing)
+    off = np.zeros(n + 1, np.int64)
+    off[1:] = np.cumsum([len(h) for h in heights])
+    flat = _f64(np.concatenate([np.asarray(h, np.float64) for h in heights]) if n else np.zeros(0))
+    lib().oracle_fields(n, cols.ctypes.data_as(_I32), rws.ctypes.data_as(_I32), sp.ctypes.data_as(_D),
+                        off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), flat.ctypes.data_as(_D))
+
+
+def field_edges(f, count):
+    """oracle_field_edges: field f's triangles' edge flags (count) and folds (6 each)."""
+    flags = np.zeros(count, np.int32)
+    folds = np.zeros(6 * count)
+    lib().oracle_field_edges(f, count, flags.ctypes.data_as(_I32), folds.ctypes.data_as(_D))
+    return flags, folds
+
+
+def level(field, position, axes, shapes, boxes, margin, tolerance, cap):
+    """oracle_level (touchLevel) for each shape record against its field: manifolds per
+    shape, then count, normal, points, separations, ids in cap slots a shape."""
+    n = len(margin)
+    field = np.ascontiguousarray(field, np.int32)
+    position, axes, shapes, boxes = _f64(position), _f64(axes), _f64(shapes), _f64(boxes)
+    margin, tolerance = _f64(margin), _f64(tolerance)
+    manifolds = np.zeros(n, np.int32)
+    count = np.zeros(cap * n, np.int32)
+    normal = np.zeros(3 * cap * n)
+    points = np.zeros(12 * cap * n)
+    seps = np.zeros(4 * cap * n)
+    ids = np.zeros(4 * cap * n, np.uint32)
+    p = lambda x, t: x.ctypes.data_as(t)
+    rc = lib().oracle_level(n, p(field, _I32), p(position, _D), p(axes, _D), p(shapes, _D), p(boxes, _D), p(margin, _D),
+                            p(tolerance, _D), cap, p(manifolds, _I32), p(count, _I32), p(normal, _D), p(points, _D),
+                            p(seps, _D), p(ids, ctypes.POINTER(ctypes.c_uint32)))
+    if rc != n:
+        raise RuntimeError(f"oracle_level: shape {-rc - 1} gave more than {cap} manifolds")
+    return manifolds, count, normal, points, seps, ids

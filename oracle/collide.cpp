@@ -20,6 +20,7 @@ using oracle::js_max;
 using oracle::js_min;
 
 constexpr int SPHERE = 0, CAPSULE = 1, BOX = 2, HULL = 3;
+constexpr int TRIANGLE_KIND = 4;  // src/collide.js:49 (TRIANGLE)
 constexpr int MAX_POINTS = 4;
 constexpr double PARALLEL = 7.450580596923828e-9;  // 2^-27, src/collide.js:686
 
@@ -32,6 +33,7 @@ struct Shape {
   double half[3];
   double weight[3];
   const Hull* hull = nullptr;  // a hull's tables, in its body's frame
+  double points[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // a triangle's corners, in the world
 };
 
 struct Manifold {
@@ -1423,13 +1425,23 @@ const uint8_t FACES[16] = {0, 1, 2, 3, 0, 1, 3, 2, 0, 2, 3, 1, 1, 2, 3, 0};
 // src/gjk.js:66-68 (radius): what the core leaves out of a round shape.
 double radius(const Shape& s) { return s.kind == SPHERE || s.kind == CAPSULE ? s.half[0] : 0; }
 
-// src/gjk.js:70-107 (support) for a sphere's, a capsule's or a box's core.
+// src/gjk.js:70-107 (support) for a sphere's, a triangle's, a hull's, a capsule's or a box's core.
 void gjk_support(const Shape& s, double x, double y, double z, double* out) {
   const double* c = s.center;
   const double* a = s.axes;
   const double* h = s.half;
   if (s.kind == SPHERE) {
     out[0] = c[0]; out[1] = c[1]; out[2] = c[2];
+  } else if (s.kind == TRIANGLE_KIND) {
+    // The corner farthest along; the first of equals.
+    const double* p = s.points;
+    int best = 0;
+    double most = -INFINITY;
+    for (int i = 0; i < 3; i++) {
+      const double d = p[3 * i] * x + p[3 * i + 1] * y + p[3 * i + 2] * z;
+      if (d > most) { most = d; best = i; }
+    }
+    out[0] = p[3 * best]; out[1] = p[3 * best + 1]; out[2] = p[3 * best + 2];
   } else if (s.kind == HULL) {
     const std::vector<double>& v = s.hull->vertices;
     const int V = s.hull->V;
@@ -1601,6 +1613,13 @@ double gjk_closest(const Shape& a, const Shape& b, double* near_a, double* near_
   near_b[0] = bx; near_b[1] = by; near_b[2] = bz;
   return std::sqrt(vx * vx + vy * vy + vz * vz);
 }
+
+// ---- level geometry: src/mesh.js, src/triangle.js ----
+
+#include "level.h"
+
+// The heightfields level records name (oracle_fields), by number.
+std::vector<Heightfield> g_fields;
 
 // ---- time of impact, src/impact.js ----
 
@@ -1929,6 +1948,49 @@ int oracle_meet(int n, const double* a, const double* b, const double* pair, int
 }
 
 /* src/core/math.js:102-122, for checking against the reference's own values. */
+/* Heightfield tables: n fields, field f's heights heights[off[f] ..] (rows[f] rows of
+   columns[f]), samples spacing[f] apart (src/mesh.js:288 Heightfield). */
+void oracle_fields(int n, const int32_t* columns, const int32_t* rows, const double* spacing, const int64_t* off,
+                   const double* heights) {
+  g_fields.assign(n, Heightfield());
+  for (int f = 0; f < n; f++) g_fields[f].init(heights + off[f], columns[f], rows[f], spacing[f]);
+}
+
+/* Field f's edge flags and folds for triangles [0, count): flags 1 a triangle, folds 6. */
+void oracle_field_edges(int f, int count, int32_t* flags, double* folds) {
+  for (int t = 0; t < count; t++) flags[t] = g_fields.at(f).edge(t, folds + 6 * t);
+}
+
+/* touchLevel (src/triangle.js:1078) for n shapes: shape k (a 19-number record) against
+   field[k] placed at position 3k with axes 9k, looking in box 6k (the field's frame: lo 3,
+   hi 3), margin and tolerance. Out: manifolds[k], then cap manifold slots a shape: count,
+   normal 3, points 12, separations 4, ids 4. Returns n, or -(k+1) when shape k gives more
+   than cap. */
+int oracle_level(int n, const int32_t* field, const double* position, const double* axes, const double* shapes,
+                 const double* boxes, const double* margin, const double* tolerance, int cap, int32_t* manifolds,
+                 int32_t* count, double* normal, double* points, double* separations, uint32_t* ids) {
+  LevelToucher L;
+  std::vector<Manifold> out;
+  for (int k = 0; k < n; k++) {
+    Shape s;
+    read_shape(shapes + 19 * k, s);
+    const int got = L.touch_level(g_fields.at(field[k]), position + 3 * k, axes + 9 * k, s, boxes + 6 * k, margin[k],
+                                  tolerance[k], out);
+    if (got > cap) return -(k + 1);
+    manifolds[k] = got;
+    for (int j = 0; j < got; j++) {
+      const size_t o = (size_t)k * cap + j;
+      const Manifold& m = out[j];
+      count[o] = m.count;
+      for (int i = 0; i < 3; i++) normal[3 * o + i] = m.normal[i];
+      for (int i = 0; i < 12; i++) points[12 * o + i] = i < 3 * m.count ? m.points[i] : 0;
+      for (int i = 0; i < 4; i++) separations[4 * o + i] = i < m.count ? m.separations[i] : 0;
+      for (int i = 0; i < 4; i++) ids[4 * o + i] = i < m.count ? m.ids[i] : 0;
+    }
+  }
+  return n;
+}
+
 void oracle_sincos(int n, const double* x, double* s, double* c) {
   for (int k = 0; k < n; k++) {
     s[k] = oracle::js_sin(x[k]);

@@ -7,6 +7,7 @@ the lower.mjs loader once per job. The fixtures are what the tests read; this
 script is not run by any test.
 """
 import json
+import math
 import os
 import subprocess
 import sys
@@ -272,6 +273,95 @@ def run_hulls(seed=2026, n=1200):
     os.remove(path)
 
 
+def level_cases(seed, n):
+    """Seeded shapes against heightfields, for touchLevel (src/triangle.js:1078): three fields
+    of value noise (24 x 20 samples, 0.75 m apart; a flat plateau for seams, a raised crest
+    line for sharp folds), placed at the origin, moved, and moved and turned about y; spheres,
+    capsules, boxes and hulls (16 points N(0, 0.3^2)) at any turn, from above the surface to
+    sunk into it, each with the box level() would look in (its own box grown by the margin,
+    in the field's frame). JSON-ready."""
+    import numpy as np
+    from bearing_amd import scenes
+    rng = np.random.default_rng(seed)
+    fields = []
+    for f in range(3):
+        cols, rows, sp = 24, 20, 0.75
+        h = scenes.value_noise(scenes.SplitMix64(100 + f), cols, rows, sp, 1.0, 4.0)
+        for j in range(rows):
+            for i in range(cols):
+                if i < 8 and j < 8:
+                    h[j * cols + i] = 0.25
+                elif i == 16:
+                    h[j * cols + i] += 0.5
+        fields.append({"columns": cols, "rows": rows, "spacing": sp, "heights": h})
+    poses = [((0.0, 0.0, 0.0), 0.0), ((1.5, -0.2, 2.25), 0.0), ((-0.75, 0.4, 0.5), math.pi / 6)]
+
+    def axes_of(q):
+        x, y, z, w = q
+        return [1 - 2 * (y * y + z * z), 2 * (x * y + z * w), 2 * (x * z - y * w),
+                2 * (x * y - z * w), 1 - 2 * (x * x + z * z), 2 * (y * z + x * w),
+                2 * (x * z + y * w), 2 * (y * z - x * w), 1 - 2 * (x * x + y * y)]
+
+    def quat():
+        r = rng.random()
+        if r < 0.25:
+            return [0.0, 0.0, 0.0, 1.0]
+        if r < 0.4:   # turned about y only: faces level
+            t = rng.uniform(0, math.pi)
+            return [0.0, math.sin(t / 2), 0.0, math.cos(t / 2)]
+        v = rng.normal(size=4)
+        return list(v / np.linalg.norm(v))
+
+    def surface(f, x, z):
+        fd = fields[f]
+        cols, rows, sp = fd["columns"], fd["rows"], fd["spacing"]
+        x0, z0 = -((cols - 1) * sp) / 2, -((rows - 1) * sp) / 2
+        i = min(max(int((x - x0) / sp), 0), cols - 1)
+        j = min(max(int((z - z0) / sp), 0), rows - 1)
+        return fd["heights"][j * cols + i]
+
+    cases = []
+    for k in range(n):
+        f = k % 3
+        pos, yaw = poses[(k // 3) % 3]
+        q0 = [0.0, math.sin(yaw / 2), 0.0, math.cos(yaw / 2)]
+        A = axes_of(q0)
+        kind = int(rng.integers(0, 4))
+        if kind == 0:
+            r = rng.uniform(0.1, 0.5)
+            shape, reach = {"kind": 0, "half": [r, 0.0, 0.0]}, r
+        elif kind == 1:
+            r, hl = rng.uniform(0.05, 0.2), rng.uniform(0.1, 0.5)
+            shape, reach = {"kind": 1, "half": [r, hl, 0.0]}, r + hl
+        elif kind == 2:
+            hh = rng.uniform(0.1, 0.6, 3)
+            shape, reach = {"kind": 2, "half": [float(x) for x in hh]}, float(np.linalg.norm(hh))
+        else:
+            pts = rng.normal(0, 0.3, (16, 3))
+            shape, reach = {"kind": 3, "points": [float(x) for x in pts.reshape(-1)]}, float(np.linalg.norm(pts, axis=1).max())
+        lx, lz = rng.uniform(-7.5, 7.5), rng.uniform(-6.0, 6.0)
+        ly = surface(f, lx, lz) + rng.uniform(-0.3, 0.5) * reach
+        centre = [pos[0] + A[0] * lx + A[3] * ly + A[6] * lz, pos[1] + A[1] * lx + A[4] * ly + A[7] * lz,
+                  pos[2] + A[2] * lx + A[5] * ly + A[8] * lz]
+        shape["center"] = centre
+        shape["q"] = quat() if kind != 0 else [0.0, 0.0, 0.0, 1.0]
+        margin = float(rng.choice([0.0, rng.uniform(0, 0.05), rng.uniform(0, 0.2)]))
+        g = reach + margin
+        box = [lx - g, ly - g, lz - g, lx + g, ly + g, lz + g]
+        cases.append({"field": f, "position": list(pos), "axes": A, "shape": shape, "box": box, "margin": margin})
+    return {"fields": fields, "cases": cases}
+
+
+def run_level(seed=2026, n=2400):
+    path = os.path.join(DATA, "level.in.json")
+    with open(path, "w") as f:
+        json.dump(level_cases(seed, n), f)
+    subprocess.run(["node", "--no-warnings", "--experimental-loader", os.path.join(HERE, "lower.mjs"),
+                    os.path.join(HERE, "make_level.mjs"), path, os.path.join(DATA, f"level.{seed}.bmfx.gz")],
+                   check=True)
+    os.remove(path)
+
+
 def run_collide(seed=2026, n=4000, capsules=False):
     pairs = capsule_pairs(seed, n) if capsules else collide_pairs(seed, n)
     name = "collide_capsule" if capsules else "collide"
@@ -361,6 +451,9 @@ if __name__ == "__main__":
         sys.exit(0)
     if sys.argv[1:] == ["collide_capsules"]:
         run_collide(n=3000, capsules=True)
+        sys.exit(0)
+    if sys.argv[1:] == ["level"]:
+        run_level()
         sys.exit(0)
     if sys.argv[1:] == ["impact"]:
         run_impact()
