@@ -129,6 +129,15 @@ struct bm_context {
   Scratch hull_ints, hull_doubles;
   bm::HullTable hulls{};
   int32_t hull_count = 0;
+  // The heightfields (bm_set_fields): info and offsets, spacing and corner, heights; and a
+  // host copy of each one's shape, for sizing touchLevel's scratch.
+  Scratch field_ints, field_doubles, level_scratch;
+  bm::FieldTable fields{};
+  struct FieldShape {
+    int columns, rows;
+    double spacing, x0, z0;
+  };
+  std::vector<FieldShape> field_shapes;
   std::vector<int32_t> key_offsets;
 };
 
@@ -774,6 +783,157 @@ int bm_set_launch(bm_context* ctx, int32_t shape, int32_t resident) {
   if (shape < 0 || shape >= bm::SOFT_SHAPES) return fail(ctx, BM_ERR_ARGUMENT, "bm_set_launch: shape must be 0, 1 or 2");
   ctx->shape = shape;
   ctx->resident = resident ? 1 : 0;
+  return BM_OK;
+}
+
+int bm_set_fields(bm_context* ctx, int32_t n, const int32_t* columns, const int32_t* rows, const double* spacing,
+                  const int64_t* offsets, const double* heights) {
+  if (!ctx) return BM_ERR_ARGUMENT;
+  ctx->error.clear();
+  if (n < 0 || (n > 0 && (!columns || !rows || !spacing || !offsets || !heights)))
+    return fail(ctx, BM_ERR_ARGUMENT, "bm_set_fields: null argument");
+  std::vector<int32_t> info(2 * (size_t)n + 2);
+  std::vector<long long> off((size_t)n + 1);
+  std::vector<double> dinfo(3 * (size_t)n + 3);
+  std::vector<bm_context::FieldShape> shapes(n);
+  for (int f = 0; f < n; f++) {
+    if (columns[f] < 2 || rows[f] < 2 || !(spacing[f] > 0) || offsets[f] < 0)
+      return fail(ctx, BM_ERR_ARGUMENT, "bm_set_fields: a heightfield has 2 or more rows and columns, spacing above 0");
+    // src/mesh.js:296-297: centred on its body, x along columns, z along rows.
+    const double x0 = -((columns[f] - 1) * spacing[f]) / 2, z0 = -((rows[f] - 1) * spacing[f]) / 2;
+    info[2 * f] = columns[f]; info[2 * f + 1] = rows[f];
+    off[f] = offsets[f];
+    dinfo[3 * f] = spacing[f]; dinfo[3 * f + 1] = x0; dinfo[3 * f + 2] = z0;
+    shapes[f] = {columns[f], rows[f], spacing[f], x0, z0};
+  }
+  size_t total = 0;
+  for (int f = 0; f < n; f++) total = std::max(total, (size_t)offsets[f] + (size_t)columns[f] * rows[f]);
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->device), "bm_set_fields: hipSetDevice");
+  const size_t ib = info.size() * 4, ob = off.size() * 8, db = dinfo.size() * 8, hb = total * 8;
+  if (int rc = grow(ctx, ctx->field_ints, ib + ob + 16, "bm_set_fields")) return rc;
+  if (int rc = grow(ctx, ctx->field_doubles, db + hb + 16, "bm_set_fields")) return rc;
+  char* fi = static_cast<char*>(ctx->field_ints.p);
+  char* fd = static_cast<char*>(ctx->field_doubles.p);
+  const size_t ob_at = (ib + 7) / 8 * 8;
+  HIP_OR_FAIL(ctx, hipMemcpy(fi, info.data(), ib, hipMemcpyHostToDevice), "bm_set_fields: copy");
+  HIP_OR_FAIL(ctx, hipMemcpy(fi + ob_at, off.data(), ob, hipMemcpyHostToDevice), "bm_set_fields: copy");
+  HIP_OR_FAIL(ctx, hipMemcpy(fd, dinfo.data(), db, hipMemcpyHostToDevice), "bm_set_fields: copy");
+  if (hb) HIP_OR_FAIL(ctx, hipMemcpy(fd + db, heights, hb, hipMemcpyHostToDevice), "bm_set_fields: copy");
+  ctx->fields.n = n;
+  ctx->fields.info = reinterpret_cast<const int*>(fi);
+  ctx->fields.off = reinterpret_cast<const long long*>(fi + ob_at);
+  ctx->fields.dinfo = reinterpret_cast<const double*>(fd);
+  ctx->fields.heights = reinterpret_cast<const double*>(fd + db);
+  ctx->field_shapes.swap(shapes);
+  return BM_OK;
+}
+
+int bm_level_host(bm_context* ctx, int32_t n, const int32_t* field, const double* position, const double* axes,
+                  const double* shapes, const double* boxes, const double* margin, const double* tolerance, int32_t cap,
+                  int32_t* manifolds, int32_t* count, double* normal, double* points, double* separations,
+                  uint32_t* ids) {
+  if (!ctx) return BM_ERR_ARGUMENT;
+  ctx->error.clear();
+  if (n < 0 || cap < 1) return fail(ctx, BM_ERR_ARGUMENT, "bm_level_host: n must be 0 or more, cap 1 or more");
+  if (n == 0) return BM_OK;
+  if (!field || !position || !axes || !shapes || !boxes || !margin || !tolerance || !manifolds || !count || !normal ||
+      !points || !separations || !ids)
+    return fail(ctx, BM_ERR_ARGUMENT, "bm_level_host: null argument");
+  // Each lane's scratch: 7 points for each triangle under its box (src/mesh.js:388-401).
+  std::vector<long long> scratch((size_t)n + 1, 0);
+  for (int32_t k = 0; k < n; k++) {
+    const double kind = shapes[19 * (size_t)k];
+    if (!(kind == 0 || kind == 1 || kind == 2 || kind == 3))
+      return fail(ctx, BM_ERR_ARGUMENT, "bm_level_host: a shape's kind must be 0, 1, 2 or 3");
+    if (!hull_ok(ctx, shapes + 19 * (size_t)k))
+      return fail(ctx, BM_ERR_ARGUMENT, "bm_level_host: a hull's number is not in the hull table (bm_set_hulls)");
+    if (field[k] < 0 || field[k] >= (int32_t)ctx->field_shapes.size())
+      return fail(ctx, BM_ERR_ARGUMENT, "bm_level_host: a field's number is not in the field table (bm_set_fields)");
+    const auto& F = ctx->field_shapes[field[k]];
+    const double* b = boxes + 6 * (size_t)k;
+    const double i0 = host_max(0, std::floor((b[0] - F.x0) / F.spacing)), i1 = host_min(F.columns - 2, std::floor((b[3] - F.x0) / F.spacing));
+    const double j0 = host_max(0, std::floor((b[2] - F.z0) / F.spacing)), j1 = host_min(F.rows - 2, std::floor((b[5] - F.z0) / F.spacing));
+    long long tris = 0;
+    if (i0 <= i1 && j0 <= j1) tris = 2 * (long long)(i1 - i0 + 1) * (long long)(j1 - j0 + 1);
+    scratch[k + 1] = scratch[k] + 7 * tris;
+  }
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->device), "bm_level_host: hipSetDevice");
+  const size_t N = (size_t)n, C = (size_t)cap;
+  const size_t in_d = (3 + 9 + 19 + 6 + 2) * N * 8, in_i = N * 4, in_s = (N + 1) * 8;
+  const size_t in_bytes = in_d + in_s + in_i + 16;
+  // Out: manifolds n, count cn, normal 3cn, points 12cn, separations 4cn, ids 4cn.
+  const size_t out_d = 19 * C * N * 8, out_i = (N + 5 * C * N) * 4, out_bytes = out_d + out_i;
+  if (int rc = grow(ctx, ctx->pairs_in, in_bytes, "bm_level_host")) return rc;
+  if (int rc = grow(ctx, ctx->pairs_out, out_bytes, "bm_level_host")) return rc;
+  const size_t points_total = (size_t)scratch[N];
+  if (int rc = grow(ctx, ctx->level_scratch, points_total * (9 * 8 + 3 * 4) + 64, "bm_level_host")) return rc;
+  char* in = static_cast<char*>(ctx->pairs_in.p);
+  char* out = static_cast<char*>(ctx->pairs_out.p);
+  bm::LevelDev d{};
+  d.n = n;
+  d.cap = cap;
+  d.position = reinterpret_cast<const double*>(in);
+  d.axes = d.position + 3 * N;
+  d.shapes = d.axes + 9 * N;
+  d.boxes = d.shapes + 19 * N;
+  d.margin = d.boxes + 6 * N;
+  d.tolerance = d.margin + N;
+  d.scratch = reinterpret_cast<const long long*>(in + in_d);
+  d.field = reinterpret_cast<const int*>(in + in_d + in_s);
+  d.normal = reinterpret_cast<double*>(out);
+  d.points = d.normal + 3 * C * N;
+  d.separations = d.points + 12 * C * N;
+  d.manifolds = reinterpret_cast<int*>(out + out_d);
+  d.count = d.manifolds + N;
+  d.ids = reinterpret_cast<uint32_t*>(d.count + C * N);
+  d.sd = static_cast<double*>(ctx->level_scratch.p);
+  d.su = reinterpret_cast<uint32_t*>(d.sd + 9 * points_total);
+  d.hulls = ctx->hulls;
+  d.fields = ctx->fields;
+  if (int rc = grow_pinned(ctx, &ctx->pstage, &ctx->pstage_cap, in_bytes + out_bytes, "bm_level_host")) return rc;
+  char* st = static_cast<char*>(ctx->pstage);
+  double* sd = reinterpret_cast<double*>(st);
+  std::memcpy(sd, position, 24 * N);
+  std::memcpy(sd + 3 * N, axes, 72 * N);
+  std::memcpy(sd + 12 * N, shapes, 152 * N);
+  std::memcpy(sd + 31 * N, boxes, 48 * N);
+  std::memcpy(sd + 37 * N, margin, 8 * N);
+  std::memcpy(sd + 38 * N, tolerance, 8 * N);
+  std::memcpy(st + in_d, scratch.data(), in_s);
+  std::memcpy(st + in_d + in_s, field, in_i);
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(in, st, in_d + in_s + in_i, hipMemcpyHostToDevice, ctx->stream), "bm_level_host: copy");
+  (void)hipGetLastError();
+  HIP_OR_FAIL(ctx, hipEventRecord(ctx->begin, ctx->stream), "bm_level_host: hipEventRecord");
+  bm::launch_level(d, ctx->stream);
+  HIP_OR_FAIL(ctx, hipGetLastError(), "bm_level_host: launch");
+  HIP_OR_FAIL(ctx, hipEventRecord(ctx->end, ctx->stream), "bm_level_host: hipEventRecord");
+  char* back = st + in_bytes;
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(back, out, out_bytes, hipMemcpyDeviceToHost, ctx->stream), "bm_level_host: copy");
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_level_host: kernel");
+  const double* bd = reinterpret_cast<const double*>(back);
+  const int32_t* bi = reinterpret_cast<const int32_t*>(back + out_d);
+  std::memcpy(manifolds, bi, 4 * N);
+  // Slots a lane did not fill are zero, as the oracle and the reference leave them.
+  for (size_t k = 0; k < N; k++) {
+    const int got = std::min<int>(bi[k], cap);
+    for (size_t j = 0; j < C; j++) {
+      const size_t o = k * C + j;
+      if ((int)j < got) {
+        count[o] = bi[N + o];
+        std::memcpy(normal + 3 * o, bd + 3 * o, 24);
+        std::memcpy(points + 12 * o, bd + 3 * C * N + 12 * o, 96);
+        std::memcpy(separations + 4 * o, bd + 15 * C * N + 4 * o, 32);
+        std::memcpy(ids + 4 * o, bi + N + C * N + 4 * o, 16);
+      } else {
+        count[o] = 0;
+        std::memset(normal + 3 * o, 0, 24);
+        std::memset(points + 12 * o, 0, 96);
+        std::memset(separations + 4 * o, 0, 32);
+        std::memset(ids + 4 * o, 0, 16);
+      }
+    }
+  }
+  HIP_OR_FAIL(ctx, hipEventElapsedTime(&ctx->last_ms, ctx->begin, ctx->end), "bm_level_host: hipEventElapsedTime");
   return BM_OK;
 }
 

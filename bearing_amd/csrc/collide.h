@@ -43,4 +43,36 @@ struct MeetDev {
 
 void launch_meet(const MeetDev& d, hipStream_t stream);
 
+// Heightfields (src/mesh.js:288 Heightfield) by number: field f has info[2f] columns and
+// info[2f + 1] rows of heights from heights + off[f], spacing dinfo[3f], x0 dinfo[3f + 1],
+// z0 dinfo[3f + 2] (its lowest corner in its body's frame).
+struct FieldTable {
+  int n;
+  const int* info;
+  const long long* off;
+  const double* dinfo;
+  const double* heights;
+};
+
+// touchLevel (src/triangle.js:1078) for n shapes, one a lane: shape record k against field
+// field[k] whose body is at position 3k turned by axes 9k, looking in box 6k (the field's
+// frame). scratch (n + 1 prefix): how many points lane k may find, 7 a triangle under its
+// box; sd / su its working storage, 9 doubles and 3 words a point. Out: manifolds[k] (every
+// direction found; slots past cap are not written), then cap manifold slots a lane.
+struct LevelDev {
+  int n, cap;
+  const int* field;
+  const double *position, *axes, *shapes, *boxes, *margin, *tolerance;
+  const long long* scratch;
+  double* sd;
+  uint32_t* su;
+  int *manifolds, *count;
+  double *normal, *points, *separations;
+  uint32_t* ids;
+  HullTable hulls;
+  FieldTable fields;
+};
+
+void launch_level(const LevelDev& d, hipStream_t stream);
+
 }  // namespace bm

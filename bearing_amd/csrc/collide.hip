@@ -34,6 +34,7 @@ struct Shape {
   double weight[3];
   int hull;                 // a hull's number in the table
   const HullTable* table;
+  const double* points;     // a triangle's corners, in the world
 };
 
 struct Manifold {
@@ -1392,6 +1393,16 @@ __device__ __forceinline__ void gjk_support(const Shape& s, double x, double y, 
   const double* h = s.half;
   if (s.kind == SPHERE) {
     out[0] = c[0]; out[1] = c[1]; out[2] = c[2];
+  } else if (s.kind == 4) {
+    // A triangle (src/collide.js:49 TRIANGLE): the corner farthest along; the first of equals.
+    const double* p = s.points;
+    int best = 0;
+    double most = -__builtin_inf();
+    for (int i = 0; i < 3; i++) {
+      const double d = p[3 * i] * x + p[3 * i + 1] * y + p[3 * i + 2] * z;
+      if (d > most) { most = d; best = i; }
+    }
+    out[0] = p[3 * best]; out[1] = p[3 * best + 1]; out[2] = p[3 * best + 2];
   } else if (s.kind == HULL) {
     const int* info = s.table->info + 8 * s.hull;
     const double* v = s.table->vertices + 3 * info[0];
@@ -1746,6 +1757,8 @@ __device__ __forceinline__ int meet(Shape& sa, Shape& sb, const Motion& ma, cons
 }
 
 
+#include "level_dev.h"
+
 }  // namespace narrow
 
 // One pair a lane: shape records (kind, centre 3, axes 9, half 3, weight 3) in, manifolds out.
@@ -1815,6 +1828,145 @@ __global__ void meet_kernel(MeetDev d) {
   for (int i = 0; i < n; i++) d.ids[4 * (long)k + i] = m.ids[i];
   for (int i = 0; i < 12; i++) d.shift[12 * (long)k + i] = shift[i];
   d.loss[k] = m.loss;
+}
+
+// touchLevel, one shape a lane (src/triangle.js:1078-1131, visit 1018-1069): the triangles
+// of every cell under the box whose heights reach it, row by row, each touched; points
+// grouped by direction, kept once where two triangles give the same one; each group sorted
+// by id and reduced to four.
+__global__ void level_kernel(LevelDev d) {
+  using namespace narrow;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= d.n) return;
+  const double* r = d.shapes + 19 * (long)k;
+  Shape s;
+  s.kind = (int)r[0];
+  for (int i = 0; i < 3; i++) s.center[i] = r[1 + i];
+  for (int i = 0; i < 9; i++) s.axes[i] = r[4 + i];
+  for (int i = 0; i < 3; i++) s.half[i] = r[13 + i];
+  for (int i = 0; i < 3; i++) s.weight[i] = r[16 + i];
+  s.hull = s.kind == HULL ? (int)r[13] : -1;
+  s.table = &d.hulls;
+  s.points = nullptr;
+  const int f = d.field[k];
+  Field F;
+  F.columns = d.fields.info[2 * f];
+  F.rows = d.fields.info[2 * f + 1];
+  F.h = d.fields.heights + d.fields.off[f];
+  F.spacing = d.fields.dinfo[3 * f];
+  F.x0 = d.fields.dinfo[3 * f + 1];
+  F.z0 = d.fields.dinfo[3 * f + 2];
+  const double* c = d.position + 3 * (long)k;
+  const double* A = d.axes + 9 * (long)k;
+  const double* box = d.boxes + 6 * (long)k;
+  const double margin = d.margin[k], tolerance = d.tolerance[k];
+  const long long base = d.scratch[k];
+  const int cap_points = (int)(d.scratch[k + 1] - base);
+  LevelScratch W;
+  W.at = d.sd + 9 * base;
+  W.gat = W.at + 4 * (long)cap_points;
+  W.normals = W.gat + 4 * (long)cap_points;
+  W.id = d.su + 3 * base;
+  W.in = W.id + cap_points;
+  W.gid = W.in + cap_points;
+  Toucher T;
+  Touch one;
+  int found = 0, groups = 0;
+  const double t2 = tolerance * tolerance;
+  // src/mesh.js:388-401 (triangles)
+  const int cells = F.columns - 1;
+  const double i0 = narrow::js_max(0, floor((box[0] - F.x0) / F.spacing)), i1 = narrow::js_min(cells - 1, floor((box[3] - F.x0) / F.spacing));
+  const double j0 = narrow::js_max(0, floor((box[2] - F.z0) / F.spacing)), j1 = narrow::js_min(F.rows - 2, floor((box[5] - F.z0) / F.spacing));
+  if (i0 <= i1 && j0 <= j1) {
+    for (int j = (int)j0; j <= (int)j1; j++) {
+      for (int i = (int)i0; i <= (int)i1; i++) {
+        const double ha = F.at(i, j), hb = F.at(i + 1, j), hc = F.at(i, j + 1), hd = F.at(i + 1, j + 1);
+        if (narrow::js_max(narrow::js_max(narrow::js_max(ha, hb), hc), hd) < box[1] || narrow::js_min(narrow::js_min(narrow::js_min(ha, hb), hc), hd) > box[4]) continue;
+        for (int half = 0; half < 2; half++) {
+          const int t = 2 * (j * cells + i) + half;
+          double local[9], p[9];
+          F.triangle(t, local);
+          for (int v = 0; v < 3; v++) {
+            const double x = local[3 * v], y = local[3 * v + 1], z = local[3 * v + 2];
+            p[3 * v] = c[0] + A[0] * x + A[3] * y + A[6] * z;
+            p[3 * v + 1] = c[1] + A[1] * x + A[4] * y + A[7] * z;
+            p[3 * v + 2] = c[2] + A[2] * x + A[5] * y + A[8] * z;
+          }
+          Shape tri;
+          tri.kind = TRIANGLE;
+          tri.points = p;
+          tri.hull = -1;
+          tri.table = nullptr;
+          for (int q = 0; q < 9; q++) tri.axes[q] = (q == 0 || q == 4 || q == 8) ? 1 : 0;
+          for (int q = 0; q < 3; q++) { tri.half[q] = 0; tri.weight[q] = 0; }
+          tri.center[0] = (p[0] + p[3] + p[6]) / 3;
+          tri.center[1] = (p[1] + p[4] + p[7]) / 3;
+          tri.center[2] = (p[2] + p[5] + p[8]) / 3;
+          triangle_normal(p, tri.axes);
+          const int flags = field_edge(F, t, T.folds);
+          if (T.touch(s, tri, flags, margin, tolerance, one, t) == 0) continue;
+          const double nx = one.normal[0], ny = one.normal[1], nz = one.normal[2];
+          int g = 0;
+          while (g < groups && W.normals[3 * g] * nx + W.normals[3 * g + 1] * ny + W.normals[3 * g + 2] * nz < 1 - SAME) g++;
+          if (g == groups) {
+            W.normals[3 * g] = nx; W.normals[3 * g + 1] = ny; W.normals[3 * g + 2] = nz;
+            groups++;
+          }
+          for (int q = 0; q < one.count; q++) {
+            const double x = one.points[3 * q], y = one.points[3 * q + 1], z = one.points[3 * q + 2], sep = one.separations[q];
+            int same = -1;
+            for (int e = 0; e < found && same < 0; e++) {
+              if ((int)W.in[e] != g) continue;
+              const double dx = W.at[4 * e] - x, dy = W.at[4 * e + 1] - y, dz = W.at[4 * e + 2] - z;
+              if (dx * dx + dy * dy + dz * dz <= t2) same = e;
+            }
+            if (same >= 0) {
+              if (sep < W.at[4 * same + 3]) {
+                W.at[4 * same] = x; W.at[4 * same + 1] = y; W.at[4 * same + 2] = z; W.at[4 * same + 3] = sep;
+              }
+              if (one.ids[q] < 512 && W.id[same] >= 512) W.id[same] = one.ids[q];
+              continue;
+            }
+            if (found == cap_points) continue;   // never: at most 7 points a triangle
+            W.at[4 * found] = x; W.at[4 * found + 1] = y; W.at[4 * found + 2] = z; W.at[4 * found + 3] = sep;
+            W.id[found] = one.ids[q];
+            W.in[found] = (uint32_t)g;
+            found++;
+          }
+        }
+      }
+    }
+  }
+  for (int g = 0; g < groups; g++) {
+    int m = 0;
+    for (int i = 0; i < found; i++) {
+      if ((int)W.in[i] != g) continue;
+      int j = m++;
+      for (; j > 0 && W.gid[j - 1] > W.id[i]; j--) {
+        for (int q = 0; q < 4; q++) W.gat[4 * j + q] = W.gat[4 * j - 4 + q];
+        W.gid[j] = W.gid[j - 1];
+      }
+      for (int q = 0; q < 4; q++) W.gat[4 * j + q] = W.at[4 * i + q];
+      W.gid[j] = W.id[i];
+    }
+    Manifold mf;
+    mf.count = 0;
+    mf.normal[0] = W.normals[3 * g]; mf.normal[1] = W.normals[3 * g + 1]; mf.normal[2] = W.normals[3 * g + 2];
+    level_reduce(W, m, tolerance, mf);
+    if (g >= d.cap) continue;
+    const long o = (long)k * d.cap + g;
+    d.count[o] = mf.count;
+    for (int q = 0; q < 3; q++) d.normal[3 * o + q] = mf.normal[q];
+    for (int q = 0; q < 12; q++) d.points[12 * o + q] = q < 3 * mf.count ? mf.points[q] : 0;
+    for (int q = 0; q < 4; q++) d.separations[4 * o + q] = q < mf.count ? mf.separations[q] : 0;
+    for (int q = 0; q < 4; q++) d.ids[4 * o + q] = q < mf.count ? mf.ids[q] : 0;
+  }
+  d.manifolds[k] = groups;
+}
+
+void launch_level(const LevelDev& d, hipStream_t stream) {
+  if (d.n == 0) return;
+  hipLaunchKernelGGL(level_kernel, dim3((d.n + 63) / 64), dim3(64), 0, stream, d);
 }
 
 void launch_meet(const MeetDev& d, hipStream_t stream) {

@@ -58,6 +58,7 @@ class Columns(ctypes.Structure):
 
 EXPORTS = ["bm_create", "bm_destroy", "bm_last_error", "bm_upload", "bm_soft_step", "bm_download",
            "bm_soft_step_host", "bm_last_kernel_ms", "bm_set_launch", "bm_lds_limit", "bm_colour", "bm_collide_host", "bm_meet_host", "bm_set_colour_lds", "bm_set_hulls",
+           "bm_set_fields", "bm_level_host",
            "bm_world_create", "bm_world_destroy", "bm_world_error", "bm_world_add", "bm_world_step", "bm_world_read",
            "bm_world_set_hulls"]
 
@@ -83,6 +84,9 @@ def load():
         lib.bm_collide_host.argtypes = [vp, ctypes.c_int32, _D, _D, _D, _D, _I32, _D, _D, _D, _U32, _D]
         lib.bm_meet_host.argtypes = [vp, ctypes.c_int32, _D, _D, _D, _I32, _D, _D, _D, _U32, _D, _I32, _D]
         lib.bm_set_hulls.argtypes = [vp, ctypes.c_int32, _I32, _I32, _I32, _D, _I32, _I32, _D]
+        lib.bm_set_fields.argtypes = [vp, ctypes.c_int32, _I32, _I32, _D, ctypes.POINTER(ctypes.c_int64), _D]
+        lib.bm_level_host.argtypes = [vp, ctypes.c_int32, _I32, _D, _D, _D, _D, _D, _D, ctypes.c_int32, _I32, _I32, _D,
+                                      _D, _D, _U32]
         lib.bm_last_kernel_ms.argtypes = [vp]
         lib.bm_last_kernel_ms.restype = ctypes.c_double
         lib.bm_set_launch.argtypes = [vp, ctypes.c_int32, ctypes.c_int32]
@@ -172,6 +176,41 @@ def set_hulls(ctx, table):
     ctx.check(ctx.lib.bm_set_hulls(ctx.ptr, len(hv) - 1, hv.ctypes.data_as(_I32), hf.ctypes.data_as(_I32),
                                    hl.ctypes.data_as(_I32), v.ctypes.data_as(_D), st.ctypes.data_as(_I32),
                                    lp.ctypes.data_as(_I32), pl.ctypes.data_as(_D)), "bm_set_hulls")
+
+
+def set_fields(ctx, columns, rows, spacing, heights):
+    """bm_set_fields: heightfields by number, field f columns[f] x rows[f] samples spacing[f]
+    apart, heights[f] its samples row by row."""
+    n = len(columns)
+    cols, rws = np.ascontiguousarray(columns, np.int32), np.ascontiguousarray(rows, np.int32)
+    sp = np.ascontiguousarray(spacing, np.float64)
+    off = np.zeros(n + 1, np.int64)
+    off[1:] = np.cumsum([len(h) for h in heights])
+    flat = np.ascontiguousarray(np.concatenate([np.asarray(h, np.float64) for h in heights]) if n else np.zeros(1))
+    ctx.check(ctx.lib.bm_set_fields(ctx.ptr, n, cols.ctypes.data_as(_I32), rws.ctypes.data_as(_I32),
+                                    sp.ctypes.data_as(_D), off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                    flat.ctypes.data_as(_D)), "bm_set_fields")
+
+
+def level(ctx, field, position, axes, shapes, boxes, margin, tolerance, cap):
+    """bm_level_host (touchLevel) for each shape record against its field: manifolds a
+    shape, then count, normal, points, separations, ids in cap slots a shape."""
+    n = len(margin)
+    f64 = lambda x: np.ascontiguousarray(x, np.float64)
+    field = np.ascontiguousarray(field, np.int32)
+    position, axes, shapes, boxes, margin, tolerance = map(f64, (position, axes, shapes, boxes, margin, tolerance))
+    manifolds = np.zeros(n, np.int32)
+    count = np.zeros(cap * n, np.int32)
+    normal = np.zeros(3 * cap * n)
+    points = np.zeros(12 * cap * n)
+    seps = np.zeros(4 * cap * n)
+    ids = np.zeros(4 * cap * n, np.uint32)
+    p = lambda x, t: x.ctypes.data_as(t)
+    ctx.check(ctx.lib.bm_level_host(ctx.ptr, n, p(field, _I32), p(position, _D), p(axes, _D), p(shapes, _D),
+                                    p(boxes, _D), p(margin, _D), p(tolerance, _D), cap, p(manifolds, _I32),
+                                    p(count, _I32), p(normal, _D), p(points, _D), p(seps, _D), p(ids, _U32)),
+              "bm_level_host")
+    return manifolds, count, normal, points, seps, ids
 
 
 def collide(ctx, a, b, margin, tolerance, with_loss=False):

@@ -192,6 +192,29 @@ int bm_meet_host(bm_context* ctx, int32_t n, const double* a, const double* b, c
 int bm_set_hulls(bm_context* ctx, int32_t n, const int32_t* hull_v, const int32_t* hull_f, const int32_t* hull_l,
                  const double* vertices, const int32_t* start, const int32_t* loops, const double* planes);
 
+/* Heightfields (src/mesh.js:288 Heightfield) that level records name by number: field f has
+ * rows[f] rows of columns[f] heights from heights + offsets[f], row by row (x along columns,
+ * z along rows, centred on its body), samples spacing[f] apart. The context keeps the table
+ * until the next call. */
+int bm_set_fields(bm_context* ctx, int32_t n, const int32_t* columns, const int32_t* rows, const double* spacing,
+                  const int64_t* offsets, const double* heights);
+
+/*
+ * touchLevel (src/triangle.js:1078-1131) on the device, one shape a lane: shape record k (as
+ * bm_collide_host's: a sphere, capsule, box or hull) against field field[k], whose body is at
+ * position 3k turned by axes 9k (columns), looking at the cells under box 6k (lo 3, hi 3, in
+ * the field's frame, as src/contact.js:832-881 level() works it out). Each triangle is touched
+ * (src/triangle.js:104 touch: faces, edges and corners, seams and folds from
+ * src/mesh.js:409 edge), its points grouped by normal and reduced to four a direction.
+ * Out: manifolds[k], how many directions (all of them, even past cap), then cap slots a
+ * shape of count, normal 3 (from the shape to the field), points 12, separations 4, ids 4;
+ * unused slots zero.
+ */
+int bm_level_host(bm_context* ctx, int32_t n, const int32_t* field, const double* position, const double* axes,
+                  const double* shapes, const double* boxes, const double* margin, const double* tolerance, int32_t cap,
+                  int32_t* manifolds, int32_t* count, double* normal, double* points, double* separations,
+                  uint32_t* ids);
+
 /* How bm_soft_step launches, for tuning (bearing_amd/tune.py picks per workload):
  * shape 0 = 256 threads a workgroup with registers for one wave a SIMD, 1 = 256
  * threads and two waves a SIMD, 2 = 512 threads and two waves; resident = 1 keeps
