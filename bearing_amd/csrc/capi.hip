@@ -205,8 +205,6 @@ int check_batch(bm_context* ctx, const bm_columns* h) {
         return fail(ctx, BM_ERR_ARGUMENT, "bm_upload: manifold body slot out of range");
       if (h->pair[m] < 0 || h->pair[m] >= np) return fail(ctx, BM_ERR_ARGUMENT, "bm_upload: manifold pair out of range");
       if (h->points[m] < 1 || h->points[m] > 4) return fail(ctx, BM_ERR_ARGUMENT, "bm_upload: a manifold has 1 to 4 points");
-      if (h->sub[m] >= 0)
-        return fail(ctx, BM_ERR_UNSUPPORTED, "bm_upload: manifolds of mesh and compound pairs are not supported");
     }
   }
   return BM_OK;

@@ -863,6 +863,8 @@ __device__ __forceinline__ void store_row(const SoftDev& d, int m, int ga, int g
     d.drift1[m] += dx * t1x + dy * t1y + dz * t1z;
     d.drift2[m] += dx * t2x + dy * t2y + dz * t2z;
   }
+  // A level pair's manifolds go into its Memory, which the host keeps.
+  if (d.sub[m] >= 0) return;
   const int p3 = 3 * gp;
   d.pair_points[gp] = (uint8_t)count;
   const double r1 = d.roll_impulse1[m], r2 = d.roll_impulse2[m], sp = d.spin_impulse[m];

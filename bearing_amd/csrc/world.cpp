@@ -24,7 +24,7 @@ namespace {
 
 constexpr int P = 4;
 constexpr int DYNAMIC = 0, STATIC = 1;
-constexpr int BOX = 0, SPHERE = 1, CAPSULE = 2, HULL = 3;
+constexpr int BOX = 0, SPHERE = 1, CAPSULE = 2, HULL = 3, MESH = 5, HEIGHTFIELD = 6;  // src/contact.js:114
 constexpr int MOVING_TREE = 1, STATIC_TREE = 0;
 constexpr double PIXEL = 1.0 / 200;      // src/contact.js:125
 constexpr double FAT = 1.0 / 4;          // src/broadphase.js:32
@@ -63,6 +63,24 @@ void rotate(const double* q, double x, double y, double z, bool inverse, double*
   out[2] = z + w * tz + (ux * ty - uy * tx);
 }
 
+// What a level pair's manifolds left behind (src/pairs.js:384-415 Memory): per manifold, its
+// normal, key and what Pairs keeps for one. Only rows below count are ever read.
+struct Memory {
+  int count = 0;
+  std::vector<double> normal, friction, slip, twist, normal_impulse;
+  std::vector<uint8_t> taken, points;
+  std::vector<int32_t> key;
+  std::vector<uint32_t> pid;
+  void ensure(int n) {
+    if ((int)points.size() >= n) return;
+    size_t c = points.size() < 1 ? 1 : points.size();
+    while ((int)c < n) c *= 2;
+    normal.resize(3 * c); friction.resize(3 * c); slip.resize(3 * c); twist.resize(3 * c);
+    normal_impulse.resize(P * c); pid.resize(P * c);
+    taken.resize(c); points.resize(c); key.resize(c);
+  }
+};
+
 // The pair list (src/pairs.js:40-378): [0, active) awake, [active, count) asleep.
 struct Pairs {
   int count = 0, active = 0;
@@ -77,6 +95,8 @@ struct Pairs {
   std::vector<uint8_t> cache_points;
   std::vector<double> cache_pose, cache_loss, cache_normal, cache_local;
   std::vector<uint32_t> cache_id;
+  // Each pair's Memory (src/pairs.js:84 memory), moved and swapped with it.
+  std::vector<Memory> memory;
 
   static uint64_t key(int x, int y) { return (uint64_t)x * 16777216ull + (uint64_t)y; }
 
@@ -84,7 +104,7 @@ struct Pairs {
     if ((int)a.size() >= n) return;
     size_t c = a.size() < 16 ? 16 : a.size();
     while ((int)c < n) c *= 2;
-    a.resize(c); b.resize(c); cache.resize(c); points.resize(c); touching.resize(c);
+    a.resize(c); b.resize(c); cache.resize(c); points.resize(c); touching.resize(c); memory.resize(c);
     pid.resize(P * c); normal_impulse.resize(P * c); friction.resize(3 * c); slip.resize(3 * c); twist.resize(3 * c);
   }
   void cache_grow(int n) {
@@ -124,6 +144,7 @@ struct Pairs {
   void forget(int p) {
     points[p] = 0;
     if (cache[p] != 0) uncache(p);
+    memory[p].count = 0;
     part(p);
   }
   // src/pairs.js:303-318
@@ -135,6 +156,7 @@ struct Pairs {
     for (int k = 0; k < 3; k++) {
       friction[3 * to + k] = friction[3 * from + k]; slip[3 * to + k] = slip[3 * from + k]; twist[3 * to + k] = twist[3 * from + k];
     }
+    std::swap(memory[to], memory[from]);
     index[key(a[to], b[to])] = to;
   }
   // src/pairs.js:321-334
@@ -149,6 +171,7 @@ struct Pairs {
       std::swap(friction[3 * i + k], friction[3 * j + k]); std::swap(slip[3 * i + k], slip[3 * j + k]);
       std::swap(twist[3 * i + k], twist[3 * j + k]);
     }
+    std::swap(memory[i], memory[j]);
     index[key(a[i], b[i])] = i;
     index[key(a[j], b[j])] = j;
   }
@@ -158,6 +181,7 @@ struct Pairs {
   void remove(int p) {
     part(p);
     if (cache[p] != 0) uncache(p);
+    memory[p].count = 0;
     index.erase(key(a[p], b[p]));
     if (p < active) {
       const int last = --active;
@@ -183,7 +207,8 @@ struct Manifold {
 struct Visit {
   int p, a, b;
   double tolerance;
-  int kind;  // 0: the boxes apart (forget); 1: the kept manifold, in m; 2: met on the device, batch row `slot`
+  int kind;  // 0: the boxes apart (forget); 1: the kept manifold, in m; 2: met on the device, batch row `slot`;
+             // 3: a level pair touched on the device, level batch row `slot`
   int slot;
   Manifold m;
 };
@@ -242,6 +267,17 @@ struct bm_world {
   };
   std::vector<HostHull> hulls;
   std::vector<double> hull_vertices;
+  // A heightfield body's field (its number in the field table), and the table as the host
+  // reads it (src/mesh.js:288-311: samples, the box in its own frame, its reach).
+  std::vector<int32_t> field;
+  struct HostField {
+    int columns, rows;
+    double spacing, x0, z0, lo[3], hi[3], reach;
+    size_t offset;
+    double at(const double* h, int i, int j) const { return h[offset + (size_t)j * columns + i]; }
+  };
+  std::vector<HostField> fields;
+  std::vector<double> field_heights;
   std::vector<int32_t> joint_head, in_awake, awake_list, island, island_next, parent, slept;
   std::vector<float> transforms;
   int awake_count = 0, frozen_count = 0, slept_count = 0;
@@ -264,6 +300,14 @@ struct bm_world {
   std::vector<double> batch_a, batch_b, batch_pair, batch_normal, batch_points, batch_seps, batch_shift, batch_loss;
   std::vector<int32_t> batch_count, batch_flags;
   std::vector<uint32_t> batch_ids;
+  // Level pairs for the device touchLevel (bm_level_host): field, the field body's place and
+  // axes, the other body's shape record, the box to look in, margin and tolerance; results
+  // in `level_cap` slots a pair.
+  std::vector<int32_t> level_field, level_manifolds, level_count;
+  std::vector<double> level_pos, level_axes, level_shape, level_box, level_margin, level_tol, level_normal, level_points,
+      level_seps;
+  std::vector<uint32_t> level_ids;
+  int level_cap = 8;
   std::vector<Visit> visits;
   // Bodies woken while a narrow pass's results are taken (their bounds changed then).
   std::vector<uint8_t> fresh;
@@ -325,13 +369,14 @@ void bounds_of(bm_world* w, int s, double dt, double gl) {
   }
   w->motion[s] = m;
   double ex, ey, ez, ox = 0, oy = 0, oz = 0;
-  if (w->shape[s] == BOX || w->shape[s] == HULL) {
+  if (w->shape[s] == BOX || w->shape[s] == HULL || w->shape[s] >= MESH) {
     double hx = w->dims[s3] * 0.5, hy = w->dims[s3 + 1] * 0.5, hz = w->dims[s3 + 2] * 0.5;
-    if (w->shape[s] == HULL) {
-      // The box round its corners in its own frame, turned into the world's.
-      const auto& h = w->hulls[w->hull[s]];
-      const double cx = (h.lo[0] + h.hi[0]) * 0.5, cy = (h.lo[1] + h.hi[1]) * 0.5, cz = (h.lo[2] + h.hi[2]) * 0.5;
-      hx = (h.hi[0] - h.lo[0]) * 0.5; hy = (h.hi[1] - h.lo[1]) * 0.5; hz = (h.hi[2] - h.lo[2]) * 0.5;
+    if (w->shape[s] != BOX) {
+      // The box round its corners (or triangles) in its own frame, turned into the world's.
+      const double* lo = w->shape[s] == HULL ? w->hulls[w->hull[s]].lo : w->fields[w->field[s]].lo;
+      const double* hi = w->shape[s] == HULL ? w->hulls[w->hull[s]].hi : w->fields[w->field[s]].hi;
+      const double cx = (lo[0] + hi[0]) * 0.5, cy = (lo[1] + hi[1]) * 0.5, cz = (lo[2] + hi[2]) * 0.5;
+      hx = (hi[0] - lo[0]) * 0.5; hy = (hi[1] - lo[1]) * 0.5; hz = (hi[2] - lo[2]) * 0.5;
       ox = ax[0] * cx + ax[3] * cy + ax[6] * cz;
       oy = ax[1] * cx + ax[4] * cy + ax[7] * cz;
       oz = ax[2] * cx + ax[5] * cy + ax[8] * cz;
@@ -488,6 +533,7 @@ void found(bm_world* w, int x, int y) {
   const int p = pr.count;
   pr.grow(p + 1);
   pr.a[p] = a; pr.b[p] = b; pr.points[p] = 0; pr.touching[p] = 0; pr.cache[p] = 0;
+  pr.memory[p].count = 0;
   pr.index[Pairs::key(a, b)] = p;
   pr.count++;
   pr.wake(p);
@@ -676,13 +722,14 @@ void basis(Contacts& c, int m) {
 
 // src/contact.js:889-921 (emit), sign 1, no sub-manifold; `shift` (3 a point) for a
 // manifold met at the moment of impact, else null.
-int emit(bm_world* w, int p, int a, int b, const Manifold& mf, double tolerance, const double* shift) {
+int emit(bm_world* w, int p, int a, int b, const Manifold& mf, double tolerance, const double* shift, int sub = -1,
+         double sign = 1) {
   Contacts& c = w->c;
   const int m = c.count++;
   c.ensure(c.count);
-  c.a[m] = a; c.b[m] = b; c.pair[m] = p; c.sub[m] = -1;
+  c.a[m] = a; c.b[m] = b; c.pair[m] = p; c.sub[m] = sub;
   c.points[m] = (uint8_t)mf.count;
-  for (int k = 0; k < 3; k++) c.normal[3 * m + k] = mf.normal[k];
+  for (int k = 0; k < 3; k++) c.normal[3 * m + k] = sign * mf.normal[k];
   c.friction[m] = std::sqrt(w->friction[a] * w->friction[b]);
   c.restitution[m] = js_max(w->restitution[a], w->restitution[b]);
   c.tolerance[m] = tolerance;
@@ -702,9 +749,9 @@ int emit(bm_world* w, int p, int a, int b, const Manifold& mf, double tolerance,
   return m;
 }
 
-// src/contact.js:999-1036 (carry) from the pair list.
-void carry(bm_world* w, int p, int m) {
-  const Pairs& pr = w->pairs;
+// src/contact.js:999-1036 (carry) from the pair list, or from a row of a level pair's Memory.
+template <class Kept>
+void carry_from(bm_world* w, const Kept& pr, int p, int m) {
   Contacts& c = w->c;
   const int remembered = pr.points[p];
   if (remembered == 0) return;
@@ -740,6 +787,8 @@ void carry(bm_world* w, int p, int m) {
   }
 }
 
+void carry(bm_world* w, int p, int m) { carry_from(w, w->pairs, p, m); }
+
 // A body record for the device meet (src/contact.js:1244-1276 load, and what moving()
 // and src/impact.js start read): the shape (kind, centre 3, axes 9, half 3, weight 3),
 // rotation 4, velocity 3, spin 3, gravity scale (0 unless dynamic), reach, flags
@@ -768,8 +817,39 @@ void load(const bm_world* w, int s, double* r) {
   r[31] = (dynamic ? 1 : 0) | (dynamic && (w->shape[s] != SPHERE || w->framed[s] == 1) ? 2 : 0);
 }
 
+// Runs the level batch gathered so far on the device (bm_level_host) and clears it; a pair
+// touched along more directions than the slots run again with room for all of them.
+int run_level_batch(bm_world* w) {
+  const int n = (int)w->level_field.size();
+  if (n == 0) return BM_OK;
+  for (;;) {
+    const size_t slots = (size_t)n * w->level_cap;
+    w->level_manifolds.resize(n);
+    w->level_count.resize(slots);
+    w->level_normal.resize(3 * slots);
+    w->level_points.resize(12 * slots);
+    w->level_seps.resize(4 * slots);
+    w->level_ids.resize(4 * slots);
+    const int rc = bm_level_host(w->ctx, n, w->level_field.data(), w->level_pos.data(), w->level_axes.data(),
+                                 w->level_shape.data(), w->level_box.data(), w->level_margin.data(),
+                                 w->level_tol.data(), w->level_cap, w->level_manifolds.data(), w->level_count.data(),
+                                 w->level_normal.data(), w->level_points.data(), w->level_seps.data(),
+                                 w->level_ids.data());
+    if (rc != BM_OK) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx));
+    int most = 0;
+    for (int k = 0; k < n; k++) most = std::max(most, (int)w->level_manifolds[k]);
+    if (most <= w->level_cap) break;
+    w->level_cap = most;
+  }
+  for (auto* v : {&w->level_pos, &w->level_axes, &w->level_shape, &w->level_box, &w->level_margin, &w->level_tol})
+    v->clear();
+  w->level_field.clear();
+  return BM_OK;
+}
+
 // Runs the narrowphase batch gathered so far on the device (bm_meet_host) and clears it.
 int run_batch(bm_world* w) {
+  if (int rc = run_level_batch(w)) return rc;
   const int n = (int)w->batch_pair.size() / 6;
   if (n == 0) return BM_OK;
   w->batch_count.resize(n);
@@ -789,8 +869,83 @@ int run_batch(bm_world* w) {
   return BM_OK;
 }
 
+// src/contact.js:832-856 (level) up to touchLevel: the other body's box, grown by its motion,
+// in the field's frame and grown by the field's own motion, appended to the level batch
+// with the other body's shape. The tolerance is the other body's: a field has no thickness.
+void prepare_level(bm_world* w, Visit& v) {
+  const int a = v.a, b = v.b;
+  const bool turned = w->shape[a] >= MESH;
+  const int g = turned ? a : b, o = turned ? b : a;
+  const double tolerance = w->size[o] * PIXEL;
+  const double* reach = w->motion.data();
+  const double margin = reach[a] + reach[b] + tolerance;
+  const double* box = w->box.data();
+  const double* A = w->axes.data();
+  const double* pos = w->position.data();
+  const int o6 = 6 * o, g3 = 3 * g, g9 = 9 * g;
+  const double cx = (box[o6] + box[o6 + 3]) * 0.5 - pos[g3], cy = (box[o6 + 1] + box[o6 + 4]) * 0.5 - pos[g3 + 1];
+  const double cz = (box[o6 + 2] + box[o6 + 5]) * 0.5 - pos[g3 + 2];
+  const double hx = (box[o6 + 3] - box[o6]) * 0.5, hy = (box[o6 + 4] - box[o6 + 1]) * 0.5, hz = (box[o6 + 5] - box[o6 + 2]) * 0.5;
+  const double lx = A[g9] * cx + A[g9 + 1] * cy + A[g9 + 2] * cz;
+  const double ly = A[g9 + 3] * cx + A[g9 + 4] * cy + A[g9 + 5] * cz;
+  const double lz = A[g9 + 6] * cx + A[g9 + 7] * cy + A[g9 + 8] * cz;
+  const double ex = std::fabs(A[g9]) * hx + std::fabs(A[g9 + 1]) * hy + std::fabs(A[g9 + 2]) * hz + reach[g];
+  const double ey = std::fabs(A[g9 + 3]) * hx + std::fabs(A[g9 + 4]) * hy + std::fabs(A[g9 + 5]) * hz + reach[g];
+  const double ez = std::fabs(A[g9 + 6]) * hx + std::fabs(A[g9 + 7]) * hy + std::fabs(A[g9 + 8]) * hz + reach[g];
+  v.kind = 3;
+  v.tolerance = tolerance;
+  v.slot = (int)w->level_field.size();
+  w->level_field.push_back(w->field[g]);
+  w->level_pos.insert(w->level_pos.end(), pos + g3, pos + g3 + 3);
+  w->level_axes.insert(w->level_axes.end(), A + g9, A + g9 + 9);
+  double r[32];
+  load(w, o, r);
+  w->level_shape.insert(w->level_shape.end(), r, r + 19);
+  const double lb[6] = {lx - ex, ly - ey, lz - ez, lx + ex, ly + ey, lz + ez};
+  w->level_box.insert(w->level_box.end(), lb, lb + 6);
+  w->level_margin.push_back(margin);
+  w->level_tol.push_back(tolerance);
+}
+
+// The rest of src/contact.js:857-881 (level): wake, then each direction the device found as a
+// manifold of its own (sub 0, 1, ...; normals from a to b), each started from the remembered
+// manifold of the pair nearest its direction; a pair that touched nothing forgets.
+int finish_level(bm_world* w, Visit& v) {
+  const int a = v.a, b = v.b, p = v.p, k = v.slot, cap = w->level_cap;
+  const bool turned = w->shape[a] >= MESH;
+  const int n = w->level_manifolds[k];
+  Memory& kept = w->pairs.memory[p];
+  for (int j = 0; j < kept.count; j++) kept.taken[j] = 0;
+  if (n > 0 && w->awake[a] != w->awake[b]) wake(w, w->awake[a] ? b : a);
+  const Contacts& c = w->c;
+  int sub = 0;
+  for (int q = 0; q < n; q++) {
+    const size_t o = (size_t)k * cap + q;
+    Manifold mf;
+    mf.count = w->level_count[o];
+    for (int i = 0; i < 3; i++) mf.normal[i] = w->level_normal[3 * o + i];
+    for (int i = 0; i < 12; i++) mf.points[i] = w->level_points[12 * o + i];
+    for (int i = 0; i < 4; i++) mf.separations[i] = w->level_seps[4 * o + i];
+    for (int i = 0; i < 4; i++) mf.ids[i] = w->level_ids[4 * o + i];
+    const int m = emit(w, p, a, b, mf, v.tolerance, nullptr, sub++, turned ? -1 : 1);
+    int best = -1;
+    double most = 0;
+    for (int j = 0; j < kept.count; j++) {
+      if (kept.taken[j] || kept.key[j] != 0) continue;
+      const double d = kept.normal[3 * j] * c.normal[3 * m] + kept.normal[3 * j + 1] * c.normal[3 * m + 1]
+          + kept.normal[3 * j + 2] * c.normal[3 * m + 2];
+      if (d > most) { most = d; best = j; }
+    }
+    if (best < 0) continue;
+    kept.taken[best] = 1;
+    carry_from(w, kept, best, m);
+  }
+  if (sub == 0) w->pairs.forget(p);
+  return BM_OK;
+}
+
 // find() up to the narrowphase (src/contact.js:334-354): the box test, the kept manifold
-// (REUSE), or a meet record appended to the device batch.
+// (REUSE), a meet record appended to the device batch, or a level pair's record.
 void prepare_visit(bm_world* w, Visit& v) {
   const int a = v.a, b = v.b;
   v.kind = 0;
@@ -798,6 +953,10 @@ void prepare_visit(bm_world* w, Visit& v) {
   const double* ba = &w->box[6 * a];
   const double* bb = &w->box[6 * b];
   if (ba[0] > bb[3] || bb[0] > ba[3] || ba[1] > bb[4] || bb[1] > ba[4] || ba[2] > bb[5] || bb[2] > ba[5]) return;
+  if (w->shape[a] >= MESH || w->shape[b] >= MESH) {
+    prepare_level(w, v);
+    return;
+  }
   const double margin = w->motion[a] + w->motion[b] + v.tolerance;
   if (reused(w, v.p, a, b, v.tolerance, margin, v.m)) {
     v.kind = 1;
@@ -822,6 +981,7 @@ int finish_visit(bm_world* w, Visit& v) {
     w->pairs.forget(v.p);
     return BM_OK;
   }
+  if (v.kind == 3) return finish_level(w, v);
   const double* shift = nullptr;
   if (v.kind == 2) {
     const int k = v.slot;
@@ -879,7 +1039,7 @@ int narrow(bm_world* w, int from) {
   for (Visit& v : visits) {
     if (w->fresh[v.a] || w->fresh[v.b]) {
       prepare_visit(w, v);
-      if (v.kind == 2) {
+      if (v.kind >= 2) {
         if (int rc = run_batch(w)) return rc;
       }
     }
@@ -1042,7 +1202,7 @@ void sleep_step(bm_world* w, double dt) {
 // ---- the continuous catch: src/world.js:1683-1743 (_catch), src/query.js:355-445 and
 // 607-642 (advance, sweep), src/gjk.js (closest), src/contact.js:1755-1776 (gapOf) ----
 
-constexpr int CORE_SPHERE = 0, CORE_CAPSULE = 1, CORE_BOX = 2, CORE_HULL = 3;  // src/collide.js:44 kinds
+constexpr int CORE_SPHERE = 0, CORE_CAPSULE = 1, CORE_BOX = 2, CORE_HULL = 3, CORE_TRIANGLE = 4;  // src/collide.js:44-49
 
 // A shape as the sweep places it: kind, centre, axes (x, y, z columns), half extents; a
 // hull's number, corners (V of them, in its own frame) and thinnest extent.
@@ -1052,17 +1212,28 @@ struct Core {
   int hull = -1, V = 0;
   const double* vertices = nullptr;
   double hull_size = 0;
+  double points[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // a triangle's corners, in the world
 };
 
 double core_radius(const Core& s) { return s.kind == CORE_SPHERE || s.kind == CORE_CAPSULE ? s.half[0] : 0; }
 
-// src/gjk.js:70-107 (support) for a sphere's, a capsule's, a box's or a hull's core.
+// src/gjk.js:70-107 (support) for a sphere's, a triangle's, a capsule's, a box's or a hull's core.
 void core_support(const Core& s, double x, double y, double z, double* out) {
   const double* c = s.center;
   const double* a = s.axes;
   const double* h = s.half;
   if (s.kind == CORE_SPHERE) {
     out[0] = c[0]; out[1] = c[1]; out[2] = c[2];
+  } else if (s.kind == CORE_TRIANGLE) {
+    // The corner farthest along; the first of equals.
+    const double* p = s.points;
+    int best = 0;
+    double most = -INFINITY;
+    for (int i = 0; i < 3; i++) {
+      const double d = p[3 * i] * x + p[3 * i + 1] * y + p[3 * i + 2] * z;
+      if (d > most) { most = d; best = i; }
+    }
+    out[0] = p[3 * best]; out[1] = p[3 * best + 1]; out[2] = p[3 * best + 2];
   } else if (s.kind == CORE_HULL) {
     // The corner farthest along, in the hull's own frame; the first of equals.
     const double* v = s.vertices;
@@ -1321,6 +1492,119 @@ double core_size(const Core& s) {
   return 2 * js_min(js_min(s.half[0], s.half[1]), s.half[2]);
 }
 
+// src/triangle.js:104-124 and 240-334 (touch, nearest, sphere) for a ball against a triangle
+// met alone, as the catch's sweep meets one (src/query.js:37 BARE: every edge a corner from
+// both sides, no folds, two-sided), with no margin: the normal from the ball to the
+// triangle and the contact point; false when apart.
+bool touch_ball(const Core& s, const Core& t, double tolerance, double* normal, double* point) {
+  const double* P = t.points;
+  const double nx0 = t.axes[0], ny0 = t.axes[1], nz0 = t.axes[2];
+  const double* c = s.center;
+  const double r = s.half[0];
+  const bool front = (c[0] - P[0]) * nx0 + (c[1] - P[1]) * ny0 + (c[2] - P[2]) * nz0 >= 0;
+  const double k = front ? 1 : -1;
+  const double up[3] = {k * nx0, k * ny0, k * nz0};
+  // Ericson 5.1.5, the closest point on the triangle (src/triangle.js:240-270).
+  double Q[3];
+  {
+    const double ax = P[0], ay = P[1], az = P[2];
+    const double abx = P[3] - ax, aby = P[4] - ay, abz = P[5] - az, acx = P[6] - ax, acy = P[7] - ay, acz = P[8] - az;
+    const double apx = c[0] - ax, apy = c[1] - ay, apz = c[2] - az;
+    const double d1 = abx * apx + aby * apy + abz * apz, d2 = acx * apx + acy * apy + acz * apz;
+    double u, v;
+    const double bpx = c[0] - P[3], bpy = c[1] - P[4], bpz = c[2] - P[5];
+    const double d3 = abx * bpx + aby * bpy + abz * bpz, d4 = acx * bpx + acy * bpy + acz * bpz;
+    const double vc = d1 * d4 - d3 * d2;
+    const double cpx = c[0] - P[6], cpy = c[1] - P[7], cpz = c[2] - P[8];
+    const double d5 = abx * cpx + aby * cpy + abz * cpz, d6 = acx * cpx + acy * cpy + acz * cpz;
+    const double vb = d5 * d2 - d1 * d6, va = d3 * d6 - d5 * d4;
+    if (d1 <= 0 && d2 <= 0) { u = 0; v = 0; }
+    else if (d3 >= 0 && d4 <= d3) { u = 1; v = 0; }
+    else if (vc <= 0 && d1 >= 0 && d3 <= 0) { u = d1 / (d1 - d3); v = 0; }
+    else if (d6 >= 0 && d5 <= d6) { u = 0; v = 1; }
+    else if (vb <= 0 && d2 >= 0 && d6 <= 0) { u = 0; v = d2 / (d2 - d6); }
+    else if (va <= 0 && d4 - d3 >= 0 && d5 - d6 >= 0) {
+      const double wt = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+      u = 1 - wt; v = wt;
+    } else {
+      const double den = 1 / (va + vb + vc);
+      u = vb * den; v = vc * den;
+    }
+    Q[0] = ax + u * abx + v * acx; Q[1] = ay + u * aby + v * acy; Q[2] = az + u * abz + v * acz;
+  }
+  double nx = c[0] - Q[0], ny = c[1] - Q[1], nz = c[2] - Q[2];
+  const double d = std::sqrt(nx * nx + ny * ny + nz * nz);
+  double sep;
+  if (d > r / 67108864) {
+    nx /= d; ny /= d; nz /= d;
+    sep = d - r;
+  } else {
+    nx = up[0]; ny = up[1]; nz = up[2];
+    sep = (c[0] - P[0]) * up[0] + (c[1] - P[1]) * up[1] + (c[2] - P[2]) * up[2] - r;
+  }
+  if (sep > 0) return false;
+  normal[0] = -nx; normal[1] = -ny; normal[2] = -nz;
+  const double m = r + sep / 2;
+  point[0] = c[0] - nx * m; point[1] = c[1] - ny * m; point[2] = c[2] - nz * m;
+  return true;
+}
+
+// src/mesh.js:317-331 (grid, triangle): triangle t of field F's corners, in its body's frame.
+void field_corners(const bm_world* w, const bm_world::HostField& F, int t, double* out) {
+  const int cell = t >> 1, cells = F.columns - 1, i = cell % cells, j = (cell - i) / cells;
+  int g[6];
+  if ((t & 1) == 0) { g[0] = i; g[1] = j; g[2] = i + 1; g[3] = j + 1; g[4] = i + 1; g[5] = j; }
+  else { g[0] = i; g[1] = j; g[2] = i; g[3] = j + 1; g[4] = i + 1; g[5] = j + 1; }
+  const double* h = w->field_heights.data();
+  for (int c = 0; c < 3; c++) {
+    out[3 * c] = F.x0 + g[2 * c] * F.spacing;
+    out[3 * c + 1] = F.at(h, g[2 * c], g[2 * c + 1]);
+    out[3 * c + 2] = F.z0 + g[2 * c + 1] * F.spacing;
+  }
+}
+
+// src/query.js:56-66 (triangle): triangle t of slot s's field placed in the world.
+void field_triangle(const bm_world* w, int s, int t, Core& tri) {
+  const auto& F = w->fields[w->field[s]];
+  double* p = tri.points;
+  field_corners(w, F, t, p);
+  const double* q = &w->rotation[4 * s];
+  const double* pos = &w->position[3 * s];
+  for (int v = 0; v < 9; v += 3) {
+    double l[3];
+    rotate(q, p[v], p[v + 1], p[v + 2], false, l);
+    p[v] = pos[0] + l[0]; p[v + 1] = pos[1] + l[1]; p[v + 2] = pos[2] + l[2];
+  }
+  for (int k = 0; k < 3; k++) tri.center[k] = (p[k] + p[k + 3] + p[k + 6]) / 3;
+  // src/mesh.js:104-113 (normal): left as it was for a triangle with no area.
+  const double ux = p[3] - p[0], uy = p[4] - p[1], uz = p[5] - p[2];
+  const double vx = p[6] - p[0], vy = p[7] - p[1], vz = p[8] - p[2];
+  const double nx = uy * vz - uz * vy, ny = uz * vx - ux * vz, nz = ux * vy - uy * vx;
+  const double l = std::sqrt(nx * nx + ny * ny + nz * nz);
+  if (l > 0) { tri.axes[0] = nx / l; tri.axes[1] = ny / l; tri.axes[2] = nz / l; }
+}
+
+// src/mesh.js:388-401 (triangles): both triangles of every cell under the box whose heights reach it.
+template <class V>
+void field_triangles(const bm_world* w, const bm_world::HostField& F, double x0, double y0, double z0, double x1,
+                     double y1, double z1, V&& visit) {
+  const double s = F.spacing;
+  const int cells = F.columns - 1;
+  const double i0 = js_max(0, std::floor((x0 - F.x0) / s)), i1 = js_min(cells - 1, std::floor((x1 - F.x0) / s));
+  const double j0 = js_max(0, std::floor((z0 - F.z0) / s)), j1 = js_min(F.rows - 2, std::floor((z1 - F.z0) / s));
+  if (!(i0 <= i1) || !(j0 <= j1)) return;
+  const double* h = w->field_heights.data();
+  for (int j = (int)j0; j <= (int)j1; j++) {
+    for (int i = (int)i0; i <= (int)i1; i++) {
+      const double a = F.at(h, i, j), b = F.at(h, i + 1, j), c = F.at(h, i, j + 1), d = F.at(h, i + 1, j + 1);
+      if (js_max(js_max(js_max(a, b), c), d) < y0 || js_min(js_min(js_min(a, b), c), d) > y1) continue;
+      const int t = 2 * (j * cells + i);
+      visit(t);
+      visit(t + 1);
+    }
+  }
+}
+
 // What advance() found: where along the sweep, the normal (body towards cast) and the point.
 struct Hit {
   double normal[3], point[3];
@@ -1347,6 +1631,18 @@ double advance(bm_world* w, Core& cast, const Core& b, double scale, const doubl
         const double nx = (na[0] - nb[0]) / dist, ny = (na[1] - nb[1]) / dist, nz = (na[2] - nb[2]) / dist;
         hit.normal[0] = nx; hit.normal[1] = ny; hit.normal[2] = nz;
         hit.point[0] = nb[0] + nx * rb; hit.point[1] = nb[1] + ny * rb; hit.point[2] = nb[2] + nz * rb;
+        return t;
+      }
+      if (b.kind == CORE_TRIANGLE) {
+        // src/query.js:388: a triangle met alone (seams BARE, no folds) asks touch().
+        if (cast.kind != CORE_SPHERE)
+          return rc = fail(w, BM_ERR_UNSUPPORTED, "step: a swept shape other than a ball deep in level geometry"), -1;
+        double n[3], at[3];
+        if (touch_ball(cast, b, tolerance, n, at)) {
+          for (int k = 0; k < 3; k++) { hit.normal[k] = -n[k]; hit.point[k] = at[k]; }
+        } else {
+          for (int k = 0; k < 3; k++) { hit.normal[k] = -d[k]; hit.point[k] = cast.center[k]; }
+        }
         return t;
       }
       double ra_rec[19], rb_rec[19];
@@ -1386,6 +1682,58 @@ double advance(bm_world* w, Core& cast, const Core& b, double scale, const doubl
 // src/query.js:607-642 (sweep, sweepVisit) for a world of plain bodies: the first slot
 // the cast meets sliding from its centre along unit d for up to max_t, or -1; out gets
 // the distance, the normal and the point. Slots `skip` marks are passed over.
+// src/query.js:644-668 (sweepLevel, sweepTriangle) and src/mesh.js:341-363 (Heightfield.ray,
+// a cast of any size): `cast` slid from o along d against the triangles of slot s's field
+// under the box the cast sweeps through, each by advance(); the nearest touch's distance, or -1.
+double sweep_level(bm_world* w, Core& cast, int s, const double* o, const double* d, const double* g, double max_t,
+                   Hit& hit, int& rc) {
+  const double* q = &w->rotation[4 * s];
+  const double* pos = &w->position[3 * s];
+  double l[3], u[3], half[3] = {0, 0, 0};
+  rotate(q, o[0] - pos[0], o[1] - pos[1], o[2] - pos[2], true, l);
+  rotate(q, d[0], d[1], d[2], true, u);
+  // src/query.js:73-81 (extentsIn)
+  for (int j = 0; j < 3; j++) {
+    double e[3];
+    rotate(q, j == 0 ? 1 : 0, j == 1 ? 1 : 0, j == 2 ? 1 : 0, true, e);
+    for (int k = 0; k < 3; k++) half[k] += std::fabs(e[k]) * g[j];
+  }
+  const auto& F = w->fields[w->field[s]];
+  double t0 = 0, t1 = max_t;
+  for (int k = 0; k < 3; k++) {
+    if (u[k] == 0) {
+      if (l[k] < F.lo[k] - half[k] || l[k] > F.hi[k] + half[k]) return -1;
+      continue;
+    }
+    double near = (F.lo[k] - half[k] - l[k]) / u[k], far = (F.hi[k] + half[k] - l[k]) / u[k];
+    if (near > far) { const double x = near; near = far; far = x; }
+    if (near > t0) t0 = near;
+    if (far < t1) t1 = far;
+    if (t0 > t1) return -1;
+  }
+  const double ax = l[0] + t0 * u[0], ay = l[1] + t0 * u[1], az = l[2] + t0 * u[2];
+  const double bx = l[0] + t1 * u[0], by = l[1] + t1 * u[1], bz = l[2] + t1 * u[2];
+  int best = -1;
+  double level_max = max_t;
+  Hit level_hit{};
+  const double scale = core_size(cast);
+  field_triangles(w, F, js_min(ax, bx) - half[0], js_min(ay, by) - half[1], js_min(az, bz) - half[2],
+                  js_max(ax, bx) + half[0], js_max(ay, by) + half[1], js_max(az, bz) + half[2], [&](int t) {
+                    if (rc != BM_OK) return;
+                    Core tri;
+                    tri.kind = CORE_TRIANGLE;
+                    for (int k = 0; k < 9; k++) tri.axes[k] = (k == 0 || k == 4 || k == 8) ? 1 : 0;
+                    for (int k = 0; k < 3; k++) { tri.half[k] = 0; tri.weight[k] = 0; }
+                    field_triangle(w, s, t, tri);
+                    Hit h;
+                    const double at = advance(w, cast, tri, scale, o, d, level_max, true, h, rc);
+                    if (at >= 0) { best = t; level_max = at; level_hit = h; }
+                  });
+  if (best < 0) return -1;
+  hit = level_hit;
+  return level_max;
+}
+
 int sweep(bm_world* w, Core& cast, int self, const double* d, double max_t, double* out, const uint8_t* skip,
           int skipped, int& rc) {
   const double o[3] = {cast.center[0], cast.center[1], cast.center[2]};
@@ -1395,11 +1743,16 @@ int sweep(bm_world* w, Core& cast, int self, const double* d, double max_t, doub
   Hit best_hit{};
   auto visit = [&](int s) -> double {
     if (rc != BM_OK || skip[s] != 0 || s == self) return max_t;
-    Core body;
-    place_body(w, s, body);
-    const double own = w->size[s], scale = own > 0 ? js_min(own, core_size(cast)) : core_size(cast);
     Hit hit;
-    const double t = advance(w, cast, body, scale, o, d, max_t, true, hit, rc);
+    double t;
+    if (w->shape[s] >= MESH) {
+      t = sweep_level(w, cast, s, o, d, g, max_t, hit, rc);
+    } else {
+      Core body;
+      place_body(w, s, body);
+      const double own = w->size[s], scale = own > 0 ? js_min(own, core_size(cast)) : core_size(cast);
+      t = advance(w, cast, body, scale, o, d, max_t, true, hit, rc);
+    }
     if (t < 0) return max_t;
     best = s;
     max_t = t;
@@ -1526,6 +1879,44 @@ void touched(bm_world* w) {
   }
 }
 
+// src/contact.js:1067-1115 (store, keep) for level pairs: the device kept the other pairs'
+// manifolds and worked out every manifold's slip; each solved level manifold goes into its
+// pair's Memory, in the order narrow() found them, so the last one sets the count.
+void store_memory(bm_world* w, int solved) {
+  Contacts& c = w->c;
+  Pairs& pr = w->pairs;
+  for (int f = 0; f < c.count; f++) {
+    const int m = c.row[f];
+    if (m >= solved || c.sub[m] < 0) continue;
+    const int sub = c.sub[m], m3 = 3 * m, s3 = 3 * sub;
+    Memory& kept = pr.memory[c.pair[m]];
+    kept.ensure(sub + 1);
+    kept.count = sub + 1;
+    for (int k = 0; k < 3; k++) kept.normal[s3 + k] = c.normal[m3 + k];
+    kept.key[sub] = 0;
+    const int count = c.points[m];
+    kept.points[sub] = (uint8_t)count;
+    const double t1x = c.t1[m3], t1y = c.t1[m3 + 1], t1z = c.t1[m3 + 2];
+    const double t2x = c.t2[m3], t2y = c.t2[m3 + 1], t2z = c.t2[m3 + 2];
+    const double r1 = c.roll_impulse1[m], r2 = c.roll_impulse2[m], sp = c.spin_impulse[m];
+    kept.twist[s3] = t1x * r1 + t2x * r2 + c.normal[m3] * sp;
+    kept.twist[s3 + 1] = t1y * r1 + t2y * r2 + c.normal[m3 + 1] * sp;
+    kept.twist[s3 + 2] = t1z * r1 + t2z * r2 + c.normal[m3 + 2] * sp;
+    const double l1 = c.tangent_impulse1[m], l2 = c.tangent_impulse2[m];
+    kept.friction[s3] = t1x * l1 + t2x * l2;
+    kept.friction[s3 + 1] = t1y * l1 + t2y * l2;
+    kept.friction[s3 + 2] = t1z * l1 + t2z * l2;
+    const double d1 = c.drift1[m], d2 = c.drift2[m];
+    kept.slip[s3] = t1x * d1 + t2x * d2;
+    kept.slip[s3 + 1] = t1y * d1 + t2y * d2;
+    kept.slip[s3 + 2] = t1z * d1 + t2z * d2;
+    for (int k = 0; k < count; k++) {
+      kept.pid[P * sub + k] = c.pid[P * m + k];
+      kept.normal_impulse[P * sub + k] = c.normal_impulse[P * m + k];
+    }
+  }
+}
+
 // The device phase: colour, rest, inertias .. store (src/world.js:956-1031), through the batch ABI.
 int device_phase(bm_world* w, double dt) {
   Contacts& c = w->c;
@@ -1569,6 +1960,7 @@ int device_phase(bm_world* w, double dt) {
   if (int rc = bm_colour(w->ctx)) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx));
   if (int rc = bm_soft_step(w->ctx)) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx));
   if (int rc = bm_download(w->ctx, &h)) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx));
+  store_memory(w, solved);
   return BM_OK;
 }
 
@@ -1610,8 +2002,13 @@ int bm_world_add(bm_world* w, const bm_bodies* b) {
         !b->inertia || !b->inv_inertia || !b->mass || !b->inv_mass || !b->gravity_scale || !b->damping ||
         !b->angular_damping || !b->size || !b->reach || !b->friction || !b->restitution)
       return fail(w, BM_ERR_ARGUMENT, "addBody: every column is needed");
-    if (b->shape[i] != BOX && b->shape[i] != SPHERE && b->shape[i] != CAPSULE && b->shape[i] != HULL)
-      return fail(w, BM_ERR_UNSUPPORTED, "addBody: this world takes boxes, spheres, capsules and hulls");
+    if (b->shape[i] != BOX && b->shape[i] != SPHERE && b->shape[i] != CAPSULE && b->shape[i] != HULL &&
+        b->shape[i] != HEIGHTFIELD)
+      return fail(w, BM_ERR_UNSUPPORTED, "addBody: this world takes boxes, spheres, capsules, hulls and heightfields");
+    if (b->shape[i] == HEIGHTFIELD && (!b->field || b->field[i] < 0 || b->field[i] >= (int)w->fields.size()))
+      return fail(w, BM_ERR_ARGUMENT, "addBody: a heightfield body names no field of the table (bm_world_set_fields)");
+    if (b->shape[i] == HEIGHTFIELD && b->type[i] != STATIC)
+      return fail(w, BM_ERR_ARGUMENT, "addBody: level geometry is never dynamic");
     if (b->shape[i] == HULL && (!b->hull || b->hull[i] < 0 || b->hull[i] >= (int)w->hulls.size()))
       return fail(w, BM_ERR_ARGUMENT, "addBody: a hull body names no hull of the table (bm_world_set_hulls)");
     const bool framed = b->framed && b->framed[i] != 0;
@@ -1627,7 +2024,7 @@ int bm_world_add(bm_world* w, const bm_bodies* b) {
   for (auto* v : {&w->position, &w->velocity, &w->angular, &w->dims, &w->inertia, &w->inv_inertia, &w->start, &w->tumbled})
     grow(*v, 3);
   for (auto* v : {&w->rotation, &w->start_q, &w->frame}) grow(*v, 4);
-  grow(w->offset, 3); grow(w->hull, 1);
+  grow(w->offset, 3); grow(w->hull, 1); grow(w->field, 1);
   grow(w->inv_i, 6); grow(w->axes, 9); grow(w->box, 6); grow(w->rest, 7);
   for (auto* v : {&w->mass, &w->inv_mass, &w->motion, &w->ceiling, &w->gravity_scale, &w->damping, &w->angular_damping,
                   &w->size, &w->reach, &w->friction, &w->restitution, &w->still, &w->least})
@@ -1659,6 +2056,7 @@ int bm_world_add(bm_world* w, const bm_bodies* b) {
     w->friction[s] = b->friction[i]; w->restitution[s] = b->restitution[i];
     w->type[s] = b->type[i]; w->shape[s] = b->shape[i]; w->lock[s] = b->lock[i];
     w->hull[s] = b->shape[i] == HULL ? b->hull[i] : -1;
+    w->field[s] = b->shape[i] == HEIGHTFIELD ? b->field[i] : -1;
     w->framed[s] = b->framed && b->framed[i] != 0 ? 1 : 0;
     for (int k = 0; k < 3; k++) w->offset[3 * s + k] = w->framed[s] ? b->offset[3 * i + k] : 0;
     for (int k = 0; k < 4; k++) w->frame[4 * s + k] = w->framed[s] ? b->frame[4 * i + k] : (k == 3 ? 1 : 0);
@@ -1706,6 +2104,39 @@ int bm_world_set_hulls(bm_world* w, int32_t n, const int32_t* hull_v, const int3
   }
   w->hulls.swap(hulls);
   w->hull_vertices.swap(hv);
+  return BM_OK;
+}
+
+int bm_world_set_fields(bm_world* w, int32_t n, const int32_t* columns, const int32_t* rows, const double* spacing,
+                        const int64_t* offsets, const double* heights) {
+  if (!w) return BM_ERR_ARGUMENT;
+  w->error.clear();
+  for (int s = 0; s < w->top; s++)
+    if (w->shape[s] == HEIGHTFIELD && w->field[s] >= n)
+      return fail(w, BM_ERR_ARGUMENT, "bm_world_set_fields: a body already added names a field the table lacks");
+  // The device's copy first: it checks the table's layout.
+  if (int rc = bm_set_fields(w->ctx, n, columns, rows, spacing, offsets, heights))
+    return fail(w, rc, bm_last_error(w->ctx));
+  std::vector<bm_world::HostField> fields(n);
+  std::vector<double> all;
+  for (int f = 0; f < n; f++) {
+    auto& F = fields[f];
+    F.columns = columns[f]; F.rows = rows[f]; F.spacing = spacing[f];
+    F.offset = all.size();
+    const double* h = heights + offsets[f];
+    const size_t count = (size_t)columns[f] * rows[f];
+    all.insert(all.end(), h, h + count);
+    // src/mesh.js:296-305: centred on its body; its box and reach.
+    F.x0 = -((F.columns - 1) * F.spacing) / 2;
+    F.z0 = -((F.rows - 1) * F.spacing) / 2;
+    double lo = INFINITY, hi = -INFINITY;
+    for (size_t k = 0; k < count; k++) { lo = js_min(lo, h[k]); hi = js_max(hi, h[k]); }
+    F.lo[0] = F.x0; F.lo[1] = lo; F.lo[2] = F.z0;
+    F.hi[0] = -F.x0; F.hi[1] = hi; F.hi[2] = -F.z0;
+    F.reach = std::sqrt(F.x0 * F.x0 + F.z0 * F.z0 + js_max(lo * lo, hi * hi));
+  }
+  w->fields.swap(fields);
+  w->field_heights.swap(all);
   return BM_OK;
 }
 

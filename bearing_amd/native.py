@@ -60,7 +60,7 @@ EXPORTS = ["bm_create", "bm_destroy", "bm_last_error", "bm_upload", "bm_soft_ste
            "bm_soft_step_host", "bm_last_kernel_ms", "bm_set_launch", "bm_lds_limit", "bm_colour", "bm_collide_host", "bm_meet_host", "bm_set_colour_lds", "bm_set_hulls",
            "bm_set_fields", "bm_level_host",
            "bm_world_create", "bm_world_destroy", "bm_world_error", "bm_world_add", "bm_world_step", "bm_world_read",
-           "bm_world_set_hulls"]
+           "bm_world_set_hulls", "bm_world_set_fields"]
 
 _lib = None
 
@@ -103,7 +103,8 @@ class Bodies(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int32), ("type", _U8), ("shape", _U8), ("lock", _U8)] + [
         (f, _D) for f in ("position", "rotation", "velocity", "angular", "dims", "inertia", "inv_inertia", "mass",
                           "inv_mass", "gravity_scale", "damping", "angular_damping", "size", "reach", "friction",
-                          "restitution")] + [("hull", _I32), ("framed", _U8), ("offset", _D), ("frame", _D)]
+                          "restitution")] + [("hull", _I32), ("framed", _U8), ("offset", _D), ("frame", _D),
+                                                         ("field", _I32)]
 
 
 class WorldState(ctypes.Structure):
@@ -125,6 +126,7 @@ def world_signatures(lib):
     lib.bm_world_step.argtypes = [vp, ctypes.c_double]
     lib.bm_world_read.argtypes = [vp, ctypes.POINTER(WorldState)]
     lib.bm_world_set_hulls.argtypes = [vp, ctypes.c_int32, _I32, _I32, _I32, _D, _I32, _I32, _D]
+    lib.bm_world_set_fields.argtypes = [vp, ctypes.c_int32, _I32, _I32, _D, ctypes.POINTER(ctypes.c_int64), _D]
 
 
 class BearingError(RuntimeError):
@@ -323,9 +325,10 @@ BODY_COLUMNS = [("type", 1, np.uint8), ("shape", 1, np.uint8), ("lock", 1, np.ui
                 ("mass", 1, np.float64), ("inv_mass", 1, np.float64), ("gravity_scale", 1, np.float64),
                 ("damping", 1, np.float64), ("angular_damping", 1, np.float64), ("size", 1, np.float64),
                 ("reach", 1, np.float64), ("friction", 1, np.float64), ("restitution", 1, np.float64),
-                ("hull", 1, np.int32), ("framed", 1, np.uint8), ("offset", 3, np.float64), ("frame", 4, np.float64)]
+                ("hull", 1, np.int32), ("framed", 1, np.uint8), ("offset", 3, np.float64), ("frame", 4, np.float64),
+                ("field", 1, np.int32)]
 # Columns a world of unframed boxes, spheres and capsules can leave out (passed as NULL).
-OPTIONAL_COLUMNS = {"hull", "framed", "offset", "frame"}
+OPTIONAL_COLUMNS = {"hull", "framed", "offset", "frame", "field"}
 
 
 class World:
@@ -373,6 +376,19 @@ class World:
         self.check(self.lib.bm_world_set_hulls(self.ptr, len(hv) - 1, hv.ctypes.data_as(_I32), hf.ctypes.data_as(_I32),
                                                hl.ctypes.data_as(_I32), v.ctypes.data_as(_D), st.ctypes.data_as(_I32),
                                                lp.ctypes.data_as(_I32), pl.ctypes.data_as(_D)), "bm_world_set_hulls")
+
+    def set_fields(self, columns, rows, spacing, heights):
+        """bm_world_set_fields: field f columns[f] x rows[f] samples spacing[f] apart, heights[f]
+        its samples row by row."""
+        n = len(columns)
+        cols, rws = np.ascontiguousarray(columns, np.int32), np.ascontiguousarray(rows, np.int32)
+        sp = np.ascontiguousarray(spacing, np.float64)
+        off = np.zeros(n + 1, np.int64)
+        off[1:] = np.cumsum([len(h) for h in heights])
+        flat = np.ascontiguousarray(np.concatenate([np.asarray(h, np.float64) for h in heights]) if n else np.zeros(1))
+        self.check(self.lib.bm_world_set_fields(self.ptr, n, cols.ctypes.data_as(_I32), rws.ctypes.data_as(_I32),
+                                                sp.ctypes.data_as(_D), off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                                flat.ctypes.data_as(_D)), "bm_world_set_fields")
 
     def step(self, dt):
         self.check(self.lib.bm_world_step(self.ptr, dt), "bm_world_step")

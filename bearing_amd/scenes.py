@@ -16,6 +16,8 @@ Scenes:
   pile_world   one world of the batched workload: a tower or a sphere pile
   throw        a thrown box landing and hitting a 6x6 wall of unit boxes
   capsules     capsules, boxes and balls dropped in a heap on a floor
+  terrain      the mixed pile small: 400 bodies on a 48 x 48 heightfield
+  mixed_pile   20,000 boxes, balls, capsules and hulls dropped onto a 256 x 256 heightfield
   rocks        hulls of random points with boxes, balls and capsules dropped in a heap, and
                small balls fired through two static hull boulders
   darts        small balls and rods fired at 20-200 m/s at a 2 cm static plate and the
@@ -283,6 +285,37 @@ def rocks(rng: SplitMix64, count: int = 40):
     return bodies
 
 
+def mixed_pile(rng: SplitMix64, count: int = 20000, columns: int = 256, volume=(60.0, 80.0, 60.0), low: float = 2.0,
+               cell: float = 8.0, amplitude: float = 2.0):
+    """The mixed pile: a static heightfield of `columns` x `columns` samples 1 m apart
+    (value_noise, amplitude 2 m, 8 m cells) and `count` bodies dropped at rest onto it from
+    a volume centred over it (y from `low` up): 40% boxes (half-extents U[0.1, 0.5]),
+    25% balls (radius U[0.1, 0.4]), 20% capsules (radius U[0.05, 0.2], half-length
+    U[0.1, 0.5]), 15% hulls of 16 points N(0, 0.3^2); every turn normalised 4D Gaussian."""
+    heights = value_noise(rng, columns, columns, 1.0, amplitude, cell)
+    bodies = [{"type": "static", "shape": "heightfield", "heights": heights, "columns": columns, "spacing": 1.0}]
+    vx, vy, vz = volume
+    for _ in range(count):
+        u = rng.uniform()
+        pos = [rng.between(-vx / 2, vx / 2), rng.between(low, low + vy), rng.between(-vz / 2, vz / 2)]
+        q = [rng.gauss(1.0) for _ in range(4)]
+        n = math.sqrt(sum(x * x for x in q))
+        rot = [x / n for x in q]
+        if u < 0.4:
+            body = {"shape": "box", "size": [2 * rng.between(0.1, 0.5) for _ in range(3)]}
+        elif u < 0.65:
+            body = {"shape": "sphere", "radius": rng.between(0.1, 0.4)}
+        elif u < 0.85:
+            r = rng.between(0.05, 0.2)
+            body = {"shape": "capsule", "radius": r, "height": 2 * r + 2 * rng.between(0.1, 0.5)}
+        else:
+            body = {"shape": "hull", "vertices": _cloud(rng, 16, 0.3)}
+        body["position"] = pos
+        body["rotation"] = rot
+        bodies.append(body)
+    return bodies
+
+
 SCENES = {
     "pyramid": lambda: pyramid(),
     "rain": lambda: rain(),
@@ -293,6 +326,8 @@ SCENES = {
     "darts": lambda: darts(SplitMix64(7)),
     "capsules": lambda: capsule_heap(SplitMix64(9)),
     "rocks": lambda: rocks(SplitMix64(11)),
+    "terrain": lambda: mixed_pile(SplitMix64(13), count=400, columns=48, volume=(24.0, 30.0, 24.0)),
+    "mixed_pile": lambda: mixed_pile(SplitMix64(3)),
 }
 
 

@@ -239,7 +239,8 @@ double bm_last_kernel_ms(const bm_context* ctx);
 
 /* ---- a whole world, stepped as src/world.js step (880-1072) steps it ----
  *
- * For worlds of dynamic and static boxes, spheres, capsules and hulls, with no joints, sensors,
+ * For worlds of dynamic and static boxes, spheres, capsules and hulls, and static heightfields,
+ * with no joints, sensors,
  * characters or compounds. The host keeps the reference's order-defining serial
  * state (the broadphase trees, the pair list, the narrow driver's order, the
  * sleeping islands); the device runs each step's narrowphase batches, colouring
@@ -252,13 +253,15 @@ typedef struct bm_world bm_world;
  * entry a body: the columns the reference keeps by slot. type: 0 dynamic,
  * 1 static; shape: 0 box (dims = full size), 1 sphere (dims[0] = radius),
  * 2 capsule (dims[0] = radius, dims[1] = height end to end), 3 hull (hull = its
- * number in the table bm_world_set_hulls set, in the frame below).
+ * number in the table bm_world_set_hulls set, in the frame below), 6 heightfield (static;
+ * field = its number in the table bm_world_set_fields set).
  * position and rotation are the step's (world._position, _rotation): a framed body's
  * centre of mass and principal axes. framed, offset (3) and frame (4) are
  * world._framed, _offset and _frame (src/world.js:2199-2205): the transforms give a
  * framed body's own origin and turn back as src/world.js:1917-1930 (_pose) does. A
  * framed box, sphere or capsule is a body with parts, which this world does not take.
- * hull may be NULL when no body is a hull; framed, offset and frame when none is framed. */
+ * hull may be NULL when no body is a hull, field when none is a heightfield; framed, offset
+ * and frame when none is framed. */
 typedef struct bm_bodies {
   int32_t n;
   const uint8_t *type, *shape, *lock;
@@ -268,6 +271,7 @@ typedef struct bm_bodies {
   const int32_t* hull;
   const uint8_t* framed;
   const double *offset, *frame; /* 3, 4 */
+  const int32_t* field;
 } bm_bodies;
 
 /* What bm_world_read copies out; any pointer may be NULL. transforms: 7 floats a
@@ -293,6 +297,11 @@ int bm_world_add(bm_world* world, const bm_bodies* bodies);
 int bm_world_set_hulls(bm_world* world, int32_t n, const int32_t* hull_v, const int32_t* hull_f,
                        const int32_t* hull_l, const double* vertices, const int32_t* start, const int32_t* loops,
                        const double* planes);
+/* The heightfields the world's heightfield bodies name, in the layout of bm_set_fields
+ * (src/mesh.js:288 Heightfield). Replaces the table; fails if a body already added names a
+ * field the new table lacks. */
+int bm_world_set_fields(bm_world* world, int32_t n, const int32_t* columns, const int32_t* rows,
+                        const double* spacing, const int64_t* offsets, const double* heights);
 /* world.step(dt) (src/world.js:880). */
 int bm_world_step(bm_world* world, double dt);
 int bm_world_read(const bm_world* world, bm_world_state* out);

@@ -798,13 +798,14 @@ void keep(OracleSoft* w, int p, int m) {
   }
 }
 
-// src/contact.js:1067-1088 (store), pairs of plain bodies only.
+// src/contact.js:1067-1088 (store): slip for every solved manifold; the pair list keeps the
+// plain pairs' (level pairs' Memory is the caller's).
 int store(OracleSoft* w) {
   for (int f = 0; f < w->count; f++) {
     const int m = w->row[f];
     if (m >= w->solved) continue;
     slipped(w, m);
-    if (w->sub[m] >= 0) return -1;
+    if (w->sub[m] >= 0) continue;   // a level pair's: into its Memory, which the caller keeps
     keep(w, w->pair[m], m);
   }
   return 0;
@@ -817,8 +818,6 @@ using namespace oracle;
 
 // src/world.js:961-1031, for a world with no joints.
 extern "C" int oracle_soft_step(OracleSoft* w) {
-  for (int m = 0; m < w->solved; m++)
-    if (w->sub[m] >= 0) return -1;
   const bool touching = w->count > 0;
   const double h = w->dt / w->substeps;
   if (touching) inertias(w);

@@ -133,6 +133,23 @@ function main() {
     init['hulls.planes'] = Float64Array.from(t.planes);
     for (const k of ['start', 'loops', 'hullV', 'hullF', 'hullL']) init['hulls.' + k] = Int32Array.from(t[k]);
   }
+  // Heightfield bodies: each one's field (world._geometry), numbered in slot order.
+  if (Array.from(world._shape.subarray(0, top0)).some((k) => k === 6)) {
+    const fieldOf = new Int32Array(top0).fill(-1);
+    const cols = [], rows = [], spacing = [], heights = [];
+    for (let s = 0; s < top0; s++) {
+      if (world._shape[s] !== 6) continue;
+      const g = world._geometry[s];
+      fieldOf[s] = cols.length;
+      cols.push(g.columns); rows.push(g.rows); spacing.push(g.spacing);
+      for (const h of g.heights) heights.push(h);
+    }
+    init['init.field'] = fieldOf;
+    init['fields.columns'] = Int32Array.from(cols);
+    init['fields.rows'] = Int32Array.from(rows);
+    init['fields.spacing'] = Float64Array.from(spacing);
+    init['fields.heights'] = Float64Array.from(heights);
+  }
   const softAt = new Set(job.soft || []), colourAt = new Set(job.colour || []), traceAt = job.trace || -1;
   let rec = null, meta = null, trace = null, traceCount = 0, colourRec = null;
   const snap = (tag) => {

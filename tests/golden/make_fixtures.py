@@ -31,6 +31,7 @@ JOBS = {
     "darts": dict(scene="darts", dt=1 / 60, steps=90, soft=[2, 20], colour=[], trace=-1, every=30),
     "capsules": dict(scene="capsules", dt=1 / 60, steps=240, soft=[60, 120], colour=[120], trace=-1, every=60),
     "rocks": dict(scene="rocks", dt=1 / 60, steps=240, soft=[60, 120], colour=[120], trace=-1, every=60),
+    "terrain": dict(scene="terrain", dt=1 / 60, steps=300, soft=[150], colour=[150], trace=-1, every=100),
     # The 2,000-sphere rain at full size: its solve phase mid-landing is the bench's workload.
     "rain": dict(scene="rain", dt=1 / 60, steps=900, soft=[300], colour=[], trace=-1, every=300),
 }

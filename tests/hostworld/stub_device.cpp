@@ -16,6 +16,12 @@ extern "C" void oracle_losses(double* out);
 extern "C" int oracle_meet(int n, const double* a, const double* b, const double* pair, int32_t* count,
                            double* normal, double* points, double* separations, uint32_t* ids, double* shift,
                            int32_t* flags, double* loss);
+extern "C" void oracle_fields(int n, const int32_t* columns, const int32_t* rows, const double* spacing,
+                              const int64_t* off, const double* heights);
+extern "C" int oracle_level(int n, const int32_t* field, const double* position, const double* axes,
+                            const double* shapes, const double* boxes, const double* margin, const double* tolerance,
+                            int cap, int32_t* manifolds, int32_t* count, double* normal, double* points,
+                            double* separations, uint32_t* ids);
 extern "C" void oracle_hulls(int n, const int32_t* hull_v, const int32_t* hull_f, const int32_t* hull_l,
                              const double* vertices, const int32_t* start, const int32_t* loops, const double* planes);
 
@@ -129,6 +135,39 @@ int bm_collide_host(bm_context*, int32_t n, const double* a, const double* b, co
                     uint32_t* ids, double* loss) {
   oracle_collide(n, a, b, margin, tolerance, count, normal, points, separations, ids);
   if (loss) oracle_losses(loss);
+  return BM_OK;
+}
+
+int bm_set_fields(bm_context*, int32_t n, const int32_t* columns, const int32_t* rows, const double* spacing,
+                  const int64_t* offsets, const double* heights) {
+  oracle_fields(n, columns, rows, spacing, offsets, heights);
+  return BM_OK;
+}
+
+// As the device does, every direction is counted even past cap; only cap are written.
+int bm_level_host(bm_context*, int32_t n, const int32_t* field, const double* position, const double* axes,
+                  const double* shapes, const double* boxes, const double* margin, const double* tolerance, int32_t cap,
+                  int32_t* manifolds, int32_t* count, double* normal, double* points, double* separations,
+                  uint32_t* ids) {
+  const int big = 256;
+  std::vector<int32_t> m(n), c((size_t)big * n);
+  std::vector<double> nr(3 * (size_t)big * n), pt(12 * (size_t)big * n), sp(4 * (size_t)big * n);
+  std::vector<uint32_t> id(4 * (size_t)big * n);
+  if (oracle_level(n, field, position, axes, shapes, boxes, margin, tolerance, big, m.data(), c.data(), nr.data(),
+                   pt.data(), sp.data(), id.data()) != n)
+    return BM_ERR_UNSUPPORTED;
+  for (int k = 0; k < n; k++) {
+    manifolds[k] = m[k];
+    for (int j = 0; j < cap; j++) {
+      const size_t o = (size_t)k * cap + j, q = (size_t)k * big + j;
+      const bool has = j < m[k];
+      count[o] = has ? c[q] : 0;
+      for (int i = 0; i < 3; i++) normal[3 * o + i] = has ? nr[3 * q + i] : 0;
+      for (int i = 0; i < 12; i++) points[12 * o + i] = has ? pt[12 * q + i] : 0;
+      for (int i = 0; i < 4; i++) separations[4 * o + i] = has ? sp[4 * q + i] : 0;
+      for (int i = 0; i < 4; i++) ids[4 * o + i] = has ? id[4 * q + i] : 0;
+    }
+  }
   return BM_OK;
 }
 

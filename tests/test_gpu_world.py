@@ -15,7 +15,7 @@ def traj(name):
     return fixtures.load(os.path.join(fixtures.DATA, f"{name}.traj.bmfx.gz"))
 
 
-@pytest.mark.parametrize("name", ["pyramid", "tower", "throw", "rain_small", "pile_small", "darts", "capsules", "rocks", "rain"])
+@pytest.mark.parametrize("name", ["pyramid", "tower", "throw", "rain_small", "pile_small", "darts", "capsules", "rocks", "terrain", "rain"])
 def test_gpu_world_matches_reference_trajectory(name):
     print("trajectory", name, flush=True)
     fx = traj(name)

@@ -26,7 +26,7 @@ def traj(name):
     return fixtures.load(os.path.join(fixtures.DATA, f"{name}.traj.bmfx.gz"))
 
 
-@pytest.mark.parametrize("name", ["pyramid", "tower", "throw", "rain_small", "pile_small", "darts", "capsules", "rocks", "rain"])
+@pytest.mark.parametrize("name", ["pyramid", "tower", "throw", "rain_small", "pile_small", "darts", "capsules", "rocks", "terrain", "rain"])
 def test_host_world_matches_reference_trajectory(lib, name):
     fx = traj(name)
     w = make_world(fx, lib=lib)

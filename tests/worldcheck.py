@@ -30,11 +30,20 @@ def hull_table(fx):
 
 
 def make_world(fx, **kw):
-    """A native.World holding a trajectory fixture's bodies (and hulls), as addBody left them."""
+    """A native.World holding a trajectory fixture's bodies (and hulls and heightfields), as
+    addBody left them."""
     w = native.World(**kw)
     table = hull_table(fx)
     if table is not None:
         w.set_hulls(table)
+    if "fields.columns" in fx:
+        cols, rows = fx["fields.columns"], fx["fields.rows"]
+        sizes = [int(c) * int(r) for c, r in zip(cols, rows)]
+        heights, at = [], 0
+        for n in sizes:
+            heights.append(fx["fields.heights"][at:at + n])
+            at += n
+        w.set_fields(cols, rows, fx["fields.spacing"], heights)
     w.add(init_columns(fx))
     return w
 
@@ -56,16 +65,21 @@ def run(world, fx, steps=None):
 
 
 def refuses(new_world):
-    """addBody fails loudly on what a world does not take: a compound (shape 4), a hull
-    naming no hull of the table, and a framed box (a body with parts)."""
+    """addBody fails loudly on what a world does not take: a compound (shape 4), a hull or a
+    heightfield naming no table entry, and a framed box (a body with parts)."""
     import fixtures
     fx = fixtures.load(os.path.join(fixtures.DATA, "tower.traj.bmfx.gz"))
     cols = init_columns(fx)
     cols["shape"] = cols["shape"].copy()
     cols["shape"][3] = 4
     w = new_world()
-    with pytest.raises(native.BearingError, match="boxes, spheres, capsules and hulls"):
+    with pytest.raises(native.BearingError, match="boxes, spheres, capsules, hulls and heightfields"):
         w.add(cols)
+    cols["shape"][3] = 6
+    cols["field"] = np.zeros(len(cols["shape"]), np.int32)
+    with pytest.raises(native.BearingError, match="names no field"):
+        w.add(cols)
+    cols.pop("field")
     cols["shape"][3] = 3
     cols["hull"] = np.zeros(len(cols["shape"]), np.int32)
     with pytest.raises(native.BearingError, match="names no hull"):
