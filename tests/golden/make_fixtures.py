@@ -32,6 +32,8 @@ JOBS = {
     "capsules": dict(scene="capsules", dt=1 / 60, steps=240, soft=[60, 120], colour=[120], trace=-1, every=60),
     "rocks": dict(scene="rocks", dt=1 / 60, steps=240, soft=[60, 120], colour=[120], trace=-1, every=60),
     "terrain": dict(scene="terrain", dt=1 / 60, steps=300, soft=[150], colour=[150], trace=-1, every=100),
+    # The 20,000-body mixed pile on a 256 x 256 heightfield at full size: hashes only.
+    "mixed_pile": dict(scene="mixed_pile", dt=1 / 60, steps=600, soft=[], colour=[], trace=-1, every=600),
     # The 2,000-sphere rain at full size: its solve phase mid-landing is the bench's workload.
     "rain": dict(scene="rain", dt=1 / 60, steps=900, soft=[300], colour=[], trace=-1, every=300),
 }
@@ -459,5 +461,5 @@ if __name__ == "__main__":
     if sys.argv[1:] == ["impact"]:
         run_impact()
         sys.exit(0)
-    for name in sys.argv[1:] or [j for j in JOBS if j != "rain"]:
+    for name in sys.argv[1:] or [j for j in JOBS if j not in ("rain", "mixed_pile")]:
         run(name)
