@@ -112,6 +112,7 @@ struct bm_context {
   void* pstage = nullptr;       // pinned host staging for narrowphase batches
   size_t pstage_cap = 0;
   std::vector<int32_t> moff_host, boff_host;
+  std::vector<int32_t> join_body, join_manifold, join_pair, join_awake;  // the joined offset tables
   size_t colour_lds_limit = 0;  // dynamic LDS a colouring workgroup may take (0: not yet asked)
   int32_t colour_in_lds = 1;    // colour worlds that fit in LDS there
   SoftDev dev{};
@@ -380,60 +381,148 @@ int64_t bm_lds_limit(bm_context* ctx) {
   return (int64_t)ctx->lds_limit;
 }
 
-int bm_upload(bm_context* ctx, const bm_columns* h) {
-  if (!ctx || !h) return fail(ctx, BM_ERR_ARGUMENT, "bm_upload: null argument");
-  ctx->error.clear();
-  ctx->uploaded = false;
-  for (int c = 0; c < NCOL; c++) {
-    const Column& col = COLUMNS[c];
-    if (col.unit <= WORLD3 && !field<const void*>(h, col.host))
-      return fail(ctx, BM_ERR_ARGUMENT, std::string("bm_upload: ") + col.name + " is null");
+// Units of kind u in one part (a bm_columns of one or more worlds).
+size_t part_units(const bm_columns* h, Unit u) {
+  const int n = h->n_worlds;
+  switch (u) {
+    case WORLD1: return (size_t)n + 1;
+    case WORLD: return (size_t)n;
+    case STARTS: return (size_t)n * BM_STARTS;
+    case WORLD3: return (size_t)n * 3;
+    case BODY: return (size_t)h->body_offset[n];
+    case AWAKE: return (size_t)h->awake_offset[n];
+    case MANIFOLD: return (size_t)h->manifold_offset[n];
+    case PAIR: return (size_t)h->pair_offset[n];
   }
-  if (int rc = check_batch(ctx, h)) return rc;
-  ctx->laid_out = laid_out(h);
-  ctx->n_worlds = h->n_worlds;
-  ctx->bodies = h->body_offset[h->n_worlds];
-  ctx->manifolds = h->manifold_offset[h->n_worlds];
-  ctx->pairs = h->pair_offset[h->n_worlds];
-  ctx->awake = h->awake_offset[h->n_worlds];
+  return 0;
+}
+
+// The batch of every part's worlds in turn, into one arena: each column the parts' columns
+// one after another (indices stay each world's own), the offset tables joined.
+int upload_parts(bm_context* ctx, int nparts, const bm_columns* const* parts, const char* who) {
+  ctx->uploaded = false;
+  if (nparts <= 0) return fail(ctx, BM_ERR_ARGUMENT, std::string(who) + ": no worlds");
+  int W = 0;
+  bool laid = true;
+  for (int q = 0; q < nparts; q++) {
+    const bm_columns* h = parts[q];
+    if (!h) return fail(ctx, BM_ERR_ARGUMENT, std::string(who) + ": null argument");
+    for (int c = 0; c < NCOL; c++) {
+      const Column& col = COLUMNS[c];
+      if (col.unit <= WORLD3 && !field<const void*>(h, col.host))
+        return fail(ctx, BM_ERR_ARGUMENT, std::string(who) + ": " + col.name + " is null");
+    }
+    if (int rc = check_batch(ctx, h)) return rc;
+    laid = laid && laid_out(h);
+    W += h->n_worlds;
+  }
+  ctx->laid_out = laid;
+  ctx->n_worlds = W;
+  // The joined offset tables.
+  std::vector<int32_t>* tables[4] = {&ctx->join_body, &ctx->join_manifold, &ctx->join_pair, &ctx->join_awake};
+  for (auto* t : tables) t->assign(1, 0);
+  for (int q = 0; q < nparts; q++) {
+    const bm_columns* h = parts[q];
+    const int32_t* src[4] = {h->body_offset, h->manifold_offset, h->pair_offset, h->awake_offset};
+    for (int t = 0; t < 4; t++) {
+      const int32_t base = tables[t]->back();
+      for (int w = 1; w <= h->n_worlds; w++) tables[t]->push_back(base + src[t][w]);
+    }
+  }
+  ctx->bodies = ctx->join_body.back();
+  ctx->manifolds = ctx->join_manifold.back();
+  ctx->pairs = ctx->join_pair.back();
+  ctx->awake = ctx->join_awake.back();
   ctx->most_bodies = 0;
-  for (int w = 0; w < h->n_worlds; w++) {
-    const int32_t nb = h->body_offset[w + 1] - h->body_offset[w];
+  for (int w = 0; w < W; w++) {
+    const int32_t nb = ctx->join_body[w + 1] - ctx->join_body[w];
     if (nb > ctx->most_bodies) ctx->most_bodies = nb;
   }
-  HIP_OR_FAIL(ctx, hipSetDevice(ctx->device), "bm_upload: hipSetDevice");
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->device), (std::string(who) + ": hipSetDevice").c_str());
   static const ArenaOrder ORDER;
   size_t at = 0;
   for (int k = 0; k < NCOL; k++) {
     const int c = ORDER.order[k];
     const Column& col = COLUMNS[c];
     const size_t bytes = units(ctx, col.unit) * col.per * col.elem;
-    if (bytes > 0 && !field<const void*>(h, col.host))
-      return fail(ctx, BM_ERR_ARGUMENT, std::string("bm_upload: ") + col.name + " is null");
     ctx->bytes[c] = bytes;
     ctx->offset[c] = at;
     at += (bytes + 255) & ~(size_t)255;
     if (k + 1 == ORDER.arranged) ctx->arranged_end = at;
   }
   ctx->total = at;
-  if (int rc = grow_device(ctx, &ctx->arena, &ctx->arena_cap, at, "bm_upload")) return rc;
-  if (int rc = grow_pinned(ctx, &ctx->stage, &ctx->stage_cap, at, "bm_upload")) return rc;
+  if (int rc = grow_device(ctx, &ctx->arena, &ctx->arena_cap, at, who)) return rc;
+  if (int rc = grow_pinned(ctx, &ctx->stage, &ctx->stage_cap, at, who)) return rc;
   char* stage = static_cast<char*>(ctx->stage);
+  const int32_t* joined[4] = {ctx->join_body.data(), ctx->join_manifold.data(), ctx->join_pair.data(),
+                              ctx->join_awake.data()};
   for (int c = 0; c < NCOL; c++) {
     const Column& col = COLUMNS[c];
-    if (ctx->bytes[c] > 0) std::memcpy(stage + ctx->offset[c], field<const void*>(h, col.host), ctx->bytes[c]);
+    char* dst = stage + ctx->offset[c];
+    if (col.unit == WORLD1) {
+      const int t = c;  // the four offset tables are the first four columns
+      std::memcpy(dst, joined[t], ctx->bytes[c]);
+    } else {
+      for (int q = 0; q < nparts; q++) {
+        const size_t bytes = part_units(parts[q], col.unit) * col.per * col.elem;
+        if (bytes == 0) continue;
+        const void* from = field<const void*>(parts[q], col.host);
+        if (!from) return fail(ctx, BM_ERR_ARGUMENT, std::string(who) + ": " + col.name + " is null");
+        std::memcpy(dst, from, bytes);
+        dst += bytes;
+      }
+    }
     void* p = static_cast<char*>(ctx->arena) + ctx->offset[c];
     ctx->buffer[c] = p;
     std::memcpy(reinterpret_cast<char*>(&ctx->dev) + col.dev, &p, sizeof(void*));
   }
   HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->arena, ctx->stage, ctx->total, hipMemcpyHostToDevice, ctx->stream),
-              "bm_upload: hipMemcpyAsync");
-  ctx->moff_host.assign(h->manifold_offset, h->manifold_offset + h->n_worlds + 1);
-  ctx->boff_host.assign(h->body_offset, h->body_offset + h->n_worlds + 1);
+              (std::string(who) + ": hipMemcpyAsync").c_str());
+  ctx->moff_host = ctx->join_manifold;
+  ctx->boff_host = ctx->join_body;
   ctx->dev.n_worlds = ctx->n_worlds;
-  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_upload: hipStreamSynchronize");
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), (std::string(who) + ": hipStreamSynchronize").c_str());
   ctx->uploaded = true;
   return BM_OK;
+}
+
+// Every IO column back into the parts it came from.
+int download_parts(bm_context* ctx, int nparts, bm_columns* const* parts, const char* who) {
+  if (!ctx->uploaded) return fail(ctx, BM_ERR_ARGUMENT, std::string(who) + ": no batch uploaded");
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->device), (std::string(who) + ": hipSetDevice").c_str());
+  for (int q = 0; q < nparts; q++) {
+    if (!parts[q]) return fail(ctx, BM_ERR_ARGUMENT, std::string(who) + ": null argument");
+    for (int c = 0; c < NCOL; c++)
+      if (COLUMNS[c].dir == IO && part_units(parts[q], COLUMNS[c].unit) > 0 && !field<void*>(parts[q], COLUMNS[c].host))
+        return fail(ctx, BM_ERR_ARGUMENT, std::string(who) + ": " + COLUMNS[c].name + " is null");
+  }
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->stage, ctx->arena, ctx->total, hipMemcpyDeviceToHost, ctx->stream),
+              (std::string(who) + ": hipMemcpyAsync").c_str());
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), (std::string(who) + ": hipStreamSynchronize").c_str());
+  const char* stage = static_cast<const char*>(ctx->stage);
+  for (int c = 0; c < NCOL; c++) {
+    if (COLUMNS[c].dir != IO || ctx->bytes[c] == 0) continue;
+    const char* from = stage + ctx->offset[c];
+    for (int q = 0; q < nparts; q++) {
+      const size_t bytes = part_units(parts[q], COLUMNS[c].unit) * COLUMNS[c].per * COLUMNS[c].elem;
+      if (bytes == 0) continue;
+      std::memcpy(field<void*>(parts[q], COLUMNS[c].host), from, bytes);
+      from += bytes;
+    }
+  }
+  return BM_OK;
+}
+
+int bm_upload(bm_context* ctx, const bm_columns* h) {
+  if (!ctx || !h) return fail(ctx, BM_ERR_ARGUMENT, "bm_upload: null argument");
+  ctx->error.clear();
+  return upload_parts(ctx, 1, &h, "bm_upload");
+}
+
+int bm_upload_worlds(bm_context* ctx, int32_t n, const bm_columns* const* worlds) {
+  if (!ctx || !worlds) return fail(ctx, BM_ERR_ARGUMENT, "bm_upload_worlds: null argument");
+  ctx->error.clear();
+  return upload_parts(ctx, n, worlds, "bm_upload_worlds");
 }
 
 int bm_soft_step(bm_context* ctx) {
@@ -459,21 +548,15 @@ int bm_soft_step(bm_context* ctx) {
 
 int bm_download(bm_context* ctx, bm_columns* h) {
   if (!ctx || !h) return fail(ctx, BM_ERR_ARGUMENT, "bm_download: null argument");
-  if (!ctx->uploaded) return fail(ctx, BM_ERR_ARGUMENT, "bm_download: no batch uploaded");
   ctx->error.clear();
-  HIP_OR_FAIL(ctx, hipSetDevice(ctx->device), "bm_download: hipSetDevice");
-  for (int c = 0; c < NCOL; c++)
-    if (COLUMNS[c].dir == IO && ctx->bytes[c] > 0 && !field<void*>(h, COLUMNS[c].host))
-      return fail(ctx, BM_ERR_ARGUMENT, std::string("bm_download: ") + COLUMNS[c].name + " is null");
-  HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->stage, ctx->arena, ctx->total, hipMemcpyDeviceToHost, ctx->stream),
-              "bm_download: hipMemcpyAsync");
-  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_download: hipStreamSynchronize");
-  const char* stage = static_cast<const char*>(ctx->stage);
-  for (int c = 0; c < NCOL; c++) {
-    if (COLUMNS[c].dir != IO || ctx->bytes[c] == 0) continue;
-    std::memcpy(field<void*>(h, COLUMNS[c].host), stage + ctx->offset[c], ctx->bytes[c]);
-  }
-  return BM_OK;
+  return download_parts(ctx, 1, &h, "bm_download");
+}
+
+int bm_download_worlds(bm_context* ctx, int32_t n, bm_columns* const* worlds) {
+  if (!ctx || !worlds) return fail(ctx, BM_ERR_ARGUMENT, "bm_download_worlds: null argument");
+  ctx->error.clear();
+  if (n <= 0) return fail(ctx, BM_ERR_ARGUMENT, "bm_download_worlds: no worlds");
+  return download_parts(ctx, n, worlds, "bm_download_worlds");
 }
 
 int bm_colour(bm_context* ctx) {

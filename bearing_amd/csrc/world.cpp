@@ -10,10 +10,15 @@
 // in batches between the points where a result can wake an island), the colouring and
 // row layout, and the whole soft step (colour.hip, collide.hip, soft.hip). Every
 // arithmetic step is the reference's, so a world's transforms are the reference's bits.
+#include <atomic>
 #include <cmath>
+#include <condition_variable>
 #include <cstdint>
 #include <cstring>
+#include <functional>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -211,6 +216,7 @@ struct Visit {
              // 3: a level pair touched on the device, level batch row `slot`
   int slot;
   Manifold m;
+  bool again = false;  // prepared a second time, a body having woken since the pass began
 };
 
 // This step's manifolds in the order narrow() found them (src/contact.js Contacts).
@@ -312,6 +318,26 @@ struct bm_world {
   // Bodies woken while a narrow pass's results are taken (their bounds changed then).
   std::vector<uint8_t> fresh;
   std::vector<int32_t> woken;
+  // Where the step's narrow passes stand: the next visit to take, and the pass's first pair.
+  size_t visit_at = 0;
+  int narrow_from = 0;
+  // The device context: the world's own, or its batch's (shared, and then every device call
+  // made while the batch's host threads run holds device_lock).
+  bool owns_ctx = true;
+  std::mutex* device_lock = nullptr;
+  // In a batch the device holds every world's hulls and fields in one table: this world's
+  // hull h is the device's hull_base + h, its field f the device's field_base + f. The
+  // world's own tables as given, for the batch to join; `joined` false until it has.
+  int hull_base = 0, field_base = 0;
+  bool* tables_stale = nullptr;
+  std::vector<int32_t> given_hull_v, given_hull_f, given_hull_l, given_start, given_loops, given_columns, given_rows;
+  std::vector<double> given_vertices, given_planes, given_spacing, given_heights;
+  std::vector<int64_t> given_offsets;
+  // The phase's columns as bm_upload takes them (offset tables of this world alone).
+  int32_t h_body_offset[2], h_manifold_offset[2], h_pair_offset[2], h_awake_offset[2];
+  int32_t h_starts[BM_STARTS], h_solved, h_substeps;
+  double h_dt;
+  bm_columns h;
 };
 
 namespace {
@@ -798,7 +824,7 @@ void load(const bm_world* w, int s, double* r) {
   for (int k = 0; k < 3; k++) r[1 + k] = w->position[3 * s + k];
   for (int k = 0; k < 9; k++) r[4 + k] = w->axes[9 * s + k];
   if (w->shape[s] == HULL) {
-    r[13] = w->hull[s]; r[14] = 0; r[15] = 0;
+    r[13] = w->hull_base + w->hull[s]; r[14] = 0; r[15] = 0;
   } else if (w->shape[s] == BOX) {
     for (int k = 0; k < 3; k++) r[13 + k] = w->dims[3 * s + k] * 0.5;
   } else if (w->shape[s] == CAPSULE) {
@@ -895,7 +921,7 @@ void prepare_level(bm_world* w, Visit& v) {
   v.kind = 3;
   v.tolerance = tolerance;
   v.slot = (int)w->level_field.size();
-  w->level_field.push_back(w->field[g]);
+  w->level_field.push_back(w->field_base + w->field[g]);
   w->level_pos.insert(w->level_pos.end(), pos + g3, pos + g3 + 3);
   w->level_axes.insert(w->level_axes.end(), A + g9, A + g9 + 9);
   double r[32];
@@ -1018,7 +1044,11 @@ int finish_visit(bm_world* w, Visit& v) {
 // changes the bounds of the bodies it wakes, so a later pair with a body woken that way
 // is looked at again from where things stand then, as the reference, which visits pairs
 // one by one, would see it.
-int narrow(bm_world* w, int from) {
+//
+// The pass is split where it needs the device: narrow_begin() prepares every pair (records for
+// the device batch), narrow_resume() takes the results in order and stops, leaving a record
+// behind, where a pair must be looked at again; the caller runs the batch between the two.
+void narrow_begin(bm_world* w, int from) {
   Pairs& pr = w->pairs;
   std::vector<Visit>& visits = w->visits;
   visits.clear();
@@ -1033,22 +1063,30 @@ int narrow(bm_world* w, int from) {
     visits.push_back(v);
     p++;
   }
-  if (int rc = run_batch(w)) return rc;
+  // Prepared after every wake so far: nothing is fresh for this pass yet.
   for (int s : w->woken) w->fresh[s] = 0;
   w->woken.clear();
-  for (Visit& v : visits) {
-    if (w->fresh[v.a] || w->fresh[v.b]) {
+  w->visit_at = 0;
+}
+
+// Takes the pass's results in order; true when it stopped for the device (a pair prepared
+// again left a record in the batch), false when the pass is done.
+bool narrow_resume(bm_world* w, int& rc) {
+  std::vector<Visit>& visits = w->visits;
+  for (; w->visit_at < visits.size(); w->visit_at++) {
+    Visit& v = visits[w->visit_at];
+    if (!v.again && (w->fresh[v.a] || w->fresh[v.b])) {
+      v.again = true;
       prepare_visit(w, v);
-      if (v.kind >= 2) {
-        if (int rc = run_batch(w)) return rc;
-      }
+      if (v.kind >= 2) return true;
     }
-    if (int rc = finish_visit(w, v)) return rc;
+    if ((rc = finish_visit(w, v)) != BM_OK) return false;
   }
   for (int s : w->woken) w->fresh[s] = 0;
   w->woken.clear();
-  return BM_OK;
+  return false;
 }
+
 
 // src/world.js:1369-1382 (_syncPairs), with no bodies removed and no static body moved by hand.
 void sync_pairs(bm_world* w) {
@@ -1655,14 +1693,19 @@ double advance(bm_world* w, Core& cast, const Core& b, double scale, const doubl
         for (int j = 0; j < 3; j++) r[1 + j] = c.center[j];
         for (int j = 0; j < 9; j++) r[4 + j] = c.axes[j];
         for (int j = 0; j < 3; j++) r[13 + j] = c.half[j];
-        if (c.kind == CORE_HULL) r[13] = c.hull;
+        if (c.kind == CORE_HULL) r[13] = w->hull_base + c.hull;
         for (int j = 0; j < 3; j++) r[16 + j] = c.weight[j];
       }
       const double zero = 0;
       int32_t count = 0;
       double normal[3], points[12], seps[4];
       uint32_t ids[4];
-      rc = bm_collide_host(w->ctx, 1, ra_rec, rb_rec, &zero, &tolerance, &count, normal, points, seps, ids, nullptr);
+      {
+        // A batch's worlds share its device context: one call at a time.
+        std::unique_lock<std::mutex> held;
+        if (w->device_lock) held = std::unique_lock<std::mutex>(*w->device_lock);
+        rc = bm_collide_host(w->ctx, 1, ra_rec, rb_rec, &zero, &tolerance, &count, normal, points, seps, ids, nullptr);
+      }
       if (rc != BM_OK) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx)), -1;
       if (count > 0) {
         for (int k = 0; k < 3; k++) { hit.normal[k] = -normal[k]; hit.point[k] = points[k]; }
@@ -1917,21 +1960,27 @@ void store_memory(bm_world* w, int solved) {
   }
 }
 
-// The device phase: colour, rest, inertias .. store (src/world.js:956-1031), through the batch ABI.
-int device_phase(bm_world* w, double dt) {
+// The device phase's columns for this world alone (src/world.js:956-1031 reads and writes
+// them), into w->h.
+void fill_columns(bm_world* w, double dt) {
   Contacts& c = w->c;
   Pairs& pr = w->pairs;
   c.ensure(c.count > 0 ? c.count : 1);
   pr.grow(pr.count > 0 ? pr.count : 1);
-  const int32_t body_offset[2] = {0, w->top}, manifold_offset[2] = {0, c.count}, pair_offset[2] = {0, pr.count},
-                awake_offset[2] = {0, w->awake_count};
-  int32_t starts[BM_STARTS] = {0}, solved = 0, substeps = w->substeps;
-  double dts = dt;
-  bm_columns h{};
+  w->h_body_offset[0] = 0; w->h_body_offset[1] = w->top;
+  w->h_manifold_offset[0] = 0; w->h_manifold_offset[1] = c.count;
+  w->h_pair_offset[0] = 0; w->h_pair_offset[1] = pr.count;
+  w->h_awake_offset[0] = 0; w->h_awake_offset[1] = w->awake_count;
+  for (int k = 0; k < BM_STARTS; k++) w->h_starts[k] = 0;
+  w->h_solved = 0;
+  w->h_substeps = w->substeps;
+  w->h_dt = dt;
+  bm_columns& h = w->h;
+  h = bm_columns{};
   h.n_worlds = 1;
-  h.body_offset = body_offset; h.manifold_offset = manifold_offset; h.pair_offset = pair_offset;
-  h.awake_offset = awake_offset;
-  h.starts = starts; h.solved = &solved; h.substeps = &substeps; h.dt = &dts; h.gravity = w->gravity;
+  h.body_offset = w->h_body_offset; h.manifold_offset = w->h_manifold_offset; h.pair_offset = w->h_pair_offset;
+  h.awake_offset = w->h_awake_offset;
+  h.starts = w->h_starts; h.solved = &w->h_solved; h.substeps = &w->h_substeps; h.dt = &w->h_dt; h.gravity = w->gravity;
   h.position = w->position.data(); h.rotation = w->rotation.data(); h.velocity = w->velocity.data();
   h.angular = w->angular.data(); h.start = w->start.data(); h.start_q = w->start_q.data(); h.inv_i = w->inv_i.data();
   h.tumbled = w->tumbled.data(); h.ceiling = w->ceiling.data(); h.dims = w->dims.data(); h.inv_mass = w->inv_mass.data();
@@ -1956,11 +2005,301 @@ int device_phase(bm_world* w, double dt) {
   h.spin_mass = c.spin_mass.data(); h.pid = c.pid.data();
   h.pair_points = pr.points.data(); h.pair_pid = pr.pid.data(); h.pair_normal_impulse = pr.normal_impulse.data();
   h.pair_twist = pr.twist.data(); h.pair_friction = pr.friction.data(); h.pair_slip = pr.slip.data();
+}
+
+// The device phase: colour, rest, inertias .. store (src/world.js:956-1031), through the batch ABI.
+int device_phase(bm_world* w, double dt) {
+  fill_columns(w, dt);
+  bm_columns& h = w->h;
   if (int rc = bm_upload(w->ctx, &h)) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx));
   if (int rc = bm_colour(w->ctx)) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx));
   if (int rc = bm_soft_step(w->ctx)) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx));
   if (int rc = bm_download(w->ctx, &h)) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx));
-  store_memory(w, solved);
+  store_memory(w, w->h_solved);
+  return BM_OK;
+}
+
+// ---- a step in phases (src/world.js:880-1072), so that worlds of a batch can share the
+// device calls between them ----
+
+// From the step's start to its first narrow pass's records (src/world.js:880-939).
+int step_begin(bm_world* w, double dt) {
+  w->error.clear();
+  if (!(dt > 0) || dt == INFINITY) return fail(w, BM_ERR_ARGUMENT, "step: dt must be a positive number of seconds");
+  ++w->step_count;
+  if (w->wake_pending) sync_pairs(w);
+  w->slept_count = 0;
+  w->step_dt = dt;
+  const double gl = gravity_length(w);
+  for (int i = 0; i < w->awake_count; i++) bounds_of(w, w->awake_list[i], dt, gl);
+  broadphase_update(w, dt);
+  candidates(w);
+  w->c.count = 0;
+  w->narrow_from = 0;
+  if (w->wake_pending) sync_pairs(w);
+  narrow_begin(w, 0);
+  return BM_OK;
+}
+
+// After the device ran the batch: the pass goes on; when it ends and woke bodies, the next
+// pass starts over the pairs they woke (src/world.js:940-946). True while records wait.
+bool narrow_continue(bm_world* w, int& rc) {
+  rc = BM_OK;
+  if (narrow_resume(w, rc)) return true;
+  if (rc != BM_OK) return false;
+  w->narrow_from = w->pairs.active;
+  if (!w->wake_pending) return false;
+  sync_pairs(w);
+  narrow_begin(w, w->narrow_from);
+  return true;
+}
+
+// src/world.js:947-955: freezing, and the solver's step count.
+void step_middle(bm_world* w) {
+  if (w->sleeping && (w->frozen_count > 0 || w->still_ready)) freeze(w);
+  if (w->c.count > 0) w->solver_steps++;
+}
+
+// src/world.js:1032-1072: the catch, touched, transforms, sleep.
+int step_end(bm_world* w, double dt) {
+  const int touching = w->c.count;
+  w->strays_at = -1;
+  if (int rc = catch_fast(w)) return rc;
+  if (touching > 0) touched(w);
+  for (int i = 0; i < w->awake_count; i++) {
+    const int s = w->awake_list[i];
+    if (w->frozen[s] == 1) continue;
+    if (w->frozen[s] == 2) w->frozen[s] = 1;
+    write_transform(w, s);
+  }
+  w->step_dt = 0;
+  if (w->sleeping) sleep_step(w, dt);
+  return BM_OK;
+}
+
+// A small pool of host threads for a batch's per-world phases: run(n, f) calls f(0..n-1)
+// across the threads and the caller, and returns when every call has.
+class Pool {
+ public:
+  explicit Pool(int threads) {
+    for (int t = 0; t < threads; t++) workers_.emplace_back([this] { work(); });
+  }
+  ~Pool() {
+    {
+      std::lock_guard<std::mutex> g(m_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto& t : workers_) t.join();
+  }
+  void run(int n, const std::function<void(int)>& f) {
+    if (workers_.empty() || n <= 1) {
+      for (int i = 0; i < n; i++) f(i);
+      return;
+    }
+    {
+      std::lock_guard<std::mutex> g(m_);
+      job_ = &f;
+      n_ = n;
+      next_.store(0);
+      busy_ = (int)workers_.size();
+      ++generation_;
+    }
+    cv_.notify_all();
+    for (int i; (i = next_.fetch_add(1)) < n;) f(i);
+    std::unique_lock<std::mutex> g(m_);
+    done_.wait(g, [this] { return busy_ == 0; });
+    job_ = nullptr;
+  }
+
+ private:
+  void work() {
+    long seen = 0;
+    for (;;) {
+      const std::function<void(int)>* job;
+      int n;
+      {
+        std::unique_lock<std::mutex> g(m_);
+        cv_.wait(g, [&] { return stop_ || generation_ != seen; });
+        if (stop_) return;
+        seen = generation_;
+        job = job_;
+        n = n_;
+      }
+      for (int i; (i = next_.fetch_add(1)) < n;) (*job)(i);
+      {
+        std::lock_guard<std::mutex> g(m_);
+        if (--busy_ == 0) done_.notify_all();
+      }
+    }
+  }
+  std::vector<std::thread> workers_;
+  std::mutex m_;
+  std::condition_variable cv_, done_;
+  const std::function<void(int)>* job_ = nullptr;
+  int n_ = 0, busy_ = 0;
+  long generation_ = 0;
+  std::atomic<int> next_{0};
+  bool stop_ = false;
+};
+
+}  // namespace
+
+// Worlds stepped together: each world's order-defining host work on a thread of the pool,
+// every narrowphase pass's records of all worlds in one device launch, and one colouring and
+// soft-step launch for every world's contact phase.
+struct bm_batch {
+  bm_context* ctx = nullptr;
+  std::string error;
+  bool tables_stale = false;  // a world's hulls or fields changed since they were joined
+  std::vector<bm_world*> worlds;
+  std::mutex lock;
+  Pool* pool = nullptr;
+  std::vector<double> a, b, pair, normal, points, seps, shift, loss;
+  std::vector<int32_t> count, flags;
+  std::vector<uint32_t> ids;
+  std::vector<int32_t> level_field, level_manifolds, level_count;
+  std::vector<double> level_pos, level_axes, level_shape, level_box, level_margin, level_tol, level_normal,
+      level_points, level_seps;
+  std::vector<uint32_t> level_ids;
+  int level_cap = 8;
+};
+
+namespace {
+
+int batch_fail(bm_batch* B, int code, const std::string& m) {
+  B->error = m;
+  return code;
+}
+
+// Every world's waiting narrowphase records in one bm_meet_host and one bm_level_host call,
+// the results handed back to each world as its own run_batch() would have left them.
+int run_merged(bm_batch* B) {
+  auto& W = B->worlds;
+  const int nw = (int)W.size();
+  std::vector<int> at(nw + 1, 0);
+  for (int i = 0; i < nw; i++) at[i + 1] = at[i] + (int)W[i]->batch_pair.size() / 6;
+  const int n = at[nw];
+  if (n > 0) {
+    B->a.clear(); B->b.clear(); B->pair.clear();
+    for (bm_world* w : W) {
+      B->a.insert(B->a.end(), w->batch_a.begin(), w->batch_a.end());
+      B->b.insert(B->b.end(), w->batch_b.begin(), w->batch_b.end());
+      B->pair.insert(B->pair.end(), w->batch_pair.begin(), w->batch_pair.end());
+    }
+    B->count.resize(n); B->flags.resize(n); B->normal.resize(3 * (size_t)n); B->points.resize(12 * (size_t)n);
+    B->seps.resize(4 * (size_t)n); B->ids.resize(4 * (size_t)n); B->shift.resize(12 * (size_t)n); B->loss.resize(n);
+    if (int rc = bm_meet_host(B->ctx, n, B->a.data(), B->b.data(), B->pair.data(), B->count.data(), B->normal.data(),
+                              B->points.data(), B->seps.data(), B->ids.data(), B->shift.data(), B->flags.data(),
+                              B->loss.data()))
+      return batch_fail(B, rc, std::string("step: ") + bm_last_error(B->ctx));
+    for (int i = 0; i < nw; i++) {
+      bm_world* w = W[i];
+      const int k0 = at[i], k = at[i + 1] - at[i];
+      w->batch_a.clear(); w->batch_b.clear(); w->batch_pair.clear();
+      if (k == 0) continue;
+      w->batch_count.assign(B->count.begin() + k0, B->count.begin() + k0 + k);
+      w->batch_flags.assign(B->flags.begin() + k0, B->flags.begin() + k0 + k);
+      w->batch_normal.assign(B->normal.begin() + 3 * k0, B->normal.begin() + 3 * (k0 + k));
+      w->batch_points.assign(B->points.begin() + 12 * k0, B->points.begin() + 12 * (k0 + k));
+      w->batch_seps.assign(B->seps.begin() + 4 * k0, B->seps.begin() + 4 * (k0 + k));
+      w->batch_ids.assign(B->ids.begin() + 4 * k0, B->ids.begin() + 4 * (k0 + k));
+      w->batch_shift.assign(B->shift.begin() + 12 * k0, B->shift.begin() + 12 * (k0 + k));
+      w->batch_loss.assign(B->loss.begin() + k0, B->loss.begin() + k0 + k);
+    }
+  }
+  std::vector<int> lat(nw + 1, 0);
+  for (int i = 0; i < nw; i++) lat[i + 1] = lat[i] + (int)W[i]->level_field.size();
+  const int ln = lat[nw];
+  if (ln > 0) {
+    for (auto* v : {&B->level_pos, &B->level_axes, &B->level_shape, &B->level_box, &B->level_margin, &B->level_tol})
+      v->clear();
+    B->level_field.clear();
+    for (bm_world* w : W) {
+      B->level_field.insert(B->level_field.end(), w->level_field.begin(), w->level_field.end());
+      B->level_pos.insert(B->level_pos.end(), w->level_pos.begin(), w->level_pos.end());
+      B->level_axes.insert(B->level_axes.end(), w->level_axes.begin(), w->level_axes.end());
+      B->level_shape.insert(B->level_shape.end(), w->level_shape.begin(), w->level_shape.end());
+      B->level_box.insert(B->level_box.end(), w->level_box.begin(), w->level_box.end());
+      B->level_margin.insert(B->level_margin.end(), w->level_margin.begin(), w->level_margin.end());
+      B->level_tol.insert(B->level_tol.end(), w->level_tol.begin(), w->level_tol.end());
+    }
+    for (;;) {
+      const size_t slots = (size_t)ln * B->level_cap;
+      B->level_manifolds.resize(ln); B->level_count.resize(slots); B->level_normal.resize(3 * slots);
+      B->level_points.resize(12 * slots); B->level_seps.resize(4 * slots); B->level_ids.resize(4 * slots);
+      if (int rc = bm_level_host(B->ctx, ln, B->level_field.data(), B->level_pos.data(), B->level_axes.data(),
+                                 B->level_shape.data(), B->level_box.data(), B->level_margin.data(),
+                                 B->level_tol.data(), B->level_cap, B->level_manifolds.data(), B->level_count.data(),
+                                 B->level_normal.data(), B->level_points.data(), B->level_seps.data(),
+                                 B->level_ids.data()))
+        return batch_fail(B, rc, std::string("step: ") + bm_last_error(B->ctx));
+      int most = 0;
+      for (int k = 0; k < ln; k++) most = std::max(most, (int)B->level_manifolds[k]);
+      if (most <= B->level_cap) break;
+      B->level_cap = most;
+    }
+    const size_t cap = B->level_cap;
+    for (int i = 0; i < nw; i++) {
+      bm_world* w = W[i];
+      const size_t k0 = lat[i], k = lat[i + 1] - lat[i];
+      for (auto* v : {&w->level_pos, &w->level_axes, &w->level_shape, &w->level_box, &w->level_margin, &w->level_tol})
+        v->clear();
+      w->level_field.clear();
+      if (k == 0) continue;
+      w->level_cap = (int)cap;
+      w->level_manifolds.assign(B->level_manifolds.begin() + k0, B->level_manifolds.begin() + k0 + k);
+      w->level_count.assign(B->level_count.begin() + cap * k0, B->level_count.begin() + cap * (k0 + k));
+      w->level_normal.assign(B->level_normal.begin() + 3 * cap * k0, B->level_normal.begin() + 3 * cap * (k0 + k));
+      w->level_points.assign(B->level_points.begin() + 12 * cap * k0, B->level_points.begin() + 12 * cap * (k0 + k));
+      w->level_seps.assign(B->level_seps.begin() + 4 * cap * k0, B->level_seps.begin() + 4 * cap * (k0 + k));
+      w->level_ids.assign(B->level_ids.begin() + 4 * cap * k0, B->level_ids.begin() + 4 * cap * (k0 + k));
+    }
+  }
+  return BM_OK;
+}
+
+// Every world's hulls and fields joined into the device's tables, each world's numbers moved
+// up by the hulls and fields before it.
+int join_tables(bm_batch* B) {
+  std::vector<int32_t> hv{0}, hf{0}, hl{0}, st, lp, cols, rows;
+  std::vector<double> vt, pl, sp, heights;
+  std::vector<int64_t> off;
+  for (bm_world* w : B->worlds) {
+    w->hull_base = (int)hv.size() - 1;
+    const int nh = w->given_hull_v.empty() ? 0 : (int)w->given_hull_v.size() - 1;
+    for (int h = 0; h < nh; h++) {
+      hv.push_back(hv.back() + w->given_hull_v[h + 1] - w->given_hull_v[h]);
+      hf.push_back(hf.back() + w->given_hull_f[h + 1] - w->given_hull_f[h]);
+      hl.push_back(hl.back() + w->given_hull_l[h + 1] - w->given_hull_l[h]);
+    }
+    vt.insert(vt.end(), w->given_vertices.begin(), w->given_vertices.end());
+    pl.insert(pl.end(), w->given_planes.begin(), w->given_planes.end());
+    st.insert(st.end(), w->given_start.begin(), w->given_start.end());
+    lp.insert(lp.end(), w->given_loops.begin(), w->given_loops.end());
+    w->field_base = (int)cols.size();
+    for (size_t f = 0; f < w->given_columns.size(); f++) {
+      cols.push_back(w->given_columns[f]);
+      rows.push_back(w->given_rows[f]);
+      sp.push_back(w->given_spacing[f]);
+      off.push_back((int64_t)heights.size() + w->given_offsets[f]);
+    }
+    heights.insert(heights.end(), w->given_heights.begin(), w->given_heights.end());
+  }
+  const int nh = (int)hv.size() - 1;
+  if (int rc = bm_set_hulls(B->ctx, nh, hv.data(), hf.data(), hl.data(), vt.data(), st.data(), lp.data(), pl.data()))
+    return batch_fail(B, rc, std::string("step: ") + bm_last_error(B->ctx));
+  if (int rc = bm_set_fields(B->ctx, (int)cols.size(), cols.data(), rows.data(), sp.data(), off.data(), heights.data()))
+    return batch_fail(B, rc, std::string("step: ") + bm_last_error(B->ctx));
+  B->tables_stale = false;
+  return BM_OK;
+}
+
+// The first world whose phase failed, as the batch's error.
+int first_failure(bm_batch* B, const std::vector<int>& rcs) {
+  for (size_t i = 0; i < rcs.size(); i++)
+    if (rcs[i] != BM_OK)
+      return batch_fail(B, rcs[i], "world " + std::to_string(i) + ": " + B->worlds[i]->error);
   return BM_OK;
 }
 
@@ -1986,6 +2325,7 @@ int bm_world_create(bm_world** out, int32_t device, const double* gravity, int32
 
 int bm_world_destroy(bm_world* w) {
   if (!w) return BM_OK;
+  if (!w->owns_ctx) return fail(w, BM_ERR_ARGUMENT, "bm_world_destroy: the world belongs to a batch (bm_batch_destroy)");
   bm_destroy(w->ctx);
   delete w;
   return BM_OK;
@@ -2077,9 +2417,24 @@ int bm_world_set_hulls(bm_world* w, int32_t n, const int32_t* hull_v, const int3
   for (int s = 0; s < w->top; s++)
     if (w->shape[s] == HULL && w->hull[s] >= n)
       return fail(w, BM_ERR_ARGUMENT, "bm_world_set_hulls: a body already added names a hull the table lacks");
-  // The device's copy first: it checks the table's layout and limits.
-  if (int rc = bm_set_hulls(w->ctx, n, hull_v, hull_f, hull_l, vertices, start, loops, planes))
-    return fail(w, rc, bm_last_error(w->ctx));
+  // The device's copy first: it checks the table's layout and limits. A batch's worlds keep
+  // theirs, and the batch joins them before its next step.
+  {
+    std::unique_lock<std::mutex> held;
+    if (w->device_lock) held = std::unique_lock<std::mutex>(*w->device_lock);
+    if (int rc = bm_set_hulls(w->ctx, n, hull_v, hull_f, hull_l, vertices, start, loops, planes))
+      return fail(w, rc, bm_last_error(w->ctx));
+  }
+  if (w->tables_stale) {
+    *w->tables_stale = true;
+    w->given_hull_v.assign(hull_v, hull_v + n + 1);
+    w->given_hull_f.assign(hull_f, hull_f + n + 1);
+    w->given_hull_l.assign(hull_l, hull_l + n + 1);
+    w->given_vertices.assign(vertices, vertices + 3 * (size_t)(n ? hull_v[n] : 0));
+    w->given_planes.assign(planes, planes + 4 * (size_t)(n ? hull_f[n] : 0));
+    w->given_start.assign(start, start + (size_t)(n ? hull_f[n] : 0));
+    w->given_loops.assign(loops, loops + (size_t)(n ? hull_l[n] : 0));
+  }
   std::vector<bm_world::HostHull> hulls(n);
   std::vector<double> hv;
   for (int h = 0; h < n; h++) {
@@ -2114,9 +2469,26 @@ int bm_world_set_fields(bm_world* w, int32_t n, const int32_t* columns, const in
   for (int s = 0; s < w->top; s++)
     if (w->shape[s] == HEIGHTFIELD && w->field[s] >= n)
       return fail(w, BM_ERR_ARGUMENT, "bm_world_set_fields: a body already added names a field the table lacks");
-  // The device's copy first: it checks the table's layout.
-  if (int rc = bm_set_fields(w->ctx, n, columns, rows, spacing, offsets, heights))
-    return fail(w, rc, bm_last_error(w->ctx));
+  // The device's copy first: it checks the table's layout. A batch joins its worlds' tables.
+  {
+    std::unique_lock<std::mutex> held;
+    if (w->device_lock) held = std::unique_lock<std::mutex>(*w->device_lock);
+    if (int rc = bm_set_fields(w->ctx, n, columns, rows, spacing, offsets, heights))
+      return fail(w, rc, bm_last_error(w->ctx));
+  }
+  if (w->tables_stale) {
+    *w->tables_stale = true;
+    w->given_columns.assign(columns, columns + n);
+    w->given_rows.assign(rows, rows + n);
+    w->given_spacing.assign(spacing, spacing + n);
+    w->given_offsets.assign(offsets, offsets + n);
+    w->given_heights.clear();
+    for (int f = 0; f < n; f++) {
+      const double* h = heights + offsets[f];
+      w->given_offsets[f] = (int64_t)w->given_heights.size();
+      w->given_heights.insert(w->given_heights.end(), h, h + (size_t)columns[f] * rows[f]);
+    }
+  }
   std::vector<bm_world::HostField> fields(n);
   std::vector<double> all;
   for (int f = 0; f < n; f++) {
@@ -2143,39 +2515,102 @@ int bm_world_set_fields(bm_world* w, int32_t n, const int32_t* columns, const in
 // src/world.js:880-1072 (step), for the bodies above.
 int bm_world_step(bm_world* w, double dt) {
   if (!w) return BM_ERR_ARGUMENT;
-  w->error.clear();
-  if (!(dt > 0) || dt == INFINITY) return fail(w, BM_ERR_ARGUMENT, "step: dt must be a positive number of seconds");
-  ++w->step_count;
-  if (w->wake_pending) sync_pairs(w);
-  w->slept_count = 0;
-  w->step_dt = dt;
-  const double gl = gravity_length(w);
-  for (int i = 0; i < w->awake_count; i++) bounds_of(w, w->awake_list[i], dt, gl);
-  broadphase_update(w, dt);
-  candidates(w);
-  w->c.count = 0;
-  int from = 0;
+  if (int rc = step_begin(w, dt)) return rc;
+  int rc = BM_OK;
   do {
-    if (w->wake_pending) sync_pairs(w);
-    if (int rc = narrow(w, from)) return rc;
-    from = w->pairs.active;
-  } while (w->wake_pending);
-  const int touching = w->c.count;
-  if (w->sleeping && (w->frozen_count > 0 || w->still_ready)) freeze(w);
-  if (touching > 0) w->solver_steps++;
-  if (int rc = device_phase(w, dt)) return rc;
-  w->strays_at = -1;
-  if (int rc = catch_fast(w)) return rc;
-  if (touching > 0) touched(w);
-  for (int i = 0; i < w->awake_count; i++) {
-    const int s = w->awake_list[i];
-    if (w->frozen[s] == 1) continue;
-    if (w->frozen[s] == 2) w->frozen[s] = 1;
-    write_transform(w, s);
+    if ((rc = run_batch(w)) != BM_OK) return rc;
+  } while (narrow_continue(w, rc));
+  if (rc != BM_OK) return rc;
+  step_middle(w);
+  if ((rc = device_phase(w, dt)) != BM_OK) return rc;
+  return step_end(w, dt);
+}
+
+int bm_batch_create(bm_batch** out, int32_t device, int32_t threads) {
+  if (!out) return BM_ERR_ARGUMENT;
+  *out = nullptr;
+  if (threads < 0) return BM_ERR_ARGUMENT;
+  bm_batch* B = new bm_batch();
+  if (int rc = bm_create(&B->ctx, device)) {
+    delete B;
+    return rc;
   }
-  w->step_dt = 0;
-  if (w->sleeping) sleep_step(w, dt);
+  B->pool = new Pool(threads > 1 ? threads - 1 : 0);
+  *out = B;
   return BM_OK;
+}
+
+int bm_batch_destroy(bm_batch* B) {
+  if (!B) return BM_OK;
+  delete B->pool;
+  for (bm_world* w : B->worlds) delete w;
+  bm_destroy(B->ctx);
+  delete B;
+  return BM_OK;
+}
+
+const char* bm_batch_error(const bm_batch* B) { return B ? B->error.c_str() : "bm_batch: no batch"; }
+
+int bm_batch_world(bm_batch* B, bm_world** out, const double* gravity, int32_t substeps, int32_t sleeping) {
+  if (!B || !out) return BM_ERR_ARGUMENT;
+  B->error.clear();
+  bm_world* w = new bm_world();
+  w->ctx = B->ctx;
+  w->owns_ctx = false;
+  w->device_lock = &B->lock;
+  w->tables_stale = &B->tables_stale;
+  if (gravity)
+    for (int k = 0; k < 3; k++) w->gravity[k] = gravity[k];
+  w->substeps = substeps > 0 ? substeps : 8;
+  w->sleeping = sleeping != 0;
+  B->worlds.push_back(w);
+  *out = w;
+  return BM_OK;
+}
+
+int bm_batch_step(bm_batch* B, double dt) {
+  if (!B) return BM_ERR_ARGUMENT;
+  B->error.clear();
+  auto& W = B->worlds;
+  const int n = (int)W.size();
+  if (n == 0) return BM_OK;
+  if (B->tables_stale) {
+    if (int rc = join_tables(B)) return rc;
+  }
+  std::vector<int> rcs(n, BM_OK);
+  B->pool->run(n, [&](int i) { rcs[i] = step_begin(W[i], dt); });
+  if (int rc = first_failure(B, rcs)) return rc;
+  std::vector<uint8_t> active(n, 1);
+  for (;;) {
+    if (int rc = run_merged(B)) return rc;
+    B->pool->run(n, [&](int i) {
+      if (!active[i]) return;
+      int rc = BM_OK;
+      active[i] = narrow_continue(W[i], rc) ? 1 : 0;
+      rcs[i] = rc;
+    });
+    if (int rc = first_failure(B, rcs)) return rc;
+    bool any = false;
+    for (int i = 0; i < n; i++) any = any || active[i];
+    if (!any) break;
+  }
+  std::vector<bm_columns*> hs(n);
+  B->pool->run(n, [&](int i) {
+    step_middle(W[i]);
+    fill_columns(W[i], dt);
+    hs[i] = &W[i]->h;
+  });
+  std::vector<const bm_columns*> ins(hs.begin(), hs.end());
+  if (int rc = bm_upload_worlds(B->ctx, n, ins.data())) return batch_fail(B, rc, std::string("step: ") + bm_last_error(B->ctx));
+  if (int rc = bm_colour(B->ctx)) return batch_fail(B, rc, std::string("step: ") + bm_last_error(B->ctx));
+  if (int rc = bm_soft_step(B->ctx)) return batch_fail(B, rc, std::string("step: ") + bm_last_error(B->ctx));
+  if (int rc = bm_download_worlds(B->ctx, n, hs.data()))
+    return batch_fail(B, rc, std::string("step: ") + bm_last_error(B->ctx));
+  B->pool->run(n, [&](int i) {
+    store_memory(W[i], W[i]->h_solved);
+    rcs[i] = step_end(W[i], dt);
+  });
+  return first_failure(B, rcs);
 }
 
 int bm_world_read(const bm_world* w, bm_world_state* out) {

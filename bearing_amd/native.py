@@ -60,7 +60,8 @@ EXPORTS = ["bm_create", "bm_destroy", "bm_last_error", "bm_upload", "bm_soft_ste
            "bm_soft_step_host", "bm_last_kernel_ms", "bm_set_launch", "bm_lds_limit", "bm_colour", "bm_collide_host", "bm_meet_host", "bm_set_colour_lds", "bm_set_hulls",
            "bm_set_fields", "bm_level_host",
            "bm_world_create", "bm_world_destroy", "bm_world_error", "bm_world_add", "bm_world_step", "bm_world_read",
-           "bm_world_set_hulls", "bm_world_set_fields"]
+           "bm_world_set_hulls", "bm_world_set_fields", "bm_upload_worlds", "bm_download_worlds",
+           "bm_batch_create", "bm_batch_destroy", "bm_batch_error", "bm_batch_world", "bm_batch_step"]
 
 _lib = None
 
@@ -127,6 +128,12 @@ def world_signatures(lib):
     lib.bm_world_read.argtypes = [vp, ctypes.POINTER(WorldState)]
     lib.bm_world_set_hulls.argtypes = [vp, ctypes.c_int32, _I32, _I32, _I32, _D, _I32, _I32, _D]
     lib.bm_world_set_fields.argtypes = [vp, ctypes.c_int32, _I32, _I32, _D, ctypes.POINTER(ctypes.c_int64), _D]
+    lib.bm_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int32, ctypes.c_int32]
+    lib.bm_batch_destroy.argtypes = [vp]
+    lib.bm_batch_error.argtypes = [vp]
+    lib.bm_batch_error.restype = ctypes.c_char_p
+    lib.bm_batch_world.argtypes = [vp, ctypes.POINTER(vp), _D, ctypes.c_int32, ctypes.c_int32]
+    lib.bm_batch_step.argtypes = [vp, ctypes.c_double]
 
 
 class BearingError(RuntimeError):
@@ -336,14 +343,21 @@ class World:
 
     `lib` is the library to bind, the built one by default."""
 
-    def __init__(self, device=0, gravity=(0.0, -9.81, 0.0), substeps=8, sleeping=True, lib=None):
-        if lib is None:
+    def __init__(self, device=0, gravity=(0.0, -9.81, 0.0), substeps=8, sleeping=True, lib=None, batch=None):
+        if batch is not None:
+            lib = batch.lib
+        elif lib is None:
             lib = load()
         else:
             world_signatures(lib)
         self.lib = lib
         self.ptr = ctypes.c_void_p()
+        self.batch = batch
         g = np.ascontiguousarray(gravity, np.float64)
+        if batch is not None:
+            batch.check(lib.bm_batch_world(batch.ptr, ctypes.byref(self.ptr), g.ctypes.data_as(_D), substeps,
+                                           1 if sleeping else 0), "bm_batch_world")
+            return
         rc = lib.bm_world_create(ctypes.byref(self.ptr), device, g.ctypes.data_as(_D), substeps, 1 if sleeping else 0)
         if rc != 0:
             raise BearingError(f"bm_world_create: no HIP device {device} (status {rc})")
@@ -413,8 +427,50 @@ class World:
         return out
 
     def close(self):
-        if self.ptr:
+        if self.ptr and self.batch is None:
             self.lib.bm_world_destroy(self.ptr)
+        self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class WorldBatch:
+    """Worlds stepped together (bm_batch_*): each as World steps it, the device shared by
+    all of them. world() makes a World of the batch; step(dt) steps every one."""
+
+    def __init__(self, device=0, threads=0, lib=None):
+        if lib is None:
+            lib = load()
+        else:
+            world_signatures(lib)
+        self.lib = lib
+        self.ptr = ctypes.c_void_p()
+        self.worlds = []
+        rc = lib.bm_batch_create(ctypes.byref(self.ptr), device, threads)
+        if rc != 0:
+            raise BearingError(f"bm_batch_create: no HIP device {device} (status {rc})")
+
+    def check(self, rc, what):
+        if rc != 0:
+            raise BearingError(f"{what}: status {rc}: {self.lib.bm_batch_error(self.ptr).decode()}")
+
+    def world(self, gravity=(0.0, -9.81, 0.0), substeps=8, sleeping=True):
+        w = World(gravity=gravity, substeps=substeps, sleeping=sleeping, batch=self)
+        self.worlds.append(w)
+        return w
+
+    def step(self, dt):
+        self.check(self.lib.bm_batch_step(self.ptr, dt), "bm_batch_step")
+
+    def close(self):
+        if self.ptr:
+            for w in self.worlds:
+                w.ptr = ctypes.c_void_p()
+            self.lib.bm_batch_destroy(self.ptr)
             self.ptr = ctypes.c_void_p()
 
     def __del__(self):

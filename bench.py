@@ -86,7 +86,56 @@ def cpu_baseline(fx, seconds):
             "sample": f"{done} soft steps of {WORKLOAD} ({rows} rows) in {elapsed:.2f} s on one host thread"}
 
 
-WORLD_RUNS = ["pyramid", "rain", "darts"]
+WORLD_RUNS = ["pyramid", "rain", "darts", "terrain"]
+# Whole worlds stepped together: the reference's own resolved bodies of these trajectories
+# (13 to 401 bodies: towers, sphere piles, a pyramid, rods and balls, capsules, hulls, a
+# heightfield), repeated to the batch size.
+BATCH_WORLDS_FROM = ["tower", "pile_small", "pyramid", "rain_small", "capsules", "darts", "rocks", "terrain"]
+
+
+def batched_world_run(count, warm, steps, threads):
+    """`count` whole worlds stepped together (bm_batch_step) against the same worlds stepped one
+    at a time through bm_world_step (a sample of them): world-steps per second both ways, and
+    whether every batched world's transforms after warm + steps steps are its reference's bits."""
+    import hashlib
+
+    import fixtures
+    from bearing_amd import native
+    from worldcheck import fill_world
+
+    fxs = [fixtures.load(os.path.join(fixtures.DATA, f"{n}.traj.bmfx.gz")) for n in BATCH_WORLDS_FROM]
+    dt = fxs[0].meta["dt"]
+    batch = native.WorldBatch(0, threads)
+    worlds = [fill_world(batch.world(), fxs[i % len(fxs)]) for i in range(count)]
+    bodies = sum(int(fxs[i % len(fxs)].meta["bodies"]) for i in range(count))
+    for _ in range(warm):
+        batch.step(dt)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.step(dt)
+    wall = time.perf_counter() - t0
+    at = warm + steps
+    exact = all(hashlib.sha256(w.read()["transforms"].tobytes()).hexdigest() == fxs[i % len(fxs)].meta["hashes"][at - 1]
+                for i, w in enumerate(worlds))
+    contacts = sum(w.read()["contacts"] for w in worlds)
+    batch.close()
+    sample = min(64, count)
+    singles = [fill_world(native.World(0), fxs[i % len(fxs)]) for i in range(sample)]
+    t_loop = 0.0
+    for w in singles:
+        for _ in range(warm):
+            w.step(dt)
+        t = time.perf_counter()
+        for _ in range(steps):
+            w.step(dt)
+        t_loop += time.perf_counter() - t
+        w.close()
+    loop_rate = sample * steps / t_loop
+    rate = count * steps / wall
+    return {"worlds": count, "bodies": bodies, "contacts_at_end": int(contacts), "warm_steps": warm, "steps": steps,
+            "host_threads": threads, "ms_per_step": wall * 1000 / steps, "world_steps_per_s": rate,
+            "looped_plain_world_steps_per_s": loop_rate, "speedup_vs_looped": rate / loop_rate,
+            "bit_exact": bool(exact), "worlds_from": BATCH_WORLDS_FROM}
 
 
 def world_run(name, lib=None):
@@ -97,12 +146,11 @@ def world_run(name, lib=None):
 
     import fixtures
     from bearing_amd import native
-    from worldcheck import init_columns
+    from worldcheck import fill_world
 
     fx = fixtures.load(os.path.join(fixtures.DATA, f"{name}.traj.bmfx.gz"))
     meta = fx.meta
-    w = native.World(0, lib=lib)
-    w.add(init_columns(fx))
+    w = fill_world(native.World(0, lib=lib), fx)
     out = {"transforms": np.zeros(7 * int(meta["bodies"]), np.float32)}
     from bearing_amd.native import WorldState
     import ctypes
@@ -130,6 +178,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--batch-worlds", type=int, default=4096)
     ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--batch-threads", type=int, default=16, help="host threads for the whole-world batch")
+    ap.add_argument("--whole-batch", type=int, default=None,
+                    help="only step this many whole worlds together (bm_batch_step) and print the timing")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--plain-launch", default=None,
                     help="shape,resident for the plain world (default: autotuned, bearing_amd/tune.py)")
@@ -155,6 +206,9 @@ def main():
 
     if args.world:
         print(json.dumps({"world": args.world, **world_run(args.world)}))
+        return
+    if args.whole_batch:
+        print(json.dumps({"whole_batch": batched_world_run(args.whole_batch, 60, 20, args.batch_threads)}))
         return
 
     from bearing_amd import native
@@ -220,6 +274,7 @@ def main():
     loop_rate = sample / t_loop
     ctx.close()
 
+    whole = batched_world_run(args.batch_worlds, 60, 20, args.batch_threads)
     worlds_run = {name: world_run(name) for name in WORLD_RUNS}
     # The same whole steps on the host with the oracle standing in for every device call
     # (tests/hostworld): the port's one-thread time for the same trajectories.
@@ -255,6 +310,7 @@ def main():
         "batched": {"worlds": args.batch_worlds, "steps": args.batch_steps, "world_steps_per_s": batched_rate,
                     "looped_plain_world_steps_per_s": loop_rate, "speedup_vs_looped": batched_rate / loop_rate,
                     "worlds_from": BATCH_SOURCES, "launch": {"shape": bshape, "resident_lds": bool(bresident)}},
+        "batched_whole_worlds": whole,
         "world": worlds_run,
         "bwd_fwd": None,
     }

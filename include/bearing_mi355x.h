@@ -145,6 +145,14 @@ int bm_soft_step(bm_context* ctx);
 /* Copies every [out]/[io] column of the uploaded batch back into `host`. */
 int bm_download(bm_context* ctx, bm_columns* host);
 
+/* bm_upload and bm_download for worlds whose columns lie apart: worlds[i] describes one or
+ * more worlds with its own offset tables, as bm_upload takes them; the batch is every part's
+ * worlds in turn, and bm_download_worlds writes each part's columns back into its own arrays.
+ * No bm_upload counterpart in the reference: it lets separately stored worlds share one launch
+ * (bm_batch_step). */
+int bm_upload_worlds(bm_context* ctx, int32_t n, const bm_columns* const* worlds);
+int bm_download_worlds(bm_context* ctx, int32_t n, bm_columns* const* worlds);
+
 /* bm_upload + bm_soft_step + bm_download: what a binding of world.step's
  * contact phase calls with the world's own arrays. */
 int bm_soft_step_host(bm_context* ctx, bm_columns* host);
@@ -305,6 +313,25 @@ int bm_world_set_fields(bm_world* world, int32_t n, const int32_t* columns, cons
 /* world.step(dt) (src/world.js:880). */
 int bm_world_step(bm_world* world, double dt);
 int bm_world_read(const bm_world* world, bm_world_state* out);
+
+/* ---- worlds stepped together (the batched form) ----
+ *
+ * A batch steps its worlds as bm_world_step steps each one, with the same results bit for
+ * bit, sharing the device between them: each world's order-defining host work (trees, pair
+ * list, narrow order, freezing, catch, sleep) runs on one of `threads` host threads (0 or 1:
+ * the calling thread alone), every narrowphase pass of every world goes to the device in one
+ * launch, and every world's colouring and soft step in one launch each. No counterpart in the
+ * reference, which steps one world at a time. */
+typedef struct bm_batch bm_batch;
+int bm_batch_create(bm_batch** out, int32_t device, int32_t threads);
+/* Destroys the batch and every world it made. */
+int bm_batch_destroy(bm_batch* batch);
+const char* bm_batch_error(const bm_batch* batch);
+/* A new world of the batch, as bm_world_create makes one (bodies, hulls and fields are added
+ * to it as to any world); it lives until bm_batch_destroy. */
+int bm_batch_world(bm_batch* batch, bm_world** out, const double* gravity, int32_t substeps, int32_t sleeping);
+/* Every world of the batch, one step of dt. */
+int bm_batch_step(bm_batch* batch, double dt);
 
 #ifdef __cplusplus
 }

@@ -26,7 +26,7 @@ extern "C" void oracle_hulls(int n, const int32_t* hull_v, const int32_t* hull_f
                              const double* vertices, const int32_t* start, const int32_t* loops, const double* planes);
 
 struct bm_context {
-  bm_columns h;
+  std::vector<bm_columns> hs;
   std::string error;
 };
 
@@ -55,13 +55,25 @@ int bm_destroy(bm_context* ctx) {
 const char* bm_last_error(const bm_context* ctx) { return ctx->error.c_str(); }
 int bm_upload(bm_context* ctx, const bm_columns* host) {
   if (host->n_worlds != 1) return BM_ERR_ARGUMENT;
-  ctx->h = *host;
+  ctx->hs.assign(1, *host);
   return BM_OK;
 }
 int bm_download(bm_context*, bm_columns*) { return BM_OK; }
+// The worlds' own arrays are worked on in place, so nothing moves either way.
+int bm_upload_worlds(bm_context* ctx, int32_t n, const bm_columns* const* worlds) {
+  ctx->hs.clear();
+  for (int i = 0; i < n; i++) {
+    if (worlds[i]->n_worlds != 1) return BM_ERR_ARGUMENT;
+    ctx->hs.push_back(*worlds[i]);
+  }
+  return BM_OK;
+}
+int bm_download_worlds(bm_context*, int32_t, bm_columns* const*) { return BM_OK; }
 
-int bm_colour(bm_context* ctx) {
-  bm_columns& h = ctx->h;
+}  // extern "C"
+
+namespace {
+int colour_one(bm_columns& h) {
   const int n = h.manifold_offset[1], top = h.body_offset[1];
   int frozen = 0;
   for (int s = 0; s < top; s++) frozen += h.frozen[s] != 0;
@@ -89,9 +101,17 @@ int bm_colour(bm_context* ctx) {
   }
   return BM_OK;
 }
+}  // namespace
 
-int bm_soft_step(bm_context* ctx) {
-  bm_columns& h = ctx->h;
+extern "C" {
+int bm_colour(bm_context* ctx) {
+  for (bm_columns& h : ctx->hs) colour_one(h);
+  return BM_OK;
+}
+}  // extern "C"
+
+namespace {
+int soft_one(bm_context* ctx, bm_columns& h) {
   OracleSoft o{};
   o.top = h.body_offset[1];
   o.awake_count = h.awake_offset[1];
@@ -127,6 +147,14 @@ int bm_soft_step(bm_context* ctx) {
     ctx->error = "oracle soft step: a manifold it does not cover";
     return BM_ERR_UNSUPPORTED;
   }
+  return BM_OK;
+}
+}  // namespace
+
+extern "C" {
+int bm_soft_step(bm_context* ctx) {
+  for (bm_columns& h : ctx->hs)
+    if (int rc = soft_one(ctx, h)) return rc;
   return BM_OK;
 }
 

@@ -30,3 +30,20 @@ def test_gpu_world_matches_reference_trajectory(name):
 
 def test_gpu_world_refuses_what_it_does_not_handle():
     refuses(lambda: native.World(0))
+
+
+def test_gpu_batch_steps_every_world_as_alone():
+    # Worlds of every kind stepped together on the device (bm_batch_step: one narrowphase
+    # launch a pass, one colouring and one soft-step launch for all of them), each matching its
+    # own reference trajectory on every step.
+    from worldcheck import fill_world, run_together
+    names = ["tower", "pile_small", "darts", "capsules", "rocks", "terrain", "tower", "rain_small", "pyramid"]
+    print("batch of", names, flush=True)
+    fxs = [traj(n) for n in names]
+    batch = native.WorldBatch(0, threads=8)
+    worlds = [fill_world(batch.world(), fx) for fx in fxs]
+    steps = min(f.meta["steps"] for f in fxs)
+    matched, why = run_together(batch, worlds, fxs, steps)
+    batch.close()
+    assert why is None, why
+    assert matched == steps

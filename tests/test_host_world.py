@@ -41,3 +41,18 @@ def test_host_world_matches_reference_trajectory(lib, name):
 
 def test_host_world_refuses_what_it_does_not_handle(lib):
     refuses(lambda: native.World(lib=lib))
+
+
+def test_host_batch_steps_every_world_as_alone(lib):
+    # Worlds of every kind stepped together (bm_batch_step), each matching its own reference
+    # trajectory on every step; two copies of one world stay the same as each other.
+    from worldcheck import fill_world, run_together
+    names = ["tower", "pile_small", "darts", "capsules", "rocks", "terrain", "tower", "rain_small"]
+    fxs = [traj(n) for n in names]
+    batch = native.WorldBatch(threads=4, lib=lib)
+    worlds = [fill_world(batch.world(), fx) for fx in fxs]
+    steps = min(f.meta["steps"] for f in fxs)
+    matched, why = run_together(batch, worlds, fxs, steps)
+    assert why is None, why
+    assert matched == steps
+    batch.close()

@@ -93,3 +93,37 @@ def refuses(new_world):
         w.add(cols)
     assert w.read()["top"] == 0
     w.close()
+
+
+def run_together(batch, worlds, fxs, steps):
+    """Steps a native.WorldBatch `steps` times; after each step every world's transforms hash
+    and counts must be its own fixture's. Returns (steps matched, the first mismatch or None)."""
+    dt = fxs[0].meta["dt"]
+    assert all(f.meta["dt"] == dt for f in fxs)
+    for s in range(1, steps + 1):
+        batch.step(dt)
+        for i, (w, fx) in enumerate(zip(worlds, fxs)):
+            r = w.read()
+            got = hashlib.sha256(r["transforms"].tobytes()).hexdigest()
+            counts = (r["awake"], r["contacts"], r["frozen"])
+            want = (int(fx["awake"][s - 1]), int(fx["contacts"][s - 1]), int(fx["frozen"][s - 1]))
+            if got != fx.meta["hashes"][s - 1] or counts != want:
+                return s - 1, f"world {i}, step {s}: transforms {'match' if got == fx.meta['hashes'][s - 1] else 'differ'}, " \
+                              f"awake/contacts/frozen {counts}, reference {want}"
+    return steps, None
+
+
+def fill_world(w, fx):
+    """Hulls, fields and bodies of a trajectory fixture into an empty native.World."""
+    table = hull_table(fx)
+    if table is not None:
+        w.set_hulls(table)
+    if "fields.columns" in fx:
+        cols, rows = fx["fields.columns"], fx["fields.rows"]
+        heights, at = [], 0
+        for c, r in zip(cols, rows):
+            heights.append(fx["fields.heights"][at:at + int(c) * int(r)])
+            at += int(c) * int(r)
+        w.set_fields(cols, rows, fx["fields.spacing"], heights)
+    w.add(init_columns(fx))
+    return w
