@@ -133,6 +133,9 @@ struct bm_context {
   // The heightfields (bm_set_fields): info and offsets, spacing and corner, heights; and a
   // host copy of each one's shape, for sizing touchLevel's scratch.
   Scratch field_ints, field_doubles, level_scratch;
+  // A single world of at least wide_min manifolds takes the grid-wide soft step (0: never);
+  // last_wide, the workgroups the last step used (0: one a world).
+  int32_t wide_min = 16384, last_wide = 0;
   bm::FieldTable fields{};
   struct FieldShape {
     int columns, rows;
@@ -537,8 +540,16 @@ int bm_soft_step(bm_context* ctx) {
   // Velocities resident in LDS when the largest world's fit: 48 bytes a body.
   const size_t lds = (size_t)ctx->most_bodies * 6 * sizeof(double);
   (void)hipGetLastError();  // a stale error of an earlier call is not this launch's
-  bm::launch_soft_step(ctx->dev, ctx->n_worlds, ctx->shape, ctx->resident && lds <= ctx->lds_limit ? lds : 0,
-                       ctx->stream);
+  const int work = std::max(ctx->manifolds, ctx->awake);
+  if (ctx->n_worlds == 1 && ctx->wide_min > 0 && ctx->manifolds >= ctx->wide_min) {
+    // One world too large to sweep in one workgroup: the whole grid, cooperatively.
+    ctx->last_wide = bm::launch_soft_wide(ctx->dev, 0, work, ctx->device, ctx->stream);
+    if (ctx->last_wide < 0) return fail(ctx, BM_ERR_DEVICE, "bm_soft_step: the grid-wide launch failed");
+  } else {
+    ctx->last_wide = 0;
+    bm::launch_soft_step(ctx->dev, ctx->n_worlds, ctx->shape, ctx->resident && lds <= ctx->lds_limit ? lds : 0,
+                         ctx->stream);
+  }
   HIP_OR_FAIL(ctx, hipGetLastError(), "bm_soft_step: launch");
   HIP_OR_FAIL(ctx, hipEventRecord(ctx->end, ctx->stream), "bm_soft_step: hipEventRecord");
   HIP_OR_FAIL(ctx, hipEventSynchronize(ctx->end), "bm_soft_step: kernel");
@@ -1015,6 +1026,14 @@ int bm_level_host(bm_context* ctx, int32_t n, const int32_t* field, const double
     }
   }
   HIP_OR_FAIL(ctx, hipEventElapsedTime(&ctx->last_ms, ctx->begin, ctx->end), "bm_level_host: hipEventElapsedTime");
+  return BM_OK;
+}
+
+int bm_set_wide(bm_context* ctx, int32_t min_manifolds) {
+  if (!ctx) return BM_ERR_ARGUMENT;
+  ctx->error.clear();
+  if (min_manifolds < 0) return fail(ctx, BM_ERR_ARGUMENT, "bm_set_wide: min_manifolds must be 0 or more");
+  ctx->wide_min = min_manifolds;
   return BM_OK;
 }
 

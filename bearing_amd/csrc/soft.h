@@ -37,6 +37,9 @@ struct SoftDev {
 // 2 = 512 threads, two waves.
 constexpr int SOFT_SHAPES = 3;
 void launch_soft_step(const SoftDev& d, int grid, int shape, size_t lds_bytes, hipStream_t stream);
+// World wld's soft step across a cooperative grid sized for `work` (its manifolds or bodies,
+// whichever is more); the workgroups launched, or -1 if the launch failed.
+int launch_soft_wide(const SoftDev& d, int wld, int work, int device, hipStream_t stream);
 size_t soft_step_lds_limit(int device);
 
 }  // namespace bm

@@ -16,6 +16,7 @@
 // Arithmetic is the reference's, operation for operation, in doubles, built with
 // -ffp-contract=off: the results are the reference's bits. Each function cites
 // the lines it follows.
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -1016,6 +1017,114 @@ size_t soft_step_lds_limit(int device) {
   if (hipDeviceGetAttribute(&most, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) return 0;
   const size_t own = (BM_STARTS_DEV + BOUNCE_CHUNK + 8) * sizeof(int) + 64;
   return (size_t)most > own ? (size_t)most - own : 0;
+}
+
+// One sweep over the world's colours across the whole grid: each colour's manifolds spread
+// over every thread of every workgroup, a grid barrier after each; the overflow in order by
+// one thread.
+template <int KIND>
+__device__ __forceinline__ void sweep_wide(cooperative_groups::grid_group& grid, const SoftDev& d, const VW& vw,
+                                           const Soft& soft, double h, bool known, int moff, int boff,
+                                           const int* starts, int tid, int nth) {
+  for (int c = 0; c < COLOURS; c++) {
+    const int from = starts[c], to = starts[c + 1];
+    if (from == to) continue;
+    for (int lm = from + tid; lm < to; lm += nth) row_of<KIND>(d, vw, soft, h, known, moff + lm, boff);
+    grid.sync();
+  }
+  const int from = starts[OVERFLOW], to = starts[OVERFLOW + 1];
+  if (from < to) {
+    if (tid == 0) {
+      for (int lm = from; lm < to; lm++) row_of<KIND>(d, vw, soft, h, known, moff + lm, boff);
+    }
+    grid.sync();
+  }
+}
+
+// World `wld`'s soft step across the whole grid, for a world too large for one workgroup to
+// sweep quickly (tens of thousands of manifolds): the phases of soft_step_kernel, each one's
+// bodies or manifolds spread over every thread, a grid barrier where that kernel has a
+// workgroup barrier (a cooperative launch: every workgroup is resident, so the barrier is
+// reached by all). Manifolds of one colour share no moving body, so the order threads take
+// them in changes no result; the overflow and the bounces stay serial in the reference's
+// order. Velocities stay in HBM.
+template <int THREADS>
+__global__ void __launch_bounds__(THREADS, 1) soft_wide_kernel(SoftDev d, int wld) {
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  __shared__ int starts[BM_STARTS_DEV];
+  __shared__ int bounce_list[BOUNCE_CHUNK];
+  __shared__ int bounce_waves[THREADS / 64];
+  const int tid = blockIdx.x * THREADS + threadIdx.x, nth = gridDim.x * THREADS;
+  const int boff = d.body_offset[wld], moff = d.manifold_offset[wld], poff = d.pair_offset[wld];
+  const int aoff = d.awake_offset[wld], awake = d.awake_offset[wld + 1] - aoff;
+  const int count = d.manifold_offset[wld + 1] - moff, solved = d.solved[wld];
+  const int substeps = d.substeps[wld];
+  const double h = d.dt[wld] / substeps;
+  const double* grav = d.gravity + 3 * wld;
+  for (int k = threadIdx.x; k < BM_STARTS_DEV; k += THREADS) starts[k] = d.starts[BM_STARTS_DEV * wld + k];
+  const bool touching = count > 0;
+  for (int i = tid; i < awake; i += nth) {
+    const int g = boff + d.awake_list[aoff + i];
+    if (touching) inertia_of(d, g);
+    for (int k = 0; k < 3; k++) d.start[3 * g + k] = d.position[3 * g + k];
+    for (int k = 0; k < 4; k++) d.start_q[4 * g + k] = d.rotation[4 * g + k];
+  }
+  const VW vw{d.velocity + 3 * boff, d.angular + 3 * boff, 3};
+  grid.sync();
+  Soft soft{0, 1, 0};
+  if (touching) {
+    for (int lm = tid; lm < solved; lm += nth) {
+      const int m = moff + lm, sa = d.a[m], sb = d.b[m];
+      prepare_row(d, vw, m, boff + sa, boff + sb, sa, sb);
+    }
+    soft = make_soft(STIFFNESS / h, DAMPING_RATIO, h);
+    grid.sync();
+  }
+  for (int sub = 0; sub < substeps; sub++) {
+    for (int i = tid; i < awake; i += nth) {
+      const int slot = d.awake_list[aoff + i];
+      accelerate_body(d, vw, boff + slot, slot, h, substeps, grav);
+    }
+    grid.sync();
+    if (touching) {
+      sweep_wide<0>(grid, d, vw, soft, h, false, moff, boff, starts, tid, nth);
+      sweep_wide<1>(grid, d, vw, soft, h, sub > 0, moff, boff, starts, tid, nth);
+    }
+    for (int i = tid; i < awake; i += nth) {
+      const int slot = d.awake_list[aoff + i];
+      move_body(d, vw, boff + slot, slot, h);
+    }
+    grid.sync();
+    if (touching) sweep_wide<2>(grid, d, vw, soft, h, false, moff, boff, starts, tid, nth);
+  }
+  if (touching) {
+    if (blockIdx.x == 0)
+      restitution_world(d, vw, moff, boff, count, solved, dsqrt(grav[0] * grav[0] + grav[1] * grav[1] + grav[2] * grav[2]),
+                        bounce_list, bounce_waves);
+    grid.sync();
+    for (int f = tid; f < count; f += nth) {
+      const int lm = d.row[moff + f];
+      if (lm >= solved) continue;
+      const int m = moff + lm;
+      store_row(d, m, boff + d.a[m], boff + d.b[m], poff + d.pair[m]);
+    }
+  }
+}
+
+// The grid-wide soft step of world `wld`: as many resident workgroups as its work fills.
+int launch_soft_wide(const SoftDev& d, int wld, int work, int device, hipStream_t stream) {
+  constexpr int T = 256;
+  int per_cu = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, soft_wide_kernel<T>, T, 0) != hipSuccess) return -1;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return -1;
+  int blocks = per_cu * cus;
+  const int wanted = (work + T - 1) / T;
+  if (wanted < blocks) blocks = wanted;
+  if (blocks < 1) blocks = 1;
+  SoftDev arg = d;
+  void* args[] = {&arg, &wld};
+  return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(soft_wide_kernel<T>), dim3(blocks), dim3(T), args, 0,
+                                    stream) == hipSuccess ? blocks : -1;
 }
 
 // Launch shapes: threads a workgroup, and registers sized for 1 or 2 waves a SIMD.

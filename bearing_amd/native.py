@@ -58,7 +58,7 @@ class Columns(ctypes.Structure):
 
 EXPORTS = ["bm_create", "bm_destroy", "bm_last_error", "bm_upload", "bm_soft_step", "bm_download",
            "bm_soft_step_host", "bm_last_kernel_ms", "bm_set_launch", "bm_lds_limit", "bm_colour", "bm_collide_host", "bm_meet_host", "bm_set_colour_lds", "bm_set_hulls",
-           "bm_set_fields", "bm_level_host",
+           "bm_set_fields", "bm_level_host", "bm_set_wide",
            "bm_world_create", "bm_world_destroy", "bm_world_error", "bm_world_add", "bm_world_step", "bm_world_read",
            "bm_world_set_hulls", "bm_world_set_fields", "bm_upload_worlds", "bm_download_worlds",
            "bm_batch_create", "bm_batch_destroy", "bm_batch_error", "bm_batch_world", "bm_batch_step"]
@@ -92,6 +92,7 @@ def load():
         lib.bm_last_kernel_ms.restype = ctypes.c_double
         lib.bm_set_launch.argtypes = [vp, ctypes.c_int32, ctypes.c_int32]
         lib.bm_set_colour_lds.argtypes = [vp, ctypes.c_int32]
+        lib.bm_set_wide.argtypes = [vp, ctypes.c_int32]
         lib.bm_lds_limit.argtypes = [vp]
         lib.bm_lds_limit.restype = ctypes.c_int64
         world_signatures(lib)
@@ -159,6 +160,11 @@ class Context:
 
     def set_colour_lds(self, allow=True):
         self.check(self.lib.bm_set_colour_lds(self.ptr, 1 if allow else 0), "bm_set_colour_lds")
+
+    def set_wide(self, min_manifolds):
+        """bm_set_wide: a single world of at least min_manifolds manifolds takes the grid-wide
+        soft step (0: never)."""
+        self.check(self.lib.bm_set_wide(self.ptr, min_manifolds), "bm_set_wide")
 
     def lds_limit(self):
         return self.lib.bm_lds_limit(self.ptr)

@@ -231,6 +231,12 @@ int bm_level_host(bm_context* ctx, int32_t n, const int32_t* field, const double
  * bits for every setting. */
 int bm_set_launch(bm_context* ctx, int32_t shape, int32_t resident);
 
+/* When bm_soft_step spreads one world over the whole device: a batch of one world with at
+ * least min_manifolds manifolds is stepped by every workgroup the device can keep resident,
+ * each colour's manifolds spread over all of them with a grid barrier between colours (0:
+ * never; default 16384). Results are the same bits either way. */
+int bm_set_wide(bm_context* ctx, int32_t min_manifolds);
+
 /* How bm_colour runs: allow = 1 (the default) colours each world in one workgroup's LDS,
  * one manifold after another as the reference does, when every world of the batch fits
  * there (8 bytes a sort key, 13 a manifold, 5 a body); allow = 0, or a world too large

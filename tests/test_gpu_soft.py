@@ -42,6 +42,21 @@ def test_gpu_soft_step_matches_reference(ctx, path):
     assert not errs, "\n".join(errs)
 
 
+@pytest.mark.parametrize("path", SOFT, ids=[p.split("/")[-1].replace(".bmfx.gz", "") for p in SOFT])
+def test_gpu_grid_wide_soft_step_matches_reference(ctx, path):
+    # Every fixture's world stepped by the whole grid (bm_set_wide), not one workgroup: the
+    # same bits.
+    print("fixture, grid-wide", path, flush=True)
+    fx = fixtures.load(path)
+    ctx.set_wide(1)
+    try:
+        got = run_world(ctx, fx, binding.columns(fx))
+    finally:
+        ctx.set_wide(16384)
+    errs = compare(fx, got)
+    assert not errs, "\n".join(errs)
+
+
 def test_gpu_batch_of_every_fixture(ctx):
     fxs = [fixtures.load(p) for p in SOFT]
     batch = native.Batch([world_from_columns(fx, binding.columns(fx)) for fx in fxs])
