@@ -133,9 +133,17 @@ struct bm_context {
   // The heightfields (bm_set_fields): info and offsets, spacing and corner, heights; and a
   // host copy of each one's shape, for sizing touchLevel's scratch.
   Scratch field_ints, field_doubles, level_scratch;
+  // A large world's colouring: sort keys and values, and the radix sort's own scratch; which
+  // path the last bm_colour took (0 HBM fixed point, 1 in LDS, 2 streamed and sorted).
+  Scratch colour_sort;
+  int last_colour_path = 0;
   // A single world of at least wide_min manifolds takes the grid-wide soft step (0: never);
   // last_wide, the workgroups the last step used (0: one a world).
   int32_t wide_min = 16384, last_wide = 0;
+  // The soft step's own columns stay on the device (bm_set_device_only); the arena's bytes
+  // up to them, which upload and download move.
+  bool keep_device_only = false;
+  size_t device_only_at = 0, moved = 0;
   bm::FieldTable fields{};
   struct FieldShape {
     int columns, rows;
@@ -261,9 +269,16 @@ int grow_pinned(bm_context* ctx, void** p, size_t* cap, size_t bytes, const char
 }
 
 // The arena's column order: the arranged columns first, then the rest.
+// The columns only the soft step itself writes and then reads, each one before it is read
+// (prepare's levers, masses and Jacobians; the relax pass's separations): a context that
+// keeps them on the device neither sends nor returns them.
+const char* const DEVICE_ONLY[] = {"now", "ra", "rb", "normal_mass", "jac", "block", "lever", "anchor_a", "anchor_b",
+                                   "fjac", "tangent_mass", "roll_mass1", "roll_mass2", "spin_mass"};
+
+// The arena's column order: those arrange() moves first, the device-only ones last.
 struct ArenaOrder {
   int order[NCOL];
-  int arranged;
+  int arranged, device_only;
   ArenaOrder() {
     int k = 0;
     bool taken[NCOL] = {};
@@ -273,10 +288,19 @@ struct ArenaOrder {
       taken[c] = true;
     }
     arranged = k;
+    for (const char* name : DEVICE_ONLY) taken[column_index(name)] = true;
     for (int c = 0; c < NCOL; c++)
       if (!taken[c]) order[k++] = c;
+    device_only = k;
+    for (const char* name : DEVICE_ONLY) order[k++] = column_index(name);
   }
 };
+
+bool is_device_only(int c) {
+  for (const char* name : DEVICE_ONLY)
+    if (column_index(name) == c) return true;
+  return false;
+}
 
 // Whether a shape record (kind first, a hull's number at 13) names a hull the table holds.
 bool hull_ok(const bm_context* ctx, const double* r) {
@@ -448,12 +472,15 @@ int upload_parts(bm_context* ctx, int nparts, const bm_columns* const* parts, co
     const int c = ORDER.order[k];
     const Column& col = COLUMNS[c];
     const size_t bytes = units(ctx, col.unit) * col.per * col.elem;
+    if (k == ORDER.device_only) ctx->device_only_at = at;
     ctx->bytes[c] = bytes;
     ctx->offset[c] = at;
     at += (bytes + 255) & ~(size_t)255;
     if (k + 1 == ORDER.arranged) ctx->arranged_end = at;
   }
   ctx->total = at;
+  // What moves between host and device: all of it, or up to the device-only columns.
+  ctx->moved = ctx->keep_device_only ? ctx->device_only_at : ctx->total;
   if (int rc = grow_device(ctx, &ctx->arena, &ctx->arena_cap, at, who)) return rc;
   if (int rc = grow_pinned(ctx, &ctx->stage, &ctx->stage_cap, at, who)) return rc;
   char* stage = static_cast<char*>(ctx->stage);
@@ -465,6 +492,8 @@ int upload_parts(bm_context* ctx, int nparts, const bm_columns* const* parts, co
     if (col.unit == WORLD1) {
       const int t = c;  // the four offset tables are the first four columns
       std::memcpy(dst, joined[t], ctx->bytes[c]);
+    } else if (ctx->keep_device_only && is_device_only(c)) {
+      // left on the device
     } else {
       for (int q = 0; q < nparts; q++) {
         const size_t bytes = part_units(parts[q], col.unit) * col.per * col.elem;
@@ -479,7 +508,7 @@ int upload_parts(bm_context* ctx, int nparts, const bm_columns* const* parts, co
     ctx->buffer[c] = p;
     std::memcpy(reinterpret_cast<char*>(&ctx->dev) + col.dev, &p, sizeof(void*));
   }
-  HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->arena, ctx->stage, ctx->total, hipMemcpyHostToDevice, ctx->stream),
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->arena, ctx->stage, ctx->moved, hipMemcpyHostToDevice, ctx->stream),
               (std::string(who) + ": hipMemcpyAsync").c_str());
   ctx->moff_host = ctx->join_manifold;
   ctx->boff_host = ctx->join_body;
@@ -496,15 +525,17 @@ int download_parts(bm_context* ctx, int nparts, bm_columns* const* parts, const 
   for (int q = 0; q < nparts; q++) {
     if (!parts[q]) return fail(ctx, BM_ERR_ARGUMENT, std::string(who) + ": null argument");
     for (int c = 0; c < NCOL; c++)
-      if (COLUMNS[c].dir == IO && part_units(parts[q], COLUMNS[c].unit) > 0 && !field<void*>(parts[q], COLUMNS[c].host))
+      if (COLUMNS[c].dir == IO && part_units(parts[q], COLUMNS[c].unit) > 0 && !field<void*>(parts[q], COLUMNS[c].host) &&
+          !(ctx->keep_device_only && is_device_only(c)))
         return fail(ctx, BM_ERR_ARGUMENT, std::string(who) + ": " + COLUMNS[c].name + " is null");
   }
-  HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->stage, ctx->arena, ctx->total, hipMemcpyDeviceToHost, ctx->stream),
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->stage, ctx->arena, ctx->moved, hipMemcpyDeviceToHost, ctx->stream),
               (std::string(who) + ": hipMemcpyAsync").c_str());
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), (std::string(who) + ": hipStreamSynchronize").c_str());
   const char* stage = static_cast<const char*>(ctx->stage);
   for (int c = 0; c < NCOL; c++) {
     if (COLUMNS[c].dir != IO || ctx->bytes[c] == 0) continue;
+    if (ctx->keep_device_only && is_device_only(c)) continue;
     const char* from = stage + ctx->offset[c];
     for (int q = 0; q < nparts; q++) {
       const size_t bytes = part_units(parts[q], COLUMNS[c].unit) * COLUMNS[c].per * COLUMNS[c].elem;
@@ -618,10 +649,23 @@ int bm_colour(bm_context* ctx) {
                                              ctx->key_offsets[w + 1] - ctx->key_offsets[w]);
     if (need > lds) lds = need;
   }
-  cd.in_lds = ctx->colour_in_lds && lds <= ctx->colour_lds_limit ? 1 : 0;
+  cd.in_lds = ctx->colour_in_lds == 1 && lds <= ctx->colour_lds_limit ? 1 : 0;
   (void)hipGetLastError();
   bm::launch_world_of(cd.manifold_offset, n, static_cast<int*>(ctx->world_of.p), ctx->stream);
-  bm::launch_colour(cd, lds, ctx->stream);
+  const int bodies0 = ctx->boff_host[1] - ctx->boff_host[0];
+  if (!cd.in_lds && n == 1 && ctx->colour_in_lds != 0 && manifolds > 0 &&
+      bm::colour_stream_lds(bodies0) <= ctx->colour_lds_limit) {
+    // One world too large to colour in LDS whose bodies' masks fit: streamed, then sorted.
+    const size_t temp = bm::colour_stream_scratch((int)manifolds);
+    const size_t need = manifolds * (8 + 8 + 4 + 4) + 16 + temp + 256;
+    if (int rc = grow(ctx, ctx->colour_sort, need, "bm_colour")) return rc;
+    if (bm::launch_colour_stream(cd, 0, (int)manifolds, bodies0, ctx->colour_sort.p, need, ctx->stream) != 0)
+      return fail(ctx, BM_ERR_DEVICE, "bm_colour: the radix sort could not run");
+    ctx->last_colour_path = 2;
+  } else {
+    bm::launch_colour(cd, lds, ctx->stream);
+    ctx->last_colour_path = cd.in_lds ? 1 : 0;
+  }
   HIP_OR_FAIL(ctx, hipGetLastError(), "bm_colour: launch");
   // Arrange: every column narrow() writes, into the spare arena at the same offsets, then
   // the arranged block is copied back in one piece.
@@ -1029,6 +1073,14 @@ int bm_level_host(bm_context* ctx, int32_t n, const int32_t* field, const double
   return BM_OK;
 }
 
+int bm_set_device_only(bm_context* ctx, int32_t keep) {
+  if (!ctx) return BM_ERR_ARGUMENT;
+  ctx->error.clear();
+  ctx->keep_device_only = keep != 0;
+  ctx->uploaded = false;
+  return BM_OK;
+}
+
 int bm_set_wide(bm_context* ctx, int32_t min_manifolds) {
   if (!ctx) return BM_ERR_ARGUMENT;
   ctx->error.clear();
@@ -1039,7 +1091,8 @@ int bm_set_wide(bm_context* ctx, int32_t min_manifolds) {
 
 int bm_set_colour_lds(bm_context* ctx, int32_t allow) {
   if (!ctx) return BM_ERR_ARGUMENT;
-  ctx->colour_in_lds = allow ? 1 : 0;
+  if (allow < 0 || allow > 2) return fail(ctx, BM_ERR_ARGUMENT, "bm_set_colour_lds: allow is 0, 1 or 2");
+  ctx->colour_in_lds = allow;
   return BM_OK;
 }
 

@@ -57,6 +57,14 @@ struct RestDev {
 };
 
 void launch_colour(const ColourDev& d, size_t lds_bytes, hipStream_t stream);
+// One world (the batch's wld-th, at manifold offset 0) coloured by streaming its manifolds
+// through one wave with the bodies' held masks in LDS, its rows ordered by a radix sort:
+// for a world too large to colour in LDS. LDS it takes; sort scratch it needs; 0, or -1 if
+// the sort could not run.
+size_t colour_stream_lds(int bodies);
+size_t colour_stream_scratch(int manifolds);
+int launch_colour_stream(const ColourDev& d, int wld, int manifolds, int bodies, void* scratch, size_t scratch_bytes,
+                         hipStream_t stream);
 void launch_world_of(const int* manifold_offset, int n_worlds, int* world_of, hipStream_t stream);
 void launch_arrange(const ArrangeDev& d, int columns, hipStream_t stream);
 void launch_rest(const RestDev& d, hipStream_t stream);

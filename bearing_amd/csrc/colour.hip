@@ -24,6 +24,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <hipcub/device/device_radix_sort.hpp>
+
 #include "colour.h"
 
 namespace bm {
@@ -263,6 +265,125 @@ __global__ void __launch_bounds__(256) colour_kernel(ColourDev d) {
     }
     __syncthreads();
   }
+}
+
+// One world too large to colour in LDS whose bodies' held masks fit there: the reference's
+// walk (src/graph.js:125-140) by the first wave, the masks and moving flags in LDS, the
+// manifolds streamed from HBM 64 at a time (each lane fetching the next 64 ends and flags
+// while the wave walks the current ones), the colours written to HBM. Then the runs per
+// colour, starts and solved, and the sort keys (colour, b) with the manifold numbers, for a
+// stable radix sort that leaves the rows as the reference's counting sorts do.
+__global__ void __launch_bounds__(256) colour_stream_kernel(ColourDev d, int wld, uint64_t* keys, int* values) {
+  __shared__ int runs[RESTING + 2];
+  extern __shared__ __align__(16) unsigned char lds[];
+  const int boff = d.body_offset[wld], moff = d.manifold_offset[wld];
+  const int nb = d.body_offset[wld + 1] - boff, n = d.manifold_offset[wld + 1] - moff;
+  int32_t* held = reinterpret_cast<int32_t*>(lds);
+  uint8_t* moving = reinterpret_cast<uint8_t*>(held + nb);
+  const int* A = d.a + moff;
+  const int* B = d.b + moff;
+  const uint8_t* type = d.type + boff;
+  const uint8_t* frozen = d.frozen + boff;
+  int* colour = d.colour + moff;
+  for (int s = threadIdx.x; s < nb; s += blockDim.x) {
+    held[s] = 0;
+    moving[s] = type[s] == DYNAMIC && frozen[s] == 0;
+  }
+  for (int k = threadIdx.x; k < RESTING + 2; k += blockDim.x) runs[k] = 0;
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    // A manifold's ends, and its flags: 1 a moves, 2 b moves, 4 resting (src/graph.js:114-124).
+    auto fetch = [&](int m, int& ea, int& eb, int& flags) {
+      ea = 0; eb = 0; flags = 0;
+      if (m >= n) return;
+      ea = A[m];
+      eb = B[m];
+      const bool rest_a = frozen[ea] != 0 || type[ea] == STATIC, rest_b = frozen[eb] != 0 || type[eb] == STATIC;
+      flags = (moving[ea] ? 1 : 0) | (moving[eb] ? 2 : 0) | (rest_a && rest_b ? 4 : 0);
+    };
+    int ea, eb, flags;
+    fetch(lane, ea, eb, flags);
+    for (int base = 0; base < n; base += 64) {
+      int na, nbb, nf;
+      fetch(base + 64 + lane, na, nbb, nf);
+      int mine = RESTING;
+      const int count = n - base < 64 ? n - base : 64;
+      for (int j = 0; j < count; j++) {
+        const int f = __builtin_amdgcn_readlane(flags, j);
+        if (f & 4) continue;
+        const int a = __builtin_amdgcn_readlane(ea, j), b = __builtin_amdgcn_readlane(eb, j);
+        const int32_t h = ((f & 1) ? held[a] : 0) | ((f & 2) ? held[b] : 0);
+        int c;
+        if (h == -1) {
+          c = OVERFLOW;
+        } else {
+          const int32_t bit = ~h & (h + 1);
+          c = 31 - __clz((uint32_t)bit);
+          if (lane == 0) {
+            if (f & 1) held[a] |= bit;
+            if (f & 2) held[b] |= bit;
+          }
+        }
+        if (lane == j) mine = c;
+      }
+      if (base + lane < n) colour[base + lane] = mine;
+      ea = na; eb = nbb; flags = nf;
+    }
+  }
+  __syncthreads();
+  for (int m = threadIdx.x; m < n; m += blockDim.x) {
+    const int c = colour[m];
+    atomicAdd(&runs[c + 1], 1);
+    keys[m] = ((uint64_t)c << 32) | (uint32_t)B[m];
+    values[m] = m;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int* starts = d.starts + BM_STARTS_DEV * wld;
+    starts[0] = 0;
+    for (int k = 0; k <= RESTING; k++) starts[k + 1] = starts[k] + runs[k + 1];
+    d.solved[wld] = starts[RESTING];
+  }
+}
+
+// order[o] = the o-th manifold in the sorted order, row[m] its row.
+__global__ void order_kernel(const int* sorted, int n, int* order, int* row) {
+  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < n; o += gridDim.x * blockDim.x) {
+    const int m = sorted[o];
+    order[o] = m;
+    row[m] = o;
+  }
+}
+
+size_t colour_stream_lds(int bodies) { return ((size_t)bodies * 5 + 15) & ~(size_t)15; }
+
+size_t colour_stream_scratch(int manifolds) {
+  size_t temp = 0;
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (const uint64_t*)nullptr, (uint64_t*)nullptr,
+                                           (const int*)nullptr, (int*)nullptr, manifolds, 0, 38);
+  return temp;
+}
+
+int launch_colour_stream(const ColourDev& d, int wld, int manifolds, int bodies, void* scratch, size_t scratch_bytes,
+                         hipStream_t stream) {
+  // Scratch: keys in and out, values in and out, then the sort's own.
+  uint64_t* keys = static_cast<uint64_t*>(scratch);
+  uint64_t* keys_out = keys + manifolds;
+  int* values = reinterpret_cast<int*>(keys_out + manifolds);
+  int* values_out = values + manifolds;
+  char* temp = reinterpret_cast<char*>(values_out + manifolds + (manifolds & 1));
+  const size_t used = (size_t)(temp - static_cast<char*>(scratch));
+  size_t temp_bytes = scratch_bytes > used ? scratch_bytes - used : 0;
+  hipLaunchKernelGGL(colour_stream_kernel, dim3(1), dim3(256), colour_stream_lds(bodies), stream, d, wld, keys, values);
+  // Colour (6 bits) above b's slot (32 bits): the stable sort keeps the order found within a
+  // colour and slot.
+  if (hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys, keys_out, values, values_out, manifolds, 0, 38,
+                                         stream) != hipSuccess)
+    return -1;
+  hipLaunchKernelGGL(order_kernel, dim3((manifolds + 255) / 256), dim3(256), 0, stream, values_out, manifolds,
+                     d.order, d.row);
+  return 0;
 }
 
 // Moves column `c` of every world's manifolds to their rows: to[o] = from[order[o]].

@@ -11,6 +11,7 @@
 // row layout, and the whole soft step (colour.hip, collide.hip, soft.hip). Every
 // arithmetic step is the reference's, so a world's transforms are the reference's bits.
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <condition_variable>
 #include <cstdint>
@@ -27,6 +28,72 @@
 
 namespace {
 
+// A small pool of host threads for per-world phases and a large world's passes: run(n, f) calls f(0..n-1)
+// across the threads and the caller, and returns when every call has.
+class Pool {
+ public:
+  explicit Pool(int threads) {
+    for (int t = 0; t < threads; t++) workers_.emplace_back([this] { work(); });
+  }
+  ~Pool() {
+    {
+      std::lock_guard<std::mutex> g(m_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto& t : workers_) t.join();
+  }
+  void run(int n, const std::function<void(int)>& f) {
+    if (workers_.empty() || n <= 1) {
+      for (int i = 0; i < n; i++) f(i);
+      return;
+    }
+    {
+      std::lock_guard<std::mutex> g(m_);
+      job_ = &f;
+      n_ = n;
+      next_.store(0);
+      busy_ = (int)workers_.size();
+      ++generation_;
+    }
+    cv_.notify_all();
+    for (int i; (i = next_.fetch_add(1)) < n;) f(i);
+    std::unique_lock<std::mutex> g(m_);
+    done_.wait(g, [this] { return busy_ == 0; });
+    job_ = nullptr;
+  }
+
+ private:
+  void work() {
+    long seen = 0;
+    for (;;) {
+      const std::function<void(int)>* job;
+      int n;
+      {
+        std::unique_lock<std::mutex> g(m_);
+        cv_.wait(g, [&] { return stop_ || generation_ != seen; });
+        if (stop_) return;
+        seen = generation_;
+        job = job_;
+        n = n_;
+      }
+      for (int i; (i = next_.fetch_add(1)) < n;) (*job)(i);
+      {
+        std::lock_guard<std::mutex> g(m_);
+        if (--busy_ == 0) done_.notify_all();
+      }
+    }
+  }
+  std::vector<std::thread> workers_;
+  std::mutex m_;
+  std::condition_variable cv_, done_;
+  const std::function<void(int)>* job_ = nullptr;
+  int n_ = 0, busy_ = 0;
+  long generation_ = 0;
+  std::atomic<int> next_{0};
+  bool stop_ = false;
+};
+constexpr int PARALLEL_VISITS = 8192;  // a narrow pass this long decides its pairs on threads
 constexpr int P = 4;
 constexpr int DYNAMIC = 0, STATIC = 1;
 constexpr int BOX = 0, SPHERE = 1, CAPSULE = 2, HULL = 3, MESH = 5, HEIGHTFIELD = 6;  // src/contact.js:114
@@ -325,6 +392,10 @@ struct bm_world {
   // made while the batch's host threads run holds device_lock).
   bool owns_ctx = true;
   std::mutex* device_lock = nullptr;
+  // Host threads for a large world's narrow passes (a world stepped alone); none in a batch.
+  Pool* pool = nullptr;
+  // Milliseconds spent in each part of the steps so far (bm_world_times).
+  double times[BM_WORLD_TIMES] = {};
   // In a batch the device holds every world's hulls and fields in one table: this world's
   // hull h is the device's hull_base + h, its field f the device's field_base + f. The
   // world's own tables as given, for the batch to join; `joined` false until it has.
@@ -896,13 +967,13 @@ int run_batch(bm_world* w) {
 }
 
 // src/contact.js:832-856 (level) up to touchLevel: the other body's box, grown by its motion,
-// in the field's frame and grown by the field's own motion, appended to the level batch
-// with the other body's shape. The tolerance is the other body's: a field has no thickness.
-void prepare_level(bm_world* w, Visit& v) {
+// in the field's frame and grown by the field's own motion, written into the level batch at
+// the visit's slot with the other body's shape.
+void fill_level(bm_world* w, const Visit& v) {
   const int a = v.a, b = v.b;
   const bool turned = w->shape[a] >= MESH;
   const int g = turned ? a : b, o = turned ? b : a;
-  const double tolerance = w->size[o] * PIXEL;
+  const double tolerance = v.tolerance;
   const double* reach = w->motion.data();
   const double margin = reach[a] + reach[b] + tolerance;
   const double* box = w->box.data();
@@ -918,19 +989,17 @@ void prepare_level(bm_world* w, Visit& v) {
   const double ex = std::fabs(A[g9]) * hx + std::fabs(A[g9 + 1]) * hy + std::fabs(A[g9 + 2]) * hz + reach[g];
   const double ey = std::fabs(A[g9 + 3]) * hx + std::fabs(A[g9 + 4]) * hy + std::fabs(A[g9 + 5]) * hz + reach[g];
   const double ez = std::fabs(A[g9 + 6]) * hx + std::fabs(A[g9 + 7]) * hy + std::fabs(A[g9 + 8]) * hz + reach[g];
-  v.kind = 3;
-  v.tolerance = tolerance;
-  v.slot = (int)w->level_field.size();
-  w->level_field.push_back(w->field_base + w->field[g]);
-  w->level_pos.insert(w->level_pos.end(), pos + g3, pos + g3 + 3);
-  w->level_axes.insert(w->level_axes.end(), A + g9, A + g9 + 9);
+  const size_t k = v.slot;
+  w->level_field[k] = w->field_base + w->field[g];
+  for (int i = 0; i < 3; i++) w->level_pos[3 * k + i] = pos[g3 + i];
+  for (int i = 0; i < 9; i++) w->level_axes[9 * k + i] = A[g9 + i];
   double r[32];
   load(w, o, r);
-  w->level_shape.insert(w->level_shape.end(), r, r + 19);
+  for (int i = 0; i < 19; i++) w->level_shape[19 * k + i] = r[i];
   const double lb[6] = {lx - ex, ly - ey, lz - ez, lx + ex, ly + ey, lz + ez};
-  w->level_box.insert(w->level_box.end(), lb, lb + 6);
-  w->level_margin.push_back(margin);
-  w->level_tol.push_back(tolerance);
+  for (int i = 0; i < 6; i++) w->level_box[6 * k + i] = lb[i];
+  w->level_margin[k] = margin;
+  w->level_tol[k] = tolerance;
 }
 
 // The rest of src/contact.js:857-881 (level): wake, then each direction the device found as a
@@ -970,9 +1039,9 @@ int finish_level(bm_world* w, Visit& v) {
   return BM_OK;
 }
 
-// find() up to the narrowphase (src/contact.js:334-354): the box test, the kept manifold
-// (REUSE), a meet record appended to the device batch, or a level pair's record.
-void prepare_visit(bm_world* w, Visit& v) {
+// find()'s decision for one visit (src/contact.js:334-354), reading only: the boxes apart
+// (0), the kept manifold rebuilt (1, REUSE, into v.m), a meet on the device (2), a level pair (3).
+void decide_visit(const bm_world* w, Visit& v) {
   const int a = v.a, b = v.b;
   v.kind = 0;
   v.slot = -1;
@@ -980,23 +1049,47 @@ void prepare_visit(bm_world* w, Visit& v) {
   const double* bb = &w->box[6 * b];
   if (ba[0] > bb[3] || bb[0] > ba[3] || ba[1] > bb[4] || bb[1] > ba[4] || ba[2] > bb[5] || bb[2] > ba[5]) return;
   if (w->shape[a] >= MESH || w->shape[b] >= MESH) {
-    prepare_level(w, v);
+    // A field has no thickness: the tolerance is the other body's (src/contact.js:836).
+    v.kind = 3;
+    v.tolerance = w->size[w->shape[a] >= MESH ? b : a] * PIXEL;
     return;
   }
   const double margin = w->motion[a] + w->motion[b] + v.tolerance;
-  if (reused(w, v.p, a, b, v.tolerance, margin, v.m)) {
-    v.kind = 1;
-    return;
-  }
-  v.kind = 2;
-  v.slot = (int)w->batch_pair.size() / 6;
-  double ra[32], rb[32];
-  load(w, a, ra);
-  load(w, b, rb);
-  w->batch_a.insert(w->batch_a.end(), ra, ra + 32);
-  w->batch_b.insert(w->batch_b.end(), rb, rb + 32);
+  v.kind = reused(w, v.p, a, b, v.tolerance, margin, v.m) ? 1 : 2;
+}
+
+// The meet record of a visit of kind 2 at its slot: both bodies and the pair's numbers.
+void fill_meet(bm_world* w, const Visit& v) {
+  const size_t k = v.slot;
+  load(w, v.a, &w->batch_a[32 * k]);
+  load(w, v.b, &w->batch_b[32 * k]);
+  const double margin = w->motion[v.a] + w->motion[v.b] + v.tolerance;
   const double pair[6] = {margin, v.tolerance, w->step_dt, w->gravity[0], w->gravity[1], w->gravity[2]};
-  w->batch_pair.insert(w->batch_pair.end(), pair, pair + 6);
+  for (int i = 0; i < 6; i++) w->batch_pair[6 * k + i] = pair[i];
+}
+
+// Room in the device batches for `meets` more meet records and `levels` more level records.
+void grow_batches(bm_world* w, size_t meets, size_t levels) {
+  const size_t m = w->batch_pair.size() / 6 + meets;
+  w->batch_a.resize(32 * m); w->batch_b.resize(32 * m); w->batch_pair.resize(6 * m);
+  const size_t l = w->level_field.size() + levels;
+  w->level_field.resize(l); w->level_pos.resize(3 * l); w->level_axes.resize(9 * l); w->level_shape.resize(19 * l);
+  w->level_box.resize(6 * l); w->level_margin.resize(l); w->level_tol.resize(l);
+}
+
+// find() up to the narrowphase (src/contact.js:334-354) for one visit: the box test, the
+// kept manifold (REUSE), a meet record appended to the device batch, or a level pair's.
+void prepare_visit(bm_world* w, Visit& v) {
+  decide_visit(w, v);
+  if (v.kind == 2) {
+    v.slot = (int)(w->batch_pair.size() / 6);
+    grow_batches(w, 1, 0);
+    fill_meet(w, v);
+  } else if (v.kind == 3) {
+    v.slot = (int)w->level_field.size();
+    grow_batches(w, 0, 1);
+    fill_level(w, v);
+  }
 }
 
 // The rest of find() and narrow()'s own steps for one visit, in the reference's order:
@@ -1058,10 +1151,36 @@ void narrow_begin(bm_world* w, int from) {
       pr.remove(p);
       continue;
     }
-    Visit v{p, a, b, js_min(w->size[a], w->size[b]) * PIXEL, 0, -1, Manifold{}};
-    prepare_visit(w, v);
-    visits.push_back(v);
+    visits.push_back(Visit{p, a, b, js_min(w->size[a], w->size[b]) * PIXEL, 0, -1, Manifold{}});
     p++;
+  }
+  // The pairs are where they will stay for the pass: each one's decision reads only, so a
+  // large pass decides on the pool's threads, then takes its slots in order and fills them.
+  const int n = (int)visits.size();
+  if (!w->pool && w->owns_ctx && n >= PARALLEL_VISITS) {
+    // A world stepped alone that grew large: its own threads, up to 16.
+    const int threads = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+    w->pool = new Pool(threads - 1);
+  }
+  if (w->pool && n >= PARALLEL_VISITS) {
+    const int chunk = 1024, chunks = (n + chunk - 1) / chunk;
+    w->pool->run(chunks, [&](int c) {
+      for (int i = c * chunk, e = std::min(n, (c + 1) * chunk); i < e; i++) decide_visit(w, visits[i]);
+    });
+    size_t meets = w->batch_pair.size() / 6, levels = w->level_field.size(), m0 = meets, l0 = levels;
+    for (Visit& v : visits) {
+      if (v.kind == 2) v.slot = (int)meets++;
+      else if (v.kind == 3) v.slot = (int)levels++;
+    }
+    grow_batches(w, meets - m0, levels - l0);
+    w->pool->run(chunks, [&](int c) {
+      for (int i = c * chunk, e = std::min(n, (c + 1) * chunk); i < e; i++) {
+        if (visits[i].kind == 2) fill_meet(w, visits[i]);
+        else if (visits[i].kind == 3) fill_level(w, visits[i]);
+      }
+    });
+  } else {
+    for (Visit& v : visits) prepare_visit(w, v);
   }
   // Prepared after every wake so far: nothing is fresh for this pass yet.
   for (int s : w->woken) w->fresh[s] = 0;
@@ -2077,71 +2196,6 @@ int step_end(bm_world* w, double dt) {
   return BM_OK;
 }
 
-// A small pool of host threads for a batch's per-world phases: run(n, f) calls f(0..n-1)
-// across the threads and the caller, and returns when every call has.
-class Pool {
- public:
-  explicit Pool(int threads) {
-    for (int t = 0; t < threads; t++) workers_.emplace_back([this] { work(); });
-  }
-  ~Pool() {
-    {
-      std::lock_guard<std::mutex> g(m_);
-      stop_ = true;
-    }
-    cv_.notify_all();
-    for (auto& t : workers_) t.join();
-  }
-  void run(int n, const std::function<void(int)>& f) {
-    if (workers_.empty() || n <= 1) {
-      for (int i = 0; i < n; i++) f(i);
-      return;
-    }
-    {
-      std::lock_guard<std::mutex> g(m_);
-      job_ = &f;
-      n_ = n;
-      next_.store(0);
-      busy_ = (int)workers_.size();
-      ++generation_;
-    }
-    cv_.notify_all();
-    for (int i; (i = next_.fetch_add(1)) < n;) f(i);
-    std::unique_lock<std::mutex> g(m_);
-    done_.wait(g, [this] { return busy_ == 0; });
-    job_ = nullptr;
-  }
-
- private:
-  void work() {
-    long seen = 0;
-    for (;;) {
-      const std::function<void(int)>* job;
-      int n;
-      {
-        std::unique_lock<std::mutex> g(m_);
-        cv_.wait(g, [&] { return stop_ || generation_ != seen; });
-        if (stop_) return;
-        seen = generation_;
-        job = job_;
-        n = n_;
-      }
-      for (int i; (i = next_.fetch_add(1)) < n;) (*job)(i);
-      {
-        std::lock_guard<std::mutex> g(m_);
-        if (--busy_ == 0) done_.notify_all();
-      }
-    }
-  }
-  std::vector<std::thread> workers_;
-  std::mutex m_;
-  std::condition_variable cv_, done_;
-  const std::function<void(int)>* job_ = nullptr;
-  int n_ = 0, busy_ = 0;
-  long generation_ = 0;
-  std::atomic<int> next_{0};
-  bool stop_ = false;
-};
 
 }  // namespace
 
@@ -2315,6 +2369,8 @@ int bm_world_create(bm_world** out, int32_t device, const double* gravity, int32
     delete w;
     return rc;
   }
+  // The host never reads the soft step's own columns between steps.
+  bm_set_device_only(w->ctx, 1);
   if (gravity)
     for (int k = 0; k < 3; k++) w->gravity[k] = gravity[k];
   w->substeps = substeps > 0 ? substeps : 8;
@@ -2326,6 +2382,7 @@ int bm_world_create(bm_world** out, int32_t device, const double* gravity, int32
 int bm_world_destroy(bm_world* w) {
   if (!w) return BM_OK;
   if (!w->owns_ctx) return fail(w, BM_ERR_ARGUMENT, "bm_world_destroy: the world belongs to a batch (bm_batch_destroy)");
+  delete w->pool;
   bm_destroy(w->ctx);
   delete w;
   return BM_OK;
@@ -2515,15 +2572,37 @@ int bm_world_set_fields(bm_world* w, int32_t n, const int32_t* columns, const in
 // src/world.js:880-1072 (step), for the bodies above.
 int bm_world_step(bm_world* w, double dt) {
   if (!w) return BM_ERR_ARGUMENT;
+  using clock = std::chrono::steady_clock;
+  auto t = clock::now();
+  auto lap = [&](int k) {
+    const auto now = clock::now();
+    w->times[k] += std::chrono::duration<double, std::milli>(now - t).count();
+    t = now;
+  };
   if (int rc = step_begin(w, dt)) return rc;
+  lap(0);
   int rc = BM_OK;
-  do {
+  for (;;) {
     if ((rc = run_batch(w)) != BM_OK) return rc;
-  } while (narrow_continue(w, rc));
+    lap(1);
+    const bool more = narrow_continue(w, rc);
+    lap(2);
+    if (!more) break;
+  }
   if (rc != BM_OK) return rc;
   step_middle(w);
+  lap(3);
   if ((rc = device_phase(w, dt)) != BM_OK) return rc;
-  return step_end(w, dt);
+  lap(4);
+  rc = step_end(w, dt);
+  lap(5);
+  return rc;
+}
+
+int bm_world_times(const bm_world* w, double* out) {
+  if (!w || !out) return BM_ERR_ARGUMENT;
+  for (int k = 0; k < BM_WORLD_TIMES; k++) out[k] = w->times[k];
+  return BM_OK;
 }
 
 int bm_batch_create(bm_batch** out, int32_t device, int32_t threads) {
@@ -2535,6 +2614,7 @@ int bm_batch_create(bm_batch** out, int32_t device, int32_t threads) {
     delete B;
     return rc;
   }
+  bm_set_device_only(B->ctx, 1);
   B->pool = new Pool(threads > 1 ? threads - 1 : 0);
   *out = B;
   return BM_OK;

@@ -58,9 +58,9 @@ class Columns(ctypes.Structure):
 
 EXPORTS = ["bm_create", "bm_destroy", "bm_last_error", "bm_upload", "bm_soft_step", "bm_download",
            "bm_soft_step_host", "bm_last_kernel_ms", "bm_set_launch", "bm_lds_limit", "bm_colour", "bm_collide_host", "bm_meet_host", "bm_set_colour_lds", "bm_set_hulls",
-           "bm_set_fields", "bm_level_host", "bm_set_wide",
+           "bm_set_fields", "bm_level_host", "bm_set_wide", "bm_set_device_only",
            "bm_world_create", "bm_world_destroy", "bm_world_error", "bm_world_add", "bm_world_step", "bm_world_read",
-           "bm_world_set_hulls", "bm_world_set_fields", "bm_upload_worlds", "bm_download_worlds",
+           "bm_world_set_hulls", "bm_world_set_fields", "bm_world_times", "bm_upload_worlds", "bm_download_worlds",
            "bm_batch_create", "bm_batch_destroy", "bm_batch_error", "bm_batch_world", "bm_batch_step"]
 
 _lib = None
@@ -93,6 +93,7 @@ def load():
         lib.bm_set_launch.argtypes = [vp, ctypes.c_int32, ctypes.c_int32]
         lib.bm_set_colour_lds.argtypes = [vp, ctypes.c_int32]
         lib.bm_set_wide.argtypes = [vp, ctypes.c_int32]
+        lib.bm_set_device_only.argtypes = [vp, ctypes.c_int32]
         lib.bm_lds_limit.argtypes = [vp]
         lib.bm_lds_limit.restype = ctypes.c_int64
         world_signatures(lib)
@@ -135,6 +136,7 @@ def world_signatures(lib):
     lib.bm_batch_error.restype = ctypes.c_char_p
     lib.bm_batch_world.argtypes = [vp, ctypes.POINTER(vp), _D, ctypes.c_int32, ctypes.c_int32]
     lib.bm_batch_step.argtypes = [vp, ctypes.c_double]
+    lib.bm_world_times.argtypes = [vp, _D]
 
 
 class BearingError(RuntimeError):
@@ -159,7 +161,10 @@ class Context:
         self.check(self.lib.bm_set_launch(self.ptr, shape, 1 if resident else 0), "bm_set_launch")
 
     def set_colour_lds(self, allow=True):
-        self.check(self.lib.bm_set_colour_lds(self.ptr, 1 if allow else 0), "bm_set_colour_lds")
+        """bm_set_colour_lds: True (1) colour in LDS where a world fits, False (0) the HBM path,
+        "stream" (2) a single world's manifolds streamed through one wave and radix-sorted."""
+        mode = 2 if allow == "stream" else 1 if allow else 0
+        self.check(self.lib.bm_set_colour_lds(self.ptr, mode), "bm_set_colour_lds")
 
     def set_wide(self, min_manifolds):
         """bm_set_wide: a single world of at least min_manifolds manifolds takes the grid-wide
@@ -431,6 +436,14 @@ class World:
         for name in ("top", "awake", "contacts", "pairs", "frozen", "impacts", "catches"):
             out[name] = getattr(s, name)
         return out
+
+    TIMES = ["prepare", "narrowphase_device", "narrow_host", "freeze", "contact_device", "catch_sleep"]
+
+    def times(self):
+        """Milliseconds the steps so far spent in each part (bm_world_times), by name."""
+        out = np.zeros(len(self.TIMES))
+        self.check(self.lib.bm_world_times(self.ptr, out.ctypes.data_as(_D)), "bm_world_times")
+        return dict(zip(self.TIMES, out.tolist()))
 
     def close(self):
         if self.ptr and self.batch is None:

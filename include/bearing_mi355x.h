@@ -231,6 +231,13 @@ int bm_level_host(bm_context* ctx, int32_t n, const int32_t* field, const double
  * bits for every setting. */
 int bm_set_launch(bm_context* ctx, int32_t shape, int32_t resident);
 
+/* keep = 1: the columns only the soft step writes and then reads (now, ra, rb, normal_mass,
+ * jac, block, lever, anchor_a, anchor_b, fjac, tangent_mass, roll_mass1, roll_mass2,
+ * spin_mass) stay on the device: bm_upload does not send them (they may be NULL) and
+ * bm_download does not write them back. For callers that never read them between steps (the
+ * whole-step path); results are the same bits. Default 0. */
+int bm_set_device_only(bm_context* ctx, int32_t keep);
+
 /* When bm_soft_step spreads one world over the whole device: a batch of one world with at
  * least min_manifolds manifolds is stepped by every workgroup the device can keep resident,
  * each colour's manifolds spread over all of them with a grid barrier between colours (0:
@@ -239,9 +246,11 @@ int bm_set_wide(bm_context* ctx, int32_t min_manifolds);
 
 /* How bm_colour runs: allow = 1 (the default) colours each world in one workgroup's LDS,
  * one manifold after another as the reference does, when every world of the batch fits
- * there (8 bytes a sort key, 13 a manifold, 5 a body); allow = 0, or a world too large
- * for LDS, takes the HBM path, a priority fixed point over each manifold's earlier
- * neighbours. Both give the reference's colours. */
+ * there (8 bytes a sort key, 13 a manifold, 5 a body); a batch of one world too large for
+ * that whose bodies fit (5 bytes a body) is walked the same way with its manifolds streamed
+ * from HBM, and its rows laid out by a stable radix sort. allow = 0, or worlds too large for
+ * either, take the HBM path, a priority fixed point over each manifold's earlier neighbours.
+ * allow = 2 streams a batch of one world whatever its size. All give the reference's colours. */
 int bm_set_colour_lds(bm_context* ctx, int32_t allow);
 
 /* Bytes of LDS a soft-step workgroup can hold velocities in on this context's
@@ -319,6 +328,13 @@ int bm_world_set_fields(bm_world* world, int32_t n, const int32_t* columns, cons
 /* world.step(dt) (src/world.js:880). */
 int bm_world_step(bm_world* world, double dt);
 int bm_world_read(const bm_world* world, bm_world_state* out);
+/* Milliseconds the world's steps have spent so far in each part, into out[BM_WORLD_TIMES]:
+ * 0 bounds, broadphase and preparing the first narrow pass; 1 the narrowphase on the device
+ * (with its copies); 2 taking the narrowphase's results in order and the later passes'
+ * preparing; 3 freezing; 4 the contact phase on the device (upload, colour, soft step,
+ * download); 5 the catch, touched, transforms and sleep. */
+#define BM_WORLD_TIMES 6
+int bm_world_times(const bm_world* world, double* out);
 
 /* ---- worlds stepped together (the batched form) ----
  *

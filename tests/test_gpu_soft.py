@@ -109,7 +109,7 @@ def test_gpu_matches_oracle_on_scrambled_worlds(ctx, seed):
 
 
 @pytest.mark.parametrize("path", SOFT, ids=[p.split("/")[-1].replace(".bmfx.gz", "") for p in SOFT])
-@pytest.mark.parametrize("in_lds", [True, False], ids=["lds", "hbm"])
+@pytest.mark.parametrize("in_lds", [True, False, "stream"], ids=["lds", "hbm", "stream"])
 def test_gpu_colour_then_soft_step_matches_reference(ctx, path, in_lds):
     """From the manifolds in the order narrow() found them: bm_colour must lay the rows out
     as the reference's colour() / arrange() / rest() did, and the soft step after it must
