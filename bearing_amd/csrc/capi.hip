@@ -2,11 +2,14 @@
 // columns, and the soft-step launch.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -408,6 +411,37 @@ int64_t bm_lds_limit(bm_context* ctx) {
   return (int64_t)ctx->lds_limit;
 }
 
+// Host copies between the caller's columns and the pinned staging: a large batch's (tens of
+// MB and more) split into 4 MB pieces over up to 16 host threads, a small one's in turn.
+struct HostCopy {
+  void* to;
+  const void* from;
+  size_t bytes;
+};
+
+void run_copies(const std::vector<HostCopy>& copies) {
+  size_t total = 0;
+  for (const HostCopy& c : copies) total += c.bytes;
+  const int threads = total >= ((size_t)64 << 20) ? std::max(1, std::min(16, (int)std::thread::hardware_concurrency())) : 1;
+  if (threads <= 1) {
+    for (const HostCopy& c : copies) std::memcpy(c.to, c.from, c.bytes);
+    return;
+  }
+  constexpr size_t PIECE = (size_t)4 << 20;
+  std::vector<HostCopy> pieces;
+  for (const HostCopy& c : copies)
+    for (size_t at = 0; at < c.bytes; at += PIECE)
+      pieces.push_back({static_cast<char*>(c.to) + at, static_cast<const char*>(c.from) + at, std::min(PIECE, c.bytes - at)});
+  std::atomic<size_t> next{0};
+  auto work = [&] {
+    for (size_t i; (i = next.fetch_add(1)) < pieces.size();) std::memcpy(pieces[i].to, pieces[i].from, pieces[i].bytes);
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < threads; t++) pool.emplace_back(work);
+  work();
+  for (auto& t : pool) t.join();
+}
+
 // Units of kind u in one part (a bm_columns of one or more worlds).
 size_t part_units(const bm_columns* h, Unit u) {
   const int n = h->n_worlds;
@@ -486,6 +520,7 @@ int upload_parts(bm_context* ctx, int nparts, const bm_columns* const* parts, co
   char* stage = static_cast<char*>(ctx->stage);
   const int32_t* joined[4] = {ctx->join_body.data(), ctx->join_manifold.data(), ctx->join_pair.data(),
                               ctx->join_awake.data()};
+  std::vector<HostCopy> copies;
   for (int c = 0; c < NCOL; c++) {
     const Column& col = COLUMNS[c];
     char* dst = stage + ctx->offset[c];
@@ -500,7 +535,7 @@ int upload_parts(bm_context* ctx, int nparts, const bm_columns* const* parts, co
         if (bytes == 0) continue;
         const void* from = field<const void*>(parts[q], col.host);
         if (!from) return fail(ctx, BM_ERR_ARGUMENT, std::string(who) + ": " + col.name + " is null");
-        std::memcpy(dst, from, bytes);
+        copies.push_back({dst, from, bytes});
         dst += bytes;
       }
     }
@@ -508,6 +543,7 @@ int upload_parts(bm_context* ctx, int nparts, const bm_columns* const* parts, co
     ctx->buffer[c] = p;
     std::memcpy(reinterpret_cast<char*>(&ctx->dev) + col.dev, &p, sizeof(void*));
   }
+  run_copies(copies);
   HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->arena, ctx->stage, ctx->moved, hipMemcpyHostToDevice, ctx->stream),
               (std::string(who) + ": hipMemcpyAsync").c_str());
   ctx->moff_host = ctx->join_manifold;
@@ -533,6 +569,7 @@ int download_parts(bm_context* ctx, int nparts, bm_columns* const* parts, const 
               (std::string(who) + ": hipMemcpyAsync").c_str());
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), (std::string(who) + ": hipStreamSynchronize").c_str());
   const char* stage = static_cast<const char*>(ctx->stage);
+  std::vector<HostCopy> copies;
   for (int c = 0; c < NCOL; c++) {
     if (COLUMNS[c].dir != IO || ctx->bytes[c] == 0) continue;
     if (ctx->keep_device_only && is_device_only(c)) continue;
@@ -540,10 +577,11 @@ int download_parts(bm_context* ctx, int nparts, bm_columns* const* parts, const 
     for (int q = 0; q < nparts; q++) {
       const size_t bytes = part_units(parts[q], COLUMNS[c].unit) * COLUMNS[c].per * COLUMNS[c].elem;
       if (bytes == 0) continue;
-      std::memcpy(field<void*>(parts[q], COLUMNS[c].host), from, bytes);
+      copies.push_back({field<void*>(parts[q], COLUMNS[c].host), from, bytes});
       from += bytes;
     }
   }
+  run_copies(copies);
   return BM_OK;
 }
 
@@ -810,9 +848,7 @@ int bm_meet_host(bm_context* ctx, int32_t n, const double* a, const double* b, c
   // One copy each way through pinned staging.
   if (int rc = grow_pinned(ctx, &ctx->pstage, &ctx->pstage_cap, in_bytes + out_bytes, "bm_meet_host")) return rc;
   char* st = static_cast<char*>(ctx->pstage);
-  std::memcpy(st, a, bodies);
-  std::memcpy(st + bodies, b, bodies);
-  std::memcpy(st + 2 * bodies, pair, pairs);
+  run_copies({{st, a, bodies}, {st + bodies, b, bodies}, {st + 2 * bodies, pair, pairs}});
   HIP_OR_FAIL(ctx, hipMemcpyAsync(in, st, in_bytes, hipMemcpyHostToDevice, ctx->stream), "bm_meet_host: copy");
   (void)hipGetLastError();
   HIP_OR_FAIL(ctx, hipEventRecord(ctx->begin, ctx->stream), "bm_meet_host: hipEventRecord");
@@ -822,14 +858,12 @@ int bm_meet_host(bm_context* ctx, int32_t n, const double* a, const double* b, c
   char* back = st + in_bytes;
   HIP_OR_FAIL(ctx, hipMemcpyAsync(back, out, 280 * N, hipMemcpyDeviceToHost, ctx->stream), "bm_meet_host: copy");
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_meet_host: kernel");
-  std::memcpy(normal, back, 24 * N);
-  std::memcpy(points, back + 24 * N, 96 * N);
-  std::memcpy(separations, back + 120 * N, 32 * N);
-  std::memcpy(shift, back + 152 * N, 96 * N);
-  std::memcpy(count, back + 248 * N, 4 * N);
-  std::memcpy(flags, back + 252 * N, 4 * N);
-  std::memcpy(ids, back + 256 * N, 16 * N);
-  if (loss) std::memcpy(loss, back + 272 * N, 8 * N);
+  std::vector<HostCopy> outs = {{normal, back, 24 * N}, {points, back + 24 * N, 96 * N},
+                                {separations, back + 120 * N, 32 * N}, {shift, back + 152 * N, 96 * N},
+                                {count, back + 248 * N, 4 * N}, {flags, back + 252 * N, 4 * N},
+                                {ids, back + 256 * N, 16 * N}};
+  if (loss) outs.push_back({loss, back + 272 * N, 8 * N});
+  run_copies(outs);
   HIP_OR_FAIL(ctx, hipEventElapsedTime(&ctx->last_ms, ctx->begin, ctx->end), "bm_meet_host: hipEventElapsedTime");
   return BM_OK;
 }

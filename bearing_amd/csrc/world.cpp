@@ -2217,6 +2217,7 @@ struct bm_batch {
       level_points, level_seps;
   std::vector<uint32_t> level_ids;
   int level_cap = 8;
+  double times[BM_WORLD_TIMES] = {};  // as bm_world_times, for the batch's steps
 };
 
 namespace {
@@ -2235,23 +2236,24 @@ int run_merged(bm_batch* B) {
   for (int i = 0; i < nw; i++) at[i + 1] = at[i] + (int)W[i]->batch_pair.size() / 6;
   const int n = at[nw];
   if (n > 0) {
-    B->a.clear(); B->b.clear(); B->pair.clear();
-    for (bm_world* w : W) {
-      B->a.insert(B->a.end(), w->batch_a.begin(), w->batch_a.end());
-      B->b.insert(B->b.end(), w->batch_b.begin(), w->batch_b.end());
-      B->pair.insert(B->pair.end(), w->batch_pair.begin(), w->batch_pair.end());
-    }
+    B->a.resize(32 * (size_t)n); B->b.resize(32 * (size_t)n); B->pair.resize(6 * (size_t)n);
+    B->pool->run(nw, [&](int i) {
+      bm_world* w = W[i];
+      std::copy(w->batch_a.begin(), w->batch_a.end(), B->a.begin() + 32 * (size_t)at[i]);
+      std::copy(w->batch_b.begin(), w->batch_b.end(), B->b.begin() + 32 * (size_t)at[i]);
+      std::copy(w->batch_pair.begin(), w->batch_pair.end(), B->pair.begin() + 6 * (size_t)at[i]);
+    });
     B->count.resize(n); B->flags.resize(n); B->normal.resize(3 * (size_t)n); B->points.resize(12 * (size_t)n);
     B->seps.resize(4 * (size_t)n); B->ids.resize(4 * (size_t)n); B->shift.resize(12 * (size_t)n); B->loss.resize(n);
     if (int rc = bm_meet_host(B->ctx, n, B->a.data(), B->b.data(), B->pair.data(), B->count.data(), B->normal.data(),
                               B->points.data(), B->seps.data(), B->ids.data(), B->shift.data(), B->flags.data(),
                               B->loss.data()))
       return batch_fail(B, rc, std::string("step: ") + bm_last_error(B->ctx));
-    for (int i = 0; i < nw; i++) {
+    B->pool->run(nw, [&](int i) {
       bm_world* w = W[i];
       const int k0 = at[i], k = at[i + 1] - at[i];
       w->batch_a.clear(); w->batch_b.clear(); w->batch_pair.clear();
-      if (k == 0) continue;
+      if (k == 0) return;
       w->batch_count.assign(B->count.begin() + k0, B->count.begin() + k0 + k);
       w->batch_flags.assign(B->flags.begin() + k0, B->flags.begin() + k0 + k);
       w->batch_normal.assign(B->normal.begin() + 3 * k0, B->normal.begin() + 3 * (k0 + k));
@@ -2260,7 +2262,7 @@ int run_merged(bm_batch* B) {
       w->batch_ids.assign(B->ids.begin() + 4 * k0, B->ids.begin() + 4 * (k0 + k));
       w->batch_shift.assign(B->shift.begin() + 12 * k0, B->shift.begin() + 12 * (k0 + k));
       w->batch_loss.assign(B->loss.begin() + k0, B->loss.begin() + k0 + k);
-    }
+    });
   }
   std::vector<int> lat(nw + 1, 0);
   for (int i = 0; i < nw; i++) lat[i + 1] = lat[i] + (int)W[i]->level_field.size();
@@ -2654,15 +2656,24 @@ int bm_batch_step(bm_batch* B, double dt) {
   auto& W = B->worlds;
   const int n = (int)W.size();
   if (n == 0) return BM_OK;
+  using clock = std::chrono::steady_clock;
+  auto t = clock::now();
+  auto lap = [&](int k) {
+    const auto now = clock::now();
+    B->times[k] += std::chrono::duration<double, std::milli>(now - t).count();
+    t = now;
+  };
   if (B->tables_stale) {
     if (int rc = join_tables(B)) return rc;
   }
   std::vector<int> rcs(n, BM_OK);
   B->pool->run(n, [&](int i) { rcs[i] = step_begin(W[i], dt); });
   if (int rc = first_failure(B, rcs)) return rc;
+  lap(0);
   std::vector<uint8_t> active(n, 1);
   for (;;) {
     if (int rc = run_merged(B)) return rc;
+    lap(1);
     B->pool->run(n, [&](int i) {
       if (!active[i]) return;
       int rc = BM_OK;
@@ -2672,6 +2683,7 @@ int bm_batch_step(bm_batch* B, double dt) {
     if (int rc = first_failure(B, rcs)) return rc;
     bool any = false;
     for (int i = 0; i < n; i++) any = any || active[i];
+    lap(2);
     if (!any) break;
   }
   std::vector<bm_columns*> hs(n);
@@ -2680,17 +2692,26 @@ int bm_batch_step(bm_batch* B, double dt) {
     fill_columns(W[i], dt);
     hs[i] = &W[i]->h;
   });
+  lap(3);
   std::vector<const bm_columns*> ins(hs.begin(), hs.end());
   if (int rc = bm_upload_worlds(B->ctx, n, ins.data())) return batch_fail(B, rc, std::string("step: ") + bm_last_error(B->ctx));
   if (int rc = bm_colour(B->ctx)) return batch_fail(B, rc, std::string("step: ") + bm_last_error(B->ctx));
   if (int rc = bm_soft_step(B->ctx)) return batch_fail(B, rc, std::string("step: ") + bm_last_error(B->ctx));
   if (int rc = bm_download_worlds(B->ctx, n, hs.data()))
     return batch_fail(B, rc, std::string("step: ") + bm_last_error(B->ctx));
+  lap(4);
   B->pool->run(n, [&](int i) {
     store_memory(W[i], W[i]->h_solved);
     rcs[i] = step_end(W[i], dt);
   });
+  lap(5);
   return first_failure(B, rcs);
+}
+
+int bm_batch_times(const bm_batch* B, double* out) {
+  if (!B || !out) return BM_ERR_ARGUMENT;
+  for (int k = 0; k < BM_WORLD_TIMES; k++) out[k] = B->times[k];
+  return BM_OK;
 }
 
 int bm_world_read(const bm_world* w, bm_world_state* out) {

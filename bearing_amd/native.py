@@ -61,7 +61,8 @@ EXPORTS = ["bm_create", "bm_destroy", "bm_last_error", "bm_upload", "bm_soft_ste
            "bm_set_fields", "bm_level_host", "bm_set_wide", "bm_set_device_only",
            "bm_world_create", "bm_world_destroy", "bm_world_error", "bm_world_add", "bm_world_step", "bm_world_read",
            "bm_world_set_hulls", "bm_world_set_fields", "bm_world_times", "bm_upload_worlds", "bm_download_worlds",
-           "bm_batch_create", "bm_batch_destroy", "bm_batch_error", "bm_batch_world", "bm_batch_step"]
+           "bm_batch_create", "bm_batch_destroy", "bm_batch_error", "bm_batch_world", "bm_batch_step",
+           "bm_batch_times"]
 
 _lib = None
 
@@ -137,6 +138,7 @@ def world_signatures(lib):
     lib.bm_batch_world.argtypes = [vp, ctypes.POINTER(vp), _D, ctypes.c_int32, ctypes.c_int32]
     lib.bm_batch_step.argtypes = [vp, ctypes.c_double]
     lib.bm_world_times.argtypes = [vp, _D]
+    lib.bm_batch_times.argtypes = [vp, _D]
 
 
 class BearingError(RuntimeError):
@@ -484,6 +486,12 @@ class WorldBatch:
 
     def step(self, dt):
         self.check(self.lib.bm_batch_step(self.ptr, dt), "bm_batch_step")
+
+    def times(self):
+        """Milliseconds the batch's steps spent in each part (bm_batch_times), by name."""
+        out = np.zeros(len(World.TIMES))
+        self.check(self.lib.bm_batch_times(self.ptr, out.ctypes.data_as(_D)), "bm_batch_times")
+        return dict(zip(World.TIMES, out.tolist()))
 
     def close(self):
         if self.ptr:

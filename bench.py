@@ -110,10 +110,13 @@ def batched_world_run(count, warm, steps, threads):
     bodies = sum(int(fxs[i % len(fxs)].meta["bodies"]) for i in range(count))
     for _ in range(warm):
         batch.step(dt)
+    before = batch.times()
     t0 = time.perf_counter()
     for _ in range(steps):
         batch.step(dt)
     wall = time.perf_counter() - t0
+    after = batch.times()
+    parts = {k: (after[k] - before[k]) / steps for k in after}
     at = warm + steps
     exact = all(hashlib.sha256(w.read()["transforms"].tobytes()).hexdigest() == fxs[i % len(fxs)].meta["hashes"][at - 1]
                 for i, w in enumerate(worlds))
@@ -135,7 +138,7 @@ def batched_world_run(count, warm, steps, threads):
     return {"worlds": count, "bodies": bodies, "contacts_at_end": int(contacts), "warm_steps": warm, "steps": steps,
             "host_threads": threads, "ms_per_step": wall * 1000 / steps, "world_steps_per_s": rate,
             "looped_plain_world_steps_per_s": loop_rate, "speedup_vs_looped": rate / loop_rate,
-            "bit_exact": bool(exact), "worlds_from": BATCH_WORLDS_FROM}
+            "bit_exact": bool(exact), "ms_per_step_by_part": parts, "worlds_from": BATCH_WORLDS_FROM}
 
 
 def world_run(name, lib=None):

@@ -354,6 +354,8 @@ const char* bm_batch_error(const bm_batch* batch);
 int bm_batch_world(bm_batch* batch, bm_world** out, const double* gravity, int32_t substeps, int32_t sleeping);
 /* Every world of the batch, one step of dt. */
 int bm_batch_step(bm_batch* batch, double dt);
+/* As bm_world_times, for the batch's steps (host parts on the batch's threads, wall time). */
+int bm_batch_times(const bm_batch* batch, double* out);
 
 #ifdef __cplusplus
 }
