@@ -86,7 +86,9 @@ def cpu_baseline(fx, seconds):
             "sample": f"{done} soft steps of {WORKLOAD} ({rows} rows) in {elapsed:.2f} s on one host thread"}
 
 
-WORLD_RUNS = ["pyramid", "rain", "darts", "terrain"]
+WORLD_RUNS = ["pyramid", "rain", "darts", "terrain", "mixed_pile"]
+# Trajectories too long for the host-only port's leg (the 20,001-body pile: ~10 minutes).
+GPU_ONLY_RUNS = {"mixed_pile"}
 # Whole worlds stepped together: the reference's own resolved bodies of these trajectories
 # (13 to 401 bodies: towers, sphere piles, a pyramid, rods and balls, capsules, hulls, a
 # heightfield), repeated to the batch size.
@@ -154,6 +156,7 @@ def world_run(name, lib=None):
     fx = fixtures.load(os.path.join(fixtures.DATA, f"{name}.traj.bmfx.gz"))
     meta = fx.meta
     w = fill_world(native.World(0, lib=lib), fx)
+    before = w.times() if lib is None else None
     out = {"transforms": np.zeros(7 * int(meta["bodies"]), np.float32)}
     from bearing_amd.native import WorldState
     import ctypes
@@ -166,12 +169,16 @@ def world_run(name, lib=None):
         w.lib.bm_world_read(w.ptr, ctypes.byref(state))
         kept.append(out["transforms"].tobytes())
     wall = time.perf_counter() - t0
+    parts = None
+    if before is not None:
+        after = w.times()
+        parts = {k: (after[k] - before[k]) / meta["steps"] for k in after}
     w.close()
     exact = all(hashlib.sha256(t).hexdigest() == h for t, h in zip(kept, meta["hashes"]))
     awake = int(np.count_nonzero(fx["awake"]))
     return {"steps": meta["steps"], "bodies": meta["bodies"], "awake_steps": awake, "ms_per_step": wall * 1000 / meta["steps"],
             "ms_per_awake_step": wall * 1000 / max(awake, 1), "world_steps_per_s": meta["steps"] / wall,
-            "bit_exact": bool(exact)}
+            "bit_exact": bool(exact), "ms_per_step_by_part": parts}
 
 
 def main():
@@ -204,7 +211,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(worlds shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
+              "(worlds would shard across GPUs with no exchange between them)", file=sys.stderr)
         sys.exit(2)
 
     if args.world:
@@ -246,6 +253,12 @@ def main():
     value = rows * 2 * SUBSTEPS * args.steps / wall
     avg_kernel_ms = float(np.mean(kernel_ms))
     achieved = step_bytes / (avg_kernel_ms * 1e-3) / 1e9
+    # The same world stepped by the whole grid (bm_set_wide) for comparison: same bits.
+    ctx.set_wide(1)
+    for _ in range(2):
+        batch.step(ctx)
+    wide_ms = float(np.mean([batch.step(ctx) for _ in range(10)]))
+    ctx.set_wide(16384)
 
     # Batched: worlds of five scenes, repeated to --batch-worlds, in one launch.
     sources = [load_world(n)[:2] for n in BATCH_SOURCES]
@@ -286,7 +299,8 @@ def main():
     if os.path.exists(host_lib):
         lib = ctypes.CDLL(host_lib)
         for name in WORLD_RUNS:
-            worlds_run[name]["cpu_port_ms_per_step"] = world_run(name, lib)["ms_per_step"]
+            if name not in GPU_ONLY_RUNS:
+                worlds_run[name]["cpu_port_ms_per_step"] = world_run(name, lib)["ms_per_step"]
 
     cpu = cpu_baseline(fx, args.cpu_seconds)
     # HBM bytes per launch of the plain kernel, from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs
@@ -305,7 +319,8 @@ def main():
         "data": "synthetic: the reference's own state of the seeded 2,000-sphere rain before its step-300 solve",
         "config": {"workload": "rain-2000 soft step (inertias..store, 8 substeps), plain drop-in, one world",
                    "bodies": meta["top"], "manifolds": solved, "rows": rows, "colours": int(np.count_nonzero(np.diff(fx["in.starts"][:33]))),
-                   "kernel_ms": avg_kernel_ms, "launch": {"shape": shape, "resident_lds": bool(resident)}},
+                   "kernel_ms": avg_kernel_ms, "launch": {"shape": shape, "resident_lds": bool(resident)},
+                   "grid_wide_kernel_ms": wide_ms},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": step_bytes},

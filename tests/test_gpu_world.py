@@ -50,7 +50,7 @@ def test_gpu_batch_steps_every_world_as_alone():
 
 
 def test_gpu_mixed_pile_first_steps():
-    # configs[2] at full size (20,001 bodies: boxes, balls, capsules and hulls onto a 256 x 256
+    # The mixed pile at full size (20,001 bodies: boxes, balls, capsules and hulls onto a 256 x 256
     # heightfield), its first 150 steps as the landing starts, every step's transforms the
     # reference's.
     print("trajectory mixed_pile, 150 steps", flush=True)
