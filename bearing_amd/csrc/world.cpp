@@ -2574,6 +2574,7 @@ int bm_world_set_fields(bm_world* w, int32_t n, const int32_t* columns, const in
 // src/world.js:880-1072 (step), for the bodies above.
 int bm_world_step(bm_world* w, double dt) {
   if (!w) return BM_ERR_ARGUMENT;
+  if (!w->owns_ctx) return fail(w, BM_ERR_ARGUMENT, "step: the world steps with its batch (bm_batch_step)");
   using clock = std::chrono::steady_clock;
   auto t = clock::now();
   auto lap = [&](int k) {

@@ -55,4 +55,7 @@ def test_host_batch_steps_every_world_as_alone(lib):
     matched, why = run_together(batch, worlds, fxs, steps)
     assert why is None, why
     assert matched == steps
+    # A batch's world steps only with its batch.
+    with pytest.raises(native.BearingError, match="steps with its batch"):
+        worlds[0].step(1 / 60)
     batch.close()
