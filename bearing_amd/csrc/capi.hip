@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/bearing_mi355x.h"
+#include "adjoint_dev.h"
 #include "collide.h"
 #include "colour.h"
 #include "soft.h"
@@ -118,6 +119,7 @@ struct bm_context {
   std::vector<int32_t> join_body, join_manifold, join_pair, join_awake;  // the joined offset tables
   size_t colour_lds_limit = 0;  // dynamic LDS a colouring workgroup may take (0: not yet asked)
   int32_t colour_in_lds = 1;    // colour worlds that fit in LDS there
+  bm::HeldDevice* held = nullptr;  // a held trajectory (bm_held_load), for the adjoint
   SoftDev dev{};
   bool uploaded = false;
   float last_ms = 0;
@@ -394,6 +396,7 @@ int bm_destroy(bm_context* ctx) {
     if (s->p) (void)hipFree(s->p);
   for (Scratch* s : {&ctx->hull_ints, &ctx->hull_doubles})
     if (s->p) (void)hipFree(s->p);
+  bm::held_free(ctx->held);
   if (ctx->begin) (void)hipEventDestroy(ctx->begin);
   if (ctx->end) (void)hipEventDestroy(ctx->end);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -402,6 +405,30 @@ int bm_destroy(bm_context* ctx) {
 }
 
 const char* bm_last_error(const bm_context* ctx) { return ctx ? ctx->error.c_str() : "bm: no context"; }
+
+int bm_held_load(bm_context* ctx, const bm_held* held) {
+  if (!ctx) return BM_ERR_ARGUMENT;
+  return bm::held_load(ctx->device, held, &ctx->held, ctx->error);
+}
+
+int bm_held_forward(bm_context* ctx, const double* state0, const double* mass, const double* friction,
+                    const double* restitution, double* state_out) {
+  if (!ctx || !state0 || !mass || !friction || !restitution || !state_out) return BM_ERR_ARGUMENT;
+  return bm::held_forward(ctx->held, ctx->stream, state0, mass, friction, restitution, state_out, ctx->error);
+}
+
+int bm_held_backward(bm_context* ctx, const double* state_bar, double* state0_bar, double* mass_bar,
+                     double* friction_bar, double* restitution_bar) {
+  if (!ctx || !state_bar || !state0_bar || !mass_bar || !friction_bar || !restitution_bar) return BM_ERR_ARGUMENT;
+  return bm::held_backward(ctx->held, ctx->stream, state_bar, state0_bar, mass_bar, friction_bar, restitution_bar,
+                           ctx->error);
+}
+
+int bm_held_times(const bm_context* ctx, double* forward_ms, double* backward_ms) {
+  if (!ctx) return BM_ERR_ARGUMENT;
+  bm::held_times(ctx->held, forward_ms, backward_ms);
+  return BM_OK;
+}
 
 double bm_last_kernel_ms(const bm_context* ctx) { return ctx ? ctx->last_ms : 0.0; }
 

@@ -1,0 +1,186 @@
+// Where a held trajectory's columns lie in one buffer (the device's, or in the tests the
+// host's), and which of them come from the bm_held description. Host code.
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+#include <cstring>
+#include <vector>
+
+#include "../../include/bearing_mi355x.h"
+#include "adjoint.h"
+
+namespace bm {
+namespace adj {
+
+struct Copy {
+  void* to;
+  const void* from;
+  size_t bytes;
+};
+
+// The per-body parameters and state the forward and backward passes take and give.
+struct HeldIO {
+  double *state0, *mass_in, *friction_in, *restitution_in, *state_out;   // state: BM_HELD_STATE a body
+  double *state_bar, *state0_bar, *mass_bar, *friction_bar, *restitution_bar;
+  double *left0, *ceiling0;
+};
+
+// The most rows any segment of `every` steps has.
+inline int segment_rows(const bm_held& h) {
+  int most = 0;
+  for (int t0 = 0; t0 < h.steps; t0 += h.every) {
+    const int t1 = t0 + h.every < h.steps ? t0 + h.every : h.steps;
+    const int rows = h.row_offset[t1 * h.worlds] - h.row_offset[t0 * h.worlds];
+    if (rows > most) most = rows;
+  }
+  return most;
+}
+
+// Lays the trajectory's columns out from `base` (nullptr: only measure), returning the
+// bytes taken; fills T and io with the columns' addresses and `copies` with what comes
+// from h.
+inline size_t lay_out(const bm_held& h, char* base, Trajectory& T, HeldIO& io, std::vector<Copy>* copies) {
+  size_t at = 0;
+  auto take = [&](size_t bytes) -> char* {
+    at = (at + 255) & ~(size_t)255;
+    char* p = base ? base + at : nullptr;
+    at += bytes;
+    return p;
+  };
+  auto from = [&](const void* src, size_t bytes) -> char* {
+    char* p = take(bytes);
+    if (copies && base && bytes) copies->push_back(Copy{p, src, bytes});
+    return p;
+  };
+  const size_t nb = h.bodies, nw = h.worlds, rows = h.rows;
+  int most_sub = 1;
+  for (int k = 0; k < h.worlds; k++)
+    if (h.substeps[k] > most_sub) most_sub = h.substeps[k];
+  const size_t work = segment_rows(h) > 0 ? segment_rows(h) : 1;
+  T.worlds = h.worlds; T.steps = h.steps; T.every = h.every; T.bodies = h.bodies;
+  T.substeps_most = most_sub; T.work_rows = (int)work;
+  auto i32 = [&](const int32_t* src, size_t n) { return reinterpret_cast<int*>(from(src, n * 4)); };
+  auto u8 = [&](const uint8_t* src, size_t n) { return reinterpret_cast<uint8_t*>(from(src, n)); };
+  auto f64 = [&](const double* src, size_t n) { return reinterpret_cast<double*>(from(src, n * 8)); };
+  auto d = [&](size_t n) { return reinterpret_cast<double*>(take(n * 8)); };
+  T.body_offset = i32(h.body_offset, nw + 1);
+  T.row_offset = i32(h.row_offset, (size_t)h.steps * nw + 1);
+  T.starts = i32(h.starts, (size_t)h.steps * nw * STARTS);
+  T.narrow = i32(h.narrow, rows);
+  T.substeps = i32(h.substeps, nw);
+  T.dt = f64(h.dt, nw);
+  T.gravity = f64(h.gravity, 3 * nw);
+  Bodies& B = T.B;
+  B.type = u8(h.type, nb); B.shape = u8(h.shape, nb); B.tumbles = u8(h.tumbles, nb); B.capped = u8(h.capped, nb);
+  B.rounded = f64(h.rounded, nb);
+  T.Z.segment = f64(h.segment, nb);
+  B.gravity_scale = f64(h.gravity_scale, nb); B.damping = f64(h.damping, nb);
+  B.angular_damping = f64(h.angular_damping, nb);
+  B.mass0 = f64(h.mass0, nb); B.inv_mass0 = f64(h.inv_mass0, nb);
+  B.inertia0 = f64(h.inertia0, 3 * nb); B.inv_inertia0 = f64(h.inv_inertia0, 3 * nb);
+  io.left0 = f64(h.left, 3 * nb); io.ceiling0 = f64(h.ceiling, nb);
+  io.mass_in = d(nb); io.friction_in = d(nb); io.restitution_in = d(nb);
+  B.mass = io.mass_in; B.friction = io.friction_in; B.restitution = io.restitution_in;
+  B.inv_mass = d(nb); B.inertia = d(3 * nb); B.inv_inertia = d(3 * nb);
+  B.p = d(3 * nb); B.q = d(4 * nb); B.v = d(3 * nb); B.w = d(3 * nb); B.left = d(3 * nb); B.ceiling = d(nb);
+  B.axes = nullptr; B.inv_i = d(6 * nb); B.start = d(3 * nb); B.start_q = d(4 * nb);
+  io.state0 = d(BM_HELD_STATE * nb); io.state_out = d(BM_HELD_STATE * nb);
+  io.state_bar = d(BM_HELD_STATE * nb); io.state0_bar = d(BM_HELD_STATE * nb);
+  io.mass_bar = d(nb); io.friction_bar = d(nb); io.restitution_bar = d(nb);
+  BodiesBar& BB = T.BB;
+  BB.p = d(3 * nb); BB.q = d(4 * nb); BB.v = d(3 * nb); BB.w = d(3 * nb); BB.left = d(3 * nb); BB.ceiling = d(nb);
+  BB.inv_i = d(6 * nb); BB.start = d(3 * nb); BB.start_q = d(4 * nb); BB.inv_mass = d(nb);
+  BB.inertia = d(3 * nb); BB.inv_inertia = d(3 * nb);
+  Rows& R = T.R;
+  R.a = i32(h.a, rows); R.b = i32(h.b, rows); R.src = i32(h.src, rows);
+  R.count = u8(h.count, rows);
+  R.pid = reinterpret_cast<const uint32_t*>(from(h.pid, (size_t)P * rows * 4));
+  R.local_normal = f64(h.local_normal, 3 * rows); R.ends = f64(h.ends, 6 * P * rows);
+  R.tolerance = f64(h.tolerance, rows); R.patch = f64(h.patch, rows);
+  // the stores: the first rows' from h, the rest written by the forward pass
+  auto stores = [&](const double* src, size_t per) {
+    double* p = d(per * rows);
+    if (copies && base && h.first) copies->push_back(Copy{p, src, per * h.first * 8});
+    return p;
+  };
+  R.store_impulse = stores(h.store_impulse, P);
+  R.store_friction = stores(h.store_friction, 3);
+  R.store_twist = stores(h.store_twist, 3);
+  R.store_slip = stores(h.store_slip, 3);
+  RowsBar& RB = T.RB;
+  RB.store_impulse = d(P * rows); RB.store_friction = d(3 * rows); RB.store_twist = d(3 * rows);
+  RB.store_slip = d(3 * rows); RB.mu = d(rows); RB.e = d(rows);
+  Work& W = T.W;
+  W.point = d(3 * P * work); W.separation = d(P * work); W.normal = d(3 * work); W.t1 = d(3 * work);
+  W.t2 = d(3 * work); W.mu = d(work); W.e = d(work); W.impulse = d(P * work); W.tangent = d(2 * work);
+  W.roll = d(2 * work); W.spin = d(work); W.drift = d(2 * work); W.most = d(P * work); W.relvel = d(P * work);
+  W.now = d(P * work); W.ra = d(3 * P * work); W.rb = d(3 * P * work); W.la = d(3 * P * work);
+  W.lb = d(3 * P * work); W.adjusted = d(P * work); W.nmass = d(P * work); W.jac = d(J * P * work);
+  W.block = d(P * P * work); W.lever = d(P * work); W.anchor_a = d(3 * work); W.anchor_b = d(3 * work);
+  W.fjac = d(FJ * work); W.tm = d(3 * work); W.rm = d(2 * work); W.sm = d(work); W.bounce = d(BOUNCE * P * work);
+  W.sliding = reinterpret_cast<uint8_t*>(take(work));
+  WorkBar& WB = T.WB;
+  WB.impulse = d(P * work); WB.tangent = d(2 * work); WB.roll = d(2 * work); WB.spin = d(work);
+  WB.drift = d(2 * work); WB.relvel = d(P * work); WB.jac = d(J * P * work); WB.fjac = d(FJ * work);
+  WB.tm = d(3 * work); WB.nmass = d(P * work); WB.block = d(P * P * work); WB.rm = d(2 * work); WB.sm = d(work);
+  WB.lever = d(P * work); WB.la = d(3 * P * work); WB.lb = d(3 * P * work); WB.adjusted = d(P * work);
+  WB.normal = d(3 * work); WB.t1 = d(3 * work); WB.t2 = d(3 * work); WB.point = d(3 * P * work);
+  WB.separation = d(P * work); WB.mu = d(work); WB.e = d(work); WB.anchor_a = d(3 * work);
+  WB.anchor_b = d(3 * work);
+  T.checkpoints = d((size_t)(h.steps / h.every + 1) * nb * STATE);
+  T.row_tape = d((size_t)2 * most_sub * work * ROW_TAPE);
+  T.body_tape = d((size_t)h.every * 2 * most_sub * nb * STATE);
+  return (at + 255) & ~(size_t)255;
+}
+
+// A world's bodies at the start of the forward pass: p q v w from state0, the tumble's
+// state as recorded, the mass columns from the mass.
+HD void start_world(const Trajectory& T, const HeldIO& io, int wld, int tid, int threads) {
+  const Bodies& B = T.B;
+  for (int g = T.body_offset[wld] + tid; g < T.body_offset[wld + 1]; g += threads) {
+    const double* s = io.state0 + (size_t)BM_HELD_STATE * g;
+    for (int k = 0; k < 3; k++) { B.p[3 * g + k] = s[k]; B.v[3 * g + k] = s[7 + k]; B.w[3 * g + k] = s[10 + k]; }
+    for (int k = 0; k < 4; k++) B.q[4 * g + k] = s[3 + k];
+    for (int k = 0; k < 3; k++) B.left[3 * g + k] = io.left0[3 * g + k];
+    B.ceiling[g] = io.ceiling0[g];
+    from_mass(B, g);
+  }
+}
+
+HD void end_world(const Trajectory& T, const HeldIO& io, int wld, int tid, int threads) {
+  const Bodies& B = T.B;
+  for (int g = T.body_offset[wld] + tid; g < T.body_offset[wld + 1]; g += threads) {
+    double* s = io.state_out + (size_t)BM_HELD_STATE * g;
+    for (int k = 0; k < 3; k++) { s[k] = B.p[3 * g + k]; s[7 + k] = B.v[3 * g + k]; s[10 + k] = B.w[3 * g + k]; }
+    for (int k = 0; k < 4; k++) s[3 + k] = B.q[4 * g + k];
+  }
+}
+
+// The backward pass's start: the final state's adjoints, every other adjoint 0.
+HD void start_bars(const Trajectory& T, const HeldIO& io, int wld, int tid, int threads) {
+  const BodiesBar& BB = T.BB;
+  for (int g = T.body_offset[wld] + tid; g < T.body_offset[wld + 1]; g += threads) {
+    const double* s = io.state_bar + (size_t)BM_HELD_STATE * g;
+    for (int k = 0; k < 3; k++) {
+      BB.p[3 * g + k] = s[k]; BB.v[3 * g + k] = s[7 + k]; BB.w[3 * g + k] = s[10 + k]; BB.left[3 * g + k] = 0;
+      BB.start[3 * g + k] = 0; BB.inertia[3 * g + k] = 0; BB.inv_inertia[3 * g + k] = 0;
+    }
+    for (int k = 0; k < 4; k++) { BB.q[4 * g + k] = s[3 + k]; BB.start_q[4 * g + k] = 0; }
+    for (int k = 0; k < 6; k++) BB.inv_i[6 * g + k] = 0;
+    BB.ceiling[g] = 0;
+    BB.inv_mass[g] = 0;
+  }
+}
+
+HD void end_bars(const Trajectory& T, const HeldIO& io, int wld, int tid, int threads) {
+  const BodiesBar& BB = T.BB;
+  for (int g = T.body_offset[wld] + tid; g < T.body_offset[wld + 1]; g += threads) {
+    double* s = io.state0_bar + (size_t)BM_HELD_STATE * g;
+    for (int k = 0; k < 3; k++) { s[k] = BB.p[3 * g + k]; s[7 + k] = BB.v[3 * g + k]; s[10 + k] = BB.w[3 * g + k]; }
+    for (int k = 0; k < 4; k++) s[3 + k] = BB.q[4 * g + k];
+  }
+}
+
+}  // namespace adj
+}  // namespace bm

@@ -11,18 +11,18 @@
 namespace bm {
 
 // Math.max: NaN if either is NaN, +0 over -0 (src/contact.js:1866 relies on it).
-__device__ __forceinline__ double js_max(double a, double b) {
+__host__ __device__ __forceinline__ double js_max(double a, double b) {
   if (a != a || b != b) return __builtin_nan("");
   if (a > b) return a;
   if (b > a) return b;
   return __builtin_signbit(a) ? b : a;
 }
 
-__device__ __forceinline__ double dsqrt(double x) { return __builtin_sqrt(x); }
-__device__ __forceinline__ double dabs(double x) { return __builtin_fabs(x); }
+__host__ __device__ __forceinline__ double dsqrt(double x) { return __builtin_sqrt(x); }
+__host__ __device__ __forceinline__ double dabs(double x) { return __builtin_fabs(x); }
 
 // fdlibm s_atan.c: reduce about 0, 0.5, 1, 1.5 or infinity, then an odd polynomial.
-__device__ __forceinline__ double js_atan(double x) {
+__host__ __device__ __forceinline__ double js_atan(double x) {
   // The arctangents of 0.5, 1, 1.5 and infinity, each in a high and a low part; picked
   // by branches rather than a table, which the compiler would put in scratch memory.
   const double HI3 = 1.57079632679489655800e+00, LO3 = 6.12323399573676603587e-17;
@@ -60,7 +60,7 @@ __device__ __forceinline__ double js_atan(double x) {
 }
 
 // fdlibm e_atan2.c: the quadrant from the signs (-0 included), then atan of |y/x|.
-__device__ __forceinline__ double js_atan2(double y, double x) {
+__host__ __device__ __forceinline__ double js_atan2(double y, double x) {
   const double PI = 3.1415926535897931160e+00, PI_LO = 1.2246467991473531772e-16;
   const double PI_O_2 = 1.5707963267948965580e+00, PI_O_4 = 7.8539816339744827900e-01;
   if (x != x || y != y) return x + y;
@@ -100,7 +100,7 @@ struct Reduced {
   double head, tail;
   int quadrant;
 };
-__device__ __forceinline__ Reduced reduce_pio2(double x) {
+__host__ __device__ __forceinline__ Reduced reduce_pio2(double x) {
   const double n0 = __builtin_floor(x * 6.36619772367581382433e-01);
   const double n = x * 6.36619772367581382433e-01 - n0 >= 0.5 ? n0 + 1 : n0;
   double r = x - n * 1.57079632673412561417e+00;
@@ -127,14 +127,14 @@ __device__ __forceinline__ Reduced reduce_pio2(double x) {
   out.quadrant = (int)((long long)n & 3);
   return out;
 }
-__device__ __forceinline__ double kernel_sin(double x, double y) {
+__host__ __device__ __forceinline__ double kernel_sin(double x, double y) {
   const double z = x * x, v = z * x;
   const double r = 8.33333333332248946124e-03 +
       z * (-1.98412698298579493134e-04 + z * (2.75573137070700676789e-06 +
       z * (-2.50507602534068634195e-08 + z * 1.58969099521155010221e-10)));
   return x - ((z * (0.5 * y - v * r) - y) - v * -1.66666666666666324348e-01);
 }
-__device__ __forceinline__ double kernel_cos(double x, double y) {
+__host__ __device__ __forceinline__ double kernel_cos(double x, double y) {
   const double z = x * x;
   const double r = z * (4.16666666666666019037e-02 + z * (-1.38888888888741095749e-03 +
       z * (2.48015872894767294178e-05 + z * (-2.75573143513906633035e-07 +
@@ -142,7 +142,7 @@ __device__ __forceinline__ double kernel_cos(double x, double y) {
   const double hz = 0.5 * z, w = 1 - hz;
   return w + (((1 - w) - hz) + (z * r - x * y));
 }
-__device__ __forceinline__ double js_sin(double x) {
+__host__ __device__ __forceinline__ double js_sin(double x) {
   const Reduced r = reduce_pio2(x);
   switch (r.quadrant) {
     case 0: return kernel_sin(r.head, r.tail);
@@ -151,7 +151,7 @@ __device__ __forceinline__ double js_sin(double x) {
     default: return -kernel_cos(r.head, r.tail);
   }
 }
-__device__ __forceinline__ double js_cos(double x) {
+__host__ __device__ __forceinline__ double js_cos(double x) {
   const Reduced r = reduce_pio2(x);
   switch (r.quadrant) {
     case 0: return kernel_cos(r.head, r.tail);
@@ -167,7 +167,7 @@ struct V3 {
   double x, y, z;
 };
 
-__device__ __forceinline__ V3 rotate(double qx, double qy, double qz, double qw, double x, double y, double z,
+__host__ __device__ __forceinline__ V3 rotate(double qx, double qy, double qz, double qw, double x, double y, double z,
                                      bool inverse) {
   const double s = inverse ? -1 : 1;
   const double ux = qx * s, uy = qy * s, uz = qz * s, w = qw;
@@ -179,7 +179,7 @@ __device__ __forceinline__ V3 rotate(double qx, double qy, double qz, double qw,
   return o;
 }
 
-__device__ __forceinline__ V3 rotate(const double* q, double x, double y, double z, bool inverse) {
+__host__ __device__ __forceinline__ V3 rotate(const double* q, double x, double y, double z, bool inverse) {
   return rotate(q[0], q[1], q[2], q[3], x, y, z, inverse);
 }
 

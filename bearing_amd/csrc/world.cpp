@@ -317,6 +317,33 @@ struct Contacts {
 
 }  // namespace
 
+// The steps a world records for the adjoint (bm_world_record): each step's solved
+// manifolds held on their bodies as REUSE keeps one (src/contact.js:540-570 remember):
+// each point's ends in their bodies' frames, the normal in a's frame; with the row order,
+// colours, narrow order and, for each, the row whose store its pair carried.
+struct Record {
+  bool on = false;
+  std::string broken;  // why the recorded steps cannot be held, if they cannot
+  int top = 0, steps = 0;
+  long catches = 0;
+  double dt = 0;
+  std::vector<double> initial, left, ceiling;  // the bodies at the start: p q v w; tumble state
+  // rows that hold what the pairs kept before recording began: count, ids and stores
+  std::vector<uint8_t> first_count;
+  std::vector<uint32_t> first_pid;
+  std::vector<double> first_impulse, first_friction, first_twist, first_slip;
+  std::vector<int32_t> row_offset;  // steps + 1, after the first rows
+  std::vector<int32_t> starts, narrow, a, b, src;
+  std::vector<uint8_t> count;
+  std::vector<uint32_t> pid;
+  std::vector<double> local_normal, ends, tolerance, patch;
+  std::unordered_map<uint64_t, int32_t> last;  // a pair of bodies: the row that stored its memory last
+  // this step's found manifolds, before the phase: held normal and ends, whether the pair
+  // carried anything
+  std::vector<double> pre_normal, pre_ends;
+  std::vector<uint8_t> pre_had;
+};
+
 struct bm_world {
   bm_context* ctx = nullptr;
   std::string error;
@@ -409,6 +436,7 @@ struct bm_world {
   int32_t h_starts[BM_STARTS], h_solved, h_substeps;
   double h_dt;
   bm_columns h;
+  Record record;
 };
 
 namespace {
@@ -2126,15 +2154,136 @@ void fill_columns(bm_world* w, double dt) {
   h.pair_twist = pr.twist.data(); h.pair_friction = pr.friction.data(); h.pair_slip = pr.slip.data();
 }
 
+// ---- recording for the adjoint ----
+
+uint64_t pair_key(int a, int b) {
+  const uint32_t lo = (uint32_t)(a < b ? a : b), hi = (uint32_t)(a < b ? b : a);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// The state the held trajectory starts from, and the pairs' memory then as first rows.
+void record_start(bm_world* w) {
+  Record& R = w->record;
+  R = Record{};
+  R.on = true;
+  R.top = w->top;
+  R.catches = w->catches;
+  R.row_offset.push_back(0);
+  for (int s = 0; s < w->top; s++) {
+    for (int k = 0; k < 3; k++) R.initial.push_back(w->position[3 * s + k]);
+    for (int k = 0; k < 4; k++) R.initial.push_back(w->rotation[4 * s + k]);
+    for (int k = 0; k < 3; k++) R.initial.push_back(w->velocity[3 * s + k]);
+    for (int k = 0; k < 3; k++) R.initial.push_back(w->angular[3 * s + k]);
+    for (int k = 0; k < 3; k++) R.left.push_back(w->tumbled[3 * s + k]);
+    R.ceiling.push_back(w->ceiling[s]);
+  }
+  const Pairs& pr = w->pairs;
+  for (int p = 0; p < pr.count; p++) {
+    if (pr.points[p] == 0) continue;
+    const int f = (int)R.first_count.size();
+    R.first_count.push_back(pr.points[p]);
+    for (int k = 0; k < P; k++) {
+      R.first_pid.push_back(pr.pid[P * p + k]);
+      R.first_impulse.push_back(pr.normal_impulse[P * p + k]);
+    }
+    for (int k = 0; k < 3; k++) {
+      R.first_friction.push_back(pr.friction[3 * p + k]);
+      R.first_twist.push_back(pr.twist[3 * p + k]);
+      R.first_slip.push_back(pr.slip[3 * p + k]);
+    }
+    R.last[pair_key(pr.a[p], pr.b[p])] = -2 - f;
+  }
+}
+
+// Before the phase: each found manifold held on its bodies (src/contact.js:540-570), and
+// whether its pair carried a memory into it.
+void record_before(bm_world* w, double dt) {
+  Record& R = w->record;
+  const Contacts& c = w->c;
+  if (R.steps == 0) R.dt = dt;
+  else if (dt != R.dt && R.broken.empty()) R.broken = "the recorded steps differ in dt";
+  if (w->top != R.top && R.broken.empty()) R.broken = "bodies were added while recording";
+  R.pre_normal.assign(3 * (size_t)c.count, 0);
+  R.pre_ends.assign(6 * P * (size_t)c.count, 0);
+  R.pre_had.assign(c.count, 0);
+  const double* pos = w->position.data();
+  const double* A = w->axes.data();
+  for (int m = 0; m < c.count; m++) {
+    if (c.sub[m] >= 0 && R.broken.empty()) R.broken = "a heightfield pair (the held step has no level pairs)";
+    const int a = c.a[m], b = c.b[m], a3 = 3 * a, b3 = 3 * b, a9 = 9 * a, b9 = 9 * b;
+    const double nx = c.normal[3 * m], ny = c.normal[3 * m + 1], nz = c.normal[3 * m + 2];
+    double* ln = &R.pre_normal[3 * m];
+    ln[0] = A[a9] * nx + A[a9 + 1] * ny + A[a9 + 2] * nz;
+    ln[1] = A[a9 + 3] * nx + A[a9 + 4] * ny + A[a9 + 5] * nz;
+    ln[2] = A[a9 + 6] * nx + A[a9 + 7] * ny + A[a9 + 8] * nz;
+    for (int k = 0; k < c.points[m]; k++) {
+      const int i = P * m + k;
+      double* L = &R.pre_ends[6 * i];
+      const double half = 0.5 * c.separation[i];
+      const double* pt = &c.point[3 * i];
+      const double* sh = &c.shift[3 * i];
+      const double ax = pt[0] - nx * half - pos[a3], ay = pt[1] - ny * half - pos[a3 + 1], az = pt[2] - nz * half - pos[a3 + 2];
+      const double bx = pt[0] + sh[0] + nx * half - pos[b3], by = pt[1] + sh[1] + ny * half - pos[b3 + 1],
+                   bz = pt[2] + sh[2] + nz * half - pos[b3 + 2];
+      L[0] = A[a9] * ax + A[a9 + 1] * ay + A[a9 + 2] * az;
+      L[1] = A[a9 + 3] * ax + A[a9 + 4] * ay + A[a9 + 5] * az;
+      L[2] = A[a9 + 6] * ax + A[a9 + 7] * ay + A[a9 + 8] * az;
+      L[3] = A[b9] * bx + A[b9 + 1] * by + A[b9 + 2] * bz;
+      L[4] = A[b9 + 3] * bx + A[b9 + 4] * by + A[b9 + 5] * bz;
+      L[5] = A[b9 + 6] * bx + A[b9 + 7] * by + A[b9 + 8] * bz;
+      cored(w, a, L);
+      cored(w, b, L + 3);
+    }
+    R.pre_had[m] = w->pairs.points[c.pair[m]] > 0 ? 1 : 0;
+  }
+}
+
+// After the phase: the solved rows in row order, their colours and narrow order, and
+// which row each pair's memory now comes from.
+void record_after(bm_world* w) {
+  Record& R = w->record;
+  const Contacts& c = w->c;
+  const int solved = w->h_solved, base = R.row_offset.back();
+  for (int k = 0; k < BM_STARTS; k++) R.starts.push_back(w->h_starts[k]);
+  for (int m = 0; m < c.count; m++)
+    if (c.row[m] < solved) R.narrow.push_back(c.row[m]);
+  for (int f = 0; f < solved; f++) {
+    const int m = c.order[f];
+    const int a = c.a[f], b = c.b[f];
+    R.a.push_back(a); R.b.push_back(b);
+    R.count.push_back(c.points[f]);
+    for (int k = 0; k < P; k++) R.pid.push_back(c.pid[P * f + k]);
+    for (int k = 0; k < 3; k++) R.local_normal.push_back(R.pre_normal[3 * m + k]);
+    for (int k = 0; k < 6 * P; k++) R.ends.push_back(R.pre_ends[6 * P * m + k]);
+    R.tolerance.push_back(c.tolerance[f]);
+    R.patch.push_back(c.patch[f]);
+    int src = -1;
+    if (R.pre_had[m]) {
+      auto it = R.last.find(pair_key(a, b));
+      if (it == R.last.end()) {
+        if (R.broken.empty()) R.broken = "a pair carried a memory the record does not hold";
+      } else {
+        src = it->second;
+      }
+    }
+    R.src.push_back(src);
+  }
+  for (int f = 0; f < solved; f++) R.last[pair_key(c.a[f], c.b[f])] = base + f;
+  R.row_offset.push_back(base + solved);
+  R.steps++;
+}
+
 // The device phase: colour, rest, inertias .. store (src/world.js:956-1031), through the batch ABI.
 int device_phase(bm_world* w, double dt) {
   fill_columns(w, dt);
+  if (w->record.on) record_before(w, dt);
   bm_columns& h = w->h;
   if (int rc = bm_upload(w->ctx, &h)) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx));
   if (int rc = bm_colour(w->ctx)) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx));
   if (int rc = bm_soft_step(w->ctx)) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx));
   if (int rc = bm_download(w->ctx, &h)) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx));
   store_memory(w, w->h_solved);
+  if (w->record.on) record_after(w);
   return BM_OK;
 }
 
@@ -2691,6 +2840,7 @@ int bm_batch_step(bm_batch* B, double dt) {
   B->pool->run(n, [&](int i) {
     step_middle(W[i]);
     fill_columns(W[i], dt);
+    if (W[i]->record.on) record_before(W[i], dt);
     hs[i] = &W[i]->h;
   });
   lap(3);
@@ -2703,6 +2853,7 @@ int bm_batch_step(bm_batch* B, double dt) {
   lap(4);
   B->pool->run(n, [&](int i) {
     store_memory(W[i], W[i]->h_solved);
+    if (W[i]->record.on) record_after(W[i]);
     rcs[i] = step_end(W[i], dt);
   });
   lap(5);
@@ -2730,6 +2881,231 @@ int bm_world_read(const bm_world* w, bm_world_state* out) {
   if (out->velocity) std::memcpy(out->velocity, w->velocity.data(), 3 * sizeof(double) * w->top);
   if (out->angular) std::memcpy(out->angular, w->angular.data(), 3 * sizeof(double) * w->top);
   return BM_OK;
+}
+
+
+// ---- the adjoint: held trajectories of recorded worlds ----
+
+int bm_world_record(bm_world* w, int32_t on) {
+  if (!w) return BM_ERR_ARGUMENT;
+  if (!on) {
+    w->record.on = false;
+    return BM_OK;
+  }
+  if (w->sleeping) return fail(w, BM_ERR_UNSUPPORTED, "record: a recorded world steps with sleeping off");
+  record_start(w);
+  return BM_OK;
+}
+
+int bm_world_recorded(const bm_world* w) { return w ? w->record.steps : 0; }
+
+}  // extern "C"
+
+struct bm_adjoint {
+  bm_context* ctx = nullptr;
+  std::string error;
+  int bodies = 0, steps = 0;
+  std::vector<double> state0, mass, friction, restitution;
+};
+
+namespace {
+int adjoint_fail(bm_adjoint* a, int code, const std::string& m) {
+  a->error = m;
+  return code;
+}
+}  // namespace
+
+extern "C" {
+
+int bm_adjoint_create(bm_adjoint** out, int32_t device, bm_world* const* worlds, int32_t n, int32_t every) {
+  if (!out || !worlds || n <= 0 || every <= 0) return BM_ERR_ARGUMENT;
+  *out = nullptr;
+  auto* A = new bm_adjoint();
+  auto bad = [&](int code, const std::string& m) {
+    A->error = m;
+    *out = A;
+    return code;
+  };
+  const int T = worlds[0] ? worlds[0]->record.steps : 0;
+  for (int k = 0; k < n; k++) {
+    const bm_world* w = worlds[k];
+    if (!w) return bad(BM_ERR_ARGUMENT, "bm_adjoint_create: no world");
+    const Record& R = w->record;
+    const std::string which = "bm_adjoint_create: world " + std::to_string(k) + ": ";
+    if (R.steps == 0) return bad(BM_ERR_ARGUMENT, which + "nothing recorded (bm_world_record, then steps)");
+    if (R.steps != T) return bad(BM_ERR_ARGUMENT, which + "the worlds recorded different numbers of steps");
+    if (!R.broken.empty()) return bad(BM_ERR_UNSUPPORTED, which + R.broken);
+    if (w->catches != R.catches) return bad(BM_ERR_UNSUPPORTED, which + "a fast body was caught (the held step has no catch)");
+    for (int s = 0; s < R.top; s++)
+      if (w->lock[s] != 0 || w->joint_head[s] != 0)
+        return bad(BM_ERR_UNSUPPORTED, which + "locked axes or joints");
+  }
+  // bodies
+  std::vector<int32_t> body_offset(n + 1, 0), substeps(n);
+  std::vector<double> dt(n), gravity(3 * n);
+  for (int k = 0; k < n; k++) {
+    body_offset[k + 1] = body_offset[k] + worlds[k]->record.top;
+    substeps[k] = worlds[k]->substeps;
+    dt[k] = worlds[k]->record.dt;
+    for (int c = 0; c < 3; c++) gravity[3 * k + c] = worlds[k]->gravity[c];
+  }
+  const int nb = body_offset[n];
+  std::vector<uint8_t> type(nb), shape(nb), tumbles(nb), capped(nb);
+  std::vector<double> rounded_(nb), segment(nb), gscale(nb), damping(nb), adamping(nb), mass0(nb), inv_mass0(nb),
+      inertia0(3 * nb), inv_inertia0(3 * nb), left(3 * nb), ceiling(nb), friction(nb), restitution(nb), state0;
+  for (int k = 0; k < n; k++) {
+    const bm_world* w = worlds[k];
+    const Record& R = w->record;
+    for (int s = 0; s < R.top; s++) {
+      const int g = body_offset[k] + s;
+      type[g] = w->type[s];
+      shape[g] = w->shape[s];
+      const double* I = &w->inertia[3 * s];
+      tumbles[g] = w->type[s] == DYNAMIC && (I[0] != I[1] || I[1] != I[2]);
+      capped[g] = w->shape[s] != SPHERE || w->framed[s] == 1;
+      rounded_[g] = rounded(w, s);
+      segment[g] = w->shape[s] == CAPSULE ? (w->dims[3 * s + 1] - 2 * w->dims[3 * s]) * 0.5 : 0;
+      gscale[g] = w->gravity_scale[s];
+      damping[g] = w->damping[s];
+      adamping[g] = w->angular_damping[s];
+      mass0[g] = w->mass[s];
+      inv_mass0[g] = w->inv_mass[s];
+      for (int c = 0; c < 3; c++) {
+        inertia0[3 * g + c] = w->inertia[3 * s + c];
+        inv_inertia0[3 * g + c] = w->inv_inertia[3 * s + c];
+        left[3 * g + c] = R.left[3 * s + c];
+      }
+      ceiling[g] = R.ceiling[s];
+      friction[g] = w->friction[s];
+      restitution[g] = w->restitution[s];
+    }
+    state0.insert(state0.end(), R.initial.begin(), R.initial.end());
+  }
+  // rows: every world's first rows, then step by step, world by world
+  std::vector<int32_t> first_base(n + 1, 0);
+  for (int k = 0; k < n; k++) first_base[k + 1] = first_base[k] + (int)worlds[k]->record.first_count.size();
+  const int first = first_base[n];
+  std::vector<int32_t> row_offset(T * n + 1);
+  {
+    int at = first;
+    for (int t = 0; t < T; t++)
+      for (int k = 0; k < n; k++) {
+        row_offset[t * n + k] = at;
+        const Record& R = worlds[k]->record;
+        at += R.row_offset[t + 1] - R.row_offset[t];
+      }
+    row_offset[T * n] = at;
+  }
+  const int rows = row_offset[T * n];
+  std::vector<int32_t> a(rows, 0), b(rows, 0), src(rows, -1), starts((size_t)T * n * BM_STARTS), narrow(rows, 0);
+  std::vector<uint8_t> count(rows, 0);
+  std::vector<uint32_t> pid((size_t)P * rows, 0);
+  std::vector<double> ln(3 * (size_t)rows, 0), ends(6 * P * (size_t)rows, 0), tol(rows, 0), patch(rows, 0);
+  std::vector<double> st_imp((size_t)P * first, 0), st_fr(3 * (size_t)first, 0), st_tw(3 * (size_t)first, 0),
+      st_sl(3 * (size_t)first, 0);
+  for (int k = 0; k < n; k++) {
+    const Record& R = worlds[k]->record;
+    for (int f = 0; f < (int)R.first_count.size(); f++) {
+      const int g = first_base[k] + f;
+      count[g] = R.first_count[f];
+      for (int c = 0; c < P; c++) { pid[P * g + c] = R.first_pid[P * f + c]; st_imp[P * g + c] = R.first_impulse[P * f + c]; }
+      for (int c = 0; c < 3; c++) {
+        st_fr[3 * g + c] = R.first_friction[3 * f + c];
+        st_tw[3 * g + c] = R.first_twist[3 * f + c];
+        st_sl[3 * g + c] = R.first_slip[3 * f + c];
+      }
+    }
+    // a world's own row numbers (steps one after another) onto the batch's
+    std::vector<int32_t> global(R.a.size());
+    for (int t = 0; t < T; t++)
+      for (int f = R.row_offset[t]; f < R.row_offset[t + 1]; f++) global[f] = row_offset[t * n + k] + f - R.row_offset[t];
+    for (int t = 0; t < T; t++) {
+      for (int c = 0; c < BM_STARTS; c++) starts[((size_t)t * n + k) * BM_STARTS + c] = R.starts[(size_t)t * BM_STARTS + c];
+      for (int f = R.row_offset[t]; f < R.row_offset[t + 1]; f++) {
+        const int g = global[f];
+        a[g] = R.a[f]; b[g] = R.b[f]; count[g] = R.count[f];
+        narrow[g] = R.narrow[f];
+        const int s = R.src[f];
+        src[g] = s >= 0 ? global[s] : s <= -2 ? first_base[k] + (-2 - s) : -1;
+        for (int c = 0; c < P; c++) pid[P * g + c] = R.pid[P * f + c];
+        for (int c = 0; c < 3; c++) ln[3 * g + c] = R.local_normal[3 * f + c];
+        for (int c = 0; c < 6 * P; c++) ends[6 * P * (size_t)g + c] = R.ends[6 * P * (size_t)f + c];
+        tol[g] = R.tolerance[f];
+        patch[g] = R.patch[f];
+      }
+    }
+  }
+  bm_held h{};
+  h.worlds = n; h.steps = T; h.every = every; h.bodies = nb; h.rows = rows; h.first = first;
+  h.body_offset = body_offset.data(); h.row_offset = row_offset.data(); h.starts = starts.data();
+  h.narrow = narrow.data(); h.substeps = substeps.data(); h.dt = dt.data(); h.gravity = gravity.data();
+  h.type = type.data(); h.shape = shape.data(); h.tumbles = tumbles.data(); h.capped = capped.data();
+  h.rounded = rounded_.data(); h.segment = segment.data(); h.gravity_scale = gscale.data(); h.damping = damping.data();
+  h.angular_damping = adamping.data(); h.mass0 = mass0.data(); h.inv_mass0 = inv_mass0.data();
+  h.inertia0 = inertia0.data(); h.inv_inertia0 = inv_inertia0.data(); h.left = left.data(); h.ceiling = ceiling.data();
+  h.a = a.data(); h.b = b.data(); h.src = src.data(); h.count = count.data(); h.pid = pid.data();
+  h.local_normal = ln.data(); h.ends = ends.data(); h.tolerance = tol.data(); h.patch = patch.data();
+  h.store_impulse = st_imp.data(); h.store_friction = st_fr.data(); h.store_twist = st_tw.data();
+  h.store_slip = st_sl.data();
+  if (int rc = bm_create(&A->ctx, device)) return bad(rc, "bm_adjoint_create: no device context");
+  if (int rc = bm_held_load(A->ctx, &h)) return bad(rc, std::string("bm_adjoint_create: ") + bm_last_error(A->ctx));
+  A->bodies = nb;
+  A->steps = T;
+  A->state0 = std::move(state0);
+  A->mass = std::move(mass0);
+  A->friction = std::move(friction);
+  A->restitution = std::move(restitution);
+  *out = A;
+  return BM_OK;
+}
+
+int bm_adjoint_destroy(bm_adjoint* a) {
+  if (!a) return BM_OK;
+  if (a->ctx) bm_destroy(a->ctx);
+  delete a;
+  return BM_OK;
+}
+
+const char* bm_adjoint_error(const bm_adjoint* a) { return a ? a->error.c_str() : "bm_adjoint: none"; }
+
+int bm_adjoint_size(const bm_adjoint* a, int32_t* bodies, int32_t* steps) {
+  if (!a) return BM_ERR_ARGUMENT;
+  if (bodies) *bodies = a->bodies;
+  if (steps) *steps = a->steps;
+  return BM_OK;
+}
+
+int bm_adjoint_recorded(const bm_adjoint* a, double* state0, double* mass, double* friction, double* restitution) {
+  if (!a || !a->ctx) return BM_ERR_ARGUMENT;
+  if (state0) std::memcpy(state0, a->state0.data(), a->state0.size() * sizeof(double));
+  if (mass) std::memcpy(mass, a->mass.data(), a->mass.size() * sizeof(double));
+  if (friction) std::memcpy(friction, a->friction.data(), a->friction.size() * sizeof(double));
+  if (restitution) std::memcpy(restitution, a->restitution.data(), a->restitution.size() * sizeof(double));
+  return BM_OK;
+}
+
+int bm_adjoint_forward(bm_adjoint* a, const double* state0, const double* mass, const double* friction,
+                       const double* restitution, double* state_out) {
+  if (!a || !a->ctx || !state0 || !mass || !friction || !restitution || !state_out) return BM_ERR_ARGUMENT;
+  a->error.clear();
+  if (int rc = bm_held_forward(a->ctx, state0, mass, friction, restitution, state_out))
+    return adjoint_fail(a, rc, std::string("forward: ") + bm_last_error(a->ctx));
+  return BM_OK;
+}
+
+int bm_adjoint_backward(bm_adjoint* a, const double* state_bar, double* state0_bar, double* mass_bar,
+                        double* friction_bar, double* restitution_bar) {
+  if (!a || !a->ctx || !state_bar || !state0_bar || !mass_bar || !friction_bar || !restitution_bar)
+    return BM_ERR_ARGUMENT;
+  a->error.clear();
+  if (int rc = bm_held_backward(a->ctx, state_bar, state0_bar, mass_bar, friction_bar, restitution_bar))
+    return adjoint_fail(a, rc, std::string("backward: ") + bm_last_error(a->ctx));
+  return BM_OK;
+}
+
+int bm_adjoint_times(const bm_adjoint* a, double* forward_ms, double* backward_ms) {
+  if (!a || !a->ctx) return BM_ERR_ARGUMENT;
+  return bm_held_times(a->ctx, forward_ms, backward_ms);
 }
 
 }  // extern "C"

@@ -62,7 +62,9 @@ EXPORTS = ["bm_create", "bm_destroy", "bm_last_error", "bm_upload", "bm_soft_ste
            "bm_world_create", "bm_world_destroy", "bm_world_error", "bm_world_add", "bm_world_step", "bm_world_read",
            "bm_world_set_hulls", "bm_world_set_fields", "bm_world_times", "bm_upload_worlds", "bm_download_worlds",
            "bm_batch_create", "bm_batch_destroy", "bm_batch_error", "bm_batch_world", "bm_batch_step",
-           "bm_batch_times"]
+           "bm_batch_times", "bm_world_record", "bm_world_recorded", "bm_adjoint_create", "bm_adjoint_destroy",
+           "bm_adjoint_error", "bm_adjoint_size", "bm_adjoint_recorded", "bm_adjoint_forward", "bm_adjoint_backward",
+           "bm_adjoint_times", "bm_held_load", "bm_held_forward", "bm_held_backward", "bm_held_times"]
 
 _lib = None
 
@@ -139,6 +141,18 @@ def world_signatures(lib):
     lib.bm_batch_step.argtypes = [vp, ctypes.c_double]
     lib.bm_world_times.argtypes = [vp, _D]
     lib.bm_batch_times.argtypes = [vp, _D]
+    lib.bm_world_record.argtypes = [vp, ctypes.c_int32]
+    lib.bm_world_recorded.argtypes = [vp]
+    lib.bm_adjoint_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int32, ctypes.POINTER(vp), ctypes.c_int32,
+                                      ctypes.c_int32]
+    lib.bm_adjoint_destroy.argtypes = [vp]
+    lib.bm_adjoint_error.argtypes = [vp]
+    lib.bm_adjoint_error.restype = ctypes.c_char_p
+    lib.bm_adjoint_size.argtypes = [vp, _I32, _I32]
+    lib.bm_adjoint_recorded.argtypes = [vp, _D, _D, _D, _D]
+    lib.bm_adjoint_forward.argtypes = [vp, _D, _D, _D, _D, _D]
+    lib.bm_adjoint_backward.argtypes = [vp, _D, _D, _D, _D, _D]
+    lib.bm_adjoint_times.argtypes = [vp, _D, _D]
 
 
 class BearingError(RuntimeError):
@@ -379,6 +393,13 @@ class World:
         if rc != 0:
             raise BearingError(f"{what}: status {rc}: {self.lib.bm_world_error(self.ptr).decode()}")
 
+    def record(self, on=True):
+        """bm_world_record: keep the steps from now on for an Adjoint (sleeping must be off)."""
+        self.check(self.lib.bm_world_record(self.ptr, 1 if on else 0), "bm_world_record")
+
+    def recorded(self):
+        return int(self.lib.bm_world_recorded(self.ptr))
+
     def add(self, columns):
         """addBody for each body of `columns` (name -> array, names as BODY_COLUMNS)."""
         n = len(columns["type"])
@@ -505,3 +526,78 @@ class WorldBatch:
             self.close()
         except Exception:
             pass
+
+
+HELD_STATE = 13  # p 3, q 4, v 3, w 3 a body (BM_HELD_STATE)
+
+
+class Adjoint:
+    """The held trajectory of recorded worlds and its gradients (bm_adjoint_*): forward()
+    steps it from an initial state with given masses, frictions and restitutions;
+    backward() gives the adjoints of those from the final state's.
+
+    Arrays are by body slot, the worlds' slots one after another."""
+
+    def __init__(self, worlds, every=10, device=0, lib=None):
+        self.lib = lib if lib is not None else load()
+        if lib is not None:
+            world_signatures(lib)
+        self.ptr = ctypes.c_void_p()
+        arr = (ctypes.c_void_p * len(worlds))(*[w.ptr for w in worlds])
+        rc = self.lib.bm_adjoint_create(ctypes.byref(self.ptr), device, arr, len(worlds), every)
+        if rc != 0:
+            msg = self.lib.bm_adjoint_error(self.ptr).decode() if self.ptr else ""
+            self.lib.bm_adjoint_destroy(self.ptr)
+            self.ptr = None
+            raise BearingError(f"bm_adjoint_create: status {rc}: {msg}")
+        b, t = np.zeros(1, np.int32), np.zeros(1, np.int32)
+        self.lib.bm_adjoint_size(self.ptr, b.ctypes.data_as(_I32), t.ctypes.data_as(_I32))
+        self.bodies, self.steps = int(b[0]), int(t[0])
+
+    def check(self, rc, what):
+        if rc != 0:
+            raise BearingError(f"{what}: status {rc}: {self.lib.bm_adjoint_error(self.ptr).decode()}")
+
+    def recorded(self):
+        """The recorded initial state (bodies x 13) and each slot's mass, friction, restitution."""
+        n = self.bodies
+        out = {"state": np.zeros((n, HELD_STATE)), "mass": np.zeros(n), "friction": np.zeros(n),
+               "restitution": np.zeros(n)}
+        self.check(self.lib.bm_adjoint_recorded(self.ptr, *[out[k].ctypes.data_as(_D) for k in
+                                                          ("state", "mass", "friction", "restitution")]),
+                   "bm_adjoint_recorded")
+        return out
+
+    def forward(self, state, mass, friction, restitution):
+        n = self.bodies
+        ins = [np.ascontiguousarray(a, np.float64) for a in (state, mass, friction, restitution)]
+        if ins[0].size != n * HELD_STATE or any(a.size != n for a in ins[1:]):
+            raise ValueError("forward: arrays of the wrong size")
+        out = np.zeros((n, HELD_STATE))
+        self.check(self.lib.bm_adjoint_forward(self.ptr, *[a.ctypes.data_as(_D) for a in ins],
+                                               out.ctypes.data_as(_D)), "bm_adjoint_forward")
+        return out
+
+    def backward(self, state_bar):
+        n = self.bodies
+        sb = np.ascontiguousarray(state_bar, np.float64)
+        if sb.size != n * HELD_STATE:
+            raise ValueError("backward: state_bar of the wrong size")
+        out = {"state": np.zeros((n, HELD_STATE)), "mass": np.zeros(n), "friction": np.zeros(n),
+               "restitution": np.zeros(n)}
+        self.check(self.lib.bm_adjoint_backward(self.ptr, sb.ctypes.data_as(_D),
+                                                *[out[k].ctypes.data_as(_D) for k in
+                                                  ("state", "mass", "friction", "restitution")]),
+                   "bm_adjoint_backward")
+        return out
+
+    def times(self):
+        f, b = np.zeros(1), np.zeros(1)
+        self.check(self.lib.bm_adjoint_times(self.ptr, f.ctypes.data_as(_D), b.ctypes.data_as(_D)),
+                   "bm_adjoint_times")
+        return float(f[0]), float(b[0])
+
+    def close(self):
+        if self.ptr:
+            self.lib.bm_adjoint_destroy(self.ptr)
+            self.ptr = None

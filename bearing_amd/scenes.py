@@ -316,6 +316,67 @@ def mixed_pile(rng: SplitMix64, count: int = 20000, columns: int = 256, volume=(
     return bodies
 
 
+def resolve(bodies):
+    """addBody's resolved columns (bm_bodies, as native.World.add takes them) for scenes of
+    boxes and spheres, with the reference's defaults and arithmetic (src/world.js:2150-2245
+    and src/shapes.js:14-56: density 1000 unless a mass is given, friction 0.5, restitution
+    0; the thinnest size and the reach; inverses where positive)."""
+    n = len(bodies)
+    cols = {name: [] for name in ("type", "shape", "lock", "position", "rotation", "velocity", "angular", "dims",
+                                  "inertia", "inv_inertia", "mass", "inv_mass", "gravity_scale", "damping",
+                                  "angular_damping", "size", "reach", "friction", "restitution")}
+    for b in bodies:
+        dynamic = b.get("type", "dynamic") == "dynamic"
+        shape = b["shape"]
+        if shape == "box":
+            w, h, d = (float(x) for x in b["size"])
+            dims = [w, h, d]
+            volume = w * h * d
+        elif shape == "sphere":
+            r = float(b["radius"])
+            dims = [r, 0.0, 0.0]
+            volume = (4 / 3) * math.pi * r * r * r
+        else:
+            raise ValueError(f"resolve: a {shape} (boxes and spheres only)")
+        mass = float(b["mass"]) if "mass" in b else float(b.get("density", 1000.0)) * volume
+        if shape == "box":
+            inertia = [mass * (h * h + d * d) / 12, mass * (w * w + d * d) / 12, mass * (w * w + h * h) / 12]
+            size, reach = min(w, h, d), 0.5 * math.sqrt(w * w + h * h + d * d)
+        else:
+            inertia = [0.4 * mass * r * r] * 3
+            size, reach = 2 * r, r
+        cols["type"].append(0 if dynamic else 1)
+        cols["shape"].append(0 if shape == "box" else 1)
+        cols["lock"].append(0)
+        cols["position"] += [float(x) for x in b.get("position", [0.0, 0.0, 0.0])]
+        cols["rotation"] += [float(x) for x in b.get("rotation", [0.0, 0.0, 0.0, 1.0])]
+        cols["velocity"] += [float(x) for x in b.get("velocity", [0.0, 0.0, 0.0])]
+        cols["angular"] += [float(x) for x in b.get("angularVelocity", [0.0, 0.0, 0.0])]
+        cols["dims"] += dims
+        cols["inertia"] += inertia
+        cols["inv_inertia"] += [1 / i if i > 0 else 0.0 for i in inertia]
+        cols["mass"].append(mass)
+        cols["inv_mass"].append(1 / mass if dynamic else 0.0)
+        cols["gravity_scale"].append(float(b.get("gravityScale", 1.0)))
+        cols["damping"].append(float(b.get("linearDamping", 0.0)))
+        cols["angular_damping"].append(float(b.get("angularDamping", 0.0)))
+        cols["size"].append(size)
+        cols["reach"].append(reach)
+        cols["friction"].append(float(b.get("friction", 0.5)))
+        cols["restitution"].append(float(b.get("restitution", 0.0)))
+    import numpy as np
+    from .native import BODY_COLUMNS
+    dtypes = {name: dtype for name, _, dtype in BODY_COLUMNS}
+    return {k: np.asarray(v, dtypes[k]) for k, v in cols.items()}
+
+
+def throws(worlds: int = 512, seed: int = 5):
+    """The differentiation form's worlds: `worlds` thrown-box scenes (throw()), each from its
+    own draws of one seeded stream."""
+    rng = SplitMix64(seed)
+    return [throw(rng) for _ in range(worlds)]
+
+
 SCENES = {
     "pyramid": lambda: pyramid(),
     "rain": lambda: rain(),

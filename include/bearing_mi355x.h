@@ -357,6 +357,94 @@ int bm_batch_step(bm_batch* batch, double dt);
 /* As bm_world_times, for the batch's steps (host parts on the batch's threads, wall time). */
 int bm_batch_times(const bm_batch* batch, double* out);
 
+/* ---- The adjoint: gradients of a recorded trajectory ----------------------------------
+ *
+ * No counterpart in the reference, which has no derivatives. A world records the steps it
+ * takes (bm_world_record); each step's solved manifolds are kept held on their bodies, as
+ * the reference's REUSE keeps a manifold (src/contact.js:388-508): each point's end on a in
+ * a's frame and on b in b's frame, the normal in a's frame, the feature ids, the colours and
+ * the order the step found them in. A held trajectory (bm_adjoint_create) steps the recorded
+ * worlds again from their first recorded state with every manifold placed from where its
+ * bodies are (the REUSE rebuild), the impulses carried as the pairs carried them
+ * (src/contact.js:999-1036), and the reference's soft step (src/world.js:961-1031); at the
+ * recorded parameters it follows the recorded steps (bit for bit where the reference itself
+ * reused the manifold; where the reference found one afresh, up to the rounding of the
+ * rebuild; a manifold found at the moment of impact is held where its points were carried
+ * back). bm_adjoint_backward gives the gradient of any function of the final state with
+ * respect to the initial state and the bodies' mass, friction and restitution, on the
+ * forward pass's active set (Murty sets, friction cone, clamps, bounces): which manifolds a
+ * step has is held, where their points lie moves with the bodies.
+ *
+ * The forward pass keeps every body's state every `every` steps and every step's stored
+ * impulses; the backward pass steps each segment again from its first state writing a tape
+ * and walks it backwards (reverse colour sweeps, the integrator's adjoint), one workgroup a
+ * world. The recorded world must step with sleeping off (so nothing freezes or sleeps),
+ * without catches, heightfields or bodies added while recording; it may step alone or in a
+ * batch (bm_batch_step records each recording world of the batch).
+ *
+ * State per body slot: p 3, q 4, v 3, w 3 (BM_HELD_STATE doubles); slots of the worlds one
+ * after another, each world's as its own slots number them. */
+#define BM_HELD_STATE 13
+/* Starts (on != 0) or stops recording. Starting clears what was recorded. */
+int bm_world_record(bm_world* world, int32_t on);
+/* Steps recorded so far. */
+int bm_world_recorded(const bm_world* world);
+typedef struct bm_adjoint bm_adjoint;
+/* The held trajectory of n recorded worlds (each from its first recorded step to its last)
+ * on a device, keeping state every `every` steps. The worlds may be destroyed afterwards. */
+int bm_adjoint_create(bm_adjoint** out, int32_t device, bm_world* const* worlds, int32_t n, int32_t every);
+int bm_adjoint_destroy(bm_adjoint* a);
+const char* bm_adjoint_error(const bm_adjoint* a);
+/* Body slots (the worlds' together) and steps. */
+int bm_adjoint_size(const bm_adjoint* a, int32_t* bodies, int32_t* steps);
+/* The recorded values: initial state (BM_HELD_STATE a slot), mass, friction, restitution. */
+int bm_adjoint_recorded(const bm_adjoint* a, double* state0, double* mass, double* friction, double* restitution);
+/* The forward pass from state0 with the given parameters; the final state into state_out. */
+int bm_adjoint_forward(bm_adjoint* a, const double* state0, const double* mass, const double* friction,
+                       const double* restitution, double* state_out);
+/* After bm_adjoint_forward: from the final state's adjoint (dL/dstate), the adjoints of the
+ * initial state and of every slot's mass, friction and restitution (0 for static bodies'
+ * mass). */
+int bm_adjoint_backward(bm_adjoint* a, const double* state_bar, double* state0_bar, double* mass_bar,
+                        double* friction_bar, double* restitution_bar);
+/* Milliseconds the last forward and backward pass took on the device. */
+int bm_adjoint_times(const bm_adjoint* a, double* forward_ms, double* backward_ms);
+
+/* The held trajectory as the device takes it (bm_held_load); host pointers. Rows are every
+ * step's solved manifolds, step after step and within a step world after world, after
+ * `first` rows that only hold what the worlds' pairs kept before recording began. */
+typedef struct bm_held {
+  int32_t worlds, steps, every, bodies, rows, first;
+  const int32_t* body_offset; /* worlds + 1 */
+  const int32_t* row_offset;  /* steps * worlds + 1 */
+  const int32_t* starts;      /* BM_STARTS per (step, world), relative to its first row */
+  const int32_t* narrow;      /* per row: its (step, world)'s rows in narrow order, relative */
+  const int32_t* substeps;    /* per world */
+  const double* dt;           /* per world */
+  const double* gravity;      /* 3 per world */
+  /* per body */
+  const uint8_t* type, *shape, *tumbles, *capped;
+  const double* rounded, *segment, *gravity_scale, *damping, *angular_damping, *mass0, *inv_mass0, *inertia0,
+      *inv_inertia0;
+  const double* left, *ceiling; /* the tumble's own state at the start: 3, 1 */
+  /* per row */
+  const int32_t* a, *b, *src;
+  const uint8_t* count;
+  const uint32_t* pid;        /* 4 */
+  const double* local_normal; /* 3 */
+  const double* ends;         /* 24: per point a's end 3, b's end 3 */
+  const double* tolerance, *patch;
+  /* the stores of the first rows: impulses 4, friction 3, twist 3, slip 3 */
+  const double* store_impulse, *store_friction, *store_twist, *store_slip;
+} bm_held;
+int bm_held_load(bm_context* ctx, const bm_held* held);
+/* state0: BM_HELD_STATE a body; the rest one a body. */
+int bm_held_forward(bm_context* ctx, const double* state0, const double* mass, const double* friction,
+                    const double* restitution, double* state_out);
+int bm_held_backward(bm_context* ctx, const double* state_bar, double* state0_bar, double* mass_bar,
+                     double* friction_bar, double* restitution_bar);
+int bm_held_times(const bm_context* ctx, double* forward_ms, double* backward_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -7,6 +7,7 @@
 #include <string>
 #include <vector>
 
+#include "../../bearing_amd/csrc/held_layout.h"
 #include "../../include/bearing_mi355x.h"
 #include "../../oracle/oracle.h"
 
@@ -25,9 +26,18 @@ extern "C" int oracle_level(int n, const int32_t* field, const double* position,
 extern "C" void oracle_hulls(int n, const int32_t* hull_v, const int32_t* hull_f, const int32_t* hull_l,
                              const double* vertices, const int32_t* start, const int32_t* loops, const double* planes);
 
+// The held trajectory (bm_held_*), stepped on the host by the library's own adjoint code
+// (bearing_amd/csrc/adjoint.h), one world after another, as one thread.
+struct HeldHost {
+  std::vector<char> mem;
+  bm::adj::Trajectory T;
+  bm::adj::HeldIO io;
+};
+
 struct bm_context {
   std::vector<bm_columns> hs;
   std::string error;
+  HeldHost* held = nullptr;
 };
 
 namespace {
@@ -49,7 +59,74 @@ int bm_create(bm_context** out, int32_t) {
   return BM_OK;
 }
 int bm_destroy(bm_context* ctx) {
+  if (ctx) delete ctx->held;
   delete ctx;
+  return BM_OK;
+}
+
+int bm_held_load(bm_context* ctx, const bm_held* h) {
+  using namespace bm::adj;
+  delete ctx->held;
+  ctx->held = new HeldHost();
+  HeldHost& H = *ctx->held;
+  const size_t bytes = lay_out(*h, nullptr, H.T, H.io, nullptr);
+  H.mem.assign(bytes + 256, 0);
+  char* base = reinterpret_cast<char*>(((uintptr_t)H.mem.data() + 255) & ~(uintptr_t)255);
+  std::vector<Copy> copies;
+  lay_out(*h, base, H.T, H.io, &copies);
+  for (const Copy& c : copies) std::memcpy(c.to, c.from, c.bytes);
+  return BM_OK;
+}
+
+int bm_held_forward(bm_context* ctx, const double* state0, const double* mass, const double* friction,
+                    const double* restitution, double* state_out) {
+  using namespace bm::adj;
+  if (!ctx->held) return BM_ERR_ARGUMENT;
+  HeldHost& H = *ctx->held;
+  const int nb = H.T.bodies;
+  std::memcpy(H.io.state0, state0, sizeof(double) * BM_HELD_STATE * nb);
+  std::memcpy(H.io.mass_in, mass, sizeof(double) * nb);
+  std::memcpy(H.io.friction_in, friction, sizeof(double) * nb);
+  std::memcpy(H.io.restitution_in, restitution, sizeof(double) * nb);
+  const Ctx c{0, 1};
+  for (int w = 0; w < H.T.worlds; w++) {
+    start_world(H.T, H.io, w, 0, 1);
+    forward_world(c, [] {}, H.T, w);
+    end_world(H.T, H.io, w, 0, 1);
+  }
+  std::memcpy(state_out, H.io.state_out, sizeof(double) * BM_HELD_STATE * nb);
+  return BM_OK;
+}
+
+int bm_held_backward(bm_context* ctx, const double* state_bar, double* state0_bar, double* mass_bar,
+                     double* friction_bar, double* restitution_bar) {
+  using namespace bm::adj;
+  if (!ctx->held) return BM_ERR_ARGUMENT;
+  HeldHost& H = *ctx->held;
+  const Trajectory& T = H.T;
+  const int nb = T.bodies;
+  const size_t rows = T.row_offset[T.steps * T.worlds];
+  std::memcpy(H.io.state_bar, state_bar, sizeof(double) * BM_HELD_STATE * nb);
+  for (double* p : {T.RB.store_impulse}) std::memset(p, 0, sizeof(double) * P * rows);
+  for (double* p : {T.RB.store_friction, T.RB.store_twist, T.RB.store_slip}) std::memset(p, 0, sizeof(double) * 3 * rows);
+  for (double* p : {T.RB.mu, T.RB.e}) std::memset(p, 0, sizeof(double) * rows);
+  const Ctx c{0, 1};
+  for (int w = 0; w < T.worlds; w++) {
+    start_bars(T, H.io, w, 0, 1);
+    backward_world(c, [] {}, T, w);
+    end_bars(T, H.io, w, 0, 1);
+    parameters_world(T, w, H.io.mass_bar, H.io.friction_bar, H.io.restitution_bar);
+  }
+  std::memcpy(state0_bar, H.io.state0_bar, sizeof(double) * BM_HELD_STATE * nb);
+  std::memcpy(mass_bar, H.io.mass_bar, sizeof(double) * nb);
+  std::memcpy(friction_bar, H.io.friction_bar, sizeof(double) * nb);
+  std::memcpy(restitution_bar, H.io.restitution_bar, sizeof(double) * nb);
+  return BM_OK;
+}
+
+int bm_held_times(const bm_context*, double* forward_ms, double* backward_ms) {
+  if (forward_ms) *forward_ms = 0;
+  if (backward_ms) *backward_ms = 0;
   return BM_OK;
 }
 const char* bm_last_error(const bm_context* ctx) { return ctx->error.c_str(); }
