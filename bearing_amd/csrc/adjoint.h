@@ -374,6 +374,7 @@ struct Rows {
   const uint8_t* count;
   const uint32_t* pid;
   const double *local_normal, *ends, *tolerance, *patch;
+  const double* shift;  // per point: where b's end was carried from at the moment of impact (0 if not met so)
   // what each row stored at its step's end (src/contact.js:1091-1115): the carried impulses
   double *store_impulse, *store_friction, *store_twist, *store_slip;
 };
@@ -564,7 +565,8 @@ HD void prepare_row(const Bodies& B, const Sizes& Z, const Work& W, const Rows& 
     const int j = P * i + k, j3 = 3 * j;
     const double px = W.point[j3], py = W.point[j3 + 1], pz = W.point[j3 + 2];
     const double rax = px - pa[0], ray = py - pa[1], raz = pz - pa[2];
-    const double rbx = px - pb[0], rby = py - pb[1], rbz = pz - pb[2];
+    const double* sh = R.shift + 3 * (P * r + k);
+    const double rbx = px + sh[0] - pb[0], rby = py + sh[1] - pb[1], rbz = pz + sh[2] - pb[2];
     W.ra[j3] = rax; W.ra[j3 + 1] = ray; W.ra[j3 + 2] = raz;
     W.rb[j3] = rbx; W.rb[j3 + 1] = rby; W.rb[j3 + 2] = rbz;
     const V3 ca = core(B, Z, ga, rax, ray, raz, W.la + j3);
@@ -1109,6 +1111,15 @@ HD void bounce_row(const Bodies& B, const Work& W, const Rows& R, int boff, int 
   }
 }
 
+// Whether row r's manifold was met at the moment of impact (its points carried back).
+HD bool impacted(const Rows& R, int r) {
+  for (int k = 0; k < R.count[r]; k++) {
+    const double* sh = R.shift + 3 * (P * r + k);
+    if (sh[0] != 0 || sh[1] != 0 || sh[2] != 0) return true;
+  }
+  return false;
+}
+
 // src/contact.js:1122-1153 (slipped), 1091-1115 (keep) for row r: its slip, then what
 // its pair keeps for the next step.
 HD void store_row(const Bodies& B, const Work& W, const Rows& R, int boff, int r, int i) {
@@ -1118,7 +1129,7 @@ HD void store_row(const Bodies& B, const Work& W, const Rows& R, int boff, int r
   for (int k = 0; k < count; k++) pressed += W.impulse[P * i + k];
   const double t1x = W.t1[i3], t1y = W.t1[i3 + 1], t1z = W.t1[i3 + 2];
   const double t2x = W.t2[i3], t2y = W.t2[i3 + 1], t2z = W.t2[i3 + 2];
-  if (W.sliding[i] != 0 || pressed == 0) {
+  if (W.sliding[i] != 0 || pressed == 0 || impacted(R, r)) {
     W.drift[2 * i] = 0; W.drift[2 * i + 1] = 0;
   } else {
     double T[6];
@@ -1997,7 +2008,7 @@ HD void store_vjp(const Bodies& B, const BodiesBar& BB, const Work& W, const Wor
   }
   double pressed = 0;
   for (int k = 0; k < count; k++) pressed += W.impulse[P * i + k];
-  if (W.sliding[i] != 0 || pressed == 0) {
+  if (W.sliding[i] != 0 || pressed == 0 || impacted(R, r)) {
     WB.drift[2 * i] = 0; WB.drift[2 * i + 1] = 0;
     return;
   }
@@ -2142,7 +2153,7 @@ HD void prepare_vjp(const Bodies& B, const BodiesBar& BB, const Sizes& Z, const 
     const int j3 = 3 * (P * i + k);
     for (int c = 0; c < 3; c++) {
       ra[k][c] = W.point[j3 + c] - pa[c];
-      rb[k][c] = W.point[j3 + c] - pb[c];
+      rb[k][c] = W.point[j3 + c] + R.shift[3 * (P * r + k) + c] - pb[c];
       rab[k][c] = 0; rbb[k][c] = 0;
     }
     sepb[k] = 0;

@@ -98,6 +98,7 @@ inline size_t lay_out(const bm_held& h, char* base, Trajectory& T, HeldIO& io, s
   R.pid = reinterpret_cast<const uint32_t*>(from(h.pid, (size_t)P * rows * 4));
   R.local_normal = f64(h.local_normal, 3 * rows); R.ends = f64(h.ends, 6 * P * rows);
   R.tolerance = f64(h.tolerance, rows); R.patch = f64(h.patch, rows);
+  R.shift = f64(h.shift, 3 * P * rows);
   // the stores: the first rows' from h, the rest written by the forward pass
   auto stores = [&](const double* src, size_t per) {
     double* p = d(per * rows);

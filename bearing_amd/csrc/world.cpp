@@ -336,11 +336,11 @@ struct Record {
   std::vector<int32_t> starts, narrow, a, b, src;
   std::vector<uint8_t> count;
   std::vector<uint32_t> pid;
-  std::vector<double> local_normal, ends, tolerance, patch;
+  std::vector<double> local_normal, ends, shift, tolerance, patch;
   std::unordered_map<uint64_t, int32_t> last;  // a pair of bodies: the row that stored its memory last
   // this step's found manifolds, before the phase: held normal and ends, whether the pair
   // carried anything
-  std::vector<double> pre_normal, pre_ends;
+  std::vector<double> pre_normal, pre_ends, pre_shift;
   std::vector<uint8_t> pre_had;
 };
 
@@ -2205,6 +2205,7 @@ void record_before(bm_world* w, double dt) {
   if (w->top != R.top && R.broken.empty()) R.broken = "bodies were added while recording";
   R.pre_normal.assign(3 * (size_t)c.count, 0);
   R.pre_ends.assign(6 * P * (size_t)c.count, 0);
+  R.pre_shift.assign(3 * P * (size_t)c.count, 0);
   R.pre_had.assign(c.count, 0);
   const double* pos = w->position.data();
   const double* A = w->axes.data();
@@ -2223,8 +2224,9 @@ void record_before(bm_world* w, double dt) {
       const double* pt = &c.point[3 * i];
       const double* sh = &c.shift[3 * i];
       const double ax = pt[0] - nx * half - pos[a3], ay = pt[1] - ny * half - pos[a3 + 1], az = pt[2] - nz * half - pos[a3 + 2];
-      const double bx = pt[0] + sh[0] + nx * half - pos[b3], by = pt[1] + sh[1] + ny * half - pos[b3 + 1],
-                   bz = pt[2] + sh[2] + nz * half - pos[b3 + 2];
+      const double bx = pt[0] + nx * half - pos[b3], by = pt[1] + ny * half - pos[b3 + 1],
+                   bz = pt[2] + nz * half - pos[b3 + 2];
+      for (int j = 0; j < 3; j++) R.pre_shift[3 * i + j] = sh[j];
       L[0] = A[a9] * ax + A[a9 + 1] * ay + A[a9 + 2] * az;
       L[1] = A[a9 + 3] * ax + A[a9 + 4] * ay + A[a9 + 5] * az;
       L[2] = A[a9 + 6] * ax + A[a9 + 7] * ay + A[a9 + 8] * az;
@@ -2255,6 +2257,7 @@ void record_after(bm_world* w) {
     for (int k = 0; k < P; k++) R.pid.push_back(c.pid[P * f + k]);
     for (int k = 0; k < 3; k++) R.local_normal.push_back(R.pre_normal[3 * m + k]);
     for (int k = 0; k < 6 * P; k++) R.ends.push_back(R.pre_ends[6 * P * m + k]);
+    for (int k = 0; k < 3 * P; k++) R.shift.push_back(R.pre_shift[3 * P * m + k]);
     R.tolerance.push_back(c.tolerance[f]);
     R.patch.push_back(c.patch[f]);
     int src = -1;
@@ -3000,7 +3003,8 @@ int bm_adjoint_create(bm_adjoint** out, int32_t device, bm_world* const* worlds,
   std::vector<int32_t> a(rows, 0), b(rows, 0), src(rows, -1), starts((size_t)T * n * BM_STARTS), narrow(rows, 0);
   std::vector<uint8_t> count(rows, 0);
   std::vector<uint32_t> pid((size_t)P * rows, 0);
-  std::vector<double> ln(3 * (size_t)rows, 0), ends(6 * P * (size_t)rows, 0), tol(rows, 0), patch(rows, 0);
+  std::vector<double> ln(3 * (size_t)rows, 0), ends(6 * P * (size_t)rows, 0), shift(3 * P * (size_t)rows, 0),
+      tol(rows, 0), patch(rows, 0);
   std::vector<double> st_imp((size_t)P * first, 0), st_fr(3 * (size_t)first, 0), st_tw(3 * (size_t)first, 0),
       st_sl(3 * (size_t)first, 0);
   for (int k = 0; k < n; k++) {
@@ -3030,6 +3034,7 @@ int bm_adjoint_create(bm_adjoint** out, int32_t device, bm_world* const* worlds,
         for (int c = 0; c < P; c++) pid[P * g + c] = R.pid[P * f + c];
         for (int c = 0; c < 3; c++) ln[3 * g + c] = R.local_normal[3 * f + c];
         for (int c = 0; c < 6 * P; c++) ends[6 * P * (size_t)g + c] = R.ends[6 * P * (size_t)f + c];
+        for (int c = 0; c < 3 * P; c++) shift[3 * P * (size_t)g + c] = R.shift[3 * P * (size_t)f + c];
         tol[g] = R.tolerance[f];
         patch[g] = R.patch[f];
       }
@@ -3044,7 +3049,8 @@ int bm_adjoint_create(bm_adjoint** out, int32_t device, bm_world* const* worlds,
   h.angular_damping = adamping.data(); h.mass0 = mass0.data(); h.inv_mass0 = inv_mass0.data();
   h.inertia0 = inertia0.data(); h.inv_inertia0 = inv_inertia0.data(); h.left = left.data(); h.ceiling = ceiling.data();
   h.a = a.data(); h.b = b.data(); h.src = src.data(); h.count = count.data(); h.pid = pid.data();
-  h.local_normal = ln.data(); h.ends = ends.data(); h.tolerance = tol.data(); h.patch = patch.data();
+  h.local_normal = ln.data(); h.ends = ends.data(); h.shift = shift.data(); h.tolerance = tol.data();
+  h.patch = patch.data();
   h.store_impulse = st_imp.data(); h.store_friction = st_fr.data(); h.store_twist = st_tw.data();
   h.store_slip = st_sl.data();
   if (int rc = bm_create(&A->ctx, device)) return bad(rc, "bm_adjoint_create: no device context");

@@ -19,6 +19,7 @@ each.
 """
 import argparse
 import json
+import re
 import os
 import sys
 import time
@@ -143,6 +144,61 @@ def batched_world_run(count, warm, steps, threads):
             "bit_exact": bool(exact), "ms_per_step_by_part": parts, "worlds_from": BATCH_WORLDS_FROM}
 
 
+def adjoint_run(worlds=512, steps=120, every=10, threads=16, repeats=3):
+    """The differentiation form: `worlds` thrown-box scenes (scenes.throws, seed 5; dt 1/120)
+    recorded while stepping together, then held and differentiated on the device: the held
+    forward pass and the backward pass (checkpoints every `every` steps) timed by their kernels'
+    HIP events, and their ratio. The loss is a fixed weighting of every final state."""
+    from bearing_amd import native, scenes
+
+    dt = 1 / 120
+    specs = scenes.throws(worlds, seed=5)
+    batch = native.WorldBatch(0, threads)
+    ws = []
+    for bodies in specs:
+        w = batch.world(sleeping=False)
+        w.add(scenes.resolve(bodies))
+        w.record(True)
+        ws.append(w)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.step(dt)
+    record_s = time.perf_counter() - t0
+    # A world whose recorded steps cannot be held (a body caught in flight) is left out.
+    keep, dropped = list(ws), 0
+    while True:
+        try:
+            adj = native.Adjoint(keep, every=every, device=0)
+            break
+        except native.BearingError as e:
+            m = re.search(r"world (\d+):", str(e))
+            if not m or len(keep) <= 1:
+                raise
+            keep.pop(int(m.group(1)))
+            dropped += 1
+    rec = adj.recorded()
+    out = adj.forward(rec["state"], rec["mass"], rec["friction"], rec["restitution"])
+    real = np.concatenate([w.read()["position"].reshape(-1, 3) for w in keep])
+    held_dev = float(np.abs(out[:, :3] - real).max())
+    W = np.random.default_rng(5).normal(size=out.shape)
+    adj.backward(W)
+    fwd, bwd = [], []
+    for _ in range(repeats):
+        adj.forward(rec["state"], rec["mass"], rec["friction"], rec["restitution"])
+        g = adj.backward(W)
+        f, b = adj.times()
+        fwd.append(f)
+        bwd.append(b)
+    finite = bool(all(np.isfinite(v).all() for v in g.values()))
+    adj.close()
+    batch.close()
+    f, b = float(np.median(fwd)), float(np.median(bwd))
+    return {"worlds": len(keep), "dropped": dropped, "bodies": int(adj.bodies), "steps": steps, "every": every,
+            "dt": dt, "record_s": record_s, "forward_ms": f, "backward_ms": b, "bwd_fwd": b / f,
+            "world_steps_per_s_forward": len(keep) * steps / (f / 1000),
+            "held_vs_recorded_max_position_m": held_dev, "gradients_finite": finite}
+
+
 def world_run(name, lib=None):
     """A whole reference trajectory stepped through bm_world (broadphase, narrow driver,
     device meet / colour / soft step, catch, sleep): wall time per step and whether every
@@ -192,6 +248,8 @@ def main():
     ap.add_argument("--whole-batch", type=int, default=None,
                     help="only step this many whole worlds together (bm_batch_step) and print the timing")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--adjoint-worlds", type=int, default=512, help="worlds of the differentiation form")
+    ap.add_argument("--adjoint", type=int, default=None, help="only the differentiation form, with this many worlds")
     ap.add_argument("--plain-launch", default=None,
                     help="shape,resident for the plain world (default: autotuned, bearing_amd/tune.py)")
     ap.add_argument("--batched-launch", default=None, help="shape,resident for the batch (default: autotuned)")
@@ -216,6 +274,9 @@ def main():
 
     if args.world:
         print(json.dumps({"world": args.world, **world_run(args.world)}))
+        return
+    if args.adjoint:
+        print(json.dumps({"adjoint": adjoint_run(args.adjoint)}))
         return
     if args.whole_batch:
         print(json.dumps({"whole_batch": batched_world_run(args.whole_batch, 60, 20, args.batch_threads)}))
@@ -302,6 +363,7 @@ def main():
             if name not in GPU_ONLY_RUNS:
                 worlds_run[name]["cpu_port_ms_per_step"] = world_run(name, lib)["ms_per_step"]
 
+    adjoint = adjoint_run(args.adjoint_worlds)
     cpu = cpu_baseline(fx, args.cpu_seconds)
     # HBM bytes per launch of the plain kernel, from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs
     # recorded in profiles/ (scaled by a copy kernel of known size); null if not measured
@@ -330,7 +392,8 @@ def main():
                     "worlds_from": BATCH_SOURCES, "launch": {"shape": bshape, "resident_lds": bool(bresident)}},
         "batched_whole_worlds": whole,
         "world": worlds_run,
-        "bwd_fwd": None,
+        "adjoint": adjoint,
+        "bwd_fwd": adjoint["bwd_fwd"],
     }
     print(json.dumps(line))
 

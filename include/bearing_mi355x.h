@@ -369,8 +369,8 @@ int bm_batch_times(const bm_batch* batch, double* out);
  * (src/contact.js:999-1036), and the reference's soft step (src/world.js:961-1031); at the
  * recorded parameters it follows the recorded steps (bit for bit where the reference itself
  * reused the manifold; where the reference found one afresh, up to the rounding of the
- * rebuild; a manifold found at the moment of impact is held where its points were carried
- * back). bm_adjoint_backward gives the gradient of any function of the final state with
+ * rebuild; a manifold found at the moment of impact keeps its carry back as a constant).
+ * bm_adjoint_backward gives the gradient of any function of the final state with
  * respect to the initial state and the bodies' mass, friction and restitution, on the
  * forward pass's active set (Murty sets, friction cone, clamps, bounces): which manifolds a
  * step has is held, where their points lie moves with the bodies.
@@ -433,6 +433,7 @@ typedef struct bm_held {
   const uint32_t* pid;        /* 4 */
   const double* local_normal; /* 3 */
   const double* ends;         /* 24: per point a's end 3, b's end 3 */
+  const double* shift;        /* 12: per point, b's carry back from the moment of impact */
   const double* tolerance, *patch;
   /* the stores of the first rows: impulses 4, friction 3, twist 3, slip 3 */
   const double* store_impulse, *store_friction, *store_twist, *store_slip;

@@ -36,8 +36,8 @@ def test_gpu_adjoint_is_the_hosts_bit_for_bit():
     subprocess.run(["make", "-s", "-C", os.path.join(HERE, "hostworld")], check=True)
     host = ctypes.CDLL(os.path.join(HERE, "hostworld", "_build", "libhostworld.so"))
     names = sorted(SCENES)
-    gw = [record(SCENES[n][0], SCENES[n][1], device=0) for n in names]
-    hw = [record(SCENES[n][0], SCENES[n][1], lib=host) for n in names]
+    gw = [record(SCENES[n][0], 30, device=0) for n in names]
+    hw = [record(SCENES[n][0], 30, lib=host) for n in names]
     g, h = native.Adjoint(gw, every=7, device=0), native.Adjoint(hw, every=7, lib=host)
     rg, rh = g.recorded(), h.recorded()
     for key in rg:
