@@ -2508,6 +2508,7 @@ struct Trajectory {
   const int* row_offset;   // steps * worlds + 1: the rows of step t, world w from [t * worlds + w]
   const int* starts;       // STARTS per (step, world), relative to its first row
   const int* narrow;       // per row: the rows of its (step, world) in narrow order, relative
+  const int* work_at;      // per (step, world): its first working row (each world has its own block)
   const int* substeps;     // per world
   const double *dt, *gravity;
   Bodies B;
@@ -2521,13 +2522,13 @@ struct Trajectory {
   double *row_tape, *body_tape;
 };
 
-HD StepView view_of(const Trajectory& T, int t, int wld, int base) {
+HD StepView view_of(const Trajectory& T, int t, int wld) {
   StepView v;
   v.boff = T.body_offset[wld];
   v.nbodies = T.body_offset[wld + 1] - v.boff;
   v.r0 = T.row_offset[t * T.worlds + wld];
   v.r1 = T.row_offset[t * T.worlds + wld + 1];
-  v.w0 = v.r0 - base;
+  v.w0 = T.work_at[t * T.worlds + wld];
   v.substeps = T.substeps[wld];
   v.dt = T.dt[wld];
   v.gx = T.gravity[3 * wld]; v.gy = T.gravity[3 * wld + 1]; v.gz = T.gravity[3 * wld + 2];
@@ -2573,8 +2574,7 @@ HD void forward_world(const Ctx& c, Sync sync, const Trajectory& T, int wld) {
       double* ck = T.checkpoints + (size_t)(t / T.every) * T.bodies * STATE;
       for (int s = c.tid; s < nb; s += c.threads) save_state(T.B, boff + s, ck + (size_t)(boff + s) * STATE);
     }
-    const int base = T.row_offset[(t / T.every) * T.every * T.worlds];
-    const StepView v = view_of(T, t, wld, base);
+    const StepView v = view_of(T, t, wld);
     step_forward(c, sync, T.B, T.Z, T.R, T.W, v, no_tape(), false);
   }
 }
@@ -2588,12 +2588,11 @@ HD void backward_world(const Ctx& c, Sync sync, const Trajectory& T, int wld) {
   const int segments = (T.steps + T.every - 1) / T.every;
   for (int seg = segments - 1; seg >= 0; seg--) {
     const int t0 = seg * T.every, t1 = t0 + T.every < T.steps ? t0 + T.every : T.steps;
-    const int base = T.row_offset[t0 * T.worlds];
     const double* ck = T.checkpoints + (size_t)seg * T.bodies * STATE;
     for (int s = c.tid; s < nb; s += c.threads) load_state(T.B, boff + s, ck + (size_t)(boff + s) * STATE);
     sync();
     for (int t = t0; t < t1; t++) {
-      const StepView v = view_of(T, t, wld, base);
+      const StepView v = view_of(T, t, wld);
       step_forward(c, sync, T.B, T.Z, T.R, T.W, v, tape_of(T, t - t0), false);
     }
     for (int t = t1 - 1; t >= t0; t--) {
@@ -2608,7 +2607,7 @@ HD void backward_world(const Ctx& c, Sync sync, const Trajectory& T, int wld) {
         world_inverse_t<double>(o + 3, T.B.inv_inertia + 3 * g, T.B.inv_i + 6 * g);
       }
       sync();
-      const StepView v = view_of(T, t, wld, base);
+      const StepView v = view_of(T, t, wld);
       step_reverse(c, sync, T.B, T.BB, T.Z, T.R, T.RB, T.W, T.WB, v, tp);
     }
   }

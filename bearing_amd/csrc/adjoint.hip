@@ -81,10 +81,11 @@ int held_load(int device, const bm_held* held, HeldDevice** out, std::string& er
   h->device = device;
   hipError_t e;
   if ((e = hipSetDevice(device)) != hipSuccess) { held_free(h); return hip_fail(e, "hipSetDevice", err); }
-  h->bytes = adj::lay_out(*held, nullptr, h->T, h->io, nullptr);
+  std::vector<int32_t> work_at;
+  h->bytes = adj::lay_out(*held, nullptr, h->T, h->io, nullptr, work_at);
   if ((e = hipMalloc(&h->mem, h->bytes)) != hipSuccess) { held_free(h); return hip_fail(e, "hipMalloc", err); }
   std::vector<adj::Copy> copies;
-  adj::lay_out(*held, h->mem, h->T, h->io, &copies);
+  adj::lay_out(*held, h->mem, h->T, h->io, &copies, work_at);
   for (const adj::Copy& c : copies)
     if ((e = hipMemcpy(c.to, c.from, c.bytes, hipMemcpyHostToDevice)) != hipSuccess) {
       held_free(h);

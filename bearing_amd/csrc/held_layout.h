@@ -26,21 +26,30 @@ struct HeldIO {
   double *left0, *ceiling0;
 };
 
-// The most rows any segment of `every` steps has.
-inline int segment_rows(const bm_held& h) {
-  int most = 0;
-  for (int t0 = 0; t0 < h.steps; t0 += h.every) {
-    const int t1 = t0 + h.every < h.steps ? t0 + h.every : h.steps;
-    const int rows = h.row_offset[t1 * h.worlds] - h.row_offset[t0 * h.worlds];
-    if (rows > most) most = rows;
+// Where each (step, world)'s rows start among the working rows: each world has a block of
+// its own, as long as its longest segment of `every` steps, so worlds at different steps
+// (their workgroups run apart) never share a working row. Returns the working rows in all.
+inline int work_blocks(const bm_held& h, std::vector<int32_t>& at) {
+  at.assign((size_t)h.steps * h.worlds, 0);
+  int offset = 0;
+  for (int w = 0; w < h.worlds; w++) {
+    int most = 0, run = 0;
+    for (int t = 0; t < h.steps; t++) {
+      if (t % h.every == 0) run = 0;
+      at[(size_t)t * h.worlds + w] = offset + run;
+      run += h.row_offset[t * h.worlds + w + 1] - h.row_offset[t * h.worlds + w];
+      if (run > most) most = run;
+    }
+    offset += most;
   }
-  return most;
+  return offset;
 }
 
 // Lays the trajectory's columns out from `base` (nullptr: only measure), returning the
 // bytes taken; fills T and io with the columns' addresses and `copies` with what comes
 // from h.
-inline size_t lay_out(const bm_held& h, char* base, Trajectory& T, HeldIO& io, std::vector<Copy>* copies) {
+inline size_t lay_out(const bm_held& h, char* base, Trajectory& T, HeldIO& io, std::vector<Copy>* copies,
+                      std::vector<int32_t>& work_at) {
   size_t at = 0;
   auto take = [&](size_t bytes) -> char* {
     at = (at + 255) & ~(size_t)255;
@@ -57,7 +66,8 @@ inline size_t lay_out(const bm_held& h, char* base, Trajectory& T, HeldIO& io, s
   int most_sub = 1;
   for (int k = 0; k < h.worlds; k++)
     if (h.substeps[k] > most_sub) most_sub = h.substeps[k];
-  const size_t work = segment_rows(h) > 0 ? segment_rows(h) : 1;
+  const int blocks = work_blocks(h, work_at);
+  const size_t work = blocks > 0 ? blocks : 1;
   T.worlds = h.worlds; T.steps = h.steps; T.every = h.every; T.bodies = h.bodies;
   T.substeps_most = most_sub; T.work_rows = (int)work;
   auto i32 = [&](const int32_t* src, size_t n) { return reinterpret_cast<int*>(from(src, n * 4)); };
@@ -68,6 +78,7 @@ inline size_t lay_out(const bm_held& h, char* base, Trajectory& T, HeldIO& io, s
   T.row_offset = i32(h.row_offset, (size_t)h.steps * nw + 1);
   T.starts = i32(h.starts, (size_t)h.steps * nw * STARTS);
   T.narrow = i32(h.narrow, rows);
+  T.work_at = i32(work_at.data(), (size_t)h.steps * nw);
   T.substeps = i32(h.substeps, nw);
   T.dt = f64(h.dt, nw);
   T.gravity = f64(h.gravity, 3 * nw);

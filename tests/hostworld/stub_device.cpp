@@ -69,11 +69,12 @@ int bm_held_load(bm_context* ctx, const bm_held* h) {
   delete ctx->held;
   ctx->held = new HeldHost();
   HeldHost& H = *ctx->held;
-  const size_t bytes = lay_out(*h, nullptr, H.T, H.io, nullptr);
+  std::vector<int32_t> work_at;
+  const size_t bytes = lay_out(*h, nullptr, H.T, H.io, nullptr, work_at);
   H.mem.assign(bytes + 256, 0);
   char* base = reinterpret_cast<char*>(((uintptr_t)H.mem.data() + 255) & ~(uintptr_t)255);
   std::vector<Copy> copies;
-  lay_out(*h, base, H.T, H.io, &copies);
+  lay_out(*h, base, H.T, H.io, &copies, work_at);
   for (const Copy& c : copies) std::memcpy(c.to, c.from, c.bytes);
   return BM_OK;
 }

@@ -53,25 +53,42 @@ def test_gpu_adjoint_is_the_hosts_bit_for_bit():
 
 
 def test_gpu_adjoint_throws_batch():
-    # The differentiation form's scene (a thrown box, a 6x6 wall, dt = 1/120), several
-    # worlds recorded together in a batch, held and differentiated on the device; every
-    # gradient finite, the held forward at the recorded values close to the recorded one.
+    # The differentiation form's scene (a thrown box, a 6x6 wall, dt = 1/120): 32 worlds
+    # recorded together in a batch, held and differentiated on the device, one workgroup a
+    # world, the worlds running apart (different segments at once): the same final state and
+    # gradients as the host's, bit for bit, on every run; the held trajectory the recorded one
+    # to rounding.
     print("adjoint throws batch", flush=True)
-    worlds = scenes.throws(8, seed=5)
-    batch = native.WorldBatch(0, threads=4)
-    ws = []
-    for bodies in worlds:
-        w = batch.world(sleeping=False)
-        w.add(scenes.resolve(bodies))
-        w.record(True)
-        ws.append(w)
-    for _ in range(40):
-        batch.step(1 / 120)
-    adj = native.Adjoint(ws, every=10, device=0)
-    rec = adj.recorded()
-    out = adj.forward(rec["state"], rec["mass"], rec["friction"], rec["restitution"])
-    g = adj.backward(np.ones_like(out))
-    assert all(np.isfinite(v).all() for v in g.values())
-    assert np.isfinite(out).all()
-    adj.close()
-    batch.close()
+    subprocess.run(["make", "-s", "-C", os.path.join(HERE, "hostworld")], check=True)
+    host = ctypes.CDLL(os.path.join(HERE, "hostworld", "_build", "libhostworld.so"))
+    specs = scenes.throws(32, seed=5)
+
+    def recorded(lib):
+        batch = native.WorldBatch(0, threads=4) if lib is None else native.WorldBatch(threads=4, lib=lib)
+        ws = []
+        for bodies in specs:
+            w = batch.world(sleeping=False)
+            w.add(scenes.resolve(bodies))
+            w.record(True)
+            ws.append(w)
+        for _ in range(120):
+            batch.step(1 / 120)
+        real = np.concatenate([w.read()["position"].reshape(-1, 3) for w in ws])
+        return batch, ws, real
+
+    bg, wg, real = recorded(None)
+    bh, wh, _ = recorded(host)
+    g = native.Adjoint(wg, every=10, device=0)
+    h = native.Adjoint(wh, every=10, lib=host)
+    rec = g.recorded()
+    W = np.random.default_rng(4).normal(size=(g.bodies, native.HELD_STATE))
+    oh = h.forward(rec["state"], rec["mass"], rec["friction"], rec["restitution"])
+    gh = h.backward(W)
+    for run in range(3):
+        og = g.forward(rec["state"], rec["mass"], rec["friction"], rec["restitution"])
+        gg = g.backward(W)
+        assert np.array_equal(og, oh), run
+        for key in gg:
+            assert np.array_equal(gg[key], gh[key]), (run, key)
+    assert np.abs(og[:, :3] - real).max() < 1e-8
+    g.close(); h.close(); bg.close(); bh.close()
