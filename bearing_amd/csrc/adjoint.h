@@ -1703,14 +1703,27 @@ HD void warm_vjp(const Work& W, const WorkBar& WB, const Rows& R, int r, int i, 
   }
 }
 
-// q's and v's adjoints of rotate(q, v, inverse) from the output's.
+// q's and v's adjoints of rotate(q, v, inverse) from the output's: with u = +-q's vector
+// part, t = 2 u x v and out = v + w t + u x t, t's adjoint is w ob + ob x u, u's gains
+// t x ob and 2 v x tb, v's ob and 2 tb x u, w's ob . t.
 HD void rotate_vjp(const double* q, const double* v, bool inverse, const double* ob, double* qb, double* vb) {
-  const double in[7] = {q[0], q[1], q[2], q[3], v[0], v[1], v[2]};
-  const double outb[3] = {ob[0], ob[1], ob[2]};
-  double inb[7] = {0, 0, 0, 0, 0, 0, 0};
-  vjp<7>(in, outb, inb, [&](const Dual<7>* x, Dual<7>* y) { rotate_t(x, x + 4, inverse, y); });
-  if (qb) for (int k = 0; k < 4; k++) qb[k] += inb[k];
-  if (vb) for (int k = 0; k < 3; k++) vb[k] += inb[4 + k];
+  const double s = inverse ? -1 : 1;
+  const double u[3] = {q[0] * s, q[1] * s, q[2] * s}, w = q[3];
+  const double t[3] = {2 * (u[1] * v[2] - u[2] * v[1]), 2 * (u[2] * v[0] - u[0] * v[2]), 2 * (u[0] * v[1] - u[1] * v[0])};
+  const double tb[3] = {w * ob[0] + (ob[1] * u[2] - ob[2] * u[1]), w * ob[1] + (ob[2] * u[0] - ob[0] * u[2]),
+                        w * ob[2] + (ob[0] * u[1] - ob[1] * u[0])};
+  if (qb) {
+    const double ub[3] = {(t[1] * ob[2] - t[2] * ob[1]) + 2 * (v[1] * tb[2] - v[2] * tb[1]),
+                          (t[2] * ob[0] - t[0] * ob[2]) + 2 * (v[2] * tb[0] - v[0] * tb[2]),
+                          (t[0] * ob[1] - t[1] * ob[0]) + 2 * (v[0] * tb[1] - v[1] * tb[0])};
+    for (int k = 0; k < 3; k++) qb[k] += s * ub[k];
+    qb[3] += ob[0] * t[0] + ob[1] * t[1] + ob[2] * t[2];
+  }
+  if (vb) {
+    vb[0] += ob[0] + 2 * (tb[1] * u[2] - tb[2] * u[1]);
+    vb[1] += ob[1] + 2 * (tb[2] * u[0] - tb[0] * u[2]);
+    vb[2] += ob[2] + 2 * (tb[0] * u[1] - tb[1] * u[0]);
+  }
 }
 
 // The reverse of separations (not known): from the separations' adjoints into the
@@ -1859,6 +1872,27 @@ HD void accelerate_vjp(const Bodies& B, const BodiesBar& BB, int g, double h, in
   }
 }
 
+// The reverse of advance_t(q, w, h): out = n / |n|, n = q + h (w, 0) q. Into qb and wb
+// (overwritten) from out's adjoint ob.
+HD void advance_vjp(const double* q, const double* w, double h, const double* ob, double* qb, double* wb) {
+  const double nx = q[0] + h * (w[0] * q[3] + w[1] * q[2] - w[2] * q[1]);
+  const double ny = q[1] + h * (w[1] * q[3] + w[2] * q[0] - w[0] * q[2]);
+  const double nz = q[2] + h * (w[2] * q[3] + w[0] * q[1] - w[1] * q[0]);
+  const double nw = q[3] - h * (w[0] * q[0] + w[1] * q[1] + w[2] * q[2]);
+  const double l = dsqrt(nx * nx + ny * ny + nz * nz + nw * nw), il = 1 / l;
+  const double ox = nx * il, oy = ny * il, oz = nz * il, ow = nw * il;
+  const double oo = ob[0] * ox + ob[1] * oy + ob[2] * oz + ob[3] * ow;
+  const double bx = (ob[0] - ox * oo) * il, by = (ob[1] - oy * oo) * il, bz = (ob[2] - oz * oo) * il,
+               bw = (ob[3] - ow * oo) * il;
+  qb[0] = bx + h * (w[2] * by - w[1] * bz - w[0] * bw);
+  qb[1] = by + h * (-w[2] * bx + w[0] * bz - w[1] * bw);
+  qb[2] = bz + h * (w[1] * bx - w[0] * by - w[2] * bw);
+  qb[3] = bw + h * (w[0] * bx + w[1] * by + w[2] * bz);
+  wb[0] = h * (q[3] * bx - q[2] * by + q[1] * bz - q[0] * bw);
+  wb[1] = h * (q[2] * bx + q[3] * by - q[0] * bz - q[1] * bw);
+  wb[2] = h * (-q[1] * bx + q[0] * by + q[3] * bz - q[2] * bw);
+}
+
 // The reverse of move_body; B holds the body's state before it.
 HD void move_vjp(const Bodies& B, const BodiesBar& BB, int g, double h) {
   for (int c = 0; c < 3; c++) BB.v[3 * g + c] += h * BB.p[3 * g + c];
@@ -1891,14 +1925,10 @@ HD void move_vjp(const Bodies& B, const BodiesBar& BB, int g, double h) {
     BB.ceiling[g] = inb[10];
   } else {
     // advance (also where w is 0: the forward's skip is advance's value there)
-    double in[7], outb[4], inb[7];
-    for (int k = 0; k < 4; k++) { in[k] = B.q[4 * g + k]; outb[k] = BB.q[4 * g + k]; }
-    for (int k = 0; k < 3; k++) in[4 + k] = w[k];
-    for (int k = 0; k < 7; k++) inb[k] = 0;
-    const double hh = h * 0.5;
-    vjp<7>(in, outb, inb, [&](const Dual<7>* x, Dual<7>* y) { advance_t(x, x + 4, hh, y); });
-    for (int k = 0; k < 4; k++) BB.q[4 * g + k] = inb[k];
-    for (int k = 0; k < 3; k++) BB.w[3 * g + k] += inb[4 + k];
+    double qb[4], wb[3];
+    advance_vjp(B.q + 4 * g, w, h * 0.5, BB.q + 4 * g, qb, wb);
+    for (int k = 0; k < 4; k++) BB.q[4 * g + k] = qb[k];
+    for (int k = 0; k < 3; k++) BB.w[3 * g + k] += wb[k];
   }
 }
 
