@@ -51,9 +51,23 @@ constexpr double SPIN_DISC = 2.0 / 3;               // src/contact.js:146
 constexpr double COMPLIANCE = 1.0 / 1048576;        // src/forest.js:67
 constexpr double ROUNDED = 1.0 / 17592186044416.0;  // src/world.js:2809
 constexpr double MATH_PI = 3.141592653589793;
-// Row tape: velocities of both bodies (12), normal impulses (4), tangent (2), roll (2),
-// spin (1), the largest pushes so far (4).
-constexpr int ROW_TAPE = 25;
+// A solve or relax row's tape, field by field (each field a column over the working rows,
+// so a colour's rows read and write it coalesced): what the row read (its bodies'
+// velocities, its impulses, the largest pushes so far) and what its reverse needs of what
+// it worked out (targets, closing speeds, the pushes, the friction cone's and the clamps'
+// outcomes and the velocities between its stages).
+enum : int {
+  T_S = 0, T_IMP = 12, T_O1 = 16, T_O2 = 17, T_R1 = 18, T_R2 = 19, T_S0 = 20, T_MOST = 21, T_TARGET = 25,
+  T_CLOSING = 29, T_RAW = 33, T_SOLVED = 37, T_NEXT = 41, T_DD = 45, T_FLAGS = 49, T_PRESSED = 50, T_TWISTED = 51,
+  T_MID = 52, T_V1 = 64, T_V2 = 65, T_B1 = 66, T_B2 = 67, T_BB = 68, T_LIMIT = 69, T_AL = 70, T_X = 71, T_Y = 72,
+  T_LL = 73, T_T1 = 74, T_T2 = 75, T_U = 76, T_RN1 = 79, T_RN2 = 80, T_RMOST = 81, T_R1OUT = 82, T_R2OUT = 83,
+  T_DW = 84, T_SOUT = 87, ROW_TAPE = 88
+};
+struct TapeRow {
+  double* p;
+  size_t stride;
+  HD double& operator[](int f) const { return p[(size_t)f * stride]; }
+};
 // Body state: p 3, q 4, v 3, w 3, left 3, ceiling 1.
 constexpr int STATE = 17;
 
@@ -380,12 +394,43 @@ struct Rows {
 };
 
 // The current step's rows (by working index: the row less the step's first row).
+// The forward pass's working columns (per point: 4 slots a row).
 struct Work {
-  double *point, *separation, *normal, *t1, *t2, *mu, *e;
-  double *impulse, *tangent, *roll, *spin, *drift, *most, *relvel, *now;
-  double *ra, *rb, *la, *lb, *adjusted, *nmass, *jac, *block, *lever, *anchor_a, *anchor_b, *fjac, *tm, *rm, *sm;
-  double* bounce;  // per point that bounced: its impulse before, its closing speed, the two bodies' velocities
+  double* base;  // the columns, one after another, each n rows wide
+  size_t n;
   uint8_t* sliding;
+  HD double* point() const { return base + 0 * n; }  // 12 a row
+  HD double* separation() const { return base + 12 * n; }  // 4 a row
+  HD double* normal() const { return base + 16 * n; }  // 3 a row
+  HD double* t1() const { return base + 19 * n; }  // 3 a row
+  HD double* t2() const { return base + 22 * n; }  // 3 a row
+  HD double* mu() const { return base + 25 * n; }  // 1 a row
+  HD double* e() const { return base + 26 * n; }  // 1 a row
+  HD double* impulse() const { return base + 27 * n; }  // 4 a row
+  HD double* tangent() const { return base + 31 * n; }  // 2 a row
+  HD double* roll() const { return base + 33 * n; }  // 2 a row
+  HD double* spin() const { return base + 35 * n; }  // 1 a row
+  HD double* drift() const { return base + 36 * n; }  // 2 a row
+  HD double* most() const { return base + 38 * n; }  // 4 a row
+  HD double* relvel() const { return base + 42 * n; }  // 4 a row
+  HD double* now() const { return base + 46 * n; }  // 4 a row
+  HD double* ra() const { return base + 50 * n; }  // 12 a row
+  HD double* rb() const { return base + 62 * n; }  // 12 a row
+  HD double* la() const { return base + 74 * n; }  // 12 a row
+  HD double* lb() const { return base + 86 * n; }  // 12 a row
+  HD double* adjusted() const { return base + 98 * n; }  // 4 a row
+  HD double* nmass() const { return base + 102 * n; }  // 4 a row
+  HD double* jac() const { return base + 106 * n; }  // 48 a row
+  HD double* block() const { return base + 154 * n; }  // 16 a row
+  HD double* lever() const { return base + 170 * n; }  // 4 a row
+  HD double* anchor_a() const { return base + 174 * n; }  // 3 a row
+  HD double* anchor_b() const { return base + 177 * n; }  // 3 a row
+  HD double* fjac() const { return base + 180 * n; }  // 24 a row
+  HD double* tm() const { return base + 204 * n; }  // 3 a row
+  HD double* rm() const { return base + 207 * n; }  // 2 a row
+  HD double* sm() const { return base + 209 * n; }  // 1 a row
+  HD double* bounce() const { return base + 210 * n; }  // 56 a row
+  static constexpr int WIDTH = 266;
 };
 constexpr int BOUNCE = 14;
 
@@ -424,48 +469,48 @@ HD void place_row(const Bodies& B, const Rows& R, const Work& W, int boff, int r
   double out[9 + 4 * P];
   place_t<double>(h, B.p + 3 * ga, B.q + 4 * ga, B.p + 3 * gb, B.q + 4 * gb, out);
   for (int k = 0; k < 3; k++) {
-    W.normal[3 * i + k] = out[k]; W.t1[3 * i + k] = out[3 + k]; W.t2[3 * i + k] = out[6 + k];
+    W.normal()[3 * i + k] = out[k]; W.t1()[3 * i + k] = out[3 + k]; W.t2()[3 * i + k] = out[6 + k];
   }
   for (int k = 0; k < h.count; k++) {
     const int j = P * i + k;
-    for (int c = 0; c < 3; c++) W.point[3 * j + c] = out[9 + 4 * k + c];
-    W.separation[j] = out[9 + 4 * k + 3];
+    for (int c = 0; c < 3; c++) W.point()[3 * j + c] = out[9 + 4 * k + c];
+    W.separation()[j] = out[9 + 4 * k + 3];
   }
-  W.mu[i] = dsqrt(B.friction[ga] * B.friction[gb]);
-  W.e[i] = js_max(B.restitution[ga], B.restitution[gb]);
+  W.mu()[i] = dsqrt(B.friction[ga] * B.friction[gb]);
+  W.e()[i] = js_max(B.restitution[ga], B.restitution[gb]);
 }
 
 HD void carry_row(const Rows& R, const Work& W, int r, int i) {
   const int count = R.count[r];
-  W.roll[2 * i] = 0; W.roll[2 * i + 1] = 0; W.spin[i] = 0;
-  W.tangent[2 * i] = 0; W.tangent[2 * i + 1] = 0; W.drift[2 * i] = 0; W.drift[2 * i + 1] = 0;
-  for (int k = 0; k < count; k++) W.impulse[P * i + k] = 0;
+  W.roll()[2 * i] = 0; W.roll()[2 * i + 1] = 0; W.spin()[i] = 0;
+  W.tangent()[2 * i] = 0; W.tangent()[2 * i + 1] = 0; W.drift()[2 * i] = 0; W.drift()[2 * i + 1] = 0;
+  for (int k = 0; k < count; k++) W.impulse()[P * i + k] = 0;
   const int s = R.src[r];
   if (s < 0) return;
-  const double *t1 = W.t1 + 3 * i, *t2 = W.t2 + 3 * i, *n = W.normal + 3 * i;
+  const double *t1 = W.t1() + 3 * i, *t2 = W.t2() + 3 * i, *n = W.normal() + 3 * i;
   const double* L = R.store_twist + 3 * s;
   if (count <= 2) {
-    W.roll[2 * i] = L[0] * t1[0] + L[1] * t1[1] + L[2] * t1[2];
-    W.roll[2 * i + 1] = L[0] * t2[0] + L[1] * t2[1] + L[2] * t2[2];
+    W.roll()[2 * i] = L[0] * t1[0] + L[1] * t1[1] + L[2] * t1[2];
+    W.roll()[2 * i + 1] = L[0] * t2[0] + L[1] * t2[1] + L[2] * t2[2];
   }
-  W.spin[i] = L[0] * n[0] + L[1] * n[1] + L[2] * n[2];
+  W.spin()[i] = L[0] * n[0] + L[1] * n[1] + L[2] * n[2];
   const double* F = R.store_friction + 3 * s;
-  W.tangent[2 * i] = F[0] * t1[0] + F[1] * t1[1] + F[2] * t1[2];
-  W.tangent[2 * i + 1] = F[0] * t2[0] + F[1] * t2[1] + F[2] * t2[2];
+  W.tangent()[2 * i] = F[0] * t1[0] + F[1] * t1[1] + F[2] * t1[2];
+  W.tangent()[2 * i + 1] = F[0] * t2[0] + F[1] * t2[1] + F[2] * t2[2];
   bool same = false;
   for (int k = 0; k < count; k++) {
     const uint32_t id = R.pid[P * r + k];
     for (int j = 0; j < R.count[s]; j++) {
       if (R.pid[P * s + j] != id) continue;
-      W.impulse[P * i + k] = R.store_impulse[P * s + j];
+      W.impulse()[P * i + k] = R.store_impulse[P * s + j];
       same = true;
       break;
     }
   }
   if (same) {
     const double* S = R.store_slip + 3 * s;
-    W.drift[2 * i] = S[0] * t1[0] + S[1] * t1[1] + S[2] * t1[2];
-    W.drift[2 * i + 1] = S[0] * t2[0] + S[1] * t2[1] + S[2] * t2[2];
+    W.drift()[2 * i] = S[0] * t1[0] + S[1] * t1[1] + S[2] * t1[2];
+    W.drift()[2 * i + 1] = S[0] * t2[0] + S[1] * t2[1] + S[2] * t2[2];
   }
 }
 
@@ -551,41 +596,41 @@ HD void prepare_row(const Bodies& B, const Sizes& Z, const Work& W, const Rows& 
   const double *wa = B.w + 3 * ga, *wb = B.w + 3 * gb;
   const double ima = B.inv_mass[ga], imb = B.inv_mass[gb];
   const int i3 = 3 * i;
-  const double nx = W.normal[i3], ny = W.normal[i3 + 1], nz = W.normal[i3 + 2];
+  const double nx = W.normal()[i3], ny = W.normal()[i3 + 1], nz = W.normal()[i3 + 2];
   const int count = R.count[r];
   if (count <= 2) {
-    W.rm[2 * i] = turning(Ia, Ib, W.t1[i3], W.t1[i3 + 1], W.t1[i3 + 2]);
-    W.rm[2 * i + 1] = turning(Ia, Ib, W.t2[i3], W.t2[i3 + 1], W.t2[i3 + 2]);
+    W.rm()[2 * i] = turning(Ia, Ib, W.t1()[i3], W.t1()[i3 + 1], W.t1()[i3 + 2]);
+    W.rm()[2 * i + 1] = turning(Ia, Ib, W.t2()[i3], W.t2()[i3 + 1], W.t2()[i3 + 2]);
   }
-  W.sm[i] = turning(Ia, Ib, nx, ny, nz);
+  W.sm()[i] = turning(Ia, Ib, nx, ny, nz);
   const double tol = R.tolerance[r];
   double weight = 0, ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0;
   double ex = 0, ey = 0, ez = 0, fx = 0, fy = 0, fz = 0;
   for (int k = 0; k < count; k++) {
     const int j = P * i + k, j3 = 3 * j;
-    const double px = W.point[j3], py = W.point[j3 + 1], pz = W.point[j3 + 2];
+    const double px = W.point()[j3], py = W.point()[j3 + 1], pz = W.point()[j3 + 2];
     const double rax = px - pa[0], ray = py - pa[1], raz = pz - pa[2];
     const double* sh = R.shift + 3 * (P * r + k);
     const double rbx = px + sh[0] - pb[0], rby = py + sh[1] - pb[1], rbz = pz + sh[2] - pb[2];
-    W.ra[j3] = rax; W.ra[j3 + 1] = ray; W.ra[j3 + 2] = raz;
-    W.rb[j3] = rbx; W.rb[j3 + 1] = rby; W.rb[j3 + 2] = rbz;
-    const V3 ca = core(B, Z, ga, rax, ray, raz, W.la + j3);
-    const V3 cb = core(B, Z, gb, rbx, rby, rbz, W.lb + j3);
-    W.adjusted[j] = W.separation[j] - ((cb.x - ca.x) * nx + (cb.y - ca.y) * ny + (cb.z - ca.z) * nz);
-    W.nmass[j] = lever(W.jac + J * j, Ia, Ib, ima, imb, rax, ray, raz, rbx, rby, rbz, nx, ny, nz);
-    W.most[j] = 0;
-    const double s = W.separation[j] / tol, near = s <= 1 ? 1 : s >= 2 ? 0 : 2 - s;
+    W.ra()[j3] = rax; W.ra()[j3 + 1] = ray; W.ra()[j3 + 2] = raz;
+    W.rb()[j3] = rbx; W.rb()[j3 + 1] = rby; W.rb()[j3 + 2] = rbz;
+    const V3 ca = core(B, Z, ga, rax, ray, raz, W.la() + j3);
+    const V3 cb = core(B, Z, gb, rbx, rby, rbz, W.lb() + j3);
+    W.adjusted()[j] = W.separation()[j] - ((cb.x - ca.x) * nx + (cb.y - ca.y) * ny + (cb.z - ca.z) * nz);
+    W.nmass()[j] = lever(W.jac() + J * j, Ia, Ib, ima, imb, rax, ray, raz, rbx, rby, rbz, nx, ny, nz);
+    W.most()[j] = 0;
+    const double s = W.separation()[j] / tol, near = s <= 1 ? 1 : s >= 2 ? 0 : 2 - s;
     weight += near;
     ax += near * rax; ay += near * ray; az += near * raz; bx += near * rbx; by += near * rby; bz += near * rbz;
     ex += rax; ey += ray; ez += raz; fx += rbx; fy += rby; fz += rbz;
     const double dvx = vb[0] + wb[1] * rbz - wb[2] * rby - va[0] - wa[1] * raz + wa[2] * ray;
     const double dvy = vb[1] + wb[2] * rbx - wb[0] * rbz - va[1] - wa[2] * rax + wa[0] * raz;
     const double dvz = vb[2] + wb[0] * rby - wb[1] * rbx - va[2] - wa[0] * ray + wa[1] * rax;
-    W.relvel[j] = dvx * nx + dvy * ny + dvz * nz;
+    W.relvel()[j] = dvx * nx + dvy * ny + dvz * nz;
   }
   if (count >= 2) {
-    const double* jac = W.jac + J * (P * i);
-    double* K = W.block + P * P * i;
+    const double* jac = W.jac() + J * (P * i);
+    double* K = W.block() + P * P * i;
     const double s = ima + imb;
     for (int k = 0; k < count; k++) {
       const double* jk = jac + J * k;
@@ -599,24 +644,24 @@ HD void prepare_row(const Bodies& B, const Sizes& Z, const Work& W, const Rows& 
   }
   if (weight == 0) { weight = count; ax = ex; ay = ey; az = ez; bx = fx; by = fy; bz = fz; }
   ax /= weight; ay /= weight; az /= weight; bx /= weight; by /= weight; bz /= weight;
-  W.anchor_a[i3] = ax; W.anchor_a[i3 + 1] = ay; W.anchor_a[i3 + 2] = az;
-  W.anchor_b[i3] = bx; W.anchor_b[i3 + 1] = by; W.anchor_b[i3 + 2] = bz;
+  W.anchor_a()[i3] = ax; W.anchor_a()[i3 + 1] = ay; W.anchor_a()[i3 + 2] = az;
+  W.anchor_b()[i3] = bx; W.anchor_b()[i3 + 1] = by; W.anchor_b()[i3 + 2] = bz;
   for (int k = 0; k < count; k++) {
     const int j3 = 3 * (P * i + k);
-    double dx = W.ra[j3] - ax, dy = W.ra[j3 + 1] - ay, dz = W.ra[j3 + 2] - az;
+    double dx = W.ra()[j3] - ax, dy = W.ra()[j3 + 1] - ay, dz = W.ra()[j3 + 2] - az;
     const double al = dx * nx + dy * ny + dz * nz;
     dx -= al * nx; dy -= al * ny; dz -= al * nz;
-    W.lever[P * i + k] = dsqrt(dx * dx + dy * dy + dz * dz);
+    W.lever()[P * i + k] = dsqrt(dx * dx + dy * dy + dz * dz);
   }
-  double* f = W.fjac + FJ * i;
-  lever(f, Ia, Ib, ima, imb, ax, ay, az, bx, by, bz, W.t1[i3], W.t1[i3 + 1], W.t1[i3 + 2]);
-  lever(f + 12, Ia, Ib, ima, imb, ax, ay, az, bx, by, bz, W.t2[i3], W.t2[i3 + 1], W.t2[i3 + 2]);
+  double* f = W.fjac() + FJ * i;
+  lever(f, Ia, Ib, ima, imb, ax, ay, az, bx, by, bz, W.t1()[i3], W.t1()[i3 + 1], W.t1()[i3 + 2]);
+  lever(f + 12, Ia, Ib, ima, imb, ax, ay, az, bx, by, bz, W.t2()[i3], W.t2()[i3 + 1], W.t2()[i3 + 2]);
   const double k11 = ima + imb + f[0] * f[6] + f[1] * f[7] + f[2] * f[8] + f[3] * f[9] + f[4] * f[10] + f[5] * f[11];
   const double k22 = ima + imb + f[12] * f[18] + f[13] * f[19] + f[14] * f[20] + f[15] * f[21] + f[16] * f[22]
       + f[17] * f[23];
   const double k12 = f[12] * f[6] + f[13] * f[7] + f[14] * f[8] + f[15] * f[9] + f[16] * f[10] + f[17] * f[11];
   const double det = k11 * k22 - k12 * k12;
-  double* tm = W.tm + i3;
+  double* tm = W.tm() + i3;
   if (det > 0) {
     tm[0] = k22 / det; tm[1] = -k12 / det; tm[2] = k11 / det;
   } else {
@@ -671,17 +716,17 @@ HD void warm_row(const Bodies& B, const Work& W, const Rows& R, int boff, int r,
   const int ga = boff + R.a[r], gb = boff + R.b[r], count = R.count[r];
   const double ima = B.inv_mass[ga], imb = B.inv_mass[gb];
   const int i3 = 3 * i;
-  const double nx = W.normal[i3], ny = W.normal[i3 + 1], nz = W.normal[i3 + 2];
-  const double t1x = W.t1[i3], t1y = W.t1[i3 + 1], t1z = W.t1[i3 + 2];
-  const double t2x = W.t2[i3], t2y = W.t2[i3 + 1], t2z = W.t2[i3 + 2];
+  const double nx = W.normal()[i3], ny = W.normal()[i3 + 1], nz = W.normal()[i3 + 2];
+  const double t1x = W.t1()[i3], t1y = W.t1()[i3 + 1], t1z = W.t1()[i3 + 2];
+  const double t2x = W.t2()[i3], t2y = W.t2()[i3 + 1], t2z = W.t2()[i3 + 2];
   Pair s = gather(B, ga, gb);
   for (int k = 0; k < count; k++) {
     const int j = P * i + k;
-    push_normal(s, ima, imb, nx, ny, nz, W.jac + J * j, W.impulse[j]);
+    push_normal(s, ima, imb, nx, ny, nz, W.jac() + J * j, W.impulse()[j]);
   }
   {
-    const double* fj = W.fjac + FJ * i;
-    const double l1 = W.tangent[2 * i], l2 = W.tangent[2 * i + 1];
+    const double* fj = W.fjac() + FJ * i;
+    const double l1 = W.tangent()[2 * i], l2 = W.tangent()[2 * i + 1];
     const double px = t1x * l1 + t2x * l2, py = t1y * l1 + t2y * l2, pz = t1z * l1 + t2z * l2;
     s.v[0] -= ima * px; s.v[1] -= ima * py; s.v[2] -= ima * pz;
     s.v[6] += imb * px; s.v[7] += imb * py; s.v[8] += imb * pz;
@@ -689,7 +734,7 @@ HD void warm_row(const Bodies& B, const Work& W, const Rows& R, int boff, int r,
     s.v[9] += fj[9] * l1 + fj[21] * l2; s.v[10] += fj[10] * l1 + fj[22] * l2; s.v[11] += fj[11] * l1 + fj[23] * l2;
   }
   {
-    const double r1 = W.roll[2 * i], r2 = W.roll[2 * i + 1], sp = W.spin[i];
+    const double r1 = W.roll()[2 * i], r2 = W.roll()[2 * i + 1], sp = W.spin()[i];
     const double lx = t1x * r1 + t2x * r2 + nx * sp, ly = t1y * r1 + t2y * r2 + ny * sp, lz = t1z * r1 + t2z * r2 + nz * sp;
     push_turn(s, B.inv_i + 6 * ga, B.inv_i + 6 * gb, lx, ly, lz);
   }
@@ -778,9 +823,9 @@ HD void solve_math(const Work& W, const Rows& R, int r, int i, double ima, doubl
                    Pair& s) {
   const double inv = 1 / h;
   const double springy = soft.impulse_scale / soft.mass_scale + COMPLIANCE;
-  const double tol = R.tolerance[r], mu = W.mu[i], patch = R.patch[r];
+  const double tol = R.tolerance[r], mu = W.mu()[i], patch = R.patch[r];
   const int i3 = 3 * i;
-  const double nx = W.normal[i3], ny = W.normal[i3 + 1], nz = W.normal[i3 + 2];
+  const double nx = W.normal()[i3], ny = W.normal()[i3 + 1], nz = W.normal()[i3 + 2];
   const double max_bias = tol * inv;
   double least = __builtin_inf();
   for (int k = 0; k < N; k++) {
@@ -799,15 +844,15 @@ HD void solve_math(const Work& W, const Rows& R, int r, int i, double ima, doubl
   }
   f.least = least;
   s = in.s;
-  const double* Kg = W.block + P * P * i;
+  const double* Kg = W.block() + P * P * i;
   f.single = N == 1 || !(Kg[0] > 0);
   f.set = 0;
   if (f.single) {
     for (int k = 0; k < N; k++) {
-      const double* jac = W.jac + J * (P * i + k);
+      const double* jac = W.jac() + J * (P * i + k);
       const double vn = closing(s, nx, ny, nz, jac);
       const double old = in.impulse[k];
-      double next = old - W.nmass[P * i + k] * f.mscale[k] * (vn + f.target[k]) - f.iscale[k] * old;
+      double next = old - W.nmass()[P * i + k] * f.mscale[k] * (vn + f.target[k]) - f.iscale[k] * old;
       f.raw[k] = next;
       if (next < 0) next = 0;
       f.closing[k] = vn; f.solved[k] = next;
@@ -816,7 +861,7 @@ HD void solve_math(const Work& W, const Rows& R, int r, int i, double ima, doubl
     double K[N][N], wanted[N];
     for (int k = 0; k < N; k++)
       for (int l = 0; l < N; l++) K[k][l] = Kg[P * k + l];
-    for (int k = 0; k < N; k++) f.closing[k] = closing(s, nx, ny, nz, W.jac + J * (P * i + k));
+    for (int k = 0; k < N; k++) f.closing[k] = closing(s, nx, ny, nz, W.jac() + J * (P * i + k));
     int set = 0;
     for (int k = 0; k < N; k++) {
       double q = -f.closing[k] - f.target[k];
@@ -849,26 +894,26 @@ HD void solve_math(const Work& W, const Rows& R, int r, int i, double ima, doubl
     f.next[k] = next; f.dd[k] = dd;
     if (dd > 0 && in.most[k] == 0) f.first |= 1u << k;
     if (dd > in.most[k]) f.larger |= 1u << k;
-    push_normal(s, ima, imb, nx, ny, nz, W.jac + J * (P * i + k), dd);
+    push_normal(s, ima, imb, nx, ny, nz, W.jac() + J * (P * i + k), dd);
   }
   f.mid = s;
-  const double t1x = W.t1[i3], t1y = W.t1[i3 + 1], t1z = W.t1[i3 + 2];
-  const double t2x = W.t2[i3], t2y = W.t2[i3 + 1], t2z = W.t2[i3 + 2];
+  const double t1x = W.t1()[i3], t1y = W.t1()[i3 + 1], t1z = W.t1()[i3 + 2];
+  const double t2x = W.t2()[i3], t2y = W.t2()[i3 + 1], t2z = W.t2()[i3 + 2];
   double pressed = 0, twisted = 0;
   for (int k = 0; k < N; k++) {
     pressed += f.next[k];
-    twisted += W.lever[P * i + k] * f.next[k];
+    twisted += W.lever()[P * i + k] * f.next[k];
   }
   f.pressed = pressed; f.twisted = twisted;
   {
-    const double* fj = W.fjac + FJ * i;
+    const double* fj = W.fjac() + FJ * i;
     const double* g = fj + 12;
     const double dvx = s.v[6] - s.v[0], dvy = s.v[7] - s.v[1], dvz = s.v[8] - s.v[2];
     double v1 = dvx * t1x + dvy * t1y + dvz * t1z + s.v[9] * fj[3] + s.v[10] * fj[4] + s.v[11] * fj[5]
         - s.v[3] * fj[0] - s.v[4] * fj[1] - s.v[5] * fj[2];
     double v2 = dvx * t2x + dvy * t2y + dvz * t2z + s.v[9] * g[3] + s.v[10] * g[4] + s.v[11] * g[5]
         - s.v[3] * g[0] - s.v[4] * g[1] - s.v[5] * g[2];
-    const double s1 = W.drift[2 * i], s2 = W.drift[2 * i + 1];
+    const double s1 = W.drift()[2 * i], s2 = W.drift()[2 * i + 1];
     f.fbias = 0;
     if (bias && (s1 != 0 || s2 != 0)) {
       double b1 = soft.bias_rate * s1, b2 = soft.bias_rate * s2;
@@ -883,7 +928,7 @@ HD void solve_math(const Work& W, const Rows& R, int r, int i, double ima, doubl
       v1 += b1; v2 += b2;
     }
     f.v1 = v1; f.v2 = v2;
-    const double tm0 = W.tm[i3], tm1 = W.tm[i3 + 1], tm2 = W.tm[i3 + 2];
+    const double tm0 = W.tm()[i3], tm1 = W.tm()[i3 + 1], tm2 = W.tm()[i3 + 2];
     double n1 = in.o1 - (tm0 * v1 + tm1 * v2), n2 = in.o2 - (tm1 * v1 + tm2 * v2);
     f.n1 = n1; f.n2 = n2;
     const double limit = mu * pressed, l2 = n1 * n1 + n2 * n2;
@@ -926,8 +971,8 @@ HD void solve_math(const Work& W, const Rows& R, int r, int i, double ima, doubl
     f.roll_most = most;
     const double ux = s.v[9] - s.v[3], uy = s.v[10] - s.v[4], uz = s.v[11] - s.v[5];
     f.u[0] = ux; f.u[1] = uy; f.u[2] = uz;
-    double n1 = in.r1 - W.rm[2 * i] * (ux * t1x + uy * t1y + uz * t1z);
-    double n2 = in.r2 - W.rm[2 * i + 1] * (ux * t2x + uy * t2y + uz * t2z);
+    double n1 = in.r1 - W.rm()[2 * i] * (ux * t1x + uy * t1y + uz * t1z);
+    double n2 = in.r2 - W.rm()[2 * i + 1] * (ux * t2x + uy * t2y + uz * t2z);
     f.rn1 = n1; f.rn2 = n2;
     const double l2 = n1 * n1 + n2 * n2;
     if (l2 > most * most) {
@@ -945,7 +990,7 @@ HD void solve_math(const Work& W, const Rows& R, int r, int i, double ima, doubl
     const double s0 = in.s0;
     const double dwx = s.v[9] - s.v[3], dwy = s.v[10] - s.v[4], dwz = s.v[11] - s.v[5];
     f.dw[0] = dwx; f.dw[1] = dwy; f.dw[2] = dwz;
-    double s1 = s0 - W.sm[i] * (dwx * nx + dwy * ny + dwz * nz);
+    double s1 = s0 - W.sm()[i] * (dwx * nx + dwy * ny + dwz * nz);
     f.sraw = s1;
     f.sclamp = 0;
     if (s1 > limit) { s1 = limit; f.sclamp = 1; }
@@ -963,11 +1008,11 @@ HD void solve_math(const Work& W, const Rows& R, int r, int i, double ima, doubl
 template <int N>
 HD void separations(const Bodies& B, const Work& W, int ga, int gb, int i, bool known, double (&sep)[N]) {
   if (known) {
-    for (int k = 0; k < N; k++) sep[k] = W.now[P * i + k];
+    for (int k = 0; k < N; k++) sep[k] = W.now()[P * i + k];
     return;
   }
   const int i3 = 3 * i;
-  const double nx = W.normal[i3], ny = W.normal[i3 + 1], nz = W.normal[i3 + 2];
+  const double nx = W.normal()[i3], ny = W.normal()[i3 + 1], nz = W.normal()[i3 + 2];
   const double *pa = B.p + 3 * ga, *pb = B.p + 3 * gb;
   const double *qa = B.start + 3 * ga, *qb = B.start + 3 * gb;
   const double dx = (pb[0] - qb[0]) - (pa[0] - qa[0]);
@@ -978,54 +1023,68 @@ HD void separations(const Bodies& B, const Work& W, int ga, int gb, int i, bool 
   const V3 nb = rotate(B.q + 4 * gb, nx, ny, nz, true);
   for (int k = 0; k < N; k++) {
     const int j3 = 3 * (P * i + k);
-    sep[k] = dn + (W.lb[j3] * nb.x + W.lb[j3 + 1] * nb.y + W.lb[j3 + 2] * nb.z)
-        - (W.la[j3] * na.x + W.la[j3 + 1] * na.y + W.la[j3 + 2] * na.z) + W.adjusted[P * i + k];
+    sep[k] = dn + (W.lb()[j3] * nb.x + W.lb()[j3 + 1] * nb.y + W.lb()[j3 + 2] * nb.z)
+        - (W.la()[j3] * na.x + W.la()[j3 + 1] * na.y + W.la()[j3 + 2] * na.z) + W.adjusted()[P * i + k];
   }
 }
 
 // A solve (bias) or relax sweep's row, forward; its inputs onto `tape` if given.
 template <int N>
 HD void row_forward(const Bodies& B, const Work& W, const Rows& R, int boff, int r, int i, const Soft& soft,
-                    double h, bool bias, bool known, double* tape) {
+                    double h, bool bias, bool known, const TapeRow& tape) {
   const int ga = boff + R.a[r], gb = boff + R.b[r];
   RowIn<N> in;
   in.s = gather(B, ga, gb);
   for (int k = 0; k < N; k++) {
-    in.impulse[k] = W.impulse[P * i + k];
-    in.most[k] = W.most[P * i + k];
+    in.impulse[k] = W.impulse()[P * i + k];
+    in.most[k] = W.most()[P * i + k];
   }
-  in.o1 = W.tangent[2 * i]; in.o2 = W.tangent[2 * i + 1];
-  in.r1 = N <= 2 ? W.roll[2 * i] : 0; in.r2 = N <= 2 ? W.roll[2 * i + 1] : 0;
-  in.s0 = W.spin[i];
+  in.o1 = W.tangent()[2 * i]; in.o2 = W.tangent()[2 * i + 1];
+  in.r1 = N <= 2 ? W.roll()[2 * i] : 0; in.r2 = N <= 2 ? W.roll()[2 * i + 1] : 0;
+  in.s0 = W.spin()[i];
   separations<N>(B, W, ga, gb, i, known, in.sep);
-  if (tape) {
-    for (int k = 0; k < 12; k++) tape[k] = in.s.v[k];
-    for (int k = 0; k < 4; k++) tape[12 + k] = k < N ? in.impulse[k] : 0;
-    tape[16] = in.o1; tape[17] = in.o2; tape[18] = in.r1; tape[19] = in.r2; tape[20] = in.s0;
-    for (int k = 0; k < 4; k++) tape[21 + k] = k < N ? in.most[k] : 0;
-  }
   const double ima = B.inv_mass[ga], imb = B.inv_mass[gb];
   RowFwd<N> f;
   Pair s;
   solve_math<N>(W, R, r, i, ima, imb, B.inv_i + 6 * ga, B.inv_i + 6 * gb, soft, h, bias, in, f, s);
   scatter(B, ga, gb, ima, imb, s);
+  if (tape.p) {
+    const TapeRow& T = tape;
+    for (int k = 0; k < 12; k++) { T[T_S + k] = in.s.v[k]; T[T_MID + k] = f.mid.v[k]; }
+    uint32_t flags = (f.single ? 1u : 0u) | (f.slide ? 2u : 0u) | (f.roll_capped ? 4u : 0u)
+        | ((uint32_t)(f.sclamp == 1 ? 1 : f.sclamp == -1 ? 2 : 0) << 3) | ((uint32_t)f.fbias << 5)
+        | ((uint32_t)f.set << 8) | ((uint32_t)f.first << 12);
+    for (int k = 0; k < N; k++) {
+      T[T_IMP + k] = in.impulse[k]; T[T_MOST + k] = in.most[k]; T[T_TARGET + k] = f.target[k];
+      T[T_CLOSING + k] = f.closing[k]; T[T_RAW + k] = f.single ? f.raw[k] : 0; T[T_SOLVED + k] = f.solved[k];
+      T[T_NEXT + k] = f.next[k]; T[T_DD + k] = f.dd[k];
+      flags |= (uint32_t)f.tcase[k] << (16 + 2 * k);
+    }
+    T[T_O1] = in.o1; T[T_O2] = in.o2; T[T_R1] = in.r1; T[T_R2] = in.r2; T[T_S0] = in.s0;
+    T[T_FLAGS] = flags; T[T_PRESSED] = f.pressed; T[T_TWISTED] = f.twisted;
+    T[T_V1] = f.v1; T[T_V2] = f.v2; T[T_B1] = f.b1; T[T_B2] = f.b2; T[T_BB] = f.bb; T[T_LIMIT] = f.limit;
+    T[T_AL] = f.al; T[T_X] = f.x; T[T_Y] = f.y; T[T_LL] = f.ll; T[T_T1] = f.t1_out; T[T_T2] = f.t2_out;
+    for (int c = 0; c < 3; c++) { T[T_U + c] = f.u[c]; T[T_DW + c] = f.dw[c]; }
+    T[T_RN1] = f.rn1; T[T_RN2] = f.rn2; T[T_RMOST] = f.roll_most; T[T_R1OUT] = f.r1_out; T[T_R2OUT] = f.r2_out;
+    T[T_SOUT] = f.s_out;
+  }
   for (int k = 0; k < N; k++) {
     const int j = P * i + k;
-    if (!known && !bias) W.now[j] = in.sep[k];
-    W.impulse[j] = f.next[k];
-    if ((f.first >> k) & 1) W.relvel[j] = f.closing[k];
-    if ((f.larger >> k) & 1) W.most[j] = f.dd[k];
+    if (!known && !bias) W.now()[j] = in.sep[k];
+    W.impulse()[j] = f.next[k];
+    if ((f.first >> k) & 1) W.relvel()[j] = f.closing[k];
+    if ((f.larger >> k) & 1) W.most()[j] = f.dd[k];
   }
   W.sliding[i] = f.slide ? 1 : 0;
-  W.tangent[2 * i] = f.t1_out; W.tangent[2 * i + 1] = f.t2_out;
-  if (N <= 2) { W.roll[2 * i] = f.r1_out; W.roll[2 * i + 1] = f.r2_out; }
-  W.spin[i] = f.s_out;
+  W.tangent()[2 * i] = f.t1_out; W.tangent()[2 * i + 1] = f.t2_out;
+  if (N <= 2) { W.roll()[2 * i] = f.r1_out; W.roll()[2 * i + 1] = f.r2_out; }
+  W.spin()[i] = f.s_out;
 }
 
 // KIND 0 warm start, 1 solve, 2 relax; the tape for this sweep's rows if given.
 template <int KIND>
 HD void row_of(const Bodies& B, const Work& W, const Rows& R, int boff, int r, int i, const Soft& soft, double h,
-               bool known, double* tape) {
+               bool known, const TapeRow& tape) {
   if (KIND == 0) { warm_row(B, W, R, boff, r, i); return; }
   switch (R.count[r]) {
     case 1: row_forward<1>(B, W, R, boff, r, i, soft, h, KIND == 1, known, tape); break;
@@ -1080,29 +1139,29 @@ HD void move_body(const Bodies& B, int g, double h) {
 
 // src/contact.js:2229-2251 (restitution) for one row, in narrow order.
 HD void bounce_row(const Bodies& B, const Work& W, const Rows& R, int boff, int r, int i, double gravity) {
-  const double e = W.e[i];
+  const double e = W.e()[i];
   if (e == 0) return;
   const double threshold = dsqrt(2 * gravity * R.tolerance[r]);
   const int ga = boff + R.a[r], gb = boff + R.b[r];
   const int i3 = 3 * i;
-  const double nx = W.normal[i3], ny = W.normal[i3 + 1], nz = W.normal[i3 + 2];
+  const double nx = W.normal()[i3], ny = W.normal()[i3 + 1], nz = W.normal()[i3 + 2];
   const double ima = B.inv_mass[ga], imb = B.inv_mass[gb];
   for (int k = 0; k < R.count[r]; k++) {
     const int j = P * i + k;
-    if (W.relvel[j] > -threshold || W.most[j] == 0) continue;
-    const double* jac = W.jac + J * j;
+    if (W.relvel()[j] > -threshold || W.most()[j] == 0) continue;
+    const double* jac = W.jac() + J * j;
     Pair s = gather(B, ga, gb);
     const double vn = closing(s, nx, ny, nz, jac);
-    const double old = W.impulse[j];
-    if (W.bounce) {
-      double* o = W.bounce + BOUNCE * j;
+    const double old = W.impulse()[j];
+    if (W.bounce()) {
+      double* o = W.bounce() + BOUNCE * j;
       o[0] = old; o[1] = vn;
       for (int c = 0; c < 12; c++) o[2 + c] = s.v[c];
     }
-    double next = old - W.nmass[j] * (vn + e * W.relvel[j]);
+    double next = old - W.nmass()[j] * (vn + e * W.relvel()[j]);
     if (next < 0) next = 0;
     const double l = next - old;
-    W.impulse[j] = next;
+    W.impulse()[j] = next;
     push_normal(s, ima, imb, nx, ny, nz, jac, l);
     for (int c = 0; c < 3; c++) {
       B.v[3 * ga + c] = s.v[c]; B.w[3 * ga + c] = s.v[3 + c];
@@ -1126,44 +1185,44 @@ HD void store_row(const Bodies& B, const Work& W, const Rows& R, int boff, int r
   const int i3 = 3 * i, count = R.count[r];
   const int ga = boff + R.a[r], gb = boff + R.b[r];
   double pressed = 0;
-  for (int k = 0; k < count; k++) pressed += W.impulse[P * i + k];
-  const double t1x = W.t1[i3], t1y = W.t1[i3 + 1], t1z = W.t1[i3 + 2];
-  const double t2x = W.t2[i3], t2y = W.t2[i3 + 1], t2z = W.t2[i3 + 2];
+  for (int k = 0; k < count; k++) pressed += W.impulse()[P * i + k];
+  const double t1x = W.t1()[i3], t1y = W.t1()[i3 + 1], t1z = W.t1()[i3 + 2];
+  const double t2x = W.t2()[i3], t2y = W.t2()[i3 + 1], t2z = W.t2()[i3 + 2];
   if (W.sliding[i] != 0 || pressed == 0 || impacted(R, r)) {
-    W.drift[2 * i] = 0; W.drift[2 * i + 1] = 0;
+    W.drift()[2 * i] = 0; W.drift()[2 * i + 1] = 0;
   } else {
     double T[6];
     double dx = 0, dy = 0, dz = 0;
     if (B.type[gb] == DYNAMIC) {
       turned_t<double>(B.p + 3 * gb, B.start + 3 * gb, B.q + 4 * gb, B.start_q + 4 * gb, T);
-      const double rx = W.anchor_b[i3], ry = W.anchor_b[i3 + 1], rz = W.anchor_b[i3 + 2];
+      const double rx = W.anchor_b()[i3], ry = W.anchor_b()[i3 + 1], rz = W.anchor_b()[i3 + 2];
       dx = T[0] + T[4] * rz - T[5] * ry;
       dy = T[1] + T[5] * rx - T[3] * rz;
       dz = T[2] + T[3] * ry - T[4] * rx;
     }
     if (B.type[ga] == DYNAMIC) {
       turned_t<double>(B.p + 3 * ga, B.start + 3 * ga, B.q + 4 * ga, B.start_q + 4 * ga, T);
-      const double rx = W.anchor_a[i3], ry = W.anchor_a[i3 + 1], rz = W.anchor_a[i3 + 2];
+      const double rx = W.anchor_a()[i3], ry = W.anchor_a()[i3 + 1], rz = W.anchor_a()[i3 + 2];
       dx -= T[0] + T[4] * rz - T[5] * ry;
       dy -= T[1] + T[5] * rx - T[3] * rz;
       dz -= T[2] + T[3] * ry - T[4] * rx;
     }
-    W.drift[2 * i] += dx * t1x + dy * t1y + dz * t1z;
-    W.drift[2 * i + 1] += dx * t2x + dy * t2y + dz * t2z;
+    W.drift()[2 * i] += dx * t1x + dy * t1y + dz * t1z;
+    W.drift()[2 * i + 1] += dx * t2x + dy * t2y + dz * t2z;
   }
-  const double nx = W.normal[i3], ny = W.normal[i3 + 1], nz = W.normal[i3 + 2];
-  const double r1 = W.roll[2 * i], r2 = W.roll[2 * i + 1], sp = W.spin[i];
+  const double nx = W.normal()[i3], ny = W.normal()[i3 + 1], nz = W.normal()[i3 + 2];
+  const double r1 = W.roll()[2 * i], r2 = W.roll()[2 * i + 1], sp = W.spin()[i];
   double* L = R.store_twist + 3 * r;
   L[0] = t1x * r1 + t2x * r2 + nx * sp;
   L[1] = t1y * r1 + t2y * r2 + ny * sp;
   L[2] = t1z * r1 + t2z * r2 + nz * sp;
-  const double l1 = W.tangent[2 * i], l2 = W.tangent[2 * i + 1];
+  const double l1 = W.tangent()[2 * i], l2 = W.tangent()[2 * i + 1];
   double* F = R.store_friction + 3 * r;
   F[0] = t1x * l1 + t2x * l2; F[1] = t1y * l1 + t2y * l2; F[2] = t1z * l1 + t2z * l2;
-  const double d1 = W.drift[2 * i], d2 = W.drift[2 * i + 1];
+  const double d1 = W.drift()[2 * i], d2 = W.drift()[2 * i + 1];
   double* S = R.store_slip + 3 * r;
   S[0] = t1x * d1 + t2x * d2; S[1] = t1y * d1 + t2y * d2; S[2] = t1z * d1 + t2z * d2;
-  for (int k = 0; k < count; k++) R.store_impulse[P * r + k] = W.impulse[P * i + k];
+  for (int k = 0; k < count; k++) R.store_impulse[P * r + k] = W.impulse()[P * i + k];
 }
 
 // The tapes of one step: a row's inputs for each solve and relax sweep
@@ -1172,7 +1231,9 @@ HD void store_row(const Bodies& B, const Work& W, const Rows& R, int boff, int r
 struct Tape {
   double *rows, *bodies;
   int row_stride, body_stride;  // working rows and bodies a sweep's block holds
-  HD double* row(int sweep, int i) const { return rows ? rows + ((size_t)sweep * row_stride + i) * ROW_TAPE : nullptr; }
+  HD TapeRow row(int sweep, int i) const {
+    return TapeRow{rows ? rows + (size_t)sweep * ROW_TAPE * row_stride + i : nullptr, (size_t)row_stride};
+  }
   HD double* body(int phase, int g) const { return bodies ? bodies + ((size_t)phase * body_stride + g) * STATE : nullptr; }
 };
 
@@ -1202,7 +1263,7 @@ HD void sweep(const Ctx& c, Sync sync, const Bodies& B, const Work& W, const Row
     if (from == to) continue;
     for (int lm = from + c.tid; lm < to; lm += c.threads) {
       const int i = v.w0 + lm;
-      row_of<KIND>(B, W, R, v.boff, v.r0 + lm, i, soft, h, known, KIND == 0 ? nullptr : tape.row(tsweep, i));
+      row_of<KIND>(B, W, R, v.boff, v.r0 + lm, i, soft, h, known, tape.row(KIND == 0 ? 0 : tsweep, i));
     }
     sync();
   }
@@ -1211,7 +1272,7 @@ HD void sweep(const Ctx& c, Sync sync, const Bodies& B, const Work& W, const Row
     if (c.tid == 0)
       for (int lm = from; lm < to; lm++) {
         const int i = v.w0 + lm;
-        row_of<KIND>(B, W, R, v.boff, v.r0 + lm, i, soft, h, known, KIND == 0 ? nullptr : tape.row(tsweep, i));
+        row_of<KIND>(B, W, R, v.boff, v.r0 + lm, i, soft, h, known, tape.row(KIND == 0 ? 0 : tsweep, i));
       }
     sync();
   }
@@ -1291,11 +1352,42 @@ struct BodiesBar {
   double *inv_mass, *inertia, *inv_inertia;  // of the mass-derived columns, summed over every step
 };
 
-// Adjoints by working row.
+// Adjoints by working row: of the row's state, of what prepare derived, of its places, and
+// what its sweeps added to its bodies' inverse mass and inertia adjoints.
 struct WorkBar {
-  double *impulse, *tangent, *roll, *spin, *drift, *relvel;       // of the row's state now
-  double *jac, *fjac, *tm, *nmass, *block, *rm, *sm, *lever;       // of what prepare derived
-  double *la, *lb, *adjusted, *normal, *t1, *t2, *point, *separation, *mu, *e, *anchor_a, *anchor_b;
+  double* base;  // the columns, one after another, each n rows wide
+  size_t n;
+  HD double* impulse() const { return base + 0 * n; }  // 4 a row
+  HD double* tangent() const { return base + 4 * n; }  // 2 a row
+  HD double* roll() const { return base + 6 * n; }  // 2 a row
+  HD double* spin() const { return base + 8 * n; }  // 1 a row
+  HD double* drift() const { return base + 9 * n; }  // 2 a row
+  HD double* relvel() const { return base + 11 * n; }  // 4 a row
+  HD double* jac() const { return base + 15 * n; }  // 48 a row
+  HD double* fjac() const { return base + 63 * n; }  // 24 a row
+  HD double* tm() const { return base + 87 * n; }  // 3 a row
+  HD double* nmass() const { return base + 90 * n; }  // 4 a row
+  HD double* block() const { return base + 94 * n; }  // 16 a row
+  HD double* rm() const { return base + 110 * n; }  // 2 a row
+  HD double* sm() const { return base + 112 * n; }  // 1 a row
+  HD double* lever() const { return base + 113 * n; }  // 4 a row
+  HD double* la() const { return base + 117 * n; }  // 12 a row
+  HD double* lb() const { return base + 129 * n; }  // 12 a row
+  HD double* adjusted() const { return base + 141 * n; }  // 4 a row
+  HD double* normal() const { return base + 145 * n; }  // 3 a row
+  HD double* t1() const { return base + 148 * n; }  // 3 a row
+  HD double* t2() const { return base + 151 * n; }  // 3 a row
+  HD double* point() const { return base + 154 * n; }  // 12 a row
+  HD double* separation() const { return base + 166 * n; }  // 4 a row
+  HD double* mu() const { return base + 170 * n; }  // 1 a row
+  HD double* e() const { return base + 171 * n; }  // 1 a row
+  HD double* anchor_a() const { return base + 172 * n; }  // 3 a row
+  HD double* anchor_b() const { return base + 175 * n; }  // 3 a row
+  HD double* ima() const { return base + 178 * n; }  // 1 a row
+  HD double* imb() const { return base + 179 * n; }  // 1 a row
+  HD double* Ia() const { return base + 180 * n; }  // 6 a row
+  HD double* Ib() const { return base + 186 * n; }  // 6 a row
+  static constexpr int WIDTH = 192;
 };
 
 // Adjoints by global row: the stored impulses, and mu and e (summed onto the bodies'
@@ -1369,103 +1461,108 @@ HD void solve2(double al, double a, double b, double c, double n1, double n2, do
   y = ((1 + al * a) * n2 - ab * n1) / det;
 }
 
-// The reverse of a solve (bias) or relax row of N points. `in` is the row's tape and
-// separations; sb holds its bodies' velocity adjoints after the row and leaves with
-// them before it; the row's state adjoints in WB go from after to before; what the row
-// read of prepare, the normal and the bodies' inverse masses and inertias gets its
-// adjoints added; sepb gets the separations' adjoints.
+// The reverse of a solve (bias) or relax row of N points, from what its forward left on
+// the tape (T, the fields below) rather than by working the row again. sb holds its bodies'
+// velocity adjoints after the row and leaves with them before it; the row's state adjoints
+// in WB go from after to before; what the row read of prepare, the normal and the bodies'
+// inverse masses and inertias gets its adjoints added; sepb gets the separations'.
 template <int N>
 HD void solve_vjp(const Work& W, const WorkBar& WB, const Rows& R, int r, int i, double ima, double imb,
-                  const double* Ia, const double* Ib, const Soft& soft, double h, bool bias, const RowIn<N>& in,
+                  const double* Ia, const double* Ib, const Soft& soft, double h, bool bias, const TapeRow& T,
                   Pair& sb, double& imab, double& imbb, double* Iab, double* Ibb, double (&sepb)[N]) {
-  RowFwd<N> f;
-  Pair out;
-  solve_math<N>(W, R, r, i, ima, imb, Ia, Ib, soft, h, bias, in, f, out);
   const int i3 = 3 * i;
-  const double* n = W.normal + i3;
-  const double* t1 = W.t1 + i3;
-  const double* t2 = W.t2 + i3;
-  double* nb = WB.normal + i3;
-  double* t1b = WB.t1 + i3;
-  double* t2b = WB.t2 + i3;
-  const double mu = W.mu[i], patch = R.patch[r];
+  const double* n = W.normal() + i3;
+  const double* t1 = W.t1() + i3;
+  const double* t2 = W.t2() + i3;
+  double* nb = WB.normal() + i3;
+  double* t1b = WB.t1() + i3;
+  double* t2b = WB.t2() + i3;
+  const double mu = W.mu()[i], patch = R.patch[r];
+  const uint32_t flags = (uint32_t)T[T_FLAGS];
+  const double pressed = T[T_PRESSED], twisted = T[T_TWISTED];
   double pressed_b = 0, twisted_b = 0;
   // spin
   {
-    const double ds = f.s_out - in.s0;
+    const double s0 = T[T_S0], ds = T[T_SOUT] - s0;
     const double L[3] = {n[0] * ds, n[1] * ds, n[2] * ds};
     double Lb[3] = {0, 0, 0};
     push_turn_vjp(sb, Ia, Ib, L, Lb, Iab, Ibb);
     const double dsb = n[0] * Lb[0] + n[1] * Lb[1] + n[2] * Lb[2];
     for (int c = 0; c < 3; c++) nb[c] += ds * Lb[c];
-    const double s1b = WB.spin[i] + dsb;
+    const double s1b = WB.spin()[i] + dsb;
     double s0b = -dsb;
     double limitb = 0;
-    if (f.sclamp == 1) limitb += s1b;
-    else if (f.sclamp == -1) limitb -= s1b;
+    const int sclamp = (flags >> 3) & 3;
+    if (sclamp == 1) limitb += s1b;
+    else if (sclamp == 2) limitb -= s1b;
     else {
       s0b += s1b;
-      const double proj = f.dw[0] * n[0] + f.dw[1] * n[1] + f.dw[2] * n[2];
-      WB.sm[i] -= s1b * proj;
-      const double sm = W.sm[i];
+      const double dw[3] = {T[T_DW], T[T_DW + 1], T[T_DW + 2]};
+      const double proj = dw[0] * n[0] + dw[1] * n[1] + dw[2] * n[2];
+      WB.sm()[i] -= s1b * proj;
+      const double sm = W.sm()[i];
       for (int c = 0; c < 3; c++) {
         const double g = -s1b * sm * n[c];
         sb.v[9 + c] += g; sb.v[3 + c] -= g;
-        nb[c] -= s1b * sm * f.dw[c];
+        nb[c] -= s1b * sm * dw[c];
       }
     }
     if (N == 1) {
-      WB.mu[i] += SPIN_DISC * patch * f.pressed * limitb;
+      WB.mu()[i] += SPIN_DISC * patch * pressed * limitb;
       pressed_b += SPIN_DISC * mu * patch * limitb;
     } else {
-      WB.mu[i] += f.twisted * limitb;
+      WB.mu()[i] += twisted * limitb;
       twisted_b += mu * limitb;
     }
-    WB.spin[i] = s0b;
+    WB.spin()[i] = s0b;
   }
   // roll
   if (N <= 2) {
-    const double d1 = f.r1_out - in.r1, d2 = f.r2_out - in.r2;
+    const double r1 = T[T_R1], r2 = T[T_R2];
+    const double d1 = T[T_R1OUT] - r1, d2 = T[T_R2OUT] - r2;
     const double L[3] = {t1[0] * d1 + t2[0] * d2, t1[1] * d1 + t2[1] * d2, t1[2] * d1 + t2[2] * d2};
     double Lb[3] = {0, 0, 0};
     push_turn_vjp(sb, Ia, Ib, L, Lb, Iab, Ibb);
     const double d1b = t1[0] * Lb[0] + t1[1] * Lb[1] + t1[2] * Lb[2];
     const double d2b = t2[0] * Lb[0] + t2[1] * Lb[1] + t2[2] * Lb[2];
     for (int c = 0; c < 3; c++) { t1b[c] += d1 * Lb[c]; t2b[c] += d2 * Lb[c]; }
-    double o1b = WB.roll[2 * i] + d1b, o2b = WB.roll[2 * i + 1] + d2b;
+    const double o1b = WB.roll()[2 * i] + d1b, o2b = WB.roll()[2 * i + 1] + d2b;
     double r1b = -d1b, r2b = -d2b;
     double n1b = o1b, n2b = o2b;
-    if (f.roll_capped) {
+    if ((flags >> 2) & 1) {
       // out = most * n / |n|
-      const double l = dsqrt(f.rn1 * f.rn1 + f.rn2 * f.rn2);
-      const double u1 = f.rn1 / l, u2 = f.rn2 / l;
+      const double rn1 = T[T_RN1], rn2 = T[T_RN2];
+      const double l = dsqrt(rn1 * rn1 + rn2 * rn2);
+      const double u1 = rn1 / l, u2 = rn2 / l;
       const double ou = o1b * u1 + o2b * u2;
       pressed_b += patch * ou;
-      const double k = f.roll_most / l;
+      const double k = T[T_RMOST] / l;
       n1b = k * (o1b - u1 * ou);
       n2b = k * (o2b - u2 * ou);
     }
     r1b += n1b; r2b += n2b;
-    const double rm1 = W.rm[2 * i], rm2 = W.rm[2 * i + 1];
-    const double p1 = f.u[0] * t1[0] + f.u[1] * t1[1] + f.u[2] * t1[2];
-    const double p2 = f.u[0] * t2[0] + f.u[1] * t2[1] + f.u[2] * t2[2];
-    WB.rm[2 * i] -= n1b * p1;
-    WB.rm[2 * i + 1] -= n2b * p2;
+    const double rm1 = W.rm()[2 * i], rm2 = W.rm()[2 * i + 1];
+    const double u[3] = {T[T_U], T[T_U + 1], T[T_U + 2]};
+    const double p1 = u[0] * t1[0] + u[1] * t1[1] + u[2] * t1[2];
+    const double p2 = u[0] * t2[0] + u[1] * t2[1] + u[2] * t2[2];
+    WB.rm()[2 * i] -= n1b * p1;
+    WB.rm()[2 * i + 1] -= n2b * p2;
     for (int c = 0; c < 3; c++) {
       const double ub = -n1b * rm1 * t1[c] - n2b * rm2 * t2[c];
       sb.v[9 + c] += ub; sb.v[3 + c] -= ub;
-      t1b[c] -= n1b * rm1 * f.u[c];
-      t2b[c] -= n2b * rm2 * f.u[c];
+      t1b[c] -= n1b * rm1 * u[c];
+      t2b[c] -= n2b * rm2 * u[c];
     }
-    WB.roll[2 * i] = r1b; WB.roll[2 * i + 1] = r2b;
+    WB.roll()[2 * i] = r1b; WB.roll()[2 * i + 1] = r2b;
   }
   // friction
   {
-    const double* fj = W.fjac + FJ * i;
+    const double* fj = W.fjac() + FJ * i;
     const double* g = fj + 12;
-    double* fjb = WB.fjac + FJ * i;
+    double* fjb = WB.fjac() + FJ * i;
     double* gb = fjb + 12;
-    const double d1 = f.t1_out - in.o1, d2 = f.t2_out - in.o2;
+    const double o1 = T[T_O1], o2 = T[T_O2];
+    const double d1 = T[T_T1] - o1, d2 = T[T_T2] - o2;
     double d1b = 0, d2b = 0;
     for (int c = 0; c < 3; c++) {
       const double dv = imb * sb.v[6 + c] - ima * sb.v[c];
@@ -1479,128 +1576,138 @@ HD void solve_vjp(const Work& W, const WorkBar& WB, const Rows& R, int r, int i,
       fjb[6 + c] -= d1 * sb.v[3 + c]; gb[6 + c] -= d2 * sb.v[3 + c];
       fjb[9 + c] += d1 * sb.v[9 + c]; gb[9 + c] += d2 * sb.v[9 + c];
     }
-    const double o1b = WB.tangent[2 * i] + d1b, o2b = WB.tangent[2 * i + 1] + d2b;
+    const double o1b = WB.tangent()[2 * i] + d1b, o2b = WB.tangent()[2 * i + 1] + d2b;
     double in1b = -d1b, in2b = -d2b;  // of the tangent impulses before
     double n1b = o1b, n2b = o2b;      // of n before the cone
-    const double tm0 = W.tm[i3], tm1 = W.tm[i3 + 1], tm2 = W.tm[i3 + 2];
-    double* tmb = WB.tm + i3;
-    if (f.slide) {
+    const double tm0 = W.tm()[i3], tm1 = W.tm()[i3 + 1], tm2 = W.tm()[i3 + 2];
+    double* tmb = WB.tm() + i3;
+    if ((flags >> 1) & 1) {
       // out = limit x / |x|, x = (I + al M)^-1 n, al where |x| = limit.
+      const double limit = T[T_LIMIT], al = T[T_AL], fx = T[T_X], fy = T[T_Y];
       double limitb = 0;
-      const double lx = dsqrt(f.ll);
-      const double u1 = f.x / lx, u2 = f.y / lx;
+      const double lx = dsqrt(T[T_LL]);
+      const double u1 = fx / lx, u2 = fy / lx;
       const double ou = o1b * u1 + o2b * u2;
       limitb += ou;
-      const double k = f.limit / lx;
+      const double k = limit / lx;
       double xb = k * (o1b - u1 * ou), yb = k * (o2b - u2 * ou);
       n1b = 0; n2b = 0;
-      if (f.limit > 0 && f.al > 0) {
-        const double al = f.al;
-        // B^-1 of the adjoint, and of M x
-        double bx, by, mx, my, bmx, bmy;
+      if (limit > 0 && al > 0) {
+        double bx, by, bmx, bmy;
         solve2(al, tm0, tm1, tm2, xb, yb, bx, by);
-        mx = tm0 * f.x + tm1 * f.y; my = tm1 * f.x + tm2 * f.y;
+        const double mx = tm0 * fx + tm1 * fy, my = tm1 * fx + tm2 * fy;
         solve2(al, tm0, tm1, tm2, mx, my, bmx, bmy);
         const double alb = -(bx * mx + by * my);
-        const double gal = -2 * (f.x * bmx + f.y * bmy);
+        const double gal = -2 * (fx * bmx + fy * bmy);
         const double cc = gal != 0 ? -alb / gal : 0;
-        const double zx = xb + 2 * cc * f.x, zy = yb + 2 * cc * f.y;
+        const double zx = xb + 2 * cc * fx, zy = yb + 2 * cc * fy;
         double yx, yy;
         solve2(al, tm0, tm1, tm2, zx, zy, yx, yy);
         n1b += yx; n2b += yy;
         // M's adjoint: -al y x^T
-        tmb[0] -= al * yx * f.x;
-        tmb[1] -= al * (yx * f.y + yy * f.x);
-        tmb[2] -= al * yy * f.y;
-        limitb += cc * (-2 * f.limit);
+        tmb[0] -= al * yx * fx;
+        tmb[1] -= al * (yx * fy + yy * fx);
+        tmb[2] -= al * yy * fy;
+        limitb += cc * (-2 * limit);
       } else {
         // limit 0, or no Newton step taken: x = n
         n1b += xb; n2b += yb;
       }
-      WB.mu[i] += f.pressed * limitb;
+      WB.mu()[i] += pressed * limitb;
       pressed_b += mu * limitb;
     }
     // n = o - M v
     in1b += n1b; in2b += n2b;
+    const double v1 = T[T_V1], v2 = T[T_V2];
     const double v1b = -(tm0 * n1b + tm1 * n2b), v2b = -(tm1 * n1b + tm2 * n2b);
-    tmb[0] -= n1b * f.v1;
-    tmb[1] -= n1b * f.v2 + n2b * f.v1;
-    tmb[2] -= n2b * f.v2;
+    tmb[0] -= n1b * v1;
+    tmb[1] -= n1b * v2 + n2b * v1;
+    tmb[2] -= n2b * v2;
     // the slip's pull (bias) depends only on the drift
-    if (f.fbias == 1) {
-      WB.drift[2 * i] += soft.bias_rate * v1b;
-      WB.drift[2 * i + 1] += soft.bias_rate * v2b;
-    } else if (f.fbias == 2) {
+    const int fbias = (flags >> 5) & 3;
+    if (fbias == 1) {
+      WB.drift()[2 * i] += soft.bias_rate * v1b;
+      WB.drift()[2 * i + 1] += soft.bias_rate * v2b;
+    } else if (fbias == 2) {
       const double tol = R.tolerance[r], max_bias = tol / h;
-      const double l = dsqrt(f.bb);
-      const double u1 = f.b1 / l, u2 = f.b2 / l, k = max_bias / l;
+      const double l = dsqrt(T[T_BB]);
+      const double u1 = T[T_B1] / l, u2 = T[T_B2] / l, k = max_bias / l;
       const double ou = v1b * u1 + v2b * u2;
-      WB.drift[2 * i] += soft.bias_rate * k * (v1b - u1 * ou);
-      WB.drift[2 * i + 1] += soft.bias_rate * k * (v2b - u2 * ou);
+      WB.drift()[2 * i] += soft.bias_rate * k * (v1b - u1 * ou);
+      WB.drift()[2 * i + 1] += soft.bias_rate * k * (v2b - u2 * ou);
     }
     // v1, v2 from the velocities after the normal pushes
-    const Pair& m = f.mid;
     for (int c = 0; c < 3; c++) {
-      const double dv = m.v[6 + c] - m.v[c];
+      const double ma = T[T_MID + c], mwa = T[T_MID + 3 + c], mb = T[T_MID + 6 + c], mwb = T[T_MID + 9 + c];
+      const double dv = mb - ma;
       t1b[c] += v1b * dv; t2b[c] += v2b * dv;
       const double gv = v1b * t1[c] + v2b * t2[c];
       sb.v[6 + c] += gv; sb.v[c] -= gv;
       sb.v[9 + c] += v1b * fj[3 + c] + v2b * g[3 + c];
       sb.v[3 + c] -= v1b * fj[c] + v2b * g[c];
-      fjb[3 + c] += v1b * m.v[9 + c]; gb[3 + c] += v2b * m.v[9 + c];
-      fjb[c] -= v1b * m.v[3 + c]; gb[c] -= v2b * m.v[3 + c];
+      fjb[3 + c] += v1b * mwb; gb[3 + c] += v2b * mwb;
+      fjb[c] -= v1b * mwa; gb[c] -= v2b * mwa;
     }
-    WB.tangent[2 * i] = in1b; WB.tangent[2 * i + 1] = in2b;
+    WB.tangent()[2 * i] = in1b; WB.tangent()[2 * i + 1] = in2b;
   }
   // normal
   double nextb[N], oldb[N];
   for (int k = 0; k < N; k++) {
-    nextb[k] = WB.impulse[P * i + k] + pressed_b + W.lever[P * i + k] * twisted_b;
-    WB.lever[P * i + k] += f.next[k] * twisted_b;
+    nextb[k] = WB.impulse()[P * i + k] + pressed_b + W.lever()[P * i + k] * twisted_b;
+    WB.lever()[P * i + k] += T[T_NEXT + k] * twisted_b;
     oldb[k] = 0;
   }
   for (int k = N - 1; k >= 0; k--) {
     const int j = P * i + k;
-    const double ddb = push_normal_vjp(sb, ima, imb, n, W.jac + J * j, f.dd[k], nb, imab, imbb, WB.jac + J * j);
+    const double ddb = push_normal_vjp(sb, ima, imb, n, W.jac() + J * j, T[T_DD + k], nb, imab, imbb, WB.jac() + J * j);
     nextb[k] += ddb;
     oldb[k] -= ddb;
   }
+  const uint32_t first = (flags >> 12) & 15;
   double closingb[N], targetb[N];
   for (int k = 0; k < N; k++) {
     closingb[k] = 0; targetb[k] = 0;
-    if ((f.first >> k) & 1) {
-      closingb[k] += WB.relvel[P * i + k];
-      WB.relvel[P * i + k] = 0;
+    if ((first >> k) & 1) {
+      closingb[k] += WB.relvel()[P * i + k];
+      WB.relvel()[P * i + k] = 0;
     }
   }
-  if (f.single) {
+  // what each point's target case made of its mass and impulse scales and give
+  const double springy = soft.impulse_scale / soft.mass_scale + COMPLIANCE;
+  auto tcase = [&](int k) { return (int)((flags >> (16 + 2 * k)) & 3); };
+  if (flags & 1) {
     for (int k = 0; k < N; k++) {
-      if (!(f.raw[k] > 0)) continue;  // clamped at 0 (or exactly 0)
+      if (!(T[T_RAW + k] > 0)) continue;  // clamped at 0 (or exactly 0)
       const double g = nextb[k];
       const int j = P * i + k;
-      const double nm = W.nmass[j];
-      oldb[k] += g * (1 - f.iscale[k]);
-      WB.nmass[j] -= g * f.mscale[k] * (f.closing[k] + f.target[k]);
-      closingb[k] -= g * nm * f.mscale[k];
-      targetb[k] -= g * nm * f.mscale[k];
+      const double nm = W.nmass()[j];
+      const bool soft_k = tcase(k) == 1 || tcase(k) == 2;
+      const double ms = soft_k ? soft.mass_scale : 1, is = soft_k ? soft.impulse_scale : 0;
+      oldb[k] += g * (1 - is);
+      WB.nmass()[j] -= g * ms * (T[T_CLOSING + k] + T[T_TARGET + k]);
+      closingb[k] -= g * nm * ms;
+      targetb[k] -= g * nm * ms;
     }
   } else {
     // solved on the set S: A_SS x = wanted_S, A = K with diagonal scaled by 1 + give
-    const double* Kg = W.block + P * P * i;
-    double* Kb = WB.block + P * P * i;
-    double xb[N], A[N][N];
+    const double* Kg = W.block() + P * P * i;
+    double* Kb = WB.block() + P * P * i;
+    const int set = (flags >> 8) & 15;
+    double xb[N], give[N], solved[N];
     int idx[N], ns = 0;
     for (int k = 0; k < N; k++) {
-      xb[k] = (f.solved[k] > 0 && ((f.set >> k) & 1)) ? nextb[k] : 0;
-      if ((f.set >> k) & 1) idx[ns++] = k;
+      solved[k] = T[T_SOLVED + k];
+      const bool soft_k = tcase(k) == 1 || tcase(k) == 2;
+      give[k] = soft_k ? springy : COMPLIANCE;
+      xb[k] = (solved[k] > 0 && ((set >> k) & 1)) ? nextb[k] : 0;
+      if ((set >> k) & 1) idx[ns++] = k;
     }
     // y = A_SS^-1 xb_S (A symmetric), by Gaussian elimination on the set
     double M[N][N + 1];
     for (int a = 0; a < ns; a++) {
       for (int b = 0; b < ns; b++) {
         const int ka = idx[a], kb = idx[b];
-        A[a][b] = Kg[P * ka + kb] * (ka == kb ? 1 + f.give[ka] : 1);
-        M[a][b] = A[a][b];
+        M[a][b] = Kg[P * ka + kb] * (ka == kb ? 1 + give[ka] : 1);
       }
       M[a][ns] = xb[idx[a]];
     }
@@ -1623,8 +1730,8 @@ HD void solve_vjp(const Work& W, const WorkBar& WB, const Rows& R, int r, int i,
       wantedb[idx[a]] = y[a];
       for (int b = 0; b < ns; b++) {
         const int ka = idx[a], kb = idx[b];
-        const double g = -y[a] * f.solved[kb];
-        Kb[P * ka + kb] += ka == kb ? g * (1 + f.give[ka]) : g;
+        const double g = -y[a] * solved[kb];
+        Kb[P * ka + kb] += ka == kb ? g * (1 + give[ka]) : g;
       }
     }
     // wanted_k = -closing_k - target_k + sum_l K_kl impulse_l
@@ -1634,20 +1741,22 @@ HD void solve_vjp(const Work& W, const WorkBar& WB, const Rows& R, int r, int i,
       closingb[k] -= g;
       targetb[k] -= g;
       for (int l = 0; l < N; l++) {
-        Kb[P * k + l] += g * in.impulse[l];
+        Kb[P * k + l] += g * T[T_IMP + l];
         oldb[l] += g * Kg[P * k + l];
       }
     }
   }
+  Pair s0;
+  for (int c = 0; c < 12; c++) s0.v[c] = T[T_S + c];
   for (int k = N - 1; k >= 0; k--) {
     const int j = P * i + k;
-    closing_vjp(in.s, n, W.jac + J * j, closingb[k], sb, nb, WB.jac + J * j);
-    WB.impulse[j] = oldb[k];
+    closing_vjp(s0, n, W.jac() + J * j, closingb[k], sb, nb, WB.jac() + J * j);
+    WB.impulse()[j] = oldb[k];
     // target(s)
     const double inv = 1 / h;
     double g = 0;
-    if (f.tcase[k] == 0) g = targetb[k] * inv;
-    else if (f.tcase[k] == 1) g = targetb[k] * soft.bias_rate;
+    if (tcase(k) == 0) g = targetb[k] * inv;
+    else if (tcase(k) == 1) g = targetb[k] * soft.bias_rate;
     sepb[k] = g;
   }
 }
@@ -1657,23 +1766,23 @@ HD void solve_vjp(const Work& W, const WorkBar& WB, const Rows& R, int r, int i,
 // through; the impulses, normal, tangents, lever terms and inertias gain theirs. The
 // impulses are the row's as the solve that follows read them (its tape).
 HD void warm_vjp(const Work& W, const WorkBar& WB, const Rows& R, int r, int i, double ima, double imb,
-                 const double* Ia, const double* Ib, const double* tape, const Pair& sb, double& imab, double& imbb,
+                 const double* Ia, const double* Ib, const TapeRow& T, const Pair& sb, double& imab, double& imbb,
                  double* Iab, double* Ibb) {
   const int count = R.count[r], i3 = 3 * i;
-  const double* n = W.normal + i3;
-  const double* t1 = W.t1 + i3;
-  const double* t2 = W.t2 + i3;
-  double* nb = WB.normal + i3;
-  double* t1b = WB.t1 + i3;
-  double* t2b = WB.t2 + i3;
+  const double* n = W.normal() + i3;
+  const double* t1 = W.t1() + i3;
+  const double* t2 = W.t2() + i3;
+  double* nb = WB.normal() + i3;
+  double* t1b = WB.t1() + i3;
+  double* t2b = WB.t2() + i3;
   for (int k = 0; k < count; k++) {
     const int j = P * i + k;
-    WB.impulse[j] += push_normal_vjp(sb, ima, imb, n, W.jac + J * j, tape[12 + k], nb, imab, imbb, WB.jac + J * j);
+    WB.impulse()[j] += push_normal_vjp(sb, ima, imb, n, W.jac() + J * j, T[T_IMP + k], nb, imab, imbb, WB.jac() + J * j);
   }
   {
-    const double* fj = W.fjac + FJ * i;
-    double* fjb = WB.fjac + FJ * i;
-    const double l1 = tape[16], l2 = tape[17];
+    const double* fj = W.fjac() + FJ * i;
+    double* fjb = WB.fjac() + FJ * i;
+    const double l1 = T[T_O1], l2 = T[T_O2];
     double l1b = 0, l2b = 0;
     for (int c = 0; c < 3; c++) {
       const double dv = imb * sb.v[6 + c] - ima * sb.v[c];
@@ -1686,19 +1795,19 @@ HD void warm_vjp(const Work& W, const WorkBar& WB, const Rows& R, int r, int i, 
       fjb[6 + c] -= l1 * sb.v[3 + c]; fjb[18 + c] -= l2 * sb.v[3 + c];
       fjb[9 + c] += l1 * sb.v[9 + c]; fjb[21 + c] += l2 * sb.v[9 + c];
     }
-    WB.tangent[2 * i] += l1b; WB.tangent[2 * i + 1] += l2b;
+    WB.tangent()[2 * i] += l1b; WB.tangent()[2 * i + 1] += l2b;
   }
   {
-    const double r1 = tape[18], r2 = tape[19], sp = tape[20];
+    const double r1 = T[T_R1], r2 = T[T_R2], sp = T[T_S0];
     const double L[3] = {t1[0] * r1 + t2[0] * r2 + n[0] * sp, t1[1] * r1 + t2[1] * r2 + n[1] * sp,
                          t1[2] * r1 + t2[2] * r2 + n[2] * sp};
     double Lb[3] = {0, 0, 0};
     push_turn_vjp(sb, Ia, Ib, L, Lb, Iab, Ibb);
     if (count <= 2) {
-      WB.roll[2 * i] += t1[0] * Lb[0] + t1[1] * Lb[1] + t1[2] * Lb[2];
-      WB.roll[2 * i + 1] += t2[0] * Lb[0] + t2[1] * Lb[1] + t2[2] * Lb[2];
+      WB.roll()[2 * i] += t1[0] * Lb[0] + t1[1] * Lb[1] + t1[2] * Lb[2];
+      WB.roll()[2 * i + 1] += t2[0] * Lb[0] + t2[1] * Lb[1] + t2[2] * Lb[2];
     }
-    WB.spin[i] += n[0] * Lb[0] + n[1] * Lb[1] + n[2] * Lb[2];
+    WB.spin()[i] += n[0] * Lb[0] + n[1] * Lb[1] + n[2] * Lb[2];
     for (int c = 0; c < 3; c++) { t1b[c] += r1 * Lb[c]; t2b[c] += r2 * Lb[c]; nb[c] += sp * Lb[c]; }
   }
 }
@@ -1732,8 +1841,8 @@ template <int N>
 HD void separations_vjp(const Bodies& B, const BodiesBar& BB, const Work& W, const WorkBar& WB, int ga, int gb, int i,
                         const double (&sepb)[N]) {
   const int i3 = 3 * i;
-  const double* n = W.normal + i3;
-  double* nb = WB.normal + i3;
+  const double* n = W.normal() + i3;
+  double* nb = WB.normal() + i3;
   const double *pa = B.p + 3 * ga, *pb = B.p + 3 * gb;
   const double *qa = B.start + 3 * ga, *qb = B.start + 3 * gb;
   const V3 na = rotate(B.q + 4 * ga, n[0], n[1], n[2], true);
@@ -1744,10 +1853,10 @@ HD void separations_vjp(const Bodies& B, const BodiesBar& BB, const Work& W, con
     const double g = sepb[k];
     if (g == 0) continue;
     dnb += g;
-    WB.adjusted[j] += g;
-    WB.lb[j3] += g * nbv.x; WB.lb[j3 + 1] += g * nbv.y; WB.lb[j3 + 2] += g * nbv.z;
-    WB.la[j3] -= g * na.x; WB.la[j3 + 1] -= g * na.y; WB.la[j3 + 2] -= g * na.z;
-    for (int c = 0; c < 3; c++) { nbb[c] += g * W.lb[j3 + c]; nab[c] -= g * W.la[j3 + c]; }
+    WB.adjusted()[j] += g;
+    WB.lb()[j3] += g * nbv.x; WB.lb()[j3 + 1] += g * nbv.y; WB.lb()[j3 + 2] += g * nbv.z;
+    WB.la()[j3] -= g * na.x; WB.la()[j3 + 1] -= g * na.y; WB.la()[j3 + 2] -= g * na.z;
+    for (int c = 0; c < 3; c++) { nbb[c] += g * W.lb()[j3 + c]; nab[c] -= g * W.la()[j3 + c]; }
   }
   for (int c = 0; c < 3; c++) {
     const double d = (pb[c] - qb[c]) - (pa[c] - qa[c]);
@@ -1768,10 +1877,13 @@ HD void separations_vjp(const Bodies& B, const BodiesBar& BB, const Work& W, con
 }
 
 // The reverse of one row of a sweep (KIND 0 warm, 1 solve, 2 relax), from the sweep's
-// tape; the bodies' positions and turns are where the sweep found them.
+// tape; the bodies' positions and turns are where the sweep found them. What the row adds
+// to its bodies' inverse mass and inertia adjoints stays with the row (WB.ima() ...) until
+// prepare's reverse hands it on, so a warm start's rows, which change no velocity
+// adjoint, can all go back at once.
 template <int N, int KIND>
 HD void row_vjp_n(const Bodies& B, const BodiesBar& BB, const Work& W, const WorkBar& WB, const Rows& R, int boff, int r,
-                  int i, const Soft& soft, double h, const double* tape) {
+                  int i, const Soft& soft, double h, const TapeRow& T) {
   const int ga = boff + R.a[r], gb = boff + R.b[r];
   const bool ma = moves(B, ga), mb = moves(B, gb);
   Pair sb;
@@ -1783,38 +1895,29 @@ HD void row_vjp_n(const Bodies& B, const BodiesBar& BB, const Work& W, const Wor
   const double* Ia = B.inv_i + 6 * ga;
   const double* Ib = B.inv_i + 6 * gb;
   double imab = 0, imbb = 0;
-  double* Iab = ma ? BB.inv_i + 6 * ga : nullptr;
-  double* Ibb = mb ? BB.inv_i + 6 * gb : nullptr;
+  double* Iab = WB.Ia() + 6 * i;
+  double* Ibb = WB.Ib() + 6 * i;
   if (KIND == 0) {
-    warm_vjp(W, WB, R, r, i, ima, imb, Ia, Ib, tape, sb, imab, imbb, Iab, Ibb);
+    warm_vjp(W, WB, R, r, i, ima, imb, Ia, Ib, T, sb, imab, imbb, Iab, Ibb);
   } else {
-    RowIn<N> in;
-    for (int k = 0; k < 12; k++) in.s.v[k] = tape[k];
-    for (int k = 0; k < N; k++) { in.impulse[k] = tape[12 + k]; in.most[k] = tape[21 + k]; }
-    in.o1 = tape[16]; in.o2 = tape[17]; in.r1 = tape[18]; in.r2 = tape[19]; in.s0 = tape[20];
-    separations<N>(B, W, ga, gb, i, false, in.sep);
     double sepb[N];
-    solve_vjp<N>(W, WB, R, r, i, ima, imb, Ia, Ib, soft, h, KIND == 1, in, sb, imab, imbb, Iab, Ibb, sepb);
+    solve_vjp<N>(W, WB, R, r, i, ima, imb, Ia, Ib, soft, h, KIND == 1, T, sb, imab, imbb, Iab, Ibb, sepb);
     separations_vjp<N>(B, BB, W, WB, ga, gb, i, sepb);
+    if (ma) for (int c = 0; c < 3; c++) { BB.v[3 * ga + c] = sb.v[c]; BB.w[3 * ga + c] = sb.v[3 + c]; }
+    if (mb) for (int c = 0; c < 3; c++) { BB.v[3 * gb + c] = sb.v[6 + c]; BB.w[3 * gb + c] = sb.v[9 + c]; }
   }
-  if (ma) {
-    for (int c = 0; c < 3; c++) { BB.v[3 * ga + c] = sb.v[c]; BB.w[3 * ga + c] = sb.v[3 + c]; }
-    BB.inv_mass[ga] += imab;
-  }
-  if (mb) {
-    for (int c = 0; c < 3; c++) { BB.v[3 * gb + c] = sb.v[6 + c]; BB.w[3 * gb + c] = sb.v[9 + c]; }
-    BB.inv_mass[gb] += imbb;
-  }
+  WB.ima()[i] += imab;
+  WB.imb()[i] += imbb;
 }
 
 template <int KIND>
 HD void row_vjp(const Bodies& B, const BodiesBar& BB, const Work& W, const WorkBar& WB, const Rows& R, int boff, int r,
-                int i, const Soft& soft, double h, const double* tape) {
+                int i, const Soft& soft, double h, const TapeRow& T) {
   switch (R.count[r]) {
-    case 1: row_vjp_n<1, KIND>(B, BB, W, WB, R, boff, r, i, soft, h, tape); break;
-    case 2: row_vjp_n<2, KIND>(B, BB, W, WB, R, boff, r, i, soft, h, tape); break;
-    case 3: row_vjp_n<3, KIND>(B, BB, W, WB, R, boff, r, i, soft, h, tape); break;
-    default: row_vjp_n<4, KIND>(B, BB, W, WB, R, boff, r, i, soft, h, tape); break;
+    case 1: row_vjp_n<1, KIND>(B, BB, W, WB, R, boff, r, i, soft, h, T); break;
+    case 2: row_vjp_n<2, KIND>(B, BB, W, WB, R, boff, r, i, soft, h, T); break;
+    case 3: row_vjp_n<3, KIND>(B, BB, W, WB, R, boff, r, i, soft, h, T); break;
+    default: row_vjp_n<4, KIND>(B, BB, W, WB, R, boff, r, i, soft, h, T); break;
   }
 }
 
@@ -1935,18 +2038,18 @@ HD void move_vjp(const Bodies& B, const BodiesBar& BB, int g, double h) {
 // The reverse of bounce_row, its points from the last.
 HD void bounce_vjp(const Bodies& B, const BodiesBar& BB, const Work& W, const WorkBar& WB, const Rows& R, int boff,
                    int r, int i, double gravity) {
-  const double e = W.e[i];
+  const double e = W.e()[i];
   if (e == 0) return;
   const double threshold = dsqrt(2 * gravity * R.tolerance[r]);
   const int ga = boff + R.a[r], gb = boff + R.b[r];
   const bool ma = moves(B, ga), mb = moves(B, gb);
-  const double* n = W.normal + 3 * i;
+  const double* n = W.normal() + 3 * i;
   const double ima = B.inv_mass[ga], imb = B.inv_mass[gb];
   for (int k = R.count[r] - 1; k >= 0; k--) {
     const int j = P * i + k;
-    if (W.relvel[j] > -threshold || W.most[j] == 0) continue;
-    const double* o = W.bounce + BOUNCE * j;
-    const double old = o[0], vn = o[1], rv = W.relvel[j], nm = W.nmass[j];
+    if (W.relvel()[j] > -threshold || W.most()[j] == 0) continue;
+    const double* o = W.bounce() + BOUNCE * j;
+    const double old = o[0], vn = o[1], rv = W.relvel()[j], nm = W.nmass()[j];
     Pair s;
     for (int c = 0; c < 12; c++) s.v[c] = o[2 + c];
     double raw = old - nm * (vn + e * rv);
@@ -1957,19 +2060,19 @@ HD void bounce_vjp(const Bodies& B, const BodiesBar& BB, const Work& W, const Wo
       sb.v[6 + c] = mb ? BB.v[3 * gb + c] : 0; sb.v[9 + c] = mb ? BB.w[3 * gb + c] : 0;
     }
     double imab = 0, imbb = 0;
-    const double lb = push_normal_vjp(sb, ima, imb, n, W.jac + J * j, next - old, WB.normal + 3 * i, imab, imbb,
-                                      WB.jac + J * j);
-    const double nextb = WB.impulse[j] + lb;
+    const double lb = push_normal_vjp(sb, ima, imb, n, W.jac() + J * j, next - old, WB.normal() + 3 * i, imab, imbb,
+                                      WB.jac() + J * j);
+    const double nextb = WB.impulse()[j] + lb;
     double oldb = -lb;
     if (raw > 0) {
       oldb += nextb;
-      WB.nmass[j] -= nextb * (vn + e * rv);
+      WB.nmass()[j] -= nextb * (vn + e * rv);
       const double vnb = -nm * nextb;
-      WB.e[i] -= nm * nextb * rv;
-      WB.relvel[j] -= nm * e * nextb;
-      closing_vjp(s, n, W.jac + J * j, vnb, sb, WB.normal + 3 * i, WB.jac + J * j);
+      WB.e()[i] -= nm * nextb * rv;
+      WB.relvel()[j] -= nm * e * nextb;
+      closing_vjp(s, n, W.jac() + J * j, vnb, sb, WB.normal() + 3 * i, WB.jac() + J * j);
     }
-    WB.impulse[j] = oldb;
+    WB.impulse()[j] = oldb;
     if (ma) {
       for (int c = 0; c < 3; c++) { BB.v[3 * ga + c] = sb.v[c]; BB.w[3 * ga + c] = sb.v[3 + c]; }
       BB.inv_mass[ga] += imab;
@@ -2014,32 +2117,32 @@ HD void store_vjp(const Bodies& B, const BodiesBar& BB, const Work& W, const Wor
                   const RowsBar& RB, int boff, int r, int i) {
   const int i3 = 3 * i, count = R.count[r];
   const int ga = boff + R.a[r], gb = boff + R.b[r];
-  const double* n = W.normal + i3;
-  const double* t1 = W.t1 + i3;
-  const double* t2 = W.t2 + i3;
-  double* nb = WB.normal + i3;
-  double* t1b = WB.t1 + i3;
-  double* t2b = WB.t2 + i3;
+  const double* n = W.normal() + i3;
+  const double* t1 = W.t1() + i3;
+  const double* t2 = W.t2() + i3;
+  double* nb = WB.normal() + i3;
+  double* t1b = WB.t1() + i3;
+  double* t2b = WB.t2() + i3;
   const double* Lb = RB.store_twist + 3 * r;
   const double* Fb = RB.store_friction + 3 * r;
   const double* Sb = RB.store_slip + 3 * r;
-  for (int k = 0; k < count; k++) WB.impulse[P * i + k] += RB.store_impulse[P * r + k];
-  const double r1 = W.roll[2 * i], r2 = W.roll[2 * i + 1], sp = W.spin[i];
-  const double l1 = W.tangent[2 * i], l2 = W.tangent[2 * i + 1];
-  const double d1 = W.drift[2 * i], d2 = W.drift[2 * i + 1];
+  for (int k = 0; k < count; k++) WB.impulse()[P * i + k] += RB.store_impulse[P * r + k];
+  const double r1 = W.roll()[2 * i], r2 = W.roll()[2 * i + 1], sp = W.spin()[i];
+  const double l1 = W.tangent()[2 * i], l2 = W.tangent()[2 * i + 1];
+  const double d1 = W.drift()[2 * i], d2 = W.drift()[2 * i + 1];
   double d1b = 0, d2b = 0;
   for (int c = 0; c < 3; c++) {
-    WB.roll[2 * i] += t1[c] * Lb[c]; WB.roll[2 * i + 1] += t2[c] * Lb[c]; WB.spin[i] += n[c] * Lb[c];
-    WB.tangent[2 * i] += t1[c] * Fb[c]; WB.tangent[2 * i + 1] += t2[c] * Fb[c];
+    WB.roll()[2 * i] += t1[c] * Lb[c]; WB.roll()[2 * i + 1] += t2[c] * Lb[c]; WB.spin()[i] += n[c] * Lb[c];
+    WB.tangent()[2 * i] += t1[c] * Fb[c]; WB.tangent()[2 * i + 1] += t2[c] * Fb[c];
     d1b += t1[c] * Sb[c]; d2b += t2[c] * Sb[c];
     t1b[c] += r1 * Lb[c] + l1 * Fb[c] + d1 * Sb[c];
     t2b[c] += r2 * Lb[c] + l2 * Fb[c] + d2 * Sb[c];
     nb[c] += sp * Lb[c];
   }
   double pressed = 0;
-  for (int k = 0; k < count; k++) pressed += W.impulse[P * i + k];
+  for (int k = 0; k < count; k++) pressed += W.impulse()[P * i + k];
   if (W.sliding[i] != 0 || pressed == 0 || impacted(R, r)) {
-    WB.drift[2 * i] = 0; WB.drift[2 * i + 1] = 0;
+    WB.drift()[2 * i] = 0; WB.drift()[2 * i + 1] = 0;
     return;
   }
   // drift_out = drift_in + (dx . t1, dx . t2)
@@ -2047,19 +2150,19 @@ HD void store_vjp(const Bodies& B, const BodiesBar& BB, const Work& W, const Wor
   double T[6];
   if (B.type[gb] == DYNAMIC) {
     turned_t<double>(B.p + 3 * gb, B.start + 3 * gb, B.q + 4 * gb, B.start_q + 4 * gb, T);
-    const double* rr = W.anchor_b + i3;
+    const double* rr = W.anchor_b() + i3;
     dx[0] += T[0] + T[4] * rr[2] - T[5] * rr[1];
     dx[1] += T[1] + T[5] * rr[0] - T[3] * rr[2];
     dx[2] += T[2] + T[3] * rr[1] - T[4] * rr[0];
   }
   if (B.type[ga] == DYNAMIC) {
     turned_t<double>(B.p + 3 * ga, B.start + 3 * ga, B.q + 4 * ga, B.start_q + 4 * ga, T);
-    const double* rr = W.anchor_a + i3;
+    const double* rr = W.anchor_a() + i3;
     dx[0] -= T[0] + T[4] * rr[2] - T[5] * rr[1];
     dx[1] -= T[1] + T[5] * rr[0] - T[3] * rr[2];
     dx[2] -= T[2] + T[3] * rr[1] - T[4] * rr[0];
   }
-  WB.drift[2 * i] = d1b; WB.drift[2 * i + 1] = d2b;
+  WB.drift()[2 * i] = d1b; WB.drift()[2 * i + 1] = d2b;
   double dxb[3];
   for (int c = 0; c < 3; c++) {
     dxb[c] = d1b * t1[c] + d2b * t2[c];
@@ -2071,8 +2174,8 @@ HD void store_vjp(const Bodies& B, const BodiesBar& BB, const Work& W, const Wor
     if (B.type[g] != DYNAMIC) continue;
     const double sg = side == 0 ? 1 : -1;
     turned_t<double>(B.p + 3 * g, B.start + 3 * g, B.q + 4 * g, B.start_q + 4 * g, T);
-    const double* rr = (side == 0 ? W.anchor_b : W.anchor_a) + i3;
-    double* rrb = (side == 0 ? WB.anchor_b : WB.anchor_a) + i3;
+    const double* rr = (side == 0 ? W.anchor_b() : W.anchor_a()) + i3;
+    double* rrb = (side == 0 ? WB.anchor_b() : WB.anchor_a()) + i3;
     const double cb[3] = {sg * dxb[0], sg * dxb[1], sg * dxb[2]};
     double Tb[6] = {cb[0], cb[1], cb[2], 0, 0, 0};
     cross_vjp(T + 3, rr, cb, Tb + 3, rrb);
@@ -2170,43 +2273,43 @@ HD void prepare_vjp(const Bodies& B, const BodiesBar& BB, const Sizes& Z, const 
   const double *wa = B.w + 3 * ga, *wb = B.w + 3 * gb;
   const double ima = B.inv_mass[ga], imb = B.inv_mass[gb];
   const int i3 = 3 * i, count = R.count[r];
-  const double* n = W.normal + i3;
-  const double* t1 = W.t1 + i3;
-  const double* t2 = W.t2 + i3;
-  double* nb = WB.normal + i3;
-  double* t1b = WB.t1 + i3;
-  double* t2b = WB.t2 + i3;
+  const double* n = W.normal() + i3;
+  const double* t1 = W.t1() + i3;
+  const double* t2 = W.t2() + i3;
+  double* nb = WB.normal() + i3;
+  double* t1b = WB.t1() + i3;
+  double* t2b = WB.t2() + i3;
   const double tol = R.tolerance[r];
   double ra[P][3], rb[P][3], rab[P][3], rbb[P][3], sepb[P];
   double near[P], weight = 0;
   for (int k = 0; k < count; k++) {
     const int j3 = 3 * (P * i + k);
     for (int c = 0; c < 3; c++) {
-      ra[k][c] = W.point[j3 + c] - pa[c];
-      rb[k][c] = W.point[j3 + c] + R.shift[3 * (P * r + k) + c] - pb[c];
+      ra[k][c] = W.point()[j3 + c] - pa[c];
+      rb[k][c] = W.point()[j3 + c] + R.shift[3 * (P * r + k) + c] - pb[c];
       rab[k][c] = 0; rbb[k][c] = 0;
     }
     sepb[k] = 0;
-    const double s = W.separation[P * i + k] / tol;
+    const double s = W.separation()[P * i + k] / tol;
     near[k] = s <= 1 ? 1 : s >= 2 ? 0 : 2 - s;
     weight += near[k];
   }
-  const double* A = W.anchor_a + i3;
-  const double* Bn = W.anchor_b + i3;
-  double Ab[3] = {WB.anchor_a[i3], WB.anchor_a[i3 + 1], WB.anchor_a[i3 + 2]};
-  double Bb[3] = {WB.anchor_b[i3], WB.anchor_b[i3 + 1], WB.anchor_b[i3 + 2]};
+  const double* A = W.anchor_a() + i3;
+  const double* Bn = W.anchor_b() + i3;
+  double Ab[3] = {WB.anchor_a()[i3], WB.anchor_a()[i3 + 1], WB.anchor_a()[i3 + 2]};
+  double Bb[3] = {WB.anchor_b()[i3], WB.anchor_b()[i3 + 1], WB.anchor_b()[i3 + 2]};
   // the tangent masses from the friction lever terms
   {
-    const double* f = W.fjac + FJ * i;
+    const double* f = W.fjac() + FJ * i;
     const double* g = f + 12;
     double fb[FJ];
-    for (int c = 0; c < FJ; c++) fb[c] = WB.fjac[FJ * i + c];
+    for (int c = 0; c < FJ; c++) fb[c] = WB.fjac()[FJ * i + c];
     const double k11 = ima + imb + f[0] * f[6] + f[1] * f[7] + f[2] * f[8] + f[3] * f[9] + f[4] * f[10] + f[5] * f[11];
     const double k22 = ima + imb + g[0] * g[6] + g[1] * g[7] + g[2] * g[8] + g[3] * g[9] + g[4] * g[10] + g[5] * g[11];
     const double k12 = g[0] * f[6] + g[1] * f[7] + g[2] * f[8] + g[3] * f[9] + g[4] * f[10] + g[5] * f[11];
     const double det = k11 * k22 - k12 * k12;
     if (det > 0) {
-      const double* tmb = WB.tm + i3;
+      const double* tmb = WB.tm() + i3;
       const double detb = (-k22 * tmb[0] + k12 * tmb[1] - k11 * tmb[2]) / (det * det);
       const double k11b = tmb[2] / det + detb * k22;
       const double k22b = tmb[0] / det + detb * k11;
@@ -2223,7 +2326,7 @@ HD void prepare_vjp(const Bodies& B, const BodiesBar& BB, const Sizes& Z, const 
   }
   // the twist levers: |e - (e . n) n|, e = ra - anchor a
   for (int k = 0; k < count; k++) {
-    const double lb = WB.lever[P * i + k];
+    const double lb = WB.lever()[P * i + k];
     if (lb == 0) continue;
     double e[3] = {ra[k][0] - A[0], ra[k][1] - A[1], ra[k][2] - A[2]};
     const double al = e[0] * n[0] + e[1] * n[1] + e[2] * n[2];
@@ -2250,19 +2353,19 @@ HD void prepare_vjp(const Bodies& B, const BodiesBar& BB, const Sizes& Z, const 
         rbb[k][c] += near[k] / weight * Bb[c];
         nearb += ((ra[k][c] - A[c]) * Ab[c] + (rb[k][c] - Bn[c]) * Bb[c]) / weight;
       }
-      const double s = W.separation[P * i + k] / tol;
+      const double s = W.separation()[P * i + k] / tol;
       if (s > 1 && s < 2) sepb[k] -= nearb / tol;
     }
   }
   // the block: K[k][l] = ima + imb + jac_k[0..5] . jac_l[6..11]
   if (count >= 2) {
-    const double* Kb = WB.block + P * P * i;
+    const double* Kb = WB.block() + P * P * i;
     for (int k = 0; k < count; k++) {
-      const double* jk = W.jac + J * (P * i + k);
-      double* jkb = WB.jac + J * (P * i + k);
+      const double* jk = W.jac() + J * (P * i + k);
+      double* jkb = WB.jac() + J * (P * i + k);
       for (int l = 0; l <= k; l++) {
-        const double* jl = W.jac + J * (P * i + l);
-        double* jlb = WB.jac + J * (P * i + l);
+        const double* jl = W.jac() + J * (P * i + l);
+        double* jlb = WB.jac() + J * (P * i + l);
         const double eb = k == l ? Kb[P * k + k] : Kb[P * k + l] + Kb[P * l + k];
         if (eb == 0) continue;
         imab += eb; imbb += eb;
@@ -2275,7 +2378,7 @@ HD void prepare_vjp(const Bodies& B, const BodiesBar& BB, const Sizes& Z, const 
   for (int k = 0; k < count; k++) {
     const int j = P * i + k, j3 = 3 * j;
     {
-      const double g = WB.relvel[j];
+      const double g = WB.relvel()[j];
       if (g != 0) {
         double dv[3];
         dv[0] = vb[0] + wb[1] * rb[k][2] - wb[2] * rb[k][1] - va[0] - wa[1] * ra[k][2] + wa[2] * ra[k][1];
@@ -2290,10 +2393,10 @@ HD void prepare_vjp(const Bodies& B, const BodiesBar& BB, const Sizes& Z, const 
         cross_vjp(wa, ra[k], mdvb, ma ? BB.w + 3 * ga : nullptr, rab[k]);
       }
     }
-    lever_vjp(Ia, Ib, ima, imb, ra[k], rb[k], n, WB.jac + J * j, WB.nmass[j], rab[k], rbb[k], nb, Iab, Ibb, imab, imbb);
+    lever_vjp(Ia, Ib, ima, imb, ra[k], rb[k], n, WB.jac() + J * j, WB.nmass()[j], rab[k], rbb[k], nb, Iab, Ibb, imab, imbb);
     {
       // adjusted = separation - ((cb - ca) . n)
-      const double g = WB.adjusted[j];
+      const double g = WB.adjusted()[j];
       double la_[3], lb_[3];
       const V3 ca = core(B, Z, ga, ra[k][0], ra[k][1], ra[k][2], la_);
       const V3 cbv = core(B, Z, gb, rb[k][0], rb[k][1], rb[k][2], lb_);
@@ -2301,28 +2404,28 @@ HD void prepare_vjp(const Bodies& B, const BodiesBar& BB, const Sizes& Z, const 
       nb[0] -= g * (cbv.x - ca.x); nb[1] -= g * (cbv.y - ca.y); nb[2] -= g * (cbv.z - ca.z);
       const double cab[3] = {g * n[0], g * n[1], g * n[2]};
       const double cbb[3] = {-g * n[0], -g * n[1], -g * n[2]};
-      core_vjp(B, BB, Z, ga, ra[k], WB.la + j3, cab, rab[k]);
-      core_vjp(B, BB, Z, gb, rb[k], WB.lb + j3, cbb, rbb[k]);
+      core_vjp(B, BB, Z, ga, ra[k], WB.la() + j3, cab, rab[k]);
+      core_vjp(B, BB, Z, gb, rb[k], WB.lb() + j3, cbb, rbb[k]);
     }
     for (int c = 0; c < 3; c++) {
-      WB.point[j3 + c] += rab[k][c] + rbb[k][c];
+      WB.point()[j3 + c] += rab[k][c] + rbb[k][c];
       if (ma) BB.p[3 * ga + c] -= rab[k][c];
       if (mb) BB.p[3 * gb + c] -= rbb[k][c];
     }
-    WB.separation[j] += sepb[k];
+    WB.separation()[j] += sepb[k];
   }
   if (count <= 2) {
-    turning_vjp(Ia, Ib, t1, WB.rm[2 * i], Iab, Ibb, t1b);
-    turning_vjp(Ia, Ib, t2, WB.rm[2 * i + 1], Iab, Ibb, t2b);
+    turning_vjp(Ia, Ib, t1, WB.rm()[2 * i], Iab, Ibb, t1b);
+    turning_vjp(Ia, Ib, t2, WB.rm()[2 * i + 1], Iab, Ibb, t2b);
   }
-  turning_vjp(Ia, Ib, n, WB.sm[i], Iab, Ibb, nb);
+  turning_vjp(Ia, Ib, n, WB.sm()[i], Iab, Ibb, nb);
   if (ma) {
-    for (int c = 0; c < 6; c++) BB.inv_i[6 * ga + c] += Iab[c];
-    BB.inv_mass[ga] += imab;
+    for (int c = 0; c < 6; c++) BB.inv_i[6 * ga + c] += Iab[c] + WB.Ia()[6 * i + c];
+    BB.inv_mass[ga] += imab + WB.ima()[i];
   }
   if (mb) {
-    for (int c = 0; c < 6; c++) BB.inv_i[6 * gb + c] += Ibb[c];
-    BB.inv_mass[gb] += imbb;
+    for (int c = 0; c < 6; c++) BB.inv_i[6 * gb + c] += Ibb[c] + WB.Ib()[6 * i + c];
+    BB.inv_mass[gb] += imbb + WB.imb()[i];
   }
 }
 
@@ -2331,9 +2434,9 @@ HD void prepare_vjp(const Bodies& B, const BodiesBar& BB, const Sizes& Z, const 
 HD void place_vjp(const Bodies& B, const BodiesBar& BB, const Work& W, const WorkBar& WB, const Rows& R,
                   const RowsBar& RB, int boff, int r, int i) {
   const int ga = boff + R.a[r], gb = boff + R.b[r], count = R.count[r], i3 = 3 * i;
-  const double* t1 = W.t1 + i3;
-  const double* t2 = W.t2 + i3;
-  const double* n = W.normal + i3;
+  const double* t1 = W.t1() + i3;
+  const double* t2 = W.t2() + i3;
+  const double* n = W.normal() + i3;
   // carry
   const int s = R.src[r];
   if (s >= 0) {
@@ -2341,21 +2444,21 @@ HD void place_vjp(const Bodies& B, const BodiesBar& BB, const Work& W, const Wor
     const double* F = R.store_friction + 3 * s;
     double* Lb = RB.store_twist + 3 * s;
     double* Fb = RB.store_friction + 3 * s;
-    const double r1b = count <= 2 ? WB.roll[2 * i] : 0, r2b = count <= 2 ? WB.roll[2 * i + 1] : 0;
-    const double sb = WB.spin[i], o1b = WB.tangent[2 * i], o2b = WB.tangent[2 * i + 1];
+    const double r1b = count <= 2 ? WB.roll()[2 * i] : 0, r2b = count <= 2 ? WB.roll()[2 * i + 1] : 0;
+    const double sb = WB.spin()[i], o1b = WB.tangent()[2 * i], o2b = WB.tangent()[2 * i + 1];
     for (int c = 0; c < 3; c++) {
       Lb[c] += r1b * t1[c] + r2b * t2[c] + sb * n[c];
       Fb[c] += o1b * t1[c] + o2b * t2[c];
-      WB.t1[i3 + c] += r1b * L[c] + o1b * F[c];
-      WB.t2[i3 + c] += r2b * L[c] + o2b * F[c];
-      WB.normal[i3 + c] += sb * L[c];
+      WB.t1()[i3 + c] += r1b * L[c] + o1b * F[c];
+      WB.t2()[i3 + c] += r2b * L[c] + o2b * F[c];
+      WB.normal()[i3 + c] += sb * L[c];
     }
     bool same = false;
     for (int k = 0; k < count; k++) {
       const uint32_t id = R.pid[P * r + k];
       for (int j = 0; j < R.count[s]; j++) {
         if (R.pid[P * s + j] != id) continue;
-        RB.store_impulse[P * s + j] += WB.impulse[P * i + k];
+        RB.store_impulse[P * s + j] += WB.impulse()[P * i + k];
         same = true;
         break;
       }
@@ -2363,16 +2466,16 @@ HD void place_vjp(const Bodies& B, const BodiesBar& BB, const Work& W, const Wor
     if (same) {
       const double* S = R.store_slip + 3 * s;
       double* Sb = RB.store_slip + 3 * s;
-      const double d1b = WB.drift[2 * i], d2b = WB.drift[2 * i + 1];
+      const double d1b = WB.drift()[2 * i], d2b = WB.drift()[2 * i + 1];
       for (int c = 0; c < 3; c++) {
         Sb[c] += d1b * t1[c] + d2b * t2[c];
-        WB.t1[i3 + c] += d1b * S[c];
-        WB.t2[i3 + c] += d2b * S[c];
+        WB.t1()[i3 + c] += d1b * S[c];
+        WB.t2()[i3 + c] += d2b * S[c];
       }
     }
   }
-  RB.mu[r] += WB.mu[i];
-  RB.e[r] += WB.e[i];
+  RB.mu[r] += WB.mu()[i];
+  RB.e[r] += WB.e()[i];
   // place
   const bool ma = moves(B, ga), mb = moves(B, gb);
   if (!ma && !mb) return;
@@ -2380,10 +2483,10 @@ HD void place_vjp(const Bodies& B, const BodiesBar& BB, const Work& W, const Wor
   double in[14], outb[9 + 4 * P], inb[14];
   for (int c = 0; c < 3; c++) { in[c] = B.p[3 * ga + c]; in[7 + c] = B.p[3 * gb + c]; }
   for (int c = 0; c < 4; c++) { in[3 + c] = B.q[4 * ga + c]; in[10 + c] = B.q[4 * gb + c]; }
-  for (int c = 0; c < 3; c++) { outb[c] = WB.normal[i3 + c]; outb[3 + c] = WB.t1[i3 + c]; outb[6 + c] = WB.t2[i3 + c]; }
+  for (int c = 0; c < 3; c++) { outb[c] = WB.normal()[i3 + c]; outb[3 + c] = WB.t1()[i3 + c]; outb[6 + c] = WB.t2()[i3 + c]; }
   for (int k = 0; k < P; k++) {
-    for (int c = 0; c < 3; c++) outb[9 + 4 * k + c] = k < count ? WB.point[3 * (P * i + k) + c] : 0;
-    outb[9 + 4 * k + 3] = k < count ? WB.separation[P * i + k] : 0;
+    for (int c = 0; c < 3; c++) outb[9 + 4 * k + c] = k < count ? WB.point()[3 * (P * i + k) + c] : 0;
+    outb[9 + 4 * k + 3] = k < count ? WB.separation()[P * i + k] : 0;
   }
   for (int c = 0; c < 14; c++) inb[c] = 0;
   vjp<7>(in, outb, inb, [&](const Dual<7>* x, Dual<7>* y) {
@@ -2419,7 +2522,7 @@ HD void inertia_vjp(const Bodies& B, const BodiesBar& BB, int g) {
 
 // The carried drift of row r, as carry_row left it (store_row adds the step's slip).
 HD void carried_drift(const Rows& R, const Work& W, int r, int i) {
-  W.drift[2 * i] = 0; W.drift[2 * i + 1] = 0;
+  W.drift()[2 * i] = 0; W.drift()[2 * i + 1] = 0;
   const int s = R.src[r];
   if (s < 0) return;
   bool same = false;
@@ -2427,27 +2530,28 @@ HD void carried_drift(const Rows& R, const Work& W, int r, int i) {
     for (int j = 0; j < R.count[s]; j++)
       if (R.pid[P * s + j] == R.pid[P * r + k]) { same = true; break; }
   if (!same) return;
-  const double *t1 = W.t1 + 3 * i, *t2 = W.t2 + 3 * i, *S = R.store_slip + 3 * s;
-  W.drift[2 * i] = S[0] * t1[0] + S[1] * t1[1] + S[2] * t1[2];
-  W.drift[2 * i + 1] = S[0] * t2[0] + S[1] * t2[1] + S[2] * t2[2];
+  const double *t1 = W.t1() + 3 * i, *t2 = W.t2() + 3 * i, *S = R.store_slip + 3 * s;
+  W.drift()[2 * i] = S[0] * t1[0] + S[1] * t1[1] + S[2] * t1[2];
+  W.drift()[2 * i + 1] = S[0] * t2[0] + S[1] * t2[1] + S[2] * t2[2];
 }
 
 HD void zero_row_bars(const WorkBar& WB, int i) {
   for (int k = 0; k < P; k++) {
     const int j = P * i + k;
-    WB.impulse[j] = 0; WB.relvel[j] = 0; WB.nmass[j] = 0; WB.lever[j] = 0; WB.adjusted[j] = 0;
-    WB.separation[j] = 0;
-    for (int c = 0; c < 3; c++) { WB.la[3 * j + c] = 0; WB.lb[3 * j + c] = 0; WB.point[3 * j + c] = 0; }
-    for (int c = 0; c < J; c++) WB.jac[J * j + c] = 0;
+    WB.impulse()[j] = 0; WB.relvel()[j] = 0; WB.nmass()[j] = 0; WB.lever()[j] = 0; WB.adjusted()[j] = 0;
+    WB.separation()[j] = 0;
+    for (int c = 0; c < 3; c++) { WB.la()[3 * j + c] = 0; WB.lb()[3 * j + c] = 0; WB.point()[3 * j + c] = 0; }
+    for (int c = 0; c < J; c++) WB.jac()[J * j + c] = 0;
   }
-  for (int c = 0; c < P * P; c++) WB.block[P * P * i + c] = 0;
-  for (int c = 0; c < FJ; c++) WB.fjac[FJ * i + c] = 0;
-  for (int c = 0; c < 2; c++) { WB.tangent[2 * i + c] = 0; WB.roll[2 * i + c] = 0; WB.drift[2 * i + c] = 0; WB.rm[2 * i + c] = 0; }
+  for (int c = 0; c < P * P; c++) WB.block()[P * P * i + c] = 0;
+  for (int c = 0; c < FJ; c++) WB.fjac()[FJ * i + c] = 0;
+  for (int c = 0; c < 2; c++) { WB.tangent()[2 * i + c] = 0; WB.roll()[2 * i + c] = 0; WB.drift()[2 * i + c] = 0; WB.rm()[2 * i + c] = 0; }
   for (int c = 0; c < 3; c++) {
-    WB.tm[3 * i + c] = 0; WB.normal[3 * i + c] = 0; WB.t1[3 * i + c] = 0; WB.t2[3 * i + c] = 0;
-    WB.anchor_a[3 * i + c] = 0; WB.anchor_b[3 * i + c] = 0;
+    WB.tm()[3 * i + c] = 0; WB.normal()[3 * i + c] = 0; WB.t1()[3 * i + c] = 0; WB.t2()[3 * i + c] = 0;
+    WB.anchor_a()[3 * i + c] = 0; WB.anchor_b()[3 * i + c] = 0;
   }
-  WB.spin[i] = 0; WB.sm[i] = 0; WB.mu[i] = 0; WB.e[i] = 0;
+  WB.spin()[i] = 0; WB.sm()[i] = 0; WB.mu()[i] = 0; WB.e()[i] = 0; WB.ima()[i] = 0; WB.imb()[i] = 0;
+  for (int c = 0; c < 6; c++) { WB.Ia()[6 * i + c] = 0; WB.Ib()[6 * i + c] = 0; }
 }
 
 // Rows of the step in colour order, a barrier after each colour, the overflow in order:
@@ -2505,7 +2609,10 @@ HD void step_reverse(const Ctx& c, Sync sync, const Bodies& B, const BodiesBar& 
     sync();
     if (touching) {
       sweep_vjp<1>(c, sync, B, BB, W, WB, R, v, soft, h, tape, 2 * sub);
-      sweep_vjp<0>(c, sync, B, BB, W, WB, R, v, soft, h, tape, 2 * sub);
+      // a warm start's rows leave every velocity adjoint as it is: all of them at once
+      for (int lm = c.tid; lm < nrows; lm += c.threads)
+        row_vjp<0>(B, BB, W, WB, R, v.boff, v.r0 + lm, v.w0 + lm, soft, h, tape.row(2 * sub, v.w0 + lm));
+      sync();
     }
     for (int s = c.tid; s < v.nbodies; s += c.threads) {
       const int g = v.boff + s;

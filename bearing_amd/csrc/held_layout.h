@@ -124,22 +124,12 @@ inline size_t lay_out(const bm_held& h, char* base, Trajectory& T, HeldIO& io, s
   RB.store_impulse = d(P * rows); RB.store_friction = d(3 * rows); RB.store_twist = d(3 * rows);
   RB.store_slip = d(3 * rows); RB.mu = d(rows); RB.e = d(rows);
   Work& W = T.W;
-  W.point = d(3 * P * work); W.separation = d(P * work); W.normal = d(3 * work); W.t1 = d(3 * work);
-  W.t2 = d(3 * work); W.mu = d(work); W.e = d(work); W.impulse = d(P * work); W.tangent = d(2 * work);
-  W.roll = d(2 * work); W.spin = d(work); W.drift = d(2 * work); W.most = d(P * work); W.relvel = d(P * work);
-  W.now = d(P * work); W.ra = d(3 * P * work); W.rb = d(3 * P * work); W.la = d(3 * P * work);
-  W.lb = d(3 * P * work); W.adjusted = d(P * work); W.nmass = d(P * work); W.jac = d(J * P * work);
-  W.block = d(P * P * work); W.lever = d(P * work); W.anchor_a = d(3 * work); W.anchor_b = d(3 * work);
-  W.fjac = d(FJ * work); W.tm = d(3 * work); W.rm = d(2 * work); W.sm = d(work); W.bounce = d(BOUNCE * P * work);
+  W.n = work;
+  W.base = d((size_t)Work::WIDTH * work);
   W.sliding = reinterpret_cast<uint8_t*>(take(work));
   WorkBar& WB = T.WB;
-  WB.impulse = d(P * work); WB.tangent = d(2 * work); WB.roll = d(2 * work); WB.spin = d(work);
-  WB.drift = d(2 * work); WB.relvel = d(P * work); WB.jac = d(J * P * work); WB.fjac = d(FJ * work);
-  WB.tm = d(3 * work); WB.nmass = d(P * work); WB.block = d(P * P * work); WB.rm = d(2 * work); WB.sm = d(work);
-  WB.lever = d(P * work); WB.la = d(3 * P * work); WB.lb = d(3 * P * work); WB.adjusted = d(P * work);
-  WB.normal = d(3 * work); WB.t1 = d(3 * work); WB.t2 = d(3 * work); WB.point = d(3 * P * work);
-  WB.separation = d(P * work); WB.mu = d(work); WB.e = d(work); WB.anchor_a = d(3 * work);
-  WB.anchor_b = d(3 * work);
+  WB.n = work;
+  WB.base = d((size_t)WorkBar::WIDTH * work);
   T.checkpoints = d((size_t)(h.steps / h.every + 1) * nb * STATE);
   T.row_tape = d((size_t)2 * most_sub * work * ROW_TAPE);
   T.body_tape = d((size_t)h.every * 2 * most_sub * nb * STATE);
