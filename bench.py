@@ -63,11 +63,10 @@ def algorithmic_bytes(points, dynamic_a, dynamic_b):
 
 def load_world(name):
     import fixtures
-    from oracle import binding
     from softcheck import world_from_columns
 
     fx = fixtures.load(os.path.join(fixtures.DATA, name + ".bmfx.gz"))
-    return fx, binding.columns(fx), world_from_columns
+    return fx, fixtures.soft_columns(fx), world_from_columns
 
 
 def cpu_baseline(fx, seconds):
@@ -85,6 +84,18 @@ def cpu_baseline(fx, seconds):
         done += 1
     return {"value": rows * 2 * SUBSTEPS * done / elapsed, "unit": "contact-iterations/s", "cores": 1, "kind": "port",
             "sample": f"{done} soft steps of {WORKLOAD} ({rows} rows) in {elapsed:.2f} s on one host thread"}
+
+
+def cpu_baseline_worlds():
+    """The CPU baseline for the whole-world runs: the same trajectories stepped on one host
+    thread with the oracle standing in for every device call (tests/hostworld), ms a step."""
+    import ctypes
+
+    host_lib = os.path.join(ROOT, "tests", "hostworld", "_build", "libhostworld.so")
+    if not os.path.exists(host_lib):
+        return {}
+    lib = ctypes.CDLL(host_lib)
+    return {name: world_run(name, lib)["ms_per_step"] for name in WORLD_RUNS if name not in GPU_ONLY_RUNS}
 
 
 WORLD_RUNS = ["pyramid", "rain", "darts", "terrain", "mixed_pile"]
@@ -199,7 +210,7 @@ def adjoint_run(worlds=512, steps=120, every=10, threads=16, repeats=3):
             "held_vs_recorded_max_position_m": held_dev, "gradients_finite": finite}
 
 
-def world_run(name, lib=None):
+def world_run(name, lib=None, steps=None):
     """A whole reference trajectory stepped through bm_world (broadphase, narrow driver,
     device meet / colour / soft step, catch, sleep): wall time per step and whether every
     step's transforms are the reference's bits (checked after the timed run)."""
@@ -219,8 +230,9 @@ def world_run(name, lib=None):
     kept = []
     state = WorldState()
     state.transforms = out["transforms"].ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    n = meta["steps"] if steps is None else min(steps, meta["steps"])
     t0 = time.perf_counter()
-    for _ in range(meta["steps"]):
+    for _ in range(n):
         w.step(meta["dt"])
         w.lib.bm_world_read(w.ptr, ctypes.byref(state))
         kept.append(out["transforms"].tobytes())
@@ -228,13 +240,52 @@ def world_run(name, lib=None):
     parts = None
     if before is not None:
         after = w.times()
-        parts = {k: (after[k] - before[k]) / meta["steps"] for k in after}
+        parts = {k: (after[k] - before[k]) / n for k in after}
     w.close()
     exact = all(hashlib.sha256(t).hexdigest() == h for t, h in zip(kept, meta["hashes"]))
-    awake = int(np.count_nonzero(fx["awake"]))
-    return {"steps": meta["steps"], "bodies": meta["bodies"], "awake_steps": awake, "ms_per_step": wall * 1000 / meta["steps"],
-            "ms_per_awake_step": wall * 1000 / max(awake, 1), "world_steps_per_s": meta["steps"] / wall,
+    awake = int(np.count_nonzero(fx["awake"][:n]))
+    return {"steps": n, "bodies": meta["bodies"], "awake_steps": awake, "ms_per_step": wall * 1000 / n,
+            "ms_per_awake_step": wall * 1000 / max(awake, 1), "world_steps_per_s": n / wall,
             "bit_exact": bool(exact), "ms_per_step_by_part": parts}
+
+
+def batched_soft_run(ctx, args):
+    """Worlds of five scenes' soft steps, repeated to --batch-worlds, in one launch, against the
+    same kernel looped one world a launch on a sample of them."""
+    from bearing_amd import native
+    from bearing_amd.tune import tune
+
+    # Batched: worlds of five scenes, repeated to --batch-worlds, in one launch.
+    sources = [load_world(n)[:2] for n in BATCH_SOURCES]
+    worlds = [world_from_columns(sources[i % len(sources)][0], {k: v.copy() for k, v in sources[i % len(sources)][1].items()})
+              for i in range(args.batch_worlds)]
+    if args.batched_launch:
+        bshape, bresident = (int(x) for x in args.batched_launch.split(","))
+    else:
+        bshape, bresident, _ = tune(worlds)
+    ctx.set_launch(bshape, bresident)
+    big = native.Batch(worlds)
+    big.upload(ctx)
+    big.step(ctx)
+    t0 = time.perf_counter()
+    for _ in range(args.batch_steps):
+        big.step(ctx)
+    batched_wall = time.perf_counter() - t0
+    batched_rate = args.batch_worlds * args.batch_steps / batched_wall
+    # The same kernel looped once per world (one world a launch), on a sample of the same worlds.
+    sample = min(64, args.batch_worlds)
+    singles = [native.Batch([w]) for w in worlds[:sample]]
+    t_loop = 0.0
+    for s in singles:
+        s.upload(ctx)
+        s.step(ctx)
+        t = time.perf_counter()
+        s.step(ctx)
+        t_loop += time.perf_counter() - t
+    loop_rate = sample / t_loop
+    return {"worlds": args.batch_worlds, "steps": args.batch_steps, "world_steps_per_s": batched_rate,
+            "looped_plain_world_steps_per_s": loop_rate, "speedup_vs_looped": batched_rate / loop_rate,
+            "worlds_from": BATCH_SOURCES, "launch": {"shape": bshape, "resident_lds": bool(bresident)}}
 
 
 def main():
@@ -255,6 +306,9 @@ def main():
     ap.add_argument("--batched-launch", default=None, help="shape,resident for the batch (default: autotuned)")
     ap.add_argument("--world", default=None,
                     help="only step this reference trajectory whole through bm_world and print its timing")
+    ap.add_argument("--world-steps", type=int, default=None, help="with --world: only its first steps")
+    ap.add_argument("--parts", default="plain,batched,whole,worlds,adjoint,cpu",
+                    help="which legs of the full line to run (the rest are null): plain,batched,whole,worlds,adjoint,cpu")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy this many MiB device to device 5 times (torch), to calibrate HBM byte counters")
     args = ap.parse_args()
@@ -273,7 +327,7 @@ def main():
         sys.exit(2)
 
     if args.world:
-        print(json.dumps({"world": args.world, **world_run(args.world)}))
+        print(json.dumps({"world": args.world, **world_run(args.world, steps=args.world_steps)}))
         return
     if args.adjoint:
         print(json.dumps({"adjoint": adjoint_run(args.adjoint)}))
@@ -321,50 +375,20 @@ def main():
     wide_ms = float(np.mean([batch.step(ctx) for _ in range(10)]))
     ctx.set_wide(16384)
 
-    # Batched: worlds of five scenes, repeated to --batch-worlds, in one launch.
-    sources = [load_world(n)[:2] for n in BATCH_SOURCES]
-    worlds = [world_from_columns(sources[i % len(sources)][0], {k: v.copy() for k, v in sources[i % len(sources)][1].items()})
-              for i in range(args.batch_worlds)]
-    if args.batched_launch:
-        bshape, bresident = (int(x) for x in args.batched_launch.split(","))
-    else:
-        bshape, bresident, _ = tune(worlds)
-    ctx.set_launch(bshape, bresident)
-    big = native.Batch(worlds)
-    big.upload(ctx)
-    big.step(ctx)
-    t0 = time.perf_counter()
-    for _ in range(args.batch_steps):
-        big.step(ctx)
-    batched_wall = time.perf_counter() - t0
-    batched_rate = args.batch_worlds * args.batch_steps / batched_wall
-    # The same kernel looped once per world (one world a launch), on a sample of the same worlds.
-    sample = min(64, args.batch_worlds)
-    singles = [native.Batch([w]) for w in worlds[:sample]]
-    t_loop = 0.0
-    for s in singles:
-        s.upload(ctx)
-        s.step(ctx)
-        t = time.perf_counter()
-        s.step(ctx)
-        t_loop += time.perf_counter() - t
-    loop_rate = sample / t_loop
+    parts = set(args.parts.split(","))
+    batched = None
+    if "batched" in parts:
+        batched = batched_soft_run(ctx, args)
     ctx.close()
-
-    whole = batched_world_run(args.batch_worlds, 60, 20, args.batch_threads)
-    worlds_run = {name: world_run(name) for name in WORLD_RUNS}
-    # The same whole steps on the host with the oracle standing in for every device call
-    # (tests/hostworld): the port's one-thread time for the same trajectories.
-    import ctypes
-    host_lib = os.path.join(ROOT, "tests", "hostworld", "_build", "libhostworld.so")
-    if os.path.exists(host_lib):
-        lib = ctypes.CDLL(host_lib)
-        for name in WORLD_RUNS:
-            if name not in GPU_ONLY_RUNS:
-                worlds_run[name]["cpu_port_ms_per_step"] = world_run(name, lib)["ms_per_step"]
-
-    adjoint = adjoint_run(args.adjoint_worlds)
-    cpu = cpu_baseline(fx, args.cpu_seconds)
+    whole = batched_world_run(args.batch_worlds, 60, 20, args.batch_threads) if "whole" in parts else None
+    worlds_run = {name: world_run(name) for name in WORLD_RUNS} if "worlds" in parts else None
+    adjoint = adjoint_run(args.adjoint_worlds) if "adjoint" in parts else None
+    cpu = None
+    if "cpu" in parts:
+        cpu = cpu_baseline(fx, args.cpu_seconds)
+        if worlds_run:
+            for name, ms in cpu_baseline_worlds().items():
+                worlds_run[name]["cpu_port_ms_per_step"] = ms
     # HBM bytes per launch of the plain kernel, from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs
     # recorded in profiles/ (scaled by a copy kernel of known size); null if not measured
     # for this launch shape.
@@ -387,13 +411,11 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": step_bytes},
         "cpu_baseline": cpu,
-        "batched": {"worlds": args.batch_worlds, "steps": args.batch_steps, "world_steps_per_s": batched_rate,
-                    "looped_plain_world_steps_per_s": loop_rate, "speedup_vs_looped": batched_rate / loop_rate,
-                    "worlds_from": BATCH_SOURCES, "launch": {"shape": bshape, "resident_lds": bool(bresident)}},
+        "batched": batched,
         "batched_whole_worlds": whole,
         "world": worlds_run,
         "adjoint": adjoint,
-        "bwd_fwd": adjoint["bwd_fwd"],
+        "bwd_fwd": adjoint["bwd_fwd"] if adjoint else None,
     }
     print(json.dumps(line))
 

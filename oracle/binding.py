@@ -92,23 +92,10 @@ def colour(a, b, top, type_, frozen, frozen_count):
 
 
 def columns(fx):
-    """The soft step's columns from a fixture's 'in.*' arrays, as fresh numpy arrays by
-    fixture column name: bodies as ``b.<name>``, manifolds ``c.<name>``, pairs ``p.<name>``."""
-    meta = fx.meta
-    top, count, pairs = meta["top"], meta["count"], meta["pairs"]
-    cols = {}
-    for _, kind, name, per in BODY:
-        src = fx.get("in." + name)
-        cols["b." + name] = src.copy() if src is not None else np.zeros(top * per, _NP[kind])
-    for _, kind, name, per in MANIFOLD:
-        src = fx.get("in.c." + name)
-        cols["c." + name] = src.copy() if src is not None else np.zeros(count * per, _NP[kind])
-    for _, kind, name, per in PAIR:
-        cols["p." + name] = np.zeros(max(pairs, 1) * per, _NP[kind])
-    cols["awake_list"] = fx["in.awakeList"].copy()
-    cols["starts"] = fx["in.starts"].copy()
-    cols["gravity"] = fx["in.gravity"].copy()
-    return cols
+    """The soft step's columns from a fixture's 'in.*' arrays (tests/golden/fixtures.py
+    soft_columns: bodies as ``b.<name>``, manifolds ``c.<name>``, pairs ``p.<name>``)."""
+    import fixtures
+    return fixtures.soft_columns(fx)
 
 
 def soft_step(fx, cols=None):
