@@ -239,9 +239,10 @@ int bm_set_launch(bm_context* ctx, int32_t shape, int32_t resident);
 int bm_set_device_only(bm_context* ctx, int32_t keep);
 
 /* When bm_soft_step spreads one world over the whole device: a batch of one world with at
- * least min_manifolds manifolds is stepped by every workgroup the device can keep resident,
- * each colour's manifolds spread over all of them with a grid barrier between colours (0:
- * never; default 16384). Results are the same bits either way. */
+ * least min_manifolds manifolds (a quarter of that if the world's velocities do not fit in one
+ * workgroup's LDS) is stepped by every workgroup the device can keep resident, each colour's
+ * manifolds spread over all of them with a grid barrier between colours (0: never; default
+ * 16384). Results are the same bits either way. */
 int bm_set_wide(bm_context* ctx, int32_t min_manifolds);
 
 /* How bm_colour runs: allow = 1 (the default) colours each world in one workgroup's LDS,
