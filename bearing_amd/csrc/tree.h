@@ -42,9 +42,15 @@ class Tree {
   }
 
   // src/tree.js:288-334 (raycast): the ray from o along d, boxes grown by g, nearer child
-  // first; visit(item) returns the new maxT. Returns the last maxT.
+  // first; visit(item) returns the new maxT. Returns the last maxT. The walk's stack is the
+  // tree's own, or the caller's (so that several threads can walk one tree at once).
   template <class F>
   double raycast(const double* o, const double* d, double max_t, const double* g, F&& visit) {
+    return raycast(o, d, max_t, g, visit, stack, at);
+  }
+  template <class F>
+  double raycast(const double* o, const double* d, double max_t, const double* g, F&& visit, std::vector<int>& stack,
+                 std::vector<double>& at) const {
     if (root == NIL) return max_t;
     const double ix = 1 / d[0], iy = 1 / d[1], iz = 1 / d[2];
     const double inv[3] = {ix, iy, iz};

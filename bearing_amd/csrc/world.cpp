@@ -99,6 +99,8 @@ constexpr int DYNAMIC = 0, STATIC = 1;
 constexpr int BOX = 0, SPHERE = 1, CAPSULE = 2, HULL = 3, MESH = 5, HEIGHTFIELD = 6;  // src/contact.js:114
 constexpr int MOVING_TREE = 1, STATIC_TREE = 0;
 constexpr double PIXEL = 1.0 / 200;      // src/contact.js:125
+// Fast bodies from which a large world's catch sweeps run on its host threads.
+constexpr int CATCH_PARALLEL = 256;
 constexpr double FAT = 1.0 / 4;          // src/broadphase.js:32
 constexpr double AHEAD = 16;             // src/broadphase.js:54
 constexpr double SLEEP_TIME = 1;         // src/world.js:154
@@ -1925,7 +1927,7 @@ double sweep_level(bm_world* w, Core& cast, int s, const double* o, const double
 }
 
 int sweep(bm_world* w, Core& cast, int self, const double* d, double max_t, double* out, const uint8_t* skip,
-          int skipped, int& rc) {
+          int skipped, int& rc, std::vector<int>* stack = nullptr, std::vector<double>* at = nullptr) {
   const double o[3] = {cast.center[0], cast.center[1], cast.center[2]};
   double g[3];
   extents(cast, g);
@@ -1951,7 +1953,7 @@ int sweep(bm_world* w, Core& cast, int self, const double* d, double max_t, doub
   };
   for (int k = 0; k < 2; k++) {
     if (k == MOVING_TREE && w->trees[k].leaves == skipped) continue;
-    max_t = w->trees[k].raycast(o, d, max_t, g, visit);
+    max_t = stack ? w->trees[k].raycast(o, d, max_t, g, visit, *stack, *at) : w->trees[k].raycast(o, d, max_t, g, visit);
   }
   for (int s : strays(w)) visit(s);
   cast.center[0] = o[0]; cast.center[1] = o[1]; cast.center[2] = o[2];
@@ -2003,10 +2005,25 @@ int catch_fast(bm_world* w) {
   }
   if (count == 0) return BM_OK;
   w->catch_steps++;
-  int caught = 0, rc = BM_OK;
-  for (int i = 0; i < n; i++) {
-    const int s = list[i], s3 = 3 * s, s4 = 4 * s;
-    if (w->fast[s] != 1) continue;
+  // Each fast body sweeps against the bodies that are not fast (the fast ones are skipped), and
+  // a catch moves only the body caught: the sweeps are independent, so a large world's run on
+  // its host threads, in any order, with the same results. Device calls (a deep core's way
+  // out) then take turns.
+  std::vector<int32_t> fast_list;
+  for (int i = 0; i < n; i++)
+    if (w->fast[list[i]] == 1) fast_list.push_back(list[i]);
+  std::vector<int> rcs(fast_list.size(), BM_OK);
+  std::vector<uint8_t> hits(fast_list.size(), 0);
+  std::mutex turns;
+  std::mutex* lock = w->device_lock;
+  const bool threads = w->pool && (int)fast_list.size() >= CATCH_PARALLEL;
+  if (threads) {
+    strays(w);  // listed once, before any body moves, as the first sweep would
+    if (!w->device_lock) w->device_lock = &turns;
+  }
+  auto one = [&](int k) {
+    const int s = fast_list[k], s3 = 3 * s, s4 = 4 * s;
+    int& rc = rcs[k];
     const double dx = p[s3] - st[s3], dy = p[s3 + 1] - st[s3 + 1], dz = p[s3 + 2] - st[s3 + 2];
     const double length = std::sqrt(dx * dx + dy * dy + dz * dz);
     Core core{};
@@ -2016,9 +2033,10 @@ int catch_fast(bm_world* w) {
     place_at(core, st[s3], st[s3 + 1], st[s3 + 2], identity);
     const double dir[3] = {dx / length, dy / length, dz / length};
     double swept[7];
-    const int hit = sweep(w, core, s, dir, length, swept, w->fast.data(), count, rc);
-    if (rc != BM_OK) return rc;
-    if (hit < 0) continue;
+    thread_local std::vector<int> stack;
+    thread_local std::vector<double> at;
+    const int hit = sweep(w, core, s, dir, length, swept, w->fast.data(), count, rc, &stack, &at);
+    if (rc != BM_OK || hit < 0) return;
     const double f = swept[0] / length;
     p[s3] = st[s3] + dx * f; p[s3 + 1] = st[s3 + 1] + dy * f; p[s3 + 2] = st[s3 + 2] + dz * f;
     const double sign = sq[s4] * q[s4] + sq[s4 + 1] * q[s4 + 1] + sq[s4 + 2] * q[s4 + 2] + sq[s4 + 3] * q[s4 + 3] < 0 ? -1 : 1;
@@ -2038,9 +2056,20 @@ int catch_fast(bm_world* w) {
       v[s3] -= k * nx; v[s3 + 1] -= k * ny; v[s3 + 2] -= k * nz;
     }
     w->caught[s] = 1;
-    w->catches++;
-    caught++;
-  }
+    hits[k] = 1;
+  };
+  if (threads) w->pool->run((int)fast_list.size(), one);
+  else
+    for (int k = 0; k < (int)fast_list.size(); k++) {
+      one(k);
+      if (rcs[k] != BM_OK) break;
+    }
+  w->device_lock = lock;
+  for (int k = 0; k < (int)fast_list.size(); k++)
+    if (rcs[k] != BM_OK) return rcs[k];
+  int caught = 0;
+  for (uint8_t h : hits) caught += h;
+  w->catches += caught;
   for (int i = 0; i < n; i++) w->fast[list[i]] = 0;
   if (caught == 0) return BM_OK;
   for (int m = 0; m < w->c.count; m++)
