@@ -22,8 +22,15 @@ class Tree {
   int insert(int it, const double* b);
   void remove(int leaf);
   void move(int leaf, const double* b);
+  // src/tree.js:259-275 (query): the leaves whose boxes overlap q, child1's side first. The
+  // walk's stack is the tree's own, or the caller's (so that several threads can walk one
+  // tree at once).
   template <class F>
   void query(const double* q, F&& visit) {
+    query(q, visit, stack);
+  }
+  template <class F>
+  void query(const double* q, F&& visit, std::vector<int>& stack) const {
     if (root == NIL) return;
     stack.clear();
     stack.push_back(root);

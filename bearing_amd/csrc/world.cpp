@@ -28,6 +28,25 @@
 
 namespace {
 
+// An allocator whose resize() leaves new elements as it finds them, for columns every slot
+// of which is written before it is read.
+template <class T>
+struct Unzeroed : std::allocator<T> {
+  template <class U>
+  struct rebind {
+    using other = Unzeroed<U>;
+  };
+  Unzeroed() = default;
+  template <class U>
+  Unzeroed(const Unzeroed<U>&) noexcept {}
+  template <class U>
+  void construct(U*) noexcept {}
+  template <class U, class... A>
+  void construct(U* p, A&&... a) {
+    ::new ((void*)p) U(std::forward<A>(a)...);
+  }
+};
+
 // A small pool of host threads for per-world phases and a large world's passes: run(n, f) calls f(0..n-1)
 // across the threads and the caller, and returns when every call has.
 class Pool {
@@ -398,19 +417,23 @@ struct bm_world {
   std::vector<uint8_t> is_moved;
   Pairs pairs;
   Contacts c;
-  // Narrowphase batches for the device.
-  std::vector<double> batch_a, batch_b, batch_pair, batch_normal, batch_points, batch_seps, batch_shift, batch_loss;
+  // Narrowphase batches for the device: the records (filled whole, so growing them zeroes
+  // nothing), then the results.
+  std::vector<double, Unzeroed<double>> batch_a, batch_b, batch_pair;
+  std::vector<double> batch_normal, batch_points, batch_seps, batch_shift, batch_loss;
   std::vector<int32_t> batch_count, batch_flags;
   std::vector<uint32_t> batch_ids;
   // Level pairs for the device touchLevel (bm_level_host): field, the field body's place and
   // axes, the other body's shape record, the box to look in, margin and tolerance; results
   // in `level_cap` slots a pair.
-  std::vector<int32_t> level_field, level_manifolds, level_count;
-  std::vector<double> level_pos, level_axes, level_shape, level_box, level_margin, level_tol, level_normal, level_points,
-      level_seps;
+  std::vector<int32_t, Unzeroed<int32_t>> level_field;
+  std::vector<int32_t> level_manifolds, level_count;
+  std::vector<double, Unzeroed<double>> level_pos, level_axes, level_shape, level_box, level_margin, level_tol;
+  std::vector<double> level_normal, level_points, level_seps;
   std::vector<uint32_t> level_ids;
   int level_cap = 8;
   std::vector<Visit> visits;
+  std::vector<uint8_t> apart;  // a large pass's pairs whose boxes no longer overlap, by slot
   // Bodies woken while a narrow pass's results are taken (their bounds changed then).
   std::vector<uint8_t> fresh;
   std::vector<int32_t> woken;
@@ -423,6 +446,7 @@ struct bm_world {
   std::mutex* device_lock = nullptr;
   // Host threads for a large world's narrow passes (a world stepped alone); none in a batch.
   Pool* pool = nullptr;
+  std::vector<std::vector<int>> candidate_hits;  // the pooled candidate walks' overlaps
   // Milliseconds spent in each part of the steps so far (bm_world_times).
   double times[BM_WORLD_TIMES] = {};
   // In a batch the device holds every world's hulls and fields in one table: this world's
@@ -666,8 +690,53 @@ void found(bm_world* w, int x, int y) {
   pr.wake(p);
 }
 
+// A large world's candidate walks run on its pool: each walk only reads the trees (found()
+// changes bodies' boxes and the pairs, never a tree's nodes), so every moved body's overlaps
+// are gathered at once, then handed to found() in the order the serial loop takes them.
+constexpr int CANDIDATES_PARALLEL = 512;
+
+bool candidates_pooled(bm_world* w) {
+  const int n = (int)w->moved.size();
+  if (!w->pool || n < CANDIDATES_PARALLEL) return false;
+  const int chunk = 128, chunks = (n + chunk - 1) / chunk;
+  auto& hits = w->candidate_hits;  // per chunk: each body's overlaps, then -1
+  if ((int)hits.size() < chunks) hits.resize(chunks);
+  w->pool->run(chunks, [&](int c) {
+    thread_local std::vector<int> stack;
+    std::vector<int>& out = hits[c];
+    out.clear();
+    for (int i = c * chunk, e = std::min(n, (c + 1) * chunk); i < e; i++) {
+      const int a = w->moved[i];
+      const int t = w->which[a];
+      if (t >= 0) {
+        double q[6];
+        std::memcpy(q, &w->trees[t].box[6 * w->leaf[a]], sizeof q);
+        auto visit = [&](int b) {
+          if (b != a) out.push_back(b);
+        };
+        if (w->type[a] != STATIC) w->trees[STATIC_TREE].query(q, visit, stack);
+        w->trees[MOVING_TREE].query(q, visit, stack);
+      }
+      out.push_back(-1);
+    }
+  });
+  for (int c = 0; c < chunks; c++) {
+    size_t k = 0;
+    const std::vector<int>& out = hits[c];
+    for (int i = c * chunk, e = std::min(n, (c + 1) * chunk); i < e; i++) {
+      const int a = w->moved[i];
+      w->is_moved[a] = 0;
+      for (; out[k] >= 0; k++) found(w, a, out[k]);
+      k++;
+    }
+  }
+  w->moved.clear();
+  return true;
+}
+
 // src/broadphase.js:161-179
 void candidates(bm_world* w) {
+  if (candidates_pooled(w)) return;
   for (size_t i = 0; i < w->moved.size(); i++) {
     const int a = w->moved[i];
     w->is_moved[a] = 0;
@@ -1174,15 +1243,44 @@ int finish_visit(bm_world* w, Visit& v) {
 void narrow_begin(bm_world* w, int from) {
   Pairs& pr = w->pairs;
   std::vector<Visit>& visits = w->visits;
-  visits.clear();
-  for (int p = from; p < pr.active;) {
+  auto visit_of = [&](int p) {
     const int a = pr.a[p], b = pr.b[p];
-    if (!overlap(w, a, b)) {
-      pr.remove(p);
-      continue;
+    return Visit{p, a, b, js_min(w->size[a], w->size[b]) * PIXEL, 0, -1, Manifold{}};
+  };
+  if (w->pool && pr.active - from >= PARALLEL_VISITS) {
+    // Each overlap reads only the trees, so a large pass tests them on the pool, then removes
+    // in the serial loop's order (a removal moves the last active pair, and its flag, into
+    // the slot); the pairs kept are then slots from..active in order, one visit each.
+    const int m = pr.active - from, chunk = 4096, chunks = (m + chunk - 1) / chunk;
+    std::vector<uint8_t>& apart = w->apart;
+    if ((int)apart.size() < pr.active) apart.resize(pr.active);
+    w->pool->run(chunks, [&](int c) {
+      for (int p = from + c * chunk, e = std::min(pr.active, from + (c + 1) * chunk); p < e; p++)
+        apart[p] = !overlap(w, pr.a[p], pr.b[p]);
+    });
+    for (int p = from; p < pr.active;) {
+      if (apart[p]) {
+        apart[p] = apart[pr.active - 1];
+        pr.remove(p);
+        continue;
+      }
+      p++;
     }
-    visits.push_back(Visit{p, a, b, js_min(w->size[a], w->size[b]) * PIXEL, 0, -1, Manifold{}});
-    p++;
+    const int kept = pr.active - from, kept_chunks = (kept + chunk - 1) / chunk;
+    visits.resize(kept);  // every visit is written below: only a longer pass's tail starts fresh
+    w->pool->run(kept_chunks, [&](int c) {
+      for (int i = c * chunk, e = std::min(kept, (c + 1) * chunk); i < e; i++) visits[i] = visit_of(from + i);
+    });
+  } else {
+    visits.clear();
+    for (int p = from; p < pr.active;) {
+      if (!overlap(w, pr.a[p], pr.b[p])) {
+        pr.remove(p);
+        continue;
+      }
+      visits.push_back(visit_of(p));
+      p++;
+    }
   }
   // The pairs are where they will stay for the pass: each one's decision reads only, so a
   // large pass decides on the pool's threads, then takes its slots in order and fills them.
