@@ -34,6 +34,7 @@ namespace {
 constexpr int DYNAMIC = 0, STATIC = 1;
 constexpr int COLOURS = 32, OVERFLOW = 32, RESTING = 33;
 constexpr int UNDECIDED = -1;
+constexpr int STREAM_OWNERS = 4096;  // the streamed walk's owner slots (a power of two)
 
 __device__ __forceinline__ int lowest_free(int32_t held) {
   const int32_t bit = ~held & (held + 1);
@@ -267,68 +268,101 @@ __global__ void __launch_bounds__(256) colour_kernel(ColourDev d) {
   }
 }
 
+// A single world's manifolds packed for the streamed walk below, one 8-byte word each: a's
+// slot, then b's slot with the flags in its top byte: 1 a moves, 2 b moves, 4 resting (both
+// ends frozen or static, src/graph.js:114-124). Done across the grid, so the walk's fetches
+// are plain loads with nothing that depends on them.
+__global__ void stream_ends_kernel(ColourDev d, int wld, int2* ends) {
+  const int boff = d.body_offset[wld], moff = d.manifold_offset[wld];
+  const int n = d.manifold_offset[wld + 1] - moff;
+  const uint8_t* type = d.type + boff;
+  const uint8_t* frozen = d.frozen + boff;
+  for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < n; m += gridDim.x * blockDim.x) {
+    const int a = d.a[moff + m], b = d.b[moff + m];
+    const bool move_a = type[a] == DYNAMIC && frozen[a] == 0, move_b = type[b] == DYNAMIC && frozen[b] == 0;
+    const bool rest = (frozen[a] != 0 || type[a] == STATIC) && (frozen[b] != 0 || type[b] == STATIC);
+    const int flags = (move_a ? 1 : 0) | (move_b ? 2 : 0) | (rest ? 4 : 0);
+    ends[m] = make_int2(a, b | (flags << 24));
+  }
+}
+
 // One world too large to colour in LDS whose bodies' held masks fit there: the reference's
-// walk (src/graph.js:125-140) by the first wave, the masks and moving flags in LDS, the
-// manifolds streamed from HBM 64 at a time (each lane fetching the next 64 ends and flags
-// while the wave walks the current ones), the colours written to HBM. Then the runs per
-// colour, starts and solved, and the sort keys (colour, b) with the manifold numbers, for a
-// stable radix sort that leaves the rows as the reference's counting sorts do.
-__global__ void __launch_bounds__(256) colour_stream_kernel(ColourDev d, int wld, uint64_t* keys, int* values) {
+// walk (src/graph.js:125-140) by the first wave, the masks in LDS, the packed manifolds
+// streamed from HBM 64 at a time (each lane fetching its next one while the wave colours the
+// current 64). Within the 64 the colours are the serial walk's, found in rounds: a manifold
+// whose earlier undecided neighbours among the 64 (those sharing a moving body with it) are
+// all decided reads its masks, takes the lowest colour they leave free and ORs it in; two
+// manifolds that decide in one round share no moving body, so each reads exactly the colours
+// of the manifolds before it on its bodies, as the serial walk does. A manifold is ready when
+// it is the lowest undecided lane holding each of its moving bodies' owner slots (a table
+// of STREAM_OWNERS, by body number modulo its size: two bodies on one slot only make a lane
+// wait a round longer, and the lowest undecided lane is always ready). The rounds are as
+// many as the longest chain of dependent manifolds in the 64. Then the runs per colour,
+// starts and solved, and the sort keys (colour, b) with the manifold numbers, for a stable
+// radix sort that leaves the rows as the reference's counting sorts do.
+__global__ void __launch_bounds__(256) colour_stream_kernel(ColourDev d, int wld, const int2* ends, uint64_t* keys,
+                                                            int* values) {
   __shared__ int runs[RESTING + 2];
   extern __shared__ __align__(16) unsigned char lds[];
   const int boff = d.body_offset[wld], moff = d.manifold_offset[wld];
   const int nb = d.body_offset[wld + 1] - boff, n = d.manifold_offset[wld + 1] - moff;
-  int32_t* held = reinterpret_cast<int32_t*>(lds);
-  uint8_t* moving = reinterpret_cast<uint8_t*>(held + nb);
-  const int* A = d.a + moff;
+  int* stage = reinterpret_cast<int*>(lds);
+  int* owner = stage + 128;
+  int32_t* held = owner + STREAM_OWNERS;
   const int* B = d.b + moff;
-  const uint8_t* type = d.type + boff;
-  const uint8_t* frozen = d.frozen + boff;
   int* colour = d.colour + moff;
-  for (int s = threadIdx.x; s < nb; s += blockDim.x) {
-    held[s] = 0;
-    moving[s] = type[s] == DYNAMIC && frozen[s] == 0;
-  }
+  for (int s = threadIdx.x; s < nb; s += blockDim.x) held[s] = 0;
   for (int k = threadIdx.x; k < RESTING + 2; k += blockDim.x) runs[k] = 0;
   __syncthreads();
   if (threadIdx.x < 64) {
     const int lane = threadIdx.x;
-    // A manifold's ends, and its flags: 1 a moves, 2 b moves, 4 resting (src/graph.js:114-124).
-    auto fetch = [&](int m, int& ea, int& eb, int& flags) {
-      ea = 0; eb = 0; flags = 0;
-      if (m >= n) return;
-      ea = A[m];
-      eb = B[m];
-      const bool rest_a = frozen[ea] != 0 || type[ea] == STATIC, rest_b = frozen[eb] != 0 || type[eb] == STATIC;
-      flags = (moving[ea] ? 1 : 0) | (moving[eb] ? 2 : 0) | (rest_a && rest_b ? 4 : 0);
-    };
-    int ea, eb, flags;
-    fetch(lane, ea, eb, flags);
+    // Each lane's next manifold waits in LDS, in its own slot (a compiler barrier after each
+    // store keeps it there rather than in registers): its load is issued before the rounds
+    // and waited for after them.
+    int* slot = &stage[2 * lane];
+    const int2 first = lane < n ? ends[lane] : make_int2(0, 0);
+    slot[0] = first.x;
+    slot[1] = first.y;
     for (int base = 0; base < n; base += 64) {
-      int na, nbb, nf;
-      fetch(base + 64 + lane, na, nbb, nf);
+      const int2 cur = make_int2(slot[0], slot[1]);
+      const int2 next = base + 64 + lane < n ? ends[base + 64 + lane] : make_int2(0, 0);
+      const int ea = cur.x, eb = cur.y & 0xffffff, flags = cur.y >> 24;
+      const bool move_a = (flags & 1) != 0, move_b = (flags & 2) != 0;
+      int* own_a = &owner[ea & (STREAM_OWNERS - 1)];
+      int* own_b = &owner[eb & (STREAM_OWNERS - 1)];
+      bool undecided = base + lane < n;
       int mine = RESTING;
-      const int count = n - base < 64 ? n - base : 64;
-      for (int j = 0; j < count; j++) {
-        const int f = __builtin_amdgcn_readlane(flags, j);
-        if (f & 4) continue;
-        const int a = __builtin_amdgcn_readlane(ea, j), b = __builtin_amdgcn_readlane(eb, j);
-        const int32_t h = ((f & 1) ? held[a] : 0) | ((f & 2) ? held[b] : 0);
-        int c;
-        if (h == -1) {
-          c = OVERFLOW;
-        } else {
-          const int32_t bit = ~h & (h + 1);
-          c = 31 - __clz((uint32_t)bit);
-          if (lane == 0) {
-            if (f & 1) held[a] |= bit;
-            if (f & 2) held[b] |= bit;
+      while (__any(undecided)) {
+        // The owner slots: cleared by every undecided lane on them, then each takes the lowest
+        // such lane (the wave's LDS operations are done in the order issued; the compiler
+        // barriers keep that order in the code).
+        if (undecided && move_a) *own_a = 64;
+        if (undecided && move_b) *own_b = 64;
+        __asm__ __volatile__("" ::: "memory");
+        if (undecided && move_a) atomicMin(own_a, lane);
+        if (undecided && move_b) atomicMin(own_b, lane);
+        __asm__ __volatile__("" ::: "memory");
+        const bool ready = undecided && (!move_a || *own_a == lane) && (!move_b || *own_b == lane);
+        if (ready) {
+          const uint32_t h = (uint32_t)((move_a ? held[ea] : 0) | (move_b ? held[eb] : 0));
+          if (flags & 4) {
+            mine = RESTING;  // no colour taken
+          } else if (h == 0xffffffffu) {
+            mine = OVERFLOW;  // every colour held: none taken
+          } else {
+            const uint32_t bit = ~h & (h + 1u);
+            mine = 31 - __clz(bit);
+            if (move_a) held[ea] |= (int32_t)bit;
+            if (move_b) held[eb] |= (int32_t)bit;
           }
+          undecided = false;
         }
-        if (lane == j) mine = c;
+        __asm__ __volatile__("" ::: "memory");
       }
       if (base + lane < n) colour[base + lane] = mine;
-      ea = na; eb = nbb; flags = nf;
+      slot[0] = next.x;
+      slot[1] = next.y;
+      __asm__ __volatile__("" ::: "memory");
     }
   }
   __syncthreads();
@@ -356,7 +390,7 @@ __global__ void order_kernel(const int* sorted, int n, int* order, int* row) {
   }
 }
 
-size_t colour_stream_lds(int bodies) { return ((size_t)bodies * 5 + 15) & ~(size_t)15; }
+size_t colour_stream_lds(int bodies) { return ((size_t)(128 + STREAM_OWNERS + bodies) * 4 + 15) & ~(size_t)15; }
 
 size_t colour_stream_scratch(int manifolds) {
   size_t temp = 0;
@@ -367,7 +401,8 @@ size_t colour_stream_scratch(int manifolds) {
 
 int launch_colour_stream(const ColourDev& d, int wld, int manifolds, int bodies, void* scratch, size_t scratch_bytes,
                          hipStream_t stream) {
-  // Scratch: keys in and out, values in and out, then the sort's own.
+  // Scratch: keys in and out, values in and out, then the sort's own; the packed manifolds
+  // take the place of the keys out until the sort writes them.
   uint64_t* keys = static_cast<uint64_t*>(scratch);
   uint64_t* keys_out = keys + manifolds;
   int* values = reinterpret_cast<int*>(keys_out + manifolds);
@@ -375,7 +410,11 @@ int launch_colour_stream(const ColourDev& d, int wld, int manifolds, int bodies,
   char* temp = reinterpret_cast<char*>(values_out + manifolds + (manifolds & 1));
   const size_t used = (size_t)(temp - static_cast<char*>(scratch));
   size_t temp_bytes = scratch_bytes > used ? scratch_bytes - used : 0;
-  hipLaunchKernelGGL(colour_stream_kernel, dim3(1), dim3(256), colour_stream_lds(bodies), stream, d, wld, keys, values);
+  int2* ends = reinterpret_cast<int2*>(keys_out);
+  const int blocks = (manifolds + 255) / 256 < 1024 ? (manifolds + 255) / 256 : 1024;
+  hipLaunchKernelGGL(stream_ends_kernel, dim3(blocks > 0 ? blocks : 1), dim3(256), 0, stream, d, wld, ends);
+  hipLaunchKernelGGL(colour_stream_kernel, dim3(1), dim3(256), colour_stream_lds(bodies), stream, d, wld, ends, keys,
+                     values);
   // Colour (6 bits) above b's slot (32 bits): the stable sort keeps the order found within a
   // colour and slot.
   if (hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys, keys_out, values, values_out, manifolds, 0, 38,
