@@ -141,6 +141,7 @@ struct bm_context {
   // A large world's colouring: sort keys and values, and the radix sort's own scratch; which
   // path the last bm_colour took (0 HBM fixed point, 1 in LDS, 2 streamed and sorted).
   Scratch colour_sort;
+  Scratch wide_bar;  // the grid-wide soft step's barrier counters (bm::WIDE_WORDS), zeroed once
   int last_colour_path = 0;
   // A single world of at least wide_min manifolds takes the grid-wide soft step (0: never);
   // last_wide, the workgroups the last step used (0: one a world).
@@ -394,7 +395,8 @@ int bm_destroy(bm_context* ctx) {
   for (Scratch* s : {&ctx->deg, &ctx->adj, &ctx->keys, &ctx->key_offset, &ctx->world_of, &ctx->pairs_in,
                      &ctx->pairs_out})
     if (s->p) (void)hipFree(s->p);
-  for (Scratch* s : {&ctx->hull_ints, &ctx->hull_doubles})
+  for (Scratch* s : {&ctx->hull_ints, &ctx->hull_doubles, &ctx->field_ints, &ctx->field_doubles, &ctx->level_scratch,
+                     &ctx->colour_sort, &ctx->wide_bar})
     if (s->p) (void)hipFree(s->p);
   bm::held_free(ctx->held);
   if (ctx->begin) (void)hipEventDestroy(ctx->begin);
@@ -644,7 +646,12 @@ int bm_soft_step(bm_context* ctx) {
   const int wide_at = resident ? ctx->wide_min : ctx->wide_min / 4;
   if (ctx->n_worlds == 1 && ctx->wide_min > 0 && ctx->manifolds >= wide_at) {
     // One world too large to sweep in one workgroup: the whole grid, cooperatively.
-    ctx->last_wide = bm::launch_soft_wide(ctx->dev, 0, work, ctx->device, ctx->stream);
+    if (!ctx->wide_bar.p) {
+      if (int rc = grow(ctx, ctx->wide_bar, bm::WIDE_WORDS * sizeof(unsigned), "bm_soft_step")) return rc;
+      HIP_OR_FAIL(ctx, hipMemset(ctx->wide_bar.p, 0, ctx->wide_bar.cap), "bm_soft_step: hipMemset");
+    }
+    ctx->last_wide = bm::launch_soft_wide(ctx->dev, 0, work, static_cast<unsigned*>(ctx->wide_bar.p), ctx->device,
+                                          ctx->stream);
     if (ctx->last_wide < 0) return fail(ctx, BM_ERR_DEVICE, "bm_soft_step: the grid-wide launch failed");
   } else {
     ctx->last_wide = 0;

@@ -38,8 +38,12 @@ struct SoftDev {
 constexpr int SOFT_SHAPES = 3;
 void launch_soft_step(const SoftDev& d, int grid, int shape, size_t lds_bytes, hipStream_t stream);
 // World wld's soft step across a cooperative grid sized for `work` (its manifolds or bodies,
-// whichever is more); the workgroups launched, or -1 if the launch failed.
-int launch_soft_wide(const SoftDev& d, int wld, int work, int device, hipStream_t stream);
+// whichever is more); the workgroups launched, or -1 if the launch failed. `words`: the grid
+// barrier's counters, WIDE_WORDS of them in device memory, zero before the first launch (each
+// launch leaves them so); launches sharing them must not overlap.
+constexpr int WIDE_GROUPS = 8, WIDE_STRIDE = 32;  // counters 128 bytes apart
+constexpr int WIDE_WORDS = WIDE_STRIDE * (WIDE_GROUPS + 2);
+int launch_soft_wide(const SoftDev& d, int wld, int work, unsigned* words, int device, hipStream_t stream);
 size_t soft_step_lds_limit(int device);
 
 }  // namespace bm

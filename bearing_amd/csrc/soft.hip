@@ -16,7 +16,6 @@
 // Arithmetic is the reference's, operation for operation, in doubles, built with
 // -ffp-contract=off: the results are the reference's bits. Each function cites
 // the lines it follows.
-#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -1019,11 +1018,55 @@ size_t soft_step_lds_limit(int device) {
   return (size_t)most > own ? (size_t)most - own : 0;
 }
 
+// A barrier over every workgroup of a cooperative launch, in two levels: a workgroup's first
+// thread arrives at its group's counter (workgroup i in group i % 8: workgroups are handed to
+// the eight XCDs in turn, so a group's arrivals come from one XCD), the last of a group at
+// the top counter, and the last there opens the next generation, which every other first
+// thread waits for. Each counter is reset by its last arrival before it goes on, so all are
+// back at 0 when the next generation opens. Every arrival releases at agent scope, a group's
+// last arrival acquires before it arrives at the top, and a waiter polls without ordering
+// (an acquire invalidates the XCD's L2, which the other workgroups' rows are reading) and
+// acquires once when the generation changes: every write before the barrier is seen by every
+// read after it, as with cooperative_groups' grid.sync() (which took 32-40 us a barrier over
+// 256 workgroups on the MI355X).
+// words: the generation at [0], group g's counter at [WIDE_STRIDE * (g + 1)], the top's at
+// [WIDE_STRIDE * (WIDE_GROUPS + 1)]; every counter 0 before the first launch.
+struct WideBar {
+  unsigned* words;
+  __device__ void sync() const {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const unsigned gen = __hip_atomic_load(&words[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned g = blockIdx.x % WIDE_GROUPS;
+      const unsigned members = (gridDim.x - g + WIDE_GROUPS - 1) / WIDE_GROUPS;
+      const unsigned groups = gridDim.x < WIDE_GROUPS ? gridDim.x : WIDE_GROUPS;
+      unsigned* group = &words[WIDE_STRIDE * (g + 1)];
+      unsigned* top = &words[WIDE_STRIDE * (WIDE_GROUPS + 1)];
+      bool last = false;
+      if (__hip_atomic_fetch_add(group, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT) == members - 1) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        __hip_atomic_store(group, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (__hip_atomic_fetch_add(top, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == groups - 1) {
+          __hip_atomic_store(top, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_fetch_add(&words[0], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+          last = true;
+        }
+      }
+      if (!last) {
+        while (__hip_atomic_load(&words[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen)
+          __builtin_amdgcn_s_sleep(1);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      }
+    }
+    __syncthreads();
+  }
+};
+
 // One sweep over the world's colours across the whole grid: each colour's manifolds spread
 // over every thread of every workgroup, a grid barrier after each; the overflow in order by
 // one thread.
 template <int KIND>
-__device__ __forceinline__ void sweep_wide(cooperative_groups::grid_group& grid, const SoftDev& d, const VW& vw,
+__device__ __forceinline__ void sweep_wide(const WideBar& grid, const SoftDev& d, const VW& vw,
                                            const Soft& soft, double h, bool known, int moff, int boff,
                                            const int* starts, int tid, int nth) {
   for (int c = 0; c < COLOURS; c++) {
@@ -1049,8 +1092,8 @@ __device__ __forceinline__ void sweep_wide(cooperative_groups::grid_group& grid,
 // them in changes no result; the overflow and the bounces stay serial in the reference's
 // order. Velocities stay in HBM.
 template <int THREADS>
-__global__ void __launch_bounds__(THREADS, 1) soft_wide_kernel(SoftDev d, int wld) {
-  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+__global__ void __launch_bounds__(THREADS, 1) soft_wide_kernel(SoftDev d, int wld, unsigned* words) {
+  const WideBar grid{words};
   __shared__ int starts[BM_STARTS_DEV];
   __shared__ int bounce_list[BOUNCE_CHUNK];
   __shared__ int bounce_waves[THREADS / 64];
@@ -1111,18 +1154,21 @@ __global__ void __launch_bounds__(THREADS, 1) soft_wide_kernel(SoftDev d, int wl
   }
 }
 
-// The grid-wide soft step of world `wld`: as many resident workgroups as its work fills.
-int launch_soft_wide(const SoftDev& d, int wld, int work, int device, hipStream_t stream) {
+// The grid-wide soft step of world `wld`: a thread for every second unit of its work (a
+// colour holds at most a fifth to a third of a world's manifolds, so a colour's rows still
+// take a thread each, and a barrier costs less the fewer workgroups arrive at it), as many
+// workgroups as can be resident at most.
+int launch_soft_wide(const SoftDev& d, int wld, int work, unsigned* words, int device, hipStream_t stream) {
   constexpr int T = 256;
   int per_cu = 0, cus = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, soft_wide_kernel<T>, T, 0) != hipSuccess) return -1;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return -1;
   int blocks = per_cu * cus;
-  const int wanted = (work + T - 1) / T;
+  const int wanted = (work + 2 * T - 1) / (2 * T);
   if (wanted < blocks) blocks = wanted;
   if (blocks < 1) blocks = 1;
   SoftDev arg = d;
-  void* args[] = {&arg, &wld};
+  void* args[] = {&arg, &wld, &words};
   return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(soft_wide_kernel<T>), dim3(blocks), dim3(T), args, 0,
                                     stream) == hipSuccess ? blocks : -1;
 }
