@@ -432,7 +432,8 @@ struct bm_world {
   std::vector<double> level_normal, level_points, level_seps;
   std::vector<uint32_t> level_ids;
   int level_cap = 8;
-  std::vector<Visit> visits;
+  std::vector<Visit> visits;  // the pass's visits are the first visit_count (the vector never shrinks)
+  size_t visit_count = 0;
   std::vector<uint8_t> apart;  // a large pass's pairs whose boxes no longer overlap, by slot
   // Bodies woken while a narrow pass's results are taken (their bounds changed then).
   std::vector<uint8_t> fresh;
@@ -1267,24 +1268,28 @@ void narrow_begin(bm_world* w, int from) {
       p++;
     }
     const int kept = pr.active - from, kept_chunks = (kept + chunk - 1) / chunk;
-    visits.resize(kept);  // every visit is written below: only a longer pass's tail starts fresh
+    if ((int)visits.size() < kept) visits.resize(kept);  // every visit is written below
     w->pool->run(kept_chunks, [&](int c) {
       for (int i = c * chunk, e = std::min(kept, (c + 1) * chunk); i < e; i++) visits[i] = visit_of(from + i);
     });
+    w->visit_count = kept;
   } else {
-    visits.clear();
+    size_t k = 0;
     for (int p = from; p < pr.active;) {
       if (!overlap(w, pr.a[p], pr.b[p])) {
         pr.remove(p);
         continue;
       }
-      visits.push_back(visit_of(p));
+      if (k < visits.size()) visits[k] = visit_of(p);
+      else visits.push_back(visit_of(p));
+      k++;
       p++;
     }
+    w->visit_count = k;
   }
   // The pairs are where they will stay for the pass: each one's decision reads only, so a
   // large pass decides on the pool's threads, then takes its slots in order and fills them.
-  const int n = (int)visits.size();
+  const int n = (int)w->visit_count;
   if (!w->pool && w->owns_ctx && n >= PARALLEL_VISITS) {
     // A world stepped alone that grew large: its own threads, up to 16.
     const int threads = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
@@ -1296,7 +1301,8 @@ void narrow_begin(bm_world* w, int from) {
       for (int i = c * chunk, e = std::min(n, (c + 1) * chunk); i < e; i++) decide_visit(w, visits[i]);
     });
     size_t meets = w->batch_pair.size() / 6, levels = w->level_field.size(), m0 = meets, l0 = levels;
-    for (Visit& v : visits) {
+    for (int i = 0; i < n; i++) {
+      Visit& v = visits[i];
       if (v.kind == 2) v.slot = (int)meets++;
       else if (v.kind == 3) v.slot = (int)levels++;
     }
@@ -1308,7 +1314,7 @@ void narrow_begin(bm_world* w, int from) {
       }
     });
   } else {
-    for (Visit& v : visits) prepare_visit(w, v);
+    for (int i = 0; i < n; i++) prepare_visit(w, visits[i]);
   }
   // Prepared after every wake so far: nothing is fresh for this pass yet.
   for (int s : w->woken) w->fresh[s] = 0;
@@ -1320,7 +1326,7 @@ void narrow_begin(bm_world* w, int from) {
 // again left a record in the batch), false when the pass is done.
 bool narrow_resume(bm_world* w, int& rc) {
   std::vector<Visit>& visits = w->visits;
-  for (; w->visit_at < visits.size(); w->visit_at++) {
+  for (; w->visit_at < w->visit_count; w->visit_at++) {
     Visit& v = visits[w->visit_at];
     if (!v.again && (w->fresh[v.a] || w->fresh[v.b])) {
       v.again = true;
