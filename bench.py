@@ -256,7 +256,9 @@ def batched_soft_run(ctx, args):
     from bearing_amd.tune import tune
 
     # Batched: worlds of five scenes, repeated to --batch-worlds, in one launch.
-    sources = [load_world(n)[:2] for n in BATCH_SOURCES]
+    loaded = [load_world(n) for n in BATCH_SOURCES]
+    world_from_columns = loaded[0][2]
+    sources = [(fx, cols) for fx, cols, _ in loaded]
     worlds = [world_from_columns(sources[i % len(sources)][0], {k: v.copy() for k, v in sources[i % len(sources)][1].items()})
               for i in range(args.batch_worlds)]
     if args.batched_launch:
