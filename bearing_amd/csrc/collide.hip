@@ -1834,9 +1834,11 @@ __global__ void meet_kernel(MeetDev d) {
 // of every cell under the box whose heights reach it, row by row, each touched; points
 // grouped by direction, kept once where two triangles give the same one; each group sorted
 // by id and reduced to four.
-__global__ void level_kernel(LevelDev d) {
+__global__ void __launch_bounds__(64) level_kernel(LevelDev d) {
   using namespace narrow;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ Touch touched[64];
+  __shared__ int hits[64];
+  const int k = blockIdx.x, lane = threadIdx.x;
   if (k >= d.n) return;
   const double* r = d.shapes + 19 * (long)k;
   Shape s;
@@ -1870,7 +1872,7 @@ __global__ void level_kernel(LevelDev d) {
   W.in = W.id + cap_points;
   W.gid = W.in + cap_points;
   Toucher T;
-  Touch one;
+  Touch mine;
   int found = 0, groups = 0;
   const double t2 = tolerance * tolerance;
   // src/mesh.js:388-401 (triangles)
@@ -1878,11 +1880,18 @@ __global__ void level_kernel(LevelDev d) {
   const double i0 = narrow::js_max(0, floor((box[0] - F.x0) / F.spacing)), i1 = narrow::js_min(cells - 1, floor((box[3] - F.x0) / F.spacing));
   const double j0 = narrow::js_max(0, floor((box[2] - F.z0) / F.spacing)), j1 = narrow::js_min(F.rows - 2, floor((box[5] - F.z0) / F.spacing));
   if (i0 <= i1 && j0 <= j1) {
-    for (int j = (int)j0; j <= (int)j1; j++) {
-      for (int i = (int)i0; i <= (int)i1; i++) {
+    // A wave a pair: lane l touches triangle start + l of the serial walk's order (rows of
+    // cells, cells, then the cell's two halves), each touch its own (touch() starts afresh),
+    // and the first lane takes the 64 touches in that order into the groups, as the walk does.
+    const int width = (int)i1 - (int)i0 + 1, total = 2 * width * ((int)j1 - (int)j0 + 1);
+    for (int start = 0; start < total; start += 64) {
+      const int o = start + lane;
+      int got = 0;
+      if (o < total) {
+        const int cell = o >> 1, half = o & 1;
+        const int i = (int)i0 + cell % width, j = (int)j0 + cell / width;
         const double ha = F.at(i, j), hb = F.at(i + 1, j), hc = F.at(i, j + 1), hd = F.at(i + 1, j + 1);
-        if (narrow::js_max(narrow::js_max(narrow::js_max(ha, hb), hc), hd) < box[1] || narrow::js_min(narrow::js_min(narrow::js_min(ha, hb), hc), hd) > box[4]) continue;
-        for (int half = 0; half < 2; half++) {
+        if (!(narrow::js_max(narrow::js_max(narrow::js_max(ha, hb), hc), hd) < box[1] || narrow::js_min(narrow::js_min(narrow::js_min(ha, hb), hc), hd) > box[4])) {
           const int t = 2 * (j * cells + i) + half;
           double local[9], p[9];
           F.triangle(t, local);
@@ -1904,7 +1913,17 @@ __global__ void level_kernel(LevelDev d) {
           tri.center[2] = (p[2] + p[5] + p[8]) / 3;
           triangle_normal(p, tri.axes);
           const int flags = field_edge(F, t, T.folds);
-          if (T.touch(s, tri, flags, margin, tolerance, one, t) == 0) continue;
+          got = T.touch(s, tri, flags, margin, tolerance, mine, t);
+        }
+      }
+      hits[lane] = got;
+      if (got) touched[lane] = mine;
+      __syncthreads();
+      if (lane == 0) {
+        const int batch = total - start < 64 ? total - start : 64;
+        for (int l = 0; l < batch; l++) {
+          if (hits[l] == 0) continue;
+          const Touch& one = touched[l];
           const double nx = one.normal[0], ny = one.normal[1], nz = one.normal[2];
           int g = 0;
           while (g < groups && W.normals[3 * g] * nx + W.normals[3 * g + 1] * ny + W.normals[3 * g + 2] * nz < 1 - SAME) g++;
@@ -1935,8 +1954,10 @@ __global__ void level_kernel(LevelDev d) {
           }
         }
       }
+      __syncthreads();
     }
   }
+  if (lane != 0) return;
   for (int g = 0; g < groups; g++) {
     int m = 0;
     for (int i = 0; i < found; i++) {
@@ -1966,7 +1987,7 @@ __global__ void level_kernel(LevelDev d) {
 
 void launch_level(const LevelDev& d, hipStream_t stream) {
   if (d.n == 0) return;
-  hipLaunchKernelGGL(level_kernel, dim3((d.n + 63) / 64), dim3(64), 0, stream, d);
+  hipLaunchKernelGGL(level_kernel, dim3(d.n), dim3(64), 0, stream, d);
 }
 
 void launch_meet(const MeetDev& d, hipStream_t stream) {
