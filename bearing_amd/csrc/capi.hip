@@ -441,7 +441,10 @@ int64_t bm_lds_limit(bm_context* ctx) {
 }
 
 // Host copies between the caller's columns and the pinned staging: a large batch's (tens of
-// MB and more) split into 4 MB pieces over up to 16 host threads, a small one's in turn.
+// MB and more) over up to 16 host threads, a small one's in turn. The list (a batch of
+// thousands of worlds holds a copy per world and column, hundreds of thousands of small
+// ones; a large copy is cut into 4 MB pieces) is cut into runs of about equal bytes, eight
+// per thread, which the threads take in turn.
 struct HostCopy {
   void* to;
   const void* from;
@@ -461,9 +464,19 @@ void run_copies(const std::vector<HostCopy>& copies) {
   for (const HostCopy& c : copies)
     for (size_t at = 0; at < c.bytes; at += PIECE)
       pieces.push_back({static_cast<char*>(c.to) + at, static_cast<const char*>(c.from) + at, std::min(PIECE, c.bytes - at)});
+  // Run r is pieces [cut[r], cut[r + 1]).
+  const size_t runs = (size_t)threads * 8, goal = total / runs + 1;
+  std::vector<size_t> cut(1, 0);
+  size_t bytes = 0;
+  for (size_t i = 0; i < pieces.size(); i++) {
+    bytes += pieces[i].bytes;
+    if (bytes >= goal * cut.size()) cut.push_back(i + 1);
+  }
+  if (cut.back() != pieces.size()) cut.push_back(pieces.size());
   std::atomic<size_t> next{0};
   auto work = [&] {
-    for (size_t i; (i = next.fetch_add(1)) < pieces.size();) std::memcpy(pieces[i].to, pieces[i].from, pieces[i].bytes);
+    for (size_t r; (r = next.fetch_add(1)) + 1 < cut.size();)
+      for (size_t i = cut[r]; i < cut[r + 1]; i++) std::memcpy(pieces[i].to, pieces[i].from, pieces[i].bytes);
   };
   std::vector<std::thread> pool;
   for (int t = 1; t < threads; t++) pool.emplace_back(work);
