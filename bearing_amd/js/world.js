@@ -1,0 +1,147 @@
+'use strict';
+/*
+ * World for a JavaScript host, stepped on the MI355X through the N-API addon
+ * (bearing_amd/_build/bearing_napi.node, over the C ABI's bm_world_*): the reference's
+ * per-step surface, new World(options) (src/world.js:314), addBody (:563), step(dt)
+ * (:880), transforms and previous (:547, :554: 7 floats a slot, position then rotation),
+ * stepCount (:526) and moved(since) (:771). Plain CommonJS that Node 12.22 runs.
+ *
+ * There is no host fallback: without the addon, or without a GPU, the constructor throws.
+ */
+const path = require('path');
+const { resolve } = require('./resolve');
+
+const ADDON = path.join(__dirname, '..', '_build', 'bearing_napi.node');
+
+function fail(caller, message) {
+  throw new Error(`${caller}: ${message}`);
+}
+
+class World {
+  /* options: {gravity: [x, y, z], substeps, sleeping}, as the reference's constructor. */
+  constructor(options) {
+    const o = options === undefined ? {} : options;
+    if (o === null || typeof o !== 'object') fail('World', 'options must be an object');
+    this._native = require(ADDON);
+    const gravity = o.gravity === undefined ? null : Float64Array.from(o.gravity);
+    const substeps = o.substeps === undefined ? 0 : o.substeps;
+    if (!Number.isInteger(substeps) || substeps < 0) fail('World', `substeps must be a whole number, got ${substeps}`);
+    this._world = this._native.create(gravity, substeps, o.sleeping === undefined ? true : Boolean(o.sleeping));
+    this._stepCount = 0;
+    this._top = 0;
+    this._counts = null;
+    this._transforms = new Float32Array(0);
+    this._previous = new Float32Array(0);
+    this._movedAt = new Float64Array(0);   // the step a slot last moved in; stepCount + 1 if added since
+    this._movedOut = new Int32Array(0);
+  }
+
+  /* Adds one body; returns its slot. */
+  addBody(options) {
+    return this.addBodies([options])[0];
+  }
+
+  /* Adds bodies in slot order, in one call to the device world; returns their slots. */
+  addBodies(list) {
+    this._live('addBody');
+    if (!Array.isArray(list)) fail('addBody', 'bodies must be an array of options');
+    const cols = resolve('addBody', list);
+    this._native.add(this._world, cols);
+    const first = this._top;
+    this._sync();
+    const slots = [];
+    for (let s = first; s < this._top; s++) {
+      this._movedAt[s] = this._stepCount + 1;
+      this._previous.set(this._transforms.subarray(7 * s, 7 * s + 7), 7 * s);
+      slots.push(s);
+    }
+    return slots;
+  }
+
+  step(dt) {
+    this._live('step');
+    if (typeof dt !== 'number' || !(dt > 0) || !Number.isFinite(dt)) fail('step', `dt must be a number above 0, got ${dt}`);
+    this._previous.set(this._transforms);
+    this._native.step(this._world, dt);
+    this._stepCount++;
+    this._sync();
+    // A slot moved in this step when its drawn pose changed.
+    const t = this._transforms, p = this._previous, at = this._movedAt, n = this._top;
+    for (let s = 0; s < n; s++) {
+      const k = 7 * s;
+      if (t[k] !== p[k] || t[k + 1] !== p[k + 1] || t[k + 2] !== p[k + 2] || t[k + 3] !== p[k + 3] ||
+          t[k + 4] !== p[k + 4] || t[k + 5] !== p[k + 5] || t[k + 6] !== p[k + 6]) at[s] = this._stepCount;
+    }
+  }
+
+  get transforms() {
+    this._live('transforms');
+    return this._transforms;
+  }
+
+  get previous() {
+    this._live('previous');
+    return this._previous;
+  }
+
+  get stepCount() {
+    this._live('stepCount');
+    return this._stepCount;
+  }
+
+  get bodyCount() {
+    this._live('bodyCount');
+    return this._top;
+  }
+
+  /* The step's counters: {awake, contacts, pairs, frozen, impacts, catches}. */
+  get counts() {
+    this._live('counts');
+    return this._counts;
+  }
+
+  /*
+   * The slots of the bodies that moved in any step after step `since`, and always of those
+   * that moved in the last one; with bodies added since the last step. A view the next
+   * call or step overwrites.
+   */
+  moved(since) {
+    this._live('moved');
+    if (!Number.isInteger(since) || since < 0) fail('moved', `since must be a whole number, 0 or more, got ${since}`);
+    const after = Math.min(since, this._stepCount - 1), at = this._movedAt, out = this._movedOut;
+    let k = 0;
+    for (let s = 0; s < this._top; s++) if (at[s] > after) out[k++] = s;
+    return out.subarray(0, k);
+  }
+
+  /* Milliseconds spent so far in each part of the step (the C ABI's bm_world_times). */
+  times() {
+    this._live('times');
+    return this._native.times(this._world);
+  }
+
+  destroy() {
+    if (this._world !== null) this._native.destroy(this._world);
+    this._world = null;
+  }
+
+  _live(caller) {
+    if (this._world === null) fail(caller, 'the world was destroyed');
+  }
+
+  /* Reads the device world's counters and transforms, growing the slot arrays with it. */
+  _sync() {
+    const top = this._native.read(this._world, null).top;
+    if (top !== this._top) {
+      const grow = (A, old, per) => { const a = new A(per * top); a.set(old.subarray(0, Math.min(old.length, per * top))); return a; };
+      this._transforms = grow(Float32Array, this._transforms, 7);
+      this._previous = grow(Float32Array, this._previous, 7);
+      this._movedAt = grow(Float64Array, this._movedAt, 1);
+      this._movedOut = new Int32Array(top);
+      this._top = top;
+    }
+    this._counts = this._native.read(this._world, this._transforms);
+  }
+}
+
+module.exports = { World, resolve };
