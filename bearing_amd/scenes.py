@@ -316,11 +316,23 @@ def mixed_pile(rng: SplitMix64, count: int = 20000, columns: int = 256, volume=(
     return bodies
 
 
+def _unit(q):
+    """A rotation as the reference keeps it (src/core/check.js:105-116 quat): scaled by its
+    largest magnitude, then to unit length."""
+    q = [float(x) for x in q]
+    m = max(abs(q[0]), abs(q[1]), abs(q[2]), abs(q[3]))
+    if not m > 0:
+        raise ValueError("resolve: rotation must be 4 finite numbers, not all zero")
+    x, y, z, w = q[0] / m, q[1] / m, q[2] / m, q[3] / m
+    s = 1 / math.sqrt(x * x + y * y + z * z + w * w)
+    return [x * s, y * s, z * s, w * s]
+
+
 def resolve(bodies):
     """addBody's resolved columns (bm_bodies, as native.World.add takes them) for scenes of
     boxes and spheres, with the reference's defaults and arithmetic (src/world.js:2150-2245
     and src/shapes.js:14-56: density 1000 unless a mass is given, friction 0.5, restitution
-    0; the thinnest size and the reach; inverses where positive)."""
+    0; the thinnest size and the reach; inverses where positive; rotations normalised)."""
     n = len(bodies)
     cols = {name: [] for name in ("type", "shape", "lock", "position", "rotation", "velocity", "angular", "dims",
                                   "inertia", "inv_inertia", "mass", "inv_mass", "gravity_scale", "damping",
@@ -349,7 +361,7 @@ def resolve(bodies):
         cols["shape"].append(0 if shape == "box" else 1)
         cols["lock"].append(0)
         cols["position"] += [float(x) for x in b.get("position", [0.0, 0.0, 0.0])]
-        cols["rotation"] += [float(x) for x in b.get("rotation", [0.0, 0.0, 0.0, 1.0])]
+        cols["rotation"] += _unit(b.get("rotation", [0.0, 0.0, 0.0, 1.0]))
         cols["velocity"] += [float(x) for x in b.get("velocity", [0.0, 0.0, 0.0])]
         cols["angular"] += [float(x) for x in b.get("angularVelocity", [0.0, 0.0, 0.0])]
         cols["dims"] += dims

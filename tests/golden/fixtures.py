@@ -93,3 +93,23 @@ def soft_columns(fx):
     cols["gravity"] = fx["in.gravity"].copy()
     return cols
 
+
+
+_NAMES = {np.dtype(v): k for k, v in _DT.items()}
+
+
+def save(path, arrays, meta):
+    """Writes named numpy arrays and a meta dict as a BMFX fixture (the layout load() reads)."""
+    entries, offset, blobs = [], 0, []
+    for name, a in arrays.items():
+        a = np.ascontiguousarray(a)
+        entries.append({"name": name, "dtype": _NAMES[a.dtype], "length": int(a.size), "offset": offset})
+        raw = a.tobytes()
+        pad = (8 - len(raw) % 8) % 8
+        blobs.append(raw + b"\0" * pad)
+        offset += len(raw) + pad
+    header = json.dumps({"arrays": entries, "meta": meta}).encode()
+    lead = 12 + len(header)
+    body = b"BMFX0001" + len(header).to_bytes(4, "little") + header + b"\0" * ((8 - lead % 8) % 8) + b"".join(blobs)
+    with gzip.open(path, "wb", compresslevel=9) as f:
+        f.write(body)

@@ -445,7 +445,80 @@ def run_impact(seed=2026, n=2000):
     os.remove(path)
 
 
+def batched_sample(worlds):
+    """The worlds of the batched scenes whose every step is recorded: of the box towers (even
+    worlds) and of the sphere piles (odd worlds) each, the smallest, the median and the
+    largest by body count (the first of equals), and the first three."""
+    picked = []
+    for kind in (0, 1):
+        idx = list(range(kind, len(worlds), 2))
+        by = sorted(idx, key=lambda i: (len(worlds[i]), i))
+        for i in (by[0], by[len(by) // 2], by[-1], *idx[:3]):
+            if i not in picked:
+                picked.append(i)
+    return sorted(picked)
+
+
+def run_batched(worlds=4096, seed=4, steps=300, dt=1 / 60, procs=8):
+    """The batched form's worlds (scenes.batched: log-uniform 8..512 bodies, box towers and
+    sphere piles, each on its own 10x1x10 ground), each stepped `steps` times by the reference
+    on its own: every world's final transforms hash and counts, and the sampled worlds' every
+    step. Written to data/batched.traj.bmfx.gz: hashes as 32 bytes a world (u8), counts as i32."""
+    import numpy as np
+    import fixtures
+
+    import shutil
+    import tempfile
+
+    scenes_list = scenes.batched(worlds, seed=seed)
+    sample = batched_sample(scenes_list)
+    tmp = tempfile.mkdtemp(prefix="bm_batched_")
+    scene_path = os.path.join(tmp, "batched.json")
+    with open(scene_path, "w") as f:
+        json.dump(scenes_list, f)
+    # Ranges of about equal body counts, one a process.
+    sizes = np.array([len(w) for w in scenes_list])
+    cuts = np.searchsorted(np.cumsum(sizes), np.linspace(0, sizes.sum(), procs + 1)[1:-1])
+    bounds = [0, *[int(c) + 1 for c in cuts], worlds]
+    jobs, running = [], []
+    for k in range(procs):
+        job = {"scenes": scene_path, "from": bounds[k], "to": bounds[k + 1], "steps": steps, "dt": dt,
+               "sample": sample, "out": os.path.join(tmp, f"batched.{k}.out.json")}
+        path = os.path.join(tmp, f"batched.{k}.job.json")
+        with open(path, "w") as f:
+            json.dump(job, f)
+        jobs.append(job)
+        running.append(subprocess.Popen(["node", "--no-warnings", "--experimental-loader",
+                                         os.path.join(HERE, "lower.mjs"), os.path.join(HERE, "make_batched.mjs"), path]))
+    for p in running:
+        if p.wait() != 0:
+            raise RuntimeError("make_batched.mjs failed")
+    final, counts, per = [], {k: [] for k in ("awake", "contacts", "frozen", "impacts", "catches")}, {}
+    for job in jobs:
+        out = json.load(open(job["out"]))
+        assert (out["from"], out["to"]) == (job["from"], job["to"])
+        final += out["final"]
+        for k in counts:
+            counts[k] += out[k]
+        per.update({int(i): r for i, r in out["sample"].items()})
+    shutil.rmtree(tmp)
+    arrays = {"final": np.frombuffer(b"".join(bytes.fromhex(h) for h in final), np.uint8).copy(),
+              "sample": np.asarray(sample, np.int32)}
+    for k, v in counts.items():
+        arrays[k] = np.asarray(v, np.int32)
+    for i in sample:
+        r = per[i]
+        arrays[f"s{i}.hashes"] = np.frombuffer(b"".join(bytes.fromhex(h) for h in r["hashes"]), np.uint8).copy()
+        for k in ("awake", "contacts", "frozen"):
+            arrays[f"s{i}.{k}"] = np.asarray(r[k], np.int32)
+    meta = {"worlds": worlds, "seed": seed, "steps": steps, "dt": dt, "bodies": int(sizes.sum())}
+    fixtures.save(os.path.join(DATA, "batched.traj.bmfx.gz"), arrays, meta)
+
+
 if __name__ == "__main__":
+    if sys.argv[1:] == ["batched"]:
+        run_batched()
+        sys.exit(0)
     if sys.argv[1:] == ["collide"]:
         run_collide()
         sys.exit(0)

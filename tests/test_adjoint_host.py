@@ -107,8 +107,6 @@ def test_resolve_matches_the_reference(lib):
         fx = fixtures.load(os.path.join(fixtures.DATA, f"{name}.traj.bmfx.gz"))
         want, got = init_columns(fx), scenes.resolve(scenes.SCENES[name]())
         for key, col in got.items():
-            if name == "tower" and key == "rotation":
-                continue  # the fixture's turns come from the reference's own sin and cos
             assert np.array_equal(col.view(np.uint8), np.asarray(want[key], col.dtype).view(np.uint8)), (name, key)
 
 

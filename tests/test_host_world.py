@@ -9,7 +9,7 @@ import subprocess
 import pytest
 
 import fixtures
-from bearing_amd import native
+from bearing_amd import native, scenes
 from worldcheck import init_columns, make_world, refuses, run
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -58,4 +58,30 @@ def test_host_batch_steps_every_world_as_alone(lib):
     # A batch's world steps only with its batch.
     with pytest.raises(native.BearingError, match="steps with its batch"):
         worlds[0].step(1 / 60)
+    batch.close()
+
+
+def test_host_batched_sample_matches_reference(lib):
+    # The batched form's sampled worlds (the smallest, median and largest box towers and sphere
+    # piles of scenes.batched(4096, seed 4), and the first three of each) stepped together for
+    # 300 steps, every step the reference's own trajectory of that world.
+    import batchedcheck
+    fx = batchedcheck.load()
+    specs = batchedcheck.scene_list(fx)
+    sample = [int(i) for i in fx["sample"]]
+    for i in sample:   # the fixture's sampled last step is its final record
+        assert bytes(fx[f"s{i}.hashes"][-32:]) == batchedcheck.final_of(fx, i)
+    batch = native.WorldBatch(threads=4, lib=lib)
+    worlds = {}
+    for i in sample:
+        worlds[i] = batch.world()
+        worlds[i].add(scenes.resolve(specs[i]))
+    for s in range(1, fx.meta["steps"] + 1):
+        batch.step(fx.meta["dt"])
+        for i, w in worlds.items():
+            why = batchedcheck.check_step(fx, i, s, w)
+            assert why is None, why
+    for i, w in worlds.items():
+        why = batchedcheck.check_final(fx, i, w)
+        assert why is None, why
     batch.close()
