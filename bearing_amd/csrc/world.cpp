@@ -368,6 +368,14 @@ struct Record {
 struct bm_world {
   bm_context* ctx = nullptr;
   std::string error;
+  // A step that failed after it began changing the world (the bodies half moved): every
+  // later step returns this code and message rather than going on from that state.
+  int broken = BM_OK;
+  std::string broken_msg;
+  // The last step's solve: rows (points of the solved manifolds), the compulsory bytes its
+  // sweeps read and write (soft_bytes), the soft-step kernel's device time.
+  int64_t step_rows = 0;
+  double step_bytes = 0, step_soft_ms = 0;
   double gravity[3] = {0, -9.81, 0};
   int substeps = 8;
   bool sleeping = true;
@@ -470,6 +478,13 @@ namespace {
 
 int fail(bm_world* w, int code, const std::string& m) {
   w->error = m;
+  return code;
+}
+
+// A step failed part-way: the world keeps the failure, and every later step returns it.
+int broke(bm_world* w, int code) {
+  w->broken = code;
+  w->broken_msg = w->error + " (the step stopped part-way; this world takes no more steps)";
   return code;
 }
 
@@ -2409,6 +2424,40 @@ void record_after(bm_world* w) {
   R.steps++;
 }
 
+// The solved rows of the step just solved (laid out by colour, solved ones first) and the
+// compulsory bytes of its soft step's sweeps, each field read or written once a pass and each
+// body's fields once a manifold end (the gather and scatter every colour does). A manifold
+// of n points, in bytes:
+//   solve (bias, separations known): 394 fixed + 56 if n <= 2 + 8 n^2 if n >= 2 (the block)
+//     + n (152 + 8 if n == 1) + bodies: v, w read 96 and written 96 (48 with one end static),
+//     inverse masses 16, inverse inertias 96;
+//   relax, and a step's first solve: 24 fewer fixed, 56 more a point, 160 more body bytes
+//     (positions, starts, rotations);
+//   warm start: 201 fixed + 16 if n <= 2 + 56 n + 304 body.
+// A step of S substeps is S warm starts, S - 1 known solves, one first solve and S relaxes.
+// `soft_ms` is the soft-step kernel's device time.
+void solve_stats(bm_world* w, double soft_ms) {
+  const Contacts& c = w->c;
+  const double S = w->substeps;
+  int64_t rows = 0;
+  double bytes = 0;
+  for (int f = 0; f < w->h_solved; f++) {
+    const int n = c.points[f];
+    rows += n;
+    const bool both = w->inv_mass[c.a[f]] > 0 && w->inv_mass[c.b[f]] > 0;
+    const double body = 96 + (both ? 96 : 48) + 16 + 96;
+    const double fixed = 394 + (n <= 2 ? 56 : 0) + (n >= 2 ? 8.0 * n * n : 0);
+    const double one = n == 1 ? 8 : 0;
+    const double solve = fixed + n * (152 + one) + body;
+    const double relax = (fixed - 24) + n * (152 + 56 + one) + body + 160;
+    const double warm = 201 + (n <= 2 ? 16 : 0) + 56.0 * n + 304;
+    bytes += S * warm + (S - 1) * solve + relax + S * relax;
+  }
+  w->step_rows = rows;
+  w->step_bytes = bytes;
+  w->step_soft_ms = soft_ms;
+}
+
 // The device phase: colour, rest, inertias .. store (src/world.js:956-1031), through the batch ABI.
 int device_phase(bm_world* w, double dt) {
   fill_columns(w, dt);
@@ -2418,6 +2467,7 @@ int device_phase(bm_world* w, double dt) {
   if (int rc = bm_colour(w->ctx)) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx));
   if (int rc = bm_soft_step(w->ctx)) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx));
   if (int rc = bm_download(w->ctx, &h)) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx));
+  solve_stats(w, bm_last_kernel_ms(w->ctx));
   store_memory(w, w->h_solved);
   if (w->record.on) record_after(w);
   return BM_OK;
@@ -2490,6 +2540,8 @@ int step_end(bm_world* w, double dt) {
 struct bm_batch {
   bm_context* ctx = nullptr;
   std::string error;
+  int broken = BM_OK;  // a step that failed part-way: every later step returns it
+  std::string broken_msg;
   bool tables_stale = false;  // a world's hulls or fields changed since they were joined
   std::vector<bm_world*> worlds;
   std::mutex lock;
@@ -2867,24 +2919,27 @@ int bm_world_step(bm_world* w, double dt) {
     w->times[k] += std::chrono::duration<double, std::milli>(now - t).count();
     t = now;
   };
+  if (w->broken != BM_OK) return fail(w, w->broken, w->broken_msg);
+  // A step that fails before anything changed (dt) leaves the world as it was; one that
+  // fails later leaves it half stepped, and every later step is refused.
   if (int rc = step_begin(w, dt)) return rc;
   lap(0);
   int rc = BM_OK;
   for (;;) {
-    if ((rc = run_batch(w)) != BM_OK) return rc;
+    if ((rc = run_batch(w)) != BM_OK) return broke(w, rc);
     lap(1);
     const bool more = narrow_continue(w, rc);
     lap(2);
     if (!more) break;
   }
-  if (rc != BM_OK) return rc;
+  if (rc != BM_OK) return broke(w, rc);
   step_middle(w);
   lap(3);
-  if ((rc = device_phase(w, dt)) != BM_OK) return rc;
+  if ((rc = device_phase(w, dt)) != BM_OK) return broke(w, rc);
   lap(4);
   rc = step_end(w, dt);
   lap(5);
-  return rc;
+  return rc != BM_OK ? broke(w, rc) : rc;
 }
 
 int bm_world_times(const bm_world* w, double* out) {
@@ -2936,9 +2991,23 @@ int bm_batch_world(bm_batch* B, bm_world** out, const double* gravity, int32_t s
   return BM_OK;
 }
 
+// Every world of the batch is left half stepped by a failure after the worlds began their
+// step: the batch, and each of its worlds, keep it.
+int batch_broke(bm_batch* B, int code) {
+  B->broken = code;
+  B->broken_msg = B->error + " (the batch's step stopped part-way; its worlds take no more steps)";
+  for (bm_world* w : B->worlds) {
+    if (w->broken != BM_OK) continue;
+    w->error = B->broken_msg;
+    broke(w, code);
+  }
+  return code;
+}
+
 int bm_batch_step(bm_batch* B, double dt) {
   if (!B) return BM_ERR_ARGUMENT;
   B->error.clear();
+  if (B->broken != BM_OK) return batch_fail(B, B->broken, B->broken_msg);
   auto& W = B->worlds;
   const int n = (int)W.size();
   if (n == 0) return BM_OK;
@@ -2952,13 +3021,14 @@ int bm_batch_step(bm_batch* B, double dt) {
   if (B->tables_stale) {
     if (int rc = join_tables(B)) return rc;
   }
+  if (!(dt > 0) || dt == INFINITY) return batch_fail(B, BM_ERR_ARGUMENT, "step: dt must be a positive number of seconds");
   std::vector<int> rcs(n, BM_OK);
   B->pool->run(n, [&](int i) { rcs[i] = step_begin(W[i], dt); });
-  if (int rc = first_failure(B, rcs)) return rc;
+  if (int rc = first_failure(B, rcs)) return batch_broke(B, rc);
   lap(0);
   std::vector<uint8_t> active(n, 1);
   for (;;) {
-    if (int rc = run_merged(B)) return rc;
+    if (int rc = run_merged(B)) return batch_broke(B, rc);
     lap(1);
     B->pool->run(n, [&](int i) {
       if (!active[i]) return;
@@ -2966,7 +3036,7 @@ int bm_batch_step(bm_batch* B, double dt) {
       active[i] = narrow_continue(W[i], rc) ? 1 : 0;
       rcs[i] = rc;
     });
-    if (int rc = first_failure(B, rcs)) return rc;
+    if (int rc = first_failure(B, rcs)) return batch_broke(B, rc);
     bool any = false;
     for (int i = 0; i < n; i++) any = any || active[i];
     lap(2);
@@ -2981,19 +3051,25 @@ int bm_batch_step(bm_batch* B, double dt) {
   });
   lap(3);
   std::vector<const bm_columns*> ins(hs.begin(), hs.end());
-  if (int rc = bm_upload_worlds(B->ctx, n, ins.data())) return batch_fail(B, rc, std::string("step: ") + bm_last_error(B->ctx));
-  if (int rc = bm_colour(B->ctx)) return batch_fail(B, rc, std::string("step: ") + bm_last_error(B->ctx));
-  if (int rc = bm_soft_step(B->ctx)) return batch_fail(B, rc, std::string("step: ") + bm_last_error(B->ctx));
-  if (int rc = bm_download_worlds(B->ctx, n, hs.data()))
-    return batch_fail(B, rc, std::string("step: ") + bm_last_error(B->ctx));
+  auto device_failed = [&](int rc) {
+    batch_fail(B, rc, std::string("step: ") + bm_last_error(B->ctx));
+    return batch_broke(B, rc);
+  };
+  if (int rc = bm_upload_worlds(B->ctx, n, ins.data())) return device_failed(rc);
+  if (int rc = bm_colour(B->ctx)) return device_failed(rc);
+  if (int rc = bm_soft_step(B->ctx)) return device_failed(rc);
+  if (int rc = bm_download_worlds(B->ctx, n, hs.data())) return device_failed(rc);
   lap(4);
+  const double soft_ms = bm_last_kernel_ms(B->ctx);
   B->pool->run(n, [&](int i) {
+    solve_stats(W[i], soft_ms);
     store_memory(W[i], W[i]->h_solved);
     if (W[i]->record.on) record_after(W[i]);
     rcs[i] = step_end(W[i], dt);
   });
   lap(5);
-  return first_failure(B, rcs);
+  if (int rc = first_failure(B, rcs)) return batch_broke(B, rc);
+  return BM_OK;
 }
 
 int bm_batch_times(const bm_batch* B, double* out) {
@@ -3011,6 +3087,9 @@ int bm_world_read(const bm_world* w, bm_world_state* out) {
   out->frozen = w->frozen_count;
   out->impacts = w->impacts;
   out->catches = w->catches;
+  out->rows = w->step_rows;
+  out->soft_bytes = w->step_bytes;
+  out->soft_ms = w->step_soft_ms;
   if (out->transforms) std::memcpy(out->transforms, w->transforms.data(), 7 * sizeof(float) * w->top);
   if (out->position) std::memcpy(out->position, w->position.data(), 3 * sizeof(double) * w->top);
   if (out->rotation) std::memcpy(out->rotation, w->rotation.data(), 4 * sizeof(double) * w->top);

@@ -118,7 +118,7 @@ class WorldState(ctypes.Structure):
                 ("pairs", ctypes.c_int32), ("frozen", ctypes.c_int32), ("impacts", ctypes.c_int64),
                 ("catches", ctypes.c_int64),
                 ("transforms", ctypes.POINTER(ctypes.c_float)), ("position", _D), ("rotation", _D), ("velocity", _D),
-                ("angular", _D)]
+                ("angular", _D), ("rows", ctypes.c_int64), ("soft_bytes", ctypes.c_double), ("soft_ms", ctypes.c_double)]
 
 
 def world_signatures(lib):
@@ -456,7 +456,8 @@ class World:
         for name in ("position", "rotation", "velocity", "angular"):
             setattr(s, name, out[name].ctypes.data_as(_D))
         self.check(self.lib.bm_world_read(self.ptr, ctypes.byref(s)), "bm_world_read")
-        for name in ("top", "awake", "contacts", "pairs", "frozen", "impacts", "catches"):
+        for name in ("top", "awake", "contacts", "pairs", "frozen", "impacts", "catches", "rows", "soft_bytes",
+                     "soft_ms"):
             out[name] = getattr(s, name)
         return out
 
@@ -553,6 +554,9 @@ class Adjoint:
         b, t = np.zeros(1, np.int32), np.zeros(1, np.int32)
         self.lib.bm_adjoint_size(self.ptr, b.ctypes.data_as(_I32), t.ctypes.data_as(_I32))
         self.bodies, self.steps = int(b[0]), int(t[0])
+        # Forward passes run so far: backward() walks the last one's checkpoints, so a caller
+        # holding an earlier pass's count knows to run that pass again first.
+        self.forwards = 0
 
     def check(self, rc, what):
         if rc != 0:
@@ -574,6 +578,7 @@ class Adjoint:
         if ins[0].size != n * HELD_STATE or any(a.size != n for a in ins[1:]):
             raise ValueError("forward: arrays of the wrong size")
         out = np.zeros((n, HELD_STATE))
+        self.forwards += 1
         self.check(self.lib.bm_adjoint_forward(self.ptr, *[a.ctypes.data_as(_D) for a in ins],
                                                out.ctypes.data_as(_D)), "bm_adjoint_forward")
         return out

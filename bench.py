@@ -1,21 +1,27 @@
-"""Benchmark: the soft step (the contact solve's hot loop) on one MI355X, FP64.
+"""Benchmark: Bearing's contact pipeline on one MI355X, FP64, every result the reference's bits.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch-worlds B] [--cpu-seconds S]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--parts ...] [--cpu-seconds S]
 
-Workload (plain drop-in): the 2,000-sphere rain (2,000 spheres r U[0.1, 0.3], restitution
-U[0, 0.7], friction 0.5, in a walled 20x20 m floor; seed 2) as the reference's own step 300
-leaves it before the solve: its manifolds coloured and laid out, its bodies' state
-(tests/golden/data/rain.soft.300). One step is one soft step of that world on the device:
-inertias, prepare, 8 substeps of integrate / warm start / solve / move / relax, restitution,
-store, one workgroup per world. Steps follow each other on the device-resident state.
+Headline (`value`): contact-iterations per second of the plain drop-in's whole step on the
+20,001-body mixed pile (8,000 boxes, 5,000 balls, 4,000 capsules and 3,000 hulls fallen from a
+60 x 80 x 60 m volume onto a 256 x 256 value-noise heightfield; seed 3) at rest: the world is
+stepped through bm_world_step (bounds, broadphase, narrowphase with time of impact, freezing,
+colouring, soft step, catch, sleep; every copy between host and device included) along the
+reference's own 600-step trajectory, and the last K steps, after W untimed warm-up steps, are
+timed: rows x 2 passes x 8 substeps a step over their wall time. A "step" is one world step.
+Every one of the 600 steps' transforms is checked against the reference's afterwards.
 
-Printed: one JSON line. `value` is contact-iterations per second (rows x 2 passes x 8
-substeps per step, rows = manifold points solved). `roofline` prices the kernel against HBM
-with the algorithmic bytes per manifold of the solve / relax / warm-start passes (see
-`algorithmic_bytes`); `cpu_baseline` is the oracle's same soft step (a single-threaded C++
-restatement) on the host on a bounded sample of the same workload; `batched` is world-steps
-per second for a batch of worlds in one launch against stepping the same worlds one launch
-each.
+`roofline` prices the step's dominant kernel, the grid-wide soft step (inertias .. store), with
+the algorithmic bytes of its sweeps (bm_world_read soft_bytes: each field once a pass, each
+body's fields once a manifold end) over its HIP-event time, against HBM's 8 TB/s.
+`cpu_baseline` is the oracle's soft step (the CPU restatement) on the host.
+
+Other legs: `batched_whole_worlds`, the batched form (4,096 worlds of scenes.batched, seed 4,
+300 steps, bm_batch_step) against bm_world_step looped per world, bit-exact against the
+reference for every world; `world`, whole reference trajectories through bm_world;
+`soft_step_microbench`, the soft step alone re-run on one fixed state of the 2,000-sphere rain
+(a microbenchmark of the solve, not a physical step); `batched`, many worlds' soft steps in one
+launch; `adjoint`, the differentiation form (bwd/fwd).
 """
 import argparse
 import json
@@ -69,10 +75,77 @@ def load_world(name):
     return fx, fixtures.soft_columns(fx), world_from_columns
 
 
-def cpu_baseline(fx, seconds):
+def traffic_of(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` as rocprofv3's FETCH_SIZE and WRITE_SIZE
+    counters measured them (each scaled by what the counter reads for a copy of known size),
+    from the records profiles/*_traffic.json keep; None if not measured."""
+    import glob
+
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_traffic.json"))):
+        rec = json.load(open(path))
+        if rec.get("kernel") == kernel and rec.get("workload") == workload:
+            return rec["bytes"]
+    return None
+
+
+def micro_run(args):
+    """The soft step alone (inertias .. store, one workgroup) re-run on one fixed state of the
+    2,000-sphere rain (its reference state before the step-300 solve): a microbenchmark of the
+    solve. After the first run the velocities are integrated against the same contacts, so the
+    later runs are not physical steps; the bits of the first are checked by the tests."""
+    from bearing_amd import native
+    from bearing_amd.tune import tune
+
+    fx, cols, world_from_columns = load_world(WORKLOAD)
+    meta = fx.meta
+    solved = meta["solved"]
+    points = fx["in.c.points"][:solved].astype(np.int64)
+    rows = int(points.sum())
+    inv_mass = fx["in.invMass"]
+    a, b = fx["in.c.a"][:solved], fx["in.c.b"][:solved]
+    step_bytes = algorithmic_bytes(points, inv_mass[a] > 0, inv_mass[b] > 0)
+    plain_world = world_from_columns(fx, cols)
+    if args.plain_launch:
+        shape, resident = (int(x) for x in args.plain_launch.split(","))
+    else:
+        shape, resident, _ = tune([plain_world])
+    ctx = native.Context(0)
+    ctx.set_launch(shape, resident)
+    batch = native.Batch([plain_world])
+    batch.upload(ctx)
+    for _ in range(5):
+        batch.step(ctx)
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.micro_steps):
+        kernel_ms.append(batch.step(ctx))
+    wall = time.perf_counter() - t0
+    avg_kernel_ms = float(np.mean(kernel_ms))
+    achieved = step_bytes / (avg_kernel_ms * 1e-3) / 1e9
+    ctx.set_wide(1)
+    for _ in range(2):
+        batch.step(ctx)
+    wide_ms = float(np.mean([batch.step(ctx) for _ in range(10)]))
+    ctx.set_wide(16384)
+    kernel = f"soft_step_kernel<{'true' if resident else 'false'}, {512 if shape == 2 else 256}, {1 if shape == 0 else 2}>"
+    out = {"workload": f"{WORKLOAD}: the rain's reference state before its step-300 solve, re-run",
+           "bodies": meta["top"], "manifolds": solved, "rows": rows,
+           "contact_iters_per_s": rows * 2 * SUBSTEPS * args.micro_steps / wall,
+           "ms_per_launch": wall * 1000 / args.micro_steps, "kernel_ms": avg_kernel_ms,
+           "launch": {"shape": shape, "resident_lds": bool(resident)}, "grid_wide_kernel_ms": wide_ms,
+           "roofline": {"kernel": kernel, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_of(kernel, WORKLOAD),
+                        "algorithmic_bytes_per_launch": step_bytes}}
+    out["batched"] = batched_soft_run(ctx, args) if "batched" in set(args.parts.split(",")) else None
+    ctx.close()
+    return out
+
+
+def cpu_baseline(seconds):
     """The oracle's soft step on the host, one thread, repeated on the same input for about `seconds`."""
     from oracle import binding
 
+    fx = load_world(WORKLOAD)[0]
     base = binding.columns(fx)
     rows = int(np.sum(fx["in.c.points"][: fx.meta["solved"]]))
     done, elapsed = 0, 0.0
@@ -98,13 +171,146 @@ def cpu_baseline_worlds():
     return {name: world_run(name, lib)["ms_per_step"] for name in WORLD_RUNS if name not in GPU_ONLY_RUNS}
 
 
-WORLD_RUNS = ["pyramid", "rain", "darts", "terrain", "mixed_pile"]
+WORLD_RUNS = ["pyramid", "rain", "darts", "terrain"]
 # Trajectories too long for the host-only port's leg (the 20,001-body pile: ~10 minutes).
 GPU_ONLY_RUNS = {"mixed_pile"}
 # Whole worlds stepped together: the reference's own resolved bodies of these trajectories
 # (13 to 401 bodies: towers, sphere piles, a pyramid, rods and balls, capsules, hulls, a
 # heightfield), repeated to the batch size.
 BATCH_WORLDS_FROM = ["tower", "pile_small", "pyramid", "rain_small", "capsules", "darts", "rocks", "terrain"]
+
+
+PILE = "mixed_pile"
+
+
+def pile_run(timed, warmup):
+    """The headline: the mixed pile's reference trajectory (600 steps) through bm_world_step;
+    the first 600 - timed - warmup steps bring it to rest, then `warmup` untimed steps, then
+    `timed` steps timed one by one (wall clock around bm_world_step; bm_world_read's rows,
+    soft-step bytes and kernel time after each). Every step's transforms hash is checked
+    against the reference's after the run."""
+    import ctypes
+    import hashlib
+
+    import fixtures
+    from bearing_amd import native
+    from worldcheck import fill_world
+
+    fx = fixtures.load(os.path.join(fixtures.DATA, f"{PILE}.traj.bmfx.gz"))
+    meta = fx.meta
+    total = int(meta["steps"])
+    timed = max(1, min(timed, total - 1))
+    warmup = max(0, min(warmup, total - timed))
+    setup = total - timed - warmup
+    w = fill_world(native.World(0), fx)
+    tr = np.zeros(7 * int(meta["bodies"]), np.float32)
+    state = native.WorldState()
+    state.transforms = tr.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    hashes, rows, soft_bytes, soft_ms, walls = [], [], [], [], []
+    before = None
+    t_all = time.perf_counter()
+    for s in range(1, total + 1):
+        if s == setup + warmup + 1:
+            before = w.times()
+        t = time.perf_counter()
+        w.step(meta["dt"])
+        wall = time.perf_counter() - t
+        w.lib.bm_world_read(w.ptr, ctypes.byref(state))
+        hashes.append(hashlib.sha256(tr.tobytes()).hexdigest())
+        rows.append(state.rows)
+        soft_bytes.append(state.soft_bytes)
+        soft_ms.append(state.soft_ms)
+        walls.append(wall)
+        if s % 100 == 0:
+            print(f"bench: {PILE} step {s}", file=sys.stderr, flush=True)
+    all_wall = time.perf_counter() - t_all
+    after = w.times()
+    w.close()
+    first = setup + warmup
+    tw = float(np.sum(walls[first:]))
+    trows = int(np.sum(rows[first:]))
+    return {
+        "steps": total, "bodies": int(meta["bodies"]), "setup_steps": setup, "warmup": warmup, "timed_steps": timed,
+        "bit_exact": bool(all(h == r for h, r in zip(hashes, meta["hashes"]))),
+        "timed_wall_s": tw, "ms_per_step": tw * 1000 / timed,
+        "rows_per_step": trows / timed, "contact_iters_per_s": trows * 2 * SUBSTEPS / tw,
+        "soft_kernel_ms": float(np.mean(soft_ms[first:])),
+        "soft_bytes_per_launch": float(np.mean(soft_bytes[first:])),
+        "soft_only_contact_iters_per_s": trows * 2 * SUBSTEPS / (float(np.sum(soft_ms[first:])) / 1000),
+        "ms_per_step_by_part": {k: (after[k] - before[k]) / timed for k in after},
+        "trajectory": {"ms_per_step": all_wall * 1000 / total, "world_steps_per_s": total / all_wall,
+                       "contact_iters_per_s": int(np.sum(rows)) * 2 * SUBSTEPS / all_wall,
+                       "note": "all 600 steps, the landing included, wall clock around bm_world_step and bm_world_read"},
+    }
+
+
+def c4_run(threads, loop_sample=16):
+    """The batched form as configured: the 4,096 worlds of scenes.batched (seed 4: body counts
+    log-uniform in [8, 512], box towers and sphere piles, each on its own ground), every world
+    stepped 300 steps together by bm_batch_step; against bm_world_step looped over every 256th
+    world for the same 300 steps; every world's final transforms against the reference's."""
+    import hashlib
+
+    import batchedcheck
+    from bearing_amd import native, scenes
+
+    fx = batchedcheck.load()
+    specs = batchedcheck.scene_list(fx)
+    steps, dt = int(fx.meta["steps"]), fx.meta["dt"]
+    batch = native.WorldBatch(0, threads)
+    worlds = []
+    for bodies in specs:
+        w = batch.world()
+        w.add(scenes.resolve(bodies))
+        worlds.append(w)
+    print(f"bench: {len(worlds)} batched worlds built", file=sys.stderr, flush=True)
+    before = batch.times()
+    soft_ms, soft_bytes, rows, walls = [], [], [], []
+    for s in range(steps):
+        t = time.perf_counter()
+        batch.step(dt)
+        walls.append(time.perf_counter() - t)
+        if s % 10 == 9:   # the shared soft launch's time and every world's bytes, every tenth step
+            st = [w.state() for w in worlds]
+            soft_ms.append(st[0].soft_ms)
+            soft_bytes.append(sum(x.soft_bytes for x in st))
+            rows.append(sum(x.rows for x in st))
+        if (s + 1) % 100 == 0:
+            print(f"bench: batched step {s + 1}", file=sys.stderr, flush=True)
+    wall = float(np.sum(walls))
+    after = batch.times()
+    exact = all(hashlib.sha256(w.read()["transforms"].tobytes()).digest() == batchedcheck.final_of(fx, i)
+                for i, w in enumerate(worlds))
+    batch.close()
+    picks = list(range(0, len(specs), max(1, len(specs) // loop_sample)))[:loop_sample]
+    t_loop, loop_exact = 0.0, True
+    for i in picks:
+        w = native.World(0)
+        w.add(scenes.resolve(specs[i]))
+        t = time.perf_counter()
+        for _ in range(steps):
+            w.step(dt)
+        t_loop += time.perf_counter() - t
+        loop_exact = loop_exact and hashlib.sha256(w.read()["transforms"].tobytes()).digest() == batchedcheck.final_of(fx, i)
+        w.close()
+    sample_bodies = sum(len(specs[i]) for i in picks)
+    rate = len(specs) * steps / wall
+    loop_rate = len(picks) * steps / t_loop
+    achieved = float(np.mean(soft_bytes)) / (float(np.mean(soft_ms)) / 1000) / 1e9
+    return {"workload": "scenes.batched(4096, seed 4): log-uniform 8..512 bodies, box towers and sphere piles, "
+                        f"{steps} steps at dt 1/60", "worlds": len(specs), "bodies": int(fx.meta["bodies"]),
+            "steps": steps, "host_threads": threads, "ms_per_step": wall * 1000 / steps, "world_steps_per_s": rate,
+            "looped_plain_world_steps_per_s": loop_rate, "speedup_vs_looped": rate / loop_rate,
+            "looped_sample": f"every {len(specs) // loop_sample}th world ({len(picks)} worlds, {sample_bodies} bodies) "
+                             f"stepped alone through bm_world_step for {steps} steps",
+            "bit_exact": bool(exact), "looped_bit_exact": bool(loop_exact),
+            "ms_per_step_by_part": {k: (after[k] - before[k]) / steps for k in after},
+            "soft_roofline": {"kernel": "soft_step_kernel (all worlds, one launch)", "bound": "hbm",
+                              "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": achieved / HBM_PEAK_GBS, "kernel_ms": float(np.mean(soft_ms)),
+                              "algorithmic_bytes_per_launch": float(np.mean(soft_bytes)),
+                              "rows_per_step": float(np.mean(rows)),
+                              "sampled": "every tenth step"}}
 
 
 def batched_world_run(count, warm, steps, threads):
@@ -293,24 +499,26 @@ def batched_soft_run(ctx, args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=50, help="timed world steps of the pile at rest (the last of its 600)")
+    ap.add_argument("--warmup", type=int, default=5, help="untimed world steps before them")
+    ap.add_argument("--micro-steps", type=int, default=50, help="soft steps of the rain's fixed state (microbenchmark)")
     ap.add_argument("--batch-worlds", type=int, default=4096)
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--batch-threads", type=int, default=16, help="host threads for the whole-world batch")
     ap.add_argument("--whole-batch", type=int, default=None,
-                    help="only step this many whole worlds together (bm_batch_step) and print the timing")
+                    help="only step this many stand-in whole worlds together (bm_batch_step) and print the timing")
+    ap.add_argument("--c4", action="store_true", help="only the batched form's 4,096 worlds and print the timing")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--adjoint-worlds", type=int, default=512, help="worlds of the differentiation form")
     ap.add_argument("--adjoint", type=int, default=None, help="only the differentiation form, with this many worlds")
     ap.add_argument("--plain-launch", default=None,
-                    help="shape,resident for the plain world (default: autotuned, bearing_amd/tune.py)")
+                    help="shape,resident for the microbenchmark's world (default: autotuned, bearing_amd/tune.py)")
     ap.add_argument("--batched-launch", default=None, help="shape,resident for the batch (default: autotuned)")
     ap.add_argument("--world", default=None,
                     help="only step this reference trajectory whole through bm_world and print its timing")
     ap.add_argument("--world-steps", type=int, default=None, help="with --world: only its first steps")
-    ap.add_argument("--parts", default="plain,batched,whole,worlds,adjoint,cpu",
-                    help="which legs of the full line to run (the rest are null): plain,batched,whole,worlds,adjoint,cpu")
+    ap.add_argument("--parts", default="pile,micro,batched,c4,worlds,adjoint,cpu",
+                    help="which legs of the full line to run (the rest are null): pile,micro,batched,c4,worlds,adjoint,cpu")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy this many MiB device to device 5 times (torch), to calibrate HBM byte counters")
     args = ap.parse_args()
@@ -337,85 +545,50 @@ def main():
     if args.whole_batch:
         print(json.dumps({"whole_batch": batched_world_run(args.whole_batch, 60, 20, args.batch_threads)}))
         return
-
-    from bearing_amd import native
-    from bearing_amd.tune import tune
-
-    fx, cols, world_from_columns = load_world(WORKLOAD)
-    meta = fx.meta
-    solved = meta["solved"]
-    points = fx["in.c.points"][:solved].astype(np.int64)
-    rows = int(points.sum())
-    inv_mass = fx["in.invMass"]
-    a, b = fx["in.c.a"][:solved], fx["in.c.b"][:solved]
-    step_bytes = algorithmic_bytes(points, inv_mass[a] > 0, inv_mass[b] > 0)
-
-    plain_world = world_from_columns(fx, cols)
-    if args.plain_launch:
-        shape, resident = (int(x) for x in args.plain_launch.split(","))
-    else:
-        shape, resident, _ = tune([plain_world])
-    ctx = native.Context(0)
-    ctx.set_launch(shape, resident)
-    batch = native.Batch([plain_world])
-    batch.upload(ctx)
-    for _ in range(args.warmup):
-        batch.step(ctx)
-    kernel_ms = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        kernel_ms.append(batch.step(ctx))
-    wall = time.perf_counter() - t0
-    ms_per_step = wall * 1000 / args.steps
-    value = rows * 2 * SUBSTEPS * args.steps / wall
-    avg_kernel_ms = float(np.mean(kernel_ms))
-    achieved = step_bytes / (avg_kernel_ms * 1e-3) / 1e9
-    # The same world stepped by the whole grid (bm_set_wide) for comparison: same bits.
-    ctx.set_wide(1)
-    for _ in range(2):
-        batch.step(ctx)
-    wide_ms = float(np.mean([batch.step(ctx) for _ in range(10)]))
-    ctx.set_wide(16384)
+    if args.c4:
+        print(json.dumps({"c4": c4_run(args.batch_threads)}))
+        return
 
     parts = set(args.parts.split(","))
-    batched = None
-    if "batched" in parts:
-        batched = batched_soft_run(ctx, args)
-    ctx.close()
-    whole = batched_world_run(args.batch_worlds, 60, 20, args.batch_threads) if "whole" in parts else None
+    pile = pile_run(args.steps, args.warmup) if "pile" in parts else None
+    micro = micro_run(args) if "micro" in parts else None
+    c4 = c4_run(args.batch_threads) if "c4" in parts else None
     worlds_run = {name: world_run(name) for name in WORLD_RUNS} if "worlds" in parts else None
     adjoint = adjoint_run(args.adjoint_worlds) if "adjoint" in parts else None
     cpu = None
     if "cpu" in parts:
-        cpu = cpu_baseline(fx, args.cpu_seconds)
+        cpu = cpu_baseline(args.cpu_seconds)
         if worlds_run:
             for name, ms in cpu_baseline_worlds().items():
                 worlds_run[name]["cpu_port_ms_per_step"] = ms
-    # HBM bytes per launch of the plain kernel, from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs
-    # recorded in profiles/ (scaled by a copy kernel of known size); null if not measured
-    # for this launch shape.
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "soft_step_rain2000_traffic.json")
-    if os.path.exists(tpath):
-        rec = json.load(open(tpath))
-        if rec.get("kernel") == f"soft_step_kernel<{'true' if resident else 'false'}, {512 if shape == 2 else 256}, {1 if shape == 0 else 2}>":
-            traffic = rec["bytes"]
+    if pile is not None:
+        achieved = pile["soft_bytes_per_launch"] / (pile["soft_kernel_ms"] * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "kernel": "soft_wide_kernel<256> (the grid-wide soft step)", "achieved": achieved,
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                    "traffic": traffic_of("soft_wide_kernel<256>", PILE),
+                    "algorithmic_bytes_per_launch": pile["soft_bytes_per_launch"], "kernel_ms": pile["soft_kernel_ms"]}
+        value, ms_per_step, steps, warmup = pile["contact_iters_per_s"], pile["ms_per_step"], pile["timed_steps"], pile["warmup"]
+    else:
+        roofline, value, ms_per_step, steps, warmup = None, None, None, args.steps, args.warmup
     line = {
         "metric": "contact-iters/s", "value": value, "unit": "contact-iterations/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "steps": steps, "warmup": warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic: the reference's own state of the seeded 2,000-sphere rain before its step-300 solve",
-        "config": {"workload": "rain-2000 soft step (inertias..store, 8 substeps), plain drop-in, one world",
-                   "bodies": meta["top"], "manifolds": solved, "rows": rows, "colours": int(np.count_nonzero(np.diff(fx["in.starts"][:33]))),
-                   "kernel_ms": avg_kernel_ms, "launch": {"shape": shape, "resident_lds": bool(resident)},
-                   "grid_wide_kernel_ms": wide_ms},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes_per_launch": step_bytes},
+        "data": "synthetic: the seeded 20,001-body mixed pile (seed 3) stepped from its start along the reference's "
+                "own trajectory; every step's transforms checked against the reference's",
+        "config": {"workload": "mixed pile (20,001 bodies on a 256x256 heightfield), plain drop-in world.step "
+                               "(bm_world_step), whole step at rest",
+                   "timed": f"steps {pile['setup_steps'] + pile['warmup'] + 1}..{pile['steps']} of {pile['steps']}"
+                   if pile else None,
+                   "rows_per_step": pile["rows_per_step"] if pile else None},
+        "roofline": roofline,
         "cpu_baseline": cpu,
-        "batched": batched,
-        "batched_whole_worlds": whole,
+        "pile": pile,
+        "batched_whole_worlds": c4,
+        "world_steps_per_s_batched": c4["world_steps_per_s"] if c4 else None,
+        "batched_speedup_vs_looped": c4["speedup_vs_looped"] if c4 else None,
         "world": worlds_run,
+        "soft_step_microbench": micro,
         "adjoint": adjoint,
         "bwd_fwd": adjoint["bwd_fwd"] if adjoint else None,
     }

@@ -131,6 +131,7 @@ int bm_held_times(const bm_context*, double* forward_ms, double* backward_ms) {
   return BM_OK;
 }
 const char* bm_last_error(const bm_context* ctx) { return ctx->error.c_str(); }
+double bm_last_kernel_ms(const bm_context*) { return 0; }
 int bm_upload(bm_context* ctx, const bm_columns* host) {
   if (host->n_worlds != 1) return BM_ERR_ARGUMENT;
   ctx->hs.assign(1, *host);
@@ -230,8 +231,19 @@ int soft_one(bm_context* ctx, bm_columns& h) {
 }
 }  // namespace
 
+// For tests of failure handling: the next soft step of any context fails as a device error
+// would, after the narrowphase has moved the world's pairs on (the step is then half done).
+static int fail_next_soft_step = 0;
+
 extern "C" {
+void stub_fail_next_soft_step() { fail_next_soft_step = 1; }
+
 int bm_soft_step(bm_context* ctx) {
+  if (fail_next_soft_step) {
+    fail_next_soft_step = 0;
+    ctx->error = "bm_soft_step: a device failure the test asked for";
+    return BM_ERR_DEVICE;
+  }
   for (bm_columns& h : ctx->hs)
     if (int rc = soft_one(ctx, h)) return rc;
   return BM_OK;

@@ -168,3 +168,28 @@ def test_host_batch_worlds_record_as_alone(lib):
     for key in ga:
         assert np.array_equal(ga[key], gb[key]), key
     a.close(); b.close(); batch.close()
+
+
+def test_host_torch_function_steps_again_after_another_forward(lib):
+    # Two forward passes through the Function from different states, then the first one's
+    # backward: it walks its own pass (run again), not the second's checkpoints, whether the
+    # Adjoint was wrapped by Held or used bare.
+    import torch
+
+    from bearing_amd.adjoint import HeldTrajectory
+    bodies, _ = SCENES["slide"]
+    w = record(bodies, 20, lib=lib)
+    adj = native.Adjoint([w], every=5, lib=lib)
+    rec = {k: torch.from_numpy(np.array(v)) for k, v in adj.recorded().items()}
+    s1 = rec["state"].clone().requires_grad_(True)
+    s2 = rec["state"].clone()
+    s2[1, 7] += 0.05   # a second pass from another initial velocity
+    W = torch.from_numpy(np.random.default_rng(9).normal(size=(2, 13)))
+    out1 = HeldTrajectory.apply(adj, s1, rec["mass"], rec["friction"], rec["restitution"])
+    HeldTrajectory.apply(adj, s2, rec["mass"], rec["friction"], rec["restitution"])
+    (out1 * W).sum().backward()
+    fresh = native.Adjoint([w], every=5, lib=lib)
+    fresh.forward(*(rec[k].numpy() for k in ("state", "mass", "friction", "restitution")))
+    want = fresh.backward(W.numpy())
+    assert np.array_equal(s1.grad.numpy(), want["state"])
+    adj.close(); fresh.close()

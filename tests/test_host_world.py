@@ -85,3 +85,35 @@ def test_host_batched_sample_matches_reference(lib):
         why = batchedcheck.check_final(fx, i, w)
         assert why is None, why
     batch.close()
+
+
+def test_host_world_refuses_steps_after_one_fails_part_way(lib):
+    # A step that fails after it began (here the soft step, with the pairs already narrowed)
+    # leaves the world half stepped: every later step returns the same failure. A dt that is
+    # refused at the door changes nothing and latches nothing.
+    fx = traj("tower")
+    w = make_world(fx, lib=lib)
+    with pytest.raises(native.BearingError, match="dt must be a positive"):
+        w.step(0.0)
+    w.step(fx.meta["dt"])
+    lib.stub_fail_next_soft_step()
+    with pytest.raises(native.BearingError, match="a device failure the test asked for"):
+        w.step(fx.meta["dt"])
+    with pytest.raises(native.BearingError, match="takes no more steps"):
+        w.step(fx.meta["dt"])
+    w.close()
+
+
+def test_host_batch_refuses_steps_after_one_fails_part_way(lib):
+    from worldcheck import fill_world
+    fx = traj("tower")
+    batch = native.WorldBatch(threads=2, lib=lib)
+    worlds = [fill_world(batch.world(), fx) for _ in range(3)]
+    batch.step(fx.meta["dt"])
+    lib.stub_fail_next_soft_step()
+    with pytest.raises(native.BearingError, match="a device failure the test asked for"):
+        batch.step(fx.meta["dt"])
+    with pytest.raises(native.BearingError, match="takes no more steps|its worlds take no more steps"):
+        batch.step(fx.meta["dt"])
+    assert "no more steps" in lib.bm_world_error(worlds[1].ptr).decode()
+    batch.close()
