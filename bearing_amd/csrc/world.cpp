@@ -408,7 +408,16 @@ struct bm_world {
   std::vector<HostField> fields;
   std::vector<double> field_heights;
   std::vector<int32_t> joint_head, in_awake, awake_list, island, island_next, parent, slept;
-  std::vector<float> transforms;
+  std::vector<float> transforms, previous;
+  // moved() (src/world.js:771-803): each slot's stamp, the step it last moved in (or the
+  // next step, for a body added since); the log of slots whose stamp changed other than by
+  // moving awake, or that left the awake list (src/world.js:1465-1492), in stamp order;
+  // entries stamped log_floor or earlier were dropped. seen marks slots already listed.
+  std::vector<int64_t> moved_at, log_at;
+  std::vector<int32_t> log_slots;
+  std::vector<uint32_t> seen;
+  int64_t log_last = 0, log_floor = 0;
+  uint32_t seen_mark = 0;
   int awake_count = 0, frozen_count = 0, slept_count = 0;
   bool still_ready = false, wake_pending = false;
   double step_dt = 0;
@@ -585,6 +594,24 @@ void awaken(bm_world* w, int s) {
   settle(w, s);
 }
 
+// src/world.js:1465-1492 (_log): slot s into the moved log with a stamp at least its own and
+// at least the last entry's; past one entry a body, the older half goes and moved() asked
+// about steps that old reads every body's stamp instead.
+void log_moved(bm_world* w, int s) {
+  const int64_t stamp = std::max(w->moved_at[s], w->log_last);
+  w->log_slots.push_back(s);
+  w->log_at.push_back(stamp);
+  w->log_last = stamp;
+  const int64_t n = (int64_t)w->log_slots.size();
+  if (n <= w->top) return;
+  const int64_t floor = w->log_at[(n - 1) >> 1];
+  int64_t from = (n - 1) >> 1;
+  while (from < n && w->log_at[from] <= floor) from++;
+  w->log_slots.erase(w->log_slots.begin(), w->log_slots.begin() + from);
+  w->log_at.erase(w->log_at.begin(), w->log_at.begin() + from);
+  w->log_floor = floor;
+}
+
 // src/world.js:1311-1320 (_unawaken)
 void unawaken(bm_world* w, int s) {
   const int i = w->in_awake[s] - 1;
@@ -594,6 +621,8 @@ void unawaken(bm_world* w, int s) {
   w->in_awake[last] = i + 1;
   w->in_awake[s] = 0;
   w->awake[s] = 0;
+  // It may have moved in steps a later moved() asks about.
+  log_moved(w, s);
 }
 
 // src/world.js:1175-1180 (_thaw)
@@ -2482,6 +2511,20 @@ int step_begin(bm_world* w, double dt) {
   if (!(dt > 0) || dt == INFINITY) return fail(w, BM_ERR_ARGUMENT, "step: dt must be a positive number of seconds");
   ++w->step_count;
   if (w->wake_pending) sync_pairs(w);
+  // src/world.js:902-918: awake bodies' `previous` is where they start this step; a body that
+  // fell asleep at the end of the last one moved in it, so it gets this once more, and its
+  // stamp, once more.
+  for (int i = 0; i < w->awake_count; i++) {
+    const size_t t7 = 7 * (size_t)w->awake_list[i];
+    for (int k = 0; k < 7; k++) w->previous[t7 + k] = w->transforms[t7 + k];
+  }
+  for (int i = 0; i < w->slept_count; i++) {
+    const int s = w->slept[i];
+    const size_t t7 = 7 * (size_t)s;
+    for (int k = 0; k < 7; k++) w->previous[t7 + k] = w->transforms[t7 + k];
+    w->moved_at[s] = w->step_count;
+    log_moved(w, s);
+  }
   w->slept_count = 0;
   w->step_dt = dt;
   const double gl = gravity_length(w);
@@ -2525,6 +2568,7 @@ int step_end(bm_world* w, double dt) {
     if (w->frozen[s] == 1) continue;
     if (w->frozen[s] == 2) w->frozen[s] = 1;
     write_transform(w, s);
+    w->moved_at[s] = w->step_count;
   }
   w->step_dt = 0;
   if (w->sleeping) sleep_step(w, dt);
@@ -2773,6 +2817,9 @@ int bm_world_add(bm_world* w, const bm_bodies* b) {
     grow(*v, 1);
   w->which.resize(top, -1);
   w->transforms.resize(7 * (size_t)top);
+  w->previous.resize(7 * (size_t)top);
+  w->moved_at.resize(top);
+  w->seen.resize(top);
   w->top = top;
   for (int i = 0; i < b->n; i++) {
     const int s = first + i;
@@ -2796,12 +2843,16 @@ int bm_world_add(bm_world* w, const bm_bodies* b) {
     w->framed[s] = b->framed && b->framed[i] != 0 ? 1 : 0;
     for (int k = 0; k < 3; k++) w->offset[3 * s + k] = w->framed[s] ? b->offset[3 * i + k] : 0;
     for (int k = 0; k < 4; k++) w->frame[4 * s + k] = w->framed[s] ? b->frame[4 * i + k] : (k == 3 ? 1 : 0);
-    // addBody (src/world.js:563-578): placed, bounded, filed, awake unless static.
+    // addBody (src/world.js:563-578): placed (transforms and previous, src/world.js:2283-2295),
+    // bounded, filed, awake unless static, stamped as moved since the last step (1450-1454).
     write_transform(w, s);
+    for (int k = 0; k < 7; k++) w->previous[7 * (size_t)s + k] = w->transforms[7 * (size_t)s + k];
     for (int k = 0; k < 3; k++) w->start[3 * s + k] = w->position[3 * s + k];
     bounds_of(w, s, 0, 0);
     broadphase_add(w, s);
     if (w->type[s] != STATIC) awaken(w, s);
+    w->moved_at[s] = w->step_count + 1;
+    log_moved(w, s);
   }
   return BM_OK;
 }
@@ -3091,6 +3142,7 @@ int bm_world_read(const bm_world* w, bm_world_state* out) {
   out->soft_bytes = w->step_bytes;
   out->soft_ms = w->step_soft_ms;
   if (out->transforms) std::memcpy(out->transforms, w->transforms.data(), 7 * sizeof(float) * w->top);
+  if (out->previous) std::memcpy(out->previous, w->previous.data(), 7 * sizeof(float) * w->top);
   if (out->position) std::memcpy(out->position, w->position.data(), 3 * sizeof(double) * w->top);
   if (out->rotation) std::memcpy(out->rotation, w->rotation.data(), 4 * sizeof(double) * w->top);
   if (out->velocity) std::memcpy(out->velocity, w->velocity.data(), 3 * sizeof(double) * w->top);
@@ -3098,6 +3150,45 @@ int bm_world_read(const bm_world* w, bm_world_state* out) {
   return BM_OK;
 }
 
+
+// src/world.js:771-803 (moved): the slots that moved in any step after `since` (and always in
+// the last one), with bodies added since the last step: the awake list's in its order, then
+// the log's from its end; a `since` older than the log reaches takes every body by its stamp.
+int bm_world_moved(bm_world* w, int32_t since, int32_t* out, int32_t* count) {
+  if (!w || !out || !count) return BM_ERR_ARGUMENT;
+  w->error.clear();
+  if (since < 0) return fail(w, BM_ERR_ARGUMENT, "moved: since must be a whole number, 0 or more");
+  const int64_t after = std::min<int64_t>(since, (int64_t)w->step_count - 1);
+  const int64_t* at = w->moved_at.data();
+  int k = 0;
+  if (after < w->log_floor) {
+    for (int s = 0; s < w->top; s++)
+      if (at[s] > after) out[k++] = s;
+    *count = k;
+    return BM_OK;
+  }
+  if (++w->seen_mark == 0) {
+    std::fill(w->seen.begin(), w->seen.end(), 0u);
+    w->seen_mark = 1;
+  }
+  const uint32_t mark = w->seen_mark;
+  for (int i = 0; i < w->awake_count; i++) {
+    const int s = w->awake_list[i];
+    if (at[s] > after) {
+      w->seen[s] = mark;
+      out[k++] = s;
+    }
+  }
+  for (int64_t i = (int64_t)w->log_slots.size() - 1; i >= 0 && w->log_at[i] > after; i--) {
+    const int s = w->log_slots[i];
+    if (s < w->top && at[s] > after && w->seen[s] != mark) {
+      w->seen[s] = mark;
+      out[k++] = s;
+    }
+  }
+  *count = k;
+  return BM_OK;
+}
 
 // ---- the adjoint: held trajectories of recorded worlds ----
 

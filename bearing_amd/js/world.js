@@ -32,7 +32,6 @@ class World {
     this._counts = null;
     this._transforms = new Float32Array(0);
     this._previous = new Float32Array(0);
-    this._movedAt = new Float64Array(0);   // the step a slot last moved in; stepCount + 1 if added since
     this._movedOut = new Int32Array(0);
   }
 
@@ -50,28 +49,27 @@ class World {
     const first = this._top;
     this._sync();
     const slots = [];
-    for (let s = first; s < this._top; s++) {
-      this._movedAt[s] = this._stepCount + 1;
-      this._previous.set(this._transforms.subarray(7 * s, 7 * s + 7), 7 * s);
-      slots.push(s);
-    }
+    for (let s = first; s < this._top; s++) slots.push(s);
     return slots;
   }
 
+  /*
+   * One step. The device world counts it only once it has begun changing the world: a step
+   * it refuses at the door (dt) leaves stepCount as it was; one that failed part-way is
+   * counted, since the world is then part-way through it (and refuses every later step).
+   */
   step(dt) {
     this._live('step');
     if (typeof dt !== 'number' || !(dt > 0) || !Number.isFinite(dt)) fail('step', `dt must be a number above 0, got ${dt}`);
-    this._previous.set(this._transforms);
-    this._native.step(this._world, dt);
+    try {
+      this._native.step(this._world, dt);
+    } catch (e) {
+      this._stepCount++;
+      this._sync();
+      throw e;
+    }
     this._stepCount++;
     this._sync();
-    // A slot moved in this step when its drawn pose changed.
-    const t = this._transforms, p = this._previous, at = this._movedAt, n = this._top;
-    for (let s = 0; s < n; s++) {
-      const k = 7 * s;
-      if (t[k] !== p[k] || t[k + 1] !== p[k + 1] || t[k + 2] !== p[k + 2] || t[k + 3] !== p[k + 3] ||
-          t[k + 4] !== p[k + 4] || t[k + 5] !== p[k + 5] || t[k + 6] !== p[k + 6]) at[s] = this._stepCount;
-    }
   }
 
   get transforms() {
@@ -102,16 +100,16 @@ class World {
 
   /*
    * The slots of the bodies that moved in any step after step `since`, and always of those
-   * that moved in the last one; with bodies added since the last step. A view the next
-   * call or step overwrites.
+   * that moved in the last one; with bodies added since the last step: the reference's list in
+   * the reference's order (the device world keeps its stamps and moved log, src/world.js:771-803).
+   * A view the next call or step overwrites.
    */
   moved(since) {
     this._live('moved');
     if (!Number.isInteger(since) || since < 0) fail('moved', `since must be a whole number, 0 or more, got ${since}`);
-    const after = Math.min(since, this._stepCount - 1), at = this._movedAt, out = this._movedOut;
-    let k = 0;
-    for (let s = 0; s < this._top; s++) if (at[s] > after) out[k++] = s;
-    return out.subarray(0, k);
+    if (this._top === 0) return this._movedOut.subarray(0, 0);
+    const k = this._native.moved(this._world, since, this._movedOut);
+    return this._movedOut.subarray(0, k);
   }
 
   /* Milliseconds spent so far in each part of the step (the C ABI's bm_world_times). */
@@ -136,11 +134,10 @@ class World {
       const grow = (A, old, per) => { const a = new A(per * top); a.set(old.subarray(0, Math.min(old.length, per * top))); return a; };
       this._transforms = grow(Float32Array, this._transforms, 7);
       this._previous = grow(Float32Array, this._previous, 7);
-      this._movedAt = grow(Float64Array, this._movedAt, 1);
       this._movedOut = new Int32Array(top);
       this._top = top;
     }
-    this._counts = this._native.read(this._world, this._transforms);
+    this._counts = this._native.read(this._world, this._transforms, this._previous);
   }
 }
 

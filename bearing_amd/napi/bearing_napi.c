@@ -209,30 +209,44 @@ static napi_value set_int(napi_env env, napi_value obj, const char* name, double
   return obj;
 }
 
-/* read(world, transforms: Float32Array | null) -> {top, awake, contacts, pairs, frozen,
- * impacts, catches}; transforms, when given, receives 7 floats a slot for `top` slots. */
+/* A Float32Array argument of `per` floats a slot, or NULL when it is null or absent; *bad is
+ * set when it is given but wrong (the exception is then pending). */
+static float* floats_arg(napi_env env, napi_value v, const char* name, size_t n, int* bad) {
+  napi_valuetype t;
+  *bad = 0;
+  if (napi_typeof(env, v, &t) != napi_ok) return (*bad = 1), NULL;
+  if (t == napi_undefined || t == napi_null) return NULL;
+  const void* data = NULL;
+  napi_value holder_obj;
+  if (napi_create_object(env, &holder_obj) != napi_ok || napi_set_named_property(env, holder_obj, name, v) != napi_ok ||
+      !column(env, holder_obj, name, napi_float32_array, n, &data))
+    return (*bad = 1), NULL;
+  return (float*)data;
+}
+
+/* read(world, transforms: Float32Array | null, previous: Float32Array | null) -> {top, awake,
+ * contacts, pairs, frozen, impacts, catches}; transforms and previous, when given, receive 7
+ * floats a slot for `top` slots. */
 static napi_value js_read(napi_env env, napi_callback_info info) {
-  size_t argc = 2;
-  napi_value argv[2];
+  size_t argc = 3;
+  napi_value argv[3];
   NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
-  if (argc < 1) return fail(env, "transforms", "read(world, transforms)");
+  if (argc < 1) return fail(env, "transforms", "read(world, transforms, previous)");
   holder* h = world_of(env, argv[0], "transforms");
   if (!h) return NULL;
   bm_world_state s = {0};
   if (bm_world_read(h->world, &s) != BM_OK) return fail(env, "transforms", bm_world_error(h->world));
+  int bad = 0;
   if (argc >= 2) {
-    napi_valuetype t;
-    NAPI(env, napi_typeof(env, argv[1], &t));
-    if (t != napi_undefined && t != napi_null) {
-      const void* data = NULL;
-      napi_value holder_obj;
-      NAPI(env, napi_create_object(env, &holder_obj));
-      NAPI(env, napi_set_named_property(env, holder_obj, "transforms", argv[1]));
-      if (!column(env, holder_obj, "transforms", napi_float32_array, (size_t)s.top * 7, &data)) return NULL;
-      s.transforms = (float*)data;
-      if (bm_world_read(h->world, &s) != BM_OK) return fail(env, "transforms", bm_world_error(h->world));
-    }
+    s.transforms = floats_arg(env, argv[1], "transforms", (size_t)s.top * 7, &bad);
+    if (bad) return NULL;
   }
+  if (argc >= 3) {
+    s.previous = floats_arg(env, argv[2], "previous", (size_t)s.top * 7, &bad);
+    if (bad) return NULL;
+  }
+  if ((s.transforms || s.previous) && bm_world_read(h->world, &s) != BM_OK)
+    return fail(env, "transforms", bm_world_error(h->world));
   napi_value out;
   NAPI(env, napi_create_object(env, &out));
   if (!set_int(env, out, "top", s.top) || !set_int(env, out, "awake", s.awake) ||
@@ -240,6 +254,31 @@ static napi_value js_read(napi_env env, napi_callback_info info) {
       !set_int(env, out, "frozen", s.frozen) || !set_int(env, out, "impacts", (double)s.impacts) ||
       !set_int(env, out, "catches", (double)s.catches))
     return NULL;
+  return out;
+}
+
+/* moved(world, since, out: Int32Array with room for every slot) -> how many slots it wrote:
+ * world.moved(since), src/world.js:771-803. */
+static napi_value js_moved(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  if (argc < 3) return fail(env, "moved", "moved(world, since, out)");
+  holder* h = world_of(env, argv[0], "moved");
+  if (!h) return NULL;
+  int32_t since = 0;
+  if (napi_get_value_int32(env, argv[1], &since) != napi_ok) return fail(env, "moved", "since must be a number");
+  bm_world_state s = {0};
+  if (bm_world_read(h->world, &s) != BM_OK) return fail(env, "moved", bm_world_error(h->world));
+  napi_value holder_obj;
+  const void* data = NULL;
+  NAPI(env, napi_create_object(env, &holder_obj));
+  NAPI(env, napi_set_named_property(env, holder_obj, "out", argv[2]));
+  if (!column(env, holder_obj, "out", napi_int32_array, (size_t)s.top, &data)) return NULL;
+  int32_t count = 0;
+  if (bm_world_moved(h->world, since, (int32_t*)data, &count) != BM_OK) return fail(env, "moved", bm_world_error(h->world));
+  napi_value out;
+  NAPI(env, napi_create_int32(env, count, &out));
   return out;
 }
 
@@ -276,7 +315,7 @@ NAPI_MODULE_INIT() {
   static const struct {
     const char* name;
     napi_callback fn;
-  } fns[] = {{"create", js_create}, {"add", js_add}, {"step", js_step}, {"read", js_read}, {"times", js_times}, {"destroy", js_destroy}};
+  } fns[] = {{"create", js_create}, {"add", js_add}, {"step", js_step}, {"read", js_read}, {"moved", js_moved}, {"times", js_times}, {"destroy", js_destroy}};
   for (size_t i = 0; i < sizeof fns / sizeof fns[0]; i++) {
     napi_value f;
     if (napi_create_function(env, fns[i].name, NAPI_AUTO_LENGTH, fns[i].fn, NULL, &f) != napi_ok ||

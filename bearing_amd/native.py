@@ -118,7 +118,8 @@ class WorldState(ctypes.Structure):
                 ("pairs", ctypes.c_int32), ("frozen", ctypes.c_int32), ("impacts", ctypes.c_int64),
                 ("catches", ctypes.c_int64),
                 ("transforms", ctypes.POINTER(ctypes.c_float)), ("position", _D), ("rotation", _D), ("velocity", _D),
-                ("angular", _D), ("rows", ctypes.c_int64), ("soft_bytes", ctypes.c_double), ("soft_ms", ctypes.c_double)]
+                ("angular", _D), ("rows", ctypes.c_int64), ("soft_bytes", ctypes.c_double), ("soft_ms", ctypes.c_double),
+                ("previous", ctypes.POINTER(ctypes.c_float))]
 
 
 def world_signatures(lib):
@@ -131,6 +132,7 @@ def world_signatures(lib):
     lib.bm_world_add.argtypes = [vp, ctypes.POINTER(Bodies)]
     lib.bm_world_step.argtypes = [vp, ctypes.c_double]
     lib.bm_world_read.argtypes = [vp, ctypes.POINTER(WorldState)]
+    lib.bm_world_moved.argtypes = [vp, ctypes.c_int32, _I32, _I32]
     lib.bm_world_set_hulls.argtypes = [vp, ctypes.c_int32, _I32, _I32, _I32, _D, _I32, _I32, _D]
     lib.bm_world_set_fields.argtypes = [vp, ctypes.c_int32, _I32, _I32, _D, ctypes.POINTER(ctypes.c_int64), _D]
     lib.bm_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int32, ctypes.c_int32]
@@ -449,10 +451,12 @@ class World:
     def read(self):
         """The counts, and transforms (float32, 7 a slot), position, rotation, velocity, angular."""
         top = self.state().top
-        out = {"transforms": np.zeros(7 * top, np.float32), "position": np.zeros(3 * top), "rotation": np.zeros(4 * top),
+        out = {"transforms": np.zeros(7 * top, np.float32), "previous": np.zeros(7 * top, np.float32),
+               "position": np.zeros(3 * top), "rotation": np.zeros(4 * top),
                "velocity": np.zeros(3 * top), "angular": np.zeros(3 * top)}
         s = WorldState()
         s.transforms = out["transforms"].ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        s.previous = out["previous"].ctypes.data_as(ctypes.POINTER(ctypes.c_float))
         for name in ("position", "rotation", "velocity", "angular"):
             setattr(s, name, out[name].ctypes.data_as(_D))
         self.check(self.lib.bm_world_read(self.ptr, ctypes.byref(s)), "bm_world_read")
@@ -460,6 +464,14 @@ class World:
                      "soft_ms"):
             out[name] = getattr(s, name)
         return out
+
+    def moved(self, since):
+        """world.moved(since): the slots that moved after step `since`, in the reference's order."""
+        out = np.zeros(max(self.state().top, 1), np.int32)
+        n = np.zeros(1, np.int32)
+        self.check(self.lib.bm_world_moved(self.ptr, since, out.ctypes.data_as(_I32), n.ctypes.data_as(_I32)),
+                   "bm_world_moved")
+        return out[:int(n[0])].copy()
 
     TIMES = ["prepare", "narrowphase_device", "narrow_host", "freeze", "contact_device", "catch_sleep"]
 

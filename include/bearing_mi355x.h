@@ -316,6 +316,8 @@ typedef struct bm_world_state {
   double *position, *rotation, *velocity, *angular;
   int64_t rows;
   double soft_bytes, soft_ms;
+  float* previous; /* world.previous (src/world.js:547-556): each slot's transform as its last
+                      step began (a body that fell asleep: as the step after began) */
 } bm_world_state;
 
 /* new World({gravity, substeps, sleeping}) (src/world.js:314): gravity may be NULL
@@ -343,6 +345,14 @@ int bm_world_set_fields(bm_world* world, int32_t n, const int32_t* columns, cons
  * every later step of the world returns the same code and message. */
 int bm_world_step(bm_world* world, double dt);
 int bm_world_read(const bm_world* world, bm_world_state* out);
+/* world.moved(since) (src/world.js:771-803): into out (room for every slot) the slots of the
+ * bodies that moved in any step after step `since`, and always of those that moved in the last
+ * one, with bodies added since the last step; *count of them. The order is the reference's:
+ * the awake list's, then the moved log's from its newest entry, or, for a `since` older than
+ * the log keeps, every slot in order (src/world.js:1465-1492). A body is stamped when it moves
+ * awake and unfrozen in a step (src/world.js:1040-1057), when it is added (1450-1454), and in
+ * the step after it fell asleep (912-918). */
+int bm_world_moved(bm_world* world, int32_t since, int32_t* out, int32_t* count);
 /* Milliseconds the world's steps have spent so far in each part, into out[BM_WORLD_TIMES]:
  * 0 bounds, broadphase and preparing the first narrow pass; 1 the narrowphase on the device
  * (with its copies); 2 taking the narrowphase's results in order and the later passes'

@@ -15,8 +15,9 @@
 //          after every call of that phase, in order.
 //   colour the manifold endpoints colour() reads and the colours, order, starts
 //          and rows it writes (src/graph.js:105-170).
-//   traj   per step, the SHA-256 of world.transforms, and the transforms
-//          themselves every `every` steps.
+//   traj   per step, the SHA-256 of world.transforms, of world.previous and of
+//          world.moved() since the step before and since seven before, and the
+//          transforms themselves every `every` steps.
 import { World } from '/root/reference/src/bearing.js';
 import { createHash } from 'crypto';
 import { readFileSync, writeFileSync } from 'fs';
@@ -196,10 +197,18 @@ function main() {
   const hashes = [], every = job.every || 0, kept = {};
   const counts = { awake: new Int32Array(job.steps), contacts: new Int32Array(job.steps),
     frozen: new Int32Array(job.steps) };
+  // world.moved() after every step, as the SHA-256 of its Int32Array (order counts): since the
+  // step before (what a renderer drawing every step asks) and since seven steps before; and
+  // world.previous. Before the first step, moved(0) (every body added).
+  const digest = (a) => createHash('sha256').update(Buffer.from(a.buffer, a.byteOffset, a.byteLength)).digest('hex');
+  const moved = { last: [], older: [], previous: [], added: digest(world.moved(0)) };
   for (let s = 1; s <= job.steps; s++) {
     world.step(job.dt);
     const t = world._transforms.subarray(0, 7 * world._slots.top);
     hashes.push(createHash('sha256').update(Buffer.from(t.buffer, t.byteOffset, t.byteLength)).digest('hex'));
+    moved.last.push(digest(world.moved(world.stepCount - 1)));
+    moved.older.push(digest(world.moved(Math.max(0, world.stepCount - 7))));
+    moved.previous.push(digest(world.previous.subarray(0, 7 * world._slots.top)));
     counts.awake[s - 1] = world._awakeCount;
     counts.contacts[s - 1] = world._contacts.count;
     counts.frozen[s - 1] = world._frozenCount;
@@ -210,7 +219,7 @@ function main() {
   writeFileSync(`${job.out}.traj.bmfx.gz`, pack(Object.assign({ 'awake': counts.awake, 'contacts': counts.contacts,
     'frozen': counts.frozen },
     kept, final, init), { steps: job.steps, dt: job.dt, hashes, bodies: scene.length, impacts: world._impacts,
-    catches: world._catches, solverSteps: world._solverSteps }));
+    catches: world._catches, solverSteps: world._solverSteps, moved }));
 }
 
 main();

@@ -6,7 +6,8 @@ init.* columns, written by src/world.js _write) bit for bit, and the addon loads
 calls and refuses to make a world without a GPU. GPU: scenes read from the reference's scene
 files step through `new World()` / `addBody` / `step(dt)` in Node and every step's
 `world.transforms` hashes to the reference's (SHA-256 of the Float32Array), with the awake,
-contact and frozen counts; `moved(stepCount - 1)` names exactly the slots whose pose changed.
+contact and frozen counts, `previous`, and `moved()` since the step before and since seven
+before, each list in the reference's order.
 """
 import json
 import os
@@ -64,7 +65,7 @@ def test_js_addon_loads_and_fails_loudly_without_a_gpu():
     import torch
 
     r = json.loads(node("load"))
-    assert r["exports"] == ["create", "add", "step", "read", "times", "destroy"]
+    assert r["exports"] == ["create", "add", "step", "read", "moved", "times", "destroy"]
     if not torch.cuda.is_available():
         assert r["threw"].startswith("World: no HIP device 0")
 
@@ -79,9 +80,12 @@ def test_gpu_js_world_steps_the_reference_trajectory(name):
     lines = node("steps", os.path.join(SCENES, f"{name}.json"), str(meta["steps"]), repr(meta["dt"]),
                  timeout=110).splitlines()
     assert len(lines) == meta["steps"] + 1
-    for s, line in enumerate(lines[:-1], 1):
-        h, awake, contacts, frozen = json.loads(line)
+    moved = meta["moved"]
+    assert json.loads(lines[0]) == [moved["added"]], f"{name}: moved(0) before the first step differs"
+    for s, line in enumerate(lines[1:], 1):
+        h, awake, contacts, frozen, last, older, previous = json.loads(line)
         want = (meta["hashes"][s - 1], int(fx["awake"][s - 1]), int(fx["contacts"][s - 1]), int(fx["frozen"][s - 1]))
         assert (h, awake, contacts, frozen) == want, f"{name}: step {s} differs from the reference"
-    tail = json.loads(lines[-1])
-    assert tail == {"moved": True, "stepCount": meta["steps"]}
+        assert previous == moved["previous"][s - 1], f"{name}: step {s}: previous differs from the reference"
+        assert last == moved["last"][s - 1], f"{name}: step {s}: moved({s - 1}) differs from the reference"
+        assert older == moved["older"][s - 1], f"{name}: step {s}: moved({max(0, s - 7)}) differs from the reference"

@@ -48,10 +48,35 @@ def make_world(fx, **kw):
     return w
 
 
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def moved_mismatch(world, fx, s, r):
+    """After step s (0: before the first), world.previous and world.moved() against the
+    reference's (the fixture's SHA-256 of each; the moved lists' order counts); None or why."""
+    rec = fx.meta.get("moved")
+    if rec is None:
+        return None
+    if s == 0:
+        return None if sha(world.moved(0)) == rec["added"] else "before the first step: moved(0) differs"
+    if sha(r["previous"]) != rec["previous"][s - 1]:
+        return f"step {s}: previous differs"
+    if sha(world.moved(s - 1)) != rec["last"][s - 1]:
+        return f"step {s}: moved({s - 1}) differs: {world.moved(s - 1)[:12]}"
+    if sha(world.moved(max(0, s - 7))) != rec["older"][s - 1]:
+        return f"step {s}: moved({max(0, s - 7)}) differs"
+    return None
+
+
 def run(world, fx, steps=None):
-    """Steps `world` along fixture `fx`; returns (steps matched, the first mismatch or None)."""
+    """Steps `world` along fixture `fx`; returns (steps matched, the first mismatch or None).
+    Each step's transforms, counts, previous and moved() lists are compared."""
     meta = fx.meta
     n = meta["steps"] if steps is None else min(steps, meta["steps"])
+    why = moved_mismatch(world, fx, 0, None)
+    if why:
+        return 0, why
     for s in range(1, n + 1):
         world.step(meta["dt"])
         r = world.read()
@@ -61,6 +86,9 @@ def run(world, fx, steps=None):
         if got != meta["hashes"][s - 1] or counts != want:
             return s - 1, f"step {s}: transforms {'match' if got == meta['hashes'][s - 1] else 'differ'}, " \
                           f"awake/contacts/frozen {counts}, reference {want}"
+        why = moved_mismatch(world, fx, s, r)
+        if why:
+            return s - 1, why
     return n, None
 
 
@@ -110,6 +138,9 @@ def run_together(batch, worlds, fxs, steps):
             if got != fx.meta["hashes"][s - 1] or counts != want:
                 return s - 1, f"world {i}, step {s}: transforms {'match' if got == fx.meta['hashes'][s - 1] else 'differ'}, " \
                               f"awake/contacts/frozen {counts}, reference {want}"
+            why = moved_mismatch(w, fx, s, r)
+            if why:
+                return s - 1, f"world {i}, {why}"
     return steps, None
 
 
