@@ -98,6 +98,7 @@ struct Scratch {
 struct bm_context {
   int device = 0;
   hipStream_t stream = nullptr;
+  void* noalloc_block = nullptr;  // this device's block for bm_level_noalloc (collide.hip)
   hipEvent_t begin = nullptr, end = nullptr;
   std::string error;
   int32_t n_worlds = 0, bodies = 0, awake = 0, manifolds = 0, pairs = 0;
@@ -377,8 +378,9 @@ int bm_create(bm_context** out, int32_t device) {
   bm_context* ctx = new bm_context();
   ctx->device = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess
-      || hipEventCreate(&ctx->begin) != hipSuccess || hipEventCreate(&ctx->end) != hipSuccess) {
-    delete ctx;
+      || hipEventCreate(&ctx->begin) != hipSuccess || hipEventCreate(&ctx->end) != hipSuccess
+      || bm::level_noalloc_block(device, &ctx->noalloc_block) != hipSuccess) {
+    bm_destroy(ctx);
     return BM_ERR_DEVICE;
   }
   *out = ctx;
@@ -1155,6 +1157,73 @@ int bm_level_host(bm_context* ctx, int32_t n, const int32_t* field, const double
     }
   }
   HIP_OR_FAIL(ctx, hipEventElapsedTime(&ctx->last_ms, ctx->begin, ctx->end), "bm_level_host: hipEventElapsedTime");
+  return BM_OK;
+}
+
+int bm_level_noalloc(bm_context* ctx, int32_t n, const int32_t* field, const double* position, const double* axes,
+                     const double* shapes, const double* boxes, const double* margin, const double* tolerance,
+                     int32_t cap, int32_t* manifolds, int32_t* count, double* normal, double* points,
+                     double* separations, uint32_t* ids) {
+  if (!ctx) return BM_ERR_ARGUMENT;
+  ctx->error.clear();
+  if (n < 0 || cap < 1) return fail(ctx, BM_ERR_ARGUMENT, "bm_level_noalloc: n must be 0 or more, cap 1 or more");
+  if (n > bm::LEVEL_NOALLOC_PAIRS)
+    return fail(ctx, BM_ERR_ARGUMENT, "bm_level_noalloc: at most " + std::to_string(bm::LEVEL_NOALLOC_PAIRS) +
+                                          " pairs a call, got " + std::to_string(n));
+  if (n == 0) return BM_OK;
+  if (!field || !position || !axes || !shapes || !boxes || !margin || !tolerance || !manifolds || !count || !normal ||
+      !points || !separations || !ids)
+    return fail(ctx, BM_ERR_ARGUMENT, "bm_level_noalloc: null argument");
+  if ((long long)n * cap > (1LL << 31) / 16)
+    return fail(ctx, BM_ERR_ARGUMENT, "bm_level_noalloc: n x cap manifold slots is too many");
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->device), "bm_level_noalloc: hipSetDevice");
+  bm::LevelDev d{};
+  d.n = n;
+  d.cap = cap;
+  d.field = field;
+  d.position = position;
+  d.axes = axes;
+  d.shapes = shapes;
+  d.boxes = boxes;
+  d.margin = margin;
+  d.tolerance = tolerance;
+  d.manifolds = manifolds;
+  d.count = count;
+  d.normal = normal;
+  d.points = points;
+  d.separations = separations;
+  d.ids = ids;
+  d.hulls = ctx->hulls;
+  d.fields = ctx->fields;
+  bm::level_on_block(d, ctx->noalloc_block);
+  (void)hipGetLastError();
+  // The plan checks every pair and gives each its share of the block; nothing reads a pair's
+  // field, hull or points until the host has seen that every pair passed.
+  bm::launch_level_plan(d, ctx->noalloc_block, ctx->stream);
+  HIP_OR_FAIL(ctx, hipGetLastError(), "bm_level_noalloc: launch");
+  long long status[3] = {0, 0, 0};
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(status, bm::level_plan_status(ctx->noalloc_block), sizeof status,
+                                  hipMemcpyDeviceToHost, ctx->stream), "bm_level_noalloc: copy");
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_level_noalloc: plan");
+  const int flags = (int)status[1];
+  const std::string pair = "pair " + std::to_string(status[2]);
+  if (flags & bm::LEVEL_PLAN_KIND) return fail(ctx, BM_ERR_ARGUMENT, "bm_level_noalloc: " + pair + ": a shape's kind must be 0, 1, 2 or 3");
+  if (flags & bm::LEVEL_PLAN_HULL)
+    return fail(ctx, BM_ERR_ARGUMENT, "bm_level_noalloc: " + pair + ": a hull's number is not in the hull table (bm_set_hulls)");
+  if (flags & bm::LEVEL_PLAN_FIELD)
+    return fail(ctx, BM_ERR_ARGUMENT, "bm_level_noalloc: " + pair + ": a field's number is not in the field table (bm_set_fields)");
+  if (flags & bm::LEVEL_PLAN_POINTS)
+    return fail(ctx, BM_ERR_ARGUMENT, "bm_level_noalloc: the boxes cover room for " + std::to_string(status[0]) +
+                                          " found points (7 a triangle), more than the " +
+                                          std::to_string(bm::LEVEL_NOALLOC_POINTS) + " a call holds");
+  HIP_OR_FAIL(ctx, hipEventRecord(ctx->begin, ctx->stream), "bm_level_noalloc: hipEventRecord");
+  bm::launch_level(d, ctx->stream);
+  HIP_OR_FAIL(ctx, hipGetLastError(), "bm_level_noalloc: launch");
+  HIP_OR_FAIL(ctx, hipEventRecord(ctx->end, ctx->stream), "bm_level_noalloc: hipEventRecord");
+  bm::launch_level_clear(d, ctx->stream);
+  HIP_OR_FAIL(ctx, hipGetLastError(), "bm_level_noalloc: launch");
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_level_noalloc: kernel");
+  HIP_OR_FAIL(ctx, hipEventElapsedTime(&ctx->last_ms, ctx->begin, ctx->end), "bm_level_noalloc: hipEventElapsedTime");
   return BM_OK;
 }
 

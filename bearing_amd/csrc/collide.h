@@ -75,4 +75,29 @@ struct LevelDev {
 
 void launch_level(const LevelDev& d, hipStream_t stream);
 
+// The no-allocation form of the level pass: its working memory is a fixed block of device
+// memory (a file-scope __device__ array of collide.hip), one copy on each device, holding for
+// up to LEVEL_NOALLOC_PAIRS pairs a call each pair's offset into the found points, a status
+// block, and up to LEVEL_NOALLOC_POINTS found points (9 doubles and 3 words each).
+constexpr int LEVEL_NOALLOC_PAIRS = 1 << 16;
+constexpr long long LEVEL_NOALLOC_POINTS = 1 << 18;
+// The block's address on the current device (taken with hipGetSymbolAddress the first time
+// a device asks, then kept for that device); hipSuccess or the error that call returned.
+hipError_t level_noalloc_block(int device, void** out);
+// Status the plan leaves in the block: [0] points the pairs need (7 a triangle under each box),
+// [1] flags (1 a shape kind other than 0-3, 2 a hull number the table lacks, 4 a field number
+// the table lacks, 8 a pair needing more points than the block holds), [2] the first pair so
+// flagged (n if none).
+constexpr int LEVEL_PLAN_KIND = 1, LEVEL_PLAN_HULL = 2, LEVEL_PLAN_FIELD = 4, LEVEL_PLAN_POINTS = 8;
+// Checks every pair's shape kind, hull and field numbers, and gives each pair its share of the
+// block's found points (one workgroup); then the pass itself may be launched on the block.
+void launch_level_plan(const LevelDev& d, void* block, hipStream_t stream);
+// Where in the block the plan leaves its status (3 words).
+const void* level_plan_status(const void* block);
+// The LevelDev of a planned pass: offsets and working storage in the block.
+void level_on_block(LevelDev& d, void* block);
+// Zeroes every manifold slot a pass left unwritten (slot j of pair k with j at least
+// min(manifolds[k], cap)), as bm_level_host leaves them.
+void launch_level_clear(const LevelDev& d, hipStream_t stream);
+
 }  // namespace bm

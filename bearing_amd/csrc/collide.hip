@@ -9,6 +9,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+#include <vector>
+
 #include "collide.h"
 #include "jsmath.h"
 
@@ -1988,6 +1991,139 @@ __global__ void __launch_bounds__(64) level_kernel(LevelDev d) {
 void launch_level(const LevelDev& d, hipStream_t stream) {
   if (d.n == 0) return;
   hipLaunchKernelGGL(level_kernel, dim3(d.n), dim3(64), 0, stream, d);
+}
+
+// ---- the level pass without allocation ----
+
+// The block: pair offsets (LEVEL_NOALLOC_PAIRS + 1 of them), the status (3 words), then the
+// found points' doubles and words.
+constexpr size_t NOALLOC_OFFSETS = 8 * ((size_t)LEVEL_NOALLOC_PAIRS + 1);
+constexpr size_t NOALLOC_STATUS = (NOALLOC_OFFSETS + 255) / 256 * 256;
+constexpr size_t NOALLOC_SD = NOALLOC_STATUS + 256;
+constexpr size_t NOALLOC_SU = NOALLOC_SD + 9 * 8 * (size_t)LEVEL_NOALLOC_POINTS;
+constexpr size_t NOALLOC_BYTES = NOALLOC_SU + 3 * 4 * (size_t)LEVEL_NOALLOC_POINTS;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+hipError_t level_noalloc_block(int device, void** out) {
+  static std::mutex lock;
+  static std::vector<void*> at;
+  std::lock_guard<std::mutex> held(lock);
+  if (device < 0) return hipErrorInvalidDevice;
+  if ((int)at.size() <= device) at.resize(device + 1, nullptr);
+  if (!at[device]) {
+    void* p = nullptr;
+    const hipError_t rc = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (rc != hipSuccess) return rc;
+    at[device] = p;
+  }
+  *out = at[device];
+  return hipSuccess;
+}
+
+const void* level_plan_status(const void* block) { return static_cast<const unsigned char*>(block) + NOALLOC_STATUS; }
+
+void level_on_block(LevelDev& d, void* block) {
+  unsigned char* b = static_cast<unsigned char*>(block);
+  d.scratch = reinterpret_cast<const long long*>(b);
+  d.sd = reinterpret_cast<double*>(b + NOALLOC_SD);
+  d.su = reinterpret_cast<uint32_t*>(b + NOALLOC_SU);
+}
+
+// One workgroup: each thread takes a run of pairs, checks them as bm_level_host does and
+// counts the points each may find (7 a triangle under its box, the cells the pass itself
+// walks); the runs' totals are scanned in LDS, then each thread writes its pairs' offsets.
+constexpr int PLAN_THREADS = 1024;
+__global__ void __launch_bounds__(PLAN_THREADS) level_plan_kernel(LevelDev d, long long* offsets, long long* status) {
+  __shared__ long long run[PLAN_THREADS];
+  __shared__ int flags[PLAN_THREADS], first[PLAN_THREADS];
+  const int t = threadIdx.x, n = d.n;
+  const int per = (n + PLAN_THREADS - 1) / PLAN_THREADS;
+  const int lo = t * per < n ? t * per : n, hi = lo + per < n ? lo + per : n;
+  long long sum = 0;
+  int bad = 0, at = n;
+  for (int k = lo; k < hi; k++) {
+    const double* r = d.shapes + 19 * (long)k;
+    const double kind = r[0];
+    int why = 0;
+    if (!(kind == 0 || kind == 1 || kind == 2 || kind == 3)) why |= LEVEL_PLAN_KIND;
+    if (kind == 3 && !(r[13] >= 0 && r[13] < d.hulls.n && r[13] == (double)(int)r[13])) why |= LEVEL_PLAN_HULL;
+    const int f = d.field[k];
+    if (f < 0 || f >= d.fields.n) why |= LEVEL_PLAN_FIELD;
+    long long need = 0;
+    if (!why) {
+      const int columns = d.fields.info[2 * f], rows = d.fields.info[2 * f + 1];
+      const double spacing = d.fields.dinfo[3 * f], x0 = d.fields.dinfo[3 * f + 1], z0 = d.fields.dinfo[3 * f + 2];
+      const double* box = d.boxes + 6 * (long)k;
+      // The pass's own walk (src/mesh.js:388-401): the cells under the box.
+      const double i0 = narrow::js_max(0, floor((box[0] - x0) / spacing)),
+                   i1 = narrow::js_min(columns - 2, floor((box[3] - x0) / spacing));
+      const double j0 = narrow::js_max(0, floor((box[2] - z0) / spacing)),
+                   j1 = narrow::js_min(rows - 2, floor((box[5] - z0) / spacing));
+      if (i0 <= i1 && j0 <= j1) {
+        const double points = 14 * (i1 - i0 + 1) * (j1 - j0 + 1);
+        if (points > (double)LEVEL_NOALLOC_POINTS) why |= LEVEL_PLAN_POINTS;
+        else need = (long long)points;
+      }
+    }
+    if (why && at == n) at = k;
+    bad |= why;
+    offsets[k + 1] = need;
+    sum += need;
+  }
+  run[t] = sum;
+  flags[t] = bad;
+  first[t] = at;
+  __syncthreads();
+  if (t == 0) {
+    long long total = 0;
+    int all = 0, earliest = n;
+    for (int i = 0; i < PLAN_THREADS; i++) {
+      const long long mine = run[i];
+      run[i] = total;
+      total += mine;
+      all |= flags[i];
+      if (first[i] < earliest) earliest = first[i];
+    }
+    if (total > LEVEL_NOALLOC_POINTS && !(all & LEVEL_PLAN_POINTS)) all |= LEVEL_PLAN_POINTS;
+    offsets[0] = 0;
+    status[0] = total;
+    status[1] = all;
+    status[2] = earliest;
+  }
+  __syncthreads();
+  long long before = run[t];
+  for (int k = lo; k < hi; k++) {
+    before += offsets[k + 1];
+    offsets[k + 1] = before;
+  }
+}
+
+void launch_level_plan(const LevelDev& d, void* block, hipStream_t stream) {
+  unsigned char* b = static_cast<unsigned char*>(block);
+  hipLaunchKernelGGL(level_plan_kernel, dim3(1), dim3(PLAN_THREADS), 0, stream, d,
+                     reinterpret_cast<long long*>(b), reinterpret_cast<long long*>(b + NOALLOC_STATUS));
+}
+
+__global__ void level_clear_kernel(LevelDev d) {
+  const long slot = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= (long)d.n * d.cap) return;
+  const long k = slot / d.cap;
+  const int j = (int)(slot - k * d.cap);
+  const int got = d.manifolds[k] < d.cap ? d.manifolds[k] : d.cap;
+  if (j < got) return;
+  d.count[slot] = 0;
+  for (int q = 0; q < 3; q++) d.normal[3 * slot + q] = 0;
+  for (int q = 0; q < 12; q++) d.points[12 * slot + q] = 0;
+  for (int q = 0; q < 4; q++) {
+    d.separations[4 * slot + q] = 0;
+    d.ids[4 * slot + q] = 0;
+  }
+}
+
+void launch_level_clear(const LevelDev& d, hipStream_t stream) {
+  const long slots = (long)d.n * d.cap;
+  if (slots == 0) return;
+  hipLaunchKernelGGL(level_clear_kernel, dim3((unsigned)((slots + 255) / 256)), dim3(256), 0, stream, d);
 }
 
 void launch_meet(const MeetDev& d, hipStream_t stream) {

@@ -59,6 +59,7 @@ class Columns(ctypes.Structure):
 EXPORTS = ["bm_create", "bm_destroy", "bm_last_error", "bm_upload", "bm_soft_step", "bm_download",
            "bm_soft_step_host", "bm_last_kernel_ms", "bm_set_launch", "bm_lds_limit", "bm_colour", "bm_collide_host", "bm_meet_host", "bm_set_colour_lds", "bm_set_hulls",
            "bm_set_fields", "bm_level_host", "bm_set_wide", "bm_set_device_only",
+           "bm_level_noalloc", "bm_world_moved",
            "bm_world_create", "bm_world_destroy", "bm_world_error", "bm_world_add", "bm_world_step", "bm_world_read",
            "bm_world_set_hulls", "bm_world_set_fields", "bm_world_times", "bm_upload_worlds", "bm_download_worlds",
            "bm_batch_create", "bm_batch_destroy", "bm_batch_error", "bm_batch_world", "bm_batch_step",
@@ -91,6 +92,7 @@ def load():
         lib.bm_set_fields.argtypes = [vp, ctypes.c_int32, _I32, _I32, _D, ctypes.POINTER(ctypes.c_int64), _D]
         lib.bm_level_host.argtypes = [vp, ctypes.c_int32, _I32, _D, _D, _D, _D, _D, _D, ctypes.c_int32, _I32, _I32, _D,
                                       _D, _D, _U32]
+        lib.bm_level_noalloc.argtypes = [vp, ctypes.c_int32] + [vp] * 7 + [ctypes.c_int32] + [vp] * 6
         lib.bm_last_kernel_ms.argtypes = [vp]
         lib.bm_last_kernel_ms.restype = ctypes.c_double
         lib.bm_set_launch.argtypes = [vp, ctypes.c_int32, ctypes.c_int32]
@@ -249,6 +251,17 @@ def level(ctx, field, position, axes, shapes, boxes, margin, tolerance, cap):
                                     p(count, _I32), p(normal, _D), p(points, _D), p(seps, _D), p(ids, _U32)),
               "bm_level_host")
     return manifolds, count, normal, points, seps, ids
+
+
+def level_noalloc(ctx, n, cap, field, position, axes, shapes, boxes, margin, tolerance, manifolds, count, normal,
+                  points, separations, ids):
+    """bm_level_noalloc: as level(), every argument after cap a device address (an int: the
+    caller's device memory, in level()'s layout); the outputs are written there."""
+    vp = ctypes.c_void_p
+    ctx.check(ctx.lib.bm_level_noalloc(ctx.ptr, n, *[vp(a) for a in (field, position, axes, shapes, boxes, margin)],
+                                       vp(tolerance), cap,
+                                       *[vp(a) for a in (manifolds, count, normal, points, separations, ids)]),
+              "bm_level_noalloc")
 
 
 def collide(ctx, a, b, margin, tolerance, with_loss=False):

@@ -223,6 +223,25 @@ int bm_level_host(bm_context* ctx, int32_t n, const int32_t* field, const double
                   int32_t* manifolds, int32_t* count, double* normal, double* points, double* separations,
                   uint32_t* ids);
 
+/*
+ * bm_level_host without allocating: for callers that cannot allocate device memory during a
+ * call (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference. The same touchLevel, bit for bit, with every array in
+ * device memory the caller owns, in bm_level_host's layout; the field and hull tables are
+ * the context's (bm_set_fields, bm_set_hulls: set up beforehand). The pass's working memory
+ * (each pair's share of the found points) is a fixed 22.5 MB block of device memory the
+ * library holds on each device until the process ends: at most 65,536 pairs a call, and at
+ * most 262,144 found points a call, 7 for each triangle under each pair's box. A call
+ * beyond either, or naming a shape kind, hull or field the tables lack, fails with
+ * BM_ERR_ARGUMENT before the pass runs. The call allocates nothing, waits for its work,
+ * and then the block holds the last call's contents; two calls that overlap on one device
+ * (two contexts on two threads) race on the block.
+ */
+int bm_level_noalloc(bm_context* ctx, int32_t n, const int32_t* field, const double* position, const double* axes,
+                     const double* shapes, const double* boxes, const double* margin, const double* tolerance,
+                     int32_t cap, int32_t* manifolds, int32_t* count, double* normal, double* points,
+                     double* separations, uint32_t* ids);
+
 /* How bm_soft_step launches, for tuning (bearing_amd/tune.py picks per workload):
  * shape 0 = 256 threads a workgroup with registers for one wave a SIMD, 1 = 256
  * threads and two waves a SIMD, 2 = 512 threads and two waves; resident = 1 keeps

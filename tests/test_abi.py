@@ -84,3 +84,17 @@ def test_world_create_without_a_gpu_fails_loudly():
         pass
     with pytest.raises(native.BearingError):
         native.World(0)
+
+
+def test_world_state_layout_matches_the_header(tmp_path):
+    fields = [f for f, _ in native.WorldState._fields_]
+    src = tmp_path / "state.c"
+    body = "".join(f'  printf("%s %zu\\n", "{f}", offsetof(bm_world_state, {f}));\n' for f in fields)
+    src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "bearing_mi355x.h"\nint main(void) {\n'
+                   f'  printf("size %zu\\n", sizeof(bm_world_state));\n{body}  return 0;\n}}\n')
+    exe = tmp_path / "state"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.dirname(HEADER), str(src), "-o", str(exe)], check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True).stdout.splitlines())
+    assert int(got["size"]) == ctypes.sizeof(native.WorldState)
+    for f in fields:
+        assert int(got[f]) == getattr(native.WorldState, f).offset, f
