@@ -275,25 +275,30 @@ HD void untwist_t(const T* q, const T* inv, const T* L, const T& most, T* out) {
 // Out: q 4, w 3 (= left), ceiling 1.
 template <class T>
 HD void tumble_t(const T* q0, const T* w, const T* left, const T& ceiling_in, const T* I, const T* inv, double dt,
-                 T* q_out, T* w_out, T& ceiling_out) {
+                 T* q_out, T* w_out, T& ceiling_out, unsigned* took = nullptr) {
   T b[3], c[3], L[3];
   rotate_t(q0, w, true, b);
   c[0] = b[0] * I[0]; c[1] = b[1] * I[1]; c[2] = b[2] * I[2];
   rotate_t(q0, c, false, L);
   T most = ceiling_in;
-  if (val(w[0]) != val(left[0]) || val(w[1]) != val(left[1]) || val(w[2]) != val(left[2]))
-    most = w[0] * L[0] + w[1] * L[1] + w[2] * L[2];
+  const bool fresh = val(w[0]) != val(left[0]) || val(w[1]) != val(left[1]) || val(w[2]) != val(left[2]);
+  if (fresh) most = w[0] * L[0] + w[1] * L[1] + w[2] * L[2];
   T q4[4], v[3], q[4];
   advance_t(q0, w, dt * 0.25, q4);
   angular_t(q4, inv, L, v);
   advance_t(q0, v, dt * 0.5, q);
   angular_t(q, inv, L, v);
-  if (val(v[0] * L[0] + v[1] * L[1] + v[2] * L[2]) > val(most) * (1 + ROUNDED)) {
+  const bool over = val(v[0] * L[0] + v[1] * L[1] + v[2] * L[2]) > val(most) * (1 + ROUNDED);
+  if (over) {
     T u[4];
     untwist_t(q, inv, L, most, u);
     for (int k = 0; k < 4; k++) q[k] = u[k];
     angular_t(q, inv, L, v);
   }
+  // The energy check above compares within 2^-44 of the ceiling: a perturbation at rounding
+  // level can flip it, and its outcomes differ by a turn of that size, so only whether the
+  // ceiling was taken afresh is a branch of the record.
+  if (took) *took = fresh ? 1u : 0u;
   for (int k = 0; k < 4; k++) q_out[k] = q[k];
   for (int k = 0; k < 3; k++) w_out[k] = v[k];
   ceiling_out = most;
@@ -380,6 +385,7 @@ struct Bodies {
   double *inv_mass, *inertia, *inv_inertia;                   // from the mass
   double *p, *q, *v, *w, *left, *ceiling;                     // the state
   double *axes, *inv_i, *start, *start_q;                     // the step's
+  uint64_t* branch;  // the forward pass's branches, a record a body (or nullptr: none kept)
 };
 
 // Rows: every solved manifold of every step, held (global row numbers).
@@ -391,7 +397,17 @@ struct Rows {
   const double* shift;  // per point: where b's end was carried from at the moment of impact (0 if not met so)
   // what each row stored at its step's end (src/contact.js:1091-1115): the carried impulses
   double *store_impulse, *store_friction, *store_twist, *store_slip;
+  uint64_t* branch;  // the forward pass's branches, a record a row (or nullptr: none kept)
 };
+
+// The branches a forward pass took (its active set: anchor weights, target cases, Murty sets,
+// friction bias and cone, roll and spin clamps, bounces, spin caps, the tumble's branches),
+// folded into one record a row and one a body in the order they are taken. Each row and each
+// body is worked on by one thread at a time, so the records need no atomics; two passes took
+// the same branches when every record is equal.
+HD void note(uint64_t* record, uint64_t code) {
+  if (record) *record = (*record ^ (code + 0x9e3779b97f4a7c15ull)) * 1099511628211ull;
+}
 
 // The current step's rows (by working index: the row less the step's first row).
 // The forward pass's working columns (per point: 4 slots a row).
@@ -620,6 +636,7 @@ HD void prepare_row(const Bodies& B, const Sizes& Z, const Work& W, const Rows& 
     W.nmass()[j] = lever(W.jac() + J * j, Ia, Ib, ima, imb, rax, ray, raz, rbx, rby, rbz, nx, ny, nz);
     W.most()[j] = 0;
     const double s = W.separation()[j] / tol, near = s <= 1 ? 1 : s >= 2 ? 0 : 2 - s;
+    note(R.branch ? R.branch + r : nullptr, 0x100u | (s <= 1 ? 0u : s >= 2 ? 1u : 2u));
     weight += near;
     ax += near * rax; ay += near * ray; az += near * raz; bx += near * rbx; by += near * rby; bz += near * rbz;
     ex += rax; ey += ray; ez += raz; fx += rbx; fy += rby; fz += rbz;
@@ -642,6 +659,7 @@ HD void prepare_row(const Bodies& B, const Sizes& Z, const Work& W, const Rows& 
       }
     }
   }
+  note(R.branch ? R.branch + r : nullptr, 0x200u | (weight == 0 ? 1u : 0u));
   if (weight == 0) { weight = count; ax = ex; ay = ey; az = ez; bx = fx; by = fy; bz = fz; }
   ax /= weight; ay /= weight; az /= weight; bx /= weight; by /= weight; bz /= weight;
   W.anchor_a()[i3] = ax; W.anchor_a()[i3 + 1] = ay; W.anchor_a()[i3 + 2] = az;
@@ -662,6 +680,7 @@ HD void prepare_row(const Bodies& B, const Sizes& Z, const Work& W, const Rows& 
   const double k12 = f[12] * f[6] + f[13] * f[7] + f[14] * f[8] + f[15] * f[9] + f[16] * f[10] + f[17] * f[11];
   const double det = k11 * k22 - k12 * k12;
   double* tm = W.tm() + i3;
+  note(R.branch ? R.branch + r : nullptr, 0x300u | (det > 0 ? 1u : 0u));
   if (det > 0) {
     tm[0] = k22 / det; tm[1] = -k12 / det; tm[2] = k11 / det;
   } else {
@@ -1048,6 +1067,14 @@ HD void row_forward(const Bodies& B, const Work& W, const Rows& R, int boff, int
   Pair s;
   solve_math<N>(W, R, r, i, ima, imb, B.inv_i + 6 * ga, B.inv_i + 6 * gb, soft, h, bias, in, f, s);
   scatter(B, ga, gb, ima, imb, s);
+  if (R.branch) {
+    uint64_t code = (f.single ? 1u : 0u) | (f.slide ? 2u : 0u) | (f.roll_capped ? 4u : 0u)
+        | ((uint64_t)(f.sclamp == 1 ? 1 : f.sclamp == -1 ? 2 : 0) << 3) | ((uint64_t)f.fbias << 5)
+        | ((uint64_t)f.set << 8) | ((uint64_t)f.first << 12)
+        | ((uint64_t)(bias ? 1 : 0) << 40) | ((uint64_t)(known ? 1 : 0) << 41) | ((uint64_t)N << 42);
+    for (int k = 0; k < N; k++) code |= (uint64_t)f.tcase[k] << (20 + 2 * k);
+    note(R.branch + r, code);
+  }
   if (tape.p) {
     const TapeRow& T = tape;
     for (int k = 0; k < 12; k++) { T[T_S + k] = in.s.v[k]; T[T_MID + k] = f.mid.v[k]; }
@@ -1111,6 +1138,7 @@ HD void accelerate_body(const Bodies& B, int g, double h, int substeps, double g
   }
   if (B.capped[g]) {
     const double w2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+    note(B.branch ? B.branch + g : nullptr, 0x10u | (w2 > most * most ? 1u : 0u));
     if (w2 > most * most) {
       const double k = most / dsqrt(w2);
       w[0] *= k; w[1] *= k; w[2] *= k;
@@ -1123,10 +1151,13 @@ HD void move_body(const Bodies& B, int g, double h) {
   double *p = B.p + 3 * g, *v = B.v + 3 * g, *w = B.w + 3 * g;
   p[0] += v[0] * h; p[1] += v[1] * h; p[2] += v[2] * h;
   const bool turning = w[0] != 0 || w[1] != 0 || w[2] != 0;
+  note(B.branch ? B.branch + g : nullptr, 0x20u | (turning ? 1u : 0u));
   if (turning && B.tumbles[g]) {
     double q[4], wo[3], c;
+    unsigned took = 0;
     tumble_t<double>(B.q + 4 * g, w, B.left + 3 * g, B.ceiling[g], B.inertia + 3 * g, B.inv_inertia + 3 * g, h, q, wo,
-                     c);
+                     c, &took);
+    note(B.branch ? B.branch + g : nullptr, 0x40u | took);
     for (int k = 0; k < 4; k++) B.q[4 * g + k] = q[k];
     for (int k = 0; k < 3; k++) { w[k] = wo[k]; B.left[3 * g + k] = wo[k]; }
     B.ceiling[g] = c;
@@ -1148,7 +1179,9 @@ HD void bounce_row(const Bodies& B, const Work& W, const Rows& R, int boff, int 
   const double ima = B.inv_mass[ga], imb = B.inv_mass[gb];
   for (int k = 0; k < R.count[r]; k++) {
     const int j = P * i + k;
-    if (W.relvel()[j] > -threshold || W.most()[j] == 0) continue;
+    const bool skip = W.relvel()[j] > -threshold || W.most()[j] == 0;
+    note(R.branch ? R.branch + r : nullptr, 0x400u | (skip ? 1u : 0u));
+    if (skip) continue;
     const double* jac = W.jac() + J * j;
     Pair s = gather(B, ga, gb);
     const double vn = closing(s, nx, ny, nz, jac);
@@ -1159,6 +1192,7 @@ HD void bounce_row(const Bodies& B, const Work& W, const Rows& R, int boff, int 
       for (int c = 0; c < 12; c++) o[2 + c] = s.v[c];
     }
     double next = old - W.nmass()[j] * (vn + e * W.relvel()[j]);
+    note(R.branch ? R.branch + r : nullptr, 0x500u | (next < 0 ? 1u : 0u));
     if (next < 0) next = 0;
     const double l = next - old;
     W.impulse()[j] = next;
@@ -1188,7 +1222,9 @@ HD void store_row(const Bodies& B, const Work& W, const Rows& R, int boff, int r
   for (int k = 0; k < count; k++) pressed += W.impulse()[P * i + k];
   const double t1x = W.t1()[i3], t1y = W.t1()[i3 + 1], t1z = W.t1()[i3 + 2];
   const double t2x = W.t2()[i3], t2y = W.t2()[i3 + 1], t2z = W.t2()[i3 + 2];
-  if (W.sliding[i] != 0 || pressed == 0 || impacted(R, r)) {
+  const bool still = W.sliding[i] != 0 || pressed == 0 || impacted(R, r);
+  note(R.branch ? R.branch + r : nullptr, 0x600u | (still ? 1u : 0u));
+  if (still) {
     W.drift()[2 * i] = 0; W.drift()[2 * i + 1] = 0;
   } else {
     double T[6];
@@ -2657,6 +2693,7 @@ struct Trajectory {
   RowsBar RB;
   double* checkpoints;  // (steps / every + 1) * bodies * STATE
   double *row_tape, *body_tape;
+  uint64_t *body_branch, *row_branch;  // the last forward pass's branch records (note())
 };
 
 HD StepView view_of(const Trajectory& T, int t, int wld) {
@@ -2706,13 +2743,33 @@ HD void from_mass(const Bodies& B, int g) {
 template <class Sync>
 HD void forward_world(const Ctx& c, Sync sync, const Trajectory& T, int wld) {
   const int boff = T.body_offset[wld], nb = T.body_offset[wld + 1] - boff;
+  // This pass keeps its branch records (the backward pass's steps again do not).
+  Bodies B = T.B;
+  Rows R = T.R;
+  B.branch = T.body_branch;
+  R.branch = T.row_branch;
+  for (int s = c.tid; s < nb; s += c.threads) B.branch[boff + s] = 0;
+  // A row's record starts from what the row is (its bodies, points, feature ids, and which
+  // row of its step its pair carried from), so records of two recordings differ where
+  // their contacts do.
+  for (int t = 0; t < T.steps; t++) {
+    const int r0 = T.row_offset[t * T.worlds + wld], r1 = T.row_offset[t * T.worlds + wld + 1];
+    for (int r = r0 + c.tid; r < r1; r += c.threads) {
+      uint64_t x = 0;
+      note(&x, ((uint64_t)R.count[r] << 56) | ((uint64_t)(uint32_t)R.a[r] << 28) | (uint64_t)(uint32_t)R.b[r]);
+      for (int k = 0; k < R.count[r]; k++) note(&x, R.pid[P * r + k]);
+      note(&x, R.src[r] < 0 ? ~0ull : (uint64_t)(r - R.src[r]));
+      R.branch[r] = x;
+    }
+  }
+  sync();
   for (int t = 0; t < T.steps; t++) {
     if (t % T.every == 0) {
       double* ck = T.checkpoints + (size_t)(t / T.every) * T.bodies * STATE;
-      for (int s = c.tid; s < nb; s += c.threads) save_state(T.B, boff + s, ck + (size_t)(boff + s) * STATE);
+      for (int s = c.tid; s < nb; s += c.threads) save_state(B, boff + s, ck + (size_t)(boff + s) * STATE);
     }
     const StepView v = view_of(T, t, wld);
-    step_forward(c, sync, T.B, T.Z, T.R, T.W, v, no_tape(), false);
+    step_forward(c, sync, B, T.Z, R, T.W, v, no_tape(), false);
   }
 }
 

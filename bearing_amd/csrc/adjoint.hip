@@ -25,6 +25,7 @@ struct HeldDevice {
   float forward_ms = 0, backward_ms = 0;
   bool forwarded = false;
   size_t rows = 0;
+  std::vector<int32_t> body_offset, row_offset;  // the host's copies, for branch records
 };
 
 namespace {
@@ -96,6 +97,8 @@ int held_load(int device, const bm_held* held, HeldDevice** out, std::string& er
     return hip_fail(e, "hipEventCreate", err);
   }
   h->rows = held->rows;
+  h->body_offset.assign(held->body_offset, held->body_offset + held->worlds + 1);
+  h->row_offset.assign(held->row_offset, held->row_offset + (size_t)held->steps * held->worlds + 1);
   *out = h;
   return BM_OK;
 }
@@ -149,6 +152,20 @@ int held_backward(HeldDevice* h, hipStream_t stream, const double* state_bar, do
       return hip_fail(e, "hipMemcpyAsync", err);
   if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "held_backward_kernel", err);
   (void)hipEventElapsedTime(&h->backward_ms, h->begin, h->end);
+  return BM_OK;
+}
+
+int held_branches(HeldDevice* h, hipStream_t stream, uint64_t* per_world, std::string& err) {
+  if (!h || !h->forwarded) { err = "bm_held_branches: no forward pass"; return BM_ERR_ARGUMENT; }
+  (void)hipSetDevice(h->device);
+  std::vector<uint64_t> body(h->T.bodies), row(h->rows ? h->rows : 1);
+  hipError_t e;
+  if ((e = hipMemcpyAsync(body.data(), h->T.body_branch, body.size() * 8, hipMemcpyDeviceToHost, stream)) != hipSuccess ||
+      (e = hipMemcpyAsync(row.data(), h->T.row_branch, row.size() * 8, hipMemcpyDeviceToHost, stream)) != hipSuccess ||
+      (e = hipStreamSynchronize(stream)) != hipSuccess)
+    return hip_fail(e, "bm_held_branches: copy", err);
+  adj::fold_branches(h->T.worlds, h->T.steps, h->body_offset.data(), h->row_offset.data(), body.data(), row.data(),
+                     per_world);
   return BM_OK;
 }
 

@@ -15,4 +15,5 @@ int held_forward(HeldDevice* h, hipStream_t stream, const double* state0, const 
 int held_backward(HeldDevice* h, hipStream_t stream, const double* state_bar, double* state0_bar, double* mass_bar,
                   double* friction_bar, double* restitution_bar, std::string& err);
 void held_times(const HeldDevice* h, double* forward_ms, double* backward_ms);
+int held_branches(HeldDevice* h, hipStream_t stream, uint64_t* per_world, std::string& err);
 }  // namespace bm

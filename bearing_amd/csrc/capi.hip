@@ -428,6 +428,12 @@ int bm_held_backward(bm_context* ctx, const double* state_bar, double* state0_ba
                            ctx->error);
 }
 
+int bm_held_branches(bm_context* ctx, uint64_t* per_world) {
+  if (!ctx || !per_world) return BM_ERR_ARGUMENT;
+  ctx->error.clear();
+  return bm::held_branches(ctx->held, ctx->stream, per_world, ctx->error);
+}
+
 int bm_held_times(const bm_context* ctx, double* forward_ms, double* backward_ms) {
   if (!ctx) return BM_ERR_ARGUMENT;
   bm::held_times(ctx->held, forward_ms, backward_ms);

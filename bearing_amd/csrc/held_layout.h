@@ -130,10 +130,35 @@ inline size_t lay_out(const bm_held& h, char* base, Trajectory& T, HeldIO& io, s
   WorkBar& WB = T.WB;
   WB.n = work;
   WB.base = d((size_t)WorkBar::WIDTH * work);
+  B.branch = nullptr;
+  R.branch = nullptr;
+  T.body_branch = reinterpret_cast<uint64_t*>(take(nb * 8));
+  T.row_branch = reinterpret_cast<uint64_t*>(take((rows ? rows : 1) * 8));
   T.checkpoints = d((size_t)(h.steps / h.every + 1) * nb * STATE);
   T.row_tape = d((size_t)2 * most_sub * work * ROW_TAPE);
   T.body_tape = d((size_t)h.every * 2 * most_sub * nb * STATE);
   return (at + 255) & ~(size_t)255;
+}
+
+// Each world's branch record from its bodies' and rows' (held_forward's note()): every record
+// mixed with its body's or row's place in the world, then summed, so the fold does not
+// depend on the order the records are read in.
+inline void fold_branches(int worlds, int steps, const int32_t* body_offset, const int32_t* row_offset,
+                          const uint64_t* body, const uint64_t* row, uint64_t* out) {
+  auto mix = [](uint64_t where, uint64_t v) {
+    uint64_t x = v ^ (where * 0x9e3779b97f4a7c15ull);
+    x = (x ^ (x >> 31)) * 0xbf58476d1ce4e5b9ull;
+    return x ^ (x >> 29);
+  };
+  for (int w = 0; w < worlds; w++) {
+    uint64_t acc = 0;
+    for (int g = body_offset[w]; g < body_offset[w + 1]; g++) acc += mix((uint64_t)(g - body_offset[w]), body[g]);
+    for (int t = 0; t < steps; t++) {
+      const int r0 = row_offset[t * worlds + w], r1 = row_offset[t * worlds + w + 1];
+      for (int r = r0; r < r1; r++) acc += mix(((uint64_t)(t + 1) << 32) | (uint64_t)(r - r0), row[r]);
+    }
+    out[w] = acc;
+  }
 }
 
 // A world's bodies at the start of the forward pass: p q v w from state0, the tumble's

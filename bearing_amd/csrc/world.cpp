@@ -3417,4 +3417,11 @@ int bm_adjoint_times(const bm_adjoint* a, double* forward_ms, double* backward_m
   return bm_held_times(a->ctx, forward_ms, backward_ms);
 }
 
+int bm_adjoint_branches(bm_adjoint* a, uint64_t* per_world) {
+  if (!a || !a->ctx || !per_world) return BM_ERR_ARGUMENT;
+  a->error.clear();
+  if (int rc = bm_held_branches(a->ctx, per_world)) return adjoint_fail(a, rc, std::string("branches: ") + bm_last_error(a->ctx));
+  return BM_OK;
+}
+
 }  // extern "C"

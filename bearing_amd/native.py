@@ -65,7 +65,8 @@ EXPORTS = ["bm_create", "bm_destroy", "bm_last_error", "bm_upload", "bm_soft_ste
            "bm_batch_create", "bm_batch_destroy", "bm_batch_error", "bm_batch_world", "bm_batch_step",
            "bm_batch_times", "bm_world_record", "bm_world_recorded", "bm_adjoint_create", "bm_adjoint_destroy",
            "bm_adjoint_error", "bm_adjoint_size", "bm_adjoint_recorded", "bm_adjoint_forward", "bm_adjoint_backward",
-           "bm_adjoint_times", "bm_held_load", "bm_held_forward", "bm_held_backward", "bm_held_times"]
+           "bm_adjoint_times", "bm_held_load", "bm_held_forward", "bm_held_backward", "bm_held_times",
+           "bm_adjoint_branches", "bm_held_branches"]
 
 _lib = None
 
@@ -157,6 +158,7 @@ def world_signatures(lib):
     lib.bm_adjoint_forward.argtypes = [vp, _D, _D, _D, _D, _D]
     lib.bm_adjoint_backward.argtypes = [vp, _D, _D, _D, _D, _D]
     lib.bm_adjoint_times.argtypes = [vp, _D, _D]
+    lib.bm_adjoint_branches.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
 
 
 class BearingError(RuntimeError):
@@ -570,6 +572,7 @@ class Adjoint:
             world_signatures(lib)
         self.ptr = ctypes.c_void_p()
         arr = (ctypes.c_void_p * len(worlds))(*[w.ptr for w in worlds])
+        self.worlds = len(worlds)
         rc = self.lib.bm_adjoint_create(ctypes.byref(self.ptr), device, arr, len(worlds), every)
         if rc != 0:
             msg = self.lib.bm_adjoint_error(self.ptr).decode() if self.ptr else ""
@@ -619,6 +622,14 @@ class Adjoint:
                                                 *[out[k].ctypes.data_as(_D) for k in
                                                   ("state", "mass", "friction", "restitution")]),
                    "bm_adjoint_backward")
+        return out
+
+    def branches(self):
+        """bm_adjoint_branches: each world's record of the branches the last forward pass took."""
+        n = int(self.worlds)
+        out = np.zeros(n, np.uint64)
+        self.check(self.lib.bm_adjoint_branches(self.ptr, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))),
+                   "bm_adjoint_branches")
         return out
 
     def times(self):

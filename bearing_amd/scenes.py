@@ -389,6 +389,15 @@ def throws(worlds: int = 512, seed: int = 5):
     return [throw(rng) for _ in range(worlds)]
 
 
+def throw_targets(worlds: int = 512, seed: int = 5):
+    """The differentiation form's loss targets: for each thrown-box world, the point the
+    box's final centre is pulled towards (the loss is its squared distance to it), drawn
+    from its own seeded stream: x in [0, 4] (from the throw towards the wall), y in
+    [0.2, 1.0], z in [-2, 2]."""
+    rng = SplitMix64(seed * 1000 + 1)
+    return [[rng.between(0.0, 4.0), rng.between(0.2, 1.0), rng.between(-2.0, 2.0)] for _ in range(worlds)]
+
+
 SCENES = {
     "pyramid": lambda: pyramid(),
     "rain": lambda: rain(),

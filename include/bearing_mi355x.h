@@ -455,6 +455,13 @@ int bm_adjoint_backward(bm_adjoint* a, const double* state_bar, double* state0_b
                         double* friction_bar, double* restitution_bar);
 /* Milliseconds the last forward and backward pass took on the device. */
 int bm_adjoint_times(const bm_adjoint* a, double* forward_ms, double* backward_ms);
+/* After bm_adjoint_forward: for each world, a 64-bit record of the branches its forward pass
+ * took — its active set: each point's anchor weight case, each solve's target cases, Murty
+ * set, friction bias and cone, roll and spin clamps, each bounce taken or not and clamped,
+ * each spin cap and the tumble's branches, in the order taken. Two forward passes of a world
+ * took the same branches when its records are equal; the gradient is that of the pass's
+ * branches, so finite differences agree with it only between inputs whose records agree. */
+int bm_adjoint_branches(bm_adjoint* a, uint64_t* per_world);
 
 /* The held trajectory as the device takes it (bm_held_load); host pointers. Rows are every
  * step's solved manifolds, step after step and within a step world after world, after
@@ -491,6 +498,8 @@ int bm_held_forward(bm_context* ctx, const double* state0, const double* mass, c
 int bm_held_backward(bm_context* ctx, const double* state_bar, double* state0_bar, double* mass_bar,
                      double* friction_bar, double* restitution_bar);
 int bm_held_times(const bm_context* ctx, double* forward_ms, double* backward_ms);
+/* bm_adjoint_branches for the held trajectory bm_held_load loaded. */
+int bm_held_branches(bm_context* ctx, uint64_t* per_world);
 
 #ifdef __cplusplus
 }

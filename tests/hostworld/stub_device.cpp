@@ -125,6 +125,13 @@ int bm_held_backward(bm_context* ctx, const double* state_bar, double* state0_ba
   return BM_OK;
 }
 
+int bm_held_branches(bm_context* ctx, uint64_t* per_world) {
+  if (!ctx->held) return BM_ERR_ARGUMENT;
+  const bm::adj::Trajectory& T = ctx->held->T;
+  bm::adj::fold_branches(T.worlds, T.steps, T.body_offset, T.row_offset, T.body_branch, T.row_branch, per_world);
+  return BM_OK;
+}
+
 int bm_held_times(const bm_context*, double* forward_ms, double* backward_ms) {
   if (forward_ms) *forward_ms = 0;
   if (backward_ms) *backward_ms = 0;

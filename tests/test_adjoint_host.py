@@ -193,3 +193,73 @@ def test_host_torch_function_steps_again_after_another_forward(lib):
     want = fresh.backward(W.numpy())
     assert np.array_equal(s1.grad.numpy(), want["state"])
     adj.close(); fresh.close()
+
+
+def test_host_adjoint_throws_matches_finite_differences(lib):
+    # The differentiation form's own scene (scenes.throws, seed 5: a thrown box landing and
+    # sliding towards a 6x6 wall; dt 1/120, 120 steps, checkpoints every 10), its loss the
+    # squared distance of the box's final centre to its seeded target: the gradients with
+    # respect to the box's initial velocity and spin, its mass, its and the ground's friction
+    # and the ground's restitution against central differences at h = 1e-6 (extrapolated with
+    # h / 2), wherever the passes at +-h and +-h/2 took the recorded pass's branches.
+    from adjointcheck import record_throws, throws_agree, throws_fd_check
+    batch, ws = record_throws(4, lib=lib)
+    adj = native.Adjoint(ws, every=10, lib=lib)
+    rows = throws_fd_check(adj, scenes.throw_targets(512, seed=5)[:4])
+    adj.close()
+    batch.close()
+    same = [r for r in rows if r[5]]
+    assert len(same) >= 12, f"only {len(same)} of {len(rows)} inputs kept their branches at +-h"
+    bad = throws_agree(rows)
+    assert not bad, bad
+
+
+def test_host_adjoint_branch_records_name_the_active_set(lib):
+    # Two forward passes from the same inputs keep the same branch record; a box thrown slow
+    # enough to stick rather than slide takes other branches; and a different recording's
+    # contacts give a different record even at the same inputs.
+    bodies, steps = SCENES["slide"]
+    w = record(bodies, steps, lib=lib)
+    adj = native.Adjoint([w], every=10, lib=lib)
+    rec = adj.recorded()
+    adj.forward(rec["state"], rec["mass"], rec["friction"], rec["restitution"])
+    first = adj.branches()
+    adj.forward(rec["state"], rec["mass"], rec["friction"], rec["restitution"])
+    assert np.array_equal(adj.branches(), first)
+    slow = np.array(rec["state"])
+    slow[1, 7] = 0.01
+    adj.forward(slow, rec["mass"], rec["friction"], rec["restitution"])
+    assert adj.branches()[0] != first[0]
+    other = dict(bodies[1], velocity=[0.01, 0.0, 0.0])
+    w2 = record([bodies[0], other], steps, lib=lib)
+    adj2 = native.Adjoint([w2], every=10, lib=lib)
+    adj2.forward(rec["state"], rec["mass"], rec["friction"], rec["restitution"])
+    assert adj2.branches()[0] != first[0]
+    adj.close(); adj2.close()
+
+
+def test_host_adjoint_is_the_engines_gradient_for_a_rolling_ball(lib):
+    # The held trajectory's gradient against finite differences of the real engine (each input
+    # perturbed in the scene, the world stepped afresh through bm_world): a ball rolling and
+    # spinning on a floor, whose contact the engine keeps finding at the same place relative to
+    # the ball, agrees on every input; every perturbed recording took the recorded branches.
+    from adjointcheck import world_fd_check
+    bodies, steps = SCENES["roll"]
+    rows = world_fd_check(bodies, steps, lib=lib)
+    assert all(r[3] for r in rows), [r[0] for r in rows if not r[3]]
+    bad = [r for r in rows if abs(r[1] - r[2]) > max(1e-6 * abs(r[2]), 3e-7)]
+    assert not bad, bad
+
+
+def test_host_adjoint_and_the_engine_part_where_contacts_are_found_afresh(lib):
+    # Where the engine finds a sliding face contact afresh each step, or meets a body at the
+    # moment of impact, its contact ends and carry back follow the state, while the held
+    # trajectory keeps each step's ends fixed in the bodies' frames and the carry back fixed:
+    # translating the sliding cube's start shows it (the engine's trajectory translates with
+    # it exactly, so d loss / d x0 is the loss's weight on the final x; the held one's is not).
+    from adjointcheck import world_fd_check
+    bodies, steps = SCENES["slide"]
+    rows = {r[0]: r for r in world_fd_check(bodies, steps, lib=lib)}
+    W = np.random.default_rng(1).normal(size=(2, native.HELD_STATE))
+    assert abs(rows["p0x"][2] - W[1, 0]) < 1e-6 * abs(W[1, 0])      # the engine: translation
+    assert abs(rows["p0x"][1] - W[1, 0]) > 1e-4 * abs(W[1, 0])       # the held model: not

@@ -45,6 +45,7 @@ def test_gpu_adjoint_is_the_hosts_bit_for_bit():
     og = g.forward(rg["state"], rg["mass"], rg["friction"], rg["restitution"])
     oh = h.forward(rh["state"], rh["mass"], rh["friction"], rh["restitution"])
     assert np.array_equal(og, oh)
+    assert np.array_equal(g.branches(), h.branches())
     W = np.random.default_rng(2).normal(size=og.shape)
     gg, gh = g.backward(W), h.backward(W)
     for key in gg:
@@ -92,3 +93,38 @@ def test_gpu_adjoint_throws_batch():
             assert np.array_equal(gg[key], gh[key]), (run, key)
     assert np.abs(og[:, :3] - real).max() < 1e-8
     g.close(); h.close(); bg.close(); bh.close()
+
+
+@pytest.mark.timeout(600)
+def test_gpu_adjoint_throws_matches_finite_differences():
+    # The differentiation form as configured, on the first 8 of its 512 worlds (scenes.throws,
+    # seed 5; dt 1/120, 120 steps, checkpoints every 10; the loss the squared distance of the
+    # box's final centre to its seeded target): the device's gradients with respect to the
+    # box's initial velocity and spin, its mass, its and the ground's friction and the ground's
+    # restitution against central differences at h = 1e-6 (extrapolated with h / 2) within
+    # 1e-6 relative, on every input whose passes at +-h and +-h/2 took the recorded pass's
+    # branches (bm_adjoint_branches: the active set is the same there).
+    from adjointcheck import record_throws, throws_agree, throws_fd_check
+    print("adjoint fd, 8 thrown-box worlds", flush=True)
+    batch, ws = record_throws(8, device=0)
+    adj = native.Adjoint(ws, every=10, device=0)
+    rows = throws_fd_check(adj, scenes.throw_targets(512, seed=5)[:8])
+    adj.close()
+    batch.close()
+    same = [r for r in rows if r[5]]
+    print(f"{len(same)} of {len(rows)} inputs kept their branches", flush=True)
+    assert len(same) >= 24, f"only {len(same)} of {len(rows)} inputs kept their branches at +-h"
+    bad = throws_agree(rows)
+    assert not bad, bad
+
+
+def test_gpu_adjoint_is_the_engines_gradient_for_a_rolling_ball():
+    # Against finite differences of the real engine on the device (each input perturbed in the
+    # scene, the world stepped afresh through bm_world).
+    from adjointcheck import world_fd_check
+    print("adjoint vs engine fd, roll", flush=True)
+    bodies, steps = SCENES["roll"]
+    rows = world_fd_check(bodies, steps, device=0)
+    assert all(r[3] for r in rows), [r[0] for r in rows if not r[3]]
+    bad = [r for r in rows if abs(r[1] - r[2]) > max(1e-6 * abs(r[2]), 3e-7)]
+    assert not bad, bad
