@@ -176,6 +176,10 @@ HD Dual<W> zero_of(const Dual<W>&) { return dual<W>(0); }
 
 // The vector-Jacobian product of f: NI inputs -> NO outputs, at `in`, by forward-mode
 // duals W directions at a time: in_bar += J^T out_bar.
+// Directions a pass of the small maps' reverses carries (placement, slip turn, tumble, world
+// inverse inertia): wider passes evaluate each map fewer times, narrower ones hold fewer
+// registers in the backward kernel.
+constexpr int DW = 2;
 template <int W, int NI, int NO, class F>
 HD void vjp(const double (&in)[NI], const double (&out_bar)[NO], double (&in_bar)[NI], F f) {
   for (int base = 0; base < NI; base += W) {
@@ -2051,8 +2055,8 @@ HD void move_vjp(const Bodies& B, const BodiesBar& BB, int g, double h) {
     for (int k = 0; k < 3; k++) outb[4 + k] = BB.w[3 * g + k] + BB.left[3 * g + k];
     outb[7] = BB.ceiling[g];
     for (int k = 0; k < 17; k++) inb[k] = 0;
-    vjp<6>(in, outb, inb, [&](const Dual<6>* x, Dual<6>* y) {
-      Dual<6> c;
+    vjp<DW>(in, outb, inb, [&](const Dual<DW>* x, Dual<DW>* y) {
+      Dual<DW> c;
       tumble_t(x, x + 4, x + 7, x[10], x + 11, x + 14, h, y, y + 4, c);
       y[7] = c;
     });
@@ -2141,7 +2145,7 @@ HD void turned_vjp(const Bodies& B, const BodiesBar& BB, int g, const double* Tb
   for (int k = 0; k < 4; k++) { in[6 + k] = B.q[4 * g + k]; in[10 + k] = B.start_q[4 * g + k]; }
   for (int k = 0; k < 6; k++) outb[k] = Tb[k];
   for (int k = 0; k < 14; k++) inb[k] = 0;
-  vjp<7>(in, outb, inb, [&](const Dual<7>* x, Dual<7>* y) { turned_t(x, x + 3, x + 6, x + 10, y); });
+  vjp<DW>(in, outb, inb, [&](const Dual<DW>* x, Dual<DW>* y) { turned_t(x, x + 3, x + 6, x + 10, y); });
   for (int k = 0; k < 3; k++) { BB.p[3 * g + k] += inb[k]; BB.start[3 * g + k] += inb[3 + k]; }
   for (int k = 0; k < 4; k++) { BB.q[4 * g + k] += inb[6 + k]; BB.start_q[4 * g + k] += inb[10 + k]; }
 }
@@ -2525,8 +2529,8 @@ HD void place_vjp(const Bodies& B, const BodiesBar& BB, const Work& W, const Wor
     outb[9 + 4 * k + 3] = k < count ? WB.separation()[P * i + k] : 0;
   }
   for (int c = 0; c < 14; c++) inb[c] = 0;
-  vjp<7>(in, outb, inb, [&](const Dual<7>* x, Dual<7>* y) {
-    for (int k = 0; k < 9 + 4 * P; k++) y[k] = dual<7>(0);
+  vjp<DW>(in, outb, inb, [&](const Dual<DW>* x, Dual<DW>* y) {
+    for (int k = 0; k < 9 + 4 * P; k++) y[k] = dual<DW>(0);
     place_t(h, x, x + 3, x + 7, x + 10, y);
   });
   if (ma) {
@@ -2546,7 +2550,7 @@ HD void inertia_vjp(const Bodies& B, const BodiesBar& BB, int g) {
   for (int k = 0; k < 3; k++) in[4 + k] = B.inv_inertia[3 * g + k];
   for (int k = 0; k < 6; k++) { outb[k] = BB.inv_i[6 * g + k]; BB.inv_i[6 * g + k] = 0; }
   for (int k = 0; k < 7; k++) inb[k] = 0;
-  vjp<7>(in, outb, inb, [&](const Dual<7>* x, Dual<7>* y) { world_inverse_t(x, x + 4, y); });
+  vjp<DW>(in, outb, inb, [&](const Dual<DW>* x, Dual<DW>* y) { world_inverse_t(x, x + 4, y); });
   for (int k = 0; k < 4; k++) BB.q[4 * g + k] += inb[k] + BB.start_q[4 * g + k];
   for (int k = 0; k < 3; k++) {
     BB.inv_inertia[3 * g + k] += inb[4 + k];
