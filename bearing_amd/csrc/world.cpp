@@ -17,6 +17,7 @@
 #include <cstdint>
 #include <cstring>
 #include <functional>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -188,8 +189,16 @@ struct Pairs {
   std::vector<uint8_t> cache_points;
   std::vector<double> cache_pose, cache_loss, cache_normal, cache_local;
   std::vector<uint32_t> cache_id;
-  // Each pair's Memory (src/pairs.js:84 memory), moved and swapped with it.
-  std::vector<Memory> memory;
+  // Each pair's Memory (src/pairs.js:84 memory), moved and swapped with it: made when a level
+  // pair first keeps something, so the other pairs (most of them) cost a null pointer.
+  std::vector<std::unique_ptr<Memory>> memory;
+  Memory& memory_of(int p) {
+    if (!memory[p]) memory[p].reset(new Memory());
+    return *memory[p];
+  }
+  void clear_memory(int p) {
+    if (memory[p]) memory[p]->count = 0;
+  }
 
   static uint64_t key(int x, int y) { return (uint64_t)x * 16777216ull + (uint64_t)y; }
 
@@ -237,7 +246,7 @@ struct Pairs {
   void forget(int p) {
     points[p] = 0;
     if (cache[p] != 0) uncache(p);
-    memory[p].count = 0;
+    clear_memory(p);
     part(p);
   }
   // src/pairs.js:303-318
@@ -274,7 +283,7 @@ struct Pairs {
   void remove(int p) {
     part(p);
     if (cache[p] != 0) uncache(p);
-    memory[p].count = 0;
+    clear_memory(p);
     index.erase(key(a[p], b[p]));
     if (p < active) {
       const int last = --active;
@@ -297,14 +306,15 @@ struct Manifold {
 };
 
 // A pair visited by narrow(): what find() decided for it, in the order visited.
+// Most visits of a settled pile find the boxes apart, so a visit is only its decision; a kept
+// manifold is rebuilt into the pass's `rebuilt` row of the same number.
 struct Visit {
   int p, a, b;
-  double tolerance;
-  int kind;  // 0: the boxes apart (forget); 1: the kept manifold, in m; 2: met on the device, batch row `slot`;
-             // 3: a level pair touched on the device, level batch row `slot`
+  int kind;  // 0: the boxes apart (forget); 1: the kept manifold, rebuilt; 2: met on the device, batch row
+             // `slot`; 3: a level pair touched on the device, level batch row `slot`
   int slot;
-  Manifold m;
   bool again = false;  // prepared a second time, a body having woken since the pass began
+  double tolerance;
 };
 
 // This step's manifolds in the order narrow() found them (src/contact.js Contacts).
@@ -450,6 +460,9 @@ struct bm_world {
   std::vector<uint32_t> level_ids;
   int level_cap = 8;
   std::vector<Visit> visits;  // the pass's visits are the first visit_count (the vector never shrinks)
+  std::vector<Manifold, Unzeroed<Manifold>> rebuilt;  // by visit: a kind-1 visit's kept manifold
+  std::vector<int32_t> live;  // the visits the serial walk takes, in order (the rest done on the pool)
+  size_t live_count = 0;
   size_t visit_count = 0;
   std::vector<uint8_t> apart;  // a large pass's pairs whose boxes no longer overlap, by slot
   // Bodies woken while a narrow pass's results are taken (their bounds changed then).
@@ -729,7 +742,7 @@ void found(bm_world* w, int x, int y) {
   const int p = pr.count;
   pr.grow(p + 1);
   pr.a[p] = a; pr.b[p] = b; pr.points[p] = 0; pr.touching[p] = 0; pr.cache[p] = 0;
-  pr.memory[p].count = 0;
+  pr.clear_memory(p);
   pr.index[Pairs::key(a, b)] = p;
   pr.count++;
   pr.wake(p);
@@ -1153,7 +1166,7 @@ int finish_level(bm_world* w, Visit& v) {
   const int a = v.a, b = v.b, p = v.p, k = v.slot, cap = w->level_cap;
   const bool turned = w->shape[a] >= MESH;
   const int n = w->level_manifolds[k];
-  Memory& kept = w->pairs.memory[p];
+  Memory& kept = w->pairs.memory_of(p);
   for (int j = 0; j < kept.count; j++) kept.taken[j] = 0;
   if (n > 0 && w->awake[a] != w->awake[b]) wake(w, w->awake[a] ? b : a);
   const Contacts& c = w->c;
@@ -1184,8 +1197,8 @@ int finish_level(bm_world* w, Visit& v) {
 }
 
 // find()'s decision for one visit (src/contact.js:334-354), reading only: the boxes apart
-// (0), the kept manifold rebuilt (1, REUSE, into v.m), a meet on the device (2), a level pair (3).
-void decide_visit(const bm_world* w, Visit& v) {
+// (0), the kept manifold rebuilt (1, REUSE, into m), a meet on the device (2), a level pair (3).
+void decide_visit(const bm_world* w, Visit& v, Manifold& m) {
   const int a = v.a, b = v.b;
   v.kind = 0;
   v.slot = -1;
@@ -1199,7 +1212,7 @@ void decide_visit(const bm_world* w, Visit& v) {
     return;
   }
   const double margin = w->motion[a] + w->motion[b] + v.tolerance;
-  v.kind = reused(w, v.p, a, b, v.tolerance, margin, v.m) ? 1 : 2;
+  v.kind = reused(w, v.p, a, b, v.tolerance, margin, m) ? 1 : 2;
 }
 
 // The meet record of a visit of kind 2 at its slot: both bodies and the pair's numbers.
@@ -1223,8 +1236,8 @@ void grow_batches(bm_world* w, size_t meets, size_t levels) {
 
 // find() up to the narrowphase (src/contact.js:334-354) for one visit: the box test, the
 // kept manifold (REUSE), a meet record appended to the device batch, or a level pair's.
-void prepare_visit(bm_world* w, Visit& v) {
-  decide_visit(w, v);
+void prepare_visit(bm_world* w, Visit& v, Manifold& m) {
+  decide_visit(w, v, m);
   if (v.kind == 2) {
     v.slot = (int)(w->batch_pair.size() / 6);
     grow_batches(w, 1, 0);
@@ -1239,7 +1252,7 @@ void prepare_visit(bm_world* w, Visit& v) {
 // The rest of find() and narrow()'s own steps for one visit, in the reference's order:
 // src/contact.js:339-363 (forget, or remember what meet() found) and 318-322 (wake,
 // emit, carry). Reads the device batch's results for a met pair.
-int finish_visit(bm_world* w, Visit& v) {
+int finish_visit(bm_world* w, Visit& v, Manifold& m) {
   if (v.kind == 0) {
     w->pairs.forget(v.p);
     return BM_OK;
@@ -1248,7 +1261,6 @@ int finish_visit(bm_world* w, Visit& v) {
   const double* shift = nullptr;
   if (v.kind == 2) {
     const int k = v.slot;
-    Manifold& m = v.m;
     m.count = w->batch_count[k];
     if (m.count < 0) return fail(w, BM_ERR_UNSUPPORTED, "step: a pair of shapes the device narrowphase does not handle");
     if (w->batch_flags[k] & 2) w->impacts++;
@@ -1270,7 +1282,7 @@ int finish_visit(bm_world* w, Visit& v) {
     }
   }
   if (w->awake[v.a] != w->awake[v.b]) wake(w, w->awake[v.a] ? v.b : v.a);
-  const int mi = emit(w, v.p, v.a, v.b, v.m, v.tolerance, shift);
+  const int mi = emit(w, v.p, v.a, v.b, m, v.tolerance, shift);
   carry(w, v.p, mi);
   return BM_OK;
 }
@@ -1290,7 +1302,7 @@ void narrow_begin(bm_world* w, int from) {
   std::vector<Visit>& visits = w->visits;
   auto visit_of = [&](int p) {
     const int a = pr.a[p], b = pr.b[p];
-    return Visit{p, a, b, js_min(w->size[a], w->size[b]) * PIXEL, 0, -1, Manifold{}};
+    return Visit{p, a, b, 0, -1, false, js_min(w->size[a], w->size[b]) * PIXEL};
   };
   if (w->pool && pr.active - from >= PARALLEL_VISITS) {
     // Each overlap reads only the trees, so a large pass tests them on the pool, then removes
@@ -1334,6 +1346,8 @@ void narrow_begin(bm_world* w, int from) {
   // The pairs are where they will stay for the pass: each one's decision reads only, so a
   // large pass decides on the pool's threads, then takes its slots in order and fills them.
   const int n = (int)w->visit_count;
+  if (w->rebuilt.size() < w->visits.size()) w->rebuilt.resize(w->visits.size());
+  Manifold* rebuilt = w->rebuilt.data();
   if (!w->pool && w->owns_ctx && n >= PARALLEL_VISITS) {
     // A world stepped alone that grew large: its own threads, up to 16.
     const int threads = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
@@ -1342,7 +1356,7 @@ void narrow_begin(bm_world* w, int from) {
   if (w->pool && n >= PARALLEL_VISITS) {
     const int chunk = 1024, chunks = (n + chunk - 1) / chunk;
     w->pool->run(chunks, [&](int c) {
-      for (int i = c * chunk, e = std::min(n, (c + 1) * chunk); i < e; i++) decide_visit(w, visits[i]);
+      for (int i = c * chunk, e = std::min(n, (c + 1) * chunk); i < e; i++) decide_visit(w, visits[i], rebuilt[i]);
     });
     size_t meets = w->batch_pair.size() / 6, levels = w->level_field.size(), m0 = meets, l0 = levels;
     for (int i = 0; i < n; i++) {
@@ -1351,14 +1365,45 @@ void narrow_begin(bm_world* w, int from) {
       else if (v.kind == 3) v.slot = (int)levels++;
     }
     grow_batches(w, meets - m0, levels - l0);
+    // A pair whose boxes are apart forgets what it kept; when it keeps no cache row and neither
+    // body can wake while the pass is taken (so the pair is never looked at again), that
+    // touches only the pair's own entries, in any order: those are forgotten here, and the
+    // serial walk takes the other visits, in order.
+    std::vector<int> kept_per((size_t)chunks + 1, 0);
+    const uint8_t* type = w->type.data();
+    const uint8_t* awake = w->awake.data();
+    auto can_wake = [&](int b) { return type[b] == DYNAMIC && awake[b] == 0; };
     w->pool->run(chunks, [&](int c) {
+      int kept_here = 0;
       for (int i = c * chunk, e = std::min(n, (c + 1) * chunk); i < e; i++) {
-        if (visits[i].kind == 2) fill_meet(w, visits[i]);
-        else if (visits[i].kind == 3) fill_level(w, visits[i]);
+        Visit& v = visits[i];
+        if (v.kind == 2) fill_meet(w, v);
+        else if (v.kind == 3) fill_level(w, v);
+        else if (v.kind == 0 && pr.cache[v.p] == 0 && !can_wake(v.a) && !can_wake(v.b)) {
+          pr.points[v.p] = 0;
+          pr.clear_memory(v.p);
+          pr.part(v.p);
+          v.kind = 4;
+          continue;
+        }
+        kept_here++;
       }
+      kept_per[c + 1] = kept_here;
     });
+    for (int c = 0; c < chunks; c++) kept_per[c + 1] += kept_per[c];
+    if (w->live.size() < (size_t)kept_per[chunks]) w->live.resize(kept_per[chunks]);
+    int32_t* live = w->live.data();
+    w->pool->run(chunks, [&](int c) {
+      int at = kept_per[c];
+      for (int i = c * chunk, e = std::min(n, (c + 1) * chunk); i < e; i++)
+        if (visits[i].kind != 4) live[at++] = i;
+    });
+    w->live_count = kept_per[chunks];
   } else {
-    for (int i = 0; i < n; i++) prepare_visit(w, visits[i]);
+    for (int i = 0; i < n; i++) prepare_visit(w, visits[i], rebuilt[i]);
+    if (w->live.size() < (size_t)n) w->live.resize(n);
+    for (int i = 0; i < n; i++) w->live[i] = i;
+    w->live_count = n;
   }
   // Prepared after every wake so far: nothing is fresh for this pass yet.
   for (int s : w->woken) w->fresh[s] = 0;
@@ -1370,14 +1415,16 @@ void narrow_begin(bm_world* w, int from) {
 // again left a record in the batch), false when the pass is done.
 bool narrow_resume(bm_world* w, int& rc) {
   std::vector<Visit>& visits = w->visits;
-  for (; w->visit_at < w->visit_count; w->visit_at++) {
-    Visit& v = visits[w->visit_at];
+  for (; w->visit_at < w->live_count; w->visit_at++) {
+    const int i = w->live[w->visit_at];
+    Visit& v = visits[i];
+    Manifold& m = w->rebuilt[i];
     if (!v.again && (w->fresh[v.a] || w->fresh[v.b])) {
       v.again = true;
-      prepare_visit(w, v);
+      prepare_visit(w, v, m);
       if (v.kind >= 2) return true;
     }
-    if ((rc = finish_visit(w, v)) != BM_OK) return false;
+    if ((rc = finish_visit(w, v, m)) != BM_OK) return false;
   }
   for (int s : w->woken) w->fresh[s] = 0;
   w->woken.clear();
@@ -2256,7 +2303,7 @@ void store_memory(bm_world* w, int solved) {
     const int m = c.row[f];
     if (m >= solved || c.sub[m] < 0) continue;
     const int sub = c.sub[m], m3 = 3 * m, s3 = 3 * sub;
-    Memory& kept = pr.memory[c.pair[m]];
+    Memory& kept = pr.memory_of(c.pair[m]);
     kept.ensure(sub + 1);
     kept.count = sub + 1;
     for (int k = 0; k < 3; k++) kept.normal[s3 + k] = c.normal[m3 + k];
