@@ -23,7 +23,9 @@ using bm::SoftDev;
 
 namespace {
 
-enum Unit { WORLD1, WORLD, STARTS, WORLD3, BODY, AWAKE, MANIFOLD, PAIR };
+// KEPT: what the pair list keeps, by pair on the host; on the device by manifold row (each
+// solved row stores its pair's memory there), never uploaded, scattered to the pairs on download.
+enum Unit { WORLD1, WORLD, STARTS, WORLD3, BODY, AWAKE, MANIFOLD, KEPT };
 enum Dir { IN = 1, IO = 3 };
 
 struct Column {
@@ -66,9 +68,9 @@ const Column COLUMNS[] = {
     COL(block, 8, 16, MANIFOLD, IO), COL(lever, 8, 4, MANIFOLD, IO), COL(anchor_a, 8, 3, MANIFOLD, IO),
     COL(anchor_b, 8, 3, MANIFOLD, IO), COL(fjac, 8, 24, MANIFOLD, IO), COL(tangent_mass, 8, 3, MANIFOLD, IO),
     COL(roll_mass1, 8, 1, MANIFOLD, IO), COL(roll_mass2, 8, 1, MANIFOLD, IO), COL(spin_mass, 8, 1, MANIFOLD, IO),
-    COL(pid, 4, 4, MANIFOLD, IO), COL(pair_points, 1, 1, PAIR, IO), COL(pair_pid, 4, 4, PAIR, IO),
-    COL(pair_normal_impulse, 8, 4, PAIR, IO), COL(pair_twist, 8, 3, PAIR, IO), COL(pair_friction, 8, 3, PAIR, IO),
-    COL(pair_slip, 8, 3, PAIR, IO),
+    COL(pid, 4, 4, MANIFOLD, IO), COL(pair_points, 1, 1, KEPT, IO), COL(pair_pid, 4, 4, KEPT, IO),
+    COL(pair_normal_impulse, 8, 4, KEPT, IO), COL(pair_twist, 8, 3, KEPT, IO), COL(pair_friction, 8, 3, KEPT, IO),
+    COL(pair_slip, 8, 3, KEPT, IO),
 };
 #undef COL
 constexpr int NCOL = sizeof(COLUMNS) / sizeof(COLUMNS[0]);
@@ -99,6 +101,7 @@ struct bm_context {
   int device = 0;
   hipStream_t stream = nullptr;
   void* noalloc_block = nullptr;  // this device's block for bm_level_noalloc (collide.hip)
+  bool stored = false;            // a soft step ran on the batch uploaded last (its pair stores are set)
   hipEvent_t begin = nullptr, end = nullptr;
   std::string error;
   int32_t n_worlds = 0, bodies = 0, awake = 0, manifolds = 0, pairs = 0;
@@ -182,7 +185,7 @@ size_t units(const bm_context* ctx, Unit u) {
     case BODY: return (size_t)ctx->bodies;
     case AWAKE: return (size_t)ctx->awake;
     case MANIFOLD: return (size_t)ctx->manifolds;
-    case PAIR: return (size_t)ctx->pairs;
+    case KEPT: return (size_t)ctx->manifolds;
   }
   return 0;
 }
@@ -503,7 +506,7 @@ size_t part_units(const bm_columns* h, Unit u) {
     case BODY: return (size_t)h->body_offset[n];
     case AWAKE: return (size_t)h->awake_offset[n];
     case MANIFOLD: return (size_t)h->manifold_offset[n];
-    case PAIR: return (size_t)h->pair_offset[n];
+    case KEPT: return (size_t)h->pair_offset[n];
   }
   return 0;
 }
@@ -577,8 +580,8 @@ int upload_parts(bm_context* ctx, int nparts, const bm_columns* const* parts, co
     if (col.unit == WORLD1) {
       const int t = c;  // the four offset tables are the first four columns
       std::memcpy(dst, joined[t], ctx->bytes[c]);
-    } else if (ctx->keep_device_only && is_device_only(c)) {
-      // left on the device
+    } else if ((ctx->keep_device_only && is_device_only(c)) || col.unit == KEPT) {
+      // left on the device (KEPT: written by the soft step before it is read)
     } else {
       for (int q = 0; q < nparts; q++) {
         const size_t bytes = part_units(parts[q], col.unit) * col.per * col.elem;
@@ -601,7 +604,47 @@ int upload_parts(bm_context* ctx, int nparts, const bm_columns* const* parts, co
   ctx->dev.n_worlds = ctx->n_worlds;
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), (std::string(who) + ": hipStreamSynchronize").c_str());
   ctx->uploaded = true;
+  ctx->stored = false;
   return BM_OK;
+}
+
+// What the soft step stored by row (src/contact.js:1067-1088 store) into each part's pair
+// columns at the row's pair: a solved row of a pair's own manifold (sub < 0) gives its pair's
+// points, twist, friction and slip, and its points' ids and impulses; every other pair keeps
+// what the part's arrays hold. Reads the part's manifold columns as the download left them.
+void scatter_kept(const bm_context* ctx, int nparts, bm_columns* const* parts, const char* stage) {
+  const auto col = [&](const char* name) { return stage + ctx->offset[column_index(name)]; };
+  const uint8_t* kp = reinterpret_cast<const uint8_t*>(col("pair_points"));
+  const uint32_t* kid = reinterpret_cast<const uint32_t*>(col("pair_pid"));
+  const double* kimp = reinterpret_cast<const double*>(col("pair_normal_impulse"));
+  const double* ktw = reinterpret_cast<const double*>(col("pair_twist"));
+  const double* kfr = reinterpret_cast<const double*>(col("pair_friction"));
+  const double* ksl = reinterpret_cast<const double*>(col("pair_slip"));
+  size_t base = 0;  // the part's first row in the batch
+  for (int q = 0; q < nparts; q++) {
+    bm_columns* h = parts[q];
+    for (int w = 0; w < h->n_worlds; w++) {
+      const int m0 = h->manifold_offset[w], p0 = h->pair_offset[w], solved = h->solved[w];
+      for (int lm = 0; lm < solved; lm++) {
+        const int m = m0 + lm;
+        if (h->sub[m] >= 0) continue;
+        const size_t g = base + (size_t)m;
+        const size_t p = (size_t)p0 + h->pair[m];
+        const int count = kp[g] < 4 ? kp[g] : 4;
+        h->pair_points[p] = kp[g];
+        for (int k = 0; k < 3; k++) {
+          h->pair_twist[3 * p + k] = ktw[3 * g + k];
+          h->pair_friction[3 * p + k] = kfr[3 * g + k];
+          h->pair_slip[3 * p + k] = ksl[3 * g + k];
+        }
+        for (int k = 0; k < count; k++) {
+          h->pair_pid[4 * p + k] = kid[4 * g + k];
+          h->pair_normal_impulse[4 * p + k] = kimp[4 * g + k];
+        }
+      }
+    }
+    base += (size_t)h->manifold_offset[h->n_worlds];
+  }
 }
 
 // Every IO column back into the parts it came from.
@@ -621,7 +664,7 @@ int download_parts(bm_context* ctx, int nparts, bm_columns* const* parts, const 
   const char* stage = static_cast<const char*>(ctx->stage);
   std::vector<HostCopy> copies;
   for (int c = 0; c < NCOL; c++) {
-    if (COLUMNS[c].dir != IO || ctx->bytes[c] == 0) continue;
+    if (COLUMNS[c].dir != IO || ctx->bytes[c] == 0 || COLUMNS[c].unit == KEPT) continue;
     if (ctx->keep_device_only && is_device_only(c)) continue;
     const char* from = stage + ctx->offset[c];
     for (int q = 0; q < nparts; q++) {
@@ -632,6 +675,8 @@ int download_parts(bm_context* ctx, int nparts, bm_columns* const* parts, const 
     }
   }
   run_copies(copies);
+  // Before a soft step ran on this upload the pairs keep what they held, as it left them.
+  if (ctx->stored) scatter_kept(ctx, nparts, parts, stage);
   return BM_OK;
 }
 
@@ -682,6 +727,7 @@ int bm_soft_step(bm_context* ctx) {
   HIP_OR_FAIL(ctx, hipEventRecord(ctx->end, ctx->stream), "bm_soft_step: hipEventRecord");
   HIP_OR_FAIL(ctx, hipEventSynchronize(ctx->end), "bm_soft_step: kernel");
   HIP_OR_FAIL(ctx, hipEventElapsedTime(&ctx->last_ms, ctx->begin, ctx->end), "bm_soft_step: hipEventElapsedTime");
+  ctx->stored = true;
   return BM_OK;
 }
 

@@ -26,6 +26,8 @@ struct SoftDev {
       *normal_mass, *jac, *block, *lever, *anchor_a, *anchor_b, *fjac, *tangent_mass, *roll_mass1, *roll_mass2,
       *spin_mass;
   const uint32_t* pid;
+  // What each solved row of a pair's own manifold stores for its pair (src/contact.js:1067-1088
+  // store), by row; the download writes it into the pair columns at the row's pair.
   uint8_t* pair_points;
   uint32_t* pair_pid;
   double *pair_normal_impulse, *pair_twist, *pair_friction, *pair_slip;

@@ -830,7 +830,7 @@ __device__ __forceinline__ void turned(const SoftDev& d, int g, double* T) {
 }
 
 // src/contact.js:1122-1153 (slipped) then 1091-1115 (keep) for row m, into pair gp.
-__device__ __forceinline__ void store_row(const SoftDev& d, int m, int ga, int gb, int gp) {
+__device__ __forceinline__ void store_row(const SoftDev& d, int m, int ga, int gb) {
   const int m3 = 3 * m, count = d.points[m];
   double pressed = 0;
   bool impact = false;
@@ -863,25 +863,26 @@ __device__ __forceinline__ void store_row(const SoftDev& d, int m, int ga, int g
     d.drift1[m] += dx * t1x + dy * t1y + dz * t1z;
     d.drift2[m] += dx * t2x + dy * t2y + dz * t2z;
   }
-  // A level pair's manifolds go into its Memory, which the host keeps.
+  // A level pair's manifolds go into its Memory, which the host keeps. What the row stores
+  // for its pair is kept by row here; the download writes it into the pair's columns.
   if (d.sub[m] >= 0) return;
-  const int p3 = 3 * gp;
-  d.pair_points[gp] = (uint8_t)count;
+  const int k3 = 3 * m;
+  d.pair_points[m] = (uint8_t)count;
   const double r1 = d.roll_impulse1[m], r2 = d.roll_impulse2[m], sp = d.spin_impulse[m];
-  d.pair_twist[p3] = t1x * r1 + t2x * r2 + d.normal[m3] * sp;
-  d.pair_twist[p3 + 1] = t1y * r1 + t2y * r2 + d.normal[m3 + 1] * sp;
-  d.pair_twist[p3 + 2] = t1z * r1 + t2z * r2 + d.normal[m3 + 2] * sp;
+  d.pair_twist[k3] = t1x * r1 + t2x * r2 + d.normal[m3] * sp;
+  d.pair_twist[k3 + 1] = t1y * r1 + t2y * r2 + d.normal[m3 + 1] * sp;
+  d.pair_twist[k3 + 2] = t1z * r1 + t2z * r2 + d.normal[m3 + 2] * sp;
   const double l1 = d.tangent_impulse1[m], l2 = d.tangent_impulse2[m];
-  d.pair_friction[p3] = t1x * l1 + t2x * l2;
-  d.pair_friction[p3 + 1] = t1y * l1 + t2y * l2;
-  d.pair_friction[p3 + 2] = t1z * l1 + t2z * l2;
+  d.pair_friction[k3] = t1x * l1 + t2x * l2;
+  d.pair_friction[k3 + 1] = t1y * l1 + t2y * l2;
+  d.pair_friction[k3 + 2] = t1z * l1 + t2z * l2;
   const double d1 = d.drift1[m], d2 = d.drift2[m];
-  d.pair_slip[p3] = t1x * d1 + t2x * d2;
-  d.pair_slip[p3 + 1] = t1y * d1 + t2y * d2;
-  d.pair_slip[p3 + 2] = t1z * d1 + t2z * d2;
+  d.pair_slip[k3] = t1x * d1 + t2x * d2;
+  d.pair_slip[k3 + 1] = t1y * d1 + t2y * d2;
+  d.pair_slip[k3 + 2] = t1z * d1 + t2z * d2;
   for (int k = 0; k < count; k++) {
-    d.pair_pid[P * gp + k] = d.pid[m * P + k];
-    d.pair_normal_impulse[P * gp + k] = d.normal_impulse[m * P + k];
+    d.pair_pid[P * m + k] = d.pid[m * P + k];
+    d.pair_normal_impulse[P * m + k] = d.normal_impulse[m * P + k];
   }
 }
 
@@ -935,7 +936,7 @@ __global__ void __launch_bounds__(THREADS, WAVES) soft_step_kernel(SoftDev d) {
   __shared__ int bounce_waves[THREADS / 64];
   extern __shared__ double lds[];
   for (int wld = blockIdx.x; wld < d.n_worlds; wld += gridDim.x) {
-    const int boff = d.body_offset[wld], moff = d.manifold_offset[wld], poff = d.pair_offset[wld];
+    const int boff = d.body_offset[wld], moff = d.manifold_offset[wld];
     const int nbodies = d.body_offset[wld + 1] - boff;
     const int aoff = d.awake_offset[wld], awake = d.awake_offset[wld + 1] - aoff;
     const int count = d.manifold_offset[wld + 1] - moff, solved = d.solved[wld];
@@ -996,7 +997,7 @@ __global__ void __launch_bounds__(THREADS, WAVES) soft_step_kernel(SoftDev d) {
         const int lm = d.row[moff + f];
         if (lm >= solved) continue;
         const int m = moff + lm;
-        store_row(d, m, boff + d.a[m], boff + d.b[m], poff + d.pair[m]);
+        store_row(d, m, boff + d.a[m], boff + d.b[m]);
       }
     }
     if (RESIDENT) {
@@ -1098,7 +1099,7 @@ __global__ void __launch_bounds__(THREADS, 1) soft_wide_kernel(SoftDev d, int wl
   __shared__ int bounce_list[BOUNCE_CHUNK];
   __shared__ int bounce_waves[THREADS / 64];
   const int tid = blockIdx.x * THREADS + threadIdx.x, nth = gridDim.x * THREADS;
-  const int boff = d.body_offset[wld], moff = d.manifold_offset[wld], poff = d.pair_offset[wld];
+  const int boff = d.body_offset[wld], moff = d.manifold_offset[wld];
   const int aoff = d.awake_offset[wld], awake = d.awake_offset[wld + 1] - aoff;
   const int count = d.manifold_offset[wld + 1] - moff, solved = d.solved[wld];
   const int substeps = d.substeps[wld];
@@ -1149,7 +1150,7 @@ __global__ void __launch_bounds__(THREADS, 1) soft_wide_kernel(SoftDev d, int wl
       const int lm = d.row[moff + f];
       if (lm >= solved) continue;
       const int m = moff + lm;
-      store_row(d, m, boff + d.a[m], boff + d.b[m], poff + d.pair[m]);
+      store_row(d, m, boff + d.a[m], boff + d.b[m]);
     }
   }
 }

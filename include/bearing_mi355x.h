@@ -98,7 +98,10 @@ typedef struct bm_columns {
   double* tangent_mass;      /* [out] 3 */
   double* roll_mass1, *roll_mass2, *spin_mass; /* [out] 1 */
   uint32_t* pid;             /* [io] 4 */
-  /* what the pair list keeps, by pair (src/pairs.js) */
+  /* what the pair list keeps, by pair (src/pairs.js): bm_download writes the entries store()
+   * writes (src/contact.js:1067-1088), those of the pairs of solved manifolds with sub < 0 (a
+   * pair's points, twist, friction, slip, and ids and impulses for its points); every other
+   * entry is left as the caller's arrays hold it. bm_upload does not read them. */
   uint8_t* pair_points;      /* [out] 1 */
   uint32_t* pair_pid;        /* [out] 4 */
   double* pair_normal_impulse; /* [out] 4 */
