@@ -3259,6 +3259,12 @@ struct bm_adjoint {
   std::string error;
   int bodies = 0, steps = 0;
   std::vector<double> state0, mass, friction, restitution;
+  // The held trajectory as the device took it (bm_adjoint_held), and the arrays it points into.
+  bm_held held{};
+  std::vector<int32_t> i32[9];
+  std::vector<uint8_t> u8[5];
+  std::vector<uint32_t> u32;
+  std::vector<double> f64[22];
 };
 
 namespace {
@@ -3405,10 +3411,23 @@ int bm_adjoint_create(bm_adjoint** out, int32_t device, bm_world* const* worlds,
   h.store_slip = st_sl.data();
   if (int rc = bm_create(&A->ctx, device)) return bad(rc, "bm_adjoint_create: no device context");
   if (int rc = bm_held_load(A->ctx, &h)) return bad(rc, std::string("bm_adjoint_create: ") + bm_last_error(A->ctx));
+  // Kept for bm_adjoint_held: a vector moved keeps its storage, so h's pointers stay good.
+  {
+    std::vector<int32_t>* I[9] = {&body_offset, &substeps, &row_offset, &a, &b, &src, &starts, &narrow, &first_base};
+    std::vector<uint8_t>* U[5] = {&type, &shape, &tumbles, &capped, &count};
+    std::vector<double>* D[22] = {&dt, &gravity, &rounded_, &segment, &gscale, &damping, &adamping, &mass0, &inv_mass0,
+                                  &inertia0, &inv_inertia0, &left, &ceiling, &ln, &ends, &shift, &tol, &patch,
+                                  &st_imp, &st_fr, &st_tw, &st_sl};
+    for (int k = 0; k < 9; k++) A->i32[k] = std::move(*I[k]);
+    for (int k = 0; k < 5; k++) A->u8[k] = std::move(*U[k]);
+    for (int k = 0; k < 22; k++) A->f64[k] = std::move(*D[k]);
+    A->u32 = std::move(pid);
+    A->held = h;
+  }
   A->bodies = nb;
   A->steps = T;
   A->state0 = std::move(state0);
-  A->mass = std::move(mass0);
+  A->mass = A->f64[7];  // mass0, kept above
   A->friction = std::move(friction);
   A->restitution = std::move(restitution);
   *out = A;
@@ -3462,6 +3481,12 @@ int bm_adjoint_backward(bm_adjoint* a, const double* state_bar, double* state0_b
 int bm_adjoint_times(const bm_adjoint* a, double* forward_ms, double* backward_ms) {
   if (!a || !a->ctx) return BM_ERR_ARGUMENT;
   return bm_held_times(a->ctx, forward_ms, backward_ms);
+}
+
+int bm_adjoint_held(const bm_adjoint* a, bm_held* out) {
+  if (!a || !a->ctx || !out) return BM_ERR_ARGUMENT;
+  *out = a->held;
+  return BM_OK;
 }
 
 int bm_adjoint_branches(bm_adjoint* a, uint64_t* per_world) {

@@ -66,7 +66,7 @@ EXPORTS = ["bm_create", "bm_destroy", "bm_last_error", "bm_upload", "bm_soft_ste
            "bm_batch_times", "bm_world_record", "bm_world_recorded", "bm_adjoint_create", "bm_adjoint_destroy",
            "bm_adjoint_error", "bm_adjoint_size", "bm_adjoint_recorded", "bm_adjoint_forward", "bm_adjoint_backward",
            "bm_adjoint_times", "bm_held_load", "bm_held_forward", "bm_held_backward", "bm_held_times",
-           "bm_adjoint_branches", "bm_held_branches"]
+           "bm_adjoint_branches", "bm_held_branches", "bm_adjoint_held"]
 
 _lib = None
 
@@ -159,6 +159,19 @@ def world_signatures(lib):
     lib.bm_adjoint_backward.argtypes = [vp, _D, _D, _D, _D, _D]
     lib.bm_adjoint_times.argtypes = [vp, _D, _D]
     lib.bm_adjoint_branches.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
+    lib.bm_adjoint_held.argtypes = [vp, ctypes.POINTER(Held)]
+
+
+class Held(ctypes.Structure):
+    """bm_held: a held trajectory as the device takes it."""
+    _fields_ = ([(f, ctypes.c_int32) for f in ("worlds", "steps", "every", "bodies", "rows", "first")]
+                + [(f, _I32) for f in ("body_offset", "row_offset", "starts", "narrow", "substeps")]
+                + [(f, _D) for f in ("dt", "gravity")] + [(f, _U8) for f in ("type", "shape", "tumbles", "capped")]
+                + [(f, _D) for f in ("rounded", "segment", "gravity_scale", "damping", "angular_damping", "mass0",
+                                     "inv_mass0", "inertia0", "inv_inertia0", "left", "ceiling")]
+                + [(f, _I32) for f in ("a", "b", "src")] + [("count", _U8), ("pid", ctypes.POINTER(ctypes.c_uint32))]
+                + [(f, _D) for f in ("local_normal", "ends", "shift", "tolerance", "patch", "store_impulse",
+                                     "store_friction", "store_twist", "store_slip")])
 
 
 class BearingError(RuntimeError):
@@ -622,6 +635,24 @@ class Adjoint:
                                                 *[out[k].ctypes.data_as(_D) for k in
                                                   ("state", "mass", "friction", "restitution")]),
                    "bm_adjoint_backward")
+        return out
+
+    def held(self):
+        """bm_adjoint_held: the held trajectory's description, every array copied out."""
+        h = Held()
+        self.check(self.lib.bm_adjoint_held(self.ptr, ctypes.byref(h)), "bm_adjoint_held")
+        W, T, B, R, F = h.worlds, h.steps, h.bodies, h.rows, h.first
+        sizes = {"body_offset": W + 1, "row_offset": T * W + 1, "starts": T * W * 35, "narrow": R, "substeps": W,
+                 "dt": W, "gravity": 3 * W, "type": B, "shape": B, "tumbles": B, "capped": B, "rounded": B,
+                 "segment": B, "gravity_scale": B, "damping": B, "angular_damping": B, "mass0": B, "inv_mass0": B,
+                 "inertia0": 3 * B, "inv_inertia0": 3 * B, "left": 3 * B, "ceiling": B, "a": R, "b": R, "src": R,
+                 "count": R, "pid": 4 * R, "local_normal": 3 * R, "ends": 24 * R, "shift": 12 * R, "tolerance": R,
+                 "patch": R, "store_impulse": 4 * F, "store_friction": 3 * F, "store_twist": 3 * F,
+                 "store_slip": 3 * F}
+        out = {k: getattr(h, k) for k in ("worlds", "steps", "every", "bodies", "rows", "first")}
+        for name, n in sizes.items():
+            ptr = getattr(h, name)
+            out[name] = np.ctypeslib.as_array(ptr, shape=(n,)).copy() if n > 0 else np.zeros(0)
         return out
 
     def branches(self):

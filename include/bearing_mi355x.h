@@ -501,6 +501,10 @@ int bm_held_forward(bm_context* ctx, const double* state0, const double* mass, c
 int bm_held_backward(bm_context* ctx, const double* state_bar, double* state0_bar, double* mass_bar,
                      double* friction_bar, double* restitution_bar);
 int bm_held_times(const bm_context* ctx, double* forward_ms, double* backward_ms);
+/* The held trajectory as bm_adjoint_create gave it to the device (bm_held_load's layout): its
+ * pointers are into the adjoint's own copy, good until bm_adjoint_destroy. No counterpart in the
+ * reference; for checking the held step against another implementation of it. */
+int bm_adjoint_held(const bm_adjoint* a, bm_held* out);
 /* bm_adjoint_branches for the held trajectory bm_held_load loaded. */
 int bm_held_branches(bm_context* ctx, uint64_t* per_world);
 

@@ -251,3 +251,32 @@ def world_fd_check(bodies, steps, h=1e-6, seed=1, **kw):
         what, idx = WORLD_GRADIENT[name]
         rows.append((name, float(g[what][idx]), (4 * diffs[1] - diffs[0]) / 3, bool(same)))
     return rows
+
+
+def torch_check(adj, seed=1):
+    """The adjoint against autograd through the plain FP64 PyTorch transcription of the held step
+    (tests/torchheld.py, written from the reference's JavaScript): the largest difference of the
+    final states, and for each of state (dynamic bodies' rows: a static body's state is a scene
+    constant, its row 0), mass, friction and restitution the largest difference of the
+    gradients and the largest gradient."""
+    import torch
+
+    import torchheld
+    held, rec = adj.held(), adj.recorded()
+    out = adj.forward(rec["state"], rec["mass"], rec["friction"], rec["restitution"])
+    ins = {k: torch.tensor(np.array(rec[k]), dtype=torch.float64, requires_grad=True)
+           for k in ("state", "mass", "friction", "restitution")}
+    got = torchheld.forward(held, ins["state"], ins["mass"], ins["friction"], ins["restitution"])
+    W = np.random.default_rng(seed).normal(size=out.shape)
+    dynamic = held["type"] == 0
+    W[~dynamic] = 0
+    (got * torch.from_numpy(W)).sum().backward()
+    g = adj.backward(W)
+    diffs = {"forward": float(np.abs(got.detach().numpy() - out).max())}
+    for key, t in ins.items():
+        mine = t.grad.numpy() if t.grad is not None else np.zeros_like(g[key])
+        theirs = g[key]
+        if key == "state":
+            mine, theirs = mine[dynamic], theirs[dynamic]
+        diffs[key] = (float(np.abs(mine - theirs).max()), float(np.abs(theirs).max()))
+    return diffs

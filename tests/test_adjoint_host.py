@@ -263,3 +263,22 @@ def test_host_adjoint_and_the_engine_part_where_contacts_are_found_afresh(lib):
     W = np.random.default_rng(1).normal(size=(2, native.HELD_STATE))
     assert abs(rows["p0x"][2] - W[1, 0]) < 1e-6 * abs(W[1, 0])      # the engine: translation
     assert abs(rows["p0x"][1] - W[1, 0]) > 1e-4 * abs(W[1, 0])       # the held model: not
+
+
+@pytest.mark.parametrize("name", ["slide", "roll", "bounce"])
+def test_host_adjoint_is_autograd_of_a_torch_transcription(lib, name):
+    # torch.autograd through a plain FP64 PyTorch transcription of the held step, written from
+    # the reference's JavaScript (tests/torchheld.py), against bm_adjoint_backward: the same
+    # final state and the same gradients to rounding (a box face's block LCP with Murty
+    # pivoting, a ball's rolling and spinning friction, a bounce).
+    from adjointcheck import torch_check
+    bodies, steps = SCENES[name]
+    w = record(bodies, steps, lib=lib)
+    adj = native.Adjoint([w], every=10, lib=lib)
+    d = torch_check(adj)
+    adj.close()
+    assert d["forward"] < 1e-13, d
+    for key in ("state", "friction", "restitution"):
+        diff, most = d[key]
+        assert diff <= 1e-12 * max(most, 1.0), (key, d)
+    assert d["mass"][0] < 1e-15, d   # both ~0: a body on a static floor moves as it would at any mass

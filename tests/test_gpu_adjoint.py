@@ -128,3 +128,22 @@ def test_gpu_adjoint_is_the_engines_gradient_for_a_rolling_ball():
     assert all(r[3] for r in rows), [r[0] for r in rows if not r[3]]
     bad = [r for r in rows if abs(r[1] - r[2]) > max(1e-6 * abs(r[2]), 3e-7)]
     assert not bad, bad
+
+
+@pytest.mark.parametrize("name", ["slide", "roll", "bounce"])
+def test_gpu_adjoint_is_autograd_of_a_torch_transcription(name):
+    # The device's adjoint against torch.autograd through the plain FP64 PyTorch transcription of
+    # the held step (tests/torchheld.py, from the reference's JavaScript), run on the CPU.
+    from adjointcheck import torch_check
+    print("adjoint vs torch transcription", name, flush=True)
+    bodies, steps = SCENES[name]
+    w = record(bodies, steps, device=0)
+    adj = native.Adjoint([w], every=10, device=0)
+    d = torch_check(adj)
+    adj.close()
+    w.close()
+    assert d["forward"] < 1e-13, d
+    for key in ("state", "friction", "restitution"):
+        diff, most = d[key]
+        assert diff <= 1e-12 * max(most, 1.0), (key, d)
+    assert d["mass"][0] < 1e-15, d
