@@ -494,6 +494,8 @@ struct bm_world {
   double h_dt;
   bm_columns h;
   Record record;
+  bm_columns_tap tap = nullptr;  // bm_world_tap
+  void* tap_user = nullptr;
 };
 
 namespace {
@@ -2539,6 +2541,7 @@ int device_phase(bm_world* w, double dt) {
   fill_columns(w, dt);
   if (w->record.on) record_before(w, dt);
   bm_columns& h = w->h;
+  if (w->tap) w->tap(w->tap_user, &h);
   if (int rc = bm_upload(w->ctx, &h)) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx));
   if (int rc = bm_colour(w->ctx)) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx));
   if (int rc = bm_soft_step(w->ctx)) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx));
@@ -3043,6 +3046,13 @@ int bm_world_step(bm_world* w, double dt) {
 int bm_world_times(const bm_world* w, double* out) {
   if (!w || !out) return BM_ERR_ARGUMENT;
   for (int k = 0; k < BM_WORLD_TIMES; k++) out[k] = w->times[k];
+  return BM_OK;
+}
+
+int bm_world_tap(bm_world* w, bm_columns_tap tap, void* user) {
+  if (!w) return BM_ERR_ARGUMENT;
+  w->tap = tap;
+  w->tap_user = user;
   return BM_OK;
 }
 

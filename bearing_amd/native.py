@@ -56,10 +56,28 @@ class Columns(ctypes.Structure):
     _fields_ = [("n_worlds", ctypes.c_int32)] + [(f, k) for f, k, _, _ in FIELDS]
 
 
+# bm_columns_tap: void (*)(void* user, const bm_columns* columns)
+COLUMNS_TAP = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(Columns))
+
+
+def copy_columns(cols):
+    """One world's bm_columns (n_worlds 1) copied into arrays by field name, as Batch takes them."""
+    counts = {"B": cols.body_offset[1], "A": cols.awake_offset[1], "M": cols.manifold_offset[1],
+              "P": cols.pair_offset[1]}
+    out = {}
+    for f, kind, unit, per in FIELDS:
+        if unit == "W1":
+            continue
+        n = {"W": 1, "W3": 3, "S": STARTS}.get(unit, counts.get(unit, 0) * per)
+        p = getattr(cols, f)
+        out[f] = np.ctypeslib.as_array(p, (n,)).copy() if n and p else np.zeros(n, _NP[kind])
+    return out
+
+
 EXPORTS = ["bm_create", "bm_destroy", "bm_last_error", "bm_upload", "bm_soft_step", "bm_download",
            "bm_soft_step_host", "bm_last_kernel_ms", "bm_set_launch", "bm_lds_limit", "bm_colour", "bm_collide_host", "bm_meet_host", "bm_set_colour_lds", "bm_set_hulls",
            "bm_set_fields", "bm_level_host", "bm_set_wide", "bm_set_device_only",
-           "bm_level_noalloc", "bm_world_moved",
+           "bm_level_noalloc", "bm_world_moved", "bm_world_tap",
            "bm_world_create", "bm_world_destroy", "bm_world_error", "bm_world_add", "bm_world_step", "bm_world_read",
            "bm_world_set_hulls", "bm_world_set_fields", "bm_world_times", "bm_upload_worlds", "bm_download_worlds",
            "bm_batch_create", "bm_batch_destroy", "bm_batch_error", "bm_batch_world", "bm_batch_step",
@@ -71,6 +89,21 @@ EXPORTS = ["bm_create", "bm_destroy", "bm_last_error", "bm_upload", "bm_soft_ste
 _lib = None
 
 
+def context_signatures(lib):
+    """The signatures of a context's calls (bm_create .. bm_last_kernel_ms) on `lib`."""
+    vp = ctypes.c_void_p
+    lib.bm_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int32]
+    lib.bm_destroy.argtypes = [vp]
+    lib.bm_last_error.argtypes = [vp]
+    lib.bm_last_error.restype = ctypes.c_char_p
+    for name in ("bm_upload", "bm_download"):
+        getattr(lib, name).argtypes = [vp, ctypes.POINTER(Columns)]
+    lib.bm_soft_step.argtypes = [vp]
+    lib.bm_colour.argtypes = [vp]
+    lib.bm_last_kernel_ms.argtypes = [vp]
+    lib.bm_last_kernel_ms.restype = ctypes.c_double
+
+
 def load():
     """The built library, with its signatures set; raises if it is not built."""
     global _lib
@@ -79,14 +112,8 @@ def load():
             raise RuntimeError(f"bearing_amd: {LIB} is not built (run __graft_entry__.build())")
         lib = ctypes.CDLL(LIB)
         vp = ctypes.c_void_p
-        lib.bm_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int32]
-        lib.bm_destroy.argtypes = [vp]
-        lib.bm_last_error.argtypes = [vp]
-        lib.bm_last_error.restype = ctypes.c_char_p
-        for name in ("bm_upload", "bm_download", "bm_soft_step_host"):
-            getattr(lib, name).argtypes = [vp, ctypes.POINTER(Columns)]
-        lib.bm_soft_step.argtypes = [vp]
-        lib.bm_colour.argtypes = [vp]
+        context_signatures(lib)
+        lib.bm_soft_step_host.argtypes = [vp, ctypes.POINTER(Columns)]
         lib.bm_collide_host.argtypes = [vp, ctypes.c_int32, _D, _D, _D, _D, _I32, _D, _D, _D, _U32, _D]
         lib.bm_meet_host.argtypes = [vp, ctypes.c_int32, _D, _D, _D, _I32, _D, _D, _D, _U32, _D, _I32, _D]
         lib.bm_set_hulls.argtypes = [vp, ctypes.c_int32, _I32, _I32, _I32, _D, _I32, _I32, _D]
@@ -94,8 +121,6 @@ def load():
         lib.bm_level_host.argtypes = [vp, ctypes.c_int32, _I32, _D, _D, _D, _D, _D, _D, ctypes.c_int32, _I32, _I32, _D,
                                       _D, _D, _U32]
         lib.bm_level_noalloc.argtypes = [vp, ctypes.c_int32] + [vp] * 7 + [ctypes.c_int32] + [vp] * 6
-        lib.bm_last_kernel_ms.argtypes = [vp]
-        lib.bm_last_kernel_ms.restype = ctypes.c_double
         lib.bm_set_launch.argtypes = [vp, ctypes.c_int32, ctypes.c_int32]
         lib.bm_set_colour_lds.argtypes = [vp, ctypes.c_int32]
         lib.bm_set_wide.argtypes = [vp, ctypes.c_int32]
@@ -136,6 +161,7 @@ def world_signatures(lib):
     lib.bm_world_step.argtypes = [vp, ctypes.c_double]
     lib.bm_world_read.argtypes = [vp, ctypes.POINTER(WorldState)]
     lib.bm_world_moved.argtypes = [vp, ctypes.c_int32, _I32, _I32]
+    lib.bm_world_tap.argtypes = [vp, COLUMNS_TAP, vp]
     lib.bm_world_set_hulls.argtypes = [vp, ctypes.c_int32, _I32, _I32, _I32, _D, _I32, _I32, _D]
     lib.bm_world_set_fields.argtypes = [vp, ctypes.c_int32, _I32, _I32, _D, ctypes.POINTER(ctypes.c_int64), _D]
     lib.bm_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int32, ctypes.c_int32]
@@ -179,10 +205,14 @@ class BearingError(RuntimeError):
 
 
 class Context:
-    """One bm_context on a HIP device."""
+    """One bm_context on a HIP device (or on `lib`, another library of the same calls)."""
 
-    def __init__(self, device=0):
-        self.lib = load()
+    def __init__(self, device=0, lib=None):
+        if lib is None:
+            lib = load()
+        else:
+            context_signatures(lib)
+        self.lib = lib
         self.ptr = ctypes.c_void_p()
         rc = self.lib.bm_create(ctypes.byref(self.ptr), device)
         if rc != 0:
@@ -500,6 +530,18 @@ class World:
         self.check(self.lib.bm_world_moved(self.ptr, since, out.ctypes.data_as(_I32), n.ctypes.data_as(_I32)),
                    "bm_world_moved")
         return out[:int(n[0])].copy()
+
+    def capture_next(self):
+        """Arms bm_world_tap for the next step alone: after it, `captured` holds the columns that
+        step handed the device (copy_columns), before the device changed them."""
+        self.captured = None
+
+        def tap(_user, cols):
+            self.captured = copy_columns(cols.contents)
+            self.lib.bm_world_tap(self.ptr, COLUMNS_TAP(), None)
+
+        self._tap = COLUMNS_TAP(tap)  # kept alive while the library holds it
+        self.check(self.lib.bm_world_tap(self.ptr, self._tap, None), "bm_world_tap")
 
     TIMES = ["prepare", "narrowphase_device", "narrow_host", "freeze", "contact_device", "catch_sleep"]
 

@@ -14,7 +14,10 @@ Every one of the 600 steps' transforms is checked against the reference's afterw
 `roofline` prices the step's dominant kernel, the grid-wide soft step (inertias .. store), with
 the algorithmic bytes of its sweeps (bm_world_read soft_bytes: each field once a pass, each
 body's fields once a manifold end) over its HIP-event time, against HBM's 8 TB/s.
-`cpu_baseline` is the oracle's soft step (the CPU restatement) on the host.
+`cpu_baseline` is the oracle (the CPU restatement) on the host, on the same contact phase: the
+columns the pile's last untimed step handed the device, replayed through the oracle's soft step
+with one thread and with the reference's 'auto' workers (half the threads less one); `value` is
+the workers' rate, `cores` their threads.
 
 Other legs: `batched_whole_worlds`, the batched form (4,096 worlds of scenes.batched, seed 4,
 300 steps, bm_batch_step) against bm_world_step looped per world, bit-exact against the
@@ -141,22 +144,61 @@ def micro_run(args):
     return out
 
 
-def cpu_baseline(seconds):
-    """The oracle's soft step on the host, one thread, repeated on the same input for about `seconds`."""
-    from oracle import binding
+def host_threads():
+    """(threads this process may use, as the machine reports them; the CPU model). The threads
+    are the CPUs this process is allowed to run on, no more than OMP_NUM_THREADS where it is set
+    (the share of the machine a job is given)."""
+    n = len(os.sched_getaffinity(0))
+    share = os.environ.get("OMP_NUM_THREADS")
+    if share and share.isdigit() and int(share) > 0:
+        n = min(n, int(share))
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return n, model
 
-    fx = load_world(WORKLOAD)[0]
-    base = binding.columns(fx)
-    rows = int(np.sum(fx["in.c.points"][: fx.meta["solved"]]))
-    done, elapsed = 0, 0.0
-    while elapsed < seconds and done < 1000:
-        cols = {k: v.copy() for k, v in base.items()}  # a fresh input each time, not timed
-        t = time.perf_counter()
-        binding.soft_step(fx, cols)
-        elapsed += time.perf_counter() - t
-        done += 1
-    return {"value": rows * 2 * SUBSTEPS * done / elapsed, "unit": "contact-iterations/s", "cores": 1, "kind": "port",
-            "sample": f"{done} soft steps of {WORKLOAD} ({rows} rows) in {elapsed:.2f} s on one host thread"}
+
+def cpu_baseline(captured, seconds):
+    """The CPU restatement (the oracle) timed on the same contact phase the headline steps: the
+    columns the pile's last untimed step handed the device (bm_world_tap), replayed through the
+    host library's context (tests/hostworld: the oracle answering bm_soft_step), once with one
+    thread and once with the reference's 'auto' workers, half the threads less one
+    (src/world.js:173-186), each about `seconds / 2`; the soft step's call alone is timed
+    (upload and colouring are not)."""
+    import ctypes
+
+    from worldcheck import replay
+
+    host_lib = os.path.join(ROOT, "tests", "hostworld", "_build", "libhostworld.so")
+    if captured is None or not os.path.exists(host_lib):
+        return None
+    lib = ctypes.CDLL(host_lib)
+    threads, model = host_threads()
+    auto = 0 if threads < 2 else max(1, threads // 2 - 1)
+    legs = {}
+    for name, workers in (("one_thread", 0), ("workers", auto)):
+        done, elapsed, rows = 0, 0.0, 0
+        while elapsed < seconds / 2 and done < 200:
+            out, t = replay(captured, lib=lib, workers=workers)
+            solved = int(out["solved"][0])
+            rows = int(np.sum(out["points"][:solved]))
+            elapsed += t
+            done += 1
+        legs[name] = {"value": rows * 2 * SUBSTEPS * done / elapsed, "workers": workers, "threads": workers + 1,
+                      "soft_steps": done, "seconds": elapsed, "rows": rows}
+    best = legs["workers"]
+    return {"value": best["value"], "unit": "contact-iterations/s", "cores": best["threads"], "kind": "port",
+            "sample": f"the mixed pile's contact phase at rest ({best['rows']} rows, the step before the timed "
+                      f"ones, tapped): {best['soft_steps']} oracle soft steps with {best['workers']} workers "
+                      f"(the reference's 'auto' for {threads} threads) in {best['seconds']:.2f} s; "
+                      f"one thread {legs['one_thread']['value']:.4g} contact-iterations/s",
+            "legs": legs, "nproc": os.cpu_count(), "threads_available": threads, "cpu_model": model}
 
 
 def cpu_baseline_worlds():
@@ -212,6 +254,8 @@ def pile_run(timed, warmup):
     for s in range(1, total + 1):
         if s == setup + warmup + 1:
             before = w.times()
+        if s == setup + warmup and s >= 1:
+            w.capture_next()   # this untimed step's contact phase, for the CPU baseline
         t = time.perf_counter()
         w.step(meta["dt"])
         wall = time.perf_counter() - t
@@ -225,6 +269,7 @@ def pile_run(timed, warmup):
             print(f"bench: {PILE} step {s}", file=sys.stderr, flush=True)
     all_wall = time.perf_counter() - t_all
     after = w.times()
+    captured = getattr(w, "captured", None)
     w.close()
     first = setup + warmup
     tw = float(np.sum(walls[first:]))
@@ -241,7 +286,7 @@ def pile_run(timed, warmup):
         "trajectory": {"ms_per_step": all_wall * 1000 / total, "world_steps_per_s": total / all_wall,
                        "contact_iters_per_s": int(np.sum(rows)) * 2 * SUBSTEPS / all_wall,
                        "note": "all 600 steps, the landing included, wall clock around bm_world_step and bm_world_read"},
-    }
+    }, captured
 
 
 def c4_run(threads, loop_sample=16):
@@ -508,7 +553,7 @@ def main():
     ap.add_argument("--whole-batch", type=int, default=None,
                     help="only step this many stand-in whole worlds together (bm_batch_step) and print the timing")
     ap.add_argument("--c4", action="store_true", help="only the batched form's 4,096 worlds and print the timing")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline time, both legs together")
     ap.add_argument("--adjoint-worlds", type=int, default=512, help="worlds of the differentiation form")
     ap.add_argument("--adjoint", type=int, default=None, help="only the differentiation form, with this many worlds")
     ap.add_argument("--plain-launch", default=None,
@@ -550,14 +595,14 @@ def main():
         return
 
     parts = set(args.parts.split(","))
-    pile = pile_run(args.steps, args.warmup) if "pile" in parts else None
+    pile, captured = pile_run(args.steps, args.warmup) if "pile" in parts else (None, None)
     micro = micro_run(args) if "micro" in parts else None
     c4 = c4_run(args.batch_threads) if "c4" in parts else None
     worlds_run = {name: world_run(name) for name in WORLD_RUNS} if "worlds" in parts else None
     adjoint = adjoint_run(args.adjoint_worlds) if "adjoint" in parts else None
     cpu = None
     if "cpu" in parts:
-        cpu = cpu_baseline(args.cpu_seconds)
+        cpu = cpu_baseline(captured, args.cpu_seconds)
         if worlds_run:
             for name, ms in cpu_baseline_worlds().items():
                 worlds_run[name]["cpu_port_ms_per_step"] = ms

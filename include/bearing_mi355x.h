@@ -382,6 +382,14 @@ int bm_world_moved(bm_world* world, int32_t since, int32_t* out, int32_t* count)
  * download); 5 the catch, touched, transforms and sleep. */
 #define BM_WORLD_TIMES 6
 int bm_world_times(const bm_world* world, double* out);
+/* Shows `tap` the columns each later step hands the device for its contact phase, before the
+ * device sees them: bm_columns of this world alone (one world's offset tables), as bm_upload
+ * takes them, the manifolds in the order the narrow pass left them (before colouring). The
+ * pointers hold only for the call, which runs on the stepping thread. For replaying one step's
+ * soft phase elsewhere: another context, a CPU restatement, a profiler. NULL stops it. No
+ * counterpart in the reference: for measurement and tests. */
+typedef void (*bm_columns_tap)(void* user, const bm_columns* columns);
+int bm_world_tap(bm_world* world, bm_columns_tap tap, void* user);
 
 /* ---- worlds stepped together (the batched form) ----
  *

@@ -69,6 +69,8 @@ def lib():
         _lib = ctypes.CDLL(LIB)
         _lib.oracle_soft_step.argtypes = [ctypes.POINTER(OracleSoft)]
         _lib.oracle_soft_step.restype = ctypes.c_int
+        _lib.oracle_soft_step_workers.argtypes = [ctypes.POINTER(OracleSoft), ctypes.c_int]
+        _lib.oracle_soft_step_workers.restype = ctypes.c_int
     return _lib
 
 
@@ -98,8 +100,9 @@ def columns(fx):
     return fixtures.soft_columns(fx)
 
 
-def soft_step(fx, cols=None):
-    """Runs oracle_soft_step on a fixture's input state; returns the columns after it."""
+def soft_step(fx, cols=None, workers=0):
+    """Runs oracle_soft_step on a fixture's input state (with `workers` more threads:
+    oracle_soft_step_workers); returns the columns after it."""
     meta = fx.meta
     cols = columns(fx) if cols is None else cols
     st = OracleSoft()
@@ -120,7 +123,8 @@ def soft_step(fx, cols=None):
         setattr(st, f, ptr(cols["p." + name], kind))
     st.awake_list = ptr(cols["awake_list"], _I32)
     st.starts = ptr(cols["starts"], _I32)
-    rc = lib().oracle_soft_step(ctypes.byref(st))
+    rc = (lib().oracle_soft_step_workers(ctypes.byref(st), workers) if workers
+          else lib().oracle_soft_step(ctypes.byref(st)))
     if rc != 0:
         raise RuntimeError(f"oracle_soft_step returned {rc}")
     return cols

@@ -37,6 +37,11 @@ typedef struct OracleSoft {
    a compound's, sub >= 0). */
 int oracle_soft_step(OracleSoft* w);
 
+/* oracle_soft_step with `workers` more threads (the reference's workers, src/core/jobs.js,
+   src/world.js:173-186): prepare, and each colour's warm start and solve, cut across them;
+   the rest on the calling thread. The same bits as oracle_soft_step. */
+int oracle_soft_step_workers(OracleSoft* w, int workers);
+
 /* src/graph.js:105-188: each manifold's colour (COLOURS = 32, 32 = overflow, 33 =
    resting), the rows' order (row o holds manifold order[o]), the way back (row), the
    35 run starts, and solved (the resting run's start). count manifolds between slots

@@ -14,7 +14,11 @@
 // velocities after every call of one step (tests/test_oracle_soft.py).
 #include "oracle.h"
 
+#include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <thread>
+#include <vector>
 
 #include "jsmath.h"
 
@@ -811,13 +815,113 @@ int store(OracleSoft* w) {
   return 0;
 }
 
-}  // namespace
-}  // namespace oracle
+// The reference's workers (src/core/jobs.js; the prepare, warm and solve jobs of
+// src/contact.js:1341, 1611, 1793, 2129-2146) as host threads: a parallel-for over a row
+// range, every thread the calling one included claiming the next chunk from one atomic
+// cursor. A dispatch of fewer rows than its job's least runs on the calling thread alone;
+// otherwise the range is cut into (threads * 4) chunks. Each dispatch ends at a barrier of
+// all threads, so no thread can carry one dispatch's arguments into the next. Between the
+// step's dispatches the workers spin, as the reference's do from begin() to end().
+enum Job { WARM = 0, SOLVE = 1, PREPARE = 2 };
+constexpr int LEAST[3] = {16, 4, 2};   // src/contact.js:2146
+constexpr int PER_THREAD = 4;          // src/core/jobs.js:94
 
-using namespace oracle;
+inline void relax_cpu() {
+#if defined(__x86_64__) || defined(__i386__)
+  __builtin_ia32_pause();
+#else
+  std::this_thread::yield();
+#endif
+}
 
-// src/world.js:961-1031, for a world with no joints.
-extern "C" int oracle_soft_step(OracleSoft* w) {
+class Crew {
+ public:
+  Crew(OracleSoft* w, int workers) : w_(w), threads_(workers + 1) {
+    for (int i = 0; i < workers; i++) pool_.emplace_back([this] { serve(); });
+  }
+  ~Crew() {
+    quit_.store(true, std::memory_order_release);
+    for (std::thread& t : pool_) t.join();
+  }
+  // Runs `job` over rows from..to-1, with the solve's arguments where it is a solve.
+  void run(Job job, int from, int to, const Soft* soft = nullptr, double h = 0, bool bias = false,
+           bool known = false) {
+    const int n = to - from;
+    if (n <= 0) return;
+    job_ = job; from_ = from; to_ = to; soft_ = soft; h_ = h; bias_ = bias; known_ = known;
+    if (pool_.empty() || n < LEAST[job]) {
+      rows(from, to);
+      return;
+    }
+    const int chunks = threads_ * PER_THREAD;
+    chunk_ = (n + chunks - 1) / chunks;
+    next_.store(0, std::memory_order_relaxed);
+    arrived_.store(0, std::memory_order_relaxed);
+    epoch_.fetch_add(1, std::memory_order_acq_rel);
+    claim();
+    arrived_.fetch_add(1, std::memory_order_acq_rel);
+    while (arrived_.load(std::memory_order_acquire) < threads_) relax_cpu();
+  }
+
+ private:
+  void rows(int from, int to) {
+    switch (job_) {
+      case WARM: warm_rows(w_, from, to); break;
+      case SOLVE: solve_rows(w_, *soft_, h_, bias_, known_, from, to); break;
+      case PREPARE: prepare(w_, from, to); break;
+    }
+  }
+  void claim() {
+    for (;;) {
+      const int f = from_ + next_.fetch_add(1, std::memory_order_relaxed) * chunk_;
+      if (f >= to_) return;
+      rows(f, std::min(to_, f + chunk_));
+    }
+  }
+  void serve() {
+    int seen = 0;
+    for (;;) {
+      int e;
+      while ((e = epoch_.load(std::memory_order_acquire)) == seen) {
+        if (quit_.load(std::memory_order_acquire)) return;
+        relax_cpu();
+      }
+      seen = e;
+      claim();
+      arrived_.fetch_add(1, std::memory_order_acq_rel);
+    }
+  }
+
+  OracleSoft* w_;
+  const int threads_;
+  std::vector<std::thread> pool_;
+  std::atomic<int> epoch_{0}, next_{0}, arrived_{0};
+  std::atomic<bool> quit_{false};
+  Job job_ = WARM;
+  int from_ = 0, to_ = 0, chunk_ = 1;
+  const Soft* soft_ = nullptr;
+  double h_ = 0;
+  bool bias_ = false, known_ = false;
+};
+
+// src/contact.js:1604-1613 (warmStart) and 1781-1796 (solve): colour by colour through the
+// workers, then the overflow's rows, which may share bodies, on this thread in order. With
+// no crew, every solved row in order on this thread (the rows lie colour by colour, so the
+// order is the same).
+void warm_pass(OracleSoft* w, Crew* crew) {
+  if (crew == nullptr) { warm_rows(w, 0, w->solved); return; }
+  for (int k = 0; k < COLOURS; k++) crew->run(WARM, w->starts[k], w->starts[k + 1]);
+  warm_rows(w, w->starts[OVERFLOW], w->starts[OVERFLOW + 1]);
+}
+void solve_pass(OracleSoft* w, Crew* crew, const Soft& soft, double h, bool bias, bool known) {
+  if (crew == nullptr) { solve_rows(w, soft, h, bias, known, 0, w->solved); return; }
+  for (int k = 0; k < COLOURS; k++) crew->run(SOLVE, w->starts[k], w->starts[k + 1], &soft, h, bias, known);
+  solve_rows(w, soft, h, bias, known, w->starts[OVERFLOW], w->starts[OVERFLOW + 1]);
+}
+
+// src/world.js:961-1031, for a world with no joints. Moving the bodies, the restitution pass
+// and store stay on this thread, as the reference keeps them.
+int step(OracleSoft* w, Crew* crew) {
   const bool touching = w->count > 0;
   const double h = w->dt / w->substeps;
   if (touching) inertias(w);
@@ -828,15 +932,16 @@ extern "C" int oracle_soft_step(OracleSoft* w) {
   }
   Soft soft;
   if (touching) {
-    prepare(w, 0, w->solved);
+    if (crew != nullptr) crew->run(PREPARE, 0, w->solved);
+    else prepare(w, 0, w->solved);
     soft = make_soft(STIFFNESS / h, DAMPING_RATIO, h);
   }
   for (int sub = 0; sub < w->substeps; sub++) {
     accelerate(w, h);
-    if (touching) warm_rows(w, 0, w->solved);
-    if (touching) solve_rows(w, soft, h, true, sub > 0, 0, w->solved);
+    if (touching) warm_pass(w, crew);
+    if (touching) solve_pass(w, crew, soft, h, true, sub > 0);
     move(w, h);
-    if (touching) solve_rows(w, soft, h, false, false, 0, w->solved);
+    if (touching) solve_pass(w, crew, soft, h, false, false);
   }
   if (touching) {
     const double* g = w->gravity;
@@ -845,4 +950,17 @@ extern "C" int oracle_soft_step(OracleSoft* w) {
   }
   (void)RESTING;
   return 0;
+}
+
+}  // namespace
+}  // namespace oracle
+
+using namespace oracle;
+
+extern "C" int oracle_soft_step(OracleSoft* w) { return step(w, nullptr); }
+
+extern "C" int oracle_soft_step_workers(OracleSoft* w, int workers) {
+  if (workers <= 0) return step(w, nullptr);
+  Crew crew(w, workers);
+  return step(w, &crew);
 }

@@ -197,6 +197,9 @@ int bm_colour(bm_context* ctx) {
 }
 }  // extern "C"
 
+// More threads for the oracle's soft step (oracle_soft_step_workers), 0 for none.
+static int soft_workers = 0;
+
 namespace {
 int soft_one(bm_context* ctx, bm_columns& h) {
   OracleSoft o{};
@@ -230,7 +233,7 @@ int soft_one(bm_context* ctx, bm_columns& h) {
   o.spin_mass = h.spin_mass; o.pid = h.pid;
   o.pair_points = h.pair_points; o.pair_pid = h.pair_pid; o.pair_normal_impulse = h.pair_normal_impulse;
   o.pair_twist = h.pair_twist; o.pair_friction = h.pair_friction; o.pair_slip = h.pair_slip;
-  if (oracle_soft_step(&o) != 0) {
+  if (oracle_soft_step_workers(&o, soft_workers) != 0) {
     ctx->error = "oracle soft step: a manifold it does not cover";
     return BM_ERR_UNSUPPORTED;
   }
@@ -244,6 +247,7 @@ static int fail_next_soft_step = 0;
 
 extern "C" {
 void stub_fail_next_soft_step() { fail_next_soft_step = 1; }
+void stub_soft_workers(int workers) { soft_workers = workers > 0 ? workers : 0; }
 
 int bm_soft_step(bm_context* ctx) {
   if (fail_next_soft_step) {

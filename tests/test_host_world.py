@@ -6,6 +6,7 @@ import ctypes
 import os
 import subprocess
 
+import numpy as np
 import pytest
 
 import fixtures
@@ -117,3 +118,43 @@ def test_host_batch_refuses_steps_after_one_fails_part_way(lib):
         batch.step(fx.meta["dt"])
     assert "no more steps" in lib.bm_world_error(worlds[1].ptr).decode()
     batch.close()
+
+
+@pytest.mark.parametrize("name", ["pile_small", "rain", "pyramid"])
+def test_host_world_with_workers_matches_reference_trajectory(lib, name):
+    # The oracle's soft step cut across 3 more threads (oracle_soft_step_workers), as the
+    # reference's workers cut it: the same trajectory bit for bit.
+    fx = traj(name)
+    w = make_world(fx, lib=lib)
+    lib.stub_soft_workers(3)
+    try:
+        matched, why = run(w, fx)
+    finally:
+        lib.stub_soft_workers(0)
+    assert why is None, why
+    assert matched == fx.meta["steps"]
+
+
+def test_tapped_columns_replay_the_step(lib):
+    # bm_world_tap shows the columns a step hands its contact phase: replayed through a
+    # context of their own (upload, colour, soft step, download) they give the world's own
+    # positions and rotations after that step, with any number of workers.
+    from worldcheck import replay
+    fx = traj("pile_small")
+    w = make_world(fx, lib=lib)
+    for _ in range(69):
+        w.step(fx.meta["dt"])
+    w.capture_next()
+    w.step(fx.meta["dt"])
+    cap = w.captured
+    assert cap is not None and len(cap["points"]) > 30
+    after = w.read()
+    w.step(fx.meta["dt"])
+    assert w.captured is None or w.captured is cap  # the tap is off after one step
+    out, _ = replay(cap, lib=lib)
+    assert after["catches"] == 0
+    assert np.array_equal(out["position"], after["position"])
+    assert np.array_equal(out["rotation"], after["rotation"])
+    out3, _ = replay(cap, lib=lib, workers=3)
+    for key in out:
+        assert np.array_equal(out[key], out3[key], equal_nan=True), key

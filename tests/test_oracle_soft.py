@@ -35,3 +35,18 @@ def test_a_planted_mistake_is_caught(column, index):
     a[index] = a[index] * (1 + 2.0 ** -30) if a[index] != 0 else 2.0 ** -30
     got = binding.soft_step(fx, cols)
     assert compare(fx, got), f"an ulp in {column}[{index}] went unnoticed"
+
+
+@pytest.mark.parametrize("workers", [1, 3, 7])
+@pytest.mark.parametrize("path", SOFT, ids=[p.split("/")[-1].replace(".bmfx.gz", "") for p in SOFT])
+def test_soft_step_with_workers_matches_reference(path, workers):
+    """The colour-parallel soft step (oracle_soft_step_workers: prepare and each colour's warm
+    start and solve cut across threads, as the reference's workers cut them) leaves the
+    reference's bits, and the single thread's, whatever the number of workers."""
+    fx = fixtures.load(path)
+    one = binding.soft_step(fx)
+    got = binding.soft_step(fx, workers=workers)
+    errs = compare(fx, got)
+    assert not errs, "\n".join(errs)
+    for key in one:
+        assert np.array_equal(one[key], got[key], equal_nan=True), key

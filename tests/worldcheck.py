@@ -158,3 +158,30 @@ def fill_world(w, fx):
         w.set_fields(cols, rows, fx["fields.spacing"], heights)
     w.add(init_columns(fx))
     return w
+
+
+def replay(captured, device=0, lib=None, workers=0):
+    """One step's tapped device-phase columns (World.capture_next) through a context's
+    bm_upload, bm_colour, bm_soft_step and bm_download: on `device`, or on `lib` (the host
+    library, the oracle answering, with `workers` more threads); the columns after, by name,
+    and the soft step's seconds (the call alone)."""
+    import time
+    from bearing_amd import native
+    ctx = native.Context(device, lib=lib)
+    batch = native.Batch([{k: v.copy() for k, v in captured.items()}])
+    if lib is not None:
+        lib.stub_soft_workers(workers)
+    try:
+        batch.upload(ctx)
+        batch.colour(ctx)
+        t = time.perf_counter()
+        batch.step(ctx)
+        seconds = time.perf_counter() - t
+        batch.download(ctx)
+    finally:
+        if lib is not None:
+            lib.stub_soft_workers(0)
+        ctx.close()
+    out = batch.world(0)
+    out["solved"], out["starts"] = batch.arrays["solved"].copy(), batch.arrays["starts"].copy()
+    return out, seconds
