@@ -465,7 +465,7 @@ struct HostCopy {
 void run_copies(const std::vector<HostCopy>& copies) {
   size_t total = 0;
   for (const HostCopy& c : copies) total += c.bytes;
-  const int threads = total >= ((size_t)64 << 20) ? std::max(1, std::min(16, (int)std::thread::hardware_concurrency())) : 1;
+  const int threads = total >= ((size_t)4 << 20) ? std::max(1, std::min(16, (int)std::thread::hardware_concurrency())) : 1;
   if (threads <= 1) {
     for (const HostCopy& c : copies) std::memcpy(c.to, c.from, c.bytes);
     return;

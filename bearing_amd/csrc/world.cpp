@@ -305,6 +305,16 @@ struct Manifold {
   double loss = 0;
 };
 
+// A row a narrow pass emitted, its columns filled when the pass stops (fill_rows): the
+// manifold (a visit's rebuilt one, or a level batch's output slot), the pair's ends and
+// numbers, and where its warm start comes from (the pair list, or row `kept` of a level
+// pair's Memory, -1 for none).
+struct RowFill {
+  const Manifold* mf;  // null for a level row
+  int m, p, a, b, sub, level_at, kept, shift_at;
+  double tolerance, sign;
+};
+
 // A pair visited by narrow(): what find() decided for it, in the order visited.
 // Most visits of a settled pile find the boxes apart, so a visit is only its decision; a kept
 // manifold is rebuilt into the pass's `rebuilt` row of the same number.
@@ -462,6 +472,7 @@ struct bm_world {
   std::vector<Visit> visits;  // the pass's visits are the first visit_count (the vector never shrinks)
   std::vector<Manifold, Unzeroed<Manifold>> rebuilt;  // by visit: a kind-1 visit's kept manifold
   std::vector<int32_t> live;  // the visits the serial walk takes, in order (the rest done on the pool)
+  std::vector<RowFill> fills;  // rows emitted since the pass last stopped, columns not yet filled
   size_t live_count = 0;
   size_t visit_count = 0;
   std::vector<uint8_t> apart;  // a large pass's pairs whose boxes no longer overlap, by slot
@@ -978,11 +989,17 @@ void basis(Contacts& c, int m) {
 
 // src/contact.js:889-921 (emit), sign 1, no sub-manifold; `shift` (3 a point) for a
 // manifold met at the moment of impact, else null.
-int emit(bm_world* w, int p, int a, int b, const Manifold& mf, double tolerance, const double* shift, int sub = -1,
-         double sign = 1) {
+// In two halves: the row's number, taken in the pass's order (emit_slot), and its columns
+// (emit_fill), which read only the manifold and the bodies' materials and write only row m.
+int emit_slot(bm_world* w) {
   Contacts& c = w->c;
   const int m = c.count++;
   c.ensure(c.count);
+  return m;
+}
+void emit_fill(bm_world* w, int m, int p, int a, int b, const Manifold& mf, double tolerance, const double* shift,
+               int sub, double sign) {
+  Contacts& c = w->c;
   c.a[m] = a; c.b[m] = b; c.pair[m] = p; c.sub[m] = sub;
   c.points[m] = (uint8_t)mf.count;
   for (int k = 0; k < 3; k++) c.normal[3 * m + k] = sign * mf.normal[k];
@@ -1002,7 +1019,6 @@ int emit(bm_world* w, int p, int a, int b, const Manifold& mf, double tolerance,
     c.normal_impulse[i] = 0;
   }
   basis(c, m);
-  return m;
 }
 
 // src/contact.js:999-1036 (carry) from the pair list, or from a row of a level pair's Memory.
@@ -1044,6 +1060,47 @@ void carry_from(bm_world* w, const Kept& pr, int p, int m) {
 }
 
 void carry(bm_world* w, int p, int m) { carry_from(w, w->pairs, p, m); }
+
+// A level batch's output slot o as a manifold.
+void level_manifold(const bm_world* w, size_t o, Manifold& mf) {
+  mf.count = w->level_count[o];
+  for (int i = 0; i < 3; i++) mf.normal[i] = w->level_normal[3 * o + i];
+  for (int i = 0; i < 12; i++) mf.points[i] = w->level_points[12 * o + i];
+  for (int i = 0; i < 4; i++) mf.separations[i] = w->level_seps[4 * o + i];
+  for (int i = 0; i < 4; i++) mf.ids[i] = w->level_ids[4 * o + i];
+}
+
+// One emitted row's columns and warm start (src/contact.js:889-921 emit, 999-1036 carry).
+void fill_row(bm_world* w, const RowFill& f) {
+  if (f.mf) {
+    const double* shift = f.shift_at >= 0 ? &w->batch_shift[12 * (size_t)f.shift_at] : nullptr;
+    emit_fill(w, f.m, f.p, f.a, f.b, *f.mf, f.tolerance, shift, f.sub, f.sign);
+    carry(w, f.p, f.m);
+    return;
+  }
+  Manifold mf;
+  level_manifold(w, (size_t)f.level_at, mf);
+  emit_fill(w, f.m, f.p, f.a, f.b, mf, f.tolerance, nullptr, f.sub, f.sign);
+  if (f.kept >= 0) carry_from(w, w->pairs.memory_of(f.p), f.kept, f.m);
+}
+
+// The rows emitted since the pass last stopped: each writes only its own row and reads what
+// no visit changes while the pass is taken (the batches' outputs, the bodies' materials, the
+// pair's own memory), so a large set is filled on the pool in any order.
+constexpr int PARALLEL_FILLS = 4096;
+void fill_rows(bm_world* w) {
+  std::vector<RowFill>& fills = w->fills;
+  const int n = (int)fills.size();
+  if (w->pool && n >= PARALLEL_FILLS) {
+    const int chunk = 1024, chunks = (n + chunk - 1) / chunk;
+    w->pool->run(chunks, [&](int c) {
+      for (int i = c * chunk, e = std::min(n, (c + 1) * chunk); i < e; i++) fill_row(w, fills[i]);
+    });
+  } else {
+    for (const RowFill& f : fills) fill_row(w, f);
+  }
+  fills.clear();
+}
 
 // A body record for the device meet (src/contact.js:1244-1276 load, and what moving()
 // and src/impact.js start read): the shape (kind, centre 3, axes 9, half 3, weight 3),
@@ -1171,28 +1228,23 @@ int finish_level(bm_world* w, Visit& v) {
   Memory& kept = w->pairs.memory_of(p);
   for (int j = 0; j < kept.count; j++) kept.taken[j] = 0;
   if (n > 0 && w->awake[a] != w->awake[b]) wake(w, w->awake[a] ? b : a);
-  const Contacts& c = w->c;
+  const double sign = turned ? -1 : 1;
   int sub = 0;
   for (int q = 0; q < n; q++) {
     const size_t o = (size_t)k * cap + q;
-    Manifold mf;
-    mf.count = w->level_count[o];
-    for (int i = 0; i < 3; i++) mf.normal[i] = w->level_normal[3 * o + i];
-    for (int i = 0; i < 12; i++) mf.points[i] = w->level_points[12 * o + i];
-    for (int i = 0; i < 4; i++) mf.separations[i] = w->level_seps[4 * o + i];
-    for (int i = 0; i < 4; i++) mf.ids[i] = w->level_ids[4 * o + i];
-    const int m = emit(w, p, a, b, mf, v.tolerance, nullptr, sub++, turned ? -1 : 1);
+    // The row's normal as emit_fill writes it, for the match against what the pair kept.
+    const double nx = sign * w->level_normal[3 * o], ny = sign * w->level_normal[3 * o + 1];
+    const double nz = sign * w->level_normal[3 * o + 2];
+    const int m = emit_slot(w);
     int best = -1;
     double most = 0;
     for (int j = 0; j < kept.count; j++) {
       if (kept.taken[j] || kept.key[j] != 0) continue;
-      const double d = kept.normal[3 * j] * c.normal[3 * m] + kept.normal[3 * j + 1] * c.normal[3 * m + 1]
-          + kept.normal[3 * j + 2] * c.normal[3 * m + 2];
+      const double d = kept.normal[3 * j] * nx + kept.normal[3 * j + 1] * ny + kept.normal[3 * j + 2] * nz;
       if (d > most) { most = d; best = j; }
     }
-    if (best < 0) continue;
-    kept.taken[best] = 1;
-    carry_from(w, kept, best, m);
+    if (best >= 0) kept.taken[best] = 1;
+    w->fills.push_back(RowFill{nullptr, m, p, a, b, sub++, (int)o, best, -1, v.tolerance, sign});
   }
   if (sub == 0) w->pairs.forget(p);
   return BM_OK;
@@ -1260,7 +1312,7 @@ int finish_visit(bm_world* w, Visit& v, Manifold& m) {
     return BM_OK;
   }
   if (v.kind == 3) return finish_level(w, v);
-  const double* shift = nullptr;
+  int shift_at = -1;
   if (v.kind == 2) {
     const int k = v.slot;
     m.count = w->batch_count[k];
@@ -1278,14 +1330,13 @@ int finish_visit(bm_world* w, Visit& v, Manifold& m) {
     if (w->batch_flags[k] & 1) {
       // A manifold met at the moment of impact is not kept (src/contact.js:541).
       if (w->pairs.cache[v.p] != 0) w->pairs.uncache(v.p);
-      shift = &w->batch_shift[12 * k];
+      shift_at = k;
     } else {
       remember(w, v.p, v.a, v.b, m);
     }
   }
   if (w->awake[v.a] != w->awake[v.b]) wake(w, w->awake[v.a] ? v.b : v.a);
-  const int mi = emit(w, v.p, v.a, v.b, m, v.tolerance, shift);
-  carry(w, v.p, mi);
+  w->fills.push_back(RowFill{&m, emit_slot(w), v.p, v.a, v.b, -1, -1, -1, shift_at, v.tolerance, 1});
   return BM_OK;
 }
 
@@ -1424,10 +1475,17 @@ bool narrow_resume(bm_world* w, int& rc) {
     if (!v.again && (w->fresh[v.a] || w->fresh[v.b])) {
       v.again = true;
       prepare_visit(w, v, m);
-      if (v.kind >= 2) return true;
+      if (v.kind >= 2) {
+        fill_rows(w);  // before the device runs the batches again
+        return true;
+      }
     }
-    if ((rc = finish_visit(w, v, m)) != BM_OK) return false;
+    if ((rc = finish_visit(w, v, m)) != BM_OK) {
+      w->fills.clear();
+      return false;
+    }
   }
+  fill_rows(w);
   for (int s : w->woken) w->fresh[s] = 0;
   w->woken.clear();
   return false;
