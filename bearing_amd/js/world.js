@@ -4,7 +4,10 @@
  * (bearing_amd/_build/bearing_napi.node, over the C ABI's bm_world_*): the reference's
  * per-step surface, new World(options) (src/world.js:314), addBody (:563), step(dt)
  * (:880), transforms and previous (:547, :554: 7 floats a slot, position then rotation),
- * stepCount (:526) and moved(since) (:771). Plain CommonJS that Node 12.22 runs.
+ * stepCount (:526) and moved(since) (:771). Bodies are boxes, spheres, capsules, static
+ * heightfields and hulls given as built (resolve.js); the world keeps every hull body's
+ * moved hull and every heightfield in the tables the device reads. Plain CommonJS that
+ * Node 12.22 runs.
  *
  * There is no host fallback: without the addon, or without a GPU, the constructor throws.
  */
@@ -15,6 +18,41 @@ const ADDON = path.join(__dirname, '..', '_build', 'bearing_napi.node');
 
 function fail(caller, message) {
   throw new Error(`${caller}: ${message}`);
+}
+
+/* Hulls (resolve.js's moved hulls, in hull number order) joined into bm_world_set_hulls's layout. */
+function joinHulls(hulls) {
+  let V = 0, F = 0, L = 0;
+  for (const h of hulls) { V += h.vertices.length / 3; F += h.start.length - 1; L += h.loops.length; }
+  const n = hulls.length;
+  const t = {
+    hullV: new Int32Array(n + 1), hullF: new Int32Array(n + 1), hullL: new Int32Array(n + 1),
+    vertices: new Float64Array(3 * V), start: new Int32Array(F), loops: new Int32Array(L), planes: new Float64Array(4 * F),
+  };
+  V = 0; F = 0; L = 0;
+  hulls.forEach((h, k) => {
+    const faces = h.start.length - 1;
+    t.vertices.set(h.vertices, 3 * V);
+    t.start.set(Int32Array.from(h.start).subarray(0, faces), F);
+    t.loops.set(h.loops, L);
+    t.planes.set(h.planes, 4 * F);
+    V += h.vertices.length / 3; F += faces; L += h.loops.length;
+    t.hullV[k + 1] = V; t.hullF[k + 1] = F; t.hullL[k + 1] = L;
+  });
+  return t;
+}
+
+/* Heightfields ({columns, rows, spacing, heights}, in field number order) joined into bm_world_set_fields's layout. */
+function joinFields(fields) {
+  const n = fields.length;
+  const t = { columns: new Int32Array(n), rows: new Int32Array(n), spacing: new Float64Array(n), offsets: new Float64Array(n + 1) };
+  fields.forEach((f, k) => {
+    t.columns[k] = f.columns; t.rows[k] = f.rows; t.spacing[k] = f.spacing;
+    t.offsets[k + 1] = t.offsets[k] + f.heights.length;
+  });
+  t.heights = new Float64Array(t.offsets[n]);
+  fields.forEach((f, k) => t.heights.set(f.heights, t.offsets[k]));
+  return t;
 }
 
 class World {
@@ -33,6 +71,8 @@ class World {
     this._transforms = new Float32Array(0);
     this._previous = new Float32Array(0);
     this._movedOut = new Int32Array(0);
+    this._hulls = [];
+    this._fields = [];
   }
 
   /* Adds one body; returns its slot. */
@@ -44,7 +84,16 @@ class World {
   addBodies(list) {
     this._live('addBody');
     if (!Array.isArray(list)) fail('addBody', 'bodies must be an array of options');
-    const cols = resolve('addBody', list);
+    const cols = resolve('addBody', list, this._hulls.length, this._fields.length);
+    // The tables first: a body is refused if it names a hull or field the world lacks.
+    if (cols.hulls.length > 0) {
+      for (const h of cols.hulls) this._hulls.push(h);
+      this._native.setHulls(this._world, joinHulls(this._hulls));
+    }
+    if (cols.fields.length > 0) {
+      for (const f of cols.fields) this._fields.push(f);
+      this._native.setFields(this._world, joinFields(this._fields));
+    }
     this._native.add(this._world, cols);
     const first = this._top;
     this._sync();
@@ -141,4 +190,4 @@ class World {
   }
 }
 
-module.exports = { World, resolve };
+module.exports = { World, resolve, joinHulls, joinFields };

@@ -188,6 +188,92 @@ static napi_value js_add(napi_env env, napi_callback_info info) {
   return NULL;
 }
 
+/* Typed array `name` of obj, of type `want`: its data and length; false with an Error thrown. */
+static bool array_of(napi_env env, napi_value obj, const char* caller, const char* name, napi_typedarray_type want,
+                     void** out, size_t* length) {
+  napi_value v, buffer;
+  bool is = false;
+  napi_typedarray_type got;
+  size_t offset = 0;
+  if (napi_get_named_property(env, obj, name, &v) != napi_ok || napi_is_typedarray(env, v, &is) != napi_ok || !is ||
+      napi_get_typedarray_info(env, v, &got, length, out, &buffer, &offset) != napi_ok || got != want) {
+    char text[160];
+    snprintf(text, sizeof text, "%s must be a typed array of its column's type", name);
+    fail(env, caller, text);
+    return false;
+  }
+  return true;
+}
+
+/* setHulls(world, table): the hull bodies' hulls in their frames, table = {hullV, hullF, hullL,
+ * start, loops (Int32Array), vertices, planes (Float64Array)}, n = hullV.length - 1 hulls, laid
+ * out as bm_world_set_hulls takes them. */
+static napi_value js_set_hulls(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  if (argc < 2) return fail(env, "addBody", "setHulls(world, table)");
+  holder* h = world_of(env, argv[0], "addBody");
+  if (!h) return NULL;
+  void *hv, *hf, *hl, *st, *lp, *vx, *pl;
+  size_t nv, nf, nl, n1, n2, n3, n4;
+  if (!array_of(env, argv[1], "addBody", "hullV", napi_int32_array, &hv, &nv) ||
+      !array_of(env, argv[1], "addBody", "hullF", napi_int32_array, &hf, &nf) ||
+      !array_of(env, argv[1], "addBody", "hullL", napi_int32_array, &hl, &nl) ||
+      !array_of(env, argv[1], "addBody", "start", napi_int32_array, &st, &n1) ||
+      !array_of(env, argv[1], "addBody", "loops", napi_int32_array, &lp, &n2) ||
+      !array_of(env, argv[1], "addBody", "vertices", napi_float64_array, &vx, &n3) ||
+      !array_of(env, argv[1], "addBody", "planes", napi_float64_array, &pl, &n4))
+    return NULL;
+  if (nv < 1 || nf != nv || nl != nv) return fail(env, "addBody", "hullV, hullF and hullL must each hold n + 1 offsets");
+  const int32_t n = (int32_t)nv - 1;
+  const int32_t* V = (const int32_t*)hv;
+  const int32_t* F = (const int32_t*)hf;
+  const int32_t* L = (const int32_t*)hl;
+  if ((size_t)V[n] * 3 > n3 || (size_t)F[n] * 4 > n4 || (size_t)F[n] > n1 || (size_t)L[n] > n2)
+    return fail(env, "addBody", "the hull offsets run past the tables");
+  if (bm_world_set_hulls(h->world, n, V, F, L, (const double*)vx, (const int32_t*)st, (const int32_t*)lp,
+                         (const double*)pl) != BM_OK)
+    return fail(env, "addBody", bm_world_error(h->world));
+  return NULL;
+}
+
+/* setFields(world, table): the heightfields, table = {columns, rows (Int32Array), spacing
+ * (Float64Array), offsets (Float64Array of whole numbers, n + 1: field f's heights at
+ * offsets[f] ..), heights (Float64Array)}, as bm_world_set_fields takes them. */
+static napi_value js_set_fields(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  if (argc < 2) return fail(env, "addBody", "setFields(world, table)");
+  holder* h = world_of(env, argv[0], "addBody");
+  if (!h) return NULL;
+  void *cl, *rw, *sp, *of, *ht;
+  size_t nc, nr, ns, no, nh;
+  if (!array_of(env, argv[1], "addBody", "columns", napi_int32_array, &cl, &nc) ||
+      !array_of(env, argv[1], "addBody", "rows", napi_int32_array, &rw, &nr) ||
+      !array_of(env, argv[1], "addBody", "spacing", napi_float64_array, &sp, &ns) ||
+      !array_of(env, argv[1], "addBody", "offsets", napi_float64_array, &of, &no) ||
+      !array_of(env, argv[1], "addBody", "heights", napi_float64_array, &ht, &nh))
+    return NULL;
+  if (nr != nc || ns != nc || no != nc + 1) return fail(env, "addBody", "a heightfield table must hold n columns, rows and spacings and n + 1 offsets");
+  int64_t* offsets = (int64_t*)malloc(sizeof(int64_t) * no);
+  if (!offsets) return fail(env, "addBody", "out of memory");
+  const double* o = (const double*)of;
+  for (size_t i = 0; i < no; i++) {
+    if (!(o[i] >= 0 && o[i] <= (double)nh && o[i] == (double)(int64_t)o[i])) {
+      free(offsets);
+      return fail(env, "addBody", "a heightfield's offsets must be whole numbers within its heights");
+    }
+    offsets[i] = (int64_t)o[i];
+  }
+  const int rc = bm_world_set_fields(h->world, (int32_t)nc, (const int32_t*)cl, (const int32_t*)rw, (const double*)sp,
+                                     offsets, (const double*)ht);
+  free(offsets);
+  if (rc != BM_OK) return fail(env, "addBody", bm_world_error(h->world));
+  return NULL;
+}
+
 /* step(world, dt) */
 static napi_value js_step(napi_env env, napi_callback_info info) {
   size_t argc = 2;
@@ -315,7 +401,7 @@ NAPI_MODULE_INIT() {
   static const struct {
     const char* name;
     napi_callback fn;
-  } fns[] = {{"create", js_create}, {"add", js_add}, {"step", js_step}, {"read", js_read}, {"moved", js_moved}, {"times", js_times}, {"destroy", js_destroy}};
+  } fns[] = {{"create", js_create}, {"add", js_add}, {"setHulls", js_set_hulls}, {"setFields", js_set_fields}, {"step", js_step}, {"read", js_read}, {"moved", js_moved}, {"times", js_times}, {"destroy", js_destroy}};
   for (size_t i = 0; i < sizeof fns / sizeof fns[0]; i++) {
     napi_value f;
     if (napi_create_function(env, fns[i].name, NAPI_AUTO_LENGTH, fns[i].fn, NULL, &f) != napi_ok ||

@@ -445,6 +445,14 @@ def run_impact(seed=2026, n=2000):
     os.remove(path)
 
 
+def run_built(name):
+    """The reference's built hulls (src/hull.js build()) of scene `name`'s hull bodies, from its
+    scene file in data/scenes, into data/<name>.built.bmfx.gz."""
+    subprocess.run(["node", "--no-warnings", "--experimental-loader", os.path.join(HERE, "lower.mjs"),
+                    os.path.join(HERE, "make_built.mjs"), os.path.join(DATA, "scenes", f"{name}.json"),
+                    os.path.join(DATA, f"{name}.built.bmfx.gz")], check=True)
+
+
 def batched_sample(worlds):
     """The worlds of the batched scenes whose every step is recorded: of the box towers (even
     worlds) and of the sphere piles (odd worlds) each, the smallest, the median and the
@@ -533,6 +541,10 @@ if __name__ == "__main__":
         sys.exit(0)
     if sys.argv[1:] == ["impact"]:
         run_impact()
+        sys.exit(0)
+    if sys.argv[1:2] == ["built"]:
+        for name in sys.argv[2:] or ["rocks", "terrain"]:
+            run_built(name)
         sys.exit(0)
     for name in sys.argv[1:] or [j for j in JOBS if j not in ("rain", "mixed_pile")]:
         run(name)
