@@ -146,10 +146,14 @@ struct bm_context {
   // path the last bm_colour took (0 HBM fixed point, 1 in LDS, 2 streamed and sorted).
   Scratch colour_sort;
   Scratch wide_bar;  // the grid-wide soft step's barrier counters (bm::WIDE_WORDS), zeroed once
+  Scratch wide_pick;  // the grid-wide soft step's rows grouped by path, an int a manifold
   int last_colour_path = 0;
   // A single world of at least wide_min manifolds takes the grid-wide soft step (0: never);
   // last_wide, the workgroups the last step used (0: one a world).
   int32_t wide_min = 16384, last_wide = 0;
+  // Grid-wide launches the device refused (the step went the one-workgroup way), and why.
+  int64_t wide_refused = 0;
+  std::string wide_error;
   // The soft step's own columns stay on the device (bm_set_device_only); the arena's bytes
   // up to them, which upload and download move.
   bool keep_device_only = false;
@@ -401,7 +405,7 @@ int bm_destroy(bm_context* ctx) {
                      &ctx->pairs_out})
     if (s->p) (void)hipFree(s->p);
   for (Scratch* s : {&ctx->hull_ints, &ctx->hull_doubles, &ctx->field_ints, &ctx->field_doubles, &ctx->level_scratch,
-                     &ctx->colour_sort, &ctx->wide_bar})
+                     &ctx->colour_sort, &ctx->wide_bar, &ctx->wide_pick})
     if (s->p) (void)hipFree(s->p);
   bm::held_free(ctx->held);
   if (ctx->begin) (void)hipEventDestroy(ctx->begin);
@@ -716,9 +720,21 @@ int bm_soft_step(bm_context* ctx) {
       if (int rc = grow(ctx, ctx->wide_bar, bm::WIDE_WORDS * sizeof(unsigned), "bm_soft_step")) return rc;
       HIP_OR_FAIL(ctx, hipMemset(ctx->wide_bar.p, 0, ctx->wide_bar.cap), "bm_soft_step: hipMemset");
     }
-    ctx->last_wide = bm::launch_soft_wide(ctx->dev, 0, work, static_cast<unsigned*>(ctx->wide_bar.p), ctx->device,
-                                          ctx->stream);
-    if (ctx->last_wide < 0) return fail(ctx, BM_ERR_DEVICE, "bm_soft_step: the grid-wide launch failed");
+    if (int rc = grow(ctx, ctx->wide_pick, sizeof(int) * (size_t)std::max(ctx->manifolds, 1), "bm_soft_step")) return rc;
+    const hipError_t e = bm::launch_soft_wide(ctx->dev, 0, work, static_cast<unsigned*>(ctx->wide_bar.p),
+                                              static_cast<int*>(ctx->wide_pick.p), ctx->device, ctx->stream,
+                                              &ctx->last_wide);
+    if (e != hipSuccess) {
+      // A cooperative launch can be refused while another process holds the device's
+      // cooperative resources: the step then takes the one-workgroup path, with the same bits.
+      (void)hipGetLastError();
+      if (ctx->wide_refused++ == 0)
+        std::fprintf(stderr, "bearing_amd: the grid-wide soft step was refused (%s); this context steps its worlds "
+                     "one workgroup each while it is\n", hipGetErrorString(e));
+      ctx->wide_error = hipGetErrorString(e);
+      ctx->last_wide = 0;
+      bm::launch_soft_step(ctx->dev, ctx->n_worlds, ctx->shape, resident ? lds : 0, ctx->stream);
+    }
   } else {
     ctx->last_wide = 0;
     bm::launch_soft_step(ctx->dev, ctx->n_worlds, ctx->shape, resident ? lds : 0, ctx->stream);

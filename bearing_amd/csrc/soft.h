@@ -45,7 +45,8 @@ void launch_soft_step(const SoftDev& d, int grid, int shape, size_t lds_bytes, h
 // launch leaves them so); launches sharing them must not overlap.
 constexpr int WIDE_GROUPS = 8, WIDE_STRIDE = 32;  // counters 128 bytes apart
 constexpr int WIDE_WORDS = WIDE_STRIDE * (WIDE_GROUPS + 2);
-int launch_soft_wide(const SoftDev& d, int wld, int work, unsigned* words, int device, hipStream_t stream);
+hipError_t launch_soft_wide(const SoftDev& d, int wld, int work, unsigned* words, int* pick, int device,
+                            hipStream_t stream, int* blocks_out);
 size_t soft_step_lds_limit(int device);
 
 }  // namespace bm

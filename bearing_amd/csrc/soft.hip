@@ -1066,14 +1066,81 @@ struct WideBar {
 // One sweep over the world's colours across the whole grid: each colour's manifolds spread
 // over every thread of every workgroup, a grid barrier after each; the overflow in order by
 // one thread.
+// Which path through row_of a solved row takes: the instance for its point count.
+__device__ __forceinline__ int path_of(const SoftDev& d, int m) {
+  const int n = d.points[m];
+  return n < 1 ? 0 : n > 4 ? 3 : n - 1;
+}
+constexpr int PATHS = 4;
+
+// Colour c's rows grouped by path into pick[from .. to), each group in row order: a wave then
+// takes one path through row_of instead of each of its rows' paths one after another. The rows
+// of a colour share no moving body, so the order threads take them in changes no result.
+// Workgroup-wide, stable: a chunk of THREADS rows at a time, each path's rows placed by a
+// ballot within the wave and a prefix over the waves.
+template <int THREADS>
+__device__ __forceinline__ void group_colour(const SoftDev& d, int moff, int from, int to, int* pick) {
+  __shared__ int counts[THREADS / 64], cursor[PATHS];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (threadIdx.x < PATHS) cursor[threadIdx.x] = 0;
+  __syncthreads();
+  // First the size of each path, so each group starts where the ones before it end.
+  for (int s = from; s < to; s += THREADS) {
+    const int i = s + (int)threadIdx.x;
+    const int path = i < to ? path_of(d, moff + i) : -1;
+    for (int k = 0; k < PATHS; k++) {
+      const unsigned long long mask = __ballot(path == k);
+      if (lane == 0 && mask) atomicAdd(&cursor[k], __popcll(mask));
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int at = from;
+    for (int k = 0; k < PATHS; k++) {
+      const int n = cursor[k];
+      cursor[k] = at;
+      at += n;
+    }
+  }
+  __syncthreads();
+  for (int s = from; s < to; s += THREADS) {
+    const int i = s + (int)threadIdx.x;
+    const int path = i < to ? path_of(d, moff + i) : -1;
+    for (int k = 0; k < PATHS; k++) {
+      const unsigned long long mask = __ballot(path == k);
+      if (lane == 0) counts[wave] = __popcll(mask);
+      __syncthreads();
+      int before = cursor[k];
+      for (int w = 0; w < wave; w++) before += counts[w];
+      if (path == k) pick[moff + before + __popcll(mask & ((1ull << lane) - 1))] = i;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        int total = 0;
+        for (int w = 0; w < THREADS / 64; w++) total += counts[w];
+        cursor[k] += total;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// World wld's colours grouped for the grid-wide soft step, workgroup c taking colour c.
+template <int THREADS>
+__global__ void __launch_bounds__(THREADS) group_kernel(SoftDev d, int wld, int* pick) {
+  const int moff = d.manifold_offset[wld];
+  const int c = blockIdx.x;
+  const int from = d.starts[BM_STARTS_DEV * wld + c], to = d.starts[BM_STARTS_DEV * wld + c + 1];
+  if (from < to) group_colour<THREADS>(d, moff, from, to, pick);
+}
+
 template <int KIND>
 __device__ __forceinline__ void sweep_wide(const WideBar& grid, const SoftDev& d, const VW& vw,
                                            const Soft& soft, double h, bool known, int moff, int boff,
-                                           const int* starts, int tid, int nth) {
+                                           const int* starts, const int* pick, int tid, int nth) {
   for (int c = 0; c < COLOURS; c++) {
     const int from = starts[c], to = starts[c + 1];
     if (from == to) continue;
-    for (int lm = from + tid; lm < to; lm += nth) row_of<KIND>(d, vw, soft, h, known, moff + lm, boff);
+    for (int j = from + tid; j < to; j += nth) row_of<KIND>(d, vw, soft, h, known, moff + pick[moff + j], boff);
     grid.sync();
   }
   const int from = starts[OVERFLOW], to = starts[OVERFLOW + 1];
@@ -1093,7 +1160,7 @@ __device__ __forceinline__ void sweep_wide(const WideBar& grid, const SoftDev& d
 // them in changes no result; the overflow and the bounces stay serial in the reference's
 // order. Velocities stay in HBM.
 template <int THREADS>
-__global__ void __launch_bounds__(THREADS, 1) soft_wide_kernel(SoftDev d, int wld, unsigned* words) {
+__global__ void __launch_bounds__(THREADS, 1) soft_wide_kernel(SoftDev d, int wld, unsigned* words, int* pick) {
   const WideBar grid{words};
   __shared__ int starts[BM_STARTS_DEV];
   __shared__ int bounce_list[BOUNCE_CHUNK];
@@ -1131,15 +1198,15 @@ __global__ void __launch_bounds__(THREADS, 1) soft_wide_kernel(SoftDev d, int wl
     }
     grid.sync();
     if (touching) {
-      sweep_wide<0>(grid, d, vw, soft, h, false, moff, boff, starts, tid, nth);
-      sweep_wide<1>(grid, d, vw, soft, h, sub > 0, moff, boff, starts, tid, nth);
+      sweep_wide<0>(grid, d, vw, soft, h, false, moff, boff, starts, pick, tid, nth);
+      sweep_wide<1>(grid, d, vw, soft, h, sub > 0, moff, boff, starts, pick, tid, nth);
     }
     for (int i = tid; i < awake; i += nth) {
       const int slot = d.awake_list[aoff + i];
       move_body(d, vw, boff + slot, slot, h);
     }
     grid.sync();
-    if (touching) sweep_wide<2>(grid, d, vw, soft, h, false, moff, boff, starts, tid, nth);
+    if (touching) sweep_wide<2>(grid, d, vw, soft, h, false, moff, boff, starts, pick, tid, nth);
   }
   if (touching) {
     if (blockIdx.x == 0)
@@ -1159,19 +1226,25 @@ __global__ void __launch_bounds__(THREADS, 1) soft_wide_kernel(SoftDev d, int wl
 // colour holds at most a fifth to a third of a world's manifolds, so a colour's rows still
 // take a thread each, and a barrier costs less the fewer workgroups arrive at it), as many
 // workgroups as can be resident at most.
-int launch_soft_wide(const SoftDev& d, int wld, int work, unsigned* words, int device, hipStream_t stream) {
+hipError_t launch_soft_wide(const SoftDev& d, int wld, int work, unsigned* words, int* pick, int device,
+                            hipStream_t stream, int* blocks_out) {
   constexpr int T = 256;
   int per_cu = 0, cus = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, soft_wide_kernel<T>, T, 0) != hipSuccess) return -1;
-  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return -1;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, soft_wide_kernel<T>, T, 0);
+  if (e != hipSuccess) return e;
+  if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess) return e;
   int blocks = per_cu * cus;
   const int wanted = (work + 2 * T - 1) / (2 * T);
   if (wanted < blocks) blocks = wanted;
   if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(group_kernel<T>, dim3(COLOURS), dim3(T), 0, stream, d, wld, pick);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
   SoftDev arg = d;
-  void* args[] = {&arg, &wld, &words};
-  return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(soft_wide_kernel<T>), dim3(blocks), dim3(T), args, 0,
-                                    stream) == hipSuccess ? blocks : -1;
+  void* args[] = {&arg, &wld, &words, &pick};
+  e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(soft_wide_kernel<T>), dim3(blocks), dim3(T), args, 0,
+                                 stream);
+  *blocks_out = e == hipSuccess ? blocks : 0;
+  return e;
 }
 
 // Launch shapes: threads a workgroup, and registers sized for 1 or 2 waves a SIMD.

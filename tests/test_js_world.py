@@ -35,6 +35,8 @@ needs_node = pytest.mark.skipif(NODE is None, reason="no node on PATH")
 def node(*args, timeout=60):
     out = subprocess.run([NODE, SCRIPT, *args], capture_output=True, text=True, timeout=timeout)
     assert out.returncode == 0, out.stderr[-2000:]
+    if out.stderr:
+        print(out.stderr[-2000:], flush=True)
     return out.stdout
 
 
