@@ -1137,10 +1137,15 @@ template <int KIND>
 __device__ __forceinline__ void sweep_wide(const WideBar& grid, const SoftDev& d, const VW& vw,
                                            const Soft& soft, double h, bool known, int moff, int boff,
                                            const int* starts, const int* pick, int tid, int nth) {
+  // A colour's rows dealt round the workgroups (row k to workgroup k mod the grid), not in
+  // blocks of a workgroup's threads: a colour is a few thousand rows and each of its loads
+  // gathers scattered manifolds, so spread over every CU the loads' addresses go through
+  // every CU's memory pipeline rather than queueing in a dozen.
+  const int slot = (int)threadIdx.x * (int)gridDim.x + (int)blockIdx.x;
   for (int c = 0; c < COLOURS; c++) {
     const int from = starts[c], to = starts[c + 1];
     if (from == to) continue;
-    for (int j = from + tid; j < to; j += nth) row_of<KIND>(d, vw, soft, h, known, moff + pick[moff + j], boff);
+    for (int j = from + slot; j < to; j += nth) row_of<KIND>(d, vw, soft, h, known, moff + pick[moff + j], boff);
     grid.sync();
   }
   const int from = starts[OVERFLOW], to = starts[OVERFLOW + 1];
