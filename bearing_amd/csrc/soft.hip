@@ -1137,15 +1137,18 @@ template <int KIND>
 __device__ __forceinline__ void sweep_wide(const WideBar& grid, const SoftDev& d, const VW& vw,
                                            const Soft& soft, double h, bool known, int moff, int boff,
                                            const int* starts, const int* pick, int tid, int nth) {
-  // A colour's rows dealt round the workgroups (row k to workgroup k mod the grid), not in
-  // blocks of a workgroup's threads: a colour is a few thousand rows and each of its loads
-  // gathers scattered manifolds, so spread over every CU the loads' addresses go through
-  // every CU's memory pipeline rather than queueing in a dozen.
-  const int slot = (int)threadIdx.x * (int)gridDim.x + (int)blockIdx.x;
+  // A colour's rows shared out in equal runs, one run a workgroup, not in blocks of one
+  // workgroup's threads: a colour is a few thousand rows, and spread over every CU their loads'
+  // addresses go through every CU's memory pipeline rather than queueing in a dozen; a run's
+  // rows are neighbours in the grouped order, so a field of a few bytes a row is read a cache
+  // line for several lanes at once.
   for (int c = 0; c < COLOURS; c++) {
     const int from = starts[c], to = starts[c + 1];
     if (from == to) continue;
-    for (int j = from + slot; j < to; j += nth) row_of<KIND>(d, vw, soft, h, known, moff + pick[moff + j], boff);
+    const int run = (to - from + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int first = from + (int)blockIdx.x * run, last = first + run < to ? first + run : to;
+    for (int j = first + (int)threadIdx.x; j < last; j += (int)blockDim.x)
+      row_of<KIND>(d, vw, soft, h, known, moff + pick[moff + j], boff);
     grid.sync();
   }
   const int from = starts[OVERFLOW], to = starts[OVERFLOW + 1];
