@@ -175,8 +175,39 @@ struct Memory {
   }
 };
 
+// The last step's contact events (src/events.js): which pairs began touching and which
+// ended, by slot (the reference names bodies by id, generation * 2^24 + slot). A begin holds
+// where the two met, the normal from a to b and the closing speed; an end holds only who.
+// Cleared as a step begins (src/world.js:888); the columns keep their storage.
+struct Events {
+  int count = 0;
+  std::vector<uint8_t> type;  // BM_EVENT_BEGIN or BM_EVENT_END
+  std::vector<int32_t> a, b;
+  std::vector<double> point, normal, speed;
+  int add(uint8_t kind, int x, int y) {
+    if (count == (int)type.size()) {
+      const size_t c = std::max<size_t>(16, 2 * type.size());
+      type.resize(c); a.resize(c); b.resize(c); point.resize(3 * c); normal.resize(3 * c); speed.resize(c);
+    }
+    const int e = count++;
+    type[e] = kind; a[e] = x; b[e] = y;
+    for (int k = 0; k < 3; k++) point[3 * e + k] = normal[3 * e + k] = 0;
+    speed[e] = 0;
+    return e;
+  }
+  // src/events.js:41-51
+  void begin(int x, int y, double px, double py, double pz, double nx, double ny, double nz, double s) {
+    const int e = add(BM_EVENT_BEGIN, x, y);
+    point[3 * e] = px; point[3 * e + 1] = py; point[3 * e + 2] = pz;
+    normal[3 * e] = nx; normal[3 * e + 1] = ny; normal[3 * e + 2] = nz;
+    speed[e] = s;
+  }
+  void end(int x, int y) { add(BM_EVENT_END, x, y); }
+};
+
 // The pair list (src/pairs.js:40-378): [0, active) awake, [active, count) asleep.
 struct Pairs {
+  Events* events = nullptr;  // the world's, where part() writes an end
   int count = 0, active = 0;
   std::vector<int32_t> a, b, cache;
   std::vector<uint8_t> points, touching;
@@ -240,8 +271,12 @@ struct Pairs {
     std::memmove(&cache_local[6 * P * r], &cache_local[6 * P * last], 6 * P * sizeof(double));
     std::memmove(&cache_id[P * r], &cache_id[P * last], P * sizeof(uint32_t));
   }
-  // src/pairs.js:271-275 (events are not kept: the flag alone)
-  void part(int p) { touching[p] = 0; }
+  // src/pairs.js:279-284: pair p is no longer touching; its end, if it was.
+  void part(int p) {
+    if (touching[p] == 0) return;
+    touching[p] = 0;
+    if (events) events->end(a[p], b[p]);
+  }
   // src/pairs.js:234-240
   void forget(int p) {
     points[p] = 0;
@@ -452,8 +487,10 @@ struct bm_world {
   std::vector<int32_t> leaf, moved;
   std::vector<int8_t> which;
   std::vector<uint8_t> is_moved;
+  Events events;
   Pairs pairs;
   Contacts c;
+  bm_world() { pairs.events = &events; }
   // Narrowphase batches for the device: the records (filled whole, so growing them zeroes
   // nothing), then the results.
   std::vector<double, Unzeroed<double>> batch_a, batch_b, batch_pair;
@@ -1418,10 +1455,10 @@ void narrow_begin(bm_world* w, int from) {
       else if (v.kind == 3) v.slot = (int)levels++;
     }
     grow_batches(w, meets - m0, levels - l0);
-    // A pair whose boxes are apart forgets what it kept; when it keeps no cache row and neither
-    // body can wake while the pass is taken (so the pair is never looked at again), that
-    // touches only the pair's own entries, in any order: those are forgotten here, and the
-    // serial walk takes the other visits, in order.
+    // A pair whose boxes are apart forgets what it kept; when it keeps no cache row, was not
+    // touching (so it writes no end event) and neither body can wake while the pass is taken
+    // (so the pair is never looked at again), that touches only the pair's own entries, in any
+    // order: those are forgotten here, and the serial walk takes the other visits, in order.
     std::vector<int> kept_per((size_t)chunks + 1, 0);
     const uint8_t* type = w->type.data();
     const uint8_t* awake = w->awake.data();
@@ -1432,7 +1469,7 @@ void narrow_begin(bm_world* w, int from) {
         Visit& v = visits[i];
         if (v.kind == 2) fill_meet(w, v);
         else if (v.kind == 3) fill_level(w, v);
-        else if (v.kind == 0 && pr.cache[v.p] == 0 && !can_wake(v.a) && !can_wake(v.b)) {
+        else if (v.kind == 0 && pr.cache[v.p] == 0 && pr.touching[v.p] == 0 && !can_wake(v.a) && !can_wake(v.b)) {
           pr.points[v.p] = 0;
           pr.clear_memory(v.p);
           pr.part(v.p);
@@ -2333,7 +2370,7 @@ int catch_fast(bm_world* w) {
   return BM_OK;
 }
 
-// src/contact.js:2178-2213 (touched): which pairs touched, the flag only (no events kept).
+// src/contact.js:2178-2213 (touched): which pairs touched, and the begin or end of each that changed.
 void touched(bm_world* w) {
   Contacts& c = w->c;
   Pairs& pr = w->pairs;
@@ -2350,6 +2387,26 @@ void touched(bm_world* w) {
     const uint8_t now = hit >= 0 ? 1 : 0;
     if (now == pr.touching[p]) continue;
     pr.touching[p] = now;
+    if (now == 0) {
+      w->events.end(pr.a[p], pr.b[p]);
+      continue;
+    }
+    // The deepest point, moved on by half of what the two bodies moved this step; the
+    // fastest closing speed among the points.
+    int deepest = hit * P;
+    double speed = 0;
+    for (int k = 0; k < c.points[hit]; k++) {
+      const int i = hit * P + k;
+      if (c.separation[i] < c.separation[deepest]) deepest = i;
+      if (-c.relative_velocity[i] > speed) speed = -c.relative_velocity[i];
+    }
+    const int a3 = 3 * c.a[hit], b3 = 3 * c.b[hit], d3 = 3 * deepest, h3 = 3 * hit;
+    const double* p3 = w->position.data();
+    const double* st = w->start.data();
+    double at[3];
+    for (int k = 0; k < 3; k++)
+      at[k] = c.point[d3 + k] + (c.shift[d3 + k] + p3[a3 + k] - st[a3 + k] + p3[b3 + k] - st[b3 + k]) * 0.5;
+    w->events.begin(pr.a[p], pr.b[p], at[0], at[1], at[2], c.normal[h3], c.normal[h3 + 1], c.normal[h3 + 2], speed);
   }
 }
 
@@ -2618,6 +2675,7 @@ int step_begin(bm_world* w, double dt) {
   w->error.clear();
   if (!(dt > 0) || dt == INFINITY) return fail(w, BM_ERR_ARGUMENT, "step: dt must be a positive number of seconds");
   ++w->step_count;
+  w->events.count = 0;
   if (w->wake_pending) sync_pairs(w);
   // src/world.js:902-918: awake bodies' `previous` is where they start this step; a body that
   // fell asleep at the end of the last one moved in it, so it gets this once more, and its
@@ -3269,6 +3327,19 @@ int bm_world_read(const bm_world* w, bm_world_state* out) {
 // src/world.js:771-803 (moved): the slots that moved in any step after `since` (and always in
 // the last one), with bodies added since the last step: the awake list's in its order, then
 // the log's from its end; a `since` older than the log reaches takes every body by its stamp.
+int bm_world_events_of(const bm_world* w, bm_world_events* out) {
+  if (!w || !out) return BM_ERR_ARGUMENT;
+  const Events& e = w->events;
+  out->count = e.count;
+  out->type = e.type.data();
+  out->a = e.a.data();
+  out->b = e.b.data();
+  out->point = e.point.data();
+  out->normal = e.normal.data();
+  out->speed = e.speed.data();
+  return BM_OK;
+}
+
 int bm_world_moved(bm_world* w, int32_t since, int32_t* out, int32_t* count) {
   if (!w || !out || !count) return BM_ERR_ARGUMENT;
   w->error.clear();

@@ -161,6 +161,35 @@ class World {
     return this._movedOut.subarray(0, k);
   }
 
+  /*
+   * The last step's contact events (src/world.js:690-723): which pairs began touching and which
+   * stopped, as fresh plain objects, bodies by slot. Given a slot, only that body's own, each
+   * turned so that `a` is that body (the normal then points the other way).
+   */
+  contacts(slot) {
+    this._live('contacts');
+    if (slot !== undefined && (!Number.isInteger(slot) || slot < 0 || slot >= this._top)) {
+      fail('contacts', `slot must be a body's slot, got ${slot}`);
+    }
+    const e = this._native.contacts(this._world), out = [];
+    for (let i = 0; i < e.length / 10; i++) {
+      const r = 10 * i;
+      let a = e[r + 1], b = e[r + 2];
+      const turned = slot !== undefined && b === slot;
+      if (slot !== undefined && a !== slot && !turned) continue;
+      if (turned) { b = a; a = slot; }
+      if (e[r] === 1) {
+        out.push({ type: 'end', a, b });
+        continue;
+      }
+      // 0 - x keeps a zero from becoming -0.
+      const k = (x) => (turned ? 0 - x : x);
+      out.push({ type: 'begin', a, b, point: [e[r + 3], e[r + 4], e[r + 5]],
+        normal: [k(e[r + 6]), k(e[r + 7]), k(e[r + 8])], speed: e[r + 9] });
+    }
+    return out;
+  }
+
   /* Milliseconds spent so far in each part of the step (the C ABI's bm_world_times). */
   times() {
     this._live('times');

@@ -368,6 +368,37 @@ static napi_value js_moved(napi_env env, napi_callback_info info) {
   return out;
 }
 
+/* contacts(world) -> Float64Array(10 n): the last step's contact events (bm_world_events_of),
+ * [type, slot a, slot b, point 3, normal 3, speed] an event, in the order the step wrote them. */
+static napi_value js_contacts(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  NAPI(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  if (argc < 1) return fail(env, "contacts", "contacts(world)");
+  holder* h = world_of(env, argv[0], "contacts");
+  if (!h) return NULL;
+  bm_world_events e = {0};
+  if (bm_world_events_of(h->world, &e) != BM_OK) return fail(env, "contacts", bm_world_error(h->world));
+  napi_value buffer, out;
+  void* data = NULL;
+  const size_t n = (size_t)e.count;
+  NAPI(env, napi_create_arraybuffer(env, sizeof(double) * 10 * (n > 0 ? n : 1), &data, &buffer));
+  double* d = (double*)data;
+  for (size_t i = 0; i < n; i++) {
+    double* r = d + 10 * i;
+    r[0] = e.type[i];
+    r[1] = e.a[i];
+    r[2] = e.b[i];
+    for (int k = 0; k < 3; k++) {
+      r[3 + k] = e.point[3 * i + k];
+      r[6 + k] = e.normal[3 * i + k];
+    }
+    r[9] = e.speed[i];
+  }
+  NAPI(env, napi_create_typedarray(env, napi_float64_array, 10 * n, buffer, 0, &out));
+  return out;
+}
+
 /* times(world) -> Float64Array(BM_WORLD_TIMES): milliseconds spent in each part so far. */
 static napi_value js_times(napi_env env, napi_callback_info info) {
   size_t argc = 1;
@@ -401,7 +432,7 @@ NAPI_MODULE_INIT() {
   static const struct {
     const char* name;
     napi_callback fn;
-  } fns[] = {{"create", js_create}, {"add", js_add}, {"setHulls", js_set_hulls}, {"setFields", js_set_fields}, {"step", js_step}, {"read", js_read}, {"moved", js_moved}, {"times", js_times}, {"destroy", js_destroy}};
+  } fns[] = {{"create", js_create}, {"add", js_add}, {"setHulls", js_set_hulls}, {"setFields", js_set_fields}, {"step", js_step}, {"read", js_read}, {"moved", js_moved}, {"contacts", js_contacts}, {"times", js_times}, {"destroy", js_destroy}};
   for (size_t i = 0; i < sizeof fns / sizeof fns[0]; i++) {
     napi_value f;
     if (napi_create_function(env, fns[i].name, NAPI_AUTO_LENGTH, fns[i].fn, NULL, &f) != napi_ok ||

@@ -77,7 +77,7 @@ def copy_columns(cols):
 EXPORTS = ["bm_create", "bm_destroy", "bm_last_error", "bm_upload", "bm_soft_step", "bm_download",
            "bm_soft_step_host", "bm_last_kernel_ms", "bm_set_launch", "bm_lds_limit", "bm_colour", "bm_collide_host", "bm_meet_host", "bm_set_colour_lds", "bm_set_hulls",
            "bm_set_fields", "bm_level_host", "bm_set_wide", "bm_set_device_only",
-           "bm_level_noalloc", "bm_world_moved", "bm_world_tap",
+           "bm_level_noalloc", "bm_world_moved", "bm_world_tap", "bm_world_events_of",
            "bm_world_create", "bm_world_destroy", "bm_world_error", "bm_world_add", "bm_world_step", "bm_world_read",
            "bm_world_set_hulls", "bm_world_set_fields", "bm_world_times", "bm_upload_worlds", "bm_download_worlds",
            "bm_batch_create", "bm_batch_destroy", "bm_batch_error", "bm_batch_world", "bm_batch_step",
@@ -150,6 +150,12 @@ class WorldState(ctypes.Structure):
                 ("previous", ctypes.POINTER(ctypes.c_float))]
 
 
+class WorldEvents(ctypes.Structure):
+    """bm_world_events: the last step's contact events, as the world holds them."""
+    _fields_ = [("count", ctypes.c_int32), ("type", _U8), ("a", _I32), ("b", _I32), ("point", _D), ("normal", _D),
+                ("speed", _D)]
+
+
 def world_signatures(lib):
     """Sets the bm_world_* signatures on a library that exports them."""
     vp = ctypes.c_void_p
@@ -162,6 +168,7 @@ def world_signatures(lib):
     lib.bm_world_read.argtypes = [vp, ctypes.POINTER(WorldState)]
     lib.bm_world_moved.argtypes = [vp, ctypes.c_int32, _I32, _I32]
     lib.bm_world_tap.argtypes = [vp, COLUMNS_TAP, vp]
+    lib.bm_world_events_of.argtypes = [vp, ctypes.POINTER(WorldEvents)]
     lib.bm_world_set_hulls.argtypes = [vp, ctypes.c_int32, _I32, _I32, _I32, _D, _I32, _I32, _D]
     lib.bm_world_set_fields.argtypes = [vp, ctypes.c_int32, _I32, _I32, _D, ctypes.POINTER(ctypes.c_int64), _D]
     lib.bm_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int32, ctypes.c_int32]
@@ -530,6 +537,23 @@ class World:
         self.check(self.lib.bm_world_moved(self.ptr, since, out.ctypes.data_as(_I32), n.ctypes.data_as(_I32)),
                    "bm_world_moved")
         return out[:int(n[0])].copy()
+
+    def events(self):
+        """The last step's contact events (world.contacts(), src/world.js:690-723) as an (n, 10)
+        float64 array, a row [type, slot a, slot b, point 3, normal 3, speed] an event (type 0
+        begin, 1 end; an end's point, normal and speed 0), in the order the step wrote them."""
+        e = WorldEvents()
+        self.check(self.lib.bm_world_events_of(self.ptr, ctypes.byref(e)), "bm_world_events_of")
+        n = e.count
+        out = np.zeros((n, 10))
+        if n:
+            out[:, 0] = np.ctypeslib.as_array(e.type, (n,))
+            out[:, 1] = np.ctypeslib.as_array(e.a, (n,))
+            out[:, 2] = np.ctypeslib.as_array(e.b, (n,))
+            out[:, 3:6] = np.ctypeslib.as_array(e.point, (3 * n,)).reshape(n, 3)
+            out[:, 6:9] = np.ctypeslib.as_array(e.normal, (3 * n,)).reshape(n, 3)
+            out[:, 9] = np.ctypeslib.as_array(e.speed, (n,))
+        return out
 
     def capture_next(self):
         """Arms bm_world_tap for the next step alone: after it, `captured` holds the columns that

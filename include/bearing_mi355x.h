@@ -375,6 +375,25 @@ int bm_world_read(const bm_world* world, bm_world_state* out);
  * awake and unfrozen in a step (src/world.js:1040-1057), when it is added (1450-1454), and in
  * the step after it fell asleep (912-918). */
 int bm_world_moved(bm_world* world, int32_t since, int32_t* out, int32_t* count);
+/* The last step's contact events (src/world.js:690-723 contacts(), src/events.js): which pairs
+ * began touching in it and which stopped, in the order the step wrote them (ends as pairs were
+ * let go in the narrow pass, src/pairs.js:279-284; then, in the order the narrow pass found the
+ * pairs, a begin or an end for each pair whose touching changed, src/contact.js:2178-2213).
+ * Bodies by slot (the reference's ids are generation * 2^24 + slot). A begin carries the
+ * deepest point moved on by half of what the two bodies moved in the step, the normal from a
+ * to b and the fastest closing speed among the points; an end's point, normal and speed are 0.
+ * The pointers are the world's own and hold until its next step or bm_world_destroy. */
+#define BM_EVENT_BEGIN 0
+#define BM_EVENT_END 1
+typedef struct bm_world_events {
+  int32_t count;
+  const uint8_t* type;       /* BM_EVENT_BEGIN or BM_EVENT_END */
+  const int32_t *a, *b;      /* slots, a < b */
+  const double* point;       /* 3 an event */
+  const double* normal;      /* 3 an event */
+  const double* speed;
+} bm_world_events;
+int bm_world_events_of(const bm_world* world, bm_world_events* out);
 /* Milliseconds the world's steps have spent so far in each part, into out[BM_WORLD_TIMES]:
  * 0 bounds, broadphase and preparing the first narrow pass; 1 the narrowphase on the device
  * (with its copies); 2 taking the narrowphase's results in order and the later passes'

@@ -34,7 +34,15 @@ def load(path):
         start = base + e["offset"]
         out[e["name"]] = np.frombuffer(raw, dtype=dt, count=e["length"], offset=start).copy()
     out.meta = header["meta"]
+    out.path = path
     return out
+
+
+def events_of(fx):
+    """The reference's contact events along trajectory fixture `fx` (make_events.mjs), or None
+    where none were written."""
+    path = getattr(fx, "path", "").replace(".traj.bmfx.gz", ".events.bmfx.gz")
+    return load(path) if path.endswith(".events.bmfx.gz") and os.path.exists(path) else None
 
 
 def find(kind, scene=None):

@@ -39,7 +39,8 @@ JOBS = {
 }
 
 
-def run(name):
+def write_job(name):
+    """Writes trajectory `name`'s scene file and job file; returns the job file's path."""
     job = dict(JOBS[name])
     os.makedirs(os.path.join(DATA, "scenes"), exist_ok=True)
     scene_path = os.path.join(DATA, "scenes", job["scene"] + ".json")
@@ -50,8 +51,19 @@ def run(name):
     job_path = os.path.join(DATA, name + ".job.json")
     with open(job_path, "w") as f:
         json.dump(job, f)
+    return job_path
+
+
+def run(name):
     subprocess.run(["node", "--no-warnings", "--experimental-loader", os.path.join(HERE, "lower.mjs"),
-                    os.path.join(HERE, "make_fixtures.mjs"), job_path], check=True)
+                    os.path.join(HERE, "make_fixtures.mjs"), write_job(name)], check=True)
+
+
+def run_events(name):
+    """The reference's contact events after every step of trajectory `name` (make_events.mjs),
+    into data/<name>.events.bmfx.gz."""
+    subprocess.run(["node", "--no-warnings", "--experimental-loader", os.path.join(HERE, "lower.mjs"),
+                    os.path.join(HERE, "make_events.mjs"), write_job(name)], check=True)
 
 
 def collide_pairs(seed, n):
@@ -541,6 +553,10 @@ if __name__ == "__main__":
         sys.exit(0)
     if sys.argv[1:] == ["impact"]:
         run_impact()
+        sys.exit(0)
+    if sys.argv[1:2] == ["events"]:
+        for name in sys.argv[2:] or list(JOBS):
+            run_events(name)
         sys.exit(0)
     if sys.argv[1:2] == ["built"]:
         for name in sys.argv[2:] or ["rocks", "terrain"]:

@@ -5,7 +5,8 @@
  *                                        joined as the world joins them (no addon, no GPU)
  *   load                                 loads the addon, prints its exports and what new World() threw, if anything
  *   steps <scene.json> <steps> <dt> [built]  steps the scene: moved(0)'s digest, then one JSON line a step:
- *                                        [sha256 of transforms, awake, contacts, frozen, of moved() twice, of previous]
+ *                                        [sha256 of transforms, awake, contacts, frozen, of moved() twice, of previous,
+ *                                        of the step's contact events]
  * `built`: a fixture of the scene's hulls as the reference's src/hull.js build() made them
  * (tests/golden/make_built.mjs); each hull body's points are replaced by its built hull.
  */
@@ -81,7 +82,8 @@ if (mode === 'resolve') {
 } else if (mode === 'steps') {
   // One JSON line before the first step, the SHA-256 of moved(0) (every body added), then one a
   // step: [sha256 of transforms, awake, contacts, frozen, sha256 of moved(stepCount - 1), of
-  // moved(stepCount - 7, or 0), of previous]; each moved() list hashed as its Int32Array, in order.
+  // moved(stepCount - 7, or 0), of previous, of the contact events]; each moved() list hashed as
+  // its Int32Array, in order.
   const { World } = require(path.join(front, 'world.js'));
   const world = new World();
   world.addBodies(bodies(scene, process.argv[6]));
@@ -93,8 +95,14 @@ if (mode === 'resolve') {
     const c = world.counts;
     const last = digest(world.moved(world.stepCount - 1));
     const older = digest(world.moved(Math.max(0, world.stepCount - 7)));
+    // The step's contact events as the reference's fixture hashes them: ten numbers an event.
+    const ev = world.contacts(), rows = new Float64Array(10 * ev.length);
+    ev.forEach((x, j) => {
+      rows.set([x.type === 'end' ? 1 : 0, x.a, x.b], 10 * j);
+      if (x.type === 'begin') rows.set([...x.point, ...x.normal, x.speed], 10 * j + 3);
+    });
     process.stdout.write(JSON.stringify([digest(world.transforms), c.awake, c.contacts, c.frozen, last, older,
-      digest(world.previous)]) + '\n');
+      digest(world.previous), digest(rows)]) + '\n');
   }
   world.destroy();
 } else {

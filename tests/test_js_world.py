@@ -104,7 +104,8 @@ def test_js_addon_loads_and_fails_loudly_without_a_gpu():
     import torch
 
     r = json.loads(node("load"))
-    assert r["exports"] == ["create", "add", "setHulls", "setFields", "step", "read", "moved", "times", "destroy"]
+    assert r["exports"] == ["create", "add", "setHulls", "setFields", "step", "read", "moved", "contacts", "times",
+                            "destroy"]
     if not torch.cuda.is_available():
         assert r["threw"].startswith("World: no HIP device 0")
 
@@ -126,10 +127,12 @@ def test_gpu_js_world_steps_the_reference_trajectory(name):
     # The mixed pile's trajectory holds the transforms and counts only, not the moved() lists.
     moved = meta.get("moved")
     assert moved is None or json.loads(lines[0]) == [moved["added"]], f"{name}: moved(0) before the first step differs"
+    events = fixtures.events_of(fx)
     for s, line in enumerate(lines[1:], 1):
-        h, awake, contacts, frozen, last, older, previous = json.loads(line)
+        h, awake, contacts, frozen, last, older, previous, ev = json.loads(line)
         want = (meta["hashes"][s - 1], int(fx["awake"][s - 1]), int(fx["contacts"][s - 1]), int(fx["frozen"][s - 1]))
         assert (h, awake, contacts, frozen) == want, f"{name}: step {s} differs from the reference"
+        assert events is None or ev == events.meta["hashes"][s - 1], f"{name}: step {s}: contact events differ"
         if moved is None:
             continue
         assert previous == moved["previous"][s - 1], f"{name}: step {s}: previous differs from the reference"

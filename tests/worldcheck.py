@@ -69,14 +69,46 @@ def moved_mismatch(world, fx, s, r):
     return None
 
 
+class Events:
+    """The reference's contact events along a trajectory (fixtures.events_of), step by step:
+    each step's count and SHA-256 (over [type, slot a, slot b, point 3, normal 3, speed] an
+    event), and every event where the fixture kept them."""
+
+    def __init__(self, fx):
+        import fixtures
+        self.ev = fixtures.events_of(fx)
+        self.at = 0
+
+    def mismatch(self, world, s):
+        """After step s: the world's events against the reference's; None or why."""
+        if self.ev is None:
+            return None
+        got = world.events()
+        n = int(self.ev["counts"][s - 1])
+        want = None
+        if "events" in self.ev:
+            want = self.ev["events"][10 * self.at:10 * (self.at + n)].reshape(n, 10)
+        self.at += n
+        if sha(got) == self.ev.meta["hashes"][s - 1]:
+            return None
+        if len(got) != n:
+            return f"step {s}: {len(got)} contact events, reference {n}"
+        if want is not None:
+            bad = np.nonzero(np.any(got != want, axis=1))[0]
+            return f"step {s}: contact event {bad[0]} is {got[bad[0]].tolist()}, reference {want[bad[0]].tolist()}"
+        return f"step {s}: contact events differ"
+
+
 def run(world, fx, steps=None):
     """Steps `world` along fixture `fx`; returns (steps matched, the first mismatch or None).
-    Each step's transforms, counts, previous and moved() lists are compared."""
+    Each step's transforms, counts, previous and moved() lists, and contact events where the
+    fixture has them, are compared."""
     meta = fx.meta
     n = meta["steps"] if steps is None else min(steps, meta["steps"])
     why = moved_mismatch(world, fx, 0, None)
     if why:
         return 0, why
+    events = Events(fx)
     for s in range(1, n + 1):
         world.step(meta["dt"])
         r = world.read()
@@ -86,7 +118,7 @@ def run(world, fx, steps=None):
         if got != meta["hashes"][s - 1] or counts != want:
             return s - 1, f"step {s}: transforms {'match' if got == meta['hashes'][s - 1] else 'differ'}, " \
                           f"awake/contacts/frozen {counts}, reference {want}"
-        why = moved_mismatch(world, fx, s, r)
+        why = moved_mismatch(world, fx, s, r) or events.mismatch(world, s)
         if why:
             return s - 1, why
     return n, None
@@ -128,6 +160,7 @@ def run_together(batch, worlds, fxs, steps):
     and counts must be its own fixture's. Returns (steps matched, the first mismatch or None)."""
     dt = fxs[0].meta["dt"]
     assert all(f.meta["dt"] == dt for f in fxs)
+    events = [Events(fx) for fx in fxs]
     for s in range(1, steps + 1):
         batch.step(dt)
         for i, (w, fx) in enumerate(zip(worlds, fxs)):
@@ -138,7 +171,7 @@ def run_together(batch, worlds, fxs, steps):
             if got != fx.meta["hashes"][s - 1] or counts != want:
                 return s - 1, f"world {i}, step {s}: transforms {'match' if got == fx.meta['hashes'][s - 1] else 'differ'}, " \
                               f"awake/contacts/frozen {counts}, reference {want}"
-            why = moved_mismatch(w, fx, s, r)
+            why = moved_mismatch(w, fx, s, r) or events[i].mismatch(w, s)
             if why:
                 return s - 1, f"world {i}, {why}"
     return steps, None
