@@ -15,6 +15,7 @@
 
 #include "../../include/bearing_mi355x.h"
 #include "adjoint_dev.h"
+#include "broad.h"
 #include "collide.h"
 #include "colour.h"
 #include "soft.h"
@@ -165,6 +166,15 @@ struct bm_context {
   };
   std::vector<FieldShape> field_shapes;
   std::vector<int32_t> key_offsets;
+  // The broadphase passes (bm_broad_*): one world's bodies by slot (25 doubles, then type and
+  // shape), its two trees (boxes; child1, child2, height and item as four runs of ints), each
+  // body's leaf and tree, and the passes' rows and results.
+  struct Broad {
+    int32_t n = 0;
+    Scratch bodies, trees[2], links[2], leaf, rows, out, hits, counters;
+    int32_t root[2] = {-1, -1}, nodes[2] = {0, 0};
+    bool have_trees = false;
+  } broad;
 };
 
 namespace {
@@ -406,6 +416,10 @@ int bm_destroy(bm_context* ctx) {
     if (s->p) (void)hipFree(s->p);
   for (Scratch* s : {&ctx->hull_ints, &ctx->hull_doubles, &ctx->field_ints, &ctx->field_doubles, &ctx->level_scratch,
                      &ctx->colour_sort, &ctx->wide_bar, &ctx->wide_pick})
+    if (s->p) (void)hipFree(s->p);
+  bm_context::Broad& bb = ctx->broad;
+  for (Scratch* s : {&bb.bodies, &bb.trees[0], &bb.trees[1], &bb.links[0], &bb.links[1], &bb.leaf, &bb.rows, &bb.out,
+                     &bb.hits, &bb.counters})
     if (s->p) (void)hipFree(s->p);
   bm::held_free(ctx->held);
   if (ctx->begin) (void)hipEventDestroy(ctx->begin);
@@ -1322,6 +1336,350 @@ int bm_soft_step_host(bm_context* ctx, bm_columns* h) {
   if (int rc = bm_upload(ctx, h)) return rc;
   if (int rc = bm_soft_step(ctx)) return rc;
   return bm_download(ctx, h);
+}
+
+}  // extern "C"
+
+// ---- the broadphase passes ----
+namespace {
+
+size_t up256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Where each per-slot field of n bodies starts in the broadphase's body block (bytes).
+struct BroadLayout {
+  size_t dims, gs, reach, size, local, position, rotation, velocity, angular, type, shape, total;
+  explicit BroadLayout(size_t n) {
+    size_t at = 0;
+    auto take = [&](size_t bytes) {
+      const size_t o = at;
+      at = up256(at + bytes);
+      return o;
+    };
+    dims = take(24 * n); gs = take(8 * n); reach = take(8 * n); size = take(8 * n); local = take(48 * n);
+    position = take(24 * n); rotation = take(32 * n); velocity = take(24 * n); angular = take(24 * n);
+    type = take(n); shape = take(n);
+    total = at;
+  }
+};
+
+bm::broad::Bodies broad_bodies(bm_context* ctx) {
+  const BroadLayout L(ctx->broad.n);
+  char* b = static_cast<char*>(ctx->broad.bodies.p);
+  bm::broad::Bodies d{};
+  d.type = reinterpret_cast<const uint8_t*>(b + L.type);
+  d.shape = reinterpret_cast<const uint8_t*>(b + L.shape);
+  d.dims = reinterpret_cast<const double*>(b + L.dims);
+  d.gravity_scale = reinterpret_cast<const double*>(b + L.gs);
+  d.reach = reinterpret_cast<const double*>(b + L.reach);
+  d.size = reinterpret_cast<const double*>(b + L.size);
+  d.local = reinterpret_cast<const double*>(b + L.local);
+  d.position = reinterpret_cast<const double*>(b + L.position);
+  d.rotation = reinterpret_cast<const double*>(b + L.rotation);
+  d.velocity = reinterpret_cast<const double*>(b + L.velocity);
+  d.angular = reinterpret_cast<const double*>(b + L.angular);
+  return d;
+}
+
+bm::broad::Tree broad_tree(bm_context* ctx, int t) {
+  const bm_context::Broad& bb = ctx->broad;
+  const int* links = static_cast<const int*>(bb.links[t].p);
+  const size_t n = bb.nodes[t];
+  return bm::broad::Tree{bb.root[t], static_cast<const double*>(bb.trees[t].p), links, links + n, links + 2 * n,
+                         links + 3 * n};
+}
+
+// The leaf (n ints) and tree (n bytes) of each body, in one block.
+const int* broad_leaf(bm_context* ctx) { return static_cast<const int*>(ctx->broad.leaf.p); }
+const int8_t* broad_which(bm_context* ctx) {
+  return reinterpret_cast<const int8_t*>(static_cast<const char*>(ctx->broad.leaf.p) + up256(4 * (size_t)ctx->broad.n));
+}
+
+// A host array going up to byte `at` of a device block.
+struct Up {
+  size_t at;
+  const void* from;
+  size_t bytes;
+};
+
+// Copies `ups` into pinned staging at their places, then to `dev` in one transfer (queued on
+// the context's stream: the staging buffer is in use until the stream has run it).
+int stage_up(bm_context* ctx, void* dev, const std::vector<Up>& ups, const char* who) {
+  size_t total = 0;
+  for (const Up& u : ups) total = std::max(total, u.at + u.bytes);
+  if (int rc = grow_pinned(ctx, &ctx->pstage, &ctx->pstage_cap, total, who)) return rc;
+  std::vector<HostCopy> copies;
+  for (const Up& u : ups) copies.push_back({static_cast<char*>(ctx->pstage) + u.at, u.from, u.bytes});
+  run_copies(copies);
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(dev, ctx->pstage, total, hipMemcpyHostToDevice, ctx->stream), who);
+  return BM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bm_broad_set_bodies(bm_context* ctx, const bm_broad_bodies* b) {
+  if (!ctx) return BM_ERR_ARGUMENT;
+  ctx->error.clear();
+  if (!b || b->n < 0) return fail(ctx, BM_ERR_ARGUMENT, "bm_broad_set_bodies: no bodies");
+  const size_t n = b->n;
+  if (n > 0 && (!b->type || !b->shape || !b->dims || !b->gravity_scale || !b->reach || !b->size || !b->local ||
+                !b->position || !b->rotation || !b->velocity || !b->angular))
+    return fail(ctx, BM_ERR_ARGUMENT, "bm_broad_set_bodies: null column");
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->device), "bm_broad_set_bodies: hipSetDevice");
+  const BroadLayout L(n);
+  if (int rc = grow(ctx, ctx->broad.bodies, L.total + 256, "bm_broad_set_bodies")) return rc;
+  ctx->broad.n = b->n;
+  ctx->broad.have_trees = false;  // the leaf block's size follows n
+  if (n == 0) return BM_OK;
+  if (int rc = stage_up(ctx, ctx->broad.bodies.p,
+                        {{L.dims, b->dims, 24 * n}, {L.gs, b->gravity_scale, 8 * n},
+                         {L.reach, b->reach, 8 * n}, {L.size, b->size, 8 * n},
+                         {L.local, b->local, 48 * n}, {L.position, b->position, 24 * n},
+                         {L.rotation, b->rotation, 32 * n}, {L.velocity, b->velocity, 24 * n},
+                         {L.angular, b->angular, 24 * n}, {L.type, b->type, n},
+                         {L.shape, b->shape, n}},
+                        "bm_broad_set_bodies"))
+    return rc;
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_broad_set_bodies: copy");
+  return BM_OK;
+}
+
+int bm_broad_set_state(bm_context* ctx, int32_t n, const int32_t* slots, const double* position,
+                       const double* rotation, const double* velocity, const double* angular) {
+  if (!ctx) return BM_ERR_ARGUMENT;
+  ctx->error.clear();
+  if (n < 0 || (n > 0 && (!slots || !position || !rotation || !velocity || !angular)))
+    return fail(ctx, BM_ERR_ARGUMENT, "bm_broad_set_state: null argument");
+  if (n == 0) return BM_OK;
+  for (int32_t i = 0; i < n; i++)
+    if (slots[i] < 0 || slots[i] >= ctx->broad.n)
+      return fail(ctx, BM_ERR_ARGUMENT, "bm_broad_set_state: a slot is not a body bm_broad_set_bodies gave");
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->device), "bm_broad_set_state: hipSetDevice");
+  const size_t N = n, rows = 13 * 8 * N, idx = up256(rows);
+  if (int rc = grow(ctx, ctx->broad.rows, idx + 4 * N, "bm_broad_set_state")) return rc;
+  if (int rc = grow_pinned(ctx, &ctx->pstage, &ctx->pstage_cap, idx + 4 * N, "bm_broad_set_state")) return rc;
+  double* r = static_cast<double*>(ctx->pstage);
+  for (size_t i = 0; i < N; i++) {
+    const size_t s = slots[i];
+    double* o = r + 13 * i;
+    for (int k = 0; k < 3; k++) o[k] = position[3 * s + k];
+    for (int k = 0; k < 4; k++) o[3 + k] = rotation[4 * s + k];
+    for (int k = 0; k < 3; k++) o[7 + k] = velocity[3 * s + k];
+    for (int k = 0; k < 3; k++) o[10 + k] = angular[3 * s + k];
+  }
+  std::memcpy(static_cast<char*>(ctx->pstage) + idx, slots, 4 * N);
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->broad.rows.p, ctx->pstage, idx + 4 * N, hipMemcpyHostToDevice, ctx->stream),
+              "bm_broad_set_state: copy");
+  const bm::broad::Bodies d = broad_bodies(ctx);
+  (void)hipGetLastError();
+  bm::launch_broad_scatter(n, reinterpret_cast<const int*>(static_cast<char*>(ctx->broad.rows.p) + idx),
+                           static_cast<const double*>(ctx->broad.rows.p), const_cast<double*>(d.position),
+                           const_cast<double*>(d.rotation), const_cast<double*>(d.velocity),
+                           const_cast<double*>(d.angular), ctx->stream);
+  HIP_OR_FAIL(ctx, hipGetLastError(), "bm_broad_set_state: launch");
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_broad_set_state: scatter");
+  return BM_OK;
+}
+
+int bm_broad_keep(bm_context* ctx) {
+  if (!ctx) return BM_ERR_ARGUMENT;
+  ctx->error.clear();
+  if (!ctx->uploaded || ctx->n_worlds != 1 || ctx->bodies != ctx->broad.n)
+    return fail(ctx, BM_ERR_ARGUMENT, "bm_broad_keep: the last batch is not one world of the broadphase's bodies");
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->device), "bm_broad_keep: hipSetDevice");
+  const bm::broad::Bodies d = broad_bodies(ctx);
+  const size_t n = ctx->broad.n;
+  const struct {
+    const double* to;
+    const char* column;
+    size_t per;
+  } moves[] = {{d.position, "position", 3}, {d.rotation, "rotation", 4}, {d.velocity, "velocity", 3},
+               {d.angular, "angular", 3}};
+  for (const auto& m : moves)
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(const_cast<double*>(m.to), ctx->buffer[column_index(m.column)], 8 * m.per * n,
+                                    hipMemcpyDeviceToDevice, ctx->stream),
+                "bm_broad_keep: copy");
+  return BM_OK;
+}
+
+int bm_broad_set_trees(bm_context* ctx, const bm_broad_tree* trees, int32_t n, const int32_t* leaf,
+                       const int8_t* which) {
+  if (!ctx) return BM_ERR_ARGUMENT;
+  ctx->error.clear();
+  if (!trees || !leaf || !which || n != ctx->broad.n)
+    return fail(ctx, BM_ERR_ARGUMENT, "bm_broad_set_trees: the trees need every body bm_broad_set_bodies gave");
+  for (int t = 0; t < 2; t++) {
+    const bm_broad_tree& T = trees[t];
+    if (T.nodes < 0 || T.root < -1 || T.root >= T.nodes ||
+        (T.nodes > 0 && (!T.box || !T.child1 || !T.child2 || !T.height || !T.item)))
+      return fail(ctx, BM_ERR_ARGUMENT, "bm_broad_set_trees: a tree's nodes or root are wrong");
+  }
+  for (int32_t s = 0; s < n; s++)
+    if (which[s] >= 0 && (which[s] > 1 || leaf[s] < 0 || leaf[s] >= trees[which[s]].nodes))
+      return fail(ctx, BM_ERR_ARGUMENT, "bm_broad_set_trees: a body's leaf is not a node of its tree");
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->device), "bm_broad_set_trees: hipSetDevice");
+  bm_context::Broad& bb = ctx->broad;
+  for (int t = 0; t < 2; t++) {
+    const bm_broad_tree& T = trees[t];
+    const size_t m = T.nodes;
+    if (int rc = grow(ctx, bb.trees[t], 48 * m + 256, "bm_broad_set_trees")) return rc;
+    if (int rc = grow(ctx, bb.links[t], 16 * m + 256, "bm_broad_set_trees")) return rc;
+    bb.root[t] = T.root;
+    bb.nodes[t] = T.nodes;
+    if (m == 0) continue;
+    if (int rc = stage_up(ctx, bb.trees[t].p, {{0, T.box, 48 * m}}, "bm_broad_set_trees")) return rc;
+    HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_broad_set_trees: copy");
+    if (int rc = stage_up(ctx, bb.links[t].p,
+                          {{0, T.child1, 4 * m}, {4 * m, T.child2, 4 * m},
+                           {8 * m, T.height, 4 * m}, {12 * m, T.item, 4 * m}},
+                          "bm_broad_set_trees"))
+      return rc;
+    HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_broad_set_trees: copy");
+  }
+  const size_t N = n;
+  if (int rc = grow(ctx, bb.leaf, up256(4 * N) + N + 256, "bm_broad_set_trees")) return rc;
+  if (N > 0) {
+    if (int rc = stage_up(ctx, bb.leaf.p, {{0, leaf, 4 * N}, {up256(4 * N), which, N}},
+                          "bm_broad_set_trees"))
+      return rc;
+    HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_broad_set_trees: copy");
+  }
+  bb.have_trees = true;
+  return BM_OK;
+}
+
+int bm_broad_bounds(bm_context* ctx, int32_t n, const int32_t* list, double dt, double gravity_length, double* out,
+                    uint8_t* refile) {
+  if (!ctx) return BM_ERR_ARGUMENT;
+  ctx->error.clear();
+  if (n < 0 || (n > 0 && (!list || !out || !refile))) return fail(ctx, BM_ERR_ARGUMENT, "bm_broad_bounds: null argument");
+  if (!ctx->broad.have_trees) return fail(ctx, BM_ERR_ARGUMENT, "bm_broad_bounds: no trees (bm_broad_set_trees)");
+  if (n == 0) return BM_OK;
+  for (int32_t i = 0; i < n; i++)
+    if (list[i] < 0 || list[i] >= ctx->broad.n) return fail(ctx, BM_ERR_ARGUMENT, "bm_broad_bounds: a slot is not a body");
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->device), "bm_broad_bounds: hipSetDevice");
+  const size_t N = n, outs = 16 * 8 * N;
+  bm_context::Broad& bb = ctx->broad;
+  if (int rc = grow(ctx, bb.rows, 4 * N, "bm_broad_bounds")) return rc;
+  if (int rc = grow(ctx, bb.out, up256(outs) + N, "bm_broad_bounds")) return rc;
+  if (int rc = grow_pinned(ctx, &ctx->pstage, &ctx->pstage_cap, std::max(4 * N, up256(outs) + N), "bm_broad_bounds"))
+    return rc;
+  std::memcpy(ctx->pstage, list, 4 * N);
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(bb.rows.p, ctx->pstage, 4 * N, hipMemcpyHostToDevice, ctx->stream),
+              "bm_broad_bounds: copy");
+  bm::BoundsDev d{};
+  d.n = n;
+  d.list = static_cast<const int*>(bb.rows.p);
+  d.dt = dt;
+  d.gl = gravity_length;
+  d.bodies = broad_bodies(ctx);
+  d.moving = broad_tree(ctx, 1);
+  d.leaf = broad_leaf(ctx);
+  d.which = broad_which(ctx);
+  d.out = static_cast<double*>(bb.out.p);
+  d.refile = static_cast<uint8_t*>(bb.out.p) + up256(outs);
+  (void)hipGetLastError();
+  bm::launch_bounds(d, ctx->stream);
+  HIP_OR_FAIL(ctx, hipGetLastError(), "bm_broad_bounds: launch");
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->pstage, bb.out.p, up256(outs) + N, hipMemcpyDeviceToHost, ctx->stream),
+              "bm_broad_bounds: copy");
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_broad_bounds: kernel");
+  run_copies({{out, ctx->pstage, outs}, {refile, static_cast<char*>(ctx->pstage) + up256(outs), N}});
+  return BM_OK;
+}
+
+int bm_broad_query(bm_context* ctx, int32_t n, const int32_t* moved, int32_t* counts, int32_t* starts, int32_t* hits,
+                   int64_t capacity, int64_t* total) {
+  if (!ctx) return BM_ERR_ARGUMENT;
+  ctx->error.clear();
+  if (n < 0 || !total || capacity < 0 || (n > 0 && (!moved || !counts || !starts)) || (capacity > 0 && !hits))
+    return fail(ctx, BM_ERR_ARGUMENT, "bm_broad_query: null argument");
+  if (!ctx->broad.have_trees) return fail(ctx, BM_ERR_ARGUMENT, "bm_broad_query: no trees (bm_broad_set_trees)");
+  *total = 0;
+  if (n == 0) return BM_OK;
+  for (int32_t i = 0; i < n; i++)
+    if (moved[i] < 0 || moved[i] >= ctx->broad.n) return fail(ctx, BM_ERR_ARGUMENT, "bm_broad_query: a slot is not a body");
+  if (capacity > INT32_MAX) capacity = INT32_MAX;
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->device), "bm_broad_query: hipSetDevice");
+  const size_t N = n, C = capacity;
+  bm_context::Broad& bb = ctx->broad;
+  if (int rc = grow(ctx, bb.rows, 4 * N, "bm_broad_query")) return rc;
+  if (int rc = grow(ctx, bb.out, up256(8 * N) + 256, "bm_broad_query")) return rc;
+  if (int rc = grow(ctx, bb.hits, 4 * C + 4, "bm_broad_query")) return rc;
+  if (int rc = grow(ctx, bb.counters, 256, "bm_broad_query")) return rc;
+  if (int rc = grow_pinned(ctx, &ctx->pstage, &ctx->pstage_cap, std::max(up256(8 * N) + 256, 4 * C + 4), "bm_broad_query"))
+    return rc;
+  std::memcpy(ctx->pstage, moved, 4 * N);
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(bb.rows.p, ctx->pstage, 4 * N, hipMemcpyHostToDevice, ctx->stream),
+              "bm_broad_query: copy");
+  HIP_OR_FAIL(ctx, hipMemsetAsync(bb.counters.p, 0, 8, ctx->stream), "bm_broad_query: hipMemsetAsync");
+  bm::QueryDev d{};
+  d.n = n;
+  d.moved = static_cast<const int*>(bb.rows.p);
+  d.trees[0] = broad_tree(ctx, 0);
+  d.trees[1] = broad_tree(ctx, 1);
+  d.leaf = broad_leaf(ctx);
+  d.which = broad_which(ctx);
+  d.type = broad_bodies(ctx).type;
+  d.counts = static_cast<int*>(bb.out.p);
+  d.starts = static_cast<int*>(bb.out.p) + N;
+  d.hits = static_cast<int*>(bb.hits.p);
+  d.cap = (int)C;
+  d.total = static_cast<int*>(bb.counters.p);
+  d.flags = static_cast<int*>(bb.counters.p) + 1;
+  (void)hipGetLastError();
+  bm::launch_query(d, ctx->stream);
+  HIP_OR_FAIL(ctx, hipGetLastError(), "bm_broad_query: launch");
+  char* st = static_cast<char*>(ctx->pstage);
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(st, bb.out.p, 8 * N, hipMemcpyDeviceToHost, ctx->stream), "bm_broad_query: copy");
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(st + up256(8 * N), bb.counters.p, 8, hipMemcpyDeviceToHost, ctx->stream),
+              "bm_broad_query: copy");
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_broad_query: kernel");
+  const int32_t* counters = reinterpret_cast<const int32_t*>(st + up256(8 * N));
+  if (counters[1] & 1)
+    return fail(ctx, BM_ERR_DEVICE, "bm_broad_query: a tree is deeper than a walk's stack holds");
+  run_copies({{counts, st, 4 * N}, {starts, st + 4 * N, 4 * N}});
+  *total = counters[0];
+  if (*total == 0 || *total > capacity) return BM_OK;
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(st, bb.hits.p, 4 * (size_t)*total, hipMemcpyDeviceToHost, ctx->stream),
+              "bm_broad_query: copy");
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_broad_query: copy");
+  run_copies({{hits, st, 4 * (size_t)*total}});
+  return BM_OK;
+}
+
+int bm_broad_holds(bm_context* ctx, int32_t n, const int32_t* a, const int32_t* b, uint8_t* apart) {
+  if (!ctx) return BM_ERR_ARGUMENT;
+  ctx->error.clear();
+  if (n < 0 || (n > 0 && (!a || !b || !apart))) return fail(ctx, BM_ERR_ARGUMENT, "bm_broad_holds: null argument");
+  if (!ctx->broad.have_trees) return fail(ctx, BM_ERR_ARGUMENT, "bm_broad_holds: no trees (bm_broad_set_trees)");
+  if (n == 0) return BM_OK;
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->device), "bm_broad_holds: hipSetDevice");
+  const size_t N = n, half = up256(4 * N);
+  bm_context::Broad& bb = ctx->broad;
+  if (int rc = grow(ctx, bb.rows, 2 * half, "bm_broad_holds")) return rc;
+  if (int rc = grow(ctx, bb.out, N, "bm_broad_holds")) return rc;
+  if (int rc = stage_up(ctx, bb.rows.p, {{0, a, 4 * N}, {half, b, 4 * N}}, "bm_broad_holds"))
+    return rc;
+  bm::HoldsDev d{};
+  d.n = n;
+  d.a = static_cast<const int*>(bb.rows.p);
+  d.b = reinterpret_cast<const int*>(static_cast<char*>(bb.rows.p) + half);
+  d.trees[0] = broad_tree(ctx, 0);
+  d.trees[1] = broad_tree(ctx, 1);
+  d.leaf = broad_leaf(ctx);
+  d.which = broad_which(ctx);
+  d.apart = static_cast<uint8_t*>(bb.out.p);
+  (void)hipGetLastError();
+  bm::launch_holds(d, ctx->stream);
+  HIP_OR_FAIL(ctx, hipGetLastError(), "bm_broad_holds: launch");
+  // The staging buffer held the pairs going up; the stream runs in order, so the flags can
+  // come back into it once the kernel has read its inputs from the device.
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->pstage, bb.out.p, N, hipMemcpyDeviceToHost, ctx->stream), "bm_broad_holds: copy");
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_broad_holds: kernel");
+  run_copies({{apart, ctx->pstage, N}});
+  return BM_OK;
 }
 
 }  // extern "C"

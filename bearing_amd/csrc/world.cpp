@@ -526,6 +526,19 @@ struct bm_world {
   // Host threads for a large world's narrow passes (a world stepped alone); none in a batch.
   Pool* pool = nullptr;
   std::vector<std::vector<int>> candidate_hits;  // the pooled candidate walks' overlaps
+  // The broadphase's per-body and per-pair passes on the device (bm_broad_*): the way this world
+  // takes them (BM_BROAD_*), and on_device for the step under way. The device holds the bodies
+  // (given once: bodies_set) and their state as the last soft step left it (state), but for the
+  // slots the host has changed since (dirty); the trees as of version tree_sent (each change to
+  // either tree counts one in tree_version). apart_ready: w->apart holds the first pass's holds.
+  struct Broad {
+    int mode = BM_BROAD_AUTO;
+    bool on_device = false, bodies_set = false, state = false, apart_ready = false;
+    std::vector<int32_t> dirty, counts, starts, hits;
+    uint64_t tree_version = 1, tree_sent = 0;
+    std::vector<double> local, out;
+    std::vector<uint8_t> refile;
+  } broad;
   // Milliseconds spent in each part of the steps so far (bm_world_times).
   double times[BM_WORLD_TIMES] = {};
   // In a batch the device holds every world's hulls and fields in one table: this world's
@@ -748,6 +761,7 @@ void broadphase_add(bm_world* w, int s) {
   fat(w, s, t, 0, out);
   w->leaf[s] = w->trees[t].insert(s, out);
   w->which[s] = (int8_t)t;
+  w->broad.tree_version++;
   mark(w, s);
 }
 
@@ -757,6 +771,7 @@ void broadphase_settle(bm_world* w, int s) {
   double out[6];
   fat(w, s, MOVING_TREE, 0, out);
   w->trees[MOVING_TREE].move(w->leaf[s], out);
+  w->broad.tree_version++;
 }
 
 // src/broadphase.js:140-159
@@ -771,6 +786,7 @@ void broadphase_update(bm_world* w, double dt) {
     double out[6];
     fat(w, s, MOVING_TREE, AHEAD * dt, out);
     tree.move(w->leaf[s], out);
+    w->broad.tree_version++;
     mark(w, s);
   }
 }
@@ -859,6 +875,161 @@ void candidates(bm_world* w) {
     w->trees[MOVING_TREE].query(q, visit);
   }
   w->moved.clear();
+}
+
+// ---- the broadphase's passes on the device (bm_broad_*) ----
+
+// A world takes them on the device when it has its own context and, in BM_BROAD_AUTO, at
+// least BROAD_AWAKE awake bodies; then the walks when at least BROAD_WALKS bodies were marked
+// and the holds when a pass starts over at least BROAD_HOLDS pairs (in BM_BROAD_DEVICE, any).
+constexpr int BROAD_AWAKE = 4096, BROAD_WALKS = 256, BROAD_HOLDS = 16384;
+
+bool broad_on(const bm_world* w) {
+  if (!w->owns_ctx || w->broad.mode == BM_BROAD_HOST) return false;
+  return w->broad.mode == BM_BROAD_DEVICE || w->awake_count >= BROAD_AWAKE;
+}
+
+int broad_fail(bm_world* w, int rc) { return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx)); }
+
+// Every body's constants and state, once (and again after bodies, hulls or fields change).
+int broad_give_bodies(bm_world* w) {
+  const int n = w->top;
+  std::vector<double>& local = w->broad.local;
+  local.assign(6 * (size_t)n, 0);
+  for (int s = 0; s < n; s++) {
+    if (w->shape[s] != HULL && w->shape[s] < MESH) continue;
+    const double* lo = w->shape[s] == HULL ? w->hulls[w->hull[s]].lo : w->fields[w->field[s]].lo;
+    const double* hi = w->shape[s] == HULL ? w->hulls[w->hull[s]].hi : w->fields[w->field[s]].hi;
+    double* l = &local[6 * (size_t)s];
+    for (int k = 0; k < 3; k++) {
+      l[k] = (lo[k] + hi[k]) * 0.5;
+      l[3 + k] = (hi[k] - lo[k]) * 0.5;
+    }
+  }
+  bm_broad_bodies b{};
+  b.n = n;
+  b.type = w->type.data(); b.shape = w->shape.data(); b.dims = w->dims.data();
+  b.gravity_scale = w->gravity_scale.data(); b.reach = w->reach.data(); b.size = w->size.data();
+  b.local = local.data(); b.position = w->position.data(); b.rotation = w->rotation.data();
+  b.velocity = w->velocity.data(); b.angular = w->angular.data();
+  if (int rc = bm_broad_set_bodies(w->ctx, &b)) return broad_fail(w, rc);
+  w->broad.bodies_set = w->broad.state = true;
+  w->broad.dirty.clear();
+  w->broad.tree_sent = 0;  // the leaf table is sized by the bodies
+  return BM_OK;
+}
+
+// The trees, when either changed since they were last sent.
+int broad_give_trees(bm_world* w) {
+  if (w->broad.tree_sent == w->broad.tree_version) return BM_OK;
+  bm_broad_tree t[2];
+  for (int k = 0; k < 2; k++) {
+    const bm::Tree& T = w->trees[k];
+    t[k].nodes = (int32_t)T.parent.size();
+    t[k].root = T.root;
+    t[k].box = T.box.data(); t[k].child1 = T.child1.data(); t[k].child2 = T.child2.data();
+    t[k].height = T.height.data(); t[k].item = T.item.data();
+  }
+  if (int rc = bm_broad_set_trees(w->ctx, t, w->top, w->leaf.data(), w->which.data())) return broad_fail(w, rc);
+  w->broad.tree_sent = w->broad.tree_version;
+  return BM_OK;
+}
+
+// src/world.js:1574-1578 (_bounds) and src/broadphase.js:140-159 (update): every awake body's
+// bounds and fat-box test on the device, then the re-filing in the awake list's order (a
+// re-filed leaf moves no other leaf, so each body's test holds however many went before it).
+int broad_update(bm_world* w, double dt, double gl) {
+  if (!w->broad.bodies_set || !w->broad.state) {
+    if (int rc = broad_give_bodies(w)) return rc;
+  } else if (!w->broad.dirty.empty()) {
+    std::vector<int32_t>& d = w->broad.dirty;
+    std::sort(d.begin(), d.end());
+    d.erase(std::unique(d.begin(), d.end()), d.end());
+    if (int rc = bm_broad_set_state(w->ctx, (int32_t)d.size(), d.data(), w->position.data(), w->rotation.data(),
+                                    w->velocity.data(), w->angular.data()))
+      return broad_fail(w, rc);
+    d.clear();
+  }
+  if (int rc = broad_give_trees(w)) return rc;
+  const int n = w->awake_count;
+  std::vector<double>& out = w->broad.out;
+  std::vector<uint8_t>& refile = w->broad.refile;
+  if (out.size() < 16 * (size_t)n) out.resize(16 * (size_t)n);
+  if (refile.size() < (size_t)n) refile.resize(n);
+  if (int rc = bm_broad_bounds(w->ctx, n, w->awake_list.data(), dt, gl, out.data(), refile.data()))
+    return broad_fail(w, rc);
+  // The results are field-major: field k of row i at out[k * n + i].
+  const int32_t* list = w->awake_list.data();
+  auto take = [&](int i0, int i1) {
+    for (int i = i0; i < i1; i++) {
+      const size_t s = list[i];
+      for (int k = 0; k < 9; k++) w->axes[9 * s + k] = out[(size_t)k * n + i];
+      w->motion[s] = out[(size_t)9 * n + i];
+      for (int k = 0; k < 6; k++) w->box[6 * s + k] = out[(size_t)(10 + k) * n + i];
+    }
+  };
+  if (w->pool && n >= 8192) {
+    const int chunk = 2048, chunks = (n + chunk - 1) / chunk;
+    w->pool->run(chunks, [&](int c) { take(c * chunk, std::min(n, (c + 1) * chunk)); });
+  } else {
+    take(0, n);
+  }
+  bm::Tree& tree = w->trees[MOVING_TREE];
+  for (int i = 0; i < n; i++) {
+    if (!refile[i]) continue;
+    const int s = list[i];
+    double f[6];
+    fat(w, s, MOVING_TREE, AHEAD * dt, f);
+    tree.move(w->leaf[s], f);
+    w->broad.tree_version++;
+    mark(w, s);
+  }
+  return BM_OK;
+}
+
+// src/broadphase.js:161-179 (candidates) with the walks on the device, their bodies handed to
+// found() in the walks' order; done false when too few bodies were marked to send.
+int broad_candidates(bm_world* w, bool& done) {
+  done = false;
+  const int n = (int)w->moved.size();
+  if (n == 0 || (w->broad.mode != BM_BROAD_DEVICE && n < BROAD_WALKS)) return BM_OK;
+  if (int rc = broad_give_trees(w)) return rc;
+  auto& b = w->broad;
+  if (b.counts.size() < (size_t)n) {
+    b.counts.resize(n);
+    b.starts.resize(n);
+  }
+  if (b.hits.size() < 8 * (size_t)n) b.hits.resize(8 * (size_t)n);
+  int64_t total = 0;
+  for (int round = 0; round < 2; round++) {
+    if (int rc = bm_broad_query(w->ctx, n, w->moved.data(), b.counts.data(), b.starts.data(), b.hits.data(),
+                                (int64_t)b.hits.size(), &total))
+      return broad_fail(w, rc);
+    if (total <= (int64_t)b.hits.size()) break;
+    b.hits.resize((size_t)total);  // the walks again, with room for every body they meet
+  }
+  for (int i = 0; i < n; i++) {
+    const int a = w->moved[i];
+    w->is_moved[a] = 0;
+    for (int k = b.starts[i], e = b.starts[i] + b.counts[i]; k < e; k++) found(w, a, b.hits[k]);
+  }
+  w->moved.clear();
+  done = true;
+  return BM_OK;
+}
+
+// src/pairs.js:340-356 (holds) for a pass over pairs [from, active) on the device, into
+// w->apart, for narrow_begin to take.
+int broad_holds(bm_world* w, int from) {
+  Pairs& pr = w->pairs;
+  const int m = pr.active - from;
+  if (m == 0 || (w->broad.mode != BM_BROAD_DEVICE && m < BROAD_HOLDS)) return BM_OK;
+  if (int rc = broad_give_trees(w)) return rc;
+  if ((int)w->apart.size() < pr.active) w->apart.resize(pr.active);
+  if (int rc = bm_broad_holds(w->ctx, m, pr.a.data() + from, pr.b.data() + from, w->apart.data() + from))
+    return broad_fail(w, rc);
+  w->broad.apart_ready = true;
+  return BM_OK;
 }
 
 // ---- narrow driver (src/contact.js:296-1036) ----
@@ -1394,17 +1565,20 @@ void narrow_begin(bm_world* w, int from) {
     const int a = pr.a[p], b = pr.b[p];
     return Visit{p, a, b, 0, -1, false, js_min(w->size[a], w->size[b]) * PIXEL};
   };
-  if (w->pool && pr.active - from >= PARALLEL_VISITS) {
-    // Each overlap reads only the trees, so a large pass tests them on the pool, then removes
-    // in the serial loop's order (a removal moves the last active pair, and its flag, into
-    // the slot); the pairs kept are then slots from..active in order, one visit each.
+  const bool ready = w->broad.apart_ready;
+  w->broad.apart_ready = false;
+  if (ready || (w->pool && pr.active - from >= PARALLEL_VISITS)) {
+    // Each overlap reads only the trees, so a large pass tests them on the device or the pool,
+    // then removes in the serial loop's order (a removal moves the last active pair, and its
+    // flag, into the slot); the pairs kept are then slots from..active in order, one visit each.
     const int m = pr.active - from, chunk = 4096, chunks = (m + chunk - 1) / chunk;
     std::vector<uint8_t>& apart = w->apart;
     if ((int)apart.size() < pr.active) apart.resize(pr.active);
-    w->pool->run(chunks, [&](int c) {
-      for (int p = from + c * chunk, e = std::min(pr.active, from + (c + 1) * chunk); p < e; p++)
-        apart[p] = !overlap(w, pr.a[p], pr.b[p]);
-    });
+    if (!ready)
+      w->pool->run(chunks, [&](int c) {
+        for (int p = from + c * chunk, e = std::min(pr.active, from + (c + 1) * chunk); p < e; p++)
+          apart[p] = !overlap(w, pr.a[p], pr.b[p]);
+      });
     for (int p = from; p < pr.active;) {
       if (apart[p]) {
         apart[p] = apart[pr.active - 1];
@@ -1415,9 +1589,12 @@ void narrow_begin(bm_world* w, int from) {
     }
     const int kept = pr.active - from, kept_chunks = (kept + chunk - 1) / chunk;
     if ((int)visits.size() < kept) visits.resize(kept);  // every visit is written below
-    w->pool->run(kept_chunks, [&](int c) {
+    auto visits_of = [&](int c) {
       for (int i = c * chunk, e = std::min(kept, (c + 1) * chunk); i < e; i++) visits[i] = visit_of(from + i);
-    });
+    };
+    if (w->pool) w->pool->run(kept_chunks, visits_of);
+    else
+      for (int c = 0; c < kept_chunks; c++) visits_of(c);
     w->visit_count = kept;
   } else {
     size_t k = 0;
@@ -2361,6 +2538,9 @@ int catch_fast(bm_world* w) {
     if (rcs[k] != BM_OK) return rcs[k];
   int caught = 0;
   for (uint8_t h : hits) caught += h;
+  if (w->broad.bodies_set)
+    for (size_t k = 0; k < fast_list.size(); k++)
+      if (hits[k]) w->broad.dirty.push_back(fast_list[k]);
   w->catches += caught;
   for (int i = 0; i < n; i++) w->fast[list[i]] = 0;
   if (caught == 0) return BM_OK;
@@ -2661,6 +2841,12 @@ int device_phase(bm_world* w, double dt) {
   if (int rc = bm_colour(w->ctx)) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx));
   if (int rc = bm_soft_step(w->ctx)) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx));
   if (int rc = bm_download(w->ctx, &h)) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx));
+  if (w->broad.bodies_set) {
+    // The step's state becomes the broadphase's on the device: every slot is as the host has it.
+    if (int rc = bm_broad_keep(w->ctx)) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx));
+    w->broad.state = true;
+    w->broad.dirty.clear();
+  }
   solve_stats(w, bm_last_kernel_ms(w->ctx));
   store_memory(w, w->h_solved);
   if (w->record.on) record_after(w);
@@ -2694,12 +2880,24 @@ int step_begin(bm_world* w, double dt) {
   w->slept_count = 0;
   w->step_dt = dt;
   const double gl = gravity_length(w);
-  for (int i = 0; i < w->awake_count; i++) bounds_of(w, w->awake_list[i], dt, gl);
-  broadphase_update(w, dt);
-  candidates(w);
+  auto& bp = w->broad;
+  bp.on_device = broad_on(w);
+  bp.apart_ready = false;
+  bool walked = false;
+  if (bp.on_device) {
+    // The world has begun its step: a device failure from here on leaves it part-way.
+    if (int rc = broad_update(w, dt, gl)) return broke(w, rc);
+    if (int rc = broad_candidates(w, walked)) return broke(w, rc);
+  } else {
+    for (int i = 0; i < w->awake_count; i++) bounds_of(w, w->awake_list[i], dt, gl);
+    broadphase_update(w, dt);
+  }
+  if (!walked) candidates(w);
   w->c.count = 0;
   w->narrow_from = 0;
   if (w->wake_pending) sync_pairs(w);
+  if (bp.on_device)
+    if (int rc = broad_holds(w, 0)) return broke(w, rc);
   narrow_begin(w, 0);
   return BM_OK;
 }
@@ -2942,6 +3140,7 @@ const char* bm_world_error(const bm_world* w) { return w ? w->error.c_str() : "b
 int bm_world_add(bm_world* w, const bm_bodies* b) {
   if (!w || !b) return BM_ERR_ARGUMENT;
   w->error.clear();
+  w->broad.bodies_set = false;  // the device's bodies are given again at the next step
   if (b->n < 0) return fail(w, BM_ERR_ARGUMENT, "addBody: n must be 0 or more");
   for (int i = 0; i < b->n; i++) {
     if (!b->type || !b->shape || !b->lock || !b->position || !b->rotation || !b->velocity || !b->angular || !b->dims ||
@@ -3025,6 +3224,7 @@ int bm_world_add(bm_world* w, const bm_bodies* b) {
 
 int bm_world_set_hulls(bm_world* w, int32_t n, const int32_t* hull_v, const int32_t* hull_f, const int32_t* hull_l,
                        const double* vertices, const int32_t* start, const int32_t* loops, const double* planes) {
+  if (w) w->broad.bodies_set = false;
   if (!w) return BM_ERR_ARGUMENT;
   w->error.clear();
   for (int s = 0; s < w->top; s++)
@@ -3077,6 +3277,7 @@ int bm_world_set_hulls(bm_world* w, int32_t n, const int32_t* hull_v, const int3
 
 int bm_world_set_fields(bm_world* w, int32_t n, const int32_t* columns, const int32_t* rows, const double* spacing,
                         const int64_t* offsets, const double* heights) {
+  if (w) w->broad.bodies_set = false;
   if (!w) return BM_ERR_ARGUMENT;
   w->error.clear();
   for (int s = 0; s < w->top; s++)
@@ -3327,6 +3528,17 @@ int bm_world_read(const bm_world* w, bm_world_state* out) {
 // src/world.js:771-803 (moved): the slots that moved in any step after `since` (and always in
 // the last one), with bodies added since the last step: the awake list's in its order, then
 // the log's from its end; a `since` older than the log reaches takes every body by its stamp.
+int bm_world_set_broadphase(bm_world* w, int32_t mode) {
+  if (!w) return BM_ERR_ARGUMENT;
+  w->error.clear();
+  if (mode != BM_BROAD_HOST && mode != BM_BROAD_DEVICE && mode != BM_BROAD_AUTO)
+    return fail(w, BM_ERR_ARGUMENT, "bm_world_set_broadphase: mode is BM_BROAD_HOST, BM_BROAD_DEVICE or BM_BROAD_AUTO");
+  if (mode == BM_BROAD_DEVICE && !w->owns_ctx)
+    return fail(w, BM_ERR_ARGUMENT, "bm_world_set_broadphase: a batch's worlds take the host's broadphase");
+  w->broad.mode = mode;
+  return BM_OK;
+}
+
 int bm_world_events_of(const bm_world* w, bm_world_events* out) {
   if (!w || !out) return BM_ERR_ARGUMENT;
   const Events& e = w->events;

@@ -77,7 +77,9 @@ def copy_columns(cols):
 EXPORTS = ["bm_create", "bm_destroy", "bm_last_error", "bm_upload", "bm_soft_step", "bm_download",
            "bm_soft_step_host", "bm_last_kernel_ms", "bm_set_launch", "bm_lds_limit", "bm_colour", "bm_collide_host", "bm_meet_host", "bm_set_colour_lds", "bm_set_hulls",
            "bm_set_fields", "bm_level_host", "bm_set_wide", "bm_set_device_only",
-           "bm_level_noalloc", "bm_world_moved", "bm_world_tap", "bm_world_events_of",
+           "bm_level_noalloc", "bm_world_moved", "bm_world_tap", "bm_world_events_of", "bm_world_set_broadphase",
+           "bm_broad_set_bodies", "bm_broad_set_state", "bm_broad_keep", "bm_broad_set_trees", "bm_broad_bounds",
+           "bm_broad_query", "bm_broad_holds",
            "bm_world_create", "bm_world_destroy", "bm_world_error", "bm_world_add", "bm_world_step", "bm_world_read",
            "bm_world_set_hulls", "bm_world_set_fields", "bm_world_times", "bm_upload_worlds", "bm_download_worlds",
            "bm_batch_create", "bm_batch_destroy", "bm_batch_error", "bm_batch_world", "bm_batch_step",
@@ -169,6 +171,7 @@ def world_signatures(lib):
     lib.bm_world_moved.argtypes = [vp, ctypes.c_int32, _I32, _I32]
     lib.bm_world_tap.argtypes = [vp, COLUMNS_TAP, vp]
     lib.bm_world_events_of.argtypes = [vp, ctypes.POINTER(WorldEvents)]
+    lib.bm_world_set_broadphase.argtypes = [vp, ctypes.c_int32]
     lib.bm_world_set_hulls.argtypes = [vp, ctypes.c_int32, _I32, _I32, _I32, _D, _I32, _I32, _D]
     lib.bm_world_set_fields.argtypes = [vp, ctypes.c_int32, _I32, _I32, _D, ctypes.POINTER(ctypes.c_int64), _D]
     lib.bm_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int32, ctypes.c_int32]
@@ -459,6 +462,12 @@ class World:
     def check(self, rc, what):
         if rc != 0:
             raise BearingError(f"{what}: status {rc}: {self.lib.bm_world_error(self.ptr).decode()}")
+
+    BROADPHASE = {"host": 0, "device": 1, "auto": 2}
+
+    def set_broadphase(self, mode):
+        """Where the broadphase's passes run: "host", "device" or "auto" (bm_world_set_broadphase)."""
+        self.check(self.lib.bm_world_set_broadphase(self.ptr, self.BROADPHASE[mode]), "bm_world_set_broadphase")
 
     def record(self, on=True):
         """bm_world_record: keep the steps from now on for an Adjoint (sleeping must be off)."""

@@ -225,7 +225,7 @@ BATCH_WORLDS_FROM = ["tower", "pile_small", "pyramid", "rain_small", "capsules",
 PILE = "mixed_pile"
 
 
-def pile_run(timed, warmup):
+def pile_run(timed, warmup, broadphase="auto"):
     """The headline: the mixed pile's reference trajectory (600 steps) through bm_world_step;
     the first 600 - timed - warmup steps bring it to rest, then `warmup` untimed steps, then
     `timed` steps timed one by one (wall clock around bm_world_step; bm_world_read's rows,
@@ -245,6 +245,7 @@ def pile_run(timed, warmup):
     warmup = max(0, min(warmup, total - timed))
     setup = total - timed - warmup
     w = fill_world(native.World(0), fx)
+    w.set_broadphase(broadphase)
     tr = np.zeros(7 * int(meta["bodies"]), np.float32)
     state = native.WorldState()
     state.transforms = tr.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
@@ -275,7 +276,7 @@ def pile_run(timed, warmup):
     tw = float(np.sum(walls[first:]))
     trows = int(np.sum(rows[first:]))
     return {
-        "steps": total, "bodies": int(meta["bodies"]), "setup_steps": setup, "warmup": warmup, "timed_steps": timed,
+        "steps": total, "bodies": int(meta["bodies"]), "broadphase": broadphase, "setup_steps": setup, "warmup": warmup, "timed_steps": timed,
         "bit_exact": bool(all(h == r for h, r in zip(hashes, meta["hashes"]))),
         "timed_wall_s": tw, "ms_per_step": tw * 1000 / timed,
         "rows_per_step": trows / timed, "contact_iters_per_s": trows * 2 * SUBSTEPS / tw,
@@ -547,6 +548,8 @@ def main():
     ap.add_argument("--steps", type=int, default=50, help="timed world steps of the pile at rest (the last of its 600)")
     ap.add_argument("--warmup", type=int, default=5, help="untimed world steps before them")
     ap.add_argument("--micro-steps", type=int, default=50, help="soft steps of the rain's fixed state (microbenchmark)")
+    ap.add_argument("--broadphase", default="auto", choices=["auto", "host", "device"],
+                    help="where the pile's broadphase passes run (bm_world_set_broadphase)")
     ap.add_argument("--batch-worlds", type=int, default=4096)
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--batch-threads", type=int, default=16, help="host threads for the whole-world batch")
@@ -595,7 +598,7 @@ def main():
         return
 
     parts = set(args.parts.split(","))
-    pile, captured = pile_run(args.steps, args.warmup) if "pile" in parts else (None, None)
+    pile, captured = pile_run(args.steps, args.warmup, args.broadphase) if "pile" in parts else (None, None)
     micro = micro_run(args) if "micro" in parts else None
     c4 = c4_run(args.batch_threads) if "c4" in parts else None
     worlds_run = {name: world_run(name) for name in WORLD_RUNS} if "worlds" in parts else None

@@ -283,6 +283,64 @@ int64_t bm_lds_limit(bm_context* ctx);
 /* Milliseconds the last bm_soft_step's kernel took on the device (HIP events). */
 double bm_last_kernel_ms(const bm_context* ctx);
 
+/* ---- the broadphase's data-parallel passes (src/world.js:1574-1630, src/broadphase.js) ----
+ *
+ * The reference's Broadphase (src/broadphase.js:56-219) re-files a body in its tree when its
+ * bounds leave its fat box, walks both trees for every body it marked, and hands what the walks
+ * meet to the pair list in that order; the narrow pass first drops pairs whose fat boxes no
+ * longer overlap (src/pairs.js:340-356 holds). These calls do the per-body and per-pair parts
+ * on the device, for one world at a time; the host keeps the trees' shape (src/tree.js insert,
+ * move, rotate) and takes the results in the reference's order. A context holds one world's
+ * bodies and trees for them (the last ones given). */
+
+/* Every body's constants and state for the bounds, by slot (src/world.js:1580-1630): type,
+ * shape, dims 3, gravityScale, reach, size, local 6 (a hull's or heightfield's box in its own
+ * frame: centre 3, half extents 3; anything for other shapes), position 3, rotation 4,
+ * velocity 3, angular 3. */
+typedef struct bm_broad_bodies {
+  int32_t n;
+  const uint8_t *type, *shape;
+  const double *dims, *gravity_scale, *reach, *size, *local;
+  const double *position, *rotation, *velocity, *angular;
+} bm_broad_bodies;
+int bm_broad_set_bodies(bm_context* ctx, const bm_broad_bodies* bodies);
+/* The state of n bodies (slots[i]) changed on the host: their position, rotation, velocity and
+ * angular, read from the per-slot host arrays, replace the device's. */
+int bm_broad_set_state(bm_context* ctx, int32_t n, const int32_t* slots, const double* position,
+                       const double* rotation, const double* velocity, const double* angular);
+/* After a soft step of a batch of one world with as many bodies as bm_broad_set_bodies gave:
+ * the step's positions, rotations, velocities and spins become the broadphase's, on the device
+ * (no copy through the host). */
+int bm_broad_keep(bm_context* ctx);
+/* The two trees (static, moving; src/broadphase.js:56-70) as src/tree.js keeps them: nodes
+ * (capacity), root (-1 empty), box 6 a node, child1, child2, height (0 a leaf), item (a leaf's
+ * body slot); and each body's leaf (leaf, by slot) and tree (which: 0 static, 1 moving, -1 none).
+ * The trees stay until the next call. */
+typedef struct bm_broad_tree {
+  int32_t nodes, root;
+  const double* box;
+  const int32_t *child1, *child2, *height, *item;
+} bm_broad_tree;
+int bm_broad_set_trees(bm_context* ctx, const bm_broad_tree* trees, int32_t n, const int32_t* leaf,
+                       const int8_t* which);
+/* src/world.js:1574-1578 (_bounds) and the test of src/broadphase.js:140-159 (update) for the n
+ * awake bodies list[0..n), stepping dt under gravity of length gravity_length: out (16 n,
+ * field-major: out[k * n + i]) each body's axes 9, motion and box 6; refile[i] = 1 where the
+ * body is in the moving tree and the box left its leaf's fat box. */
+int bm_broad_bounds(bm_context* ctx, int32_t n, const int32_t* list, double dt, double gravity_length, double* out,
+                    uint8_t* refile);
+/* src/broadphase.js:161-179 (candidates): for each of the n marked bodies moved[0..n), its leaf's
+ * box walked down the static tree (unless the body is static, by the type bm_broad_set_bodies
+ * gave), then the moving tree (src/tree.js
+ * query, child1 first): counts[i] bodies met other than itself, written from hits[starts[i]] in
+ * walk order. *total: every body's count together; hits is written only when that is at most
+ * capacity (call again with room for *total). */
+int bm_broad_query(bm_context* ctx, int32_t n, const int32_t* moved, int32_t* counts, int32_t* starts, int32_t* hits,
+                   int64_t capacity, int64_t* total);
+/* src/pairs.js:340-356 (holds): apart[i] = 1 where the fat boxes of pair (a[i], b[i])'s leaves
+ * no longer overlap (src/broadphase.js:181-187). */
+int bm_broad_holds(bm_context* ctx, int32_t n, const int32_t* a, const int32_t* b, uint8_t* apart);
+
 /* ---- a whole world, stepped as src/world.js step (880-1072) steps it ----
  *
  * For worlds of dynamic and static boxes, spheres, capsules and hulls, and static heightfields,
@@ -383,6 +441,16 @@ int bm_world_moved(bm_world* world, int32_t since, int32_t* out, int32_t* count)
  * deepest point moved on by half of what the two bodies moved in the step, the normal from a
  * to b and the fastest closing speed among the points; an end's point, normal and speed are 0.
  * The pointers are the world's own and hold until its next step or bm_world_destroy. */
+/* Where a world's broadphase passes run (src/world.js:1574-1630 bounds, src/broadphase.js:140-187
+ * update and candidates, src/pairs.js:340-356 holds): BM_BROAD_HOST on the host's threads;
+ * BM_BROAD_DEVICE on the device (bm_broad_*), the host re-filing bodies in the trees and taking
+ * what the walks met in the reference's order; BM_BROAD_AUTO (the default) on the device for a
+ * world with its own context and at least 4,096 awake bodies. A batch's worlds take the host's.
+ * Results are the same bits every way. No counterpart in the reference. */
+#define BM_BROAD_HOST 0
+#define BM_BROAD_DEVICE 1
+#define BM_BROAD_AUTO 2
+int bm_world_set_broadphase(bm_world* world, int32_t mode);
 #define BM_EVENT_BEGIN 0
 #define BM_EVENT_END 1
 typedef struct bm_world_events {

@@ -7,6 +7,7 @@
 #include <string>
 #include <vector>
 
+#include "../../bearing_amd/csrc/broad_dev.h"
 #include "../../bearing_amd/csrc/held_layout.h"
 #include "../../include/bearing_mi355x.h"
 #include "../../oracle/oracle.h"
@@ -34,10 +35,35 @@ struct HeldHost {
   bm::adj::HeldIO io;
 };
 
+// The broadphase passes' device memory, as copies: what bm_broad_set_bodies, _set_state, _keep
+// and _set_trees gave, so a body the host changed without saying so keeps its old state here.
+struct BroadHost {
+  int n = 0;
+  std::vector<uint8_t> type, shape;
+  std::vector<double> dims, gs, reach, size, local, position, rotation, velocity, angular;
+  struct T {
+    int root = -1;
+    std::vector<double> box;
+    std::vector<int> child1, child2, height, item;
+  } trees[2];
+  std::vector<int> leaf;
+  std::vector<int8_t> which;
+  bool have_trees = false;
+  bm::broad::Bodies bodies() const {
+    return bm::broad::Bodies{type.data(), shape.data(), dims.data(), gs.data(), reach.data(), size.data(),
+                             local.data(), position.data(), rotation.data(), velocity.data(), angular.data()};
+  }
+  bm::broad::Tree tree(int t) const {
+    const T& x = trees[t];
+    return bm::broad::Tree{x.root, x.box.data(), x.child1.data(), x.child2.data(), x.height.data(), x.item.data()};
+  }
+};
+
 struct bm_context {
   std::vector<bm_columns> hs;
   std::string error;
   HeldHost* held = nullptr;
+  BroadHost broad;
 };
 
 namespace {
@@ -313,6 +339,153 @@ int bm_meet_host(bm_context* ctx, int32_t n, const double* a, const double* b, c
   if (oracle_meet(n, a, b, pair, count, normal, points, separations, ids, shift, flags, loss) != n) {
     ctx->error = "oracle meet: a pair of kinds it does not cover";
     return BM_ERR_UNSUPPORTED;
+  }
+  return BM_OK;
+}
+
+}  // extern "C"
+
+// The broadphase passes (bm_broad_*) on the CPU, with the library's own per-body and per-pair
+// arithmetic (bearing_amd/csrc/broad_dev.h) on the copies above. broad_calls counts the rows
+// each pass was given (bounds, walks, holds), for tests to see that the passes ran.
+static int64_t broad_calls[3] = {0, 0, 0};
+extern "C" {
+int64_t stub_broad_rows(int pass) { return pass >= 0 && pass < 3 ? broad_calls[pass] : -1; }
+
+int bm_broad_set_bodies(bm_context* ctx, const bm_broad_bodies* b) {
+  BroadHost& B = ctx->broad;
+  const size_t n = b->n;
+  B.n = b->n;
+  B.type.assign(b->type, b->type + n);
+  B.shape.assign(b->shape, b->shape + n);
+  B.dims.assign(b->dims, b->dims + 3 * n);
+  B.gs.assign(b->gravity_scale, b->gravity_scale + n);
+  B.reach.assign(b->reach, b->reach + n);
+  B.size.assign(b->size, b->size + n);
+  B.local.assign(b->local, b->local + 6 * n);
+  B.position.assign(b->position, b->position + 3 * n);
+  B.rotation.assign(b->rotation, b->rotation + 4 * n);
+  B.velocity.assign(b->velocity, b->velocity + 3 * n);
+  B.angular.assign(b->angular, b->angular + 3 * n);
+  B.have_trees = false;
+  return BM_OK;
+}
+
+int bm_broad_set_state(bm_context* ctx, int32_t n, const int32_t* slots, const double* position,
+                       const double* rotation, const double* velocity, const double* angular) {
+  BroadHost& B = ctx->broad;
+  for (int32_t i = 0; i < n; i++) {
+    const size_t s = slots[i];
+    if (slots[i] < 0 || slots[i] >= B.n) {
+      ctx->error = "bm_broad_set_state: a slot is not a body bm_broad_set_bodies gave";
+      return BM_ERR_ARGUMENT;
+    }
+    for (int k = 0; k < 3; k++) B.position[3 * s + k] = position[3 * s + k];
+    for (int k = 0; k < 4; k++) B.rotation[4 * s + k] = rotation[4 * s + k];
+    for (int k = 0; k < 3; k++) B.velocity[3 * s + k] = velocity[3 * s + k];
+    for (int k = 0; k < 3; k++) B.angular[3 * s + k] = angular[3 * s + k];
+  }
+  return BM_OK;
+}
+
+int bm_broad_keep(bm_context* ctx) {
+  BroadHost& B = ctx->broad;
+  if (ctx->hs.size() != 1 || ctx->hs[0].body_offset[1] != B.n) {
+    ctx->error = "bm_broad_keep: the last batch is not one world of the broadphase's bodies";
+    return BM_ERR_ARGUMENT;
+  }
+  const bm_columns& h = ctx->hs[0];
+  const size_t n = B.n;
+  B.position.assign(h.position, h.position + 3 * n);
+  B.rotation.assign(h.rotation, h.rotation + 4 * n);
+  B.velocity.assign(h.velocity, h.velocity + 3 * n);
+  B.angular.assign(h.angular, h.angular + 3 * n);
+  return BM_OK;
+}
+
+int bm_broad_set_trees(bm_context* ctx, const bm_broad_tree* trees, int32_t n, const int32_t* leaf,
+                       const int8_t* which) {
+  BroadHost& B = ctx->broad;
+  if (n != B.n) {
+    ctx->error = "bm_broad_set_trees: the trees need every body bm_broad_set_bodies gave";
+    return BM_ERR_ARGUMENT;
+  }
+  for (int t = 0; t < 2; t++) {
+    const bm_broad_tree& x = trees[t];
+    const size_t m = x.nodes;
+    BroadHost::T& y = B.trees[t];
+    y.root = x.root;
+    y.box.assign(x.box, x.box + 6 * m);
+    y.child1.assign(x.child1, x.child1 + m);
+    y.child2.assign(x.child2, x.child2 + m);
+    y.height.assign(x.height, x.height + m);
+    y.item.assign(x.item, x.item + m);
+  }
+  B.leaf.assign(leaf, leaf + n);
+  B.which.assign(which, which + n);
+  B.have_trees = true;
+  return BM_OK;
+}
+
+int bm_broad_bounds(bm_context* ctx, int32_t n, const int32_t* list, double dt, double gravity_length, double* out,
+                    uint8_t* refile) {
+  const BroadHost& B = ctx->broad;
+  if (!B.have_trees) {
+    ctx->error = "bm_broad_bounds: no trees (bm_broad_set_trees)";
+    return BM_ERR_ARGUMENT;
+  }
+  const bm::broad::Bodies bodies = B.bodies();
+  const bm::broad::Tree moving = B.tree(1);
+  broad_calls[0] += n;
+  for (int32_t i = 0; i < n; i++) {
+    const int s = list[i];
+    double o[16];
+    bm::broad::bounds(bodies, s, dt, gravity_length, o);
+    for (int k = 0; k < 16; k++) out[(size_t)k * n + i] = o[k];
+    refile[i] = B.which[s] == bm::broad::MOVING_TREE && !bm::broad::contains(moving.box + 6 * (size_t)B.leaf[s], o + 10);
+  }
+  return BM_OK;
+}
+
+int bm_broad_query(bm_context* ctx, int32_t n, const int32_t* moved, int32_t* counts, int32_t* starts, int32_t* hits,
+                   int64_t capacity, int64_t* total) {
+  const BroadHost& B = ctx->broad;
+  std::vector<int32_t> all;
+  broad_calls[1] += n;
+  for (int32_t i = 0; i < n; i++) {
+    const int a = moved[i], t = B.which[a];
+    starts[i] = (int32_t)all.size();
+    counts[i] = 0;
+    if (t < 0) continue;
+    double q[6];
+    for (int k = 0; k < 6; k++) q[k] = B.trees[t].box[6 * (size_t)B.leaf[a] + k];
+    auto put = [&](int b) {
+      if (b != a) all.push_back(b);
+    };
+    bool ok = true;
+    if (B.type[a] != bm::broad::STATIC) ok = bm::broad::query(B.tree(0), q, put);
+    if (ok) ok = bm::broad::query(B.tree(1), q, put);
+    if (!ok) {
+      ctx->error = "bm_broad_query: a tree is deeper than a walk's stack holds";
+      return BM_ERR_DEVICE;
+    }
+    counts[i] = (int32_t)all.size() - starts[i];
+  }
+  *total = (int64_t)all.size();
+  if (*total <= capacity && !all.empty()) std::memcpy(hits, all.data(), 4 * all.size());
+  return BM_OK;
+}
+
+int bm_broad_holds(bm_context* ctx, int32_t n, const int32_t* a, const int32_t* b, uint8_t* apart) {
+  const BroadHost& B = ctx->broad;
+  broad_calls[2] += n;
+  for (int32_t i = 0; i < n; i++) {
+    const int ta = B.which[a[i]], tb = B.which[b[i]];
+    if (ta < 0 || tb < 0) {
+      apart[i] = 0;
+      continue;
+    }
+    apart[i] = !bm::broad::overlap(&B.trees[ta].box[6 * (size_t)B.leaf[a[i]]], &B.trees[tb].box[6 * (size_t)B.leaf[b[i]]]);
   }
   return BM_OK;
 }

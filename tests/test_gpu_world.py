@@ -28,6 +28,22 @@ def test_gpu_world_matches_reference_trajectory(name):
     assert (state["impacts"], state["catches"]) == (fx.meta["impacts"], fx.meta["catches"])
 
 
+@pytest.mark.parametrize("name", ["pyramid", "throw", "rain_small", "darts", "rocks", "terrain", "rain"])
+def test_gpu_world_with_device_broadphase_matches_reference_trajectory(name):
+    # Every step's bounds and fat-box tests, tree walks and pair holds on the device
+    # (bm_broad_*), the host re-filing bodies and taking what the walks met in order.
+    print("trajectory", name, "device broadphase", flush=True)
+    fx = traj(name)
+    w = make_world(fx, device=0)
+    w.set_broadphase("device")
+    matched, why = run(w, fx)
+    state = w.read()
+    w.close()
+    assert why is None, why
+    assert matched == fx.meta["steps"]
+    assert (state["impacts"], state["catches"]) == (fx.meta["impacts"], fx.meta["catches"])
+
+
 def test_gpu_world_refuses_what_it_does_not_handle():
     refuses(lambda: native.World(0))
 
@@ -53,10 +69,14 @@ def test_gpu_mixed_pile_first_steps():
     # The mixed pile at full size (20,001 bodies: boxes, balls, capsules and hulls onto a 256 x 256
     # heightfield), its first 150 steps as the landing starts, every step's transforms the
     # reference's.
-    print("trajectory mixed_pile, 150 steps", flush=True)
+    # By default a world this size takes its broadphase passes on the device; the host's
+    # give the same steps.
     fx = traj("mixed_pile")
-    w = make_world(fx, device=0)
-    matched, why = run(w, fx, steps=150)
-    w.close()
-    assert why is None, why
-    assert matched == 150
+    for mode in ("auto", "host"):
+        print("trajectory mixed_pile, 150 steps, broadphase", mode, flush=True)
+        w = make_world(fx, device=0)
+        w.set_broadphase(mode)
+        matched, why = run(w, fx, steps=150)
+        w.close()
+        assert why is None, why
+        assert matched == 150

@@ -135,6 +135,27 @@ def test_host_world_with_workers_matches_reference_trajectory(lib, name):
     assert matched == fx.meta["steps"]
 
 
+@pytest.mark.parametrize("name", ["pyramid", "tower", "throw", "rain_small", "pile_small", "darts", "capsules", "rocks",
+                                  "terrain"])
+def test_host_world_with_device_broadphase_matches_reference_trajectory(lib, name):
+    # Every step's bounds, fat-box tests, tree walks and holds through the bm_broad_* calls
+    # (answered here by copies of what the world sent them, so a body the host changed without
+    # sending it keeps its old state there), the host re-filing and taking the results in order.
+    fx = traj(name)
+    w = make_world(fx, lib=lib)
+    w.set_broadphase("device")
+    lib.stub_broad_rows.restype = ctypes.c_int64
+    before = [lib.stub_broad_rows(k) for k in range(3)]
+    matched, why = run(w, fx)
+    assert why is None, why
+    assert matched == fx.meta["steps"]
+    state = w.read()
+    assert (state["impacts"], state["catches"]) == (fx.meta["impacts"], fx.meta["catches"])
+    rows = [lib.stub_broad_rows(k) - before[k] for k in range(3)]
+    print(f"{name}: bounds, walks and holds of {rows} rows through bm_broad_*")
+    assert all(r > 0 for r in rows), rows
+
+
 def test_tapped_columns_replay_the_step(lib):
     # bm_world_tap shows the columns a step hands its contact phase: replayed through a
     # context of their own (upload, colour, soft step, download) they give the world's own
