@@ -18,13 +18,16 @@ __global__ void __launch_bounds__(256) bounds_kernel(BoundsDev d) {
   broad::bounds(d.bodies, s, d.dt, d.gl, o);
   for (int k = 0; k < 16; k++) d.out[(size_t)k * d.n + i] = o[k];
   uint8_t r = 0;
-  if (d.which[s] == broad::MOVING_TREE) r = broad::contains(d.moving.box + 6 * (size_t)d.leaf[s], o + 10) ? 0 : 1;
+  if (d.which[s] == broad::MOVING_TREE) r = broad::contains(d.moving.node[d.leaf[s]].box, o + 10) ? 0 : 1;
   d.refile[i] = r;
 }
 
-// One marked body a lane: the walks once to count the leaves, a reservation of that many
-// places, then the walks again to write them, so each body's leaves stay in walk order.
-__global__ void __launch_bounds__(256) query_kernel(QueryDev d) {
+// One marked body a lane, a wave a workgroup: the walks once to count the leaves, a
+// reservation of that many places, then the walks again to write them, so each body's leaves
+// stay in walk order. Each lane's stack lives in LDS, entry k of lane l at k * 64 + l.
+__global__ void __launch_bounds__(64) query_kernel(QueryDev d) {
+  __shared__ int stacks[broad::STACK * 64];
+  int* stack = stacks + threadIdx.x;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= d.n) return;
   const int a = d.moved[i];
@@ -33,7 +36,7 @@ __global__ void __launch_bounds__(256) query_kernel(QueryDev d) {
   d.starts[i] = 0;
   if (t < 0) return;
   double q[6];
-  const double* lb = d.trees[t].box + 6 * (size_t)d.leaf[a];
+  const double* lb = d.trees[t].node[d.leaf[a]].box;
   for (int k = 0; k < 6; k++) q[k] = lb[k];
   const bool stat = d.type[a] == broad::STATIC;
   int c = 0;
@@ -41,8 +44,8 @@ __global__ void __launch_bounds__(256) query_kernel(QueryDev d) {
     if (b != a) c++;
   };
   bool ok = true;
-  if (!stat) ok = broad::query(d.trees[broad::STATIC_TREE], q, count);
-  if (ok) ok = broad::query(d.trees[broad::MOVING_TREE], q, count);
+  if (!stat) ok = broad::query(d.trees[broad::STATIC_TREE], q, stack, 64, count);
+  if (ok) ok = broad::query(d.trees[broad::MOVING_TREE], q, stack, 64, count);
   if (!ok) {
     atomicOr(d.flags, 1);
     return;
@@ -55,8 +58,8 @@ __global__ void __launch_bounds__(256) query_kernel(QueryDev d) {
   auto put = [&](int b) {
     if (b != a) d.hits[at++] = b;
   };
-  if (!stat) broad::query(d.trees[broad::STATIC_TREE], q, put);
-  broad::query(d.trees[broad::MOVING_TREE], q, put);
+  if (!stat) broad::query(d.trees[broad::STATIC_TREE], q, stack, 64, put);
+  broad::query(d.trees[broad::MOVING_TREE], q, stack, 64, put);
 }
 
 __global__ void __launch_bounds__(256) holds_kernel(HoldsDev d) {
@@ -68,8 +71,8 @@ __global__ void __launch_bounds__(256) holds_kernel(HoldsDev d) {
     d.apart[i] = 0;
     return;
   }
-  const double* fa = d.trees[d.which[a]].box + 6 * (size_t)d.leaf[a];
-  const double* fb = d.trees[d.which[b]].box + 6 * (size_t)d.leaf[b];
+  const double* fa = d.trees[d.which[a]].node[d.leaf[a]].box;
+  const double* fb = d.trees[d.which[b]].node[d.leaf[b]].box;
   d.apart[i] = broad::overlap(fa, fb) ? 0 : 1;
 }
 
@@ -95,7 +98,7 @@ void launch_bounds(const BoundsDev& d, hipStream_t stream) {
 }
 
 void launch_query(const QueryDev& d, hipStream_t stream) {
-  if (d.n > 0) query_kernel<<<(d.n + 255) / 256, 256, 0, stream>>>(d);
+  if (d.n > 0) query_kernel<<<(d.n + 63) / 64, 64, 0, stream>>>(d);
 }
 
 void launch_holds(const HoldsDev& d, hipStream_t stream) {

@@ -19,8 +19,9 @@ constexpr int DYNAMIC = 0, STATIC = 1;
 constexpr int BOX = 0, SPHERE = 1, CAPSULE = 2, HULL = 3, MESH = 5;
 constexpr int STATIC_TREE = 0, MOVING_TREE = 1;
 // The deepest a walk's stack may grow. The trees are AVL-balanced (src/tree.js:219-252), so a
-// tree of 2^24 leaves stays within about 1.44 * 24 + 2 levels; a deeper one is reported.
-constexpr int STACK = 96;
+// tree of 2^24 leaves stays within about 1.44 * 24 + 2 levels, and a walk holds at most one
+// entry a level and one more; a deeper tree is reported.
+constexpr int STACK = 64;
 
 // What the bounds read of each body, by slot. `local`: for a hull or a heightfield, the centre
 // (3) and half extents (3) of the box round its corners or samples in its own frame.
@@ -30,12 +31,17 @@ struct Bodies {
   const double *position, *rotation, *velocity, *angular;
 };
 
-// One AABB tree as src/tree.js keeps it: 6 numbers a node (min x y z, max x y z), children,
-// height (0 for a leaf) and a leaf's item (its body's slot).
+// A node of an AABB tree as src/tree.js keeps it, in one 64-byte record: its box (min x y z,
+// max x y z), children, height (0 for a leaf) and a leaf's item (its body's slot).
+struct Node {
+  double box[6];
+  int32_t child1, child2, height, item;
+};
+static_assert(sizeof(Node) == 64, "a node is one 64-byte record");
+
 struct Tree {
   int root;
-  const double* box;
-  const int *child1, *child2, *height, *item;
+  const Node* node;
 };
 
 // src/core/math.js:223-228 (axesOf)
@@ -103,24 +109,24 @@ __host__ __device__ __forceinline__ bool overlap(const double* fa, const double*
 }
 
 // src/tree.js:259-275 (query): visit(item) for each leaf whose box overlaps q, child1's side
-// first. False if the walk needed more than STACK entries (it then stops).
+// first. The walk's stack holds STACK entries, entry k at stack[k * stride]; false if the walk
+// needed more (it then stops).
 template <class F>
-__host__ __device__ __forceinline__ bool query(const Tree& t, const double* q, F&& visit) {
+__host__ __device__ __forceinline__ bool query(const Tree& t, const double* q, int* stack, int stride, F&& visit) {
   if (t.root < 0) return true;
-  int stack[STACK];
-  int top = 0;
-  stack[top++] = t.root;
+  stack[0] = t.root;
+  int top = 1;
   while (top > 0) {
-    const int n = stack[--top];
-    const double* b = t.box + 6 * n;
+    const Node& n = t.node[stack[--top * stride]];
+    const double* b = n.box;
     if (b[0] > q[3] || b[3] < q[0] || b[1] > q[4] || b[4] < q[1] || b[2] > q[5] || b[5] < q[2]) continue;
-    if (t.height[n] == 0) {
-      visit(t.item[n]);
+    if (n.height == 0) {
+      visit(n.item);
       continue;
     }
     if (top + 2 > STACK) return false;
-    stack[top++] = t.child2[n];
-    stack[top++] = t.child1[n];
+    stack[top++ * stride] = n.child2;
+    stack[top++ * stride] = n.child1;
   }
   return true;
 }

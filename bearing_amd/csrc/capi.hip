@@ -167,11 +167,11 @@ struct bm_context {
   std::vector<FieldShape> field_shapes;
   std::vector<int32_t> key_offsets;
   // The broadphase passes (bm_broad_*): one world's bodies by slot (25 doubles, then type and
-  // shape), its two trees (boxes; child1, child2, height and item as four runs of ints), each
-  // body's leaf and tree, and the passes' rows and results.
+  // shape), its two trees (a 64-byte record a node), each body's leaf and tree, and the passes'
+  // rows and results.
   struct Broad {
     int32_t n = 0;
-    Scratch bodies, trees[2], links[2], leaf, rows, out, hits, counters;
+    Scratch bodies, trees[2], leaf, rows, out, hits, counters;
     int32_t root[2] = {-1, -1}, nodes[2] = {0, 0};
     bool have_trees = false;
   } broad;
@@ -418,8 +418,7 @@ int bm_destroy(bm_context* ctx) {
                      &ctx->colour_sort, &ctx->wide_bar, &ctx->wide_pick})
     if (s->p) (void)hipFree(s->p);
   bm_context::Broad& bb = ctx->broad;
-  for (Scratch* s : {&bb.bodies, &bb.trees[0], &bb.trees[1], &bb.links[0], &bb.links[1], &bb.leaf, &bb.rows, &bb.out,
-                     &bb.hits, &bb.counters})
+  for (Scratch* s : {&bb.bodies, &bb.trees[0], &bb.trees[1], &bb.leaf, &bb.rows, &bb.out, &bb.hits, &bb.counters})
     if (s->p) (void)hipFree(s->p);
   bm::held_free(ctx->held);
   if (ctx->begin) (void)hipEventDestroy(ctx->begin);
@@ -1382,10 +1381,7 @@ bm::broad::Bodies broad_bodies(bm_context* ctx) {
 
 bm::broad::Tree broad_tree(bm_context* ctx, int t) {
   const bm_context::Broad& bb = ctx->broad;
-  const int* links = static_cast<const int*>(bb.links[t].p);
-  const size_t n = bb.nodes[t];
-  return bm::broad::Tree{bb.root[t], static_cast<const double*>(bb.trees[t].p), links, links + n, links + 2 * n,
-                         links + 3 * n};
+  return bm::broad::Tree{bb.root[t], static_cast<const bm::broad::Node*>(bb.trees[t].p)};
 }
 
 // The leaf (n ints) and tree (n bytes) of each body, in one block.
@@ -1523,18 +1519,33 @@ int bm_broad_set_trees(bm_context* ctx, const bm_broad_tree* trees, int32_t n, c
   for (int t = 0; t < 2; t++) {
     const bm_broad_tree& T = trees[t];
     const size_t m = T.nodes;
-    if (int rc = grow(ctx, bb.trees[t], 48 * m + 256, "bm_broad_set_trees")) return rc;
-    if (int rc = grow(ctx, bb.links[t], 16 * m + 256, "bm_broad_set_trees")) return rc;
+    if (int rc = grow(ctx, bb.trees[t], sizeof(bm::broad::Node) * m + 256, "bm_broad_set_trees")) return rc;
     bb.root[t] = T.root;
     bb.nodes[t] = T.nodes;
     if (m == 0) continue;
-    if (int rc = stage_up(ctx, bb.trees[t].p, {{0, T.box, 48 * m}}, "bm_broad_set_trees")) return rc;
-    HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_broad_set_trees: copy");
-    if (int rc = stage_up(ctx, bb.links[t].p,
-                          {{0, T.child1, 4 * m}, {4 * m, T.child2, 4 * m},
-                           {8 * m, T.height, 4 * m}, {12 * m, T.item, 4 * m}},
-                          "bm_broad_set_trees"))
+    // Each node's box and links into one record, straight into the staging buffer.
+    if (int rc = grow_pinned(ctx, &ctx->pstage, &ctx->pstage_cap, sizeof(bm::broad::Node) * m, "bm_broad_set_trees"))
       return rc;
+    bm::broad::Node* nodes = static_cast<bm::broad::Node*>(ctx->pstage);
+    auto fill = [&](size_t i0, size_t i1) {
+      for (size_t i = i0; i < i1; i++) {
+        bm::broad::Node& o = nodes[i];
+        for (int k = 0; k < 6; k++) o.box[k] = T.box[6 * i + k];
+        o.child1 = T.child1[i]; o.child2 = T.child2[i]; o.height = T.height[i]; o.item = T.item[i];
+      }
+    };
+    if (m >= 32768) {
+      const int threads = std::max(1, std::min(8, (int)std::thread::hardware_concurrency()));
+      std::vector<std::thread> pool;
+      for (int k = 1; k < threads; k++) pool.emplace_back(fill, m * k / threads, m * (k + 1) / threads);
+      fill(0, m / threads);
+      for (std::thread& th : pool) th.join();
+    } else {
+      fill(0, m);
+    }
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(bb.trees[t].p, ctx->pstage, sizeof(bm::broad::Node) * m, hipMemcpyHostToDevice,
+                                    ctx->stream),
+                "bm_broad_set_trees: copy");
     HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_broad_set_trees: copy");
   }
   const size_t N = n;

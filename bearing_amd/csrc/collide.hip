@@ -1351,6 +1351,9 @@ __device__ __forceinline__ void capsule_hull(Scratch& S, const Shape& a, const S
 }
 
 // src/collide.js:204-242 (collide, pair), for spheres and boxes.
+// HULLS: the pairs with a hull (or a box against a hull); the rest, without. Each kernel is
+// built for one kind of pair, so the other kind's working arrays take no per-lane memory.
+template <bool HULLS>
 __device__ __forceinline__ int collide(const Shape& a, const Shape& b, double margin, double tolerance, Manifold& out) {
   Scratch S;
   out.count = 0;
@@ -1360,16 +1363,20 @@ __device__ __forceinline__ int collide(const Shape& a, const Shape& b, double ma
   const bool swap = a.kind > b.kind;
   const Shape& lo = swap ? b : a;
   const Shape& hi = swap ? a : b;
-  if (lo.kind == SPHERE && hi.kind == SPHERE) sphere_sphere(S, lo, hi, margin, out);
-  else if (lo.kind == SPHERE && hi.kind == CAPSULE) sphere_capsule(S, lo, hi, margin, out);
-  else if (lo.kind == SPHERE && hi.kind == BOX) sphere_box(S, lo, hi, margin, out);
-  else if (lo.kind == CAPSULE && hi.kind == CAPSULE) capsule_capsule(S, lo, hi, margin, tolerance, out);
-  else if (lo.kind == CAPSULE && hi.kind == BOX) capsule_box(S, lo, hi, margin, tolerance, out);
-  else if (lo.kind == BOX && hi.kind == BOX) box_box(S, lo, hi, margin, tolerance, out);
-  else if (lo.kind == SPHERE && hi.kind == HULL) sphere_hull(S, lo, hi, margin, out);
-  else if (lo.kind == CAPSULE && hi.kind == HULL) capsule_hull(S, lo, hi, margin, tolerance, out);
-  else if ((lo.kind == BOX || lo.kind == HULL) && hi.kind == HULL) hull_hull(S, lo, hi, margin, tolerance, out);
-  else return -1;
+  if constexpr (!HULLS) {
+    if (lo.kind == SPHERE && hi.kind == SPHERE) sphere_sphere(S, lo, hi, margin, out);
+    else if (lo.kind == SPHERE && hi.kind == CAPSULE) sphere_capsule(S, lo, hi, margin, out);
+    else if (lo.kind == SPHERE && hi.kind == BOX) sphere_box(S, lo, hi, margin, out);
+    else if (lo.kind == CAPSULE && hi.kind == CAPSULE) capsule_capsule(S, lo, hi, margin, tolerance, out);
+    else if (lo.kind == CAPSULE && hi.kind == BOX) capsule_box(S, lo, hi, margin, tolerance, out);
+    else if (lo.kind == BOX && hi.kind == BOX) box_box(S, lo, hi, margin, tolerance, out);
+    else return -1;
+  } else {
+    if (lo.kind == SPHERE && hi.kind == HULL) sphere_hull(S, lo, hi, margin, out);
+    else if (lo.kind == CAPSULE && hi.kind == HULL) capsule_hull(S, lo, hi, margin, tolerance, out);
+    else if ((lo.kind == BOX || lo.kind == HULL) && hi.kind == HULL) hull_hull(S, lo, hi, margin, tolerance, out);
+    else return -1;
+  }
   if (swap) { out.normal[0] = -out.normal[0]; out.normal[1] = -out.normal[1]; out.normal[2] = -out.normal[2]; }
   return out.count;
 }
@@ -1736,27 +1743,32 @@ __device__ __forceinline__ double least_separation(const Manifold& m) {
 
 // src/contact.js:627-640 (meet) for two whole bodies. Returns the point count (0: none),
 // or -1 for a pair of kinds not handled; flags 1 shifted, 2 a time of impact asked.
+template <bool HULLS>
 __device__ __forceinline__ int meet(Shape& sa, Shape& sb, const Motion& ma, const Motion& mb, const double* gravity,
                                     double dt, double margin, double tolerance, Manifold& out, double* shift,
                                     int& flags) {
   flags = 0;
-  const int n = collide(sa, sb, margin, tolerance, out);
-  if (n <= 0) return n;
-  const double moves = least_separation(out) > tolerance ? moving(ma, mb, gravity, dt) : 0;
-  if (moves > tolerance) {
-    Flight F;
-    const double t = impact(F, sa, sb, ma, mb, dt, tolerance);
-    flags |= 2;
-    const double ahead = t >= 0 ? moves * (1 - t / dt) + 2 * tolerance : 0;
-    if (t == -1) return 0;
-    if (t != UNSURE) {
-      const int m = collide(sa, sb, ahead, tolerance, out);
-      if (m <= 0) return m;
+  // Two rounds of one collide() call: the step's start, then (after a time of impact) the
+  // moment of impact with the shapes placed there, so the dispatch and its scratch are
+  // inlined once.
+  Flight F;
+  double t = 0, grow = margin;
+  for (int round = 0;; round++) {
+    const int n = collide<HULLS>(sa, sb, grow, tolerance, out);
+    if (n <= 0) return n;
+    if (round == 1) {
       back(F, sa, sb, ma, mb, t, out, shift);
       flags |= 1;
+      return out.count;
     }
+    const double moves = least_separation(out) > tolerance ? moving(ma, mb, gravity, dt) : 0;
+    if (!(moves > tolerance)) return out.count;
+    t = impact(F, sa, sb, ma, mb, dt, tolerance);
+    flags |= 2;
+    grow = t >= 0 ? moves * (1 - t / dt) + 2 * tolerance : 0;
+    if (t == -1) return 0;
+    if (t == UNSURE) return out.count;
   }
-  return out.count;
 }
 
 
@@ -1764,10 +1776,16 @@ __device__ __forceinline__ int meet(Shape& sa, Shape& sb, const Motion& ma, cons
 
 }  // namespace narrow
 
-// One pair a lane: shape records (kind, centre 3, axes 9, half 3, weight 3) in, manifolds out.
+// One pair a lane: shape records (kind, centre 3, axes 9, half 3, weight 3) in, manifolds out;
+// the pairs with a hull in the HULLS kernel, the others in the other one.
+__device__ __forceinline__ bool has_hull(const double* a, const double* b) {
+  return (int)a[0] == narrow::HULL || (int)b[0] == narrow::HULL;
+}
+
+template <bool HULLS>
 __global__ void collide_kernel(CollideDev d) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= d.n) return;
+  if (k >= d.n || has_hull(d.a + 19 * (long)k, d.b + 19 * (long)k) != HULLS) return;
   narrow::Shape s[2];
   const double* src[2] = {d.a + 19 * (long)k, d.b + 19 * (long)k};
   for (int t = 0; t < 2; t++) {
@@ -1782,7 +1800,7 @@ __global__ void collide_kernel(CollideDev d) {
   }
   narrow::Manifold m;
   m.count = 0;
-  const int n = narrow::collide(s[0], s[1], d.margin[k], d.tolerance[k], m);
+  const int n = narrow::collide<HULLS>(s[0], s[1], d.margin[k], d.tolerance[k], m);
   d.count[k] = n;
   for (int i = 0; i < 3; i++) d.normal[3 * k + i] = m.normal[i];
   for (int i = 0; i < 3 * n; i++) d.points[12 * (long)k + i] = m.points[i];
@@ -1793,9 +1811,10 @@ __global__ void collide_kernel(CollideDev d) {
 
 // One pair a lane: body records (32 numbers) and pair records (margin, tolerance, dt,
 // gravity 3) in; meet's manifold, shift and flags out.
+template <bool HULLS>
 __global__ void meet_kernel(MeetDev d) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= d.n) return;
+  if (k >= d.n || has_hull(d.a + 32 * (long)k, d.b + 32 * (long)k) != HULLS) return;
   narrow::Shape s[2];
   narrow::Motion mo[2];
   const double* src[2] = {d.a + 32 * (long)k, d.b + 32 * (long)k};
@@ -1822,7 +1841,7 @@ __global__ void meet_kernel(MeetDev d) {
   double shift[12];
   for (int i = 0; i < 12; i++) shift[i] = 0;
   int flags = 0;
-  const int n = narrow::meet(s[0], s[1], mo[0], mo[1], pr + 3, pr[2], pr[0], pr[1], m, shift, flags);
+  const int n = narrow::meet<HULLS>(s[0], s[1], mo[0], mo[1], pr + 3, pr[2], pr[0], pr[1], m, shift, flags);
   d.count[k] = n;
   d.flags[k] = flags;
   for (int i = 0; i < 3; i++) d.normal[3 * k + i] = m.normal[i];
@@ -2128,12 +2147,14 @@ void launch_level_clear(const LevelDev& d, hipStream_t stream) {
 
 void launch_meet(const MeetDev& d, hipStream_t stream) {
   if (d.n == 0) return;
-  hipLaunchKernelGGL(meet_kernel, dim3((d.n + 63) / 64), dim3(64), 0, stream, d);
+  hipLaunchKernelGGL(meet_kernel<false>, dim3((d.n + 63) / 64), dim3(64), 0, stream, d);
+  hipLaunchKernelGGL(meet_kernel<true>, dim3((d.n + 63) / 64), dim3(64), 0, stream, d);
 }
 
 void launch_collide(const CollideDev& d, hipStream_t stream) {
   if (d.n == 0) return;
-  hipLaunchKernelGGL(collide_kernel, dim3((d.n + 63) / 64), dim3(64), 0, stream, d);
+  hipLaunchKernelGGL(collide_kernel<false>, dim3((d.n + 63) / 64), dim3(64), 0, stream, d);
+  hipLaunchKernelGGL(collide_kernel<true>, dim3((d.n + 63) / 64), dim3(64), 0, stream, d);
 }
 
 }  // namespace bm

@@ -43,8 +43,7 @@ struct BroadHost {
   std::vector<double> dims, gs, reach, size, local, position, rotation, velocity, angular;
   struct T {
     int root = -1;
-    std::vector<double> box;
-    std::vector<int> child1, child2, height, item;
+    std::vector<bm::broad::Node> node;
   } trees[2];
   std::vector<int> leaf;
   std::vector<int8_t> which;
@@ -53,10 +52,7 @@ struct BroadHost {
     return bm::broad::Bodies{type.data(), shape.data(), dims.data(), gs.data(), reach.data(), size.data(),
                              local.data(), position.data(), rotation.data(), velocity.data(), angular.data()};
   }
-  bm::broad::Tree tree(int t) const {
-    const T& x = trees[t];
-    return bm::broad::Tree{x.root, x.box.data(), x.child1.data(), x.child2.data(), x.height.data(), x.item.data()};
-  }
+  bm::broad::Tree tree(int t) const { return bm::broad::Tree{trees[t].root, trees[t].node.data()}; }
 };
 
 struct bm_context {
@@ -415,11 +411,12 @@ int bm_broad_set_trees(bm_context* ctx, const bm_broad_tree* trees, int32_t n, c
     const size_t m = x.nodes;
     BroadHost::T& y = B.trees[t];
     y.root = x.root;
-    y.box.assign(x.box, x.box + 6 * m);
-    y.child1.assign(x.child1, x.child1 + m);
-    y.child2.assign(x.child2, x.child2 + m);
-    y.height.assign(x.height, x.height + m);
-    y.item.assign(x.item, x.item + m);
+    y.node.resize(m);
+    for (size_t i = 0; i < m; i++) {
+      for (int k = 0; k < 6; k++) y.node[i].box[k] = x.box[6 * i + k];
+      y.node[i].child1 = x.child1[i]; y.node[i].child2 = x.child2[i];
+      y.node[i].height = x.height[i]; y.node[i].item = x.item[i];
+    }
   }
   B.leaf.assign(leaf, leaf + n);
   B.which.assign(which, which + n);
@@ -442,7 +439,7 @@ int bm_broad_bounds(bm_context* ctx, int32_t n, const int32_t* list, double dt, 
     double o[16];
     bm::broad::bounds(bodies, s, dt, gravity_length, o);
     for (int k = 0; k < 16; k++) out[(size_t)k * n + i] = o[k];
-    refile[i] = B.which[s] == bm::broad::MOVING_TREE && !bm::broad::contains(moving.box + 6 * (size_t)B.leaf[s], o + 10);
+    refile[i] = B.which[s] == bm::broad::MOVING_TREE && !bm::broad::contains(moving.node[B.leaf[s]].box, o + 10);
   }
   return BM_OK;
 }
@@ -458,13 +455,14 @@ int bm_broad_query(bm_context* ctx, int32_t n, const int32_t* moved, int32_t* co
     counts[i] = 0;
     if (t < 0) continue;
     double q[6];
-    for (int k = 0; k < 6; k++) q[k] = B.trees[t].box[6 * (size_t)B.leaf[a] + k];
+    for (int k = 0; k < 6; k++) q[k] = B.trees[t].node[B.leaf[a]].box[k];
     auto put = [&](int b) {
       if (b != a) all.push_back(b);
     };
     bool ok = true;
-    if (B.type[a] != bm::broad::STATIC) ok = bm::broad::query(B.tree(0), q, put);
-    if (ok) ok = bm::broad::query(B.tree(1), q, put);
+    int stack[bm::broad::STACK];
+    if (B.type[a] != bm::broad::STATIC) ok = bm::broad::query(B.tree(0), q, stack, 1, put);
+    if (ok) ok = bm::broad::query(B.tree(1), q, stack, 1, put);
     if (!ok) {
       ctx->error = "bm_broad_query: a tree is deeper than a walk's stack holds";
       return BM_ERR_DEVICE;
@@ -485,7 +483,7 @@ int bm_broad_holds(bm_context* ctx, int32_t n, const int32_t* a, const int32_t* 
       apart[i] = 0;
       continue;
     }
-    apart[i] = !bm::broad::overlap(&B.trees[ta].box[6 * (size_t)B.leaf[a[i]]], &B.trees[tb].box[6 * (size_t)B.leaf[b[i]]]);
+    apart[i] = !bm::broad::overlap(B.trees[ta].node[B.leaf[a[i]]].box, B.trees[tb].node[B.leaf[b[i]]].box);
   }
   return BM_OK;
 }
