@@ -134,6 +134,10 @@ struct bm_context {
   int32_t resident = 1;     // keep velocities in LDS when they fit
   bool laid_out = false;    // rows are laid out by colour (checked at upload, or by bm_colour)
   Scratch deg, adj, keys, key_offset, world_of;
+  // Worlds by decreasing manifold count, the order the colouring's and soft step's workgroups
+  // take them in (a batch of more than one world), and its host copy.
+  Scratch world_order;
+  std::vector<int32_t> order_host;
   Scratch pairs_in, pairs_out;
   // The hull table (bm_set_hulls): ints (info, start, loops, edges, sides) and doubles
   // (dinfo, vertices, planes) on the device, and where each part starts.
@@ -412,7 +416,7 @@ int bm_destroy(bm_context* ctx) {
   if (ctx->stage) (void)hipHostFree(ctx->stage);
   if (ctx->pstage) (void)hipHostFree(ctx->pstage);
   for (Scratch* s : {&ctx->deg, &ctx->adj, &ctx->keys, &ctx->key_offset, &ctx->world_of, &ctx->pairs_in,
-                     &ctx->pairs_out})
+                     &ctx->pairs_out, &ctx->world_order})
     if (s->p) (void)hipFree(s->p);
   for (Scratch* s : {&ctx->hull_ints, &ctx->hull_doubles, &ctx->field_ints, &ctx->field_doubles, &ctx->level_scratch,
                      &ctx->colour_sort, &ctx->wide_bar, &ctx->wide_pick})
@@ -619,6 +623,23 @@ int upload_parts(bm_context* ctx, int nparts, const bm_columns* const* parts, co
   ctx->moff_host = ctx->join_manifold;
   ctx->boff_host = ctx->join_body;
   ctx->dev.n_worlds = ctx->n_worlds;
+  ctx->dev.world_order = nullptr;
+  if (ctx->n_worlds > 1) {
+    // The largest worlds first (longest-processing-time order): workgroups are handed out in
+    // order as others finish, so an uneven batch's longest worlds do not start last.
+    const int n = ctx->n_worlds;
+    std::vector<int32_t>& order = ctx->order_host;
+    order.resize(n);
+    for (int w = 0; w < n; w++) order[w] = w;
+    const std::vector<int32_t>& moff = ctx->moff_host;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return moff[x + 1] - moff[x] > moff[y + 1] - moff[y]; });
+    if (int rc = grow(ctx, ctx->world_order, sizeof(int32_t) * (size_t)n, who)) return rc;
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->world_order.p, order.data(), sizeof(int32_t) * (size_t)n,
+                                    hipMemcpyHostToDevice, ctx->stream),
+                (std::string(who) + ": hipMemcpyAsync").c_str());
+    ctx->dev.world_order = static_cast<const int*>(ctx->world_order.p);
+  }
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), (std::string(who) + ": hipStreamSynchronize").c_str());
   ctx->uploaded = true;
   ctx->stored = false;
@@ -813,6 +834,7 @@ int bm_colour(bm_context* ctx) {
   cd.adj = static_cast<int*>(ctx->adj.p);
   cd.keys = static_cast<uint64_t*>(ctx->keys.p);
   cd.key_offset = static_cast<const int*>(ctx->key_offset.p);
+  cd.world_order = ctx->dev.world_order;
   // Colour in LDS when every world's data fits there.
   if (ctx->colour_lds_limit == 0) ctx->colour_lds_limit = bm::colour_lds_limit(ctx->device);
   size_t lds = 0;

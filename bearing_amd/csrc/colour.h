@@ -19,6 +19,7 @@ struct ColourDev {
   uint64_t* keys;                  // scratch: key_offset[n_worlds]
   const int* key_offset;           // per world, a power of two at least its manifolds
   int in_lds;                      // 1: each world's colouring and sort run in dynamic LDS
+  const int* world_order;          // the order workgroups take the worlds in (null: as numbered)
 };
 
 // Dynamic LDS a world takes when coloured in LDS: first the held masks, the manifolds' ends

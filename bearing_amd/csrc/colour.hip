@@ -48,7 +48,8 @@ __global__ void __launch_bounds__(256) colour_kernel(ColourDev d) {
   __shared__ int flag;
   __shared__ int runs[RESTING + 2];
   extern __shared__ __align__(16) unsigned char lds[];
-  for (int wld = blockIdx.x; wld < d.n_worlds; wld += gridDim.x) {
+  for (int at = blockIdx.x; at < d.n_worlds; at += gridDim.x) {
+    const int wld = d.world_order ? d.world_order[at] : at;
     if (d.in_lds) {
       const int boff = d.body_offset[wld], moff = d.manifold_offset[wld];
       const int nb = d.body_offset[wld + 1] - boff, n = d.manifold_offset[wld + 1] - moff;

@@ -31,6 +31,9 @@ struct SoftDev {
   uint8_t* pair_points;
   uint32_t* pair_pid;
   double *pair_normal_impulse, *pair_twist, *pair_friction, *pair_slip;
+  // The order workgroups take the worlds in (null: as numbered): the largest first, so the
+  // longest worlds of an uneven batch start first and the launch ends with short ones.
+  const int* world_order;
 };
 
 // One workgroup per world; lds_bytes > 0 keeps each world's body velocities in that

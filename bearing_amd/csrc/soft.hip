@@ -935,7 +935,8 @@ __global__ void __launch_bounds__(THREADS, WAVES) soft_step_kernel(SoftDev d) {
   __shared__ int bounce_list[BOUNCE_CHUNK];
   __shared__ int bounce_waves[THREADS / 64];
   extern __shared__ double lds[];
-  for (int wld = blockIdx.x; wld < d.n_worlds; wld += gridDim.x) {
+  for (int at = blockIdx.x; at < d.n_worlds; at += gridDim.x) {
+    const int wld = d.world_order ? d.world_order[at] : at;
     const int boff = d.body_offset[wld], moff = d.manifold_offset[wld];
     const int nbodies = d.body_offset[wld + 1] - boff;
     const int aoff = d.awake_offset[wld], awake = d.awake_offset[wld + 1] - aoff;
