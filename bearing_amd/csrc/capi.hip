@@ -147,6 +147,7 @@ struct bm_context {
   // The heightfields (bm_set_fields): info and offsets, spacing and corner, heights; and a
   // host copy of each one's shape, for sizing touchLevel's scratch.
   Scratch field_ints, field_doubles, level_scratch;
+  Scratch level_packed;  // bm_level_packed's starts and packed rows
   // A large world's colouring: sort keys and values, and the radix sort's own scratch; which
   // path the last bm_colour took (0 HBM fixed point, 1 in LDS, 2 streamed and sorted).
   Scratch colour_sort;
@@ -419,7 +420,7 @@ int bm_destroy(bm_context* ctx) {
                      &ctx->pairs_out, &ctx->world_order})
     if (s->p) (void)hipFree(s->p);
   for (Scratch* s : {&ctx->hull_ints, &ctx->hull_doubles, &ctx->field_ints, &ctx->field_doubles, &ctx->level_scratch,
-                     &ctx->colour_sort, &ctx->wide_bar, &ctx->wide_pick})
+                     &ctx->colour_sort, &ctx->wide_bar, &ctx->wide_pick, &ctx->level_packed})
     if (s->p) (void)hipFree(s->p);
   bm_context::Broad& bb = ctx->broad;
   for (Scratch* s : {&bb.bodies, &bb.trees[0], &bb.trees[1], &bb.leaf, &bb.rows, &bb.out, &bb.hits, &bb.counters})
@@ -1154,27 +1155,30 @@ int bm_set_fields(bm_context* ctx, int32_t n, const int32_t* columns, const int3
   return BM_OK;
 }
 
-int bm_level_host(bm_context* ctx, int32_t n, const int32_t* field, const double* position, const double* axes,
-                  const double* shapes, const double* boxes, const double* margin, const double* tolerance, int32_t cap,
-                  int32_t* manifolds, int32_t* count, double* normal, double* points, double* separations,
-                  uint32_t* ids) {
-  if (!ctx) return BM_ERR_ARGUMENT;
-  ctx->error.clear();
-  if (n < 0 || cap < 1) return fail(ctx, BM_ERR_ARGUMENT, "bm_level_host: n must be 0 or more, cap 1 or more");
-  if (n == 0) return BM_OK;
-  if (!field || !position || !axes || !shapes || !boxes || !margin || !tolerance || !manifolds || !count || !normal ||
-      !points || !separations || !ids)
-    return fail(ctx, BM_ERR_ARGUMENT, "bm_level_host: null argument");
-  // Each lane's scratch: 7 points for each triangle under its box (src/mesh.js:388-401).
+}  // extern "C"
+
+namespace {
+
+// The level pass up to its launch, for bm_level_host and bm_level_packed: the inputs checked,
+// each pair's share of scratch (7 points for each triangle under its box, src/mesh.js:388-401),
+// the inputs sent up through pinned staging and the kernel queued; d holds where everything is.
+// The staging buffer's first in_bytes hold the inputs until the stream has sent them.
+int level_launch(bm_context* ctx, const char* who, int32_t n, const int32_t* field, const double* position,
+                 const double* axes, const double* shapes, const double* boxes, const double* margin,
+                 const double* tolerance, int32_t cap, bm::LevelDev& d, size_t* in_bytes_out) {
+  const std::string w(who);
+  if (n < 0 || cap < 1) return fail(ctx, BM_ERR_ARGUMENT, w + ": n must be 0 or more, cap 1 or more");
+  if (!field || !position || !axes || !shapes || !boxes || !margin || !tolerance)
+    return fail(ctx, BM_ERR_ARGUMENT, w + ": null argument");
   std::vector<long long> scratch((size_t)n + 1, 0);
   for (int32_t k = 0; k < n; k++) {
     const double kind = shapes[19 * (size_t)k];
     if (!(kind == 0 || kind == 1 || kind == 2 || kind == 3))
-      return fail(ctx, BM_ERR_ARGUMENT, "bm_level_host: a shape's kind must be 0, 1, 2 or 3");
+      return fail(ctx, BM_ERR_ARGUMENT, w + ": a shape's kind must be 0, 1, 2 or 3");
     if (!hull_ok(ctx, shapes + 19 * (size_t)k))
-      return fail(ctx, BM_ERR_ARGUMENT, "bm_level_host: a hull's number is not in the hull table (bm_set_hulls)");
+      return fail(ctx, BM_ERR_ARGUMENT, w + ": a hull's number is not in the hull table (bm_set_hulls)");
     if (field[k] < 0 || field[k] >= (int32_t)ctx->field_shapes.size())
-      return fail(ctx, BM_ERR_ARGUMENT, "bm_level_host: a field's number is not in the field table (bm_set_fields)");
+      return fail(ctx, BM_ERR_ARGUMENT, w + ": a field's number is not in the field table (bm_set_fields)");
     const auto& F = ctx->field_shapes[field[k]];
     const double* b = boxes + 6 * (size_t)k;
     const double i0 = host_max(0, std::floor((b[0] - F.x0) / F.spacing)), i1 = host_min(F.columns - 2, std::floor((b[3] - F.x0) / F.spacing));
@@ -1183,19 +1187,19 @@ int bm_level_host(bm_context* ctx, int32_t n, const int32_t* field, const double
     if (i0 <= i1 && j0 <= j1) tris = 2 * (long long)(i1 - i0 + 1) * (long long)(j1 - j0 + 1);
     scratch[k + 1] = scratch[k] + 7 * tris;
   }
-  HIP_OR_FAIL(ctx, hipSetDevice(ctx->device), "bm_level_host: hipSetDevice");
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->device), (w + ": hipSetDevice").c_str());
   const size_t N = (size_t)n, C = (size_t)cap;
   const size_t in_d = (3 + 9 + 19 + 6 + 2) * N * 8, in_i = N * 4, in_s = (N + 1) * 8;
   const size_t in_bytes = in_d + in_s + in_i + 16;
   // Out: manifolds n, count cn, normal 3cn, points 12cn, separations 4cn, ids 4cn.
   const size_t out_d = 19 * C * N * 8, out_i = (N + 5 * C * N) * 4, out_bytes = out_d + out_i;
-  if (int rc = grow(ctx, ctx->pairs_in, in_bytes, "bm_level_host")) return rc;
-  if (int rc = grow(ctx, ctx->pairs_out, out_bytes, "bm_level_host")) return rc;
+  if (int rc = grow(ctx, ctx->pairs_in, in_bytes, who)) return rc;
+  if (int rc = grow(ctx, ctx->pairs_out, out_bytes, who)) return rc;
   const size_t points_total = (size_t)scratch[N];
-  if (int rc = grow(ctx, ctx->level_scratch, points_total * (9 * 8 + 3 * 4) + 64, "bm_level_host")) return rc;
+  if (int rc = grow(ctx, ctx->level_scratch, points_total * (9 * 8 + 3 * 4) + 64, who)) return rc;
   char* in = static_cast<char*>(ctx->pairs_in.p);
   char* out = static_cast<char*>(ctx->pairs_out.p);
-  bm::LevelDev d{};
+  d = bm::LevelDev{};
   d.n = n;
   d.cap = cap;
   d.position = reinterpret_cast<const double*>(in);
@@ -1216,7 +1220,7 @@ int bm_level_host(bm_context* ctx, int32_t n, const int32_t* field, const double
   d.su = reinterpret_cast<uint32_t*>(d.sd + 9 * points_total);
   d.hulls = ctx->hulls;
   d.fields = ctx->fields;
-  if (int rc = grow_pinned(ctx, &ctx->pstage, &ctx->pstage_cap, in_bytes + out_bytes, "bm_level_host")) return rc;
+  if (int rc = grow_pinned(ctx, &ctx->pstage, &ctx->pstage_cap, in_bytes + out_bytes, who)) return rc;
   char* st = static_cast<char*>(ctx->pstage);
   double* sd = reinterpret_cast<double*>(st);
   std::memcpy(sd, position, 24 * N);
@@ -1227,14 +1231,40 @@ int bm_level_host(bm_context* ctx, int32_t n, const int32_t* field, const double
   std::memcpy(sd + 38 * N, tolerance, 8 * N);
   std::memcpy(st + in_d, scratch.data(), in_s);
   std::memcpy(st + in_d + in_s, field, in_i);
-  HIP_OR_FAIL(ctx, hipMemcpyAsync(in, st, in_d + in_s + in_i, hipMemcpyHostToDevice, ctx->stream), "bm_level_host: copy");
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(in, st, in_d + in_s + in_i, hipMemcpyHostToDevice, ctx->stream), (w + ": copy").c_str());
   (void)hipGetLastError();
-  HIP_OR_FAIL(ctx, hipEventRecord(ctx->begin, ctx->stream), "bm_level_host: hipEventRecord");
+  HIP_OR_FAIL(ctx, hipEventRecord(ctx->begin, ctx->stream), (w + ": hipEventRecord").c_str());
   bm::launch_level(d, ctx->stream);
-  HIP_OR_FAIL(ctx, hipGetLastError(), "bm_level_host: launch");
-  HIP_OR_FAIL(ctx, hipEventRecord(ctx->end, ctx->stream), "bm_level_host: hipEventRecord");
-  char* back = st + in_bytes;
-  HIP_OR_FAIL(ctx, hipMemcpyAsync(back, out, out_bytes, hipMemcpyDeviceToHost, ctx->stream), "bm_level_host: copy");
+  HIP_OR_FAIL(ctx, hipGetLastError(), (w + ": launch").c_str());
+  HIP_OR_FAIL(ctx, hipEventRecord(ctx->end, ctx->stream), (w + ": hipEventRecord").c_str());
+  *in_bytes_out = in_bytes;
+  return BM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bm_level_host(bm_context* ctx, int32_t n, const int32_t* field, const double* position, const double* axes,
+                  const double* shapes, const double* boxes, const double* margin, const double* tolerance, int32_t cap,
+                  int32_t* manifolds, int32_t* count, double* normal, double* points, double* separations,
+                  uint32_t* ids) {
+  if (!ctx) return BM_ERR_ARGUMENT;
+  ctx->error.clear();
+  if (n < 0 || cap < 1) return fail(ctx, BM_ERR_ARGUMENT, "bm_level_host: n must be 0 or more, cap 1 or more");
+  if (n == 0) return BM_OK;
+  if (!manifolds || !count || !normal || !points || !separations || !ids)
+    return fail(ctx, BM_ERR_ARGUMENT, "bm_level_host: null argument");
+  bm::LevelDev d{};
+  size_t in_bytes = 0;
+  if (int rc = level_launch(ctx, "bm_level_host", n, field, position, axes, shapes, boxes, margin, tolerance, cap, d,
+                            &in_bytes))
+    return rc;
+  const size_t N = (size_t)n, C = (size_t)cap;
+  const size_t out_d = 19 * C * N * 8, out_i = (N + 5 * C * N) * 4, out_bytes = out_d + out_i;
+  char* back = static_cast<char*>(ctx->pstage) + in_bytes;
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(back, ctx->pairs_out.p, out_bytes, hipMemcpyDeviceToHost, ctx->stream),
+              "bm_level_host: copy");
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_level_host: kernel");
   const double* bd = reinterpret_cast<const double*>(back);
   const int32_t* bi = reinterpret_cast<const int32_t*>(back + out_d);
@@ -1260,6 +1290,75 @@ int bm_level_host(bm_context* ctx, int32_t n, const int32_t* field, const double
     }
   }
   HIP_OR_FAIL(ctx, hipEventElapsedTime(&ctx->last_ms, ctx->begin, ctx->end), "bm_level_host: hipEventElapsedTime");
+  return BM_OK;
+}
+
+int bm_level_packed(bm_context* ctx, int32_t n, const int32_t* field, const double* position, const double* axes,
+                    const double* shapes, const double* boxes, const double* margin, const double* tolerance,
+                    int32_t cap, int32_t* manifolds, int32_t* start, int32_t* count, double* normal, double* points,
+                    double* separations, uint32_t* ids) {
+  if (!ctx) return BM_ERR_ARGUMENT;
+  ctx->error.clear();
+  if (n < 0 || cap < 1) return fail(ctx, BM_ERR_ARGUMENT, "bm_level_packed: n must be 0 or more, cap 1 or more");
+  if (!start) return fail(ctx, BM_ERR_ARGUMENT, "bm_level_packed: null argument");
+  start[0] = 0;
+  if (n == 0) return BM_OK;
+  if (!manifolds || !count || !normal || !points || !separations || !ids)
+    return fail(ctx, BM_ERR_ARGUMENT, "bm_level_packed: null argument");
+  bm::LevelDev d{};
+  size_t in_bytes = 0;
+  if (int rc = level_launch(ctx, "bm_level_packed", n, field, position, axes, shapes, boxes, margin, tolerance, cap,
+                            d, &in_bytes))
+    return rc;
+  const size_t N = (size_t)n, R = (size_t)n * cap;
+  // The packed columns, and the starts, in the scratch the narrowphase's meet pass uses for its
+  // inputs (the level pass's own inputs are in pairs_in).
+  const size_t pk_start = 0, pk_count = (4 * (N + 1) + 255) & ~(size_t)255, pk_normal = pk_count + ((4 * R + 255) & ~(size_t)255);
+  const size_t pk_points = pk_normal + 24 * R, pk_seps = pk_points + 96 * R, pk_ids = pk_seps + 32 * R;
+  const size_t pk_bytes = pk_ids + 16 * R;
+  if (int rc = grow(ctx, ctx->level_packed, pk_bytes, "bm_level_packed")) return rc;
+  char* pk = static_cast<char*>(ctx->level_packed.p);
+  bm::LevelPacked p{};
+  p.start = reinterpret_cast<int*>(pk + pk_start);
+  p.count = reinterpret_cast<int*>(pk + pk_count);
+  p.normal = reinterpret_cast<double*>(pk + pk_normal);
+  p.points = reinterpret_cast<double*>(pk + pk_points);
+  p.separations = reinterpret_cast<double*>(pk + pk_seps);
+  p.ids = reinterpret_cast<uint32_t*>(pk + pk_ids);
+  bm::launch_level_pack(d, p, ctx->stream);
+  HIP_OR_FAIL(ctx, hipGetLastError(), "bm_level_packed: pack");
+  // The counts and starts first, then only the rows found.
+  char* back = static_cast<char*>(ctx->pstage) + in_bytes;
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(back, d.manifolds, 4 * N, hipMemcpyDeviceToHost, ctx->stream), "bm_level_packed: copy");
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(back + 4 * N, p.start, 4 * (N + 1), hipMemcpyDeviceToHost, ctx->stream),
+              "bm_level_packed: copy");
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_level_packed: kernel");
+  std::memcpy(manifolds, back, 4 * N);
+  std::memcpy(start, back + 4 * N, 4 * (N + 1));
+  const size_t rows = (size_t)start[N];
+  if (rows > 0) {
+    const struct {
+      void* to;
+      const void* from;
+      size_t bytes;
+    } cols[] = {{count, p.count, 4 * rows}, {normal, p.normal, 24 * rows}, {points, p.points, 96 * rows},
+                {separations, p.separations, 32 * rows}, {ids, p.ids, 16 * rows}};
+    size_t at = 0;
+    for (const auto& c : cols) {
+      HIP_OR_FAIL(ctx, hipMemcpyAsync(back + at, c.from, c.bytes, hipMemcpyDeviceToHost, ctx->stream),
+                  "bm_level_packed: copy");
+      at += (c.bytes + 255) & ~(size_t)255;
+    }
+    HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_level_packed: copy");
+    std::vector<HostCopy> copies;
+    at = 0;
+    for (const auto& c : cols) {
+      copies.push_back({c.to, back + at, c.bytes});
+      at += (c.bytes + 255) & ~(size_t)255;
+    }
+    run_copies(copies);
+  }
+  HIP_OR_FAIL(ctx, hipEventElapsedTime(&ctx->last_ms, ctx->begin, ctx->end), "bm_level_packed: hipEventElapsedTime");
   return BM_OK;
 }
 

@@ -96,6 +96,17 @@ void launch_level_plan(const LevelDev& d, void* block, hipStream_t stream);
 const void* level_plan_status(const void* block);
 // The LevelDev of a planned pass: offsets and working storage in the block.
 void level_on_block(LevelDev& d, void* block);
+// The pass's results packed by pair: start[k] (n + 1 of them) the first packed row of pair k,
+// each pair's min(manifolds[k], cap) slots in order; out_* the packed columns (room for n * cap
+// rows each). One workgroup scans, then a lane a pair copies its slots.
+struct LevelPacked {
+  int* start;
+  int* count;
+  double *normal, *points, *separations;
+  uint32_t* ids;
+};
+void launch_level_pack(const LevelDev& d, const LevelPacked& p, hipStream_t stream);
+
 // Zeroes every manifold slot a pass left unwritten (slot j of pair k with j at least
 // min(manifolds[k], cap)), as bm_level_host leaves them.
 void launch_level_clear(const LevelDev& d, hipStream_t stream);

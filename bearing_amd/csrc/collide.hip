@@ -2012,6 +2012,51 @@ void launch_level(const LevelDev& d, hipStream_t stream) {
   hipLaunchKernelGGL(level_kernel, dim3(d.n), dim3(64), 0, stream, d);
 }
 
+// The pass's results packed by pair (LevelPacked): first every pair's share, min(manifolds, cap),
+// scanned by one workgroup (each thread a run of pairs, then the runs' sums in LDS), then a lane
+// a pair copying its slots to their packed rows.
+__global__ void __launch_bounds__(1024) level_scan_kernel(LevelDev d, int* start) {
+  __shared__ int sums[1024];
+  const int t = threadIdx.x, T = blockDim.x;
+  const int per = (d.n + T - 1) / T, k0 = t * per, k1 = k0 + per < d.n ? k0 + per : d.n;
+  int run = 0;
+  for (int k = k0; k < k1; k++) run += d.manifolds[k] < d.cap ? d.manifolds[k] : d.cap;
+  sums[t] = run;
+  __syncthreads();
+  for (int step = 1; step < T; step *= 2) {
+    const int add = t >= step ? sums[t - step] : 0;
+    __syncthreads();
+    sums[t] += add;
+    __syncthreads();
+  }
+  int at = sums[t] - run;
+  for (int k = k0; k < k1; k++) {
+    start[k] = at;
+    at += d.manifolds[k] < d.cap ? d.manifolds[k] : d.cap;
+  }
+  if (t == T - 1) start[d.n] = sums[t];
+}
+
+__global__ void __launch_bounds__(256) level_pack_kernel(LevelDev d, LevelPacked p) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= d.n) return;
+  const int got = d.manifolds[k] < d.cap ? d.manifolds[k] : d.cap;
+  for (int j = 0; j < got; j++) {
+    const long o = (long)k * d.cap + j, r = p.start[k] + j;
+    p.count[r] = d.count[o];
+    for (int i = 0; i < 3; i++) p.normal[3 * r + i] = d.normal[3 * o + i];
+    for (int i = 0; i < 12; i++) p.points[12 * r + i] = d.points[12 * o + i];
+    for (int i = 0; i < 4; i++) p.separations[4 * r + i] = d.separations[4 * o + i];
+    for (int i = 0; i < 4; i++) p.ids[4 * r + i] = d.ids[4 * o + i];
+  }
+}
+
+void launch_level_pack(const LevelDev& d, const LevelPacked& p, hipStream_t stream) {
+  if (d.n == 0) return;
+  hipLaunchKernelGGL(level_scan_kernel, dim3(1), dim3(1024), 0, stream, d, p.start);
+  hipLaunchKernelGGL(level_pack_kernel, dim3((d.n + 255) / 256), dim3(256), 0, stream, d, p);
+}
+
 // ---- the level pass without allocation ----
 
 // The block: pair offsets (LEVEL_NOALLOC_PAIRS + 1 of them), the status (3 words), then the

@@ -497,11 +497,11 @@ struct bm_world {
   std::vector<double> batch_normal, batch_points, batch_seps, batch_shift, batch_loss;
   std::vector<int32_t> batch_count, batch_flags;
   std::vector<uint32_t> batch_ids;
-  // Level pairs for the device touchLevel (bm_level_host): field, the field body's place and
+  // Level pairs for the device touchLevel (bm_level_packed): field, the field body's place and
   // axes, the other body's shape record, the box to look in, margin and tolerance; results
-  // in `level_cap` slots a pair.
+  // packed by pair (pair k's manifolds from row level_start[k]), at most `level_cap` a pair.
   std::vector<int32_t, Unzeroed<int32_t>> level_field;
-  std::vector<int32_t> level_manifolds, level_count;
+  std::vector<int32_t> level_manifolds, level_count, level_start;
   std::vector<double, Unzeroed<double>> level_pos, level_axes, level_shape, level_box, level_margin, level_tol;
   std::vector<double> level_normal, level_points, level_seps;
   std::vector<uint32_t> level_ids;
@@ -1269,7 +1269,7 @@ void carry_from(bm_world* w, const Kept& pr, int p, int m) {
 
 void carry(bm_world* w, int p, int m) { carry_from(w, w->pairs, p, m); }
 
-// A level batch's output slot o as a manifold.
+// A level batch's packed output row o as a manifold.
 void level_manifold(const bm_world* w, size_t o, Manifold& mf) {
   mf.count = w->level_count[o];
   for (int i = 0; i < 3; i++) mf.normal[i] = w->level_normal[3 * o + i];
@@ -1346,16 +1346,17 @@ int run_level_batch(bm_world* w) {
   for (;;) {
     const size_t slots = (size_t)n * w->level_cap;
     w->level_manifolds.resize(n);
+    w->level_start.resize(n + 1);
     w->level_count.resize(slots);
     w->level_normal.resize(3 * slots);
     w->level_points.resize(12 * slots);
     w->level_seps.resize(4 * slots);
     w->level_ids.resize(4 * slots);
-    const int rc = bm_level_host(w->ctx, n, w->level_field.data(), w->level_pos.data(), w->level_axes.data(),
-                                 w->level_shape.data(), w->level_box.data(), w->level_margin.data(),
-                                 w->level_tol.data(), w->level_cap, w->level_manifolds.data(), w->level_count.data(),
-                                 w->level_normal.data(), w->level_points.data(), w->level_seps.data(),
-                                 w->level_ids.data());
+    const int rc = bm_level_packed(w->ctx, n, w->level_field.data(), w->level_pos.data(), w->level_axes.data(),
+                                   w->level_shape.data(), w->level_box.data(), w->level_margin.data(),
+                                   w->level_tol.data(), w->level_cap, w->level_manifolds.data(), w->level_start.data(),
+                                   w->level_count.data(), w->level_normal.data(), w->level_points.data(),
+                                   w->level_seps.data(), w->level_ids.data());
     if (rc != BM_OK) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx));
     int most = 0;
     for (int k = 0; k < n; k++) most = std::max(most, (int)w->level_manifolds[k]);
@@ -1430,7 +1431,7 @@ void fill_level(bm_world* w, const Visit& v) {
 // manifold of its own (sub 0, 1, ...; normals from a to b), each started from the remembered
 // manifold of the pair nearest its direction; a pair that touched nothing forgets.
 int finish_level(bm_world* w, Visit& v) {
-  const int a = v.a, b = v.b, p = v.p, k = v.slot, cap = w->level_cap;
+  const int a = v.a, b = v.b, p = v.p, k = v.slot;
   const bool turned = w->shape[a] >= MESH;
   const int n = w->level_manifolds[k];
   Memory& kept = w->pairs.memory_of(p);
@@ -1439,7 +1440,7 @@ int finish_level(bm_world* w, Visit& v) {
   const double sign = turned ? -1 : 1;
   int sub = 0;
   for (int q = 0; q < n; q++) {
-    const size_t o = (size_t)k * cap + q;
+    const size_t o = (size_t)w->level_start[k] + q;
     // The row's normal as emit_fill writes it, for the match against what the pair kept.
     const double nx = sign * w->level_normal[3 * o], ny = sign * w->level_normal[3 * o + 1];
     const double nz = sign * w->level_normal[3 * o + 2];
@@ -2957,7 +2958,7 @@ struct bm_batch {
   std::vector<double> a, b, pair, normal, points, seps, shift, loss;
   std::vector<int32_t> count, flags;
   std::vector<uint32_t> ids;
-  std::vector<int32_t> level_field, level_manifolds, level_count;
+  std::vector<int32_t> level_field, level_manifolds, level_count, level_start;
   std::vector<double> level_pos, level_axes, level_shape, level_box, level_margin, level_tol, level_normal,
       level_points, level_seps;
   std::vector<uint32_t> level_ids;
@@ -3027,13 +3028,14 @@ int run_merged(bm_batch* B) {
     }
     for (;;) {
       const size_t slots = (size_t)ln * B->level_cap;
-      B->level_manifolds.resize(ln); B->level_count.resize(slots); B->level_normal.resize(3 * slots);
-      B->level_points.resize(12 * slots); B->level_seps.resize(4 * slots); B->level_ids.resize(4 * slots);
-      if (int rc = bm_level_host(B->ctx, ln, B->level_field.data(), B->level_pos.data(), B->level_axes.data(),
-                                 B->level_shape.data(), B->level_box.data(), B->level_margin.data(),
-                                 B->level_tol.data(), B->level_cap, B->level_manifolds.data(), B->level_count.data(),
-                                 B->level_normal.data(), B->level_points.data(), B->level_seps.data(),
-                                 B->level_ids.data()))
+      B->level_manifolds.resize(ln); B->level_start.resize(ln + 1); B->level_count.resize(slots);
+      B->level_normal.resize(3 * slots); B->level_points.resize(12 * slots); B->level_seps.resize(4 * slots);
+      B->level_ids.resize(4 * slots);
+      if (int rc = bm_level_packed(B->ctx, ln, B->level_field.data(), B->level_pos.data(), B->level_axes.data(),
+                                   B->level_shape.data(), B->level_box.data(), B->level_margin.data(),
+                                   B->level_tol.data(), B->level_cap, B->level_manifolds.data(),
+                                   B->level_start.data(), B->level_count.data(), B->level_normal.data(),
+                                   B->level_points.data(), B->level_seps.data(), B->level_ids.data()))
         return batch_fail(B, rc, std::string("step: ") + bm_last_error(B->ctx));
       int most = 0;
       for (int k = 0; k < ln; k++) most = std::max(most, (int)B->level_manifolds[k]);
@@ -3050,11 +3052,15 @@ int run_merged(bm_batch* B) {
       if (k == 0) continue;
       w->level_cap = (int)cap;
       w->level_manifolds.assign(B->level_manifolds.begin() + k0, B->level_manifolds.begin() + k0 + k);
-      w->level_count.assign(B->level_count.begin() + cap * k0, B->level_count.begin() + cap * (k0 + k));
-      w->level_normal.assign(B->level_normal.begin() + 3 * cap * k0, B->level_normal.begin() + 3 * cap * (k0 + k));
-      w->level_points.assign(B->level_points.begin() + 12 * cap * k0, B->level_points.begin() + 12 * cap * (k0 + k));
-      w->level_seps.assign(B->level_seps.begin() + 4 * cap * k0, B->level_seps.begin() + 4 * cap * (k0 + k));
-      w->level_ids.assign(B->level_ids.begin() + 4 * cap * k0, B->level_ids.begin() + 4 * cap * (k0 + k));
+      // The world's pairs' packed rows, their starts counted from the world's first.
+      const size_t r0 = B->level_start[k0], r1 = B->level_start[k0 + k];
+      w->level_start.resize(k + 1);
+      for (size_t j = 0; j <= k; j++) w->level_start[j] = B->level_start[k0 + j] - (int32_t)r0;
+      w->level_count.assign(B->level_count.begin() + r0, B->level_count.begin() + r1);
+      w->level_normal.assign(B->level_normal.begin() + 3 * r0, B->level_normal.begin() + 3 * r1);
+      w->level_points.assign(B->level_points.begin() + 12 * r0, B->level_points.begin() + 12 * r1);
+      w->level_seps.assign(B->level_seps.begin() + 4 * r0, B->level_seps.begin() + 4 * r1);
+      w->level_ids.assign(B->level_ids.begin() + 4 * r0, B->level_ids.begin() + 4 * r1);
     }
   }
   return BM_OK;

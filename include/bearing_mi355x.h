@@ -225,6 +225,15 @@ int bm_level_host(bm_context* ctx, int32_t n, const int32_t* field, const double
                   const double* shapes, const double* boxes, const double* margin, const double* tolerance, int32_t cap,
                   int32_t* manifolds, int32_t* count, double* normal, double* points, double* separations,
                   uint32_t* ids);
+/* bm_level_host with the results packed by pair, for a caller that reads only the manifolds
+ * found: pair k's min(manifolds[k], cap) manifolds are rows start[k] .. start[k + 1] - 1 of
+ * count, normal (3 a row), points (12), separations (4) and ids (4), in the order bm_level_host
+ * gives them; start has n + 1 entries, the outputs room for n * cap rows. Only the rows found
+ * come back from the device, and rows past start[n] are left as they were. */
+int bm_level_packed(bm_context* ctx, int32_t n, const int32_t* field, const double* position, const double* axes,
+                    const double* shapes, const double* boxes, const double* margin, const double* tolerance,
+                    int32_t cap, int32_t* manifolds, int32_t* start, int32_t* count, double* normal, double* points,
+                    double* separations, uint32_t* ids);
 
 /*
  * bm_level_host without allocating: for callers that cannot allocate device memory during a

@@ -323,6 +323,34 @@ int bm_level_host(bm_context*, int32_t n, const int32_t* field, const double* po
   return BM_OK;
 }
 
+int bm_level_packed(bm_context* ctx, int32_t n, const int32_t* field, const double* position, const double* axes,
+                    const double* shapes, const double* boxes, const double* margin, const double* tolerance,
+                    int32_t cap, int32_t* manifolds, int32_t* start, int32_t* count, double* normal, double* points,
+                    double* separations, uint32_t* ids) {
+  const size_t slots = (size_t)n * cap;
+  std::vector<int32_t> c(slots);
+  std::vector<double> nr(3 * slots), pt(12 * slots), sp(4 * slots);
+  std::vector<uint32_t> id(4 * slots);
+  if (int rc = bm_level_host(ctx, n, field, position, axes, shapes, boxes, margin, tolerance, cap, manifolds, c.data(),
+                             nr.data(), pt.data(), sp.data(), id.data()))
+    return rc;
+  int at = 0;
+  for (int k = 0; k < n; k++) {
+    start[k] = at;
+    const int got = manifolds[k] < cap ? manifolds[k] : cap;
+    for (int j = 0; j < got; j++, at++) {
+      const size_t o = (size_t)k * cap + j;
+      count[at] = c[o];
+      for (int i = 0; i < 3; i++) normal[3 * (size_t)at + i] = nr[3 * o + i];
+      for (int i = 0; i < 12; i++) points[12 * (size_t)at + i] = pt[12 * o + i];
+      for (int i = 0; i < 4; i++) separations[4 * (size_t)at + i] = sp[4 * o + i];
+      for (int i = 0; i < 4; i++) ids[4 * (size_t)at + i] = id[4 * o + i];
+    }
+  }
+  start[n] = at;
+  return BM_OK;
+}
+
 int bm_set_hulls(bm_context*, int32_t n, const int32_t* hull_v, const int32_t* hull_f, const int32_t* hull_l,
                  const double* vertices, const int32_t* start, const int32_t* loops, const double* planes) {
   oracle_hulls(n, hull_v, hull_f, hull_l, vertices, start, loops, planes);
