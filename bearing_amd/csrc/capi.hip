@@ -1025,6 +1025,74 @@ int bm_meet_host(bm_context* ctx, int32_t n, const double* a, const double* b, c
   return BM_OK;
 }
 
+int bm_meet_indexed(bm_context* ctx, int32_t bodies, const double* table, int32_t n, const int32_t* ia,
+                    const int32_t* ib, const double* pair, int32_t* count, double* normal, double* points,
+                    double* separations, uint32_t* ids, double* shift, int32_t* flags, double* loss) {
+  if (!ctx) return BM_ERR_ARGUMENT;
+  ctx->error.clear();
+  if (n < 0 || bodies < 0) return fail(ctx, BM_ERR_ARGUMENT, "bm_meet_indexed: n and bodies must be 0 or more");
+  if (n == 0) return BM_OK;
+  if (!table || !ia || !ib || !pair || !count || !normal || !points || !separations || !ids || !shift || !flags)
+    return fail(ctx, BM_ERR_ARGUMENT, "bm_meet_indexed: null argument");
+  for (int32_t r = 0; r < bodies; r++) {
+    const double kind = table[32 * (size_t)r];
+    if (!(kind == 0 || kind == 1 || kind == 2 || kind == 3))
+      return fail(ctx, BM_ERR_ARGUMENT, "bm_meet_indexed: a shape's kind must be 0, 1, 2 or 3");
+    if (!hull_ok(ctx, table + 32 * (size_t)r))
+      return fail(ctx, BM_ERR_ARGUMENT, "bm_meet_indexed: a hull's number is not in the hull table (bm_set_hulls)");
+  }
+  for (int32_t k = 0; k < n; k++) {
+    if (ia[k] < 0 || ia[k] >= bodies || ib[k] < 0 || ib[k] >= bodies)
+      return fail(ctx, BM_ERR_ARGUMENT, "bm_meet_indexed: a pair names a body the table lacks");
+    if (!(pair[6 * (size_t)k + 2] > 0)) return fail(ctx, BM_ERR_ARGUMENT, "bm_meet_indexed: dt must be positive");
+  }
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->device), "bm_meet_indexed: hipSetDevice");
+  const size_t N = (size_t)n, T = 32 * sizeof(double) * (size_t)bodies;
+  const size_t pairs = 6 * sizeof(double) * N, idx = (4 * N + 255) & ~(size_t)255;
+  const size_t in_bytes = T + pairs + 2 * idx;
+  const size_t out_bytes = 280 * N + 64;
+  if (int rc = grow(ctx, ctx->pairs_in, in_bytes, "bm_meet_indexed")) return rc;
+  if (int rc = grow(ctx, ctx->pairs_out, out_bytes, "bm_meet_indexed")) return rc;
+  char* in = static_cast<char*>(ctx->pairs_in.p);
+  char* out = static_cast<char*>(ctx->pairs_out.p);
+  bm::MeetDev d{};
+  d.n = n;
+  d.a = reinterpret_cast<const double*>(in);
+  d.b = d.a;
+  d.pair = reinterpret_cast<const double*>(in + T);
+  d.ia = reinterpret_cast<const int*>(in + T + pairs);
+  d.ib = reinterpret_cast<const int*>(in + T + pairs + idx);
+  d.normal = reinterpret_cast<double*>(out);
+  d.points = reinterpret_cast<double*>(out + 24 * N);
+  d.separations = reinterpret_cast<double*>(out + 120 * N);
+  d.shift = reinterpret_cast<double*>(out + 152 * N);
+  d.count = reinterpret_cast<int32_t*>(out + 248 * N);
+  d.flags = reinterpret_cast<int32_t*>(out + 252 * N);
+  d.ids = reinterpret_cast<uint32_t*>(out + 256 * N);
+  d.loss = reinterpret_cast<double*>(out + 272 * N);
+  d.hulls = ctx->hulls;
+  if (int rc = grow_pinned(ctx, &ctx->pstage, &ctx->pstage_cap, in_bytes + out_bytes, "bm_meet_indexed")) return rc;
+  char* st = static_cast<char*>(ctx->pstage);
+  run_copies({{st, table, T}, {st + T, pair, pairs}, {st + T + pairs, ia, 4 * N}, {st + T + pairs + idx, ib, 4 * N}});
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(in, st, in_bytes, hipMemcpyHostToDevice, ctx->stream), "bm_meet_indexed: copy");
+  (void)hipGetLastError();
+  HIP_OR_FAIL(ctx, hipEventRecord(ctx->begin, ctx->stream), "bm_meet_indexed: hipEventRecord");
+  bm::launch_meet(d, ctx->stream);
+  HIP_OR_FAIL(ctx, hipGetLastError(), "bm_meet_indexed: launch");
+  HIP_OR_FAIL(ctx, hipEventRecord(ctx->end, ctx->stream), "bm_meet_indexed: hipEventRecord");
+  char* back = st + in_bytes;
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(back, out, 280 * N, hipMemcpyDeviceToHost, ctx->stream), "bm_meet_indexed: copy");
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_meet_indexed: kernel");
+  std::vector<HostCopy> outs = {{normal, back, 24 * N}, {points, back + 24 * N, 96 * N},
+                                {separations, back + 120 * N, 32 * N}, {shift, back + 152 * N, 96 * N},
+                                {count, back + 248 * N, 4 * N}, {flags, back + 252 * N, 4 * N},
+                                {ids, back + 256 * N, 16 * N}};
+  if (loss) outs.push_back({loss, back + 272 * N, 8 * N});
+  run_copies(outs);
+  HIP_OR_FAIL(ctx, hipEventElapsedTime(&ctx->last_ms, ctx->begin, ctx->end), "bm_meet_indexed: hipEventElapsedTime");
+  return BM_OK;
+}
+
 int bm_set_hulls(bm_context* ctx, int32_t n, const int32_t* hull_v, const int32_t* hull_f, const int32_t* hull_l,
                  const double* vertices, const int32_t* start, const int32_t* loops, const double* planes) {
   if (!ctx) return BM_ERR_ARGUMENT;

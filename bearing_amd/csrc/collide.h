@@ -33,6 +33,8 @@ struct MeetDev {
   int n;
   const double *a, *b;  // body records: the shape record (19), rotation 4, velocity 3, spin 3,
                         // gravity scale, reach, flags (1 dynamic, 2 spin capped)
+  const int *ia, *ib;   // when set, pair k's bodies are records ia[k] of a and ib[k] of b (one
+                        // table of each body once); else records k
   const double* pair;   // margin, tolerance, dt, gravity 3
   int *count, *flags;   // flags: 1 shifted, 2 a time of impact was asked
   double *normal, *points, *separations, *shift;

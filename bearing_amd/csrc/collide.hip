@@ -1814,10 +1814,11 @@ __global__ void collide_kernel(CollideDev d) {
 template <bool HULLS>
 __global__ void meet_kernel(MeetDev d) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= d.n || has_hull(d.a + 32 * (long)k, d.b + 32 * (long)k) != HULLS) return;
+  if (k >= d.n) return;
+  const double* src[2] = {d.a + 32 * (long)(d.ia ? d.ia[k] : k), d.b + 32 * (long)(d.ib ? d.ib[k] : k)};
+  if (has_hull(src[0], src[1]) != HULLS) return;
   narrow::Shape s[2];
   narrow::Motion mo[2];
-  const double* src[2] = {d.a + 32 * (long)k, d.b + 32 * (long)k};
   for (int t = 0; t < 2; t++) {
     const double* r = src[t];
     s[t].kind = (int)r[0];

@@ -493,7 +493,12 @@ struct bm_world {
   bm_world() { pairs.events = &events; }
   // Narrowphase batches for the device: the records (filled whole, so growing them zeroes
   // nothing), then the results.
-  std::vector<double, Unzeroed<double>> batch_a, batch_b, batch_pair;
+  // A meet record is its bodies' slots (batch_ia, batch_ib) and the pair's numbers; when the
+  // pass goes to the device each body that meets any gives one row of 32 numbers (meet_table,
+  // rows in first-met order: meet_bodies), and each pair its two rows (meet_ta, meet_tb).
+  std::vector<int32_t, Unzeroed<int32_t>> batch_ia, batch_ib, meet_ta, meet_tb;
+  std::vector<double, Unzeroed<double>> batch_pair, meet_table;
+  std::vector<int32_t> meet_row_of, meet_bodies;  // meet_row_of: by slot, -1 when not in the table
   std::vector<double> batch_normal, batch_points, batch_seps, batch_shift, batch_loss;
   std::vector<int32_t> batch_count, batch_flags;
   std::vector<uint32_t> batch_ids;
@@ -1369,11 +1374,53 @@ int run_level_batch(bm_world* w) {
   return BM_OK;
 }
 
-// Runs the narrowphase batch gathered so far on the device (bm_meet_host) and clears it.
+// The rows of the pass's n meet records: each body once, in the order the records first name
+// it (meet_bodies), and each record's two rows (meet_ta, meet_tb). Returns the bodies.
+int meet_rows(bm_world* w, int n) {
+  std::vector<int32_t>& of = w->meet_row_of;
+  if ((int)of.size() < w->top) of.resize(w->top, -1);
+  std::vector<int32_t>& list = w->meet_bodies;
+  list.clear();
+  w->meet_ta.resize(n);
+  w->meet_tb.resize(n);
+  auto row = [&](int s) {
+    if (of[s] < 0) {
+      of[s] = (int32_t)list.size();
+      list.push_back(s);
+    }
+    return of[s];
+  };
+  for (int k = 0; k < n; k++) {
+    w->meet_ta[k] = row(w->batch_ia[k]);
+    w->meet_tb[k] = row(w->batch_ib[k]);
+  }
+  for (int s : list) of[s] = -1;
+  return (int)list.size();
+}
+
+// The bodies' rows of the meet table into out (32 numbers a row), on the pool when many.
+void meet_table_fill(bm_world* w, double* out) {
+  const std::vector<int32_t>& list = w->meet_bodies;
+  const int nb = (int)list.size();
+  auto fill = [&](int i0, int i1) {
+    for (int i = i0; i < i1; i++) load(w, list[i], out + 32 * (size_t)i);
+  };
+  if (w->pool && nb >= 8192) {
+    const int chunk = 2048, chunks = (nb + chunk - 1) / chunk;
+    w->pool->run(chunks, [&](int c) { fill(c * chunk, std::min(nb, (c + 1) * chunk)); });
+  } else {
+    fill(0, nb);
+  }
+}
+
+// Runs the narrowphase batch gathered so far on the device (bm_meet_indexed) and clears it.
 int run_batch(bm_world* w) {
   if (int rc = run_level_batch(w)) return rc;
   const int n = (int)w->batch_pair.size() / 6;
   if (n == 0) return BM_OK;
+  const int nb = meet_rows(w, n);
+  w->meet_table.resize(32 * (size_t)nb);
+  meet_table_fill(w, w->meet_table.data());
   w->batch_count.resize(n);
   w->batch_flags.resize(n);
   w->batch_normal.resize(3 * n);
@@ -1382,11 +1429,11 @@ int run_batch(bm_world* w) {
   w->batch_ids.resize(4 * n);
   w->batch_shift.resize(12 * n);
   w->batch_loss.resize(n);
-  const int rc = bm_meet_host(w->ctx, n, w->batch_a.data(), w->batch_b.data(), w->batch_pair.data(),
-                              w->batch_count.data(), w->batch_normal.data(), w->batch_points.data(),
-                              w->batch_seps.data(), w->batch_ids.data(), w->batch_shift.data(), w->batch_flags.data(),
-                              w->batch_loss.data());
-  w->batch_a.clear(); w->batch_b.clear(); w->batch_pair.clear();
+  const int rc = bm_meet_indexed(w->ctx, nb, w->meet_table.data(), n, w->meet_ta.data(), w->meet_tb.data(),
+                                 w->batch_pair.data(), w->batch_count.data(), w->batch_normal.data(),
+                                 w->batch_points.data(), w->batch_seps.data(), w->batch_ids.data(),
+                                 w->batch_shift.data(), w->batch_flags.data(), w->batch_loss.data());
+  w->batch_ia.clear(); w->batch_ib.clear(); w->batch_pair.clear();
   if (rc != BM_OK) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx));
   return BM_OK;
 }
@@ -1478,11 +1525,14 @@ void decide_visit(const bm_world* w, Visit& v, Manifold& m) {
   v.kind = reused(w, v.p, a, b, v.tolerance, margin, m) ? 1 : 2;
 }
 
-// The meet record of a visit of kind 2 at its slot: both bodies and the pair's numbers.
+// The meet record of a visit of kind 2 at its slot: both bodies and the pair's numbers. The
+// bodies' own records are taken when the pass goes to the device: nothing they hold (pose,
+// axes, velocities) changes while a pass is prepared, a wake changing only a body's motion,
+// which is the pair's.
 void fill_meet(bm_world* w, const Visit& v) {
   const size_t k = v.slot;
-  load(w, v.a, &w->batch_a[32 * k]);
-  load(w, v.b, &w->batch_b[32 * k]);
+  w->batch_ia[k] = v.a;
+  w->batch_ib[k] = v.b;
   const double margin = w->motion[v.a] + w->motion[v.b] + v.tolerance;
   const double pair[6] = {margin, v.tolerance, w->step_dt, w->gravity[0], w->gravity[1], w->gravity[2]};
   for (int i = 0; i < 6; i++) w->batch_pair[6 * k + i] = pair[i];
@@ -1491,7 +1541,7 @@ void fill_meet(bm_world* w, const Visit& v) {
 // Room in the device batches for `meets` more meet records and `levels` more level records.
 void grow_batches(bm_world* w, size_t meets, size_t levels) {
   const size_t m = w->batch_pair.size() / 6 + meets;
-  w->batch_a.resize(32 * m); w->batch_b.resize(32 * m); w->batch_pair.resize(6 * m);
+  w->batch_ia.resize(m); w->batch_ib.resize(m); w->batch_pair.resize(6 * m);
   const size_t l = w->level_field.size() + levels;
   w->level_field.resize(l); w->level_pos.resize(3 * l); w->level_axes.resize(9 * l); w->level_shape.resize(19 * l);
   w->level_box.resize(6 * l); w->level_margin.resize(l); w->level_tol.resize(l);
@@ -2955,7 +3005,11 @@ struct bm_batch {
   std::vector<bm_world*> worlds;
   std::mutex lock;
   Pool* pool = nullptr;
-  std::vector<double> a, b, pair, normal, points, seps, shift, loss;
+  // A merged meet pass: every world's body rows one after another (table), each world's pairs'
+  // rows moved up by its first (ia, ib), and the pairs' numbers; then the results.
+  std::vector<double, Unzeroed<double>> table, pair;
+  std::vector<int32_t, Unzeroed<int32_t>> ia, ib;
+  std::vector<double> normal, points, seps, shift, loss;
   std::vector<int32_t> count, flags;
   std::vector<uint32_t> ids;
   std::vector<int32_t> level_field, level_manifolds, level_count, level_start;
@@ -2982,23 +3036,37 @@ int run_merged(bm_batch* B) {
   for (int i = 0; i < nw; i++) at[i + 1] = at[i] + (int)W[i]->batch_pair.size() / 6;
   const int n = at[nw];
   if (n > 0) {
-    B->a.resize(32 * (size_t)n); B->b.resize(32 * (size_t)n); B->pair.resize(6 * (size_t)n);
+    // Each world's bodies that meet, once each; then their rows, the pairs' rows moved up by
+    // the world's first, and the pairs' numbers, each world into its place.
+    std::vector<int> bt(nw + 1, 0);
     B->pool->run(nw, [&](int i) {
       bm_world* w = W[i];
-      std::copy(w->batch_a.begin(), w->batch_a.end(), B->a.begin() + 32 * (size_t)at[i]);
-      std::copy(w->batch_b.begin(), w->batch_b.end(), B->b.begin() + 32 * (size_t)at[i]);
-      std::copy(w->batch_pair.begin(), w->batch_pair.end(), B->pair.begin() + 6 * (size_t)at[i]);
+      bt[i + 1] = at[i + 1] > at[i] ? meet_rows(w, at[i + 1] - at[i]) : 0;
+    });
+    for (int i = 0; i < nw; i++) bt[i + 1] += bt[i];
+    B->table.resize(32 * (size_t)bt[nw]); B->ia.resize(n); B->ib.resize(n); B->pair.resize(6 * (size_t)n);
+    B->pool->run(nw, [&](int i) {
+      bm_world* w = W[i];
+      const int k = at[i + 1] - at[i];
+      if (k == 0) return;
+      const std::vector<int32_t>& list = w->meet_bodies;
+      for (size_t r = 0; r < list.size(); r++) load(w, list[r], &B->table[32 * ((size_t)bt[i] + r)]);
+      for (int j = 0; j < k; j++) {
+        B->ia[at[i] + j] = bt[i] + w->meet_ta[j];
+        B->ib[at[i] + j] = bt[i] + w->meet_tb[j];
+      }
+      std::copy(w->batch_pair.begin(), w->batch_pair.begin() + 6 * (size_t)k, B->pair.begin() + 6 * (size_t)at[i]);
     });
     B->count.resize(n); B->flags.resize(n); B->normal.resize(3 * (size_t)n); B->points.resize(12 * (size_t)n);
     B->seps.resize(4 * (size_t)n); B->ids.resize(4 * (size_t)n); B->shift.resize(12 * (size_t)n); B->loss.resize(n);
-    if (int rc = bm_meet_host(B->ctx, n, B->a.data(), B->b.data(), B->pair.data(), B->count.data(), B->normal.data(),
-                              B->points.data(), B->seps.data(), B->ids.data(), B->shift.data(), B->flags.data(),
-                              B->loss.data()))
+    if (int rc = bm_meet_indexed(B->ctx, bt[nw], B->table.data(), n, B->ia.data(), B->ib.data(), B->pair.data(),
+                                 B->count.data(), B->normal.data(), B->points.data(), B->seps.data(), B->ids.data(),
+                                 B->shift.data(), B->flags.data(), B->loss.data()))
       return batch_fail(B, rc, std::string("step: ") + bm_last_error(B->ctx));
     B->pool->run(nw, [&](int i) {
       bm_world* w = W[i];
       const int k0 = at[i], k = at[i + 1] - at[i];
-      w->batch_a.clear(); w->batch_b.clear(); w->batch_pair.clear();
+      w->batch_ia.clear(); w->batch_ib.clear(); w->batch_pair.clear();
       if (k == 0) return;
       w->batch_count.assign(B->count.begin() + k0, B->count.begin() + k0 + k);
       w->batch_flags.assign(B->flags.begin() + k0, B->flags.begin() + k0 + k);

@@ -192,6 +192,13 @@ int bm_collide_host(bm_context* ctx, int32_t n, const double* a, const double* b
 int bm_meet_host(bm_context* ctx, int32_t n, const double* a, const double* b, const double* pair, int32_t* count,
                  double* normal, double* points, double* separations, uint32_t* ids, double* shift, int32_t* flags,
                  double* loss);
+/* bm_meet_host with each body's record given once: table holds `bodies` records of 32 numbers
+ * (the layout bm_meet_host's a and b have), and pair k is the bodies table[ia[k]] and
+ * table[ib[k]] with its 6 numbers in pair; results as bm_meet_host's. For a pass whose bodies
+ * meet several others each, what crosses to the device is a record a body, not two a pair. */
+int bm_meet_indexed(bm_context* ctx, int32_t bodies, const double* table, int32_t n, const int32_t* ia,
+                    const int32_t* ib, const double* pair, int32_t* count, double* normal, double* points,
+                    double* separations, uint32_t* ids, double* shift, int32_t* flags, double* loss);
 
 /* The hulls shape records name by number (a hull's record: kind 3, its number in place of
  * half[0]): n hulls in their bodies' frames, as src/hull.js's Hull holds them. Hull h's

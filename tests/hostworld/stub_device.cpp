@@ -367,6 +367,21 @@ int bm_meet_host(bm_context* ctx, int32_t n, const double* a, const double* b, c
   return BM_OK;
 }
 
+int bm_meet_indexed(bm_context* ctx, int32_t bodies, const double* table, int32_t n, const int32_t* ia,
+                    const int32_t* ib, const double* pair, int32_t* count, double* normal, double* points,
+                    double* separations, uint32_t* ids, double* shift, int32_t* flags, double* loss) {
+  std::vector<double> a(32 * (size_t)n), b(32 * (size_t)n);
+  for (int32_t k = 0; k < n; k++) {
+    if (ia[k] < 0 || ia[k] >= bodies || ib[k] < 0 || ib[k] >= bodies) {
+      ctx->error = "bm_meet_indexed: a pair names a body the table lacks";
+      return BM_ERR_ARGUMENT;
+    }
+    std::memcpy(&a[32 * (size_t)k], table + 32 * (size_t)ia[k], 32 * sizeof(double));
+    std::memcpy(&b[32 * (size_t)k], table + 32 * (size_t)ib[k], 32 * sizeof(double));
+  }
+  return bm_meet_host(ctx, n, a.data(), b.data(), pair, count, normal, points, separations, ids, shift, flags, loss);
+}
+
 }  // extern "C"
 
 // The broadphase passes (bm_broad_*) on the CPU, with the library's own per-body and per-pair
