@@ -163,7 +163,7 @@ struct bm_context {
   // The soft step's own columns stay on the device (bm_set_device_only); the arena's bytes
   // up to them, which upload and download move.
   bool keep_device_only = false;
-  size_t device_only_at = 0, moved = 0;
+  size_t device_only_at = 0, results_only_at = 0, moved = 0;
   bm::FieldTable fields{};
   struct FieldShape {
     int columns, rows;
@@ -304,10 +304,18 @@ int grow_pinned(bm_context* ctx, void** p, size_t* cap, size_t bytes, const char
 const char* const DEVICE_ONLY[] = {"now", "ra", "rb", "normal_mass", "jac", "block", "lever", "anchor_a", "anchor_b",
                                    "fjac", "tangent_mass", "roll_mass1", "roll_mass2", "spin_mass"};
 
-// The arena's column order: those arrange() moves first, the device-only ones last.
+// The columns the step writes in every row before it reads them there (prepare: anchors,
+// adjusted separations, first pushes and closing speeds; the solve: gap and sliding; rest()
+// the resting rows' pushes, speeds and gap) and the host reads only after the step: a context
+// that keeps the device-only columns on the device returns these but does not send them.
+const char* const RESULTS_ONLY[] = {"la", "lb", "adjusted", "max_normal_impulse", "relative_velocity", "gap",
+                                    "sliding"};
+
+// The arena's column order: those arrange() moves first, then the rest, the results-only ones
+// and last the device-only ones.
 struct ArenaOrder {
   int order[NCOL];
-  int arranged, device_only;
+  int arranged, results_only, device_only;
   ArenaOrder() {
     int k = 0;
     bool taken[NCOL] = {};
@@ -318,12 +326,21 @@ struct ArenaOrder {
     }
     arranged = k;
     for (const char* name : DEVICE_ONLY) taken[column_index(name)] = true;
+    for (const char* name : RESULTS_ONLY) taken[column_index(name)] = true;
     for (int c = 0; c < NCOL; c++)
       if (!taken[c]) order[k++] = c;
+    results_only = k;
+    for (const char* name : RESULTS_ONLY) order[k++] = column_index(name);
     device_only = k;
     for (const char* name : DEVICE_ONLY) order[k++] = column_index(name);
   }
 };
+
+bool is_results_only(int c) {
+  for (const char* name : RESULTS_ONLY)
+    if (column_index(name) == c) return true;
+  return false;
+}
 
 bool is_device_only(int c) {
   for (const char* name : DEVICE_ONLY)
@@ -582,14 +599,17 @@ int upload_parts(bm_context* ctx, int nparts, const bm_columns* const* parts, co
     const Column& col = COLUMNS[c];
     const size_t bytes = units(ctx, col.unit) * col.per * col.elem;
     if (k == ORDER.device_only) ctx->device_only_at = at;
+    if (k == ORDER.results_only) ctx->results_only_at = at;
     ctx->bytes[c] = bytes;
     ctx->offset[c] = at;
     at += (bytes + 255) & ~(size_t)255;
     if (k + 1 == ORDER.arranged) ctx->arranged_end = at;
   }
   ctx->total = at;
-  // What moves between host and device: all of it, or up to the device-only columns.
+  // What moves between host and device: all of it, or up to the device-only columns (and goes
+  // up to the results-only ones).
   ctx->moved = ctx->keep_device_only ? ctx->device_only_at : ctx->total;
+  const size_t sent = ctx->keep_device_only ? ctx->results_only_at : ctx->total;
   if (int rc = grow_device(ctx, &ctx->arena, &ctx->arena_cap, at, who)) return rc;
   if (int rc = grow_pinned(ctx, &ctx->stage, &ctx->stage_cap, at, who)) return rc;
   char* stage = static_cast<char*>(ctx->stage);
@@ -602,7 +622,7 @@ int upload_parts(bm_context* ctx, int nparts, const bm_columns* const* parts, co
     if (col.unit == WORLD1) {
       const int t = c;  // the four offset tables are the first four columns
       std::memcpy(dst, joined[t], ctx->bytes[c]);
-    } else if ((ctx->keep_device_only && is_device_only(c)) || col.unit == KEPT) {
+    } else if ((ctx->keep_device_only && (is_device_only(c) || is_results_only(c))) || col.unit == KEPT) {
       // left on the device (KEPT: written by the soft step before it is read)
     } else {
       for (int q = 0; q < nparts; q++) {
@@ -619,7 +639,7 @@ int upload_parts(bm_context* ctx, int nparts, const bm_columns* const* parts, co
     std::memcpy(reinterpret_cast<char*>(&ctx->dev) + col.dev, &p, sizeof(void*));
   }
   run_copies(copies);
-  HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->arena, ctx->stage, ctx->moved, hipMemcpyHostToDevice, ctx->stream),
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->arena, ctx->stage, sent, hipMemcpyHostToDevice, ctx->stream),
               (std::string(who) + ": hipMemcpyAsync").c_str());
   ctx->moff_host = ctx->join_manifold;
   ctx->boff_host = ctx->join_body;

@@ -272,7 +272,9 @@ int bm_set_launch(bm_context* ctx, int32_t shape, int32_t resident);
 /* keep = 1: the columns only the soft step writes and then reads (now, ra, rb, normal_mass,
  * jac, block, lever, anchor_a, anchor_b, fjac, tangent_mass, roll_mass1, roll_mass2,
  * spin_mass) stay on the device: bm_upload does not send them (they may be NULL) and
- * bm_download does not write them back. For callers that never read them between steps (the
+ * bm_download does not write them back; and the columns the step writes in every row before
+ * reading them there (la, lb, adjusted, max_normal_impulse, relative_velocity, gap, sliding)
+ * are not sent, only returned. For callers that never read them between steps (the
  * whole-step path); results are the same bits. Default 0. */
 int bm_set_device_only(bm_context* ctx, int32_t keep);
 
