@@ -164,6 +164,13 @@ struct bm_context {
   // up to them, which upload and download move.
   bool keep_device_only = false;
   size_t device_only_at = 0, results_only_at = 0, moved = 0;
+  // The bodies' constant columns (CONSTANT), when a body version is set: their block, the
+  // version and bodies it holds, and whether each column lives there for the current upload.
+  uint64_t body_version = 0, kept_version = 0;
+  int32_t kept_bodies = -1;
+  void* constants = nullptr;
+  size_t constants_cap = 0;
+  bool in_block[NCOL] = {};
   bm::FieldTable fields{};
   struct FieldShape {
     int columns, rows;
@@ -336,6 +343,18 @@ struct ArenaOrder {
   }
 };
 
+// The bodies' columns no step changes (src/world.js _write sets them as a body is added): with a
+// body version set (bm_set_body_version) they live in a block of their own, sent only when the
+// version changes.
+const char* const CONSTANT[] = {"dims", "inertia", "inv_inertia", "gravity_scale", "damping", "angular_damping",
+                                "type", "shape", "lock", "framed", "joint_head"};
+
+bool is_constant(int c) {
+  for (const char* name : CONSTANT)
+    if (column_index(name) == c) return true;
+  return false;
+}
+
 bool is_results_only(int c) {
   for (const char* name : RESULTS_ONLY)
     if (column_index(name) == c) return true;
@@ -430,6 +449,7 @@ int bm_destroy(bm_context* ctx) {
   if (!ctx) return BM_OK;
   (void)hipSetDevice(ctx->device);
   if (ctx->arena) (void)hipFree(ctx->arena);
+  if (ctx->constants) (void)hipFree(ctx->constants);
   if (ctx->spare_arena) (void)hipFree(ctx->spare_arena);
   if (ctx->stage) (void)hipHostFree(ctx->stage);
   if (ctx->pstage) (void)hipHostFree(ctx->pstage);
@@ -593,7 +613,8 @@ int upload_parts(bm_context* ctx, int nparts, const bm_columns* const* parts, co
   }
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->device), (std::string(who) + ": hipSetDevice").c_str());
   static const ArenaOrder ORDER;
-  size_t at = 0;
+  size_t at = 0, block = 0;
+  const bool constants = ctx->body_version != 0;
   for (int k = 0; k < NCOL; k++) {
     const int c = ORDER.order[k];
     const Column& col = COLUMNS[c];
@@ -601,9 +622,26 @@ int upload_parts(bm_context* ctx, int nparts, const bm_columns* const* parts, co
     if (k == ORDER.device_only) ctx->device_only_at = at;
     if (k == ORDER.results_only) ctx->results_only_at = at;
     ctx->bytes[c] = bytes;
-    ctx->offset[c] = at;
-    at += (bytes + 255) & ~(size_t)255;
+    ctx->in_block[c] = constants && is_constant(c);
+    if (ctx->in_block[c]) {
+      ctx->offset[c] = block;
+      block += (bytes + 255) & ~(size_t)255;
+    } else {
+      ctx->offset[c] = at;
+      at += (bytes + 255) & ~(size_t)255;
+    }
     if (k + 1 == ORDER.arranged) ctx->arranged_end = at;
+  }
+  // The constant columns go up only when the block holds another version or other bodies.
+  bool send_constants = constants && (ctx->kept_version != ctx->body_version || ctx->kept_bodies != ctx->bodies);
+  ctx->kept_version = 0;  // set again below once the block holds this upload's constants
+  if (constants && block > ctx->constants_cap) {
+    if (ctx->constants) (void)hipFree(ctx->constants);
+    ctx->constants = nullptr;
+    ctx->constants_cap = 0;
+    if (hipMalloc(&ctx->constants, block) != hipSuccess) return fail(ctx, BM_ERR_DEVICE, std::string(who) + ": hipMalloc");
+    ctx->constants_cap = block;
+    send_constants = true;
   }
   ctx->total = at;
   // What moves between host and device: all of it, or up to the device-only columns (and goes
@@ -615,11 +653,22 @@ int upload_parts(bm_context* ctx, int nparts, const bm_columns* const* parts, co
   char* stage = static_cast<char*>(ctx->stage);
   const int32_t* joined[4] = {ctx->join_body.data(), ctx->join_manifold.data(), ctx->join_pair.data(),
                               ctx->join_awake.data()};
-  std::vector<HostCopy> copies;
+  std::vector<HostCopy> copies, constant_copies;
+  if (send_constants)
+    if (int rc = grow_pinned(ctx, &ctx->pstage, &ctx->pstage_cap, block, who)) return rc;
   for (int c = 0; c < NCOL; c++) {
     const Column& col = COLUMNS[c];
-    char* dst = stage + ctx->offset[c];
-    if (col.unit == WORLD1) {
+    char* dst = (ctx->in_block[c] ? static_cast<char*>(ctx->pstage) : stage) + ctx->offset[c];
+    if (ctx->in_block[c]) {
+      for (int q = 0; q < nparts && send_constants; q++) {
+        const size_t bytes = part_units(parts[q], col.unit) * col.per * col.elem;
+        if (bytes == 0) continue;
+        const void* from = field<const void*>(parts[q], col.host);
+        if (!from) return fail(ctx, BM_ERR_ARGUMENT, std::string(who) + ": " + col.name + " is null");
+        constant_copies.push_back({dst, from, bytes});
+        dst += bytes;
+      }
+    } else if (col.unit == WORLD1) {
       const int t = c;  // the four offset tables are the first four columns
       std::memcpy(dst, joined[t], ctx->bytes[c]);
     } else if ((ctx->keep_device_only && (is_device_only(c) || is_results_only(c))) || col.unit == KEPT) {
@@ -634,9 +683,19 @@ int upload_parts(bm_context* ctx, int nparts, const bm_columns* const* parts, co
         dst += bytes;
       }
     }
-    void* p = static_cast<char*>(ctx->arena) + ctx->offset[c];
+    void* p = static_cast<char*>(ctx->in_block[c] ? ctx->constants : ctx->arena) + ctx->offset[c];
     ctx->buffer[c] = p;
     std::memcpy(reinterpret_cast<char*>(&ctx->dev) + col.dev, &p, sizeof(void*));
+  }
+  if (send_constants) {
+    run_copies(constant_copies);
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->constants, ctx->pstage, block, hipMemcpyHostToDevice, ctx->stream),
+                (std::string(who) + ": hipMemcpyAsync").c_str());
+    HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), (std::string(who) + ": hipStreamSynchronize").c_str());
+  }
+  if (constants) {
+    ctx->kept_version = ctx->body_version;
+    ctx->kept_bodies = ctx->bodies;
   }
   run_copies(copies);
   HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->arena, ctx->stage, sent, hipMemcpyHostToDevice, ctx->stream),
@@ -1514,6 +1573,13 @@ int bm_level_noalloc(bm_context* ctx, int32_t n, const int32_t* field, const dou
   HIP_OR_FAIL(ctx, hipGetLastError(), "bm_level_noalloc: launch");
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_level_noalloc: kernel");
   HIP_OR_FAIL(ctx, hipEventElapsedTime(&ctx->last_ms, ctx->begin, ctx->end), "bm_level_noalloc: hipEventElapsedTime");
+  return BM_OK;
+}
+
+int bm_set_body_version(bm_context* ctx, uint64_t version) {
+  if (!ctx) return BM_ERR_ARGUMENT;
+  ctx->error.clear();
+  ctx->body_version = version;
   return BM_OK;
 }
 

@@ -490,6 +490,10 @@ struct bm_world {
   Events events;
   Pairs pairs;
   Contacts c;
+  // Names this world's bodies' constant columns (dims, inertias, damping, type, shape, ...): a
+  // new number, unique across worlds, each time bodies are added; the device keeps those columns
+  // while the number it was given stays the same (bm_set_body_version).
+  uint64_t body_version = 0;
   bm_world() { pairs.events = &events; }
   // Narrowphase batches for the device: the records (filled whole, so growing them zeroes
   // nothing), then the results.
@@ -2888,6 +2892,7 @@ int device_phase(bm_world* w, double dt) {
   if (w->record.on) record_before(w, dt);
   bm_columns& h = w->h;
   if (w->tap) w->tap(w->tap_user, &h);
+  bm_set_body_version(w->ctx, w->body_version);
   if (int rc = bm_upload(w->ctx, &h)) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx));
   if (int rc = bm_colour(w->ctx)) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx));
   if (int rc = bm_soft_step(w->ctx)) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx));
@@ -3182,6 +3187,12 @@ int first_failure(bm_batch* B, const std::vector<int>& rcs) {
 
 extern "C" {
 
+// A number no world's bodies have had: names a world's constant body columns after each add.
+uint64_t next_body_version() {
+  static std::atomic<uint64_t> next{1};
+  return next.fetch_add(1);
+}
+
 int bm_world_create(bm_world** out, int32_t device, const double* gravity, int32_t substeps, int32_t sleeping) {
   if (!out) return BM_ERR_ARGUMENT;
   *out = nullptr;
@@ -3215,6 +3226,7 @@ int bm_world_add(bm_world* w, const bm_bodies* b) {
   if (!w || !b) return BM_ERR_ARGUMENT;
   w->error.clear();
   w->broad.bodies_set = false;  // the device's bodies are given again at the next step
+  w->body_version = next_body_version();
   if (b->n < 0) return fail(w, BM_ERR_ARGUMENT, "addBody: n must be 0 or more");
   for (int i = 0; i < b->n; i++) {
     if (!b->type || !b->shape || !b->lock || !b->position || !b->rotation || !b->velocity || !b->angular || !b->dims ||
@@ -3554,6 +3566,10 @@ int bm_batch_step(bm_batch* B, double dt) {
     batch_fail(B, rc, std::string("step: ") + bm_last_error(B->ctx));
     return batch_broke(B, rc);
   };
+  // The batch's bodies' constant columns are named by its worlds' versions, in order.
+  uint64_t version = 1469598103934665603ull;
+  for (int i = 0; i < n; i++) version = (version ^ W[i]->body_version) * 1099511628211ull;
+  bm_set_body_version(B->ctx, version != 0 ? version : 1);
   if (int rc = bm_upload_worlds(B->ctx, n, ins.data())) return device_failed(rc);
   if (int rc = bm_colour(B->ctx)) return device_failed(rc);
   if (int rc = bm_soft_step(B->ctx)) return device_failed(rc);

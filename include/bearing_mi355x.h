@@ -278,6 +278,14 @@ int bm_set_launch(bm_context* ctx, int32_t shape, int32_t resident);
  * whole-step path); results are the same bits. Default 0. */
 int bm_set_device_only(bm_context* ctx, int32_t keep);
 
+/* version != 0: the bodies' columns no step changes (dims, inertia, inv_inertia, gravity_scale,
+ * damping, angular_damping, type, shape, lock, framed, joint_head) are kept on the device in a
+ * block of their own, sent by the first upload under this version and not again while uploads
+ * keep the same version and number of bodies. The caller gives a new version whenever any of
+ * those columns, or the worlds or bodies it uploads, change. 0 (the default) sends them with
+ * every upload. Results are the same bits. */
+int bm_set_body_version(bm_context* ctx, uint64_t version);
+
 /* When bm_soft_step spreads one world over the whole device: a batch of one world with at
  * least min_manifolds manifolds (a quarter of that if the world's velocities do not fit in one
  * workgroup's LDS) is stepped by every workgroup the device can keep resident, each colour's

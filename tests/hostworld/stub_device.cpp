@@ -178,6 +178,7 @@ int bm_upload_worlds(bm_context* ctx, int32_t n, const bm_columns* const* worlds
 }
 int bm_download_worlds(bm_context*, int32_t, bm_columns* const*) { return BM_OK; }
 int bm_set_device_only(bm_context*, int32_t) { return BM_OK; }
+int bm_set_body_version(bm_context*, uint64_t) { return BM_OK; }
 
 }  // extern "C"
 
