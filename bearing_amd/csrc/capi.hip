@@ -164,6 +164,11 @@ struct bm_context {
   // up to them, which upload and download move.
   bool keep_device_only = false;
   size_t device_only_at = 0, results_only_at = 0, moved = 0;
+  // bm_contact_phase's graph (bm_set_graph): the executable graph of the colouring's and soft
+  // step's launches, and how many times it was launched, made and updated.
+  bool graph_on = false;
+  hipGraphExec_t graph_exec = nullptr;
+  int64_t graph_launches = 0, graph_instantiations = 0, graph_updates = 0;
   // The bodies' constant columns (CONSTANT), when a body version is set: their block, the
   // version and bodies it holds, and whether each column lives there for the current upload.
   uint64_t body_version = 0, kept_version = 0;
@@ -448,6 +453,7 @@ int bm_create(bm_context** out, int32_t device) {
 int bm_destroy(bm_context* ctx) {
   if (!ctx) return BM_OK;
   (void)hipSetDevice(ctx->device);
+  if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
   if (ctx->arena) (void)hipFree(ctx->arena);
   if (ctx->constants) (void)hipFree(ctx->constants);
   if (ctx->spare_arena) (void)hipFree(ctx->spare_arena);
@@ -810,14 +816,23 @@ int bm_upload_worlds(bm_context* ctx, int32_t n, const bm_columns* const* worlds
   return upload_parts(ctx, n, worlds, "bm_upload_worlds");
 }
 
-int bm_soft_step(bm_context* ctx) {
+}  // extern "C"
+
+namespace {
+
+int colour_run(bm_context* ctx, bool capture, bool* captured);
+
+// bm_soft_step's work; capture: the stream is capturing (bm_contact_phase), so the launch is
+// queued and nothing waits (a batch of one world, which may take the cooperative grid-wide
+// form, is never captured).
+int soft_run(bm_context* ctx, bool capture) {
   if (!ctx) return BM_ERR_ARGUMENT;
   if (!ctx->uploaded) return fail(ctx, BM_ERR_ARGUMENT, "bm_soft_step: no batch uploaded");
   ctx->error.clear();
   if (!ctx->laid_out)
     return fail(ctx, BM_ERR_ARGUMENT, "bm_soft_step: the batch's rows are not laid out by colour (call bm_colour)");
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->device), "bm_soft_step: hipSetDevice");
-  HIP_OR_FAIL(ctx, hipEventRecord(ctx->begin, ctx->stream), "bm_soft_step: hipEventRecord");
+  if (!capture) HIP_OR_FAIL(ctx, hipEventRecord(ctx->begin, ctx->stream), "bm_soft_step: hipEventRecord");
   if (ctx->lds_limit == 0) ctx->lds_limit = bm::soft_step_lds_limit(ctx->device);
   // Velocities resident in LDS when the largest world's fit: 48 bytes a body.
   const size_t lds = (size_t)ctx->most_bodies * 6 * sizeof(double);
@@ -854,10 +869,86 @@ int bm_soft_step(bm_context* ctx) {
     bm::launch_soft_step(ctx->dev, ctx->n_worlds, ctx->shape, resident ? lds : 0, ctx->stream);
   }
   HIP_OR_FAIL(ctx, hipGetLastError(), "bm_soft_step: launch");
+  ctx->stored = true;
+  if (capture) return BM_OK;
   HIP_OR_FAIL(ctx, hipEventRecord(ctx->end, ctx->stream), "bm_soft_step: hipEventRecord");
   HIP_OR_FAIL(ctx, hipEventSynchronize(ctx->end), "bm_soft_step: kernel");
   HIP_OR_FAIL(ctx, hipEventElapsedTime(&ctx->last_ms, ctx->begin, ctx->end), "bm_soft_step: hipEventElapsedTime");
-  ctx->stored = true;
+  return BM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bm_soft_step(bm_context* ctx) { return soft_run(ctx, false); }
+
+int bm_set_graph(bm_context* ctx, int32_t on) {
+  if (!ctx) return BM_ERR_ARGUMENT;
+  ctx->error.clear();
+  ctx->graph_on = on != 0;
+  return BM_OK;
+}
+
+int bm_graph_counts(const bm_context* ctx, int64_t* out) {
+  if (!ctx || !out) return BM_ERR_ARGUMENT;
+  out[0] = ctx->graph_launches;
+  out[1] = ctx->graph_instantiations;
+  out[2] = ctx->graph_updates;
+  return BM_OK;
+}
+
+int bm_contact_phase(bm_context* ctx) {
+  if (!ctx) return BM_ERR_ARGUMENT;
+  if (!ctx->graph_on || ctx->n_worlds < 2) {
+    if (int rc = colour_run(ctx, false, nullptr)) return rc;
+    return soft_run(ctx, false);
+  }
+  // The colouring's and the soft step's launches captured as a graph: the executable graph of
+  // the step before is updated to this step's arguments when the shapes allow, else made anew.
+  bool captured = false;
+  int rc = colour_run(ctx, true, &captured);
+  if (rc == BM_OK && captured) rc = soft_run(ctx, true);
+  if (!captured) {
+    if (rc != BM_OK) return rc;
+    return soft_run(ctx, false);
+  }
+  hipGraph_t graph = nullptr;
+  const hipError_t ended = hipStreamEndCapture(ctx->stream, &graph);
+  if (rc != BM_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (ended != hipSuccess) return fail(ctx, BM_ERR_DEVICE, std::string("bm_contact_phase: hipStreamEndCapture: ") + hipGetErrorString(ended));
+  bool ready = false;
+  if (ctx->graph_exec) {
+    hipGraphNode_t bad = nullptr;
+    hipGraphExecUpdateResult why;
+    if (hipGraphExecUpdate(ctx->graph_exec, graph, &bad, &why) == hipSuccess) {
+      ready = true;
+      ctx->graph_updates++;
+    } else {
+      (void)hipGetLastError();
+      (void)hipGraphExecDestroy(ctx->graph_exec);
+      ctx->graph_exec = nullptr;
+    }
+  }
+  if (!ready) {
+    const hipError_t e = hipGraphInstantiate(&ctx->graph_exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      ctx->graph_exec = nullptr;
+      (void)hipGraphDestroy(graph);
+      return fail(ctx, BM_ERR_DEVICE, std::string("bm_contact_phase: hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    ctx->graph_instantiations++;
+  }
+  (void)hipGraphDestroy(graph);
+  HIP_OR_FAIL(ctx, hipEventRecord(ctx->begin, ctx->stream), "bm_contact_phase: hipEventRecord");
+  HIP_OR_FAIL(ctx, hipGraphLaunch(ctx->graph_exec, ctx->stream), "bm_contact_phase: hipGraphLaunch");
+  HIP_OR_FAIL(ctx, hipEventRecord(ctx->end, ctx->stream), "bm_contact_phase: hipEventRecord");
+  ctx->graph_launches++;
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_contact_phase: kernel");
+  HIP_OR_FAIL(ctx, hipEventElapsedTime(&ctx->last_ms, ctx->begin, ctx->end), "bm_contact_phase: hipEventElapsedTime");
   return BM_OK;
 }
 
@@ -874,7 +965,15 @@ int bm_download_worlds(bm_context* ctx, int32_t n, bm_columns* const* worlds) {
   return download_parts(ctx, n, worlds, "bm_download_worlds");
 }
 
-int bm_colour(bm_context* ctx) {
+}  // extern "C"
+
+namespace {
+
+// bm_colour's work. capture: once the host's part is done and its one copy queued, the stream
+// starts capturing (for bm_contact_phase's graph), and the launches are queued without a wait;
+// *captured says whether it did (not for the streamed path, whose sort allocates as it goes).
+int colour_run(bm_context* ctx, bool capture, bool* captured) {
+  if (captured) *captured = false;
   if (!ctx) return BM_ERR_ARGUMENT;
   if (!ctx->uploaded) return fail(ctx, BM_ERR_ARGUMENT, "bm_colour: no batch uploaded");
   ctx->error.clear();
@@ -925,10 +1024,17 @@ int bm_colour(bm_context* ctx) {
   }
   cd.in_lds = ctx->colour_in_lds == 1 && lds <= ctx->colour_lds_limit ? 1 : 0;
   (void)hipGetLastError();
-  bm::launch_world_of(cd.manifold_offset, n, static_cast<int*>(ctx->world_of.p), ctx->stream);
   const int bodies0 = ctx->boff_host[1] - ctx->boff_host[0];
-  if (!cd.in_lds && n == 1 && ctx->colour_in_lds != 0 && manifolds > 0 &&
-      bm::colour_stream_lds(bodies0) <= ctx->colour_lds_limit) {
+  const bool streamed = !cd.in_lds && n == 1 && ctx->colour_in_lds != 0 && manifolds > 0 &&
+                        bm::colour_stream_lds(bodies0) <= ctx->colour_lds_limit;
+  if (int rc = grow_device(ctx, &ctx->spare_arena, &ctx->spare_cap, ctx->arranged_end, "bm_colour")) return rc;
+  if (capture && !streamed) {
+    HIP_OR_FAIL(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal),
+                "bm_contact_phase: hipStreamBeginCapture");
+    *captured = true;
+  }
+  bm::launch_world_of(cd.manifold_offset, n, static_cast<int*>(ctx->world_of.p), ctx->stream);
+  if (streamed) {
     // One world too large to colour in LDS whose bodies' masks fit: streamed, then sorted.
     const size_t temp = bm::colour_stream_scratch((int)manifolds);
     const size_t need = manifolds * (8 + 8 + 4 + 4) + 16 + temp + 256;
@@ -943,7 +1049,6 @@ int bm_colour(bm_context* ctx) {
   HIP_OR_FAIL(ctx, hipGetLastError(), "bm_colour: launch");
   // Arrange: every column narrow() writes, into the spare arena at the same offsets, then
   // the arranged block is copied back in one piece.
-  if (int rc = grow_device(ctx, &ctx->spare_arena, &ctx->spare_cap, ctx->arranged_end, "bm_colour")) return rc;
   bm::ArrangeDev ad{};
   ad.manifolds = (long)manifolds;
   ad.manifold_offset = cd.manifold_offset;
@@ -972,10 +1077,16 @@ int bm_colour(bm_context* ctx) {
   rd.gap = static_cast<double*>(buf("gap"));
   bm::launch_rest(rd, ctx->stream);
   HIP_OR_FAIL(ctx, hipGetLastError(), "bm_colour: rest");
-  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_colour: kernel");
+  if (!capture) HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_colour: kernel");
   ctx->laid_out = true;
   return BM_OK;
 }
+
+}  // namespace
+
+extern "C" {
+
+int bm_colour(bm_context* ctx) { return colour_run(ctx, false, nullptr); }
 
 int bm_collide_host(bm_context* ctx, int32_t n, const double* a, const double* b, const double* margin,
                     const double* tolerance, int32_t* count, double* normal, double* points, double* separations,

@@ -3469,6 +3469,7 @@ int bm_batch_create(bm_batch** out, int32_t device, int32_t threads) {
     return rc;
   }
   bm_set_device_only(B->ctx, 1);
+  bm_set_graph(B->ctx, 1);
   B->pool = new Pool(threads > 1 ? threads - 1 : 0);
   *out = B;
   return BM_OK;
@@ -3571,8 +3572,7 @@ int bm_batch_step(bm_batch* B, double dt) {
   for (int i = 0; i < n; i++) version = (version ^ W[i]->body_version) * 1099511628211ull;
   bm_set_body_version(B->ctx, version != 0 ? version : 1);
   if (int rc = bm_upload_worlds(B->ctx, n, ins.data())) return device_failed(rc);
-  if (int rc = bm_colour(B->ctx)) return device_failed(rc);
-  if (int rc = bm_soft_step(B->ctx)) return device_failed(rc);
+  if (int rc = bm_contact_phase(B->ctx)) return device_failed(rc);
   if (int rc = bm_download_worlds(B->ctx, n, hs.data())) return device_failed(rc);
   lap(4);
   const double soft_ms = bm_last_kernel_ms(B->ctx);
@@ -3585,6 +3585,16 @@ int bm_batch_step(bm_batch* B, double dt) {
   lap(5);
   if (int rc = first_failure(B, rcs)) return batch_broke(B, rc);
   return BM_OK;
+}
+
+int bm_batch_set_graph(bm_batch* B, int32_t on) {
+  if (!B) return BM_ERR_ARGUMENT;
+  return bm_set_graph(B->ctx, on);
+}
+
+int bm_batch_graph_counts(const bm_batch* B, int64_t* out) {
+  if (!B || !out) return BM_ERR_ARGUMENT;
+  return bm_graph_counts(B->ctx, out);
 }
 
 int bm_batch_times(const bm_batch* B, double* out) {

@@ -77,7 +77,8 @@ def copy_columns(cols):
 EXPORTS = ["bm_create", "bm_destroy", "bm_last_error", "bm_upload", "bm_soft_step", "bm_download",
            "bm_soft_step_host", "bm_last_kernel_ms", "bm_set_launch", "bm_lds_limit", "bm_colour", "bm_collide_host", "bm_meet_host", "bm_set_colour_lds", "bm_set_hulls",
            "bm_set_fields", "bm_level_host", "bm_set_wide", "bm_set_device_only", "bm_set_body_version",
-           "bm_level_noalloc", "bm_level_packed", "bm_meet_indexed", "bm_world_moved", "bm_world_tap", "bm_world_events_of", "bm_world_set_broadphase",
+           "bm_level_noalloc", "bm_level_packed", "bm_meet_indexed", "bm_contact_phase", "bm_set_graph",
+           "bm_graph_counts", "bm_batch_set_graph", "bm_batch_graph_counts", "bm_world_moved", "bm_world_tap", "bm_world_events_of", "bm_world_set_broadphase",
            "bm_broad_set_bodies", "bm_broad_set_state", "bm_broad_keep", "bm_broad_set_trees", "bm_broad_bounds",
            "bm_broad_query", "bm_broad_holds",
            "bm_world_create", "bm_world_destroy", "bm_world_error", "bm_world_add", "bm_world_step", "bm_world_read",
@@ -180,6 +181,8 @@ def world_signatures(lib):
     lib.bm_batch_error.restype = ctypes.c_char_p
     lib.bm_batch_world.argtypes = [vp, ctypes.POINTER(vp), _D, ctypes.c_int32, ctypes.c_int32]
     lib.bm_batch_step.argtypes = [vp, ctypes.c_double]
+    lib.bm_batch_set_graph.argtypes = [vp, ctypes.c_int32]
+    lib.bm_batch_graph_counts.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
     lib.bm_world_times.argtypes = [vp, _D]
     lib.bm_batch_times.argtypes = [vp, _D]
     lib.bm_world_record.argtypes = [vp, ctypes.c_int32]
@@ -629,6 +632,17 @@ class WorldBatch:
         out = np.zeros(len(World.TIMES))
         self.check(self.lib.bm_batch_times(self.ptr, out.ctypes.data_as(_D)), "bm_batch_times")
         return dict(zip(World.TIMES, out.tolist()))
+
+    def set_graph(self, on):
+        """Whether the contact phase goes to the device as a HIP graph (bm_batch_set_graph)."""
+        self.check(self.lib.bm_batch_set_graph(self.ptr, 1 if on else 0), "bm_batch_set_graph")
+
+    def graph_counts(self):
+        """{launches, made, updated}: the contact phase's graphs so far (bm_batch_graph_counts)."""
+        out = np.zeros(3, np.int64)
+        self.check(self.lib.bm_batch_graph_counts(self.ptr, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))),
+                   "bm_batch_graph_counts")
+        return dict(zip(("launches", "made", "updated"), out.tolist()))
 
     def close(self):
         if self.ptr:

@@ -313,6 +313,9 @@ def c4_run(threads, loop_sample=16):
     before = batch.times()
     soft_ms, soft_bytes, rows, walls = [], [], [], []
     for s in range(steps):
+        # Every tenth step launch by launch, so the shared soft launch's own time can be read;
+        # the others with the contact phase as one HIP graph.
+        batch.set_graph(s % 10 != 9)
         t = time.perf_counter()
         batch.step(dt)
         walls.append(time.perf_counter() - t)
@@ -325,6 +328,7 @@ def c4_run(threads, loop_sample=16):
             print(f"bench: batched step {s + 1}", file=sys.stderr, flush=True)
     wall = float(np.sum(walls))
     after = batch.times()
+    graphs = batch.graph_counts()
     exact = all(hashlib.sha256(w.read()["transforms"].tobytes()).digest() == batchedcheck.final_of(fx, i)
                 for i, w in enumerate(worlds))
     batch.close()
@@ -349,7 +353,7 @@ def c4_run(threads, loop_sample=16):
             "looped_plain_world_steps_per_s": loop_rate, "speedup_vs_looped": rate / loop_rate,
             "looped_sample": f"every {len(specs) // loop_sample}th world ({len(picks)} worlds, {sample_bodies} bodies) "
                              f"stepped alone through bm_world_step for {steps} steps",
-            "bit_exact": bool(exact), "looped_bit_exact": bool(loop_exact),
+            "bit_exact": bool(exact), "looped_bit_exact": bool(loop_exact), "contact_phase_graphs": graphs,
             "ms_per_step_by_part": {k: (after[k] - before[k]) / steps for k in after},
             "soft_roofline": {"kernel": "soft_step_kernel (all worlds, one launch)", "bound": "hbm",
                               "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

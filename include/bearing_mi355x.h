@@ -278,6 +278,20 @@ int bm_set_launch(bm_context* ctx, int32_t shape, int32_t resident);
  * whole-step path); results are the same bits. Default 0. */
 int bm_set_device_only(bm_context* ctx, int32_t keep);
 
+/* The phase's device part in one call: bm_colour then bm_soft_step. With the graph on
+ * (bm_set_graph) and a batch of more than one world, their launches (the world table, the
+ * colouring, arrange and its copy back, rest, the soft step and its two timing events) are
+ * captured from the context's stream into a HIP graph and launched as one; the executable graph
+ * of the call before is updated to this call's arguments (the arena is laid out by each step's
+ * counts, so addresses move) and made anew only when the update is refused. Results are the
+ * same bits either way; bm_last_kernel_ms is then the whole graph's time (colouring and soft
+ * step), not the soft step's alone. */
+int bm_contact_phase(bm_context* ctx);
+int bm_set_graph(bm_context* ctx, int32_t on);
+/* bm_contact_phase's graphs so far: out[0] launches, out[1] executable graphs made, out[2]
+ * updates of the one before. */
+int bm_graph_counts(const bm_context* ctx, int64_t* out);
+
 /* version != 0: the bodies' columns no step changes (dims, inertia, inv_inertia, gravity_scale,
  * damping, angular_damping, type, shape, lock, framed, joint_head) are kept on the device in a
  * block of their own, sent by the first upload under this version and not again while uploads
@@ -520,6 +534,10 @@ const char* bm_batch_error(const bm_batch* batch);
 /* A new world of the batch, as bm_world_create makes one (bodies, hulls and fields are added
  * to it as to any world); it lives until bm_batch_destroy. */
 int bm_batch_world(bm_batch* batch, bm_world** out, const double* gravity, int32_t substeps, int32_t sleeping);
+/* Whether the batch's contact phase goes to the device as a HIP graph (bm_contact_phase);
+ * default on. */
+int bm_batch_set_graph(bm_batch* batch, int32_t on);
+int bm_batch_graph_counts(const bm_batch* batch, int64_t* out);
 /* Every world of the batch, one step of dt. A step that fails after the worlds began theirs
  * leaves all of them part-way: every later step of the batch returns the same code and
  * message, and so does the message of each of its worlds. */

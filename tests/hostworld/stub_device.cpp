@@ -179,6 +179,11 @@ int bm_upload_worlds(bm_context* ctx, int32_t n, const bm_columns* const* worlds
 int bm_download_worlds(bm_context*, int32_t, bm_columns* const*) { return BM_OK; }
 int bm_set_device_only(bm_context*, int32_t) { return BM_OK; }
 int bm_set_body_version(bm_context*, uint64_t) { return BM_OK; }
+int bm_set_graph(bm_context*, int32_t) { return BM_OK; }
+int bm_graph_counts(const bm_context*, int64_t* out) {
+  out[0] = out[1] = out[2] = 0;
+  return BM_OK;
+}
 
 }  // extern "C"
 
@@ -281,6 +286,11 @@ int bm_soft_step(bm_context* ctx) {
   for (bm_columns& h : ctx->hs)
     if (int rc = soft_one(ctx, h)) return rc;
   return BM_OK;
+}
+
+int bm_contact_phase(bm_context* ctx) {
+  if (int rc = bm_colour(ctx)) return rc;
+  return bm_soft_step(ctx);
 }
 
 int bm_collide_host(bm_context*, int32_t n, const double* a, const double* b, const double* margin,

@@ -48,21 +48,28 @@ def test_gpu_world_refuses_what_it_does_not_handle():
     refuses(lambda: native.World(0))
 
 
-def test_gpu_batch_steps_every_world_as_alone():
+@pytest.mark.parametrize("graph", [True, False])
+def test_gpu_batch_steps_every_world_as_alone(graph):
     # Worlds of every kind stepped together on the device (bm_batch_step: one narrowphase
-    # launch a pass, one colouring and one soft-step launch for all of them), each matching its
-    # own reference trajectory on every step.
+    # launch a pass, one colouring and one soft-step launch for all of them, as a HIP graph or
+    # launch by launch), each matching its own reference trajectory on every step.
     from worldcheck import fill_world, run_together
     names = ["tower", "pile_small", "darts", "capsules", "rocks", "terrain", "tower", "rain_small", "pyramid"]
-    print("batch of", names, flush=True)
+    print("batch of", names, "graph" if graph else "launches", flush=True)
     fxs = [traj(n) for n in names]
     batch = native.WorldBatch(0, threads=8)
+    batch.set_graph(graph)
     worlds = [fill_world(batch.world(), fx) for fx in fxs]
     steps = min(f.meta["steps"] for f in fxs)
     matched, why = run_together(batch, worlds, fxs, steps)
+    counts = batch.graph_counts()
     batch.close()
     assert why is None, why
     assert matched == steps
+    # One graph launch a step when on (every step had contacts to colour), none when off.
+    print("graphs", counts, flush=True)
+    assert counts["launches"] == (steps if graph else 0), counts
+    assert counts["made"] + counts["updated"] == counts["launches"], counts
 
 
 def test_gpu_mixed_pile_first_steps():
