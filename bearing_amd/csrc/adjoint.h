@@ -465,6 +465,9 @@ struct StepView {
 
 struct Ctx {
   int tid, threads;
+  // This thread's block of ROW_SLOT doubles for a row's adjoints while its reverse runs (the
+  // workgroup's LDS on the device); only the backward pass uses it.
+  double* slot;
 };
 
 
@@ -1438,6 +1441,66 @@ struct RowsBar {
 
 HD bool moves(const Bodies& B, int g) { return B.type[g] == DYNAMIC; }
 
+// One working row's adjoints (its WorkBar fields but the anchors' and adjusted separations')
+// while a sweep's reverse works on the row, in the thread's slot (Ctx::slot: LDS on the
+// device): read once before, written once after. Between the two the row's reverse adds to
+// them in the slot and stores nothing to global memory until its separations, so its reads of
+// the tape, the forward's columns and the bodies need not wait on stores before them. A warm
+// start's reverse (SOLVE false) touches only the first group. Every sum is taken in the same
+// order as on the columns themselves.
+template <int N, bool SOLVE>
+struct RowAcc {
+  double impulse[N], tangent[2], roll[2], spin, jac[N * J], fjac[FJ], normal[3], t1[3], t2[3], ima, imb, Ia[6], Ib[6];
+  double drift[2], relvel[N], tm[3], nmass[N], block[P * P], rm[2], sm, lever[N], mu;
+
+  HD void load(const WorkBar& WB, int i) {
+    for (int k = 0; k < N; k++) impulse[k] = WB.impulse()[P * i + k];
+    for (int c = 0; c < 2; c++) { tangent[c] = WB.tangent()[2 * i + c]; roll[c] = WB.roll()[2 * i + c]; }
+    spin = WB.spin()[i];
+    for (int c = 0; c < N * J; c++) jac[c] = WB.jac()[(size_t)J * P * i + c];
+    for (int c = 0; c < FJ; c++) fjac[c] = WB.fjac()[(size_t)FJ * i + c];
+    for (int c = 0; c < 3; c++) { normal[c] = WB.normal()[3 * i + c]; t1[c] = WB.t1()[3 * i + c]; t2[c] = WB.t2()[3 * i + c]; }
+    ima = WB.ima()[i];
+    imb = WB.imb()[i];
+    for (int c = 0; c < 6; c++) { Ia[c] = WB.Ia()[6 * i + c]; Ib[c] = WB.Ib()[6 * i + c]; }
+    if (!SOLVE) return;
+    for (int c = 0; c < 2; c++) { drift[c] = WB.drift()[2 * i + c]; rm[c] = WB.rm()[2 * i + c]; }
+    for (int k = 0; k < N; k++) {
+      relvel[k] = WB.relvel()[P * i + k]; nmass[k] = WB.nmass()[P * i + k];
+      lever[k] = WB.lever()[P * i + k];
+    }
+    for (int c = 0; c < 3; c++) tm[c] = WB.tm()[3 * i + c];
+    for (int c = 0; c < P * P; c++) block[c] = WB.block()[P * P * i + c];
+    sm = WB.sm()[i];
+    mu = WB.mu()[i];
+  }
+  HD void store(const WorkBar& WB, int i) const {
+    for (int k = 0; k < N; k++) WB.impulse()[P * i + k] = impulse[k];
+    for (int c = 0; c < 2; c++) { WB.tangent()[2 * i + c] = tangent[c]; WB.roll()[2 * i + c] = roll[c]; }
+    WB.spin()[i] = spin;
+    for (int c = 0; c < N * J; c++) WB.jac()[(size_t)J * P * i + c] = jac[c];
+    for (int c = 0; c < FJ; c++) WB.fjac()[(size_t)FJ * i + c] = fjac[c];
+    for (int c = 0; c < 3; c++) { WB.normal()[3 * i + c] = normal[c]; WB.t1()[3 * i + c] = t1[c]; WB.t2()[3 * i + c] = t2[c]; }
+    WB.ima()[i] = ima;
+    WB.imb()[i] = imb;
+    for (int c = 0; c < 6; c++) { WB.Ia()[6 * i + c] = Ia[c]; WB.Ib()[6 * i + c] = Ib[c]; }
+    if (!SOLVE) return;
+    for (int c = 0; c < 2; c++) { WB.drift()[2 * i + c] = drift[c]; WB.rm()[2 * i + c] = rm[c]; }
+    for (int k = 0; k < N; k++) {
+      WB.relvel()[P * i + k] = relvel[k]; WB.nmass()[P * i + k] = nmass[k];
+      WB.lever()[P * i + k] = lever[k];
+    }
+    for (int c = 0; c < 3; c++) WB.tm()[3 * i + c] = tm[c];
+    for (int c = 0; c < P * P; c++) WB.block()[P * P * i + c] = block[c];
+    WB.sm()[i] = sm;
+    WB.mu()[i] = mu;
+  }
+};
+// A slot's doubles: room for the largest RowAcc; odd, so the slots of a wave's lanes start in
+// different LDS banks.
+constexpr int ROW_SLOT = 141;
+static_assert(sizeof(RowAcc<4, true>) <= ROW_SLOT * sizeof(double), "a row's adjoints fit a slot");
+
 // y = I x for a symmetric I (6: xx xy xz yy yz zz): adds the adjoints of I and x.
 HD void sym_vjp(const double* I, const double* x, const double* yb, double* Ib, double* xb) {
   if (Ib) {
@@ -1507,16 +1570,18 @@ HD void solve2(double al, double a, double b, double c, double n1, double n2, do
 // in WB go from after to before; what the row read of prepare, the normal and the bodies'
 // inverse masses and inertias gets its adjoints added; sepb gets the separations'.
 template <int N>
-HD void solve_vjp(const Work& W, const WorkBar& WB, const Rows& R, int r, int i, double ima, double imb,
+HD void solve_vjp(const Work& W, RowAcc<N, true>& A, const Rows& R, int r, int i, double ima, double imb,
                   const double* Ia, const double* Ib, const Soft& soft, double h, bool bias, const TapeRow& T,
-                  Pair& sb, double& imab, double& imbb, double* Iab, double* Ibb, double (&sepb)[N]) {
+                  Pair& sb, double& imab, double& imbb, double (&sepb)[N]) {
+  double* Iab = A.Ia;
+  double* Ibb = A.Ib;
   const int i3 = 3 * i;
   const double* n = W.normal() + i3;
   const double* t1 = W.t1() + i3;
   const double* t2 = W.t2() + i3;
-  double* nb = WB.normal() + i3;
-  double* t1b = WB.t1() + i3;
-  double* t2b = WB.t2() + i3;
+  double* nb = A.normal;
+  double* t1b = A.t1;
+  double* t2b = A.t2;
   const double mu = W.mu()[i], patch = R.patch[r];
   const uint32_t flags = (uint32_t)T[T_FLAGS];
   const double pressed = T[T_PRESSED], twisted = T[T_TWISTED];
@@ -1529,7 +1594,7 @@ HD void solve_vjp(const Work& W, const WorkBar& WB, const Rows& R, int r, int i,
     push_turn_vjp(sb, Ia, Ib, L, Lb, Iab, Ibb);
     const double dsb = n[0] * Lb[0] + n[1] * Lb[1] + n[2] * Lb[2];
     for (int c = 0; c < 3; c++) nb[c] += ds * Lb[c];
-    const double s1b = WB.spin()[i] + dsb;
+    const double s1b = A.spin + dsb;
     double s0b = -dsb;
     double limitb = 0;
     const int sclamp = (flags >> 3) & 3;
@@ -1539,7 +1604,7 @@ HD void solve_vjp(const Work& W, const WorkBar& WB, const Rows& R, int r, int i,
       s0b += s1b;
       const double dw[3] = {T[T_DW], T[T_DW + 1], T[T_DW + 2]};
       const double proj = dw[0] * n[0] + dw[1] * n[1] + dw[2] * n[2];
-      WB.sm()[i] -= s1b * proj;
+      A.sm -= s1b * proj;
       const double sm = W.sm()[i];
       for (int c = 0; c < 3; c++) {
         const double g = -s1b * sm * n[c];
@@ -1548,13 +1613,13 @@ HD void solve_vjp(const Work& W, const WorkBar& WB, const Rows& R, int r, int i,
       }
     }
     if (N == 1) {
-      WB.mu()[i] += SPIN_DISC * patch * pressed * limitb;
+      A.mu += SPIN_DISC * patch * pressed * limitb;
       pressed_b += SPIN_DISC * mu * patch * limitb;
     } else {
-      WB.mu()[i] += twisted * limitb;
+      A.mu += twisted * limitb;
       twisted_b += mu * limitb;
     }
-    WB.spin()[i] = s0b;
+    A.spin = s0b;
   }
   // roll
   if (N <= 2) {
@@ -1566,7 +1631,7 @@ HD void solve_vjp(const Work& W, const WorkBar& WB, const Rows& R, int r, int i,
     const double d1b = t1[0] * Lb[0] + t1[1] * Lb[1] + t1[2] * Lb[2];
     const double d2b = t2[0] * Lb[0] + t2[1] * Lb[1] + t2[2] * Lb[2];
     for (int c = 0; c < 3; c++) { t1b[c] += d1 * Lb[c]; t2b[c] += d2 * Lb[c]; }
-    const double o1b = WB.roll()[2 * i] + d1b, o2b = WB.roll()[2 * i + 1] + d2b;
+    const double o1b = A.roll[0] + d1b, o2b = A.roll[1] + d2b;
     double r1b = -d1b, r2b = -d2b;
     double n1b = o1b, n2b = o2b;
     if ((flags >> 2) & 1) {
@@ -1585,21 +1650,21 @@ HD void solve_vjp(const Work& W, const WorkBar& WB, const Rows& R, int r, int i,
     const double u[3] = {T[T_U], T[T_U + 1], T[T_U + 2]};
     const double p1 = u[0] * t1[0] + u[1] * t1[1] + u[2] * t1[2];
     const double p2 = u[0] * t2[0] + u[1] * t2[1] + u[2] * t2[2];
-    WB.rm()[2 * i] -= n1b * p1;
-    WB.rm()[2 * i + 1] -= n2b * p2;
+    A.rm[0] -= n1b * p1;
+    A.rm[1] -= n2b * p2;
     for (int c = 0; c < 3; c++) {
       const double ub = -n1b * rm1 * t1[c] - n2b * rm2 * t2[c];
       sb.v[9 + c] += ub; sb.v[3 + c] -= ub;
       t1b[c] -= n1b * rm1 * u[c];
       t2b[c] -= n2b * rm2 * u[c];
     }
-    WB.roll()[2 * i] = r1b; WB.roll()[2 * i + 1] = r2b;
+    A.roll[0] = r1b; A.roll[1] = r2b;
   }
   // friction
   {
     const double* fj = W.fjac() + FJ * i;
     const double* g = fj + 12;
-    double* fjb = WB.fjac() + FJ * i;
+    double* fjb = A.fjac;
     double* gb = fjb + 12;
     const double o1 = T[T_O1], o2 = T[T_O2];
     const double d1 = T[T_T1] - o1, d2 = T[T_T2] - o2;
@@ -1616,11 +1681,11 @@ HD void solve_vjp(const Work& W, const WorkBar& WB, const Rows& R, int r, int i,
       fjb[6 + c] -= d1 * sb.v[3 + c]; gb[6 + c] -= d2 * sb.v[3 + c];
       fjb[9 + c] += d1 * sb.v[9 + c]; gb[9 + c] += d2 * sb.v[9 + c];
     }
-    const double o1b = WB.tangent()[2 * i] + d1b, o2b = WB.tangent()[2 * i + 1] + d2b;
+    const double o1b = A.tangent[0] + d1b, o2b = A.tangent[1] + d2b;
     double in1b = -d1b, in2b = -d2b;  // of the tangent impulses before
     double n1b = o1b, n2b = o2b;      // of n before the cone
     const double tm0 = W.tm()[i3], tm1 = W.tm()[i3 + 1], tm2 = W.tm()[i3 + 2];
-    double* tmb = WB.tm() + i3;
+    double* tmb = A.tm;
     if ((flags >> 1) & 1) {
       // out = limit x / |x|, x = (I + al M)^-1 n, al where |x| = limit.
       const double limit = T[T_LIMIT], al = T[T_AL], fx = T[T_X], fy = T[T_Y];
@@ -1653,7 +1718,7 @@ HD void solve_vjp(const Work& W, const WorkBar& WB, const Rows& R, int r, int i,
         // limit 0, or no Newton step taken: x = n
         n1b += xb; n2b += yb;
       }
-      WB.mu()[i] += pressed * limitb;
+      A.mu += pressed * limitb;
       pressed_b += mu * limitb;
     }
     // n = o - M v
@@ -1666,15 +1731,15 @@ HD void solve_vjp(const Work& W, const WorkBar& WB, const Rows& R, int r, int i,
     // the slip's pull (bias) depends only on the drift
     const int fbias = (flags >> 5) & 3;
     if (fbias == 1) {
-      WB.drift()[2 * i] += soft.bias_rate * v1b;
-      WB.drift()[2 * i + 1] += soft.bias_rate * v2b;
+      A.drift[0] += soft.bias_rate * v1b;
+      A.drift[1] += soft.bias_rate * v2b;
     } else if (fbias == 2) {
       const double tol = R.tolerance[r], max_bias = tol / h;
       const double l = dsqrt(T[T_BB]);
       const double u1 = T[T_B1] / l, u2 = T[T_B2] / l, k = max_bias / l;
       const double ou = v1b * u1 + v2b * u2;
-      WB.drift()[2 * i] += soft.bias_rate * k * (v1b - u1 * ou);
-      WB.drift()[2 * i + 1] += soft.bias_rate * k * (v2b - u2 * ou);
+      A.drift[0] += soft.bias_rate * k * (v1b - u1 * ou);
+      A.drift[1] += soft.bias_rate * k * (v2b - u2 * ou);
     }
     // v1, v2 from the velocities after the normal pushes
     for (int c = 0; c < 3; c++) {
@@ -1688,18 +1753,18 @@ HD void solve_vjp(const Work& W, const WorkBar& WB, const Rows& R, int r, int i,
       fjb[3 + c] += v1b * mwb; gb[3 + c] += v2b * mwb;
       fjb[c] -= v1b * mwa; gb[c] -= v2b * mwa;
     }
-    WB.tangent()[2 * i] = in1b; WB.tangent()[2 * i + 1] = in2b;
+    A.tangent[0] = in1b; A.tangent[1] = in2b;
   }
   // normal
   double nextb[N], oldb[N];
   for (int k = 0; k < N; k++) {
-    nextb[k] = WB.impulse()[P * i + k] + pressed_b + W.lever()[P * i + k] * twisted_b;
-    WB.lever()[P * i + k] += T[T_NEXT + k] * twisted_b;
+    nextb[k] = A.impulse[k] + pressed_b + W.lever()[P * i + k] * twisted_b;
+    A.lever[k] += T[T_NEXT + k] * twisted_b;
     oldb[k] = 0;
   }
   for (int k = N - 1; k >= 0; k--) {
     const int j = P * i + k;
-    const double ddb = push_normal_vjp(sb, ima, imb, n, W.jac() + J * j, T[T_DD + k], nb, imab, imbb, WB.jac() + J * j);
+    const double ddb = push_normal_vjp(sb, ima, imb, n, W.jac() + J * j, T[T_DD + k], nb, imab, imbb, A.jac + J * k);
     nextb[k] += ddb;
     oldb[k] -= ddb;
   }
@@ -1708,8 +1773,8 @@ HD void solve_vjp(const Work& W, const WorkBar& WB, const Rows& R, int r, int i,
   for (int k = 0; k < N; k++) {
     closingb[k] = 0; targetb[k] = 0;
     if ((first >> k) & 1) {
-      closingb[k] += WB.relvel()[P * i + k];
-      WB.relvel()[P * i + k] = 0;
+      closingb[k] += A.relvel[k];
+      A.relvel[k] = 0;
     }
   }
   // what each point's target case made of its mass and impulse scales and give
@@ -1724,14 +1789,14 @@ HD void solve_vjp(const Work& W, const WorkBar& WB, const Rows& R, int r, int i,
       const bool soft_k = tcase(k) == 1 || tcase(k) == 2;
       const double ms = soft_k ? soft.mass_scale : 1, is = soft_k ? soft.impulse_scale : 0;
       oldb[k] += g * (1 - is);
-      WB.nmass()[j] -= g * ms * (T[T_CLOSING + k] + T[T_TARGET + k]);
+      A.nmass[k] -= g * ms * (T[T_CLOSING + k] + T[T_TARGET + k]);
       closingb[k] -= g * nm * ms;
       targetb[k] -= g * nm * ms;
     }
   } else {
     // solved on the set S: A_SS x = wanted_S, A = K with diagonal scaled by 1 + give
     const double* Kg = W.block() + P * P * i;
-    double* Kb = WB.block() + P * P * i;
+    double* Kb = A.block;
     const int set = (flags >> 8) & 15;
     double xb[N], give[N], solved[N];
     int idx[N], ns = 0;
@@ -1790,8 +1855,8 @@ HD void solve_vjp(const Work& W, const WorkBar& WB, const Rows& R, int r, int i,
   for (int c = 0; c < 12; c++) s0.v[c] = T[T_S + c];
   for (int k = N - 1; k >= 0; k--) {
     const int j = P * i + k;
-    closing_vjp(s0, n, W.jac() + J * j, closingb[k], sb, nb, WB.jac() + J * j);
-    WB.impulse()[j] = oldb[k];
+    closing_vjp(s0, n, W.jac() + J * j, closingb[k], sb, nb, A.jac + J * k);
+    A.impulse[k] = oldb[k];
     // target(s)
     const double inv = 1 / h;
     double g = 0;
@@ -1805,23 +1870,26 @@ HD void solve_vjp(const Work& W, const WorkBar& WB, const Rows& R, int r, int i,
 // The reverse of warm_row: the velocities only gain pushes, so their adjoints pass
 // through; the impulses, normal, tangents, lever terms and inertias gain theirs. The
 // impulses are the row's as the solve that follows read them (its tape).
-HD void warm_vjp(const Work& W, const WorkBar& WB, const Rows& R, int r, int i, double ima, double imb,
-                 const double* Ia, const double* Ib, const TapeRow& T, const Pair& sb, double& imab, double& imbb,
-                 double* Iab, double* Ibb) {
+template <int N>
+HD void warm_vjp(const Work& W, RowAcc<N, false>& A, const Rows& R, int r, int i, double ima, double imb,
+                 const double* Ia, const double* Ib, const TapeRow& T, const Pair& sb, double& imab, double& imbb) {
+  double* Iab = A.Ia;
+  double* Ibb = A.Ib;
   const int count = R.count[r], i3 = 3 * i;
   const double* n = W.normal() + i3;
   const double* t1 = W.t1() + i3;
   const double* t2 = W.t2() + i3;
-  double* nb = WB.normal() + i3;
-  double* t1b = WB.t1() + i3;
-  double* t2b = WB.t2() + i3;
-  for (int k = 0; k < count; k++) {
+  double* nb = A.normal;
+  double* t1b = A.t1;
+  double* t2b = A.t2;
+  for (int k = 0; k < N; k++) {
+    if (k >= count) break;
     const int j = P * i + k;
-    WB.impulse()[j] += push_normal_vjp(sb, ima, imb, n, W.jac() + J * j, T[T_IMP + k], nb, imab, imbb, WB.jac() + J * j);
+    A.impulse[k] += push_normal_vjp(sb, ima, imb, n, W.jac() + J * j, T[T_IMP + k], nb, imab, imbb, A.jac + J * k);
   }
   {
     const double* fj = W.fjac() + FJ * i;
-    double* fjb = WB.fjac() + FJ * i;
+    double* fjb = A.fjac;
     const double l1 = T[T_O1], l2 = T[T_O2];
     double l1b = 0, l2b = 0;
     for (int c = 0; c < 3; c++) {
@@ -1835,7 +1903,7 @@ HD void warm_vjp(const Work& W, const WorkBar& WB, const Rows& R, int r, int i, 
       fjb[6 + c] -= l1 * sb.v[3 + c]; fjb[18 + c] -= l2 * sb.v[3 + c];
       fjb[9 + c] += l1 * sb.v[9 + c]; fjb[21 + c] += l2 * sb.v[9 + c];
     }
-    WB.tangent()[2 * i] += l1b; WB.tangent()[2 * i + 1] += l2b;
+    A.tangent[0] += l1b; A.tangent[1] += l2b;
   }
   {
     const double r1 = T[T_R1], r2 = T[T_R2], sp = T[T_S0];
@@ -1844,10 +1912,10 @@ HD void warm_vjp(const Work& W, const WorkBar& WB, const Rows& R, int r, int i, 
     double Lb[3] = {0, 0, 0};
     push_turn_vjp(sb, Ia, Ib, L, Lb, Iab, Ibb);
     if (count <= 2) {
-      WB.roll()[2 * i] += t1[0] * Lb[0] + t1[1] * Lb[1] + t1[2] * Lb[2];
-      WB.roll()[2 * i + 1] += t2[0] * Lb[0] + t2[1] * Lb[1] + t2[2] * Lb[2];
+      A.roll[0] += t1[0] * Lb[0] + t1[1] * Lb[1] + t1[2] * Lb[2];
+      A.roll[1] += t2[0] * Lb[0] + t2[1] * Lb[1] + t2[2] * Lb[2];
     }
-    WB.spin()[i] += n[0] * Lb[0] + n[1] * Lb[1] + n[2] * Lb[2];
+    A.spin += n[0] * Lb[0] + n[1] * Lb[1] + n[2] * Lb[2];
     for (int c = 0; c < 3; c++) { t1b[c] += r1 * Lb[c]; t2b[c] += r2 * Lb[c]; nb[c] += sp * Lb[c]; }
   }
 }
@@ -1878,11 +1946,12 @@ HD void rotate_vjp(const double* q, const double* v, bool inverse, const double*
 // The reverse of separations (not known): from the separations' adjoints into the
 // bodies' positions, starts and turns, the row's anchors, adjusted and normal.
 template <int N>
-HD void separations_vjp(const Bodies& B, const BodiesBar& BB, const Work& W, const WorkBar& WB, int ga, int gb, int i,
+HD void separations_vjp(const Bodies& B, const BodiesBar& BB, const Work& W, const WorkBar& WB, RowAcc<N, true>& A,
+                        int ga, int gb, int i,
                         const double (&sepb)[N]) {
   const int i3 = 3 * i;
   const double* n = W.normal() + i3;
-  double* nb = WB.normal() + i3;
+  double* nb = A.normal;
   const double *pa = B.p + 3 * ga, *pb = B.p + 3 * gb;
   const double *qa = B.start + 3 * ga, *qb = B.start + 3 * gb;
   const V3 na = rotate(B.q + 4 * ga, n[0], n[1], n[2], true);
@@ -1923,7 +1992,7 @@ HD void separations_vjp(const Bodies& B, const BodiesBar& BB, const Work& W, con
 // adjoint, can all go back at once.
 template <int N, int KIND>
 HD void row_vjp_n(const Bodies& B, const BodiesBar& BB, const Work& W, const WorkBar& WB, const Rows& R, int boff, int r,
-                  int i, const Soft& soft, double h, const TapeRow& T) {
+                  int i, const Soft& soft, double h, const TapeRow& T, double* slot) {
   const int ga = boff + R.a[r], gb = boff + R.b[r];
   const bool ma = moves(B, ga), mb = moves(B, gb);
   Pair sb;
@@ -1935,29 +2004,35 @@ HD void row_vjp_n(const Bodies& B, const BodiesBar& BB, const Work& W, const Wor
   const double* Ia = B.inv_i + 6 * ga;
   const double* Ib = B.inv_i + 6 * gb;
   double imab = 0, imbb = 0;
-  double* Iab = WB.Ia() + 6 * i;
-  double* Ibb = WB.Ib() + 6 * i;
   if (KIND == 0) {
-    warm_vjp(W, WB, R, r, i, ima, imb, Ia, Ib, T, sb, imab, imbb, Iab, Ibb);
+    RowAcc<N, false>& A = *reinterpret_cast<RowAcc<N, false>*>(slot);
+    A.load(WB, i);
+    warm_vjp<N>(W, A, R, r, i, ima, imb, Ia, Ib, T, sb, imab, imbb);
+    A.ima += imab;
+    A.imb += imbb;
+    A.store(WB, i);
   } else {
+    RowAcc<N, true>& A = *reinterpret_cast<RowAcc<N, true>*>(slot);
+    A.load(WB, i);
     double sepb[N];
-    solve_vjp<N>(W, WB, R, r, i, ima, imb, Ia, Ib, soft, h, KIND == 1, T, sb, imab, imbb, Iab, Ibb, sepb);
-    separations_vjp<N>(B, BB, W, WB, ga, gb, i, sepb);
+    solve_vjp<N>(W, A, R, r, i, ima, imb, Ia, Ib, soft, h, KIND == 1, T, sb, imab, imbb, sepb);
+    separations_vjp<N>(B, BB, W, WB, A, ga, gb, i, sepb);
+    A.ima += imab;
+    A.imb += imbb;
+    A.store(WB, i);
     if (ma) for (int c = 0; c < 3; c++) { BB.v[3 * ga + c] = sb.v[c]; BB.w[3 * ga + c] = sb.v[3 + c]; }
     if (mb) for (int c = 0; c < 3; c++) { BB.v[3 * gb + c] = sb.v[6 + c]; BB.w[3 * gb + c] = sb.v[9 + c]; }
   }
-  WB.ima()[i] += imab;
-  WB.imb()[i] += imbb;
 }
 
 template <int KIND>
 HD void row_vjp(const Bodies& B, const BodiesBar& BB, const Work& W, const WorkBar& WB, const Rows& R, int boff, int r,
-                int i, const Soft& soft, double h, const TapeRow& T) {
+                int i, const Soft& soft, double h, const TapeRow& T, double* slot) {
   switch (R.count[r]) {
-    case 1: row_vjp_n<1, KIND>(B, BB, W, WB, R, boff, r, i, soft, h, T); break;
-    case 2: row_vjp_n<2, KIND>(B, BB, W, WB, R, boff, r, i, soft, h, T); break;
-    case 3: row_vjp_n<3, KIND>(B, BB, W, WB, R, boff, r, i, soft, h, T); break;
-    default: row_vjp_n<4, KIND>(B, BB, W, WB, R, boff, r, i, soft, h, T); break;
+    case 1: row_vjp_n<1, KIND>(B, BB, W, WB, R, boff, r, i, soft, h, T, slot); break;
+    case 2: row_vjp_n<2, KIND>(B, BB, W, WB, R, boff, r, i, soft, h, T, slot); break;
+    case 3: row_vjp_n<3, KIND>(B, BB, W, WB, R, boff, r, i, soft, h, T, slot); break;
+    default: row_vjp_n<4, KIND>(B, BB, W, WB, R, boff, r, i, soft, h, T, slot); break;
   }
 }
 
@@ -1970,7 +2045,7 @@ HD void sweep_vjp(const Ctx& c, Sync sync, const Bodies& B, const BodiesBar& BB,
     if (c.tid == 0)
       for (int lm = to - 1; lm >= from; lm--) {
         const int i = v.w0 + lm;
-        row_vjp<KIND>(B, BB, W, WB, R, v.boff, v.r0 + lm, i, soft, h, tape.row(tsweep, i));
+        row_vjp<KIND>(B, BB, W, WB, R, v.boff, v.r0 + lm, i, soft, h, tape.row(tsweep, i), c.slot);
       }
     sync();
   }
@@ -1979,7 +2054,7 @@ HD void sweep_vjp(const Ctx& c, Sync sync, const Bodies& B, const BodiesBar& BB,
     if (f0 == t0) continue;
     for (int lm = f0 + c.tid; lm < t0; lm += c.threads) {
       const int i = v.w0 + lm;
-      row_vjp<KIND>(B, BB, W, WB, R, v.boff, v.r0 + lm, i, soft, h, tape.row(tsweep, i));
+      row_vjp<KIND>(B, BB, W, WB, R, v.boff, v.r0 + lm, i, soft, h, tape.row(tsweep, i), c.slot);
     }
     sync();
   }
@@ -2651,7 +2726,7 @@ HD void step_reverse(const Ctx& c, Sync sync, const Bodies& B, const BodiesBar& 
       sweep_vjp<1>(c, sync, B, BB, W, WB, R, v, soft, h, tape, 2 * sub);
       // a warm start's rows leave every velocity adjoint as it is: all of them at once
       for (int lm = c.tid; lm < nrows; lm += c.threads)
-        row_vjp<0>(B, BB, W, WB, R, v.boff, v.r0 + lm, v.w0 + lm, soft, h, tape.row(2 * sub, v.w0 + lm));
+        row_vjp<0>(B, BB, W, WB, R, v.boff, v.r0 + lm, v.w0 + lm, soft, h, tape.row(2 * sub, v.w0 + lm), c.slot);
       sync();
     }
     for (int s = c.tid; s < v.nbodies; s += c.threads) {

@@ -36,7 +36,7 @@ struct Barrier {
 
 __global__ void __launch_bounds__(HELD_THREADS) held_forward_kernel(adj::Trajectory T, adj::HeldIO io) {
   const int wld = blockIdx.x;
-  const adj::Ctx c{(int)threadIdx.x, (int)blockDim.x};
+  const adj::Ctx c{(int)threadIdx.x, (int)blockDim.x, nullptr};
   adj::start_world(T, io, wld, c.tid, c.threads);
   __syncthreads();
   adj::forward_world(c, Barrier{}, T, wld);
@@ -45,8 +45,10 @@ __global__ void __launch_bounds__(HELD_THREADS) held_forward_kernel(adj::Traject
 }
 
 __global__ void __launch_bounds__(HELD_THREADS) held_backward_kernel(adj::Trajectory T, adj::HeldIO io) {
+  // Each thread's row adjoints while a row's reverse runs (72 KB a workgroup: two worlds a CU).
+  __shared__ double slots[HELD_THREADS * adj::ROW_SLOT];
   const int wld = blockIdx.x;
-  const adj::Ctx c{(int)threadIdx.x, (int)blockDim.x};
+  const adj::Ctx c{(int)threadIdx.x, (int)blockDim.x, slots + threadIdx.x * adj::ROW_SLOT};
   adj::start_bars(T, io, wld, c.tid, c.threads);
   __syncthreads();
   adj::backward_world(c, Barrier{}, T, wld);

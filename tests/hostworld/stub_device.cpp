@@ -111,7 +111,7 @@ int bm_held_forward(bm_context* ctx, const double* state0, const double* mass, c
   std::memcpy(H.io.mass_in, mass, sizeof(double) * nb);
   std::memcpy(H.io.friction_in, friction, sizeof(double) * nb);
   std::memcpy(H.io.restitution_in, restitution, sizeof(double) * nb);
-  const Ctx c{0, 1};
+  const Ctx c{0, 1, nullptr};
   for (int w = 0; w < H.T.worlds; w++) {
     start_world(H.T, H.io, w, 0, 1);
     forward_world(c, [] {}, H.T, w);
@@ -133,7 +133,8 @@ int bm_held_backward(bm_context* ctx, const double* state_bar, double* state0_ba
   for (double* p : {T.RB.store_impulse}) std::memset(p, 0, sizeof(double) * P * rows);
   for (double* p : {T.RB.store_friction, T.RB.store_twist, T.RB.store_slip}) std::memset(p, 0, sizeof(double) * 3 * rows);
   for (double* p : {T.RB.mu, T.RB.e}) std::memset(p, 0, sizeof(double) * rows);
-  const Ctx c{0, 1};
+  double slot[ROW_SLOT];
+  const Ctx c{0, 1, slot};
   for (int w = 0; w < T.worlds; w++) {
     start_bars(T, H.io, w, 0, 1);
     backward_world(c, [] {}, T, w);
