@@ -1944,44 +1944,73 @@ HD void rotate_vjp(const double* q, const double* v, bool inverse, const double*
 }
 
 // The reverse of separations (not known): from the separations' adjoints into the
-// bodies' positions, starts and turns, the row's anchors, adjusted and normal.
+// bodies' positions, starts and turns, the row's anchors, adjusted and normal. Everything it
+// reads comes first (the bodies, the row's anchors, the adjoints it adds to), then the sums in
+// the order the reference's terms take, then the stores: no read waits on a store.
 template <int N>
 HD void separations_vjp(const Bodies& B, const BodiesBar& BB, const Work& W, const WorkBar& WB, RowAcc<N, true>& A,
-                        int ga, int gb, int i,
-                        const double (&sepb)[N]) {
+                        int ga, int gb, bool ma, bool mb, int i, const double (&sepb)[N]) {
   const int i3 = 3 * i;
-  const double* n = W.normal() + i3;
-  double* nb = A.normal;
-  const double *pa = B.p + 3 * ga, *pb = B.p + 3 * gb;
-  const double *qa = B.start + 3 * ga, *qb = B.start + 3 * gb;
-  const V3 na = rotate(B.q + 4 * ga, n[0], n[1], n[2], true);
-  const V3 nbv = rotate(B.q + 4 * gb, n[0], n[1], n[2], true);
-  double dnb = 0, nab[3] = {0, 0, 0}, nbb[3] = {0, 0, 0};
+  double n[3], pa[3], pb[3], sa[3], sb[3], qa[4], qb[4];
+  for (int c = 0; c < 3; c++) {
+    n[c] = W.normal()[i3 + c];
+    pa[c] = B.p[3 * ga + c]; pb[c] = B.p[3 * gb + c];
+    sa[c] = B.start[3 * ga + c]; sb[c] = B.start[3 * gb + c];
+  }
+  for (int c = 0; c < 4; c++) { qa[c] = B.q[4 * ga + c]; qb[c] = B.q[4 * gb + c]; }
+  double wla[N][3], wlb[N][3], lab[N][3], lbb[N][3], adj[N];
   for (int k = 0; k < N; k++) {
     const int j = P * i + k, j3 = 3 * j;
+    adj[k] = WB.adjusted()[j];
+    for (int c = 0; c < 3; c++) {
+      wla[k][c] = W.la()[j3 + c]; wlb[k][c] = W.lb()[j3 + c];
+      lab[k][c] = WB.la()[j3 + c]; lbb[k][c] = WB.lb()[j3 + c];
+    }
+  }
+  double pab[3], sab[3], qab[4], pbb[3], sbb[3], qbb[4];
+  for (int c = 0; c < 3; c++) {
+    pab[c] = BB.p[3 * ga + c]; sab[c] = BB.start[3 * ga + c];
+    pbb[c] = BB.p[3 * gb + c]; sbb[c] = BB.start[3 * gb + c];
+  }
+  for (int c = 0; c < 4; c++) { qab[c] = BB.q[4 * ga + c]; qbb[c] = BB.q[4 * gb + c]; }
+  double* nb = A.normal;
+  const V3 na = rotate(qa, n[0], n[1], n[2], true);
+  const V3 nbv = rotate(qb, n[0], n[1], n[2], true);
+  double dnb = 0, nab[3] = {0, 0, 0}, nbb[3] = {0, 0, 0};
+  for (int k = 0; k < N; k++) {
     const double g = sepb[k];
     if (g == 0) continue;
     dnb += g;
-    WB.adjusted()[j] += g;
-    WB.lb()[j3] += g * nbv.x; WB.lb()[j3 + 1] += g * nbv.y; WB.lb()[j3 + 2] += g * nbv.z;
-    WB.la()[j3] -= g * na.x; WB.la()[j3 + 1] -= g * na.y; WB.la()[j3 + 2] -= g * na.z;
-    for (int c = 0; c < 3; c++) { nbb[c] += g * W.lb()[j3 + c]; nab[c] -= g * W.la()[j3 + c]; }
+    adj[k] += g;
+    lbb[k][0] += g * nbv.x; lbb[k][1] += g * nbv.y; lbb[k][2] += g * nbv.z;
+    lab[k][0] -= g * na.x; lab[k][1] -= g * na.y; lab[k][2] -= g * na.z;
+    for (int c = 0; c < 3; c++) { nbb[c] += g * wlb[k][c]; nab[c] -= g * wla[k][c]; }
+  }
+  for (int k = 0; k < N; k++) {
+    if (sepb[k] == 0) continue;
+    const int j = P * i + k, j3 = 3 * j;
+    WB.adjusted()[j] = adj[k];
+    for (int c = 0; c < 3; c++) { WB.la()[j3 + c] = lab[k][c]; WB.lb()[j3 + c] = lbb[k][c]; }
   }
   for (int c = 0; c < 3; c++) {
-    const double d = (pb[c] - qb[c]) - (pa[c] - qa[c]);
+    const double d = (pb[c] - sb[c]) - (pa[c] - sa[c]);
     nb[c] += dnb * d;
   }
-  if (moves(B, gb)) {
-    for (int c = 0; c < 3; c++) { BB.p[3 * gb + c] += dnb * n[c]; BB.start[3 * gb + c] -= dnb * n[c]; }
-    rotate_vjp(B.q + 4 * gb, n, true, nbb, BB.q + 4 * gb, nb);
+  if (mb) {
+    for (int c = 0; c < 3; c++) { pbb[c] += dnb * n[c]; sbb[c] -= dnb * n[c]; }
+    rotate_vjp(qb, n, true, nbb, qbb, nb);
+    for (int c = 0; c < 3; c++) { BB.p[3 * gb + c] = pbb[c]; BB.start[3 * gb + c] = sbb[c]; }
+    for (int c = 0; c < 4; c++) BB.q[4 * gb + c] = qbb[c];
   } else {
-    rotate_vjp(B.q + 4 * gb, n, true, nbb, nullptr, nb);
+    rotate_vjp(qb, n, true, nbb, nullptr, nb);
   }
-  if (moves(B, ga)) {
-    for (int c = 0; c < 3; c++) { BB.p[3 * ga + c] -= dnb * n[c]; BB.start[3 * ga + c] += dnb * n[c]; }
-    rotate_vjp(B.q + 4 * ga, n, true, nab, BB.q + 4 * ga, nb);
+  if (ma) {
+    for (int c = 0; c < 3; c++) { pab[c] -= dnb * n[c]; sab[c] += dnb * n[c]; }
+    rotate_vjp(qa, n, true, nab, qab, nb);
+    for (int c = 0; c < 3; c++) { BB.p[3 * ga + c] = pab[c]; BB.start[3 * ga + c] = sab[c]; }
+    for (int c = 0; c < 4; c++) BB.q[4 * ga + c] = qab[c];
   } else {
-    rotate_vjp(B.q + 4 * ga, n, true, nab, nullptr, nb);
+    rotate_vjp(qa, n, true, nab, nullptr, nb);
   }
 }
 
@@ -1997,8 +2026,10 @@ HD void row_vjp_n(const Bodies& B, const BodiesBar& BB, const Work& W, const Wor
   const bool ma = moves(B, ga), mb = moves(B, gb);
   Pair sb;
   for (int c = 0; c < 3; c++) {
-    sb.v[c] = ma ? BB.v[3 * ga + c] : 0; sb.v[3 + c] = ma ? BB.w[3 * ga + c] : 0;
-    sb.v[6 + c] = mb ? BB.v[3 * gb + c] : 0; sb.v[9 + c] = mb ? BB.w[3 * gb + c] : 0;
+    // read whether or not the body moves (every body has them), then kept only if it does
+    const double va = BB.v[3 * ga + c], wa = BB.w[3 * ga + c], vb = BB.v[3 * gb + c], wb = BB.w[3 * gb + c];
+    sb.v[c] = ma ? va : 0; sb.v[3 + c] = ma ? wa : 0;
+    sb.v[6 + c] = mb ? vb : 0; sb.v[9 + c] = mb ? wb : 0;
   }
   const double ima = B.inv_mass[ga], imb = B.inv_mass[gb];
   const double* Ia = B.inv_i + 6 * ga;
@@ -2016,7 +2047,7 @@ HD void row_vjp_n(const Bodies& B, const BodiesBar& BB, const Work& W, const Wor
     A.load(WB, i);
     double sepb[N];
     solve_vjp<N>(W, A, R, r, i, ima, imb, Ia, Ib, soft, h, KIND == 1, T, sb, imab, imbb, sepb);
-    separations_vjp<N>(B, BB, W, WB, A, ga, gb, i, sepb);
+    separations_vjp<N>(B, BB, W, WB, A, ga, gb, ma, mb, i, sepb);
     A.ima += imab;
     A.imb += imbb;
     A.store(WB, i);
