@@ -2045,8 +2045,12 @@ HD void row_vjp_n(const Bodies& B, const BodiesBar& BB, const Work& W, const Wor
   } else {
     RowAcc<N, true>& A = *reinterpret_cast<RowAcc<N, true>*>(slot);
     A.load(WB, i);
+    // the row's tape read at once (the fields the reverse does not read are never loaded)
+    double tl[ROW_TAPE];
+    for (int f = 0; f < ROW_TAPE; f++) tl[f] = T[f];
+    const TapeRow TL{tl, 1};
     double sepb[N];
-    solve_vjp<N>(W, A, R, r, i, ima, imb, Ia, Ib, soft, h, KIND == 1, T, sb, imab, imbb, sepb);
+    solve_vjp<N>(W, A, R, r, i, ima, imb, Ia, Ib, soft, h, KIND == 1, TL, sb, imab, imbb, sepb);
     separations_vjp<N>(B, BB, W, WB, A, ga, gb, ma, mb, i, sepb);
     A.ima += imab;
     A.imb += imbb;
