@@ -1576,9 +1576,11 @@ HD void solve_vjp(const Work& W, RowAcc<N, true>& A, const Rows& R, int r, int i
   double* Iab = A.Ia;
   double* Ibb = A.Ib;
   const int i3 = 3 * i;
-  const double* n = W.normal() + i3;
-  const double* t1 = W.t1() + i3;
-  const double* t2 = W.t2() + i3;
+  // The row's own columns read at once, before the reverse's first use of any.
+  double n[3], t1[3], t2[3], tmv[3], lever_in[N], nmass_in[N];
+  for (int c = 0; c < 3; c++) { n[c] = W.normal()[i3 + c]; t1[c] = W.t1()[i3 + c]; t2[c] = W.t2()[i3 + c]; tmv[c] = W.tm()[i3 + c]; }
+  for (int k = 0; k < N; k++) { lever_in[k] = W.lever()[P * i + k]; nmass_in[k] = W.nmass()[P * i + k]; }
+  const double sm_in = W.sm()[i], rm1_in = W.rm()[2 * i], rm2_in = W.rm()[2 * i + 1], tol_in = R.tolerance[r];
   double* nb = A.normal;
   double* t1b = A.t1;
   double* t2b = A.t2;
@@ -1605,7 +1607,7 @@ HD void solve_vjp(const Work& W, RowAcc<N, true>& A, const Rows& R, int r, int i
       const double dw[3] = {T[T_DW], T[T_DW + 1], T[T_DW + 2]};
       const double proj = dw[0] * n[0] + dw[1] * n[1] + dw[2] * n[2];
       A.sm -= s1b * proj;
-      const double sm = W.sm()[i];
+      const double sm = sm_in;
       for (int c = 0; c < 3; c++) {
         const double g = -s1b * sm * n[c];
         sb.v[9 + c] += g; sb.v[3 + c] -= g;
@@ -1646,7 +1648,7 @@ HD void solve_vjp(const Work& W, RowAcc<N, true>& A, const Rows& R, int r, int i
       n2b = k * (o2b - u2 * ou);
     }
     r1b += n1b; r2b += n2b;
-    const double rm1 = W.rm()[2 * i], rm2 = W.rm()[2 * i + 1];
+    const double rm1 = rm1_in, rm2 = rm2_in;
     const double u[3] = {T[T_U], T[T_U + 1], T[T_U + 2]};
     const double p1 = u[0] * t1[0] + u[1] * t1[1] + u[2] * t1[2];
     const double p2 = u[0] * t2[0] + u[1] * t2[1] + u[2] * t2[2];
@@ -1684,7 +1686,7 @@ HD void solve_vjp(const Work& W, RowAcc<N, true>& A, const Rows& R, int r, int i
     const double o1b = A.tangent[0] + d1b, o2b = A.tangent[1] + d2b;
     double in1b = -d1b, in2b = -d2b;  // of the tangent impulses before
     double n1b = o1b, n2b = o2b;      // of n before the cone
-    const double tm0 = W.tm()[i3], tm1 = W.tm()[i3 + 1], tm2 = W.tm()[i3 + 2];
+    const double tm0 = tmv[0], tm1 = tmv[1], tm2 = tmv[2];
     double* tmb = A.tm;
     if ((flags >> 1) & 1) {
       // out = limit x / |x|, x = (I + al M)^-1 n, al where |x| = limit.
@@ -1734,7 +1736,7 @@ HD void solve_vjp(const Work& W, RowAcc<N, true>& A, const Rows& R, int r, int i
       A.drift[0] += soft.bias_rate * v1b;
       A.drift[1] += soft.bias_rate * v2b;
     } else if (fbias == 2) {
-      const double tol = R.tolerance[r], max_bias = tol / h;
+      const double tol = tol_in, max_bias = tol / h;
       const double l = dsqrt(T[T_BB]);
       const double u1 = T[T_B1] / l, u2 = T[T_B2] / l, k = max_bias / l;
       const double ou = v1b * u1 + v2b * u2;
@@ -1758,7 +1760,7 @@ HD void solve_vjp(const Work& W, RowAcc<N, true>& A, const Rows& R, int r, int i
   // normal
   double nextb[N], oldb[N];
   for (int k = 0; k < N; k++) {
-    nextb[k] = A.impulse[k] + pressed_b + W.lever()[P * i + k] * twisted_b;
+    nextb[k] = A.impulse[k] + pressed_b + lever_in[k] * twisted_b;
     A.lever[k] += T[T_NEXT + k] * twisted_b;
     oldb[k] = 0;
   }
@@ -1785,7 +1787,7 @@ HD void solve_vjp(const Work& W, RowAcc<N, true>& A, const Rows& R, int r, int i
       if (!(T[T_RAW + k] > 0)) continue;  // clamped at 0 (or exactly 0)
       const double g = nextb[k];
       const int j = P * i + k;
-      const double nm = W.nmass()[j];
+      const double nm = nmass_in[k];
       const bool soft_k = tcase(k) == 1 || tcase(k) == 2;
       const double ms = soft_k ? soft.mass_scale : 1, is = soft_k ? soft.impulse_scale : 0;
       oldb[k] += g * (1 - is);
@@ -1876,21 +1878,22 @@ HD void warm_vjp(const Work& W, RowAcc<N, false>& A, const Rows& R, int r, int i
   double* Iab = A.Ia;
   double* Ibb = A.Ib;
   const int count = R.count[r], i3 = 3 * i;
-  const double* n = W.normal() + i3;
-  const double* t1 = W.t1() + i3;
-  const double* t2 = W.t2() + i3;
+  // the row's columns and tape read at once, before the first use of any
+  double n[3], t1[3], t2[3], imp[N], fj[FJ];
+  for (int c = 0; c < 3; c++) { n[c] = W.normal()[i3 + c]; t1[c] = W.t1()[i3 + c]; t2[c] = W.t2()[i3 + c]; }
+  for (int k = 0; k < N; k++) imp[k] = T[T_IMP + k];
+  for (int c = 0; c < FJ; c++) fj[c] = W.fjac()[FJ * i + c];
+  const double l1 = T[T_O1], l2 = T[T_O2], r1 = T[T_R1], r2 = T[T_R2], sp = T[T_S0];
   double* nb = A.normal;
   double* t1b = A.t1;
   double* t2b = A.t2;
   for (int k = 0; k < N; k++) {
     if (k >= count) break;
     const int j = P * i + k;
-    A.impulse[k] += push_normal_vjp(sb, ima, imb, n, W.jac() + J * j, T[T_IMP + k], nb, imab, imbb, A.jac + J * k);
+    A.impulse[k] += push_normal_vjp(sb, ima, imb, n, W.jac() + J * j, imp[k], nb, imab, imbb, A.jac + J * k);
   }
   {
-    const double* fj = W.fjac() + FJ * i;
     double* fjb = A.fjac;
-    const double l1 = T[T_O1], l2 = T[T_O2];
     double l1b = 0, l2b = 0;
     for (int c = 0; c < 3; c++) {
       const double dv = imb * sb.v[6 + c] - ima * sb.v[c];
@@ -1906,7 +1909,6 @@ HD void warm_vjp(const Work& W, RowAcc<N, false>& A, const Rows& R, int r, int i
     A.tangent[0] += l1b; A.tangent[1] += l2b;
   }
   {
-    const double r1 = T[T_R1], r2 = T[T_R2], sp = T[T_S0];
     const double L[3] = {t1[0] * r1 + t2[0] * r2 + n[0] * sp, t1[1] * r1 + t2[1] * r2 + n[1] * sp,
                          t1[2] * r1 + t2[2] * r2 + n[2] * sp};
     double Lb[3] = {0, 0, 0};
