@@ -528,6 +528,8 @@ struct bm_world {
   // Where the step's narrow passes stand: the next visit to take, and the pass's first pair.
   size_t visit_at = 0;
   int narrow_from = 0;
+  // A large pass decided on the pool whose first batch's results have not been looked at yet.
+  bool first_results = false;
   // The device context: the world's own, or its batch's (shared, and then every device call
   // made while the batch's host threads run holds device_lock).
   bool owns_ctx = true;
@@ -1721,6 +1723,7 @@ void narrow_begin(bm_world* w, int from) {
         if (visits[i].kind != 4) live[at++] = i;
     });
     w->live_count = kept_per[chunks];
+    w->first_results = true;
   } else {
     for (int i = 0; i < n; i++) prepare_visit(w, visits[i], rebuilt[i]);
     if (w->live.size() < (size_t)n) w->live.resize(n);
@@ -1735,7 +1738,58 @@ void narrow_begin(bm_world* w, int from) {
 
 // Takes the pass's results in order; true when it stopped for the device (a pair prepared
 // again left a record in the batch), false when the pass is done.
+// A large pass's first batch back: the meets that found nothing for a pair that keeps no
+// cache row, was not touching (so it writes no end event) and neither of whose bodies can wake
+// while the pass is taken (so it is never prepared again) forget their pair, as finish_visit
+// would, on the pool and in any order: each touches only its own pair's entries, and the impact
+// count is a sum. The serial walk then takes the other visits, in order. On a pile at rest these
+// are most of the walk (a pair whose boxes touch but whose shapes do not).
+void drop_empty_meets(bm_world* w) {
+  const int n = (int)w->live_count, chunk = 2048, chunks = (n + chunk - 1) / chunk;
+  Pairs& pr = w->pairs;
+  const uint8_t* type = w->type.data();
+  const uint8_t* awake = w->awake.data();
+  auto can_wake = [&](int b) { return type[b] == DYNAMIC && awake[b] == 0; };
+  std::vector<int> kept_per((size_t)chunks + 1, 0), impacts_per((size_t)chunks, 0);
+  int32_t* live = w->live.data();
+  Visit* visits = w->visits.data();
+  w->pool->run(chunks, [&](int c) {
+    int kept = 0, impacts = 0;
+    for (int j = c * chunk, e = std::min(n, (c + 1) * chunk); j < e; j++) {
+      Visit& v = visits[live[j]];
+      if (v.kind == 2 && w->batch_count[v.slot] == 0 && pr.cache[v.p] == 0 && pr.touching[v.p] == 0 &&
+          !can_wake(v.a) && !can_wake(v.b)) {
+        if (w->batch_flags[v.slot] & 2) impacts++;
+        pr.points[v.p] = 0;
+        pr.clear_memory(v.p);
+        v.kind = 4;
+        continue;
+      }
+      kept++;
+    }
+    kept_per[c + 1] = kept;
+    impacts_per[c] = impacts;
+  });
+  for (int c = 0; c < chunks; c++) {
+    kept_per[c + 1] += kept_per[c];
+    w->impacts += impacts_per[c];
+  }
+  // The rest keep their order: each chunk's survivors move down to where the chunks before end.
+  std::vector<int32_t> rest((size_t)kept_per[chunks]);
+  w->pool->run(chunks, [&](int c) {
+    int at = kept_per[c];
+    for (int j = c * chunk, e = std::min(n, (c + 1) * chunk); j < e; j++)
+      if (visits[live[j]].kind != 4) rest[at++] = live[j];
+  });
+  std::copy(rest.begin(), rest.end(), live);
+  w->live_count = kept_per[chunks];
+}
+
 bool narrow_resume(bm_world* w, int& rc) {
+  if (w->first_results) {
+    w->first_results = false;
+    if (w->pool && w->visit_at == 0) drop_empty_meets(w);
+  }
   std::vector<Visit>& visits = w->visits;
   for (; w->visit_at < w->live_count; w->visit_at++) {
     const int i = w->live[w->visit_at];
