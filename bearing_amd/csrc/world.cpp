@@ -2705,9 +2705,9 @@ void touched(bm_world* w) {
 void store_memory(bm_world* w, int solved) {
   Contacts& c = w->c;
   Pairs& pr = w->pairs;
-  for (int f = 0; f < c.count; f++) {
+  auto one = [&](int f) {
     const int m = c.row[f];
-    if (m >= solved || c.sub[m] < 0) continue;
+    if (m >= solved || c.sub[m] < 0) return;
     const int sub = c.sub[m], m3 = 3 * m, s3 = 3 * sub;
     Memory& kept = pr.memory_of(c.pair[m]);
     kept.ensure(sub + 1);
@@ -2734,7 +2734,24 @@ void store_memory(bm_world* w, int solved) {
       kept.pid[P * sub + k] = c.pid[P * m + k];
       kept.normal_impulse[P * sub + k] = c.normal_impulse[P * m + k];
     }
+  };
+  const int n = c.count;
+  if (!w->pool || n < PARALLEL_VISITS) {
+    for (int f = 0; f < n; f++) one(f);
+    return;
   }
+  // A pair's rows are neighbours in narrow order (touched() reads them so too), and only they
+  // write the pair's Memory: chunks cut where one pair's rows end and the next's begin write
+  // disjoint Memories, each pair's rows still in narrow order.
+  const int chunk = 4096, chunks = (n + chunk - 1) / chunk;
+  w->pool->run(chunks, [&](int k) {
+    int f = k * chunk, e = std::min(n, f + chunk);
+    if (k > 0)
+      for (const int before = c.pair[c.row[f - 1]]; f < n && c.pair[c.row[f]] == before;) f++;
+    if (e < n)
+      for (const int last = c.pair[c.row[e - 1]]; e < n && c.pair[c.row[e]] == last;) e++;
+    for (; f < e; f++) one(f);
+  });
 }
 
 // The device phase's columns for this world alone (src/world.js:956-1031 reads and writes
