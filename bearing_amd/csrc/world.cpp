@@ -530,6 +530,11 @@ struct bm_world {
   int narrow_from = 0;
   // A large pass decided on the pool whose first batch's results have not been looked at yet.
   bool first_results = false;
+  // Within a pass, each run of the device batches appends its results: the meet results and
+  // level pairs before these slots are earlier runs', still read by the visits not yet taken
+  // (a visit prepared again after a wake gets a new slot past them). The records waiting for
+  // the next run are those slots on.
+  size_t meet_base = 0, level_base = 0;
   // The device context: the world's own, or its batch's (shared, and then every device call
   // made while the batch's host threads run holds device_lock).
   bool owns_ctx = true;
@@ -1354,10 +1359,12 @@ void load(const bm_world* w, int s, double* r) {
 int run_level_batch(bm_world* w) {
   const int n = (int)w->level_field.size();
   if (n == 0) return BM_OK;
+  // This run's pairs and rows go after the pass's earlier runs'.
+  const size_t k0 = w->level_base, r0 = k0 > 0 ? (size_t)w->level_start[k0] : 0;
   for (;;) {
-    const size_t slots = (size_t)n * w->level_cap;
-    w->level_manifolds.resize(n);
-    w->level_start.resize(n + 1);
+    const size_t slots = r0 + (size_t)n * w->level_cap;
+    w->level_manifolds.resize(k0 + n);
+    w->level_start.resize(k0 + n + 1);
     w->level_count.resize(slots);
     w->level_normal.resize(3 * slots);
     w->level_points.resize(12 * slots);
@@ -1365,15 +1372,18 @@ int run_level_batch(bm_world* w) {
     w->level_ids.resize(4 * slots);
     const int rc = bm_level_packed(w->ctx, n, w->level_field.data(), w->level_pos.data(), w->level_axes.data(),
                                    w->level_shape.data(), w->level_box.data(), w->level_margin.data(),
-                                   w->level_tol.data(), w->level_cap, w->level_manifolds.data(), w->level_start.data(),
-                                   w->level_count.data(), w->level_normal.data(), w->level_points.data(),
-                                   w->level_seps.data(), w->level_ids.data());
+                                   w->level_tol.data(), w->level_cap, w->level_manifolds.data() + k0,
+                                   w->level_start.data() + k0, w->level_count.data() + r0,
+                                   w->level_normal.data() + 3 * r0, w->level_points.data() + 12 * r0,
+                                   w->level_seps.data() + 4 * r0, w->level_ids.data() + 4 * r0);
     if (rc != BM_OK) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx));
     int most = 0;
-    for (int k = 0; k < n; k++) most = std::max(most, (int)w->level_manifolds[k]);
+    for (int k = 0; k < n; k++) most = std::max(most, (int)w->level_manifolds[k0 + k]);
     if (most <= w->level_cap) break;
     w->level_cap = most;
   }
+  for (int k = 0; k <= n; k++) w->level_start[k0 + k] += (int32_t)r0;
+  w->level_base = k0 + n;
   for (auto* v : {&w->level_pos, &w->level_axes, &w->level_shape, &w->level_box, &w->level_margin, &w->level_tol})
     v->clear();
   w->level_field.clear();
@@ -1427,20 +1437,23 @@ int run_batch(bm_world* w) {
   const int nb = meet_rows(w, n);
   w->meet_table.resize(32 * (size_t)nb);
   meet_table_fill(w, w->meet_table.data());
-  w->batch_count.resize(n);
-  w->batch_flags.resize(n);
-  w->batch_normal.resize(3 * n);
-  w->batch_points.resize(12 * n);
-  w->batch_seps.resize(4 * n);
-  w->batch_ids.resize(4 * n);
-  w->batch_shift.resize(12 * n);
-  w->batch_loss.resize(n);
+  const size_t k = w->meet_base, e = k + n;  // this run's results go after the pass's earlier runs'
+  w->batch_count.resize(e);
+  w->batch_flags.resize(e);
+  w->batch_normal.resize(3 * e);
+  w->batch_points.resize(12 * e);
+  w->batch_seps.resize(4 * e);
+  w->batch_ids.resize(4 * e);
+  w->batch_shift.resize(12 * e);
+  w->batch_loss.resize(e);
   const int rc = bm_meet_indexed(w->ctx, nb, w->meet_table.data(), n, w->meet_ta.data(), w->meet_tb.data(),
-                                 w->batch_pair.data(), w->batch_count.data(), w->batch_normal.data(),
-                                 w->batch_points.data(), w->batch_seps.data(), w->batch_ids.data(),
-                                 w->batch_shift.data(), w->batch_flags.data(), w->batch_loss.data());
+                                 w->batch_pair.data(), w->batch_count.data() + k, w->batch_normal.data() + 3 * k,
+                                 w->batch_points.data() + 12 * k, w->batch_seps.data() + 4 * k,
+                                 w->batch_ids.data() + 4 * k, w->batch_shift.data() + 12 * k,
+                                 w->batch_flags.data() + k, w->batch_loss.data() + k);
   w->batch_ia.clear(); w->batch_ib.clear(); w->batch_pair.clear();
   if (rc != BM_OK) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx));
+  w->meet_base = e;
   return BM_OK;
 }
 
@@ -1467,7 +1480,7 @@ void fill_level(bm_world* w, const Visit& v) {
   const double ex = std::fabs(A[g9]) * hx + std::fabs(A[g9 + 1]) * hy + std::fabs(A[g9 + 2]) * hz + reach[g];
   const double ey = std::fabs(A[g9 + 3]) * hx + std::fabs(A[g9 + 4]) * hy + std::fabs(A[g9 + 5]) * hz + reach[g];
   const double ez = std::fabs(A[g9 + 6]) * hx + std::fabs(A[g9 + 7]) * hy + std::fabs(A[g9 + 8]) * hz + reach[g];
-  const size_t k = v.slot;
+  const size_t k = v.slot - w->level_base;
   w->level_field[k] = w->field_base + w->field[g];
   for (int i = 0; i < 3; i++) w->level_pos[3 * k + i] = pos[g3 + i];
   for (int i = 0; i < 9; i++) w->level_axes[9 * k + i] = A[g9 + i];
@@ -1536,7 +1549,7 @@ void decide_visit(const bm_world* w, Visit& v, Manifold& m) {
 // axes, velocities) changes while a pass is prepared, a wake changing only a body's motion,
 // which is the pair's.
 void fill_meet(bm_world* w, const Visit& v) {
-  const size_t k = v.slot;
+  const size_t k = v.slot - w->meet_base;
   w->batch_ia[k] = v.a;
   w->batch_ib[k] = v.b;
   const double margin = w->motion[v.a] + w->motion[v.b] + v.tolerance;
@@ -1558,11 +1571,11 @@ void grow_batches(bm_world* w, size_t meets, size_t levels) {
 void prepare_visit(bm_world* w, Visit& v, Manifold& m) {
   decide_visit(w, v, m);
   if (v.kind == 2) {
-    v.slot = (int)(w->batch_pair.size() / 6);
+    v.slot = (int)(w->meet_base + w->batch_pair.size() / 6);
     grow_batches(w, 1, 0);
     fill_meet(w, v);
   } else if (v.kind == 3) {
-    v.slot = (int)w->level_field.size();
+    v.slot = (int)(w->level_base + w->level_field.size());
     grow_batches(w, 0, 1);
     fill_level(w, v);
   }
@@ -1617,6 +1630,9 @@ int finish_visit(bm_world* w, Visit& v, Manifold& m) {
 // behind, where a pair must be looked at again; the caller runs the batch between the two.
 void narrow_begin(bm_world* w, int from) {
   Pairs& pr = w->pairs;
+  // The last pass's runs are all taken: this one's results start afresh.
+  w->meet_base = 0;
+  w->level_base = 0;
   std::vector<Visit>& visits = w->visits;
   auto visit_of = [&](int p) {
     const int a = pr.a[p], b = pr.b[p];
@@ -1682,7 +1698,8 @@ void narrow_begin(bm_world* w, int from) {
     w->pool->run(chunks, [&](int c) {
       for (int i = c * chunk, e = std::min(n, (c + 1) * chunk); i < e; i++) decide_visit(w, visits[i], rebuilt[i]);
     });
-    size_t meets = w->batch_pair.size() / 6, levels = w->level_field.size(), m0 = meets, l0 = levels;
+    size_t meets = w->meet_base + w->batch_pair.size() / 6, levels = w->level_base + w->level_field.size();
+    const size_t m0 = meets, l0 = levels;
     for (int i = 0; i < n; i++) {
       Visit& v = visits[i];
       if (v.kind == 2) v.slot = (int)meets++;
@@ -3144,14 +3161,21 @@ int run_merged(bm_batch* B) {
       const int k0 = at[i], k = at[i + 1] - at[i];
       w->batch_ia.clear(); w->batch_ib.clear(); w->batch_pair.clear();
       if (k == 0) return;
-      w->batch_count.assign(B->count.begin() + k0, B->count.begin() + k0 + k);
-      w->batch_flags.assign(B->flags.begin() + k0, B->flags.begin() + k0 + k);
-      w->batch_normal.assign(B->normal.begin() + 3 * k0, B->normal.begin() + 3 * (k0 + k));
-      w->batch_points.assign(B->points.begin() + 12 * k0, B->points.begin() + 12 * (k0 + k));
-      w->batch_seps.assign(B->seps.begin() + 4 * k0, B->seps.begin() + 4 * (k0 + k));
-      w->batch_ids.assign(B->ids.begin() + 4 * k0, B->ids.begin() + 4 * (k0 + k));
-      w->batch_shift.assign(B->shift.begin() + 12 * k0, B->shift.begin() + 12 * (k0 + k));
-      w->batch_loss.assign(B->loss.begin() + k0, B->loss.begin() + k0 + k);
+      // after the results of the world's pass's earlier runs
+      const size_t b = w->meet_base, e = b + k;
+      auto put = [&](auto& to, const auto& from, size_t width) {
+        to.resize(width * e);
+        std::copy(from.begin() + width * k0, from.begin() + width * (k0 + k), to.begin() + width * b);
+      };
+      put(w->batch_count, B->count, 1);
+      put(w->batch_flags, B->flags, 1);
+      put(w->batch_normal, B->normal, 3);
+      put(w->batch_points, B->points, 12);
+      put(w->batch_seps, B->seps, 4);
+      put(w->batch_ids, B->ids, 4);
+      put(w->batch_shift, B->shift, 12);
+      put(w->batch_loss, B->loss, 1);
+      w->meet_base = e;
     });
   }
   std::vector<int> lat(nw + 1, 0);
@@ -3195,16 +3219,24 @@ int run_merged(bm_batch* B) {
       w->level_field.clear();
       if (k == 0) continue;
       w->level_cap = (int)cap;
-      w->level_manifolds.assign(B->level_manifolds.begin() + k0, B->level_manifolds.begin() + k0 + k);
-      // The world's pairs' packed rows, their starts counted from the world's first.
-      const size_t r0 = B->level_start[k0], r1 = B->level_start[k0 + k];
-      w->level_start.resize(k + 1);
-      for (size_t j = 0; j <= k; j++) w->level_start[j] = B->level_start[k0 + j] - (int32_t)r0;
-      w->level_count.assign(B->level_count.begin() + r0, B->level_count.begin() + r1);
-      w->level_normal.assign(B->level_normal.begin() + 3 * r0, B->level_normal.begin() + 3 * r1);
-      w->level_points.assign(B->level_points.begin() + 12 * r0, B->level_points.begin() + 12 * r1);
-      w->level_seps.assign(B->level_seps.begin() + 4 * r0, B->level_seps.begin() + 4 * r1);
-      w->level_ids.assign(B->level_ids.begin() + 4 * r0, B->level_ids.begin() + 4 * r1);
+      // The world's pairs and their packed rows, after its pass's earlier runs', the starts
+      // counted from the world's first row.
+      const size_t b = w->level_base, wr = b > 0 ? (size_t)w->level_start[b] : 0;
+      const size_t r0 = B->level_start[k0], r1 = B->level_start[k0 + k], rows = r1 - r0;
+      w->level_manifolds.resize(b + k);
+      std::copy(B->level_manifolds.begin() + k0, B->level_manifolds.begin() + k0 + k, w->level_manifolds.begin() + b);
+      w->level_start.resize(b + k + 1);
+      for (size_t j = 0; j <= k; j++) w->level_start[b + j] = B->level_start[k0 + j] - (int32_t)r0 + (int32_t)wr;
+      auto put = [&](auto& to, const auto& from, size_t width) {
+        to.resize(width * (wr + rows));
+        std::copy(from.begin() + width * r0, from.begin() + width * r1, to.begin() + width * wr);
+      };
+      put(w->level_count, B->level_count, 1);
+      put(w->level_normal, B->level_normal, 3);
+      put(w->level_points, B->level_points, 12);
+      put(w->level_seps, B->level_seps, 4);
+      put(w->level_ids, B->level_ids, 4);
+      w->level_base = b + k;
     }
   }
   return BM_OK;
