@@ -535,6 +535,10 @@ struct bm_world {
   // (a visit prepared again after a wake gets a new slot past them). The records waiting for
   // the next run are those slots on.
   size_t meet_base = 0, level_base = 0;
+  // For a large pass's first run: each level row's kept entry (what finish_level would match it
+  // with), worked out on the pool; level_matched says these hold for visits not prepared again.
+  std::vector<int32_t> level_best;
+  bool level_matched = false;
   // The device context: the world's own, or its batch's (shared, and then every device call
   // made while the batch's host threads run holds device_lock).
   bool owns_ctx = true;
@@ -1496,21 +1500,20 @@ void fill_level(bm_world* w, const Visit& v) {
 // The rest of src/contact.js:857-881 (level): wake, then each direction the device found as a
 // manifold of its own (sub 0, 1, ...; normals from a to b), each started from the remembered
 // manifold of the pair nearest its direction; a pair that touched nothing forgets.
-int finish_level(bm_world* w, Visit& v) {
-  const int a = v.a, b = v.b, p = v.p, k = v.slot;
-  const bool turned = w->shape[a] >= MESH;
+// A level pair's rows matched to what the pair kept, in order (src/contact.js:857-880): each
+// row takes the untaken entry of no key whose normal is nearest its own (the largest positive
+// dot product), or none. Reads and writes only the pair's own Memory.
+void match_level(bm_world* w, const Visit& v, int32_t* best_of) {
+  const int a = v.a, p = v.p, k = v.slot;
+  const double sign = w->shape[a] >= MESH ? -1 : 1;
   const int n = w->level_manifolds[k];
   Memory& kept = w->pairs.memory_of(p);
   for (int j = 0; j < kept.count; j++) kept.taken[j] = 0;
-  if (n > 0 && w->awake[a] != w->awake[b]) wake(w, w->awake[a] ? b : a);
-  const double sign = turned ? -1 : 1;
-  int sub = 0;
   for (int q = 0; q < n; q++) {
     const size_t o = (size_t)w->level_start[k] + q;
     // The row's normal as emit_fill writes it, for the match against what the pair kept.
     const double nx = sign * w->level_normal[3 * o], ny = sign * w->level_normal[3 * o + 1];
     const double nz = sign * w->level_normal[3 * o + 2];
-    const int m = emit_slot(w);
     int best = -1;
     double most = 0;
     for (int j = 0; j < kept.count; j++) {
@@ -1519,7 +1522,34 @@ int finish_level(bm_world* w, Visit& v) {
       if (d > most) { most = d; best = j; }
     }
     if (best >= 0) kept.taken[best] = 1;
-    w->fills.push_back(RowFill{nullptr, m, p, a, b, sub++, (int)o, best, -1, v.tolerance, sign});
+    best_of[q] = best;
+  }
+}
+
+int finish_level(bm_world* w, Visit& v) {
+  const int a = v.a, b = v.b, p = v.p, k = v.slot;
+  const bool turned = w->shape[a] >= MESH;
+  const int n = w->level_manifolds[k];
+  if (n > 0 && w->awake[a] != w->awake[b]) wake(w, w->awake[a] ? b : a);
+  const double sign = turned ? -1 : 1;
+  // Matched on the pool when the pass's first run came back, unless prepared again since.
+  const size_t o0 = (size_t)w->level_start[k];
+  int32_t own[64];
+  const int32_t* best = nullptr;
+  if (w->level_matched && !v.again) {
+    best = &w->level_best[o0];
+  } else if (n <= 64) {
+    match_level(w, v, own);
+    best = own;
+  } else {
+    w->level_best.resize(std::max(w->level_best.size(), o0 + n));
+    match_level(w, v, &w->level_best[o0]);
+    best = &w->level_best[o0];
+  }
+  int sub = 0;
+  for (int q = 0; q < n; q++) {
+    const int m = emit_slot(w);
+    w->fills.push_back(RowFill{nullptr, m, p, a, b, sub++, (int)(o0 + q), best[q], -1, v.tolerance, sign});
   }
   if (sub == 0) w->pairs.forget(p);
   return BM_OK;
@@ -1633,6 +1663,7 @@ void narrow_begin(bm_world* w, int from) {
   // The last pass's runs are all taken: this one's results start afresh.
   w->meet_base = 0;
   w->level_base = 0;
+  w->level_matched = false;
   std::vector<Visit>& visits = w->visits;
   auto visit_of = [&](int p) {
     const int a = pr.a[p], b = pr.b[p];
@@ -1802,10 +1833,31 @@ void drop_empty_meets(bm_world* w) {
   w->live_count = kept_per[chunks];
 }
 
+// A large pass's first run back: every level pair's rows matched to what it kept, on the pool
+// (each pair reads and writes only its own Memory, and no visit before it in the walk changes
+// that), so the serial walk only takes their rows' places.
+void match_levels(bm_world* w) {
+  const size_t rows = w->level_base > 0 ? (size_t)w->level_start[w->level_base] : 0;
+  if (w->level_best.size() < rows) w->level_best.resize(rows);
+  const int n = (int)w->live_count, chunk = 2048, chunks = (n + chunk - 1) / chunk;
+  const int32_t* live = w->live.data();
+  Visit* visits = w->visits.data();
+  w->pool->run(chunks, [&](int c) {
+    for (int j = c * chunk, e = std::min(n, (c + 1) * chunk); j < e; j++) {
+      const Visit& v = visits[live[j]];
+      if (v.kind == 3) match_level(w, v, &w->level_best[(size_t)w->level_start[v.slot]]);
+    }
+  });
+  w->level_matched = true;
+}
+
 bool narrow_resume(bm_world* w, int& rc) {
   if (w->first_results) {
     w->first_results = false;
-    if (w->pool && w->visit_at == 0) drop_empty_meets(w);
+    if (w->pool && w->visit_at == 0) {
+      match_levels(w);
+      drop_empty_meets(w);
+    }
   }
   std::vector<Visit>& visits = w->visits;
   for (; w->visit_at < w->live_count; w->visit_at++) {
