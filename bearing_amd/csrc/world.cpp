@@ -1683,14 +1683,25 @@ void narrow_begin(bm_world* w, int from) {
         for (int p = from + c * chunk, e = std::min(pr.active, from + (c + 1) * chunk); p < e; p++)
           apart[p] = !overlap(w, pr.a[p], pr.b[p]);
       });
-    for (int p = from; p < pr.active;) {
-      if (apart[p]) {
-        apart[p] = apart[pr.active - 1];
-        pr.remove(p);
-        continue;
-      }
-      p++;
-    }
+    // Only the slots whose pair is apart, found on the pool, in rising order: a slot that is not
+    // apart stays so (a removal moves the last active pair, and its flag, into the removed
+    // slot only), so the serial loop acts at these slots alone.
+    std::vector<std::vector<int32_t>> found_per((size_t)chunks);
+    auto find = [&](int c) {
+      std::vector<int32_t>& f = found_per[c];
+      f.clear();
+      for (int p = from + c * chunk, e = std::min(pr.active, from + (c + 1) * chunk); p < e; p++)
+        if (apart[p]) f.push_back(p);
+    };
+    if (w->pool) w->pool->run(chunks, find);
+    else
+      for (int c = 0; c < chunks; c++) find(c);
+    for (const std::vector<int32_t>& f : found_per)
+      for (const int p : f)
+        while (p < pr.active && apart[p]) {
+          apart[p] = apart[pr.active - 1];
+          pr.remove(p);
+        }
     const int kept = pr.active - from, kept_chunks = (kept + chunk - 1) / chunk;
     if ((int)visits.size() < kept) visits.resize(kept);  // every visit is written below
     auto visits_of = [&](int c) {
@@ -1726,17 +1737,27 @@ void narrow_begin(bm_world* w, int from) {
   }
   if (w->pool && n >= PARALLEL_VISITS) {
     const int chunk = 1024, chunks = (n + chunk - 1) / chunk;
+    // Each chunk's meets and level pairs counted as it decides, so their slots (in visit
+    // order) are known per chunk and taken on the pool below.
+    std::vector<size_t> meets_at((size_t)chunks + 1, 0), levels_at((size_t)chunks + 1, 0);
     w->pool->run(chunks, [&](int c) {
-      for (int i = c * chunk, e = std::min(n, (c + 1) * chunk); i < e; i++) decide_visit(w, visits[i], rebuilt[i]);
+      size_t nm = 0, nl = 0;
+      for (int i = c * chunk, e = std::min(n, (c + 1) * chunk); i < e; i++) {
+        decide_visit(w, visits[i], rebuilt[i]);
+        nm += visits[i].kind == 2;
+        nl += visits[i].kind == 3;
+      }
+      meets_at[c + 1] = nm;
+      levels_at[c + 1] = nl;
     });
-    size_t meets = w->meet_base + w->batch_pair.size() / 6, levels = w->level_base + w->level_field.size();
-    const size_t m0 = meets, l0 = levels;
-    for (int i = 0; i < n; i++) {
-      Visit& v = visits[i];
-      if (v.kind == 2) v.slot = (int)meets++;
-      else if (v.kind == 3) v.slot = (int)levels++;
+    meets_at[0] = w->meet_base + w->batch_pair.size() / 6;
+    levels_at[0] = w->level_base + w->level_field.size();
+    const size_t m0 = meets_at[0], l0 = levels_at[0];
+    for (int c = 0; c < chunks; c++) {
+      meets_at[c + 1] += meets_at[c];
+      levels_at[c + 1] += levels_at[c];
     }
-    grow_batches(w, meets - m0, levels - l0);
+    grow_batches(w, meets_at[chunks] - m0, levels_at[chunks] - l0);
     // A pair whose boxes are apart forgets what it kept; when it keeps no cache row, was not
     // touching (so it writes no end event) and neither body can wake while the pass is taken
     // (so the pair is never looked at again), that touches only the pair's own entries, in any
@@ -1747,11 +1768,16 @@ void narrow_begin(bm_world* w, int from) {
     auto can_wake = [&](int b) { return type[b] == DYNAMIC && awake[b] == 0; };
     w->pool->run(chunks, [&](int c) {
       int kept_here = 0;
+      size_t meet = meets_at[c], level = levels_at[c];
       for (int i = c * chunk, e = std::min(n, (c + 1) * chunk); i < e; i++) {
         Visit& v = visits[i];
-        if (v.kind == 2) fill_meet(w, v);
-        else if (v.kind == 3) fill_level(w, v);
-        else if (v.kind == 0 && pr.cache[v.p] == 0 && pr.touching[v.p] == 0 && !can_wake(v.a) && !can_wake(v.b)) {
+        if (v.kind == 2) {
+          v.slot = (int)meet++;
+          fill_meet(w, v);
+        } else if (v.kind == 3) {
+          v.slot = (int)level++;
+          fill_level(w, v);
+        } else if (v.kind == 0 && pr.cache[v.p] == 0 && pr.touching[v.p] == 0 && !can_wake(v.a) && !can_wake(v.b)) {
           pr.points[v.p] = 0;
           pr.clear_memory(v.p);
           pr.part(v.p);
