@@ -1786,7 +1786,6 @@ HD void solve_vjp(const Work& W, RowAcc<N, true>& A, const Rows& R, int r, int i
     for (int k = 0; k < N; k++) {
       if (!(T[T_RAW + k] > 0)) continue;  // clamped at 0 (or exactly 0)
       const double g = nextb[k];
-      const int j = P * i + k;
       const double nm = nmass_in[k];
       const bool soft_k = tcase(k) == 1 || tcase(k) == 2;
       const double ms = soft_k ? soft.mass_scale : 1, is = soft_k ? soft.impulse_scale : 0;

@@ -1215,37 +1215,38 @@ int bm_meet_host(bm_context* ctx, int32_t n, const double* a, const double* b, c
   return BM_OK;
 }
 
-int bm_meet_indexed(bm_context* ctx, int32_t bodies, const double* table, int32_t n, const int32_t* ia,
-                    const int32_t* ib, const double* pair, int32_t* count, double* normal, double* points,
-                    double* separations, uint32_t* ids, double* shift, int32_t* flags, double* loss) {
-  if (!ctx) return BM_ERR_ARGUMENT;
-  ctx->error.clear();
-  if (n < 0 || bodies < 0) return fail(ctx, BM_ERR_ARGUMENT, "bm_meet_indexed: n and bodies must be 0 or more");
-  if (n == 0) return BM_OK;
-  if (!table || !ia || !ib || !pair || !count || !normal || !points || !separations || !ids || !shift || !flags)
-    return fail(ctx, BM_ERR_ARGUMENT, "bm_meet_indexed: null argument");
+}  // extern "C"
+
+namespace {
+
+// The common start of bm_meet_indexed and bm_meet_packed: the arguments checked, pairs_out
+// grown to out_bytes (the pass's outputs first, 280 bytes a pair), the records and the table
+// sent, the pass launched (timed by ctx's events); the staging for the results starts at
+// *in_bytes_out in ctx->pstage.
+int meet_start(bm_context* ctx, const char* who, int32_t bodies, const double* table, int32_t n, const int32_t* ia,
+               const int32_t* ib, const double* pair, size_t out_bytes, bm::MeetDev& d, size_t* in_bytes_out) {
+  const std::string w(who);
+  if (!table || !ia || !ib || !pair) return fail(ctx, BM_ERR_ARGUMENT, w + ": null argument");
   for (int32_t r = 0; r < bodies; r++) {
     const double kind = table[32 * (size_t)r];
     if (!(kind == 0 || kind == 1 || kind == 2 || kind == 3))
-      return fail(ctx, BM_ERR_ARGUMENT, "bm_meet_indexed: a shape's kind must be 0, 1, 2 or 3");
+      return fail(ctx, BM_ERR_ARGUMENT, w + ": a shape's kind must be 0, 1, 2 or 3");
     if (!hull_ok(ctx, table + 32 * (size_t)r))
-      return fail(ctx, BM_ERR_ARGUMENT, "bm_meet_indexed: a hull's number is not in the hull table (bm_set_hulls)");
+      return fail(ctx, BM_ERR_ARGUMENT, w + ": a hull's number is not in the hull table (bm_set_hulls)");
   }
   for (int32_t k = 0; k < n; k++) {
     if (ia[k] < 0 || ia[k] >= bodies || ib[k] < 0 || ib[k] >= bodies)
-      return fail(ctx, BM_ERR_ARGUMENT, "bm_meet_indexed: a pair names a body the table lacks");
-    if (!(pair[6 * (size_t)k + 2] > 0)) return fail(ctx, BM_ERR_ARGUMENT, "bm_meet_indexed: dt must be positive");
+      return fail(ctx, BM_ERR_ARGUMENT, w + ": a pair names a body the table lacks");
+    if (!(pair[6 * (size_t)k + 2] > 0)) return fail(ctx, BM_ERR_ARGUMENT, w + ": dt must be positive");
   }
-  HIP_OR_FAIL(ctx, hipSetDevice(ctx->device), "bm_meet_indexed: hipSetDevice");
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->device), (w + ": hipSetDevice").c_str());
   const size_t N = (size_t)n, T = 32 * sizeof(double) * (size_t)bodies;
   const size_t pairs = 6 * sizeof(double) * N, idx = (4 * N + 255) & ~(size_t)255;
   const size_t in_bytes = T + pairs + 2 * idx;
-  const size_t out_bytes = 280 * N + 64;
-  if (int rc = grow(ctx, ctx->pairs_in, in_bytes, "bm_meet_indexed")) return rc;
-  if (int rc = grow(ctx, ctx->pairs_out, out_bytes, "bm_meet_indexed")) return rc;
+  if (int rc = grow(ctx, ctx->pairs_in, in_bytes, who)) return rc;
+  if (int rc = grow(ctx, ctx->pairs_out, out_bytes, who)) return rc;
   char* in = static_cast<char*>(ctx->pairs_in.p);
   char* out = static_cast<char*>(ctx->pairs_out.p);
-  bm::MeetDev d{};
   d.n = n;
   d.a = reinterpret_cast<const double*>(in);
   d.b = d.a;
@@ -1261,17 +1262,41 @@ int bm_meet_indexed(bm_context* ctx, int32_t bodies, const double* table, int32_
   d.ids = reinterpret_cast<uint32_t*>(out + 256 * N);
   d.loss = reinterpret_cast<double*>(out + 272 * N);
   d.hulls = ctx->hulls;
-  if (int rc = grow_pinned(ctx, &ctx->pstage, &ctx->pstage_cap, in_bytes + out_bytes, "bm_meet_indexed")) return rc;
+  if (int rc = grow_pinned(ctx, &ctx->pstage, &ctx->pstage_cap, in_bytes + 400 * N + 4096, who)) return rc;
   char* st = static_cast<char*>(ctx->pstage);
   run_copies({{st, table, T}, {st + T, pair, pairs}, {st + T + pairs, ia, 4 * N}, {st + T + pairs + idx, ib, 4 * N}});
-  HIP_OR_FAIL(ctx, hipMemcpyAsync(in, st, in_bytes, hipMemcpyHostToDevice, ctx->stream), "bm_meet_indexed: copy");
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(in, st, in_bytes, hipMemcpyHostToDevice, ctx->stream), (w + ": copy").c_str());
   (void)hipGetLastError();
-  HIP_OR_FAIL(ctx, hipEventRecord(ctx->begin, ctx->stream), "bm_meet_indexed: hipEventRecord");
+  HIP_OR_FAIL(ctx, hipEventRecord(ctx->begin, ctx->stream), (w + ": hipEventRecord").c_str());
   bm::launch_meet(d, ctx->stream);
-  HIP_OR_FAIL(ctx, hipGetLastError(), "bm_meet_indexed: launch");
-  HIP_OR_FAIL(ctx, hipEventRecord(ctx->end, ctx->stream), "bm_meet_indexed: hipEventRecord");
+  HIP_OR_FAIL(ctx, hipGetLastError(), (w + ": launch").c_str());
+  HIP_OR_FAIL(ctx, hipEventRecord(ctx->end, ctx->stream), (w + ": hipEventRecord").c_str());
+  *in_bytes_out = in_bytes;
+  return BM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bm_meet_indexed(bm_context* ctx, int32_t bodies, const double* table, int32_t n, const int32_t* ia,
+                    const int32_t* ib, const double* pair, int32_t* count, double* normal, double* points,
+                    double* separations, uint32_t* ids, double* shift, int32_t* flags, double* loss) {
+  if (!ctx) return BM_ERR_ARGUMENT;
+  ctx->error.clear();
+  if (n < 0 || bodies < 0) return fail(ctx, BM_ERR_ARGUMENT, "bm_meet_indexed: n and bodies must be 0 or more");
+  if (n == 0) return BM_OK;
+  if (!count || !normal || !points || !separations || !ids || !shift || !flags)
+    return fail(ctx, BM_ERR_ARGUMENT, "bm_meet_indexed: null argument");
+  bm::MeetDev d{};
+  size_t in_bytes = 0;
+  const size_t N = (size_t)n;
+  if (int rc = meet_start(ctx, "bm_meet_indexed", bodies, table, n, ia, ib, pair, 280 * N + 64, d, &in_bytes))
+    return rc;
+  char* st = static_cast<char*>(ctx->pstage);
   char* back = st + in_bytes;
-  HIP_OR_FAIL(ctx, hipMemcpyAsync(back, out, 280 * N, hipMemcpyDeviceToHost, ctx->stream), "bm_meet_indexed: copy");
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(back, ctx->pairs_out.p, 280 * N, hipMemcpyDeviceToHost, ctx->stream),
+              "bm_meet_indexed: copy");
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_meet_indexed: kernel");
   std::vector<HostCopy> outs = {{normal, back, 24 * N}, {points, back + 24 * N, 96 * N},
                                 {separations, back + 120 * N, 32 * N}, {shift, back + 152 * N, 96 * N},
@@ -1280,6 +1305,72 @@ int bm_meet_indexed(bm_context* ctx, int32_t bodies, const double* table, int32_
   if (loss) outs.push_back({loss, back + 272 * N, 8 * N});
   run_copies(outs);
   HIP_OR_FAIL(ctx, hipEventElapsedTime(&ctx->last_ms, ctx->begin, ctx->end), "bm_meet_indexed: hipEventElapsedTime");
+  return BM_OK;
+}
+
+int bm_meet_packed(bm_context* ctx, int32_t bodies, const double* table, int32_t n, const int32_t* ia,
+                   const int32_t* ib, const double* pair, int32_t* count, int32_t* flags, int32_t* row,
+                   double* normal, double* points, double* separations, uint32_t* ids, double* shift, double* loss) {
+  if (!ctx) return BM_ERR_ARGUMENT;
+  ctx->error.clear();
+  if (n < 0 || bodies < 0) return fail(ctx, BM_ERR_ARGUMENT, "bm_meet_packed: n and bodies must be 0 or more");
+  if (n == 0) return BM_OK;
+  if (!count || !flags || !row || !normal || !points || !separations || !ids || !shift || !loss)
+    return fail(ctx, BM_ERR_ARGUMENT, "bm_meet_packed: null argument");
+  bm::MeetDev d{};
+  size_t in_bytes = 0;
+  const size_t N = (size_t)n;
+  // The packed columns after the pass's own outputs, in the same scratch: the rows (N + 1),
+  // then room for N found rows of each column.
+  const size_t pk_row = 280 * N + 64, pk_normal = pk_row + ((4 * (N + 1) + 255) & ~(size_t)255);
+  const size_t pk_points = pk_normal + 24 * N, pk_seps = pk_points + 96 * N, pk_shift = pk_seps + 32 * N;
+  const size_t pk_loss = pk_shift + 96 * N, pk_ids = pk_loss + 8 * N, pk_end = pk_ids + 16 * N;
+  if (int rc = meet_start(ctx, "bm_meet_packed", bodies, table, n, ia, ib, pair, pk_end, d, &in_bytes)) return rc;
+  char* out = static_cast<char*>(ctx->pairs_out.p);
+  bm::MeetPacked p{};
+  p.row = reinterpret_cast<int*>(out + pk_row);
+  p.found = p.row + N;
+  p.normal = reinterpret_cast<double*>(out + pk_normal);
+  p.points = reinterpret_cast<double*>(out + pk_points);
+  p.separations = reinterpret_cast<double*>(out + pk_seps);
+  p.shift = reinterpret_cast<double*>(out + pk_shift);
+  p.loss = reinterpret_cast<double*>(out + pk_loss);
+  p.ids = reinterpret_cast<uint32_t*>(out + pk_ids);
+  bm::launch_meet_pack(d, p, ctx->stream);
+  HIP_OR_FAIL(ctx, hipGetLastError(), "bm_meet_packed: pack");
+  // Every pair's count, flags and row first, then only the rows found.
+  char* back = static_cast<char*>(ctx->pstage) + in_bytes;
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(back, d.count, 8 * N, hipMemcpyDeviceToHost, ctx->stream), "bm_meet_packed: copy");
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(back + 8 * N, p.row, 4 * (N + 1), hipMemcpyDeviceToHost, ctx->stream),
+              "bm_meet_packed: copy");
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_meet_packed: kernel");
+  const int32_t* bi = reinterpret_cast<const int32_t*>(back);
+  const size_t found = (size_t)bi[2 * N + N];
+  if (found > 0) {
+    const struct {
+      void* to;
+      const void* from;
+      size_t bytes;
+    } cols[] = {{normal, p.normal, 24 * found}, {points, p.points, 96 * found}, {separations, p.separations, 32 * found},
+                {shift, p.shift, 96 * found},   {loss, p.loss, 8 * found},       {ids, p.ids, 16 * found}};
+    size_t at = (12 * N + 4 + 255) & ~(size_t)255;
+    for (const auto& c : cols) {
+      HIP_OR_FAIL(ctx, hipMemcpyAsync(back + at, c.from, c.bytes, hipMemcpyDeviceToHost, ctx->stream),
+                  "bm_meet_packed: copy");
+      at += (c.bytes + 255) & ~(size_t)255;
+    }
+    HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), "bm_meet_packed: copy");
+    std::vector<HostCopy> copies;
+    at = (12 * N + 4 + 255) & ~(size_t)255;
+    for (const auto& c : cols) {
+      copies.push_back({c.to, back + at, c.bytes});
+      at += (c.bytes + 255) & ~(size_t)255;
+    }
+    run_copies(copies);
+  }
+  std::vector<HostCopy> heads = {{count, back, 4 * N}, {flags, back + 4 * N, 4 * N}, {row, back + 8 * N, 4 * N}};
+  run_copies(heads);
+  HIP_OR_FAIL(ctx, hipEventElapsedTime(&ctx->last_ms, ctx->begin, ctx->end), "bm_meet_packed: hipEventElapsedTime");
   return BM_OK;
 }
 

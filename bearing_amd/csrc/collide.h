@@ -44,6 +44,15 @@ struct MeetDev {
 };
 
 void launch_meet(const MeetDev& d, hipStream_t stream);
+// A meet pass's results packed by pair: row[k] the packed row of pair k when it found a
+// manifold (count[k] > 0, rows in pair order), else -1; *found the rows; the packed columns
+// have room for n rows. One workgroup scans, then a lane a pair copies its row.
+struct MeetPacked {
+  int *row, *found;
+  double *normal, *points, *separations, *shift, *loss;
+  uint32_t* ids;
+};
+void launch_meet_pack(const MeetDev& d, const MeetPacked& p, hipStream_t stream);
 
 // Heightfields (src/mesh.js:288 Heightfield) by number: field f has info[2f] columns and
 // info[2f + 1] rows of heights from heights + off[f], spacing dinfo[3f], x0 dinfo[3f + 1],

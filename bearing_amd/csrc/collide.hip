@@ -2197,6 +2197,45 @@ void launch_meet(const MeetDev& d, hipStream_t stream) {
   hipLaunchKernelGGL(meet_kernel<true>, dim3((d.n + 63) / 64), dim3(64), 0, stream, d);
 }
 
+// Which pairs of a meet pass found a manifold, numbered in pair order.
+__global__ void __launch_bounds__(1024) meet_scan_kernel(MeetDev d, int* row, int* found) {
+  __shared__ int sums[1024];
+  const int t = threadIdx.x, T = blockDim.x;
+  const int per = (d.n + T - 1) / T, k0 = t * per, k1 = k0 + per < d.n ? k0 + per : d.n;
+  int run = 0;
+  for (int k = k0; k < k1; k++) run += d.count[k] > 0 ? 1 : 0;
+  sums[t] = run;
+  __syncthreads();
+  for (int step = 1; step < T; step *= 2) {
+    const int add = t >= step ? sums[t - step] : 0;
+    __syncthreads();
+    sums[t] += add;
+    __syncthreads();
+  }
+  int at = sums[t] - run;
+  for (int k = k0; k < k1; k++) row[k] = d.count[k] > 0 ? at++ : -1;
+  if (t == T - 1) *found = sums[t];
+}
+
+__global__ void __launch_bounds__(256) meet_pack_kernel(MeetDev d, MeetPacked p) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= d.n) return;
+  const long r = p.row[k];
+  if (r < 0) return;
+  for (int i = 0; i < 3; i++) p.normal[3 * r + i] = d.normal[3 * (long)k + i];
+  for (int i = 0; i < 12; i++) p.points[12 * r + i] = d.points[12 * (long)k + i];
+  for (int i = 0; i < 4; i++) p.separations[4 * r + i] = d.separations[4 * (long)k + i];
+  for (int i = 0; i < 12; i++) p.shift[12 * r + i] = d.shift[12 * (long)k + i];
+  for (int i = 0; i < 4; i++) p.ids[4 * r + i] = d.ids[4 * (long)k + i];
+  p.loss[r] = d.loss[k];
+}
+
+void launch_meet_pack(const MeetDev& d, const MeetPacked& p, hipStream_t stream) {
+  if (d.n == 0) return;
+  hipLaunchKernelGGL(meet_scan_kernel, dim3(1), dim3(1024), 0, stream, d, p.row, p.found);
+  hipLaunchKernelGGL(meet_pack_kernel, dim3((d.n + 255) / 256), dim3(256), 0, stream, d, p);
+}
+
 void launch_collide(const CollideDev& d, hipStream_t stream) {
   if (d.n == 0) return;
   hipLaunchKernelGGL(collide_kernel<false>, dim3((d.n + 63) / 64), dim3(64), 0, stream, d);

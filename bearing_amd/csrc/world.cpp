@@ -503,8 +503,10 @@ struct bm_world {
   std::vector<int32_t, Unzeroed<int32_t>> batch_ia, batch_ib, meet_ta, meet_tb;
   std::vector<double, Unzeroed<double>> batch_pair, meet_table;
   std::vector<int32_t> meet_row_of, meet_bodies;  // meet_row_of: by slot, -1 when not in the table
+  // The meet pass's results: count and flags by slot; the manifolds found packed, slot k's in
+  // row batch_row[k] (-1 when it found none) of normal, points, seps, ids, shift and loss.
   std::vector<double> batch_normal, batch_points, batch_seps, batch_shift, batch_loss;
-  std::vector<int32_t> batch_count, batch_flags;
+  std::vector<int32_t> batch_count, batch_flags, batch_row;
   std::vector<uint32_t> batch_ids;
   // Level pairs for the device touchLevel (bm_level_packed): field, the field body's place and
   // axes, the other body's shape record, the box to look in, margin and tolerance; results
@@ -534,7 +536,7 @@ struct bm_world {
   // level pairs before these slots are earlier runs', still read by the visits not yet taken
   // (a visit prepared again after a wake gets a new slot past them). The records waiting for
   // the next run are those slots on.
-  size_t meet_base = 0, level_base = 0;
+  size_t meet_base = 0, level_base = 0, meet_found = 0;
   // For a large pass's first run: each level row's kept entry (what finish_level would match it
   // with), worked out on the pool; level_matched says these hold for visits not prepared again.
   std::vector<int32_t> level_best;
@@ -1301,7 +1303,7 @@ void level_manifold(const bm_world* w, size_t o, Manifold& mf) {
 // One emitted row's columns and warm start (src/contact.js:889-921 emit, 999-1036 carry).
 void fill_row(bm_world* w, const RowFill& f) {
   if (f.mf) {
-    const double* shift = f.shift_at >= 0 ? &w->batch_shift[12 * (size_t)f.shift_at] : nullptr;
+    const double* shift = f.shift_at >= 0 ? &w->batch_shift[12 * (size_t)w->batch_row[f.shift_at]] : nullptr;
     emit_fill(w, f.m, f.p, f.a, f.b, *f.mf, f.tolerance, shift, f.sub, f.sign);
     carry(w, f.p, f.m);
     return;
@@ -1441,23 +1443,32 @@ int run_batch(bm_world* w) {
   const int nb = meet_rows(w, n);
   w->meet_table.resize(32 * (size_t)nb);
   meet_table_fill(w, w->meet_table.data());
-  const size_t k = w->meet_base, e = k + n;  // this run's results go after the pass's earlier runs'
+  // This run's results go after the pass's earlier runs', its rows after theirs.
+  const size_t k = w->meet_base, e = k + n, r = w->meet_found, re = r + n;
   w->batch_count.resize(e);
   w->batch_flags.resize(e);
-  w->batch_normal.resize(3 * e);
-  w->batch_points.resize(12 * e);
-  w->batch_seps.resize(4 * e);
-  w->batch_ids.resize(4 * e);
-  w->batch_shift.resize(12 * e);
-  w->batch_loss.resize(e);
-  const int rc = bm_meet_indexed(w->ctx, nb, w->meet_table.data(), n, w->meet_ta.data(), w->meet_tb.data(),
-                                 w->batch_pair.data(), w->batch_count.data() + k, w->batch_normal.data() + 3 * k,
-                                 w->batch_points.data() + 12 * k, w->batch_seps.data() + 4 * k,
-                                 w->batch_ids.data() + 4 * k, w->batch_shift.data() + 12 * k,
-                                 w->batch_flags.data() + k, w->batch_loss.data() + k);
+  w->batch_row.resize(e);
+  w->batch_normal.resize(3 * re);
+  w->batch_points.resize(12 * re);
+  w->batch_seps.resize(4 * re);
+  w->batch_ids.resize(4 * re);
+  w->batch_shift.resize(12 * re);
+  w->batch_loss.resize(re);
+  const int rc = bm_meet_packed(w->ctx, nb, w->meet_table.data(), n, w->meet_ta.data(), w->meet_tb.data(),
+                                w->batch_pair.data(), w->batch_count.data() + k, w->batch_flags.data() + k,
+                                w->batch_row.data() + k, w->batch_normal.data() + 3 * r,
+                                w->batch_points.data() + 12 * r, w->batch_seps.data() + 4 * r,
+                                w->batch_ids.data() + 4 * r, w->batch_shift.data() + 12 * r, w->batch_loss.data() + r);
   w->batch_ia.clear(); w->batch_ib.clear(); w->batch_pair.clear();
   if (rc != BM_OK) return fail(w, rc, std::string("step: ") + bm_last_error(w->ctx));
+  size_t found = 0;
+  for (size_t j = k; j < e; j++)
+    if (w->batch_row[j] >= 0) {
+      w->batch_row[j] += (int32_t)r;
+      found++;
+    }
   w->meet_base = e;
+  w->meet_found = r + found;
   return BM_OK;
 }
 
@@ -1630,11 +1641,12 @@ int finish_visit(bm_world* w, Visit& v, Manifold& m) {
       w->pairs.forget(v.p);
       return BM_OK;
     }
-    m.loss = w->batch_loss[k];
-    for (int i = 0; i < 3; i++) m.normal[i] = w->batch_normal[3 * k + i];
-    for (int i = 0; i < 12; i++) m.points[i] = w->batch_points[12 * k + i];
-    for (int i = 0; i < 4; i++) m.separations[i] = w->batch_seps[4 * k + i];
-    for (int i = 0; i < 4; i++) m.ids[i] = w->batch_ids[4 * k + i];
+    const size_t r = w->batch_row[k];
+    m.loss = w->batch_loss[r];
+    for (int i = 0; i < 3; i++) m.normal[i] = w->batch_normal[3 * r + i];
+    for (int i = 0; i < 12; i++) m.points[i] = w->batch_points[12 * r + i];
+    for (int i = 0; i < 4; i++) m.separations[i] = w->batch_seps[4 * r + i];
+    for (int i = 0; i < 4; i++) m.ids[i] = w->batch_ids[4 * r + i];
     if (w->batch_flags[k] & 1) {
       // A manifold met at the moment of impact is not kept (src/contact.js:541).
       if (w->pairs.cache[v.p] != 0) w->pairs.uncache(v.p);
@@ -1663,6 +1675,7 @@ void narrow_begin(bm_world* w, int from) {
   // The last pass's runs are all taken: this one's results start afresh.
   w->meet_base = 0;
   w->level_base = 0;
+  w->meet_found = 0;
   w->level_matched = false;
   std::vector<Visit>& visits = w->visits;
   auto visit_of = [&](int p) {
@@ -3181,7 +3194,7 @@ struct bm_batch {
   std::vector<double, Unzeroed<double>> table, pair;
   std::vector<int32_t, Unzeroed<int32_t>> ia, ib;
   std::vector<double> normal, points, seps, shift, loss;
-  std::vector<int32_t> count, flags;
+  std::vector<int32_t> count, flags, row;
   std::vector<uint32_t> ids;
   std::vector<int32_t> level_field, level_manifolds, level_count, level_start;
   std::vector<double> level_pos, level_axes, level_shape, level_box, level_margin, level_tol, level_normal,
@@ -3228,25 +3241,40 @@ int run_merged(bm_batch* B) {
       }
       std::copy(w->batch_pair.begin(), w->batch_pair.begin() + 6 * (size_t)k, B->pair.begin() + 6 * (size_t)at[i]);
     });
-    B->count.resize(n); B->flags.resize(n); B->normal.resize(3 * (size_t)n); B->points.resize(12 * (size_t)n);
-    B->seps.resize(4 * (size_t)n); B->ids.resize(4 * (size_t)n); B->shift.resize(12 * (size_t)n); B->loss.resize(n);
-    if (int rc = bm_meet_indexed(B->ctx, bt[nw], B->table.data(), n, B->ia.data(), B->ib.data(), B->pair.data(),
-                                 B->count.data(), B->normal.data(), B->points.data(), B->seps.data(), B->ids.data(),
-                                 B->shift.data(), B->flags.data(), B->loss.data()))
+    B->count.resize(n); B->flags.resize(n); B->row.resize(n); B->normal.resize(3 * (size_t)n);
+    B->points.resize(12 * (size_t)n); B->seps.resize(4 * (size_t)n); B->ids.resize(4 * (size_t)n);
+    B->shift.resize(12 * (size_t)n); B->loss.resize(n);
+    if (int rc = bm_meet_packed(B->ctx, bt[nw], B->table.data(), n, B->ia.data(), B->ib.data(), B->pair.data(),
+                                B->count.data(), B->flags.data(), B->row.data(), B->normal.data(), B->points.data(),
+                                B->seps.data(), B->ids.data(), B->shift.data(), B->loss.data()))
       return batch_fail(B, rc, std::string("step: ") + bm_last_error(B->ctx));
     B->pool->run(nw, [&](int i) {
       bm_world* w = W[i];
       const int k0 = at[i], k = at[i + 1] - at[i];
       w->batch_ia.clear(); w->batch_ib.clear(); w->batch_pair.clear();
       if (k == 0) return;
-      // after the results of the world's pass's earlier runs
-      const size_t b = w->meet_base, e = b + k;
+      // After the results of the world's pass's earlier runs: its pairs' counts and flags, and
+      // their packed rows (the batch's rows are in pair order, so a world's are one run).
+      const size_t b = w->meet_base, e = b + k, rb = w->meet_found;
+      w->batch_count.resize(e);
+      w->batch_flags.resize(e);
+      w->batch_row.resize(e);
+      std::copy(B->count.begin() + k0, B->count.begin() + k0 + k, w->batch_count.begin() + b);
+      std::copy(B->flags.begin() + k0, B->flags.begin() + k0 + k, w->batch_flags.begin() + b);
+      int32_t lo = -1, hi = -1;
+      for (int j = 0; j < k; j++) {
+        const int32_t r = B->row[k0 + j];
+        if (r >= 0) {
+          if (lo < 0) lo = r;
+          hi = r + 1;
+        }
+        w->batch_row[b + j] = r >= 0 ? r - lo + (int32_t)rb : -1;
+      }
+      const size_t rows = lo >= 0 ? (size_t)(hi - lo) : 0;
       auto put = [&](auto& to, const auto& from, size_t width) {
-        to.resize(width * e);
-        std::copy(from.begin() + width * k0, from.begin() + width * (k0 + k), to.begin() + width * b);
+        to.resize(width * (rb + rows));
+        if (rows > 0) std::copy(from.begin() + width * lo, from.begin() + width * hi, to.begin() + width * rb);
       };
-      put(w->batch_count, B->count, 1);
-      put(w->batch_flags, B->flags, 1);
       put(w->batch_normal, B->normal, 3);
       put(w->batch_points, B->points, 12);
       put(w->batch_seps, B->seps, 4);
@@ -3254,6 +3282,7 @@ int run_merged(bm_batch* B) {
       put(w->batch_shift, B->shift, 12);
       put(w->batch_loss, B->loss, 1);
       w->meet_base = e;
+      w->meet_found = rb + rows;
     });
   }
   std::vector<int> lat(nw + 1, 0);

@@ -199,6 +199,14 @@ int bm_meet_host(bm_context* ctx, int32_t n, const double* a, const double* b, c
 int bm_meet_indexed(bm_context* ctx, int32_t bodies, const double* table, int32_t n, const int32_t* ia,
                     const int32_t* ib, const double* pair, int32_t* count, double* normal, double* points,
                     double* separations, uint32_t* ids, double* shift, int32_t* flags, double* loss);
+/* bm_meet_indexed with the results packed, for a caller that reads the manifolds found only:
+ * count[k] and flags[k] for every pair; row[k] the packed row of pair k when count[k] > 0 (rows
+ * numbered in pair order), else -1; its normal (3 a row), points (12), separations (4), ids
+ * (4), shift (12) and loss (1) in that row. The outputs have room for n rows; only the rows
+ * found come back from the device (most pairs of a settled pile whose boxes touch find none). */
+int bm_meet_packed(bm_context* ctx, int32_t bodies, const double* table, int32_t n, const int32_t* ia,
+                   const int32_t* ib, const double* pair, int32_t* count, int32_t* flags, int32_t* row,
+                   double* normal, double* points, double* separations, uint32_t* ids, double* shift, double* loss);
 
 /* The hulls shape records name by number (a hull's record: kind 3, its number in place of
  * half[0]): n hulls in their bodies' frames, as src/hull.js's Hull holds them. Hull h's

@@ -394,6 +394,35 @@ int bm_meet_indexed(bm_context* ctx, int32_t bodies, const double* table, int32_
   return bm_meet_host(ctx, n, a.data(), b.data(), pair, count, normal, points, separations, ids, shift, flags, loss);
 }
 
+// bm_meet_indexed's results packed as bm_meet_packed leaves them: the pairs that found a
+// manifold in pair order, each one row.
+int bm_meet_packed(bm_context* ctx, int32_t bodies, const double* table, int32_t n, const int32_t* ia,
+                   const int32_t* ib, const double* pair, int32_t* count, int32_t* flags, int32_t* row,
+                   double* normal, double* points, double* separations, uint32_t* ids, double* shift, double* loss) {
+  const size_t N = (size_t)n;
+  std::vector<double> nm(3 * N), pt(12 * N), sp(4 * N), sh(12 * N), ls(N);
+  std::vector<uint32_t> id(4 * N);
+  if (int rc = bm_meet_indexed(ctx, bodies, table, n, ia, ib, pair, count, nm.data(), pt.data(), sp.data(), id.data(),
+                               sh.data(), flags, ls.data()))
+    return rc;
+  int32_t r = 0;
+  for (size_t k = 0; k < N; k++) {
+    if (count[k] <= 0) {
+      row[k] = -1;
+      continue;
+    }
+    row[k] = r;
+    std::memcpy(normal + 3 * (size_t)r, &nm[3 * k], 3 * sizeof(double));
+    std::memcpy(points + 12 * (size_t)r, &pt[12 * k], 12 * sizeof(double));
+    std::memcpy(separations + 4 * (size_t)r, &sp[4 * k], 4 * sizeof(double));
+    std::memcpy(shift + 12 * (size_t)r, &sh[12 * k], 12 * sizeof(double));
+    std::memcpy(ids + 4 * (size_t)r, &id[4 * k], 4 * sizeof(uint32_t));
+    loss[r] = ls[k];
+    r++;
+  }
+  return BM_OK;
+}
+
 }  // extern "C"
 
 // The broadphase passes (bm_broad_*) on the CPU, with the library's own per-body and per-pair
