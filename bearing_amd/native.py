@@ -362,6 +362,29 @@ def meet(ctx, a, b, pair):
     return count, normal, points, separations, ids, shift, flags, loss
 
 
+def meet_packed(ctx, table, ia, ib, pair):
+    """bm_meet_packed on a table of body records (32 numbers each), each pair two row numbers
+    into it, and pair records: (count, flags, row, normal, points, separations, ids, shift,
+    loss), the last six packed by row (the pairs with count > 0, in pair order)."""
+    table = np.ascontiguousarray(table, np.float64)
+    ia = np.ascontiguousarray(ia, np.int32)
+    ib = np.ascontiguousarray(ib, np.int32)
+    pair = np.ascontiguousarray(pair, np.float64)
+    n, bodies = pair.size // 6, table.size // 32
+    count, flags, row = np.zeros(n, np.int32), np.zeros(n, np.int32), np.zeros(n, np.int32)
+    normal, points, separations = np.zeros(3 * n), np.zeros(12 * n), np.zeros(4 * n)
+    ids, shift, loss = np.zeros(4 * n, np.uint32), np.zeros(12 * n), np.zeros(n)
+    ctx.check(ctx.lib.bm_meet_packed(ctx.ptr, bodies, table.ctypes.data_as(_D), n, ia.ctypes.data_as(_I32),
+                                     ib.ctypes.data_as(_I32), pair.ctypes.data_as(_D), count.ctypes.data_as(_I32),
+                                     flags.ctypes.data_as(_I32), row.ctypes.data_as(_I32), normal.ctypes.data_as(_D),
+                                     points.ctypes.data_as(_D), separations.ctypes.data_as(_D),
+                                     ids.ctypes.data_as(_U32), shift.ctypes.data_as(_D), loss.ctypes.data_as(_D)),
+              "bm_meet_packed")
+    found = int((count > 0).sum())
+    return (count, flags, row, normal[:3 * found], points[:12 * found], separations[:4 * found], ids[:4 * found],
+            shift[:12 * found], loss[:found])
+
+
 class Batch:
     """Worlds' soft-step columns, concatenated. Each world is a dict by bm_columns field
     name (per-body, per-manifold, per-pair arrays plus 'awake_list', 'starts', 'solved',

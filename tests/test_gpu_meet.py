@@ -74,3 +74,31 @@ def test_gpu_meet_matches_oracle_on_fresh_pairs(ctx, seed):
     want = binding.meet(a, b, pair)
     got = native.meet(ctx, a, b, pair)
     compare(got, want)
+
+
+def test_gpu_meet_packed_is_the_found_pairs_of_the_oracle(ctx):
+    # bm_meet_packed on the reference's time-of-impact pairs, each body given once in a table
+    # (pair k: rows 2k and 2k + 1), shuffled so a row is met out of order: every pair's count
+    # and flags, and the pairs with points packed in pair order, against the oracle's meet.
+    fx = fixtures.load(os.path.join(fixtures.DATA, "impact.2026.bmfx.gz"))
+    a, b, pair = pairs_from_fixture(fx, 5)
+    n = len(pair)
+    perm = np.random.default_rng(5).permutation(2 * n)
+    table = np.empty((2 * n, 32))
+    table[perm[0::2]] = a
+    table[perm[1::2]] = b
+    count, flags, row, normal, points, seps, ids, shift, loss = native.meet_packed(
+        ctx, table, perm[0::2], perm[1::2], pair)
+    want = binding.meet(a, b, pair)
+    assert np.array_equal(count, want[0]) and np.array_equal(flags, want[6])
+    has = np.flatnonzero(count > 0)
+    assert has.size > 100 and has.size < n, "too few or too many pairs found to test"
+    assert np.array_equal(row[has], np.arange(has.size)) and (row[count <= 0] == -1).all()
+    for got, w, width in ((normal, want[1], 3), (shift, want[5], 12), (loss, want[7], 1)):
+        assert np.array_equal(bits(got), bits(np.asarray(w).reshape(n, width)[has].ravel()))
+    for got, w, per in ((points, want[2], 3), (seps, want[3], 1), (ids, want[4], 1)):
+        g = np.asarray(got).reshape(has.size, 4 * per)
+        ww = np.asarray(w).reshape(n, 4 * per)[has]
+        for r, k in enumerate(has):
+            used = min(count[k], 4) * per
+            assert np.array_equal(bits(g[r, :used]), bits(ww[r, :used])), (k, r)
