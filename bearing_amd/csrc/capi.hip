@@ -4,6 +4,9 @@
 
 #include <algorithm>
 #include <atomic>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
 #include <cmath>
 #include <cstddef>
 #include <cstdio>
@@ -527,6 +530,61 @@ struct HostCopy {
   size_t bytes;
 };
 
+// Host threads kept for the staging copies, made on first use and kept until the process ends
+// (they wait on a condition variable between jobs), so a step's copies do not each start and
+// join threads. One job at a time: a caller waits for the one before it to end.
+class CopyThreads {
+ public:
+  explicit CopyThreads(int n) {
+    for (int t = 0; t < n; t++) threads_.emplace_back([this] { loop(); });
+  }
+  int size() const { return (int)threads_.size(); }
+  // The process's one set, never destroyed: its threads wait for work until the process ends.
+  static CopyThreads& kept() {
+    static CopyThreads* one =
+        new CopyThreads(std::max(0, std::min(16, (int)std::thread::hardware_concurrency()) - 1));
+    return *one;
+  }
+  // f on every kept thread and the caller's, returning when all have returned.
+  void run(const std::function<void()>& f) {
+    std::lock_guard<std::mutex> one(job_lock_);
+    {
+      std::lock_guard<std::mutex> l(m_);
+      job_ = &f;
+      busy_ = (int)threads_.size();
+      gen_++;
+    }
+    start_.notify_all();
+    f();
+    std::unique_lock<std::mutex> l(m_);
+    done_.wait(l, [&] { return busy_ == 0; });
+    job_ = nullptr;
+  }
+
+ private:
+  void loop() {
+    uint64_t seen = 0;
+    for (;;) {
+      const std::function<void()>* f;
+      {
+        std::unique_lock<std::mutex> l(m_);
+        start_.wait(l, [&] { return gen_ != seen; });
+        seen = gen_;
+        f = job_;
+      }
+      (*f)();
+      std::lock_guard<std::mutex> l(m_);
+      if (--busy_ == 0) done_.notify_all();
+    }
+  }
+  std::vector<std::thread> threads_;
+  std::mutex job_lock_, m_;
+  std::condition_variable start_, done_;
+  const std::function<void()>* job_ = nullptr;
+  int busy_ = 0;
+  uint64_t gen_ = 0;
+};
+
 void run_copies(const std::vector<HostCopy>& copies) {
   size_t total = 0;
   for (const HostCopy& c : copies) total += c.bytes;
@@ -554,10 +612,7 @@ void run_copies(const std::vector<HostCopy>& copies) {
     for (size_t r; (r = next.fetch_add(1)) + 1 < cut.size();)
       for (size_t i = cut[r]; i < cut[r + 1]; i++) std::memcpy(pieces[i].to, pieces[i].from, pieces[i].bytes);
   };
-  std::vector<std::thread> pool;
-  for (int t = 1; t < threads; t++) pool.emplace_back(work);
-  work();
-  for (auto& t : pool) t.join();
+  CopyThreads::kept().run(work);
 }
 
 // Units of kind u in one part (a bm_columns of one or more worlds).
