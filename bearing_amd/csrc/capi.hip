@@ -233,37 +233,37 @@ T field(const void* base, size_t off) {
 
 // Checks a batch's offset tables and the indices inside its columns, so no kernel
 // can read or write outside the buffers it is given.
-int check_batch(bm_context* ctx, const bm_columns* h) {
+std::string check_part(const bm_columns* h) {
   const int n = h->n_worlds;
-  if (n <= 0) return fail(ctx, BM_ERR_ARGUMENT, "bm_upload: n_worlds must be 1 or more");
+  if (n <= 0) return "bm_upload: n_worlds must be 1 or more";
   const int32_t* tables[4] = {h->body_offset, h->manifold_offset, h->pair_offset, h->awake_offset};
   const char* names[4] = {"body_offset", "manifold_offset", "pair_offset", "awake_offset"};
   for (int t = 0; t < 4; t++) {
-    if (!tables[t]) return fail(ctx, BM_ERR_ARGUMENT, std::string("bm_upload: ") + names[t] + " is null");
-    if (tables[t][0] != 0) return fail(ctx, BM_ERR_ARGUMENT, std::string("bm_upload: ") + names[t] + "[0] must be 0");
+    if (!tables[t]) return std::string("bm_upload: ") + names[t] + " is null";
+    if (tables[t][0] != 0) return std::string("bm_upload: ") + names[t] + "[0] must be 0";
     for (int w = 0; w < n; w++)
       if (tables[t][w + 1] < tables[t][w])
-        return fail(ctx, BM_ERR_ARGUMENT, std::string("bm_upload: ") + names[t] + " must not decrease");
+        return std::string("bm_upload: ") + names[t] + " must not decrease";
   }
-  if (h->awake_offset[n] > 0 && !h->awake_list) return fail(ctx, BM_ERR_ARGUMENT, "bm_upload: awake_list is null");
+  if (h->awake_offset[n] > 0 && !h->awake_list) return "bm_upload: awake_list is null";
   if (h->manifold_offset[n] > 0 && (!h->a || !h->b || !h->pair || !h->sub || !h->row || !h->points))
-    return fail(ctx, BM_ERR_ARGUMENT, "bm_upload: a manifold index column is null");
+    return "bm_upload: a manifold index column is null";
   for (int w = 0; w < n; w++) {
     const int nb = h->body_offset[w + 1] - h->body_offset[w];
     const int nm = h->manifold_offset[w + 1] - h->manifold_offset[w];
     const int np = h->pair_offset[w + 1] - h->pair_offset[w];
     const int moff = h->manifold_offset[w], aoff = h->awake_offset[w];
-    if (h->substeps[w] < 1) return fail(ctx, BM_ERR_ARGUMENT, "bm_upload: substeps must be 1 or more");
+    if (h->substeps[w] < 1) return "bm_upload: substeps must be 1 or more";
     for (int i = aoff; i < h->awake_offset[w + 1]; i++)
-      if (h->awake_list[i] < 0 || h->awake_list[i] >= nb) return fail(ctx, BM_ERR_ARGUMENT, "bm_upload: awake_list slot out of range");
+      if (h->awake_list[i] < 0 || h->awake_list[i] >= nb) return "bm_upload: awake_list slot out of range";
     for (int m = moff; m < moff + nm; m++) {
       if (h->a[m] < 0 || h->a[m] >= nb || h->b[m] < 0 || h->b[m] >= nb)
-        return fail(ctx, BM_ERR_ARGUMENT, "bm_upload: manifold body slot out of range");
-      if (h->pair[m] < 0 || h->pair[m] >= np) return fail(ctx, BM_ERR_ARGUMENT, "bm_upload: manifold pair out of range");
-      if (h->points[m] < 1 || h->points[m] > 4) return fail(ctx, BM_ERR_ARGUMENT, "bm_upload: a manifold has 1 to 4 points");
+        return "bm_upload: manifold body slot out of range";
+      if (h->pair[m] < 0 || h->pair[m] >= np) return "bm_upload: manifold pair out of range";
+      if (h->points[m] < 1 || h->points[m] > 4) return "bm_upload: a manifold has 1 to 4 points";
     }
   }
-  return BM_OK;
+  return std::string();
 }
 
 // Whether starts, solved and row describe rows laid out by colour: runs in order within
@@ -594,10 +594,16 @@ void run_copies(const std::vector<HostCopy>& copies) {
     return;
   }
   constexpr size_t PIECE = (size_t)4 << 20;
-  std::vector<HostCopy> pieces;
-  for (const HostCopy& c : copies)
-    for (size_t at = 0; at < c.bytes; at += PIECE)
-      pieces.push_back({static_cast<char*>(c.to) + at, static_cast<const char*>(c.from) + at, std::min(PIECE, c.bytes - at)});
+  // Copies larger than a piece are cut into pieces; a list with none (a batch's many small
+  // copies) is taken as it is.
+  bool large = false;
+  for (const HostCopy& c : copies) large = large || c.bytes > PIECE;
+  std::vector<HostCopy> cut_up;
+  if (large)
+    for (const HostCopy& c : copies)
+      for (size_t at = 0; at < c.bytes; at += PIECE)
+        cut_up.push_back({static_cast<char*>(c.to) + at, static_cast<const char*>(c.from) + at, std::min(PIECE, c.bytes - at)});
+  const std::vector<HostCopy>& pieces = large ? cut_up : copies;
   // Run r is pieces [cut[r], cut[r + 1]).
   const size_t runs = (size_t)threads * 8, goal = total / runs + 1;
   std::vector<size_t> cut(1, 0);
@@ -646,9 +652,27 @@ int upload_parts(bm_context* ctx, int nparts, const bm_columns* const* parts, co
       if (col.unit <= WORLD3 && !field<const void*>(h, col.host))
         return fail(ctx, BM_ERR_ARGUMENT, std::string(who) + ": " + col.name + " is null");
     }
-    if (int rc = check_batch(ctx, h)) return rc;
-    laid = laid && laid_out(h);
     W += h->n_worlds;
+  }
+  // Each part's indices checked and its layout read, a batch's parts on the copy threads; the
+  // first part found wanting (in order) names the error.
+  std::vector<std::string> why(nparts);
+  std::vector<uint8_t> part_laid(nparts, 0);
+  auto check = [&](int q) {
+    why[q] = check_part(parts[q]);
+    part_laid[q] = why[q].empty() && laid_out(parts[q]) ? 1 : 0;
+  };
+  if (nparts >= 64) {
+    std::atomic<int> next{0};
+    CopyThreads::kept().run([&] {
+      for (int q; (q = next.fetch_add(1)) < nparts;) check(q);
+    });
+  } else {
+    for (int q = 0; q < nparts; q++) check(q);
+  }
+  for (int q = 0; q < nparts; q++) {
+    if (!why[q].empty()) return fail(ctx, BM_ERR_ARGUMENT, why[q]);
+    laid = laid && part_laid[q];
   }
   ctx->laid_out = laid;
   ctx->n_worlds = W;
