@@ -107,6 +107,10 @@ struct bm_context {
   void* noalloc_block = nullptr;  // this device's block for bm_level_noalloc (collide.hip)
   bool stored = false;            // a soft step ran on the batch uploaded last (its pair stores are set)
   hipEvent_t begin = nullptr, end = nullptr;
+  // A download's pieces (made on first use): each piece's columns go to the worlds while the
+  // pieces after it are still coming back.
+  static constexpr int PIECES = 4;
+  hipEvent_t piece[PIECES] = {};
   std::string error;
   int32_t n_worlds = 0, bodies = 0, awake = 0, manifolds = 0, pairs = 0;
   // Every column lives in one device arena at a 256-byte aligned offset, the columns
@@ -474,6 +478,8 @@ int bm_destroy(bm_context* ctx) {
   bm::held_free(ctx->held);
   if (ctx->begin) (void)hipEventDestroy(ctx->begin);
   if (ctx->end) (void)hipEventDestroy(ctx->end);
+  for (hipEvent_t e : ctx->piece)
+    if (e) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return BM_OK;
@@ -861,23 +867,48 @@ int download_parts(bm_context* ctx, int nparts, bm_columns* const* parts, const 
           !(ctx->keep_device_only && is_device_only(c)))
         return fail(ctx, BM_ERR_ARGUMENT, std::string(who) + ": " + COLUMNS[c].name + " is null");
   }
-  HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->stage, ctx->arena, ctx->moved, hipMemcpyDeviceToHost, ctx->stream),
-              (std::string(who) + ": hipMemcpyAsync").c_str());
-  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream), (std::string(who) + ": hipStreamSynchronize").c_str());
-  const char* stage = static_cast<const char*>(ctx->stage);
-  std::vector<HostCopy> copies;
-  for (int c = 0; c < NCOL; c++) {
-    if (COLUMNS[c].dir != IO || ctx->bytes[c] == 0 || COLUMNS[c].unit == KEPT) continue;
-    if (ctx->keep_device_only && is_device_only(c)) continue;
-    const char* from = stage + ctx->offset[c];
-    for (int q = 0; q < nparts; q++) {
-      const size_t bytes = part_units(parts[q], COLUMNS[c].unit) * COLUMNS[c].per * COLUMNS[c].elem;
-      if (bytes == 0) continue;
-      copies.push_back({field<void*>(parts[q], COLUMNS[c].host), from, bytes});
-      from += bytes;
-    }
+  // The arena comes back in pieces cut at column starts, a column whole in one piece; each
+  // piece's columns are scattered to the worlds while the pieces after it are in flight.
+  constexpr int K = bm_context::PIECES;
+  for (hipEvent_t& e : ctx->piece)
+    if (!e) HIP_OR_FAIL(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming), (std::string(who) + ": hipEventCreate").c_str());
+  const size_t moved = ctx->moved;
+  size_t cut[K + 1];
+  cut[0] = 0;
+  cut[K] = moved;
+  for (int k = 1; k < K; k++) {
+    // the first column start at or after an equal share of the bytes
+    const size_t goal = moved / K * k;
+    size_t at = moved;
+    for (int c = 0; c < NCOL; c++)
+      if (!ctx->in_block[c] && ctx->offset[c] >= goal && ctx->offset[c] < at) at = ctx->offset[c];
+    cut[k] = std::max(cut[k - 1], at);
   }
-  run_copies(copies);
+  char* stage = static_cast<char*>(ctx->stage);
+  const char* arena = static_cast<const char*>(ctx->arena);
+  for (int k = 0; k < K; k++) {
+    if (cut[k + 1] > cut[k])
+      HIP_OR_FAIL(ctx, hipMemcpyAsync(stage + cut[k], arena + cut[k], cut[k + 1] - cut[k], hipMemcpyDeviceToHost, ctx->stream),
+                  (std::string(who) + ": hipMemcpyAsync").c_str());
+    HIP_OR_FAIL(ctx, hipEventRecord(ctx->piece[k], ctx->stream), (std::string(who) + ": hipEventRecord").c_str());
+  }
+  for (int k = 0; k < K; k++) {
+    HIP_OR_FAIL(ctx, hipEventSynchronize(ctx->piece[k]), (std::string(who) + ": hipEventSynchronize").c_str());
+    std::vector<HostCopy> copies;
+    for (int c = 0; c < NCOL; c++) {
+      if (COLUMNS[c].dir != IO || ctx->bytes[c] == 0 || COLUMNS[c].unit == KEPT) continue;
+      if (ctx->keep_device_only && is_device_only(c)) continue;
+      if (ctx->in_block[c] || ctx->offset[c] < cut[k] || ctx->offset[c] >= cut[k + 1]) continue;
+      const char* from = stage + ctx->offset[c];
+      for (int q = 0; q < nparts; q++) {
+        const size_t bytes = part_units(parts[q], COLUMNS[c].unit) * COLUMNS[c].per * COLUMNS[c].elem;
+        if (bytes == 0) continue;
+        copies.push_back({field<void*>(parts[q], COLUMNS[c].host), from, bytes});
+        from += bytes;
+      }
+    }
+    run_copies(copies);
+  }
   // Before a soft step ran on this upload the pairs keep what they held, as it left them.
   if (ctx->stored) scatter_kept(ctx, nparts, parts, stage);
   return BM_OK;
