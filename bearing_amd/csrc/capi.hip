@@ -788,9 +788,35 @@ int upload_parts(bm_context* ctx, int nparts, const bm_columns* const* parts, co
     ctx->kept_version = ctx->body_version;
     ctx->kept_bodies = ctx->bodies;
   }
-  run_copies(copies);
-  HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->arena, ctx->stage, sent, hipMemcpyHostToDevice, ctx->stream),
-              (std::string(who) + ": hipMemcpyAsync").c_str());
+  {
+    // The staging filled and sent in pieces cut at column starts: each piece goes up while the
+    // next one is filled.
+    constexpr int K = bm_context::PIECES;
+    size_t cut[K + 1];
+    cut[0] = 0;
+    cut[K] = sent;
+    for (int k = 1; k < K; k++) {
+      const size_t goal = sent / K * k;
+      size_t at = sent;
+      for (int c = 0; c < NCOL; c++)
+        if (!ctx->in_block[c] && ctx->offset[c] >= goal && ctx->offset[c] < at) at = ctx->offset[c];
+      cut[k] = std::max(cut[k - 1], at);
+    }
+    std::vector<HostCopy> piece[K];
+    for (const HostCopy& c : copies) {
+      const size_t o = static_cast<size_t>(static_cast<char*>(c.to) - stage);
+      int k = 0;
+      while (k + 1 < K && o >= cut[k + 1]) k++;
+      piece[k].push_back(c);
+    }
+    for (int k = 0; k < K; k++) {
+      run_copies(piece[k]);
+      if (cut[k + 1] > cut[k])
+        HIP_OR_FAIL(ctx, hipMemcpyAsync(static_cast<char*>(ctx->arena) + cut[k], stage + cut[k], cut[k + 1] - cut[k],
+                                        hipMemcpyHostToDevice, ctx->stream),
+                    (std::string(who) + ": hipMemcpyAsync").c_str());
+    }
+  }
   ctx->moff_host = ctx->join_manifold;
   ctx->boff_host = ctx->join_body;
   ctx->dev.n_worlds = ctx->n_worlds;
