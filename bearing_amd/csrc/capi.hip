@@ -1363,18 +1363,36 @@ int meet_start(bm_context* ctx, const char* who, int32_t bodies, const double* t
                const int32_t* ib, const double* pair, size_t out_bytes, bm::MeetDev& d, size_t* in_bytes_out) {
   const std::string w(who);
   if (!table || !ia || !ib || !pair) return fail(ctx, BM_ERR_ARGUMENT, w + ": null argument");
-  for (int32_t r = 0; r < bodies; r++) {
-    const double kind = table[32 * (size_t)r];
-    if (!(kind == 0 || kind == 1 || kind == 2 || kind == 3))
-      return fail(ctx, BM_ERR_ARGUMENT, w + ": a shape's kind must be 0, 1, 2 or 3");
-    if (!hull_ok(ctx, table + 32 * (size_t)r))
-      return fail(ctx, BM_ERR_ARGUMENT, w + ": a hull's number is not in the hull table (bm_set_hulls)");
+  // The records and pairs checked in chunks (a large pass's on the copy threads); the first
+  // chunk found wanting names the error.
+  const int64_t items = (int64_t)bodies + n, chunk = 16384, chunks = (items + chunk - 1) / chunk;
+  std::vector<uint8_t> why((size_t)chunks, 0);  // 0 good; 1 kind, 2 hull, 3 body, 4 dt
+  auto check = [&](int64_t c) {
+    for (int64_t i = c * chunk, e = std::min(items, (c + 1) * chunk); i < e && !why[c]; i++) {
+      if (i < bodies) {
+        const double kind = table[32 * (size_t)i];
+        if (!(kind == 0 || kind == 1 || kind == 2 || kind == 3)) why[c] = 1;
+        else if (!hull_ok(ctx, table + 32 * (size_t)i)) why[c] = 2;
+      } else {
+        const int64_t k = i - bodies;
+        if (ia[k] < 0 || ia[k] >= bodies || ib[k] < 0 || ib[k] >= bodies) why[c] = 3;
+        else if (!(pair[6 * (size_t)k + 2] > 0)) why[c] = 4;
+      }
+    }
+  };
+  if (chunks >= 4) {
+    std::atomic<int64_t> next{0};
+    CopyThreads::kept().run([&] {
+      for (int64_t c; (c = next.fetch_add(1)) < chunks;) check(c);
+    });
+  } else {
+    for (int64_t c = 0; c < chunks; c++) check(c);
   }
-  for (int32_t k = 0; k < n; k++) {
-    if (ia[k] < 0 || ia[k] >= bodies || ib[k] < 0 || ib[k] >= bodies)
-      return fail(ctx, BM_ERR_ARGUMENT, w + ": a pair names a body the table lacks");
-    if (!(pair[6 * (size_t)k + 2] > 0)) return fail(ctx, BM_ERR_ARGUMENT, w + ": dt must be positive");
-  }
+  static const char* const WHY[] = {"", ": a shape's kind must be 0, 1, 2 or 3",
+                                    ": a hull's number is not in the hull table (bm_set_hulls)",
+                                    ": a pair names a body the table lacks", ": dt must be positive"};
+  for (int64_t c = 0; c < chunks; c++)
+    if (why[c]) return fail(ctx, BM_ERR_ARGUMENT, w + WHY[why[c]]);
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->device), (w + ": hipSetDevice").c_str());
   const size_t N = (size_t)n, T = 32 * sizeof(double) * (size_t)bodies;
   const size_t pairs = 6 * sizeof(double) * N, idx = (4 * N + 255) & ~(size_t)255;
