@@ -2023,8 +2023,10 @@ void fall_asleep(bm_world* w, int s, int r) {
 void sleep_step(bm_world* w, double dt) {
   const double* p = w->position.data();
   const double* q = w->rotation.data();
+  // Each awake body's timer and rest pose are its own: a large world's on the pool.
+  auto timers = [&](int i0, int i1) {
   bool ready = false;
-  for (int i = 0; i < w->awake_count; i++) {
+  for (int i = i0; i < i1; i++) {
     const int s = w->awake_list[i], s3 = 3 * s, s4 = 4 * s, r7 = 7 * s;
     const double* rest = &w->rest[r7];
     const double dx = p[s3] - rest[0], dy = p[s3 + 1] - rest[1], dz = p[s3 + 2] - rest[2];
@@ -2045,6 +2047,18 @@ void sleep_step(bm_world* w, double dt) {
     } else {
       settle(w, s);
     }
+  }
+  return ready;
+  };
+  bool ready = false;
+  const int n_awake = w->awake_count;
+  if (w->pool && n_awake >= PARALLEL_VISITS) {
+    const int chunk = 2048, chunks = (n_awake + chunk - 1) / chunk;
+    std::vector<uint8_t> any((size_t)chunks, 0);
+    w->pool->run(chunks, [&](int c) { any[c] = timers(c * chunk, std::min(n_awake, (c + 1) * chunk)) ? 1 : 0; });
+    for (uint8_t a : any) ready = ready || a;
+  } else {
+    ready = timers(0, n_awake);
   }
   w->still_ready = ready;
   if (!ready) return;
@@ -2771,7 +2785,9 @@ int catch_fast(bm_world* w) {
 void touched(bm_world* w) {
   Contacts& c = w->c;
   Pairs& pr = w->pairs;
-  for (int f = 0; f < c.count;) {
+  // Each pair's rows are neighbours in narrow order: whether any of them meets (its gap within
+  // tolerance, or a point pushed), and the first that does.
+  auto first_hit = [&](int& f) {
     const int p = c.pair[c.row[f]];
     int hit = -1;
     for (; f < c.count && c.pair[c.row[f]] == p; f++) {
@@ -2781,12 +2797,12 @@ void touched(bm_world* w) {
       for (int k = 0; k < c.points[m] && !meets; k++) meets = c.max_normal_impulse[m * P + k] > 0;
       if (meets) hit = m;
     }
-    const uint8_t now = hit >= 0 ? 1 : 0;
-    if (now == pr.touching[p]) continue;
-    pr.touching[p] = now;
-    if (now == 0) {
+    return std::make_pair(p, hit);
+  };
+  auto event = [&](int p, int hit) {
+    if (hit < 0) {
       w->events.end(pr.a[p], pr.b[p]);
-      continue;
+      return;
     }
     // The deepest point, moved on by half of what the two bodies moved this step; the
     // fastest closing speed among the points.
@@ -2804,7 +2820,41 @@ void touched(bm_world* w) {
     for (int k = 0; k < 3; k++)
       at[k] = c.point[d3 + k] + (c.shift[d3 + k] + p3[a3 + k] - st[a3 + k] + p3[b3 + k] - st[b3 + k]) * 0.5;
     w->events.begin(pr.a[p], pr.b[p], at[0], at[1], at[2], c.normal[h3], c.normal[h3 + 1], c.normal[h3 + 2], speed);
+  };
+  const int n = c.count;
+  if (!w->pool || n < PARALLEL_VISITS) {
+    for (int f = 0; f < n;) {
+      const auto [p, hit] = first_hit(f);
+      const uint8_t now = hit >= 0 ? 1 : 0;
+      if (now == pr.touching[p]) continue;
+      pr.touching[p] = now;
+      event(p, hit);
+    }
+    return;
   }
+  // A large world: the pairs' flags on the pool, in chunks cut between one pair's rows and the
+  // next's (each pair's entries touched by one chunk alone); the pairs that changed listed per
+  // chunk, and their events written after, in narrow order.
+  const int chunk = 4096, chunks = (n + chunk - 1) / chunk;
+  std::vector<std::vector<std::pair<int, int>>> changed((size_t)chunks);
+  w->pool->run(chunks, [&](int k) {
+    int f = k * chunk, e = std::min(n, f + chunk);
+    if (k > 0)
+      for (const int before = c.pair[c.row[f - 1]]; f < n && c.pair[c.row[f]] == before;) f++;
+    if (e < n)
+      for (const int last = c.pair[c.row[e - 1]]; e < n && c.pair[c.row[e]] == last;) e++;
+    std::vector<std::pair<int, int>>& list = changed[k];
+    list.clear();
+    while (f < e) {
+      const auto [p, hit] = first_hit(f);
+      const uint8_t now = hit >= 0 ? 1 : 0;
+      if (now == pr.touching[p]) continue;
+      pr.touching[p] = now;
+      list.emplace_back(p, hit);
+    }
+  });
+  for (const auto& list : changed)
+    for (const auto& [p, hit] : list) event(p, hit);
 }
 
 // src/contact.js:1067-1115 (store, keep) for level pairs: the device kept the other pairs'
