@@ -1478,7 +1478,8 @@ int bm_meet_packed(bm_context* ctx, int32_t bodies, const double* table, int32_t
   // then room for N found rows of each column.
   const size_t pk_row = 280 * N + 64, pk_normal = pk_row + ((4 * (N + 1) + 255) & ~(size_t)255);
   const size_t pk_points = pk_normal + 24 * N, pk_seps = pk_points + 96 * N, pk_shift = pk_seps + 32 * N;
-  const size_t pk_loss = pk_shift + 96 * N, pk_ids = pk_loss + 8 * N, pk_end = pk_ids + 16 * N;
+  const size_t pk_loss = pk_shift + 96 * N, pk_ids = pk_loss + 8 * N, pk_sums = pk_ids + 16 * N;
+  const size_t pk_end = pk_sums + 4 * (N / 256 + 2);
   if (int rc = meet_start(ctx, "bm_meet_packed", bodies, table, n, ia, ib, pair, pk_end, d, &in_bytes)) return rc;
   char* out = static_cast<char*>(ctx->pairs_out.p);
   bm::MeetPacked p{};
@@ -1490,6 +1491,7 @@ int bm_meet_packed(bm_context* ctx, int32_t bodies, const double* table, int32_t
   p.shift = reinterpret_cast<double*>(out + pk_shift);
   p.loss = reinterpret_cast<double*>(out + pk_loss);
   p.ids = reinterpret_cast<uint32_t*>(out + pk_ids);
+  p.sums = reinterpret_cast<int*>(out + pk_sums);
   bm::launch_meet_pack(d, p, ctx->stream);
   HIP_OR_FAIL(ctx, hipGetLastError(), "bm_meet_packed: pack");
   // Every pair's count, flags and row first, then only the rows found.

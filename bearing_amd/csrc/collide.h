@@ -46,9 +46,11 @@ struct MeetDev {
 void launch_meet(const MeetDev& d, hipStream_t stream);
 // A meet pass's results packed by pair: row[k] the packed row of pair k when it found a
 // manifold (count[k] > 0, rows in pair order), else -1; *found the rows; the packed columns
-// have room for n rows. One workgroup scans, then a lane a pair copies its row.
+// have room for n rows; sums: room for an int per 256 pairs (the workgroups' counts, then
+// their first rows). Workgroups count, one workgroup numbers them, then a lane a pair copies
+// its row.
 struct MeetPacked {
-  int *row, *found;
+  int *row, *found, *sums;
   double *normal, *points, *separations, *shift, *loss;
   uint32_t* ids;
 };

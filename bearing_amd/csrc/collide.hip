@@ -2197,31 +2197,56 @@ void launch_meet(const MeetDev& d, hipStream_t stream) {
   hipLaunchKernelGGL(meet_kernel<true>, dim3((d.n + 63) / 64), dim3(64), 0, stream, d);
 }
 
-// Which pairs of a meet pass found a manifold, numbered in pair order.
-__global__ void __launch_bounds__(1024) meet_scan_kernel(MeetDev d, int* row, int* found) {
-  __shared__ int sums[1024];
-  const int t = threadIdx.x, T = blockDim.x;
-  const int per = (d.n + T - 1) / T, k0 = t * per, k1 = k0 + per < d.n ? k0 + per : d.n;
-  int run = 0;
-  for (int k = k0; k < k1; k++) run += d.count[k] > 0 ? 1 : 0;
-  sums[t] = run;
-  __syncthreads();
-  for (int step = 1; step < T; step *= 2) {
-    const int add = t >= step ? sums[t - step] : 0;
-    __syncthreads();
-    sums[t] += add;
-    __syncthreads();
-  }
-  int at = sums[t] - run;
-  for (int k = k0; k < k1; k++) row[k] = d.count[k] > 0 ? at++ : -1;
-  if (t == T - 1) *found = sums[t];
+// Which pairs of a meet pass found a manifold, numbered in pair order, in three launches: each
+// workgroup of 256 pairs counts its own, one workgroup turns the counts into each workgroup's
+// first row (and the total), then each pair takes its row (the workgroup's first plus the pairs
+// before it in the workgroup that found one) and copies its results there.
+constexpr int MEET_PACK_THREADS = 256;
+
+__global__ void __launch_bounds__(MEET_PACK_THREADS) meet_count_kernel(MeetDev d, int* sums) {
+  const int k = blockIdx.x * MEET_PACK_THREADS + threadIdx.x;
+  const int found = __syncthreads_count(k < d.n && d.count[k] > 0);
+  if (threadIdx.x == 0) sums[blockIdx.x] = found;
 }
 
-__global__ void __launch_bounds__(256) meet_pack_kernel(MeetDev d, MeetPacked p) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(1024) meet_first_kernel(int blocks, int* sums, int* found) {
+  __shared__ int part[1024];
+  const int t = threadIdx.x, T = blockDim.x;
+  const int per = (blocks + T - 1) / T, b0 = t * per, b1 = b0 + per < blocks ? b0 + per : blocks;
+  int run = 0;
+  for (int b = b0; b < b1; b++) run += sums[b];
+  part[t] = run;
+  __syncthreads();
+  for (int step = 1; step < T; step *= 2) {
+    const int add = t >= step ? part[t - step] : 0;
+    __syncthreads();
+    part[t] += add;
+    __syncthreads();
+  }
+  int at = part[t] - run;
+  for (int b = b0; b < b1; b++) {
+    const int n = sums[b];
+    sums[b] = at;
+    at += n;
+  }
+  if (t == T - 1) *found = part[t];
+}
+
+__global__ void __launch_bounds__(MEET_PACK_THREADS) meet_pack_kernel(MeetDev d, MeetPacked p, const int* first) {
+  __shared__ int waves[MEET_PACK_THREADS / 64];
+  const int k = blockIdx.x * MEET_PACK_THREADS + threadIdx.x;
+  const bool has = k < d.n && d.count[k] > 0;
+  const unsigned long long mask = __ballot(has);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) waves[wave] = __popcll(mask);
+  __syncthreads();
+  int before = first[blockIdx.x];
+  for (int w = 0; w < wave; w++) before += waves[w];
+  before += __popcll(mask & ((1ull << lane) - 1));
   if (k >= d.n) return;
-  const long r = p.row[k];
-  if (r < 0) return;
+  p.row[k] = has ? before : -1;
+  if (!has) return;
+  const long r = before;
   for (int i = 0; i < 3; i++) p.normal[3 * r + i] = d.normal[3 * (long)k + i];
   for (int i = 0; i < 12; i++) p.points[12 * r + i] = d.points[12 * (long)k + i];
   for (int i = 0; i < 4; i++) p.separations[4 * r + i] = d.separations[4 * (long)k + i];
@@ -2232,8 +2257,11 @@ __global__ void __launch_bounds__(256) meet_pack_kernel(MeetDev d, MeetPacked p)
 
 void launch_meet_pack(const MeetDev& d, const MeetPacked& p, hipStream_t stream) {
   if (d.n == 0) return;
-  hipLaunchKernelGGL(meet_scan_kernel, dim3(1), dim3(1024), 0, stream, d, p.row, p.found);
-  hipLaunchKernelGGL(meet_pack_kernel, dim3((d.n + 255) / 256), dim3(256), 0, stream, d, p);
+  const int blocks = (d.n + MEET_PACK_THREADS - 1) / MEET_PACK_THREADS;
+  hipLaunchKernelGGL(meet_count_kernel, dim3(blocks), dim3(MEET_PACK_THREADS), 0, stream, d, p.sums);
+  hipLaunchKernelGGL(meet_first_kernel, dim3(1), dim3(1024), 0, stream, blocks, p.sums, p.found);
+  hipLaunchKernelGGL(meet_pack_kernel, dim3(blocks), dim3(MEET_PACK_THREADS), 0, stream, d, p,
+                     static_cast<const int*>(p.sums));
 }
 
 void launch_collide(const CollideDev& d, hipStream_t stream) {
