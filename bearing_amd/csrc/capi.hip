@@ -591,6 +591,9 @@ class CopyThreads {
   uint64_t gen_ = 0;
 };
 
+// Uploads and downloads of at least this many bytes go in pieces (bm_context::PIECES).
+constexpr size_t PIECED_BYTES = (size_t)8 << 20;
+
 void run_copies(const std::vector<HostCopy>& copies) {
   size_t total = 0;
   for (const HostCopy& c : copies) total += c.bytes;
@@ -794,8 +797,8 @@ int upload_parts(bm_context* ctx, int nparts, const bm_columns* const* parts, co
     constexpr int K = bm_context::PIECES;
     size_t cut[K + 1];
     cut[0] = 0;
-    cut[K] = sent;
-    for (int k = 1; k < K; k++) {
+    for (int k = 1; k <= K; k++) cut[k] = sent;
+    for (int k = 1; k < K && sent >= PIECED_BYTES; k++) {
       const size_t goal = sent / K * k;
       size_t at = sent;
       for (int c = 0; c < NCOL; c++)
@@ -901,8 +904,9 @@ int download_parts(bm_context* ctx, int nparts, bm_columns* const* parts, const 
   const size_t moved = ctx->moved;
   size_t cut[K + 1];
   cut[0] = 0;
-  cut[K] = moved;
-  for (int k = 1; k < K; k++) {
+  // A small world's arena comes back in one piece (each piece costs a copy, an event and a wait).
+  for (int k = 1; k <= K; k++) cut[k] = moved;
+  for (int k = 1; k < K && moved >= PIECED_BYTES; k++) {
     // the first column start at or after an equal share of the bytes
     const size_t goal = moved / K * k;
     size_t at = moved;
@@ -913,12 +917,13 @@ int download_parts(bm_context* ctx, int nparts, bm_columns* const* parts, const 
   char* stage = static_cast<char*>(ctx->stage);
   const char* arena = static_cast<const char*>(ctx->arena);
   for (int k = 0; k < K; k++) {
-    if (cut[k + 1] > cut[k])
-      HIP_OR_FAIL(ctx, hipMemcpyAsync(stage + cut[k], arena + cut[k], cut[k + 1] - cut[k], hipMemcpyDeviceToHost, ctx->stream),
-                  (std::string(who) + ": hipMemcpyAsync").c_str());
+    if (cut[k + 1] == cut[k]) continue;
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(stage + cut[k], arena + cut[k], cut[k + 1] - cut[k], hipMemcpyDeviceToHost, ctx->stream),
+                (std::string(who) + ": hipMemcpyAsync").c_str());
     HIP_OR_FAIL(ctx, hipEventRecord(ctx->piece[k], ctx->stream), (std::string(who) + ": hipEventRecord").c_str());
   }
   for (int k = 0; k < K; k++) {
+    if (cut[k + 1] == cut[k]) continue;
     HIP_OR_FAIL(ctx, hipEventSynchronize(ctx->piece[k]), (std::string(who) + ": hipEventSynchronize").c_str());
     std::vector<HostCopy> copies;
     for (int c = 0; c < NCOL; c++) {
