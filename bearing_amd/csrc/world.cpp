@@ -1899,7 +1899,9 @@ bool narrow_resume(bm_world* w, int& rc) {
     }
   }
   std::vector<Visit>& visits = w->visits;
+  constexpr size_t AHEAD_VISITS = 16;  // visits asked for before the walk reaches them
   for (; w->visit_at < w->live_count; w->visit_at++) {
+    if (w->visit_at + AHEAD_VISITS < w->live_count) __builtin_prefetch(&visits[w->live[w->visit_at + AHEAD_VISITS]]);
     const int i = w->live[w->visit_at];
     Visit& v = visits[i];
     Manifold& m = w->rebuilt[i];
