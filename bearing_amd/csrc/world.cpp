@@ -113,7 +113,7 @@ class Pool {
   std::atomic<int> next_{0};
   bool stop_ = false;
 };
-constexpr int PARALLEL_VISITS = 8192;  // a narrow pass this long decides its pairs on threads
+constexpr int PARALLEL_VISITS = 8192;  // by default, a narrow pass this long decides its pairs on threads
 constexpr int P = 4;
 constexpr int DYNAMIC = 0, STATIC = 1;
 constexpr int BOX = 0, SPHERE = 1, CAPSULE = 2, HULL = 3, MESH = 5, HEIGHTFIELD = 6;  // src/contact.js:114
@@ -532,6 +532,9 @@ struct bm_world {
   int narrow_from = 0;
   // A large pass decided on the pool whose first batch's results have not been looked at yet.
   bool first_results = false;
+  // From how many pairs (or awake bodies, or rows) a world stepped alone takes a pass's work to
+  // its host threads (bm_world_set_parallel).
+  int parallel_from = PARALLEL_VISITS;
   // Within a pass, each run of the device batches appends its results: the meet results and
   // level pairs before these slots are earlier runs', still read by the visits not yet taken
   // (a visit prepared again after a wake gets a new slot past them). The records waiting for
@@ -1684,7 +1687,7 @@ void narrow_begin(bm_world* w, int from) {
   };
   const bool ready = w->broad.apart_ready;
   w->broad.apart_ready = false;
-  if (ready || (w->pool && pr.active - from >= PARALLEL_VISITS)) {
+  if (ready || (w->pool && pr.active - from >= w->parallel_from)) {
     // Each overlap reads only the trees, so a large pass tests them on the device or the pool,
     // then removes in the serial loop's order (a removal moves the last active pair, and its
     // flag, into the slot); the pairs kept are then slots from..active in order, one visit each.
@@ -1743,12 +1746,12 @@ void narrow_begin(bm_world* w, int from) {
   const int n = (int)w->visit_count;
   if (w->rebuilt.size() < w->visits.size()) w->rebuilt.resize(w->visits.size());
   Manifold* rebuilt = w->rebuilt.data();
-  if (!w->pool && w->owns_ctx && n >= PARALLEL_VISITS) {
+  if (!w->pool && w->owns_ctx && n >= w->parallel_from) {
     // A world stepped alone that grew large: its own threads, up to 16.
     const int threads = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
     w->pool = new Pool(threads - 1);
   }
-  if (w->pool && n >= PARALLEL_VISITS) {
+  if (w->pool && n >= w->parallel_from) {
     const int chunk = 1024, chunks = (n + chunk - 1) / chunk;
     // Each chunk's meets and level pairs counted as it decides, so their slots (in visit
     // order) are known per chunk and taken on the pool below.
@@ -2054,7 +2057,7 @@ void sleep_step(bm_world* w, double dt) {
   };
   bool ready = false;
   const int n_awake = w->awake_count;
-  if (w->pool && n_awake >= PARALLEL_VISITS) {
+  if (w->pool && n_awake >= w->parallel_from) {
     const int chunk = 2048, chunks = (n_awake + chunk - 1) / chunk;
     std::vector<uint8_t> any((size_t)chunks, 0);
     w->pool->run(chunks, [&](int c) { any[c] = timers(c * chunk, std::min(n_awake, (c + 1) * chunk)) ? 1 : 0; });
@@ -2824,7 +2827,7 @@ void touched(bm_world* w) {
     w->events.begin(pr.a[p], pr.b[p], at[0], at[1], at[2], c.normal[h3], c.normal[h3 + 1], c.normal[h3 + 2], speed);
   };
   const int n = c.count;
-  if (!w->pool || n < PARALLEL_VISITS) {
+  if (!w->pool || n < w->parallel_from) {
     for (int f = 0; f < n;) {
       const auto [p, hit] = first_hit(f);
       const uint8_t now = hit >= 0 ? 1 : 0;
@@ -2896,7 +2899,7 @@ void store_memory(bm_world* w, int solved) {
     }
   };
   const int n = c.count;
-  if (!w->pool || n < PARALLEL_VISITS) {
+  if (!w->pool || n < w->parallel_from) {
     for (int f = 0; f < n; f++) one(f);
     return;
   }
@@ -3890,6 +3893,13 @@ int bm_world_read(const bm_world* w, bm_world_state* out) {
 // src/world.js:771-803 (moved): the slots that moved in any step after `since` (and always in
 // the last one), with bodies added since the last step: the awake list's in its order, then
 // the log's from its end; a `since` older than the log reaches takes every body by its stamp.
+int bm_world_set_parallel(bm_world* w, int32_t from) {
+  if (!w) return BM_ERR_ARGUMENT;
+  if (from < 1) return fail(w, BM_ERR_ARGUMENT, "bm_world_set_parallel: from must be 1 or more");
+  w->parallel_from = from;
+  return BM_OK;
+}
+
 int bm_world_set_broadphase(bm_world* w, int32_t mode) {
   if (!w) return BM_ERR_ARGUMENT;
   w->error.clear();

@@ -77,7 +77,7 @@ def copy_columns(cols):
 EXPORTS = ["bm_create", "bm_destroy", "bm_last_error", "bm_upload", "bm_soft_step", "bm_download",
            "bm_soft_step_host", "bm_last_kernel_ms", "bm_set_launch", "bm_lds_limit", "bm_colour", "bm_collide_host", "bm_meet_host", "bm_set_colour_lds", "bm_set_hulls",
            "bm_set_fields", "bm_level_host", "bm_set_wide", "bm_set_device_only", "bm_set_body_version",
-           "bm_level_noalloc", "bm_level_packed", "bm_meet_indexed", "bm_meet_packed", "bm_contact_phase", "bm_set_graph",
+           "bm_level_noalloc", "bm_level_packed", "bm_meet_indexed", "bm_meet_packed", "bm_world_set_parallel", "bm_contact_phase", "bm_set_graph",
            "bm_graph_counts", "bm_batch_set_graph", "bm_batch_graph_counts", "bm_world_moved", "bm_world_tap", "bm_world_events_of", "bm_world_set_broadphase",
            "bm_broad_set_bodies", "bm_broad_set_state", "bm_broad_keep", "bm_broad_set_trees", "bm_broad_bounds",
            "bm_broad_query", "bm_broad_holds",
@@ -173,6 +173,7 @@ def world_signatures(lib):
     lib.bm_world_tap.argtypes = [vp, COLUMNS_TAP, vp]
     lib.bm_world_events_of.argtypes = [vp, ctypes.POINTER(WorldEvents)]
     lib.bm_world_set_broadphase.argtypes = [vp, ctypes.c_int32]
+    lib.bm_world_set_parallel.argtypes = [vp, ctypes.c_int32]
     lib.bm_world_set_hulls.argtypes = [vp, ctypes.c_int32, _I32, _I32, _I32, _D, _I32, _I32, _D]
     lib.bm_world_set_fields.argtypes = [vp, ctypes.c_int32, _I32, _I32, _D, ctypes.POINTER(ctypes.c_int64), _D]
     lib.bm_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int32, ctypes.c_int32]
@@ -490,6 +491,11 @@ class World:
             raise BearingError(f"{what}: status {rc}: {self.lib.bm_world_error(self.ptr).decode()}")
 
     BROADPHASE = {"host": 0, "device": 1, "auto": 2}
+
+    def set_parallel(self, from_):
+        """bm_world_set_parallel: from how many pairs (bodies, rows) a pass's work goes to
+        the world's host threads."""
+        self.check(self.lib.bm_world_set_parallel(self.ptr, int(from_)), "bm_world_set_parallel")
 
     def set_broadphase(self, mode):
         """Where the broadphase's passes run: "host", "device" or "auto" (bm_world_set_broadphase)."""

@@ -489,6 +489,12 @@ int bm_world_moved(bm_world* world, int32_t since, int32_t* out, int32_t* count)
  * deepest point moved on by half of what the two bodies moved in the step, the normal from a
  * to b and the fastest closing speed among the points; an end's point, normal and speed are 0.
  * The pointers are the world's own and hold until its next step or bm_world_destroy. */
+/* From how many pairs a world stepped alone (bm_world_step) takes a narrow pass's work, and
+ * from how many awake bodies or contact rows its other large loops, to its host threads
+ * (default 8,192). No counterpart in the reference, whose workers take a pass's pairs from a
+ * fixed size (src/world.js:186); the results are the same bits either way, so this only
+ * moves work between the calling thread and the world's threads. */
+int bm_world_set_parallel(bm_world* w, int32_t from);
 /* Where a world's broadphase passes run (src/world.js:1574-1630 bounds, src/broadphase.js:140-187
  * update and candidates, src/pairs.js:340-356 holds): BM_BROAD_HOST on the host's threads;
  * BM_BROAD_DEVICE on the device (bm_broad_*), the host re-filing bodies in the trees and taking

@@ -40,6 +40,22 @@ def test_host_world_matches_reference_trajectory(lib, name):
     assert w.read()["awake"] == int(fx["awake"][-1])
 
 
+@pytest.mark.parametrize("name", ["pyramid", "rain_small", "pile_small", "darts", "rocks", "terrain"])
+def test_host_world_on_its_threads_matches_reference_trajectory(lib, name):
+    # Every pass's work taken to the world's host threads (the parallel paths a large world
+    # takes: decisions and slots per chunk, the apart pairs found per chunk, empty meets
+    # dropped and level pairs matched on the pool, pair memory, touched and sleep timers in
+    # chunks) from 16 pairs on: the same trajectory, bit for bit, with its events.
+    fx = traj(name)
+    w = make_world(fx, lib=lib)
+    w.set_parallel(16)
+    matched, why = run(w, fx)
+    assert why is None, why
+    assert matched == fx.meta["steps"]
+    state = w.read()
+    assert (state["impacts"], state["catches"]) == (fx.meta["impacts"], fx.meta["catches"])
+
+
 def test_host_world_refuses_what_it_does_not_handle(lib):
     refuses(lambda: native.World(lib=lib))
 
