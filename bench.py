@@ -360,6 +360,7 @@ def c4_run(threads, loop_sample=16):
                               "frac": achieved / HBM_PEAK_GBS, "kernel_ms": float(np.mean(soft_ms)),
                               "algorithmic_bytes_per_launch": float(np.mean(soft_bytes)),
                               "rows_per_step": float(np.mean(rows)),
+                              "traffic": traffic_of("soft_step_kernel<true, 256, 1>", "scenes.batched"),
                               "sampled": "every tenth step"}}
 
 
