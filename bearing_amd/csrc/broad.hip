@@ -22,12 +22,17 @@ __global__ void __launch_bounds__(256) bounds_kernel(BoundsDev d) {
   d.refile[i] = r;
 }
 
-// One marked body a lane, a wave a workgroup: the walks once to count the leaves, a
-// reservation of that many places, then the walks again to write them, so each body's leaves
-// stay in walk order. Each lane's stack lives in LDS, entry k of lane l at k * 64 + l.
+// One marked body a lane, a wave a workgroup: the walks once, keeping the first FOUND leaves
+// met and counting them all, a reservation of that many places, the kept leaves written there,
+// and only for a body that met more the walks again to write the rest, so each body's leaves
+// stay in walk order. Each lane's stack and kept leaves live in LDS, entry k of lane l at
+// k * 64 + l.
+constexpr int FOUND = 32;
 __global__ void __launch_bounds__(64) query_kernel(QueryDev d) {
   __shared__ int stacks[broad::STACK * 64];
+  __shared__ int kept[FOUND * 64];
   int* stack = stacks + threadIdx.x;
+  int* mine = kept + threadIdx.x;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= d.n) return;
   const int a = d.moved[i];
@@ -41,7 +46,9 @@ __global__ void __launch_bounds__(64) query_kernel(QueryDev d) {
   const bool stat = d.type[a] == broad::STATIC;
   int c = 0;
   auto count = [&](int b) {
-    if (b != a) c++;
+    if (b == a) return;
+    if (c < FOUND) mine[c * 64] = b;
+    c++;
   };
   bool ok = true;
   if (!stat) ok = broad::query(d.trees[broad::STATIC_TREE], q, stack, 64, count);
@@ -54,9 +61,14 @@ __global__ void __launch_bounds__(64) query_kernel(QueryDev d) {
   d.counts[i] = c;
   d.starts[i] = base;
   if (c == 0 || base + c > d.cap) return;
-  int at = base;
+  for (int k = 0; k < c && k < FOUND; k++) d.hits[base + k] = mine[k * 64];
+  if (c <= FOUND) return;
+  // more than were kept: the same walks again, writing from the first leaf not kept on
+  int seen = 0;
   auto put = [&](int b) {
-    if (b != a) d.hits[at++] = b;
+    if (b == a) return;
+    if (seen >= FOUND && seen < c) d.hits[base + seen] = b;
+    seen++;
   };
   if (!stat) broad::query(d.trees[broad::STATIC_TREE], q, stack, 64, put);
   broad::query(d.trees[broad::MOVING_TREE], q, stack, 64, put);
