@@ -1857,7 +1857,7 @@ __global__ void meet_kernel(MeetDev d) {
 // of every cell under the box whose heights reach it, row by row, each touched; points
 // grouped by direction, kept once where two triangles give the same one; each group sorted
 // by id and reduced to four.
-__global__ void __launch_bounds__(64) level_kernel(LevelDev d) {
+__global__ void __launch_bounds__(64, 2) level_kernel(LevelDev d) {
   using namespace narrow;
   __shared__ Touch touched[64];
   __shared__ int hits[64];
