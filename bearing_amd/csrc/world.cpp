@@ -3857,6 +3857,13 @@ int bm_batch_set_graph(bm_batch* B, int32_t on) {
   return bm_set_graph(B->ctx, on);
 }
 
+int bm_batch_set_launch(bm_batch* B, int32_t shape, int32_t resident) {
+  if (!B) return BM_ERR_ARGUMENT;
+  const int rc = bm_set_launch(B->ctx, shape, resident);
+  if (rc != BM_OK) batch_fail(B, rc, bm_last_error(B->ctx));
+  return rc;
+}
+
 int bm_batch_graph_counts(const bm_batch* B, int64_t* out) {
   if (!B || !out) return BM_ERR_ARGUMENT;
   return bm_graph_counts(B->ctx, out);

@@ -77,7 +77,7 @@ def copy_columns(cols):
 EXPORTS = ["bm_create", "bm_destroy", "bm_last_error", "bm_upload", "bm_soft_step", "bm_download",
            "bm_soft_step_host", "bm_last_kernel_ms", "bm_set_launch", "bm_lds_limit", "bm_colour", "bm_collide_host", "bm_meet_host", "bm_set_colour_lds", "bm_set_hulls",
            "bm_set_fields", "bm_level_host", "bm_set_wide", "bm_set_device_only", "bm_set_body_version",
-           "bm_level_noalloc", "bm_level_packed", "bm_meet_indexed", "bm_meet_packed", "bm_world_set_parallel", "bm_contact_phase", "bm_set_graph",
+           "bm_level_noalloc", "bm_level_packed", "bm_meet_indexed", "bm_meet_packed", "bm_world_set_parallel", "bm_batch_set_launch", "bm_contact_phase", "bm_set_graph",
            "bm_graph_counts", "bm_batch_set_graph", "bm_batch_graph_counts", "bm_world_moved", "bm_world_tap", "bm_world_events_of", "bm_world_set_broadphase",
            "bm_broad_set_bodies", "bm_broad_set_state", "bm_broad_keep", "bm_broad_set_trees", "bm_broad_bounds",
            "bm_broad_query", "bm_broad_holds",
@@ -183,6 +183,7 @@ def world_signatures(lib):
     lib.bm_batch_world.argtypes = [vp, ctypes.POINTER(vp), _D, ctypes.c_int32, ctypes.c_int32]
     lib.bm_batch_step.argtypes = [vp, ctypes.c_double]
     lib.bm_batch_set_graph.argtypes = [vp, ctypes.c_int32]
+    lib.bm_batch_set_launch.argtypes = [vp, ctypes.c_int32, ctypes.c_int32]
     lib.bm_batch_graph_counts.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
     lib.bm_world_times.argtypes = [vp, _D]
     lib.bm_batch_times.argtypes = [vp, _D]
@@ -665,6 +666,10 @@ class WorldBatch:
     def set_graph(self, on):
         """Whether the contact phase goes to the device as a HIP graph (bm_batch_set_graph)."""
         self.check(self.lib.bm_batch_set_graph(self.ptr, 1 if on else 0), "bm_batch_set_graph")
+
+    def set_launch(self, shape, resident=True):
+        """The soft step's launch shape (0, 1, 2) and velocity residency (bm_batch_set_launch)."""
+        self.check(self.lib.bm_batch_set_launch(self.ptr, int(shape), 1 if resident else 0), "bm_batch_set_launch")
 
     def graph_counts(self):
         """{launches, made, updated}: the contact phase's graphs so far (bm_batch_graph_counts)."""

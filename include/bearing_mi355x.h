@@ -551,6 +551,9 @@ int bm_batch_world(bm_batch* batch, bm_world** out, const double* gravity, int32
 /* Whether the batch's contact phase goes to the device as a HIP graph (bm_contact_phase);
  * default on. */
 int bm_batch_set_graph(bm_batch* batch, int32_t on);
+/* The batch's soft-step launch shape and velocity residency, as bm_set_launch sets a context's
+ * (an autotuner times them on the batch's own steps); every world's bits are the same for each. */
+int bm_batch_set_launch(bm_batch* batch, int32_t shape, int32_t resident);
 int bm_batch_graph_counts(const bm_batch* batch, int64_t* out);
 /* Every world of the batch, one step of dt. A step that fails after the worlds began theirs
  * leaves all of them part-way: every later step of the batch returns the same code and

@@ -379,6 +379,16 @@ int bm_meet_host(bm_context* ctx, int32_t n, const double* a, const double* b, c
   return BM_OK;
 }
 
+// The launch shape has no meaning on the CPU; its arguments are checked as the device's are.
+int bm_set_launch(bm_context* ctx, int32_t shape, int32_t resident) {
+  (void)resident;
+  if (shape < 0 || shape > 2) {
+    ctx->error = "bm_set_launch: shape must be 0, 1 or 2";
+    return BM_ERR_ARGUMENT;
+  }
+  return BM_OK;
+}
+
 int bm_meet_indexed(bm_context* ctx, int32_t bodies, const double* table, int32_t n, const int32_t* ia,
                     const int32_t* ib, const double* pair, int32_t* count, double* normal, double* points,
                     double* separations, uint32_t* ids, double* shift, int32_t* flags, double* loss) {
