@@ -1423,8 +1423,23 @@ int meet_start(bm_context* ctx, const char* who, int32_t bodies, const double* t
   d.hulls = ctx->hulls;
   if (int rc = grow_pinned(ctx, &ctx->pstage, &ctx->pstage_cap, in_bytes + 400 * N + 4096, who)) return rc;
   char* st = static_cast<char*>(ctx->pstage);
-  run_copies({{st, table, T}, {st + T, pair, pairs}, {st + T + pairs, ia, 4 * N}, {st + T + pairs + idx, ib, 4 * N}});
-  HIP_OR_FAIL(ctx, hipMemcpyAsync(in, st, in_bytes, hipMemcpyHostToDevice, ctx->stream), (w + ": copy").c_str());
+  // Staged and sent in pieces when large (a batch's table of every meeting body): each piece
+  // goes up while the next one is staged.
+  const std::vector<HostCopy> all = {
+      {st, table, T}, {st + T, pair, pairs}, {st + T + pairs, ia, 4 * N}, {st + T + pairs + idx, ib, 4 * N}};
+  constexpr int K = bm_context::PIECES;
+  const int pieces = in_bytes >= PIECED_BYTES ? K : 1;
+  for (int k = 0; k < pieces; k++) {
+    const size_t lo = in_bytes / pieces * k, hi = k + 1 == pieces ? in_bytes : in_bytes / pieces * (k + 1);
+    std::vector<HostCopy> part;
+    for (const HostCopy& c : all) {
+      const size_t at = static_cast<size_t>(static_cast<char*>(c.to) - st);
+      const size_t a = std::max(at, lo), b = std::min(at + c.bytes, hi);
+      if (a < b) part.push_back({st + a, static_cast<const char*>(c.from) + (a - at), b - a});
+    }
+    run_copies(part);
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(in + lo, st + lo, hi - lo, hipMemcpyHostToDevice, ctx->stream), (w + ": copy").c_str());
+  }
   (void)hipGetLastError();
   HIP_OR_FAIL(ctx, hipEventRecord(ctx->begin, ctx->stream), (w + ": hipEventRecord").c_str());
   bm::launch_meet(d, ctx->stream);
